@@ -1,0 +1,209 @@
+"""Throughput of the MI355X LW-FR step (FP64 DOF-updates/s) with roofline and CPU baseline.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-cpu-baseline]
+
+A step is one full LW step of solve_lwfr's loop on the device (CFL dt from the
+current state, volume kernel, surface kernel, update, dt/t update), over the
+whole mesh.  Inputs are resident in HBM before the timed region.  Prints one
+JSON line on stdout; progress goes to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "trixilw.jl_amd"))
+sys.path.insert(0, str(ROOT / "oracle"))
+
+METRIC = "FP64 DOF-updates/s on 1 MI355X, % HBM roofline; adjoint/forward step ratio"
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    # isentropic vortex (strength 5, Mach 0.5), periodic [-10,10]^2 TreeMesh, Rusanov, CFL 0.98·LW limit
+    "lw_euler_vortex_1024x1024_N4": dict(level=10, N=4, td="lw"),
+    "lw_euler_vortex_256x256_N4": dict(level=8, N=4, td="lw"),
+    "lw_euler_vortex_32x32_N4": dict(level=5, N=4, td="lw"),
+    "mdrk_euler_vortex_1024x1024_N3": dict(level=10, N=3, td="mdrk"),
+}
+DEFAULT_WORKLOAD = "lw_euler_vortex_1024x1024_N4"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def alg_bytes_per_dof(N: int, td: str):
+    """Algorithmic HBM bytes per DOF-update: (whole step, volume kernel).
+
+    Whole step (LW): read uⁿ + write uⁿ⁺¹ + one write and one read of the face
+    traces (uⁿ, Ũ, F̃·n: 3 × 4 variables × n nodes × 4 faces):
+    8 (2·4 n² + 2·3·4·4 n) / n².  The volume kernel's share is the state read,
+    the state write and the trace write: 8 (2·4 n² + 3·4·4 n) / n².
+    MDRK: 8 (5·4 n² + 4·48 n) / n² per step (two stages).
+    """
+    n = N + 1
+    if td == "lw":
+        step = 8.0 * (2 * 4 * n * n + 2 * 3 * 4 * 4 * n) / (n * n)
+        vol = 8.0 * (2 * 4 * n * n + 3 * 4 * 4 * n) / (n * n)
+    else:
+        step = 8.0 * (5 * 4 * n * n + 4 * 48 * n) / (n * n)
+        vol = None
+    return step, vol
+
+
+def load_traffic(workload: str):
+    p = ROOT / "profiles" / "traffic.json"
+    if not p.exists():
+        return None
+    try:
+        d = json.loads(p.read_text())
+        return d.get(workload)
+    except Exception:
+        return None
+
+
+def cpu_baseline(o, u0, cfl, mdrk, dofs, steps, threads):
+    import oracle_ctypes
+
+    oracle_ctypes.set_threads(threads)
+    t0 = time.perf_counter()
+    _, _, dts = o.solve(u0, 0.0, 1e30, cfl, max_steps=steps, mdrk=mdrk)
+    el = time.perf_counter() - t0
+    return dofs * len(dts) / el, el
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default=DEFAULT_WORKLOAD, choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (the sharding design is in "
+              "DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import trixilw_hip as tlw
+    from trixilw_hip import _lib
+
+    wl = WORKLOADS[args.workload]
+    N, td = wl["N"], wl["td"]
+    eq = tlw.CompressibleEulerEquations2D(1.4)
+    ic = tlw.IsentropicVortex()
+    mesh = tlw.TreeMesh((-10.0, -10.0), (10.0, 10.0), wl["level"], ordering="morton")
+    solver = tlw.DGSEM(N, volume_integral=tlw.VolumeIntegralFR(tlw.MDRK() if td == "mdrk" else tlw.LW()))
+    semi = tlw.SemidiscretizationHyperbolic(mesh, tlw.get_time_discretization(solver), eq, ic, solver)
+    cfl = tlw.trixi2lw(0.98, solver) * (0.5 if td == "mdrk" else 1.0)
+    dofs = semi.ndofs()
+    log(f"bench: {args.workload}: {mesh.n_elements} elements, {dofs} DOFs")
+
+    x, y = semi.node_coordinates()
+    u0 = np.ascontiguousarray(ic(x, y, 0.0))
+    L = _lib.lib()
+    h = semi.handle
+    dp = C.POINTER(C.c_double)
+    semi._bind_stream()
+    _lib.check(L.tlw_set_state(h, u0.ctypes.data_as(dp), u0.size), h)
+    _lib.check(L.tlw_solve_init(h, 0.0, float("inf"), cfl), h)
+    t, dt = C.c_double(), C.c_double()
+    it, fin = C.c_int64(), C.c_int32()
+    if args.warmup > 0:
+        _lib.check(L.tlw_advance(h, args.warmup, C.byref(t), C.byref(dt), C.byref(it), C.byref(fin)), h)
+    _lib.check(L.tlw_enable_kernel_timing(h, 1), h)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _lib.check(L.tlw_advance(h, args.steps, C.byref(t), C.byref(dt), C.byref(it), C.byref(fin)), h)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    vms, vn, sms, sn = C.c_double(), C.c_int64(), C.c_double(), C.c_int64()
+    _lib.check(L.tlw_kernel_times(h, C.byref(vms), C.byref(vn), C.byref(sms), C.byref(sn)), h)
+    _lib.check(L.tlw_enable_kernel_timing(h, 0), h)
+    steps_done = int(it.value) - args.warmup
+    assert steps_done == args.steps, (steps_done, args.steps)
+    value = dofs * args.steps / el
+    ms_per_step = 1e3 * el / args.steps
+    log(f"bench: {value / 1e9:.3f} G DOF-updates/s, {ms_per_step:.3f} ms/step, "
+        f"volume {vms.value / max(vn.value, 1):.3f} ms x{vn.value}, surface {sms.value / max(sn.value, 1):.3f} ms")
+
+    step_b, vol_b = alg_bytes_per_dof(N, td)
+    vol_avg_s = vms.value / max(vn.value, 1) * 1e-3
+    launches_per_step = 1 if td == "lw" else 2
+    if vol_b is not None:
+        achieved = vol_b * dofs / vol_avg_s / 1e9
+        rl_unit_bytes = vol_b * dofs
+    else:
+        achieved = step_b * dofs / (launches_per_step * vol_avg_s) / 1e9
+        rl_unit_bytes = step_b * dofs / launches_per_step
+    traffic = load_traffic(args.workload)
+    roofline = {
+        "bound": "hbm",
+        "kernel": "k_volume_lw4" if (td == "lw" and N == 4) else ("k_volume_lw3" if td == "lw" else "k_volume_mdrk*"),
+        "achieved": round(achieved, 2),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": traffic,
+        "alg_bytes_per_launch": rl_unit_bytes,
+        "avg_launch_ms": round(vol_avg_s * 1e3, 4),
+        "step_alg_bytes_per_dof": step_b,
+        "step_frac": round(step_b * value / 1e9 / HBM_PEAK_GBS, 4),
+        "surface_avg_launch_ms": round(sms.value / max(sn.value, 1), 4),
+    }
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        import oracle_ctypes
+
+        threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+        threads = max(1, min(threads, os.cpu_count() or threads))
+        nx = mesh.nx
+        o = oracle_ctypes.Oracle(N, nx, nx, -10.0, -10.0, 20.0 / nx)
+        xo, yo = o.node_coords()
+        uo = np.ascontiguousarray(ic(xo, yo, 0.0))
+        log(f"bench: CPU oracle baseline, {args.cpu_steps} steps on {threads} threads")
+        v_all, el_all = cpu_baseline(o, uo, cfl, td == "mdrk", dofs, args.cpu_steps, threads)
+        log(f"bench: CPU {v_all / 1e6:.2f} M DOF-updates/s ({el_all:.1f} s)")
+        cpu = {"value": v_all, "unit": "DOF-updates/s", "cores": threads, "kind": "port",
+               "sample": f"{args.cpu_steps} steps of the same {nx}x{nx} N={N} mesh ({dofs} DOFs), "
+                         f"C++/OpenMP oracle, g++ -O3 -march=x86-64-v3",
+               "gpu_over_cpu": round(value / v_all, 2)}
+        del o
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "DOF-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "mesh": f"{mesh.nx}x{mesh.nx} periodic TreeMesh [-10,10]^2",
+                   "polydeg": N, "time_discretization": td.upper(), "dofs": dofs,
+                   "initial_condition": "isentropic vortex strength 5, Mach 0.5", "cfl": cfl},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    semi.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,157 @@
+/*
+ * trixilw_hip.h -- C ABI of libtrixilw_hip.so, the MI355X (gfx950) implementation
+ * of TrixiLW's Lax-Wendroff flux-reconstruction and MDRK right-hand side on a
+ * 2-D TreeMesh, compressible Euler, FP64.
+ *
+ * The entry points replace the reference's Julia methods on
+ * SemidiscretizationHyperbolic (paths relative to the TrixiLW.jl tree):
+ *   rhs!        src/semidiscretization/semidiscretization_hyperbolic.jl:4-20
+ *   rhs_mdrk1!  src/semidiscretization/semidiscretization_hyperbolic.jl:22-38
+ *   rhs_mdrk2!  src/semidiscretization/semidiscretization_hyperbolic.jl:40-56
+ *   compute_dt  src/solvers/lwfr.jl:176-190 (Trixi max_dt)
+ *   perform_step!/solve_lwfr loop  src/solvers/lwfr.jl:212-263, src/solvers/mdrk.jl:1-29
+ *   SemidiscretizationHyperbolic + create_cache_serial
+ *               src/semidiscretization/semidiscretization_hyperbolic.jl:58-84,
+ *               src/solvers/dgsem_tree/containers.jl:7-64
+ *
+ * Conventions
+ *   - Every function returns TLW_OK (0) or an error code; tlw_last_error()
+ *     gives a message.  TLW_EADMISSIBLE plays the role of the reference's
+ *     DomainError (src/solvers/adaptive.jl:6-21).
+ *   - State arrays use Trixi's layout [v, i, j, e] (column-major, variable
+ *     fastest), 4 variables (ρ, ρv1, ρv2, ρE), (N+1)^2 nodes per element.
+ *   - Pointers named *_dev are device pointers; *_host are host pointers.
+ *   - All work is ordered on the context's stream.  A context is not
+ *     reentrant (one host thread at a time), like the reference's cache.
+ */
+#ifndef TRIXILW_HIP_H
+#define TRIXILW_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    TLW_OK = 0,
+    TLW_EINVAL = 1,      /* bad argument or description */
+    TLW_EHIP = 2,        /* HIP runtime error */
+    TLW_EADMISSIBLE = 3, /* non-admissible state (ρ <= 0 or p <= 0) */
+    TLW_ENOTSUP = 4      /* configuration not supported */
+};
+
+/* time discretizations (src/basic_types.jl:4-6) */
+enum { TLW_LW = 0, TLW_MDRK = 1 };
+
+/* Argument order of the 8-argument Rusanov flux (src/equations/numerical_fluxes.jl:14-38):
+ * TreeMesh passes (F̃,F̃,Ũ,Ũ,uⁿ,uⁿ): λ from Ũ, jump on uⁿ (src/solvers/dgsem_tree/dg_2d.jl:1928).
+ * Structured/P4est pass (F̃n,F̃n,uⁿ,uⁿ,Ũ,Ũ): λ from uⁿ, jump on Ũ (src/solvers/dgsem_structured/dg_2d.jl:2271). */
+enum { TLW_FLUX_TREE = 0, TLW_FLUX_STRUCTURED = 1 };
+
+/* boundary value functions of the time-averaged Dirichlet condition
+ * (src/equations/equations.jl:6-37) */
+enum { TLW_BV_NONE = 0, TLW_BV_VORTEX = 1, TLW_BV_CONSTANT = 2, TLW_BV_RIEMANN4 = 3 };
+
+/* tlw_get_aux selectors (element_cache fields, src/solvers/dgsem_tree/containers.jl:73-98) */
+enum { TLW_AUX_U = 0, TLW_AUX_F = 1 };
+
+/* solver flags */
+enum { TLW_FLAG_KEEP_AUX = 1 /* store full Ũ [v,i,j,e] and F̃ [v,d,i,j,e] on every rhs */ };
+
+/* Mesh: Trixi TreeMesh element data (cache.elements) plus face neighbours
+ * derived from cache.interfaces / cache.boundaries.  Conforming meshes only. */
+typedef struct {
+    int32_t n_elements;
+    const int32_t* neighbors_host;        /* [4*E]: neighbour of faces -x,+x,-y,+y; -1 = domain boundary */
+    const double* inverse_jacobian_host;  /* [E] */
+    const double* element_center_host;    /* [2*E] (x, y); used for boundary node coordinates */
+    const double* element_length_host;    /* [E] */
+} tlw_mesh_desc;
+
+/* Basis: Trixi LobattoLegendreBasis fields, arrays column-major as Julia stores them. */
+typedef struct {
+    int32_t polydeg;                /* 3 or 4 */
+    const double* nodes_host;       /* [n] */
+    const double* weights_host;     /* [n] */
+    const double* derivative_matrix_host; /* [n*n], D[i,k] at i + n*k */
+    const double* derivative_dhat_host;   /* [n*n] */
+} tlw_basis_desc;
+
+typedef struct {
+    int32_t time_discretization;    /* TLW_LW or TLW_MDRK */
+    double gamma;                   /* CompressibleEulerEquations2D */
+    int32_t surface_flux_order;     /* TLW_FLUX_TREE or TLW_FLUX_STRUCTURED */
+    int32_t boundary_kind;          /* TLW_BV_* applied on every face with neighbour -1 */
+    double boundary_params[24];
+    int32_t flags;                  /* TLW_FLAG_* */
+} tlw_solver_desc;
+
+typedef struct tlw_ctx tlw_ctx;
+
+/* SemidiscretizationHyperbolic(mesh, time_disc, equations, ic, solver) + cache
+ * (semidiscretization_hyperbolic.jl:58-84, containers.jl:7-64): uploads mesh and
+ * basis, allocates the device state, traces and MDRK stage buffer. */
+int tlw_create(const tlw_mesh_desc* mesh, const tlw_basis_desc* basis, const tlw_solver_desc* solver,
+               tlw_ctx** out);
+/* AMR or any host-side mesh change: rebuild the device mesh (src/callbacks/amr.jl:2-20 resize!) */
+int tlw_rebuild_mesh(tlw_ctx* ctx, const tlw_mesh_desc* mesh);
+void tlw_destroy(tlw_ctx* ctx);
+const char* tlw_last_error(const tlw_ctx* ctx);
+
+/* Stream on which all work of the context is ordered (hipStream_t passed as void*; NULL = default). */
+int tlw_set_stream(tlw_ctx* ctx, void* stream);
+
+/* Number of doubles in a state vector: 4 * (N+1)^2 * E (Trixi ndofs * nvariables). */
+size_t tlw_state_len(const tlw_ctx* ctx);
+
+/* rhs!(du_ode, u_ode, semi, t) with cache.dt[1] = dt (semidiscretization_hyperbolic.jl:4-20,
+ * dg_2d.jl:28-83).  du is overwritten. */
+int tlw_rhs_lw(tlw_ctx* ctx, const double* u_dev, double* du_dev, double t, double dt);
+/* rhs_mdrk1!/rhs_mdrk2! (semidiscretization_hyperbolic.jl:22-56, tree/multi_derivative_2d.jl:1-116).
+ * Stage 2 reads us from the context's stage buffer (tlw_us_ptr), and u must be the
+ * same uⁿ that stage 1 received, as in mdrk.jl:13-25. */
+int tlw_rhs_mdrk1(tlw_ctx* ctx, const double* u_dev, double* du_dev, double t, double dt);
+int tlw_rhs_mdrk2(tlw_ctx* ctx, const double* u_dev, double* du_dev, double t, double dt);
+/* Device pointer of the MDRK stage state `us` (element_cache.mdrk_cache._us), and a
+ * stream-ordered device-to-device copy into it (update_soln!(integrator, _us, uprev, du),
+ * src/solvers/mdrk.jl:17). */
+int tlw_us_ptr(tlw_ctx* ctx, double** us_dev);
+int tlw_set_us(tlw_ctx* ctx, const double* us_dev);
+
+/* Trixi max_dt(u, t, mesh::TreeMesh{2}, ...) = 2 / (n * max_e J⁻¹ (max λ1 + max λ2));
+ * compute_dt multiplies it by the CFL number (lwfr.jl:176-190). */
+int tlw_max_dt(tlw_ctx* ctx, const double* u_dev, double* max_dt_host);
+
+/* Device-resident time loop: the state lives in the context (tlw_set_state /
+ * tlw_get_state / tlw_state_ptr).  tlw_solve_init sets t, the final time and the
+ * CFL number and computes the first dt on the device; tlw_advance runs up to
+ * max_steps steps of solve_lwfr's loop (compute_dt, set_dt! truncation at t_final,
+ * rhs!/rhs_mdrk*, u = uprev + dt du, set_t_and_iter!, isfinished = t ≈ t_final)
+ * without host synchronisation between steps, then reports the time and the
+ * number of steps actually taken. */
+int tlw_set_state(tlw_ctx* ctx, const double* u_host, size_t len);
+int tlw_get_state(tlw_ctx* ctx, double* u_host, size_t len);
+int tlw_state_ptr(tlw_ctx* ctx, double** u_dev);
+int tlw_solve_init(tlw_ctx* ctx, double t0, double t_final, double cfl);
+int tlw_advance(tlw_ctx* ctx, int32_t max_steps, double* t_host, double* dt_host, int64_t* iter_host,
+                int32_t* finished_host);
+
+/* Measurement hook (no counterpart in the reference; its TimerOutputs/performance_counter
+ * timing, semidiscretization_hyperbolic.jl:11-17, is per rhs! call): when enabled,
+ * tlw_advance records HIP events around every volume and surface kernel on the context's
+ * stream; tlw_kernel_times returns the accumulated milliseconds and launch counts and
+ * resets them. */
+int tlw_enable_kernel_timing(tlw_ctx* ctx, int32_t enable);
+int tlw_kernel_times(tlw_ctx* ctx, double* volume_ms, int64_t* volume_launches, double* surface_ms,
+                     int64_t* surface_launches);
+
+/* Copy an element-cache array of the last rhs call to the host (needs TLW_FLAG_KEEP_AUX):
+ * TLW_AUX_U: Ũ [v,i,j,e], len 4 n^2 E; TLW_AUX_F: F̃ [v,d,i,j,e], len 8 n^2 E. */
+int tlw_get_aux(tlw_ctx* ctx, int32_t which, double* out_host, size_t len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TRIXILW_HIP_H */

@@ -1,0 +1,201 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle_tlw.so).
+
+TEST INFRASTRUCTURE ONLY.  Imported by tests/, ``__graft_entry__.smoke()`` and
+the ``cpu_baseline`` leg of ``bench.py`` as the checker; the product path never
+imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle_tlw.so"
+
+BV_NONE, BV_VORTEX, BV_CONSTANT, BV_RIEMANN4 = 0, 1, 2, 3
+
+
+class _Desc(C.Structure):
+    _fields_ = [
+        ("polydeg", C.c_int),
+        ("nx", C.c_int),
+        ("ny", C.c_int),
+        ("xmin", C.c_double),
+        ("ymin", C.c_double),
+        ("h", C.c_double),
+        ("periodic_x", C.c_int),
+        ("periodic_y", C.c_int),
+        ("equation", C.c_int),
+        ("gamma", C.c_double),
+        ("adv_a1", C.c_double),
+        ("adv_a2", C.c_double),
+        ("flux_order", C.c_int),
+        ("bc_kind", C.c_int),
+        ("bc_par", C.c_double * 24),
+        ("abstol", C.c_double),
+        ("reltol", C.c_double),
+        ("nthreads", C.c_int),
+    ]
+
+
+def build(force: bool = False) -> Path:
+    """Compile the oracle with its Makefile (g++, OpenMP)."""
+    if force or not LIB_PATH.exists():
+        subprocess.run(["make", "-C", str(HERE), "-B" if force else "all"], check=True,
+                       stdout=subprocess.DEVNULL)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        L = C.CDLL(str(LIB_PATH))
+        dp = C.POINTER(C.c_double)
+        L.tlwo_create.restype = C.c_void_p
+        L.tlwo_create.argtypes = [C.POINTER(_Desc)]
+        L.tlwo_destroy.argtypes = [C.c_void_p]
+        L.tlwo_set_threads.argtypes = [C.c_int]
+        L.tlwo_max_threads.restype = C.c_int
+        L.tlwo_ndofs.restype = C.c_longlong
+        L.tlwo_ndofs.argtypes = [C.c_void_p]
+        L.tlwo_basis.argtypes = [C.c_void_p, dp, dp, dp, dp]
+        L.tlwo_node_coords.argtypes = [C.c_void_p, dp, dp]
+        L.tlwo_neighbors.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+        L.tlwo_max_dt.restype = C.c_double
+        L.tlwo_max_dt.argtypes = [C.c_void_p, dp]
+        for nm in ("tlwo_rhs", "tlwo_rhs_mdrk1", "tlwo_rhs_mdrk2"):
+            getattr(L, nm).argtypes = [C.c_void_p, dp, dp, C.c_double, C.c_double]
+        L.tlwo_set_us.argtypes = [C.c_void_p, dp]
+        L.tlwo_get_aux.restype = C.c_longlong
+        L.tlwo_get_aux.argtypes = [C.c_void_p, C.c_int, dp]
+        L.tlwo_euler_fluxes.argtypes = [dp, C.c_double, dp, dp]
+        L.tlwo_max_abs_speed_naive.restype = C.c_double
+        L.tlwo_max_abs_speed_naive.argtypes = [dp, dp, C.c_int, C.c_double]
+        L.tlwo_solve.restype = C.c_int
+        L.tlwo_solve.argtypes = [C.c_void_p, dp, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, dp, dp]
+        _lib = L
+    return _lib
+
+
+def _ptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+AUX = {"U": 0, "F": 1, "temporal_errors": 2, "surface_flux_values": 3, "us": 4, "U2": 5, "F2": 6}
+
+
+class Oracle:
+    """One oracle solver instance on a uniform TreeMesh (elements e = ix + nx*iy)."""
+
+    def __init__(self, polydeg, nx, ny, xmin, ymin, h, periodic=(True, True), equation="euler", gamma=1.4,
+                 adv=(1.0, 1.0), flux_order=0, bc_kind=BV_NONE, bc_par=(), tolerances=(0.0, 0.0), nthreads=0):
+        d = _Desc()
+        d.polydeg, d.nx, d.ny = polydeg, nx, ny
+        d.xmin, d.ymin, d.h = xmin, ymin, h
+        d.periodic_x, d.periodic_y = int(periodic[0]), int(periodic[1])
+        d.equation = 0 if equation == "euler" else 1
+        d.gamma = gamma
+        d.adv_a1, d.adv_a2 = adv
+        d.flux_order = flux_order
+        d.bc_kind = bc_kind
+        for k, v in enumerate(bc_par):
+            d.bc_par[k] = v
+        d.abstol, d.reltol = tolerances
+        d.nthreads = nthreads
+        self._h = lib().tlwo_create(C.byref(d))
+        if not self._h:
+            raise ValueError("oracle supports polydeg 3 and 4")
+        self.n = polydeg + 1
+        self.nv = 4 if equation == "euler" else 1
+        self.E = nx * ny
+        self.nx, self.ny = nx, ny
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().tlwo_destroy(self._h)
+            self._h = None
+
+    @property
+    def shape(self):
+        return (self.E, self.n, self.n, self.nv)
+
+    def basis(self):
+        n = self.n
+        nodes, w = np.zeros(n), np.zeros(n)
+        D, Dh = np.zeros((n, n)), np.zeros((n, n))
+        lib().tlwo_basis(self._h, _ptr(nodes), _ptr(w), _ptr(D), _ptr(Dh))
+        return nodes, w, D, Dh
+
+    def node_coords(self):
+        x = np.zeros((self.E, self.n, self.n))
+        y = np.zeros_like(x)
+        lib().tlwo_node_coords(self._h, _ptr(x), _ptr(y))
+        return x, y  # [e, j, i]
+
+    def neighbors(self):
+        nb = np.zeros((self.E, 4), dtype=np.int32)
+        lib().tlwo_neighbors(self._h, nb.ctypes.data_as(C.POINTER(C.c_int)))
+        return nb
+
+    def max_dt(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        return lib().tlwo_max_dt(self._h, _ptr(u))
+
+    def rhs(self, u, t, dt, kind="lw"):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        du = np.zeros_like(u)
+        fn = {"lw": lib().tlwo_rhs, "mdrk1": lib().tlwo_rhs_mdrk1, "mdrk2": lib().tlwo_rhs_mdrk2}[kind]
+        fn(self._h, _ptr(u), _ptr(du), t, dt)
+        return du
+
+    def set_us(self, us):
+        us = np.ascontiguousarray(us, dtype=np.float64)
+        lib().tlwo_set_us(self._h, _ptr(us))
+
+    def aux(self, name):
+        k = AUX[name]
+        m = lib().tlwo_get_aux(self._h, k, None)
+        out = np.zeros(m)
+        lib().tlwo_get_aux(self._h, k, _ptr(out))
+        return out
+
+    def solve(self, u0, t0, tf, cfl, max_steps=10**9, mdrk=False):
+        """solve_lwfr with CFLBased stepping; returns (u, t, dts)."""
+        u = np.array(u0, dtype=np.float64, copy=True, order="C")
+        cap = max_steps if max_steps < 10**6 else 10**6
+        dts = np.zeros(cap)
+        tout = C.c_double(0.0)
+        k = lib().tlwo_solve(self._h, _ptr(u), t0, tf, cfl, int(min(max_steps, cap)), int(mdrk), C.byref(tout),
+                             _ptr(dts))
+        return u, tout.value, dts[:k]
+
+
+def euler_fluxes(u, gamma=1.4):
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    f, g = np.zeros(4), np.zeros(4)
+    lib().tlwo_euler_fluxes(_ptr(u), gamma, _ptr(f), _ptr(g))
+    return f, g
+
+
+def max_abs_speed_naive(ul, ur, orientation, gamma=1.4):
+    ul = np.ascontiguousarray(ul, dtype=np.float64)
+    ur = np.ascontiguousarray(ur, dtype=np.float64)
+    return lib().tlwo_max_abs_speed_naive(_ptr(ul), _ptr(ur), int(orientation), gamma)
+
+
+def set_threads(n: int) -> None:
+    lib().tlwo_set_threads(int(n))
+
+
+def max_threads() -> int:
+    return int(lib().tlwo_max_threads())

@@ -1,0 +1,1008 @@
+// tlw_oracle.hpp -- CPU restatement of TrixiLW's TreeMesh Lax-Wendroff / MDRK
+// right-hand side.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the parity oracle and the timed CPU
+// baseline.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+// leg may load it, and only as the checker.  The product (libtrixilw_hip.so)
+// never links or calls it.
+//
+// Every function cites the reference file:line it restates (paths relative to
+// the TrixiLW.jl source tree).  Functions that live in the un-vendored Trixi.jl
+// dependency (basis, Euler flux, wave speeds, surface integral, Jacobian,
+// max_dt) are restated from their published definitions; how each is pinned
+// is listed next to it.
+//
+// Layouts follow Trixi's column-major arrays, first index fastest:
+//   u, du        [v, i, j, e]        -> ((e*n + j)*n + i)*NV + v
+//   U (Ũ)        [v, i, j, e]
+//   F (F̃)        [v, d, i, j, e]     -> (((e*n + j)*n + i)*2 + d)*NV + v
+//   sfv          [v, l, face, e]     -> ((e*4 + face)*n + l)*NV + v
+// Faces are numbered as Trixi's `direction`: 0 = -x, 1 = +x, 2 = -y, 3 = +y.
+#pragma once
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+namespace tlwo {
+
+constexpr int MAXN = 8;
+
+// ---------------------------------------------------------------------------
+// Scalar helpers.  A forward-mode dual type provides the same overloads.
+inline double sqrt_(double x) { return std::sqrt(x); }
+inline double abs_(double x) { return std::fabs(x); }
+inline double value(double x) { return x; }
+// Julia Base.max for non-NaN floats: y if y > x, else x.
+template <class T> inline T max_(const T& x, const T& y) { return (value(y) > value(x)) ? y : x; }
+
+// ---------------------------------------------------------------------------
+// Gauss-Lobatto-Legendre basis.  Restates Trixi's LobattoLegendreBasis
+// (gauss_lobatto_nodes_weights, barycentric_weights,
+// polynomial_derivative_matrix, calc_dhat, calc_lhat), which TrixiLW reads as
+// `derivative_matrix`, `derivative_dhat`, `weights`, `inverse_weights` and
+// `boundary_interpolation` (src/solvers/dgsem_tree/dg_2d.jl:932, 1211-1226).
+// Pinned by closed forms (N=3: ±1, ±sqrt(1/5), w = 1/6, 5/6; N=4: 0,
+// ±sqrt(3/7), ±1, w = 32/45, 49/90, 1/10), exact differentiation of
+// polynomials of degree <= N, and the SBP identity W D + D^T W = B.
+struct Basis {
+    int N = 0, n = 0;
+    double nodes[MAXN] = {}, weights[MAXN] = {}, wbary[MAXN] = {}, inv_w[MAXN] = {};
+    double D[MAXN][MAXN] = {}, Dhat[MAXN][MAXN] = {};
+    double bnd_lo = 0, bnd_hi = 0;  // boundary_interpolation[1,1], [n,2]
+
+    void init(int polydeg) {
+        N = polydeg;
+        n = N + 1;
+        // Newton iteration on q = L_{N+1} - L_{N-1} in extended precision
+        // (Kopriva, "Implementing Spectral Methods", Alg. 24/25).
+        long double x[MAXN], w[MAXN];
+        auto qL = [&](long double xx, long double& q, long double& dq, long double& LN) {
+            long double Lm2 = 1, Lm1 = xx, dLm2 = 0, dLm1 = 1, Lk = 0, dLk = 0;
+            for (int k = 2; k <= N; ++k) {
+                Lk = (2.0L * k - 1) / k * xx * Lm1 - (k - 1.0L) / k * Lm2;
+                dLk = dLm2 + (2.0L * k - 1) * Lm1;
+                Lm2 = Lm1; Lm1 = Lk; dLm2 = dLm1; dLm1 = dLk;
+            }
+            if (N == 1) { Lk = xx; dLk = 1; Lm2 = 1; dLm2 = 0; }
+            const int k = N + 1;
+            long double Lk1 = (2.0L * k - 1) / k * xx * Lk - (k - 1.0L) / k * Lm2;
+            long double dLk1 = dLm2 + (2.0L * k - 1) * Lk;
+            q = Lk1 - Lm2;
+            dq = dLk1 - dLm2;
+            LN = Lk;
+        };
+        const long double pi = 3.141592653589793238462643383279502884L;
+        x[0] = -1; x[N] = 1;
+        w[0] = w[N] = 2.0L / (N * (N + 1.0L));
+        for (int j = 1; j <= (N + 1) / 2 - 1; ++j) {
+            long double xx = -cosl((j + 0.25L) * pi / N - 3.0L / (8.0L * N * pi * (j + 0.25L)));
+            for (int it = 0; it < 100; ++it) {
+                long double q, dq, LN;
+                qL(xx, q, dq, LN);
+                long double d = -q / dq;
+                xx += d;
+                if (fabsl(d) <= 1e-19L * fabsl(xx) + 1e-30L) break;
+            }
+            long double q, dq, LN;
+            qL(xx, q, dq, LN);
+            x[j] = xx;
+            x[N - j] = -xx;
+            w[j] = w[N - j] = 2.0L / (N * (N + 1.0L) * LN * LN);
+        }
+        if (N % 2 == 0) {
+            long double q, dq, LN;
+            qL(0.0L, q, dq, LN);
+            x[N / 2] = 0;
+            w[N / 2] = 2.0L / (N * (N + 1.0L) * LN * LN);
+        }
+        for (int i = 0; i < n; ++i) {
+            nodes[i] = (double)x[i];
+            weights[i] = (double)w[i];
+            inv_w[i] = 1.0 / weights[i];
+        }
+        // barycentric weights, in double as Trixi computes them
+        for (int i = 0; i < n; ++i) wbary[i] = 1.0;
+        for (int j = 1; j < n; ++j)
+            for (int k = 0; k < j; ++k) {
+                wbary[k] *= nodes[k] - nodes[j];
+                wbary[j] *= nodes[j] - nodes[k];
+            }
+        for (int j = 0; j < n; ++j) wbary[j] = 1.0 / wbary[j];
+        // polynomial_derivative_matrix: negative-sum diagonal
+        for (int i = 0; i < n; ++i) {
+            D[i][i] = 0;
+            for (int j = 0; j < n; ++j)
+                if (j != i) {
+                    D[i][j] = wbary[j] / wbary[i] * 1.0 / (nodes[i] - nodes[j]);
+                    D[i][i] -= D[i][j];
+                }
+        }
+        // calc_dhat: dhat = D', dhat[j,k] *= -w[k]/w[j]
+        for (int j = 0; j < n; ++j)
+            for (int k = 0; k < n; ++k) Dhat[j][k] = D[k][j] * (-weights[k] / weights[j]);
+        // calc_lhat at -1 and +1 (the interpolant is exact at the end nodes)
+        bnd_lo = 1.0 / weights[0];
+        bnd_hi = 1.0 / weights[n - 1];
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Equations.
+//
+// Trixi CompressibleEulerEquations2D: flux(u, orientation), cons2prim,
+// max_abs_speed_naive, max_abs_speeds.  TrixiLW calls them as
+// `fluxes(u, eq) = (flux(u,1), flux(u,2))` (src/solvers/dgsem_tree/dg_2d.jl:368)
+// and through FluxPlusDissipation (src/equations/numerical_fluxes.jl:14-38).
+// Pinned by the closed form, the x<->y mirror symmetry, free-stream
+// preservation and the isentropic-vortex convergence order.
+template <class T> struct Euler2D {
+    static constexpr int NV = 4;
+    double gamma = 1.4;
+    void fluxes(const T* u, T* f, T* g) const {
+        const T rho = u[0], m1 = u[1], m2 = u[2], E = u[3];
+        const T v1 = m1 / rho, v2 = m2 / rho;
+        const T p = (gamma - 1.0) * (E - 0.5 * (m1 * v1 + m2 * v2));
+        f[0] = m1; f[1] = m1 * v1 + p; f[2] = m1 * v2; f[3] = (E + p) * v1;
+        g[0] = m2; g[1] = m2 * v1; g[2] = m2 * v2 + p; g[3] = (E + p) * v2;
+    }
+    void flux(const T* u, int orientation, T* f) const {
+        T ff[4], gg[4];
+        fluxes(u, ff, gg);
+        for (int v = 0; v < 4; ++v) f[v] = orientation == 1 ? ff[v] : gg[v];
+    }
+    void cons2prim(const T* u, T& rho, T& v1, T& v2, T& p) const {
+        rho = u[0];
+        v1 = u[1] / rho;
+        v2 = u[2] / rho;
+        p = (gamma - 1.0) * (u[3] - 0.5 * (u[1] * v1 + u[2] * v2));
+    }
+    // |v_n| and c of one state, the two ingredients of max_abs_speed_naive
+    void speed_parts(const T* u, int orientation, T& vn_abs, T& c) const {
+        T rho, v1, v2, p;
+        cons2prim(u, rho, v1, v2, p);
+        vn_abs = abs_(orientation == 1 ? v1 : v2);
+        c = sqrt_(gamma * p / rho);
+    }
+    T max_abs_speed_naive(const T* ul, const T* ur, int orientation) const {
+        T al, cl, ar, cr;
+        speed_parts(ul, orientation, al, cl);
+        speed_parts(ur, orientation, ar, cr);
+        return max_(al, ar) + max_(cl, cr);
+    }
+    void max_abs_speeds(const T* u, T& l1, T& l2) const {
+        T rho, v1, v2, p;
+        cons2prim(u, rho, v1, v2, p);
+        const T c = sqrt_(gamma * p / rho);
+        l1 = abs_(v1) + c;
+        l2 = abs_(v2) + c;
+    }
+    // src/equations/compressible_euler_2d.jl:4-7
+    bool is_admissible(const T* u) const {
+        T rho, v1, v2, p;
+        cons2prim(u, rho, v1, v2, p);
+        return value(rho) > 0.0 && value(p) > 0.0;
+    }
+};
+
+// Trixi LinearScalarAdvectionEquation2D (flux a_d * u, constant speed).  Used
+// by the linear-case pin: with a linear flux the finite-difference time
+// derivatives are exact and one step has a closed matrix form.
+template <class T> struct LinAdv2D {
+    static constexpr int NV = 1;
+    double a1 = 1.0, a2 = 1.0;
+    void fluxes(const T* u, T* f, T* g) const { f[0] = a1 * u[0]; g[0] = a2 * u[0]; }
+    void flux(const T* u, int orientation, T* f) const { f[0] = (orientation == 1 ? a1 : a2) * u[0]; }
+    void speed_parts(const T*, int orientation, T& vn_abs, T& c) const {
+        vn_abs = T(std::fabs(orientation == 1 ? a1 : a2));
+        c = T(0.0);
+    }
+    T max_abs_speed_naive(const T*, const T*, int orientation) const {
+        return T(std::fabs(orientation == 1 ? a1 : a2));
+    }
+    void max_abs_speeds(const T*, T& l1, T& l2) const { l1 = T(std::fabs(a1)); l2 = T(std::fabs(a2)); }
+    bool is_admissible(const T*) const { return true; }
+};
+
+// ---------------------------------------------------------------------------
+// Boundary value functions for the time-averaged Dirichlet condition
+// (src/equations/equations.jl:6-37).  Same set as the HIP library.
+enum BoundaryValueKind { BV_NONE = 0, BV_VORTEX = 1, BV_CONSTANT = 2, BV_RIEMANN4 = 3 };
+
+struct BoundaryValue {
+    int kind = BV_NONE;
+    double par[24] = {};
+    // isentropic vortex: par = {eps, mach, xc0, yc0, Lx, Ly, gamma}
+    // constant: par = {rho, v1, v2, p, gamma}
+    // four-quadrant Riemann: par = {x0, y0, gamma, then (rho, v1, v2, p) for
+    //   quadrants NE, NW, SW, SE}
+    void eval(double x, double y, double t, double* u) const {
+        double rho, v1, v2, p, gamma;
+        if (kind == BV_VORTEX) {
+            const double eps = par[0], mach = par[1], Lx = par[4], Ly = par[5];
+            gamma = par[6];
+            const double vinf = mach * std::sqrt(gamma) / std::sqrt(2.0);
+            double dx = x - (par[2] + vinf * t), dy = y - (par[3] + vinf * t);
+            dx -= Lx * std::floor(dx / Lx + 0.5);
+            dy -= Ly * std::floor(dy / Ly + 0.5);
+            const double r2 = dx * dx + dy * dy;
+            const double pi = 3.141592653589793;
+            const double du = eps / (2 * pi) * std::exp(0.5 * (1 - r2));
+            const double T = 1 - (gamma - 1) * eps * eps / (8 * gamma * pi * pi) * std::exp(1 - r2);
+            rho = std::pow(T, 1 / (gamma - 1));
+            v1 = vinf - du * dy;
+            v2 = vinf + du * dx;
+            p = rho * T;
+        } else if (kind == BV_CONSTANT) {
+            rho = par[0]; v1 = par[1]; v2 = par[2]; p = par[3]; gamma = par[4];
+        } else if (kind == BV_RIEMANN4) {
+            gamma = par[2];
+            int q;
+            if (x >= par[0]) q = (y >= par[1]) ? 0 : 3;
+            else q = (y >= par[1]) ? 1 : 2;
+            rho = par[3 + 4 * q]; v1 = par[4 + 4 * q]; v2 = par[5 + 4 * q]; p = par[6 + 4 * q];
+        } else {
+            u[0] = u[1] = u[2] = u[3] = std::numeric_limits<double>::quiet_NaN();
+            return;
+        }
+        u[0] = rho;
+        u[1] = rho * v1;
+        u[2] = rho * v2;
+        u[3] = p / (gamma - 1) + 0.5 * rho * (v1 * v1 + v2 * v2);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Uniform 2-D TreeMesh (square elements), elements numbered e = ix + nx*iy.
+// Trixi numbers TreeMesh leaves in depth-first (Z) order; every routine here
+// uses the neighbour and interface tables, so the element order is a free
+// choice and parity is compared by node coordinate.
+struct Mesh {
+    int nx = 0, ny = 0, E = 0;
+    double xmin = 0, ymin = 0, h = 0;  // element edge length
+    bool per_x = true, per_y = true;
+    std::vector<double> inv_jac;       // 2/h per element (Trixi TreeMesh)
+    std::vector<int> nbr;              // [e*4 + face], -1 on a domain boundary
+    // Trixi interface container: neighbor_ids[1|2], orientations
+    std::vector<int> if_left, if_right, if_orient;
+    // Trixi boundary container, sorted by direction 1..4
+    std::vector<int> bd_elem, bd_orient, bd_side, bd_dir;
+
+    void init(int nx_, int ny_, double xmin_, double ymin_, double h_, bool px, bool py) {
+        nx = nx_; ny = ny_; E = nx * ny; xmin = xmin_; ymin = ymin_; h = h_; per_x = px; per_y = py;
+        inv_jac.assign(E, 2.0 / h);
+        nbr.assign(4 * E, -1);
+        for (int iy = 0; iy < ny; ++iy)
+            for (int ix = 0; ix < nx; ++ix) {
+                const int e = ix + nx * iy;
+                int l = ix - 1, r = ix + 1, b = iy - 1, t = iy + 1;
+                if (per_x) { l = (l + nx) % nx; r = r % nx; }
+                if (per_y) { b = (b + ny) % ny; t = t % ny; }
+                nbr[4 * e + 0] = (l >= 0 && l < nx) ? l + nx * iy : -1;
+                nbr[4 * e + 1] = (r >= 0 && r < nx) ? r + nx * iy : -1;
+                nbr[4 * e + 2] = (b >= 0 && b < ny) ? ix + nx * b : -1;
+                nbr[4 * e + 3] = (t >= 0 && t < ny) ? ix + nx * t : -1;
+            }
+        if_left.clear(); if_right.clear(); if_orient.clear();
+        for (int e = 0; e < E; ++e) {
+            if (nbr[4 * e + 1] >= 0) { if_left.push_back(e); if_right.push_back(nbr[4 * e + 1]); if_orient.push_back(1); }
+            if (nbr[4 * e + 3] >= 0) { if_left.push_back(e); if_right.push_back(nbr[4 * e + 3]); if_orient.push_back(2); }
+        }
+        bd_elem.clear(); bd_orient.clear(); bd_side.clear(); bd_dir.clear();
+        for (int dir = 1; dir <= 4; ++dir)
+            for (int e = 0; e < E; ++e)
+                if (nbr[4 * e + (dir - 1)] < 0) {
+                    bd_elem.push_back(e);
+                    bd_orient.push_back(dir <= 2 ? 1 : 2);
+                    // neighbor_sides: 1 = element left of the boundary (+ face)
+                    bd_side.push_back((dir % 2 == 0) ? 1 : 2);
+                    bd_dir.push_back(dir);
+                }
+    }
+    double node_x(const Basis& b, int i, int e) const { return xmin + h * ((e % nx) + 0.5) + 0.5 * h * b.nodes[i]; }
+    double node_y(const Basis& b, int j, int e) const { return ymin + h * ((e / nx) + 0.5) + 0.5 * h * b.nodes[j]; }
+};
+
+// ---------------------------------------------------------------------------
+// Solver state (the subset of `semi.cache` on the LW path,
+// src/solvers/dgsem_tree/containers.jl:7-165).
+template <class T, class Eq> struct Solver {
+    static constexpr int NV = Eq::NV;
+    Basis basis;
+    Mesh mesh;
+    Eq eq;
+    BoundaryValue bval;
+    int n = 0;
+    // 0: TreeMesh order (λ from Ũ, jump on uⁿ; dg_2d.jl:1928)
+    // 1: Structured/P4est order (λ from uⁿ, jump on Ũ; structured/dg_2d.jl:2271)
+    int flux_order = 0;
+    double abstol = 0.0, reltol = 0.0;
+    std::vector<T> U, F, sfv, temporal_errors;
+    std::vector<T> us, U2, F2;  // MDRK stage cache (containers.jl:83-98)
+
+    void setup() {
+        n = basis.n;
+        const size_t nd = (size_t)mesh.E * n * n * NV;
+        U.assign(nd, T(0));
+        F.assign(2 * nd, T(0));
+        sfv.assign((size_t)mesh.E * 4 * n * NV, T(0));
+        temporal_errors.assign(mesh.E, T(0));
+    }
+    // the MDRK stage cache is allocated on first use
+    void ensure_mdrk() {
+        const size_t nd = ndofs_total();
+        if (us.size() != nd) us.assign(nd, T(0));
+        if (U2.size() != nd) U2.assign(nd, T(0));
+        if (F2.size() != 2 * nd) F2.assign(2 * nd, T(0));
+    }
+    size_t ndofs_total() const { return (size_t)mesh.E * n * n * NV; }
+    size_t idx(int v, int i, int j, int e) const { return (((size_t)e * n + j) * n + i) * NV + v; }
+    size_t fidx(int v, int d, int i, int j, int e) const { return ((((size_t)e * n + j) * n + i) * 2 + d) * NV + v; }
+    size_t sidx(int v, int l, int face, int e) const { return (((size_t)e * 4 + face) * n + l) * NV + v; }
+
+    // ---------------- LW volume kernel, N = 4 ----------------
+    // lw_volume_kernel_4!, src/solvers/dgsem_tree/dg_2d.jl:918-1267.
+    // Source terms are `nothing` on this path (the calc_source* methods at
+    // dg_2d.jl:92-176 return zero), so their zero contributions are omitted.
+    void lw_volume_kernel_4(T* du, const T* u, double dt, int e, double alpha) {
+        const auto& D = basis.D;
+        const auto& Dh = basis.Dhat;
+        const double ij = mesh.inv_jac[e];
+        T f[MAXN][MAXN][NV], g[MAXN][MAXN][NV], Fc[MAXN][MAXN][NV], Gc[MAXN][MAXN][NV];
+        T ut[MAXN][MAXN][NV], utt[MAXN][MAXN][NV], uttt[MAXN][MAXN][NV], utttt[MAXN][MAXN][NV];
+        T Uc[MAXN][MAXN][NV], up[MAXN][MAXN][NV], um[MAXN][MAXN][NV], upp[MAXN][MAXN][NV], umm[MAXN][MAXN][NV];
+        T unp1[MAXN][MAXN][NV], unp1low[MAXN][MAXN][NV];
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i)
+                for (int v = 0; v < NV; ++v) ut[j][i][v] = utt[j][i][v] = uttt[j][i][v] = utttt[j][i][v] = T(0);
+        // 953-987: fluxes at uⁿ and ut = -dt*J⁻¹*(D f + D g)
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) {
+                const T* un = &u[idx(0, i, j, e)];
+                T fl[NV], gl[NV];
+                eq.fluxes(un, fl, gl);
+                for (int v = 0; v < NV; ++v) { f[j][i][v] = fl[v]; g[j][i][v] = gl[v]; Fc[j][i][v] = fl[v]; }
+                for (int ii = 0; ii < n; ++ii)
+                    for (int v = 0; v < NV; ++v) ut[j][ii][v] = ut[j][ii][v] + (-dt * D[ii][i]) * fl[v];
+                for (int v = 0; v < NV; ++v) Gc[j][i][v] = gl[v];
+                for (int jj = 0; jj < n; ++jj)
+                    for (int v = 0; v < NV; ++v) ut[jj][i][v] = ut[jj][i][v] + (-dt * D[jj][j]) * gl[v];
+                for (int v = 0; v < NV; ++v)
+                    unp1[j][i][v] = unp1low[j][i][v] = um[j][i][v] = up[j][i][v] = umm[j][i][v] = upp[j][i][v] =
+                        Uc[j][i][v] = un[v];
+            }
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i)
+                for (int v = 0; v < NV; ++v) ut[j][i][v] = ut[j][i][v] * ij;
+        // 995-1022: first Taylor states, f_t, utt
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) {
+                for (int v = 0; v < NV; ++v) {
+                    const T a = ut[j][i][v];
+                    Uc[j][i][v] = Uc[j][i][v] + 0.5 * a;
+                    up[j][i][v] = up[j][i][v] + 1.0 * a;
+                    um[j][i][v] = um[j][i][v] + (-1.0) * a;
+                    umm[j][i][v] = umm[j][i][v] + (-2.0) * a;
+                    upp[j][i][v] = upp[j][i][v] + 2.0 * a;
+                }
+                T fm[NV], gm[NV], fp[NV], gp[NV], fmm[NV], gmm[NV], fpp[NV], gpp[NV];
+                eq.fluxes(um[j][i], fm, gm);
+                eq.fluxes(up[j][i], fp, gp);
+                eq.fluxes(umm[j][i], fmm, gmm);
+                eq.fluxes(upp[j][i], fpp, gpp);
+                T ft[NV], gt[NV];
+                for (int v = 0; v < NV; ++v) {
+                    ft[v] = 1.0 / 12.0 * (-fpp[v] + 8.0 * fp[v] - 8.0 * fm[v] + fmm[v]);
+                    gt[v] = 1.0 / 12.0 * (-gpp[v] + 8.0 * gp[v] - 8.0 * gm[v] + gmm[v]);
+                    Fc[j][i][v] = Fc[j][i][v] + 0.5 * ft[v];
+                    Gc[j][i][v] = Gc[j][i][v] + 0.5 * gt[v];
+                }
+                for (int ii = 0; ii < n; ++ii)
+                    for (int v = 0; v < NV; ++v) utt[j][ii][v] = utt[j][ii][v] + (-dt * D[ii][i]) * ft[v];
+                for (int jj = 0; jj < n; ++jj)
+                    for (int v = 0; v < NV; ++v) utt[jj][i][v] = utt[jj][i][v] + (-dt * D[jj][j]) * gt[v];
+            }
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i)
+                for (int v = 0; v < NV; ++v) utt[j][i][v] = utt[j][i][v] * ij;
+        // 1047-1080: f_tt, uttt
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) {
+                for (int v = 0; v < NV; ++v) {
+                    const T a = utt[j][i][v];
+                    Uc[j][i][v] = Uc[j][i][v] + (1.0 / 6.0) * a;
+                    up[j][i][v] = up[j][i][v] + 0.5 * a;
+                    um[j][i][v] = um[j][i][v] + 0.5 * a;
+                    upp[j][i][v] = upp[j][i][v] + 2.0 * a;
+                    umm[j][i][v] = umm[j][i][v] + 2.0 * a;
+                }
+                T fm[NV], gm[NV], fp[NV], gp[NV], fmm[NV], gmm[NV], fpp[NV], gpp[NV];
+                eq.fluxes(um[j][i], fm, gm);
+                eq.fluxes(up[j][i], fp, gp);
+                eq.fluxes(umm[j][i], fmm, gmm);
+                eq.fluxes(upp[j][i], fpp, gpp);
+                T ftt[NV], gtt[NV];
+                for (int v = 0; v < NV; ++v) {
+                    ftt[v] = (1.0 / 12.0) * (-fpp[v] + 16.0 * fp[v] - 30.0 * f[j][i][v] + 16.0 * fm[v] - fmm[v]);
+                    gtt[v] = (1.0 / 12.0) * (-gpp[v] + 16.0 * gp[v] - 30.0 * g[j][i][v] + 16.0 * gm[v] - gmm[v]);
+                    Fc[j][i][v] = Fc[j][i][v] + (1.0 / 6.0) * ftt[v];
+                    Gc[j][i][v] = Gc[j][i][v] + (1.0 / 6.0) * gtt[v];
+                }
+                for (int ii = 0; ii < n; ++ii)
+                    for (int v = 0; v < NV; ++v) uttt[j][ii][v] = uttt[j][ii][v] + (-dt * D[ii][i]) * ftt[v];
+                for (int jj = 0; jj < n; ++jj)
+                    for (int v = 0; v < NV; ++v) uttt[jj][i][v] = uttt[jj][i][v] + (-dt * D[jj][j]) * gtt[v];
+            }
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i)
+                for (int v = 0; v < NV; ++v) uttt[j][i][v] = uttt[j][i][v] * ij;
+        // 1103-1136: f_ttt, utttt
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) {
+                for (int v = 0; v < NV; ++v) {
+                    const T a = uttt[j][i][v];
+                    Uc[j][i][v] = Uc[j][i][v] + (1.0 / 24.0) * a;
+                    um[j][i][v] = um[j][i][v] + (-1.0 / 6.0) * a;
+                    up[j][i][v] = up[j][i][v] + (1.0 / 6.0) * a;
+                    umm[j][i][v] = umm[j][i][v] + (-4.0 / 3.0) * a;
+                    upp[j][i][v] = upp[j][i][v] + (4.0 / 3.0) * a;
+                }
+                T fm[NV], gm[NV], fp[NV], gp[NV], fmm[NV], gmm[NV], fpp[NV], gpp[NV];
+                eq.fluxes(um[j][i], fm, gm);
+                eq.fluxes(up[j][i], fp, gp);
+                eq.fluxes(umm[j][i], fmm, gmm);
+                eq.fluxes(upp[j][i], fpp, gpp);
+                T fttt[NV], gttt[NV];
+                for (int v = 0; v < NV; ++v) {
+                    fttt[v] = 0.5 * (fpp[v] - 2.0 * fp[v] + 2.0 * fm[v] - fmm[v]);
+                    Fc[j][i][v] = Fc[j][i][v] + (1.0 / 24.0) * fttt[v];
+                    gttt[v] = 0.5 * (gpp[v] - 2.0 * gp[v] + 2.0 * gm[v] - gmm[v]);
+                    Gc[j][i][v] = Gc[j][i][v] + (1.0 / 24.0) * gttt[v];
+                }
+                for (int ii = 0; ii < n; ++ii)
+                    for (int v = 0; v < NV; ++v) utttt[j][ii][v] = utttt[j][ii][v] + (-dt * D[ii][i]) * fttt[v];
+                for (int jj = 0; jj < n; ++jj)
+                    for (int v = 0; v < NV; ++v) utttt[jj][i][v] = utttt[jj][i][v] + (-dt * D[jj][j]) * gttt[v];
+            }
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i)
+                for (int v = 0; v < NV; ++v) utttt[j][i][v] = utttt[j][i][v] * ij;
+        // 1160-1249: f_tttt (with the reference's 0.5 factor), F̃, volume term
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) {
+                for (int v = 0; v < NV; ++v) {
+                    const T a = utttt[j][i][v];
+                    Uc[j][i][v] = Uc[j][i][v] + (1.0 / 120.0) * a;
+                    um[j][i][v] = um[j][i][v] + (1.0 / 24.0) * a;
+                    up[j][i][v] = up[j][i][v] + (1.0 / 24.0) * a;
+                    umm[j][i][v] = umm[j][i][v] + (2.0 / 3.0) * a;
+                    upp[j][i][v] = upp[j][i][v] + (2.0 / 3.0) * a;
+                }
+                T fm[NV], gm[NV], fp[NV], gp[NV], fmm[NV], gmm[NV], fpp[NV], gpp[NV];
+                eq.fluxes(um[j][i], fm, gm);
+                eq.fluxes(up[j][i], fp, gp);
+                eq.fluxes(umm[j][i], fmm, gmm);
+                eq.fluxes(upp[j][i], fpp, gpp);
+                // u_np1_low uses F̃ before the f_tttt term (1183-1199)
+                for (int ii = 0; ii < n; ++ii)
+                    for (int v = 0; v < NV; ++v)
+                        unp1low[j][ii][v] = unp1low[j][ii][v] + (-dt * ij * D[ii][i]) * Fc[j][i][v];
+                for (int jj = 0; jj < n; ++jj)
+                    for (int v = 0; v < NV; ++v)
+                        unp1low[jj][i][v] = unp1low[jj][i][v] + (-dt * ij * D[jj][j]) * Gc[j][i][v];
+                for (int v = 0; v < NV; ++v) {
+                    const T ftttt = 0.5 * (fpp[v] - 4.0 * fp[v] + 6.0 * f[j][i][v] - 4.0 * fm[v] + fmm[v]);
+                    const T gtttt = 0.5 * (gpp[v] - 4.0 * gp[v] + 6.0 * g[j][i][v] - 4.0 * gm[v] + gmm[v]);
+                    Fc[j][i][v] = Fc[j][i][v] + (1.0 / 120.0) * ftttt;
+                    Gc[j][i][v] = Gc[j][i][v] + (1.0 / 120.0) * gtttt;
+                }
+                for (int ii = 0; ii < n; ++ii)
+                    for (int v = 0; v < NV; ++v) {
+                        du[idx(v, ii, j, e)] = du[idx(v, ii, j, e)] + (alpha * Dh[ii][i]) * Fc[j][i][v];
+                        unp1[j][ii][v] = unp1[j][ii][v] + (-dt * ij * D[ii][i]) * Fc[j][i][v];
+                    }
+                for (int jj = 0; jj < n; ++jj)
+                    for (int v = 0; v < NV; ++v) {
+                        du[idx(v, i, jj, e)] = du[idx(v, i, jj, e)] + (alpha * Dh[jj][j]) * Gc[j][i][v];
+                        unp1[jj][i][v] = unp1[jj][i][v] + (-dt * ij * D[jj][j]) * Gc[j][i][v];
+                    }
+                for (int v = 0; v < NV; ++v) {
+                    F[fidx(v, 0, i, j, e)] = Fc[j][i][v];
+                    F[fidx(v, 1, i, j, e)] = Gc[j][i][v];
+                    U[idx(v, i, j, e)] = Uc[j][i][v];
+                }
+            }
+        // 1251-1265: temporal error estimate for the adaptive controller
+        T err = T(0);
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i)
+                for (int v = 0; v < NV; ++v) {
+                    const T d = unp1[j][i][v] - unp1low[j][i][v];
+                    const T s = abstol + reltol * max_(abs_(unp1[j][i][v]), abs_(unp1low[j][i][v]));
+                    const T r = d / s;
+                    err = err + r * r;
+                }
+        temporal_errors[e] = err;
+    }
+
+    // ---------------- LW volume kernel, N = 3 ----------------
+    // lw_volume_kernel_3!, src/solvers/dgsem_tree/dg_2d.jl:674-916
+    void lw_volume_kernel_3(T* du, const T* u, double dt, int e, double alpha) {
+        const auto& D = basis.D;
+        const auto& Dh = basis.Dhat;
+        const double ij = mesh.inv_jac[e];
+        T f[MAXN][MAXN][NV], g[MAXN][MAXN][NV], Fc[MAXN][MAXN][NV], Gc[MAXN][MAXN][NV];
+        T ut[MAXN][MAXN][NV], utt[MAXN][MAXN][NV], uttt[MAXN][MAXN][NV];
+        T Uc[MAXN][MAXN][NV], up[MAXN][MAXN][NV], um[MAXN][MAXN][NV], upp[MAXN][MAXN][NV], umm[MAXN][MAXN][NV];
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i)
+                for (int v = 0; v < NV; ++v) ut[j][i][v] = utt[j][i][v] = uttt[j][i][v] = T(0);
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) {
+                const T* un = &u[idx(0, i, j, e)];
+                T fl[NV], gl[NV];
+                eq.fluxes(un, fl, gl);
+                for (int v = 0; v < NV; ++v) { f[j][i][v] = fl[v]; g[j][i][v] = gl[v]; Fc[j][i][v] = fl[v]; }
+                for (int ii = 0; ii < n; ++ii)
+                    for (int v = 0; v < NV; ++v) ut[j][ii][v] = ut[j][ii][v] + (-dt * D[ii][i]) * fl[v];
+                for (int v = 0; v < NV; ++v) Gc[j][i][v] = gl[v];
+                for (int jj = 0; jj < n; ++jj)
+                    for (int v = 0; v < NV; ++v) ut[jj][i][v] = ut[jj][i][v] + (-dt * D[jj][j]) * gl[v];
+                for (int v = 0; v < NV; ++v)
+                    um[j][i][v] = up[j][i][v] = umm[j][i][v] = upp[j][i][v] = Uc[j][i][v] = un[v];
+            }
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i)
+                for (int v = 0; v < NV; ++v) ut[j][i][v] = ut[j][i][v] * ij;
+        // 740-769
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) {
+                for (int v = 0; v < NV; ++v) {
+                    const T a = ut[j][i][v];
+                    Uc[j][i][v] = Uc[j][i][v] + 0.5 * a;
+                    up[j][i][v] = up[j][i][v] + 1.0 * a;
+                    um[j][i][v] = um[j][i][v] + (-1.0) * a;
+                    umm[j][i][v] = umm[j][i][v] + (-2.0) * a;
+                    upp[j][i][v] = upp[j][i][v] + 2.0 * a;
+                }
+                T fm[NV], gm[NV], fp[NV], gp[NV], fmm[NV], gmm[NV], fpp[NV], gpp[NV];
+                eq.fluxes(um[j][i], fm, gm);
+                eq.fluxes(up[j][i], fp, gp);
+                eq.fluxes(umm[j][i], fmm, gmm);
+                eq.fluxes(upp[j][i], fpp, gpp);
+                T ft[NV], gt[NV];
+                for (int v = 0; v < NV; ++v) {
+                    ft[v] = 1.0 / 12.0 * (-fpp[v] + 8.0 * fp[v] - 8.0 * fm[v] + fmm[v]);
+                    gt[v] = 1.0 / 12.0 * (-gpp[v] + 8.0 * gp[v] - 8.0 * gm[v] + gmm[v]);
+                    Fc[j][i][v] = Fc[j][i][v] + 0.5 * ft[v];
+                    Gc[j][i][v] = Gc[j][i][v] + 0.5 * gt[v];
+                }
+                for (int ii = 0; ii < n; ++ii)
+                    for (int v = 0; v < NV; ++v) utt[j][ii][v] = utt[j][ii][v] + (-dt * D[ii][i]) * ft[v];
+                for (int jj = 0; jj < n; ++jj)
+                    for (int v = 0; v < NV; ++v) utt[jj][i][v] = utt[jj][i][v] + (-dt * D[jj][j]) * gt[v];
+            }
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i)
+                for (int v = 0; v < NV; ++v) utt[j][i][v] = utt[j][i][v] * ij;
+        // 799-827: f_tt = fp - 2f + fm (two-point second difference)
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) {
+                for (int v = 0; v < NV; ++v) {
+                    const T a = utt[j][i][v];
+                    Uc[j][i][v] = Uc[j][i][v] + (1.0 / 6.0) * a;
+                    up[j][i][v] = up[j][i][v] + 0.5 * a;
+                    um[j][i][v] = um[j][i][v] + 0.5 * a;
+                    upp[j][i][v] = upp[j][i][v] + 2.0 * a;
+                    umm[j][i][v] = umm[j][i][v] + 2.0 * a;
+                }
+                T fm[NV], gm[NV], fp[NV], gp[NV];
+                eq.fluxes(um[j][i], fm, gm);
+                eq.fluxes(up[j][i], fp, gp);
+                T ftt[NV], gtt[NV];
+                for (int v = 0; v < NV; ++v) {
+                    ftt[v] = fp[v] - 2.0 * f[j][i][v] + fm[v];
+                    gtt[v] = gp[v] - 2.0 * g[j][i][v] + gm[v];
+                    Fc[j][i][v] = Fc[j][i][v] + (1.0 / 6.0) * ftt[v];
+                    Gc[j][i][v] = Gc[j][i][v] + (1.0 / 6.0) * gtt[v];
+                }
+                for (int ii = 0; ii < n; ++ii)
+                    for (int v = 0; v < NV; ++v) uttt[j][ii][v] = uttt[j][ii][v] + (-dt * D[ii][i]) * ftt[v];
+                for (int jj = 0; jj < n; ++jj)
+                    for (int v = 0; v < NV; ++v) uttt[jj][i][v] = uttt[jj][i][v] + (-dt * D[jj][j]) * gtt[v];
+            }
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i)
+                for (int v = 0; v < NV; ++v) uttt[j][i][v] = uttt[j][i][v] * ij;
+        // 853-913: f_ttt, F̃, volume term
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) {
+                for (int v = 0; v < NV; ++v) {
+                    const T a = uttt[j][i][v];
+                    Uc[j][i][v] = Uc[j][i][v] + (1.0 / 24.0) * a;
+                    um[j][i][v] = um[j][i][v] + (-1.0 / 6.0) * a;
+                    up[j][i][v] = up[j][i][v] + (1.0 / 6.0) * a;
+                    umm[j][i][v] = umm[j][i][v] + (-4.0 / 3.0) * a;
+                    upp[j][i][v] = upp[j][i][v] + (4.0 / 3.0) * a;
+                }
+                T fm[NV], gm[NV], fp[NV], gp[NV], fmm[NV], gmm[NV], fpp[NV], gpp[NV];
+                eq.fluxes(um[j][i], fm, gm);
+                eq.fluxes(up[j][i], fp, gp);
+                eq.fluxes(umm[j][i], fmm, gmm);
+                eq.fluxes(upp[j][i], fpp, gpp);
+                for (int v = 0; v < NV; ++v) {
+                    const T fttt = 0.5 * (fpp[v] - 2.0 * fp[v] + 2.0 * fm[v] - fmm[v]);
+                    Fc[j][i][v] = Fc[j][i][v] + (1.0 / 24.0) * fttt;
+                    const T gttt = 0.5 * (gpp[v] - 2.0 * gp[v] + 2.0 * gm[v] - gmm[v]);
+                    Gc[j][i][v] = Gc[j][i][v] + (1.0 / 24.0) * gttt;
+                }
+                for (int ii = 0; ii < n; ++ii)
+                    for (int v = 0; v < NV; ++v)
+                        du[idx(v, ii, j, e)] = du[idx(v, ii, j, e)] + (alpha * Dh[ii][i]) * Fc[j][i][v];
+                for (int jj = 0; jj < n; ++jj)
+                    for (int v = 0; v < NV; ++v)
+                        du[idx(v, i, jj, e)] = du[idx(v, i, jj, e)] + (alpha * Dh[jj][j]) * Gc[j][i][v];
+                for (int v = 0; v < NV; ++v) {
+                    U[idx(v, i, j, e)] = Uc[j][i][v];
+                    F[fidx(v, 0, i, j, e)] = Fc[j][i][v];
+                    F[fidx(v, 1, i, j, e)] = Gc[j][i][v];
+                }
+            }
+    }
+
+    // ---------------- MDRK stage kernels (TreeMesh) ----------------
+    // mdrk_kernel_1!, src/solvers/dgsem_tree/multi_derivative_2d.jl:263-420.
+    // The reference kernel also writes `mdrk_cache.u_low`, a field that
+    // MDRKElementCache does not have (containers.jl:83-98), so the reference
+    // raises there; that low-order update feeds only the admissibility
+    // machinery and is left out.  du, Ũ, F̃, U2, F2 follow the kernel.
+    void mdrk_kernel_1(T* du, const T* u, double dt, int e, double alpha) {
+        const auto& D = basis.D;
+        const auto& Dh = basis.Dhat;
+        const double ij = mesh.inv_jac[e];
+        T Fc[MAXN][MAXN][NV], F2c[MAXN][MAXN][NV], Gc[MAXN][MAXN][NV], G2c[MAXN][MAXN][NV];
+        T ut[MAXN][MAXN][NV], Uc[MAXN][MAXN][NV], U2c[MAXN][MAXN][NV];
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i)
+                for (int v = 0; v < NV; ++v) ut[j][i][v] = T(0);
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) {
+                const T* un = &u[idx(0, i, j, e)];
+                T fl[NV], gl[NV];
+                eq.fluxes(un, fl, gl);
+                for (int v = 0; v < NV; ++v) { Fc[j][i][v] = 0.5 * fl[v]; F2c[j][i][v] = fl[v]; }
+                for (int ii = 0; ii < n; ++ii)
+                    for (int v = 0; v < NV; ++v) ut[j][ii][v] = ut[j][ii][v] + (-dt * D[ii][i]) * fl[v];
+                for (int v = 0; v < NV; ++v) { Gc[j][i][v] = 0.5 * gl[v]; G2c[j][i][v] = gl[v]; }
+                for (int jj = 0; jj < n; ++jj)
+                    for (int v = 0; v < NV; ++v) ut[jj][i][v] = ut[jj][i][v] + (-dt * D[jj][j]) * gl[v];
+                for (int v = 0; v < NV; ++v) { Uc[j][i][v] = 0.5 * un[v]; U2c[j][i][v] = un[v]; }
+            }
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i)
+                for (int v = 0; v < NV; ++v) ut[j][i][v] = ut[j][i][v] * ij;
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) {
+                const T* un = &u[idx(0, i, j, e)];
+                T um[NV], up[NV], umm[NV], upp[NV];
+                for (int v = 0; v < NV; ++v) {
+                    const T a = ut[j][i][v];
+                    um[v] = un[v] - a;
+                    up[v] = un[v] + a;
+                    umm[v] = un[v] - 2.0 * a;
+                    upp[v] = un[v] + 2.0 * a;
+                }
+                T fm[NV], gm[NV], fp[NV], gp[NV], fmm[NV], gmm[NV], fpp[NV], gpp[NV];
+                eq.fluxes(um, fm, gm);
+                eq.fluxes(up, fp, gp);
+                eq.fluxes(umm, fmm, gmm);
+                eq.fluxes(upp, fpp, gpp);
+                for (int v = 0; v < NV; ++v) {
+                    const T ft = 1.0 / 12.0 * (-fpp[v] + 8.0 * fp[v] - 8.0 * fm[v] + fmm[v]);
+                    const T gt = 1.0 / 12.0 * (-gpp[v] + 8.0 * gp[v] - 8.0 * gm[v] + gmm[v]);
+                    Fc[j][i][v] = Fc[j][i][v] + 0.125 * ft;
+                    Gc[j][i][v] = Gc[j][i][v] + 0.125 * gt;
+                    Uc[j][i][v] = Uc[j][i][v] + 0.125 * ut[j][i][v];
+                    F2c[j][i][v] = F2c[j][i][v] + (1.0 / 6.0) * ft;
+                    G2c[j][i][v] = G2c[j][i][v] + (1.0 / 6.0) * gt;
+                    U2c[j][i][v] = U2c[j][i][v] + (1.0 / 6.0) * ut[j][i][v];
+                }
+                for (int ii = 0; ii < n; ++ii)
+                    for (int v = 0; v < NV; ++v)
+                        du[idx(v, ii, j, e)] = du[idx(v, ii, j, e)] + (alpha * Dh[ii][i]) * Fc[j][i][v];
+                for (int jj = 0; jj < n; ++jj)
+                    for (int v = 0; v < NV; ++v)
+                        du[idx(v, i, jj, e)] = du[idx(v, i, jj, e)] + (alpha * Dh[jj][j]) * Gc[j][i][v];
+                for (int v = 0; v < NV; ++v) {
+                    U[idx(v, i, j, e)] = Uc[j][i][v];
+                    F[fidx(v, 0, i, j, e)] = Fc[j][i][v];
+                    F[fidx(v, 1, i, j, e)] = Gc[j][i][v];
+                    U2[idx(v, i, j, e)] = U2c[j][i][v];
+                    F2[fidx(v, 0, i, j, e)] = F2c[j][i][v];
+                    F2[fidx(v, 1, i, j, e)] = G2c[j][i][v];
+                }
+            }
+    }
+
+    // mdrk_kernel_2!, src/solvers/dgsem_tree/multi_derivative_2d.jl:423-555
+    void mdrk_kernel_2(T* du, double dt, int e, double alpha) {
+        const auto& D = basis.D;
+        const auto& Dh = basis.Dhat;
+        const double ij = mesh.inv_jac[e];
+        T Fc[MAXN][MAXN][NV], Gc[MAXN][MAXN][NV], ust[MAXN][MAXN][NV], Uc[MAXN][MAXN][NV];
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i)
+                for (int v = 0; v < NV; ++v) {
+                    Fc[j][i][v] = F2[fidx(v, 0, i, j, e)];
+                    Gc[j][i][v] = F2[fidx(v, 1, i, j, e)];
+                    Uc[j][i][v] = U2[idx(v, i, j, e)];
+                    ust[j][i][v] = T(0);
+                }
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) {
+                const T* usn = &us[idx(0, i, j, e)];
+                T fl[NV], gl[NV];
+                eq.fluxes(usn, fl, gl);
+                for (int ii = 0; ii < n; ++ii)
+                    for (int v = 0; v < NV; ++v) ust[j][ii][v] = ust[j][ii][v] + (-dt * D[ii][i]) * fl[v];
+                for (int jj = 0; jj < n; ++jj)
+                    for (int v = 0; v < NV; ++v) ust[jj][i][v] = ust[jj][i][v] + (-dt * D[jj][j]) * gl[v];
+            }
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i)
+                for (int v = 0; v < NV; ++v) ust[j][i][v] = ust[j][i][v] * ij;
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) {
+                const T* usn = &us[idx(0, i, j, e)];
+                T um[NV], up[NV], umm[NV], upp[NV];
+                for (int v = 0; v < NV; ++v) {
+                    const T a = ust[j][i][v];
+                    um[v] = usn[v] - a;
+                    up[v] = usn[v] + a;
+                    umm[v] = usn[v] - 2.0 * a;
+                    upp[v] = usn[v] + 2.0 * a;
+                }
+                T fm[NV], gm[NV], fp[NV], gp[NV], fmm[NV], gmm[NV], fpp[NV], gpp[NV];
+                eq.fluxes(um, fm, gm);
+                eq.fluxes(up, fp, gp);
+                eq.fluxes(umm, fmm, gmm);
+                eq.fluxes(upp, fpp, gpp);
+                for (int v = 0; v < NV; ++v) {
+                    const T ft = 1.0 / 12.0 * (-fpp[v] + 8.0 * fp[v] - 8.0 * fm[v] + fmm[v]);
+                    const T gt = 1.0 / 12.0 * (-gpp[v] + 8.0 * gp[v] - 8.0 * gm[v] + gmm[v]);
+                    Fc[j][i][v] = Fc[j][i][v] + (1.0 / 3.0) * ft;
+                    Gc[j][i][v] = Gc[j][i][v] + (1.0 / 3.0) * gt;
+                    Uc[j][i][v] = Uc[j][i][v] + (1.0 / 3.0) * ust[j][i][v];
+                }
+                for (int ii = 0; ii < n; ++ii)
+                    for (int v = 0; v < NV; ++v)
+                        du[idx(v, ii, j, e)] = du[idx(v, ii, j, e)] + (alpha * Dh[ii][i]) * Fc[j][i][v];
+                for (int jj = 0; jj < n; ++jj)
+                    for (int v = 0; v < NV; ++v)
+                        du[idx(v, i, jj, e)] = du[idx(v, i, jj, e)] + (alpha * Dh[jj][j]) * Gc[j][i][v];
+                for (int v = 0; v < NV; ++v) {
+                    U[idx(v, i, j, e)] = Uc[j][i][v];
+                    F[fidx(v, 0, i, j, e)] = Fc[j][i][v];
+                    F[fidx(v, 1, i, j, e)] = Gc[j][i][v];
+                }
+            }
+    }
+
+    // ---------------- interface flux ----------------
+    // The 8-argument FluxPlusDissipation with flux_central and the 6-argument
+    // DissipationLocalLaxFriedrichs (src/equations/numerical_fluxes.jl:7-38):
+    //   F* = 0.5*(f_ll + f_rr) + (-0.5*λ)*(q_rr - q_ll).
+    // TreeMesh passes (F̃, F̃, Ũ, Ũ, uⁿ, uⁿ), so λ = max_abs_speed_naive(Ũ)
+    // and the jump is on uⁿ (dg_2d.jl:1928); the Structured/P4est order is
+    // (F̃n, F̃n, uⁿ, uⁿ, Ũ, Ũ) (structured/dg_2d.jl:2271-2274).
+    void lw_numflux(const T* fl, const T* fr, const T* Ul, const T* Ur, const T* ul, const T* ur, int orientation,
+                    T* out) const {
+        const T* sl = flux_order == 0 ? Ul : ul;  // state for λ
+        const T* sr = flux_order == 0 ? Ur : ur;
+        const T* jl = flux_order == 0 ? ul : Ul;  // state for the jump
+        const T* jr = flux_order == 0 ? ur : Ur;
+        const T lam = eq.max_abs_speed_naive(sl, sr, orientation);
+        for (int v = 0; v < NV; ++v) out[v] = 0.5 * (fl[v] + fr[v]) + (-0.5 * lam) * (jr[v] - jl[v]);
+    }
+
+    // prolong2interfaces! + calc_interface_flux! for TreeMesh and plain FR
+    // (dg_2d.jl:1269-1326, 1903-1946).  compute_alp returns 0 for
+    // VolumeIntegralFR (dg_2d.jl:1873-1877), so Fn = F*; the low-order
+    // Rusanov flux `fn` it also evaluates is multiplied by zero.
+    void calc_interface_flux(const T* u) {
+        const int nI = (int)mesh.if_left.size();
+#pragma omp parallel for schedule(static)
+        for (int I = 0; I < nI; ++I) {
+            const int L = mesh.if_left[I], R = mesh.if_right[I], o = mesh.if_orient[I];
+            const int ldir = 2 * o - 1, rdir = 2 * o - 2;  // 0-based: +face of L, -face of R
+            for (int l = 0; l < n; ++l) {
+                int li, lj, ri, rj;
+                if (o == 1) { li = n - 1; lj = l; ri = 0; rj = l; }
+                else { li = l; lj = n - 1; ri = l; rj = 0; }
+                const T* ul = &u[idx(0, li, lj, L)];
+                const T* ur = &u[idx(0, ri, rj, R)];
+                const T* Ul = &U[idx(0, li, lj, L)];
+                const T* Ur = &U[idx(0, ri, rj, R)];
+                const T* fl = &F[fidx(0, o - 1, li, lj, L)];
+                const T* fr = &F[fidx(0, o - 1, ri, rj, R)];
+                T Fn[NV];
+                lw_numflux(fl, fr, Ul, Ur, ul, ur, o, Fn);
+                for (int v = 0; v < NV; ++v) {
+                    sfv[sidx(v, l, ldir, L)] = Fn[v];
+                    sfv[sidx(v, l, rdir, R)] = Fn[v];
+                }
+            }
+        }
+    }
+
+    // prolong2boundaries! + calc_boundary_flux! with the LW time-averaged
+    // Dirichlet condition (dg_2d.jl:1328-1454; equations.jl:6-37).
+    // `scaling_factor` is 0.5 in MDRK stage 1 (tree/multi_derivative_2d.jl:34).
+    void calc_boundary_flux(const T* u, double t, double dt, double scaling_factor) {
+        const int nB = (int)mesh.bd_elem.size();
+        const auto& nodes = basis.nodes;
+        const auto& w = basis.weights;
+#pragma omp parallel for schedule(static)
+        for (int B = 0; B < nB; ++B) {
+            const int e = mesh.bd_elem[B], o = mesh.bd_orient[B], side = mesh.bd_side[B], dir = mesh.bd_dir[B];
+            for (int l = 0; l < n; ++l) {
+                int ii, jj;
+                if (o == 1) { ii = (side == 1) ? n - 1 : 0; jj = l; }
+                else { ii = l; jj = (side == 1) ? n - 1 : 0; }
+                const T* u_in = &u[idx(0, ii, jj, e)];
+                const T* U_in = &U[idx(0, ii, jj, e)];
+                const T* F_in = &F[fidx(0, o - 1, ii, jj, e)];
+                const double x = mesh.node_x(basis, ii, e), y = mesh.node_y(basis, jj, e);
+                T Uo[NV], Fo[NV], uo[NV];
+                for (int v = 0; v < NV; ++v) { Uo[v] = T(0); Fo[v] = T(0); }
+                const double dts = scaling_factor * dt;
+                for (int q = 0; q < n; ++q) {
+                    const double ts = t + 0.5 * dts * (nodes[q] + 1.0);
+                    double ubd[4];
+                    bval.eval(x, y, ts, ubd);
+                    T ub[NV], fb[NV];
+                    for (int v = 0; v < NV; ++v) ub[v] = T(ubd[v]);
+                    eq.flux(ub, o, fb);
+                    for (int v = 0; v < NV; ++v) {
+                        Uo[v] = Uo[v] + 0.5 * scaling_factor * ub[v] * w[q];
+                        Fo[v] = Fo[v] + 0.5 * scaling_factor * fb[v] * w[q];
+                    }
+                }
+                double uod[4];
+                bval.eval(x, y, t, uod);
+                for (int v = 0; v < NV; ++v) uo[v] = T(uod[v]);
+                T Fn[NV];
+                if (dir % 2 == 0) lw_numflux(F_in, Fo, U_in, Uo, u_in, uo, o, Fn);
+                else lw_numflux(Fo, F_in, Uo, U_in, uo, u_in, o, Fn);
+                for (int v = 0; v < NV; ++v) sfv[sidx(v, l, dir - 1, e)] = Fn[v];
+            }
+        }
+    }
+
+    // Trixi calc_surface_integral! (weak form, TreeMesh 2-D) followed by
+    // apply_jacobian! (du *= -J⁻¹), called at dg_2d.jl:75-80.
+    void surface_integral_and_jacobian(T* du) {
+        const double f1 = basis.bnd_lo, f2 = basis.bnd_hi;
+#pragma omp parallel for schedule(static)
+        for (int e = 0; e < mesh.E; ++e) {
+            for (int l = 0; l < n; ++l)
+                for (int v = 0; v < NV; ++v) {
+                    du[idx(v, 0, l, e)] = du[idx(v, 0, l, e)] - sfv[sidx(v, l, 0, e)] * f1;
+                    du[idx(v, n - 1, l, e)] = du[idx(v, n - 1, l, e)] + sfv[sidx(v, l, 1, e)] * f2;
+                    du[idx(v, l, 0, e)] = du[idx(v, l, 0, e)] - sfv[sidx(v, l, 2, e)] * f1;
+                    du[idx(v, l, n - 1, e)] = du[idx(v, l, n - 1, e)] + sfv[sidx(v, l, 3, e)] * f2;
+                }
+            const double factor = -mesh.inv_jac[e];
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < n; ++i)
+                    for (int v = 0; v < NV; ++v) du[idx(v, i, j, e)] = du[idx(v, i, j, e)] * factor;
+        }
+    }
+
+    // rhs!(du, u, t, dt, mesh::TreeMesh{2}, ...), dg_2d.jl:28-83, plain FR.
+    void reset_du(T* du) const {
+        const long long nd = (long long)ndofs_total();
+#pragma omp parallel for schedule(static)
+        for (long long k = 0; k < nd; ++k) du[k] = T(0);
+    }
+
+    void rhs_lw(T* du, const T* u, double t, double dt) {
+        reset_du(du);  // reset_du!
+#pragma omp parallel for schedule(static)
+        for (int e = 0; e < mesh.E; ++e) {
+            if (basis.N == 4) lw_volume_kernel_4(du, u, dt, e, 1.0);
+            else lw_volume_kernel_3(du, u, dt, e, 1.0);
+        }
+        calc_interface_flux(u);
+        if (!mesh.bd_elem.empty()) calc_boundary_flux(u, t, dt, 1.0);
+        surface_integral_and_jacobian(du);
+    }
+
+    // rhs_mdrk1! / rhs_mdrk2!, tree/multi_derivative_2d.jl:1-116.  Both stages
+    // prolong uⁿ (the integrator's u) to the interfaces; stage 2 reads `us`.
+    void rhs_mdrk1(T* du, const T* u, double t, double dt) {
+        ensure_mdrk();
+        reset_du(du);
+#pragma omp parallel for schedule(static)
+        for (int e = 0; e < mesh.E; ++e) mdrk_kernel_1(du, u, dt, e, 1.0);
+        calc_interface_flux(u);
+        if (!mesh.bd_elem.empty()) calc_boundary_flux(u, t, dt, 0.5);
+        surface_integral_and_jacobian(du);
+    }
+    void rhs_mdrk2(T* du, const T* u, double t, double dt) {
+        ensure_mdrk();
+        reset_du(du);
+#pragma omp parallel for schedule(static)
+        for (int e = 0; e < mesh.E; ++e) mdrk_kernel_2(du, dt, e, 1.0);
+        calc_interface_flux(u);
+        if (!mesh.bd_elem.empty()) calc_boundary_flux(u, t, dt, 1.0);
+        surface_integral_and_jacobian(du);
+    }
+
+    // Trixi max_dt(u, t, mesh::TreeMesh{2}, constant_speed::False, ...):
+    // 2 / (nnodes * max_e J⁻¹ (max λ1 + max λ2)); called by compute_dt
+    // (src/solvers/lwfr.jl:176-190).
+    double max_dt(const T* u) const {
+        double mss = std::nextafter(0.0, 1.0);
+#pragma omp parallel for schedule(static) reduction(max : mss)
+        for (int e = 0; e < mesh.E; ++e) {
+            double m1 = 0, m2 = 0;
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < n; ++i) {
+                    T l1, l2;
+                    eq.max_abs_speeds(&u[idx(0, i, j, e)], l1, l2);
+                    m1 = max_(m1, value(l1));
+                    m2 = max_(m2, value(l2));
+                }
+            mss = max_(mss, mesh.inv_jac[e] * (m1 + m2));
+        }
+        return 2.0 / (n * mss);
+    }
+};
+
+// Julia isapprox(x, y) with default tolerances for Float64 (atol = 0,
+// rtol = sqrt(eps)).  Used by isfinished (src/solvers/lwfr.jl:146-148).
+inline bool julia_isapprox(double x, double y) {
+    if (x == y) return true;
+    if (!std::isfinite(x) || !std::isfinite(y)) return false;
+    return std::fabs(x - y) <= 1.4901161193847656e-08 * std::max(std::fabs(x), std::fabs(y));
+}
+
+// solve_lwfr loop with CFLBased time stepping (src/solvers/lwfr.jl:76-103,
+// 176-190, 212-263; src/solvers/mdrk.jl:1-29).  Callbacks and limiters are
+// empty.  Returns the number of steps taken; dts[k] receives the k-th dt.
+template <class T, class Eq>
+int solve(Solver<T, Eq>& s, T* u, double t0, double tf, double cfl, int max_steps, int mdrk, double* t_out,
+          double* dts) {
+    const long long nd = (long long)s.ndofs_total();
+    std::vector<T> du(nd), uprev(nd);
+    double t = t0;
+    int it = 0;
+    while (!julia_isapprox(t, tf) && it < max_steps) {
+        double dt = cfl * s.max_dt(u);
+        if (t + dt > tf) dt = tf - t;  // set_dt!
+#pragma omp parallel for schedule(static)
+        for (long long k = 0; k < nd; ++k) uprev[k] = u[k];
+        if (!mdrk) {
+            s.rhs_lw(du.data(), u, t, dt);
+#pragma omp parallel for schedule(static)
+            for (long long k = 0; k < nd; ++k) u[k] = uprev[k] + dt * du[k];
+        } else {
+            s.rhs_mdrk1(du.data(), u, t, dt);
+#pragma omp parallel for schedule(static)
+            for (long long k = 0; k < nd; ++k) s.us[k] = uprev[k] + dt * du[k];
+            s.rhs_mdrk2(du.data(), u, t, dt);
+#pragma omp parallel for schedule(static)
+            for (long long k = 0; k < nd; ++k) u[k] = uprev[k] + dt * du[k];
+        }
+        if (dts) dts[it] = dt;
+        t += dt;  // set_t_and_iter!
+        ++it;
+    }
+    if (t_out) *t_out = t;
+    return it;
+}
+
+}  // namespace tlwo

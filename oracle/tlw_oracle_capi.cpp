@@ -1,0 +1,190 @@
+// tlw_oracle_capi.cpp -- C entry points of the CPU oracle for ctypes.
+//
+// TEST INFRASTRUCTURE ONLY: loaded by tests/, __graft_entry__.smoke() and the
+// cpu_baseline leg of bench.py as the checker.  Never part of the product.
+#include <omp.h>
+
+#include <cstdio>
+#include <memory>
+
+#include "tlw_oracle.hpp"
+
+using namespace tlwo;
+
+struct tlwo_desc {
+    int polydeg;
+    int nx, ny;
+    double xmin, ymin, h;  // lower-left corner and element edge length
+    int periodic_x, periodic_y;
+    int equation;  // 0 compressible Euler, 1 linear scalar advection
+    double gamma;
+    double adv_a1, adv_a2;
+    int flux_order;  // 0 TreeMesh, 1 Structured/P4est argument order
+    int bc_kind;     // BoundaryValueKind
+    double bc_par[24];
+    double abstol, reltol;
+    int nthreads;    // 0: leave OpenMP's default
+};
+
+struct tlwo_ctx {
+    int equation;
+    std::unique_ptr<Solver<double, Euler2D<double>>> euler;
+    std::unique_ptr<Solver<double, LinAdv2D<double>>> linadv;
+};
+
+template <class S> static void configure(S& s, const tlwo_desc* d) {
+    s.basis.init(d->polydeg);
+    s.mesh.init(d->nx, d->ny, d->xmin, d->ymin, d->h, d->periodic_x != 0, d->periodic_y != 0);
+    s.flux_order = d->flux_order;
+    s.bval.kind = d->bc_kind;
+    for (int k = 0; k < 24; ++k) s.bval.par[k] = d->bc_par[k];
+    s.abstol = d->abstol;
+    s.reltol = d->reltol;
+    s.setup();
+}
+
+extern "C" {
+
+void* tlwo_create(const tlwo_desc* d) {
+    if (d->polydeg != 3 && d->polydeg != 4) return nullptr;
+    if (d->nthreads > 0) omp_set_num_threads(d->nthreads);
+    auto* c = new tlwo_ctx;
+    c->equation = d->equation;
+    if (d->equation == 0) {
+        c->euler.reset(new Solver<double, Euler2D<double>>);
+        c->euler->eq.gamma = d->gamma;
+        configure(*c->euler, d);
+    } else {
+        c->linadv.reset(new Solver<double, LinAdv2D<double>>);
+        c->linadv->eq.a1 = d->adv_a1;
+        c->linadv->eq.a2 = d->adv_a2;
+        configure(*c->linadv, d);
+    }
+    return c;
+}
+
+void tlwo_destroy(void* p) { delete static_cast<tlwo_ctx*>(p); }
+
+void tlwo_set_threads(int nthreads) {
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+}
+
+int tlwo_max_threads(void) { return omp_get_max_threads(); }
+
+#define DISPATCH(ctx, call)                               \
+    do {                                                  \
+        auto* c_ = static_cast<tlwo_ctx*>(ctx);           \
+        if (c_->equation == 0) { auto& s = *c_->euler; call; } \
+        else { auto& s = *c_->linadv; call; }             \
+    } while (0)
+
+long long tlwo_ndofs(void* ctx) {
+    long long r = 0;
+    DISPATCH(ctx, r = (long long)s.ndofs_total());
+    return r;
+}
+
+// Basis export: nodes, weights (n), D, Dhat (n*n, row-major [i][k])
+void tlwo_basis(void* ctx, double* nodes, double* weights, double* D, double* Dhat) {
+    DISPATCH(ctx, {
+        const int n = s.n;
+        for (int i = 0; i < n; ++i) {
+            nodes[i] = s.basis.nodes[i];
+            weights[i] = s.basis.weights[i];
+            for (int k = 0; k < n; ++k) {
+                D[i * n + k] = s.basis.D[i][k];
+                Dhat[i * n + k] = s.basis.Dhat[i][k];
+            }
+        }
+    });
+}
+
+// Node coordinates in [i, j, e] order
+void tlwo_node_coords(void* ctx, double* x, double* y) {
+    DISPATCH(ctx, {
+        const int n = s.n;
+        for (int e = 0; e < s.mesh.E; ++e)
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < n; ++i) {
+                    const size_t k = ((size_t)e * n + j) * n + i;
+                    x[k] = s.mesh.node_x(s.basis, i, e);
+                    y[k] = s.mesh.node_y(s.basis, j, e);
+                }
+    });
+}
+
+void tlwo_neighbors(void* ctx, int* nbr) {
+    DISPATCH(ctx, {
+        for (size_t k = 0; k < s.mesh.nbr.size(); ++k) nbr[k] = s.mesh.nbr[k];
+    });
+}
+
+double tlwo_max_dt(void* ctx, const double* u) {
+    double r = 0;
+    DISPATCH(ctx, r = s.max_dt(u));
+    return r;
+}
+
+void tlwo_rhs(void* ctx, const double* u, double* du, double t, double dt) {
+    DISPATCH(ctx, s.rhs_lw(du, u, t, dt));
+}
+
+void tlwo_rhs_mdrk1(void* ctx, const double* u, double* du, double t, double dt) {
+    DISPATCH(ctx, s.rhs_mdrk1(du, u, t, dt));
+}
+
+void tlwo_rhs_mdrk2(void* ctx, const double* u, double* du, double t, double dt) {
+    DISPATCH(ctx, s.rhs_mdrk2(du, u, t, dt));
+}
+
+void tlwo_set_us(void* ctx, const double* us) {
+    DISPATCH(ctx, {
+        s.ensure_mdrk();
+        std::copy(us, us + s.ndofs_total(), s.us.begin());
+    });
+}
+
+// which: 0 = Ũ [v,i,j,e], 1 = F̃ [v,d,i,j,e], 2 = temporal_errors [e],
+//        3 = surface_flux_values [v,l,face,e], 4 = us, 5 = U2, 6 = F2
+long long tlwo_get_aux(void* ctx, int which, double* out) {
+    long long len = -1;
+    DISPATCH(ctx, {
+        const std::vector<double>* src = nullptr;
+        switch (which) {
+            case 0: src = &s.U; break;
+            case 1: src = &s.F; break;
+            case 2: src = &s.temporal_errors; break;
+            case 3: src = &s.sfv; break;
+            case 4: src = &s.us; break;
+            case 5: src = &s.U2; break;
+            case 6: src = &s.F2; break;
+        }
+        if (src) {
+            len = (long long)src->size();
+            if (out) std::copy(src->begin(), src->end(), out);
+        }
+    });
+    return len;
+}
+
+int tlwo_solve(void* ctx, double* u, double t0, double tf, double cfl, int max_steps, int mdrk, double* t_out,
+               double* dts) {
+    int r = 0;
+    DISPATCH(ctx, r = solve(s, u, t0, tf, cfl, max_steps, mdrk, t_out, dts));
+    return r;
+}
+
+// Pointwise Euler pieces, for closed-form checks.
+void tlwo_euler_fluxes(const double* u, double gamma, double* f, double* g) {
+    Euler2D<double> eq;
+    eq.gamma = gamma;
+    eq.fluxes(u, f, g);
+}
+
+double tlwo_max_abs_speed_naive(const double* ul, const double* ur, int orientation, double gamma) {
+    Euler2D<double> eq;
+    eq.gamma = gamma;
+    return eq.max_abs_speed_naive(ul, ur, orientation);
+}
+
+}  // extern "C"

@@ -1,0 +1,111 @@
+"""Closed-form LW-FR / MDRK step for linear advection (test infrastructure).
+
+For f = a u the finite-difference time derivatives of the approximate LW
+procedure are exact on the Taylor states, so with the element-local operator
+A = -Δt J⁻¹ (a1 Dx + a2 Dy) the procedure collapses to matrix polynomials:
+  N=4 (src/solvers/dgsem_tree/dg_2d.jl:918-1267):
+      Ũ = (I + A/2 + A²/6 + A³/24 + A⁴/120) u
+      F̃ = a (I + A/2 + A²/6 + A³/24 + A⁴/240) u     (f_tttt carries a factor 0.5,
+                                                      dg_2d.jl:1203-1206)
+  N=3 (dg_2d.jl:674-916): Ũ = F̃/a = (I + A/2 + A²/6 + A³/24) u
+  MDRK stage 1 (tree/multi_derivative_2d.jl:263-420): F̃/a = Ũ = u/2 + Au/8
+  MDRK stage 2 (423-555): F̃/a = Ũ = u + Au/6 + A us/3
+and the FR update is u + Δt · (-J⁻¹)(D̂ F̃ + lifted Rusanov face fluxes).
+This module evaluates those formulas with dense numpy operators, in gather
+form, independently of the oracle's loops.  It is a pin, not a restatement:
+a wrong Taylor or finite-difference coefficient, a transposed D, a wrong
+lifting sign or the wrong jump/λ arguments of the interface flux changes the
+result beyond round-off.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+class LinearLW:
+    def __init__(self, nodes, weights, D, Dhat, nx, ny, h, a=(1.0, 0.7), flux_order="tree"):
+        self.nodes, self.w, self.D, self.Dh = nodes, weights, D, Dhat
+        self.n = len(nodes)
+        self.nx, self.ny = nx, ny
+        self.E = nx * ny
+        self.invJ = 2.0 / h
+        self.a1, self.a2 = a
+        self.flux_order = flux_order
+
+    # u has shape (E, n, n) indexed [e, j, i]
+    def dx(self, u, M):
+        return np.einsum("ik,ejk->eji", M, u)
+
+    def dy(self, u, M):
+        return np.einsum("jk,eki->eji", M, u)
+
+    def A(self, u, dt):
+        return -dt * self.invJ * (self.a1 * self.dx(u, self.D) + self.a2 * self.dy(u, self.D))
+
+    def nbr(self, e, face):
+        ix, iy = e % self.nx, e // self.nx
+        if face == 0:
+            ix = (ix - 1) % self.nx
+        elif face == 1:
+            ix = (ix + 1) % self.nx
+        elif face == 2:
+            iy = (iy - 1) % self.ny
+        else:
+            iy = (iy + 1) % self.ny
+        return ix + self.nx * iy
+
+    def fr_update(self, u, Ut, Qt, dt):
+        """u + dt * du with F̃ = a Qt, Ũ = Ut, Rusanov jump on u (tree) or Ũ (structured)."""
+        n = self.n
+        F, G = self.a1 * Qt, self.a2 * Qt
+        vol = self.dx(F, self.Dh) + self.dy(G, self.Dh)
+        jump = u if self.flux_order == "tree" else Ut
+        surf = np.zeros_like(u)
+        l1, l2 = abs(self.a1), abs(self.a2)
+        for e in range(self.E):
+            L = self.nbr(e, 0)
+            fs = 0.5 * (F[L, :, n - 1] + F[e, :, 0]) - 0.5 * l1 * (jump[e, :, 0] - jump[L, :, n - 1])
+            surf[e, :, 0] -= fs / self.w[0]
+            R = self.nbr(e, 1)
+            fs = 0.5 * (F[e, :, n - 1] + F[R, :, 0]) - 0.5 * l1 * (jump[R, :, 0] - jump[e, :, n - 1])
+            surf[e, :, n - 1] += fs / self.w[n - 1]
+            B = self.nbr(e, 2)
+            gs = 0.5 * (G[B, n - 1, :] + G[e, 0, :]) - 0.5 * l2 * (jump[e, 0, :] - jump[B, n - 1, :])
+            surf[e, 0, :] -= gs / self.w[0]
+            T = self.nbr(e, 3)
+            gs = 0.5 * (G[e, n - 1, :] + G[T, 0, :]) - 0.5 * l2 * (jump[T, 0, :] - jump[e, n - 1, :])
+            surf[e, n - 1, :] += gs / self.w[n - 1]
+        du = -self.invJ * (vol + surf)
+        return u + dt * du, du
+
+    def lw_step(self, u, dt):
+        N = self.n - 1
+        terms = [u]
+        for k in range(1, N + 1):
+            terms.append(self.A(terms[-1], dt))
+        fact = [1.0, 1.0 / 2, 1.0 / 6, 1.0 / 24, 1.0 / 120]
+        Ut = sum(fact[k] * terms[k] for k in range(N + 1))
+        qf = list(fact)
+        if N == 4:
+            qf[4] = 1.0 / 240
+        Qt = sum(qf[k] * terms[k] for k in range(N + 1))
+        return self.fr_update(u, Ut, Qt, dt)
+
+    def mdrk_step(self, u, dt):
+        Au = self.A(u, dt)
+        S1 = 0.5 * u + 0.125 * Au
+        us, du1 = self.fr_update(u, S1, S1, dt)
+        S2 = u + Au / 6.0 + self.A(us, dt) / 3.0
+        unew, du2 = self.fr_update(u, S2, S2, dt)
+        return unew, du1, du2, us
+
+    def step_matrix(self, dt, kind="lw"):
+        m = self.E * self.n * self.n
+        M = np.zeros((m, m))
+        for k in range(m):
+            e = np.zeros(m)
+            e[k] = 1.0
+            u = e.reshape(self.E, self.n, self.n)
+            r = self.lw_step(u, dt)[0] if kind == "lw" else self.mdrk_step(u, dt)[0]
+            M[:, k] = r.ravel()
+        return M

@@ -1,0 +1,95 @@
+"""Host-side logic and the C ABI surface (no GPU)."""
+import ctypes
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def header_functions():
+    txt = (ROOT / "include" / "trixilw_hip.h").read_text()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(tlw_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    from trixilw_hip import _lib
+
+    so = _lib.LIB_PATH
+    assert so.exists(), "libtrixilw_hip.so not built"
+    L = ctypes.CDLL(str(so))
+    names = header_functions()
+    assert len(names) >= 17
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    assert set(names) == set(_lib.EXPORTED)
+
+
+def test_product_does_not_use_the_oracle():
+    pkg = ROOT / "trixilw.jl_amd"
+    for p in list(pkg.rglob("*.py")) + list(pkg.rglob("*.hip")) + list(pkg.rglob("*.hpp")):
+        src = p.read_text()
+        assert "oracle" not in src.lower() or p.name == "build.py", p
+
+
+def test_treemesh_tables():
+    from trixilw_hip import TreeMesh
+
+    for ordering in ("morton", "lexicographic"):
+        m = TreeMesh((-10, -10), (10, 10), 3, periodicity=True, ordering=ordering)
+        E = m.n_elements
+        assert E == 64 and m.h == 2.5
+        nb = m.neighbors
+        for f in range(4):
+            assert np.all(nb[nb[:, f], f ^ 1] == np.arange(E))
+        assert np.all(m.inverse_jacobian == 2 / 2.5)
+        assert sorted(m.element_of_cell.tolist()) == list(range(E))
+    m = TreeMesh((0, 0), (1, 1), 2, periodicity=False)
+    assert (m.neighbors < 0).sum() == 4 * 4
+    cx = m.element_center[:, 0]
+    assert np.all(m.neighbors[cx < 0.25, 0] == -1)
+
+
+def test_node_coordinates_match_oracle(oracle_mod):
+    from trixilw_hip import LobattoLegendreBasis, TreeMesh
+
+    m = TreeMesh((-10, -10), (10, 10), 2, ordering="lexicographic")
+    b = LobattoLegendreBasis(4)
+    x, y = m.node_coordinates(b.nodes)
+    o = oracle_mod.Oracle(4, 4, 4, -10.0, -10.0, 5.0)
+    xo, yo = o.node_coords()
+    assert np.allclose(x, xo, rtol=0, atol=1e-14) and np.allclose(y, yo, rtol=0, atol=1e-14)
+
+
+def test_cfl_mapping_and_isapprox():
+    from trixilw_hip import DGSEM, isapprox, lw_fourier_cfl, trixi2lw
+
+    assert lw_fourier_cfl(4) == 0.103 and lw_fourier_cfl(3) == 0.170
+    assert trixi2lw(0.98, DGSEM(4)) == pytest.approx(0.98 * 5 * 0.103, rel=1e-15)
+    assert isapprox(1.0, 1.0 + 1e-9) and not isapprox(1.0, 1.0 + 1e-7)
+    assert isapprox(0.0, 0.0) and not isapprox(0.0, 1e-300)
+
+
+def test_set_dt_truncates():
+    from trixilw_hip.integrator import set_dt, set_t_and_iter
+
+    class Semi:
+        cache = {"dt": [0.0]}
+
+    class It:
+        t = 0.9
+        tspan = (0.0, 1.0)
+        p = Semi()
+        iter = 0
+
+        class stats:
+            naccept = 0
+
+    it = It()
+    assert set_dt(it, 0.3) == pytest.approx(0.1)
+    assert Semi.cache["dt"][0] == pytest.approx(0.1)
+    set_t_and_iter(it, 0.1)
+    assert it.t == pytest.approx(1.0) and it.iter == 1

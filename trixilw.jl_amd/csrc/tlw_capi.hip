@@ -1,0 +1,510 @@
+// tlw_capi.hip -- C ABI of libtrixilw_hip.so (declared in include/trixilw_hip.h).
+//
+// Host side of the drop-in boundary: validates and uploads the mesh/basis
+// description (what SemidiscretizationHyperbolic + create_cache_serial build,
+// src/semidiscretization/semidiscretization_hyperbolic.jl:58-84,
+// src/solvers/dgsem_tree/containers.jl:7-64), owns the device buffers, and
+// sequences the kernels of tlw_device.hpp for rhs!, rhs_mdrk1!/2!, max_dt and
+// the device-resident solve_lwfr loop.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/trixilw_hip.h"
+#include "tlw_device.hpp"
+
+using namespace tlw;
+
+struct tlw_ctx {
+    int N = 0, n = 0, E = 0;
+    int time_disc = TLW_LW;
+    int flags = 0;
+    hipStream_t stream = nullptr;
+    DevConst hconst{};
+    DevConst* d_const = nullptr;
+    DevState* d_state = nullptr;
+    DevState* h_state = nullptr;  // pinned mirror
+    int* d_nbr = nullptr;
+    double* d_invj = nullptr;
+    double* d_center = nullptr;
+    double* d_length = nullptr;
+    double* d_trace = nullptr;
+    double* d_buf[2] = {nullptr, nullptr};
+    double* d_us = nullptr;
+    double* d_auxU = nullptr;
+    double* d_auxF = nullptr;
+    int host_cur = 0;
+    bool aux_valid = false;
+    // kernel timing (tlw_enable_kernel_timing)
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<int, int>> ev_pairs;  // (start index, kind 0 volume / 1 surface)
+    size_t ev_used = 0;
+    double t_ms[2] = {0.0, 0.0};
+    int64_t t_n[2] = {0, 0};
+    std::string err;
+};
+
+namespace {
+
+int fail(tlw_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPCHK(ctx, call)                                                                         \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            return fail((ctx), TLW_EHIP, std::string(#call) + ": " + hipGetErrorString(e_));      \
+    } while (0)
+
+size_t state_len(const tlw_ctx* c) { return (size_t)4 * c->n * c->n * c->E; }
+
+void free_mesh(tlw_ctx* c) {
+    (void)hipFree(c->d_nbr);
+    (void)hipFree(c->d_invj);
+    (void)hipFree(c->d_center);
+    (void)hipFree(c->d_length);
+    (void)hipFree(c->d_trace);
+    (void)hipFree(c->d_buf[0]);
+    (void)hipFree(c->d_buf[1]);
+    (void)hipFree(c->d_us);
+    (void)hipFree(c->d_auxU);
+    (void)hipFree(c->d_auxF);
+    c->d_nbr = nullptr;
+    c->d_invj = c->d_center = c->d_length = c->d_trace = c->d_us = c->d_auxU = c->d_auxF = nullptr;
+    c->d_buf[0] = c->d_buf[1] = nullptr;
+}
+
+int upload_mesh(tlw_ctx* c, const tlw_mesh_desc* m) {
+    if (!m || m->n_elements <= 0 || !m->neighbors_host || !m->inverse_jacobian_host || !m->element_center_host ||
+        !m->element_length_host)
+        return fail(c, TLW_EINVAL, "mesh description: missing arrays or n_elements <= 0");
+    const int E = m->n_elements;
+    for (size_t k = 0; k < (size_t)4 * E; ++k) {
+        const int nb = m->neighbors_host[k];
+        if (nb < -1 || nb >= E) return fail(c, TLW_EINVAL, "mesh description: neighbour id out of range");
+    }
+    // conformity: the neighbour across a face must point back through the opposite face
+    for (int e = 0; e < E; ++e)
+        for (int f = 0; f < 4; ++f) {
+            const int nb = m->neighbors_host[4 * e + f];
+            if (nb >= 0 && m->neighbors_host[4 * nb + (f ^ 1)] != e)
+                return fail(c, TLW_EINVAL, "mesh description: neighbour table is not symmetric (non-conforming mesh)");
+        }
+    for (int e = 0; e < E; ++e)
+        if (!(m->inverse_jacobian_host[e] > 0.0) || !(m->element_length_host[e] > 0.0))
+            return fail(c, TLW_EINVAL, "mesh description: non-positive Jacobian or element length");
+    free_mesh(c);
+    c->E = E;
+    const size_t nd = state_len(c);
+    HIPCHK(c, hipMalloc(&c->d_nbr, sizeof(int) * 4 * E));
+    HIPCHK(c, hipMalloc(&c->d_invj, sizeof(double) * E));
+    HIPCHK(c, hipMalloc(&c->d_center, sizeof(double) * 2 * E));
+    HIPCHK(c, hipMalloc(&c->d_length, sizeof(double) * E));
+    HIPCHK(c, hipMalloc(&c->d_trace, sizeof(double) * (size_t)E * 4 * c->n * kTraceQ));
+    HIPCHK(c, hipMalloc(&c->d_buf[0], sizeof(double) * nd));
+    HIPCHK(c, hipMalloc(&c->d_buf[1], sizeof(double) * nd));
+    HIPCHK(c, hipMalloc(&c->d_us, sizeof(double) * nd));
+    if (c->flags & TLW_FLAG_KEEP_AUX) {
+        HIPCHK(c, hipMalloc(&c->d_auxU, sizeof(double) * nd));
+        HIPCHK(c, hipMalloc(&c->d_auxF, sizeof(double) * 2 * nd));
+    }
+    HIPCHK(c, hipMemcpy(c->d_nbr, m->neighbors_host, sizeof(int) * 4 * E, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_invj, m->inverse_jacobian_host, sizeof(double) * E, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_center, m->element_center_host, sizeof(double) * 2 * E, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_length, m->element_length_host, sizeof(double) * E, hipMemcpyHostToDevice));
+    c->aux_valid = false;
+    return TLW_OK;
+}
+
+KArgs base_args(tlw_ctx* c) {
+    KArgs a{};
+    a.E = c->E;
+    a.bufs[0] = c->d_buf[0];
+    a.bufs[1] = c->d_buf[1];
+    a.trace = c->d_trace;
+    a.nbr = c->d_nbr;
+    a.inv_jac = c->d_invj;
+    a.center = c->d_center;
+    a.length = c->d_length;
+    a.c = c->d_const;
+    a.st = c->d_state;
+    a.bc_scale = 1.0;
+    return a;
+}
+
+template <int N> int grid_of(const tlw_ctx* c) { return (c->E + Geo<N>::EB - 1) / Geo<N>::EB; }
+
+// Kernel launches, dispatched on the polynomial degree.
+enum VolKind { VOL_LW = 0, VOL_MDRK1 = 1, VOL_MDRK2 = 2 };
+
+template <int N> void launch_volume_t(tlw_ctx* c, const KArgs& a, int kind) {
+    dim3 g(grid_of<N>(c)), b(kBlock);
+    if (kind == VOL_LW) {
+        if constexpr (N == 4) hipLaunchKernelGGL(k_volume_lw4<4>, g, b, 0, c->stream, a);
+        else hipLaunchKernelGGL(k_volume_lw3<3>, g, b, 0, c->stream, a);
+    } else if (kind == VOL_MDRK1) {
+        hipLaunchKernelGGL(k_volume_mdrk1<N>, g, b, 0, c->stream, a);
+    } else {
+        hipLaunchKernelGGL(k_volume_mdrk2<N>, g, b, 0, c->stream, a);
+    }
+}
+void launch_volume(tlw_ctx* c, const KArgs& a, int kind) {
+    if (c->N == 4) launch_volume_t<4>(c, a, kind);
+    else launch_volume_t<3>(c, a, kind);
+}
+void launch_surface(tlw_ctx* c, const KArgs& a) {
+    if (c->N == 4) hipLaunchKernelGGL(k_surface<4>, dim3(grid_of<4>(c)), dim3(kBlock), 0, c->stream, a);
+    else hipLaunchKernelGGL(k_surface<3>, dim3(grid_of<3>(c)), dim3(kBlock), 0, c->stream, a);
+}
+void launch_max_speed(tlw_ctx* c, const KArgs& a) {
+    if (c->N == 4) hipLaunchKernelGGL(k_max_speed<4>, dim3(grid_of<4>(c)), dim3(kBlock), 0, c->stream, a);
+    else hipLaunchKernelGGL(k_max_speed<3>, dim3(grid_of<3>(c)), dim3(kBlock), 0, c->stream, a);
+}
+
+int check_launch(tlw_ctx* c) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(c, TLW_EHIP, std::string("kernel launch: ") + hipGetErrorString(e));
+    return TLW_OK;
+}
+
+hipEvent_t next_event(tlw_ctx* c) {
+    if (c->ev_used == c->ev_pool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        c->ev_pool.push_back(e);
+    }
+    return c->ev_pool[c->ev_used++];
+}
+
+// Records an event pair around a launch when timing is on.
+template <class F> void timed(tlw_ctx* c, int kind, F&& launch) {
+    if (!c->timing) {
+        launch();
+        return;
+    }
+    const size_t i0 = c->ev_used;
+    hipEvent_t a = next_event(c), b = next_event(c);
+    if (!a || !b) {
+        launch();
+        return;
+    }
+    (void)hipEventRecord(a, c->stream);
+    launch();
+    (void)hipEventRecord(b, c->stream);
+    c->ev_pairs.push_back({(int)i0, kind});
+}
+
+// Folds recorded event pairs into the totals (stream must be synchronised).
+void collect_times(tlw_ctx* c) {
+    for (auto& pr : c->ev_pairs) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, c->ev_pool[pr.first], c->ev_pool[pr.first + 1]) == hipSuccess) {
+            c->t_ms[pr.second] += ms;
+            c->t_n[pr.second] += 1;
+        }
+    }
+    c->ev_pairs.clear();
+    c->ev_used = 0;
+}
+
+// One rhs evaluation (volume + surface) on explicit pointers.
+int rhs_common(tlw_ctx* c, int kind, const double* u, double* du, double t, double dt) {
+    if (!c) return TLW_EINVAL;
+    if (!u || !du) return fail(c, TLW_EINVAL, "null state pointer");
+    if (!std::isfinite(dt) || !std::isfinite(t)) return fail(c, TLW_EINVAL, "t and dt must be finite");
+    if (kind != VOL_LW && c->time_disc != TLW_MDRK)
+        return fail(c, TLW_EINVAL, "rhs_mdrk needs a context created with TLW_MDRK");
+    KArgs a = base_args(c);
+    a.u = u;
+    a.us = c->d_us;
+    a.out = du;
+    a.in_sel = SEL_EXPLICIT;
+    a.out_sel = SEL_EXPLICIT;
+    a.t_arg = t;
+    a.dt_arg = dt;
+    a.use_state_dt = 0;
+    a.step_mode = 0;
+    a.reduce = RED_NONE;
+    a.bc_scale = (kind == VOL_MDRK1) ? 0.5 : 1.0;  // tree/multi_derivative_2d.jl:34
+    if (c->flags & TLW_FLAG_KEEP_AUX) {
+        a.auxU = c->d_auxU;
+        a.auxF = c->d_auxF;
+    }
+    launch_volume(c, a, kind);
+    launch_surface(c, a);
+    int rc = check_launch(c);
+    if (rc) return rc;
+    c->aux_valid = (c->flags & TLW_FLAG_KEEP_AUX) != 0;
+    return TLW_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tlw_create(const tlw_mesh_desc* mesh, const tlw_basis_desc* basis, const tlw_solver_desc* solver,
+               tlw_ctx** out) {
+    if (!out) return TLW_EINVAL;
+    *out = nullptr;
+    if (!mesh || !basis || !solver) return TLW_EINVAL;
+    if (basis->polydeg != 3 && basis->polydeg != 4) return TLW_ENOTSUP;
+    if (!basis->nodes_host || !basis->weights_host || !basis->derivative_matrix_host || !basis->derivative_dhat_host)
+        return TLW_EINVAL;
+    if (solver->time_discretization != TLW_LW && solver->time_discretization != TLW_MDRK) return TLW_EINVAL;
+    if (solver->surface_flux_order != TLW_FLUX_TREE && solver->surface_flux_order != TLW_FLUX_STRUCTURED)
+        return TLW_EINVAL;
+    if (solver->boundary_kind < TLW_BV_NONE || solver->boundary_kind > TLW_BV_RIEMANN4) return TLW_EINVAL;
+    if (!(solver->gamma > 1.0)) return TLW_EINVAL;
+    tlw_ctx* c = new tlw_ctx;
+    c->N = basis->polydeg;
+    c->n = c->N + 1;
+    c->time_disc = solver->time_discretization;
+    c->flags = solver->flags;
+    DevConst& h = c->hconst;
+    const int n = c->n;
+    for (int i = 0; i < n; ++i) {
+        h.nodes[i] = basis->nodes_host[i];
+        h.weights[i] = basis->weights_host[i];
+        for (int k = 0; k < n; ++k) {
+            h.D[i][k] = basis->derivative_matrix_host[i + n * k];
+            h.Dhat[i][k] = basis->derivative_dhat_host[i + n * k];
+        }
+    }
+    h.bnd_lo = 1.0 / h.weights[0];
+    h.bnd_hi = 1.0 / h.weights[n - 1];
+    h.gamma = solver->gamma;
+    for (int k = 0; k < 24; ++k) h.bc_par[k] = solver->boundary_params[k];
+    h.bc_kind = solver->boundary_kind;
+    h.flux_order = solver->surface_flux_order;
+    // every domain-boundary face needs a boundary value function
+    bool has_boundary = false;
+    for (size_t k = 0; k < (size_t)4 * std::max(mesh->n_elements, 0) && mesh->neighbors_host; ++k)
+        if (mesh->neighbors_host[k] < 0) has_boundary = true;
+    if (has_boundary && h.bc_kind == TLW_BV_NONE) {
+        delete c;
+        return TLW_EINVAL;
+    }
+    auto bail = [&](int rc) {
+        tlw_destroy(c);
+        return rc;
+    };
+    if (hipMalloc(&c->d_const, sizeof(DevConst)) != hipSuccess) return bail(TLW_EHIP);
+    if (hipMalloc(&c->d_state, sizeof(DevState)) != hipSuccess) return bail(TLW_EHIP);
+    if (hipHostMalloc(&c->h_state, sizeof(DevState), hipHostMallocDefault) != hipSuccess) return bail(TLW_EHIP);
+    if (hipMemcpy(c->d_const, &h, sizeof(DevConst), hipMemcpyHostToDevice) != hipSuccess) return bail(TLW_EHIP);
+    std::memset(c->h_state, 0, sizeof(DevState));
+    if (hipMemcpy(c->d_state, c->h_state, sizeof(DevState), hipMemcpyHostToDevice) != hipSuccess)
+        return bail(TLW_EHIP);
+    int rc = upload_mesh(c, mesh);
+    if (rc) return bail(rc);
+    *out = c;
+    return TLW_OK;
+}
+
+int tlw_rebuild_mesh(tlw_ctx* c, const tlw_mesh_desc* mesh) {
+    if (!c) return TLW_EINVAL;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return upload_mesh(c, mesh);
+}
+
+void tlw_destroy(tlw_ctx* c) {
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    else (void)hipDeviceSynchronize();
+    free_mesh(c);
+    for (auto e : c->ev_pool) (void)hipEventDestroy(e);
+    (void)hipFree(c->d_const);
+    (void)hipFree(c->d_state);
+    if (c->h_state) (void)hipHostFree(c->h_state);
+    delete c;
+}
+
+const char* tlw_last_error(const tlw_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int tlw_set_stream(tlw_ctx* c, void* stream) {
+    if (!c) return TLW_EINVAL;
+    c->stream = static_cast<hipStream_t>(stream);
+    return TLW_OK;
+}
+
+size_t tlw_state_len(const tlw_ctx* c) { return c ? state_len(c) : 0; }
+
+int tlw_rhs_lw(tlw_ctx* c, const double* u, double* du, double t, double dt) {
+    return rhs_common(c, VOL_LW, u, du, t, dt);
+}
+int tlw_rhs_mdrk1(tlw_ctx* c, const double* u, double* du, double t, double dt) {
+    return rhs_common(c, VOL_MDRK1, u, du, t, dt);
+}
+int tlw_rhs_mdrk2(tlw_ctx* c, const double* u, double* du, double t, double dt) {
+    return rhs_common(c, VOL_MDRK2, u, du, t, dt);
+}
+
+int tlw_us_ptr(tlw_ctx* c, double** us) {
+    if (!c || !us) return TLW_EINVAL;
+    *us = c->d_us;
+    return TLW_OK;
+}
+
+int tlw_set_us(tlw_ctx* c, const double* us) {
+    if (!c || !us) return fail(c, TLW_EINVAL, "null pointer");
+    HIPCHK(c, hipMemcpyAsync(c->d_us, us, sizeof(double) * state_len(c), hipMemcpyDeviceToDevice, c->stream));
+    return TLW_OK;
+}
+
+int tlw_max_dt(tlw_ctx* c, const double* u, double* max_dt) {
+    if (!c || !u || !max_dt) return fail(c, TLW_EINVAL, "null pointer");
+    hipLaunchKernelGGL(k_clear_bad_state, dim3(1), dim3(64), 0, c->stream, c->d_state);
+    KArgs a = base_args(c);
+    a.u = u;
+    a.in_sel = SEL_EXPLICIT;
+    a.reduce = RED_QUERY;
+    launch_max_speed(c, a);
+    int rc = check_launch(c);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->h_state, c->d_state, sizeof(DevState), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *max_dt = c->h_state->query;
+    if (c->h_state->bad_state) return fail(c, TLW_EADMISSIBLE, "non-admissible state (rho <= 0 or p <= 0)");
+    return TLW_OK;
+}
+
+int tlw_set_state(tlw_ctx* c, const double* u, size_t len) {
+    if (!c || !u) return fail(c, TLW_EINVAL, "null pointer");
+    if (len != state_len(c)) return fail(c, TLW_EINVAL, "state length mismatch");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(c->d_buf[c->host_cur], u, sizeof(double) * len, hipMemcpyHostToDevice));
+    return TLW_OK;
+}
+
+int tlw_get_state(tlw_ctx* c, double* u, size_t len) {
+    if (!c || !u) return fail(c, TLW_EINVAL, "null pointer");
+    if (len != state_len(c)) return fail(c, TLW_EINVAL, "state length mismatch");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(u, c->d_buf[c->host_cur], sizeof(double) * len, hipMemcpyDeviceToHost));
+    return TLW_OK;
+}
+
+int tlw_state_ptr(tlw_ctx* c, double** u) {
+    if (!c || !u) return TLW_EINVAL;
+    *u = c->d_buf[c->host_cur];
+    return TLW_OK;
+}
+
+int tlw_solve_init(tlw_ctx* c, double t0, double t_final, double cfl) {
+    if (!c) return TLW_EINVAL;
+    if (!std::isfinite(t0) || std::isnan(t_final) || !(cfl > 0.0))
+        return fail(c, TLW_EINVAL, "t0 finite, t_final not NaN and cfl > 0 required");
+    hipLaunchKernelGGL(k_init_state, dim3(1), dim3(64), 0, c->stream, c->d_state, t0, t_final, cfl, c->host_cur);
+    KArgs a = base_args(c);
+    a.in_sel = SEL_CUR;
+    a.reduce = RED_INIT;
+    launch_max_speed(c, a);
+    int rc = check_launch(c);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->h_state, c->d_state, sizeof(DevState), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->h_state->bad_state) return fail(c, TLW_EADMISSIBLE, "non-admissible initial state");
+    return TLW_OK;
+}
+
+int tlw_advance(tlw_ctx* c, int32_t max_steps, double* t_host, double* dt_host, int64_t* iter_host,
+                int32_t* finished_host) {
+    if (!c) return TLW_EINVAL;
+    if (max_steps < 0) return fail(c, TLW_EINVAL, "max_steps < 0");
+    KArgs a = base_args(c);
+    a.use_state_dt = 1;
+    a.step_mode = 1;
+    for (int s = 0; s < max_steps; ++s) {
+        if (c->time_disc == TLW_LW) {
+            KArgs v = a;
+            v.in_sel = SEL_CUR;
+            v.out_sel = SEL_OTHER;
+            v.reduce = RED_NONE;
+            timed(c, 0, [&] { launch_volume(c, v, VOL_LW); });
+            KArgs sf = a;
+            sf.out_sel = SEL_OTHER;
+            sf.reduce = RED_STEP;
+            timed(c, 1, [&] { launch_surface(c, sf); });
+        } else {
+            // stage 1: us = uⁿ + dt rhs_mdrk1(uⁿ)  (mdrk.jl:13-17)
+            KArgs v1 = a;
+            v1.in_sel = SEL_CUR;
+            v1.out_sel = SEL_EXPLICIT;
+            v1.out = c->d_us;
+            v1.reduce = RED_NONE;
+            timed(c, 0, [&] { launch_volume(c, v1, VOL_MDRK1); });
+            KArgs s1 = v1;
+            s1.bc_scale = 0.5;
+            timed(c, 1, [&] { launch_surface(c, s1); });
+            // stage 2: uⁿ⁺¹ = uⁿ + dt rhs_mdrk2(uⁿ, us)  (mdrk.jl:21-24)
+            KArgs v2 = a;
+            v2.in_sel = SEL_CUR;
+            v2.out_sel = SEL_OTHER;
+            v2.us = c->d_us;
+            v2.reduce = RED_NONE;
+            timed(c, 0, [&] { launch_volume(c, v2, VOL_MDRK2); });
+            KArgs s2 = v2;
+            s2.reduce = RED_STEP;
+            timed(c, 1, [&] { launch_surface(c, s2); });
+        }
+    }
+    int rc = check_launch(c);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->h_state, c->d_state, sizeof(DevState), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->timing) collect_times(c);
+    const DevState& s = *c->h_state;
+    c->host_cur = s.cur;
+    if (t_host) *t_host = s.t;
+    if (dt_host) *dt_host = s.dt;
+    if (iter_host) *iter_host = s.iter;
+    if (finished_host) *finished_host = s.finished;
+    if (s.bad_state) return fail(c, TLW_EADMISSIBLE, "non-admissible state (rho <= 0 or p <= 0)");
+    return TLW_OK;
+}
+
+int tlw_enable_kernel_timing(tlw_ctx* c, int32_t enable) {
+    if (!c) return TLW_EINVAL;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    collect_times(c);
+    c->timing = enable != 0;
+    return TLW_OK;
+}
+
+int tlw_kernel_times(tlw_ctx* c, double* vol_ms, int64_t* vol_n, double* surf_ms, int64_t* surf_n) {
+    if (!c) return TLW_EINVAL;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    collect_times(c);
+    if (vol_ms) *vol_ms = c->t_ms[0];
+    if (vol_n) *vol_n = c->t_n[0];
+    if (surf_ms) *surf_ms = c->t_ms[1];
+    if (surf_n) *surf_n = c->t_n[1];
+    c->t_ms[0] = c->t_ms[1] = 0.0;
+    c->t_n[0] = c->t_n[1] = 0;
+    return TLW_OK;
+}
+
+int tlw_get_aux(tlw_ctx* c, int32_t which, double* out, size_t len) {
+    if (!c || !out) return fail(c, TLW_EINVAL, "null pointer");
+    if (!(c->flags & TLW_FLAG_KEEP_AUX)) return fail(c, TLW_EINVAL, "context created without TLW_FLAG_KEEP_AUX");
+    if (!c->aux_valid) return fail(c, TLW_EINVAL, "no rhs evaluated yet");
+    const size_t nd = state_len(c);
+    const double* src = nullptr;
+    size_t need = 0;
+    if (which == TLW_AUX_U) { src = c->d_auxU; need = nd; }
+    else if (which == TLW_AUX_F) { src = c->d_auxF; need = 2 * nd; }
+    else return fail(c, TLW_EINVAL, "unknown aux selector");
+    if (len != need) return fail(c, TLW_EINVAL, "aux length mismatch");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(out, src, sizeof(double) * len, hipMemcpyDeviceToHost));
+    return TLW_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,24 @@
+"""trixilw_hip -- MI355X (gfx950) implementation of TrixiLW's LW-FR / MDRK right-hand side.
+
+Host-side mirror of the reference's API (TreeMesh, DGSEM, LW/MDRK,
+SemidiscretizationHyperbolic, rhs!/rhs_mdrk1!/rhs_mdrk2!, solve_lwfr) over the
+C ABI of libtrixilw_hip.so.  Importing this package does not load the
+library; the first call that needs the GPU does, and fails if it is missing.
+"""
+from .basis import LobattoLegendreBasis
+from .equations import (CompressibleEulerEquations2D, ConstantState, FourQuadrantRiemann, IsentropicVortex,
+                        LinearScalarAdvectionEquation2D)
+from .integrator import (LWIntegrator, LWSolution, compute_dt, isapprox, isfinished, perform_step, set_dt,
+                         set_t_and_iter, solve_lwfr)
+from .mesh import TreeMesh
+from .semidiscretization import (DGSEM, LW, MDRK, CFLBased, LWUpdate, SemidiscretizationHyperbolic,
+                                 VolumeIntegralFR, flux_lax_friedrichs, get_time_discretization, lw_fourier_cfl, rhs,
+                                 rhs_mdrk1, rhs_mdrk2, semidiscretize, trixi2lw)
+
+__all__ = [
+    "LobattoLegendreBasis", "CompressibleEulerEquations2D", "ConstantState", "FourQuadrantRiemann",
+    "IsentropicVortex", "LinearScalarAdvectionEquation2D", "LWIntegrator", "LWSolution", "compute_dt", "isapprox",
+    "isfinished", "perform_step", "set_dt", "set_t_and_iter", "solve_lwfr", "TreeMesh", "DGSEM", "LW", "MDRK",
+    "CFLBased", "LWUpdate", "SemidiscretizationHyperbolic", "VolumeIntegralFR", "flux_lax_friedrichs",
+    "get_time_discretization", "lw_fourier_cfl", "rhs", "rhs_mdrk1", "rhs_mdrk2", "semidiscretize", "trixi2lw",
+]

@@ -1,0 +1,129 @@
+"""ctypes binding of libtrixilw_hip.so (the C ABI in include/trixilw_hip.h).
+
+The library must exist: there is no CPU fallback.  Build it with
+``python trixilw.jl_amd/build.py`` (or ``__graft_entry__.build()``).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libtrixilw_hip.so"
+
+TLW_OK, TLW_EINVAL, TLW_EHIP, TLW_EADMISSIBLE, TLW_ENOTSUP = 0, 1, 2, 3, 4
+TLW_LW, TLW_MDRK = 0, 1
+TLW_FLUX_TREE, TLW_FLUX_STRUCTURED = 0, 1
+TLW_BV_NONE, TLW_BV_VORTEX, TLW_BV_CONSTANT, TLW_BV_RIEMANN4 = 0, 1, 2, 3
+TLW_AUX_U, TLW_AUX_F = 0, 1
+TLW_FLAG_KEEP_AUX = 1
+
+EXPORTED = (
+    "tlw_create", "tlw_rebuild_mesh", "tlw_destroy", "tlw_last_error", "tlw_set_stream", "tlw_state_len",
+    "tlw_rhs_lw", "tlw_rhs_mdrk1", "tlw_rhs_mdrk2", "tlw_us_ptr", "tlw_set_us", "tlw_max_dt", "tlw_set_state",
+    "tlw_get_state", "tlw_state_ptr", "tlw_solve_init", "tlw_advance", "tlw_get_aux",
+    "tlw_enable_kernel_timing", "tlw_kernel_times",
+)
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [
+        ("n_elements", C.c_int32),
+        ("neighbors_host", C.POINTER(C.c_int32)),
+        ("inverse_jacobian_host", C.POINTER(C.c_double)),
+        ("element_center_host", C.POINTER(C.c_double)),
+        ("element_length_host", C.POINTER(C.c_double)),
+    ]
+
+
+class BasisDesc(C.Structure):
+    _fields_ = [
+        ("polydeg", C.c_int32),
+        ("nodes_host", C.POINTER(C.c_double)),
+        ("weights_host", C.POINTER(C.c_double)),
+        ("derivative_matrix_host", C.POINTER(C.c_double)),
+        ("derivative_dhat_host", C.POINTER(C.c_double)),
+    ]
+
+
+class SolverDesc(C.Structure):
+    _fields_ = [
+        ("time_discretization", C.c_int32),
+        ("gamma", C.c_double),
+        ("surface_flux_order", C.c_int32),
+        ("boundary_kind", C.c_int32),
+        ("boundary_params", C.c_double * 24),
+        ("flags", C.c_int32),
+    ]
+
+
+class TLWError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"libtrixilw_hip error {code}: {msg}")
+        self.code = code
+
+
+class NonAdmissibleState(TLWError):
+    """The device found ρ <= 0 or p <= 0 (the reference raises DomainError)."""
+
+
+_lib = None
+
+
+def lib():
+    """Load libtrixilw_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python trixilw.jl_amd/build.py`")
+    L = C.CDLL(str(LIB_PATH))
+    vp, dp, i32, sz = C.c_void_p, C.POINTER(C.c_double), C.c_int32, C.c_size_t
+    L.tlw_create.restype = C.c_int
+    L.tlw_create.argtypes = [C.POINTER(MeshDesc), C.POINTER(BasisDesc), C.POINTER(SolverDesc), C.POINTER(vp)]
+    L.tlw_rebuild_mesh.restype = C.c_int
+    L.tlw_rebuild_mesh.argtypes = [vp, C.POINTER(MeshDesc)]
+    L.tlw_destroy.restype = None
+    L.tlw_destroy.argtypes = [vp]
+    L.tlw_last_error.restype = C.c_char_p
+    L.tlw_last_error.argtypes = [vp]
+    L.tlw_set_stream.restype = C.c_int
+    L.tlw_set_stream.argtypes = [vp, vp]
+    L.tlw_state_len.restype = sz
+    L.tlw_state_len.argtypes = [vp]
+    for nm in ("tlw_rhs_lw", "tlw_rhs_mdrk1", "tlw_rhs_mdrk2"):
+        f = getattr(L, nm)
+        f.restype = C.c_int
+        f.argtypes = [vp, vp, vp, C.c_double, C.c_double]
+    L.tlw_us_ptr.restype = C.c_int
+    L.tlw_us_ptr.argtypes = [vp, C.POINTER(vp)]
+    L.tlw_set_us.restype = C.c_int
+    L.tlw_set_us.argtypes = [vp, vp]
+    L.tlw_max_dt.restype = C.c_int
+    L.tlw_max_dt.argtypes = [vp, vp, dp]
+    L.tlw_set_state.restype = C.c_int
+    L.tlw_set_state.argtypes = [vp, dp, sz]
+    L.tlw_get_state.restype = C.c_int
+    L.tlw_get_state.argtypes = [vp, dp, sz]
+    L.tlw_state_ptr.restype = C.c_int
+    L.tlw_state_ptr.argtypes = [vp, C.POINTER(vp)]
+    L.tlw_solve_init.restype = C.c_int
+    L.tlw_solve_init.argtypes = [vp, C.c_double, C.c_double, C.c_double]
+    L.tlw_advance.restype = C.c_int
+    L.tlw_advance.argtypes = [vp, i32, dp, dp, C.POINTER(C.c_int64), C.POINTER(i32)]
+    L.tlw_enable_kernel_timing.restype = C.c_int
+    L.tlw_enable_kernel_timing.argtypes = [vp, i32]
+    L.tlw_kernel_times.restype = C.c_int
+    L.tlw_kernel_times.argtypes = [vp, dp, C.POINTER(C.c_int64), dp, C.POINTER(C.c_int64)]
+    L.tlw_get_aux.restype = C.c_int
+    L.tlw_get_aux.argtypes = [vp, i32, dp, sz]
+    _lib = L
+    return L
+
+
+def check(rc: int, handle=None) -> None:
+    if rc == TLW_OK:
+        return
+    msg = lib().tlw_last_error(handle).decode() if handle else "(no context)"
+    if rc == TLW_EADMISSIBLE:
+        raise NonAdmissibleState(rc, msg)
+    raise TLWError(rc, msg)

@@ -1,0 +1,134 @@
+"""Equations and initial/boundary value functions (host side, numpy).
+
+`CompressibleEulerEquations2D` mirrors Trixi's type of the same name as used
+by TrixiLW (src/equations/compressible_euler_2d.jl); the kernels evaluate the
+flux on the device, this class only converts variables and evaluates initial
+and boundary data on the host.
+
+Boundary-value functions that the device must evaluate (time-averaged
+Dirichlet condition, src/equations/equations.jl:6-37) are described by a kind
+code and a parameter vector shared with the HIP library.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+BV_NONE, BV_VORTEX, BV_CONSTANT, BV_RIEMANN4 = 0, 1, 2, 3
+
+
+@dataclass
+class CompressibleEulerEquations2D:
+    gamma: float = 1.4
+    nvariables: int = 4
+
+    def prim2cons(self, rho, v1, v2, p):
+        g = self.gamma
+        return np.stack([rho, rho * v1, rho * v2, p / (g - 1) + 0.5 * rho * (v1 * v1 + v2 * v2)], axis=-1)
+
+    def cons2prim(self, u):
+        rho = u[..., 0]
+        v1 = u[..., 1] / rho
+        v2 = u[..., 2] / rho
+        p = (self.gamma - 1) * (u[..., 3] - 0.5 * (u[..., 1] * v1 + u[..., 2] * v2))
+        return rho, v1, v2, p
+
+
+@dataclass
+class LinearScalarAdvectionEquation2D:
+    advection_velocity: tuple = (1.0, 1.0)
+    nvariables: int = 1
+
+
+@dataclass
+class IsentropicVortex:
+    """Shu/Yee isentropic vortex translating with the free stream.
+
+    dv = eps/(2π) exp((1-r²)/2) (-y, x), T = 1 - (γ-1) eps²/(8γπ²) exp(1-r²),
+    ρ = T^{1/(γ-1)}, p = ρT, background ρ = p = 1 and v∞ = M √γ (1,1)/√2.
+    The exact solution at time t is the periodic translate by v∞ t; the
+    images beyond the nearest one contribute below 1e-20 on [-10,10]².
+    """
+
+    strength: float = 5.0
+    mach: float = 0.5
+    center: tuple = (0.0, 0.0)
+    period: tuple = (20.0, 20.0)
+    gamma: float = 1.4
+
+    kind = BV_VORTEX
+
+    @property
+    def params(self):
+        return [self.strength, self.mach, self.center[0], self.center[1], self.period[0], self.period[1],
+                self.gamma]
+
+    def __call__(self, x, y, t, equations=None):
+        g = self.gamma
+        vinf = self.mach * np.sqrt(g) / np.sqrt(2.0)
+        dx = x - (self.center[0] + vinf * t)
+        dy = y - (self.center[1] + vinf * t)
+        Lx, Ly = self.period
+        dx = dx - Lx * np.floor(dx / Lx + 0.5)
+        dy = dy - Ly * np.floor(dy / Ly + 0.5)
+        r2 = dx * dx + dy * dy
+        eps = self.strength
+        du = eps / (2 * np.pi) * np.exp(0.5 * (1 - r2))
+        T = 1 - (g - 1) * eps * eps / (8 * g * np.pi * np.pi) * np.exp(1 - r2)
+        rho = T ** (1 / (g - 1))
+        return CompressibleEulerEquations2D(g).prim2cons(rho, vinf - du * dy, vinf + du * dx, rho * T)
+
+
+@dataclass
+class ConstantState:
+    rho: float = 1.0
+    v1: float = 0.1
+    v2: float = -0.2
+    p: float = 1.0
+    gamma: float = 1.4
+
+    kind = BV_CONSTANT
+
+    @property
+    def params(self):
+        return [self.rho, self.v1, self.v2, self.p, self.gamma]
+
+    def __call__(self, x, y, t, equations=None):
+        one = np.ones_like(np.asarray(x, dtype=np.float64))
+        return CompressibleEulerEquations2D(self.gamma).prim2cons(self.rho * one, self.v1 * one, self.v2 * one,
+                                                                  self.p * one)
+
+
+@dataclass
+class FourQuadrantRiemann:
+    """Piecewise-constant (ρ, v1, v2, p) per quadrant around (x0, y0).
+
+    Default states: configuration 3 of Lax & Liu (SIAM J. Sci. Comput. 19,
+    1998), ρ ∈ [0.138, 1.5] and p ∈ [0.029, 1.5]; quadrant order NE, NW, SW, SE.
+    """
+
+    x0: float = 0.5
+    y0: float = 0.5
+    states: tuple = ((1.5, 0.0, 0.0, 1.5),
+                     (0.5323, 1.206, 0.0, 0.3),
+                     (0.138, 1.206, 1.206, 0.029),
+                     (0.5323, 0.0, 1.206, 0.3))
+    gamma: float = 1.4
+
+    kind = BV_RIEMANN4
+
+    @property
+    def params(self):
+        out = [self.x0, self.y0, self.gamma]
+        for s in self.states:
+            out.extend(s)
+        return out
+
+    def __call__(self, x, y, t, equations=None):
+        x = np.asarray(x, dtype=np.float64)
+        y = np.asarray(y, dtype=np.float64)
+        q = np.where(x >= self.x0, np.where(y >= self.y0, 0, 3), np.where(y >= self.y0, 1, 2))
+        st = np.asarray(self.states, dtype=np.float64)
+        rho, v1, v2, p = (st[q, k] for k in range(4))
+        return CompressibleEulerEquations2D(self.gamma).prim2cons(rho, v1, v2, p)

@@ -1,0 +1,282 @@
+"""Host mirror of TrixiLW's solver types and SemidiscretizationHyperbolic.
+
+Names, arguments and semantics follow the reference:
+  LW, MDRK, CFLBased                src/basic_types.jl:1-16
+  VolumeIntegralFR, get_time_discretization   src/solvers/fr.jl:15-30, 81-84
+  lw_fourier_cfl, trixi2lw          src/auxiliary/auxiliary.jl:14-23
+  SemidiscretizationHyperbolic      src/semidiscretization/semidiscretization_hyperbolic.jl:58-84
+  rhs!, rhs_mdrk1!, rhs_mdrk2!      src/semidiscretization/semidiscretization_hyperbolic.jl:4-56
+  semidiscretize                    src/semidiscretization/semidiscretization.jl:4-14
+The right-hand side itself runs in libtrixilw_hip.so; state vectors are
+torch float64 tensors on the GPU in Trixi's [v, i, j, e] layout, i.e. shape
+(E, n, n, 4) in row-major order.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from .basis import LobattoLegendreBasis
+from .equations import CompressibleEulerEquations2D
+
+
+class LW:
+    def __repr__(self):
+        return "Lax-Wendroff"
+
+
+class MDRK:
+    def __repr__(self):
+        return "Multi-Derivative Runge-Kutta"
+
+
+@dataclass
+class CFLBased:
+    cfl_number: float
+
+
+@dataclass
+class VolumeIntegralFR:
+    time_discretization: object
+
+
+flux_lax_friedrichs = "flux_lax_friedrichs"
+
+
+class DGSEM:
+    """DGSEM(polydeg, surface_flux=flux_lax_friedrichs, volume_integral=VolumeIntegralFR(LW()))."""
+
+    def __init__(self, polydeg, surface_flux=flux_lax_friedrichs, volume_integral=None):
+        if surface_flux != flux_lax_friedrichs:
+            raise NotImplementedError("only flux_lax_friedrichs (Rusanov) is implemented")
+        self.basis = LobattoLegendreBasis(polydeg)
+        self.surface_flux = surface_flux
+        self.volume_integral = volume_integral if volume_integral is not None else VolumeIntegralFR(LW())
+
+    def polydeg(self):
+        return self.basis.polydeg
+
+    def nnodes(self):
+        return self.basis.polydeg + 1
+
+
+def get_time_discretization(solver_or_integral):
+    vi = solver_or_integral.volume_integral if isinstance(solver_or_integral, DGSEM) else solver_or_integral
+    return vi.time_discretization
+
+
+def lw_fourier_cfl(N: int) -> float:
+    return (1.000, 0.333, 0.170, 0.103)[N - 1]
+
+
+def trixi2lw(cfl_number: float, dg: DGSEM) -> float:
+    N = dg.polydeg()
+    return cfl_number * (N + 1) * lw_fourier_cfl(N)
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _torch():
+    import torch
+    return torch
+
+
+class SemidiscretizationHyperbolic:
+    """SemidiscretizationHyperbolic(mesh, time_discretization, equations, initial_condition, solver;
+    boundary_conditions=periodic).
+
+    ``boundary_conditions`` is a boundary-value function object (IsentropicVortex,
+    ConstantState, FourQuadrantRiemann) used as the time-averaged Dirichlet
+    condition on every non-periodic face, or None for a fully periodic mesh.
+    ``surface_flux_order`` selects the TreeMesh ("tree") or Structured/P4est
+    ("structured") argument order of the LW Rusanov flux.
+    """
+
+    def __init__(self, mesh, time_discretization, equations, initial_condition, solver, boundary_conditions=None,
+                 source_terms=None, surface_flux_order="tree", keep_aux=False, device=None):
+        if source_terms is not None:
+            raise NotImplementedError("source terms are not on the device path")
+        if not isinstance(equations, CompressibleEulerEquations2D):
+            raise NotImplementedError("the device path implements CompressibleEulerEquations2D")
+        torch = _torch()
+        self.mesh = mesh
+        self.equations = equations
+        self.initial_condition = initial_condition
+        self.solver = solver
+        self.time_discretization = time_discretization
+        self.boundary_conditions = boundary_conditions
+        self.device = torch.device(device) if device is not None else torch.device("cuda")
+        self.n = solver.nnodes()
+        self.cache = {"dt": [math.nan]}
+        self.performance_counter = []
+        b = solver.basis
+        self._nodes = np.ascontiguousarray(b.nodes)
+        self._weights = np.ascontiguousarray(b.weights)
+        self._D = np.asfortranarray(b.derivative_matrix).ravel(order="F").copy()
+        self._Dh = np.asfortranarray(b.derivative_dhat).ravel(order="F").copy()
+        basis = _lib.BasisDesc(b.polydeg, _dptr(self._nodes), _dptr(self._weights), _dptr(self._D), _dptr(self._Dh))
+        sd = _lib.SolverDesc()
+        sd.time_discretization = _lib.TLW_MDRK if isinstance(time_discretization, MDRK) else _lib.TLW_LW
+        sd.gamma = equations.gamma
+        sd.surface_flux_order = {"tree": _lib.TLW_FLUX_TREE, "structured": _lib.TLW_FLUX_STRUCTURED}[
+            surface_flux_order]
+        if boundary_conditions is None:
+            sd.boundary_kind = _lib.TLW_BV_NONE
+        else:
+            sd.boundary_kind = int(boundary_conditions.kind)
+            for k, v in enumerate(boundary_conditions.params):
+                sd.boundary_params[k] = float(v)
+        sd.flags = _lib.TLW_FLAG_KEEP_AUX if keep_aux else 0
+        self._keep = self._mesh_arrays(mesh)
+        md = self._mesh_desc()
+        h = C.c_void_p()
+        torch.cuda.init()
+        rc = _lib.lib().tlw_create(C.byref(md), C.byref(basis), C.byref(sd), C.byref(h))
+        if rc != _lib.TLW_OK:
+            raise _lib.TLWError(rc, "tlw_create failed (invalid description or HIP error)")
+        self._h = h
+
+    def _mesh_arrays(self, mesh):
+        return {
+            "nbr": np.ascontiguousarray(mesh.neighbors, dtype=np.int32),
+            "invj": np.ascontiguousarray(mesh.inverse_jacobian, dtype=np.float64),
+            "center": np.ascontiguousarray(mesh.element_center, dtype=np.float64),
+            "length": np.ascontiguousarray(mesh.element_length, dtype=np.float64),
+        }
+
+    def _mesh_desc(self):
+        k = self._keep
+        return _lib.MeshDesc(int(self.mesh.n_elements), k["nbr"].ctypes.data_as(C.POINTER(C.c_int32)),
+                             _dptr(k["invj"]), _dptr(k["center"]), _dptr(k["length"]))
+
+    def rebuild_mesh(self, mesh):
+        """Host-side mesh change (AMR) followed by a device re-upload."""
+        self.mesh = mesh
+        self._keep = self._mesh_arrays(mesh)
+        md = self._mesh_desc()
+        _lib.check(_lib.lib().tlw_rebuild_mesh(self._h, C.byref(md)), self._h)
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _lib.lib().tlw_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- helpers ---------------------------------------------------------------
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def state_shape(self):
+        return (self.mesh.n_elements, self.n, self.n, 4)
+
+    def ndofs(self):
+        return self.mesh.n_elements * self.n * self.n
+
+    def node_coordinates(self):
+        return self.mesh.node_coordinates(self.solver.basis.nodes)
+
+    def compute_coefficients(self, t):
+        """Initial condition at the nodes as a device tensor (Trixi compute_coefficients)."""
+        torch = _torch()
+        x, y = self.node_coordinates()
+        u = np.ascontiguousarray(self.initial_condition(x, y, t), dtype=np.float64)
+        return torch.from_numpy(u).to(self.device)
+
+    def _bind_stream(self):
+        torch = _torch()
+        s = torch.cuda.current_stream(self.device).cuda_stream
+        _lib.check(_lib.lib().tlw_set_stream(self._h, C.c_void_p(s)), self._h)
+
+    def _check_state(self, x, name):
+        torch = _torch()
+        if not isinstance(x, torch.Tensor) or x.dtype != torch.float64 or not x.is_cuda:
+            raise TypeError(f"{name} must be a float64 CUDA tensor")
+        if not x.is_contiguous() or x.numel() != self.ndofs() * 4:
+            raise ValueError(f"{name} must be contiguous with {self.ndofs() * 4} entries")
+
+    def max_dt(self, u_ode):
+        """Trixi max_dt(u, t, mesh, constant_speed=False, equations, solver, cache)."""
+        self._check_state(u_ode, "u")
+        self._bind_stream()
+        out = C.c_double(0.0)
+        _lib.check(_lib.lib().tlw_max_dt(self._h, C.c_void_p(u_ode.data_ptr()), C.byref(out)), self._h)
+        return out.value
+
+    def us(self):
+        """Device pointer of the MDRK stage buffer (element_cache.mdrk_cache._us)."""
+        p = C.c_void_p()
+        _lib.check(_lib.lib().tlw_us_ptr(self._h, C.byref(p)), self._h)
+        return p.value
+
+    def set_us(self, us_ode):
+        """Stream-ordered copy of a device tensor into the MDRK stage buffer."""
+        self._check_state(us_ode, "us")
+        self._bind_stream()
+        _lib.check(_lib.lib().tlw_set_us(self._h, C.c_void_p(us_ode.data_ptr())), self._h)
+
+    def get_aux(self, which):
+        n = self.n
+        E = self.mesh.n_elements
+        if which == "U":
+            out = np.zeros((E, n, n, 4))
+            code = _lib.TLW_AUX_U
+        elif which == "F":
+            out = np.zeros((E, n, n, 2, 4))
+            code = _lib.TLW_AUX_F
+        else:
+            raise ValueError(which)
+        _lib.check(_lib.lib().tlw_get_aux(self._h, code, _dptr(out), out.size), self._h)
+        return out
+
+
+def _rhs_call(fn, du_ode, u_ode, semi, t):
+    semi._check_state(u_ode, "u")
+    semi._check_state(du_ode, "du")
+    semi._bind_stream()
+    dt = semi.cache["dt"][0]
+    _lib.check(fn(semi.handle, C.c_void_p(u_ode.data_ptr()), C.c_void_p(du_ode.data_ptr()), float(t), float(dt)),
+               semi.handle)
+
+
+def rhs(du_ode, u_ode, semi, t, tolerances=None):
+    """rhs!(du_ode, u_ode, semi, t): du = LW-FR residual with dt = semi.cache["dt"][0]."""
+    _rhs_call(_lib.lib().tlw_rhs_lw, du_ode, u_ode, semi, t)
+
+
+def rhs_mdrk1(du_ode, u_ode, semi, t, tolerances=None):
+    """rhs_mdrk1!(du_ode, u_ode, semi, t): first MDRK stage."""
+    _rhs_call(_lib.lib().tlw_rhs_mdrk1, du_ode, u_ode, semi, t)
+
+
+def rhs_mdrk2(du_ode, u_ode, semi, t, tolerances=None):
+    """rhs_mdrk2!(du_ode, u_ode, semi, t): second MDRK stage; reads us from the cache."""
+    _rhs_call(_lib.lib().tlw_rhs_mdrk2, du_ode, u_ode, semi, t)
+
+
+@dataclass
+class LWUpdate:
+    f: object
+    u0: object
+    soln_arrays: dict
+    tspan: tuple
+    p: SemidiscretizationHyperbolic
+
+
+def semidiscretize(semi, time_discretization, tspan):
+    """semidiscretize(semi, time_discretization, tspan) -> LWUpdate (semidiscretization.jl:4-14)."""
+    u0_ode = semi.compute_coefficients(tspan[0])
+    du_ode = _torch().empty_like(u0_ode)
+    return LWUpdate(rhs, u0_ode, {"u0_ode": u0_ode, "du_ode": du_ode}, (float(tspan[0]), float(tspan[1])), semi)
