@@ -253,6 +253,26 @@ def test_conservation_and_free_stream_at_scale():
         semi.close()
 
 
+def test_nonadmissible_step_is_reported(oracle_mod):
+    """The under-resolved 4×4 vortex reaches p < 0 in its third step (the oracle
+    shows the same); the device loop reports it instead of continuing."""
+    import trixilw_hip as tlw
+    from trixilw_hip._lib import NonAdmissibleState
+
+    semi, o, u_lex, mesh = setup(oracle_mod, 4, 2, ic=tlw.IsentropicVortex(), amp=0.0, keep_aux=False)
+    cfl = tlw.trixi2lw(0.98, semi.solver)
+    u2, _, _ = o.solve(u_lex, 0.0, 1e9, cfl, max_steps=2)
+    u3, _, _ = o.solve(u_lex, 0.0, 1e9, cfl, max_steps=3)
+    p = lambda u: 0.4 * (u[..., 3] - 0.5 * (u[..., 1] ** 2 + u[..., 2] ** 2) / u[..., 0])
+    assert p(u2).min() > 0 and p(u3).min() < 0
+    lw_update = tlw.semidiscretize(semi, semi.time_discretization, (0.0, 1e9))
+    sol = tlw.solve_lwfr(lw_update, (), 1e-6, None, time_step_computation=tlw.CFLBased(cfl), max_steps=2)
+    assert sol.stats.naccept == 2
+    lw_update = tlw.semidiscretize(semi, semi.time_discretization, (0.0, 1e9))
+    with pytest.raises(NonAdmissibleState):
+        tlw.solve_lwfr(lw_update, (), 1e-6, None, time_step_computation=tlw.CFLBased(cfl), max_steps=3)
+
+
 def test_nonadmissible_state_is_reported():
     """A state with negative pressure is rejected (the reference raises DomainError)."""
     import trixilw_hip as tlw

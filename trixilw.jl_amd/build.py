@@ -20,6 +20,8 @@ OUT = PKG_ROOT / "trixilw_hip" / "libtrixilw_hip.so"
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
+    # torch's bundled HIP runtime (ROCm 7.0) is the one loaded in-process
+    "-mcode-object-version=5",
     "-O3",
     "-std=c++17",
     "-fPIC",

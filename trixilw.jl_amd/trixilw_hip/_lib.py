@@ -76,6 +76,10 @@ def lib():
         return _lib
     if not LIB_PATH.exists():
         raise RuntimeError(f"{LIB_PATH} is missing: build it with `python trixilw.jl_amd/build.py`")
+    # torch bundles its own libamdhip64.so.7; loading torch first makes this
+    # library bind to that single HIP runtime (same soname), so streams and
+    # device pointers are shared.
+    import torch  # noqa: F401
     L = C.CDLL(str(LIB_PATH))
     vp, dp, i32, sz = C.c_void_p, C.POINTER(C.c_double), C.c_int32, C.c_size_t
     L.tlw_create.restype = C.c_int
