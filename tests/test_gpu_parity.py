@@ -183,7 +183,8 @@ def test_device_loop_k_steps(oracle_mod, N, td):
     cfl = 0.98 * tlw.trixi2lw(1.0, semi.solver) * (0.5 if td == "mdrk" else 1.0)
     K = 12
     u_ref, t_ref, dts = o.solve(u_lex, 0.0, 1e9, cfl, max_steps=K, mdrk=(td == "mdrk"))
-    lw_update = tlw.semidiscretize(semi, semi.time_discretization, (0.0, 1e9))
+    # an infinite final time never finishes (Julia: isapprox(t, Inf) is false)
+    lw_update = tlw.semidiscretize(semi, semi.time_discretization, (0.0, float("inf")))
     lw_update.u0.copy_(to_dev(mesh, u_lex))
     sol = tlw.solve_lwfr(lw_update, (), 1e-6, None, time_step_computation=tlw.CFLBased(cfl), max_steps=K)
     assert sol.stats.naccept == K

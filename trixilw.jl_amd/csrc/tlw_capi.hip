@@ -37,6 +37,8 @@ struct tlw_ctx {
     double* d_us = nullptr;
     double* d_auxU = nullptr;
     double* d_auxF = nullptr;
+    double* d_partial = nullptr;
+    bool has_boundary = false;
     int host_cur = 0;
     bool aux_valid = false;
     // kernel timing (tlw_enable_kernel_timing)
@@ -76,6 +78,8 @@ void free_mesh(tlw_ctx* c) {
     (void)hipFree(c->d_us);
     (void)hipFree(c->d_auxU);
     (void)hipFree(c->d_auxF);
+    (void)hipFree(c->d_partial);
+    c->d_partial = nullptr;
     c->d_nbr = nullptr;
     c->d_invj = c->d_center = c->d_length = c->d_trace = c->d_us = c->d_auxU = c->d_auxF = nullptr;
     c->d_buf[0] = c->d_buf[1] = nullptr;
@@ -102,6 +106,11 @@ int upload_mesh(tlw_ctx* c, const tlw_mesh_desc* m) {
             return fail(c, TLW_EINVAL, "mesh description: non-positive Jacobian or element length");
     free_mesh(c);
     c->E = E;
+    c->has_boundary = false;
+    for (size_t k = 0; k < (size_t)4 * E; ++k)
+        if (m->neighbors_host[k] < 0) c->has_boundary = true;
+    if (c->has_boundary && c->hconst.bc_kind == TLW_BV_NONE)
+        return fail(c, TLW_EINVAL, "mesh has boundary faces but no boundary value function was given");
     const size_t nd = state_len(c);
     HIPCHK(c, hipMalloc(&c->d_nbr, sizeof(int) * 4 * E));
     HIPCHK(c, hipMalloc(&c->d_invj, sizeof(double) * E));
@@ -111,6 +120,7 @@ int upload_mesh(tlw_ctx* c, const tlw_mesh_desc* m) {
     HIPCHK(c, hipMalloc(&c->d_buf[0], sizeof(double) * nd));
     HIPCHK(c, hipMalloc(&c->d_buf[1], sizeof(double) * nd));
     HIPCHK(c, hipMalloc(&c->d_us, sizeof(double) * nd));
+    HIPCHK(c, hipMalloc(&c->d_partial, sizeof(double) * (size_t)(E + 1)));
     if (c->flags & TLW_FLAG_KEEP_AUX) {
         HIPCHK(c, hipMalloc(&c->d_auxU, sizeof(double) * nd));
         HIPCHK(c, hipMalloc(&c->d_auxF, sizeof(double) * 2 * nd));
@@ -135,6 +145,7 @@ KArgs base_args(tlw_ctx* c) {
     a.length = c->d_length;
     a.c = c->d_const;
     a.st = c->d_state;
+    a.partial = c->d_partial;
     a.bc_scale = 1.0;
     return a;
 }
@@ -159,13 +170,27 @@ void launch_volume(tlw_ctx* c, const KArgs& a, int kind) {
     if (c->N == 4) launch_volume_t<4>(c, a, kind);
     else launch_volume_t<3>(c, a, kind);
 }
+template <int N> void launch_finalize_t(tlw_ctx* c, const KArgs& a) {
+    hipLaunchKernelGGL(k_finalize<N>, dim3(1), dim3(kFinalizeBlock), 0, c->stream, c->d_state, c->d_partial,
+                       grid_of<N>(c), a.reduce);
+}
+template <int N> void launch_surface_t(tlw_ctx* c, const KArgs& a) {
+    dim3 g(grid_of<N>(c)), b(kBlock);
+    if (c->has_boundary) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_surface<N, true>), g, b, 0, c->stream, a);
+    else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_surface<N, false>), g, b, 0, c->stream, a);
+    if (a.reduce != RED_NONE) launch_finalize_t<N>(c, a);
+}
 void launch_surface(tlw_ctx* c, const KArgs& a) {
-    if (c->N == 4) hipLaunchKernelGGL(k_surface<4>, dim3(grid_of<4>(c)), dim3(kBlock), 0, c->stream, a);
-    else hipLaunchKernelGGL(k_surface<3>, dim3(grid_of<3>(c)), dim3(kBlock), 0, c->stream, a);
+    if (c->N == 4) launch_surface_t<4>(c, a);
+    else launch_surface_t<3>(c, a);
+}
+template <int N> void launch_max_speed_t(tlw_ctx* c, const KArgs& a) {
+    hipLaunchKernelGGL(k_max_speed<N>, dim3(grid_of<N>(c)), dim3(kBlock), 0, c->stream, a);
+    launch_finalize_t<N>(c, a);
 }
 void launch_max_speed(tlw_ctx* c, const KArgs& a) {
-    if (c->N == 4) hipLaunchKernelGGL(k_max_speed<4>, dim3(grid_of<4>(c)), dim3(kBlock), 0, c->stream, a);
-    else hipLaunchKernelGGL(k_max_speed<3>, dim3(grid_of<3>(c)), dim3(kBlock), 0, c->stream, a);
+    if (c->N == 4) launch_max_speed_t<4>(c, a);
+    else launch_max_speed_t<3>(c, a);
 }
 
 int check_launch(tlw_ctx* c) {

@@ -53,8 +53,6 @@ struct DevConst {
 struct DevState {
     double t, dt, t_final, cfl;
     double query;                 // Trixi max_dt result of a query reduction
-    unsigned long long max_bits;  // running max of J⁻¹(max λ1 + max λ2) as bits (values >= 0)
-    unsigned int blocks_done;
     int finished;
     int cur;                      // state buffer holding uⁿ
     int bad_state;                // set when a non-admissible state was met
@@ -80,6 +78,7 @@ struct KArgs {
     DevState* st;
     double* auxU;  // optional full Ũ
     double* auxF;  // optional full F̃
+    double* partial;  // per-workgroup max of J⁻¹(max λ1 + max λ2) for the CFL reduction
     double t_arg, dt_arg;
     int use_state_dt;  // 1: t, dt from DevState (device loop); 0: from t_arg/dt_arg
     int step_mode;     // 1: out = u + dt * rhs; 0: out = rhs
@@ -88,10 +87,23 @@ struct KArgs {
 };
 
 // ---------------------------------------------------------------------------
-// Euler flux, both directions (Trixi flux(u, 1|2, CompressibleEulerEquations2D))
+// 1/x from the hardware reciprocal estimate and two Newton-Raphson steps
+// r <- r + r (1 - x r); the result is within a few ulp of the correctly
+// rounded reciprocal for the positive, normal densities it is applied to.
+__device__ __forceinline__ double rcp_nr(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
+// Euler flux, both directions (Trixi flux(u, 1|2, CompressibleEulerEquations2D)).
+// The velocities use one reciprocal of ρ instead of two divisions.
 __device__ __forceinline__ void euler_fluxes(const double* u, double gm1, double* f, double* g) {
     const double rho = u[0], m1 = u[1], m2 = u[2], E = u[3];
-    const double v1 = m1 / rho, v2 = m2 / rho;
+    const double ir = rcp_nr(rho);
+    const double v1 = m1 * ir, v2 = m2 * ir;
     const double p = gm1 * (E - 0.5 * (m1 * v1 + m2 * v2));
     f[0] = m1;
     f[1] = m1 * v1 + p;
@@ -120,6 +132,14 @@ __device__ __forceinline__ void speed_parts(const double* s, double gamma, int o
 }
 
 __device__ __forceinline__ double dmax(double x, double y) { return (y > x) ? y : x; }
+
+// Julia's x ≈ y for Float64 (atol = 0, rtol = sqrt(eps)): equal, or both
+// finite and |x - y| <= rtol max(|x|, |y|).  isfinished (lwfr.jl:146-148).
+__device__ __forceinline__ int julia_isapprox(double x, double y) {
+    if (x == y) return 1;
+    if (!isfinite(x) || !isfinite(y)) return 0;
+    return fabs(x - y) <= 1.4901161193847656e-08 * dmax(fabs(x), fabs(y));
+}
 
 // Boundary value functions of the time-averaged Dirichlet condition.
 __device__ inline void boundary_value(const DevConst* c, double x, double y, double t, double* u) {
@@ -783,50 +803,56 @@ __device__ inline void boundary_record(const KArgs& a, int e, int face, int l, d
     r[9] = c_;
 }
 
-template <int N>
+template <int N, bool BC>
 __device__ __forceinline__ void face_flux(const KArgs& a, int e, int face, int l, double t, double dt, double* Fs) {
     constexpr int n = N + 1;
     double own[kTraceQ], oth[kTraceQ];
     load_record(a.trace + ((size_t)e * 4 * n + face * n + l) * kTraceQ, own);
     const int ne = a.nbr[4 * e + face];
-    if (ne >= 0) load_record(a.trace + ((size_t)ne * 4 * n + (face ^ 1) * n + l) * kTraceQ, oth);
+    if (!BC || ne >= 0) load_record(a.trace + ((size_t)ne * 4 * n + (face ^ 1) * n + l) * kTraceQ, oth);
     else boundary_record<N>(a, e, face, l, t, dt, oth);
     if (face & 1) face_numflux(own, oth, Fs);  // +x, +y face: own side is left
     else face_numflux(oth, own, Fs);
 }
 
-// Last-workgroup finalisation of a reduction of J⁻¹(max λ1 + max λ2).
-template <int N> __device__ void finalize_reduction(const KArgs& a, double block_max) {
+// Final reduction of the per-workgroup partial maxima of J⁻¹(max λ1 + max λ2)
+// by one workgroup, then compute_dt / set_dt! / set_t_and_iter! / isfinished
+// on the device state (lwfr.jl:76-103, 146-148, 176-190; Trixi max_dt).
+constexpr int kFinalizeBlock = 1024;
+template <int N>
+__global__ __launch_bounds__(kFinalizeBlock) void k_finalize(DevState* st, const double* partial, int nparts,
+                                                           int reduce) {
     constexpr int n = N + 1;
-    DevState* st = a.st;
-    atomicMax(&st->max_bits, (unsigned long long)__double_as_longlong(block_max));
-    __threadfence();
-    const unsigned int prev = atomicAdd(&st->blocks_done, 1u);
-    if (prev != gridDim.x - 1) return;
-    __threadfence();
-    double M = __longlong_as_double((long long)atomicAdd(&st->max_bits, 0ull));
-    const double tiny = 4.9406564584124654e-324;  // nextfloat(0.0), Trixi's start value
-    M = dmax(tiny, M);
-    const double max_dt = 2.0 / (n * M);
-    if (a.reduce == RED_QUERY) {
-        st->query = max_dt;
-    } else {
-        double t = st->t;
-        if (a.reduce == RED_STEP) {
-            t = t + st->dt;  // set_t_and_iter!
-            st->t = t;
-            st->iter = st->iter + 1;
-            st->cur = 1 - st->cur;
-        }
-        const double tf = st->t_final;
-        const int fin = (t == tf) || (fabs(t - tf) <= 1.4901161193847656e-08 * dmax(fabs(t), fabs(tf)));
-        double dt = st->cfl * max_dt;  // compute_dt
-        if (t + dt > tf) dt = tf - t;  // set_dt!
-        st->dt = dt;
-        st->finished = fin;
+    __shared__ double s[kFinalizeBlock];
+    if (reduce == RED_STEP && st->finished) return;
+    double m = 0.0;
+    for (int k = threadIdx.x; k < nparts; k += kFinalizeBlock) m = dmax(m, partial[k]);
+    s[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = kFinalizeBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] = dmax(s[threadIdx.x], s[threadIdx.x + w]);
+        __syncthreads();
     }
-    st->max_bits = 0ull;
-    st->blocks_done = 0u;
+    if (threadIdx.x != 0) return;
+    const double tiny = 4.9406564584124654e-324;  // nextfloat(0.0), Trixi's start value
+    const double M = dmax(tiny, s[0]);
+    const double max_dt = 2.0 / (n * M);
+    if (reduce == RED_QUERY) {
+        st->query = max_dt;
+        return;
+    }
+    double t = st->t;
+    if (reduce == RED_STEP) {
+        t = t + st->dt;  // set_t_and_iter!
+        st->t = t;
+        st->iter = st->iter + 1;
+        st->cur = 1 - st->cur;
+    }
+    const double tf = st->t_final;
+    double dt = st->cfl * max_dt;  // compute_dt
+    if (t + dt > tf) dt = tf - t;  // set_dt!
+    st->dt = dt;
+    st->finished = julia_isapprox(t, tf);
 }
 
 // Per-element max of λ1 = |v1| + c and λ2 = |v2| + c (Trixi max_abs_speeds),
@@ -867,7 +893,7 @@ __device__ __forceinline__ double block_speed_max(const KArgs& a, const double* 
 // interface is evaluated by both neighbours from identical inputs, so both
 // see the same bits), weak-form surface integral with ℓ̂ = 1/w at the end
 // nodes, Jacobian, finalisation of the update, CFL reduction.
-template <int N>
+template <int N, bool BC>
 __global__ __launch_bounds__(kBlock) void k_surface(KArgs a) {
     TLW_ELEMENT_PROLOGUE(N)
     __shared__ double s_l1[Geo<N>::S], s_l2[Geo<N>::S], s_m[Geo<N>::EB];
@@ -885,22 +911,22 @@ __global__ __launch_bounds__(kBlock) void k_surface(KArgs a) {
     if (act && (i == 0 || i == n - 1 || j == 0 || j == n - 1)) {
         double surf[4] = {0.0, 0.0, 0.0, 0.0}, Fs[4];
         if (i == 0) {
-            face_flux<N>(a, e, 0, j, t, dt, Fs);
+            face_flux<N, BC>(a, e, 0, j, t, dt, Fs);
 #pragma unroll
             for (int v = 0; v < 4; ++v) surf[v] -= Fs[v] * c->bnd_lo;
         }
         if (i == n - 1) {
-            face_flux<N>(a, e, 1, j, t, dt, Fs);
+            face_flux<N, BC>(a, e, 1, j, t, dt, Fs);
 #pragma unroll
             for (int v = 0; v < 4; ++v) surf[v] += Fs[v] * c->bnd_hi;
         }
         if (j == 0) {
-            face_flux<N>(a, e, 2, i, t, dt, Fs);
+            face_flux<N, BC>(a, e, 2, i, t, dt, Fs);
 #pragma unroll
             for (int v = 0; v < 4; ++v) surf[v] -= Fs[v] * c->bnd_lo;
         }
         if (j == n - 1) {
-            face_flux<N>(a, e, 3, i, t, dt, Fs);
+            face_flux<N, BC>(a, e, 3, i, t, dt, Fs);
 #pragma unroll
             for (int v = 0; v < 4; ++v) surf[v] += Fs[v] * c->bnd_hi;
         }
@@ -913,7 +939,7 @@ __global__ __launch_bounds__(kBlock) void k_surface(KArgs a) {
     }
     if (a.reduce != RED_NONE) {
         const double bm = block_speed_max<N>(a, val, act, lec, node, ec, s_l1, s_l2, s_m);
-        if (threadIdx.x == 0) finalize_reduction<N>(a, bm);
+        if (threadIdx.x == 0) a.partial[blockIdx.x] = bm;
     }
 }
 
@@ -929,7 +955,7 @@ __global__ __launch_bounds__(kBlock) void k_max_speed(KArgs a) {
     double val[4];
     load4(uin + off, val);
     const double bm = block_speed_max<N>(a, val, act, lec, node, ec, s_l1, s_l2, s_m);
-    if (threadIdx.x == 0) finalize_reduction<N>(a, bm);
+    if (threadIdx.x == 0) a.partial[blockIdx.x] = bm;
 }
 
 // Initialisation of the device integrator state (one thread).
@@ -940,9 +966,7 @@ __global__ void k_init_state(DevState* st, double t0, double tf, double cfl, int
         st->t_final = tf;
         st->cfl = cfl;
         st->query = 0.0;
-        st->max_bits = 0ull;
-        st->blocks_done = 0u;
-        st->finished = (t0 == tf) || (fabs(t0 - tf) <= 1.4901161193847656e-08 * dmax(fabs(t0), fabs(tf)));
+        st->finished = julia_isapprox(t0, tf);
         st->cur = cur;
         st->bad_state = 0;
         st->iter = 0;
