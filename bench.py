@@ -88,6 +88,8 @@ def main():
     ap.add_argument("--workload", default=DEFAULT_WORKLOAD, choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a plain 1 GiB device copy (the reference kernel for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the sharding design is in "
@@ -95,6 +97,20 @@ def main():
         sys.exit(2)
 
     import torch
+
+    if args.copy_calibration:
+        a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(10):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        el = (time.perf_counter() - t0) / 10
+        log(f"copy calibration: 1 GiB read + 1 GiB write per copy, {2 * (1 << 30) / el / 1e9:.1f} GB/s")
+        return
 
     import trixilw_hip as tlw
     from trixilw_hip import _lib

@@ -825,9 +825,17 @@ __global__ __launch_bounds__(kFinalizeBlock) void k_finalize(DevState* st, const
     constexpr int n = N + 1;
     __shared__ double s[kFinalizeBlock];
     if (reduce == RED_STEP && st->finished) return;
-    double m = 0.0;
-    for (int k = threadIdx.x; k < nparts; k += kFinalizeBlock) m = dmax(m, partial[k]);
-    s[threadIdx.x] = m;
+    // eight independent accumulators keep several loads in flight per lane
+    double m[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int k = threadIdx.x;
+    for (; k + 7 * kFinalizeBlock < nparts; k += 8 * kFinalizeBlock) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) m[q] = dmax(m[q], partial[k + q * kFinalizeBlock]);
+    }
+    for (; k < nparts; k += kFinalizeBlock) m[0] = dmax(m[0], partial[k]);
+#pragma unroll
+    for (int q = 1; q < 8; ++q) m[0] = dmax(m[0], m[q]);
+    s[threadIdx.x] = m[0];
     __syncthreads();
     for (int w = kFinalizeBlock / 2; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) s[threadIdx.x] = dmax(s[threadIdx.x], s[threadIdx.x + w]);
