@@ -92,8 +92,8 @@ def main():
                     help="run only a plain 1 GiB device copy (the reference kernel for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (the sharding design is in "
-              "DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (element-block sharding with a "
+              "face-record exchange is a design only); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import torch
