@@ -60,14 +60,16 @@ def alg_bytes_per_dof(N: int, td: str):
 
 
 def load_traffic(workload: str):
+    """HBM bytes per launch of the dominant kernel, measured with rocprofv3
+    FETCH_SIZE/WRITE_SIZE scaled by a 1 GiB copy (committed summary)."""
     p = ROOT / "profiles" / "traffic.json"
     if not p.exists():
-        return None
+        return None, None
     try:
-        d = json.loads(p.read_text())
-        return d.get(workload)
+        d = json.loads(p.read_text()).get(workload)
+        return (float(d["bytes"]), d.get("source")) if d else (None, None)
     except Exception:
-        return None
+        return None, None
 
 
 def cpu_baseline(o, u0, cfl, mdrk, dofs, steps, threads):
@@ -163,7 +165,7 @@ def main():
     else:
         achieved = step_b * dofs / (launches_per_step * vol_avg_s) / 1e9
         rl_unit_bytes = step_b * dofs / launches_per_step
-    traffic = load_traffic(args.workload)
+    traffic, traffic_src = load_traffic(args.workload)
     roofline = {
         "bound": "hbm",
         "kernel": "k_volume_lw4" if (td == "lw" and N == 4) else ("k_volume_lw3" if td == "lw" else "k_volume_mdrk*"),
@@ -172,6 +174,7 @@ def main():
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": traffic,
+        "traffic_source": traffic_src,
         "alg_bytes_per_launch": rl_unit_bytes,
         "avg_launch_ms": round(vol_avg_s * 1e3, 4),
         "step_alg_bytes_per_dof": step_b,
