@@ -38,6 +38,7 @@ struct tlw_ctx {
     double* d_auxU = nullptr;
     double* d_auxF = nullptr;
     double* d_partial = nullptr;
+    double* d_elem_max = nullptr;
     bool has_boundary = false;
     int host_cur = 0;
     bool aux_valid = false;
@@ -67,6 +68,9 @@ int fail(tlw_ctx* c, int code, const std::string& msg) {
 
 size_t state_len(const tlw_ctx* c) { return (size_t)4 * c->n * c->n * c->E; }
 
+int vgrid(const tlw_ctx* c);
+int sgrid(const tlw_ctx* c);
+
 void free_mesh(tlw_ctx* c) {
     (void)hipFree(c->d_nbr);
     (void)hipFree(c->d_invj);
@@ -79,7 +83,9 @@ void free_mesh(tlw_ctx* c) {
     (void)hipFree(c->d_auxU);
     (void)hipFree(c->d_auxF);
     (void)hipFree(c->d_partial);
+    (void)hipFree(c->d_elem_max);
     c->d_partial = nullptr;
+    c->d_elem_max = nullptr;
     c->d_nbr = nullptr;
     c->d_invj = c->d_center = c->d_length = c->d_trace = c->d_us = c->d_auxU = c->d_auxF = nullptr;
     c->d_buf[0] = c->d_buf[1] = nullptr;
@@ -120,7 +126,8 @@ int upload_mesh(tlw_ctx* c, const tlw_mesh_desc* m) {
     HIPCHK(c, hipMalloc(&c->d_buf[0], sizeof(double) * nd));
     HIPCHK(c, hipMalloc(&c->d_buf[1], sizeof(double) * nd));
     HIPCHK(c, hipMalloc(&c->d_us, sizeof(double) * nd));
-    HIPCHK(c, hipMalloc(&c->d_partial, sizeof(double) * (size_t)(E + 1)));
+    HIPCHK(c, hipMalloc(&c->d_partial, sizeof(double) * (size_t)(vgrid(c) + sgrid(c) + 1)));
+    HIPCHK(c, hipMalloc(&c->d_elem_max, sizeof(double) * 2 * (size_t)E));
     if (c->flags & TLW_FLAG_KEEP_AUX) {
         HIPCHK(c, hipMalloc(&c->d_auxU, sizeof(double) * nd));
         HIPCHK(c, hipMalloc(&c->d_auxF, sizeof(double) * 2 * nd));
@@ -146,47 +153,52 @@ KArgs base_args(tlw_ctx* c) {
     a.c = c->d_const;
     a.st = c->d_state;
     a.partial = c->d_partial;
+    a.elem_max = c->d_elem_max;
+    a.vol_grid = vgrid(c);
     a.bc_scale = 1.0;
     return a;
 }
 
-template <int N> int grid_of(const tlw_ctx* c) { return (c->E + Geo<N>::EB - 1) / Geo<N>::EB; }
+// Elements per volume workgroup.  N=3: 16 (256 lanes, 16 consecutive Z-ordered
+// elements form a 4x4 tree block); N=4: 15 (375 of 384 lanes, two workgroups per
+// CU at three waves per SIMD).
+template <int N> constexpr int kEBV = (N == 4) ? 15 : 16;
+
+template <int N> int vgrid_of(const tlw_ctx* c) { return (c->E + kEBV<N> - 1) / kEBV<N>; }
+template <int N> int sgrid_of(const tlw_ctx* c) { return (c->E + SGeo<N>::EB - 1) / SGeo<N>::EB; }
+int vgrid(const tlw_ctx* c) { return c->N == 4 ? vgrid_of<4>(c) : vgrid_of<3>(c); }
+int sgrid(const tlw_ctx* c) { return c->N == 4 ? sgrid_of<4>(c) : sgrid_of<3>(c); }
 
 // Kernel launches, dispatched on the polynomial degree.
-enum VolKind { VOL_LW = 0, VOL_MDRK1 = 1, VOL_MDRK2 = 2 };
-
 template <int N> void launch_volume_t(tlw_ctx* c, const KArgs& a, int kind) {
-    dim3 g(grid_of<N>(c)), b(kBlock);
-    if (kind == VOL_LW) {
-        if constexpr (N == 4) hipLaunchKernelGGL(k_volume_lw4<4>, g, b, 0, c->stream, a);
-        else hipLaunchKernelGGL(k_volume_lw3<3>, g, b, 0, c->stream, a);
-    } else if (kind == VOL_MDRK1) {
-        hipLaunchKernelGGL(k_volume_mdrk1<N>, g, b, 0, c->stream, a);
-    } else {
-        hipLaunchKernelGGL(k_volume_mdrk2<N>, g, b, 0, c->stream, a);
-    }
+    constexpr int EB = kEBV<N>;
+    dim3 g(vgrid_of<N>(c)), b(VGeo<N, EB>::NT);
+    if (kind == VOL_LW) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_LW>), g, b, 0, c->stream, a);
+    else if (kind == VOL_MDRK1)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_MDRK1>), g, b, 0, c->stream, a);
+    else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_MDRK2>), g, b, 0, c->stream, a);
 }
 void launch_volume(tlw_ctx* c, const KArgs& a, int kind) {
     if (c->N == 4) launch_volume_t<4>(c, a, kind);
     else launch_volume_t<3>(c, a, kind);
 }
-template <int N> void launch_finalize_t(tlw_ctx* c, const KArgs& a) {
-    hipLaunchKernelGGL(k_finalize<N>, dim3(1), dim3(kFinalizeBlock), 0, c->stream, c->d_state, c->d_partial,
-                       grid_of<N>(c), a.reduce);
+template <int N> void launch_finalize_t(tlw_ctx* c, int reduce, int nparts) {
+    hipLaunchKernelGGL(k_finalize<N>, dim3(1), dim3(kFinalizeBlock), 0, c->stream, c->d_state, c->d_partial, nparts,
+                       reduce);
 }
 template <int N> void launch_surface_t(tlw_ctx* c, const KArgs& a) {
-    dim3 g(grid_of<N>(c)), b(kBlock);
-    if (c->has_boundary) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_surface<N, true>), g, b, 0, c->stream, a);
-    else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_surface<N, false>), g, b, 0, c->stream, a);
-    if (a.reduce != RED_NONE) launch_finalize_t<N>(c, a);
+    dim3 g(sgrid_of<N>(c)), b(kSurfBlock);
+    if (c->has_boundary) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_surface<N, kEBV<N>, true>), g, b, 0, c->stream, a);
+    else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_surface<N, kEBV<N>, false>), g, b, 0, c->stream, a);
+    if (a.reduce != RED_NONE) launch_finalize_t<N>(c, a.reduce, vgrid_of<N>(c) + sgrid_of<N>(c));
 }
 void launch_surface(tlw_ctx* c, const KArgs& a) {
     if (c->N == 4) launch_surface_t<4>(c, a);
     else launch_surface_t<3>(c, a);
 }
 template <int N> void launch_max_speed_t(tlw_ctx* c, const KArgs& a) {
-    hipLaunchKernelGGL(k_max_speed<N>, dim3(grid_of<N>(c)), dim3(kBlock), 0, c->stream, a);
-    launch_finalize_t<N>(c, a);
+    hipLaunchKernelGGL(k_max_speed<N>, dim3(sgrid_of<N>(c)), dim3(kSurfBlock), 0, c->stream, a);
+    launch_finalize_t<N>(c, a.reduce, sgrid_of<N>(c));
 }
 void launch_max_speed(tlw_ctx* c, const KArgs& a) {
     if (c->N == 4) launch_max_speed_t<4>(c, a);

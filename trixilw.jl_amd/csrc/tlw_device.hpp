@@ -1,17 +1,22 @@
 // tlw_device.hpp -- gfx950 kernels of the TreeMesh Lax-Wendroff / MDRK step.
 //
-// Work decomposition: one thread per GLL node, EB elements per 128-thread
-// workgroup (N=4: 5 elements = 125 lanes; N=3: 8 elements = 128 lanes).  Each
-// element's fluxes and flux time derivatives are exchanged through LDS for the
-// derivative-matrix contractions; everything else stays in registers.
+// Work decomposition: one thread per GLL node; a volume workgroup owns EB
+// consecutive elements (with Z-ordered meshes, 16 consecutive elements form
+// a 4x4 block of the tree).  Fluxes and flux time derivatives are exchanged
+// through LDS for the derivative-matrix contractions; Taylor states, Ũ and F̃
+// stay in registers.
 //
-// One step is two kernels:
-//   volume   uⁿ -> approximate LW procedure (Taylor states, finite-difference
-//            flux time derivatives, Ũ, F̃), volume term D̂F̃, partial update of
-//            every node, and a compact face record per element face node
-//   surface  Rusanov flux on the time-averaged F̃/Ũ from the two face records,
-//            FR correction of the face nodes, finalisation of the update,
-//            CFL reduction of the new state and (last workgroup) dt, t update
+// One step is three launches:
+//   volume    uⁿ -> approximate LW procedure, volume term D̂F̃, then the
+//             Rusanov flux of every interface whose two elements belong to
+//             the workgroup (both face records are in LDS), the lifted
+//             surface terms, the Jacobian and the update of every node.
+//             Faces leaving the workgroup write a 10-double face record.
+//   surface   the remaining faces: Rusanov flux from the two face records
+//             (or the time-averaged Dirichlet state on a domain boundary),
+//             lifted correction of those face nodes.
+//   finalize  CFL reduction and compute_dt / set_dt! / set_t_and_iter! /
+//             isfinished on the device state.
 // The reference computes the same quantities in
 //   lw_volume_kernel_3!/4!         src/solvers/dgsem_tree/dg_2d.jl:674-1267
 //   prolong2interfaces!            src/solvers/dgsem_tree/dg_2d.jl:1269-1326
@@ -26,14 +31,23 @@
 
 namespace tlw {
 
-constexpr int kBlock = 128;
 constexpr int kTraceQ = 10;  // face record: jump state (4), normal flux (4), |v_n|, c
+constexpr int kSurfBlock = 128;
 
-template <int N> struct Geo {
+// Volume workgroup geometry: EB elements, NT threads (a multiple of 64).
+template <int N, int EB> struct VGeo {
     static constexpr int n = N + 1;
     static constexpr int NN = n * n;
-    static constexpr int EB = kBlock / NN;  // elements per workgroup
-    static constexpr int S = EB * NN;       // LDS stride of one variable
+    static constexpr int S = EB * NN;  // LDS stride of one exchanged variable
+    static constexpr int NT = ((EB * NN + 63) / 64) * 64;
+    static constexpr int LDS = 3 * 8 * S;  // three exchange buffers
+};
+// Surface / reduction workgroup geometry: 128 threads.
+template <int N> struct SGeo {
+    static constexpr int n = N + 1;
+    static constexpr int NN = n * n;
+    static constexpr int EB = kSurfBlock / NN;
+    static constexpr int S = EB * NN;
 };
 
 // Basis and boundary data, uploaded once per context.
@@ -49,36 +63,39 @@ struct DevConst {
 };
 
 // Device-resident integrator state (LWIntegrator fields t, dt, iter and the
-// isfinished test), updated by the last workgroup of each surface kernel.
+// isfinished test), updated by the finalize kernel.
 struct DevState {
     double t, dt, t_final, cfl;
-    double query;                 // Trixi max_dt result of a query reduction
+    double query;   // Trixi max_dt result of a query reduction
     int finished;
-    int cur;                      // state buffer holding uⁿ
-    int bad_state;                // set when a non-admissible state was met
+    int cur;        // state buffer holding uⁿ
+    int bad_state;  // set when a non-admissible state was met
     long long iter;
 };
 
 enum BufSel { SEL_EXPLICIT = 0, SEL_CUR = 1, SEL_OTHER = 2 };
 enum Reduce { RED_NONE = 0, RED_STEP = 1, RED_INIT = 2, RED_QUERY = 3 };
+enum VolKind { VOL_LW = 0, VOL_MDRK1 = 1, VOL_MDRK2 = 2 };
 
 struct KArgs {
     int E;
     const double* u;      // uⁿ (explicit)
     const double* us;     // MDRK stage state
-    double* out;          // volume: partial result; surface: finalised in place
+    double* out;          // volume: result; surface: corrected in place
     double* bufs[2];      // ping-pong state buffers for the device-resident loop
     int in_sel, out_sel;  // BufSel
-    double* trace;        // [E][4][n][kTraceQ]
+    double* trace;        // [E][4][n][kTraceQ], written for faces leaving a volume workgroup
     const int* nbr;       // [E][4]
     const double* inv_jac;
     const double* center;  // [E][2]
     const double* length;  // [E]
     const DevConst* c;
     DevState* st;
-    double* auxU;  // optional full Ũ
-    double* auxF;  // optional full F̃
-    double* partial;  // per-workgroup max of J⁻¹(max λ1 + max λ2) for the CFL reduction
+    double* auxU;      // optional full Ũ
+    double* auxF;      // optional full F̃
+    double* partial;   // per-workgroup max of J⁻¹(max λ1 + max λ2): volume part, then surface part
+    double* elem_max;  // [E][2]: max λ1, λ2 over the nodes the volume kernel finalised
+    int vol_grid;      // number of volume workgroups (offset of the surface partials)
     double t_arg, dt_arg;
     int use_state_dt;  // 1: t, dt from DevState (device loop); 0: from t_arg/dt_arg
     int step_mode;     // 1: out = u + dt * rhs; 0: out = rhs
@@ -175,12 +192,17 @@ __device__ inline void boundary_value(const DevConst* c, double x, double y, dou
     u[3] = p / (gamma - 1) + 0.5 * rho * (v1 * v1 + v2 * v2);
 }
 
-// Derivative contraction of one element: out[v] = Σ_k Di[k] s_f[v][j,k] + Σ_k Dj[k] s_g[v][k,i]
-// where s holds f (variables 0..3) and g (4..7) of every node of the workgroup.
-template <int N>
-__device__ __forceinline__ void contract(const double* s, int base, int i, int j, const double* Di, const double* Dj,
-                                         double* out) {
-    constexpr int n = Geo<N>::n, S = Geo<N>::S;
+// Derivative contraction of one element: out[v] = Σ_k M[i,k] s_f[v][j,k] + Σ_k M[j,k] s_g[v][k,i]
+// where s holds f (variables 0..3) and g (4..7) of every node of the workgroup
+// with stride S between variables, and M (n x n, row-major) is in LDS.
+template <int n, int S>
+__device__ __forceinline__ void contract(const double* s, int base, int i, int j, const double* M, double* out) {
+    double Di[n], Dj[n];
+#pragma unroll
+    for (int k = 0; k < n; ++k) {
+        Di[k] = M[i * n + k];
+        Dj[k] = M[j * n + k];
+    }
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         double acc = 0.0;
@@ -192,43 +214,12 @@ __device__ __forceinline__ void contract(const double* s, int base, int i, int j
     }
 }
 
-template <int N> __device__ __forceinline__ void put8(double* s, int idx, const double* f, const double* g) {
-    constexpr int S = Geo<N>::S;
+template <int S> __device__ __forceinline__ void put8(double* s, int idx, const double* f, const double* g) {
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         s[v * S + idx] = f[v];
         s[(4 + v) * S + idx] = g[v];
     }
-}
-
-// Face record of one element face node.
-__device__ __forceinline__ void write_record(double* rec, const double* jump, const double* Fn, const double* lam_state,
-                                             double gamma, int orientation) {
-    double a, c;
-    speed_parts(lam_state, gamma, orientation, a, c);
-    double2* r2 = reinterpret_cast<double2*>(rec);
-    r2[0] = make_double2(jump[0], jump[1]);
-    r2[1] = make_double2(jump[2], jump[3]);
-    r2[2] = make_double2(Fn[0], Fn[1]);
-    r2[3] = make_double2(Fn[2], Fn[3]);
-    r2[4] = make_double2(a, c);
-}
-
-// Writes the face records of node (i, j) (jump state, normal component of
-// F̃, λ ingredients): the data prolong2interfaces!/prolong2boundaries! copy.
-template <int N>
-__device__ __forceinline__ void write_traces(const KArgs& a, int e, int i, int j, const double* uN, const double* Uc,
-                                             const double* F, const double* G) {
-    constexpr int n = N + 1;
-    const bool tree = a.c->flux_order == 0;
-    const double* jump = tree ? uN : Uc;
-    const double* lam = tree ? Uc : uN;
-    const double gamma = a.c->gamma;
-    double* tr = a.trace + (size_t)e * 4 * n * kTraceQ;
-    if (i == 0) write_record(tr + (0 * n + j) * kTraceQ, jump, F, lam, gamma, 1);
-    if (i == n - 1) write_record(tr + (1 * n + j) * kTraceQ, jump, F, lam, gamma, 1);
-    if (j == 0) write_record(tr + (2 * n + i) * kTraceQ, jump, G, lam, gamma, 2);
-    if (j == n - 1) write_record(tr + (3 * n + i) * kTraceQ, jump, G, lam, gamma, 2);
 }
 
 __device__ __forceinline__ void load4(const double* p, double* u) {
@@ -242,6 +233,21 @@ __device__ __forceinline__ void store4(double* p, const double* u) {
     p2[1] = make_double2(u[2], u[3]);
 }
 
+__device__ __forceinline__ void load_record(const double* p, double* r) {
+    const double2* p2 = reinterpret_cast<const double2*>(p);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        const double2 x = p2[k];
+        r[2 * k] = x.x;
+        r[2 * k + 1] = x.y;
+    }
+}
+__device__ __forceinline__ void store_record(double* p, const double* r) {
+    double2* p2 = reinterpret_cast<double2*>(p);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) p2[k] = make_double2(r[2 * k], r[2 * k + 1]);
+}
+
 __device__ __forceinline__ const double* sel_in(const KArgs& a, int cur) {
     return a.in_sel == SEL_CUR ? a.bufs[cur] : (a.in_sel == SEL_OTHER ? a.bufs[1 - cur] : a.u);
 }
@@ -249,81 +255,66 @@ __device__ __forceinline__ double* sel_out(const KArgs& a, int cur) {
     return a.out_sel == SEL_CUR ? a.bufs[cur] : (a.out_sel == SEL_OTHER ? a.bufs[1 - cur] : a.out);
 }
 
-// Writes the volume result: rhs mode out = -J⁻¹ vol; step mode out = u + dt (-J⁻¹ vol).
-__device__ __forceinline__ void emit_volume(const KArgs& a, double* out, size_t off, const double* uN, const double* vol,
-                                            double invJ, double dt) {
-    double r[4];
+// Rusanov flux of one face node from the two face records (left = -side):
+// F* = ½(F̃n_L + F̃n_R) − ½ λ (q_R − q_L), λ = max(|v_n|) + max(c)
+// (FluxPlusDissipation with flux_central and DissipationLocalLaxFriedrichs,
+// src/equations/numerical_fluxes.jl:7-38).
+__device__ __forceinline__ void face_numflux(const double* L, const double* R, double* Fs) {
+    const double lam = dmax(L[8], R[8]) + dmax(L[9], R[9]);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) Fs[v] = 0.5 * (L[4 + v] + R[4 + v]) + (-0.5 * lam) * (R[v] - L[v]);
+}
+
+// Face record of a face node: jump state, normal component of F̃ and the λ
+// ingredients of the λ state (TreeMesh order: λ from Ũ, jump on uⁿ).
+__device__ __forceinline__ void make_record(double* r, const double* jump, const double* Fn, const double* lam_state,
+                                            double gamma, int orientation) {
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-        const double d = (-invJ) * vol[v];
-        r[v] = a.step_mode ? fma(dt, d, uN[v]) : d;
+        r[v] = jump[v];
+        r[4 + v] = Fn[v];
     }
-    store4(out + off, r);
+    speed_parts(lam_state, gamma, orientation, r[8], r[9]);
 }
 
-template <int N>
-__device__ __forceinline__ void load_drows(const DevConst* c, int i, int j, double scale, double* Di, double* Dj) {
-    constexpr int n = N + 1;
-#pragma unroll
-    for (int k = 0; k < n; ++k) {
-        Di[k] = scale * c->D[i][k];
-        Dj[k] = scale * c->D[j][k];
-    }
+// Per-node maxima λ1 = |v1| + c, λ2 = |v2| + c (Trixi max_abs_speeds).
+__device__ __forceinline__ void node_speeds(const double* un, double gamma, double& l1, double& l2, bool& bad) {
+    const double rho = un[0], v1 = un[1] / rho, v2 = un[2] / rho;
+    const double p = (gamma - 1.0) * (un[3] - 0.5 * (un[1] * v1 + un[2] * v2));
+    bad = !(rho > 0.0) || !(p > 0.0);
+    const double cs = sqrt(gamma * p / rho);
+    l1 = fabs(v1) + cs;
+    l2 = fabs(v2) + cs;
 }
-
-// Shared prologue of the element kernels.
-#define TLW_ELEMENT_PROLOGUE(N)                                  \
-    constexpr int n = Geo<N>::n, NN = Geo<N>::NN, EB = Geo<N>::EB; \
-    const int tid = threadIdx.x;                                   \
-    const int le = tid / NN, node = tid - (tid / NN) * NN;         \
-    const int i = node % n, j = node / n;                          \
-    const int e = blockIdx.x * EB + le;                            \
-    const bool act = (le < EB) && (e < a.E);                       \
-    const int ec = e < a.E ? e : a.E - 1;                          \
-    const int lec = le < EB ? le : EB - 1;                         \
-    const int sidx = lec * NN + node;                              \
-    const int sbase = lec * NN;                                    \
-    (void)sidx; (void)sbase; (void)act;
 
 // ---------------------------------------------------------------------------
-// LW volume kernel, N = 4: lw_volume_kernel_4!, dg_2d.jl:918-1267 (the
-// temporal-error estimate at 1183-1199, 1251-1265 is an adaptive-mode output
-// and is not computed).
-template <int N>
-__global__ __launch_bounds__(kBlock) void k_volume_lw4(KArgs a) {
-    TLW_ELEMENT_PROLOGUE(N)
-    static_assert(N == 4, "N = 4 kernel");
-    __shared__ double s0[8 * Geo<N>::S], s1[8 * Geo<N>::S], s2[8 * Geo<N>::S];
-    DevState* st = a.st;
-    if (a.use_state_dt && st->finished) return;
-    const int cur = a.use_state_dt ? st->cur : 0;
-    const double* uin = sel_in(a, cur);
-    double* out = sel_out(a, cur);
-    const double dt = a.use_state_dt ? st->dt : a.dt_arg;
-    const DevConst* c = a.c;
-    const double gm1 = c->gamma - 1.0;
-    const double invJ = a.inv_jac[ec];
-    const size_t off = ((size_t)ec * NN + node) * 4;
+// Element volume computations.  Each fills uN (state), Uc (Ũ), F, G (F̃) and
+// vol (D̂F̃ contraction) for the calling node; the final F̃ exchange is in s2,
+// so s0 and s1 are free once every thread has passed the last barrier.
 
-    double Di[n], Dj[n];
-    load_drows<N>(c, i, j, -dt, Di, Dj);
-    double u[4];
-    load4(uin + off, u);
-    double F[4], G[4];
-    euler_fluxes(u, gm1, F, G);
-    if (act) put8<N>(s0, sidx, F, G);
+// lw_volume_kernel_4!, dg_2d.jl:918-1267 (the temporal-error estimate at
+// 1183-1199, 1251-1265 is an adaptive-mode output and is not computed).
+template <int N, int EB>
+__device__ __forceinline__ void lw4_element(const double* uN, double dt, double invJ, double gm1, const DevConst* c,
+                                            int i, int j, bool act, int sidx, int sbase, double* s0, double* s1,
+                                            double* s2, const double* sDm, const double* sDh, double* Uc, double* F,
+                                            double* G, double* vol) {
+    using G_ = VGeo<N, EB>;
+    constexpr int n = G_::n, S = G_::S;
+    euler_fluxes(uN, gm1, F, G);
+    if (act) put8<S>(s0, sidx, F, G);
     __syncthreads();
     double ut[4];
-    contract<N>(s0, sbase, i, j, Di, Dj, ut);
-    double Uc[4], up[4], um[4], upp[4], umm[4];
+    contract<n, S>(s0, sbase, i, j, sDm, ut);
+    double up[4], um[4], upp[4], umm[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         ut[v] *= invJ;
-        Uc[v] = fma(0.5, ut[v], u[v]);
-        up[v] = u[v] + ut[v];
-        um[v] = u[v] - ut[v];
-        umm[v] = fma(-2.0, ut[v], u[v]);
-        upp[v] = fma(2.0, ut[v], u[v]);
+        Uc[v] = fma(0.5, ut[v], uN[v]);
+        up[v] = uN[v] + ut[v];
+        um[v] = uN[v] - ut[v];
+        umm[v] = fma(-2.0, ut[v], uN[v]);
+        upp[v] = fma(2.0, ut[v], uN[v]);
     }
     double fa[4], ga[4], dx[4], dy[4];
     // stage 1: f_t = (-f(u+2) + 8 f(u+1) - 8 f(u-1) + f(u-2)) / 12
@@ -344,10 +335,10 @@ __global__ __launch_bounds__(kBlock) void k_volume_lw4(KArgs a) {
         F[v] = fma(0.5, dx[v], F[v]);
         G[v] = fma(0.5, dy[v], G[v]);
     }
-    if (act) put8<N>(s1, sidx, dx, dy);
+    if (act) put8<S>(s1, sidx, dx, dy);
     __syncthreads();
     double uk[4];
-    contract<N>(s1, sbase, i, j, Di, Dj, uk);
+    contract<n, S>(s1, sbase, i, j, sDm, uk);
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         const double w = uk[v] * invJ;  // utt
@@ -366,8 +357,8 @@ __global__ __launch_bounds__(kBlock) void k_volume_lw4(KArgs a) {
     for (int v = 0; v < 4; ++v) {
         dx[v] = fma(16.0, fa[v], dx[v]);
         dy[v] = fma(16.0, ga[v], dy[v]);
-        dx[v] = fma(-30.0, s0[v * Geo<N>::S + sidx], dx[v]);
-        dy[v] = fma(-30.0, s0[(4 + v) * Geo<N>::S + sidx], dy[v]);
+        dx[v] = fma(-30.0, s0[v * S + sidx], dx[v]);
+        dy[v] = fma(-30.0, s0[(4 + v) * S + sidx], dy[v]);
     }
     euler_fluxes(um, gm1, fa, ga);
 #pragma unroll
@@ -380,9 +371,9 @@ __global__ __launch_bounds__(kBlock) void k_volume_lw4(KArgs a) {
         F[v] = fma(1.0 / 6.0, dx[v], F[v]);
         G[v] = fma(1.0 / 6.0, dy[v], G[v]);
     }
-    if (act) put8<N>(s2, sidx, dx, dy);
+    if (act) put8<S>(s2, sidx, dx, dy);
     __syncthreads();
-    contract<N>(s2, sbase, i, j, Di, Dj, uk);
+    contract<n, S>(s2, sbase, i, j, sDm, uk);
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         const double w = uk[v] * invJ;  // uttt
@@ -410,9 +401,9 @@ __global__ __launch_bounds__(kBlock) void k_volume_lw4(KArgs a) {
         F[v] = fma(1.0 / 24.0, dx[v], F[v]);
         G[v] = fma(1.0 / 24.0, dy[v], G[v]);
     }
-    if (act) put8<N>(s1, sidx, dx, dy);
+    if (act) put8<S>(s1, sidx, dx, dy);
     __syncthreads();
-    contract<N>(s1, sbase, i, j, Di, Dj, uk);
+    contract<n, S>(s1, sbase, i, j, sDm, uk);
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         const double w = uk[v] * invJ;  // utttt
@@ -432,8 +423,8 @@ __global__ __launch_bounds__(kBlock) void k_volume_lw4(KArgs a) {
     for (int v = 0; v < 4; ++v) {
         dx[v] = fma(-4.0, fa[v], dx[v]);
         dy[v] = fma(-4.0, ga[v], dy[v]);
-        dx[v] = fma(6.0, s0[v * Geo<N>::S + sidx], dx[v]);
-        dy[v] = fma(6.0, s0[(4 + v) * Geo<N>::S + sidx], dy[v]);
+        dx[v] = fma(6.0, s0[v * S + sidx], dx[v]);
+        dy[v] = fma(6.0, s0[(4 + v) * S + sidx], dy[v]);
     }
     euler_fluxes(um, gm1, fa, ga);
 #pragma unroll
@@ -446,60 +437,33 @@ __global__ __launch_bounds__(kBlock) void k_volume_lw4(KArgs a) {
         F[v] = fma(1.0 / 120.0, dx[v], F[v]);
         G[v] = fma(1.0 / 120.0, dy[v], G[v]);
     }
-    if (act) put8<N>(s2, sidx, F, G);
+    if (act) put8<S>(s2, sidx, F, G);
     __syncthreads();
-    double Hi[n], Hj[n];
-#pragma unroll
-    for (int k = 0; k < n; ++k) { Hi[k] = c->Dhat[i][k]; Hj[k] = c->Dhat[j][k]; }
-    double vol[4];
-    contract<N>(s2, sbase, i, j, Hi, Hj, vol);
-    if (act) {
-        emit_volume(a, out, off, u, vol, invJ, dt);
-        write_traces<N>(a, e, i, j, u, Uc, F, G);
-        if (a.auxU) {
-            store4(a.auxU + off, Uc);
-            store4(a.auxF + ((size_t)e * NN + node) * 8, F);
-            store4(a.auxF + ((size_t)e * NN + node) * 8 + 4, G);
-        }
-    }
+    contract<n, S>(s2, sbase, i, j, sDh, vol);
 }
 
-// LW volume kernel, N = 3: lw_volume_kernel_3!, dg_2d.jl:674-916.
-template <int N>
-__global__ __launch_bounds__(kBlock) void k_volume_lw3(KArgs a) {
-    TLW_ELEMENT_PROLOGUE(N)
-    static_assert(N == 3, "N = 3 kernel");
-    __shared__ double s0[8 * Geo<N>::S], s1[8 * Geo<N>::S], s2[8 * Geo<N>::S];
-    DevState* st = a.st;
-    if (a.use_state_dt && st->finished) return;
-    const int cur = a.use_state_dt ? st->cur : 0;
-    const double* uin = sel_in(a, cur);
-    double* out = sel_out(a, cur);
-    const double dt = a.use_state_dt ? st->dt : a.dt_arg;
-    const DevConst* c = a.c;
-    const double gm1 = c->gamma - 1.0;
-    const double invJ = a.inv_jac[ec];
-    const size_t off = ((size_t)ec * NN + node) * 4;
-
-    double Di[n], Dj[n];
-    load_drows<N>(c, i, j, -dt, Di, Dj);
-    double u[4];
-    load4(uin + off, u);
-    double F[4], G[4];
-    euler_fluxes(u, gm1, F, G);
-    if (act) put8<N>(s0, sidx, F, G);
+// lw_volume_kernel_3!, dg_2d.jl:674-916
+template <int N, int EB>
+__device__ __forceinline__ void lw3_element(const double* uN, double dt, double invJ, double gm1, const DevConst* c,
+                                            int i, int j, bool act, int sidx, int sbase, double* s0, double* s1,
+                                            double* s2, const double* sDm, const double* sDh, double* Uc, double* F,
+                                            double* G, double* vol) {
+    using G_ = VGeo<N, EB>;
+    constexpr int n = G_::n, S = G_::S;
+    euler_fluxes(uN, gm1, F, G);
+    if (act) put8<S>(s0, sidx, F, G);
     __syncthreads();
     double ut[4];
-    contract<N>(s0, sbase, i, j, Di, Dj, ut);
-    double Uc[4], up[4], um[4], upp[4], umm[4];
+    contract<n, S>(s0, sbase, i, j, sDm, ut);
+    double up[4], um[4], upp[4], umm[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         ut[v] *= invJ;
-        Uc[v] = fma(0.5, ut[v], u[v]);
-        up[v] = u[v] + ut[v];
-        um[v] = u[v] - ut[v];
-        umm[v] = fma(-2.0, ut[v], u[v]);
-        upp[v] = fma(2.0, ut[v], u[v]);
+        Uc[v] = fma(0.5, ut[v], uN[v]);
+        up[v] = uN[v] + ut[v];
+        um[v] = uN[v] - ut[v];
+        umm[v] = fma(-2.0, ut[v], uN[v]);
+        upp[v] = fma(2.0, ut[v], uN[v]);
     }
     double fa[4], ga[4], dx[4], dy[4];
     euler_fluxes(upp, gm1, fa, ga);
@@ -519,10 +483,10 @@ __global__ __launch_bounds__(kBlock) void k_volume_lw3(KArgs a) {
         F[v] = fma(0.5, dx[v], F[v]);
         G[v] = fma(0.5, dy[v], G[v]);
     }
-    if (act) put8<N>(s1, sidx, dx, dy);
+    if (act) put8<S>(s1, sidx, dx, dy);
     __syncthreads();
     double uk[4];
-    contract<N>(s1, sbase, i, j, Di, Dj, uk);
+    contract<n, S>(s1, sbase, i, j, sDm, uk);
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         const double w = uk[v] * invJ;  // utt
@@ -536,8 +500,8 @@ __global__ __launch_bounds__(kBlock) void k_volume_lw3(KArgs a) {
     euler_fluxes(up, gm1, fa, ga);
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-        dx[v] = fma(-2.0, s0[v * Geo<N>::S + sidx], fa[v]);
-        dy[v] = fma(-2.0, s0[(4 + v) * Geo<N>::S + sidx], ga[v]);
+        dx[v] = fma(-2.0, s0[v * S + sidx], fa[v]);
+        dy[v] = fma(-2.0, s0[(4 + v) * S + sidx], ga[v]);
     }
     euler_fluxes(um, gm1, fa, ga);
 #pragma unroll
@@ -547,9 +511,9 @@ __global__ __launch_bounds__(kBlock) void k_volume_lw3(KArgs a) {
         F[v] = fma(1.0 / 6.0, dx[v], F[v]);
         G[v] = fma(1.0 / 6.0, dy[v], G[v]);
     }
-    if (act) put8<N>(s2, sidx, dx, dy);
+    if (act) put8<S>(s2, sidx, dx, dy);
     __syncthreads();
-    contract<N>(s2, sbase, i, j, Di, Dj, uk);
+    contract<n, S>(s2, sbase, i, j, sDm, uk);
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         const double w = uk[v] * invJ;  // uttt
@@ -577,22 +541,10 @@ __global__ __launch_bounds__(kBlock) void k_volume_lw3(KArgs a) {
         F[v] = fma(1.0 / 24.0, dx[v], F[v]);
         G[v] = fma(1.0 / 24.0, dy[v], G[v]);
     }
-    if (act) put8<N>(s1, sidx, F, G);
+    __syncthreads();  // every thread has finished reading s2 before it is reused
+    if (act) put8<S>(s2, sidx, F, G);
     __syncthreads();
-    double Hi[n], Hj[n];
-#pragma unroll
-    for (int k = 0; k < n; ++k) { Hi[k] = c->Dhat[i][k]; Hj[k] = c->Dhat[j][k]; }
-    double vol[4];
-    contract<N>(s1, sbase, i, j, Hi, Hj, vol);
-    if (act) {
-        emit_volume(a, out, off, u, vol, invJ, dt);
-        write_traces<N>(a, e, i, j, u, Uc, F, G);
-        if (a.auxU) {
-            store4(a.auxU + off, Uc);
-            store4(a.auxF + ((size_t)e * NN + node) * 8, F);
-            store4(a.auxF + ((size_t)e * NN + node) * 8 + 4, G);
-        }
-    }
+    contract<n, S>(s2, sbase, i, j, sDh, vol);
 }
 
 // f_t of the two-level Taylor states u ± ut, u ± 2ut:
@@ -622,102 +574,66 @@ __device__ __forceinline__ void mdrk_ft(const double* u, const double* ut, doubl
     }
 }
 
-// MDRK stage 1 volume kernel: mdrk_kernel_1!, tree/multi_derivative_2d.jl:263-420.
-// F̃ = f/2 + f_t/8, Ũ = u/2 + ut/8; F2 = f + f_t/6 and U2 = u + ut/6 are
-// recomputed by the stage-2 kernel instead of being stored.
-template <int N>
-__global__ __launch_bounds__(kBlock) void k_volume_mdrk1(KArgs a) {
-    TLW_ELEMENT_PROLOGUE(N)
-    __shared__ double s0[8 * Geo<N>::S], s1[8 * Geo<N>::S];
-    DevState* st = a.st;
-    if (a.use_state_dt && st->finished) return;
-    const int cur = a.use_state_dt ? st->cur : 0;
-    const double* uin = sel_in(a, cur);
-    double* out = sel_out(a, cur);
-    const double dt = a.use_state_dt ? st->dt : a.dt_arg;
-    const DevConst* c = a.c;
-    const double gm1 = c->gamma - 1.0;
-    const double invJ = a.inv_jac[ec];
-    const size_t off = ((size_t)ec * NN + node) * 4;
-
-    double Di[n], Dj[n];
-    load_drows<N>(c, i, j, -dt, Di, Dj);
-    double u[4], f[4], g[4];
-    load4(uin + off, u);
-    euler_fluxes(u, gm1, f, g);
-    if (act) put8<N>(s0, sidx, f, g);
+// mdrk_kernel_1!, tree/multi_derivative_2d.jl:263-420: F̃ = f/2 + f_t/8,
+// Ũ = u/2 + ut/8.  F2 = f + f_t/6 and U2 = u + ut/6 are recomputed by the
+// stage-2 kernel instead of being stored.
+template <int N, int EB>
+__device__ __forceinline__ void mdrk1_element(const double* uN, double dt, double invJ, double gm1, const DevConst* c,
+                                              int i, int j, bool act, int sidx, int sbase, double* s0, double* s1,
+                                              double* s2, const double* sDm, const double* sDh, double* Uc,
+                                              double* F, double* G, double* vol) {
+    using G_ = VGeo<N, EB>;
+    constexpr int n = G_::n, S = G_::S;
+    double f[4], g[4];
+    euler_fluxes(uN, gm1, f, g);
+    if (act) put8<S>(s0, sidx, f, g);
     __syncthreads();
     double ut[4];
-    contract<N>(s0, sbase, i, j, Di, Dj, ut);
+    contract<n, S>(s0, sbase, i, j, sDm, ut);
 #pragma unroll
     for (int v = 0; v < 4; ++v) ut[v] *= invJ;
-    double dx[4], dy[4], F[4], G[4], Uc[4];
-    mdrk_ft(u, ut, gm1, dx, dy);
+    double dx[4], dy[4];
+    mdrk_ft(uN, ut, gm1, dx, dy);
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         F[v] = fma(0.125, dx[v], 0.5 * f[v]);
         G[v] = fma(0.125, dy[v], 0.5 * g[v]);
-        Uc[v] = fma(0.125, ut[v], 0.5 * u[v]);
+        Uc[v] = fma(0.125, ut[v], 0.5 * uN[v]);
     }
-    if (act) put8<N>(s1, sidx, F, G);
+    if (act) put8<S>(s2, sidx, F, G);
     __syncthreads();
-    double Hi[n], Hj[n];
-#pragma unroll
-    for (int k = 0; k < n; ++k) { Hi[k] = c->Dhat[i][k]; Hj[k] = c->Dhat[j][k]; }
-    double vol[4];
-    contract<N>(s1, sbase, i, j, Hi, Hj, vol);
-    if (act) {
-        emit_volume(a, out, off, u, vol, invJ, dt);
-        write_traces<N>(a, e, i, j, u, Uc, F, G);
-        if (a.auxU) {
-            store4(a.auxU + off, Uc);
-            store4(a.auxF + ((size_t)e * NN + node) * 8, F);
-            store4(a.auxF + ((size_t)e * NN + node) * 8 + 4, G);
-        }
-    }
+    contract<n, S>(s2, sbase, i, j, sDh, vol);
 }
 
-// MDRK stage 2 volume kernel: mdrk_kernel_2!, tree/multi_derivative_2d.jl:423-555.
-// F̃ = F2 + f_t(us)/3, Ũ = U2 + ust/3, with F2, U2 recomputed from uⁿ.
-template <int N>
-__global__ __launch_bounds__(kBlock) void k_volume_mdrk2(KArgs a) {
-    TLW_ELEMENT_PROLOGUE(N)
-    __shared__ double s0[8 * Geo<N>::S], s1[8 * Geo<N>::S], s2[8 * Geo<N>::S];
-    DevState* st = a.st;
-    if (a.use_state_dt && st->finished) return;
-    const int cur = a.use_state_dt ? st->cur : 0;
-    const double* uin = sel_in(a, cur);
-    double* out = sel_out(a, cur);
-    const double dt = a.use_state_dt ? st->dt : a.dt_arg;
-    const DevConst* c = a.c;
-    const double gm1 = c->gamma - 1.0;
-    const double invJ = a.inv_jac[ec];
-    const size_t off = ((size_t)ec * NN + node) * 4;
-
-    double Di[n], Dj[n];
-    load_drows<N>(c, i, j, -dt, Di, Dj);
-    double u[4], f[4], g[4], usn[4], fs[4], gs[4];
-    load4(uin + off, u);
-    load4(a.us + off, usn);
-    euler_fluxes(u, gm1, f, g);
+// mdrk_kernel_2!, tree/multi_derivative_2d.jl:423-555: F̃ = F2 + f_t(us)/3,
+// Ũ = U2 + ust/3 with F2, U2 recomputed from uⁿ.
+template <int N, int EB>
+__device__ __forceinline__ void mdrk2_element(const double* uN, const double* usn, double dt, double invJ, double gm1,
+                                              const DevConst* c, int i, int j, bool act, int sidx, int sbase,
+                                              double* s0, double* s1, double* s2, const double* sDm,
+                                              const double* sDh, double* Uc, double* F, double* G, double* vol) {
+    using G_ = VGeo<N, EB>;
+    constexpr int n = G_::n, S = G_::S;
+    double f[4], g[4], fs[4], gs[4];
+    euler_fluxes(uN, gm1, f, g);
     euler_fluxes(usn, gm1, fs, gs);
     if (act) {
-        put8<N>(s0, sidx, f, g);
-        put8<N>(s1, sidx, fs, gs);
+        put8<S>(s0, sidx, f, g);
+        put8<S>(s1, sidx, fs, gs);
     }
     __syncthreads();
     double ut[4], ust[4];
-    contract<N>(s0, sbase, i, j, Di, Dj, ut);
-    contract<N>(s1, sbase, i, j, Di, Dj, ust);
+    contract<n, S>(s0, sbase, i, j, sDm, ut);
+    contract<n, S>(s1, sbase, i, j, sDm, ust);
 #pragma unroll
     for (int v = 0; v < 4; ++v) { ut[v] *= invJ; ust[v] *= invJ; }
-    double dx[4], dy[4], F[4], G[4], Uc[4];
-    mdrk_ft(u, ut, gm1, dx, dy);
+    double dx[4], dy[4];
+    mdrk_ft(uN, ut, gm1, dx, dy);
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         F[v] = fma(1.0 / 6.0, dx[v], f[v]);   // F2
         G[v] = fma(1.0 / 6.0, dy[v], g[v]);
-        Uc[v] = fma(1.0 / 6.0, ut[v], u[v]);  // U2
+        Uc[v] = fma(1.0 / 6.0, ut[v], uN[v]);  // U2
     }
     mdrk_ft(usn, ust, gm1, dx, dy);
 #pragma unroll
@@ -726,40 +642,162 @@ __global__ __launch_bounds__(kBlock) void k_volume_mdrk2(KArgs a) {
         G[v] = fma(1.0 / 3.0, dy[v], G[v]);
         Uc[v] = fma(1.0 / 3.0, ust[v], Uc[v]);
     }
-    if (act) put8<N>(s2, sidx, F, G);
+    if (act) put8<S>(s2, sidx, F, G);
     __syncthreads();
-    double Hi[n], Hj[n];
-#pragma unroll
-    for (int k = 0; k < n; ++k) { Hi[k] = c->Dhat[i][k]; Hj[k] = c->Dhat[j][k]; }
-    double vol[4];
-    contract<N>(s2, sbase, i, j, Hi, Hj, vol);
-    if (act) {
-        emit_volume(a, out, off, u, vol, invJ, dt);
-        write_traces<N>(a, e, i, j, u, Uc, F, G);
-        if (a.auxU) {
-            store4(a.auxU + off, Uc);
-            store4(a.auxF + ((size_t)e * NN + node) * 8, F);
-            store4(a.auxF + ((size_t)e * NN + node) * 8 + 4, G);
-        }
-    }
+    contract<n, S>(s2, sbase, i, j, sDh, vol);
 }
 
 // ---------------------------------------------------------------------------
-// Rusanov flux of one face node from the two face records (left = -side).
-__device__ __forceinline__ void face_numflux(const double* L, const double* R, double* Fs) {
-    const double lam = dmax(L[8], R[8]) + dmax(L[9], R[9]);
-#pragma unroll
-    for (int v = 0; v < 4; ++v) Fs[v] = 0.5 * (L[4 + v] + R[4 + v]) + (-0.5 * lam) * (R[v] - L[v]);
-}
+// Volume kernel: element computation, then the interfaces inside the
+// workgroup, the lifted surface terms (weak form, ℓ̂ = 1/w at the end nodes),
+// the Jacobian and the update.  Faces leaving the workgroup write their face
+// record for the surface kernel.  In step mode the CFL reduction covers the
+// nodes finalised here: whole elements go to the workgroup partial, elements
+// with an outside face leave their per-node maxima in elem_max.
+template <int N, int EB, int KIND>
+__global__ __launch_bounds__((VGeo<N, EB>::NT), 3) void k_volume(KArgs a) {
+    using G_ = VGeo<N, EB>;
+    constexpr int n = G_::n, NN = G_::NN, S = G_::S, NT = G_::NT;
+    __shared__ double lds[G_::LDS];
+    __shared__ double sDm[NN], sDh[NN];  // -dt D and D̂, row-major
+    double* s0 = lds;
+    double* s1 = lds + 8 * S;
+    double* s2 = lds + 16 * S;
+    DevState* st = a.st;
+    if (a.use_state_dt && st->finished) return;
+    const int tid = threadIdx.x;
+    const int le = tid / NN, node = tid - (tid / NN) * NN;
+    const int i = node % n, j = node / n;
+    const int e0 = blockIdx.x * EB;
+    const int nel = min(EB, a.E - e0);
+    const bool act = le < nel;
+    const int lec = le < nel ? le : nel - 1;
+    const int e = e0 + lec;
+    const int sidx = lec * NN + node, sbase = lec * NN;
+    const int cur = a.use_state_dt ? st->cur : 0;
+    const double* uin = sel_in(a, cur);
+    double* out = sel_out(a, cur);
+    const double dt = a.use_state_dt ? st->dt : a.dt_arg;
+    const DevConst* c = a.c;
+    const double gamma = c->gamma, gm1 = gamma - 1.0;
+    const double invJ = a.inv_jac[e];
+    const size_t off = ((size_t)e * NN + node) * 4;
 
-__device__ __forceinline__ void load_record(const double* p, double* r) {
-    const double2* p2 = reinterpret_cast<const double2*>(p);
-#pragma unroll
-    for (int k = 0; k < 5; ++k) {
-        const double2 x = p2[k];
-        r[2 * k] = x.x;
-        r[2 * k + 1] = x.y;
+    if (tid < NN) {  // published by the first barrier of the element computation
+        sDm[tid] = -dt * c->D[tid / n][tid % n];
+        sDh[tid] = c->Dhat[tid / n][tid % n];
     }
+    double uN[4], Uc[4], F[4], G[4], vol[4];
+    load4(uin + off, uN);
+    if (KIND == VOL_LW) {
+        if constexpr (N == 4)
+            lw4_element<N, EB>(uN, dt, invJ, gm1, c, i, j, act, sidx, sbase, s0, s1, s2, sDm, sDh, Uc, F, G, vol);
+        else
+            lw3_element<N, EB>(uN, dt, invJ, gm1, c, i, j, act, sidx, sbase, s0, s1, s2, sDm, sDh, Uc, F, G, vol);
+    } else if (KIND == VOL_MDRK1) {
+        mdrk1_element<N, EB>(uN, dt, invJ, gm1, c, i, j, act, sidx, sbase, s0, s1, s2, sDm, sDh, Uc, F, G, vol);
+    } else {
+        double usn[4];
+        load4(a.us + off, usn);
+        mdrk2_element<N, EB>(uN, usn, dt, invJ, gm1, c, i, j, act, sidx, sbase, s0, s1, s2, sDm, sDh, Uc, F, G,
+                             vol);
+    }
+    if (act && a.auxU) {
+        store4(a.auxU + off, Uc);
+        store4(a.auxF + ((size_t)e * NN + node) * 8, F);
+        store4(a.auxF + ((size_t)e * NN + node) * 8 + 4, G);
+    }
+
+    // --- face records of this node (s0 and s1 are free: records live there).
+    // A node lies on at most one x face (fx) and one y face (fy).
+    double* rec = lds;  // [EB][4][n][kTraceQ]
+    const bool tree = c->flux_order == 0;
+    const double* jump = tree ? uN : Uc;
+    const double* lamst = tree ? Uc : uN;
+    const int fx = (i == 0) ? 0 : ((i == n - 1) ? 1 : -1);
+    const int fy = (j == 0) ? 2 : ((j == n - 1) ? 3 : -1);
+    if (act && fx >= 0) {
+        double r[kTraceQ];
+        make_record(r, jump, F, lamst, gamma, 1);
+        store_record(rec + ((lec * 4 + fx) * n + j) * kTraceQ, r);
+    }
+    if (act && fy >= 0) {
+        double r[kTraceQ];
+        make_record(r, jump, G, lamst, gamma, 2);
+        store_record(rec + ((lec * 4 + fy) * n + i) * kTraceQ, r);
+    }
+    __syncthreads();
+    // --- interfaces inside the workgroup; records of outside faces to HBM
+    double surf[4] = {0.0, 0.0, 0.0, 0.0};
+    bool fin_node = true;
+    auto do_face = [&](int f, int l) {
+        double own[kTraceQ];
+        load_record(rec + ((lec * 4 + f) * n + l) * kTraceQ, own);
+        const int ne = a.nbr[4 * e + f];
+        if (ne >= e0 && ne < e0 + nel) {
+            double oth[kTraceQ], Fs[4];
+            load_record(rec + (((ne - e0) * 4 + (f ^ 1)) * n + l) * kTraceQ, oth);
+            if (f & 1) face_numflux(own, oth, Fs);  // +x, +y face: own side is left
+            else face_numflux(oth, own, Fs);
+            const double wf = (f & 1) ? c->bnd_hi : -c->bnd_lo;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) surf[v] = fma(Fs[v], wf, surf[v]);
+        } else {
+            fin_node = false;
+            if (act) store_record(a.trace + (((size_t)e * 4 + f) * n + l) * kTraceQ, own);
+        }
+    };
+    if (fx >= 0) do_face(fx, j);
+    if (fy >= 0) do_face(fy, i);
+    double val[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const double d = (-invJ) * (vol[v] + surf[v]);
+        val[v] = a.step_mode ? fma(dt, d, uN[v]) : d;
+    }
+    if (act) store4(out + off, val);
+    if (a.reduce == RED_NONE) return;
+    // --- CFL reduction over the nodes finalised here (s2 is free)
+    double l1 = -1.0, l2 = -1.0;
+    if (fin_node) {
+        bool bad;
+        node_speeds(val, gamma, l1, l2, bad);
+        if (act && bad) st->bad_state = 1;
+    }
+    double* sl1 = s2;
+    double* sl2 = s2 + S;
+    double* sm = s2 + 2 * S;
+    if (act) {
+        sl1[sidx] = l1;
+        sl2[sidx] = l2;
+    }
+    __syncthreads();
+    if (act && node == 0) {
+        double m1 = 0.0, m2 = 0.0;
+        for (int k = 0; k < NN; ++k) {
+            m1 = dmax(m1, sl1[sbase + k]);
+            m2 = dmax(m2, sl2[sbase + k]);
+        }
+        bool whole = true;
+        for (int f = 0; f < 4; ++f) {
+            const int ne = a.nbr[4 * e + f];
+            if (!(ne >= e0 && ne < e0 + nel)) whole = false;
+        }
+        if (whole) {
+            sm[le] = invJ * (m1 + m2);
+        } else {
+            sm[le] = 0.0;
+            a.elem_max[2 * e] = m1;
+            a.elem_max[2 * e + 1] = m2;
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double bm = 0.0;
+        for (int k = 0; k < nel; ++k) bm = dmax(bm, sm[k]);
+        a.partial[blockIdx.x] = bm;
+    }
+    (void)NT;
 }
 
 // Outer record of a domain-boundary face node: time average of the boundary
@@ -794,25 +832,145 @@ __device__ inline void boundary_record(const KArgs& a, int e, int face, int l, d
     }
     boundary_value(c, x, y, t, uo);
     const bool tree = c->flux_order == 0;
-    const double* jump = tree ? uo : Uo;
-    const double* lam = tree ? Uo : uo;
-    double a_, c_;
-    speed_parts(lam, c->gamma, orientation, a_, c_);
-    for (int v = 0; v < 4; ++v) { r[v] = jump[v]; r[4 + v] = Fo[v]; }
-    r[8] = a_;
-    r[9] = c_;
+    make_record(r, tree ? uo : Uo, Fo, tree ? Uo : uo, c->gamma, orientation);
 }
 
-template <int N, bool BC>
-__device__ __forceinline__ void face_flux(const KArgs& a, int e, int face, int l, double t, double dt, double* Fs) {
-    constexpr int n = N + 1;
-    double own[kTraceQ], oth[kTraceQ];
-    load_record(a.trace + ((size_t)e * 4 * n + face * n + l) * kTraceQ, own);
-    const int ne = a.nbr[4 * e + face];
-    if (!BC || ne >= 0) load_record(a.trace + ((size_t)ne * 4 * n + (face ^ 1) * n + l) * kTraceQ, oth);
-    else boundary_record<N>(a, e, face, l, t, dt, oth);
-    if (face & 1) face_numflux(own, oth, Fs);  // +x, +y face: own side is left
-    else face_numflux(oth, own, Fs);
+// Surface kernel: the faces that leave a volume workgroup (EBV elements).
+// Each interface is evaluated by both neighbours from identical inputs, so
+// both see the same bits.  Elements with such faces finish their CFL
+// reduction here, combining elem_max with the corrected face nodes.
+template <int N, int EBV, bool BC>
+__global__ __launch_bounds__(kSurfBlock) void k_surface(KArgs a) {
+    using G_ = SGeo<N>;
+    constexpr int n = G_::n, NN = G_::NN, EB = G_::EB, S = G_::S;
+    __shared__ double s_l1[S], s_l2[S], s_m[EB];
+    DevState* st = a.st;
+    if (a.use_state_dt && st->finished) return;
+    const int tid = threadIdx.x;
+    const int le = tid / NN, node = tid - (tid / NN) * NN;
+    const int i = node % n, j = node / n;
+    const int e0 = blockIdx.x * EB;
+    const int nel = min(EB, a.E - e0);
+    const bool act = le < nel;
+    const int lec = le < nel ? le : nel - 1;
+    const int e = e0 + lec;
+    const int cur = a.use_state_dt ? st->cur : 0;
+    double* out = sel_out(a, cur);
+    const double dt = a.use_state_dt ? st->dt : a.dt_arg;
+    const double t = a.use_state_dt ? st->t : a.t_arg;
+    const DevConst* c = a.c;
+    const double invJ = a.inv_jac[e];
+    const int eblk = e / EBV;
+    const int fx = (i == 0) ? 0 : ((i == n - 1) ? 1 : -1);
+    const int fy = (j == 0) ? 2 : ((j == n - 1) ? 3 : -1);
+    bool elem_ext = false;
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+        const int nb = a.nbr[4 * e + f];
+        elem_ext = elem_ext || nb < 0 || nb / EBV != eblk;
+    }
+    const int nbx = fx >= 0 ? a.nbr[4 * e + fx] : 0;
+    const int nby = fy >= 0 ? a.nbr[4 * e + fy] : 0;
+    const bool extx = fx >= 0 && (nbx < 0 || nbx / EBV != eblk);
+    const bool exty = fy >= 0 && (nby < 0 || nby / EBV != eblk);
+    const bool any_ext = extx || exty;
+    double val[4];
+    if (act && any_ext) {
+        const size_t off = ((size_t)e * NN + node) * 4;
+        load4(out + off, val);
+        double surf[4] = {0.0, 0.0, 0.0, 0.0};
+        auto do_face = [&](int f, int l, int nb) {
+            double own[kTraceQ], oth[kTraceQ], Fs[4];
+            load_record(a.trace + (((size_t)e * 4 + f) * n + l) * kTraceQ, own);
+            if (!BC || nb >= 0) load_record(a.trace + (((size_t)nb * 4 + (f ^ 1)) * n + l) * kTraceQ, oth);
+            else boundary_record<N>(a, e, f, l, t, dt, oth);
+            if (f & 1) face_numflux(own, oth, Fs);
+            else face_numflux(oth, own, Fs);
+            const double wf = (f & 1) ? c->bnd_hi : -c->bnd_lo;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) surf[v] = fma(Fs[v], wf, surf[v]);
+        };
+        if (extx) do_face(fx, j, nbx);
+        if (exty) do_face(fy, i, nby);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const double d = (-invJ) * surf[v];
+            val[v] = a.step_mode ? fma(dt, d, val[v]) : val[v] + d;
+        }
+        store4(out + off, val);
+    }
+    if (a.reduce == RED_NONE) return;
+    double l1 = -1.0, l2 = -1.0;
+    if (act && any_ext) {
+        bool bad;
+        node_speeds(val, c->gamma, l1, l2, bad);
+        if (bad) st->bad_state = 1;
+    }
+    if (act) {
+        s_l1[lec * NN + node] = l1;
+        s_l2[lec * NN + node] = l2;
+    }
+    __syncthreads();
+    if (act && node == 0) {
+        double m = 0.0;
+        if (elem_ext) {
+            double m1 = a.elem_max[2 * e], m2 = a.elem_max[2 * e + 1];
+            for (int k = 0; k < NN; ++k) {
+                m1 = dmax(m1, s_l1[lec * NN + k]);
+                m2 = dmax(m2, s_l2[lec * NN + k]);
+            }
+            m = invJ * (m1 + m2);
+        }
+        s_m[lec] = m;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double bm = 0.0;
+        for (int k = 0; k < nel; ++k) bm = dmax(bm, s_m[k]);
+        a.partial[a.vol_grid + blockIdx.x] = bm;
+    }
+}
+
+// CFL reduction over a whole state (Trixi max_dt) without an update: first dt
+// of a solve (RED_INIT) or a max_dt query (RED_QUERY).
+template <int N>
+__global__ __launch_bounds__(kSurfBlock) void k_max_speed(KArgs a) {
+    using G_ = SGeo<N>;
+    constexpr int NN = G_::NN, EB = G_::EB, S = G_::S;
+    __shared__ double s_l1[S], s_l2[S], s_m[EB];
+    const int tid = threadIdx.x;
+    const int le = tid / NN, node = tid - (tid / NN) * NN;
+    const int e0 = blockIdx.x * EB;
+    const int nel = min(EB, a.E - e0);
+    const bool act = le < nel;
+    const int lec = le < nel ? le : nel - 1;
+    const int e = e0 + lec;
+    const double* uin = sel_in(a, a.st->cur);
+    double val[4];
+    load4(uin + ((size_t)e * NN + node) * 4, val);
+    double l1, l2;
+    bool bad;
+    node_speeds(val, a.c->gamma, l1, l2, bad);
+    if (act && bad) a.st->bad_state = 1;
+    if (act) {
+        s_l1[lec * NN + node] = l1;
+        s_l2[lec * NN + node] = l2;
+    }
+    __syncthreads();
+    if (act && node == 0) {
+        double m1 = 0.0, m2 = 0.0;
+        for (int k = 0; k < NN; ++k) {
+            m1 = dmax(m1, s_l1[lec * NN + k]);
+            m2 = dmax(m2, s_l2[lec * NN + k]);
+        }
+        s_m[lec] = a.inv_jac[e] * (m1 + m2);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double bm = 0.0;
+        for (int k = 0; k < nel; ++k) bm = dmax(bm, s_m[k]);
+        a.partial[blockIdx.x] = bm;
+    }
 }
 
 // Final reduction of the per-workgroup partial maxima of J⁻¹(max λ1 + max λ2)
@@ -861,109 +1019,6 @@ __global__ __launch_bounds__(kFinalizeBlock) void k_finalize(DevState* st, const
     if (t + dt > tf) dt = tf - t;  // set_dt!
     st->dt = dt;
     st->finished = julia_isapprox(t, tf);
-}
-
-// Per-element max of λ1 = |v1| + c and λ2 = |v2| + c (Trixi max_abs_speeds),
-// scaled max over the workgroup.
-template <int N>
-__device__ __forceinline__ double block_speed_max(const KArgs& a, const double* un, bool act, int lec, int node,
-                                                  int ec, double* s_l1, double* s_l2, double* s_m) {
-    constexpr int NN = Geo<N>::NN, EB = Geo<N>::EB;
-    const double gamma = a.c->gamma;
-    const double rho = un[0], v1 = un[1] / rho, v2 = un[2] / rho;
-    const double p = (gamma - 1.0) * (un[3] - 0.5 * (un[1] * v1 + un[2] * v2));
-    const bool bad = !(rho > 0.0) || !(p > 0.0);
-    const double cs = sqrt(gamma * p / rho);
-    if (act && bad) a.st->bad_state = 1;
-    if (act) {
-        s_l1[lec * NN + node] = fabs(v1) + cs;
-        s_l2[lec * NN + node] = fabs(v2) + cs;
-    }
-    __syncthreads();
-    if (act && node == 0) {
-        double m1 = 0.0, m2 = 0.0;
-        for (int k = 0; k < NN; ++k) {
-            m1 = dmax(m1, s_l1[lec * NN + k]);
-            m2 = dmax(m2, s_l2[lec * NN + k]);
-        }
-        s_m[lec] = a.inv_jac[ec] * (m1 + m2);
-    }
-    __syncthreads();
-    double bm = 0.0;
-    if (threadIdx.x == 0) {
-        const int nel = min(EB, a.E - (int)blockIdx.x * EB);
-        for (int k = 0; k < nel; ++k) bm = dmax(bm, s_m[k]);
-    }
-    return bm;
-}
-
-// Surface kernel: Rusanov flux at every face node of the element (each
-// interface is evaluated by both neighbours from identical inputs, so both
-// see the same bits), weak-form surface integral with ℓ̂ = 1/w at the end
-// nodes, Jacobian, finalisation of the update, CFL reduction.
-template <int N, bool BC>
-__global__ __launch_bounds__(kBlock) void k_surface(KArgs a) {
-    TLW_ELEMENT_PROLOGUE(N)
-    __shared__ double s_l1[Geo<N>::S], s_l2[Geo<N>::S], s_m[Geo<N>::EB];
-    DevState* st = a.st;
-    if (a.use_state_dt && st->finished) return;
-    const int cur = a.use_state_dt ? st->cur : 0;
-    double* out = sel_out(a, cur);
-    const double dt = a.use_state_dt ? st->dt : a.dt_arg;
-    const double t = a.use_state_dt ? st->t : a.t_arg;
-    const DevConst* c = a.c;
-    const double invJ = a.inv_jac[ec];
-    const size_t off = ((size_t)ec * NN + node) * 4;
-    double val[4];
-    load4(out + off, val);
-    if (act && (i == 0 || i == n - 1 || j == 0 || j == n - 1)) {
-        double surf[4] = {0.0, 0.0, 0.0, 0.0}, Fs[4];
-        if (i == 0) {
-            face_flux<N, BC>(a, e, 0, j, t, dt, Fs);
-#pragma unroll
-            for (int v = 0; v < 4; ++v) surf[v] -= Fs[v] * c->bnd_lo;
-        }
-        if (i == n - 1) {
-            face_flux<N, BC>(a, e, 1, j, t, dt, Fs);
-#pragma unroll
-            for (int v = 0; v < 4; ++v) surf[v] += Fs[v] * c->bnd_hi;
-        }
-        if (j == 0) {
-            face_flux<N, BC>(a, e, 2, i, t, dt, Fs);
-#pragma unroll
-            for (int v = 0; v < 4; ++v) surf[v] -= Fs[v] * c->bnd_lo;
-        }
-        if (j == n - 1) {
-            face_flux<N, BC>(a, e, 3, i, t, dt, Fs);
-#pragma unroll
-            for (int v = 0; v < 4; ++v) surf[v] += Fs[v] * c->bnd_hi;
-        }
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const double d = (-invJ) * surf[v];
-            val[v] = a.step_mode ? fma(dt, d, val[v]) : val[v] + d;
-        }
-        store4(out + off, val);
-    }
-    if (a.reduce != RED_NONE) {
-        const double bm = block_speed_max<N>(a, val, act, lec, node, ec, s_l1, s_l2, s_m);
-        if (threadIdx.x == 0) a.partial[blockIdx.x] = bm;
-    }
-}
-
-// CFL reduction over a state (Trixi max_dt) without an update: first dt of a
-// solve (RED_INIT) or a max_dt query (RED_QUERY).
-template <int N>
-__global__ __launch_bounds__(kBlock) void k_max_speed(KArgs a) {
-    TLW_ELEMENT_PROLOGUE(N)
-    __shared__ double s_l1[Geo<N>::S], s_l2[Geo<N>::S], s_m[Geo<N>::EB];
-    const int cur = a.st->cur;
-    const double* uin = sel_in(a, cur);
-    const size_t off = ((size_t)ec * NN + node) * 4;
-    double val[4];
-    load4(uin + off, val);
-    const double bm = block_speed_max<N>(a, val, act, lec, node, ec, s_l1, s_l2, s_m);
-    if (threadIdx.x == 0) a.partial[blockIdx.x] = bm;
 }
 
 // Initialisation of the device integrator state (one thread).
