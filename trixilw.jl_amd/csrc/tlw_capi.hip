@@ -460,10 +460,12 @@ int tlw_advance(tlw_ctx* c, int32_t max_steps, double* t_host, double* dt_host, 
     a.step_mode = 1;
     for (int s = 0; s < max_steps; ++s) {
         if (c->time_disc == TLW_LW) {
+            // the CFL reduction of uⁿ⁺¹ is split between the volume kernel
+            // (nodes it finalises) and the surface kernel (the rest)
             KArgs v = a;
             v.in_sel = SEL_CUR;
             v.out_sel = SEL_OTHER;
-            v.reduce = RED_NONE;
+            v.reduce = RED_STEP;
             timed(c, 0, [&] { launch_volume(c, v, VOL_LW); });
             KArgs sf = a;
             sf.out_sel = SEL_OTHER;
@@ -485,10 +487,9 @@ int tlw_advance(tlw_ctx* c, int32_t max_steps, double* t_host, double* dt_host, 
             v2.in_sel = SEL_CUR;
             v2.out_sel = SEL_OTHER;
             v2.us = c->d_us;
-            v2.reduce = RED_NONE;
+            v2.reduce = RED_STEP;
             timed(c, 0, [&] { launch_volume(c, v2, VOL_MDRK2); });
             KArgs s2 = v2;
-            s2.reduce = RED_STEP;
             timed(c, 1, [&] { launch_surface(c, s2); });
         }
     }
