@@ -32,6 +32,12 @@ WORKLOADS = {
     "lw_euler_vortex_256x256_N4": dict(level=8, N=4, td="lw"),
     "lw_euler_vortex_32x32_N4": dict(level=5, N=4, td="lw"),
     "mdrk_euler_vortex_1024x1024_N3": dict(level=10, N=3, td="mdrk"),
+    # four-quadrant Riemann problem (Lax-Liu configuration 3) on [0,1]^2 with a time-averaged
+    # Dirichlet boundary, Hennemann-Gassner indicator on density_pressure (alpha_max 0.5) and
+    # first-order subcell FV blending; CFL 0.3 (the 0.98·LW limit loses positivity within 500 steps
+    # without the reference's positivity limiter, on the CPU oracle as on the device)
+    "lw_euler_riemann_sc_256x256_N4": dict(level=8, N=4, td="lw", problem="riemann_sc", cfl=0.3),
+    "lw_euler_riemann_sc_1024x1024_N4": dict(level=10, N=4, td="lw", problem="riemann_sc", cfl=0.3),
 }
 DEFAULT_WORKLOAD = "lw_euler_vortex_1024x1024_N4"
 
@@ -90,6 +96,7 @@ def main():
     ap.add_argument("--workload", default=DEFAULT_WORKLOAD, choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--tile", type=int, default=0, help="elements per volume workgroup (0: autotune)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device copy (the reference kernel for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -119,12 +126,21 @@ def main():
 
     wl = WORKLOADS[args.workload]
     N, td = wl["N"], wl["td"]
+    sc = wl.get("problem") == "riemann_sc"
     eq = tlw.CompressibleEulerEquations2D(1.4)
-    ic = tlw.IsentropicVortex()
-    mesh = tlw.TreeMesh((-10.0, -10.0), (10.0, 10.0), wl["level"], ordering="morton")
-    solver = tlw.DGSEM(N, volume_integral=tlw.VolumeIntegralFR(tlw.MDRK() if td == "mdrk" else tlw.LW()))
-    semi = tlw.SemidiscretizationHyperbolic(mesh, tlw.get_time_discretization(solver), eq, ic, solver)
-    cfl = tlw.trixi2lw(0.98, solver) * (0.5 if td == "mdrk" else 1.0)
+    vi = tlw.VolumeIntegralFR(tlw.MDRK() if td == "mdrk" else tlw.LW())
+    if sc:
+        ic = tlw.FourQuadrantRiemann()
+        lo, hi, bc, periodic = 0.0, 1.0, ic, False
+        vi = tlw.VolumeIntegralFRShockCapturing(tlw.IndicatorHennemannGassner(eq, None, alpha_max=0.5), vi)
+    else:
+        ic = tlw.IsentropicVortex()
+        lo, hi, bc, periodic = -10.0, 10.0, None, True
+    mesh = tlw.TreeMesh((lo, lo), (hi, hi), wl["level"], periodicity=periodic, ordering="morton")
+    solver = tlw.DGSEM(N, volume_integral=vi)
+    semi = tlw.SemidiscretizationHyperbolic(mesh, tlw.get_time_discretization(solver), eq, ic, solver,
+                                            boundary_conditions=bc)
+    cfl = wl.get("cfl") or tlw.trixi2lw(0.98, solver) * (0.5 if td == "mdrk" else 1.0)
     dofs = semi.ndofs()
     log(f"bench: {args.workload}: {mesh.n_elements} elements, {dofs} DOFs")
 
@@ -134,6 +150,15 @@ def main():
     h = semi.handle
     dp = C.POINTER(C.c_double)
     semi._bind_stream()
+    _lib.check(L.tlw_set_state(h, u0.ctypes.data_as(dp), u0.size), h)
+    if args.tile:
+        semi.elements_per_workgroup(args.tile)
+        tune = None
+    else:
+        chosen, tune = semi.autotune(u0, reps=3)
+        log("bench: tile autotune (ms per rhs): " + ", ".join(f"{k} el/wg {v:.3f}" for k, v in tune.items())
+            + f" -> {chosen}")
+    tile = semi.elements_per_workgroup()
     _lib.check(L.tlw_set_state(h, u0.ctypes.data_as(dp), u0.size), h)
     _lib.check(L.tlw_solve_init(h, 0.0, float("inf"), cfl), h)
     t, dt = C.c_double(), C.c_double()
@@ -189,7 +214,9 @@ def main():
         threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
         threads = max(1, min(threads, os.cpu_count() or threads))
         nx = mesh.nx
-        o = oracle_ctypes.Oracle(N, nx, nx, -10.0, -10.0, 20.0 / nx)
+        o = oracle_ctypes.Oracle(N, nx, nx, lo, lo, (hi - lo) / nx, periodic=(periodic, periodic),
+                                 bc_kind=bc.kind if bc is not None else 0, bc_par=bc.params if bc is not None else (),
+                                 shock_capturing=sc, alpha_max=0.5)
         xo, yo = o.node_coords()
         uo = np.ascontiguousarray(ic(xo, yo, 0.0))
         log(f"bench: CPU oracle baseline, {args.cpu_steps} steps on {threads} threads")
@@ -214,9 +241,15 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": args.workload, "mesh": f"{mesh.nx}x{mesh.nx} periodic TreeMesh [-10,10]^2",
+        "config": {"workload": args.workload,
+                   "mesh": (f"{mesh.nx}x{mesh.nx} TreeMesh [0,1]^2, time-averaged Dirichlet boundary" if sc else
+                            f"{mesh.nx}x{mesh.nx} periodic TreeMesh [-10,10]^2"),
                    "polydeg": N, "time_discretization": td.upper(), "dofs": dofs,
-                   "initial_condition": "isentropic vortex strength 5, Mach 0.5", "cfl": cfl},
+                   "initial_condition": ("four-quadrant Riemann problem (Lax-Liu 3)" if sc else
+                                         "isentropic vortex strength 5, Mach 0.5"),
+                   "shock_capturing": ("Hennemann-Gassner on density_pressure, alpha_max 0.5, first-order subcell FV"
+                                       if sc else None),
+                   "cfl": cfl, "elements_per_workgroup": tile, "tile_autotune_ms": tune},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

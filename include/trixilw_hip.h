@@ -54,8 +54,9 @@ enum { TLW_FLUX_TREE = 0, TLW_FLUX_STRUCTURED = 1 };
  * (src/equations/equations.jl:6-37) */
 enum { TLW_BV_NONE = 0, TLW_BV_VORTEX = 1, TLW_BV_CONSTANT = 2, TLW_BV_RIEMANN4 = 3 };
 
-/* tlw_get_aux selectors (element_cache fields, src/solvers/dgsem_tree/containers.jl:73-98) */
-enum { TLW_AUX_U = 0, TLW_AUX_F = 1 };
+/* tlw_get_aux selectors (element_cache fields, src/solvers/dgsem_tree/containers.jl:73-98,
+ * and the shock-capturing blending factors, indicator.cache.alpha) */
+enum { TLW_AUX_U = 0, TLW_AUX_F = 1, TLW_AUX_ALPHA = 2 };
 
 /* solver flags */
 enum { TLW_FLAG_KEEP_AUX = 1 /* store full Ũ [v,i,j,e] and F̃ [v,d,i,j,e] on every rhs */ };
@@ -86,6 +87,13 @@ typedef struct {
     int32_t boundary_kind;          /* TLW_BV_* applied on every face with neighbour -1 */
     double boundary_params[24];
     int32_t flags;                  /* TLW_FLAG_* */
+    /* VolumeIntegralFRShockCapturing(IndicatorHennemannGassner(...; variable=density_pressure),
+     * volume_flux_fv=flux_lax_friedrichs, reconstruction=FirstOrderReconstruction())
+     * (src/solvers/fr.jl:63-86, src/solvers/dgsem_tree/dg_2d.jl:1648-1991); TreeMesh order only */
+    int32_t shock_capturing;        /* 0 = VolumeIntegralFR */
+    double alpha_max, alpha_min;
+    int32_t alpha_smooth;
+    const double* inverse_vandermonde_host; /* [n*n] Trixi basis.inverse_vandermonde_legendre, column-major */
 } tlw_solver_desc;
 
 typedef struct tlw_ctx tlw_ctx;
@@ -147,8 +155,21 @@ int tlw_enable_kernel_timing(tlw_ctx* ctx, int32_t enable);
 int tlw_kernel_times(tlw_ctx* ctx, double* volume_ms, int64_t* volume_launches, double* surface_ms,
                      int64_t* surface_launches);
 
-/* Copy an element-cache array of the last rhs call to the host (needs TLW_FLAG_KEEP_AUX):
- * TLW_AUX_U: Ũ [v,i,j,e], len 4 n^2 E; TLW_AUX_F: F̃ [v,d,i,j,e], len 8 n^2 E. */
+/* Tile of the fused volume kernel (no counterpart in the reference, whose @threaded loop
+ * has no tile): elements per workgroup, one lane per GLL node.  polydeg 4: 5, 10 or 15
+ * (default 15); polydeg 3: 4, 8 or 16 (default 16).  Results do not depend on the tile.
+ * tlw_autotune times `reps` rhs evaluations per tile on the current state buffer (after
+ * tlw_set_state), keeps the fastest and writes it to *chosen and the per-rhs milliseconds
+ * of the three tiles, in the order above, to ms[3] (may be NULL).  The other state buffer
+ * (and, for MDRK, the stage buffer) is used as scratch. */
+int tlw_set_elements_per_workgroup(tlw_ctx* ctx, int32_t elements);
+int tlw_get_elements_per_workgroup(const tlw_ctx* ctx, int32_t* elements);
+int tlw_autotune(tlw_ctx* ctx, int32_t reps, int32_t* chosen, double* ms);
+
+/* Copy an element-cache array to the host.  TLW_AUX_U: Ũ [v,i,j,e], len 4 n^2 E, and
+ * TLW_AUX_F: F̃ [v,d,i,j,e], len 8 n^2 E, of the last rhs call (need TLW_FLAG_KEEP_AUX);
+ * TLW_AUX_ALPHA: the smoothed blending factors [E] of the last rhs call or device-loop
+ * stage (shock capturing; Trixi's indicator cache alpha). */
 int tlw_get_aux(tlw_ctx* ctx, int32_t which, double* out_host, size_t len);
 
 #ifdef __cplusplus

@@ -39,6 +39,10 @@ class _Desc(C.Structure):
         ("abstol", C.c_double),
         ("reltol", C.c_double),
         ("nthreads", C.c_int),
+        ("shock_capturing", C.c_int),
+        ("alpha_max", C.c_double),
+        ("alpha_min", C.c_double),
+        ("alpha_smooth", C.c_int),
     ]
 
 
@@ -78,6 +82,8 @@ def lib():
         L.tlwo_get_aux.restype = C.c_longlong
         L.tlwo_get_aux.argtypes = [C.c_void_p, C.c_int, dp]
         L.tlwo_euler_fluxes.argtypes = [dp, C.c_double, dp, dp]
+        L.tlwo_set_alpha.argtypes = [C.c_void_p, dp]
+        L.tlwo_inverse_vandermonde.argtypes = [C.c_void_p, dp]
         L.tlwo_max_abs_speed_naive.restype = C.c_double
         L.tlwo_max_abs_speed_naive.argtypes = [dp, dp, C.c_int, C.c_double]
         L.tlwo_solve.restype = C.c_int
@@ -91,14 +97,16 @@ def _ptr(a: np.ndarray):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
-AUX = {"U": 0, "F": 1, "temporal_errors": 2, "surface_flux_values": 3, "us": 4, "U2": 5, "F2": 6}
+AUX = {"U": 0, "F": 1, "temporal_errors": 2, "surface_flux_values": 3, "us": 4, "U2": 5, "F2": 6, "alpha": 7,
+       "fn_low": 8}
 
 
 class Oracle:
     """One oracle solver instance on a uniform TreeMesh (elements e = ix + nx*iy)."""
 
     def __init__(self, polydeg, nx, ny, xmin, ymin, h, periodic=(True, True), equation="euler", gamma=1.4,
-                 adv=(1.0, 1.0), flux_order=0, bc_kind=BV_NONE, bc_par=(), tolerances=(0.0, 0.0), nthreads=0):
+                 adv=(1.0, 1.0), flux_order=0, bc_kind=BV_NONE, bc_par=(), tolerances=(0.0, 0.0), nthreads=0,
+                 shock_capturing=False, alpha_max=0.5, alpha_min=0.001, alpha_smooth=True):
         d = _Desc()
         d.polydeg, d.nx, d.ny = polydeg, nx, ny
         d.xmin, d.ymin, d.h = xmin, ymin, h
@@ -112,6 +120,9 @@ class Oracle:
             d.bc_par[k] = v
         d.abstol, d.reltol = tolerances
         d.nthreads = nthreads
+        d.shock_capturing = int(shock_capturing)
+        d.alpha_max, d.alpha_min = alpha_max, alpha_min
+        d.alpha_smooth = int(alpha_smooth)
         self._h = lib().tlwo_create(C.byref(d))
         if not self._h:
             raise ValueError("oracle supports polydeg 3 and 4")
@@ -157,6 +168,19 @@ class Oracle:
         fn = {"lw": lib().tlwo_rhs, "mdrk1": lib().tlwo_rhs_mdrk1, "mdrk2": lib().tlwo_rhs_mdrk2}[kind]
         fn(self._h, _ptr(u), _ptr(du), t, dt)
         return du
+
+    def set_alpha(self, alpha):
+        """Prescribe the blending factors (None returns to the indicator)."""
+        if alpha is None:
+            lib().tlwo_set_alpha(self._h, None)
+        else:
+            a = np.ascontiguousarray(alpha, dtype=np.float64)
+            lib().tlwo_set_alpha(self._h, _ptr(a))
+
+    def inverse_vandermonde(self):
+        out = np.zeros((self.n, self.n))
+        lib().tlwo_inverse_vandermonde(self._h, _ptr(out))
+        return out
 
     def set_us(self, us):
         us = np.ascontiguousarray(us, dtype=np.float64)

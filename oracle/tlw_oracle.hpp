@@ -126,7 +126,41 @@ struct Basis {
         // calc_lhat at -1 and +1 (the interpolant is exact at the end nodes)
         bnd_lo = 1.0 / weights[0];
         bnd_hi = 1.0 / weights[n - 1];
+        // Trixi vandermonde_legendre: V[i][m] = sqrt(m + 1/2) L_m(x_i); its inverse
+        // maps nodal values to modal coefficients (IndicatorHennemannGassner)
+        long double V[MAXN][2 * MAXN];
+        for (int i = 0; i < n; ++i) {
+            long double Lm2 = 1, Lm1 = x[i];
+            for (int m = 0; m < n; ++m) {
+                long double L;
+                if (m == 0) L = 1;
+                else if (m == 1) L = x[i];
+                else {
+                    L = ((2.0L * m - 1) * x[i] * Lm1 - (m - 1.0L) * Lm2) / m;
+                    Lm2 = Lm1;
+                    Lm1 = L;
+                }
+                V[i][m] = L * sqrtl(m + 0.5L);
+                V[i][n + m] = (i == m) ? 1.0L : 0.0L;
+            }
+        }
+        for (int col = 0; col < n; ++col) {  // Gauss-Jordan with partial pivoting
+            int piv = col;
+            for (int r = col + 1; r < n; ++r)
+                if (fabsl(V[r][col]) > fabsl(V[piv][col])) piv = r;
+            for (int q = 0; q < 2 * n; ++q) std::swap(V[col][q], V[piv][q]);
+            const long double d = V[col][col];
+            for (int q = 0; q < 2 * n; ++q) V[col][q] /= d;
+            for (int r = 0; r < n; ++r)
+                if (r != col) {
+                    const long double f = V[r][col];
+                    for (int q = 0; q < 2 * n; ++q) V[r][q] -= f * V[col][q];
+                }
+        }
+        for (int i = 0; i < n; ++i)
+            for (int m = 0; m < n; ++m) Vinv[i][m] = (double)V[i][n + m];
     }
+    double Vinv[MAXN][MAXN] = {};
 };
 
 // ---------------------------------------------------------------------------
@@ -179,6 +213,11 @@ template <class T> struct Euler2D {
         l1 = abs_(v1) + c;
         l2 = abs_(v2) + c;
     }
+    // Trixi density_pressure: ρ p = (γ-1)(ρ ρe - ½(ρv1² + ρv2²)), the
+    // indicator variable of the shock-capturing configuration
+    T density_pressure(const T* u) const {
+        return (gamma - 1.0) * (u[0] * u[3] - 0.5 * (u[1] * u[1] + u[2] * u[2]));
+    }
     // src/equations/compressible_euler_2d.jl:4-7
     bool is_admissible(const T* u) const {
         T rho, v1, v2, p;
@@ -203,6 +242,7 @@ template <class T> struct LinAdv2D {
         return T(std::fabs(orientation == 1 ? a1 : a2));
     }
     void max_abs_speeds(const T*, T& l1, T& l2) const { l1 = T(std::fabs(a1)); l2 = T(std::fabs(a2)); }
+    T density_pressure(const T* u) const { return u[0]; }
     bool is_admissible(const T*) const { return true; }
 };
 
@@ -319,6 +359,14 @@ template <class T, class Eq> struct Solver {
     // 1: Structured/P4est order (λ from uⁿ, jump on Ũ; structured/dg_2d.jl:2271)
     int flux_order = 0;
     double abstol = 0.0, reltol = 0.0;
+    // shock capturing (VolumeIntegralFRShockCapturing with IndicatorHennemannGassner
+    // on density_pressure and first-order subcell FV, src/solvers/fr.jl:63-86)
+    bool sc = false;
+    double alpha_max = 0.5, alpha_min = 0.001;
+    bool alpha_smooth = true;
+    std::vector<double> alpha, alpha_tmp, alpha_override;
+    double sc_dt = 0.0;  // dt passed to compute_alp
+    std::vector<T> fn_low;  // [v, l, face, e]
     std::vector<T> U, F, sfv, temporal_errors;
     std::vector<T> us, U2, F2;  // MDRK stage cache (containers.jl:83-98)
 
@@ -329,6 +377,9 @@ template <class T, class Eq> struct Solver {
         F.assign(2 * nd, T(0));
         sfv.assign((size_t)mesh.E * 4 * n * NV, T(0));
         temporal_errors.assign(mesh.E, T(0));
+        alpha.assign(mesh.E, 0.0);
+        alpha_tmp.assign(mesh.E, 0.0);
+        fn_low.assign((size_t)mesh.E * 4 * n * NV, T(0));
     }
     // the MDRK stage cache is allocated on first use
     void ensure_mdrk() {
@@ -790,6 +841,123 @@ template <class T, class Eq> struct Solver {
             }
     }
 
+
+    // ---------------- shock capturing ----------------
+    // Trixi IndicatorHennemannGassner (Hennemann et al., JCP 426, 2021):
+    // modal energy of density_pressure, threshold 0.5·10^(-1.8 n^0.25),
+    // sharpness s = ln((1-1e-4)/1e-4), clipping to 0 below alpha_min and to 1
+    // above 1-alpha_min, alpha_max cap, then smoothing over interfaces
+    // (alpha_e >= ½ alpha_neighbour), called at dg_2d.jl:1665.
+    void indicator_hg(const T* u) {
+        const double threshold = 0.5 * std::pow(10.0, -1.8 * std::pow((double)n, 0.25));
+        const double par_s = std::log((1 - 0.0001) / 0.0001);
+#pragma omp parallel for schedule(static)
+        for (int e = 0; e < mesh.E; ++e) {
+            double ind[MAXN][MAXN], tmp[MAXN][MAXN], modal[MAXN][MAXN];
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < n; ++i) ind[j][i] = value(eq.density_pressure(&u[idx(0, i, j, e)]));
+            // modal[j][i] = Σ Vinv[i][ii] Vinv[j][jj] ind[jj][ii]
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < n; ++i) {
+                    double acc = 0;
+                    for (int ii = 0; ii < n; ++ii) acc += basis.Vinv[i][ii] * ind[j][ii];
+                    tmp[j][i] = acc;
+                }
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < n; ++i) {
+                    double acc = 0;
+                    for (int jj = 0; jj < n; ++jj) acc += basis.Vinv[j][jj] * tmp[jj][i];
+                    modal[j][i] = acc;
+                }
+            double total = 0, clip1 = 0, clip2 = 0;
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < n; ++i) total += modal[j][i] * modal[j][i];
+            for (int j = 0; j < n - 1; ++j)
+                for (int i = 0; i < n - 1; ++i) clip1 += modal[j][i] * modal[j][i];
+            for (int j = 0; j < n - 2; ++j)
+                for (int i = 0; i < n - 2; ++i) clip2 += modal[j][i] * modal[j][i];
+            const double f1 = total != 0 ? (total - clip1) / total : 0.0;
+            const double f2 = clip1 != 0 ? (clip1 - clip2) / clip1 : 0.0;
+            const double energy = std::max(f1, f2);
+            double a = 1.0 / (1.0 + std::exp(-par_s / threshold * (energy - threshold)));
+            if (a < alpha_min) a = 0.0;
+            if (a > 1 - alpha_min) a = 1.0;
+            alpha[e] = std::min(alpha_max, a);
+        }
+        if (alpha_smooth) {
+            alpha_tmp = alpha;
+            const int nI = (int)mesh.if_left.size();
+            for (int I = 0; I < nI; ++I) {
+                const int L = mesh.if_left[I], R = mesh.if_right[I];
+                alpha[L] = std::max(std::max(alpha_tmp[L], 0.5 * alpha_tmp[R]), alpha[L]);
+                alpha[R] = std::max(std::max(alpha_tmp[R], 0.5 * alpha_tmp[L]), alpha[R]);
+            }
+        }
+    }
+
+    // Trixi flux_lax_friedrichs on two states (FluxPlusDissipation of
+    // flux_central and DissipationLocalLaxFriedrichs with max_abs_speed_naive)
+    void rusanov(const T* ul, const T* ur, int orientation, T* out) const {
+        T fl[NV], fr[NV];
+        eq.flux(ul, orientation, fl);
+        eq.flux(ur, orientation, fr);
+        const T lam = eq.max_abs_speed_naive(ul, ur, orientation);
+        for (int v = 0; v < NV; ++v) out[v] = 0.5 * (fl[v] + fr[v]) + (-0.5 * lam) * (ur[v] - ul[v]);
+    }
+
+    // First-order subcell FV of a blended element (fv_kernel!, FirstOrderReconstruction,
+    // dg_2d.jl:251-282, with Trixi calcflux_fv!) and the fluxes next to the element
+    // faces stored in fn_low (load_fn_low!, dg_2d.jl:220-234).  With fv_scale = 0
+    // only fn_low is produced (calc_fn_low!, dg_2d.jl:1948-1991).
+    void fv_kernel_fo(T* du, const T* u, int e, double fv_scale, bool add) {
+        // fx[j][i]: flux between nodes i-1 and i (i = 1..n-1); fx[j][0] = fx[j][n] = 0
+        T fx[MAXN][MAXN + 1][NV], gy[MAXN + 1][MAXN][NV];
+        for (int j = 0; j < n; ++j)
+            for (int v = 0; v < NV; ++v) { fx[j][0][v] = T(0); fx[j][n][v] = T(0); }
+        for (int i = 0; i < n; ++i)
+            for (int v = 0; v < NV; ++v) { gy[0][i][v] = T(0); gy[n][i][v] = T(0); }
+        for (int j = 0; j < n; ++j)
+            for (int i = 1; i < n; ++i) rusanov(&u[idx(0, i - 1, j, e)], &u[idx(0, i, j, e)], 1, fx[j][i]);
+        for (int j = 1; j < n; ++j)
+            for (int i = 0; i < n; ++i) rusanov(&u[idx(0, i, j - 1, e)], &u[idx(0, i, j, e)], 2, gy[j][i]);
+        if (add) {
+            const auto& iw = basis.inv_w;
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < n; ++i)
+                    for (int v = 0; v < NV; ++v)
+                        du[idx(v, i, j, e)] = du[idx(v, i, j, e)] +
+                                              fv_scale * (iw[i] * (fx[j][i + 1][v] - fx[j][i][v]) +
+                                                          iw[j] * (gy[j + 1][i][v] - gy[j][i][v]));
+        }
+        for (int k = 0; k < n; ++k)
+            for (int v = 0; v < NV; ++v) {
+                fn_low[sidx(v, k, 0, e)] = fx[k][1][v];
+                fn_low[sidx(v, k, 1, e)] = fx[k][n - 1][v];
+                fn_low[sidx(v, k, 2, e)] = gy[1][k][v];
+                fn_low[sidx(v, k, 3, e)] = gy[n - 1][k][v];
+            }
+    }
+
+    // compute_alp for VolumeIntegralFRShockCapturing on TreeMesh (dg_2d.jl:1879-1901):
+    // average of the two blending factors, reset to 1 when the low-order update
+    // of either adjacent node with the blended flux is not admissible.  J is the
+    // interface cache's inverse Jacobian, the first element's value on a TreeMesh
+    // (containers_2d.jl load_inverse_jacobian!).
+    double compute_alp(const T* ul, const T* ur, int L, int R, double J, double dt, const T* fn, const T* Fn_,
+                       const T* fin_ll, const T* fin_rr) const {
+        const double alp = 0.5 * (alpha[L] + alpha[R]);
+        T Fn[NV], low[NV];
+        for (int v = 0; v < NV; ++v) Fn[v] = (1.0 - alp) * Fn_[v] + alp * fn[v];
+        const double lx = 0.5;
+        const double cl = dt * J / (basis.weights[n - 1] * lx);
+        for (int v = 0; v < NV; ++v) low[v] = ul[v] - cl * (Fn[v] - fin_ll[v]);
+        if (!eq.is_admissible(low)) return 1.0;
+        const double cr = dt * J / (basis.weights[0] * lx);
+        for (int v = 0; v < NV; ++v) low[v] = ur[v] - cr * (fin_rr[v] - Fn[v]);
+        if (!eq.is_admissible(low)) return 1.0;
+        return alp;
+    }
+
     // ---------------- interface flux ----------------
     // The 8-argument FluxPlusDissipation with flux_central and the 6-argument
     // DissipationLocalLaxFriedrichs (src/equations/numerical_fluxes.jl:7-38):
@@ -829,6 +997,15 @@ template <class T, class Eq> struct Solver {
                 const T* fr = &F[fidx(0, o - 1, ri, rj, R)];
                 T Fn[NV];
                 lw_numflux(fl, fr, Ul, Ur, ul, ur, o, Fn);
+                if (sc) {
+                    T fn[NV];
+                    rusanov(ul, ur, o, fn);
+                    const T* fin_ll = &fn_low[sidx(0, l, ldir, L)];
+                    const T* fin_rr = &fn_low[sidx(0, l, rdir, R)];
+                    const double J = mesh.inv_jac[0];
+                    const double alp = compute_alp(ul, ur, L, R, J, sc_dt, fn, Fn, fin_ll, fin_rr);
+                    for (int v = 0; v < NV; ++v) Fn[v] = alp * fn[v] + (1.0 - alp) * Fn[v];
+                }
                 for (int v = 0; v < NV; ++v) {
                     sfv[sidx(v, l, ldir, L)] = Fn[v];
                     sfv[sidx(v, l, rdir, R)] = Fn[v];
@@ -910,10 +1087,18 @@ template <class T, class Eq> struct Solver {
 
     void rhs_lw(T* du, const T* u, double t, double dt) {
         reset_du(du);  // reset_du!
+        // calc_volume_integral! for VolumeIntegralFRShockCapturing (dg_2d.jl:1648-1804)
+        if (sc) sc_prepare(u, dt);
 #pragma omp parallel for schedule(static)
         for (int e = 0; e < mesh.E; ++e) {
-            if (basis.N == 4) lw_volume_kernel_4(du, u, dt, e, 1.0);
-            else lw_volume_kernel_3(du, u, dt, e, 1.0);
+            const double a_e = sc ? alpha[e] : 0.0;
+            if (basis.N == 4) lw_volume_kernel_4(du, u, dt, e, 1.0 - a_e);
+            else lw_volume_kernel_3(du, u, dt, e, 1.0 - a_e);
+            if (sc) {
+                // pure_and_blended_element_ids!: alpha ≈ 0 (atol 1e-12) is pure DG
+                const bool dg_only = std::fabs(a_e) <= 1e-12;
+                fv_kernel_fo(du, u, e, a_e, !dg_only);
+            }
         }
         calc_interface_flux(u);
         if (!mesh.bd_elem.empty()) calc_boundary_flux(u, t, dt, 1.0);
@@ -922,11 +1107,28 @@ template <class T, class Eq> struct Solver {
 
     // rhs_mdrk1! / rhs_mdrk2!, tree/multi_derivative_2d.jl:1-116.  Both stages
     // prolong uⁿ (the integrator's u) to the interfaces; stage 2 reads `us`.
+    // shock-capturing prologue of an rhs call: blending factors from uⁿ
+    void sc_prepare(const T* u, double dt) {
+        if (alpha_override.size() == (size_t)mesh.E) alpha = alpha_override;
+        else indicator_hg(u);
+        sc_dt = dt;
+    }
+    void sc_element(T* du, const T* u, int e) {
+        const double a_e = alpha[e];
+        fv_kernel_fo(du, u, e, a_e, !(std::fabs(a_e) <= 1e-12));
+    }
+
+    // calc_volume_integral_mdrk1!/2! with shock capturing (tree/multi_derivative_2d.jl:151-261):
+    // the MDRK kernel scaled by 1 - α plus first-order FV of uⁿ scaled by α.
     void rhs_mdrk1(T* du, const T* u, double t, double dt) {
         ensure_mdrk();
         reset_du(du);
+        if (sc) sc_prepare(u, dt);
 #pragma omp parallel for schedule(static)
-        for (int e = 0; e < mesh.E; ++e) mdrk_kernel_1(du, u, dt, e, 1.0);
+        for (int e = 0; e < mesh.E; ++e) {
+            mdrk_kernel_1(du, u, dt, e, sc ? 1.0 - alpha[e] : 1.0);
+            if (sc) sc_element(du, u, e);
+        }
         calc_interface_flux(u);
         if (!mesh.bd_elem.empty()) calc_boundary_flux(u, t, dt, 0.5);
         surface_integral_and_jacobian(du);
@@ -934,8 +1136,12 @@ template <class T, class Eq> struct Solver {
     void rhs_mdrk2(T* du, const T* u, double t, double dt) {
         ensure_mdrk();
         reset_du(du);
+        if (sc) sc_prepare(u, dt);
 #pragma omp parallel for schedule(static)
-        for (int e = 0; e < mesh.E; ++e) mdrk_kernel_2(du, dt, e, 1.0);
+        for (int e = 0; e < mesh.E; ++e) {
+            mdrk_kernel_2(du, dt, e, sc ? 1.0 - alpha[e] : 1.0);
+            if (sc) sc_element(du, u, e);
+        }
         calc_interface_flux(u);
         if (!mesh.bd_elem.empty()) calc_boundary_flux(u, t, dt, 1.0);
         surface_integral_and_jacobian(du);

@@ -24,6 +24,9 @@ struct tlwo_desc {
     double bc_par[24];
     double abstol, reltol;
     int nthreads;    // 0: leave OpenMP's default
+    int shock_capturing;
+    double alpha_max, alpha_min;
+    int alpha_smooth;
 };
 
 struct tlwo_ctx {
@@ -40,6 +43,10 @@ template <class S> static void configure(S& s, const tlwo_desc* d) {
     for (int k = 0; k < 24; ++k) s.bval.par[k] = d->bc_par[k];
     s.abstol = d->abstol;
     s.reltol = d->reltol;
+    s.sc = d->shock_capturing != 0;
+    s.alpha_max = d->alpha_max;
+    s.alpha_min = d->alpha_min;
+    s.alpha_smooth = d->alpha_smooth != 0;
     s.setup();
 }
 
@@ -145,7 +152,8 @@ void tlwo_set_us(void* ctx, const double* us) {
 }
 
 // which: 0 = Ũ [v,i,j,e], 1 = F̃ [v,d,i,j,e], 2 = temporal_errors [e],
-//        3 = surface_flux_values [v,l,face,e], 4 = us, 5 = U2, 6 = F2
+//        3 = surface_flux_values [v,l,face,e], 4 = us, 5 = U2, 6 = F2,
+//        7 = alpha [e], 8 = fn_low [v,l,face,e]
 long long tlwo_get_aux(void* ctx, int which, double* out) {
     long long len = -1;
     DISPATCH(ctx, {
@@ -158,6 +166,8 @@ long long tlwo_get_aux(void* ctx, int which, double* out) {
             case 4: src = &s.us; break;
             case 5: src = &s.U2; break;
             case 6: src = &s.F2; break;
+            case 7: src = &s.alpha; break;
+            case 8: src = &s.fn_low; break;
         }
         if (src) {
             len = (long long)src->size();
@@ -172,6 +182,22 @@ int tlwo_solve(void* ctx, double* u, double t0, double tf, double cfl, int max_s
     int r = 0;
     DISPATCH(ctx, r = solve(s, u, t0, tf, cfl, max_steps, mdrk, t_out, dts));
     return r;
+}
+
+// Prescribed blending factors (len E) used instead of the indicator; NULL clears.
+void tlwo_set_alpha(void* ctx, const double* alpha) {
+    DISPATCH(ctx, {
+        if (alpha) s.alpha_override.assign(alpha, alpha + s.mesh.E);
+        else s.alpha_override.clear();
+    });
+}
+
+// Inverse Legendre Vandermonde of the basis (n*n, row-major [i][m]).
+void tlwo_inverse_vandermonde(void* ctx, double* out) {
+    DISPATCH(ctx, {
+        for (int i = 0; i < s.n; ++i)
+            for (int m = 0; m < s.n; ++m) out[i * s.n + m] = s.basis.Vinv[i][m];
+    });
 }
 
 // Pointwise Euler pieces, for closed-form checks.

@@ -1,0 +1,61 @@
+"""Shared set-up of the GPU parity tests: one device semidiscretization and
+the oracle on the same mesh, state and dt, plus element-order conversion."""
+import numpy as np
+
+from conftest import rel_l2, rel_linf
+
+TOL = 1e-12
+
+
+def torch_mod():
+    import torch
+
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    return torch
+
+
+def setup(oracle_mod, N, level, td="lw", periodic=True, domain=(-10.0, 10.0), bc=None, flux_order="tree",
+          ordering="lexicographic", keep_aux=True, seed=0, amp=1e-3, ic=None, sc=False, alpha_max=0.5,
+          alpha_smooth=True):
+    """``sc`` selects VolumeIntegralFRShockCapturing (Hennemann-Gassner on
+    density_pressure, first-order subcell FV) on both sides."""
+    import trixilw_hip as tlw
+
+    eq = tlw.CompressibleEulerEquations2D(1.4)
+    ic = ic or tlw.IsentropicVortex(center=(0.7, -0.4), period=(domain[1] - domain[0],) * 2)
+    mesh = tlw.TreeMesh((domain[0],) * 2, (domain[1],) * 2, level, periodicity=periodic, ordering=ordering)
+    vi = tlw.VolumeIntegralFR(tlw.MDRK() if td == "mdrk" else tlw.LW())
+    if sc:
+        ind = tlw.IndicatorHennemannGassner(eq, None, alpha_max=alpha_max, alpha_min=0.001, alpha_smooth=alpha_smooth)
+        vi = tlw.VolumeIntegralFRShockCapturing(ind, volume_integralFR=vi)
+    solver = tlw.DGSEM(N, volume_integral=vi)
+    semi = tlw.SemidiscretizationHyperbolic(mesh, tlw.get_time_discretization(solver), eq, ic, solver,
+                                            boundary_conditions=bc, surface_flux_order=flux_order,
+                                            keep_aux=keep_aux)
+    nx = mesh.nx
+    o = oracle_mod.Oracle(N, nx, nx, domain[0], domain[0], (domain[1] - domain[0]) / nx, periodic=(periodic,) * 2,
+                          flux_order=0 if flux_order == "tree" else 1,
+                          bc_kind=bc.kind if bc is not None else 0, bc_par=bc.params if bc is not None else (),
+                          shock_capturing=sc, alpha_max=alpha_max, alpha_min=0.001, alpha_smooth=alpha_smooth)
+    x, y = o.node_coords()
+    rng = np.random.default_rng(seed)
+    u_lex = ic(x, y, 0.0) * (1.0 + amp * rng.standard_normal(x.shape + (4,)))
+    return semi, o, u_lex, mesh
+
+
+def to_dev(mesh, u_lex):
+    torch = torch_mod()
+    perm = mesh.lexicographic_permutation()  # lex index -> element
+    u = np.empty_like(u_lex)
+    u[perm] = u_lex
+    return torch.from_numpy(np.ascontiguousarray(u)).cuda()
+
+
+def to_lex(mesh, u_dev):
+    a = u_dev.detach().cpu().numpy() if hasattr(u_dev, "detach") else u_dev
+    return a[mesh.lexicographic_permutation()]
+
+
+def assert_close(a, b, tol=TOL, what=""):
+    l2, li = rel_l2(a, b), rel_linf(a, b)
+    assert l2 <= tol and li <= tol, f"{what}: rel L2 {l2:.3e}, rel Linf {li:.3e}"

@@ -54,31 +54,64 @@ class LinearLW:
             iy = (iy + 1) % self.ny
         return ix + self.nx * iy
 
-    def fr_update(self, u, Ut, Qt, dt):
-        """u + dt * du with F̃ = a Qt, Ũ = Ut, Rusanov jump on u (tree) or Ũ (structured)."""
+    def upwind(self, ul, ur, a):
+        """Rusanov flux of f = a u: 0.5 a (ul + ur) - 0.5 |a| (ur - ul)."""
+        return 0.5 * a * (ul + ur) - 0.5 * abs(a) * (ur - ul)
+
+    def subcell_fv(self, u):
+        """First-order subcell FV divergence on the GLL subcells of each element,
+        (f_{i+1/2} - f_{i-1/2}) / w_i with zero flux at the element faces."""
+        n = self.n
+        out = np.zeros_like(u)
+        fx = np.zeros(u.shape[:2] + (n + 1,))
+        fx[:, :, 1:n] = self.upwind(u[:, :, :-1], u[:, :, 1:], self.a1)
+        gy = np.zeros((u.shape[0], n + 1, n))
+        gy[:, 1:n, :] = self.upwind(u[:, :-1, :], u[:, 1:, :], self.a2)
+        out += (fx[:, :, 1:] - fx[:, :, :-1]) / self.w[None, None, :]
+        out += (gy[:, 1:, :] - gy[:, :-1, :]) / self.w[None, :, None]
+        return out
+
+    def fr_update(self, u, Ut, Qt, dt, alpha=None):
+        """u + dt * du with F̃ = a Qt, Ũ = Ut, Rusanov jump on u (tree) or Ũ (structured).
+
+        With ``alpha`` (one blending factor per element) the update is the
+        shock-capturing blend: (1 - α_e) times the LW volume term, α_e times
+        the subcell FV divergence of u, and interface fluxes
+        alp·fn + (1 - alp)·F* with alp = (α_L + α_R)/2 and fn the Rusanov
+        flux of u (linear advection is always admissible).
+        """
         n = self.n
         F, G = self.a1 * Qt, self.a2 * Qt
         vol = self.dx(F, self.Dh) + self.dy(G, self.Dh)
+        if alpha is not None:
+            vol = (1.0 - alpha)[:, None, None] * vol + alpha[:, None, None] * self.subcell_fv(u)
         jump = u if self.flux_order == "tree" else Ut
         surf = np.zeros_like(u)
         l1, l2 = abs(self.a1), abs(self.a2)
+
+        def blend(fs, ul, ur, a, L, R):
+            if alpha is None:
+                return fs
+            alp = 0.5 * (alpha[L] + alpha[R])
+            return alp * self.upwind(ul, ur, a) + (1.0 - alp) * fs
+
         for e in range(self.E):
             L = self.nbr(e, 0)
             fs = 0.5 * (F[L, :, n - 1] + F[e, :, 0]) - 0.5 * l1 * (jump[e, :, 0] - jump[L, :, n - 1])
-            surf[e, :, 0] -= fs / self.w[0]
+            surf[e, :, 0] -= blend(fs, u[L, :, n - 1], u[e, :, 0], self.a1, L, e) / self.w[0]
             R = self.nbr(e, 1)
             fs = 0.5 * (F[e, :, n - 1] + F[R, :, 0]) - 0.5 * l1 * (jump[R, :, 0] - jump[e, :, n - 1])
-            surf[e, :, n - 1] += fs / self.w[n - 1]
+            surf[e, :, n - 1] += blend(fs, u[e, :, n - 1], u[R, :, 0], self.a1, e, R) / self.w[n - 1]
             B = self.nbr(e, 2)
             gs = 0.5 * (G[B, n - 1, :] + G[e, 0, :]) - 0.5 * l2 * (jump[e, 0, :] - jump[B, n - 1, :])
-            surf[e, 0, :] -= gs / self.w[0]
+            surf[e, 0, :] -= blend(gs, u[B, n - 1, :], u[e, 0, :], self.a2, B, e) / self.w[0]
             T = self.nbr(e, 3)
             gs = 0.5 * (G[e, n - 1, :] + G[T, 0, :]) - 0.5 * l2 * (jump[T, 0, :] - jump[e, n - 1, :])
-            surf[e, n - 1, :] += gs / self.w[n - 1]
+            surf[e, n - 1, :] += blend(gs, u[e, n - 1, :], u[T, 0, :], self.a2, e, T) / self.w[n - 1]
         du = -self.invJ * (vol + surf)
         return u + dt * du, du
 
-    def lw_step(self, u, dt):
+    def lw_step(self, u, dt, alpha=None):
         N = self.n - 1
         terms = [u]
         for k in range(1, N + 1):
@@ -89,14 +122,14 @@ class LinearLW:
         if N == 4:
             qf[4] = 1.0 / 240
         Qt = sum(qf[k] * terms[k] for k in range(N + 1))
-        return self.fr_update(u, Ut, Qt, dt)
+        return self.fr_update(u, Ut, Qt, dt, alpha)
 
-    def mdrk_step(self, u, dt):
+    def mdrk_step(self, u, dt, alpha=None):
         Au = self.A(u, dt)
         S1 = 0.5 * u + 0.125 * Au
-        us, du1 = self.fr_update(u, S1, S1, dt)
+        us, du1 = self.fr_update(u, S1, S1, dt, alpha)
         S2 = u + Au / 6.0 + self.A(us, dt) / 3.0
-        unew, du2 = self.fr_update(u, S2, S2, dt)
+        unew, du2 = self.fr_update(u, S2, S2, dt, alpha)
         return unew, du1, du2, us
 
     def step_matrix(self, dt, kind="lw"):

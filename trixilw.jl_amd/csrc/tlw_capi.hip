@@ -39,9 +39,15 @@ struct tlw_ctx {
     double* d_auxF = nullptr;
     double* d_partial = nullptr;
     double* d_elem_max = nullptr;
+    double* d_alpha_tmp = nullptr;
+    double* d_alpha = nullptr;
+    bool sc = false;
+    int ebv = 0;  // elements per volume workgroup
+    double invJ0 = 0.0;
     bool has_boundary = false;
     int host_cur = 0;
     bool aux_valid = false;
+    bool alpha_valid = false;  // blending factors of the last rhs or step (shock capturing)
     // kernel timing (tlw_enable_kernel_timing)
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
@@ -70,6 +76,7 @@ size_t state_len(const tlw_ctx* c) { return (size_t)4 * c->n * c->n * c->E; }
 
 int vgrid(const tlw_ctx* c);
 int sgrid(const tlw_ctx* c);
+int min_eb(int N);
 
 void free_mesh(tlw_ctx* c) {
     (void)hipFree(c->d_nbr);
@@ -84,8 +91,11 @@ void free_mesh(tlw_ctx* c) {
     (void)hipFree(c->d_auxF);
     (void)hipFree(c->d_partial);
     (void)hipFree(c->d_elem_max);
+    (void)hipFree(c->d_alpha_tmp);
+    (void)hipFree(c->d_alpha);
     c->d_partial = nullptr;
     c->d_elem_max = nullptr;
+    c->d_alpha_tmp = c->d_alpha = nullptr;
     c->d_nbr = nullptr;
     c->d_invj = c->d_center = c->d_length = c->d_trace = c->d_us = c->d_auxU = c->d_auxF = nullptr;
     c->d_buf[0] = c->d_buf[1] = nullptr;
@@ -111,6 +121,7 @@ int upload_mesh(tlw_ctx* c, const tlw_mesh_desc* m) {
         if (!(m->inverse_jacobian_host[e] > 0.0) || !(m->element_length_host[e] > 0.0))
             return fail(c, TLW_EINVAL, "mesh description: non-positive Jacobian or element length");
     free_mesh(c);
+    c->aux_valid = c->alpha_valid = false;
     c->E = E;
     c->has_boundary = false;
     for (size_t k = 0; k < (size_t)4 * E; ++k)
@@ -122,12 +133,19 @@ int upload_mesh(tlw_ctx* c, const tlw_mesh_desc* m) {
     HIPCHK(c, hipMalloc(&c->d_invj, sizeof(double) * E));
     HIPCHK(c, hipMalloc(&c->d_center, sizeof(double) * 2 * E));
     HIPCHK(c, hipMalloc(&c->d_length, sizeof(double) * E));
-    HIPCHK(c, hipMalloc(&c->d_trace, sizeof(double) * (size_t)E * 4 * c->n * kTraceQ));
+    HIPCHK(c, hipMalloc(&c->d_trace, sizeof(double) * (size_t)E * 4 * c->n * (c->sc ? kTraceQSC : kTraceQ)));
     HIPCHK(c, hipMalloc(&c->d_buf[0], sizeof(double) * nd));
     HIPCHK(c, hipMalloc(&c->d_buf[1], sizeof(double) * nd));
     HIPCHK(c, hipMalloc(&c->d_us, sizeof(double) * nd));
-    HIPCHK(c, hipMalloc(&c->d_partial, sizeof(double) * (size_t)(vgrid(c) + sgrid(c) + 1)));
+    // sized for the smallest tile, so the tile can change without a reallocation
+    const int vmax = (E + min_eb(c->N) - 1) / min_eb(c->N);
+    HIPCHK(c, hipMalloc(&c->d_partial, sizeof(double) * (size_t)(vmax + sgrid(c) + 1)));
     HIPCHK(c, hipMalloc(&c->d_elem_max, sizeof(double) * 2 * (size_t)E));
+    if (c->sc) {
+        HIPCHK(c, hipMalloc(&c->d_alpha_tmp, sizeof(double) * (size_t)E));
+        HIPCHK(c, hipMalloc(&c->d_alpha, sizeof(double) * (size_t)E));
+    }
+    c->invJ0 = m->inverse_jacobian_host[0];
     if (c->flags & TLW_FLAG_KEEP_AUX) {
         HIPCHK(c, hipMalloc(&c->d_auxU, sizeof(double) * nd));
         HIPCHK(c, hipMalloc(&c->d_auxF, sizeof(double) * 2 * nd));
@@ -155,28 +173,54 @@ KArgs base_args(tlw_ctx* c) {
     a.partial = c->d_partial;
     a.elem_max = c->d_elem_max;
     a.vol_grid = vgrid(c);
+    a.ebv = c->ebv;
+    a.alpha_tmp = c->d_alpha_tmp;
+    a.alpha_out = c->d_alpha;
+    a.invJ0 = c->invJ0;
     a.bc_scale = 1.0;
     return a;
 }
 
-// Elements per volume workgroup.  N=3: 16 (256 lanes, 16 consecutive Z-ordered
-// elements form a 4x4 tree block); N=4: 15 (375 of 384 lanes, two workgroups per
-// CU at three waves per SIMD).
-template <int N> constexpr int kEBV = (N == 4) ? 15 : 16;
+// Elements per volume workgroup: the compiled tile sizes, selectable at run
+// time (tlw_set_elements_per_workgroup, tlw_autotune).  polydeg 4 (25 nodes):
+// 5, 10, 15 elements = 125, 250, 375 lanes of 128, 256, 384; polydeg 3
+// (16 nodes): 4, 8, 16 elements = 64, 128, 256 lanes, no idle lanes.
+constexpr int kEbChoices4[3] = {5, 10, 15};
+constexpr int kEbChoices3[3] = {4, 8, 16};
+const int* eb_choices(int N) { return N == 4 ? kEbChoices4 : kEbChoices3; }
+int default_eb(int N) { return N == 4 ? 15 : 16; }
+int min_eb(int N) { return N == 4 ? 5 : 4; }
+bool eb_supported(int N, int eb) {
+    const int* ch = eb_choices(N);
+    return eb == ch[0] || eb == ch[1] || eb == ch[2];
+}
 
-template <int N> int vgrid_of(const tlw_ctx* c) { return (c->E + kEBV<N> - 1) / kEBV<N>; }
 template <int N> int sgrid_of(const tlw_ctx* c) { return (c->E + SGeo<N>::EB - 1) / SGeo<N>::EB; }
-int vgrid(const tlw_ctx* c) { return c->N == 4 ? vgrid_of<4>(c) : vgrid_of<3>(c); }
+int vgrid(const tlw_ctx* c) { return (c->E + c->ebv - 1) / c->ebv; }
 int sgrid(const tlw_ctx* c) { return c->N == 4 ? sgrid_of<4>(c) : sgrid_of<3>(c); }
 
-// Kernel launches, dispatched on the polynomial degree.
-template <int N> void launch_volume_t(tlw_ctx* c, const KArgs& a, int kind) {
-    constexpr int EB = kEBV<N>;
-    dim3 g(vgrid_of<N>(c)), b(VGeo<N, EB>::NT);
-    if (kind == VOL_LW) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_LW>), g, b, 0, c->stream, a);
+// Kernel launches, dispatched on the polynomial degree and the tile size.
+template <int N, int EB, bool SC> void launch_volume_eb(tlw_ctx* c, const KArgs& a, int kind) {
+    dim3 g((c->E + EB - 1) / EB), b(VGeo<N, EB>::NT);
+    if (kind == VOL_LW) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_LW, SC>), g, b, 0, c->stream, a);
     else if (kind == VOL_MDRK1)
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_MDRK1>), g, b, 0, c->stream, a);
-    else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_MDRK2>), g, b, 0, c->stream, a);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_MDRK1, SC>), g, b, 0, c->stream, a);
+    else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_MDRK2, SC>), g, b, 0, c->stream, a);
+}
+template <int N, bool SC> void launch_volume_sc(tlw_ctx* c, const KArgs& a, int kind) {
+    constexpr const int* ch = N == 4 ? kEbChoices4 : kEbChoices3;
+    if (c->ebv == ch[0]) launch_volume_eb<N, ch[0], SC>(c, a, kind);
+    else if (c->ebv == ch[1]) launch_volume_eb<N, ch[1], SC>(c, a, kind);
+    else launch_volume_eb<N, ch[2], SC>(c, a, kind);
+}
+template <int N> void launch_volume_t(tlw_ctx* c, const KArgs& a, int kind) {
+    if (c->sc) {
+        // IndicatorHennemannGassner on uⁿ before the volume integral (dg_2d.jl:1665)
+        hipLaunchKernelGGL(k_indicator<N>, dim3(sgrid_of<N>(c)), dim3(kSurfBlock), 0, c->stream, a, c->d_alpha_tmp);
+        launch_volume_sc<N, true>(c, a, kind);
+    } else {
+        launch_volume_sc<N, false>(c, a, kind);
+    }
 }
 void launch_volume(tlw_ctx* c, const KArgs& a, int kind) {
     if (c->N == 4) launch_volume_t<4>(c, a, kind);
@@ -188,9 +232,14 @@ template <int N> void launch_finalize_t(tlw_ctx* c, int reduce, int nparts) {
 }
 template <int N> void launch_surface_t(tlw_ctx* c, const KArgs& a) {
     dim3 g(sgrid_of<N>(c)), b(kSurfBlock);
-    if (c->has_boundary) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_surface<N, kEBV<N>, true>), g, b, 0, c->stream, a);
-    else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_surface<N, kEBV<N>, false>), g, b, 0, c->stream, a);
-    if (a.reduce != RED_NONE) launch_finalize_t<N>(c, a.reduce, vgrid_of<N>(c) + sgrid_of<N>(c));
+    if (c->sc) {
+        if (c->has_boundary) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_surface<N, true, true>), g, b, 0, c->stream, a);
+        else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_surface<N, false, true>), g, b, 0, c->stream, a);
+    } else {
+        if (c->has_boundary) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_surface<N, true, false>), g, b, 0, c->stream, a);
+        else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_surface<N, false, false>), g, b, 0, c->stream, a);
+    }
+    if (a.reduce != RED_NONE) launch_finalize_t<N>(c, a.reduce, vgrid(c) + sgrid_of<N>(c));
 }
 void launch_surface(tlw_ctx* c, const KArgs& a) {
     if (c->N == 4) launch_surface_t<4>(c, a);
@@ -279,6 +328,7 @@ int rhs_common(tlw_ctx* c, int kind, const double* u, double* du, double t, doub
     int rc = check_launch(c);
     if (rc) return rc;
     c->aux_valid = (c->flags & TLW_FLAG_KEEP_AUX) != 0;
+    c->alpha_valid = c->sc;
     return TLW_OK;
 }
 
@@ -302,6 +352,7 @@ int tlw_create(const tlw_mesh_desc* mesh, const tlw_basis_desc* basis, const tlw
     tlw_ctx* c = new tlw_ctx;
     c->N = basis->polydeg;
     c->n = c->N + 1;
+    c->ebv = default_eb(c->N);
     c->time_disc = solver->time_discretization;
     c->flags = solver->flags;
     DevConst& h = c->hconst;
@@ -320,6 +371,19 @@ int tlw_create(const tlw_mesh_desc* mesh, const tlw_basis_desc* basis, const tlw
     for (int k = 0; k < 24; ++k) h.bc_par[k] = solver->boundary_params[k];
     h.bc_kind = solver->boundary_kind;
     h.flux_order = solver->surface_flux_order;
+    c->sc = solver->shock_capturing != 0;
+    if (c->sc) {
+        if (!solver->inverse_vandermonde_host || solver->surface_flux_order != TLW_FLUX_TREE ||
+            !(solver->alpha_max >= 0.0) || !(solver->alpha_min >= 0.0)) {
+            delete c;
+            return TLW_EINVAL;
+        }
+        for (int i = 0; i < n; ++i)
+            for (int k = 0; k < n; ++k) h.Vinv[i][k] = solver->inverse_vandermonde_host[i + n * k];
+        h.alpha_max = solver->alpha_max;
+        h.alpha_min = solver->alpha_min;
+        h.alpha_smooth = solver->alpha_smooth;
+    }
     // every domain-boundary face needs a boundary value function
     bool has_boundary = false;
     for (size_t k = 0; k < (size_t)4 * std::max(mesh->n_elements, 0) && mesh->neighbors_host; ++k)
@@ -500,6 +564,7 @@ int tlw_advance(tlw_ctx* c, int32_t max_steps, double* t_host, double* dt_host, 
     if (c->timing) collect_times(c);
     const DevState& s = *c->h_state;
     c->host_cur = s.cur;
+    if (max_steps > 0) c->alpha_valid = c->sc;
     if (t_host) *t_host = s.t;
     if (dt_host) *dt_host = s.dt;
     if (iter_host) *iter_host = s.iter;
@@ -529,16 +594,91 @@ int tlw_kernel_times(tlw_ctx* c, double* vol_ms, int64_t* vol_n, double* surf_ms
     return TLW_OK;
 }
 
+int tlw_set_elements_per_workgroup(tlw_ctx* c, int32_t elements) {
+    if (!c) return TLW_EINVAL;
+    if (!eb_supported(c->N, elements))
+        return fail(c, TLW_EINVAL, c->N == 4 ? "elements per workgroup must be 5, 10 or 15 for polydeg 4"
+                                             : "elements per workgroup must be 4, 8 or 16 for polydeg 3");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->ebv = elements;
+    return TLW_OK;
+}
+
+int tlw_get_elements_per_workgroup(const tlw_ctx* c, int32_t* elements) {
+    if (!c || !elements) return TLW_EINVAL;
+    *elements = c->ebv;
+    return TLW_OK;
+}
+
+int tlw_autotune(tlw_ctx* c, int32_t reps, int32_t* chosen, double* ms_out) {
+    if (!c) return TLW_EINVAL;
+    if (reps < 1) return fail(c, TLW_EINVAL, "reps >= 1 required");
+    // one rhs evaluation of the context's scheme (LW, or MDRK stage 2, the
+    // heavier stage) on the current state buffer, into the other buffer
+    const int kind = c->time_disc == TLW_LW ? VOL_LW : VOL_MDRK2;
+    const double* u = c->d_buf[c->host_cur];
+    double* scratch = c->d_buf[1 - c->host_cur];
+    if (kind == VOL_MDRK2)
+        HIPCHK(c, hipMemcpyAsync(c->d_us, u, sizeof(double) * state_len(c), hipMemcpyDeviceToDevice, c->stream));
+    hipEvent_t e0, e1;
+    HIPCHK(c, hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return fail(c, TLW_EHIP, "hipEventCreate failed");
+    }
+    const int saved = c->ebv;
+    const double dt = 1e-6;  // any finite step: the work does not depend on it
+    int best = saved;
+    double best_ms = 0.0;
+    int rc = TLW_OK;
+    for (int k = 0; k < 3 && rc == TLW_OK; ++k) {
+        c->ebv = eb_choices(c->N)[k];
+        rc = rhs_common(c, kind, u, scratch, 0.0, dt);  // warm-up
+        if (rc) break;
+        if (hipEventRecord(e0, c->stream) != hipSuccess) { rc = fail(c, TLW_EHIP, "hipEventRecord"); break; }
+        for (int r = 0; r < reps && rc == TLW_OK; ++r) rc = rhs_common(c, kind, u, scratch, 0.0, dt);
+        if (rc) break;
+        float ms = 0.f;
+        if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+            rc = fail(c, TLW_EHIP, "event timing failed");
+            break;
+        }
+        const double per = ms / reps;
+        if (ms_out) ms_out[k] = per;
+        if (k == 0 || per < best_ms) {
+            best_ms = per;
+            best = c->ebv;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->ebv = rc == TLW_OK ? best : saved;
+    c->aux_valid = false;
+    c->alpha_valid = false;
+    if (chosen) *chosen = c->ebv;
+    return rc;
+}
+
 int tlw_get_aux(tlw_ctx* c, int32_t which, double* out, size_t len) {
     if (!c || !out) return fail(c, TLW_EINVAL, "null pointer");
-    if (!(c->flags & TLW_FLAG_KEEP_AUX)) return fail(c, TLW_EINVAL, "context created without TLW_FLAG_KEEP_AUX");
-    if (!c->aux_valid) return fail(c, TLW_EINVAL, "no rhs evaluated yet");
     const size_t nd = state_len(c);
     const double* src = nullptr;
     size_t need = 0;
-    if (which == TLW_AUX_U) { src = c->d_auxU; need = nd; }
-    else if (which == TLW_AUX_F) { src = c->d_auxF; need = 2 * nd; }
-    else return fail(c, TLW_EINVAL, "unknown aux selector");
+    if (which == TLW_AUX_ALPHA) {
+        if (!c->sc) return fail(c, TLW_EINVAL, "context created without shock capturing");
+        if (!c->alpha_valid) return fail(c, TLW_EINVAL, "no rhs evaluated or step taken yet");
+        src = c->d_alpha;
+        need = (size_t)c->E;
+    } else if (which == TLW_AUX_U || which == TLW_AUX_F) {
+        if (!(c->flags & TLW_FLAG_KEEP_AUX))
+            return fail(c, TLW_EINVAL, "context created without TLW_FLAG_KEEP_AUX");
+        if (!c->aux_valid) return fail(c, TLW_EINVAL, "no rhs evaluated yet");
+        src = which == TLW_AUX_U ? c->d_auxU : c->d_auxF;
+        need = which == TLW_AUX_U ? nd : 2 * nd;
+    } else {
+        return fail(c, TLW_EINVAL, "unknown aux selector");
+    }
     if (len != need) return fail(c, TLW_EINVAL, "aux length mismatch");
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipMemcpy(out, src, sizeof(double) * len, hipMemcpyDeviceToHost));

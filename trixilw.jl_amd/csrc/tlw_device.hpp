@@ -32,6 +32,9 @@
 namespace tlw {
 
 constexpr int kTraceQ = 10;  // face record: jump state (4), normal flux (4), |v_n|, c
+// shock-capturing face record: + fn_low (4), blending factor, padding
+constexpr int kTraceQSC = 16;
+template <bool SC> constexpr int kRQ = SC ? kTraceQSC : kTraceQ;
 constexpr int kSurfBlock = 128;
 
 // Volume workgroup geometry: EB elements, NT threads (a multiple of 64).
@@ -60,6 +63,10 @@ struct DevConst {
     double bc_par[24];
     int bc_kind;
     int flux_order;
+    // shock capturing (IndicatorHennemannGassner on density_pressure)
+    double Vinv[8][8];  // inverse normalised-Legendre Vandermonde, row-major
+    double alpha_max, alpha_min;
+    int alpha_smooth;
 };
 
 // Device-resident integrator state (LWIntegrator fields t, dt, iter and the
@@ -96,6 +103,10 @@ struct KArgs {
     double* partial;   // per-workgroup max of J⁻¹(max λ1 + max λ2): volume part, then surface part
     double* elem_max;  // [E][2]: max λ1, λ2 over the nodes the volume kernel finalised
     int vol_grid;      // number of volume workgroups (offset of the surface partials)
+    int ebv;           // elements per volume workgroup
+    const double* alpha_tmp;  // indicator values before smoothing [E]
+    double* alpha_out;        // smoothed blending factors [E] (optional)
+    double invJ0;             // interface inverse Jacobian of the TreeMesh cache (first element)
     double t_arg, dt_arg;
     int use_state_dt;  // 1: t, dt from DevState (device loop); 0: from t_arg/dt_arg
     int step_mode;     // 1: out = u + dt * rhs; 0: out = rhs
@@ -233,19 +244,19 @@ __device__ __forceinline__ void store4(double* p, const double* u) {
     p2[1] = make_double2(u[2], u[3]);
 }
 
-__device__ __forceinline__ void load_record(const double* p, double* r) {
+template <int RQ = kTraceQ> __device__ __forceinline__ void load_record(const double* p, double* r) {
     const double2* p2 = reinterpret_cast<const double2*>(p);
 #pragma unroll
-    for (int k = 0; k < 5; ++k) {
+    for (int k = 0; k < RQ / 2; ++k) {
         const double2 x = p2[k];
         r[2 * k] = x.x;
         r[2 * k + 1] = x.y;
     }
 }
-__device__ __forceinline__ void store_record(double* p, const double* r) {
+template <int RQ = kTraceQ> __device__ __forceinline__ void store_record(double* p, const double* r) {
     double2* p2 = reinterpret_cast<double2*>(p);
 #pragma unroll
-    for (int k = 0; k < 5; ++k) p2[k] = make_double2(r[2 * k], r[2 * k + 1]);
+    for (int k = 0; k < RQ / 2; ++k) p2[k] = make_double2(r[2 * k], r[2 * k + 1]);
 }
 
 __device__ __forceinline__ const double* sel_in(const KArgs& a, int cur) {
@@ -263,6 +274,51 @@ __device__ __forceinline__ void face_numflux(const double* L, const double* R, d
     const double lam = dmax(L[8], R[8]) + dmax(L[9], R[9]);
 #pragma unroll
     for (int v = 0; v < 4; ++v) Fs[v] = 0.5 * (L[4 + v] + R[4 + v]) + (-0.5 * lam) * (R[v] - L[v]);
+}
+
+// Trixi flux_lax_friedrichs (Rusanov with max_abs_speed_naive) of two states.
+__device__ __forceinline__ void rusanov(const double* ul, const double* ur, double gamma, int orientation, double* out) {
+    const double gm1 = gamma - 1.0;
+    double fl[4], fr[4], al, cl, ar, cr;
+    euler_flux_dir(ul, gm1, orientation, fl);
+    euler_flux_dir(ur, gm1, orientation, fr);
+    speed_parts(ul, gamma, orientation, al, cl);
+    speed_parts(ur, gamma, orientation, ar, cr);
+    const double lam = dmax(al, ar) + dmax(cl, cr);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) out[v] = 0.5 * (fl[v] + fr[v]) + (-0.5 * lam) * (ur[v] - ul[v]);
+}
+
+__device__ __forceinline__ bool admissible(const double* u, double gamma) {
+    const double rho = u[0];
+    const double v1 = u[1] / rho, v2 = u[2] / rho;
+    const double p = (gamma - 1.0) * (u[3] - 0.5 * (u[1] * v1 + u[2] * v2));
+    return rho > 0.0 && p > 0.0;
+}
+
+// Shock-capturing interface flux (dg_2d.jl:1879-1946): the LW flux blended
+// with the first-order Rusanov flux of uⁿ by the average blending factor of
+// the two elements, falling back to the first-order flux when the low-order
+// update of an adjacent node with the blended flux is not admissible.
+// Records: [0..3] uⁿ, [4..7] F̃n, [8] |v_n|(Ũ), [9] c(Ũ), [10..13] fn_low, [14] α.
+__device__ __forceinline__ void face_numflux_sc(const double* L, const double* R, int orientation, double gamma,
+                                                double J, double dt, double w_n, double w_1, double* Fs) {
+    double Fh[4], fn[4], Fb[4], low[4];
+    face_numflux(L, R, Fh);
+    rusanov(L, R, gamma, orientation, fn);
+    double alp = 0.5 * (L[14] + R[14]);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) Fb[v] = (1.0 - alp) * Fh[v] + alp * fn[v];
+    const double cl = dt * J / (w_n * 0.5), cr = dt * J / (w_1 * 0.5);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) low[v] = L[v] - cl * (Fb[v] - L[10 + v]);
+    bool ok = admissible(low, gamma);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) low[v] = R[v] - cr * (R[10 + v] - Fb[v]);
+    ok = ok && admissible(low, gamma);
+    if (!ok) alp = 1.0;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) Fs[v] = alp * fn[v] + (1.0 - alp) * Fh[v];
 }
 
 // Face record of a face node: jump state, normal component of F̃ and the λ
@@ -654,8 +710,13 @@ __device__ __forceinline__ void mdrk2_element(const double* uN, const double* us
 // record for the surface kernel.  In step mode the CFL reduction covers the
 // nodes finalised here: whole elements go to the workgroup partial, elements
 // with an outside face leave their per-node maxima in elem_max.
-template <int N, int EB, int KIND>
-__global__ __launch_bounds__((VGeo<N, EB>::NT), 3) void k_volume(KArgs a) {
+// Waves per SIMD the register budget is sized for: 3 (168 VGPRs) for the
+// plain scheme; 2 (256 VGPRs) with shock capturing, whose subcell FV and
+// blended face fluxes would otherwise spill.
+template <bool SC> constexpr int kVolWaves = SC ? 2 : 3;
+
+template <int N, int EB, int KIND, bool SC>
+__global__ __launch_bounds__((VGeo<N, EB>::NT), kVolWaves<SC>) void k_volume(KArgs a) {
     using G_ = VGeo<N, EB>;
     constexpr int n = G_::n, NN = G_::NN, S = G_::S, NT = G_::NT;
     __shared__ double lds[G_::LDS];
@@ -708,43 +769,104 @@ __global__ __launch_bounds__((VGeo<N, EB>::NT), 3) void k_volume(KArgs a) {
         store4(a.auxF + ((size_t)e * NN + node) * 8 + 4, G);
     }
 
+    const int fx = (i == 0) ? 0 : ((i == n - 1) ? 1 : -1);
+    const int fy = (j == 0) ? 2 : ((j == n - 1) ? 3 : -1);
+    constexpr int RQ = kRQ<SC>;
+    double fnlx[4] = {0, 0, 0, 0}, fnly[4] = {0, 0, 0, 0};
+    double alpha_e = 0.0;
+    if constexpr (SC) {
+        // blending factor, smoothed over the face neighbours (Trixi apply_smoothing!)
+        alpha_e = a.alpha_tmp[e];
+        if (c->alpha_smooth) {
+#pragma unroll
+            for (int f = 0; f < 4; ++f) {
+                const int nb = a.nbr[4 * e + f];
+                if (nb >= 0) alpha_e = dmax(alpha_e, 0.5 * a.alpha_tmp[nb]);
+            }
+        }
+        if (act && node == 0 && a.alpha_out) a.alpha_out[e] = alpha_e;
+        // first-order subcell FV (fv_kernel!, dg_2d.jl:251-282; Trixi calcflux_fv!):
+        // Rusanov fluxes between neighbouring nodes, zero at the element faces
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+            if (act) s1[v * S + sidx] = uN[v];
+        __syncthreads();
+        double fxl[4] = {0, 0, 0, 0}, gyl[4] = {0, 0, 0, 0}, w[4];
+        if (i > 0) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) w[v] = s1[v * S + sidx - 1];
+            rusanov(w, uN, gamma, 1, fxl);
+        }
+        if (j > 0) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) w[v] = s1[v * S + sidx - n];
+            rusanov(w, uN, gamma, 2, gyl);
+        }
+        if (act) put8<S>(s0, sidx, fxl, gyl);
+        __syncthreads();
+        const double iwi = 1.0 / c->weights[i], iwj = 1.0 / c->weights[j];
+        const bool blended = !(fabs(alpha_e) <= 1e-12);  // pure_and_blended_element_ids!
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const double fxr = (i < n - 1) ? s0[v * S + sidx + 1] : 0.0;
+            const double gyr = (j < n - 1) ? s0[(4 + v) * S + sidx + n] : 0.0;
+            const double fv = iwi * (fxr - fxl[v]) + iwj * (gyr - gyl[v]);
+            vol[v] = (1.0 - alpha_e) * vol[v] + (blended ? alpha_e * fv : 0.0);
+            // fn_low: the subcell fluxes next to the element faces (load_fn_low!)
+            fnlx[v] = (fx == 0) ? s0[v * S + sidx + 1] : fxl[v];
+            fnly[v] = (fy == 2) ? s0[(4 + v) * S + sidx + n] : gyl[v];
+        }
+        __syncthreads();  // s0 and s1 are reused for the face records
+    }
     // --- face records of this node (s0 and s1 are free: records live there).
     // A node lies on at most one x face (fx) and one y face (fy).
-    double* rec = lds;  // [EB][4][n][kTraceQ]
+    double* rec = lds;  // [EB][4][n][RQ]
     const bool tree = c->flux_order == 0;
     const double* jump = tree ? uN : Uc;
     const double* lamst = tree ? Uc : uN;
-    const int fx = (i == 0) ? 0 : ((i == n - 1) ? 1 : -1);
-    const int fy = (j == 0) ? 2 : ((j == n - 1) ? 3 : -1);
+    auto fill = [&](double* r, const double* fnl) {
+        if constexpr (SC) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) r[10 + v] = fnl[v];
+            r[14] = alpha_e;
+            r[15] = 0.0;
+        }
+    };
     if (act && fx >= 0) {
-        double r[kTraceQ];
+        double r[RQ];
         make_record(r, jump, F, lamst, gamma, 1);
-        store_record(rec + ((lec * 4 + fx) * n + j) * kTraceQ, r);
+        fill(r, fnlx);
+        store_record<RQ>(rec + ((lec * 4 + fx) * n + j) * RQ, r);
     }
     if (act && fy >= 0) {
-        double r[kTraceQ];
+        double r[RQ];
         make_record(r, jump, G, lamst, gamma, 2);
-        store_record(rec + ((lec * 4 + fy) * n + i) * kTraceQ, r);
+        fill(r, fnly);
+        store_record<RQ>(rec + ((lec * 4 + fy) * n + i) * RQ, r);
     }
     __syncthreads();
     // --- interfaces inside the workgroup; records of outside faces to HBM
     double surf[4] = {0.0, 0.0, 0.0, 0.0};
     bool fin_node = true;
     auto do_face = [&](int f, int l) {
-        double own[kTraceQ];
-        load_record(rec + ((lec * 4 + f) * n + l) * kTraceQ, own);
+        double own[RQ];
+        load_record<RQ>(rec + ((lec * 4 + f) * n + l) * RQ, own);
         const int ne = a.nbr[4 * e + f];
         if (ne >= e0 && ne < e0 + nel) {
-            double oth[kTraceQ], Fs[4];
-            load_record(rec + (((ne - e0) * 4 + (f ^ 1)) * n + l) * kTraceQ, oth);
-            if (f & 1) face_numflux(own, oth, Fs);  // +x, +y face: own side is left
-            else face_numflux(oth, own, Fs);
+            double oth[RQ], Fs[4];
+            load_record<RQ>(rec + (((ne - e0) * 4 + (f ^ 1)) * n + l) * RQ, oth);
+            const double* Lr = (f & 1) ? own : oth;  // +x, +y face: own side is left
+            const double* Rr = (f & 1) ? oth : own;
+            if constexpr (SC)
+                face_numflux_sc(Lr, Rr, f < 2 ? 1 : 2, gamma, a.invJ0, dt, c->weights[n - 1], c->weights[0], Fs);
+            else
+                face_numflux(Lr, Rr, Fs);
             const double wf = (f & 1) ? c->bnd_hi : -c->bnd_lo;
 #pragma unroll
             for (int v = 0; v < 4; ++v) surf[v] = fma(Fs[v], wf, surf[v]);
         } else {
             fin_node = false;
-            if (act) store_record(a.trace + (((size_t)e * 4 + f) * n + l) * kTraceQ, own);
+            if (act) store_record<RQ>(a.trace + (((size_t)e * 4 + f) * n + l) * RQ, own);
         }
     };
     if (fx >= 0) do_face(fx, j);
@@ -835,11 +957,11 @@ __device__ inline void boundary_record(const KArgs& a, int e, int face, int l, d
     make_record(r, tree ? uo : Uo, Fo, tree ? Uo : uo, c->gamma, orientation);
 }
 
-// Surface kernel: the faces that leave a volume workgroup (EBV elements).
+// Surface kernel: the faces that leave a volume workgroup (a.ebv elements).
 // Each interface is evaluated by both neighbours from identical inputs, so
 // both see the same bits.  Elements with such faces finish their CFL
 // reduction here, combining elem_max with the corrected face nodes.
-template <int N, int EBV, bool BC>
+template <int N, bool BC, bool SC>
 __global__ __launch_bounds__(kSurfBlock) void k_surface(KArgs a) {
     using G_ = SGeo<N>;
     constexpr int n = G_::n, NN = G_::NN, EB = G_::EB, S = G_::S;
@@ -860,6 +982,7 @@ __global__ __launch_bounds__(kSurfBlock) void k_surface(KArgs a) {
     const double t = a.use_state_dt ? st->t : a.t_arg;
     const DevConst* c = a.c;
     const double invJ = a.inv_jac[e];
+    const int EBV = a.ebv;
     const int eblk = e / EBV;
     const int fx = (i == 0) ? 0 : ((i == n - 1) ? 1 : -1);
     const int fy = (j == 0) ? 2 : ((j == n - 1) ? 3 : -1);
@@ -879,13 +1002,25 @@ __global__ __launch_bounds__(kSurfBlock) void k_surface(KArgs a) {
         const size_t off = ((size_t)e * NN + node) * 4;
         load4(out + off, val);
         double surf[4] = {0.0, 0.0, 0.0, 0.0};
+        constexpr int RQ = kRQ<SC>;
         auto do_face = [&](int f, int l, int nb) {
-            double own[kTraceQ], oth[kTraceQ], Fs[4];
-            load_record(a.trace + (((size_t)e * 4 + f) * n + l) * kTraceQ, own);
-            if (!BC || nb >= 0) load_record(a.trace + (((size_t)nb * 4 + (f ^ 1)) * n + l) * kTraceQ, oth);
-            else boundary_record<N>(a, e, f, l, t, dt, oth);
-            if (f & 1) face_numflux(own, oth, Fs);
-            else face_numflux(oth, own, Fs);
+            double own[RQ], oth[RQ], Fs[4];
+            load_record<RQ>(a.trace + (((size_t)e * 4 + f) * n + l) * RQ, own);
+            const double* Lr = (f & 1) ? own : oth;
+            const double* Rr = (f & 1) ? oth : own;
+            if (!BC || nb >= 0) {
+                load_record<RQ>(a.trace + (((size_t)nb * 4 + (f ^ 1)) * n + l) * RQ, oth);
+                if constexpr (SC)
+                    face_numflux_sc(Lr, Rr, f < 2 ? 1 : 2, c->gamma, a.invJ0, dt, c->weights[n - 1], c->weights[0],
+                                    Fs);
+                else
+                    face_numflux(Lr, Rr, Fs);
+            } else {
+                // domain boundary: time-averaged Dirichlet state, no blending
+                // (calc_boundary_flux!, dg_2d.jl:1379-1454)
+                boundary_record<N>(a, e, f, l, t, dt, oth);
+                face_numflux(Lr, Rr, Fs);
+            }
             const double wf = (f & 1) ? c->bnd_hi : -c->bnd_lo;
 #pragma unroll
             for (int v = 0; v < 4; ++v) surf[v] = fma(Fs[v], wf, surf[v]);
@@ -970,6 +1105,66 @@ __global__ __launch_bounds__(kSurfBlock) void k_max_speed(KArgs a) {
         double bm = 0.0;
         for (int k = 0; k < nel; ++k) bm = dmax(bm, s_m[k]);
         a.partial[blockIdx.x] = bm;
+    }
+}
+
+// Hennemann-Gassner indicator (Trixi IndicatorHennemannGassner, Hennemann et
+// al., JCP 426, 2021) on density_pressure, before smoothing: modal energy of
+// the highest and second-highest modes, logistic blending with threshold
+// 0.5·10^(-1.8 n^0.25) and sharpness ln((1-1e-4)/1e-4), clipped below
+// alpha_min to 0, above 1-alpha_min to 1, capped at alpha_max.
+template <int N>
+__global__ __launch_bounds__(kSurfBlock) void k_indicator(KArgs a, double* alpha_tmp) {
+    using G_ = SGeo<N>;
+    constexpr int n = G_::n, NN = G_::NN, EB = G_::EB, S = G_::S;
+    __shared__ double s_a[S], s_b[S], s_e[3][S];
+    const int tid = threadIdx.x;
+    const int le = tid / NN, node = tid - (tid / NN) * NN;
+    const int i = node % n, j = node / n;
+    const int e0 = blockIdx.x * EB;
+    const int nel = min(EB, a.E - e0);
+    const bool act = le < nel;
+    const int lec = le < nel ? le : nel - 1;
+    const int e = e0 + lec;
+    const DevConst* c = a.c;
+    const double* uin = sel_in(a, a.use_state_dt ? a.st->cur : 0);
+    if (a.use_state_dt && a.st->finished) return;
+    double un[4];
+    load4(uin + ((size_t)e * NN + node) * 4, un);
+    const double rp = (c->gamma - 1.0) * (un[0] * un[3] - 0.5 * (un[1] * un[1] + un[2] * un[2]));
+    if (act) s_a[lec * NN + node] = rp;
+    __syncthreads();
+    double acc = 0.0;  // x transform: tmp[j][i] = Σ_ii Vinv[i][ii] ind[j][ii]
+#pragma unroll
+    for (int k = 0; k < n; ++k) acc = fma(c->Vinv[i][k], s_a[lec * NN + j * n + k], acc);
+    if (act) s_b[lec * NN + node] = acc;
+    __syncthreads();
+    double m = 0.0;  // y transform: modal[j][i] = Σ_jj Vinv[j][jj] tmp[jj][i]
+#pragma unroll
+    for (int k = 0; k < n; ++k) m = fma(c->Vinv[j][k], s_b[lec * NN + k * n + i], m);
+    const double m2 = m * m;
+    if (act) {
+        s_e[0][lec * NN + node] = m2;
+        s_e[1][lec * NN + node] = (i < n - 1 && j < n - 1) ? m2 : 0.0;
+        s_e[2][lec * NN + node] = (i < n - 2 && j < n - 2) ? m2 : 0.0;
+    }
+    __syncthreads();
+    if (act && node == 0) {
+        double tot = 0.0, c1 = 0.0, c2 = 0.0;
+        for (int k = 0; k < NN; ++k) {
+            tot += s_e[0][lec * NN + k];
+            c1 += s_e[1][lec * NN + k];
+            c2 += s_e[2][lec * NN + k];
+        }
+        const double f1 = tot != 0.0 ? (tot - c1) / tot : 0.0;
+        const double f2 = c1 != 0.0 ? (c1 - c2) / c1 : 0.0;
+        const double energy = dmax(f1, f2);
+        const double threshold = 0.5 * pow(10.0, -1.8 * pow((double)n, 0.25));
+        const double par_s = log((1.0 - 0.0001) / 0.0001);
+        double al = 1.0 / (1.0 + exp(-par_s / threshold * (energy - threshold)));
+        if (al < c->alpha_min) al = 0.0;
+        if (al > 1.0 - c->alpha_min) al = 1.0;
+        alpha_tmp[e] = al < c->alpha_max ? al : c->alpha_max;
     }
 }
 

@@ -11,9 +11,11 @@ from .equations import (CompressibleEulerEquations2D, ConstantState, FourQuadran
 from .integrator import (LWIntegrator, LWSolution, compute_dt, isapprox, isfinished, perform_step, set_dt,
                          set_t_and_iter, solve_lwfr)
 from .mesh import TreeMesh
-from .semidiscretization import (DGSEM, LW, MDRK, CFLBased, LWUpdate, SemidiscretizationHyperbolic,
-                                 VolumeIntegralFR, flux_lax_friedrichs, get_time_discretization, lw_fourier_cfl, rhs,
-                                 rhs_mdrk1, rhs_mdrk2, semidiscretize, trixi2lw)
+from .semidiscretization import (DGSEM, LW, MDRK, CFLBased, FirstOrderReconstruction, IndicatorHennemannGassner,
+                                 LWUpdate, MUSCLHancockReconstruction, SemidiscretizationHyperbolic, VolumeIntegralFR,
+                                 VolumeIntegralFRShockCapturing, density_pressure, flux_lax_friedrichs,
+                                 get_time_discretization, lw_fourier_cfl, rhs, rhs_mdrk1, rhs_mdrk2, semidiscretize,
+                                 trixi2lw)
 
 __all__ = [
     "LobattoLegendreBasis", "CompressibleEulerEquations2D", "ConstantState", "FourQuadrantRiemann",
@@ -21,4 +23,6 @@ __all__ = [
     "isfinished", "perform_step", "set_dt", "set_t_and_iter", "solve_lwfr", "TreeMesh", "DGSEM", "LW", "MDRK",
     "CFLBased", "LWUpdate", "SemidiscretizationHyperbolic", "VolumeIntegralFR", "flux_lax_friedrichs",
     "get_time_discretization", "lw_fourier_cfl", "rhs", "rhs_mdrk1", "rhs_mdrk2", "semidiscretize", "trixi2lw",
+    "FirstOrderReconstruction", "IndicatorHennemannGassner", "MUSCLHancockReconstruction",
+    "VolumeIntegralFRShockCapturing", "density_pressure",
 ]

@@ -14,14 +14,15 @@ TLW_OK, TLW_EINVAL, TLW_EHIP, TLW_EADMISSIBLE, TLW_ENOTSUP = 0, 1, 2, 3, 4
 TLW_LW, TLW_MDRK = 0, 1
 TLW_FLUX_TREE, TLW_FLUX_STRUCTURED = 0, 1
 TLW_BV_NONE, TLW_BV_VORTEX, TLW_BV_CONSTANT, TLW_BV_RIEMANN4 = 0, 1, 2, 3
-TLW_AUX_U, TLW_AUX_F = 0, 1
+TLW_AUX_U, TLW_AUX_F, TLW_AUX_ALPHA = 0, 1, 2
 TLW_FLAG_KEEP_AUX = 1
 
 EXPORTED = (
     "tlw_create", "tlw_rebuild_mesh", "tlw_destroy", "tlw_last_error", "tlw_set_stream", "tlw_state_len",
     "tlw_rhs_lw", "tlw_rhs_mdrk1", "tlw_rhs_mdrk2", "tlw_us_ptr", "tlw_set_us", "tlw_max_dt", "tlw_set_state",
     "tlw_get_state", "tlw_state_ptr", "tlw_solve_init", "tlw_advance", "tlw_get_aux",
-    "tlw_enable_kernel_timing", "tlw_kernel_times",
+    "tlw_enable_kernel_timing", "tlw_kernel_times", "tlw_set_elements_per_workgroup",
+    "tlw_get_elements_per_workgroup", "tlw_autotune",
 )
 
 
@@ -53,6 +54,11 @@ class SolverDesc(C.Structure):
         ("boundary_kind", C.c_int32),
         ("boundary_params", C.c_double * 24),
         ("flags", C.c_int32),
+        ("shock_capturing", C.c_int32),
+        ("alpha_max", C.c_double),
+        ("alpha_min", C.c_double),
+        ("alpha_smooth", C.c_int32),
+        ("inverse_vandermonde_host", C.POINTER(C.c_double)),
     ]
 
 
@@ -120,6 +126,12 @@ def lib():
     L.tlw_kernel_times.argtypes = [vp, dp, C.POINTER(C.c_int64), dp, C.POINTER(C.c_int64)]
     L.tlw_get_aux.restype = C.c_int
     L.tlw_get_aux.argtypes = [vp, i32, dp, sz]
+    L.tlw_set_elements_per_workgroup.restype = C.c_int
+    L.tlw_set_elements_per_workgroup.argtypes = [vp, i32]
+    L.tlw_get_elements_per_workgroup.restype = C.c_int
+    L.tlw_get_elements_per_workgroup.argtypes = [vp, C.POINTER(i32)]
+    L.tlw_autotune.restype = C.c_int
+    L.tlw_autotune.argtypes = [vp, i32, C.POINTER(i32), dp]
     _lib = L
     return L
 

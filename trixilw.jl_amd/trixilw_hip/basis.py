@@ -87,6 +87,29 @@ def calc_dhat(nodes, weights):
     return Dh
 
 
+def legendre_normalized(m: int, x):
+    """sqrt(m + 1/2) L_m(x) (Trixi legendre_polynomial_and_derivative)."""
+    x = np.asarray(x, dtype=np.longdouble)
+    Lm2, Lm1 = np.ones_like(x), x
+    if m == 0:
+        L = np.ones_like(x)
+    elif m == 1:
+        L = x
+    else:
+        for k in range(2, m + 1):
+            L = ((2 * k - 1) * x * Lm1 - (k - 1) * Lm2) / k
+            Lm2, Lm1 = Lm1, L
+    return L * np.sqrt(np.longdouble(m) + np.longdouble(0.5))
+
+
+def vandermonde_legendre(nodes):
+    n = len(nodes)
+    V = np.zeros((n, n), dtype=np.longdouble)
+    for m in range(n):
+        V[:, m] = legendre_normalized(m, np.asarray(nodes, dtype=np.longdouble))
+    return V.astype(np.float64), np.linalg.inv(V.astype(np.float64))
+
+
 class LobattoLegendreBasis:
     def __init__(self, polydeg: int):
         self.polydeg = int(polydeg)
@@ -99,6 +122,7 @@ class LobattoLegendreBasis:
         self.boundary_interpolation = np.zeros((n, 2))
         self.boundary_interpolation[0, 0] = 1.0 / self.weights[0]
         self.boundary_interpolation[n - 1, 1] = 1.0 / self.weights[n - 1]
+        self.vandermonde_legendre, self.inverse_vandermonde_legendre = vandermonde_legendre(self.nodes)
 
     def nnodes(self) -> int:
         return self.polydeg + 1

@@ -45,6 +45,47 @@ class VolumeIntegralFR:
 
 
 flux_lax_friedrichs = "flux_lax_friedrichs"
+density_pressure = "density_pressure"
+
+
+class FirstOrderReconstruction:
+    """Subcell FV reconstruction tag (src/solvers/fr.jl:58)."""
+
+
+class MUSCLHancockReconstruction:
+    """Not implemented on the device: the reference has no TreeMesh method of calcflux_mh!."""
+
+
+@dataclass
+class IndicatorHennemannGassner:
+    """IndicatorHennemannGassner(equations, basis; alpha_max, alpha_min, alpha_smooth, variable)."""
+
+    equations: object = None
+    basis: object = None
+    alpha_max: float = 0.5
+    alpha_min: float = 0.001
+    alpha_smooth: bool = True
+    variable: str = density_pressure
+
+
+class VolumeIntegralFRShockCapturing:
+    """VolumeIntegralFRShockCapturing(indicator; volume_integralFR, volume_flux_fv, reconstruction)
+    (src/solvers/fr.jl:63-86)."""
+
+    def __init__(self, indicator, volume_integralFR=None, volume_flux_fv=flux_lax_friedrichs,
+                 reconstruction=None):
+        self.indicator = indicator
+        self.volume_integralFR = volume_integralFR if volume_integralFR is not None else VolumeIntegralFR(LW())
+        self.volume_flux_fv = volume_flux_fv
+        self.reconstruction = reconstruction if reconstruction is not None else FirstOrderReconstruction()
+        if not isinstance(self.reconstruction, FirstOrderReconstruction):
+            raise NotImplementedError("only FirstOrderReconstruction is implemented on the device")
+        if volume_flux_fv != flux_lax_friedrichs or indicator.variable != density_pressure:
+            raise NotImplementedError("flux_lax_friedrichs subcell fluxes and density_pressure only")
+
+    @property
+    def time_discretization(self):
+        return self.volume_integralFR.time_discretization
 
 
 class DGSEM:
@@ -133,6 +174,14 @@ class SemidiscretizationHyperbolic:
             for k, v in enumerate(boundary_conditions.params):
                 sd.boundary_params[k] = float(v)
         sd.flags = _lib.TLW_FLAG_KEEP_AUX if keep_aux else 0
+        vi = solver.volume_integral
+        if isinstance(vi, VolumeIntegralFRShockCapturing):
+            ind = vi.indicator
+            self._vinv = np.asfortranarray(b.inverse_vandermonde_legendre).ravel(order="F").copy()
+            sd.shock_capturing = 1
+            sd.alpha_max, sd.alpha_min = ind.alpha_max, ind.alpha_min
+            sd.alpha_smooth = int(bool(ind.alpha_smooth))
+            sd.inverse_vandermonde_host = _dptr(self._vinv)
         self._keep = self._mesh_arrays(mesh)
         md = self._mesh_desc()
         h = C.c_void_p()
@@ -227,6 +276,30 @@ class SemidiscretizationHyperbolic:
         self._bind_stream()
         _lib.check(_lib.lib().tlw_set_us(self._h, C.c_void_p(us_ode.data_ptr())), self._h)
 
+    def elements_per_workgroup(self, elements=None):
+        """Tile of the fused volume kernel: returns it, or sets it when given."""
+        if elements is not None:
+            _lib.check(_lib.lib().tlw_set_elements_per_workgroup(self._h, int(elements)), self._h)
+        out = C.c_int32()
+        _lib.check(_lib.lib().tlw_get_elements_per_workgroup(self._h, C.byref(out)), self._h)
+        return out.value
+
+    def autotune(self, u0, reps=5):
+        """Loads the state ``u0`` (host array or device tensor) into the
+        context, times each compiled tile and keeps the fastest.  Returns
+        (chosen tile, {tile: ms per rhs evaluation})."""
+        u = u0.detach().cpu().numpy() if hasattr(u0, "detach") else np.asarray(u0)
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        if u.size != self.ndofs() * 4:
+            raise ValueError(f"state must have {self.ndofs() * 4} entries")
+        self._bind_stream()
+        _lib.check(_lib.lib().tlw_set_state(self._h, _dptr(u), u.size), self._h)
+        chosen = C.c_int32()
+        ms = (C.c_double * 3)()
+        _lib.check(_lib.lib().tlw_autotune(self._h, int(reps), C.byref(chosen), ms), self._h)
+        tiles = (5, 10, 15) if self.n == 5 else (4, 8, 16)
+        return chosen.value, dict(zip(tiles, list(ms)))
+
     def get_aux(self, which):
         n = self.n
         E = self.mesh.n_elements
@@ -236,6 +309,9 @@ class SemidiscretizationHyperbolic:
         elif which == "F":
             out = np.zeros((E, n, n, 2, 4))
             code = _lib.TLW_AUX_F
+        elif which == "alpha":
+            out = np.zeros(E)
+            code = _lib.TLW_AUX_ALPHA
         else:
             raise ValueError(which)
         _lib.check(_lib.lib().tlw_get_aux(self._h, code, _dptr(out), out.size), self._h)
