@@ -97,6 +97,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--tile", type=int, default=0, help="elements per volume workgroup (0: autotune)")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward step ratio")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device copy (the reference kernel for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -207,6 +208,35 @@ def main():
         "surface_avg_launch_ms": round(sms.value / max(sn.value, 1), 4),
     }
 
+    adjoint = None
+    if td == "lw" and not sc and not args.no_adjoint:
+        # adjoint/forward step ratio: one adjoint step (tlw_adjoint_lw: face records + reverse
+        # sweep) against one forward step of the loop above, same state and dt
+        _lib.check(L.tlw_get_state(h, u0.ctypes.data_as(dp), u0.size), h)  # the state after the timed steps
+        u = torch.from_numpy(u0).to("cuda")
+        lam_a = torch.randn_like(u)
+        lam_b = torch.empty_like(u)
+        dt_a = cfl * semi.max_dt(u)
+        for _ in range(2):
+            tlw.adjoint_step(lam_b, u, lam_a, semi, 0.0, dt_a)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        reps = max(3, min(args.steps, 10))
+        e0.record()
+        for r in range(reps):
+            if r % 2 == 0:
+                tlw.adjoint_step(lam_b, u, lam_a, semi, 0.0, dt_a)
+            else:
+                tlw.adjoint_step(lam_a, u, lam_b, semi, 0.0, dt_a)
+        e1.record()
+        torch.cuda.synchronize()
+        adj_ms = e0.elapsed_time(e1) / reps
+        adjoint = {"step_ms": round(adj_ms, 4), "forward_step_ms": round(ms_per_step, 4),
+                   "ratio": round(adj_ms / ms_per_step, 3), "reps": reps,
+                   "note": "one discrete-adjoint LW step (k_adj_records + k_adjoint) on the same mesh and state"}
+        log(f"bench: adjoint step {adj_ms:.3f} ms = {adj_ms / ms_per_step:.2f} x forward step")
+        del u, lam_a, lam_b
+
     cpu = None
     if not args.no_cpu_baseline:
         import oracle_ctypes
@@ -252,6 +282,7 @@ def main():
                    "cfl": cfl, "elements_per_workgroup": tile, "tile_autotune_ms": tune},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "adjoint": adjoint,
     }
     print(json.dumps(out), flush=True)
     semi.close()

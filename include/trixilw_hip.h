@@ -155,6 +155,24 @@ int tlw_enable_kernel_timing(tlw_ctx* ctx, int32_t enable);
 int tlw_kernel_times(tlw_ctx* ctx, double* volume_ms, int64_t* volume_launches, double* surface_ms,
                      int64_t* surface_launches);
 
+/* Discrete adjoint of one LW step u' = u + dt rhs!(u) (plain LW, periodic mesh, TreeMesh
+ * flux order): lam = lam_next + dt (∂rhs/∂u)ᵀ lam_next at the state u.  No counterpart in the
+ * reference, whose gradients go through ForwardDiff; it differentiates the same rhs!
+ * (dg_2d.jl:28-83 with lw_volume_kernel_4!/3!, 674-1267, and the TreeMesh Rusanov flux,
+ * 1903-1946), with dt frozen.  The max in λ passes its derivative to the side the forward
+ * max picked.  Uses the context's trace buffer as scratch; lam must not alias u or lam_next. */
+int tlw_adjoint_lw(tlw_ctx* ctx, const double* u_dev, const double* lam_next_dev, double* lam_dev, double t,
+                   double dt);
+
+/* PositivityPreservingLimiterZhangShu(thresholds = (threshold_density, threshold_pressure),
+ * variables = (density, pressure)) as a stage limiter of solve_lwfr
+ * (src/callbacks/callbacks_stage/positivity_zhang_shu.jl:10-41, positivity_zhang_shu_dg2d.jl:8-45;
+ * applied after each update, src/solvers/lwfr.jl:226, 251, 257).  tlw_set_positivity_limiter
+ * turns it on for tlw_advance (the next step's dt is then taken from the limited state);
+ * tlw_apply_positivity_limiter is limiter!(u_ode, integrator, semi, t) on a device state. */
+int tlw_set_positivity_limiter(tlw_ctx* ctx, int32_t enable, double threshold_density, double threshold_pressure);
+int tlw_apply_positivity_limiter(tlw_ctx* ctx, double* u_dev);
+
 /* Tile of the fused volume kernel (no counterpart in the reference, whose @threaded loop
  * has no tile): elements per workgroup, one lane per GLL node.  polydeg 4: 5, 10 or 15
  * (default 15); polydeg 3: 4, 8 or 16 (default 16).  Results do not depend on the tile.

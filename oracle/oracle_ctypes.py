@@ -43,6 +43,8 @@ class _Desc(C.Structure):
         ("alpha_max", C.c_double),
         ("alpha_min", C.c_double),
         ("alpha_smooth", C.c_int),
+        ("positivity", C.c_int),
+        ("pos_threshold", C.c_double * 2),
     ]
 
 
@@ -83,6 +85,10 @@ def lib():
         L.tlwo_get_aux.argtypes = [C.c_void_p, C.c_int, dp]
         L.tlwo_euler_fluxes.argtypes = [dp, C.c_double, dp, dp]
         L.tlwo_set_alpha.argtypes = [C.c_void_p, dp]
+        L.tlwo_positivity_limiter.argtypes = [C.c_void_p, dp]
+        L.tlwo_solve_fixed_dt.argtypes = [C.c_void_p, dp, C.c_double, dp, C.c_int]
+        L.tlwo_tangent_lw.restype = C.c_int
+        L.tlwo_tangent_lw.argtypes = [C.POINTER(_Desc), dp, dp, dp, C.c_int, dp, dp]
         L.tlwo_inverse_vandermonde.argtypes = [C.c_void_p, dp]
         L.tlwo_max_abs_speed_naive.restype = C.c_double
         L.tlwo_max_abs_speed_naive.argtypes = [dp, dp, C.c_int, C.c_double]
@@ -106,7 +112,9 @@ class Oracle:
 
     def __init__(self, polydeg, nx, ny, xmin, ymin, h, periodic=(True, True), equation="euler", gamma=1.4,
                  adv=(1.0, 1.0), flux_order=0, bc_kind=BV_NONE, bc_par=(), tolerances=(0.0, 0.0), nthreads=0,
-                 shock_capturing=False, alpha_max=0.5, alpha_min=0.001, alpha_smooth=True):
+                 shock_capturing=False, alpha_max=0.5, alpha_min=0.001, alpha_smooth=True, positivity=None):
+        """``positivity``: (threshold_density, threshold_pressure) of a Zhang-Shu
+        limiter that ``solve`` applies as a stage limiter, or None."""
         d = _Desc()
         d.polydeg, d.nx, d.ny = polydeg, nx, ny
         d.xmin, d.ymin, d.h = xmin, ymin, h
@@ -123,6 +131,10 @@ class Oracle:
         d.shock_capturing = int(shock_capturing)
         d.alpha_max, d.alpha_min = alpha_max, alpha_min
         d.alpha_smooth = int(alpha_smooth)
+        if positivity is not None:
+            d.positivity = 1
+            d.pos_threshold[0], d.pos_threshold[1] = positivity
+        self._desc = d
         self._h = lib().tlwo_create(C.byref(d))
         if not self._h:
             raise ValueError("oracle supports polydeg 3 and 4")
@@ -177,10 +189,36 @@ class Oracle:
             a = np.ascontiguousarray(alpha, dtype=np.float64)
             lib().tlwo_set_alpha(self._h, _ptr(a))
 
+    def positivity_limiter(self, u):
+        """my_limiter_zhang_shu! on (density, pressure) with the thresholds given
+        as ``positivity`` at construction; returns the limited copy."""
+        out = np.array(u, dtype=np.float64, copy=True, order="C")
+        lib().tlwo_positivity_limiter(self._h, _ptr(out))
+        return out
+
     def inverse_vandermonde(self):
         out = np.zeros((self.n, self.n))
         lib().tlwo_inverse_vandermonde(self._h, _ptr(out))
         return out
+
+    def solve_fixed_dt(self, u0, dts, t0=0.0):
+        """The LW loop with the given dt per step; returns the final state."""
+        u = np.array(u0, dtype=np.float64, copy=True, order="C")
+        d = np.ascontiguousarray(dts, dtype=np.float64)
+        lib().tlwo_solve_fixed_dt(self._h, _ptr(u), t0, _ptr(d), len(d))
+        return u
+
+    def tangent_lw(self, u0, du0, dts):
+        """Forward-mode (dual-number) tangent of the fixed-dt LW loop:
+        returns (u_K, du_K) for the seed du0."""
+        u0 = np.ascontiguousarray(u0, dtype=np.float64)
+        du0 = np.ascontiguousarray(du0, dtype=np.float64)
+        d = np.ascontiguousarray(dts, dtype=np.float64)
+        u, du = np.zeros_like(u0), np.zeros_like(u0)
+        rc = lib().tlwo_tangent_lw(C.byref(self._desc), _ptr(u0), _ptr(du0), _ptr(d), len(d), _ptr(u), _ptr(du))
+        if rc:
+            raise ValueError("tangent needs polydeg 3 or 4")
+        return u, du
 
     def set_us(self, us):
         us = np.ascontiguousarray(us, dtype=np.float64)

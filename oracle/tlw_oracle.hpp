@@ -38,6 +38,41 @@ inline double value(double x) { return x; }
 // Julia Base.max for non-NaN floats: y if y > x, else x.
 template <class T> inline T max_(const T& x, const T& y) { return (value(y) > value(x)) ? y : x; }
 
+// Forward-mode dual number (value, one directional derivative), the role
+// ForwardDiff.Dual plays for the reference: Solver<Dual, Euler2D<Dual>> runs
+// the same code and carries d/dε of every quantity along a seed direction.
+struct Dual {
+    double v = 0.0, d = 0.0;
+    Dual() = default;
+    Dual(double x) : v(x), d(0.0) {}
+    Dual(double x, double dx) : v(x), d(dx) {}
+};
+inline double value(const Dual& x) { return x.v; }
+inline Dual operator+(const Dual& a, const Dual& b) { return {a.v + b.v, a.d + b.d}; }
+inline Dual operator-(const Dual& a, const Dual& b) { return {a.v - b.v, a.d - b.d}; }
+inline Dual operator-(const Dual& a) { return {-a.v, -a.d}; }
+inline Dual operator*(const Dual& a, const Dual& b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+inline Dual operator/(const Dual& a, const Dual& b) {
+    const double q = a.v / b.v;
+    return {q, (a.d - q * b.d) / b.v};
+}
+inline Dual operator+(const Dual& a, double b) { return {a.v + b, a.d}; }
+inline Dual operator+(double a, const Dual& b) { return {a + b.v, b.d}; }
+inline Dual operator-(const Dual& a, double b) { return {a.v - b, a.d}; }
+inline Dual operator-(double a, const Dual& b) { return {a - b.v, -b.d}; }
+inline Dual operator*(const Dual& a, double b) { return {a.v * b, a.d * b}; }
+inline Dual operator*(double a, const Dual& b) { return {a * b.v, a * b.d}; }
+inline Dual operator/(const Dual& a, double b) { return {a.v / b, a.d / b}; }
+inline Dual operator/(double a, const Dual& b) {
+    const double q = a / b.v;
+    return {q, -q * b.d / b.v};
+}
+inline Dual sqrt_(const Dual& x) {
+    const double r = std::sqrt(x.v);
+    return {r, x.d / (2.0 * r)};
+}
+inline Dual abs_(const Dual& x) { return x.v < 0.0 ? -x : x; }
+
 // ---------------------------------------------------------------------------
 // Gauss-Lobatto-Legendre basis.  Restates Trixi's LobattoLegendreBasis
 // (gauss_lobatto_nodes_weights, barycentric_weights,
@@ -213,6 +248,9 @@ template <class T> struct Euler2D {
         l1 = abs_(v1) + c;
         l2 = abs_(v2) + c;
     }
+    // Trixi density and pressure, the variables of the positivity limiter
+    T density(const T* u) const { return u[0]; }
+    T pressure(const T* u) const { return (gamma - 1.0) * (u[3] - 0.5 * (u[1] * u[1] + u[2] * u[2]) / u[0]); }
     // Trixi density_pressure: ρ p = (γ-1)(ρ ρe - ½(ρv1² + ρv2²)), the
     // indicator variable of the shock-capturing configuration
     T density_pressure(const T* u) const {
@@ -243,6 +281,8 @@ template <class T> struct LinAdv2D {
     }
     void max_abs_speeds(const T*, T& l1, T& l2) const { l1 = T(std::fabs(a1)); l2 = T(std::fabs(a2)); }
     T density_pressure(const T* u) const { return u[0]; }
+    T density(const T* u) const { return u[0]; }
+    T pressure(const T* u) const { return u[0]; }
     bool is_admissible(const T*) const { return true; }
 };
 
@@ -366,6 +406,10 @@ template <class T, class Eq> struct Solver {
     bool alpha_smooth = true;
     std::vector<double> alpha, alpha_tmp, alpha_override;
     double sc_dt = 0.0;  // dt passed to compute_alp
+    // PositivityPreservingLimiterZhangShu(thresholds, variables = (density, pressure))
+    // passed to solve_lwfr as a stage limiter (lwfr.jl:131-137, 226, 251, 257)
+    bool positivity = false;
+    double pos_threshold[2] = {5.0e-6, 5.0e-6};
     std::vector<T> fn_low;  // [v, l, face, e]
     std::vector<T> U, F, sfv, temporal_errors;
     std::vector<T> us, U2, F2;  // MDRK stage cache (containers.jl:83-98)
@@ -1147,6 +1191,40 @@ template <class T, class Eq> struct Solver {
         surface_integral_and_jacobian(du);
     }
 
+    // my_limiter_zhang_shu! on a conforming TreeMesh (no mortars)
+    // (src/callbacks/callbacks_stage/positivity_zhang_shu_dg2d.jl:8-45): where
+    // the minimum of the variable over an element's nodes is below the
+    // threshold, every node is pulled toward the element mean,
+    // u <- θ u + (1-θ) ū with θ = (var(ū) - threshold) / (var(ū) - min var).
+    // The mean is Σ u w_i w_j / 4 in the reference's loop order.
+    template <class Var> void zhang_shu_pass(T* u, double threshold, Var var) {
+        const auto& w = basis.weights;
+#pragma omp parallel for schedule(static)
+        for (int e = 0; e < mesh.E; ++e) {
+            double vmin = std::numeric_limits<double>::infinity();
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < n; ++i) vmin = std::min(vmin, value(var(&u[idx(0, i, j, e)])));
+            if (!(vmin < threshold)) continue;
+            T mean[NV];
+            for (int v = 0; v < NV; ++v) mean[v] = T(0);
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < n; ++i)
+                    for (int v = 0; v < NV; ++v) mean[v] = mean[v] + u[idx(v, i, j, e)] * w[i] * w[j];
+            for (int v = 0; v < NV; ++v) mean[v] = mean[v] / 4.0;
+            const double vmean = value(var(mean));
+            const double theta = (vmean - threshold) / (vmean - vmin);
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < n; ++i)
+                    for (int v = 0; v < NV; ++v)
+                        u[idx(v, i, j, e)] = theta * u[idx(v, i, j, e)] + (1.0 - theta) * mean[v];
+        }
+    }
+    // thresholds and variables are processed in order (positivity_zhang_shu.jl:26-41)
+    void positivity_limiter(T* u) {
+        zhang_shu_pass(u, pos_threshold[0], [this](const T* q) { return eq.density(q); });
+        zhang_shu_pass(u, pos_threshold[1], [this](const T* q) { return eq.pressure(q); });
+    }
+
     // Trixi max_dt(u, t, mesh::TreeMesh{2}, constant_speed::False, ...):
     // 2 / (nnodes * max_e J⁻¹ (max λ1 + max λ2)); called by compute_dt
     // (src/solvers/lwfr.jl:176-190).
@@ -1186,6 +1264,7 @@ int solve(Solver<T, Eq>& s, T* u, double t0, double tf, double cfl, int max_step
     std::vector<T> du(nd), uprev(nd);
     double t = t0;
     int it = 0;
+    if (s.positivity) s.positivity_limiter(u);  // apply_limiters! before the loop (lwfr.jl:251)
     while (!julia_isapprox(t, tf) && it < max_steps) {
         double dt = cfl * s.max_dt(u);
         if (t + dt > tf) dt = tf - t;  // set_dt!
@@ -1203,12 +1282,37 @@ int solve(Solver<T, Eq>& s, T* u, double t0, double tf, double cfl, int max_step
 #pragma omp parallel for schedule(static)
             for (long long k = 0; k < nd; ++k) u[k] = uprev[k] + dt * du[k];
         }
+        // perform_step! applies the limiters after the update (lwfr.jl:226) and the loop
+        // again after the callbacks (lwfr.jl:257); the second pass leaves a limited
+        // state unchanged.  The MDRK step applies them to uⁿ between the stages
+        // (mdrk.jl:19), which was already limited.
+        if (s.positivity) s.positivity_limiter(u);
         if (dts) dts[it] = dt;
         t += dt;  // set_t_and_iter!
         ++it;
     }
     if (t_out) *t_out = t;
     return it;
+}
+
+// The LW loop of solve_lwfr with a prescribed dt per step, as the gradient
+// checks use it: the CFL dts of a reference run are frozen, so the map
+// u0 -> u_K is differentiable (compute_dt's max is not).
+template <class T, class Eq> void solve_fixed_dt(Solver<T, Eq>& s, T* u, double t0, const double* dts, int nsteps) {
+    const long long nd = (long long)s.ndofs_total();
+    std::vector<T> du(nd), uprev(nd);
+    double t = t0;
+    if (s.positivity) s.positivity_limiter(u);
+    for (int k = 0; k < nsteps; ++k) {
+        const double dt = dts[k];
+#pragma omp parallel for schedule(static)
+        for (long long q = 0; q < nd; ++q) uprev[q] = u[q];
+        s.rhs_lw(du.data(), u, t, dt);
+#pragma omp parallel for schedule(static)
+        for (long long q = 0; q < nd; ++q) u[q] = uprev[q] + dt * du[q];
+        if (s.positivity) s.positivity_limiter(u);
+        t += dt;
+    }
 }
 
 }  // namespace tlwo

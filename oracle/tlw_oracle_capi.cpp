@@ -27,6 +27,8 @@ struct tlwo_desc {
     int shock_capturing;
     double alpha_max, alpha_min;
     int alpha_smooth;
+    int positivity;              // Zhang-Shu limiter on (density, pressure) in solve
+    double pos_threshold[2];
 };
 
 struct tlwo_ctx {
@@ -47,6 +49,9 @@ template <class S> static void configure(S& s, const tlwo_desc* d) {
     s.alpha_max = d->alpha_max;
     s.alpha_min = d->alpha_min;
     s.alpha_smooth = d->alpha_smooth != 0;
+    s.positivity = d->positivity != 0;
+    s.pos_threshold[0] = d->pos_threshold[0];
+    s.pos_threshold[1] = d->pos_threshold[1];
     s.setup();
 }
 
@@ -185,6 +190,12 @@ int tlwo_solve(void* ctx, double* u, double t0, double tf, double cfl, int max_s
 }
 
 // Prescribed blending factors (len E) used instead of the indicator; NULL clears.
+void tlwo_positivity_limiter(void* ctx, double* u) {
+    auto* c = static_cast<tlwo_ctx*>(ctx);
+    if (c->euler) c->euler->positivity_limiter(u);
+    else c->linadv->positivity_limiter(u);
+}
+
 void tlwo_set_alpha(void* ctx, const double* alpha) {
     DISPATCH(ctx, {
         if (alpha) s.alpha_override.assign(alpha, alpha + s.mesh.E);
@@ -213,4 +224,30 @@ double tlwo_max_abs_speed_naive(const double* ul, const double* ur, int orientat
     return eq.max_abs_speed_naive(ul, ur, orientation);
 }
 
+
+void tlwo_solve_fixed_dt(void* ctx, double* u, double t0, const double* dts, int nsteps) {
+    auto* c = static_cast<tlwo_ctx*>(ctx);
+    if (c->euler) solve_fixed_dt(*c->euler, u, t0, dts, nsteps);
+    else solve_fixed_dt(*c->linadv, u, t0, dts, nsteps);
+}
+
+// Tangent of the fixed-dt LW loop: u0 + ε du0 -> u_K + ε du_K with dual numbers
+// (the forward-mode gradient check; Euler only).
+int tlwo_tangent_lw(const tlwo_desc* d, const double* u0, const double* du0, const double* dts, int nsteps,
+                    double* u_out, double* du_out) {
+    if (d->polydeg != 3 && d->polydeg != 4) return 1;
+    if (d->nthreads > 0) omp_set_num_threads(d->nthreads);
+    Solver<Dual, Euler2D<Dual>> s;
+    s.eq.gamma = d->gamma;
+    configure(s, d);
+    const size_t nd = s.ndofs_total();
+    std::vector<Dual> u(nd);
+    for (size_t k = 0; k < nd; ++k) u[k] = Dual(u0[k], du0[k]);
+    solve_fixed_dt(s, u.data(), 0.0, dts, nsteps);
+    for (size_t k = 0; k < nd; ++k) {
+        u_out[k] = u[k].v;
+        du_out[k] = u[k].d;
+    }
+    return 0;
+}
 }  // extern "C"

@@ -239,3 +239,28 @@ def test_nonadmissible_state_is_reported():
     u[3, 2, 2, 3] = -1.0  # negative energy -> negative pressure
     with pytest.raises(NonAdmissibleState):
         semi.max_dt(u)
+
+
+@pytest.mark.parametrize("N", [3, 4])
+def test_every_tile_matches_oracle(oracle_mod, N):
+    """Each compiled volume tile (elements per workgroup) gives the oracle's
+    residual, and the autotuner picks one of them."""
+    import trixilw_hip as tlw
+
+    torch = torch_mod()
+    semi, o, u_lex, mesh = setup(oracle_mod, N, 4, ordering="morton", seed=31 + N)
+    dt = 0.5 * o.max_dt(u_lex)
+    du_ref = o.rhs(u_lex, 0.0, dt)
+    u = to_dev(mesh, u_lex)
+    du = torch.empty_like(u)
+    semi.cache["dt"][0] = dt
+    tiles = (5, 10, 15) if N == 4 else (4, 8, 16)
+    for tile in tiles:
+        semi.elements_per_workgroup(tile)
+        tlw.rhs(du, u, semi, 0.0)
+        torch.cuda.synchronize()
+        assert_close(to_lex(mesh, du), du_ref, what=f"tile {tile}")
+    chosen, ms = semi.autotune(u, reps=1)
+    assert chosen in tiles and set(ms) == set(tiles) and all(v > 0 for v in ms.values())
+    with pytest.raises(tlw._lib.TLWError):
+        semi.elements_per_workgroup(7)

@@ -207,3 +207,25 @@ def test_solve_truncates_last_step(oracle_mod):
     u, t, dts = o.solve(IsentropicVortex()(x, y, 0.0), 0.0, 0.37, 0.5)
     assert t == pytest.approx(0.37, rel=0, abs=1e-15)
     assert dts[-1] <= dts[-2]
+
+
+# ---------------------------------------------------------------- tangent (dual numbers)
+def test_dual_tangent_matches_central_differences(oracle_mod):
+    """The dual-number tangent of the fixed-dt LW loop (the forward-mode check
+    of the device adjoint) against central differences, and its value part
+    against the plain double loop."""
+    from trixilw_hip.equations import IsentropicVortex
+
+    o = make(oracle_mod, 4, 8)
+    x, y = o.node_coords()
+    u0 = IsentropicVortex()(x, y, 0.0)
+    rng = np.random.default_rng(12)
+    v = np.zeros_like(u0)
+    v[..., 0] = 1e-2 * rng.standard_normal(x.shape)
+    v[..., 3] = 1e-2 * rng.standard_normal(x.shape)
+    dts = [0.3 * o.max_dt(u0)] * 6
+    uK, duK = o.tangent_lw(u0, v, dts)
+    assert np.max(np.abs(uK - o.solve_fixed_dt(u0, dts))) <= 1e-14 * np.max(np.abs(uK))
+    h = 1e-6
+    fd = (o.solve_fixed_dt(u0 + h * v, dts) - o.solve_fixed_dt(u0 - h * v, dts)) / (2 * h)
+    assert rel_linf(duK, fd) < 1e-7

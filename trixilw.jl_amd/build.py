@@ -40,10 +40,10 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
-def build(verbose: bool = False) -> Path:
+def build(verbose: bool = False, out: Path = OUT, defines=()) -> Path:
     srcs = [str(CSRC / "tlw_capi.hip")]
-    tmp = OUT.with_suffix(".so.tmp")
-    cmd = [hipcc(), *HIPCC_FLAGS, "-o", str(tmp), *srcs]
+    tmp = Path(out).with_suffix(".so.tmp")
+    cmd = [hipcc(), *HIPCC_FLAGS, *[f"-D{d}" for d in defines], "-o", str(tmp), *srcs]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -55,9 +55,15 @@ def build(verbose: bool = False) -> Path:
         lines = [ln for ln in r.stderr.splitlines() if "packed-fp32-ops" not in ln]
         if lines:
             sys.stderr.write("\n".join(lines) + "\n")
-    os.replace(tmp, OUT)
-    return OUT
+    os.replace(tmp, out)
+    return Path(out)
 
 
 if __name__ == "__main__":
-    print(build(verbose=True))
+    import argparse
+
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default=str(OUT))
+    ap.add_argument("-D", dest="defines", action="append", default=[], help="extra preprocessor definition")
+    a = ap.parse_args()
+    print(build(verbose=True, out=Path(a.out), defines=a.defines))

@@ -1,0 +1,368 @@
+// tlw_adjoint.hpp -- discrete adjoint of one LW step on a periodic TreeMesh
+// (plain FR, TreeMesh flux order):  u' = u + dt R(u)  =>  λ = λ' + dt (∂R/∂u)ᵀ λ'.
+//
+// R is the reference's LW residual (rhs!, src/solvers/dgsem_tree/dg_2d.jl:28-83):
+//   R = -J⁻¹ [ D̂·(F̃, G̃) + lifted Rusanov fluxes F* ],
+//   F* = ½(F̃n_L + F̃n_R) − ½ λ(Ũ_L, Ũ_R) (u_R − u_L),  λ = max|v_n| + max c,
+// with Ũ, F̃ from the approximate LW procedure (lw_volume_kernel_4!/3!,
+// dg_2d.jl:674-1267).  The adjoint reverses that chain by hand: the transposed
+// D̂ and D contractions through LDS, the transpose of the Euler flux Jacobian at
+// each Taylor state (17 states at N=4, 11 at N=3), and the interface flux with
+// the derivative of the max going to the side the forward max picked.
+// Two kernels per step: k_adj_records recomputes the forward element
+// computation and writes every face node's (u, |v_n|(Ũ), c(Ũ), s̄) with
+// s̄ = -J⁻¹ dt λ'; k_adjoint recomputes it again, keeping the time
+// derivatives, and runs the reverse sweep.
+#pragma once
+
+#include "tlw_device.hpp"
+
+namespace tlw {
+
+constexpr int kAdjQ = 10;  // adjoint face record: u 4, |v_n| 1, c 1, s̄ 4
+
+// Taylor states of the FD flux derivatives: w = u + Σ_d a[kind][d] u^(d+1) for
+// kind = u+2, u+1, u-1, u-2 (the incremental updates of lw4_element/lw3_element)
+struct LWTaylor {
+    static constexpr double a[4][4] = {{2.0, 2.0, 4.0 / 3.0, 2.0 / 3.0},
+                                       {1.0, 0.5, 1.0 / 6.0, 1.0 / 24.0},
+                                       {-1.0, 0.5, -1.0 / 6.0, 1.0 / 24.0},
+                                       {-2.0, 2.0, -4.0 / 3.0, 2.0 / 3.0}};
+    // Ũ = u + Σ_d w[d] u^(d+1);  F̃ = F0 + Σ_k w[k] dx_k
+    static constexpr double w[4] = {0.5, 1.0 / 6.0, 1.0 / 24.0, 1.0 / 120.0};
+};
+// FD stencil of stage k: dx_k = Σ_kind c[k][kind] f(w_kind) + c[k][4] F0
+template <int N> struct LWStencil;
+template <> struct LWStencil<4> {
+    static constexpr double c[4][5] = {{-1.0 / 12.0, 8.0 / 12.0, -8.0 / 12.0, 1.0 / 12.0, 0.0},
+                                       {-1.0 / 12.0, 16.0 / 12.0, 16.0 / 12.0, -1.0 / 12.0, -30.0 / 12.0},
+                                       {0.5, -1.0, 1.0, -0.5, 0.0},
+                                       {0.5, -2.0, -2.0, 0.5, 3.0}};
+};
+template <> struct LWStencil<3> {
+    static constexpr double c[3][5] = {{-1.0 / 12.0, 8.0 / 12.0, -8.0 / 12.0, 1.0 / 12.0, 0.0},
+                                       {0.0, 1.0, 1.0, 0.0, -2.0},
+                                       {0.5, -1.0, 1.0, -0.5, 0.0}};
+};
+
+// wb = (∂f/∂w)ᵀ fb + (∂g/∂w)ᵀ gb for the Euler fluxes f, g of euler_fluxes.
+__device__ __forceinline__ void euler_fluxes_vjp(const double* w, double gm1, const double* fb, const double* gb,
+                                                 double* wb) {
+    const double m1 = w[1], m2 = w[2], E = w[3];
+    const double ir = rcp_nr(w[0]);
+    const double v1 = m1 * ir, v2 = m2 * ir;
+    const double p = gm1 * (E - 0.5 * (m1 * v1 + m2 * v2));
+    const double T = E + p;
+    // f = (m1, m1 v1 + p, m1 v2, T v1), g = (m2, m2 v1, m2 v2 + p, T v2)
+    double m1b = fb[0] + fb[1] * v1 + fb[2] * v2;
+    double m2b = gb[0] + gb[1] * v1 + gb[2] * v2;
+    const double Tb = fb[3] * v1 + gb[3] * v2;
+    double v1b = fb[1] * m1 + gb[1] * m2 + fb[3] * T;
+    double v2b = fb[2] * m1 + gb[2] * m2 + gb[3] * T;
+    const double pb = fb[1] + gb[2] + Tb;
+    // T = E + p, p = gm1 (E - ½ (m1 v1 + m2 v2))
+    const double Eb = Tb + gm1 * pb;
+    const double q = -0.5 * gm1 * pb;
+    m1b = fma(q, v1, m1b);
+    v1b = fma(q, m1, v1b);
+    m2b = fma(q, v2, m2b);
+    v2b = fma(q, m2, v2b);
+    // v = m / ρ
+    m1b = fma(v1b, ir, m1b);
+    m2b = fma(v2b, ir, m2b);
+    const double irb = v1b * m1 + v2b * m2;
+    wb[0] = -irb * ir * ir;
+    wb[1] = m1b;
+    wb[2] = m2b;
+    wb[3] = Eb;
+}
+
+// Gradients of |v_n| and of c = sqrt(γ p / ρ) with respect to the state.
+__device__ __forceinline__ void speed_grads(const double* s, double gamma, int orientation, double* da, double* dc) {
+    const double ir = rcp_nr(s[0]);
+    const double v1 = s[1] * ir, v2 = s[2] * ir;
+    const double gm1 = gamma - 1.0;
+    const double p = gm1 * (s[3] - 0.5 * (s[1] * v1 + s[2] * v2));
+    const double vn = orientation == 1 ? v1 : v2;
+    const double sg = vn < 0.0 ? -1.0 : 1.0;
+    da[0] = -sg * vn * ir;
+    da[1] = orientation == 1 ? sg * ir : 0.0;
+    da[2] = orientation == 1 ? 0.0 : sg * ir;
+    da[3] = 0.0;
+    // c² = γ p/ρ,  ∂(p/ρ) = gm1/ρ (-E/ρ + |v|², -v1, -v2, 1)
+    const double cs = sqrt_gs(gamma * p * ir);
+    const double k = 0.5 * gamma * gm1 * ir / cs;
+    dc[0] = k * (v1 * v1 + v2 * v2 - s[3] * ir);
+    dc[1] = -k * v1;
+    dc[2] = -k * v2;
+    dc[3] = k;
+}
+
+// Transposed derivative contraction: with a[v] of every node of the
+// workgroup in s (stride S), Fo[v] = Σ_k M[k,i] a[v](k, j) and
+// Go[v] = Σ_k M[k,j] a[v](i, k) -- the adjoint of contract<n, S>.
+template <int n, int S>
+__device__ __forceinline__ void contract_t(const double* s, int base, int i, int j, const double* M, double* Fo,
+                                           double* Go) {
+    double Mi[n], Mj[n];
+#pragma unroll
+    for (int k = 0; k < n; ++k) {
+        Mi[k] = M[k * n + i];
+        Mj[k] = M[k * n + j];
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        double af = 0.0, ag = 0.0;
+#pragma unroll
+        for (int k = 0; k < n; ++k) {
+            af = fma(Mi[k], s[v * S + base + j * n + k], af);
+            ag = fma(Mj[k], s[v * S + base + k * n + i], ag);
+        }
+        Fo[v] = af;
+        Go[v] = ag;
+    }
+}
+
+template <int S> __device__ __forceinline__ void put4(double* s, int idx, const double* a) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) s[v * S + idx] = a[v];
+}
+
+template <int N> constexpr int kEBA = (N == 4) ? 5 : 8;  // elements per adjoint workgroup
+
+template <int N, int EB, bool KEEP>
+__device__ __forceinline__ void adj_forward(const double* uN, double dt, double invJ, double gm1, const DevConst* c,
+                                            int i, int j, bool act, int sidx, int sbase, double* lds, double* sDm,
+                                            double* sDh, double* Uc, double* tder) {
+    using G_ = VGeo<N, EB>;
+    constexpr int n = G_::n, NN = G_::NN, S = G_::S;
+    if (threadIdx.x < NN) {
+        sDm[threadIdx.x] = -dt * c->D[threadIdx.x / n][threadIdx.x % n];
+        sDh[threadIdx.x] = c->Dhat[threadIdx.x / n][threadIdx.x % n];
+    }
+    double F[4], G[4], vol[4];
+    if constexpr (N == 4)
+        lw4_element<N, EB, KEEP>(uN, dt, invJ, gm1, c, i, j, act, sidx, sbase, lds, lds + 8 * S, lds + 16 * S, sDm,
+                                 sDh, Uc, F, G, vol, tder);
+    else
+        lw3_element<N, EB, KEEP>(uN, dt, invJ, gm1, c, i, j, act, sidx, sbase, lds, lds + 8 * S, lds + 16 * S, sDm,
+                                 sDh, Uc, F, G, vol, tder);
+}
+
+// Face records of the adjoint: every face node's u, the λ ingredients of Ũ and
+// s̄ = -J⁻¹ dt λ' (a.u = state, a.us = λ').
+template <int N, int EB>
+__global__ __launch_bounds__((VGeo<N, EB>::NT), 2) void k_adj_records(KArgs a) {
+    using G_ = VGeo<N, EB>;
+    constexpr int n = G_::n, NN = G_::NN;
+    __shared__ double lds[G_::LDS];
+    __shared__ double sDm[NN], sDh[NN];
+    const int tid = threadIdx.x;
+    const int le = tid / NN, node = tid - (tid / NN) * NN;
+    const int i = node % n, j = node / n;
+    const int e0 = blockIdx.x * EB;
+    const int nel = min(EB, a.E - e0);
+    const bool act = le < nel;
+    const int lec = le < nel ? le : nel - 1;
+    const int e = e0 + lec;
+    const int sidx = lec * NN + node, sbase = lec * NN;
+    const DevConst* c = a.c;
+    const double dt = a.dt_arg, invJ = a.inv_jac[e];
+    const size_t off = ((size_t)e * NN + node) * 4;
+    double uN[4], Uc[4];
+    load4(a.u + off, uN);
+    adj_forward<N, EB, false>(uN, dt, invJ, c->gamma - 1.0, c, i, j, act, sidx, sbase, lds, sDm, sDh, Uc, nullptr);
+    const int fx = (i == 0) ? 0 : ((i == n - 1) ? 1 : -1);
+    const int fy = (j == 0) ? 2 : ((j == n - 1) ? 3 : -1);
+    if (!act || (fx < 0 && fy < 0)) return;
+    double lam[4], r[kAdjQ], av1, av2, cs;
+    load4(a.us + off, lam);
+    speed_parts2(Uc, c->gamma, av1, av2, cs);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        r[v] = uN[v];
+        r[6 + v] = (-invJ * dt) * lam[v];
+    }
+    r[5] = cs;
+    if (fx >= 0) {
+        r[4] = av1;
+        store_record<kAdjQ>(a.trace + (((size_t)e * 4 + fx) * n + j) * kAdjQ, r);
+    }
+    if (fy >= 0) {
+        r[4] = av2;
+        store_record<kAdjQ>(a.trace + (((size_t)e * 4 + fy) * n + i) * kAdjQ, r);
+    }
+}
+
+// One adjoint step: a.out = λ = λ' + dt (∂R/∂u)ᵀ λ'  (a.u = u, a.us = λ').
+template <int N, int EB>
+__global__ __launch_bounds__((VGeo<N, EB>::NT), 2) void k_adjoint(KArgs a) {
+    using G_ = VGeo<N, EB>;
+    constexpr int n = G_::n, NN = G_::NN, S = G_::S;
+    __shared__ double lds[G_::LDS];
+    __shared__ double sDm[NN], sDh[NN];
+    const int tid = threadIdx.x;
+    const int le = tid / NN, node = tid - (tid / NN) * NN;
+    const int i = node % n, j = node / n;
+    const int e0 = blockIdx.x * EB;
+    const int nel = min(EB, a.E - e0);
+    const bool act = le < nel;
+    const int lec = le < nel ? le : nel - 1;
+    const int e = e0 + lec;
+    const int sidx = lec * NN + node, sbase = lec * NN;
+    const DevConst* c = a.c;
+    const double gamma = c->gamma, gm1 = gamma - 1.0;
+    const double dt = a.dt_arg, invJ = a.inv_jac[e];
+    const size_t off = ((size_t)e * NN + node) * 4;
+    double uN[4], lam[4], Uc[4], ud[N][4];
+    load4(a.u + off, uN);
+    load4(a.us + off, lam);
+    adj_forward<N, EB, true>(uN, dt, invJ, gm1, c, i, j, act, sidx, sbase, lds, sDm, sDh, Uc, &ud[0][0]);
+
+    // --- interface fluxes of this node's faces
+    double sb[4], Fb[4] = {0, 0, 0, 0}, Gb[4] = {0, 0, 0, 0}, Ucb[4] = {0, 0, 0, 0}, ub[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int v = 0; v < 4; ++v) sb[v] = (-invJ * dt) * lam[v];
+    const int fx = (i == 0) ? 0 : ((i == n - 1) ? 1 : -1);
+    const int fy = (j == 0) ? 2 : ((j == n - 1) ? 3 : -1);
+    auto face = [&](int f, int l, double* Fnb) {
+        const int nb = a.nbr[4 * e + f];
+        double r[kAdjQ];
+        load_record<kAdjQ>(a.trace + (((size_t)nb * 4 + (f ^ 1)) * n + l) * kAdjQ, r);
+        const int orientation = f < 2 ? 1 : 2;
+        double av1, av2, cs;
+        speed_parts2(Uc, gamma, av1, av2, cs);
+        const double aown = orientation == 1 ? av1 : av2;
+        const bool own_left = (f & 1) != 0;
+        const double aL = own_left ? aown : r[4], aR = own_left ? r[4] : aown;
+        const double cL = own_left ? cs : r[5], cR = own_left ? r[5] : cs;
+        const double lamf = dmax(aL, aR) + dmax(cL, cR);
+        double Fsb[4], lamb = 0.0;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const double sL = own_left ? sb[v] : r[6 + v];
+            const double sR = own_left ? r[6 + v] : sb[v];
+            Fsb[v] = c->bnd_hi * sL - c->bnd_lo * sR;
+            const double jump = own_left ? r[v] - uN[v] : uN[v] - r[v];  // u_R - u_L
+            lamb = fma(-0.5 * Fsb[v], jump, lamb);
+            Fnb[v] = fma(0.5, Fsb[v], Fnb[v]);
+            ub[v] = fma(own_left ? 0.5 * lamf : -0.5 * lamf, Fsb[v], ub[v]);
+        }
+        // dmax(x, y) = y > x ? y : x picks the right side only when it is larger
+        const bool a_own = own_left ? !(aR > aL) : (aR > aL);
+        const bool c_own = own_left ? !(cR > cL) : (cR > cL);
+        if (a_own || c_own) {
+            double da[4], dc[4];
+            speed_grads(Uc, gamma, orientation, da, dc);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                if (a_own) Ucb[v] = fma(lamb, da[v], Ucb[v]);
+                if (c_own) Ucb[v] = fma(lamb, dc[v], Ucb[v]);
+            }
+        }
+    };
+    if (fx >= 0) face(fx, j, Fb);
+    if (fy >= 0) face(fy, i, Gb);
+
+    // --- volume term D̂·(F̃, G̃): (F̄, Ḡ) += D̂ᵀ s̄
+    double* ex0 = lds;          // exchange buffers, 4 doubles per node each
+    double* ex1 = lds + 4 * S;  // (the forward's last reads were of s2)
+    if (act) put4<S>(ex0, sidx, sb);
+    __syncthreads();
+    {
+        double cf[4], cg[4];
+        contract_t<n, S>(ex0, sbase, i, j, sDh, cf, cg);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            Fb[v] += cf[v];
+            Gb[v] += cg[v];
+        }
+    }
+    // --- reverse of the LW procedure
+    double F0b[4], G0b[4], udb[N][4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        F0b[v] = Fb[v];
+        G0b[v] = Gb[v];
+        ub[v] += Ucb[v];
+#pragma unroll
+        for (int d = 0; d < N; ++d) udb[d][v] = LWTaylor::w[d] * Ucb[v];
+    }
+    int buf = 1;
+#pragma unroll
+    for (int k = N; k >= 1; --k) {
+        double dxb[4], dyb[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            dxb[v] = LWTaylor::w[k - 1] * Fb[v];
+            dyb[v] = LWTaylor::w[k - 1] * Gb[v];
+        }
+        if (k < N) {  // u^(k+1) = J⁻¹ C(dx_k, dy_k) with C the (-dt D) contraction
+            double* ex = buf ? ex1 : ex0;
+            buf ^= 1;
+            double t4[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) t4[v] = invJ * udb[k][v];
+            if (act) put4<S>(ex, sidx, t4);
+            __syncthreads();
+            double cf[4], cg[4];
+            contract_t<n, S>(ex, sbase, i, j, sDm, cf, cg);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                dxb[v] += cf[v];
+                dyb[v] += cg[v];
+            }
+        }
+        const double f0 = LWStencil<N>::c[k - 1][4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            F0b[v] = fma(f0, dxb[v], F0b[v]);
+            G0b[v] = fma(f0, dyb[v], G0b[v]);
+        }
+#pragma unroll
+        for (int kind = 0; kind < 4; ++kind) {
+            const double cc = LWStencil<N>::c[k - 1][kind];
+            if (cc == 0.0) continue;
+            double w[4], fb[4], gb[4], wb[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                double x = uN[v];
+#pragma unroll
+                for (int d = 0; d < k; ++d) x = fma(LWTaylor::a[kind][d], ud[d][v], x);
+                w[v] = x;
+                fb[v] = cc * dxb[v];
+                gb[v] = cc * dyb[v];
+            }
+            euler_fluxes_vjp(w, gm1, fb, gb, wb);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                ub[v] += wb[v];
+#pragma unroll
+                for (int d = 0; d < k; ++d) udb[d][v] = fma(LWTaylor::a[kind][d], wb[v], udb[d][v]);
+            }
+        }
+    }
+    {  // ut = J⁻¹ C(F0, G0)
+        double* ex = buf ? ex1 : ex0;
+        double t4[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) t4[v] = invJ * udb[0][v];
+        if (act) put4<S>(ex, sidx, t4);
+        __syncthreads();
+        double cf[4], cg[4];
+        contract_t<n, S>(ex, sbase, i, j, sDm, cf, cg);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            F0b[v] += cf[v];
+            G0b[v] += cg[v];
+        }
+    }
+    double wb[4];
+    euler_fluxes_vjp(uN, gm1, F0b, G0b, wb);
+    double out[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) out[v] = lam[v] + (ub[v] + wb[v]);
+    if (act) store4(a.out + off, out);
+}
+
+}  // namespace tlw

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../include/trixilw_hip.h"
+#include "tlw_adjoint.hpp"
 #include "tlw_device.hpp"
 
 using namespace tlw;
@@ -43,6 +44,7 @@ struct tlw_ctx {
     double* d_alpha = nullptr;
     bool sc = false;
     int ebv = 0;  // elements per volume workgroup
+    bool positivity = false;  // Zhang-Shu limiter in the device loop
     double invJ0 = 0.0;
     bool has_boundary = false;
     int host_cur = 0;
@@ -51,10 +53,10 @@ struct tlw_ctx {
     // kernel timing (tlw_enable_kernel_timing)
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
-    std::vector<std::pair<int, int>> ev_pairs;  // (start index, kind 0 volume / 1 surface)
+    std::vector<std::pair<int, int>> ev_pairs;  // (start index, kind 0 volume / 1 surface / 2 limiter)
     size_t ev_used = 0;
-    double t_ms[2] = {0.0, 0.0};
-    int64_t t_n[2] = {0, 0};
+    double t_ms[3] = {0.0, 0.0, 0.0};
+    int64_t t_n[3] = {0, 0, 0};
     std::string err;
 };
 
@@ -244,6 +246,14 @@ template <int N> void launch_surface_t(tlw_ctx* c, const KArgs& a) {
 void launch_surface(tlw_ctx* c, const KArgs& a) {
     if (c->N == 4) launch_surface_t<4>(c, a);
     else launch_surface_t<3>(c, a);
+}
+template <int N> void launch_limiter_t(tlw_ctx* c, const KArgs& a) {
+    hipLaunchKernelGGL(k_limiter<N>, dim3(sgrid_of<N>(c)), dim3(kSurfBlock), 0, c->stream, a);
+    if (a.reduce != RED_NONE) launch_finalize_t<N>(c, a.reduce, sgrid_of<N>(c));
+}
+void launch_limiter(tlw_ctx* c, const KArgs& a) {
+    if (c->N == 4) launch_limiter_t<4>(c, a);
+    else launch_limiter_t<3>(c, a);
 }
 template <int N> void launch_max_speed_t(tlw_ctx* c, const KArgs& a) {
     hipLaunchKernelGGL(k_max_speed<N>, dim3(sgrid_of<N>(c)), dim3(kSurfBlock), 0, c->stream, a);
@@ -523,17 +533,20 @@ int tlw_advance(tlw_ctx* c, int32_t max_steps, double* t_host, double* dt_host, 
     a.use_state_dt = 1;
     a.step_mode = 1;
     for (int s = 0; s < max_steps; ++s) {
+        // Without a limiter the CFL reduction of uⁿ⁺¹ is split between the
+        // volume kernel (nodes it finalises) and the surface kernel (the rest);
+        // with the positivity limiter (applied after the update, lwfr.jl:226)
+        // the limiter kernel takes it over the limited state.
+        const int red = c->positivity ? RED_NONE : RED_STEP;
         if (c->time_disc == TLW_LW) {
-            // the CFL reduction of uⁿ⁺¹ is split between the volume kernel
-            // (nodes it finalises) and the surface kernel (the rest)
             KArgs v = a;
             v.in_sel = SEL_CUR;
             v.out_sel = SEL_OTHER;
-            v.reduce = RED_STEP;
+            v.reduce = red;
             timed(c, 0, [&] { launch_volume(c, v, VOL_LW); });
             KArgs sf = a;
             sf.out_sel = SEL_OTHER;
-            sf.reduce = RED_STEP;
+            sf.reduce = red;
             timed(c, 1, [&] { launch_surface(c, sf); });
         } else {
             // stage 1: us = uⁿ + dt rhs_mdrk1(uⁿ)  (mdrk.jl:13-17)
@@ -551,10 +564,16 @@ int tlw_advance(tlw_ctx* c, int32_t max_steps, double* t_host, double* dt_host, 
             v2.in_sel = SEL_CUR;
             v2.out_sel = SEL_OTHER;
             v2.us = c->d_us;
-            v2.reduce = RED_STEP;
+            v2.reduce = red;
             timed(c, 0, [&] { launch_volume(c, v2, VOL_MDRK2); });
             KArgs s2 = v2;
             timed(c, 1, [&] { launch_surface(c, s2); });
+        }
+        if (c->positivity) {
+            KArgs l = a;
+            l.out_sel = SEL_OTHER;
+            l.reduce = RED_STEP;
+            timed(c, 2, [&] { launch_limiter(c, l); });
         }
     }
     int rc = check_launch(c);
@@ -589,9 +608,61 @@ int tlw_kernel_times(tlw_ctx* c, double* vol_ms, int64_t* vol_n, double* surf_ms
     if (vol_n) *vol_n = c->t_n[0];
     if (surf_ms) *surf_ms = c->t_ms[1];
     if (surf_n) *surf_n = c->t_n[1];
-    c->t_ms[0] = c->t_ms[1] = 0.0;
-    c->t_n[0] = c->t_n[1] = 0;
+    c->t_ms[0] = c->t_ms[1] = c->t_ms[2] = 0.0;
+    c->t_n[0] = c->t_n[1] = c->t_n[2] = 0;
     return TLW_OK;
+}
+
+int tlw_adjoint_lw(tlw_ctx* c, const double* u, const double* lam_next, double* lam, double t, double dt) {
+    if (!c) return TLW_EINVAL;
+    if (!u || !lam_next || !lam) return fail(c, TLW_EINVAL, "null state pointer");
+    if (!std::isfinite(dt) || !std::isfinite(t)) return fail(c, TLW_EINVAL, "t and dt must be finite");
+    if (c->time_disc != TLW_LW || c->sc || c->has_boundary || c->hconst.flux_order != TLW_FLUX_TREE)
+        return fail(c, TLW_ENOTSUP, "the adjoint covers plain LW on a periodic mesh with the TreeMesh flux order");
+    if (lam == lam_next || lam == u) return fail(c, TLW_EINVAL, "lam must not alias lam_next or u");
+    KArgs a = base_args(c);
+    a.u = u;
+    a.us = lam_next;
+    a.out = lam;
+    a.t_arg = t;
+    a.dt_arg = dt;
+    if (c->N == 4) {
+        constexpr int EB = kEBA<4>;
+        dim3 g((c->E + EB - 1) / EB), b(VGeo<4, EB>::NT);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_adj_records<4, EB>), g, b, 0, c->stream, a);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_adjoint<4, EB>), g, b, 0, c->stream, a);
+    } else {
+        constexpr int EB = kEBA<3>;
+        dim3 g((c->E + EB - 1) / EB), b(VGeo<3, EB>::NT);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_adj_records<3, EB>), g, b, 0, c->stream, a);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_adjoint<3, EB>), g, b, 0, c->stream, a);
+    }
+    return check_launch(c);
+}
+
+int tlw_set_positivity_limiter(tlw_ctx* c, int32_t enable, double threshold_density, double threshold_pressure) {
+    if (!c) return TLW_EINVAL;
+    if (enable && (std::isnan(threshold_density) || std::isnan(threshold_pressure)))
+        return fail(c, TLW_EINVAL, "positivity thresholds must not be NaN");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->positivity = enable != 0;
+    if (c->positivity) {
+        c->hconst.pos_threshold[0] = threshold_density;
+        c->hconst.pos_threshold[1] = threshold_pressure;
+        HIPCHK(c, hipMemcpy(c->d_const, &c->hconst, sizeof(DevConst), hipMemcpyHostToDevice));
+    }
+    return TLW_OK;
+}
+
+int tlw_apply_positivity_limiter(tlw_ctx* c, double* u) {
+    if (!c || !u) return fail(c, TLW_EINVAL, "null pointer");
+    if (!c->positivity) return fail(c, TLW_EINVAL, "positivity limiter not configured (tlw_set_positivity_limiter)");
+    KArgs a = base_args(c);
+    a.out = u;
+    a.out_sel = SEL_EXPLICIT;
+    a.reduce = RED_NONE;
+    launch_limiter(c, a);
+    return check_launch(c);
 }
 
 int tlw_set_elements_per_workgroup(tlw_ctx* c, int32_t elements) {

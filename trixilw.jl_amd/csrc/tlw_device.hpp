@@ -67,6 +67,8 @@ struct DevConst {
     double Vinv[8][8];  // inverse normalised-Legendre Vandermonde, row-major
     double alpha_max, alpha_min;
     int alpha_smooth;
+    // positivity limiter thresholds for density and pressure
+    double pos_threshold[2];
 };
 
 // Device-resident integrator state (LWIntegrator fields t, dt, iter and the
@@ -149,17 +151,49 @@ __device__ __forceinline__ void euler_flux_dir(const double* u, double gm1, int 
     for (int v = 0; v < 4; ++v) f[v] = orientation == 1 ? ff[v] : gg[v];
 }
 
+__device__ __forceinline__ double dmax(double x, double y) { return (y > x) ? y : x; }
+
+// Square root of a positive normal argument: the hardware reciprocal
+// square-root estimate refined by two steps of Goldschmidt's iteration and a
+// final Newton correction (P. Markstein, "IA-64 and Elementary Functions",
+// 2000, ch. 8), within an ulp or two of the rounded root.  Zero, negative and
+// NaN arguments give NaN; callers apply it to sound speeds, whose states are
+// checked for ρ > 0 and p > 0.
+__device__ __forceinline__ double sqrt_gs(double x) {
+    const double r = __builtin_amdgcn_rsq(x);
+    double g = x * r, h = 0.5 * r;
+    double e = fma(-g, h, 0.5);
+    g = fma(g, e, g);
+    h = fma(h, e, h);
+    e = fma(-g, h, 0.5);
+    g = fma(g, e, g);
+    h = fma(h, e, h);
+    const double d = fma(-g, g, x);
+    return fma(d, h, g);
+}
+
 // |v_n| and the sound speed: the per-side ingredients of Trixi's
 // max_abs_speed_naive (λ = max(|v_n,L|,|v_n,R|) + max(c_L, c_R)).
 __device__ __forceinline__ void speed_parts(const double* s, double gamma, int orientation, double& a, double& c) {
-    const double rho = s[0];
-    const double v1 = s[1] / rho, v2 = s[2] / rho;
+    const double ir = rcp_nr(s[0]);
+    const double v1 = s[1] * ir, v2 = s[2] * ir;
     const double p = (gamma - 1.0) * (s[3] - 0.5 * (s[1] * v1 + s[2] * v2));
     a = fabs(orientation == 1 ? v1 : v2);
-    c = sqrt(gamma * p / rho);
+    c = sqrt_gs(gamma * p * ir);
 }
 
-__device__ __forceinline__ double dmax(double x, double y) { return (y > x) ? y : x; }
+// |v1|, |v2| and the sound speed of one state: the λ ingredients of both
+// orientations with one reciprocal of ρ and one square root.
+__device__ __forceinline__ void speed_parts2(const double* s, double gamma, double& a1, double& a2, double& c) {
+    const double ir = rcp_nr(s[0]);
+    const double v1 = s[1] * ir, v2 = s[2] * ir;
+    const double p = (gamma - 1.0) * (s[3] - 0.5 * (s[1] * v1 + s[2] * v2));
+    a1 = fabs(v1);
+    a2 = fabs(v2);
+    c = sqrt_gs(gamma * p * ir);
+}
+
+
 
 // Julia's x ≈ y for Float64 (atol = 0, rtol = sqrt(eps)): equal, or both
 // finite and |x - y| <= rtol max(|x|, |y|).  isfinished (lwfr.jl:146-148).
@@ -322,23 +356,24 @@ __device__ __forceinline__ void face_numflux_sc(const double* L, const double* R
 }
 
 // Face record of a face node: jump state, normal component of F̃ and the λ
-// ingredients of the λ state (TreeMesh order: λ from Ũ, jump on uⁿ).
-__device__ __forceinline__ void make_record(double* r, const double* jump, const double* Fn, const double* lam_state,
-                                            double gamma, int orientation) {
+// ingredients |v_n| and c of the λ state (TreeMesh order: λ from Ũ, jump on uⁿ).
+__device__ __forceinline__ void make_record(double* r, const double* jump, const double* Fn, double vn_abs, double c) {
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         r[v] = jump[v];
         r[4 + v] = Fn[v];
     }
-    speed_parts(lam_state, gamma, orientation, r[8], r[9]);
+    r[8] = vn_abs;
+    r[9] = c;
 }
 
 // Per-node maxima λ1 = |v1| + c, λ2 = |v2| + c (Trixi max_abs_speeds).
 __device__ __forceinline__ void node_speeds(const double* un, double gamma, double& l1, double& l2, bool& bad) {
-    const double rho = un[0], v1 = un[1] / rho, v2 = un[2] / rho;
+    const double ir = rcp_nr(un[0]);
+    const double v1 = un[1] * ir, v2 = un[2] * ir;
     const double p = (gamma - 1.0) * (un[3] - 0.5 * (un[1] * v1 + un[2] * v2));
-    bad = !(rho > 0.0) || !(p > 0.0);
-    const double cs = sqrt(gamma * p / rho);
+    bad = !(un[0] > 0.0) || !(p > 0.0);
+    const double cs = sqrt_gs(gamma * p * ir);
     l1 = fabs(v1) + cs;
     l2 = fabs(v2) + cs;
 }
@@ -350,11 +385,13 @@ __device__ __forceinline__ void node_speeds(const double* un, double gamma, doub
 
 // lw_volume_kernel_4!, dg_2d.jl:918-1267 (the temporal-error estimate at
 // 1183-1199, 1251-1265 is an adaptive-mode output and is not computed).
-template <int N, int EB>
+// With KEEP, tder receives the time derivatives ut, utt, ... (4 per order),
+// the intermediates an adjoint sweep needs.
+template <int N, int EB, bool KEEP = false>
 __device__ __forceinline__ void lw4_element(const double* uN, double dt, double invJ, double gm1, const DevConst* c,
                                             int i, int j, bool act, int sidx, int sbase, double* s0, double* s1,
                                             double* s2, const double* sDm, const double* sDh, double* Uc, double* F,
-                                            double* G, double* vol) {
+                                            double* G, double* vol, double* tder = nullptr) {
     using G_ = VGeo<N, EB>;
     constexpr int n = G_::n, S = G_::S;
     euler_fluxes(uN, gm1, F, G);
@@ -366,6 +403,7 @@ __device__ __forceinline__ void lw4_element(const double* uN, double dt, double 
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         ut[v] *= invJ;
+        if constexpr (KEEP) tder[v] = ut[v];
         Uc[v] = fma(0.5, ut[v], uN[v]);
         up[v] = uN[v] + ut[v];
         um[v] = uN[v] - ut[v];
@@ -398,6 +436,7 @@ __device__ __forceinline__ void lw4_element(const double* uN, double dt, double 
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         const double w = uk[v] * invJ;  // utt
+        if constexpr (KEEP) tder[4 + v] = w;
         Uc[v] = fma(1.0 / 6.0, w, Uc[v]);
         up[v] = fma(0.5, w, up[v]);
         um[v] = fma(0.5, w, um[v]);
@@ -433,6 +472,7 @@ __device__ __forceinline__ void lw4_element(const double* uN, double dt, double 
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         const double w = uk[v] * invJ;  // uttt
+        if constexpr (KEEP) tder[8 + v] = w;
         Uc[v] = fma(1.0 / 24.0, w, Uc[v]);
         um[v] = fma(-1.0 / 6.0, w, um[v]);
         up[v] = fma(1.0 / 6.0, w, up[v]);
@@ -463,6 +503,7 @@ __device__ __forceinline__ void lw4_element(const double* uN, double dt, double 
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         const double w = uk[v] * invJ;  // utttt
+        if constexpr (KEEP) tder[12 + v] = w;
         Uc[v] = fma(1.0 / 120.0, w, Uc[v]);
         um[v] = fma(1.0 / 24.0, w, um[v]);
         up[v] = fma(1.0 / 24.0, w, up[v]);
@@ -499,11 +540,13 @@ __device__ __forceinline__ void lw4_element(const double* uN, double dt, double 
 }
 
 // lw_volume_kernel_3!, dg_2d.jl:674-916
-template <int N, int EB>
+// With KEEP, tder receives the time derivatives ut, utt, ... (4 per order),
+// the intermediates an adjoint sweep needs.
+template <int N, int EB, bool KEEP = false>
 __device__ __forceinline__ void lw3_element(const double* uN, double dt, double invJ, double gm1, const DevConst* c,
                                             int i, int j, bool act, int sidx, int sbase, double* s0, double* s1,
                                             double* s2, const double* sDm, const double* sDh, double* Uc, double* F,
-                                            double* G, double* vol) {
+                                            double* G, double* vol, double* tder = nullptr) {
     using G_ = VGeo<N, EB>;
     constexpr int n = G_::n, S = G_::S;
     euler_fluxes(uN, gm1, F, G);
@@ -515,6 +558,7 @@ __device__ __forceinline__ void lw3_element(const double* uN, double dt, double 
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         ut[v] *= invJ;
+        if constexpr (KEEP) tder[v] = ut[v];
         Uc[v] = fma(0.5, ut[v], uN[v]);
         up[v] = uN[v] + ut[v];
         um[v] = uN[v] - ut[v];
@@ -546,6 +590,7 @@ __device__ __forceinline__ void lw3_element(const double* uN, double dt, double 
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         const double w = uk[v] * invJ;  // utt
+        if constexpr (KEEP) tder[4 + v] = w;
         Uc[v] = fma(1.0 / 6.0, w, Uc[v]);
         up[v] = fma(0.5, w, up[v]);
         um[v] = fma(0.5, w, um[v]);
@@ -573,6 +618,7 @@ __device__ __forceinline__ void lw3_element(const double* uN, double dt, double 
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         const double w = uk[v] * invJ;  // uttt
+        if constexpr (KEEP) tder[8 + v] = w;
         Uc[v] = fma(1.0 / 24.0, w, Uc[v]);
         um[v] = fma(-1.0 / 6.0, w, um[v]);
         up[v] = fma(1.0 / 6.0, w, up[v]);
@@ -711,9 +757,12 @@ __device__ __forceinline__ void mdrk2_element(const double* uN, const double* us
 // nodes finalised here: whole elements go to the workgroup partial, elements
 // with an outside face leave their per-node maxima in elem_max.
 // Waves per SIMD the register budget is sized for: 3 (168 VGPRs) for the
-// plain scheme; 2 (256 VGPRs) with shock capturing, whose subcell FV and
-// blended face fluxes would otherwise spill.
-template <bool SC> constexpr int kVolWaves = SC ? 2 : 3;
+// plain scheme (TLW_VOL_WAVES at build time); 2 (256 VGPRs) with shock
+// capturing, whose subcell FV and blended face fluxes would otherwise spill.
+#ifndef TLW_VOL_WAVES
+#define TLW_VOL_WAVES 3
+#endif
+template <bool SC> constexpr int kVolWaves = SC ? 2 : TLW_VOL_WAVES;
 
 template <int N, int EB, int KIND, bool SC>
 __global__ __launch_bounds__((VGeo<N, EB>::NT), kVolWaves<SC>) void k_volume(KArgs a) {
@@ -818,59 +867,94 @@ __global__ __launch_bounds__((VGeo<N, EB>::NT), kVolWaves<SC>) void k_volume(KAr
         }
         __syncthreads();  // s0 and s1 are reused for the face records
     }
-    // --- face records of this node (s0 and s1 are free: records live there).
-    // A node lies on at most one x face (fx) and one y face (fy).
+    // --- face records into LDS (s0, s1 are free), one per face the node lies on.
+    // λ ingredients: |v1|, |v2| and c of the λ state, once per node.
     double* rec = lds;  // [EB][4][n][RQ]
     const bool tree = c->flux_order == 0;
     const double* jump = tree ? uN : Uc;
     const double* lamst = tree ? Uc : uN;
-    auto fill = [&](double* r, const double* fnl) {
-        if constexpr (SC) {
+    const int nbx = fx >= 0 ? a.nbr[4 * e + fx] : -1;
+    const int nby = fy >= 0 ? a.nbr[4 * e + fy] : -1;
+    {
+        double av1, av2, cs;
+        speed_parts2(lamst, gamma, av1, av2, cs);
+        auto fill = [&](double* r, const double* fnl) {
+            if constexpr (SC) {
 #pragma unroll
-            for (int v = 0; v < 4; ++v) r[10 + v] = fnl[v];
-            r[14] = alpha_e;
-            r[15] = 0.0;
+                for (int v = 0; v < 4; ++v) r[10 + v] = fnl[v];
+                r[14] = alpha_e;
+                r[15] = 0.0;
+            }
+        };
+        if (act && fx >= 0) {
+            double r[RQ];
+            make_record(r, jump, F, av1, cs);
+            fill(r, fnlx);
+            store_record<RQ>(rec + ((lec * 4 + fx) * n + j) * RQ, r);
         }
-    };
-    if (act && fx >= 0) {
-        double r[RQ];
-        make_record(r, jump, F, lamst, gamma, 1);
-        fill(r, fnlx);
-        store_record<RQ>(rec + ((lec * 4 + fx) * n + j) * RQ, r);
-    }
-    if (act && fy >= 0) {
-        double r[RQ];
-        make_record(r, jump, G, lamst, gamma, 2);
-        fill(r, fnly);
-        store_record<RQ>(rec + ((lec * 4 + fy) * n + i) * RQ, r);
+        if (act && fy >= 0) {
+            double r[RQ];
+            make_record(r, jump, G, av2, cs);
+            fill(r, fnly);
+            store_record<RQ>(rec + ((lec * 4 + fy) * n + i) * RQ, r);
+        }
     }
     __syncthreads();
-    // --- interfaces inside the workgroup; records of outside faces to HBM
-    double surf[4] = {0.0, 0.0, 0.0, 0.0};
-    bool fin_node = true;
-    auto do_face = [&](int f, int l) {
-        double own[RQ];
-        load_record<RQ>(rec + ((lec * 4 + f) * n + l) * RQ, own);
-        const int ne = a.nbr[4 * e + f];
-        if (ne >= e0 && ne < e0 + nel) {
-            double oth[RQ], Fs[4];
-            load_record<RQ>(rec + (((ne - e0) * 4 + (f ^ 1)) * n + l) * RQ, oth);
-            const double* Lr = (f & 1) ? own : oth;  // +x, +y face: own side is left
-            const double* Rr = (f & 1) ? oth : own;
-            if constexpr (SC)
-                face_numflux_sc(Lr, Rr, f < 2 ? 1 : 2, gamma, a.invJ0, dt, c->weights[n - 1], c->weights[0], Fs);
-            else
-                face_numflux(Lr, Rr, Fs);
-            const double wf = (f & 1) ? c->bnd_hi : -c->bnd_lo;
+    // --- one lane per face slot (element, face, node).  An interface inside the
+    // workgroup is evaluated once, by the slot of its left element's + face, and
+    // its flux goes to both adjacent nodes (accx for x faces, accy for y faces);
+    // the record of a face leaving the workgroup goes to HBM for the surface kernel.
+    double* accx = lds + 16 * S;  // [4][S], in s2 (free after the last contraction)
+    double* accy = accx + 4 * S;
+    constexpr int NSLOT = EB * 4 * n;
+    static_assert(NSLOT <= NT, "one face slot per lane");
+    static_assert(EB * 4 * n * RQ <= 16 * S, "records fit in s0 and s1");
+    if (tid < NSLOT) {
+        const int ls = tid / (4 * n), f = (tid / n) % 4, l = tid % n;
+        if (ls < nel) {
+            const int es = e0 + ls;
+            const int ne = a.nbr[4 * es + f];
+            const double* own = rec + tid * RQ;
+            if (!(ne >= e0 && ne < e0 + nel)) {
+                double r[RQ];
+                load_record<RQ>(own, r);
+                store_record<RQ>(a.trace + ((size_t)es * 4 * n + f * n + l) * RQ, r);
+            } else if (f & 1) {
+                double Lr[RQ], Rr[RQ], Fs[4];
+                load_record<RQ>(own, Lr);
+                load_record<RQ>(rec + (((ne - e0) * 4 + (f ^ 1)) * n + l) * RQ, Rr);
+                if constexpr (SC)
+                    face_numflux_sc(Lr, Rr, f < 2 ? 1 : 2, gamma, a.invJ0, dt, c->weights[n - 1], c->weights[0], Fs);
+                else
+                    face_numflux(Lr, Rr, Fs);
+                // + face of ls: node (n-1, l) or (l, n-1); - face of ne: node (0, l) or (l, 0)
+                const int nodeL = (f == 1) ? l * n + (n - 1) : (n - 1) * n + l;
+                const int nodeR = (f == 1) ? l * n : l;
+                double* acc = (f == 1) ? accx : accy;
 #pragma unroll
-            for (int v = 0; v < 4; ++v) surf[v] = fma(Fs[v], wf, surf[v]);
-        } else {
-            fin_node = false;
-            if (act) store_record<RQ>(a.trace + (((size_t)e * 4 + f) * n + l) * RQ, own);
+                for (int v = 0; v < 4; ++v) {
+                    acc[v * S + ls * NN + nodeL] = Fs[v];
+                    acc[v * S + (ne - e0) * NN + nodeR] = Fs[v];
+                }
+            }
         }
-    };
-    if (fx >= 0) do_face(fx, j);
-    if (fy >= 0) do_face(fy, i);
+    }
+    __syncthreads();
+    // --- lifted internal fluxes, Jacobian, output
+    const bool intx = fx >= 0 && nbx >= e0 && nbx < e0 + nel;
+    const bool inty = fy >= 0 && nby >= e0 && nby < e0 + nel;
+    const bool fin_node = (fx < 0 || intx) && (fy < 0 || inty);
+    double surf[4] = {0.0, 0.0, 0.0, 0.0};
+    if (intx) {
+        const double wf = (fx & 1) ? c->bnd_hi : -c->bnd_lo;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) surf[v] = accx[v * S + sidx] * wf;
+    }
+    if (inty) {
+        const double wf = (fy & 1) ? c->bnd_hi : -c->bnd_lo;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) surf[v] = fma(accy[v * S + sidx], wf, surf[v]);
+    }
     double val[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
@@ -879,31 +963,45 @@ __global__ __launch_bounds__((VGeo<N, EB>::NT), kVolWaves<SC>) void k_volume(KAr
     }
     if (act) store4(out + off, val);
     if (a.reduce == RED_NONE) return;
-    // --- CFL reduction over the nodes finalised here (s2 is free)
+    // --- CFL reduction over the nodes finalised here (records are dead: reuse s0)
     double l1 = -1.0, l2 = -1.0;
     if (fin_node) {
         bool bad;
         node_speeds(val, gamma, l1, l2, bad);
         if (act && bad) st->bad_state = 1;
     }
-    double* sl1 = s2;
-    double* sl2 = s2 + S;
-    double* sm = s2 + 2 * S;
+    double* sl1 = lds;
+    double* sl2 = lds + S;
+    double* rm = lds + 2 * S;            // row maxima [EB][n][2]
+    double* sm = rm + 2 * EB * n;        // element values [EB]
     if (act) {
         sl1[sidx] = l1;
         sl2[sidx] = l2;
     }
     __syncthreads();
+    if (act && i == 0) {
+        double m1 = 0.0, m2 = 0.0;
+#pragma unroll
+        for (int k = 0; k < n; ++k) {
+            m1 = dmax(m1, sl1[sbase + j * n + k]);
+            m2 = dmax(m2, sl2[sbase + j * n + k]);
+        }
+        rm[(lec * n + j) * 2] = m1;
+        rm[(lec * n + j) * 2 + 1] = m2;
+    }
+    __syncthreads();
     if (act && node == 0) {
         double m1 = 0.0, m2 = 0.0;
-        for (int k = 0; k < NN; ++k) {
-            m1 = dmax(m1, sl1[sbase + k]);
-            m2 = dmax(m2, sl2[sbase + k]);
+#pragma unroll
+        for (int k = 0; k < n; ++k) {
+            m1 = dmax(m1, rm[(lec * n + k) * 2]);
+            m2 = dmax(m2, rm[(lec * n + k) * 2 + 1]);
         }
         bool whole = true;
+#pragma unroll
         for (int f = 0; f < 4; ++f) {
             const int ne = a.nbr[4 * e + f];
-            if (!(ne >= e0 && ne < e0 + nel)) whole = false;
+            whole = whole && ne >= e0 && ne < e0 + nel;
         }
         if (whole) {
             sm[le] = invJ * (m1 + m2);
@@ -919,7 +1017,6 @@ __global__ __launch_bounds__((VGeo<N, EB>::NT), kVolWaves<SC>) void k_volume(KAr
         for (int k = 0; k < nel; ++k) bm = dmax(bm, sm[k]);
         a.partial[blockIdx.x] = bm;
     }
-    (void)NT;
 }
 
 // Outer record of a domain-boundary face node: time average of the boundary
@@ -954,7 +1051,9 @@ __device__ inline void boundary_record(const KArgs& a, int e, int face, int l, d
     }
     boundary_value(c, x, y, t, uo);
     const bool tree = c->flux_order == 0;
-    make_record(r, tree ? uo : Uo, Fo, tree ? Uo : uo, c->gamma, orientation);
+    double av1, av2, cs;
+    speed_parts2(tree ? Uo : uo, c->gamma, av1, av2, cs);
+    make_record(r, tree ? uo : Uo, Fo, orientation == 1 ? av1 : av2, cs);
 }
 
 // Surface kernel: the faces that leave a volume workgroup (a.ebv elements).
@@ -1165,6 +1264,107 @@ __global__ __launch_bounds__(kSurfBlock) void k_indicator(KArgs a, double* alpha
         if (al < c->alpha_min) al = 0.0;
         if (al > 1.0 - c->alpha_min) al = 1.0;
         alpha_tmp[e] = al < c->alpha_max ? al : c->alpha_max;
+    }
+}
+
+// Zhang-Shu positivity limiter on (density, pressure), in that order
+// (my_limiter_zhang_shu!, src/callbacks/callbacks_stage/positivity_zhang_shu_dg2d.jl:8-45,
+// positivity_zhang_shu.jl:26-41): where the variable's minimum over an
+// element's nodes is below its threshold, u <- θ u + (1-θ) ū with
+// θ = (var(ū) - threshold) / (var(ū) - min var) and ū = Σ u w_i w_j / 4 summed
+// in the reference's node order.  Limits a.out (explicit) or the new state of
+// the device loop in place and, in the loop, takes the CFL reduction of the
+// limited state (the next step's compute_dt).
+template <int N>
+__global__ __launch_bounds__(kSurfBlock) void k_limiter(KArgs a) {
+    using G_ = SGeo<N>;
+    constexpr int n = G_::n, NN = G_::NN, EB = G_::EB, S = G_::S;
+    __shared__ double s_var[S], s_q[4][S];
+    __shared__ double s_theta[EB], s_mean[EB][4];
+    __shared__ int s_flag[EB];
+    DevState* st = a.st;
+    if (a.use_state_dt && st->finished) return;
+    const int tid = threadIdx.x;
+    const int le = tid / NN, node = tid - (tid / NN) * NN;
+    const int i = node % n, j = node / n;
+    const int e0 = blockIdx.x * EB;
+    const int nel = min(EB, a.E - e0);
+    const bool act = le < nel;
+    const int lec = le < nel ? le : nel - 1;
+    const int e = e0 + lec;
+    const int sidx = lec * NN + node, sbase = lec * NN;
+    const int cur = a.use_state_dt ? st->cur : 0;
+    double* u = sel_out(a, cur);
+    const DevConst* c = a.c;
+    const double gm1 = c->gamma - 1.0;
+    const size_t off = ((size_t)e * NN + node) * 4;
+    double un[4];
+    load4(u + off, un);
+    bool changed = false;
+#pragma unroll
+    for (int pass = 0; pass < 2; ++pass) {
+        const double thr = c->pos_threshold[pass];
+        const double var = pass == 0 ? un[0] : gm1 * (un[3] - 0.5 * (un[1] * un[1] + un[2] * un[2]) / un[0]);
+        if (act) {
+            s_var[sidx] = var;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) s_q[v][sidx] = un[v] * c->weights[i] * c->weights[j];
+        }
+        __syncthreads();
+        if (act && node == 0) {
+            double vmin = s_var[sbase];
+            for (int k = 1; k < NN; ++k) vmin = s_var[sbase + k] < vmin ? s_var[sbase + k] : vmin;
+            int flag = vmin < thr;
+            if (flag) {
+                double m[4] = {0.0, 0.0, 0.0, 0.0};
+                for (int k = 0; k < NN; ++k)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) m[v] += s_q[v][sbase + k];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) m[v] = m[v] / 4.0;
+                const double vmean = pass == 0 ? m[0] : gm1 * (m[3] - 0.5 * (m[1] * m[1] + m[2] * m[2]) / m[0]);
+                s_theta[le] = (vmean - thr) / (vmean - vmin);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) s_mean[le][v] = m[v];
+            }
+            s_flag[le] = flag;
+        }
+        __syncthreads();
+        if (s_flag[lec]) {
+            const double th = s_theta[lec];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) un[v] = th * un[v] + (1.0 - th) * s_mean[lec][v];
+            changed = true;
+        }
+        __syncthreads();  // s_var, s_q, s_flag are rewritten by the next pass
+    }
+    if (act && changed) store4(u + off, un);
+    if (a.reduce == RED_NONE) return;
+    // per-element max of λ1, λ2 of the limited state, then one partial per workgroup
+    double l1, l2;
+    bool bad;
+    node_speeds(un, c->gamma, l1, l2, bad);
+    if (act && bad) st->bad_state = 1;
+    double* s_l1 = s_var;
+    double* s_l2 = s_q[0];
+    if (act) {
+        s_l1[sidx] = l1;
+        s_l2[sidx] = l2;
+    }
+    __syncthreads();
+    if (act && node == 0) {
+        double m1 = 0.0, m2 = 0.0;
+        for (int k = 0; k < NN; ++k) {
+            m1 = dmax(m1, s_l1[sbase + k]);
+            m2 = dmax(m2, s_l2[sbase + k]);
+        }
+        s_theta[lec] = a.inv_jac[e] * (m1 + m2);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double bm = 0.0;
+        for (int k = 0; k < nel; ++k) bm = dmax(bm, s_theta[k]);
+        a.partial[blockIdx.x] = bm;
     }
 }
 

@@ -5,11 +5,13 @@ SemidiscretizationHyperbolic, rhs!/rhs_mdrk1!/rhs_mdrk2!, solve_lwfr) over the
 C ABI of libtrixilw_hip.so.  Importing this package does not load the
 library; the first call that needs the GPU does, and fails if it is missing.
 """
+from .adjoint import adjoint_step, gradient_lw, kinetic_energy
 from .basis import LobattoLegendreBasis
 from .equations import (CompressibleEulerEquations2D, ConstantState, FourQuadrantRiemann, IsentropicVortex,
                         LinearScalarAdvectionEquation2D)
 from .integrator import (LWIntegrator, LWSolution, compute_dt, isapprox, isfinished, perform_step, set_dt,
                          set_t_and_iter, solve_lwfr)
+from .limiters import PositivityPreservingLimiterZhangShu, density, pressure
 from .mesh import TreeMesh
 from .semidiscretization import (DGSEM, LW, MDRK, CFLBased, FirstOrderReconstruction, IndicatorHennemannGassner,
                                  LWUpdate, MUSCLHancockReconstruction, SemidiscretizationHyperbolic, VolumeIntegralFR,
@@ -24,5 +26,6 @@ __all__ = [
     "CFLBased", "LWUpdate", "SemidiscretizationHyperbolic", "VolumeIntegralFR", "flux_lax_friedrichs",
     "get_time_discretization", "lw_fourier_cfl", "rhs", "rhs_mdrk1", "rhs_mdrk2", "semidiscretize", "trixi2lw",
     "FirstOrderReconstruction", "IndicatorHennemannGassner", "MUSCLHancockReconstruction",
-    "VolumeIntegralFRShockCapturing", "density_pressure",
+    "VolumeIntegralFRShockCapturing", "density_pressure", "PositivityPreservingLimiterZhangShu", "density",
+    "pressure", "adjoint_step", "gradient_lw", "kinetic_energy",
 ]

@@ -6,9 +6,11 @@ The library must exist: there is no CPU fallback.  Build it with
 from __future__ import annotations
 
 import ctypes as C
+import os
 from pathlib import Path
 
-LIB_PATH = Path(__file__).resolve().parent / "libtrixilw_hip.so"
+# TRIXILW_HIP_LIBRARY selects another build of the same ABI (e.g. other compile-time options)
+LIB_PATH = Path(os.environ.get("TRIXILW_HIP_LIBRARY") or Path(__file__).resolve().parent / "libtrixilw_hip.so")
 
 TLW_OK, TLW_EINVAL, TLW_EHIP, TLW_EADMISSIBLE, TLW_ENOTSUP = 0, 1, 2, 3, 4
 TLW_LW, TLW_MDRK = 0, 1
@@ -22,7 +24,8 @@ EXPORTED = (
     "tlw_rhs_lw", "tlw_rhs_mdrk1", "tlw_rhs_mdrk2", "tlw_us_ptr", "tlw_set_us", "tlw_max_dt", "tlw_set_state",
     "tlw_get_state", "tlw_state_ptr", "tlw_solve_init", "tlw_advance", "tlw_get_aux",
     "tlw_enable_kernel_timing", "tlw_kernel_times", "tlw_set_elements_per_workgroup",
-    "tlw_get_elements_per_workgroup", "tlw_autotune",
+    "tlw_get_elements_per_workgroup", "tlw_autotune", "tlw_set_positivity_limiter", "tlw_apply_positivity_limiter",
+    "tlw_adjoint_lw",
 )
 
 
@@ -130,6 +133,12 @@ def lib():
     L.tlw_set_elements_per_workgroup.argtypes = [vp, i32]
     L.tlw_get_elements_per_workgroup.restype = C.c_int
     L.tlw_get_elements_per_workgroup.argtypes = [vp, C.POINTER(i32)]
+    L.tlw_set_positivity_limiter.restype = C.c_int
+    L.tlw_set_positivity_limiter.argtypes = [vp, i32, C.c_double, C.c_double]
+    L.tlw_apply_positivity_limiter.restype = C.c_int
+    L.tlw_apply_positivity_limiter.argtypes = [vp, vp]
+    L.tlw_adjoint_lw.restype = C.c_int
+    L.tlw_adjoint_lw.argtypes = [vp, vp, vp, vp, C.c_double, C.c_double]
     L.tlw_autotune.restype = C.c_int
     L.tlw_autotune.argtypes = [vp, i32, C.POINTER(i32), dp]
     _lib = L

@@ -5,10 +5,11 @@ Two drivers with the reference's semantics:
     rhs entry points (compute_dt -> set_dt! -> uprev = u -> rhs! ->
     u = uprev + dt du -> set_t_and_iter!), with callbacks and limiters
     applied between steps as ``solve_lwfr`` does (lwfr.jl:233-263);
-  * without callbacks or limiters, ``solve_lwfr`` runs the same loop on the
-    device (tlw_solve_init / tlw_advance): dt, the truncation at the final
-    time, t and the isfinished test all live on the GPU and steps are
-    launched in chunks without host synchronisation.
+  * without callbacks (and with at most the positivity limiter),
+    ``solve_lwfr`` runs the same loop on the device (tlw_solve_init /
+    tlw_advance): dt, the truncation at the final time, t, the limiter and
+    the isfinished test all live on the GPU and steps are launched in chunks
+    without host synchronisation.
 """
 from __future__ import annotations
 
@@ -19,6 +20,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _lib
+from .limiters import PositivityPreservingLimiterZhangShu
 from .semidiscretization import (LW, MDRK, CFLBased, LWUpdate, rhs, rhs_mdrk1, rhs_mdrk2,
                                  get_time_discretization)
 
@@ -132,7 +134,8 @@ def solve_lwfr(lw_update: LWUpdate, callbacks=(), dt_initial=1e-6, tolerances=No
     """solve_lwfr(lw_update, callbacks, dt_initial, tolerances; time_step_computation, limiters).
 
     Returns an LWSolution whose ``u`` is the final state (device tensor).
-    ``device_loop`` defaults to True when there are no callbacks and no limiters.
+    ``device_loop`` defaults to True when there are no callbacks and every
+    limiter is a PositivityPreservingLimiterZhangShu (at most one).
     ``max_steps`` optionally stops the loop early (for fixed-step runs).
     """
     if time_step_computation is None:
@@ -148,9 +151,13 @@ def solve_lwfr(lw_update: LWUpdate, callbacks=(), dt_initial=1e-6, tolerances=No
     for cb in callbacks:
         if cb.condition(integrator.u, integrator.t, integrator):
             cb.affect(integrator)
+    dev_limiters = all(isinstance(lim, PositivityPreservingLimiterZhangShu) for lim in limiters) and len(limiters) <= 1
     if device_loop is None:
-        device_loop = not callbacks and not limiters
+        device_loop = not callbacks and dev_limiters
     if device_loop:
+        if callbacks or not dev_limiters:
+            raise NotImplementedError("the device loop runs without callbacks and with at most one positivity limiter")
+        semi.set_positivity_limiter(limiters[0].device_thresholds if limiters else None)
         _device_solve(integrator, time_step_computation, chunk, max_steps)
     else:
         while not isfinished(integrator) and (max_steps is None or integrator.iter < max_steps):

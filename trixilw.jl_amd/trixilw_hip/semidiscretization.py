@@ -276,6 +276,21 @@ class SemidiscretizationHyperbolic:
         self._bind_stream()
         _lib.check(_lib.lib().tlw_set_us(self._h, C.c_void_p(us_ode.data_ptr())), self._h)
 
+    def set_positivity_limiter(self, thresholds):
+        """Zhang-Shu limiter of the device loop: (threshold_density,
+        threshold_pressure), or None to turn it off."""
+        if thresholds is None:
+            _lib.check(_lib.lib().tlw_set_positivity_limiter(self._h, 0, 0.0, 0.0), self._h)
+        else:
+            _lib.check(_lib.lib().tlw_set_positivity_limiter(self._h, 1, float(thresholds[0]), float(thresholds[1])),
+                       self._h)
+
+    def apply_positivity_limiter(self, u_ode, thresholds):
+        self._check_state(u_ode, "u")
+        self._bind_stream()
+        self.set_positivity_limiter(thresholds)
+        _lib.check(_lib.lib().tlw_apply_positivity_limiter(self._h, C.c_void_p(u_ode.data_ptr())), self._h)
+
     def elements_per_workgroup(self, elements=None):
         """Tile of the fused volume kernel: returns it, or sets it when given."""
         if elements is not None:
