@@ -88,6 +88,76 @@ def cpu_baseline(o, u0, cfl, mdrk, dofs, steps, threads):
     return dofs * len(dts) / el, el
 
 
+def measure_batch(steps=200, n_instances=64, seed=2024, single_value=None):
+    """64 isentropic-vortex instances (mesh 16²..128², polydeg 3 or 4, Mach U[0.2, 0.8],
+    strength U[2, 5], drawn with numpy default_rng(seed)), grouped by polydeg into two batch
+    contexts; each group runs `steps` LW steps of its instances (own CFL dt per instance,
+    0.98·LW limit) replayed from a captured graph.  Returns the sub-record of the JSON line."""
+    import torch
+    import trixilw_hip as tlw
+
+    rng = np.random.default_rng(seed)
+    levels = rng.integers(4, 8, n_instances)
+    degs = rng.choice([3, 4], n_instances)
+    machs = rng.uniform(0.2, 0.8, n_instances)
+    strengths = rng.uniform(2.0, 5.0, n_instances)
+    eq = tlw.CompressibleEulerEquations2D(1.4)
+    total_dofs, total_s, groups = 0, 0.0, {}
+    stream = torch.cuda.Stream()
+    for N in (3, 4):
+        idx = np.nonzero(degs == N)[0]
+        if len(idx) == 0:
+            continue
+        meshes = [tlw.TreeMesh((-10.0, -10.0), (10.0, 10.0), int(levels[k]), ordering="morton") for k in idx]
+        ics = [tlw.IsentropicVortex(strength=float(strengths[k]), mach=float(machs[k])) for k in idx]
+        solver = tlw.DGSEM(int(N))
+        batch = tlw.SemidiscretizationBatch(meshes, tlw.LW(), eq, ics, solver)
+        cfl = tlw.trixi2lw(0.98, solver)
+        u0 = batch.compute_coefficients()
+        batch.set_stream(stream)
+        L = tlw._lib.lib()
+        h = batch.handle
+        dp = C.POINTER(C.c_double)
+        tlw._lib.check(L.tlw_set_state(h, u0.ctypes.data_as(dp), u0.size), h)
+        chosen, _ = batch_autotune(batch, u0)
+        tlw._lib.check(L.tlw_enable_graphs(h, 1), h)
+        tlw._lib.check(L.tlw_solve_init(h, 0.0, float("inf"), cfl), h)
+        batch.advance(5)  # capture + warm-up
+        tlw._lib.check(L.tlw_set_state(h, u0.ctypes.data_as(dp), u0.size), h)
+        tlw._lib.check(L.tlw_solve_init(h, 0.0, float("inf"), cfl), h)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        batch.advance(steps)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        st = batch.status()
+        assert np.all(st["iter"] == steps), st["iter"]
+        dofs = batch.ndofs()
+        groups[f"N{N}"] = {"instances": int(len(idx)), "dofs": int(dofs), "seconds": round(el, 5),
+                           "dof_updates_per_s": dofs * steps / el, "elements_per_workgroup": chosen}
+        total_dofs += dofs
+        total_s += el
+        batch.close()
+    value = total_dofs * steps / total_s
+    out = {"value": value, "unit": "DOF-updates/s", "instances": n_instances, "steps": steps,
+           "total_dofs": int(total_dofs), "groups": groups,
+           "config": "64 isentropic-vortex instances, 16^2..128^2 elements, polydeg 3/4, Mach U[0.2,0.8], "
+                     "strength U[2,5], LW, graph-replayed steps, one batch context per polydeg"}
+    if single_value:
+        out["vs_single_instance"] = round(value / single_value, 3)
+    return out
+
+
+def batch_autotune(batch, u0):
+    """Tile choice of a batch context on its initial state."""
+    import trixilw_hip as tlw
+
+    chosen = C.c_int32()
+    ms = (C.c_double * 3)()
+    tlw._lib.check(tlw._lib.lib().tlw_autotune(batch.handle, 2, C.byref(chosen), ms), batch.handle)
+    return chosen.value, list(ms)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,6 +168,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--tile", type=int, default=0, help="elements per volume workgroup (0: autotune)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward step ratio")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-instance batch measurement")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device copy (the reference kernel for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -237,6 +308,13 @@ def main():
         log(f"bench: adjoint step {adj_ms:.3f} ms = {adj_ms / ms_per_step:.2f} x forward step")
         del u, lam_a, lam_b
 
+    batch = None
+    if args.workload == DEFAULT_WORKLOAD and not args.no_batch:
+        log("bench: 64-instance batch")
+        batch = measure_batch(single_value=value)
+        log(f"bench: batch {batch['value'] / 1e9:.3f} G DOF-updates/s "
+            f"({batch.get('vs_single_instance')} x the single 1024^2 instance)")
+
     cpu = None
     if not args.no_cpu_baseline:
         import oracle_ctypes
@@ -283,6 +361,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "adjoint": adjoint,
+        "batch": batch,
     }
     print(json.dumps(out), flush=True)
     semi.close()

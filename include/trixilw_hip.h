@@ -103,6 +103,22 @@ typedef struct tlw_ctx tlw_ctx;
  * basis, allocates the device state, traces and MDRK stage buffer. */
 int tlw_create(const tlw_mesh_desc* mesh, const tlw_basis_desc* basis, const tlw_solver_desc* solver,
                tlw_ctx** out);
+/* A batch: n_instances independent meshes (meshes[k], element ids local to each) with the same
+ * basis and solver, advanced together by one launch per kernel and step of tlw_advance; each
+ * instance keeps its own t, dt (its own CFL reduction), iteration count and finished flag.
+ * No counterpart in the reference, which runs one semidiscretization per solve_lwfr.  States
+ * are the instances' states concatenated in order; tlw_advance reports instance 0 and
+ * `finished` when all are; tlw_batch_status reports every instance.  tlw_max_dt returns the
+ * smallest of the instances' values. */
+int tlw_create_batch(int32_t n_instances, const tlw_mesh_desc* meshes, const tlw_basis_desc* basis,
+                     const tlw_solver_desc* solver, tlw_ctx** out);
+int tlw_n_instances(const tlw_ctx* ctx, int32_t* n_instances);
+int tlw_batch_status(tlw_ctx* ctx, int32_t n_instances, double* t_host, double* dt_host, int64_t* iter_host,
+                     int32_t* finished_host);
+/* Replay tlw_advance's steps from a captured HIP graph (one graph per step count, captured on
+ * first use; needs an explicit stream from tlw_set_stream).  No counterpart in the reference. */
+int tlw_enable_graphs(tlw_ctx* ctx, int32_t enable);
+
 /* AMR or any host-side mesh change: rebuild the device mesh (src/callbacks/amr.jl:2-20 resize!) */
 int tlw_rebuild_mesh(tlw_ctx* ctx, const tlw_mesh_desc* mesh);
 void tlw_destroy(tlw_ctx* ctx);

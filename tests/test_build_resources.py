@@ -1,0 +1,49 @@
+"""Register and private-memory budgets of the hot kernels (compiler remarks
+of the gfx950 build; no GPU).  A kernel whose arguments or registers fall back
+to private memory runs several times slower, so the budgets are checked here."""
+import re
+import subprocess
+import sys
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def resource_usage(tmp_path):
+    sys.path.insert(0, str(ROOT / "trixilw.jl_amd"))
+    import build
+
+    out = tmp_path / "probe.so"
+    cmd = [build.hipcc(), *build.HIPCC_FLAGS, "-Rpass-analysis=kernel-resource-usage", "-o", str(out),
+           str(build.CSRC / "tlw_capi.hip")]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-2000:]
+    usage, cur = {}, None
+    for ln in r.stderr.splitlines():
+        m = re.search(r"remark:\s+(Function Name|ScratchSize \[bytes/lane\]|Occupancy \[waves/SIMD\]): (\S+)", ln)
+        if not m:
+            continue
+        if m.group(1) == "Function Name":
+            cur = m.group(2)
+            usage[cur] = {}
+        else:
+            usage[cur][m.group(1).split()[0]] = int(m.group(2))
+    return usage
+
+
+@pytest.fixture(scope="module")
+def usage(tmp_path_factory):
+    return resource_usage(tmp_path_factory.mktemp("res"))
+
+
+def test_hot_kernels_stay_out_of_private_memory(usage):
+    vol = {k: v for k, v in usage.items() if "k_volume" in k}
+    surf = {k: v for k, v in usage.items() if "k_surface" in k}
+    assert vol and surf
+    for name, u in surf.items():
+        assert u["ScratchSize"] == 0, (name, u)
+    for name, u in vol.items():
+        assert u["ScratchSize"] <= 64, (name, u)  # a few spilled registers at most
+        assert u["Occupancy"] >= 2, (name, u)

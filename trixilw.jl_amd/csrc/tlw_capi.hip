@@ -5,12 +5,15 @@
 // src/semidiscretization/semidiscretization_hyperbolic.jl:58-84,
 // src/solvers/dgsem_tree/containers.jl:7-64), owns the device buffers, and
 // sequences the kernels of tlw_device.hpp for rhs!, rhs_mdrk1!/2!, max_dt and
-// the device-resident solve_lwfr loop.
+// the device-resident solve_lwfr loop.  A context holds one mesh or a batch of
+// independent meshes of the same polynomial degree (instances), each with its
+// own time, dt and CFL reduction, advanced by the same launches.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -22,13 +25,16 @@ using namespace tlw;
 
 struct tlw_ctx {
     int N = 0, n = 0, E = 0;
+    int ninst = 1;
+    std::vector<int> inst_first;  // [ninst + 1] first element of each instance
     int time_disc = TLW_LW;
     int flags = 0;
     hipStream_t stream = nullptr;
     DevConst hconst{};
     DevConst* d_const = nullptr;
-    DevState* d_state = nullptr;
-    DevState* h_state = nullptr;  // pinned mirror
+    DevState* d_state = nullptr;  // [ninst]
+    DevState* h_state = nullptr;  // pinned mirror [ninst]
+    int* d_cur = nullptr;         // [ninst] staging for k_init_state
     int* d_nbr = nullptr;
     double* d_invj = nullptr;
     double* d_center = nullptr;
@@ -42,12 +48,17 @@ struct tlw_ctx {
     double* d_elem_max = nullptr;
     double* d_alpha_tmp = nullptr;
     double* d_alpha = nullptr;
+    // workgroup tables (KArgs::vblk, sblk, etile) and per-instance partial ranges
+    int4* d_vblk = nullptr;
+    int4* d_sblk = nullptr;
+    int* d_etile = nullptr;
+    int4* d_rng = nullptr;
+    int vgrid = 0, sgrid = 0, vcap = 0;
     bool sc = false;
-    int ebv = 0;  // elements per volume workgroup
+    int ebv = 0;              // elements per volume workgroup
     bool positivity = false;  // Zhang-Shu limiter in the device loop
-    double invJ0 = 0.0;
     bool has_boundary = false;
-    int host_cur = 0;
+    std::vector<int> host_cur;  // [ninst] state buffer holding each instance's uⁿ
     bool aux_valid = false;
     bool alpha_valid = false;  // blending factors of the last rhs or step (shock capturing)
     // kernel timing (tlw_enable_kernel_timing)
@@ -57,6 +68,10 @@ struct tlw_ctx {
     size_t ev_used = 0;
     double t_ms[3] = {0.0, 0.0, 0.0};
     int64_t t_n[3] = {0, 0, 0};
+    // captured step sequences of tlw_advance, by step count (tlw_enable_graphs)
+    bool graphs = false;
+    std::map<int, hipGraphExec_t> graph_cache;
+    hipStream_t graph_stream = nullptr;
     std::string err;
 };
 
@@ -75,35 +90,87 @@ int fail(tlw_ctx* c, int code, const std::string& msg) {
     } while (0)
 
 size_t state_len(const tlw_ctx* c) { return (size_t)4 * c->n * c->n * c->E; }
+size_t inst_offset(const tlw_ctx* c, int k) { return (size_t)4 * c->n * c->n * c->inst_first[k]; }
+size_t inst_len(const tlw_ctx* c, int k) {
+    return (size_t)4 * c->n * c->n * (c->inst_first[k + 1] - c->inst_first[k]);
+}
 
-int vgrid(const tlw_ctx* c);
-int sgrid(const tlw_ctx* c);
-int min_eb(int N);
+// Elements per volume workgroup: the compiled tile sizes, selectable at run
+// time (tlw_set_elements_per_workgroup, tlw_autotune).  polydeg 4 (25 nodes):
+// 5, 10, 15 elements = 125, 250, 375 lanes of 128, 256, 384; polydeg 3
+// (16 nodes): 4, 8, 16 elements = 64, 128, 256 lanes, no idle lanes.
+constexpr int kEbChoices4[3] = {5, 10, 15};
+constexpr int kEbChoices3[3] = {4, 8, 16};
+const int* eb_choices(int N) { return N == 4 ? kEbChoices4 : kEbChoices3; }
+int default_eb(int N) { return N == 4 ? 15 : 16; }
+int min_eb(int N) { return N == 4 ? 5 : 4; }
+bool eb_supported(int N, int eb) {
+    const int* ch = eb_choices(N);
+    return eb == ch[0] || eb == ch[1] || eb == ch[2];
+}
+int surf_eb(int N) { return N == 4 ? SGeo<4>::EB : SGeo<3>::EB; }
+
+void drop_graphs(tlw_ctx* c) {
+    for (auto& kv : c->graph_cache) (void)hipGraphExecDestroy(kv.second);
+    c->graph_cache.clear();
+}
 
 void free_mesh(tlw_ctx* c) {
-    (void)hipFree(c->d_nbr);
-    (void)hipFree(c->d_invj);
-    (void)hipFree(c->d_center);
-    (void)hipFree(c->d_length);
-    (void)hipFree(c->d_trace);
-    (void)hipFree(c->d_buf[0]);
-    (void)hipFree(c->d_buf[1]);
-    (void)hipFree(c->d_us);
-    (void)hipFree(c->d_auxU);
-    (void)hipFree(c->d_auxF);
-    (void)hipFree(c->d_partial);
-    (void)hipFree(c->d_elem_max);
-    (void)hipFree(c->d_alpha_tmp);
-    (void)hipFree(c->d_alpha);
-    c->d_partial = nullptr;
-    c->d_elem_max = nullptr;
-    c->d_alpha_tmp = c->d_alpha = nullptr;
+    drop_graphs(c);
+    void* ptrs[] = {c->d_nbr,       c->d_invj,  c->d_center, c->d_length, c->d_trace,   c->d_buf[0],    c->d_buf[1],
+                    c->d_us,        c->d_auxU,  c->d_auxF,   c->d_partial, c->d_elem_max, c->d_alpha_tmp, c->d_alpha,
+                    c->d_vblk,      c->d_sblk,  c->d_etile,  c->d_rng,    c->d_state,   c->d_cur};
+    for (void* p : ptrs) (void)hipFree(p);
+    if (c->h_state) (void)hipHostFree(c->h_state);
     c->d_nbr = nullptr;
     c->d_invj = c->d_center = c->d_length = c->d_trace = c->d_us = c->d_auxU = c->d_auxF = nullptr;
     c->d_buf[0] = c->d_buf[1] = nullptr;
+    c->d_partial = c->d_elem_max = c->d_alpha_tmp = c->d_alpha = nullptr;
+    c->d_vblk = c->d_sblk = c->d_rng = nullptr;
+    c->d_etile = nullptr;
+    c->d_state = nullptr;
+    c->h_state = nullptr;
+    c->d_cur = nullptr;
 }
 
-int upload_mesh(tlw_ctx* c, const tlw_mesh_desc* m) {
+// Workgroup tables for the current tile: volume workgroups of ebv elements and
+// node-per-lane workgroups of the surface size, both restarted at every
+// instance boundary, the element -> volume workgroup map and each instance's
+// range of partials (volume workgroups, then the node-per-lane grid after them).
+int build_tables(tlw_ctx* c) {
+    std::vector<int4> vb, sb, rng(c->ninst);
+    std::vector<int> et(c->E);
+    const int seb = surf_eb(c->N);
+    for (int k = 0; k < c->ninst; ++k) {
+        const int f0 = c->inst_first[k], f1 = c->inst_first[k + 1];
+        rng[k].x = (int)vb.size();
+        for (int e = f0; e < f1; e += c->ebv) {
+            const int nel = std::min(c->ebv, f1 - e);
+            for (int q = 0; q < nel; ++q) et[e + q] = (int)vb.size();
+            vb.push_back(make_int4(e, nel, k, 0));
+        }
+        rng[k].y = (int)vb.size();
+        rng[k].z = (int)sb.size();
+        for (int e = f0; e < f1; e += seb) sb.push_back(make_int4(e, std::min(seb, f1 - e), k, 0));
+        rng[k].w = (int)sb.size();
+    }
+    if ((int)vb.size() > c->vcap) return fail(c, TLW_EINVAL, "volume workgroup table exceeds its allocation");
+    c->vgrid = (int)vb.size();
+    c->sgrid = (int)sb.size();
+    for (auto& r : rng) {  // surface partials follow the volume ones
+        r.z += c->vgrid;
+        r.w += c->vgrid;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(c->d_vblk, vb.data(), sizeof(int4) * vb.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_sblk, sb.data(), sizeof(int4) * sb.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_etile, et.data(), sizeof(int) * et.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_rng, rng.data(), sizeof(int4) * rng.size(), hipMemcpyHostToDevice));
+    drop_graphs(c);
+    return TLW_OK;
+}
+
+int validate_mesh(tlw_ctx* c, const tlw_mesh_desc* m) {
     if (!m || m->n_elements <= 0 || !m->neighbors_host || !m->inverse_jacobian_host || !m->element_center_host ||
         !m->element_length_host)
         return fail(c, TLW_EINVAL, "mesh description: missing arrays or n_elements <= 0");
@@ -122,14 +189,47 @@ int upload_mesh(tlw_ctx* c, const tlw_mesh_desc* m) {
     for (int e = 0; e < E; ++e)
         if (!(m->inverse_jacobian_host[e] > 0.0) || !(m->element_length_host[e] > 0.0))
             return fail(c, TLW_EINVAL, "mesh description: non-positive Jacobian or element length");
+    return TLW_OK;
+}
+
+// Uploads ninst meshes as one element range each (element ids offset by the
+// instance's first element) and allocates everything that depends on E.
+int upload_meshes(tlw_ctx* c, const tlw_mesh_desc* ms, int ninst) {
+    if (ninst < 1) return fail(c, TLW_EINVAL, "at least one instance");
+    long long total = 0;
+    for (int k = 0; k < ninst; ++k) {
+        int rc = validate_mesh(c, &ms[k]);
+        if (rc) return rc;
+        total += ms[k].n_elements;
+    }
+    if (total > (1LL << 30)) return fail(c, TLW_EINVAL, "too many elements");
+    bool boundary = false;
+    for (int k = 0; k < ninst; ++k)
+        for (size_t q = 0; q < (size_t)4 * ms[k].n_elements; ++q)
+            if (ms[k].neighbors_host[q] < 0) boundary = true;
+    if (boundary && c->hconst.bc_kind == TLW_BV_NONE)
+        return fail(c, TLW_EINVAL, "mesh has boundary faces but no boundary value function was given");
     free_mesh(c);
     c->aux_valid = c->alpha_valid = false;
-    c->E = E;
-    c->has_boundary = false;
-    for (size_t k = 0; k < (size_t)4 * E; ++k)
-        if (m->neighbors_host[k] < 0) c->has_boundary = true;
-    if (c->has_boundary && c->hconst.bc_kind == TLW_BV_NONE)
-        return fail(c, TLW_EINVAL, "mesh has boundary faces but no boundary value function was given");
+    c->ninst = ninst;
+    c->E = (int)total;
+    c->has_boundary = boundary;
+    c->inst_first.assign(ninst + 1, 0);
+    for (int k = 0; k < ninst; ++k) c->inst_first[k + 1] = c->inst_first[k] + ms[k].n_elements;
+    c->host_cur.assign(ninst, 0);
+    const int E = c->E;
+    std::vector<int> nbr(4 * (size_t)E);
+    std::vector<double> invj(E), center(2 * (size_t)E), length(E);
+    for (int k = 0; k < ninst; ++k) {
+        const int f0 = c->inst_first[k], Ek = ms[k].n_elements;
+        for (size_t q = 0; q < (size_t)4 * Ek; ++q) {
+            const int nb = ms[k].neighbors_host[q];
+            nbr[4 * (size_t)f0 + q] = nb < 0 ? -1 : nb + f0;
+        }
+        std::memcpy(&invj[f0], ms[k].inverse_jacobian_host, sizeof(double) * Ek);
+        std::memcpy(&center[2 * (size_t)f0], ms[k].element_center_host, sizeof(double) * 2 * Ek);
+        std::memcpy(&length[f0], ms[k].element_length_host, sizeof(double) * Ek);
+    }
     const size_t nd = state_len(c);
     HIPCHK(c, hipMalloc(&c->d_nbr, sizeof(int) * 4 * E));
     HIPCHK(c, hipMalloc(&c->d_invj, sizeof(double) * E));
@@ -139,25 +239,34 @@ int upload_mesh(tlw_ctx* c, const tlw_mesh_desc* m) {
     HIPCHK(c, hipMalloc(&c->d_buf[0], sizeof(double) * nd));
     HIPCHK(c, hipMalloc(&c->d_buf[1], sizeof(double) * nd));
     HIPCHK(c, hipMalloc(&c->d_us, sizeof(double) * nd));
-    // sized for the smallest tile, so the tile can change without a reallocation
-    const int vmax = (E + min_eb(c->N) - 1) / min_eb(c->N);
-    HIPCHK(c, hipMalloc(&c->d_partial, sizeof(double) * (size_t)(vmax + sgrid(c) + 1)));
+    // tables and partials sized for the smallest tile, so the tile can change freely
+    c->vcap = (E + min_eb(c->N) - 1) / min_eb(c->N) + ninst;
+    const int scap = (E + surf_eb(c->N) - 1) / surf_eb(c->N) + ninst;
+    HIPCHK(c, hipMalloc(&c->d_partial, sizeof(double) * (size_t)(c->vcap + scap + 1)));
+    HIPCHK(c, hipMalloc(&c->d_vblk, sizeof(int4) * (size_t)c->vcap));
+    HIPCHK(c, hipMalloc(&c->d_sblk, sizeof(int4) * (size_t)scap));
+    HIPCHK(c, hipMalloc(&c->d_etile, sizeof(int) * (size_t)E));
+    HIPCHK(c, hipMalloc(&c->d_rng, sizeof(int4) * (size_t)ninst));
     HIPCHK(c, hipMalloc(&c->d_elem_max, sizeof(double) * 2 * (size_t)E));
     if (c->sc) {
         HIPCHK(c, hipMalloc(&c->d_alpha_tmp, sizeof(double) * (size_t)E));
         HIPCHK(c, hipMalloc(&c->d_alpha, sizeof(double) * (size_t)E));
     }
-    c->invJ0 = m->inverse_jacobian_host[0];
     if (c->flags & TLW_FLAG_KEEP_AUX) {
         HIPCHK(c, hipMalloc(&c->d_auxU, sizeof(double) * nd));
         HIPCHK(c, hipMalloc(&c->d_auxF, sizeof(double) * 2 * nd));
     }
-    HIPCHK(c, hipMemcpy(c->d_nbr, m->neighbors_host, sizeof(int) * 4 * E, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->d_invj, m->inverse_jacobian_host, sizeof(double) * E, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->d_center, m->element_center_host, sizeof(double) * 2 * E, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->d_length, m->element_length_host, sizeof(double) * E, hipMemcpyHostToDevice));
-    c->aux_valid = false;
-    return TLW_OK;
+    HIPCHK(c, hipMalloc(&c->d_state, sizeof(DevState) * ninst));
+    HIPCHK(c, hipMalloc(&c->d_cur, sizeof(int) * ninst));
+    HIPCHK(c, hipHostMalloc(&c->h_state, sizeof(DevState) * ninst, hipHostMallocDefault));
+    std::memset(c->h_state, 0, sizeof(DevState) * ninst);
+    for (int k = 0; k < ninst; ++k) c->h_state[k].invJ0 = ms[k].inverse_jacobian_host[0];
+    HIPCHK(c, hipMemcpy(c->d_state, c->h_state, sizeof(DevState) * ninst, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_nbr, nbr.data(), sizeof(int) * 4 * E, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_invj, invj.data(), sizeof(double) * E, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_center, center.data(), sizeof(double) * 2 * E, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_length, length.data(), sizeof(double) * E, hipMemcpyHostToDevice));
+    return build_tables(c);
 }
 
 KArgs base_args(tlw_ctx* c) {
@@ -174,36 +283,20 @@ KArgs base_args(tlw_ctx* c) {
     a.st = c->d_state;
     a.partial = c->d_partial;
     a.elem_max = c->d_elem_max;
-    a.vol_grid = vgrid(c);
+    a.vol_grid = c->vgrid;
     a.ebv = c->ebv;
+    a.vblk = c->d_vblk;
+    a.sblk = c->d_sblk;
+    a.etile = c->d_etile;
     a.alpha_tmp = c->d_alpha_tmp;
     a.alpha_out = c->d_alpha;
-    a.invJ0 = c->invJ0;
     a.bc_scale = 1.0;
     return a;
 }
 
-// Elements per volume workgroup: the compiled tile sizes, selectable at run
-// time (tlw_set_elements_per_workgroup, tlw_autotune).  polydeg 4 (25 nodes):
-// 5, 10, 15 elements = 125, 250, 375 lanes of 128, 256, 384; polydeg 3
-// (16 nodes): 4, 8, 16 elements = 64, 128, 256 lanes, no idle lanes.
-constexpr int kEbChoices4[3] = {5, 10, 15};
-constexpr int kEbChoices3[3] = {4, 8, 16};
-const int* eb_choices(int N) { return N == 4 ? kEbChoices4 : kEbChoices3; }
-int default_eb(int N) { return N == 4 ? 15 : 16; }
-int min_eb(int N) { return N == 4 ? 5 : 4; }
-bool eb_supported(int N, int eb) {
-    const int* ch = eb_choices(N);
-    return eb == ch[0] || eb == ch[1] || eb == ch[2];
-}
-
-template <int N> int sgrid_of(const tlw_ctx* c) { return (c->E + SGeo<N>::EB - 1) / SGeo<N>::EB; }
-int vgrid(const tlw_ctx* c) { return (c->E + c->ebv - 1) / c->ebv; }
-int sgrid(const tlw_ctx* c) { return c->N == 4 ? sgrid_of<4>(c) : sgrid_of<3>(c); }
-
 // Kernel launches, dispatched on the polynomial degree and the tile size.
 template <int N, int EB, bool SC> void launch_volume_eb(tlw_ctx* c, const KArgs& a, int kind) {
-    dim3 g((c->E + EB - 1) / EB), b(VGeo<N, EB>::NT);
+    dim3 g(c->vgrid), b(VGeo<N, EB>::NT);
     if (kind == VOL_LW) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_LW, SC>), g, b, 0, c->stream, a);
     else if (kind == VOL_MDRK1)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_MDRK1, SC>), g, b, 0, c->stream, a);
@@ -218,7 +311,7 @@ template <int N, bool SC> void launch_volume_sc(tlw_ctx* c, const KArgs& a, int 
 template <int N> void launch_volume_t(tlw_ctx* c, const KArgs& a, int kind) {
     if (c->sc) {
         // IndicatorHennemannGassner on uⁿ before the volume integral (dg_2d.jl:1665)
-        hipLaunchKernelGGL(k_indicator<N>, dim3(sgrid_of<N>(c)), dim3(kSurfBlock), 0, c->stream, a, c->d_alpha_tmp);
+        hipLaunchKernelGGL(k_indicator<N>, dim3(c->sgrid), dim3(kSurfBlock), 0, c->stream, a, c->d_alpha_tmp);
         launch_volume_sc<N, true>(c, a, kind);
     } else {
         launch_volume_sc<N, false>(c, a, kind);
@@ -228,12 +321,16 @@ void launch_volume(tlw_ctx* c, const KArgs& a, int kind) {
     if (c->N == 4) launch_volume_t<4>(c, a, kind);
     else launch_volume_t<3>(c, a, kind);
 }
-template <int N> void launch_finalize_t(tlw_ctx* c, int reduce, int nparts) {
-    hipLaunchKernelGGL(k_finalize<N>, dim3(1), dim3(kFinalizeBlock), 0, c->stream, c->d_state, c->d_partial, nparts,
-                       reduce);
+void launch_finalize(tlw_ctx* c, int reduce, int use_vol) {
+    if (c->N == 4)
+        hipLaunchKernelGGL(k_finalize<4>, dim3(c->ninst), dim3(kFinalizeBlock), 0, c->stream, c->d_state,
+                           c->d_partial, c->d_rng, use_vol, reduce);
+    else
+        hipLaunchKernelGGL(k_finalize<3>, dim3(c->ninst), dim3(kFinalizeBlock), 0, c->stream, c->d_state,
+                           c->d_partial, c->d_rng, use_vol, reduce);
 }
 template <int N> void launch_surface_t(tlw_ctx* c, const KArgs& a) {
-    dim3 g(sgrid_of<N>(c)), b(kSurfBlock);
+    dim3 g(c->sgrid), b(kSurfBlock);
     if (c->sc) {
         if (c->has_boundary) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_surface<N, true, true>), g, b, 0, c->stream, a);
         else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_surface<N, false, true>), g, b, 0, c->stream, a);
@@ -241,27 +338,21 @@ template <int N> void launch_surface_t(tlw_ctx* c, const KArgs& a) {
         if (c->has_boundary) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_surface<N, true, false>), g, b, 0, c->stream, a);
         else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_surface<N, false, false>), g, b, 0, c->stream, a);
     }
-    if (a.reduce != RED_NONE) launch_finalize_t<N>(c, a.reduce, vgrid(c) + sgrid_of<N>(c));
+    if (a.reduce != RED_NONE) launch_finalize(c, a.reduce, 1);
 }
 void launch_surface(tlw_ctx* c, const KArgs& a) {
     if (c->N == 4) launch_surface_t<4>(c, a);
     else launch_surface_t<3>(c, a);
 }
-template <int N> void launch_limiter_t(tlw_ctx* c, const KArgs& a) {
-    hipLaunchKernelGGL(k_limiter<N>, dim3(sgrid_of<N>(c)), dim3(kSurfBlock), 0, c->stream, a);
-    if (a.reduce != RED_NONE) launch_finalize_t<N>(c, a.reduce, sgrid_of<N>(c));
-}
 void launch_limiter(tlw_ctx* c, const KArgs& a) {
-    if (c->N == 4) launch_limiter_t<4>(c, a);
-    else launch_limiter_t<3>(c, a);
-}
-template <int N> void launch_max_speed_t(tlw_ctx* c, const KArgs& a) {
-    hipLaunchKernelGGL(k_max_speed<N>, dim3(sgrid_of<N>(c)), dim3(kSurfBlock), 0, c->stream, a);
-    launch_finalize_t<N>(c, a.reduce, sgrid_of<N>(c));
+    if (c->N == 4) hipLaunchKernelGGL(k_limiter<4>, dim3(c->sgrid), dim3(kSurfBlock), 0, c->stream, a);
+    else hipLaunchKernelGGL(k_limiter<3>, dim3(c->sgrid), dim3(kSurfBlock), 0, c->stream, a);
+    if (a.reduce != RED_NONE) launch_finalize(c, a.reduce, 0);
 }
 void launch_max_speed(tlw_ctx* c, const KArgs& a) {
-    if (c->N == 4) launch_max_speed_t<4>(c, a);
-    else launch_max_speed_t<3>(c, a);
+    if (c->N == 4) hipLaunchKernelGGL(k_max_speed<4>, dim3(c->sgrid), dim3(kSurfBlock), 0, c->stream, a);
+    else hipLaunchKernelGGL(k_max_speed<3>, dim3(c->sgrid), dim3(kSurfBlock), 0, c->stream, a);
+    launch_finalize(c, a.reduce, 0);
 }
 
 int check_launch(tlw_ctx* c) {
@@ -342,15 +433,478 @@ int rhs_common(tlw_ctx* c, int kind, const double* u, double* du, double t, doub
     return TLW_OK;
 }
 
+// The launches of one step of the device loop (solve_lwfr's perform_step! with
+// compute_dt of the next step folded into the reductions).
+void enqueue_step(tlw_ctx* c) {
+    KArgs a = base_args(c);
+    a.use_state_dt = 1;
+    a.step_mode = 1;
+    // Without a limiter the CFL reduction of uⁿ⁺¹ is split between the volume
+    // kernel (nodes it finalises) and the surface kernel (the rest); with the
+    // positivity limiter (applied after the update, lwfr.jl:226) the limiter
+    // kernel takes it over the limited state.
+    const int red = c->positivity ? RED_NONE : RED_STEP;
+    if (c->time_disc == TLW_LW) {
+        KArgs v = a;
+        v.in_sel = SEL_CUR;
+        v.out_sel = SEL_OTHER;
+        v.reduce = red;
+        timed(c, 0, [&] { launch_volume(c, v, VOL_LW); });
+        KArgs sf = a;
+        sf.out_sel = SEL_OTHER;
+        sf.reduce = red;
+        timed(c, 1, [&] { launch_surface(c, sf); });
+    } else {
+        // stage 1: us = uⁿ + dt rhs_mdrk1(uⁿ)  (mdrk.jl:13-17)
+        KArgs v1 = a;
+        v1.in_sel = SEL_CUR;
+        v1.out_sel = SEL_EXPLICIT;
+        v1.out = c->d_us;
+        v1.reduce = RED_NONE;
+        timed(c, 0, [&] { launch_volume(c, v1, VOL_MDRK1); });
+        KArgs s1 = v1;
+        s1.bc_scale = 0.5;
+        timed(c, 1, [&] { launch_surface(c, s1); });
+        // stage 2: uⁿ⁺¹ = uⁿ + dt rhs_mdrk2(uⁿ, us)  (mdrk.jl:21-24)
+        KArgs v2 = a;
+        v2.in_sel = SEL_CUR;
+        v2.out_sel = SEL_OTHER;
+        v2.us = c->d_us;
+        v2.reduce = red;
+        timed(c, 0, [&] { launch_volume(c, v2, VOL_MDRK2); });
+        KArgs s2 = v2;
+        timed(c, 1, [&] { launch_surface(c, s2); });
+    }
+    if (c->positivity) {
+        KArgs l = a;
+        l.out_sel = SEL_OTHER;
+        l.reduce = RED_STEP;
+        timed(c, 2, [&] { launch_limiter(c, l); });
+    }
+}
+
+// Replays max_steps steps from a graph captured on first use for this step
+// count and stream.  The captured launches contain no memory sets.
+int advance_graph(tlw_ctx* c, int max_steps) {
+    if (!c->stream) return fail(c, TLW_EINVAL, "graph replay needs an explicit stream (tlw_set_stream)");
+    if (c->graph_stream != c->stream) {
+        drop_graphs(c);
+        c->graph_stream = c->stream;
+    }
+    auto it = c->graph_cache.find(max_steps);
+    if (it == c->graph_cache.end()) {
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            for (int k = 0; k < max_steps; ++k) enqueue_step(c);
+            e = hipStreamEndCapture(c->stream, &g);
+        }
+        hipGraphExec_t ge = nullptr;
+        if (e == hipSuccess) e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        if (g) (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return fail(c, TLW_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
+        it = c->graph_cache.emplace(max_steps, ge).first;
+    }
+    HIPCHK(c, hipGraphLaunch(it->second, c->stream));
+    return TLW_OK;
+}
+
+int sync_state(tlw_ctx* c) {
+    HIPCHK(c, hipMemcpyAsync(c->h_state, c->d_state, sizeof(DevState) * c->ninst, hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return TLW_OK;
+}
+
+bool any_bad(const tlw_ctx* c) {
+    for (int k = 0; k < c->ninst; ++k)
+        if (c->h_state[k].bad_state) return true;
+    return false;
+}
+
+int create_common(int ninst, const tlw_mesh_desc* meshes, const tlw_basis_desc* basis, const tlw_solver_desc* solver,
+                  tlw_ctx** out);
+
 }  // namespace
 
 extern "C" {
 
 int tlw_create(const tlw_mesh_desc* mesh, const tlw_basis_desc* basis, const tlw_solver_desc* solver,
                tlw_ctx** out) {
+    return create_common(1, mesh, basis, solver, out);
+}
+
+int tlw_create_batch(int32_t n_instances, const tlw_mesh_desc* meshes, const tlw_basis_desc* basis,
+                     const tlw_solver_desc* solver, tlw_ctx** out) {
+    if (n_instances < 1 || !meshes) {
+        if (out) *out = nullptr;
+        return TLW_EINVAL;
+    }
+    return create_common(n_instances, meshes, basis, solver, out);
+}
+
+int tlw_n_instances(const tlw_ctx* c, int32_t* n) {
+    if (!c || !n) return TLW_EINVAL;
+    *n = c->ninst;
+    return TLW_OK;
+}
+
+int tlw_rebuild_mesh(tlw_ctx* c, const tlw_mesh_desc* mesh) {
+    if (!c) return TLW_EINVAL;
+    if (c->ninst != 1) return fail(c, TLW_EINVAL, "tlw_rebuild_mesh needs a single-instance context");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return upload_meshes(c, mesh, 1);
+}
+
+void tlw_destroy(tlw_ctx* c) {
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    else (void)hipDeviceSynchronize();
+    free_mesh(c);
+    for (auto e : c->ev_pool) (void)hipEventDestroy(e);
+    (void)hipFree(c->d_const);
+    delete c;
+}
+
+const char* tlw_last_error(const tlw_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int tlw_set_stream(tlw_ctx* c, void* stream) {
+    if (!c) return TLW_EINVAL;
+    c->stream = static_cast<hipStream_t>(stream);
+    return TLW_OK;
+}
+
+size_t tlw_state_len(const tlw_ctx* c) { return c ? state_len(c) : 0; }
+
+int tlw_rhs_lw(tlw_ctx* c, const double* u, double* du, double t, double dt) {
+    return rhs_common(c, VOL_LW, u, du, t, dt);
+}
+int tlw_rhs_mdrk1(tlw_ctx* c, const double* u, double* du, double t, double dt) {
+    return rhs_common(c, VOL_MDRK1, u, du, t, dt);
+}
+int tlw_rhs_mdrk2(tlw_ctx* c, const double* u, double* du, double t, double dt) {
+    return rhs_common(c, VOL_MDRK2, u, du, t, dt);
+}
+
+int tlw_us_ptr(tlw_ctx* c, double** us) {
+    if (!c || !us) return TLW_EINVAL;
+    *us = c->d_us;
+    return TLW_OK;
+}
+
+int tlw_set_us(tlw_ctx* c, const double* us) {
+    if (!c || !us) return fail(c, TLW_EINVAL, "null pointer");
+    HIPCHK(c, hipMemcpyAsync(c->d_us, us, sizeof(double) * state_len(c), hipMemcpyDeviceToDevice, c->stream));
+    return TLW_OK;
+}
+
+int tlw_max_dt(tlw_ctx* c, const double* u, double* max_dt) {
+    if (!c || !u || !max_dt) return fail(c, TLW_EINVAL, "null pointer");
+    hipLaunchKernelGGL(k_clear_bad_state, dim3((c->ninst + 255) / 256), dim3(256), 0, c->stream, c->d_state,
+                       c->ninst);
+    KArgs a = base_args(c);
+    a.u = u;
+    a.in_sel = SEL_EXPLICIT;
+    a.reduce = RED_QUERY;
+    launch_max_speed(c, a);
+    int rc = check_launch(c);
+    if (rc) return rc;
+    rc = sync_state(c);
+    if (rc) return rc;
+    // a batch answers with the smallest of its instances' values
+    double m = c->h_state[0].query;
+    for (int k = 1; k < c->ninst; ++k) m = std::min(m, c->h_state[k].query);
+    *max_dt = m;
+    if (any_bad(c)) return fail(c, TLW_EADMISSIBLE, "non-admissible state (rho <= 0 or p <= 0)");
+    return TLW_OK;
+}
+
+int tlw_set_state(tlw_ctx* c, const double* u, size_t len) {
+    if (!c || !u) return fail(c, TLW_EINVAL, "null pointer");
+    if (len != state_len(c)) return fail(c, TLW_EINVAL, "state length mismatch");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int k = 0; k < c->ninst; ++k)
+        HIPCHK(c, hipMemcpy(c->d_buf[c->host_cur[k]] + inst_offset(c, k), u + inst_offset(c, k),
+                            sizeof(double) * inst_len(c, k), hipMemcpyHostToDevice));
+    return TLW_OK;
+}
+
+int tlw_get_state(tlw_ctx* c, double* u, size_t len) {
+    if (!c || !u) return fail(c, TLW_EINVAL, "null pointer");
+    if (len != state_len(c)) return fail(c, TLW_EINVAL, "state length mismatch");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int k = 0; k < c->ninst; ++k)
+        HIPCHK(c, hipMemcpy(u + inst_offset(c, k), c->d_buf[c->host_cur[k]] + inst_offset(c, k),
+                            sizeof(double) * inst_len(c, k), hipMemcpyDeviceToHost));
+    return TLW_OK;
+}
+
+int tlw_state_ptr(tlw_ctx* c, double** u) {
+    if (!c || !u) return TLW_EINVAL;
+    for (int k = 1; k < c->ninst; ++k)
+        if (c->host_cur[k] != c->host_cur[0])
+            return fail(c, TLW_EINVAL, "the instances' states are in different buffers (use tlw_get_state)");
+    *u = c->d_buf[c->host_cur[0]];
+    return TLW_OK;
+}
+
+int tlw_solve_init(tlw_ctx* c, double t0, double t_final, double cfl) {
+    if (!c) return TLW_EINVAL;
+    if (!std::isfinite(t0) || std::isnan(t_final) || !(cfl > 0.0))
+        return fail(c, TLW_EINVAL, "t0 finite, t_final not NaN and cfl > 0 required");
+    HIPCHK(c, hipMemcpyAsync(c->d_cur, c->host_cur.data(), sizeof(int) * c->ninst, hipMemcpyHostToDevice,
+                             c->stream));
+    hipLaunchKernelGGL(k_init_state, dim3((c->ninst + 255) / 256), dim3(256), 0, c->stream, c->d_state, c->ninst, t0,
+                       t_final, cfl, c->d_cur);
+    KArgs a = base_args(c);
+    a.in_sel = SEL_CUR;
+    a.reduce = RED_INIT;
+    launch_max_speed(c, a);
+    int rc = check_launch(c);
+    if (rc) return rc;
+    rc = sync_state(c);
+    if (rc) return rc;
+    if (any_bad(c)) return fail(c, TLW_EADMISSIBLE, "non-admissible initial state");
+    return TLW_OK;
+}
+
+int tlw_advance(tlw_ctx* c, int32_t max_steps, double* t_host, double* dt_host, int64_t* iter_host,
+                int32_t* finished_host) {
+    if (!c) return TLW_EINVAL;
+    if (max_steps < 0) return fail(c, TLW_EINVAL, "max_steps < 0");
+    if (max_steps > 0) {
+        if (c->graphs && !c->timing) {
+            int rc = advance_graph(c, max_steps);
+            if (rc) return rc;
+        } else {
+            for (int s = 0; s < max_steps; ++s) enqueue_step(c);
+        }
+    }
+    int rc = check_launch(c);
+    if (rc) return rc;
+    rc = sync_state(c);
+    if (rc) return rc;
+    if (c->timing) collect_times(c);
+    for (int k = 0; k < c->ninst; ++k) c->host_cur[k] = c->h_state[k].cur;
+    const DevState& s = c->h_state[0];
+    if (t_host) *t_host = s.t;
+    if (dt_host) *dt_host = s.dt;
+    if (iter_host) *iter_host = s.iter;
+    if (finished_host) {
+        int all = 1;
+        for (int k = 0; k < c->ninst; ++k) all = all && c->h_state[k].finished;
+        *finished_host = all;
+    }
+    if (max_steps > 0) c->alpha_valid = c->sc;
+    if (any_bad(c)) return fail(c, TLW_EADMISSIBLE, "non-admissible state (rho <= 0 or p <= 0)");
+    return TLW_OK;
+}
+
+int tlw_batch_status(tlw_ctx* c, int32_t n, double* t, double* dt, int64_t* iter, int32_t* finished) {
+    if (!c) return TLW_EINVAL;
+    if (n != c->ninst) return fail(c, TLW_EINVAL, "n must equal the number of instances");
+    int rc = sync_state(c);
+    if (rc) return rc;
+    for (int k = 0; k < n; ++k) {
+        if (t) t[k] = c->h_state[k].t;
+        if (dt) dt[k] = c->h_state[k].dt;
+        if (iter) iter[k] = c->h_state[k].iter;
+        if (finished) finished[k] = c->h_state[k].finished;
+    }
+    return TLW_OK;
+}
+
+int tlw_enable_graphs(tlw_ctx* c, int32_t enable) {
+    if (!c) return TLW_EINVAL;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->graphs = enable != 0;
+    if (!c->graphs) drop_graphs(c);
+    return TLW_OK;
+}
+
+int tlw_enable_kernel_timing(tlw_ctx* c, int32_t enable) {
+    if (!c) return TLW_EINVAL;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    collect_times(c);
+    c->timing = enable != 0;
+    return TLW_OK;
+}
+
+int tlw_kernel_times(tlw_ctx* c, double* vol_ms, int64_t* vol_n, double* surf_ms, int64_t* surf_n) {
+    if (!c) return TLW_EINVAL;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    collect_times(c);
+    if (vol_ms) *vol_ms = c->t_ms[0];
+    if (vol_n) *vol_n = c->t_n[0];
+    if (surf_ms) *surf_ms = c->t_ms[1];
+    if (surf_n) *surf_n = c->t_n[1];
+    c->t_ms[0] = c->t_ms[1] = c->t_ms[2] = 0.0;
+    c->t_n[0] = c->t_n[1] = c->t_n[2] = 0;
+    return TLW_OK;
+}
+
+int tlw_adjoint_lw(tlw_ctx* c, const double* u, const double* lam_next, double* lam, double t, double dt) {
+    if (!c) return TLW_EINVAL;
+    if (!u || !lam_next || !lam) return fail(c, TLW_EINVAL, "null state pointer");
+    if (!std::isfinite(dt) || !std::isfinite(t)) return fail(c, TLW_EINVAL, "t and dt must be finite");
+    if (c->time_disc != TLW_LW || c->sc || c->has_boundary || c->hconst.flux_order != TLW_FLUX_TREE ||
+        c->ninst != 1)
+        return fail(c, TLW_ENOTSUP, "the adjoint covers plain LW on one periodic mesh with the TreeMesh flux order");
+    if (lam == lam_next || lam == u) return fail(c, TLW_EINVAL, "lam must not alias lam_next or u");
+    KArgs a = base_args(c);
+    a.u = u;
+    a.us = lam_next;
+    a.out = lam;
+    a.t_arg = t;
+    a.dt_arg = dt;
+    if (c->N == 4) {
+        constexpr int EB = kEBA<4>;
+        dim3 g((c->E + EB - 1) / EB), b(VGeo<4, EB>::NT);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_adj_records<4, EB>), g, b, 0, c->stream, a);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_adjoint<4, EB>), g, b, 0, c->stream, a);
+    } else {
+        constexpr int EB = kEBA<3>;
+        dim3 g((c->E + EB - 1) / EB), b(VGeo<3, EB>::NT);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_adj_records<3, EB>), g, b, 0, c->stream, a);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_adjoint<3, EB>), g, b, 0, c->stream, a);
+    }
+    return check_launch(c);
+}
+
+int tlw_set_positivity_limiter(tlw_ctx* c, int32_t enable, double threshold_density, double threshold_pressure) {
+    if (!c) return TLW_EINVAL;
+    if (enable && (std::isnan(threshold_density) || std::isnan(threshold_pressure)))
+        return fail(c, TLW_EINVAL, "positivity thresholds must not be NaN");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if ((enable != 0) != c->positivity) drop_graphs(c);
+    c->positivity = enable != 0;
+    if (c->positivity) {
+        c->hconst.pos_threshold[0] = threshold_density;
+        c->hconst.pos_threshold[1] = threshold_pressure;
+        HIPCHK(c, hipMemcpy(c->d_const, &c->hconst, sizeof(DevConst), hipMemcpyHostToDevice));
+    }
+    return TLW_OK;
+}
+
+int tlw_apply_positivity_limiter(tlw_ctx* c, double* u) {
+    if (!c || !u) return fail(c, TLW_EINVAL, "null pointer");
+    if (!c->positivity) return fail(c, TLW_EINVAL, "positivity limiter not configured (tlw_set_positivity_limiter)");
+    KArgs a = base_args(c);
+    a.out = u;
+    a.out_sel = SEL_EXPLICIT;
+    a.reduce = RED_NONE;
+    launch_limiter(c, a);
+    return check_launch(c);
+}
+
+int tlw_set_elements_per_workgroup(tlw_ctx* c, int32_t elements) {
+    if (!c) return TLW_EINVAL;
+    if (!eb_supported(c->N, elements))
+        return fail(c, TLW_EINVAL, c->N == 4 ? "elements per workgroup must be 5, 10 or 15 for polydeg 4"
+                                             : "elements per workgroup must be 4, 8 or 16 for polydeg 3");
+    if (elements == c->ebv) return TLW_OK;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->ebv = elements;
+    return build_tables(c);
+}
+
+int tlw_get_elements_per_workgroup(const tlw_ctx* c, int32_t* elements) {
+    if (!c || !elements) return TLW_EINVAL;
+    *elements = c->ebv;
+    return TLW_OK;
+}
+
+int tlw_autotune(tlw_ctx* c, int32_t reps, int32_t* chosen, double* ms_out) {
+    if (!c) return TLW_EINVAL;
+    if (reps < 1) return fail(c, TLW_EINVAL, "reps >= 1 required");
+    for (int k = 1; k < c->ninst; ++k)
+        if (c->host_cur[k] != c->host_cur[0])
+            return fail(c, TLW_EINVAL, "the instances' states are in different buffers");
+    // one rhs evaluation of the context's scheme (LW, or MDRK stage 2, the
+    // heavier stage) on the current state buffer, into the other buffer
+    const int kind = c->time_disc == TLW_LW ? VOL_LW : VOL_MDRK2;
+    const double* u = c->d_buf[c->host_cur[0]];
+    double* scratch = c->d_buf[1 - c->host_cur[0]];
+    if (kind == VOL_MDRK2)
+        HIPCHK(c, hipMemcpyAsync(c->d_us, u, sizeof(double) * state_len(c), hipMemcpyDeviceToDevice, c->stream));
+    hipEvent_t e0, e1;
+    HIPCHK(c, hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return fail(c, TLW_EHIP, "hipEventCreate failed");
+    }
+    const int saved = c->ebv;
+    const double dt = 1e-6;  // any finite step: the work does not depend on it
+    int best = saved;
+    double best_ms = 0.0;
+    int rc = TLW_OK;
+    for (int k = 0; k < 3 && rc == TLW_OK; ++k) {
+        rc = tlw_set_elements_per_workgroup(c, eb_choices(c->N)[k]);
+        if (rc) break;
+        rc = rhs_common(c, kind, u, scratch, 0.0, dt);  // warm-up
+        if (rc) break;
+        if (hipEventRecord(e0, c->stream) != hipSuccess) {
+            rc = fail(c, TLW_EHIP, "hipEventRecord");
+            break;
+        }
+        for (int r = 0; r < reps && rc == TLW_OK; ++r) rc = rhs_common(c, kind, u, scratch, 0.0, dt);
+        if (rc) break;
+        float ms = 0.f;
+        if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+            rc = fail(c, TLW_EHIP, "event timing failed");
+            break;
+        }
+        const double per = ms / reps;
+        if (ms_out) ms_out[k] = per;
+        if (k == 0 || per < best_ms) {
+            best_ms = per;
+            best = c->ebv;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    const int rc2 = tlw_set_elements_per_workgroup(c, rc == TLW_OK ? best : saved);
+    c->aux_valid = false;
+    c->alpha_valid = false;
+    if (chosen) *chosen = c->ebv;
+    return rc ? rc : rc2;
+}
+
+int tlw_get_aux(tlw_ctx* c, int32_t which, double* out, size_t len) {
+    if (!c || !out) return fail(c, TLW_EINVAL, "null pointer");
+    const size_t nd = state_len(c);
+    const double* src = nullptr;
+    size_t need = 0;
+    if (which == TLW_AUX_ALPHA) {
+        if (!c->sc) return fail(c, TLW_EINVAL, "context created without shock capturing");
+        if (!c->alpha_valid) return fail(c, TLW_EINVAL, "no rhs evaluated or step taken yet");
+        src = c->d_alpha;
+        need = (size_t)c->E;
+    } else if (which == TLW_AUX_U || which == TLW_AUX_F) {
+        if (!(c->flags & TLW_FLAG_KEEP_AUX))
+            return fail(c, TLW_EINVAL, "context created without TLW_FLAG_KEEP_AUX");
+        if (!c->aux_valid) return fail(c, TLW_EINVAL, "no rhs evaluated yet");
+        src = which == TLW_AUX_U ? c->d_auxU : c->d_auxF;
+        need = which == TLW_AUX_U ? nd : 2 * nd;
+    } else {
+        return fail(c, TLW_EINVAL, "unknown aux selector");
+    }
+    if (len != need) return fail(c, TLW_EINVAL, "aux length mismatch");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(out, src, sizeof(double) * len, hipMemcpyDeviceToHost));
+    return TLW_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+int create_common(int ninst, const tlw_mesh_desc* meshes, const tlw_basis_desc* basis, const tlw_solver_desc* solver,
+                  tlw_ctx** out) {
     if (!out) return TLW_EINVAL;
     *out = nullptr;
-    if (!mesh || !basis || !solver) return TLW_EINVAL;
+    if (!meshes || !basis || !solver) return TLW_EINVAL;
     if (basis->polydeg != 3 && basis->polydeg != 4) return TLW_ENOTSUP;
     if (!basis->nodes_host || !basis->weights_host || !basis->derivative_matrix_host || !basis->derivative_dhat_host)
         return TLW_EINVAL;
@@ -394,366 +948,16 @@ int tlw_create(const tlw_mesh_desc* mesh, const tlw_basis_desc* basis, const tlw
         h.alpha_min = solver->alpha_min;
         h.alpha_smooth = solver->alpha_smooth;
     }
-    // every domain-boundary face needs a boundary value function
-    bool has_boundary = false;
-    for (size_t k = 0; k < (size_t)4 * std::max(mesh->n_elements, 0) && mesh->neighbors_host; ++k)
-        if (mesh->neighbors_host[k] < 0) has_boundary = true;
-    if (has_boundary && h.bc_kind == TLW_BV_NONE) {
-        delete c;
-        return TLW_EINVAL;
-    }
     auto bail = [&](int rc) {
         tlw_destroy(c);
         return rc;
     };
     if (hipMalloc(&c->d_const, sizeof(DevConst)) != hipSuccess) return bail(TLW_EHIP);
-    if (hipMalloc(&c->d_state, sizeof(DevState)) != hipSuccess) return bail(TLW_EHIP);
-    if (hipHostMalloc(&c->h_state, sizeof(DevState), hipHostMallocDefault) != hipSuccess) return bail(TLW_EHIP);
     if (hipMemcpy(c->d_const, &h, sizeof(DevConst), hipMemcpyHostToDevice) != hipSuccess) return bail(TLW_EHIP);
-    std::memset(c->h_state, 0, sizeof(DevState));
-    if (hipMemcpy(c->d_state, c->h_state, sizeof(DevState), hipMemcpyHostToDevice) != hipSuccess)
-        return bail(TLW_EHIP);
-    int rc = upload_mesh(c, mesh);
+    int rc = upload_meshes(c, meshes, ninst);
     if (rc) return bail(rc);
     *out = c;
     return TLW_OK;
 }
 
-int tlw_rebuild_mesh(tlw_ctx* c, const tlw_mesh_desc* mesh) {
-    if (!c) return TLW_EINVAL;
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    return upload_mesh(c, mesh);
-}
-
-void tlw_destroy(tlw_ctx* c) {
-    if (!c) return;
-    if (c->stream) (void)hipStreamSynchronize(c->stream);
-    else (void)hipDeviceSynchronize();
-    free_mesh(c);
-    for (auto e : c->ev_pool) (void)hipEventDestroy(e);
-    (void)hipFree(c->d_const);
-    (void)hipFree(c->d_state);
-    if (c->h_state) (void)hipHostFree(c->h_state);
-    delete c;
-}
-
-const char* tlw_last_error(const tlw_ctx* c) { return c ? c->err.c_str() : "null context"; }
-
-int tlw_set_stream(tlw_ctx* c, void* stream) {
-    if (!c) return TLW_EINVAL;
-    c->stream = static_cast<hipStream_t>(stream);
-    return TLW_OK;
-}
-
-size_t tlw_state_len(const tlw_ctx* c) { return c ? state_len(c) : 0; }
-
-int tlw_rhs_lw(tlw_ctx* c, const double* u, double* du, double t, double dt) {
-    return rhs_common(c, VOL_LW, u, du, t, dt);
-}
-int tlw_rhs_mdrk1(tlw_ctx* c, const double* u, double* du, double t, double dt) {
-    return rhs_common(c, VOL_MDRK1, u, du, t, dt);
-}
-int tlw_rhs_mdrk2(tlw_ctx* c, const double* u, double* du, double t, double dt) {
-    return rhs_common(c, VOL_MDRK2, u, du, t, dt);
-}
-
-int tlw_us_ptr(tlw_ctx* c, double** us) {
-    if (!c || !us) return TLW_EINVAL;
-    *us = c->d_us;
-    return TLW_OK;
-}
-
-int tlw_set_us(tlw_ctx* c, const double* us) {
-    if (!c || !us) return fail(c, TLW_EINVAL, "null pointer");
-    HIPCHK(c, hipMemcpyAsync(c->d_us, us, sizeof(double) * state_len(c), hipMemcpyDeviceToDevice, c->stream));
-    return TLW_OK;
-}
-
-int tlw_max_dt(tlw_ctx* c, const double* u, double* max_dt) {
-    if (!c || !u || !max_dt) return fail(c, TLW_EINVAL, "null pointer");
-    hipLaunchKernelGGL(k_clear_bad_state, dim3(1), dim3(64), 0, c->stream, c->d_state);
-    KArgs a = base_args(c);
-    a.u = u;
-    a.in_sel = SEL_EXPLICIT;
-    a.reduce = RED_QUERY;
-    launch_max_speed(c, a);
-    int rc = check_launch(c);
-    if (rc) return rc;
-    HIPCHK(c, hipMemcpyAsync(c->h_state, c->d_state, sizeof(DevState), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    *max_dt = c->h_state->query;
-    if (c->h_state->bad_state) return fail(c, TLW_EADMISSIBLE, "non-admissible state (rho <= 0 or p <= 0)");
-    return TLW_OK;
-}
-
-int tlw_set_state(tlw_ctx* c, const double* u, size_t len) {
-    if (!c || !u) return fail(c, TLW_EINVAL, "null pointer");
-    if (len != state_len(c)) return fail(c, TLW_EINVAL, "state length mismatch");
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    HIPCHK(c, hipMemcpy(c->d_buf[c->host_cur], u, sizeof(double) * len, hipMemcpyHostToDevice));
-    return TLW_OK;
-}
-
-int tlw_get_state(tlw_ctx* c, double* u, size_t len) {
-    if (!c || !u) return fail(c, TLW_EINVAL, "null pointer");
-    if (len != state_len(c)) return fail(c, TLW_EINVAL, "state length mismatch");
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    HIPCHK(c, hipMemcpy(u, c->d_buf[c->host_cur], sizeof(double) * len, hipMemcpyDeviceToHost));
-    return TLW_OK;
-}
-
-int tlw_state_ptr(tlw_ctx* c, double** u) {
-    if (!c || !u) return TLW_EINVAL;
-    *u = c->d_buf[c->host_cur];
-    return TLW_OK;
-}
-
-int tlw_solve_init(tlw_ctx* c, double t0, double t_final, double cfl) {
-    if (!c) return TLW_EINVAL;
-    if (!std::isfinite(t0) || std::isnan(t_final) || !(cfl > 0.0))
-        return fail(c, TLW_EINVAL, "t0 finite, t_final not NaN and cfl > 0 required");
-    hipLaunchKernelGGL(k_init_state, dim3(1), dim3(64), 0, c->stream, c->d_state, t0, t_final, cfl, c->host_cur);
-    KArgs a = base_args(c);
-    a.in_sel = SEL_CUR;
-    a.reduce = RED_INIT;
-    launch_max_speed(c, a);
-    int rc = check_launch(c);
-    if (rc) return rc;
-    HIPCHK(c, hipMemcpyAsync(c->h_state, c->d_state, sizeof(DevState), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    if (c->h_state->bad_state) return fail(c, TLW_EADMISSIBLE, "non-admissible initial state");
-    return TLW_OK;
-}
-
-int tlw_advance(tlw_ctx* c, int32_t max_steps, double* t_host, double* dt_host, int64_t* iter_host,
-                int32_t* finished_host) {
-    if (!c) return TLW_EINVAL;
-    if (max_steps < 0) return fail(c, TLW_EINVAL, "max_steps < 0");
-    KArgs a = base_args(c);
-    a.use_state_dt = 1;
-    a.step_mode = 1;
-    for (int s = 0; s < max_steps; ++s) {
-        // Without a limiter the CFL reduction of uⁿ⁺¹ is split between the
-        // volume kernel (nodes it finalises) and the surface kernel (the rest);
-        // with the positivity limiter (applied after the update, lwfr.jl:226)
-        // the limiter kernel takes it over the limited state.
-        const int red = c->positivity ? RED_NONE : RED_STEP;
-        if (c->time_disc == TLW_LW) {
-            KArgs v = a;
-            v.in_sel = SEL_CUR;
-            v.out_sel = SEL_OTHER;
-            v.reduce = red;
-            timed(c, 0, [&] { launch_volume(c, v, VOL_LW); });
-            KArgs sf = a;
-            sf.out_sel = SEL_OTHER;
-            sf.reduce = red;
-            timed(c, 1, [&] { launch_surface(c, sf); });
-        } else {
-            // stage 1: us = uⁿ + dt rhs_mdrk1(uⁿ)  (mdrk.jl:13-17)
-            KArgs v1 = a;
-            v1.in_sel = SEL_CUR;
-            v1.out_sel = SEL_EXPLICIT;
-            v1.out = c->d_us;
-            v1.reduce = RED_NONE;
-            timed(c, 0, [&] { launch_volume(c, v1, VOL_MDRK1); });
-            KArgs s1 = v1;
-            s1.bc_scale = 0.5;
-            timed(c, 1, [&] { launch_surface(c, s1); });
-            // stage 2: uⁿ⁺¹ = uⁿ + dt rhs_mdrk2(uⁿ, us)  (mdrk.jl:21-24)
-            KArgs v2 = a;
-            v2.in_sel = SEL_CUR;
-            v2.out_sel = SEL_OTHER;
-            v2.us = c->d_us;
-            v2.reduce = red;
-            timed(c, 0, [&] { launch_volume(c, v2, VOL_MDRK2); });
-            KArgs s2 = v2;
-            timed(c, 1, [&] { launch_surface(c, s2); });
-        }
-        if (c->positivity) {
-            KArgs l = a;
-            l.out_sel = SEL_OTHER;
-            l.reduce = RED_STEP;
-            timed(c, 2, [&] { launch_limiter(c, l); });
-        }
-    }
-    int rc = check_launch(c);
-    if (rc) return rc;
-    HIPCHK(c, hipMemcpyAsync(c->h_state, c->d_state, sizeof(DevState), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    if (c->timing) collect_times(c);
-    const DevState& s = *c->h_state;
-    c->host_cur = s.cur;
-    if (max_steps > 0) c->alpha_valid = c->sc;
-    if (t_host) *t_host = s.t;
-    if (dt_host) *dt_host = s.dt;
-    if (iter_host) *iter_host = s.iter;
-    if (finished_host) *finished_host = s.finished;
-    if (s.bad_state) return fail(c, TLW_EADMISSIBLE, "non-admissible state (rho <= 0 or p <= 0)");
-    return TLW_OK;
-}
-
-int tlw_enable_kernel_timing(tlw_ctx* c, int32_t enable) {
-    if (!c) return TLW_EINVAL;
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    collect_times(c);
-    c->timing = enable != 0;
-    return TLW_OK;
-}
-
-int tlw_kernel_times(tlw_ctx* c, double* vol_ms, int64_t* vol_n, double* surf_ms, int64_t* surf_n) {
-    if (!c) return TLW_EINVAL;
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    collect_times(c);
-    if (vol_ms) *vol_ms = c->t_ms[0];
-    if (vol_n) *vol_n = c->t_n[0];
-    if (surf_ms) *surf_ms = c->t_ms[1];
-    if (surf_n) *surf_n = c->t_n[1];
-    c->t_ms[0] = c->t_ms[1] = c->t_ms[2] = 0.0;
-    c->t_n[0] = c->t_n[1] = c->t_n[2] = 0;
-    return TLW_OK;
-}
-
-int tlw_adjoint_lw(tlw_ctx* c, const double* u, const double* lam_next, double* lam, double t, double dt) {
-    if (!c) return TLW_EINVAL;
-    if (!u || !lam_next || !lam) return fail(c, TLW_EINVAL, "null state pointer");
-    if (!std::isfinite(dt) || !std::isfinite(t)) return fail(c, TLW_EINVAL, "t and dt must be finite");
-    if (c->time_disc != TLW_LW || c->sc || c->has_boundary || c->hconst.flux_order != TLW_FLUX_TREE)
-        return fail(c, TLW_ENOTSUP, "the adjoint covers plain LW on a periodic mesh with the TreeMesh flux order");
-    if (lam == lam_next || lam == u) return fail(c, TLW_EINVAL, "lam must not alias lam_next or u");
-    KArgs a = base_args(c);
-    a.u = u;
-    a.us = lam_next;
-    a.out = lam;
-    a.t_arg = t;
-    a.dt_arg = dt;
-    if (c->N == 4) {
-        constexpr int EB = kEBA<4>;
-        dim3 g((c->E + EB - 1) / EB), b(VGeo<4, EB>::NT);
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_adj_records<4, EB>), g, b, 0, c->stream, a);
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_adjoint<4, EB>), g, b, 0, c->stream, a);
-    } else {
-        constexpr int EB = kEBA<3>;
-        dim3 g((c->E + EB - 1) / EB), b(VGeo<3, EB>::NT);
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_adj_records<3, EB>), g, b, 0, c->stream, a);
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_adjoint<3, EB>), g, b, 0, c->stream, a);
-    }
-    return check_launch(c);
-}
-
-int tlw_set_positivity_limiter(tlw_ctx* c, int32_t enable, double threshold_density, double threshold_pressure) {
-    if (!c) return TLW_EINVAL;
-    if (enable && (std::isnan(threshold_density) || std::isnan(threshold_pressure)))
-        return fail(c, TLW_EINVAL, "positivity thresholds must not be NaN");
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    c->positivity = enable != 0;
-    if (c->positivity) {
-        c->hconst.pos_threshold[0] = threshold_density;
-        c->hconst.pos_threshold[1] = threshold_pressure;
-        HIPCHK(c, hipMemcpy(c->d_const, &c->hconst, sizeof(DevConst), hipMemcpyHostToDevice));
-    }
-    return TLW_OK;
-}
-
-int tlw_apply_positivity_limiter(tlw_ctx* c, double* u) {
-    if (!c || !u) return fail(c, TLW_EINVAL, "null pointer");
-    if (!c->positivity) return fail(c, TLW_EINVAL, "positivity limiter not configured (tlw_set_positivity_limiter)");
-    KArgs a = base_args(c);
-    a.out = u;
-    a.out_sel = SEL_EXPLICIT;
-    a.reduce = RED_NONE;
-    launch_limiter(c, a);
-    return check_launch(c);
-}
-
-int tlw_set_elements_per_workgroup(tlw_ctx* c, int32_t elements) {
-    if (!c) return TLW_EINVAL;
-    if (!eb_supported(c->N, elements))
-        return fail(c, TLW_EINVAL, c->N == 4 ? "elements per workgroup must be 5, 10 or 15 for polydeg 4"
-                                             : "elements per workgroup must be 4, 8 or 16 for polydeg 3");
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    c->ebv = elements;
-    return TLW_OK;
-}
-
-int tlw_get_elements_per_workgroup(const tlw_ctx* c, int32_t* elements) {
-    if (!c || !elements) return TLW_EINVAL;
-    *elements = c->ebv;
-    return TLW_OK;
-}
-
-int tlw_autotune(tlw_ctx* c, int32_t reps, int32_t* chosen, double* ms_out) {
-    if (!c) return TLW_EINVAL;
-    if (reps < 1) return fail(c, TLW_EINVAL, "reps >= 1 required");
-    // one rhs evaluation of the context's scheme (LW, or MDRK stage 2, the
-    // heavier stage) on the current state buffer, into the other buffer
-    const int kind = c->time_disc == TLW_LW ? VOL_LW : VOL_MDRK2;
-    const double* u = c->d_buf[c->host_cur];
-    double* scratch = c->d_buf[1 - c->host_cur];
-    if (kind == VOL_MDRK2)
-        HIPCHK(c, hipMemcpyAsync(c->d_us, u, sizeof(double) * state_len(c), hipMemcpyDeviceToDevice, c->stream));
-    hipEvent_t e0, e1;
-    HIPCHK(c, hipEventCreate(&e0));
-    if (hipEventCreate(&e1) != hipSuccess) {
-        (void)hipEventDestroy(e0);
-        return fail(c, TLW_EHIP, "hipEventCreate failed");
-    }
-    const int saved = c->ebv;
-    const double dt = 1e-6;  // any finite step: the work does not depend on it
-    int best = saved;
-    double best_ms = 0.0;
-    int rc = TLW_OK;
-    for (int k = 0; k < 3 && rc == TLW_OK; ++k) {
-        c->ebv = eb_choices(c->N)[k];
-        rc = rhs_common(c, kind, u, scratch, 0.0, dt);  // warm-up
-        if (rc) break;
-        if (hipEventRecord(e0, c->stream) != hipSuccess) { rc = fail(c, TLW_EHIP, "hipEventRecord"); break; }
-        for (int r = 0; r < reps && rc == TLW_OK; ++r) rc = rhs_common(c, kind, u, scratch, 0.0, dt);
-        if (rc) break;
-        float ms = 0.f;
-        if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-            hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
-            rc = fail(c, TLW_EHIP, "event timing failed");
-            break;
-        }
-        const double per = ms / reps;
-        if (ms_out) ms_out[k] = per;
-        if (k == 0 || per < best_ms) {
-            best_ms = per;
-            best = c->ebv;
-        }
-    }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    c->ebv = rc == TLW_OK ? best : saved;
-    c->aux_valid = false;
-    c->alpha_valid = false;
-    if (chosen) *chosen = c->ebv;
-    return rc;
-}
-
-int tlw_get_aux(tlw_ctx* c, int32_t which, double* out, size_t len) {
-    if (!c || !out) return fail(c, TLW_EINVAL, "null pointer");
-    const size_t nd = state_len(c);
-    const double* src = nullptr;
-    size_t need = 0;
-    if (which == TLW_AUX_ALPHA) {
-        if (!c->sc) return fail(c, TLW_EINVAL, "context created without shock capturing");
-        if (!c->alpha_valid) return fail(c, TLW_EINVAL, "no rhs evaluated or step taken yet");
-        src = c->d_alpha;
-        need = (size_t)c->E;
-    } else if (which == TLW_AUX_U || which == TLW_AUX_F) {
-        if (!(c->flags & TLW_FLAG_KEEP_AUX))
-            return fail(c, TLW_EINVAL, "context created without TLW_FLAG_KEEP_AUX");
-        if (!c->aux_valid) return fail(c, TLW_EINVAL, "no rhs evaluated yet");
-        src = which == TLW_AUX_U ? c->d_auxU : c->d_auxF;
-        need = which == TLW_AUX_U ? nd : 2 * nd;
-    } else {
-        return fail(c, TLW_EINVAL, "unknown aux selector");
-    }
-    if (len != need) return fail(c, TLW_EINVAL, "aux length mismatch");
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    HIPCHK(c, hipMemcpy(out, src, sizeof(double) * len, hipMemcpyDeviceToHost));
-    return TLW_OK;
-}
-
-}  // extern "C"
+}  // namespace

@@ -75,6 +75,7 @@ struct DevConst {
 // isfinished test), updated by the finalize kernel.
 struct DevState {
     double t, dt, t_final, cfl;
+    double invJ0;   // interface inverse Jacobian of compute_alp: the instance's first element
     double query;   // Trixi max_dt result of a query reduction
     int finished;
     int cur;        // state buffer holding uⁿ
@@ -106,9 +107,14 @@ struct KArgs {
     double* elem_max;  // [E][2]: max λ1, λ2 over the nodes the volume kernel finalised
     int vol_grid;      // number of volume workgroups (offset of the surface partials)
     int ebv;           // elements per volume workgroup
+    // Workgroup tables: {first element, elements, instance, 0} per workgroup of the
+    // volume grid (vblk) and of the node-per-lane grids (sblk).  Workgroups never
+    // straddle two instances of a batch, so each reads one instance's state.
+    const int4* vblk;
+    const int4* sblk;
+    const int* etile;  // [E] volume workgroup of each element
     const double* alpha_tmp;  // indicator values before smoothing [E]
     double* alpha_out;        // smoothed blending factors [E] (optional)
-    double invJ0;             // interface inverse Jacobian of the TreeMesh cache (first element)
     double t_arg, dt_arg;
     int use_state_dt;  // 1: t, dt from DevState (device loop); 0: from t_arg/dt_arg
     int step_mode;     // 1: out = u + dt * rhs; 0: out = rhs
@@ -293,11 +299,31 @@ template <int RQ = kTraceQ> __device__ __forceinline__ void store_record(double*
     for (int k = 0; k < RQ / 2; ++k) p2[k] = make_double2(r[2 * k], r[2 * k + 1]);
 }
 
+// Buffer selection by integer arithmetic on the loaded pointers: a select
+// between fields of the argument struct would be folded into a load from a
+// computed offset, which moves the whole struct to private memory.
+__device__ __forceinline__ uintptr_t pick_addr(uintptr_t p0, uintptr_t p1, bool second) {
+    const uintptr_t m = (uintptr_t)0 - (uintptr_t)second;
+    return p0 ^ ((p0 ^ p1) & m);
+}
 __device__ __forceinline__ const double* sel_in(const KArgs& a, int cur) {
-    return a.in_sel == SEL_CUR ? a.bufs[cur] : (a.in_sel == SEL_OTHER ? a.bufs[1 - cur] : a.u);
+    const uintptr_t b0 = (uintptr_t)a.bufs[0], b1 = (uintptr_t)a.bufs[1], ue = (uintptr_t)a.u;
+    const int sel = a.in_sel;
+    const bool second = (sel == SEL_CUR) != (cur == 0);
+    return (const double*)pick_addr(pick_addr(b0, b1, second), ue, sel == SEL_EXPLICIT);
 }
 __device__ __forceinline__ double* sel_out(const KArgs& a, int cur) {
-    return a.out_sel == SEL_CUR ? a.bufs[cur] : (a.out_sel == SEL_OTHER ? a.bufs[1 - cur] : a.out);
+    const uintptr_t b0 = (uintptr_t)a.bufs[0], b1 = (uintptr_t)a.bufs[1], ue = (uintptr_t)a.out;
+    const int sel = a.out_sel;
+    const bool second = (sel == SEL_CUR) != (cur == 0);
+    return (double*)pick_addr(pick_addr(b0, b1, second), ue, sel == SEL_EXPLICIT);
+}
+
+// A workgroup-table entry, read once per workgroup into scalar registers.
+__device__ __forceinline__ int4 load_blk(const int4* t) {
+    const int4 b = t[blockIdx.x];
+    return make_int4(__builtin_amdgcn_readfirstlane(b.x), __builtin_amdgcn_readfirstlane(b.y),
+                     __builtin_amdgcn_readfirstlane(b.z), 0);
 }
 
 // Rusanov flux of one face node from the two face records (left = -side):
@@ -773,13 +799,14 @@ __global__ __launch_bounds__((VGeo<N, EB>::NT), kVolWaves<SC>) void k_volume(KAr
     double* s0 = lds;
     double* s1 = lds + 8 * S;
     double* s2 = lds + 16 * S;
-    DevState* st = a.st;
+    const int4 blk = load_blk(a.vblk);
+    DevState* st = a.st + blk.z;
     if (a.use_state_dt && st->finished) return;
     const int tid = threadIdx.x;
     const int le = tid / NN, node = tid - (tid / NN) * NN;
     const int i = node % n, j = node / n;
-    const int e0 = blockIdx.x * EB;
-    const int nel = min(EB, a.E - e0);
+    const int e0 = blk.x;
+    const int nel = blk.y;
     const bool act = le < nel;
     const int lec = le < nel ? le : nel - 1;
     const int e = e0 + lec;
@@ -924,7 +951,7 @@ __global__ __launch_bounds__((VGeo<N, EB>::NT), kVolWaves<SC>) void k_volume(KAr
                 load_record<RQ>(own, Lr);
                 load_record<RQ>(rec + (((ne - e0) * 4 + (f ^ 1)) * n + l) * RQ, Rr);
                 if constexpr (SC)
-                    face_numflux_sc(Lr, Rr, f < 2 ? 1 : 2, gamma, a.invJ0, dt, c->weights[n - 1], c->weights[0], Fs);
+                    face_numflux_sc(Lr, Rr, f < 2 ? 1 : 2, gamma, st->invJ0, dt, c->weights[n - 1], c->weights[0], Fs);
                 else
                     face_numflux(Lr, Rr, Fs);
                 // + face of ls: node (n-1, l) or (l, n-1); - face of ne: node (0, l) or (l, 0)
@@ -1065,13 +1092,14 @@ __global__ __launch_bounds__(kSurfBlock) void k_surface(KArgs a) {
     using G_ = SGeo<N>;
     constexpr int n = G_::n, NN = G_::NN, EB = G_::EB, S = G_::S;
     __shared__ double s_l1[S], s_l2[S], s_m[EB];
-    DevState* st = a.st;
+    const int4 blk = load_blk(a.sblk);
+    DevState* st = a.st + blk.z;
     if (a.use_state_dt && st->finished) return;
     const int tid = threadIdx.x;
     const int le = tid / NN, node = tid - (tid / NN) * NN;
     const int i = node % n, j = node / n;
-    const int e0 = blockIdx.x * EB;
-    const int nel = min(EB, a.E - e0);
+    const int e0 = blk.x;
+    const int nel = blk.y;
     const bool act = le < nel;
     const int lec = le < nel ? le : nel - 1;
     const int e = e0 + lec;
@@ -1081,20 +1109,19 @@ __global__ __launch_bounds__(kSurfBlock) void k_surface(KArgs a) {
     const double t = a.use_state_dt ? st->t : a.t_arg;
     const DevConst* c = a.c;
     const double invJ = a.inv_jac[e];
-    const int EBV = a.ebv;
-    const int eblk = e / EBV;
+    const int eblk = a.etile[e];
     const int fx = (i == 0) ? 0 : ((i == n - 1) ? 1 : -1);
     const int fy = (j == 0) ? 2 : ((j == n - 1) ? 3 : -1);
     bool elem_ext = false;
 #pragma unroll
     for (int f = 0; f < 4; ++f) {
         const int nb = a.nbr[4 * e + f];
-        elem_ext = elem_ext || nb < 0 || nb / EBV != eblk;
+        elem_ext = elem_ext || nb < 0 || a.etile[nb] != eblk;
     }
     const int nbx = fx >= 0 ? a.nbr[4 * e + fx] : 0;
     const int nby = fy >= 0 ? a.nbr[4 * e + fy] : 0;
-    const bool extx = fx >= 0 && (nbx < 0 || nbx / EBV != eblk);
-    const bool exty = fy >= 0 && (nby < 0 || nby / EBV != eblk);
+    const bool extx = fx >= 0 && (nbx < 0 || a.etile[nbx] != eblk);
+    const bool exty = fy >= 0 && (nby < 0 || a.etile[nby] != eblk);
     const bool any_ext = extx || exty;
     double val[4];
     if (act && any_ext) {
@@ -1110,7 +1137,7 @@ __global__ __launch_bounds__(kSurfBlock) void k_surface(KArgs a) {
             if (!BC || nb >= 0) {
                 load_record<RQ>(a.trace + (((size_t)nb * 4 + (f ^ 1)) * n + l) * RQ, oth);
                 if constexpr (SC)
-                    face_numflux_sc(Lr, Rr, f < 2 ? 1 : 2, c->gamma, a.invJ0, dt, c->weights[n - 1], c->weights[0],
+                    face_numflux_sc(Lr, Rr, f < 2 ? 1 : 2, c->gamma, st->invJ0, dt, c->weights[n - 1], c->weights[0],
                                     Fs);
                 else
                     face_numflux(Lr, Rr, Fs);
@@ -1172,20 +1199,22 @@ __global__ __launch_bounds__(kSurfBlock) void k_max_speed(KArgs a) {
     using G_ = SGeo<N>;
     constexpr int NN = G_::NN, EB = G_::EB, S = G_::S;
     __shared__ double s_l1[S], s_l2[S], s_m[EB];
+    const int4 blk = load_blk(a.sblk);
+    DevState* st = a.st + blk.z;
     const int tid = threadIdx.x;
     const int le = tid / NN, node = tid - (tid / NN) * NN;
-    const int e0 = blockIdx.x * EB;
-    const int nel = min(EB, a.E - e0);
+    const int e0 = blk.x;
+    const int nel = blk.y;
     const bool act = le < nel;
     const int lec = le < nel ? le : nel - 1;
     const int e = e0 + lec;
-    const double* uin = sel_in(a, a.st->cur);
+    const double* uin = sel_in(a, st->cur);
     double val[4];
     load4(uin + ((size_t)e * NN + node) * 4, val);
     double l1, l2;
     bool bad;
     node_speeds(val, a.c->gamma, l1, l2, bad);
-    if (act && bad) a.st->bad_state = 1;
+    if (act && bad) st->bad_state = 1;
     if (act) {
         s_l1[lec * NN + node] = l1;
         s_l2[lec * NN + node] = l2;
@@ -1203,7 +1232,7 @@ __global__ __launch_bounds__(kSurfBlock) void k_max_speed(KArgs a) {
     if (tid == 0) {
         double bm = 0.0;
         for (int k = 0; k < nel; ++k) bm = dmax(bm, s_m[k]);
-        a.partial[blockIdx.x] = bm;
+        a.partial[a.vol_grid + blockIdx.x] = bm;
     }
 }
 
@@ -1220,14 +1249,16 @@ __global__ __launch_bounds__(kSurfBlock) void k_indicator(KArgs a, double* alpha
     const int tid = threadIdx.x;
     const int le = tid / NN, node = tid - (tid / NN) * NN;
     const int i = node % n, j = node / n;
-    const int e0 = blockIdx.x * EB;
-    const int nel = min(EB, a.E - e0);
+    const int4 blk = load_blk(a.sblk);
+    const DevState* st = a.st + blk.z;
+    const int e0 = blk.x;
+    const int nel = blk.y;
     const bool act = le < nel;
     const int lec = le < nel ? le : nel - 1;
     const int e = e0 + lec;
     const DevConst* c = a.c;
-    const double* uin = sel_in(a, a.use_state_dt ? a.st->cur : 0);
-    if (a.use_state_dt && a.st->finished) return;
+    const double* uin = sel_in(a, a.use_state_dt ? st->cur : 0);
+    if (a.use_state_dt && st->finished) return;
     double un[4];
     load4(uin + ((size_t)e * NN + node) * 4, un);
     const double rp = (c->gamma - 1.0) * (un[0] * un[3] - 0.5 * (un[1] * un[1] + un[2] * un[2]));
@@ -1282,13 +1313,14 @@ __global__ __launch_bounds__(kSurfBlock) void k_limiter(KArgs a) {
     __shared__ double s_var[S], s_q[4][S];
     __shared__ double s_theta[EB], s_mean[EB][4];
     __shared__ int s_flag[EB];
-    DevState* st = a.st;
+    const int4 blk = load_blk(a.sblk);
+    DevState* st = a.st + blk.z;
     if (a.use_state_dt && st->finished) return;
     const int tid = threadIdx.x;
     const int le = tid / NN, node = tid - (tid / NN) * NN;
     const int i = node % n, j = node / n;
-    const int e0 = blockIdx.x * EB;
-    const int nel = min(EB, a.E - e0);
+    const int e0 = blk.x;
+    const int nel = blk.y;
     const bool act = le < nel;
     const int lec = le < nel ? le : nel - 1;
     const int e = e0 + lec;
@@ -1364,28 +1396,36 @@ __global__ __launch_bounds__(kSurfBlock) void k_limiter(KArgs a) {
     if (tid == 0) {
         double bm = 0.0;
         for (int k = 0; k < nel; ++k) bm = dmax(bm, s_theta[k]);
-        a.partial[blockIdx.x] = bm;
+        a.partial[a.vol_grid + blockIdx.x] = bm;
     }
 }
 
-// Final reduction of the per-workgroup partial maxima of J⁻¹(max λ1 + max λ2)
-// by one workgroup, then compute_dt / set_dt! / set_t_and_iter! / isfinished
-// on the device state (lwfr.jl:76-103, 146-148, 176-190; Trixi max_dt).
+// Final reduction of the per-workgroup partial maxima of J⁻¹(max λ1 + max λ2),
+// one workgroup per instance over that instance's partials (rng[inst] = volume
+// begin/end, surface begin/end; the volume range is skipped when use_vol is 0),
+// then compute_dt / set_dt! / set_t_and_iter! / isfinished on the instance's
+// state (lwfr.jl:76-103, 146-148, 176-190; Trixi max_dt).
 constexpr int kFinalizeBlock = 1024;
 template <int N>
-__global__ __launch_bounds__(kFinalizeBlock) void k_finalize(DevState* st, const double* partial, int nparts,
-                                                           int reduce) {
+__global__ __launch_bounds__(kFinalizeBlock) void k_finalize(DevState* st_all, const double* partial, const int4* rng,
+                                                           int use_vol, int reduce) {
     constexpr int n = N + 1;
     __shared__ double s[kFinalizeBlock];
+    DevState* st = st_all + blockIdx.x;
+    const int4 r = rng[blockIdx.x];
     if (reduce == RED_STEP && st->finished) return;
     // eight independent accumulators keep several loads in flight per lane
     double m[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    int k = threadIdx.x;
-    for (; k + 7 * kFinalizeBlock < nparts; k += 8 * kFinalizeBlock) {
+    auto scan = [&](int b, int e) {
+        int k = b + (int)threadIdx.x;
+        for (; k + 7 * kFinalizeBlock < e; k += 8 * kFinalizeBlock) {
 #pragma unroll
-        for (int q = 0; q < 8; ++q) m[q] = dmax(m[q], partial[k + q * kFinalizeBlock]);
-    }
-    for (; k < nparts; k += kFinalizeBlock) m[0] = dmax(m[0], partial[k]);
+            for (int q = 0; q < 8; ++q) m[q] = dmax(m[q], partial[k + q * kFinalizeBlock]);
+        }
+        for (; k < e; k += kFinalizeBlock) m[0] = dmax(m[0], partial[k]);
+    };
+    if (use_vol) scan(r.x, r.y);
+    scan(r.z, r.w);
 #pragma unroll
     for (int q = 1; q < 8; ++q) m[0] = dmax(m[0], m[q]);
     s[threadIdx.x] = m[0];
@@ -1416,23 +1456,25 @@ __global__ __launch_bounds__(kFinalizeBlock) void k_finalize(DevState* st, const
     st->finished = julia_isapprox(t, tf);
 }
 
-// Initialisation of the device integrator state (one thread).
-__global__ void k_init_state(DevState* st, double t0, double tf, double cfl, int cur) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        st->t = t0;
-        st->dt = 0.0;
-        st->t_final = tf;
-        st->cfl = cfl;
-        st->query = 0.0;
-        st->finished = julia_isapprox(t0, tf);
-        st->cur = cur;
-        st->bad_state = 0;
-        st->iter = 0;
-    }
+// Initialisation of the device integrator state of every instance.
+__global__ void k_init_state(DevState* st_all, int ninst, double t0, double tf, double cfl, const int* cur) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ninst) return;
+    DevState* st = st_all + k;
+    st->t = t0;
+    st->dt = 0.0;
+    st->t_final = tf;
+    st->cfl = cfl;
+    st->query = 0.0;
+    st->finished = julia_isapprox(t0, tf);
+    st->cur = cur[k];
+    st->bad_state = 0;
+    st->iter = 0;
 }
 
-__global__ void k_clear_bad_state(DevState* st) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) st->bad_state = 0;
+__global__ void k_clear_bad_state(DevState* st_all, int ninst) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < ninst) st_all[k].bad_state = 0;
 }
 
 }  // namespace tlw

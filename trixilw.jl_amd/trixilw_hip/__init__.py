@@ -7,6 +7,7 @@ library; the first call that needs the GPU does, and fails if it is missing.
 """
 from .adjoint import adjoint_step, gradient_lw, kinetic_energy
 from .basis import LobattoLegendreBasis
+from .batch import SemidiscretizationBatch
 from .equations import (CompressibleEulerEquations2D, ConstantState, FourQuadrantRiemann, IsentropicVortex,
                         LinearScalarAdvectionEquation2D)
 from .integrator import (LWIntegrator, LWSolution, compute_dt, isapprox, isfinished, perform_step, set_dt,
@@ -28,4 +29,5 @@ __all__ = [
     "FirstOrderReconstruction", "IndicatorHennemannGassner", "MUSCLHancockReconstruction",
     "VolumeIntegralFRShockCapturing", "density_pressure", "PositivityPreservingLimiterZhangShu", "density",
     "pressure", "adjoint_step", "gradient_lw", "kinetic_energy",
+    "SemidiscretizationBatch",
 ]

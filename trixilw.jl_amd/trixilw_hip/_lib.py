@@ -25,7 +25,7 @@ EXPORTED = (
     "tlw_get_state", "tlw_state_ptr", "tlw_solve_init", "tlw_advance", "tlw_get_aux",
     "tlw_enable_kernel_timing", "tlw_kernel_times", "tlw_set_elements_per_workgroup",
     "tlw_get_elements_per_workgroup", "tlw_autotune", "tlw_set_positivity_limiter", "tlw_apply_positivity_limiter",
-    "tlw_adjoint_lw",
+    "tlw_adjoint_lw", "tlw_create_batch", "tlw_n_instances", "tlw_batch_status", "tlw_enable_graphs",
 )
 
 
@@ -137,6 +137,15 @@ def lib():
     L.tlw_set_positivity_limiter.argtypes = [vp, i32, C.c_double, C.c_double]
     L.tlw_apply_positivity_limiter.restype = C.c_int
     L.tlw_apply_positivity_limiter.argtypes = [vp, vp]
+    L.tlw_create_batch.restype = C.c_int
+    L.tlw_create_batch.argtypes = [i32, C.POINTER(MeshDesc), C.POINTER(BasisDesc), C.POINTER(SolverDesc),
+                                   C.POINTER(vp)]
+    L.tlw_n_instances.restype = C.c_int
+    L.tlw_n_instances.argtypes = [vp, C.POINTER(i32)]
+    L.tlw_batch_status.restype = C.c_int
+    L.tlw_batch_status.argtypes = [vp, i32, dp, dp, C.POINTER(C.c_int64), C.POINTER(i32)]
+    L.tlw_enable_graphs.restype = C.c_int
+    L.tlw_enable_graphs.argtypes = [vp, i32]
     L.tlw_adjoint_lw.restype = C.c_int
     L.tlw_adjoint_lw.argtypes = [vp, vp, vp, vp, C.c_double, C.c_double]
     L.tlw_autotune.restype = C.c_int

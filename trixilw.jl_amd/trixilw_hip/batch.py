@@ -1,0 +1,151 @@
+"""Batches of independent semidiscretizations in one device context.
+
+An ensemble of instances (each its own TreeMesh and initial condition, same
+polynomial degree, equations and solver) is advanced by one launch per kernel
+and step (tlw_create_batch); every instance keeps its own t, CFL dt and
+iteration count, exactly as if it ran alone through solve_lwfr.  Steps can be
+replayed from a captured HIP graph.  The reference runs one semidiscretization
+per solve_lwfr call; this is the batched form of that loop.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from .semidiscretization import MDRK, SemidiscretizationHyperbolic, _dptr
+
+
+def _torch():
+    import torch
+    return torch
+
+
+class SemidiscretizationBatch:
+    """SemidiscretizationBatch(meshes, time_discretization, equations, initial_conditions, solver).
+
+    ``meshes`` and ``initial_conditions`` are lists of equal length; all
+    instances share ``equations``, ``solver`` (basis, volume integral) and the
+    boundary treatment (periodic, or the given boundary value function).
+    """
+
+    def __init__(self, meshes, time_discretization, equations, initial_conditions, solver, boundary_conditions=None,
+                 device=None):
+        torch = _torch()
+        if len(meshes) != len(initial_conditions) or not meshes:
+            raise ValueError("one initial condition per mesh")
+        # the solver description is built exactly as for a single instance
+        first = SemidiscretizationHyperbolic(meshes[0], time_discretization, equations, initial_conditions[0], solver,
+                                             boundary_conditions=boundary_conditions, device=device)
+        self._proto = first
+        self.meshes = list(meshes)
+        self.initial_conditions = list(initial_conditions)
+        self.solver = solver
+        self.equations = equations
+        self.time_discretization = time_discretization
+        self.device = first.device
+        self.n = first.n
+        self._keep = []
+        descs = (_lib.MeshDesc * len(meshes))()
+        for k, m in enumerate(meshes):
+            arrs = first._mesh_arrays(m)
+            self._keep.append(arrs)
+            descs[k] = _lib.MeshDesc(int(m.n_elements), arrs["nbr"].ctypes.data_as(C.POINTER(C.c_int32)),
+                                     _dptr(arrs["invj"]), _dptr(arrs["center"]), _dptr(arrs["length"]))
+        b = solver.basis
+        self._basis_arrays = (np.ascontiguousarray(b.nodes), np.ascontiguousarray(b.weights),
+                              np.asfortranarray(b.derivative_matrix).ravel(order="F").copy(),
+                              np.asfortranarray(b.derivative_dhat).ravel(order="F").copy())
+        basis = _lib.BasisDesc(b.polydeg, *[_dptr(a) for a in self._basis_arrays])
+        h = C.c_void_p()
+        rc = _lib.lib().tlw_create_batch(len(meshes), descs, C.byref(basis), C.byref(first._solver_desc), C.byref(h))
+        if rc != _lib.TLW_OK:
+            raise _lib.TLWError(rc, "tlw_create_batch failed (invalid description or HIP error)")
+        self._h = h
+        first.close()
+        self.sizes = [m.n_elements for m in meshes]
+        self.offsets = np.concatenate([[0], np.cumsum(self.sizes)]).astype(int)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _lib.lib().tlw_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def ndofs(self):
+        return sum(self.sizes) * self.n * self.n
+
+    def instance_ndofs(self, k):
+        return self.sizes[k] * self.n * self.n
+
+    def compute_coefficients(self, t=0.0):
+        """Initial states of all instances, concatenated, as a host array (E_total, n, n, 4)."""
+        parts = []
+        for m, ic in zip(self.meshes, self.initial_conditions):
+            x, y = m.node_coordinates(self.solver.basis.nodes)
+            parts.append(np.ascontiguousarray(ic(x, y, t), dtype=np.float64))
+        return np.concatenate(parts)
+
+    def split(self, u):
+        """Per-instance views of a concatenated state."""
+        return [u[self.offsets[k]:self.offsets[k + 1]] for k in range(len(self.sizes))]
+
+    def set_stream(self, stream):
+        _lib.check(_lib.lib().tlw_set_stream(self._h, C.c_void_p(stream.cuda_stream)), self._h)
+
+    def solve(self, u0, cfl, nsteps, t0=0.0, t_final=float("inf"), chunk=None, graphs=False, stream=None):
+        """solve_lwfr's CFL loop for every instance: nsteps steps (or until every
+        instance reaches t_final).  Returns (u, status) with u the concatenated
+        final states (host array) and status a dict of per-instance t, dt, iter."""
+        torch = _torch()
+        L = _lib.lib()
+        h = self._h
+        if stream is not None:
+            self.set_stream(stream)
+        elif graphs:
+            raise ValueError("graph replay needs an explicit stream")
+        u = np.ascontiguousarray(u0, dtype=np.float64)
+        dp = C.POINTER(C.c_double)
+        _lib.check(L.tlw_enable_graphs(h, int(bool(graphs))), h)
+        _lib.check(L.tlw_set_state(h, u.ctypes.data_as(dp), u.size), h)
+        _lib.check(L.tlw_solve_init(h, t0, t_final, cfl), h)
+        t, dt = C.c_double(), C.c_double()
+        it, fin = C.c_int64(), C.c_int32()
+        chunk = chunk or nsteps
+        done = 0
+        while done < nsteps:
+            k = min(chunk, nsteps - done)
+            _lib.check(L.tlw_advance(h, k, C.byref(t), C.byref(dt), C.byref(it), C.byref(fin)), h)
+            done += k
+            if fin.value:
+                break
+        out = np.empty_like(u)
+        _lib.check(L.tlw_get_state(h, out.ctypes.data_as(dp), out.size), h)
+        return out, self.status()
+
+    def advance(self, nsteps):
+        """Advance an initialised batch by nsteps (no host synchronisation inside)."""
+        L = _lib.lib()
+        t, dt = C.c_double(), C.c_double()
+        it, fin = C.c_int64(), C.c_int32()
+        _lib.check(L.tlw_advance(self._h, int(nsteps), C.byref(t), C.byref(dt), C.byref(it), C.byref(fin)), self._h)
+
+    def status(self):
+        n = len(self.sizes)
+        t, dt = np.zeros(n), np.zeros(n)
+        it = np.zeros(n, dtype=np.int64)
+        fin = np.zeros(n, dtype=np.int32)
+        _lib.check(_lib.lib().tlw_batch_status(self._h, n, _dptr(t), _dptr(dt),
+                                               it.ctypes.data_as(C.POINTER(C.c_int64)),
+                                               fin.ctypes.data_as(C.POINTER(C.c_int32))), self._h)
+        return {"t": t, "dt": dt, "iter": it, "finished": fin.astype(bool)}

@@ -182,6 +182,7 @@ class SemidiscretizationHyperbolic:
             sd.alpha_max, sd.alpha_min = ind.alpha_max, ind.alpha_min
             sd.alpha_smooth = int(bool(ind.alpha_smooth))
             sd.inverse_vandermonde_host = _dptr(self._vinv)
+        self._solver_desc = sd
         self._keep = self._mesh_arrays(mesh)
         md = self._mesh_desc()
         h = C.c_void_p()
