@@ -45,6 +45,9 @@ class _Desc(C.Structure):
         ("alpha_smooth", C.c_int),
         ("positivity", C.c_int),
         ("pos_threshold", C.c_double * 2),
+        ("parabolic", C.c_int),
+        ("visc_mu", C.c_double),
+        ("prandtl", C.c_double),
     ]
 
 
@@ -92,6 +95,7 @@ def lib():
         L.tlwo_inverse_vandermonde.argtypes = [C.c_void_p, dp]
         L.tlwo_max_abs_speed_naive.restype = C.c_double
         L.tlwo_max_abs_speed_naive.argtypes = [dp, dp, C.c_int, C.c_double]
+        L.tlwo_ns_viscous_fluxes.argtypes = [dp, dp, dp, C.c_double, C.c_double, C.c_double, dp, dp]
         L.tlwo_solve.restype = C.c_int
         L.tlwo_solve.argtypes = [C.c_void_p, dp, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, dp, dp]
         _lib = L
@@ -104,7 +108,8 @@ def _ptr(a: np.ndarray):
 
 
 AUX = {"U": 0, "F": 1, "temporal_errors": 2, "surface_flux_values": 3, "us": 4, "U2": 5, "F2": 6, "alpha": 7,
-       "fn_low": 8}
+       "fn_low": 8, "gradients_x": 9, "gradients_y": 10, "Fv": 11, "Fv2": 12}
+PARABOLIC = {None: 0, "laplace": 1, "navier_stokes": 2}
 
 
 class Oracle:
@@ -112,9 +117,14 @@ class Oracle:
 
     def __init__(self, polydeg, nx, ny, xmin, ymin, h, periodic=(True, True), equation="euler", gamma=1.4,
                  adv=(1.0, 1.0), flux_order=0, bc_kind=BV_NONE, bc_par=(), tolerances=(0.0, 0.0), nthreads=0,
-                 shock_capturing=False, alpha_max=0.5, alpha_min=0.001, alpha_smooth=True, positivity=None):
+                 shock_capturing=False, alpha_max=0.5, alpha_min=0.001, alpha_smooth=True, positivity=None,
+                 parabolic=None, mu=0.0, prandtl=0.72):
         """``positivity``: (threshold_density, threshold_pressure) of a Zhang-Shu
-        limiter that ``solve`` applies as a stage limiter, or None."""
+        limiter that ``solve`` applies as a stage limiter, or None.
+        ``parabolic``: None, "laplace" (LaplaceDiffusion2D with diffusivity ``mu``,
+        linear advection) or "navier_stokes" (CompressibleNavierStokesDiffusion2D,
+        dynamic viscosity ``mu``, Prandtl number ``prandtl``, conservative
+        gradient variables); periodic meshes only."""
         d = _Desc()
         d.polydeg, d.nx, d.ny = polydeg, nx, ny
         d.xmin, d.ymin, d.h = xmin, ymin, h
@@ -134,6 +144,8 @@ class Oracle:
         if positivity is not None:
             d.positivity = 1
             d.pos_threshold[0], d.pos_threshold[1] = positivity
+        d.parabolic = PARABOLIC[parabolic]
+        d.visc_mu, d.prandtl = mu, prandtl
         self._desc = d
         self._h = lib().tlwo_create(C.byref(d))
         if not self._h:
@@ -253,6 +265,15 @@ def max_abs_speed_naive(ul, ur, orientation, gamma=1.4):
     ul = np.ascontiguousarray(ul, dtype=np.float64)
     ur = np.ascontiguousarray(ur, dtype=np.float64)
     return lib().tlwo_max_abs_speed_naive(_ptr(ul), _ptr(ur), int(orientation), gamma)
+
+
+def ns_viscous_fluxes(u, gx, gy, gamma=1.4, mu=1.0, prandtl=0.72):
+    """Viscous fluxes (x, y) of CompressibleNavierStokesDiffusion2D with
+    conservative gradient variables at one node."""
+    u, gx, gy = (np.ascontiguousarray(a, dtype=np.float64) for a in (u, gx, gy))
+    fx, fy = np.zeros(4), np.zeros(4)
+    lib().tlwo_ns_viscous_fluxes(_ptr(u), _ptr(gx), _ptr(gy), gamma, mu, prandtl, _ptr(fx), _ptr(fy))
+    return fx, fy
 
 
 def set_threads(n: int) -> None:

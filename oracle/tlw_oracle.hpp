@@ -414,6 +414,10 @@ template <class T, class Eq> struct Solver {
     std::vector<T> U, F, sfv, temporal_errors;
     std::vector<T> us, U2, F2;  // MDRK stage cache (containers.jl:83-98)
 
+    virtual ~Solver() = default;
+    // gradients and viscous caches of the hyperbolic-parabolic solver (tlwo_get_aux 9-12)
+    virtual const std::vector<T>* extra_aux(int) const { return nullptr; }
+
     void setup() {
         n = basis.n;
         const size_t nd = (size_t)mesh.E * n * n * NV;
@@ -1129,7 +1133,7 @@ template <class T, class Eq> struct Solver {
         for (long long k = 0; k < nd; ++k) du[k] = T(0);
     }
 
-    void rhs_lw(T* du, const T* u, double t, double dt) {
+    virtual void rhs_lw(T* du, const T* u, double t, double dt) {
         reset_du(du);  // reset_du!
         // calc_volume_integral! for VolumeIntegralFRShockCapturing (dg_2d.jl:1648-1804)
         if (sc) sc_prepare(u, dt);
@@ -1164,7 +1168,7 @@ template <class T, class Eq> struct Solver {
 
     // calc_volume_integral_mdrk1!/2! with shock capturing (tree/multi_derivative_2d.jl:151-261):
     // the MDRK kernel scaled by 1 - α plus first-order FV of uⁿ scaled by α.
-    void rhs_mdrk1(T* du, const T* u, double t, double dt) {
+    virtual void rhs_mdrk1(T* du, const T* u, double t, double dt) {
         ensure_mdrk();
         reset_du(du);
         if (sc) sc_prepare(u, dt);
@@ -1177,7 +1181,7 @@ template <class T, class Eq> struct Solver {
         if (!mesh.bd_elem.empty()) calc_boundary_flux(u, t, dt, 0.5);
         surface_integral_and_jacobian(du);
     }
-    void rhs_mdrk2(T* du, const T* u, double t, double dt) {
+    virtual void rhs_mdrk2(T* du, const T* u, double t, double dt) {
         ensure_mdrk();
         reset_du(du);
         if (sc) sc_prepare(u, dt);
@@ -1228,7 +1232,7 @@ template <class T, class Eq> struct Solver {
     // Trixi max_dt(u, t, mesh::TreeMesh{2}, constant_speed::False, ...):
     // 2 / (nnodes * max_e J⁻¹ (max λ1 + max λ2)); called by compute_dt
     // (src/solvers/lwfr.jl:176-190).
-    double max_dt(const T* u) const {
+    virtual double max_dt(const T* u) const {
         double mss = std::nextafter(0.0, 1.0);
 #pragma omp parallel for schedule(static) reduction(max : mss)
         for (int e = 0; e < mesh.E; ++e) {

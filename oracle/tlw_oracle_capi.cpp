@@ -7,7 +7,7 @@
 #include <cstdio>
 #include <memory>
 
-#include "tlw_oracle.hpp"
+#include "tlw_oracle_parabolic.hpp"
 
 using namespace tlwo;
 
@@ -29,6 +29,12 @@ struct tlwo_desc {
     int alpha_smooth;
     int positivity;              // Zhang-Shu limiter on (density, pressure) in solve
     double pos_threshold[2];
+    // SemidiscretizationHyperbolicParabolic: 0 none, 1 LaplaceDiffusion2D (with
+    // linear advection), 2 CompressibleNavierStokesDiffusion2D with
+    // GradientVariablesConservative (with Euler); periodic meshes only
+    int parabolic;
+    double visc_mu;  // diffusivity (Laplace) or dynamic viscosity μ (Navier-Stokes)
+    double prandtl;
 };
 
 struct tlwo_ctx {
@@ -60,9 +66,28 @@ extern "C" {
 void* tlwo_create(const tlwo_desc* d) {
     if (d->polydeg != 3 && d->polydeg != 4) return nullptr;
     if (d->nthreads > 0) omp_set_num_threads(d->nthreads);
+    if (d->parabolic) {
+        if (!d->periodic_x || !d->periodic_y || d->shock_capturing) return nullptr;
+        if ((d->parabolic == 1) != (d->equation == 1)) return nullptr;
+    }
     auto* c = new tlwo_ctx;
     c->equation = d->equation;
-    if (d->equation == 0) {
+    if (d->parabolic == 2) {
+        auto* p = new ParabolicSolver<double, Euler2D<double>, NavierStokesDiffusion2D<double>>;
+        p->eq.gamma = d->gamma;
+        p->visc.set(d->gamma, d->visc_mu, d->prandtl);
+        configure(*p, d);
+        p->setup_parabolic();
+        c->euler.reset(p);
+    } else if (d->parabolic == 1) {
+        auto* p = new ParabolicSolver<double, LinAdv2D<double>, LaplaceDiffusion2D<double>>;
+        p->eq.a1 = d->adv_a1;
+        p->eq.a2 = d->adv_a2;
+        p->visc.diffusivity = d->visc_mu;
+        configure(*p, d);
+        p->setup_parabolic();
+        c->linadv.reset(p);
+    } else if (d->equation == 0) {
         c->euler.reset(new Solver<double, Euler2D<double>>);
         c->euler->eq.gamma = d->gamma;
         configure(*c->euler, d);
@@ -158,7 +183,9 @@ void tlwo_set_us(void* ctx, const double* us) {
 
 // which: 0 = Ũ [v,i,j,e], 1 = F̃ [v,d,i,j,e], 2 = temporal_errors [e],
 //        3 = surface_flux_values [v,l,face,e], 4 = us, 5 = U2, 6 = F2,
-//        7 = alpha [e], 8 = fn_low [v,l,face,e]
+//        7 = alpha [e], 8 = fn_low [v,l,face,e],
+//        hyperbolic-parabolic only: 9 = gradients_x, 10 = gradients_y [v,i,j,e],
+//        11 = Fv [v,d,i,j,e], 12 = mdrk Fv2
 long long tlwo_get_aux(void* ctx, int which, double* out) {
     long long len = -1;
     DISPATCH(ctx, {
@@ -173,6 +200,7 @@ long long tlwo_get_aux(void* ctx, int which, double* out) {
             case 6: src = &s.F2; break;
             case 7: src = &s.alpha; break;
             case 8: src = &s.fn_low; break;
+            default: src = s.extra_aux(which); break;
         }
         if (src) {
             len = (long long)src->size();
@@ -249,5 +277,16 @@ int tlwo_tangent_lw(const tlwo_desc* d, const double* u0, const double* du0, con
         du_out[k] = u[k].d;
     }
     return 0;
+}
+}  // extern "C"
+
+extern "C" {
+// Pointwise viscous flux of CompressibleNavierStokesDiffusion2D with conservative
+// gradient variables, for the chain-rule check.
+void tlwo_ns_viscous_fluxes(const double* u, const double* gx, const double* gy, double gamma, double mu,
+                            double prandtl, double* fx, double* fy) {
+    NavierStokesDiffusion2D<double> v;
+    v.set(gamma, mu, prandtl);
+    v.fluxes<4>(u, gx, gy, fx, fy);
 }
 }  // extern "C"

@@ -142,3 +142,101 @@ class LinearLW:
             r = self.lw_step(u, dt)[0] if kind == "lw" else self.mdrk_step(u, dt)[0]
             M[:, k] = r.ravel()
         return M
+
+
+class LinearParabolicLW(LinearLW):
+    """Closed-form LW / MDRK step for linear advection-diffusion, f = a u - ν ∇u
+    (TrixiLW's hyperbolic-parabolic kernels, src/solvers/dgsem_tree/
+    dg_2d_parabolic.jl:725-1948 and multi_derivative_2d_parabolic.jl:221-711,
+    with Trixi's BR1 gradient).  With a linear flux the finite-difference time
+    derivatives of both parts are exact on the Taylor states, so with
+      g₀ = BR1 gradient of u (weak-form D̂ volume term, central face value,
+           lifting by 1/w at the end nodes, ×J⁻¹),
+      u_{k+1} = -Δt J⁻¹ (Dx (a1 u_k - ν gx_k) + Dy (a2 u_k - ν gy_k)),
+      g_{k+1} = J⁻¹ (Dx, Dy) u_{k+1}      (element-local gradient of u_k),
+    the LW fluxes are F̃ = Σ q_k (a u_k - ν g_k) with q = 1, 1/2, 1/6, 1/24 and,
+    at N=4, 1/240 (the halved f_tttt also applies to the viscous part), and the
+    interface flux is ½(F̃_L + F̃_R) - ½|a| (u_R - u_L): the viscous part enters
+    as the central average.  Evaluated with dense numpy operators in gather
+    form, independently of the oracle's loops.
+    """
+
+    def __init__(self, nodes, weights, D, Dhat, nx, ny, h, a=(1.5, 1.0), nu=0.05):
+        super().__init__(nodes, weights, D, Dhat, nx, ny, h, a=a, flux_order="tree")
+        self.nu = nu
+
+    def neighbours(self):
+        E = self.E
+        return [np.array([self.nbr(e, f) for e in range(E)]) for f in range(4)]
+
+    def br1_gradient(self, u):
+        n = self.n
+        L, R, B, T = self.neighbours()
+        gx = self.dx(u, self.Dh)
+        gy = self.dy(u, self.Dh)
+        gx[:, :, 0] -= 0.5 * (u[L][:, :, n - 1] + u[:, :, 0]) / self.w[0]
+        gx[:, :, n - 1] += 0.5 * (u[:, :, n - 1] + u[R][:, :, 0]) / self.w[n - 1]
+        gy[:, 0, :] -= 0.5 * (u[B][:, n - 1, :] + u[:, 0, :]) / self.w[0]
+        gy[:, n - 1, :] += 0.5 * (u[:, n - 1, :] + u[T][:, 0, :]) / self.w[n - 1]
+        return self.invJ * gx, self.invJ * gy
+
+    def local_gradient(self, u):
+        return self.invJ * self.dx(u, self.D), self.invJ * self.dy(u, self.D)
+
+    def advance(self, u, gx, gy, dt):
+        fx = self.a1 * u - self.nu * gx
+        fy = self.a2 * u - self.nu * gy
+        return -dt * self.invJ * (self.dx(fx, self.D) + self.dy(fy, self.D))
+
+    def update(self, u, Fx, Fy, dt):
+        """u + dt du for the flux F̃ = (Fx, Fy) (advective minus viscous) and the
+        Rusanov jump on uⁿ."""
+        n = self.n
+        L, R, B, T = self.neighbours()
+        l1, l2 = abs(self.a1), abs(self.a2)
+        vol = self.dx(Fx, self.Dh) + self.dy(Fy, self.Dh)
+        surf = np.zeros_like(u)
+        fsL = 0.5 * (Fx[L][:, :, n - 1] + Fx[:, :, 0]) - 0.5 * l1 * (u[:, :, 0] - u[L][:, :, n - 1])
+        fsR = 0.5 * (Fx[:, :, n - 1] + Fx[R][:, :, 0]) - 0.5 * l1 * (u[R][:, :, 0] - u[:, :, n - 1])
+        gsB = 0.5 * (Fy[B][:, n - 1, :] + Fy[:, 0, :]) - 0.5 * l2 * (u[:, 0, :] - u[B][:, n - 1, :])
+        gsT = 0.5 * (Fy[:, n - 1, :] + Fy[T][:, 0, :]) - 0.5 * l2 * (u[T][:, 0, :] - u[:, n - 1, :])
+        surf[:, :, 0] -= fsL / self.w[0]
+        surf[:, :, n - 1] += fsR / self.w[n - 1]
+        surf[:, 0, :] -= gsB / self.w[0]
+        surf[:, n - 1, :] += gsT / self.w[n - 1]
+        du = -self.invJ * (vol + surf)
+        return u + dt * du, du
+
+    def lw_step(self, u, dt, alpha=None):
+        N = self.n - 1
+        us_, gxs, gys = [u], [], []
+        gx, gy = self.br1_gradient(u)
+        gxs.append(gx)
+        gys.append(gy)
+        for k in range(1, N + 1):
+            w = self.advance(us_[-1], gxs[-1], gys[-1], dt)
+            us_.append(w)
+            wx, wy = self.local_gradient(w)
+            gxs.append(wx)
+            gys.append(wy)
+        q = [1.0, 1.0 / 2, 1.0 / 6, 1.0 / 24, 1.0 / 240 if N == 4 else 1.0 / 120]
+        Fx = sum(q[k] * (self.a1 * us_[k] - self.nu * gxs[k]) for k in range(N + 1))
+        Fy = sum(q[k] * (self.a2 * us_[k] - self.nu * gys[k]) for k in range(N + 1))
+        return self.update(u, Fx, Fy, dt)
+
+    def mdrk_step(self, u, dt, alpha=None):
+        gx, gy = self.br1_gradient(u)
+        ut = self.advance(u, gx, gy, dt)
+        utx, uty = self.local_gradient(ut)
+        f = (self.a1 * u - self.nu * gx, self.a2 * u - self.nu * gy)
+        ft = (self.a1 * ut - self.nu * utx, self.a2 * ut - self.nu * uty)
+        F1 = [0.5 * f[d] + 0.125 * ft[d] for d in range(2)]
+        us, du1 = self.update(u, F1[0], F1[1], dt)
+        F2 = [f[d] + ft[d] / 6.0 for d in range(2)]
+        sx, sy = self.br1_gradient(us)
+        ust = self.advance(us, sx, sy, dt)
+        ustx, usty = self.local_gradient(ust)
+        fst = (self.a1 * ust - self.nu * ustx, self.a2 * ust - self.nu * usty)
+        Fs = [F2[d] + fst[d] / 3.0 for d in range(2)]
+        unew, du2 = self.update(u, Fs[0], Fs[1], dt)
+        return unew, du1, du2, us
