@@ -96,6 +96,20 @@ typedef struct {
     const double* inverse_vandermonde_host; /* [n*n] Trixi basis.inverse_vandermonde_legendre, column-major */
 } tlw_solver_desc;
 
+/* Parabolic terms of SemidiscretizationHyperbolicParabolic
+ * (src/semidiscretization/semidiscretization_hyperbolic_parabolic.jl:8-118):
+ * CompressibleNavierStokesDiffusion2D(equations; mu, Prandtl, gradient_variables)
+ * with constant dynamic viscosity (src/equations/compressible_navier_stokes_2d.jl). */
+enum { TLW_PAR_NAVIER_STOKES = 1 };
+/* gradient variables: GradientVariablesConservative (compressible_navier_stokes_2d.jl:14-57)
+ * is the one implemented; Trixi's GradientVariablesPrimitive is rejected (TLW_ENOTSUP). */
+enum { TLW_GRAD_CONSERVATIVE = 0, TLW_GRAD_PRIMITIVE = 1 };
+typedef struct {
+    int32_t equations;           /* TLW_PAR_NAVIER_STOKES */
+    double mu;                   /* dynamic viscosity */
+    double prandtl;              /* Prandtl number; κ = γ / ((γ-1) Pr) */
+    int32_t gradient_variables;  /* TLW_GRAD_CONSERVATIVE */
+} tlw_parabolic_desc;
 typedef struct tlw_ctx tlw_ctx;
 
 /* SemidiscretizationHyperbolic(mesh, time_disc, equations, ic, solver) + cache
@@ -112,6 +126,18 @@ int tlw_create(const tlw_mesh_desc* mesh, const tlw_basis_desc* basis, const tlw
  * smallest of the instances' values. */
 int tlw_create_batch(int32_t n_instances, const tlw_mesh_desc* meshes, const tlw_basis_desc* basis,
                      const tlw_solver_desc* solver, tlw_ctx** out);
+/* SemidiscretizationHyperbolicParabolic(mesh, time_disc, (equations, equations_parabolic), ic, solver)
+ * (semidiscretization_hyperbolic_parabolic.jl:8-118) on a periodic mesh: every rhs and step of the
+ * context then follows rhs! / rhs_mdrk1! / rhs_mdrk2! of the parabolic path
+ * (src/solvers/dgsem_tree/dg_2d_parabolic.jl:15-98, multi_derivative_2d_parabolic.jl:59-219):
+ * BR1 gradient of the conservative variables, LW procedure on f_a - f_v with Taylor states of
+ * the gradient, interface flux LW-Rusanov - ½(F̃v_L + F̃v_R); tlw_max_dt and the device loop use
+ * compute_dt's min of the advective and viscous bounds (src/solvers/lwfr.jl:192-210,
+ * compressible_navier_stokes_2d.jl:204-261).  Plain VolumeIntegralFR only; the positivity
+ * limiter and tlw_adjoint_lw return TLW_ENOTSUP on such a context.  With TLW_FLAG_KEEP_AUX,
+ * TLW_AUX_F holds the total F̃a - F̃v. */
+int tlw_create_parabolic(const tlw_mesh_desc* mesh, const tlw_basis_desc* basis, const tlw_solver_desc* solver,
+                         const tlw_parabolic_desc* parabolic, tlw_ctx** out);
 int tlw_n_instances(const tlw_ctx* ctx, int32_t* n_instances);
 int tlw_batch_status(tlw_ctx* ctx, int32_t n_instances, double* t_host, double* dt_host, int64_t* iter_host,
                      int32_t* finished_host);

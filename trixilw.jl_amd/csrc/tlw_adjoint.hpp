@@ -123,11 +123,6 @@ __device__ __forceinline__ void contract_t(const double* s, int base, int i, int
     }
 }
 
-template <int S> __device__ __forceinline__ void put4(double* s, int idx, const double* a) {
-#pragma unroll
-    for (int v = 0; v < 4; ++v) s[v * S + idx] = a[v];
-}
-
 template <int N> constexpr int kEBA = (N == 4) ? 5 : 8;  // elements per adjoint workgroup
 
 template <int N, int EB, bool KEEP>

@@ -55,6 +55,9 @@ struct tlw_ctx {
     int4* d_rng = nullptr;
     int vgrid = 0, sgrid = 0, vcap = 0;
     bool sc = false;
+    bool parabolic = false;   // compressible Navier-Stokes (tlw_create_parabolic)
+    int pv_off = 0;           // offset of the viscous CFL partials (0: none)
+    double* d_elem_vmax = nullptr;
     int ebv = 0;              // elements per volume workgroup
     bool positivity = false;  // Zhang-Shu limiter in the device loop
     bool has_boundary = false;
@@ -119,13 +122,14 @@ void free_mesh(tlw_ctx* c) {
     drop_graphs(c);
     void* ptrs[] = {c->d_nbr,       c->d_invj,  c->d_center, c->d_length, c->d_trace,   c->d_buf[0],    c->d_buf[1],
                     c->d_us,        c->d_auxU,  c->d_auxF,   c->d_partial, c->d_elem_max, c->d_alpha_tmp, c->d_alpha,
-                    c->d_vblk,      c->d_sblk,  c->d_etile,  c->d_rng,    c->d_state,   c->d_cur};
+                    c->d_vblk,      c->d_sblk,  c->d_etile,  c->d_rng,    c->d_state,   c->d_cur,
+                    c->d_elem_vmax};
     for (void* p : ptrs) (void)hipFree(p);
     if (c->h_state) (void)hipHostFree(c->h_state);
     c->d_nbr = nullptr;
     c->d_invj = c->d_center = c->d_length = c->d_trace = c->d_us = c->d_auxU = c->d_auxF = nullptr;
     c->d_buf[0] = c->d_buf[1] = nullptr;
-    c->d_partial = c->d_elem_max = c->d_alpha_tmp = c->d_alpha = nullptr;
+    c->d_partial = c->d_elem_max = c->d_alpha_tmp = c->d_alpha = c->d_elem_vmax = nullptr;
     c->d_vblk = c->d_sblk = c->d_rng = nullptr;
     c->d_etile = nullptr;
     c->d_state = nullptr;
@@ -209,6 +213,8 @@ int upload_meshes(tlw_ctx* c, const tlw_mesh_desc* ms, int ninst) {
             if (ms[k].neighbors_host[q] < 0) boundary = true;
     if (boundary && c->hconst.bc_kind == TLW_BV_NONE)
         return fail(c, TLW_EINVAL, "mesh has boundary faces but no boundary value function was given");
+    if (boundary && c->parabolic)
+        return fail(c, TLW_ENOTSUP, "the hyperbolic-parabolic path needs a periodic mesh");
     free_mesh(c);
     c->aux_valid = c->alpha_valid = false;
     c->ninst = ninst;
@@ -242,7 +248,10 @@ int upload_meshes(tlw_ctx* c, const tlw_mesh_desc* ms, int ninst) {
     // tables and partials sized for the smallest tile, so the tile can change freely
     c->vcap = (E + min_eb(c->N) - 1) / min_eb(c->N) + ninst;
     const int scap = (E + surf_eb(c->N) - 1) / surf_eb(c->N) + ninst;
-    HIPCHK(c, hipMalloc(&c->d_partial, sizeof(double) * (size_t)(c->vcap + scap + 1)));
+    // hyperbolic-parabolic: the viscous partials follow in a second block of the same size
+    c->pv_off = c->parabolic ? c->vcap + scap : 0;
+    HIPCHK(c, hipMalloc(&c->d_partial, sizeof(double) * (size_t)((c->parabolic ? 2 : 1) * (c->vcap + scap) + 1)));
+    if (c->parabolic) HIPCHK(c, hipMalloc(&c->d_elem_vmax, sizeof(double) * (size_t)E));
     HIPCHK(c, hipMalloc(&c->d_vblk, sizeof(int4) * (size_t)c->vcap));
     HIPCHK(c, hipMalloc(&c->d_sblk, sizeof(int4) * (size_t)scap));
     HIPCHK(c, hipMalloc(&c->d_etile, sizeof(int) * (size_t)E));
@@ -283,6 +292,8 @@ KArgs base_args(tlw_ctx* c) {
     a.st = c->d_state;
     a.partial = c->d_partial;
     a.elem_max = c->d_elem_max;
+    a.pv_off = c->pv_off;
+    a.elem_vmax = c->d_elem_vmax;
     a.vol_grid = c->vgrid;
     a.ebv = c->ebv;
     a.vblk = c->d_vblk;
@@ -295,21 +306,24 @@ KArgs base_args(tlw_ctx* c) {
 }
 
 // Kernel launches, dispatched on the polynomial degree and the tile size.
-template <int N, int EB, bool SC> void launch_volume_eb(tlw_ctx* c, const KArgs& a, int kind) {
+template <int N, int EB, bool SC, bool PAR = false> void launch_volume_eb(tlw_ctx* c, const KArgs& a, int kind) {
     dim3 g(c->vgrid), b(VGeo<N, EB>::NT);
-    if (kind == VOL_LW) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_LW, SC>), g, b, 0, c->stream, a);
+    if (kind == VOL_LW)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_LW, SC, PAR>), g, b, 0, c->stream, a);
     else if (kind == VOL_MDRK1)
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_MDRK1, SC>), g, b, 0, c->stream, a);
-    else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_MDRK2, SC>), g, b, 0, c->stream, a);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_MDRK1, SC, PAR>), g, b, 0, c->stream, a);
+    else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_MDRK2, SC, PAR>), g, b, 0, c->stream, a);
 }
-template <int N, bool SC> void launch_volume_sc(tlw_ctx* c, const KArgs& a, int kind) {
+template <int N, bool SC, bool PAR = false> void launch_volume_sc(tlw_ctx* c, const KArgs& a, int kind) {
     constexpr const int* ch = N == 4 ? kEbChoices4 : kEbChoices3;
-    if (c->ebv == ch[0]) launch_volume_eb<N, ch[0], SC>(c, a, kind);
-    else if (c->ebv == ch[1]) launch_volume_eb<N, ch[1], SC>(c, a, kind);
-    else launch_volume_eb<N, ch[2], SC>(c, a, kind);
+    if (c->ebv == ch[0]) launch_volume_eb<N, ch[0], SC, PAR>(c, a, kind);
+    else if (c->ebv == ch[1]) launch_volume_eb<N, ch[1], SC, PAR>(c, a, kind);
+    else launch_volume_eb<N, ch[2], SC, PAR>(c, a, kind);
 }
 template <int N> void launch_volume_t(tlw_ctx* c, const KArgs& a, int kind) {
-    if (c->sc) {
+    if (c->parabolic) {
+        launch_volume_sc<N, false, true>(c, a, kind);
+    } else if (c->sc) {
         // IndicatorHennemannGassner on uⁿ before the volume integral (dg_2d.jl:1665)
         hipLaunchKernelGGL(k_indicator<N>, dim3(c->sgrid), dim3(kSurfBlock), 0, c->stream, a, c->d_alpha_tmp);
         launch_volume_sc<N, true>(c, a, kind);
@@ -324,10 +338,10 @@ void launch_volume(tlw_ctx* c, const KArgs& a, int kind) {
 void launch_finalize(tlw_ctx* c, int reduce, int use_vol) {
     if (c->N == 4)
         hipLaunchKernelGGL(k_finalize<4>, dim3(c->ninst), dim3(kFinalizeBlock), 0, c->stream, c->d_state,
-                           c->d_partial, c->d_rng, use_vol, reduce);
+                           c->d_partial, c->d_rng, use_vol, reduce, c->pv_off);
     else
         hipLaunchKernelGGL(k_finalize<3>, dim3(c->ninst), dim3(kFinalizeBlock), 0, c->stream, c->d_state,
-                           c->d_partial, c->d_rng, use_vol, reduce);
+                           c->d_partial, c->d_rng, use_vol, reduce, c->pv_off);
 }
 template <int N> void launch_surface_t(tlw_ctx* c, const KArgs& a) {
     dim3 g(c->sgrid), b(kSurfBlock);
@@ -523,7 +537,7 @@ bool any_bad(const tlw_ctx* c) {
 }
 
 int create_common(int ninst, const tlw_mesh_desc* meshes, const tlw_basis_desc* basis, const tlw_solver_desc* solver,
-                  tlw_ctx** out);
+                  tlw_ctx** out, const tlw_parabolic_desc* par = nullptr);
 
 }  // namespace
 
@@ -532,6 +546,15 @@ extern "C" {
 int tlw_create(const tlw_mesh_desc* mesh, const tlw_basis_desc* basis, const tlw_solver_desc* solver,
                tlw_ctx** out) {
     return create_common(1, mesh, basis, solver, out);
+}
+
+int tlw_create_parabolic(const tlw_mesh_desc* mesh, const tlw_basis_desc* basis, const tlw_solver_desc* solver,
+                         const tlw_parabolic_desc* parabolic, tlw_ctx** out) {
+    if (!parabolic) {
+        if (out) *out = nullptr;
+        return TLW_EINVAL;
+    }
+    return create_common(1, mesh, basis, solver, out, parabolic);
 }
 
 int tlw_create_batch(int32_t n_instances, const tlw_mesh_desc* meshes, const tlw_basis_desc* basis,
@@ -747,7 +770,7 @@ int tlw_adjoint_lw(tlw_ctx* c, const double* u, const double* lam_next, double* 
     if (!c) return TLW_EINVAL;
     if (!u || !lam_next || !lam) return fail(c, TLW_EINVAL, "null state pointer");
     if (!std::isfinite(dt) || !std::isfinite(t)) return fail(c, TLW_EINVAL, "t and dt must be finite");
-    if (c->time_disc != TLW_LW || c->sc || c->has_boundary || c->hconst.flux_order != TLW_FLUX_TREE ||
+    if (c->time_disc != TLW_LW || c->sc || c->parabolic || c->has_boundary || c->hconst.flux_order != TLW_FLUX_TREE ||
         c->ninst != 1)
         return fail(c, TLW_ENOTSUP, "the adjoint covers plain LW on one periodic mesh with the TreeMesh flux order");
     if (lam == lam_next || lam == u) return fail(c, TLW_EINVAL, "lam must not alias lam_next or u");
@@ -775,6 +798,8 @@ int tlw_set_positivity_limiter(tlw_ctx* c, int32_t enable, double threshold_dens
     if (!c) return TLW_EINVAL;
     if (enable && (std::isnan(threshold_density) || std::isnan(threshold_pressure)))
         return fail(c, TLW_EINVAL, "positivity thresholds must not be NaN");
+    if (enable && c->parabolic)
+        return fail(c, TLW_ENOTSUP, "the positivity limiter is not combined with the hyperbolic-parabolic path");
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if ((enable != 0) != c->positivity) drop_graphs(c);
     c->positivity = enable != 0;
@@ -901,7 +926,7 @@ int tlw_get_aux(tlw_ctx* c, int32_t which, double* out, size_t len) {
 namespace {
 
 int create_common(int ninst, const tlw_mesh_desc* meshes, const tlw_basis_desc* basis, const tlw_solver_desc* solver,
-                  tlw_ctx** out) {
+                  tlw_ctx** out, const tlw_parabolic_desc* par) {
     if (!out) return TLW_EINVAL;
     *out = nullptr;
     if (!meshes || !basis || !solver) return TLW_EINVAL;
@@ -947,6 +972,21 @@ int create_common(int ninst, const tlw_mesh_desc* meshes, const tlw_basis_desc* 
         h.alpha_max = solver->alpha_max;
         h.alpha_min = solver->alpha_min;
         h.alpha_smooth = solver->alpha_smooth;
+    }
+    if (par) {
+        // CompressibleNavierStokesDiffusion2D(equations; mu, Prandtl,
+        // gradient_variables = GradientVariablesConservative()), constant μ
+        const bool ok = par->equations == TLW_PAR_NAVIER_STOKES && par->gradient_variables == TLW_GRAD_CONSERVATIVE &&
+                        std::isfinite(par->mu) && par->mu >= 0.0 && std::isfinite(par->prandtl) && par->prandtl > 0.0;
+        const bool supported = !c->sc && ninst == 1;
+        if (!ok || !supported) {
+            delete c;
+            return ok ? TLW_ENOTSUP : TLW_EINVAL;
+        }
+        c->parabolic = true;
+        h.visc_mu = par->mu;
+        h.visc_Pr = par->prandtl;
+        h.visc_kappa = solver->gamma * (1.0 / (solver->gamma - 1.0)) / par->prandtl;
     }
     auto bail = [&](int rc) {
         tlw_destroy(c);

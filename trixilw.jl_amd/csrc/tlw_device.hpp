@@ -69,6 +69,9 @@ struct DevConst {
     int alpha_smooth;
     // positivity limiter thresholds for density and pressure
     double pos_threshold[2];
+    // CompressibleNavierStokesDiffusion2D (conservative gradient variables):
+    // dynamic viscosity, Prandtl number, κ = γ/((γ-1) Pr)
+    double visc_mu, visc_Pr, visc_kappa;
 };
 
 // Device-resident integrator state (LWIntegrator fields t, dt, iter and the
@@ -115,6 +118,12 @@ struct KArgs {
     const int* etile;  // [E] volume workgroup of each element
     const double* alpha_tmp;  // indicator values before smoothing [E]
     double* alpha_out;        // smoothed blending factors [E] (optional)
+    // hyperbolic-parabolic CFL reduction: the per-workgroup maxima of the
+    // viscous bound J⁻² max(4μ/(3ρ), μκ/(ρ Pr)) sit pv_off entries after the
+    // advective ones (0: no parabolic terms); elem_vmax is elem_max's
+    // counterpart for that bound
+    int pv_off;
+    double* elem_vmax;  // [E]
     double t_arg, dt_arg;
     int use_state_dt;  // 1: t, dt from DevState (device loop); 0: from t_arg/dt_arg
     int step_mode;     // 1: out = u + dt * rhs; 0: out = rhs
@@ -776,6 +785,460 @@ __device__ __forceinline__ void mdrk2_element(const double* uN, const double* us
 }
 
 // ---------------------------------------------------------------------------
+// Hyperbolic-parabolic element procedures (compressible Navier-Stokes with
+// conservative gradient variables).  The BR1 gradient of the input state is
+// formed in the volume workgroup itself (weak-form D̂ volume term, central
+// face value ½(u + u_neighbour) lifted by 1/w at the end nodes, ×J⁻¹), so no
+// gradient array goes through HBM.  The LW procedure then runs on the total
+// flux f_a(u) - f_v(u, ∇u) with Taylor states of u and of ∇u, the time
+// derivatives' gradients being element-local (J⁻¹ D ∂ₜᵏu).  Only the total
+// F̃ = F̃a - F̃v is kept: the interface flux of the reference,
+// LW-Rusanov(F̃a) - ½(F̃v_L + F̃v_R), equals ½(F̃_L + F̃_R) - ½λ(u_R - u_L),
+// so the face records and the surface kernel are those of the Euler path.
+// Reference: lw_volume_kernel_3!/4!, dg_2d_parabolic.jl:725-1860;
+// calc_interface_flux_hyperbolic_parabolic!, dg_2d_parabolic.jl:1911-1948;
+// mdrk_kernel_1!/2!, multi_derivative_2d_parabolic.jl:221-711.
+
+// Separate x and y contractions of the 4 variables in s (stride S):
+// ox[v] = Σ_k M[i,k] s[v][j,k], oy[v] = Σ_k M[j,k] s[v][k,i].
+template <int n, int S>
+__device__ __forceinline__ void contract_xy(const double* s, int base, int i, int j, const double* M, double* ox,
+                                            double* oy) {
+    double Mi[n], Mj[n];
+#pragma unroll
+    for (int k = 0; k < n; ++k) {
+        Mi[k] = M[i * n + k];
+        Mj[k] = M[j * n + k];
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        double ax = 0.0, ay = 0.0;
+#pragma unroll
+        for (int k = 0; k < n; ++k) {
+            ax = fma(Mi[k], s[v * S + base + j * n + k], ax);
+            ay = fma(Mj[k], s[v * S + base + k * n + i], ay);
+        }
+        ox[v] = ax;
+        oy[v] = ay;
+    }
+}
+
+template <int S> __device__ __forceinline__ void put4(double* s, int idx, const double* w) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) s[v * S + idx] = w[v];
+}
+
+// Viscous fluxes of CompressibleNavierStokesDiffusion2D from the conservative
+// state and conservative gradients (convert_derivative_to_primitive,
+// compressible_navier_stokes_2d.jl:37-57; Trixi flux(u, ∇u, orientation)).
+__device__ __forceinline__ void ns_visc_fluxes(const double* u, const double* gx, const double* gy, double gm1,
+                                               const DevConst* c, double* fx, double* fy) {
+    const double ir = rcp_nr(u[0]);
+    const double v1 = u[1] * ir, v2 = u[2] * ir;
+    const double p = gm1 * (u[3] - 0.5 * (u[1] * v1 + u[2] * v2));
+    const double T = p * ir;
+    const double vv = v1 * v1 + v2 * v2;
+    auto prim = [&](const double* W, double& v1d, double& v2d, double& Td) {
+        const double rd = W[0];
+        v1d = (W[1] - rd * v1) * ir;
+        v2d = (W[2] - rd * v2) * ir;
+        const double pd = (W[3] - 0.5 * rd * vv - u[0] * (v1 * v1d + v2 * v2d)) * gm1;
+        Td = (pd - rd * T) * ir;
+    };
+    double v1x, v2x, Tx, v1y, v2y, Ty;
+    prim(gx, v1x, v2x, Tx);
+    prim(gy, v1y, v2y, Ty);
+    const double mu = c->visc_mu, kappa = c->visc_kappa;
+    const double t11 = (4.0 / 3.0) * v1x - (2.0 / 3.0) * v2y;
+    const double t12 = v1y + v2x;
+    const double t22 = (4.0 / 3.0) * v2y - (2.0 / 3.0) * v1x;
+    fx[0] = 0.0;
+    fx[1] = t11 * mu;
+    fx[2] = t12 * mu;
+    fx[3] = (v1 * t11 + v2 * t12 + kappa * Tx) * mu;
+    fy[0] = 0.0;
+    fy[1] = t12 * mu;
+    fy[2] = t22 * mu;
+    fy[3] = (v1 * t12 + v2 * t22 + kappa * Ty) * mu;
+}
+
+// f = f_a(u) - f_v(u, ∇u)
+__device__ __forceinline__ void ns_total_fluxes(const double* u, const double* gx, const double* gy, double gm1,
+                                                const DevConst* c, double* f, double* g) {
+    double fx[4], fy[4];
+    euler_fluxes(u, gm1, f, g);
+    ns_visc_fluxes(u, gx, gy, gm1, c, fx, fy);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        f[v] -= fx[v];
+        g[v] -= fy[v];
+    }
+}
+
+// Viscous time-step bound of one node, J⁻² max(4μ/(3ρ), μκ/(ρ Pr))
+// (compressible_navier_stokes_2d.jl:230-237), with IEEE divisions.
+__device__ __forceinline__ double lam_visc(const double* u, double invJ, const DevConst* c) {
+    const double mu = c->visc_mu, rho = u[0];
+    return invJ * invJ * dmax(4.0 * mu / (3.0 * rho), mu * c->visc_kappa / (rho * c->visc_Pr));
+}
+
+// BR1 gradient of the node's state: s holds the element's state (put4 before
+// the barrier), src the state array the face neighbours are read from.
+template <int N, int EB>
+__device__ __forceinline__ void br1_gradient(const double* s, const double* src, const int* nbr, const DevConst* c,
+                                             int e, int i, int j, int sbase, double invJ, const double* sDh,
+                                             const double* uown, double* gx, double* gy) {
+    using G_ = VGeo<N, EB>;
+    constexpr int n = G_::n, NN = G_::NN, S = G_::S;
+    contract_xy<n, S>(s, sbase, i, j, sDh, gx, gy);
+    if (i == 0 || i == n - 1) {
+        const int f = (i == 0) ? 0 : 1;
+        const int nb = nbr[4 * e + f];
+        if (nb >= 0) {
+            double un[4];
+            load4(src + ((size_t)nb * NN + j * n + (n - 1 - i)) * 4, un);
+            const double wf = f ? c->bnd_hi : -c->bnd_lo;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) gx[v] = fma(0.5 * (uown[v] + un[v]), wf, gx[v]);
+        }
+    }
+    if (j == 0 || j == n - 1) {
+        const int f = (j == 0) ? 2 : 3;
+        const int nb = nbr[4 * e + f];
+        if (nb >= 0) {
+            double un[4];
+            load4(src + ((size_t)nb * NN + (n - 1 - j) * n + i) * 4, un);
+            const double wf = (f & 1) ? c->bnd_hi : -c->bnd_lo;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) gy[v] = fma(0.5 * (uown[v] + un[v]), wf, gy[v]);
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        gx[v] *= invJ;
+        gy[v] *= invJ;
+    }
+}
+
+// Taylor states of u and ∇u: q = 0..3 for u+, u-, u++, u--.
+struct ParStates {
+    double u[4][4], x[4][4], y[4][4];
+    __device__ __forceinline__ void init(const double* w, const double* wx, const double* wy) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                u[q][v] = w[v];
+                x[q][v] = wx[v];
+                y[q][v] = wy[v];
+            }
+    }
+    // states += c[q] · (w, wx, wy)
+    __device__ __forceinline__ void add(double c0, double c1, double c2, double c3, const double* w,
+                                        const double* wx, const double* wy) {
+        const double cq[4] = {c0, c1, c2, c3};
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                u[q][v] = fma(cq[q], w[v], u[q][v]);
+                x[q][v] = fma(cq[q], wx[v], x[q][v]);
+                y[q][v] = fma(cq[q], wy[v], y[q][v]);
+            }
+    }
+    __device__ __forceinline__ void flux(int q, double gm1, const DevConst* c, double* f, double* g) const {
+        ns_total_fluxes(u[q], x[q], y[q], gm1, c, f, g);
+    }
+};
+
+// Time derivative of the next level and its element-local gradient:
+// w = J⁻¹ (-dt D) · (dx, dy) through sa, then (wx, wy) = J⁻¹ D w through sg.
+template <int N, int EB>
+__device__ __forceinline__ void par_level(bool act, int sidx, int sbase, int i, int j, double invJ, double* sa,
+                                          double* sg, const double* sDm, const double* sD, const double* dx,
+                                          const double* dy, double* w, double* wx, double* wy) {
+    using G_ = VGeo<N, EB>;
+    constexpr int n = G_::n, S = G_::S;
+    if (act) put8<S>(sa, sidx, dx, dy);
+    __syncthreads();
+    contract<n, S>(sa, sbase, i, j, sDm, w);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) w[v] *= invJ;
+    if (act) put4<S>(sg, sidx, w);
+    __syncthreads();
+    contract_xy<n, S>(sg, sbase, i, j, sD, wx, wy);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        wx[v] *= invJ;
+        wy[v] *= invJ;
+    }
+}
+
+// lw_volume_kernel_3!/4! with viscous terms (dg_2d_parabolic.jl:725-1189,
+// 1191-1860): the Euler procedure of lw3_element/lw4_element on the total
+// flux, with the states of ∇u advanced by the same Taylor coefficients.
+// Buffer use: s0 keeps f(uⁿ) for the 5- and 3-point stencils, s1 takes each
+// flux derivative, s2 each time derivative; the final F̃ exchange is in s2.
+template <int N, int EB>
+__device__ __forceinline__ void lwp_element(const double* uN, const double* gxN, const double* gyN, double invJ,
+                                            double gm1, const DevConst* c, int i, int j, bool act, int sidx,
+                                            int sbase, double* s0, double* s1, double* s2, const double* sDm,
+                                            const double* sD, const double* sDh, double* Uc, double* F, double* G,
+                                            double* vol) {
+    using G_ = VGeo<N, EB>;
+    constexpr int n = G_::n, S = G_::S;
+    ns_total_fluxes(uN, gxN, gyN, gm1, c, F, G);
+    double w[4], wx[4], wy[4];
+    par_level<N, EB>(act, sidx, sbase, i, j, invJ, s0, s2, sDm, sD, F, G, w, wx, wy);  // ut
+    ParStates P;
+    P.init(uN, gxN, gyN);
+    P.add(1.0, -1.0, 2.0, -2.0, w, wx, wy);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) Uc[v] = fma(0.5, w[v], uN[v]);
+    double fa[4], ga[4], dx[4], dy[4];
+    // stage 1: f_t = (-f(u++) + 8 f(u+) - 8 f(u-) + f(u--)) / 12
+    P.flux(2, gm1, c, fa, ga);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) { dx[v] = -fa[v]; dy[v] = -ga[v]; }
+    P.flux(0, gm1, c, fa, ga);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) { dx[v] = fma(8.0, fa[v], dx[v]); dy[v] = fma(8.0, ga[v], dy[v]); }
+    P.flux(1, gm1, c, fa, ga);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) { dx[v] = fma(-8.0, fa[v], dx[v]); dy[v] = fma(-8.0, ga[v], dy[v]); }
+    P.flux(3, gm1, c, fa, ga);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        dx[v] = (1.0 / 12.0) * (dx[v] + fa[v]);
+        dy[v] = (1.0 / 12.0) * (dy[v] + ga[v]);
+        F[v] = fma(0.5, dx[v], F[v]);
+        G[v] = fma(0.5, dy[v], G[v]);
+    }
+    par_level<N, EB>(act, sidx, sbase, i, j, invJ, s1, s2, sDm, sD, dx, dy, w, wx, wy);  // utt
+    P.add(0.5, 0.5, 2.0, 2.0, w, wx, wy);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) Uc[v] = fma(1.0 / 6.0, w[v], Uc[v]);
+    if constexpr (N == 4) {
+        // stage 2: f_tt = (-f(u++) + 16 f(u+) - 30 f + 16 f(u-) - f(u--)) / 12
+        P.flux(2, gm1, c, fa, ga);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) { dx[v] = -fa[v]; dy[v] = -ga[v]; }
+        P.flux(0, gm1, c, fa, ga);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            dx[v] = fma(16.0, fa[v], dx[v]);
+            dy[v] = fma(16.0, ga[v], dy[v]);
+            dx[v] = fma(-30.0, s0[v * S + sidx], dx[v]);
+            dy[v] = fma(-30.0, s0[(4 + v) * S + sidx], dy[v]);
+        }
+        P.flux(1, gm1, c, fa, ga);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) { dx[v] = fma(16.0, fa[v], dx[v]); dy[v] = fma(16.0, ga[v], dy[v]); }
+        P.flux(3, gm1, c, fa, ga);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            dx[v] = (1.0 / 12.0) * (dx[v] - fa[v]);
+            dy[v] = (1.0 / 12.0) * (dy[v] - ga[v]);
+        }
+    } else {
+        // stage 2: f_tt = f(u+) - 2 f + f(u-)
+        P.flux(0, gm1, c, fa, ga);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            dx[v] = fma(-2.0, s0[v * S + sidx], fa[v]);
+            dy[v] = fma(-2.0, s0[(4 + v) * S + sidx], ga[v]);
+        }
+        P.flux(1, gm1, c, fa, ga);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) { dx[v] += fa[v]; dy[v] += ga[v]; }
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        F[v] = fma(1.0 / 6.0, dx[v], F[v]);
+        G[v] = fma(1.0 / 6.0, dy[v], G[v]);
+    }
+    par_level<N, EB>(act, sidx, sbase, i, j, invJ, s1, s2, sDm, sD, dx, dy, w, wx, wy);  // uttt
+    P.add(1.0 / 6.0, -1.0 / 6.0, 4.0 / 3.0, -4.0 / 3.0, w, wx, wy);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) Uc[v] = fma(1.0 / 24.0, w[v], Uc[v]);
+    // stage 3: f_ttt = (f(u++) - 2 f(u+) + 2 f(u-) - f(u--)) / 2
+    P.flux(2, gm1, c, fa, ga);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) { dx[v] = fa[v]; dy[v] = ga[v]; }
+    P.flux(0, gm1, c, fa, ga);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) { dx[v] = fma(-2.0, fa[v], dx[v]); dy[v] = fma(-2.0, ga[v], dy[v]); }
+    P.flux(1, gm1, c, fa, ga);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) { dx[v] = fma(2.0, fa[v], dx[v]); dy[v] = fma(2.0, ga[v], dy[v]); }
+    P.flux(3, gm1, c, fa, ga);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        dx[v] = 0.5 * (dx[v] - fa[v]);
+        dy[v] = 0.5 * (dy[v] - ga[v]);
+        F[v] = fma(1.0 / 24.0, dx[v], F[v]);
+        G[v] = fma(1.0 / 24.0, dy[v], G[v]);
+    }
+    if constexpr (N == 4) {
+        par_level<N, EB>(act, sidx, sbase, i, j, invJ, s1, s2, sDm, sD, dx, dy, w, wx, wy);  // utttt
+        P.add(1.0 / 24.0, 1.0 / 24.0, 2.0 / 3.0, 2.0 / 3.0, w, wx, wy);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Uc[v] = fma(1.0 / 120.0, w[v], Uc[v]);
+        // stage 4: f_tttt = 0.5 (f(u++) - 4 f(u+) + 6 f - 4 f(u-) + f(u--)), with
+        // the reference's factor 0.5 (dg_2d_parabolic.jl:1746-1747)
+        P.flux(2, gm1, c, fa, ga);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) { dx[v] = fa[v]; dy[v] = ga[v]; }
+        P.flux(0, gm1, c, fa, ga);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            dx[v] = fma(-4.0, fa[v], dx[v]);
+            dy[v] = fma(-4.0, ga[v], dy[v]);
+            dx[v] = fma(6.0, s0[v * S + sidx], dx[v]);
+            dy[v] = fma(6.0, s0[(4 + v) * S + sidx], dy[v]);
+        }
+        P.flux(1, gm1, c, fa, ga);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) { dx[v] = fma(-4.0, fa[v], dx[v]); dy[v] = fma(-4.0, ga[v], dy[v]); }
+        P.flux(3, gm1, c, fa, ga);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            dx[v] = 0.5 * (dx[v] + fa[v]);
+            dy[v] = 0.5 * (dy[v] + ga[v]);
+            F[v] = fma(1.0 / 120.0, dx[v], F[v]);
+            G[v] = fma(1.0 / 120.0, dy[v], G[v]);
+        }
+    }
+    __syncthreads();  // every thread has finished reading s2 before it is reused
+    if (act) put8<S>(s2, sidx, F, G);
+    __syncthreads();
+    contract<n, S>(s2, sbase, i, j, sDh, vol);
+}
+
+// f_t of the states w ± wt, w ± 2wt (and their gradients) for the MDRK stages
+__device__ __forceinline__ void mdrkp_ft(const double* u, const double* ut, const double* gx, const double* gtx,
+                                         const double* gy, const double* gty, double gm1, const DevConst* c,
+                                         double* dx, double* dy) {
+    const double cq[4] = {2.0, 1.0, -1.0, -2.0};
+    const double wq[4] = {-1.0, 8.0, -8.0, 1.0};
+#pragma unroll
+    for (int v = 0; v < 4; ++v) dx[v] = dy[v] = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        double s[4], sx[4], sy[4], fa[4], ga[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            s[v] = fma(cq[q], ut[v], u[v]);
+            sx[v] = fma(cq[q], gtx[v], gx[v]);
+            sy[v] = fma(cq[q], gty[v], gy[v]);
+        }
+        ns_total_fluxes(s, sx, sy, gm1, c, fa, ga);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            dx[v] = fma(wq[q], fa[v], dx[v]);
+            dy[v] = fma(wq[q], ga[v], dy[v]);
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        dx[v] *= 1.0 / 12.0;
+        dy[v] *= 1.0 / 12.0;
+    }
+}
+
+// mdrk_kernel_1! with viscous terms (multi_derivative_2d_parabolic.jl:221-464):
+// F̃ = f/2 + f_t/8, Ũ = u/2 + ut/8 on the total flux.
+template <int N, int EB>
+__device__ __forceinline__ void mdrkp1_element(const double* uN, const double* gxN, const double* gyN, double invJ,
+                                               double gm1, const DevConst* c, int i, int j, bool act, int sidx,
+                                               int sbase, double* s0, double* s1, double* s2, const double* sDm,
+                                               const double* sD, const double* sDh, double* Uc, double* F,
+                                               double* G, double* vol) {
+    using G_ = VGeo<N, EB>;
+    constexpr int n = G_::n, S = G_::S;
+    double f[4], g[4];
+    ns_total_fluxes(uN, gxN, gyN, gm1, c, f, g);
+    double ut[4], utx[4], uty[4];
+    par_level<N, EB>(act, sidx, sbase, i, j, invJ, s0, s2, sDm, sD, f, g, ut, utx, uty);
+    double dx[4], dy[4];
+    mdrkp_ft(uN, ut, gxN, utx, gyN, uty, gm1, c, dx, dy);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        F[v] = fma(0.125, dx[v], 0.5 * f[v]);
+        G[v] = fma(0.125, dy[v], 0.5 * g[v]);
+        Uc[v] = fma(0.125, ut[v], 0.5 * uN[v]);
+    }
+    __syncthreads();  // every thread has finished reading s2 before it is reused
+    if (act) put8<S>(s2, sidx, F, G);
+    __syncthreads();
+    contract<n, S>(s2, sbase, i, j, sDh, vol);
+}
+
+// mdrk_kernel_2! with viscous terms (multi_derivative_2d_parabolic.jl:506-711):
+// F̃ = F2 + f_t(us)/3, Ũ = U2 + ust/3, with F2 = f + f_t/6 and U2 = u + ut/6
+// recomputed from uⁿ and its BR1 gradient.
+template <int N, int EB>
+__device__ __forceinline__ void mdrkp2_element(const double* uN, const double* gxN, const double* gyN,
+                                               const double* usn, const double* gxS, const double* gyS,
+                                               double invJ, double gm1, const DevConst* c, int i, int j, bool act,
+                                               int sidx, int sbase, double* s0, double* s1, double* s2,
+                                               const double* sDm, const double* sD, const double* sDh, double* Uc,
+                                               double* F, double* G, double* vol) {
+    using G_ = VGeo<N, EB>;
+    constexpr int n = G_::n, S = G_::S;
+    double f[4], g[4], fs[4], gs[4];
+    ns_total_fluxes(uN, gxN, gyN, gm1, c, f, g);
+    ns_total_fluxes(usn, gxS, gyS, gm1, c, fs, gs);
+    if (act) {
+        put8<S>(s0, sidx, f, g);
+        put8<S>(s1, sidx, fs, gs);
+    }
+    __syncthreads();
+    double ut[4], ust[4];
+    contract<n, S>(s0, sbase, i, j, sDm, ut);
+    contract<n, S>(s1, sbase, i, j, sDm, ust);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        ut[v] *= invJ;
+        ust[v] *= invJ;
+    }
+    if (act) {
+        put4<S>(s2, sidx, ut);
+        put4<S>(s2 + 4 * S, sidx, ust);
+    }
+    __syncthreads();
+    double utx[4], uty[4], stx[4], sty[4];
+    contract_xy<n, S>(s2, sbase, i, j, sD, utx, uty);
+    contract_xy<n, S>(s2 + 4 * S, sbase, i, j, sD, stx, sty);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        utx[v] *= invJ;
+        uty[v] *= invJ;
+        stx[v] *= invJ;
+        sty[v] *= invJ;
+    }
+    double dx[4], dy[4];
+    mdrkp_ft(uN, ut, gxN, utx, gyN, uty, gm1, c, dx, dy);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        F[v] = fma(1.0 / 6.0, dx[v], f[v]);
+        G[v] = fma(1.0 / 6.0, dy[v], g[v]);
+        Uc[v] = fma(1.0 / 6.0, ut[v], uN[v]);
+    }
+    mdrkp_ft(usn, ust, gxS, stx, gyS, sty, gm1, c, dx, dy);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        F[v] = fma(1.0 / 3.0, dx[v], F[v]);
+        G[v] = fma(1.0 / 3.0, dy[v], G[v]);
+        Uc[v] = fma(1.0 / 3.0, ust[v], Uc[v]);
+    }
+    __syncthreads();  // every thread has finished reading s2 before it is reused
+    if (act) put8<S>(s2, sidx, F, G);
+    __syncthreads();
+    contract<n, S>(s2, sbase, i, j, sDh, vol);
+}
+
+// ---------------------------------------------------------------------------
 // Volume kernel: element computation, then the interfaces inside the
 // workgroup, the lifted surface terms (weak form, ℓ̂ = 1/w at the end nodes),
 // the Jacobian and the update.  Faces leaving the workgroup write their face
@@ -788,14 +1251,17 @@ __device__ __forceinline__ void mdrk2_element(const double* uN, const double* us
 #ifndef TLW_VOL_WAVES
 #define TLW_VOL_WAVES 3
 #endif
-template <bool SC> constexpr int kVolWaves = SC ? 2 : TLW_VOL_WAVES;
+template <bool TWO> constexpr int kVolWaves = TWO ? 2 : TLW_VOL_WAVES;
 
-template <int N, int EB, int KIND, bool SC>
-__global__ __launch_bounds__((VGeo<N, EB>::NT), kVolWaves<SC>) void k_volume(KArgs a) {
+// PAR: hyperbolic-parabolic (compressible Navier-Stokes) element procedures,
+// same register budget as shock capturing.
+template <int N, int EB, int KIND, bool SC, bool PAR = false>
+__global__ __launch_bounds__((VGeo<N, EB>::NT), kVolWaves<SC || PAR>) void k_volume(KArgs a) {
+    static_assert(!(SC && PAR), "shock capturing has no hyperbolic-parabolic form");
     using G_ = VGeo<N, EB>;
     constexpr int n = G_::n, NN = G_::NN, S = G_::S, NT = G_::NT;
     __shared__ double lds[G_::LDS];
-    __shared__ double sDm[NN], sDh[NN];  // -dt D and D̂, row-major
+    __shared__ double sDm[NN], sDh[NN], sD[NN];  // -dt D, D̂ and D, row-major
     double* s0 = lds;
     double* s1 = lds + 8 * S;
     double* s2 = lds + 16 * S;
@@ -823,10 +1289,36 @@ __global__ __launch_bounds__((VGeo<N, EB>::NT), kVolWaves<SC>) void k_volume(KAr
     if (tid < NN) {  // published by the first barrier of the element computation
         sDm[tid] = -dt * c->D[tid / n][tid % n];
         sDh[tid] = c->Dhat[tid / n][tid % n];
+        if constexpr (PAR) sD[tid] = c->D[tid / n][tid % n];
     }
     double uN[4], Uc[4], F[4], G[4], vol[4];
     load4(uin + off, uN);
-    if (KIND == VOL_LW) {
+    if constexpr (PAR) {
+        // BR1 gradients of uⁿ (and of us in MDRK stage 2), exchanged through a
+        // buffer the element procedure writes only after its first barrier
+        double* sg = (KIND == VOL_MDRK2) ? s2 : s1;
+        double gxN[4], gyN[4];
+        if (act) put4<S>(sg, sidx, uN);
+        if constexpr (KIND == VOL_MDRK2) {
+            double usn[4], gxS[4], gyS[4];
+            load4(a.us + off, usn);
+            if (act) put4<S>(sg + 4 * S, sidx, usn);
+            __syncthreads();
+            br1_gradient<N, EB>(sg, uin, a.nbr, c, e, i, j, sbase, invJ, sDh, uN, gxN, gyN);
+            br1_gradient<N, EB>(sg + 4 * S, a.us, a.nbr, c, e, i, j, sbase, invJ, sDh, usn, gxS, gyS);
+            mdrkp2_element<N, EB>(uN, gxN, gyN, usn, gxS, gyS, invJ, gm1, c, i, j, act, sidx, sbase, s0, s1, s2, sDm,
+                                  sD, sDh, Uc, F, G, vol);
+        } else {
+            __syncthreads();
+            br1_gradient<N, EB>(sg, uin, a.nbr, c, e, i, j, sbase, invJ, sDh, uN, gxN, gyN);
+            if constexpr (KIND == VOL_LW)
+                lwp_element<N, EB>(uN, gxN, gyN, invJ, gm1, c, i, j, act, sidx, sbase, s0, s1, s2, sDm, sD, sDh, Uc, F,
+                                   G, vol);
+            else
+                mdrkp1_element<N, EB>(uN, gxN, gyN, invJ, gm1, c, i, j, act, sidx, sbase, s0, s1, s2, sDm, sD, sDh,
+                                      Uc, F, G, vol);
+        }
+    } else if (KIND == VOL_LW) {
         if constexpr (N == 4)
             lw4_element<N, EB>(uN, dt, invJ, gm1, c, i, j, act, sidx, sbase, s0, s1, s2, sDm, sDh, Uc, F, G, vol);
         else
@@ -991,19 +1483,24 @@ __global__ __launch_bounds__((VGeo<N, EB>::NT), kVolWaves<SC>) void k_volume(KAr
     if (act) store4(out + off, val);
     if (a.reduce == RED_NONE) return;
     // --- CFL reduction over the nodes finalised here (records are dead: reuse s0)
-    double l1 = -1.0, l2 = -1.0;
+    double l1 = -1.0, l2 = -1.0, lv = -1.0;
     if (fin_node) {
         bool bad;
         node_speeds(val, gamma, l1, l2, bad);
         if (act && bad) st->bad_state = 1;
+        if constexpr (PAR) lv = lam_visc(val, invJ, c);
     }
     double* sl1 = lds;
     double* sl2 = lds + S;
     double* rm = lds + 2 * S;            // row maxima [EB][n][2]
     double* sm = rm + 2 * EB * n;        // element values [EB]
+    double* slv = lds + 3 * S;           // viscous bound: node values, row maxima, element values
+    double* rmv = lds + 4 * S;
+    double* smv = rmv + EB * n;
     if (act) {
         sl1[sidx] = l1;
         sl2[sidx] = l2;
+        if constexpr (PAR) slv[sidx] = lv;
     }
     __syncthreads();
     if (act && i == 0) {
@@ -1015,14 +1512,21 @@ __global__ __launch_bounds__((VGeo<N, EB>::NT), kVolWaves<SC>) void k_volume(KAr
         }
         rm[(lec * n + j) * 2] = m1;
         rm[(lec * n + j) * 2 + 1] = m2;
+        if constexpr (PAR) {
+            double mv = 0.0;
+#pragma unroll
+            for (int k = 0; k < n; ++k) mv = dmax(mv, slv[sbase + j * n + k]);
+            rmv[lec * n + j] = mv;
+        }
     }
     __syncthreads();
     if (act && node == 0) {
-        double m1 = 0.0, m2 = 0.0;
+        double m1 = 0.0, m2 = 0.0, mv = 0.0;
 #pragma unroll
         for (int k = 0; k < n; ++k) {
             m1 = dmax(m1, rm[(lec * n + k) * 2]);
             m2 = dmax(m2, rm[(lec * n + k) * 2 + 1]);
+            if constexpr (PAR) mv = dmax(mv, rmv[lec * n + k]);
         }
         bool whole = true;
 #pragma unroll
@@ -1032,17 +1536,26 @@ __global__ __launch_bounds__((VGeo<N, EB>::NT), kVolWaves<SC>) void k_volume(KAr
         }
         if (whole) {
             sm[le] = invJ * (m1 + m2);
+            if constexpr (PAR) smv[le] = mv;
         } else {
             sm[le] = 0.0;
             a.elem_max[2 * e] = m1;
             a.elem_max[2 * e + 1] = m2;
+            if constexpr (PAR) {
+                smv[le] = 0.0;
+                a.elem_vmax[e] = mv;
+            }
         }
     }
     __syncthreads();
     if (tid == 0) {
-        double bm = 0.0;
-        for (int k = 0; k < nel; ++k) bm = dmax(bm, sm[k]);
+        double bm = 0.0, bv = 0.0;
+        for (int k = 0; k < nel; ++k) {
+            bm = dmax(bm, sm[k]);
+            if constexpr (PAR) bv = dmax(bv, smv[k]);
+        }
         a.partial[blockIdx.x] = bm;
+        if constexpr (PAR) a.partial[a.pv_off + blockIdx.x] = bv;
     }
 }
 
@@ -1091,7 +1604,7 @@ template <int N, bool BC, bool SC>
 __global__ __launch_bounds__(kSurfBlock) void k_surface(KArgs a) {
     using G_ = SGeo<N>;
     constexpr int n = G_::n, NN = G_::NN, EB = G_::EB, S = G_::S;
-    __shared__ double s_l1[S], s_l2[S], s_m[EB];
+    __shared__ double s_l1[S], s_l2[S], s_m[EB], s_lv[S], s_mv[EB];
     const int4 blk = load_blk(a.sblk);
     DevState* st = a.st + blk.z;
     if (a.use_state_dt && st->finished) return;
@@ -1161,19 +1674,22 @@ __global__ __launch_bounds__(kSurfBlock) void k_surface(KArgs a) {
         store4(out + off, val);
     }
     if (a.reduce == RED_NONE) return;
-    double l1 = -1.0, l2 = -1.0;
+    const bool par = a.pv_off > 0;
+    double l1 = -1.0, l2 = -1.0, lv = -1.0;
     if (act && any_ext) {
         bool bad;
         node_speeds(val, c->gamma, l1, l2, bad);
         if (bad) st->bad_state = 1;
+        if (par) lv = lam_visc(val, invJ, c);
     }
     if (act) {
         s_l1[lec * NN + node] = l1;
         s_l2[lec * NN + node] = l2;
+        s_lv[lec * NN + node] = lv;
     }
     __syncthreads();
     if (act && node == 0) {
-        double m = 0.0;
+        double m = 0.0, mv = 0.0;
         if (elem_ext) {
             double m1 = a.elem_max[2 * e], m2 = a.elem_max[2 * e + 1];
             for (int k = 0; k < NN; ++k) {
@@ -1181,14 +1697,23 @@ __global__ __launch_bounds__(kSurfBlock) void k_surface(KArgs a) {
                 m2 = dmax(m2, s_l2[lec * NN + k]);
             }
             m = invJ * (m1 + m2);
+            if (par) {
+                mv = a.elem_vmax[e];
+                for (int k = 0; k < NN; ++k) mv = dmax(mv, s_lv[lec * NN + k]);
+            }
         }
         s_m[lec] = m;
+        s_mv[lec] = mv;
     }
     __syncthreads();
     if (tid == 0) {
-        double bm = 0.0;
-        for (int k = 0; k < nel; ++k) bm = dmax(bm, s_m[k]);
+        double bm = 0.0, bv = 0.0;
+        for (int k = 0; k < nel; ++k) {
+            bm = dmax(bm, s_m[k]);
+            bv = dmax(bv, s_mv[k]);
+        }
         a.partial[a.vol_grid + blockIdx.x] = bm;
+        if (par) a.partial[a.pv_off + a.vol_grid + blockIdx.x] = bv;
     }
 }
 
@@ -1198,7 +1723,7 @@ template <int N>
 __global__ __launch_bounds__(kSurfBlock) void k_max_speed(KArgs a) {
     using G_ = SGeo<N>;
     constexpr int NN = G_::NN, EB = G_::EB, S = G_::S;
-    __shared__ double s_l1[S], s_l2[S], s_m[EB];
+    __shared__ double s_l1[S], s_l2[S], s_m[EB], s_lv[S], s_mv[EB];
     const int4 blk = load_blk(a.sblk);
     DevState* st = a.st + blk.z;
     const int tid = threadIdx.x;
@@ -1215,24 +1740,32 @@ __global__ __launch_bounds__(kSurfBlock) void k_max_speed(KArgs a) {
     bool bad;
     node_speeds(val, a.c->gamma, l1, l2, bad);
     if (act && bad) st->bad_state = 1;
+    const bool par = a.pv_off > 0;
     if (act) {
         s_l1[lec * NN + node] = l1;
         s_l2[lec * NN + node] = l2;
+        s_lv[lec * NN + node] = par ? lam_visc(val, a.inv_jac[e], a.c) : 0.0;
     }
     __syncthreads();
     if (act && node == 0) {
-        double m1 = 0.0, m2 = 0.0;
+        double m1 = 0.0, m2 = 0.0, mv = 0.0;
         for (int k = 0; k < NN; ++k) {
             m1 = dmax(m1, s_l1[lec * NN + k]);
             m2 = dmax(m2, s_l2[lec * NN + k]);
+            mv = dmax(mv, s_lv[lec * NN + k]);
         }
         s_m[lec] = a.inv_jac[e] * (m1 + m2);
+        s_mv[lec] = mv;
     }
     __syncthreads();
     if (tid == 0) {
-        double bm = 0.0;
-        for (int k = 0; k < nel; ++k) bm = dmax(bm, s_m[k]);
+        double bm = 0.0, bv = 0.0;
+        for (int k = 0; k < nel; ++k) {
+            bm = dmax(bm, s_m[k]);
+            bv = dmax(bv, s_mv[k]);
+        }
         a.partial[a.vol_grid + blockIdx.x] = bm;
+        if (par) a.partial[a.pv_off + a.vol_grid + blockIdx.x] = bv;
     }
 }
 
@@ -1406,38 +1939,54 @@ __global__ __launch_bounds__(kSurfBlock) void k_limiter(KArgs a) {
 // then compute_dt / set_dt! / set_t_and_iter! / isfinished on the instance's
 // state (lwfr.jl:76-103, 146-148, 176-190; Trixi max_dt).
 constexpr int kFinalizeBlock = 1024;
+// With pv_off > 0 the viscous maxima (pv_off entries further on) give the
+// parabolic bound 1/((λa + λv)·2(2N+1)), λv = 2(2N+1)·max J⁻² max(4μ/(3ρ), μκ/(ρPr)),
+// and max_dt is the smaller of the two (compute_dt for
+// SemidiscretizationHyperbolicParabolic, lwfr.jl:192-210).
 template <int N>
 __global__ __launch_bounds__(kFinalizeBlock) void k_finalize(DevState* st_all, const double* partial, const int4* rng,
-                                                           int use_vol, int reduce) {
+                                                           int use_vol, int reduce, int pv_off) {
     constexpr int n = N + 1;
-    __shared__ double s[kFinalizeBlock];
+    __shared__ double s[kFinalizeBlock], sv[kFinalizeBlock];
     DevState* st = st_all + blockIdx.x;
     const int4 r = rng[blockIdx.x];
     if (reduce == RED_STEP && st->finished) return;
     // eight independent accumulators keep several loads in flight per lane
-    double m[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    auto scan = [&](int b, int e) {
+    auto scan = [&](const double* p, int b, int e, double* m) {
         int k = b + (int)threadIdx.x;
         for (; k + 7 * kFinalizeBlock < e; k += 8 * kFinalizeBlock) {
 #pragma unroll
-            for (int q = 0; q < 8; ++q) m[q] = dmax(m[q], partial[k + q * kFinalizeBlock]);
+            for (int q = 0; q < 8; ++q) m[q] = dmax(m[q], p[k + q * kFinalizeBlock]);
         }
-        for (; k < e; k += kFinalizeBlock) m[0] = dmax(m[0], partial[k]);
+        for (; k < e; k += kFinalizeBlock) m[0] = dmax(m[0], p[k]);
     };
-    if (use_vol) scan(r.x, r.y);
-    scan(r.z, r.w);
+    auto reduce_all = [&](const double* p, double* sh) {
+        double m[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        if (use_vol) scan(p, r.x, r.y, m);
+        scan(p, r.z, r.w, m);
 #pragma unroll
-    for (int q = 1; q < 8; ++q) m[0] = dmax(m[0], m[q]);
-    s[threadIdx.x] = m[0];
+        for (int q = 1; q < 8; ++q) m[0] = dmax(m[0], m[q]);
+        sh[threadIdx.x] = m[0];
+    };
+    reduce_all(partial, s);
+    if (pv_off > 0) reduce_all(partial + pv_off, sv);
     __syncthreads();
     for (int w = kFinalizeBlock / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) s[threadIdx.x] = dmax(s[threadIdx.x], s[threadIdx.x + w]);
+        if ((int)threadIdx.x < w) {
+            s[threadIdx.x] = dmax(s[threadIdx.x], s[threadIdx.x + w]);
+            if (pv_off > 0) sv[threadIdx.x] = dmax(sv[threadIdx.x], sv[threadIdx.x + w]);
+        }
         __syncthreads();
     }
     if (threadIdx.x != 0) return;
     const double tiny = 4.9406564584124654e-324;  // nextfloat(0.0), Trixi's start value
     const double M = dmax(tiny, s[0]);
-    const double max_dt = 2.0 / (n * M);
+    double max_dt = 2.0 / (n * M);
+    if (pv_off > 0) {
+        const double lam_v = dmax(tiny, sv[0]) * (2.0 * (2.0 * N + 1.0));
+        const double dtv = 1.0 / ((M + lam_v) * (2 * (2.0 * N + 1.0)));
+        if (dtv < max_dt) max_dt = dtv;
+    }
     if (reduce == RED_QUERY) {
         st->query = max_dt;
         return;

@@ -8,14 +8,16 @@ library; the first call that needs the GPU does, and fails if it is missing.
 from .adjoint import adjoint_step, gradient_lw, kinetic_energy
 from .basis import LobattoLegendreBasis
 from .batch import SemidiscretizationBatch
-from .equations import (CompressibleEulerEquations2D, ConstantState, FourQuadrantRiemann, IsentropicVortex,
-                        LinearScalarAdvectionEquation2D)
+from .equations import (CompressibleEulerEquations2D, CompressibleNavierStokesDiffusion2D, ConstantState,
+                        FourQuadrantRiemann, GradientVariablesConservative, GradientVariablesPrimitive,
+                        IsentropicVortex, LinearScalarAdvectionEquation2D)
 from .integrator import (LWIntegrator, LWSolution, compute_dt, isapprox, isfinished, perform_step, set_dt,
                          set_t_and_iter, solve_lwfr)
 from .limiters import PositivityPreservingLimiterZhangShu, density, pressure
 from .mesh import TreeMesh
 from .semidiscretization import (DGSEM, LW, MDRK, CFLBased, FirstOrderReconstruction, IndicatorHennemannGassner,
-                                 LWUpdate, MUSCLHancockReconstruction, SemidiscretizationHyperbolic, VolumeIntegralFR,
+                                 LWUpdate, MUSCLHancockReconstruction, SemidiscretizationHyperbolic,
+                                 SemidiscretizationHyperbolicParabolic, VolumeIntegralFR,
                                  VolumeIntegralFRShockCapturing, density_pressure, flux_lax_friedrichs,
                                  get_time_discretization, lw_fourier_cfl, rhs, rhs_mdrk1, rhs_mdrk2, semidiscretize,
                                  trixi2lw)
@@ -29,5 +31,6 @@ __all__ = [
     "FirstOrderReconstruction", "IndicatorHennemannGassner", "MUSCLHancockReconstruction",
     "VolumeIntegralFRShockCapturing", "density_pressure", "PositivityPreservingLimiterZhangShu", "density",
     "pressure", "adjoint_step", "gradient_lw", "kinetic_energy",
-    "SemidiscretizationBatch",
+    "SemidiscretizationBatch", "SemidiscretizationHyperbolicParabolic", "CompressibleNavierStokesDiffusion2D",
+    "GradientVariablesConservative", "GradientVariablesPrimitive",
 ]

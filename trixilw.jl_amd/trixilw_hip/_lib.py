@@ -18,6 +18,8 @@ TLW_FLUX_TREE, TLW_FLUX_STRUCTURED = 0, 1
 TLW_BV_NONE, TLW_BV_VORTEX, TLW_BV_CONSTANT, TLW_BV_RIEMANN4 = 0, 1, 2, 3
 TLW_AUX_U, TLW_AUX_F, TLW_AUX_ALPHA = 0, 1, 2
 TLW_FLAG_KEEP_AUX = 1
+TLW_PAR_NAVIER_STOKES = 1
+TLW_GRAD_CONSERVATIVE, TLW_GRAD_PRIMITIVE = 0, 1
 
 EXPORTED = (
     "tlw_create", "tlw_rebuild_mesh", "tlw_destroy", "tlw_last_error", "tlw_set_stream", "tlw_state_len",
@@ -26,6 +28,7 @@ EXPORTED = (
     "tlw_enable_kernel_timing", "tlw_kernel_times", "tlw_set_elements_per_workgroup",
     "tlw_get_elements_per_workgroup", "tlw_autotune", "tlw_set_positivity_limiter", "tlw_apply_positivity_limiter",
     "tlw_adjoint_lw", "tlw_create_batch", "tlw_n_instances", "tlw_batch_status", "tlw_enable_graphs",
+    "tlw_create_parabolic",
 )
 
 
@@ -65,6 +68,15 @@ class SolverDesc(C.Structure):
     ]
 
 
+class ParabolicDesc(C.Structure):
+    _fields_ = [
+        ("equations", C.c_int32),
+        ("mu", C.c_double),
+        ("prandtl", C.c_double),
+        ("gradient_variables", C.c_int32),
+    ]
+
+
 class TLWError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"libtrixilw_hip error {code}: {msg}")
@@ -93,6 +105,9 @@ def lib():
     vp, dp, i32, sz = C.c_void_p, C.POINTER(C.c_double), C.c_int32, C.c_size_t
     L.tlw_create.restype = C.c_int
     L.tlw_create.argtypes = [C.POINTER(MeshDesc), C.POINTER(BasisDesc), C.POINTER(SolverDesc), C.POINTER(vp)]
+    L.tlw_create_parabolic.restype = C.c_int
+    L.tlw_create_parabolic.argtypes = [C.POINTER(MeshDesc), C.POINTER(BasisDesc), C.POINTER(SolverDesc),
+                                       C.POINTER(ParabolicDesc), C.POINTER(vp)]
     L.tlw_rebuild_mesh.restype = C.c_int
     L.tlw_rebuild_mesh.argtypes = [vp, C.POINTER(MeshDesc)]
     L.tlw_destroy.restype = None

@@ -38,6 +38,36 @@ class CompressibleEulerEquations2D:
 @dataclass
 class LinearScalarAdvectionEquation2D:
     advection_velocity: tuple = (1.0, 1.0)
+
+
+class GradientVariablesConservative:
+    """Gradients of the conservative variables (TrixiLW's
+    GradientVariablesConservative, src/equations/compressible_navier_stokes_2d.jl:14-57)."""
+
+
+class GradientVariablesPrimitive:
+    """Trixi's default gradient variables; not implemented on the device."""
+
+
+class CompressibleNavierStokesDiffusion2D:
+    """CompressibleNavierStokesDiffusion2D(equations; mu, Prandtl, gradient_variables)
+    with constant dynamic viscosity μ and κ = γ/((γ-1) Pr) (Trixi's constructor;
+    used through TrixiLW's compressible_navier_stokes_2d.jl)."""
+
+    def __init__(self, equations, mu, Prandtl, gradient_variables=None):
+        if not isinstance(equations, CompressibleEulerEquations2D):
+            raise TypeError("the hyperbolic part must be CompressibleEulerEquations2D")
+        gv = gradient_variables if gradient_variables is not None else GradientVariablesPrimitive()
+        if not isinstance(gv, GradientVariablesConservative):
+            raise NotImplementedError("the device path implements GradientVariablesConservative")
+        if not (np.isfinite(mu) and mu >= 0.0 and np.isfinite(Prandtl) and Prandtl > 0.0):
+            raise ValueError("mu must be >= 0 and Prandtl > 0")
+        self.equations_hyperbolic = equations
+        self.gamma = equations.gamma
+        self.mu = float(mu)
+        self.Pr = float(Prandtl)
+        self.kappa = self.gamma / ((self.gamma - 1.0) * self.Pr)
+        self.gradient_variables = gv
     nvariables: int = 1
 
 

@@ -187,10 +187,13 @@ class SemidiscretizationHyperbolic:
         md = self._mesh_desc()
         h = C.c_void_p()
         torch.cuda.init()
+        self._create(md, basis, sd, h)
+        self._h = h
+
+    def _create(self, md, basis, sd, h):
         rc = _lib.lib().tlw_create(C.byref(md), C.byref(basis), C.byref(sd), C.byref(h))
         if rc != _lib.TLW_OK:
             raise _lib.TLWError(rc, "tlw_create failed (invalid description or HIP error)")
-        self._h = h
 
     def _mesh_arrays(self, mesh):
         return {
@@ -332,6 +335,46 @@ class SemidiscretizationHyperbolic:
             raise ValueError(which)
         _lib.check(_lib.lib().tlw_get_aux(self._h, code, _dptr(out), out.size), self._h)
         return out
+
+
+class SemidiscretizationHyperbolicParabolic(SemidiscretizationHyperbolic):
+    """SemidiscretizationHyperbolicParabolic(mesh, time_discretization,
+    (equations, equations_parabolic), initial_condition, solver;
+    boundary_conditions=(periodic, periodic))
+    (src/semidiscretization/semidiscretization_hyperbolic_parabolic.jl:8-118).
+
+    ``equations_parabolic`` is a CompressibleNavierStokesDiffusion2D with
+    GradientVariablesConservative; the mesh must be periodic (boundary
+    conditions None).  rhs / rhs_mdrk1 / rhs_mdrk2, max_dt and the device loop
+    then include the viscous terms (BR1 gradient, Taylor states of ∇u, central
+    viscous interface flux, compute_dt's min of advective and viscous bounds).
+    """
+
+    def __init__(self, mesh, time_discretization, equations, initial_condition, solver, boundary_conditions=None,
+                 source_terms=None, keep_aux=False, device=None):
+        from .equations import CompressibleNavierStokesDiffusion2D
+
+        if not isinstance(equations, tuple) or len(equations) != 2:
+            raise TypeError("equations must be the tuple (equations, equations_parabolic)")
+        eq, eqp = equations
+        if not isinstance(eqp, CompressibleNavierStokesDiffusion2D):
+            raise NotImplementedError("the device path implements CompressibleNavierStokesDiffusion2D")
+        bcs = boundary_conditions if boundary_conditions is not None else (None, None)
+        if any(b is not None for b in bcs):
+            raise NotImplementedError("the hyperbolic-parabolic device path needs a periodic mesh")
+        if isinstance(solver.volume_integral, VolumeIntegralFRShockCapturing):
+            raise NotImplementedError("the reference's parabolic kernels take VolumeIntegralFR only")
+        self.equations_parabolic = eqp
+        self._par = _lib.ParabolicDesc(_lib.TLW_PAR_NAVIER_STOKES, eqp.mu, eqp.Pr, _lib.TLW_GRAD_CONSERVATIVE)
+        super().__init__(mesh, time_discretization, eq, initial_condition, solver, boundary_conditions=None,
+                         source_terms=source_terms, keep_aux=keep_aux, device=device)
+
+    def _create(self, md, basis, sd, h):
+        rc = _lib.lib().tlw_create_parabolic(C.byref(md), C.byref(basis), C.byref(sd), C.byref(self._par),
+                                             C.byref(h))
+        if rc != _lib.TLW_OK:
+            raise _lib.TLWError(rc, "tlw_create_parabolic failed (invalid description, non-periodic mesh or HIP "
+                                    "error)")
 
 
 def _rhs_call(fn, du_ode, u_ode, semi, t):
