@@ -38,7 +38,14 @@ WORKLOADS = {
     # without the reference's positivity limiter, on the CPU oracle as on the device)
     "lw_euler_riemann_sc_256x256_N4": dict(level=8, N=4, td="lw", problem="riemann_sc", cfl=0.3),
     "lw_euler_riemann_sc_1024x1024_N4": dict(level=10, N=4, td="lw", problem="riemann_sc", cfl=0.3),
+    # compressible Navier-Stokes (conservative gradient variables, μ = 0.01, Pr = 0.72, the
+    # parameters of the reference's elixir_navierstokes_convergence.jl) on a periodic [-1,1]^2
+    # TreeMesh, smooth shear/density/pressure waves, CFL 0.5
+    "lw_navierstokes_shear_1024x1024_N4": dict(level=10, N=4, td="lw", problem="ns_shear", cfl=0.5),
+    "mdrk_navierstokes_shear_1024x1024_N3": dict(level=10, N=3, td="mdrk", problem="ns_shear", cfl=0.5),
+    "lw_navierstokes_shear_256x256_N4": dict(level=8, N=4, td="lw", problem="ns_shear", cfl=0.5),
 }
+NS_MU, NS_PR = 0.01, 0.72
 DEFAULT_WORKLOAD = "lw_euler_vortex_1024x1024_N4"
 
 
@@ -46,7 +53,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def alg_bytes_per_dof(N: int, td: str):
+def alg_bytes_per_dof(N: int, td: str, parabolic: bool = False):
     """Algorithmic HBM bytes per DOF-update: (whole step, volume kernel).
 
     Whole step (LW): read uⁿ + write uⁿ⁺¹ + one write and one read of the face
@@ -54,13 +61,16 @@ def alg_bytes_per_dof(N: int, td: str):
     8 (2·4 n² + 2·3·4·4 n) / n².  The volume kernel's share is the state read,
     the state write and the trace write: 8 (2·4 n² + 3·4·4 n) / n².
     MDRK: 8 (5·4 n² + 4·48 n) / n² per step (two stages).
+    Navier-Stokes adds, per stage, the read of the neighbours' face states for
+    the BR1 gradient (4 variables × n nodes × 4 faces): + 8·4·4 n / n².
     """
     n = N + 1
+    br1 = 8.0 * 4 * 4 * n / (n * n) if parabolic else 0.0
     if td == "lw":
-        step = 8.0 * (2 * 4 * n * n + 2 * 3 * 4 * 4 * n) / (n * n)
-        vol = 8.0 * (2 * 4 * n * n + 3 * 4 * 4 * n) / (n * n)
+        step = 8.0 * (2 * 4 * n * n + 2 * 3 * 4 * 4 * n) / (n * n) + br1
+        vol = 8.0 * (2 * 4 * n * n + 3 * 4 * 4 * n) / (n * n) + br1
     else:
-        step = 8.0 * (5 * 4 * n * n + 4 * 48 * n) / (n * n)
+        step = 8.0 * (5 * 4 * n * n + 4 * 48 * n) / (n * n) + (3 if parabolic else 0) * br1
         vol = None
     return step, vol
 
@@ -158,6 +168,129 @@ def batch_autotune(batch, u0):
     return chosen.value, list(ms)
 
 
+def ns_shear(x, y, t=0.0, gamma=1.4):
+    """Smooth periodic state on [-1, 1]²: shear flow with density and pressure waves."""
+    rho = 1.0 + 0.2 * np.sin(np.pi * x) * np.cos(np.pi * y)
+    v1 = 0.3 + 0.1 * np.sin(np.pi * y)
+    v2 = -0.2 + 0.1 * np.cos(np.pi * x)
+    p = 1.0 + 0.1 * np.cos(np.pi * (x + y))
+    return np.stack([rho, rho * v1, rho * v2, p / (gamma - 1) + 0.5 * rho * (v1**2 + v2**2)], axis=-1)
+
+
+def make_problem(wl):
+    """Semidiscretization, initial condition, CFL number and the oracle's
+    constructor arguments of a workload."""
+    import trixilw_hip as tlw
+
+    N, td = wl["N"], wl["td"]
+    problem = wl.get("problem")
+    eq = tlw.CompressibleEulerEquations2D(1.4)
+    vi = tlw.VolumeIntegralFR(tlw.MDRK() if td == "mdrk" else tlw.LW())
+    okw = {}
+    if problem == "riemann_sc":
+        ic = tlw.FourQuadrantRiemann()
+        lo, hi, bc, periodic = 0.0, 1.0, ic, False
+        vi = tlw.VolumeIntegralFRShockCapturing(tlw.IndicatorHennemannGassner(eq, None, alpha_max=0.5), vi)
+        okw = dict(bc_kind=bc.kind, bc_par=bc.params, shock_capturing=True, alpha_max=0.5)
+    elif problem == "ns_shear":
+        ic = ns_shear
+        lo, hi, bc, periodic = -1.0, 1.0, None, True
+        okw = dict(parabolic="navier_stokes", mu=NS_MU, prandtl=NS_PR)
+    else:
+        ic = tlw.IsentropicVortex()
+        lo, hi, bc, periodic = -10.0, 10.0, None, True
+    mesh = tlw.TreeMesh((lo, lo), (hi, hi), wl["level"], periodicity=periodic, ordering="morton")
+    solver = tlw.DGSEM(N, volume_integral=vi)
+    if problem == "ns_shear":
+        eqp = tlw.CompressibleNavierStokesDiffusion2D(eq, mu=NS_MU, Prandtl=NS_PR,
+                                                     gradient_variables=tlw.GradientVariablesConservative())
+        semi = tlw.SemidiscretizationHyperbolicParabolic(mesh, tlw.get_time_discretization(solver), (eq, eqp), ic,
+                                                         solver)
+    else:
+        semi = tlw.SemidiscretizationHyperbolic(mesh, tlw.get_time_discretization(solver), eq, ic, solver,
+                                                boundary_conditions=bc)
+    cfl = wl.get("cfl") or tlw.trixi2lw(0.98, solver) * (0.5 if td == "mdrk" else 1.0)
+    okw.update(periodic=(periodic, periodic))
+    return semi, mesh, ic, cfl, (lo, hi), okw
+
+
+def run_steps(semi, u0, cfl, steps, warmup, tile=0):
+    """Device loop: optional tile autotune, `warmup` untimed steps, then `steps`
+    timed steps with HIP-event kernel timing.  Returns a dict of the results."""
+    import torch
+    from trixilw_hip import _lib
+
+    L = _lib.lib()
+    h = semi.handle
+    dp = C.POINTER(C.c_double)
+    semi._bind_stream()
+    _lib.check(L.tlw_set_state(h, u0.ctypes.data_as(dp), u0.size), h)
+    if tile:
+        semi.elements_per_workgroup(tile)
+        tune = None
+    else:
+        chosen, tune = semi.autotune(u0, reps=3)
+        log("bench: tile autotune (ms per rhs): " + ", ".join(f"{k} el/wg {v:.3f}" for k, v in tune.items())
+            + f" -> {chosen}")
+    tile = semi.elements_per_workgroup()
+    _lib.check(L.tlw_set_state(h, u0.ctypes.data_as(dp), u0.size), h)
+    _lib.check(L.tlw_solve_init(h, 0.0, float("inf"), cfl), h)
+    t, dt = C.c_double(), C.c_double()
+    it, fin = C.c_int64(), C.c_int32()
+    if warmup > 0:
+        _lib.check(L.tlw_advance(h, warmup, C.byref(t), C.byref(dt), C.byref(it), C.byref(fin)), h)
+    _lib.check(L.tlw_enable_kernel_timing(h, 1), h)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _lib.check(L.tlw_advance(h, steps, C.byref(t), C.byref(dt), C.byref(it), C.byref(fin)), h)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    vms, vn, sms, sn = C.c_double(), C.c_int64(), C.c_double(), C.c_int64()
+    _lib.check(L.tlw_kernel_times(h, C.byref(vms), C.byref(vn), C.byref(sms), C.byref(sn)), h)
+    _lib.check(L.tlw_enable_kernel_timing(h, 0), h)
+    steps_done = int(it.value) - warmup
+    assert steps_done == steps, (steps_done, steps)
+    dofs = semi.ndofs()
+    return {"value": dofs * steps / el, "ms_per_step": 1e3 * el / steps, "elapsed": el,
+            "vol_ms": vms.value / max(vn.value, 1), "vol_n": int(vn.value),
+            "surf_ms": sms.value / max(sn.value, 1), "surf_n": int(sn.value), "tile": tile, "tune": tune}
+
+
+def measure_navier_stokes(steps, cpu_steps, threads, with_cpu=True):
+    """Sub-record: LW N=4 Navier-Stokes on the 1024² periodic mesh, device
+    loop throughput, volume kernel bytes per launch and the CPU oracle."""
+    wl = WORKLOADS["lw_navierstokes_shear_1024x1024_N4"]
+    semi, mesh, ic, cfl, (lo, hi), okw = make_problem(wl)
+    x, y = semi.node_coordinates()
+    u0 = np.ascontiguousarray(ic(x, y, 0.0))
+    r = run_steps(semi, u0, cfl, steps, 2)
+    dofs = semi.ndofs()
+    _, vol_b = alg_bytes_per_dof(4, "lw", parabolic=True)
+    out = {"workload": "lw_navierstokes_shear_1024x1024_N4", "value": r["value"], "unit": "DOF-updates/s",
+           "ms_per_step": round(r["ms_per_step"], 4), "steps": steps, "dofs": dofs,
+           "volume_avg_launch_ms": round(r["vol_ms"], 4), "surface_avg_launch_ms": round(r["surf_ms"], 4),
+           "volume_achieved_GBs": round(vol_b * dofs / (r["vol_ms"] * 1e-3) / 1e9, 2),
+           "volume_alg_bytes_per_dof": vol_b, "elements_per_workgroup": r["tile"], "tile_autotune_ms": r["tune"],
+           "config": f"compressible Navier-Stokes, conservative gradients, mu {NS_MU}, Pr {NS_PR}, periodic "
+                     f"[-1,1]^2 {mesh.nx}x{mesh.nx} TreeMesh, polydeg 4, LW, CFL {cfl}"}
+    log(f"bench: Navier-Stokes {r['value'] / 1e9:.3f} G DOF-updates/s, {r['ms_per_step']:.3f} ms/step, "
+        f"volume {r['vol_ms']:.3f} ms, surface {r['surf_ms']:.3f} ms")
+    semi.close()
+    if with_cpu:
+        import oracle_ctypes
+
+        nx = mesh.nx
+        o = oracle_ctypes.Oracle(4, nx, nx, lo, lo, (hi - lo) / nx, **okw)
+        xo, yo = o.node_coords()
+        v, el = cpu_baseline(o, np.ascontiguousarray(ic(xo, yo, 0.0)), cfl, False, dofs, cpu_steps, threads)
+        log(f"bench: Navier-Stokes CPU {v / 1e6:.2f} M DOF-updates/s ({el:.1f} s)")
+        out["cpu_baseline"] = {"value": v, "unit": "DOF-updates/s", "cores": threads, "kind": "port",
+                               "sample": f"{cpu_steps} step(s) of the same mesh, C++/OpenMP oracle"}
+        out["gpu_over_cpu"] = round(r["value"] / v, 2)
+        del o
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -169,6 +302,7 @@ def main():
     ap.add_argument("--tile", type=int, default=0, help="elements per volume workgroup (0: autotune)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward step ratio")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-instance batch measurement")
+    ap.add_argument("--no-ns", action="store_true", help="skip the Navier-Stokes sub-record")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device copy (the reference kernel for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -199,62 +333,24 @@ def main():
     wl = WORKLOADS[args.workload]
     N, td = wl["N"], wl["td"]
     sc = wl.get("problem") == "riemann_sc"
-    eq = tlw.CompressibleEulerEquations2D(1.4)
-    vi = tlw.VolumeIntegralFR(tlw.MDRK() if td == "mdrk" else tlw.LW())
-    if sc:
-        ic = tlw.FourQuadrantRiemann()
-        lo, hi, bc, periodic = 0.0, 1.0, ic, False
-        vi = tlw.VolumeIntegralFRShockCapturing(tlw.IndicatorHennemannGassner(eq, None, alpha_max=0.5), vi)
-    else:
-        ic = tlw.IsentropicVortex()
-        lo, hi, bc, periodic = -10.0, 10.0, None, True
-    mesh = tlw.TreeMesh((lo, lo), (hi, hi), wl["level"], periodicity=periodic, ordering="morton")
-    solver = tlw.DGSEM(N, volume_integral=vi)
-    semi = tlw.SemidiscretizationHyperbolic(mesh, tlw.get_time_discretization(solver), eq, ic, solver,
-                                            boundary_conditions=bc)
-    cfl = wl.get("cfl") or tlw.trixi2lw(0.98, solver) * (0.5 if td == "mdrk" else 1.0)
+    par = wl.get("problem") == "ns_shear"
+    semi, mesh, ic, cfl, (lo, hi), okw = make_problem(wl)
     dofs = semi.ndofs()
     log(f"bench: {args.workload}: {mesh.n_elements} elements, {dofs} DOFs")
-
     x, y = semi.node_coordinates()
     u0 = np.ascontiguousarray(ic(x, y, 0.0))
     L = _lib.lib()
     h = semi.handle
     dp = C.POINTER(C.c_double)
-    semi._bind_stream()
-    _lib.check(L.tlw_set_state(h, u0.ctypes.data_as(dp), u0.size), h)
-    if args.tile:
-        semi.elements_per_workgroup(args.tile)
-        tune = None
-    else:
-        chosen, tune = semi.autotune(u0, reps=3)
-        log("bench: tile autotune (ms per rhs): " + ", ".join(f"{k} el/wg {v:.3f}" for k, v in tune.items())
-            + f" -> {chosen}")
-    tile = semi.elements_per_workgroup()
-    _lib.check(L.tlw_set_state(h, u0.ctypes.data_as(dp), u0.size), h)
-    _lib.check(L.tlw_solve_init(h, 0.0, float("inf"), cfl), h)
-    t, dt = C.c_double(), C.c_double()
-    it, fin = C.c_int64(), C.c_int32()
-    if args.warmup > 0:
-        _lib.check(L.tlw_advance(h, args.warmup, C.byref(t), C.byref(dt), C.byref(it), C.byref(fin)), h)
-    _lib.check(L.tlw_enable_kernel_timing(h, 1), h)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    _lib.check(L.tlw_advance(h, args.steps, C.byref(t), C.byref(dt), C.byref(it), C.byref(fin)), h)
-    torch.cuda.synchronize()
-    el = time.perf_counter() - t0
-    vms, vn, sms, sn = C.c_double(), C.c_int64(), C.c_double(), C.c_int64()
-    _lib.check(L.tlw_kernel_times(h, C.byref(vms), C.byref(vn), C.byref(sms), C.byref(sn)), h)
-    _lib.check(L.tlw_enable_kernel_timing(h, 0), h)
-    steps_done = int(it.value) - args.warmup
-    assert steps_done == args.steps, (steps_done, args.steps)
-    value = dofs * args.steps / el
-    ms_per_step = 1e3 * el / args.steps
+    r = run_steps(semi, u0, cfl, args.steps, args.warmup, args.tile)
+    tile, tune = r["tile"], r["tune"]
+    value, ms_per_step = r["value"], r["ms_per_step"]
+    vms_avg, sms_avg, vn = r["vol_ms"], r["surf_ms"], r["vol_n"]
     log(f"bench: {value / 1e9:.3f} G DOF-updates/s, {ms_per_step:.3f} ms/step, "
-        f"volume {vms.value / max(vn.value, 1):.3f} ms x{vn.value}, surface {sms.value / max(sn.value, 1):.3f} ms")
+        f"volume {vms_avg:.3f} ms x{vn}, surface {sms_avg:.3f} ms")
 
-    step_b, vol_b = alg_bytes_per_dof(N, td)
-    vol_avg_s = vms.value / max(vn.value, 1) * 1e-3
+    step_b, vol_b = alg_bytes_per_dof(N, td, parabolic=par)
+    vol_avg_s = vms_avg * 1e-3
     launches_per_step = 1 if td == "lw" else 2
     if vol_b is not None:
         achieved = vol_b * dofs / vol_avg_s / 1e9
@@ -265,7 +361,8 @@ def main():
     traffic, traffic_src = load_traffic(args.workload)
     roofline = {
         "bound": "hbm",
-        "kernel": "k_volume_lw4" if (td == "lw" and N == 4) else ("k_volume_lw3" if td == "lw" else "k_volume_mdrk*"),
+        "kernel": (("k_volume_lw4" if (td == "lw" and N == 4) else ("k_volume_lw3" if td == "lw" else "k_volume_mdrk*"))
+                   + ("_navier_stokes" if par else "")),
         "achieved": round(achieved, 2),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
@@ -276,11 +373,11 @@ def main():
         "avg_launch_ms": round(vol_avg_s * 1e3, 4),
         "step_alg_bytes_per_dof": step_b,
         "step_frac": round(step_b * value / 1e9 / HBM_PEAK_GBS, 4),
-        "surface_avg_launch_ms": round(sms.value / max(sn.value, 1), 4),
+        "surface_avg_launch_ms": round(sms_avg, 4),
     }
 
     adjoint = None
-    if td == "lw" and not sc and not args.no_adjoint:
+    if td == "lw" and not sc and not par and not args.no_adjoint:
         # adjoint/forward step ratio: one adjoint step (tlw_adjoint_lw: face records + reverse
         # sweep) against one forward step of the loop above, same state and dt
         _lib.check(L.tlw_get_state(h, u0.ctypes.data_as(dp), u0.size), h)  # the state after the timed steps
@@ -315,16 +412,19 @@ def main():
         log(f"bench: batch {batch['value'] / 1e9:.3f} G DOF-updates/s "
             f"({batch.get('vs_single_instance')} x the single 1024^2 instance)")
 
+    threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+    threads = max(1, min(threads, os.cpu_count() or threads))
+    navier_stokes = None
+    if args.workload == DEFAULT_WORKLOAD and not args.no_ns:
+        navier_stokes = measure_navier_stokes(max(3, min(args.steps, 10)), 1, threads,
+                                              with_cpu=not args.no_cpu_baseline)
+
     cpu = None
     if not args.no_cpu_baseline:
         import oracle_ctypes
 
-        threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
-        threads = max(1, min(threads, os.cpu_count() or threads))
         nx = mesh.nx
-        o = oracle_ctypes.Oracle(N, nx, nx, lo, lo, (hi - lo) / nx, periodic=(periodic, periodic),
-                                 bc_kind=bc.kind if bc is not None else 0, bc_par=bc.params if bc is not None else (),
-                                 shock_capturing=sc, alpha_max=0.5)
+        o = oracle_ctypes.Oracle(N, nx, nx, lo, lo, (hi - lo) / nx, **okw)
         xo, yo = o.node_coords()
         uo = np.ascontiguousarray(ic(xo, yo, 0.0))
         log(f"bench: CPU oracle baseline, {args.cpu_steps} steps on {threads} threads")
@@ -351,10 +451,13 @@ def main():
         "data": "synthetic",
         "config": {"workload": args.workload,
                    "mesh": (f"{mesh.nx}x{mesh.nx} TreeMesh [0,1]^2, time-averaged Dirichlet boundary" if sc else
-                            f"{mesh.nx}x{mesh.nx} periodic TreeMesh [-10,10]^2"),
+                            f"{mesh.nx}x{mesh.nx} periodic TreeMesh [{lo:g},{hi:g}]^2"),
                    "polydeg": N, "time_discretization": td.upper(), "dofs": dofs,
                    "initial_condition": ("four-quadrant Riemann problem (Lax-Liu 3)" if sc else
+                                         "shear flow with density/pressure waves" if par else
                                          "isentropic vortex strength 5, Mach 0.5"),
+                   "parabolic": (f"CompressibleNavierStokesDiffusion2D mu {NS_MU}, Pr {NS_PR}, conservative "
+                                 "gradients" if par else None),
                    "shock_capturing": ("Hennemann-Gassner on density_pressure, alpha_max 0.5, first-order subcell FV"
                                        if sc else None),
                    "cfl": cfl, "elements_per_workgroup": tile, "tile_autotune_ms": tune},
@@ -362,6 +465,7 @@ def main():
         "cpu_baseline": cpu,
         "adjoint": adjoint,
         "batch": batch,
+        "navier_stokes": navier_stokes,
     }
     print(json.dumps(out), flush=True)
     semi.close()
