@@ -188,6 +188,21 @@ int tlw_solve_init(tlw_ctx* ctx, double t0, double t_final, double cfl);
 int tlw_advance(tlw_ctx* ctx, int32_t max_steps, double* t_host, double* dt_host, int64_t* iter_host,
                 int32_t* finished_host);
 
+/* Adaptive (error-controlled) time stepping, solve_lwfr with time_step_computation = Adaptive()
+ * (src/solvers/adaptive.jl:95-173, lwfr.jl:141-171): replaces tlw_solve_init.  dt starts at
+ * dt_initial (set_dt!); every step of tlw_advance is one attempt: the LW step with the temporal
+ * error estimate of lw_volume_kernel_4! (dg_2d.jl:1183-1199, 1251-1265; abstol/reltol are the
+ * reference's `tolerances`), the admissibility test of the updated state, then the controller
+ * (β = 0.6, -0.2, 0, k = nnodes): a rejected attempt (non-admissible state or factor <= 0.81)
+ * restarts from uⁿ with min(factor, 0.95)·dt, an accepted one advances t and proposes
+ * min(factor, 1.5)·dt.  tlw_advance's iter counts accepted steps.  Polydeg 4, LW, Euler only (the
+ * reference computes the estimate only in that kernel on TreeMesh); not combined with the
+ * positivity limiter (TLW_ENOTSUP).  A rejected step whose factor is not finite (NaN state)
+ * shrinks dt by 0.95. */
+int tlw_solve_init_adaptive(tlw_ctx* ctx, double t0, double t_final, double dt_initial, double abstol,
+                            double reltol);
+/* integrator.stats.nreject: rejected attempts of the adaptive loop, summed over instances. */
+int tlw_rejected_steps(tlw_ctx* ctx, int64_t* nreject);
 /* Measurement hook (no counterpart in the reference; its TimerOutputs/performance_counter
  * timing, semidiscretization_hyperbolic.jl:11-17, is per rhs! call): when enabled,
  * tlw_advance records HIP events around every volume and surface kernel on the context's

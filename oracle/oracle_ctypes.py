@@ -96,6 +96,9 @@ def lib():
         L.tlwo_max_abs_speed_naive.restype = C.c_double
         L.tlwo_max_abs_speed_naive.argtypes = [dp, dp, C.c_int, C.c_double]
         L.tlwo_ns_viscous_fluxes.argtypes = [dp, dp, dp, C.c_double, C.c_double, C.c_double, dp, dp]
+        L.tlwo_solve_adaptive.restype = C.c_int
+        L.tlwo_solve_adaptive.argtypes = [C.c_void_p, dp, C.c_double, C.c_double, C.c_double, C.c_int, dp, dp,
+                                          C.POINTER(C.c_longlong)]
         L.tlwo_solve.restype = C.c_int
         L.tlwo_solve.argtypes = [C.c_void_p, dp, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, dp, dp]
         _lib = L
@@ -252,6 +255,18 @@ class Oracle:
         k = lib().tlwo_solve(self._h, _ptr(u), t0, tf, cfl, int(min(max_steps, cap)), int(mdrk), C.byref(tout),
                              _ptr(dts))
         return u, tout.value, dts[:k]
+
+
+    def solve_adaptive(self, u0, t0, tf, dt_initial, max_attempts=10**6):
+        """solve_lwfr with Adaptive() time stepping (LW; tolerances from the
+        constructor); returns (u, t, accepted dts, rejected attempts)."""
+        u = np.array(u0, dtype=np.float64, copy=True, order="C")
+        dts = np.zeros(max_attempts)
+        tout = C.c_double(0.0)
+        nrej = C.c_longlong(0)
+        k = lib().tlwo_solve_adaptive(self._h, _ptr(u), t0, tf, dt_initial, int(max_attempts), C.byref(tout),
+                                      _ptr(dts), C.byref(nrej))
+        return u, tout.value, dts[:k], int(nrej.value)
 
 
 def euler_fluxes(u, gamma=1.4):

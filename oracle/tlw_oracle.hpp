@@ -452,10 +452,10 @@ template <class T, class Eq> struct Solver {
         T f[MAXN][MAXN][NV], g[MAXN][MAXN][NV], Fc[MAXN][MAXN][NV], Gc[MAXN][MAXN][NV];
         T ut[MAXN][MAXN][NV], utt[MAXN][MAXN][NV], uttt[MAXN][MAXN][NV], utttt[MAXN][MAXN][NV];
         T Uc[MAXN][MAXN][NV], up[MAXN][MAXN][NV], um[MAXN][MAXN][NV], upp[MAXN][MAXN][NV], umm[MAXN][MAXN][NV];
-        T unp1[MAXN][MAXN][NV], unp1low[MAXN][MAXN][NV];
+        T unp1[MAXN][MAXN][NV], unp1low[MAXN][MAXN][NV], dlow[MAXN][MAXN][NV];
         for (int j = 0; j < n; ++j)
             for (int i = 0; i < n; ++i)
-                for (int v = 0; v < NV; ++v) ut[j][i][v] = utt[j][i][v] = uttt[j][i][v] = utttt[j][i][v] = T(0);
+                for (int v = 0; v < NV; ++v) ut[j][i][v] = utt[j][i][v] = uttt[j][i][v] = utttt[j][i][v] = dlow[j][i][v] = T(0);
         // 953-987: fluxes at uⁿ and ut = -dt*J⁻¹*(D f + D g)
         for (int j = 0; j < n; ++j)
             for (int i = 0; i < n; ++i) {
@@ -596,6 +596,11 @@ template <class T, class Eq> struct Solver {
                     const T gtttt = 0.5 * (gpp[v] - 4.0 * gp[v] + 6.0 * g[j][i][v] - 4.0 * gm[v] + gmm[v]);
                     Fc[j][i][v] = Fc[j][i][v] + (1.0 / 120.0) * ftttt;
                     Gc[j][i][v] = Gc[j][i][v] + (1.0 / 120.0) * gtttt;
+                    // u_np1 - u_np1_low: the f_tttt term's own update
+                    for (int ii = 0; ii < n; ++ii)
+                        dlow[j][ii][v] = dlow[j][ii][v] + (-dt * ij * D[ii][i]) * ((1.0 / 120.0) * ftttt);
+                    for (int jj = 0; jj < n; ++jj)
+                        dlow[jj][i][v] = dlow[jj][i][v] + (-dt * ij * D[jj][j]) * ((1.0 / 120.0) * gtttt);
                 }
                 for (int ii = 0; ii < n; ++ii)
                     for (int v = 0; v < NV; ++v) {
@@ -613,12 +618,14 @@ template <class T, class Eq> struct Solver {
                     U[idx(v, i, j, e)] = Uc[j][i][v];
                 }
             }
-        // 1251-1265: temporal error estimate for the adaptive controller
+        // 1251-1265: temporal error estimate for the adaptive controller.  The
+        // numerator u_np1 - u_np1_low is the update of the f_tttt term itself
+        // (the same quantity without the cancellation of the difference).
         T err = T(0);
         for (int j = 0; j < n; ++j)
             for (int i = 0; i < n; ++i)
                 for (int v = 0; v < NV; ++v) {
-                    const T d = unp1[j][i][v] - unp1low[j][i][v];
+                    const T d = dlow[j][i][v];
                     const T s = abstol + reltol * max_(abs_(unp1[j][i][v]), abs_(unp1low[j][i][v]));
                     const T r = d / s;
                     err = err + r * r;
@@ -1317,6 +1324,74 @@ template <class T, class Eq> void solve_fixed_dt(Solver<T, Eq>& s, T* u, double 
         if (s.positivity) s.positivity_limiter(u);
         t += dt;
     }
+}
+
+// solve_lwfr with time_step_computation = Adaptive() for LW (perform_step!,
+// src/solvers/adaptive.jl:95-173; test_updated_solution 30-46;
+// load_temporal_errors! 48-64; dt_factor 88-93; controller β = (0.6, -0.2, 0),
+// epsilon = (1, 1, 1) at the start, lwfr.jl:162-168).  The temporal errors come
+// from lw_volume_kernel_4 with the solver's abstol/reltol.  A rejected attempt
+// (non-admissible state or factor <= 0.81) restarts from uⁿ with
+// min(factor, 0.95)·dt; a non-finite factor there gives 0.95·dt.  Returns the
+// number of accepted steps; dts[k] receives the k-th accepted dt, *nreject the
+// rejected attempts.
+template <class T, class Eq>
+int solve_adaptive(Solver<T, Eq>& s, T* u, double t0, double tf, double dt_initial, int max_attempts, double* t_out,
+                   double* dts, long long* nreject) {
+    const long long nd = (long long)s.ndofs_total();
+    const double ndofs = (double)s.mesh.E * s.n * s.n;  // Trixi.ndofs
+    const double k = s.n;                                // nnodes
+    std::vector<T> du(nd), uprev(nd);
+    double eps[3] = {1.0, 1.0, 1.0};  // epsilon[-1], epsilon[0], epsilon[1]
+    double t = t0;
+    double dt = dt_initial;
+    if (t + dt > tf) dt = tf - t;  // set_dt!
+    if (s.positivity) s.positivity_limiter(u);
+    int it = 0, attempts = 0, nfail = 0;
+    long long nrej = 0;
+    while (!julia_isapprox(t, tf) && attempts < max_attempts) {
+        ++attempts;
+#pragma omp parallel for schedule(static)
+        for (long long q = 0; q < nd; ++q) uprev[q] = u[q];
+        s.rhs_lw(du.data(), u, t, dt);
+#pragma omp parallel for schedule(static)
+        for (long long q = 0; q < nd; ++q) u[q] = uprev[q] + dt * du[q];
+        if (s.positivity) s.positivity_limiter(u);
+        bool valid = true;
+        for (int e = 0; e < s.mesh.E && valid; ++e)
+            for (int q = 0; q < s.n * s.n && valid; ++q) valid = s.eq.is_admissible(&u[(size_t)(e * s.n * s.n + q) * Eq::NV]);
+        double total = 0.0;
+        for (int e = 0; e < s.mesh.E; ++e) total += value(s.temporal_errors[e]);
+        const double err = std::sqrt(total / ndofs);
+        if (nfail == 0) {
+            eps[0] = eps[1];
+            eps[1] = eps[2];
+        }
+        eps[2] = 1.0 / (err + 1e-12);
+        const double fac = std::pow(eps[2], 0.6 / k) * std::pow(eps[1], -0.2 / k) * std::pow(eps[0], 0.0 / k);
+        const double factor = 1.0 + std::atan(fac - 1.0);
+        if (!valid || !(factor > 0.81)) {
+            ++nfail;
+            dt = (factor < 0.95 ? factor : 0.95) * dt;
+            if (t + dt > tf) dt = tf - t;
+#pragma omp parallel for schedule(static)
+            for (long long q = 0; q < nd; ++q) u[q] = uprev[q];
+            continue;
+        }
+        nrej += nfail;
+        nfail = 0;
+        double dtn = (factor < 1.5 ? factor : 1.5) * dt;
+        double dts_ = dt;
+        if (t + dts_ > tf) dts_ = tf - t;  // set_t_and_iter!
+        if (dts) dts[it] = dts_;
+        t += dts_;
+        ++it;
+        if (t + dtn > tf) dtn = tf - t;  // set_dt!
+        dt = dtn;
+    }
+    if (t_out) *t_out = t;
+    if (nreject) *nreject = nrej;
+    return it;
 }
 
 }  // namespace tlwo

@@ -217,6 +217,14 @@ int tlwo_solve(void* ctx, double* u, double t0, double tf, double cfl, int max_s
     return r;
 }
 
+// Adaptive LW loop (tolerances from the descriptor's abstol/reltol); returns accepted steps.
+int tlwo_solve_adaptive(void* ctx, double* u, double t0, double tf, double dt_initial, int max_attempts,
+                        double* t_out, double* dts, long long* nreject) {
+    int r = 0;
+    DISPATCH(ctx, r = solve_adaptive(s, u, t0, tf, dt_initial, max_attempts, t_out, dts, nreject));
+    return r;
+}
+
 // Prescribed blending factors (len E) used instead of the indicator; NULL clears.
 void tlwo_positivity_limiter(void* ctx, double* u) {
     auto* c = static_cast<tlwo_ctx*>(ctx);

@@ -60,6 +60,7 @@ struct tlw_ctx {
     double* d_elem_vmax = nullptr;
     int ebv = 0;              // elements per volume workgroup
     bool positivity = false;  // Zhang-Shu limiter in the device loop
+    bool adaptive = false;    // error-controlled steps (tlw_solve_init_adaptive)
     bool has_boundary = false;
     std::vector<int> host_cur;  // [ninst] state buffer holding each instance's uⁿ
     bool aux_valid = false;
@@ -269,7 +270,10 @@ int upload_meshes(tlw_ctx* c, const tlw_mesh_desc* ms, int ninst) {
     HIPCHK(c, hipMalloc(&c->d_cur, sizeof(int) * ninst));
     HIPCHK(c, hipHostMalloc(&c->h_state, sizeof(DevState) * ninst, hipHostMallocDefault));
     std::memset(c->h_state, 0, sizeof(DevState) * ninst);
-    for (int k = 0; k < ninst; ++k) c->h_state[k].invJ0 = ms[k].inverse_jacobian_host[0];
+    for (int k = 0; k < ninst; ++k) {
+        c->h_state[k].invJ0 = ms[k].inverse_jacobian_host[0];
+        c->h_state[k].ndofs = (double)ms[k].n_elements * c->n * c->n;  // Trixi.ndofs
+    }
     HIPCHK(c, hipMemcpy(c->d_state, c->h_state, sizeof(DevState) * ninst, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_nbr, nbr.data(), sizeof(int) * 4 * E, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_invj, invj.data(), sizeof(double) * E, hipMemcpyHostToDevice));
@@ -312,7 +316,10 @@ template <int N, int EB, bool SC, bool PAR = false> void launch_volume_eb(tlw_ct
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_LW, SC, PAR>), g, b, 0, c->stream, a);
     else if (kind == VOL_MDRK1)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_MDRK1, SC, PAR>), g, b, 0, c->stream, a);
-    else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_MDRK2, SC, PAR>), g, b, 0, c->stream, a);
+    else if (kind == VOL_MDRK2)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_MDRK2, SC, PAR>), g, b, 0, c->stream, a);
+    else if constexpr (N == 4 && !PAR)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_LW_ADAPT, SC, PAR>), g, b, 0, c->stream, a);
 }
 template <int N, bool SC, bool PAR = false> void launch_volume_sc(tlw_ctx* c, const KArgs& a, int kind) {
     constexpr const int* ch = N == 4 ? kEbChoices4 : kEbChoices3;
@@ -352,7 +359,7 @@ template <int N> void launch_surface_t(tlw_ctx* c, const KArgs& a) {
         if (c->has_boundary) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_surface<N, true, false>), g, b, 0, c->stream, a);
         else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_surface<N, false, false>), g, b, 0, c->stream, a);
     }
-    if (a.reduce != RED_NONE) launch_finalize(c, a.reduce, 1);
+    if (a.reduce != RED_NONE) launch_finalize(c, a.reduce == RED_ADAPT ? RED_STEP : a.reduce, 1);
 }
 void launch_surface(tlw_ctx* c, const KArgs& a) {
     if (c->N == 4) launch_surface_t<4>(c, a);
@@ -457,13 +464,13 @@ void enqueue_step(tlw_ctx* c) {
     // kernel (nodes it finalises) and the surface kernel (the rest); with the
     // positivity limiter (applied after the update, lwfr.jl:226) the limiter
     // kernel takes it over the limited state.
-    const int red = c->positivity ? RED_NONE : RED_STEP;
+    const int red = c->adaptive ? RED_ADAPT : (c->positivity ? RED_NONE : RED_STEP);
     if (c->time_disc == TLW_LW) {
         KArgs v = a;
         v.in_sel = SEL_CUR;
         v.out_sel = SEL_OTHER;
         v.reduce = red;
-        timed(c, 0, [&] { launch_volume(c, v, VOL_LW); });
+        timed(c, 0, [&] { launch_volume(c, v, c->adaptive ? VOL_LW_ADAPT : VOL_LW); });
         KArgs sf = a;
         sf.out_sel = SEL_OTHER;
         sf.reduce = red;
@@ -489,7 +496,7 @@ void enqueue_step(tlw_ctx* c) {
         KArgs s2 = v2;
         timed(c, 1, [&] { launch_surface(c, s2); });
     }
-    if (c->positivity) {
+    if (c->positivity && !c->adaptive) {
         KArgs l = a;
         l.out_sel = SEL_OTHER;
         l.reduce = RED_STEP;
@@ -677,8 +684,10 @@ int tlw_solve_init(tlw_ctx* c, double t0, double t_final, double cfl) {
         return fail(c, TLW_EINVAL, "t0 finite, t_final not NaN and cfl > 0 required");
     HIPCHK(c, hipMemcpyAsync(c->d_cur, c->host_cur.data(), sizeof(int) * c->ninst, hipMemcpyHostToDevice,
                              c->stream));
+    if (c->adaptive) drop_graphs(c);
+    c->adaptive = false;
     hipLaunchKernelGGL(k_init_state, dim3((c->ninst + 255) / 256), dim3(256), 0, c->stream, c->d_state, c->ninst, t0,
-                       t_final, cfl, c->d_cur);
+                       t_final, cfl, c->d_cur, 0, 0.0, 0.0, 0.0);
     KArgs a = base_args(c);
     a.in_sel = SEL_CUR;
     a.reduce = RED_INIT;
@@ -688,6 +697,39 @@ int tlw_solve_init(tlw_ctx* c, double t0, double t_final, double cfl) {
     rc = sync_state(c);
     if (rc) return rc;
     if (any_bad(c)) return fail(c, TLW_EADMISSIBLE, "non-admissible initial state");
+    return TLW_OK;
+}
+
+int tlw_solve_init_adaptive(tlw_ctx* c, double t0, double t_final, double dt_initial, double abstol,
+                            double reltol) {
+    if (!c) return TLW_EINVAL;
+    if (!std::isfinite(t0) || std::isnan(t_final) || !(dt_initial > 0.0) || !std::isfinite(dt_initial))
+        return fail(c, TLW_EINVAL, "t0 finite, t_final not NaN and a finite dt_initial > 0 required");
+    if (!(abstol >= 0.0) || !(reltol >= 0.0) || !(abstol > 0.0 || reltol > 0.0) || !std::isfinite(abstol) ||
+        !std::isfinite(reltol))
+        return fail(c, TLW_EINVAL, "tolerances: abstol, reltol >= 0, not both zero");
+    if (c->N != 4 || c->time_disc != TLW_LW || c->parabolic)
+        return fail(c, TLW_ENOTSUP,
+                    "adaptive steps need the temporal-error estimate of the Euler LW kernel for polydeg 4");
+    if (c->positivity) return fail(c, TLW_ENOTSUP, "adaptive steps are not combined with the positivity limiter");
+    HIPCHK(c, hipMemcpyAsync(c->d_cur, c->host_cur.data(), sizeof(int) * c->ninst, hipMemcpyHostToDevice,
+                             c->stream));
+    if (!c->adaptive) drop_graphs(c);
+    c->adaptive = true;
+    hipLaunchKernelGGL(k_init_state, dim3((c->ninst + 255) / 256), dim3(256), 0, c->stream, c->d_state, c->ninst, t0,
+                       t_final, 1.0, c->d_cur, 1, dt_initial, abstol, reltol);
+    int rc = check_launch(c);
+    if (rc) return rc;
+    return sync_state(c);
+}
+
+int tlw_rejected_steps(tlw_ctx* c, int64_t* nreject) {
+    if (!c || !nreject) return fail(c, TLW_EINVAL, "null pointer");
+    int rc = sync_state(c);
+    if (rc) return rc;
+    int64_t r = 0;
+    for (int k = 0; k < c->ninst; ++k) r += c->h_state[k].nreject;
+    *nreject = r;
     return TLW_OK;
 }
 
@@ -800,6 +842,8 @@ int tlw_set_positivity_limiter(tlw_ctx* c, int32_t enable, double threshold_dens
         return fail(c, TLW_EINVAL, "positivity thresholds must not be NaN");
     if (enable && c->parabolic)
         return fail(c, TLW_ENOTSUP, "the positivity limiter is not combined with the hyperbolic-parabolic path");
+    if (enable && c->adaptive)
+        return fail(c, TLW_ENOTSUP, "the positivity limiter is not combined with adaptive steps (tlw_solve_init first)");
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if ((enable != 0) != c->positivity) drop_graphs(c);
     c->positivity = enable != 0;

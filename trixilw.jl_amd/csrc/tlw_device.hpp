@@ -84,11 +84,22 @@ struct DevState {
     int cur;        // state buffer holding uⁿ
     int bad_state;  // set when a non-admissible state was met
     long long iter;
+    // Adaptive time stepping (src/solvers/adaptive.jl): the error estimates
+    // epsilon[-1], epsilon[0], epsilon[1], tolerances, the instance's Trixi ndofs,
+    // the retry count of the current step and the accumulated rejections
+    double eps[3];
+    double abstol, reltol, ndofs;
+    int adaptive, nfail;
+    long long nreject;
 };
 
 enum BufSel { SEL_EXPLICIT = 0, SEL_CUR = 1, SEL_OTHER = 2 };
-enum Reduce { RED_NONE = 0, RED_STEP = 1, RED_INIT = 2, RED_QUERY = 3 };
-enum VolKind { VOL_LW = 0, VOL_MDRK1 = 1, VOL_MDRK2 = 2 };
+// RED_ADAPT: adaptive step (per-workgroup sums of the temporal errors and the
+// admissibility test of the updated state instead of CFL maxima)
+enum Reduce { RED_NONE = 0, RED_STEP = 1, RED_INIT = 2, RED_QUERY = 3, RED_ADAPT = 4 };
+// VOL_LW_ADAPT: the LW kernel with the temporal-error estimate of adaptive steps
+// (a separate instantiation, so the plain LW kernel carries none of its registers)
+enum VolKind { VOL_LW = 0, VOL_MDRK1 = 1, VOL_MDRK2 = 2, VOL_LW_ADAPT = 3 };
 
 struct KArgs {
     int E;
@@ -422,11 +433,17 @@ __device__ __forceinline__ void node_speeds(const double* un, double gamma, doub
 // 1183-1199, 1251-1265 is an adaptive-mode output and is not computed).
 // With KEEP, tder receives the time derivatives ut, utt, ... (4 per order),
 // the intermediates an adjoint sweep needs.
+// With terr, the temporal error of the node for the adaptive controller
+// (dg_2d.jl:1183-1199, 1251-1265): u_np1 = u - dt J⁻¹ D F̃ and u_np1_low, the
+// same with F̃ before the f_tttt term, compared variable by variable as
+// ((u_np1 - u_np1_low) / (abstol + reltol max(|u_np1|, |u_np1_low|)))², summed.
+// terr must be uniformly null or non-null over the workgroup (it adds barriers).
 template <int N, int EB, bool KEEP = false>
 __device__ __forceinline__ void lw4_element(const double* uN, double dt, double invJ, double gm1, const DevConst* c,
                                             int i, int j, bool act, int sidx, int sbase, double* s0, double* s1,
                                             double* s2, const double* sDm, const double* sDh, double* Uc, double* F,
-                                            double* G, double* vol, double* tder = nullptr) {
+                                            double* G, double* vol, double* tder = nullptr, double* terr = nullptr,
+                                            double abstol = 0.0, double reltol = 0.0) {
     using G_ = VGeo<N, EB>;
     constexpr int n = G_::n, S = G_::S;
     euler_fluxes(uN, gm1, F, G);
@@ -570,8 +587,28 @@ __device__ __forceinline__ void lw4_element(const double* uN, double dt, double 
         G[v] = fma(1.0 / 120.0, dy[v], G[v]);
     }
     if (act) put8<S>(s2, sidx, F, G);
+    // s0 is read only at the own node since the first contraction: it takes f_tttt
+    if (terr && act) put8<S>(s0, sidx, dx, dy);
     __syncthreads();
     contract<n, S>(s2, sbase, i, j, sDh, vol);
+    if (terr) {
+        double a1[4], a2[4];
+        contract<n, S>(s2, sbase, i, j, sDm, a1);  // -dt D F̃
+        contract<n, S>(s0, sbase, i, j, sDm, a2);  // -dt D f_tttt
+        // u_np1 - u_np1_low is formed as the contraction of the f_tttt term itself,
+        // not as a difference of the two updates (which cancels to ~1e-7 relative)
+        double e = 0.0;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const double d = invJ * ((1.0 / 120.0) * a2[v]);
+            const double hi = fma(invJ, a1[v], uN[v]);
+            const double lo = hi - d;
+            const double r = d / (abstol + reltol * dmax(fabs(hi), fabs(lo)));
+            e = fma(r, r, e);
+        }
+        *terr = e;
+        __syncthreads();  // s0 is reused for the face records
+    }
 }
 
 // lw_volume_kernel_3!, dg_2d.jl:674-916
@@ -1292,6 +1329,7 @@ __global__ __launch_bounds__((VGeo<N, EB>::NT), kVolWaves<SC || PAR>) void k_vol
         if constexpr (PAR) sD[tid] = c->D[tid / n][tid % n];
     }
     double uN[4], Uc[4], F[4], G[4], vol[4];
+    double terr = 0.0;  // temporal error of the node (adaptive steps, LW N=4)
     load4(uin + off, uN);
     if constexpr (PAR) {
         // BR1 gradients of uⁿ (and of us in MDRK stage 2), exchanged through a
@@ -1318,8 +1356,11 @@ __global__ __launch_bounds__((VGeo<N, EB>::NT), kVolWaves<SC || PAR>) void k_vol
                 mdrkp1_element<N, EB>(uN, gxN, gyN, invJ, gm1, c, i, j, act, sidx, sbase, s0, s1, s2, sDm, sD, sDh,
                                       Uc, F, G, vol);
         }
-    } else if (KIND == VOL_LW) {
-        if constexpr (N == 4)
+    } else if (KIND == VOL_LW || KIND == VOL_LW_ADAPT) {
+        if constexpr (N == 4 && KIND == VOL_LW_ADAPT)
+            lw4_element<N, EB>(uN, dt, invJ, gm1, c, i, j, act, sidx, sbase, s0, s1, s2, sDm, sDh, Uc, F, G, vol,
+                               nullptr, &terr, st->abstol, st->reltol);
+        else if constexpr (N == 4)
             lw4_element<N, EB>(uN, dt, invJ, gm1, c, i, j, act, sidx, sbase, s0, s1, s2, sDm, sDh, Uc, F, G, vol);
         else
             lw3_element<N, EB>(uN, dt, invJ, gm1, c, i, j, act, sidx, sbase, s0, s1, s2, sDm, sDh, Uc, F, G, vol);
@@ -1482,6 +1523,36 @@ __global__ __launch_bounds__((VGeo<N, EB>::NT), kVolWaves<SC || PAR>) void k_vol
     }
     if (act) store4(out + off, val);
     if (a.reduce == RED_NONE) return;
+    if constexpr (KIND == VOL_LW_ADAPT) {
+        // test_updated_solution on the nodes finalised here, and the workgroup's
+        // sum of its elements' temporal errors (adaptive.jl:30-46, 48-64)
+        if (act && fin_node && !admissible(val, gamma)) st->bad_state = 1;
+        double* se = lds;
+        double* rs = lds + S;
+        double* es = rs + EB * n;
+        if (act) se[sidx] = terr;
+        __syncthreads();
+        if (act && i == 0) {
+            double r = 0.0;
+#pragma unroll
+            for (int k = 0; k < n; ++k) r += se[sbase + j * n + k];
+            rs[lec * n + j] = r;
+        }
+        __syncthreads();
+        if (act && node == 0) {
+            double r = 0.0;
+#pragma unroll
+            for (int k = 0; k < n; ++k) r += rs[lec * n + k];
+            es[lec] = r;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double r = 0.0;
+            for (int k = 0; k < nel; ++k) r += es[k];
+            a.partial[blockIdx.x] = r;
+        }
+        return;
+    }
     // --- CFL reduction over the nodes finalised here (records are dead: reuse s0)
     double l1 = -1.0, l2 = -1.0, lv = -1.0;
     if (fin_node) {
@@ -1674,6 +1745,10 @@ __global__ __launch_bounds__(kSurfBlock) void k_surface(KArgs a) {
         store4(out + off, val);
     }
     if (a.reduce == RED_NONE) return;
+    if (a.reduce == RED_ADAPT) {  // test_updated_solution on the nodes corrected here
+        if (act && any_ext && !admissible(val, c->gamma)) st->bad_state = 1;
+        return;
+    }
     const bool par = a.pv_off > 0;
     double l1 = -1.0, l2 = -1.0, lv = -1.0;
     if (act && any_ext) {
@@ -1933,6 +2008,46 @@ __global__ __launch_bounds__(kSurfBlock) void k_limiter(KArgs a) {
     }
 }
 
+// Step control of the adaptive LW loop (perform_step! for Adaptive and LW,
+// adaptive.jl:48-64, 95-173): total error sqrt(Σ temporal_errors / ndofs),
+// epsilon shift on a first attempt, factor 1 + atan(ε1^(β1/k) ε0^(β2/k) ε-1^(β3/k) - 1)
+// with (β1, β2, β3) = (0.6, -0.2, 0) (lwfr.jl:162) and k = nnodes; the step is
+// rejected when the updated state is not admissible or factor <= 0.81 (retry
+// from uⁿ with min(factor, 0.95)·dt), else accepted with next dt min(factor, 1.5)·dt.
+// A non-finite factor on a rejected step shrinks dt by 0.95.
+__device__ inline void adaptive_control(DevState* st, double total, double k) {
+    const double err = sqrt(total / st->ndofs);
+    if (st->nfail == 0) {
+        st->eps[0] = st->eps[1];
+        st->eps[1] = st->eps[2];
+    }
+    st->eps[2] = 1.0 / (err + 1e-12);
+    const double fac = pow(st->eps[2], 0.6 / k) * pow(st->eps[1], -0.2 / k) * pow(st->eps[0], 0.0 / k);
+    const double factor = 1.0 + atan(fac - 1.0);
+    const double t = st->t, tf = st->t_final;
+    double dt = st->dt;
+    if (st->bad_state || !(factor > 0.81)) {
+        st->nfail = st->nfail + 1;
+        dt = (factor < 0.95 ? factor : 0.95) * dt;
+        if (t + dt > tf) dt = tf - t;  // set_dt!
+        st->dt = dt;
+        st->bad_state = 0;
+        return;
+    }
+    st->nreject = st->nreject + st->nfail;
+    st->nfail = 0;
+    double dtn = (factor < 1.5 ? factor : 1.5) * dt;
+    double dts = dt;
+    if (t + dts > tf) dts = tf - t;  // set_t_and_iter!
+    const double t1 = t + dts;
+    st->t = t1;
+    st->iter = st->iter + 1;
+    st->cur = 1 - st->cur;
+    if (t1 + dtn > tf) dtn = tf - t1;  // set_dt!
+    st->dt = dtn;
+    st->finished = julia_isapprox(t1, tf);
+}
+
 // Final reduction of the per-workgroup partial maxima of J⁻¹(max λ1 + max λ2),
 // one workgroup per instance over that instance's partials (rng[inst] = volume
 // begin/end, surface begin/end; the volume range is skipped when use_vol is 0),
@@ -1951,6 +2066,24 @@ __global__ __launch_bounds__(kFinalizeBlock) void k_finalize(DevState* st_all, c
     DevState* st = st_all + blockIdx.x;
     const int4 r = rng[blockIdx.x];
     if (reduce == RED_STEP && st->finished) return;
+    if (reduce == RED_STEP && st->adaptive) {
+        // sum of the volume workgroups' temporal-error partials, then the controller
+        double m[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        int k = r.x + (int)threadIdx.x;
+        for (; k + 7 * kFinalizeBlock < r.y; k += 8 * kFinalizeBlock) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) m[q] += partial[k + q * kFinalizeBlock];
+        }
+        for (; k < r.y; k += kFinalizeBlock) m[0] += partial[k];
+        s[threadIdx.x] = ((m[0] + m[1]) + (m[2] + m[3])) + ((m[4] + m[5]) + (m[6] + m[7]));
+        __syncthreads();
+        for (int w = kFinalizeBlock / 2; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) adaptive_control(st, s[0], (double)n);
+        return;
+    }
     // eight independent accumulators keep several loads in flight per lane
     auto scan = [&](const double* p, int b, int e, double* m) {
         int k = b + (int)threadIdx.x;
@@ -2006,12 +2139,22 @@ __global__ __launch_bounds__(kFinalizeBlock) void k_finalize(DevState* st_all, c
 }
 
 // Initialisation of the device integrator state of every instance.
-__global__ void k_init_state(DevState* st_all, int ninst, double t0, double tf, double cfl, const int* cur) {
+// adaptive: dt_initial, tolerances and epsilon = (1, 1, 1) of the adaptive loop
+// (solve_lwfr's set_dt!(integrator, dt_initial); LWIntegrator, lwfr.jl:162-168)
+__global__ void k_init_state(DevState* st_all, int ninst, double t0, double tf, double cfl, const int* cur,
+                             int adaptive, double dt0, double abstol, double reltol) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= ninst) return;
     DevState* st = st_all + k;
     st->t = t0;
     st->dt = 0.0;
+    st->adaptive = adaptive;
+    st->nfail = 0;
+    st->nreject = 0;
+    st->eps[0] = st->eps[1] = st->eps[2] = 1.0;
+    st->abstol = abstol;
+    st->reltol = reltol;
+    if (adaptive) st->dt = (t0 + dt0 > tf) ? tf - t0 : dt0;
     st->t_final = tf;
     st->cfl = cfl;
     st->query = 0.0;

@@ -15,7 +15,7 @@ from .integrator import (LWIntegrator, LWSolution, compute_dt, isapprox, isfinis
                          set_t_and_iter, solve_lwfr)
 from .limiters import PositivityPreservingLimiterZhangShu, density, pressure
 from .mesh import TreeMesh
-from .semidiscretization import (DGSEM, LW, MDRK, CFLBased, FirstOrderReconstruction, IndicatorHennemannGassner,
+from .semidiscretization import (DGSEM, LW, MDRK, Adaptive, CFLBased, FirstOrderReconstruction, IndicatorHennemannGassner,
                                  LWUpdate, MUSCLHancockReconstruction, SemidiscretizationHyperbolic,
                                  SemidiscretizationHyperbolicParabolic, VolumeIntegralFR,
                                  VolumeIntegralFRShockCapturing, density_pressure, flux_lax_friedrichs,
@@ -26,7 +26,7 @@ __all__ = [
     "LobattoLegendreBasis", "CompressibleEulerEquations2D", "ConstantState", "FourQuadrantRiemann",
     "IsentropicVortex", "LinearScalarAdvectionEquation2D", "LWIntegrator", "LWSolution", "compute_dt", "isapprox",
     "isfinished", "perform_step", "set_dt", "set_t_and_iter", "solve_lwfr", "TreeMesh", "DGSEM", "LW", "MDRK",
-    "CFLBased", "LWUpdate", "SemidiscretizationHyperbolic", "VolumeIntegralFR", "flux_lax_friedrichs",
+    "CFLBased", "Adaptive", "LWUpdate", "SemidiscretizationHyperbolic", "VolumeIntegralFR", "flux_lax_friedrichs",
     "get_time_discretization", "lw_fourier_cfl", "rhs", "rhs_mdrk1", "rhs_mdrk2", "semidiscretize", "trixi2lw",
     "FirstOrderReconstruction", "IndicatorHennemannGassner", "MUSCLHancockReconstruction",
     "VolumeIntegralFRShockCapturing", "density_pressure", "PositivityPreservingLimiterZhangShu", "density",

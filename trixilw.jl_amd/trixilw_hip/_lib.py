@@ -28,7 +28,7 @@ EXPORTED = (
     "tlw_enable_kernel_timing", "tlw_kernel_times", "tlw_set_elements_per_workgroup",
     "tlw_get_elements_per_workgroup", "tlw_autotune", "tlw_set_positivity_limiter", "tlw_apply_positivity_limiter",
     "tlw_adjoint_lw", "tlw_create_batch", "tlw_n_instances", "tlw_batch_status", "tlw_enable_graphs",
-    "tlw_create_parabolic",
+    "tlw_create_parabolic", "tlw_solve_init_adaptive", "tlw_rejected_steps",
 )
 
 
@@ -136,6 +136,10 @@ def lib():
     L.tlw_state_ptr.argtypes = [vp, C.POINTER(vp)]
     L.tlw_solve_init.restype = C.c_int
     L.tlw_solve_init.argtypes = [vp, C.c_double, C.c_double, C.c_double]
+    L.tlw_solve_init_adaptive.restype = C.c_int
+    L.tlw_solve_init_adaptive.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double]
+    L.tlw_rejected_steps.restype = C.c_int
+    L.tlw_rejected_steps.argtypes = [vp, C.POINTER(C.c_int64)]
     L.tlw_advance.restype = C.c_int
     L.tlw_advance.argtypes = [vp, i32, dp, dp, C.POINTER(C.c_int64), C.POINTER(i32)]
     L.tlw_enable_kernel_timing.restype = C.c_int

@@ -21,7 +21,7 @@ import numpy as np
 
 from . import _lib
 from .limiters import PositivityPreservingLimiterZhangShu
-from .semidiscretization import (LW, MDRK, CFLBased, LWUpdate, rhs, rhs_mdrk1, rhs_mdrk2,
+from .semidiscretization import (LW, MDRK, Adaptive, CFLBased, LWUpdate, rhs, rhs_mdrk1, rhs_mdrk2,
                                  get_time_discretization)
 
 RTOL_ISAPPROX = 1.4901161193847656e-08  # sqrt(eps(Float64)), Julia's default for ≈
@@ -139,8 +139,19 @@ def solve_lwfr(lw_update: LWUpdate, callbacks=(), dt_initial=1e-6, tolerances=No
     ``max_steps`` optionally stops the loop early (for fixed-step runs).
     """
     if time_step_computation is None:
-        raise ValueError("time_step_computation=CFLBased(cfl) is required")
+        raise ValueError("time_step_computation=CFLBased(cfl) or Adaptive() is required")
     semi = lw_update.p
+    if isinstance(time_step_computation, Adaptive):
+        # perform_step! for Adaptive (adaptive.jl:95-173) runs on the device only
+        if callbacks or limiters or device_loop is False:
+            raise NotImplementedError("adaptive steps run in the device loop, without callbacks and limiters")
+        tol = tolerances or {}
+        tol = tol if isinstance(tol, dict) else tol._asdict()
+        integrator = LWIntegrator(lw_update, get_time_discretization(semi.solver), tol, time_step_computation)
+        set_dt(integrator, dt_initial)
+        semi.set_positivity_limiter(None)
+        _device_solve(integrator, time_step_computation, chunk, max_steps, dt_initial=dt_initial)
+        return LWSolution(integrator.u, lw_update, integrator.t, integrator.stats, integrator.dts)
     td = get_time_discretization(semi.solver)
     integrator = LWIntegrator(lw_update, td, tolerances, time_step_computation)
     set_dt(integrator, dt_initial)
@@ -170,7 +181,7 @@ def solve_lwfr(lw_update: LWUpdate, callbacks=(), dt_initial=1e-6, tolerances=No
     return LWSolution(integrator.u, lw_update, integrator.t, integrator.stats, integrator.dts)
 
 
-def _device_solve(integrator: LWIntegrator, tsc: CFLBased, chunk: int, max_steps):
+def _device_solve(integrator: LWIntegrator, tsc, chunk: int, max_steps, dt_initial=None):
     torch = __import__("torch")
     semi = integrator.p
     L = _lib.lib()
@@ -179,16 +190,30 @@ def _device_solve(integrator: LWIntegrator, tsc: CFLBased, chunk: int, max_steps
     u_host = np.ascontiguousarray(integrator.u.detach().cpu().numpy())
     dp = C.POINTER(C.c_double)
     _lib.check(L.tlw_set_state(h, u_host.ctypes.data_as(dp), u_host.size), h)
-    _lib.check(L.tlw_solve_init(h, integrator.t, integrator.tspan[1], tsc.cfl_number), h)
+    adaptive = isinstance(tsc, Adaptive)
+    if adaptive:
+        tol = integrator.tolerances
+        _lib.check(L.tlw_solve_init_adaptive(h, integrator.t, integrator.tspan[1], float(dt_initial),
+                                             float(tol.get("abstol", 0.0)), float(tol.get("reltol", 0.0))), h)
+    else:
+        _lib.check(L.tlw_solve_init(h, integrator.t, integrator.tspan[1], tsc.cfl_number), h)
     t, dt = C.c_double(), C.c_double()
     it, fin = C.c_int64(), C.c_int32()
+    attempts = 0
     while True:
         k = chunk if max_steps is None else min(chunk, max_steps - it.value)
         if k <= 0:
             break
         _lib.check(L.tlw_advance(h, k, C.byref(t), C.byref(dt), C.byref(it), C.byref(fin)), h)
+        attempts += k
         if fin.value:
             break
+        if adaptive and attempts > 100 * (it.value + 100):
+            raise RuntimeError(f"adaptive steps make no progress (t = {t.value}, dt = {dt.value})")
+    if adaptive:
+        nrej = C.c_int64()
+        _lib.check(L.tlw_rejected_steps(h, C.byref(nrej)), h)
+        integrator.stats.nreject = int(nrej.value)
     _lib.check(L.tlw_get_state(h, u_host.ctypes.data_as(dp), u_host.size), h)
     integrator.u = torch.from_numpy(u_host).to(semi.device)
     integrator.t = t.value

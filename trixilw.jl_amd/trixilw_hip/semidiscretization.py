@@ -39,6 +39,13 @@ class CFLBased:
     cfl_number: float
 
 
+class Adaptive:
+    """Error-controlled time stepping (src/basic_types.jl:10, src/solvers/adaptive.jl)."""
+
+    def __repr__(self):
+        return "Adaptive"
+
+
 @dataclass
 class VolumeIntegralFR:
     time_discretization: object
