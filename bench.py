@@ -291,6 +291,41 @@ def measure_navier_stokes(steps, cpu_steps, threads, with_cpu=True):
     return out
 
 
+def measure_adaptive(semi, u0, attempts, tol=(1e-6, 1e-6), dt0=1e-4):
+    """Sub-record: the adaptive (error-controlled) LW loop on the same mesh and
+    state: `attempts` device-loop attempts after 10 untimed ones."""
+    import torch
+    from trixilw_hip import _lib
+
+    L = _lib.lib()
+    h = semi.handle
+    dp = C.POINTER(C.c_double)
+    _lib.check(L.tlw_set_state(h, u0.ctypes.data_as(dp), u0.size), h)
+    _lib.check(L.tlw_solve_init_adaptive(h, 0.0, float("inf"), dt0, tol[0], tol[1]), h)
+    t, dt = C.c_double(), C.c_double()
+    it, fin = C.c_int64(), C.c_int32()
+    _lib.check(L.tlw_advance(h, 10, C.byref(t), C.byref(dt), C.byref(it), C.byref(fin)), h)
+    it0, t0s = int(it.value), t.value
+    nrej0 = C.c_int64()
+    _lib.check(L.tlw_rejected_steps(h, C.byref(nrej0)), h)
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    _lib.check(L.tlw_advance(h, attempts, C.byref(t), C.byref(dt), C.byref(it), C.byref(fin)), h)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - w0
+    nrej = C.c_int64()
+    _lib.check(L.tlw_rejected_steps(h, C.byref(nrej)), h)
+    acc = int(it.value) - it0
+    dofs = semi.ndofs()
+    out = {"value": dofs * acc / el, "unit": "DOF-updates/s (accepted steps)", "attempts": attempts,
+           "accepted": acc, "rejected": int(nrej.value - nrej0.value), "ms_per_attempt": round(1e3 * el / attempts, 4),
+           "simulated_time": t.value - t0s, "last_dt": dt.value,
+           "config": f"Adaptive(), abstol {tol[0]}, reltol {tol[1]}, dt_initial {dt0}, same mesh and state"}
+    log(f"bench: adaptive {out['value'] / 1e9:.3f} G DOF-updates/s, {out['ms_per_attempt']:.3f} ms/attempt, "
+        f"{acc} accepted / {out['rejected']} rejected")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -303,6 +338,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward step ratio")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-instance batch measurement")
     ap.add_argument("--no-ns", action="store_true", help="skip the Navier-Stokes sub-record")
+    ap.add_argument("--no-adaptive", action="store_true", help="skip the adaptive-stepping sub-record")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device copy (the reference kernel for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -405,6 +441,10 @@ def main():
         log(f"bench: adjoint step {adj_ms:.3f} ms = {adj_ms / ms_per_step:.2f} x forward step")
         del u, lam_a, lam_b
 
+    adaptive = None
+    if args.workload == DEFAULT_WORKLOAD and not args.no_adaptive:
+        adaptive = measure_adaptive(semi, u0, max(5, min(args.steps, 20)))  # tile chosen above
+
     batch = None
     if args.workload == DEFAULT_WORKLOAD and not args.no_batch:
         log("bench: 64-instance batch")
@@ -466,6 +506,7 @@ def main():
         "adjoint": adjoint,
         "batch": batch,
         "navier_stokes": navier_stokes,
+        "adaptive": adaptive,
     }
     print(json.dumps(out), flush=True)
     semi.close()
