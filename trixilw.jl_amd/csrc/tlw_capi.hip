@@ -45,7 +45,6 @@ struct tlw_ctx {
     double* d_auxU = nullptr;
     double* d_auxF = nullptr;
     double* d_partial = nullptr;
-    double* d_elem_max = nullptr;
     double* d_alpha_tmp = nullptr;
     double* d_alpha = nullptr;
     // workgroup tables (KArgs::vblk, sblk, etile) and per-instance partial ranges
@@ -57,7 +56,6 @@ struct tlw_ctx {
     bool sc = false;
     bool parabolic = false;   // compressible Navier-Stokes (tlw_create_parabolic)
     int pv_off = 0;           // offset of the viscous CFL partials (0: none)
-    double* d_elem_vmax = nullptr;
     int ebv = 0;              // elements per volume workgroup
     bool positivity = false;  // Zhang-Shu limiter in the device loop
     bool adaptive = false;    // error-controlled steps (tlw_solve_init_adaptive)
@@ -122,15 +120,14 @@ void drop_graphs(tlw_ctx* c) {
 void free_mesh(tlw_ctx* c) {
     drop_graphs(c);
     void* ptrs[] = {c->d_nbr,       c->d_invj,  c->d_center, c->d_length, c->d_trace,   c->d_buf[0],    c->d_buf[1],
-                    c->d_us,        c->d_auxU,  c->d_auxF,   c->d_partial, c->d_elem_max, c->d_alpha_tmp, c->d_alpha,
-                    c->d_vblk,      c->d_sblk,  c->d_etile,  c->d_rng,    c->d_state,   c->d_cur,
-                    c->d_elem_vmax};
+                    c->d_us,        c->d_auxU,  c->d_auxF,   c->d_partial, c->d_alpha_tmp, c->d_alpha,
+                    c->d_vblk,      c->d_sblk,  c->d_etile,  c->d_rng,    c->d_state,   c->d_cur};
     for (void* p : ptrs) (void)hipFree(p);
     if (c->h_state) (void)hipHostFree(c->h_state);
     c->d_nbr = nullptr;
     c->d_invj = c->d_center = c->d_length = c->d_trace = c->d_us = c->d_auxU = c->d_auxF = nullptr;
     c->d_buf[0] = c->d_buf[1] = nullptr;
-    c->d_partial = c->d_elem_max = c->d_alpha_tmp = c->d_alpha = c->d_elem_vmax = nullptr;
+    c->d_partial = c->d_alpha_tmp = c->d_alpha = nullptr;
     c->d_vblk = c->d_sblk = c->d_rng = nullptr;
     c->d_etile = nullptr;
     c->d_state = nullptr;
@@ -252,12 +249,10 @@ int upload_meshes(tlw_ctx* c, const tlw_mesh_desc* ms, int ninst) {
     // hyperbolic-parabolic: the viscous partials follow in a second block of the same size
     c->pv_off = c->parabolic ? c->vcap + scap : 0;
     HIPCHK(c, hipMalloc(&c->d_partial, sizeof(double) * (size_t)((c->parabolic ? 2 : 1) * (c->vcap + scap) + 1)));
-    if (c->parabolic) HIPCHK(c, hipMalloc(&c->d_elem_vmax, sizeof(double) * (size_t)E));
     HIPCHK(c, hipMalloc(&c->d_vblk, sizeof(int4) * (size_t)c->vcap));
     HIPCHK(c, hipMalloc(&c->d_sblk, sizeof(int4) * (size_t)scap));
     HIPCHK(c, hipMalloc(&c->d_etile, sizeof(int) * (size_t)E));
     HIPCHK(c, hipMalloc(&c->d_rng, sizeof(int4) * (size_t)ninst));
-    HIPCHK(c, hipMalloc(&c->d_elem_max, sizeof(double) * 2 * (size_t)E));
     if (c->sc) {
         HIPCHK(c, hipMalloc(&c->d_alpha_tmp, sizeof(double) * (size_t)E));
         HIPCHK(c, hipMalloc(&c->d_alpha, sizeof(double) * (size_t)E));
@@ -295,9 +290,7 @@ KArgs base_args(tlw_ctx* c) {
     a.c = c->d_const;
     a.st = c->d_state;
     a.partial = c->d_partial;
-    a.elem_max = c->d_elem_max;
     a.pv_off = c->pv_off;
-    a.elem_vmax = c->d_elem_vmax;
     a.vol_grid = c->vgrid;
     a.ebv = c->ebv;
     a.vblk = c->d_vblk;
@@ -359,7 +352,9 @@ template <int N> void launch_surface_t(tlw_ctx* c, const KArgs& a) {
         if (c->has_boundary) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_surface<N, true, false>), g, b, 0, c->stream, a);
         else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_surface<N, false, false>), g, b, 0, c->stream, a);
     }
-    if (a.reduce != RED_NONE) launch_finalize(c, a.reduce == RED_ADAPT ? RED_STEP : a.reduce, 1);
+    // the step's CFL partials come from the surface kernel alone; with adaptive
+    // steps the volume kernel's temporal-error sums are read by the finalize kernel
+    if (a.reduce != RED_NONE) launch_finalize(c, a.reduce == RED_ADAPT ? RED_STEP : a.reduce, 0);
 }
 void launch_surface(tlw_ctx* c, const KArgs& a) {
     if (c->N == 4) launch_surface_t<4>(c, a);

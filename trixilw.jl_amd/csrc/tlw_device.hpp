@@ -117,9 +117,10 @@ struct KArgs {
     DevState* st;
     double* auxU;      // optional full Ũ
     double* auxF;      // optional full F̃
-    double* partial;   // per-workgroup max of J⁻¹(max λ1 + max λ2): volume part, then surface part
-    double* elem_max;  // [E][2]: max λ1, λ2 over the nodes the volume kernel finalised
-    int vol_grid;      // number of volume workgroups (offset of the surface partials)
+    // per-workgroup partials: the volume kernel's temporal-error sums (adaptive
+    // steps), then the node-per-lane kernels' maxima of J⁻¹(max λ1 + max λ2)
+    double* partial;
+    int vol_grid;      // number of volume workgroups (offset of the node-per-lane partials)
     int ebv;           // elements per volume workgroup
     // Workgroup tables: {first element, elements, instance, 0} per workgroup of the
     // volume grid (vblk) and of the node-per-lane grids (sblk).  Workgroups never
@@ -131,10 +132,8 @@ struct KArgs {
     double* alpha_out;        // smoothed blending factors [E] (optional)
     // hyperbolic-parabolic CFL reduction: the per-workgroup maxima of the
     // viscous bound J⁻² max(4μ/(3ρ), μκ/(ρ Pr)) sit pv_off entries after the
-    // advective ones (0: no parabolic terms); elem_vmax is elem_max's
-    // counterpart for that bound
+    // advective ones (0: no parabolic terms)
     int pv_off;
-    double* elem_vmax;  // [E]
     double t_arg, dt_arg;
     int use_state_dt;  // 1: t, dt from DevState (device loop); 0: from t_arg/dt_arg
     int step_mode;     // 1: out = u + dt * rhs; 0: out = rhs
@@ -1279,9 +1278,9 @@ __device__ __forceinline__ void mdrkp2_element(const double* uN, const double* g
 // Volume kernel: element computation, then the interfaces inside the
 // workgroup, the lifted surface terms (weak form, ℓ̂ = 1/w at the end nodes),
 // the Jacobian and the update.  Faces leaving the workgroup write their face
-// record for the surface kernel.  In step mode the CFL reduction covers the
-// nodes finalised here: whole elements go to the workgroup partial, elements
-// with an outside face leave their per-node maxima in elem_max.
+// record for the surface kernel.  The CFL reduction of a step is left to the
+// surface kernel, which reads every node once after its corrections (a
+// reduction here costs barriers and LDS passes worth 15% of this kernel).
 // Waves per SIMD the register budget is sized for: 3 (168 VGPRs) for the
 // plain scheme (TLW_VOL_WAVES at build time); 2 (256 VGPRs) with shock
 // capturing, whose subcell FV and blended face fluxes would otherwise spill.
@@ -1503,7 +1502,6 @@ __global__ __launch_bounds__((VGeo<N, EB>::NT), kVolWaves<SC || PAR>) void k_vol
     // --- lifted internal fluxes, Jacobian, output
     const bool intx = fx >= 0 && nbx >= e0 && nbx < e0 + nel;
     const bool inty = fy >= 0 && nby >= e0 && nby < e0 + nel;
-    const bool fin_node = (fx < 0 || intx) && (fy < 0 || inty);
     double surf[4] = {0.0, 0.0, 0.0, 0.0};
     if (intx) {
         const double wf = (fx & 1) ? c->bnd_hi : -c->bnd_lo;
@@ -1522,11 +1520,9 @@ __global__ __launch_bounds__((VGeo<N, EB>::NT), kVolWaves<SC || PAR>) void k_vol
         val[v] = a.step_mode ? fma(dt, d, uN[v]) : d;
     }
     if (act) store4(out + off, val);
-    if (a.reduce == RED_NONE) return;
     if constexpr (KIND == VOL_LW_ADAPT) {
-        // test_updated_solution on the nodes finalised here, and the workgroup's
-        // sum of its elements' temporal errors (adaptive.jl:30-46, 48-64)
-        if (act && fin_node && !admissible(val, gamma)) st->bad_state = 1;
+        if (a.reduce != RED_ADAPT) return;
+        // the workgroup's sum of its elements' temporal errors (adaptive.jl:48-64)
         double* se = lds;
         double* rs = lds + S;
         double* es = rs + EB * n;
@@ -1551,83 +1547,9 @@ __global__ __launch_bounds__((VGeo<N, EB>::NT), kVolWaves<SC || PAR>) void k_vol
             for (int k = 0; k < nel; ++k) r += es[k];
             a.partial[blockIdx.x] = r;
         }
-        return;
     }
-    // --- CFL reduction over the nodes finalised here (records are dead: reuse s0)
-    double l1 = -1.0, l2 = -1.0, lv = -1.0;
-    if (fin_node) {
-        bool bad;
-        node_speeds(val, gamma, l1, l2, bad);
-        if (act && bad) st->bad_state = 1;
-        if constexpr (PAR) lv = lam_visc(val, invJ, c);
-    }
-    double* sl1 = lds;
-    double* sl2 = lds + S;
-    double* rm = lds + 2 * S;            // row maxima [EB][n][2]
-    double* sm = rm + 2 * EB * n;        // element values [EB]
-    double* slv = lds + 3 * S;           // viscous bound: node values, row maxima, element values
-    double* rmv = lds + 4 * S;
-    double* smv = rmv + EB * n;
-    if (act) {
-        sl1[sidx] = l1;
-        sl2[sidx] = l2;
-        if constexpr (PAR) slv[sidx] = lv;
-    }
-    __syncthreads();
-    if (act && i == 0) {
-        double m1 = 0.0, m2 = 0.0;
-#pragma unroll
-        for (int k = 0; k < n; ++k) {
-            m1 = dmax(m1, sl1[sbase + j * n + k]);
-            m2 = dmax(m2, sl2[sbase + j * n + k]);
-        }
-        rm[(lec * n + j) * 2] = m1;
-        rm[(lec * n + j) * 2 + 1] = m2;
-        if constexpr (PAR) {
-            double mv = 0.0;
-#pragma unroll
-            for (int k = 0; k < n; ++k) mv = dmax(mv, slv[sbase + j * n + k]);
-            rmv[lec * n + j] = mv;
-        }
-    }
-    __syncthreads();
-    if (act && node == 0) {
-        double m1 = 0.0, m2 = 0.0, mv = 0.0;
-#pragma unroll
-        for (int k = 0; k < n; ++k) {
-            m1 = dmax(m1, rm[(lec * n + k) * 2]);
-            m2 = dmax(m2, rm[(lec * n + k) * 2 + 1]);
-            if constexpr (PAR) mv = dmax(mv, rmv[lec * n + k]);
-        }
-        bool whole = true;
-#pragma unroll
-        for (int f = 0; f < 4; ++f) {
-            const int ne = a.nbr[4 * e + f];
-            whole = whole && ne >= e0 && ne < e0 + nel;
-        }
-        if (whole) {
-            sm[le] = invJ * (m1 + m2);
-            if constexpr (PAR) smv[le] = mv;
-        } else {
-            sm[le] = 0.0;
-            a.elem_max[2 * e] = m1;
-            a.elem_max[2 * e + 1] = m2;
-            if constexpr (PAR) {
-                smv[le] = 0.0;
-                a.elem_vmax[e] = mv;
-            }
-        }
-    }
-    __syncthreads();
-    if (tid == 0) {
-        double bm = 0.0, bv = 0.0;
-        for (int k = 0; k < nel; ++k) {
-            bm = dmax(bm, sm[k]);
-            if constexpr (PAR) bv = dmax(bv, smv[k]);
-        }
-        a.partial[blockIdx.x] = bm;
-        if constexpr (PAR) a.partial[a.pv_off + blockIdx.x] = bv;
-    }
+    // the CFL reduction of a step (and the admissibility test) is the surface
+    // kernel's: it reads every node once after the last correction
 }
 
 // Outer record of a domain-boundary face node: time average of the boundary
@@ -1669,8 +1591,9 @@ __device__ inline void boundary_record(const KArgs& a, int e, int face, int l, d
 
 // Surface kernel: the faces that leave a volume workgroup (a.ebv elements).
 // Each interface is evaluated by both neighbours from identical inputs, so
-// both see the same bits.  Elements with such faces finish their CFL
-// reduction here, combining elem_max with the corrected face nodes.
+// both see the same bits.  In step mode every lane then takes its node's
+// final value (corrected here or finalised by the volume kernel) into the
+// CFL maxima and the admissibility test.
 template <int N, bool BC, bool SC>
 __global__ __launch_bounds__(kSurfBlock) void k_surface(KArgs a) {
     using G_ = SGeo<N>;
@@ -1696,12 +1619,6 @@ __global__ __launch_bounds__(kSurfBlock) void k_surface(KArgs a) {
     const int eblk = a.etile[e];
     const int fx = (i == 0) ? 0 : ((i == n - 1) ? 1 : -1);
     const int fy = (j == 0) ? 2 : ((j == n - 1) ? 3 : -1);
-    bool elem_ext = false;
-#pragma unroll
-    for (int f = 0; f < 4; ++f) {
-        const int nb = a.nbr[4 * e + f];
-        elem_ext = elem_ext || nb < 0 || a.etile[nb] != eblk;
-    }
     const int nbx = fx >= 0 ? a.nbr[4 * e + fx] : 0;
     const int nby = fy >= 0 ? a.nbr[4 * e + fy] : 0;
     const bool extx = fx >= 0 && (nbx < 0 || a.etile[nbx] != eblk);
@@ -1745,39 +1662,33 @@ __global__ __launch_bounds__(kSurfBlock) void k_surface(KArgs a) {
         store4(out + off, val);
     }
     if (a.reduce == RED_NONE) return;
-    if (a.reduce == RED_ADAPT) {  // test_updated_solution on the nodes corrected here
-        if (act && any_ext && !admissible(val, c->gamma)) st->bad_state = 1;
+    // Every node's final value: the corrected ones are in registers, the volume
+    // kernel finalised the others.  CFL maxima (Trixi max_dt) and the
+    // admissibility test of the updated state cover all nodes here.
+    if (act && !any_ext) load4(out + ((size_t)e * NN + node) * 4, val);
+    if (a.reduce == RED_ADAPT) {  // test_updated_solution (adaptive.jl:30-46)
+        if (act && !admissible(val, c->gamma)) st->bad_state = 1;
         return;
     }
     const bool par = a.pv_off > 0;
-    double l1 = -1.0, l2 = -1.0, lv = -1.0;
-    if (act && any_ext) {
+    if (act) {
+        double l1, l2;
         bool bad;
         node_speeds(val, c->gamma, l1, l2, bad);
         if (bad) st->bad_state = 1;
-        if (par) lv = lam_visc(val, invJ, c);
-    }
-    if (act) {
         s_l1[lec * NN + node] = l1;
         s_l2[lec * NN + node] = l2;
-        s_lv[lec * NN + node] = lv;
+        s_lv[lec * NN + node] = par ? lam_visc(val, invJ, c) : 0.0;
     }
     __syncthreads();
     if (act && node == 0) {
-        double m = 0.0, mv = 0.0;
-        if (elem_ext) {
-            double m1 = a.elem_max[2 * e], m2 = a.elem_max[2 * e + 1];
-            for (int k = 0; k < NN; ++k) {
-                m1 = dmax(m1, s_l1[lec * NN + k]);
-                m2 = dmax(m2, s_l2[lec * NN + k]);
-            }
-            m = invJ * (m1 + m2);
-            if (par) {
-                mv = a.elem_vmax[e];
-                for (int k = 0; k < NN; ++k) mv = dmax(mv, s_lv[lec * NN + k]);
-            }
+        double m1 = 0.0, m2 = 0.0, mv = 0.0;
+        for (int k = 0; k < NN; ++k) {
+            m1 = dmax(m1, s_l1[lec * NN + k]);
+            m2 = dmax(m2, s_l2[lec * NN + k]);
+            mv = dmax(mv, s_lv[lec * NN + k]);
         }
-        s_m[lec] = m;
+        s_m[lec] = invJ * (m1 + m2);
         s_mv[lec] = mv;
     }
     __syncthreads();
