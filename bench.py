@@ -291,9 +291,10 @@ def measure_navier_stokes(steps, cpu_steps, threads, with_cpu=True):
     return out
 
 
-def measure_adaptive(semi, u0, attempts, tol=(1e-6, 1e-6), dt0=1e-4):
+def measure_adaptive(semi, u0, attempts, tol=(1e-10, 1e-10), dt0=1e-5):
     """Sub-record: the adaptive (error-controlled) LW loop on the same mesh and
-    state: `attempts` device-loop attempts after 10 untimed ones."""
+    state: `attempts` device-loop attempts after 30 untimed ones (the controller
+    grows dt by at most 1.5 per step from dt0)."""
     import torch
     from trixilw_hip import _lib
 
@@ -304,7 +305,7 @@ def measure_adaptive(semi, u0, attempts, tol=(1e-6, 1e-6), dt0=1e-4):
     _lib.check(L.tlw_solve_init_adaptive(h, 0.0, float("inf"), dt0, tol[0], tol[1]), h)
     t, dt = C.c_double(), C.c_double()
     it, fin = C.c_int64(), C.c_int32()
-    _lib.check(L.tlw_advance(h, 10, C.byref(t), C.byref(dt), C.byref(it), C.byref(fin)), h)
+    _lib.check(L.tlw_advance(h, 30, C.byref(t), C.byref(dt), C.byref(it), C.byref(fin)), h)
     it0, t0s = int(it.value), t.value
     nrej0 = C.c_int64()
     _lib.check(L.tlw_rejected_steps(h, C.byref(nrej0)), h)
