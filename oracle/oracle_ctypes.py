@@ -48,6 +48,9 @@ class _Desc(C.Structure):
         ("parabolic", C.c_int),
         ("visc_mu", C.c_double),
         ("prandtl", C.c_double),
+        ("curved", C.c_int),
+        ("map_kind", C.c_int),
+        ("map_par", C.c_double * 4),
     ]
 
 
@@ -111,7 +114,9 @@ def _ptr(a: np.ndarray):
 
 
 AUX = {"U": 0, "F": 1, "temporal_errors": 2, "surface_flux_values": 3, "us": 4, "U2": 5, "F2": 6, "alpha": 7,
-       "fn_low": 8, "gradients_x": 9, "gradients_y": 10, "Fv": 11, "Fv2": 12}
+       "fn_low": 8, "gradients_x": 9, "gradients_y": 10, "Fv": 11, "Fv2": 12, "contravariant_vectors": 13,
+       "inverse_jacobian": 14}
+MAP_AFFINE, MAP_WARP = 0, 1
 PARABOLIC = {None: 0, "laplace": 1, "navier_stokes": 2}
 
 
@@ -121,13 +126,16 @@ class Oracle:
     def __init__(self, polydeg, nx, ny, xmin, ymin, h, periodic=(True, True), equation="euler", gamma=1.4,
                  adv=(1.0, 1.0), flux_order=0, bc_kind=BV_NONE, bc_par=(), tolerances=(0.0, 0.0), nthreads=0,
                  shock_capturing=False, alpha_max=0.5, alpha_min=0.001, alpha_smooth=True, positivity=None,
-                 parabolic=None, mu=0.0, prandtl=0.72):
+                 parabolic=None, mu=0.0, prandtl=0.72, mapping=None):
         """``positivity``: (threshold_density, threshold_pressure) of a Zhang-Shu
         limiter that ``solve`` applies as a stage limiter, or None.
         ``parabolic``: None, "laplace" (LaplaceDiffusion2D with diffusivity ``mu``,
         linear advection) or "navier_stokes" (CompressibleNavierStokesDiffusion2D,
         dynamic viscosity ``mu``, Prandtl number ``prandtl``, conservative
-        gradient variables); periodic meshes only."""
+        gradient variables); periodic meshes only.
+        ``mapping``: None (TreeMesh) or (MAP_AFFINE, (x0, y0, Lx, Ly)) /
+        (MAP_WARP, (L, A)) for a periodic curvilinear StructuredMesh of nx × ny
+        cells (xmin, ymin, h are then unused)."""
         d = _Desc()
         d.polydeg, d.nx, d.ny = polydeg, nx, ny
         d.xmin, d.ymin, d.h = xmin, ymin, h
@@ -149,6 +157,11 @@ class Oracle:
             d.pos_threshold[0], d.pos_threshold[1] = positivity
         d.parabolic = PARABOLIC[parabolic]
         d.visc_mu, d.prandtl = mu, prandtl
+        if mapping is not None:
+            d.curved = 1
+            d.map_kind = int(mapping[0])
+            for k, v in enumerate(mapping[1]):
+                d.map_par[k] = v
         self._desc = d
         self._h = lib().tlwo_create(C.byref(d))
         if not self._h:

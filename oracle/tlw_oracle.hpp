@@ -248,6 +248,17 @@ template <class T> struct Euler2D {
         l1 = abs_(v1) + c;
         l2 = abs_(v2) + c;
     }
+    // max_abs_speed_naive with a normal direction (curvilinear meshes):
+    // max(|v_ll·n|, |v_rr·n|) + max(c_ll, c_rr) |n|
+    T max_abs_speed_naive_normal(const T* ul, const T* ur, const double* nrm) const {
+        T rl, v1l, v2l, pl, rr, v1r, v2r, pr;
+        cons2prim(ul, rl, v1l, v2l, pl);
+        cons2prim(ur, rr, v1r, v2r, pr);
+        const T vl = v1l * nrm[0] + v2l * nrm[1];
+        const T vr = v1r * nrm[0] + v2r * nrm[1];
+        const T cl = sqrt_(gamma * pl / rl), cr = sqrt_(gamma * pr / rr);
+        return max_(abs_(vl), abs_(vr)) + max_(cl, cr) * std::sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1]);
+    }
     // Trixi density and pressure, the variables of the positivity limiter
     T density(const T* u) const { return u[0]; }
     T pressure(const T* u) const { return (gamma - 1.0) * (u[3] - 0.5 * (u[1] * u[1] + u[2] * u[2]) / u[0]); }
@@ -280,6 +291,9 @@ template <class T> struct LinAdv2D {
         return T(std::fabs(orientation == 1 ? a1 : a2));
     }
     void max_abs_speeds(const T*, T& l1, T& l2) const { l1 = T(std::fabs(a1)); l2 = T(std::fabs(a2)); }
+    T max_abs_speed_naive_normal(const T*, const T*, const double* nrm) const {
+        return T(std::fabs(a1 * nrm[0] + a2 * nrm[1]));
+    }
     T density_pressure(const T* u) const { return u[0]; }
     T density(const T* u) const { return u[0]; }
     T pressure(const T* u) const { return u[0]; }
@@ -415,6 +429,11 @@ template <class T, class Eq> struct Solver {
     std::vector<T> us, U2, F2;  // MDRK stage cache (containers.jl:83-98)
 
     virtual ~Solver() = default;
+    // node coordinates (TreeMesh: from the element's centre and edge length)
+    virtual void node_xy(int i, int j, int e, double& x, double& y) const {
+        x = mesh.node_x(basis, i, e);
+        y = mesh.node_y(basis, j, e);
+    }
     // gradients and viscous caches of the hyperbolic-parabolic solver (tlwo_get_aux 9-12)
     virtual const std::vector<T>* extra_aux(int) const { return nullptr; }
 

@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <memory>
 
+#include "tlw_oracle_curved.hpp"
 #include "tlw_oracle_parabolic.hpp"
 
 using namespace tlwo;
@@ -35,6 +36,11 @@ struct tlwo_desc {
     int parabolic;
     double visc_mu;  // diffusivity (Laplace) or dynamic viscosity μ (Navier-Stokes)
     double prandtl;
+    // StructuredMesh(cells, mapping): 0 TreeMesh, 1 curvilinear with map_kind
+    // (MappingKind) and map_par; periodic, plain FR, LW
+    int curved;
+    int map_kind;
+    double map_par[4];
 };
 
 struct tlwo_ctx {
@@ -70,9 +76,25 @@ void* tlwo_create(const tlwo_desc* d) {
         if (!d->periodic_x || !d->periodic_y || d->shock_capturing) return nullptr;
         if ((d->parabolic == 1) != (d->equation == 1)) return nullptr;
     }
+    if (d->curved && (d->parabolic || d->shock_capturing || !d->periodic_x || !d->periodic_y)) return nullptr;
     auto* c = new tlwo_ctx;
     c->equation = d->equation;
-    if (d->parabolic == 2) {
+    if (d->curved) {
+        if (d->equation == 0) {
+            auto* p = new CurvedSolver<double, Euler2D<double>>;
+            p->eq.gamma = d->gamma;
+            configure(*p, d);
+            p->setup_curved(d->map_kind, d->map_par);
+            c->euler.reset(p);
+        } else {
+            auto* p = new CurvedSolver<double, LinAdv2D<double>>;
+            p->eq.a1 = d->adv_a1;
+            p->eq.a2 = d->adv_a2;
+            configure(*p, d);
+            p->setup_curved(d->map_kind, d->map_par);
+            c->linadv.reset(p);
+        }
+    } else if (d->parabolic == 2) {
         auto* p = new ParabolicSolver<double, Euler2D<double>, NavierStokesDiffusion2D<double>>;
         p->eq.gamma = d->gamma;
         p->visc.set(d->gamma, d->visc_mu, d->prandtl);
@@ -144,8 +166,7 @@ void tlwo_node_coords(void* ctx, double* x, double* y) {
             for (int j = 0; j < n; ++j)
                 for (int i = 0; i < n; ++i) {
                     const size_t k = ((size_t)e * n + j) * n + i;
-                    x[k] = s.mesh.node_x(s.basis, i, e);
-                    y[k] = s.mesh.node_y(s.basis, j, e);
+                    s.node_xy(i, j, e, x[k], y[k]);
                 }
     });
 }
@@ -185,7 +206,9 @@ void tlwo_set_us(void* ctx, const double* us) {
 //        3 = surface_flux_values [v,l,face,e], 4 = us, 5 = U2, 6 = F2,
 //        7 = alpha [e], 8 = fn_low [v,l,face,e],
 //        hyperbolic-parabolic only: 9 = gradients_x, 10 = gradients_y [v,i,j,e],
-//        11 = Fv [v,d,i,j,e], 12 = mdrk Fv2
+//        11 = Fv [v,d,i,j,e], 12 = mdrk Fv2,
+//        curvilinear only: 13 = contravariant vectors [4,i,j,e] (Ja11, Ja12, Ja21, Ja22),
+//        14 = inverse Jacobian [i,j,e]
 long long tlwo_get_aux(void* ctx, int which, double* out) {
     long long len = -1;
     DISPATCH(ctx, {

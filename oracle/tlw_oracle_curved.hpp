@@ -1,0 +1,354 @@
+// tlw_oracle_curved.hpp -- CPU restatement of TrixiLW's Lax-Wendroff right-hand
+// side on a curvilinear 2-D StructuredMesh (periodic), compressible Euler.
+//
+// TEST INFRASTRUCTURE ONLY (parity oracle and CPU baseline of the curvilinear
+// path); the product never links or calls it.
+//
+// Reference (paths relative to the TrixiLW.jl tree):
+//   rhs!                          src/solvers/dgsem_structured/dg_2d.jl:17-53
+//   contravariant_flux            dg_2d.jl:541-556
+//   lw_volume_kernel_3! / 4!      dg_2d.jl:1379-1685, 1687-2105
+//   calc_interface_flux!          dg_2d.jl:2107-2290 (λ from uⁿ, jump on Ũ, F̃ in the
+//                                 right element's contravariant direction)
+//   normal_product                src/solvers/fr.jl:146-150
+// and, from the un-vendored Trixi.jl (restated from their published
+// definitions): StructuredMesh node coordinates (the element's GLL nodes in the
+// global reference square, mapped), calc_jacobian_matrix! (derivatives of the
+// interpolated coordinates with D), the 2-D cross-product contravariant vectors
+// Ja¹ = (y_η, -x_η), Ja² = (-y_ξ, x_ξ), inverse_jacobian = 1/(x_ξ y_η - x_η y_ξ),
+// max_abs_speed_naive with a normal direction, the weak-form surface integral of
+// StructuredMesh (the TreeMesh one), apply_jacobian! with the node's J⁻¹, and
+// max_dt of curvilinear meshes (|Ja^k · (λ1, λ2)| |J⁻¹| per node).
+//
+// Pins (tests/test_oracle_curved_pins.py): an affine mapping reproduces the
+// TreeMesh solver with the Structured argument order; free-stream
+// preservation and discrete conservation on a warped mesh; convergence of the
+// isentropic vortex on it.
+#pragma once
+#include <type_traits>
+
+#include "tlw_oracle.hpp"
+
+namespace tlwo {
+
+// Mappings of the global reference square [-1, 1]² (StructuredMesh(cells, mapping)):
+//   MAP_AFFINE: the box [p0, p0 + p2] × [p1, p1 + p3];
+//   MAP_WARP:   [0, L]² (p0 = L) with x = L (s + A sin 2πs sin 2πt),
+//               y = L (t + A sin 2πs sin 2πt), s, t ∈ [0, 1], A = p1; the warp
+//               vanishes on the domain boundary, so the mesh is periodic.
+enum MappingKind { MAP_AFFINE = 0, MAP_WARP = 1 };
+
+inline void apply_mapping(int kind, const double* p, double xi, double eta, double& x, double& y) {
+    if (kind == MAP_AFFINE) {
+        x = p[0] + 0.5 * (xi + 1.0) * p[2];
+        y = p[1] + 0.5 * (eta + 1.0) * p[3];
+    } else {
+        const double pi = 3.141592653589793;
+        const double s = 0.5 * (xi + 1.0), t = 0.5 * (eta + 1.0);
+        const double w = p[1] * std::sin(2.0 * pi * s) * std::sin(2.0 * pi * t);
+        x = p[0] * (s + w);
+        y = p[0] * (t + w);
+    }
+}
+
+template <class T, class Eq> struct CurvedSolver : Solver<T, Eq> {
+    using B = Solver<T, Eq>;
+    static constexpr int NV = Eq::NV;
+    using B::basis;
+    using B::eq;
+    using B::fidx;
+    using B::idx;
+    using B::mesh;
+    using B::n;
+    using B::sidx;
+    std::vector<double> xn, yn;  // node coordinates [e][j][i]
+    std::vector<double> Ja;      // [e][j][i][4]: Ja11, Ja12, Ja21, Ja22
+    std::vector<double> ijn;     // inverse Jacobian [e][j][i]
+
+    size_t nidx(int i, int j, int e) const { return ((size_t)e * n + j) * n + i; }
+    void node_xy(int i, int j, int e, double& x, double& y) const override {
+        x = xn[nidx(i, j, e)];
+        y = yn[nidx(i, j, e)];
+    }
+    const std::vector<T>* extra_aux(int which) const override {
+        if constexpr (std::is_same<T, double>::value) {
+            if (which == 13) return &Ja;
+            if (which == 14) return &ijn;
+        }
+        return nullptr;
+    }
+
+    // Trixi StructuredMesh cache: node coordinates, Jacobian matrix from the
+    // derivative matrix, contravariant vectors, inverse Jacobian.
+    void setup_curved(int kind, const double* p) {
+        const int nx = mesh.nx, ny = mesh.ny, E = mesh.E;
+        xn.assign((size_t)E * n * n, 0.0);
+        yn.assign(xn.size(), 0.0);
+        Ja.assign(4 * xn.size(), 0.0);
+        ijn.assign(xn.size(), 0.0);
+        const double dx = 2.0 / nx, dy = 2.0 / ny;
+        for (int e = 0; e < E; ++e) {
+            const int cx = e % nx, cy = e / nx;
+            const double ox = -1.0 + cx * dx + dx / 2, oy = -1.0 + cy * dy + dy / 2;
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < n; ++i)
+                    apply_mapping(kind, p, ox + dx / 2 * basis.nodes[i], oy + dy / 2 * basis.nodes[j], xn[nidx(i, j, e)],
+                                  yn[nidx(i, j, e)]);
+            const auto& D = basis.D;
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < n; ++i) {
+                    double x_xi = 0, y_xi = 0, x_eta = 0, y_eta = 0;
+                    for (int k = 0; k < n; ++k) {
+                        x_xi += D[i][k] * xn[nidx(k, j, e)];
+                        y_xi += D[i][k] * yn[nidx(k, j, e)];
+                        x_eta += xn[nidx(i, k, e)] * D[j][k];
+                        y_eta += yn[nidx(i, k, e)] * D[j][k];
+                    }
+                    double* J = &Ja[4 * nidx(i, j, e)];
+                    J[0] = y_eta;
+                    J[1] = -x_eta;
+                    J[2] = -y_xi;
+                    J[3] = x_xi;
+                    ijn[nidx(i, j, e)] = 1.0 / (x_xi * y_eta - x_eta * y_xi);
+                }
+        }
+    }
+
+    // contravariant_flux (dg_2d.jl:550-556)
+    void cv_fluxes(const T* u, const double* J, T* f, T* g, T* fc, T* gc) const {
+        eq.fluxes(u, f, g);
+        for (int v = 0; v < NV; ++v) {
+            fc[v] = J[0] * f[v] + J[1] * g[v];
+            gc[v] = J[2] * f[v] + J[3] * g[v];
+        }
+    }
+
+    // lw_volume_kernel_3!/4! for StructuredMesh (dg_2d.jl:1379-2105): the
+    // TreeMesh procedure with contravariant fluxes in the derivative
+    // contractions and the node's J⁻¹; the Cartesian time average F̃ is kept
+    // for the interfaces.  Source terms are `nothing`; the temporal-error
+    // estimate (adaptive mode) is not computed.
+    void lw_volume_curved(T* du, const T* u, double dt, int e) {
+        const auto& D = basis.D;
+        const auto& Dh = basis.Dhat;
+        const int N = basis.N;
+        T f[MAXN][MAXN][NV], g[MAXN][MAXN][NV], fc[MAXN][MAXN][NV], gc[MAXN][MAXN][NV];
+        T F[MAXN][MAXN][NV], G[MAXN][MAXN][NV], Fc[MAXN][MAXN][NV], Gc[MAXN][MAXN][NV];
+        T Uc[MAXN][MAXN][NV], w[MAXN][MAXN][NV], dfc[MAXN][MAXN][NV], dgc[MAXN][MAXN][NV];
+        T st[4][MAXN][MAXN][NV];  // up, um, upp, umm
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) {
+                const T* un = &u[idx(0, i, j, e)];
+                cv_fluxes(un, &Ja[4 * nidx(i, j, e)], f[j][i], g[j][i], fc[j][i], gc[j][i]);
+                for (int v = 0; v < NV; ++v) {
+                    F[j][i][v] = f[j][i][v];
+                    G[j][i][v] = g[j][i][v];
+                    Fc[j][i][v] = fc[j][i][v];
+                    Gc[j][i][v] = gc[j][i][v];
+                    Uc[j][i][v] = un[v];
+                    for (int q = 0; q < 4; ++q) st[q][j][i][v] = un[v];
+                }
+            }
+        // w = J⁻¹ (-dt D) (a, b) for the contravariant pair (a, b)
+        auto advance = [&](const T (*a)[MAXN][NV], const T (*b)[MAXN][NV]) {
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < n; ++i)
+                    for (int v = 0; v < NV; ++v) w[j][i][v] = T(0);
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < n; ++i) {
+                    for (int ii = 0; ii < n; ++ii)
+                        for (int v = 0; v < NV; ++v) w[j][ii][v] = w[j][ii][v] + (-dt * D[ii][i]) * a[j][i][v];
+                    for (int jj = 0; jj < n; ++jj)
+                        for (int v = 0; v < NV; ++v) w[jj][i][v] = w[jj][i][v] + (-dt * D[jj][j]) * b[j][i][v];
+                }
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < n; ++i)
+                    for (int v = 0; v < NV; ++v) w[j][i][v] = w[j][i][v] * ijn[nidx(i, j, e)];
+        };
+        auto add_states = [&](double cU, double c0, double c1, double c2, double c3) {
+            const double c[4] = {c0, c1, c2, c3};
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < n; ++i)
+                    for (int v = 0; v < NV; ++v) {
+                        Uc[j][i][v] = Uc[j][i][v] + cU * w[j][i][v];
+                        for (int q = 0; q < 4; ++q) st[q][j][i][v] = st[q][j][i][v] + c[q] * w[j][i][v];
+                    }
+        };
+        // stencil weights on (u++, u+, u, u-, u--), scale and the F̃ weight of a stage
+        auto stage = [&](const double* wts, double scale, double fw) {
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < n; ++i) {
+                    const double* J = &Ja[4 * nidx(i, j, e)];
+                    T sf[5][NV], sg[5][NV], sfc[5][NV], sgc[5][NV];
+                    const int q_of[5] = {2, 0, -1, 1, 3};
+                    for (int s = 0; s < 5; ++s) {
+                        if (wts[s] == 0.0) continue;
+                        if (q_of[s] < 0) {
+                            for (int v = 0; v < NV; ++v) {
+                                sf[s][v] = f[j][i][v];
+                                sg[s][v] = g[j][i][v];
+                                sfc[s][v] = fc[j][i][v];
+                                sgc[s][v] = gc[j][i][v];
+                            }
+                        } else {
+                            cv_fluxes(st[q_of[s]][j][i], J, sf[s], sg[s], sfc[s], sgc[s]);
+                        }
+                    }
+                    for (int v = 0; v < NV; ++v) {
+                        T a = T(0), b = T(0), ac = T(0), bc = T(0);
+                        for (int s = 0; s < 5; ++s) {
+                            if (wts[s] == 0.0) continue;
+                            a = a + wts[s] * sf[s][v];
+                            b = b + wts[s] * sg[s][v];
+                            ac = ac + wts[s] * sfc[s][v];
+                            bc = bc + wts[s] * sgc[s][v];
+                        }
+                        a = scale * a;
+                        b = scale * b;
+                        ac = scale * ac;
+                        bc = scale * bc;
+                        F[j][i][v] = F[j][i][v] + fw * a;
+                        G[j][i][v] = G[j][i][v] + fw * b;
+                        Fc[j][i][v] = Fc[j][i][v] + fw * ac;
+                        Gc[j][i][v] = Gc[j][i][v] + fw * bc;
+                        dfc[j][i][v] = ac;
+                        dgc[j][i][v] = bc;
+                    }
+                }
+        };
+        advance(fc, gc);  // ut
+        add_states(0.5, 1.0, -1.0, 2.0, -2.0);
+        {
+            const double wt[5] = {-1.0, 8.0, 0.0, -8.0, 1.0};  // f_t
+            stage(wt, 1.0 / 12.0, 0.5);
+        }
+        advance(dfc, dgc);  // utt
+        add_states(1.0 / 6.0, 0.5, 0.5, 2.0, 2.0);
+        if (N == 4) {
+            const double wt[5] = {-1.0, 16.0, -30.0, 16.0, -1.0};  // f_tt
+            stage(wt, 1.0 / 12.0, 1.0 / 6.0);
+        } else {
+            const double wt[5] = {0.0, 1.0, -2.0, 1.0, 0.0};
+            stage(wt, 1.0, 1.0 / 6.0);
+        }
+        advance(dfc, dgc);  // uttt
+        add_states(1.0 / 24.0, 1.0 / 6.0, -1.0 / 6.0, 4.0 / 3.0, -4.0 / 3.0);
+        {
+            const double wt[5] = {1.0, -2.0, 0.0, 2.0, -1.0};  // f_ttt
+            stage(wt, 0.5, 1.0 / 24.0);
+        }
+        if (N == 4) {
+            advance(dfc, dgc);  // utttt
+            add_states(1.0 / 120.0, 1.0 / 24.0, 1.0 / 24.0, 2.0 / 3.0, 2.0 / 3.0);
+            const double wt[5] = {1.0, -4.0, 6.0, -4.0, 1.0};  // f_tttt, with the 0.5 of dg_2d.jl:2018-2021
+            stage(wt, 0.5, 1.0 / 120.0);
+        }
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) {
+                for (int ii = 0; ii < n; ++ii)
+                    for (int v = 0; v < NV; ++v) du[idx(v, ii, j, e)] = du[idx(v, ii, j, e)] + Dh[ii][i] * Fc[j][i][v];
+                for (int jj = 0; jj < n; ++jj)
+                    for (int v = 0; v < NV; ++v) du[idx(v, i, jj, e)] = du[idx(v, i, jj, e)] + Dh[jj][j] * Gc[j][i][v];
+                for (int v = 0; v < NV; ++v) {
+                    this->U[idx(v, i, j, e)] = Uc[j][i][v];
+                    this->F[fidx(v, 0, i, j, e)] = F[j][i][v];
+                    this->F[fidx(v, 1, i, j, e)] = G[j][i][v];
+                }
+            }
+    }
+
+    // Normal of the face node shared by left and right elements: the right
+    // element's contravariant vector of the interface orientation (dg_2d.jl:2229-2264)
+    void face_normal(int right, int orientation, int l, double* nrm, double& sgn) const {
+        const size_t k = orientation == 1 ? nidx(0, l, right) : nidx(l, 0, right);
+        sgn = ijn[k] > 0 ? 1.0 : (ijn[k] < 0 ? -1.0 : 0.0);
+        const double* J = &Ja[4 * k];
+        nrm[0] = sgn * (orientation == 1 ? J[0] : J[2]);
+        nrm[1] = sgn * (orientation == 1 ? J[1] : J[3]);
+    }
+
+    // calc_interface_flux! for StructuredMesh (dg_2d.jl:2107-2290), plain FR:
+    // Fn = sign · [½(F̃_ll·n + F̃_rr·n) - ½ λ(uⁿ, n) (Ũ_rr - Ũ_ll)] (periodic mesh)
+    void calc_interface_flux_curved(const T* u) {
+#pragma omp parallel for schedule(static)
+        for (int e = 0; e < mesh.E; ++e)
+            for (int o = 1; o <= 2; ++o) {
+                const int left = mesh.nbr[4 * e + (o == 1 ? 0 : 2)];
+                if (left < 0) continue;
+                for (int l = 0; l < n; ++l) {
+                    int li, lj, ri, rj;
+                    if (o == 1) { li = n - 1; lj = l; ri = 0; rj = l; }
+                    else { li = l; lj = n - 1; ri = l; rj = 0; }
+                    double nrm[2], sgn;
+                    face_normal(e, o, l, nrm, sgn);
+                    T fll[NV], frr[NV];
+                    for (int v = 0; v < NV; ++v) {
+                        fll[v] = this->F[fidx(v, 0, li, lj, left)] * nrm[0] + this->F[fidx(v, 1, li, lj, left)] * nrm[1];
+                        frr[v] = this->F[fidx(v, 0, ri, rj, e)] * nrm[0] + this->F[fidx(v, 1, ri, rj, e)] * nrm[1];
+                    }
+                    const T* ull = &u[idx(0, li, lj, left)];
+                    const T* urr = &u[idx(0, ri, rj, e)];
+                    const T lam = eq.max_abs_speed_naive_normal(ull, urr, nrm);
+                    for (int v = 0; v < NV; ++v) {
+                        const T Fn = sgn * (0.5 * (fll[v] + frr[v]) -
+                                            0.5 * lam * (this->U[idx(v, ri, rj, e)] - this->U[idx(v, li, lj, left)]));
+                        this->sfv[sidx(v, l, 2 * o - 1, left)] = Fn;
+                        this->sfv[sidx(v, l, 2 * o - 2, e)] = Fn;
+                    }
+                }
+            }
+    }
+
+    // Trixi calc_surface_integral! (StructuredMesh uses the TreeMesh form) and
+    // apply_jacobian! with the node's J⁻¹
+    void surface_integral_and_jacobian_curved(T* du) {
+        const double f1 = basis.bnd_lo, f2 = basis.bnd_hi;
+#pragma omp parallel for schedule(static)
+        for (int e = 0; e < mesh.E; ++e) {
+            for (int l = 0; l < n; ++l)
+                for (int v = 0; v < NV; ++v) {
+                    du[idx(v, 0, l, e)] = du[idx(v, 0, l, e)] - this->sfv[sidx(v, l, 0, e)] * f1;
+                    du[idx(v, n - 1, l, e)] = du[idx(v, n - 1, l, e)] + this->sfv[sidx(v, l, 1, e)] * f2;
+                    du[idx(v, l, 0, e)] = du[idx(v, l, 0, e)] - this->sfv[sidx(v, l, 2, e)] * f1;
+                    du[idx(v, l, n - 1, e)] = du[idx(v, l, n - 1, e)] + this->sfv[sidx(v, l, 3, e)] * f2;
+                }
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < n; ++i) {
+                    const double factor = -ijn[nidx(i, j, e)];
+                    for (int v = 0; v < NV; ++v) du[idx(v, i, j, e)] = du[idx(v, i, j, e)] * factor;
+                }
+        }
+    }
+
+    void rhs_lw(T* du, const T* u, double t, double dt) override {
+        (void)t;
+        this->reset_du(du);
+#pragma omp parallel for schedule(static)
+        for (int e = 0; e < mesh.E; ++e) lw_volume_curved(du, u, dt, e);
+        calc_interface_flux_curved(u);
+        surface_integral_and_jacobian_curved(du);
+    }
+
+    // Trixi max_dt for curvilinear meshes: per node the speeds transformed to the
+    // reference element, |Ja^k · (λ1, λ2)| |J⁻¹|, element maxima summed,
+    // 2 / (nnodes · max)
+    double max_dt(const T* u) const override {
+        double mss = std::nextafter(0.0, 1.0);
+        for (int e = 0; e < mesh.E; ++e) {
+            double m1 = 0, m2 = 0;
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < n; ++i) {
+                    T l1, l2;
+                    eq.max_abs_speeds(&u[idx(0, i, j, e)], l1, l2);
+                    const double* J = &Ja[4 * nidx(i, j, e)];
+                    const double ij = std::fabs(ijn[nidx(i, j, e)]);
+                    m1 = max_(m1, std::fabs(J[0] * value(l1) + J[1] * value(l2)) * ij);
+                    m2 = max_(m2, std::fabs(J[2] * value(l1) + J[3] * value(l2)) * ij);
+                }
+            mss = max_(mss, m1 + m2);
+        }
+        return 2.0 / (n * mss);
+    }
+};
+
+}  // namespace tlwo

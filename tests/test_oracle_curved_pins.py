@@ -1,0 +1,101 @@
+"""Pins of the oracle's curvilinear StructuredMesh path (no GPU).
+
+An affine mapping must reproduce the TreeMesh solver run with the
+Structured/P4est argument order of the LW Rusanov flux (λ from uⁿ, jump on Ũ,
+src/solvers/dgsem_structured/dg_2d.jl:2271): every metric term, the contravariant
+fluxes, the normal scaling of the interface flux and the per-node Jacobian
+then reduce to the Cartesian ones.  On a warped periodic mesh the discrete
+metric identities give free-stream preservation, the flux form gives
+conservation (Σ w_i w_j J du = 0), and the isentropic vortex converges at high
+order.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_linf
+
+
+def vortex(L=20.0):
+    from trixilw_hip.equations import IsentropicVortex
+
+    return IsentropicVortex(center=(L / 2, L / 2), period=(L, L))
+
+
+@pytest.mark.parametrize("N", [3, 4])
+def test_affine_mapping_equals_treemesh_structured_order(oracle_mod, N):
+    nx, ny, L = 6, 4, 20.0
+    h = L / nx
+    Ly = h * ny
+    ic = vortex(L)
+    tree = oracle_mod.Oracle(N, nx, ny, 0.0, 0.0, h, flux_order=1)
+    curv = oracle_mod.Oracle(N, nx, ny, 0.0, 0.0, h, mapping=(oracle_mod.MAP_AFFINE, (0.0, 0.0, L, Ly)))
+    xt, yt = tree.node_coords()
+    xc, yc = curv.node_coords()
+    assert np.max(np.abs(xt - xc)) < 1e-13 and np.max(np.abs(yt - yc)) < 1e-13
+    rng = np.random.default_rng(21 + N)
+    u = ic(xt, yt, 0.0) * (1 + 1e-2 * rng.standard_normal(xt.shape + (4,)))
+    assert abs(tree.max_dt(u) - curv.max_dt(u)) < 1e-14 * tree.max_dt(u)
+    dt = 0.8 * tree.max_dt(u)
+    # the metric terms carry round-off of D applied to the coordinates (~1e-15 relative)
+    assert rel_linf(curv.rhs(u, 0.0, dt), tree.rhs(u, 0.0, dt)) < 1e-12
+    # the metric terms are the constant scalings of the box
+    Ja = curv.aux("contravariant_vectors").reshape(-1, 4)
+    assert np.allclose(Ja[:, 0], Ly / ny / 2, rtol=1e-13, atol=0) and np.allclose(Ja[:, 3], h / 2, rtol=1e-13, atol=0)
+    assert np.max(np.abs(Ja[:, 1:3])) < 1e-12
+
+
+@pytest.mark.parametrize("N", [3, 4])
+def test_affine_mapping_linear_advection(oracle_mod, N):
+    nx, L = 4, 2.0
+    tree = oracle_mod.Oracle(N, nx, nx, 0.0, 0.0, L / nx, equation="linadv", adv=(1.0, 0.6), flux_order=1)
+    curv = oracle_mod.Oracle(N, nx, nx, 0.0, 0.0, L / nx, equation="linadv", adv=(1.0, 0.6),
+                             mapping=(oracle_mod.MAP_AFFINE, (0.0, 0.0, L, L)))
+    u = np.random.default_rng(3).standard_normal((nx * nx, N + 1, N + 1, 1))
+    dt = 0.05
+    assert rel_linf(curv.rhs(u, 0.0, dt), tree.rhs(u, 0.0, dt)) < 1e-13
+
+
+def warped(oracle_mod, N, nx, L=20.0, A=0.05, **kw):
+    return oracle_mod.Oracle(N, nx, nx, 0.0, 0.0, L / nx, mapping=(oracle_mod.MAP_WARP, (L, A)), **kw)
+
+
+@pytest.mark.parametrize("N", [3, 4])
+def test_warped_mesh_metrics_and_free_stream(oracle_mod, N):
+    o = warped(oracle_mod, N, 6)
+    Ja = o.aux("contravariant_vectors").reshape(-1, 4)
+    assert np.max(np.abs(Ja[:, 1])) > 1e-2  # the mesh is really curved
+    assert np.all(o.aux("inverse_jacobian") > 0)
+    x, _ = o.node_coords()
+    u = np.zeros(x.shape + (4,))
+    u[...] = [1.1, 0.4, -0.3, 2.7]
+    du = o.rhs(u, 0.0, 0.5 * o.max_dt(u))
+    assert np.max(np.abs(du)) < 1e-12
+
+
+@pytest.mark.parametrize("N", [3, 4])
+def test_warped_mesh_conservation(oracle_mod, N):
+    o = warped(oracle_mod, N, 6)
+    x, y = o.node_coords()
+    rng = np.random.default_rng(8)
+    u = vortex()(x, y, 0.0) * (1 + 1e-3 * rng.standard_normal(x.shape + (4,)))
+    du = o.rhs(u, 0.0, 0.5 * o.max_dt(u))
+    _, w, _, _ = o.basis()
+    W = np.outer(w, w)[None] / o.aux("inverse_jacobian").reshape(x.shape)
+    total = np.einsum("eji,ejiv->v", W, du)
+    scale = np.einsum("eji,ejiv->v", W, np.abs(du))
+    assert np.all(np.abs(total) <= 1e-13 * scale)
+
+
+@pytest.mark.parametrize("N,eoc_min", [(3, 3.3), (4, 4.0)])
+def test_warped_mesh_vortex_convergence(oracle_mod, N, eoc_min):
+    ic = vortex()
+    errs = []
+    for nx in (8, 16, 32):
+        o = warped(oracle_mod, N, nx)
+        x, y = o.node_coords()
+        cfl = 0.9 * (N + 1) * {3: 0.170, 4: 0.103}[N]
+        u, t, _ = o.solve(ic(x, y, 0.0), 0.0, 1.0, cfl)
+        assert abs(t - 1.0) < 1e-12
+        errs.append(np.sqrt(np.mean((u - ic(x, y, t))[..., 0] ** 2)))
+    eoc = np.log(errs[1] / errs[2]) / np.log(2)
+    assert eoc > eoc_min, (errs, eoc)
