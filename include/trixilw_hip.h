@@ -110,6 +110,15 @@ typedef struct {
     double prandtl;              /* Prandtl number; κ = γ / ((γ-1) Pr) */
     int32_t gradient_variables;  /* TLW_GRAD_CONSERVATIVE */
 } tlw_parabolic_desc;
+/* Curvilinear StructuredMesh metrics (Trixi cache.elements of a StructuredMesh,
+ * src/solvers/dgsem_structured/containers_2d.jl; TrixiLW's kernels for it in
+ * src/solvers/dgsem_structured/dg_2d.jl).  Layouts are Julia's column-major arrays:
+ * contravariant_vectors[m, l, i, j, element] = component m of Ja^l at node (i, j), so the four
+ * entries of a node (Ja11, Ja12, Ja21, Ja22) are contiguous; inverse_jacobian[i, j, element]. */
+typedef struct {
+    const double* contravariant_vectors_host; /* [2*2*n*n*E] */
+    const double* inverse_jacobian_host;      /* [n*n*E], every entry > 0 */
+} tlw_curved_desc;
 typedef struct tlw_ctx tlw_ctx;
 
 /* SemidiscretizationHyperbolic(mesh, time_disc, equations, ic, solver) + cache
@@ -138,6 +147,20 @@ int tlw_create_batch(int32_t n_instances, const tlw_mesh_desc* meshes, const tlw
  * TLW_AUX_F holds the total F̃a - F̃v. */
 int tlw_create_parabolic(const tlw_mesh_desc* mesh, const tlw_basis_desc* basis, const tlw_solver_desc* solver,
                          const tlw_parabolic_desc* parabolic, tlw_ctx** out);
+/* SemidiscretizationHyperbolic on a curvilinear StructuredMesh: rhs! of
+ * src/solvers/dgsem_structured/dg_2d.jl:17-53 with lw_volume_kernel_3!/4! (dg_2d.jl:1379-2105:
+ * the Taylor procedure on the contravariant fluxes Ja^l · (f, g) and the node's inverse Jacobian)
+ * and calc_interface_flux! (dg_2d.jl:2194-2290: the normal of the right element's face node,
+ * F* = ½(F̃_L·n + F̃_R·n) - ½ λ(uⁿ, n)(Ũ_R - Ũ_L) with λ = max|v·n| + max c |n|); tlw_max_dt and
+ * the device loop use Trixi's curvilinear max_dt (per node |Ja^l · (λ1, λ2)| |J⁻¹|).
+ * mesh->neighbors_host gives the face neighbours (a periodic mesh: no -1 entries);
+ * mesh->inverse_jacobian_host, element_center_host and element_length_host are not read and
+ * may be NULL.  The structured flux order applies whatever solver->surface_flux_order says.
+ * Plain VolumeIntegralFR with TLW_LW; shock capturing, boundaries, MDRK return TLW_ENOTSUP, and
+ * so do the positivity limiter, adaptive steps, tlw_adjoint_lw and tlw_rebuild_mesh on such a
+ * context.  A non-positive inverse Jacobian returns TLW_EINVAL. */
+int tlw_create_curved(const tlw_mesh_desc* mesh, const tlw_basis_desc* basis, const tlw_solver_desc* solver,
+                      const tlw_curved_desc* metrics, tlw_ctx** out);
 int tlw_n_instances(const tlw_ctx* ctx, int32_t* n_instances);
 int tlw_batch_status(tlw_ctx* ctx, int32_t n_instances, double* t_host, double* dt_host, int64_t* iter_host,
                      int32_t* finished_host);

@@ -55,6 +55,12 @@ struct tlw_ctx {
     int vgrid = 0, sgrid = 0, vcap = 0;
     bool sc = false;
     bool parabolic = false;   // compressible Navier-Stokes (tlw_create_parabolic)
+    // curvilinear StructuredMesh (tlw_create_curved): per node Ja [E][n²][4] and
+    // J⁻¹ [E][n²], per face node the interface normal (n1, n2, |n|, 0) [E][4][n][4]
+    bool curved = false;
+    double* d_Ja = nullptr;
+    double* d_ijn = nullptr;
+    double* d_fnorm = nullptr;
     int pv_off = 0;           // offset of the viscous CFL partials (0: none)
     int ebv = 0;              // elements per volume workgroup
     bool positivity = false;  // Zhang-Shu limiter in the device loop
@@ -121,7 +127,8 @@ void free_mesh(tlw_ctx* c) {
     drop_graphs(c);
     void* ptrs[] = {c->d_nbr,       c->d_invj,  c->d_center, c->d_length, c->d_trace,   c->d_buf[0],    c->d_buf[1],
                     c->d_us,        c->d_auxU,  c->d_auxF,   c->d_partial, c->d_alpha_tmp, c->d_alpha,
-                    c->d_vblk,      c->d_sblk,  c->d_etile,  c->d_rng,    c->d_state,   c->d_cur};
+                    c->d_vblk,      c->d_sblk,  c->d_etile,  c->d_rng,    c->d_state,   c->d_cur,
+                    c->d_Ja,        c->d_ijn,   c->d_fnorm};
     for (void* p : ptrs) (void)hipFree(p);
     if (c->h_state) (void)hipHostFree(c->h_state);
     c->d_nbr = nullptr;
@@ -133,6 +140,7 @@ void free_mesh(tlw_ctx* c) {
     c->d_state = nullptr;
     c->h_state = nullptr;
     c->d_cur = nullptr;
+    c->d_Ja = c->d_ijn = c->d_fnorm = nullptr;
 }
 
 // Workgroup tables for the current tile: volume workgroups of ebv elements and
@@ -173,9 +181,10 @@ int build_tables(tlw_ctx* c) {
 }
 
 int validate_mesh(tlw_ctx* c, const tlw_mesh_desc* m) {
-    if (!m || m->n_elements <= 0 || !m->neighbors_host || !m->inverse_jacobian_host || !m->element_center_host ||
-        !m->element_length_host)
-        return fail(c, TLW_EINVAL, "mesh description: missing arrays or n_elements <= 0");
+    if (!m || m->n_elements <= 0 || !m->neighbors_host)
+        return fail(c, TLW_EINVAL, "mesh description: missing neighbour table or n_elements <= 0");
+    if (!c->curved && (!m->inverse_jacobian_host || !m->element_center_host || !m->element_length_host))
+        return fail(c, TLW_EINVAL, "mesh description: missing arrays");
     const int E = m->n_elements;
     for (size_t k = 0; k < (size_t)4 * E; ++k) {
         const int nb = m->neighbors_host[k];
@@ -188,16 +197,54 @@ int validate_mesh(tlw_ctx* c, const tlw_mesh_desc* m) {
             if (nb >= 0 && m->neighbors_host[4 * nb + (f ^ 1)] != e)
                 return fail(c, TLW_EINVAL, "mesh description: neighbour table is not symmetric (non-conforming mesh)");
         }
+    if (c->curved) return TLW_OK;  // metrics are checked with the curved description
     for (int e = 0; e < E; ++e)
         if (!(m->inverse_jacobian_host[e] > 0.0) || !(m->element_length_host[e] > 0.0))
             return fail(c, TLW_EINVAL, "mesh description: non-positive Jacobian or element length");
     return TLW_OK;
 }
 
+// Curvilinear metrics: finite contravariant vectors, positive inverse Jacobians
+// (the normal's sign factor of calc_interface_flux! is then 1), and the
+// interface normals of every face node, taken from the right element's face
+// node so that both elements of an interface use the same bits.
+int prepare_curved(tlw_ctx* c, const tlw_mesh_desc* m, const tlw_curved_desc* cv, std::vector<double>& fnorm,
+                   std::vector<double>& invj) {
+    if (!cv || !cv->contravariant_vectors_host || !cv->inverse_jacobian_host)
+        return fail(c, TLW_EINVAL, "curved description: missing arrays");
+    const int E = m->n_elements, n = c->n, NN = n * n;
+    const double* Ja = cv->contravariant_vectors_host;
+    const double* ijn = cv->inverse_jacobian_host;
+    for (size_t k = 0; k < (size_t)4 * NN * E; ++k)
+        if (!std::isfinite(Ja[k])) return fail(c, TLW_EINVAL, "curved description: non-finite contravariant vector");
+    for (size_t k = 0; k < (size_t)NN * E; ++k)
+        if (!(ijn[k] > 0.0) || !std::isfinite(ijn[k]))
+            return fail(c, TLW_EINVAL, "curved description: inverse Jacobian must be finite and > 0");
+    fnorm.assign((size_t)E * 4 * n * 4, 0.0);
+    invj.resize(E);
+    for (int e = 0; e < E; ++e) {
+        invj[e] = ijn[(size_t)e * NN];
+        for (int f = 0; f < 4; ++f) {
+            const int right = (f & 1) ? m->neighbors_host[4 * e + f] : e;
+            for (int l = 0; l < n; ++l) {
+                // x faces: Ja¹ at the right element's node (1, l); y faces: Ja² at (l, 1)
+                const size_t node = (size_t)right * NN + (f < 2 ? (size_t)n * l : (size_t)l);
+                const double n1 = Ja[4 * node + (f < 2 ? 0 : 2)], n2 = Ja[4 * node + (f < 2 ? 1 : 3)];
+                double* o = &fnorm[(((size_t)e * 4 + f) * n + l) * 4];
+                o[0] = n1;
+                o[1] = n2;
+                o[2] = std::sqrt(n1 * n1 + n2 * n2);
+            }
+        }
+    }
+    return TLW_OK;
+}
+
 // Uploads ninst meshes as one element range each (element ids offset by the
 // instance's first element) and allocates everything that depends on E.
-int upload_meshes(tlw_ctx* c, const tlw_mesh_desc* ms, int ninst) {
+int upload_meshes(tlw_ctx* c, const tlw_mesh_desc* ms, int ninst, const tlw_curved_desc* cv = nullptr) {
     if (ninst < 1) return fail(c, TLW_EINVAL, "at least one instance");
+    if (c->curved && ninst != 1) return fail(c, TLW_ENOTSUP, "a curvilinear mesh is a single instance");
     long long total = 0;
     for (int k = 0; k < ninst; ++k) {
         int rc = validate_mesh(c, &ms[k]);
@@ -209,10 +256,16 @@ int upload_meshes(tlw_ctx* c, const tlw_mesh_desc* ms, int ninst) {
     for (int k = 0; k < ninst; ++k)
         for (size_t q = 0; q < (size_t)4 * ms[k].n_elements; ++q)
             if (ms[k].neighbors_host[q] < 0) boundary = true;
+    if (boundary && c->curved) return fail(c, TLW_ENOTSUP, "the curvilinear path needs a periodic mesh");
     if (boundary && c->hconst.bc_kind == TLW_BV_NONE)
         return fail(c, TLW_EINVAL, "mesh has boundary faces but no boundary value function was given");
     if (boundary && c->parabolic)
         return fail(c, TLW_ENOTSUP, "the hyperbolic-parabolic path needs a periodic mesh");
+    std::vector<double> fnorm, cinvj;
+    if (c->curved) {
+        int rc = prepare_curved(c, &ms[0], cv, fnorm, cinvj);
+        if (rc) return rc;
+    }
     free_mesh(c);
     c->aux_valid = c->alpha_valid = false;
     c->ninst = ninst;
@@ -229,6 +282,10 @@ int upload_meshes(tlw_ctx* c, const tlw_mesh_desc* ms, int ninst) {
         for (size_t q = 0; q < (size_t)4 * Ek; ++q) {
             const int nb = ms[k].neighbors_host[q];
             nbr[4 * (size_t)f0 + q] = nb < 0 ? -1 : nb + f0;
+        }
+        if (c->curved) {  // element data of a TreeMesh: unused, J⁻¹ of each element's first node
+            std::memcpy(&invj[f0], cinvj.data(), sizeof(double) * Ek);
+            continue;
         }
         std::memcpy(&invj[f0], ms[k].inverse_jacobian_host, sizeof(double) * Ek);
         std::memcpy(&center[2 * (size_t)f0], ms[k].element_center_host, sizeof(double) * 2 * Ek);
@@ -266,7 +323,7 @@ int upload_meshes(tlw_ctx* c, const tlw_mesh_desc* ms, int ninst) {
     HIPCHK(c, hipHostMalloc(&c->h_state, sizeof(DevState) * ninst, hipHostMallocDefault));
     std::memset(c->h_state, 0, sizeof(DevState) * ninst);
     for (int k = 0; k < ninst; ++k) {
-        c->h_state[k].invJ0 = ms[k].inverse_jacobian_host[0];
+        c->h_state[k].invJ0 = invj[c->inst_first[k]];
         c->h_state[k].ndofs = (double)ms[k].n_elements * c->n * c->n;  // Trixi.ndofs
     }
     HIPCHK(c, hipMemcpy(c->d_state, c->h_state, sizeof(DevState) * ninst, hipMemcpyHostToDevice));
@@ -274,6 +331,16 @@ int upload_meshes(tlw_ctx* c, const tlw_mesh_desc* ms, int ninst) {
     HIPCHK(c, hipMemcpy(c->d_invj, invj.data(), sizeof(double) * E, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_center, center.data(), sizeof(double) * 2 * E, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_length, length.data(), sizeof(double) * E, hipMemcpyHostToDevice));
+    if (c->curved) {
+        const size_t NN = (size_t)c->n * c->n;
+        HIPCHK(c, hipMalloc(&c->d_Ja, sizeof(double) * 4 * NN * E));
+        HIPCHK(c, hipMalloc(&c->d_ijn, sizeof(double) * NN * E));
+        HIPCHK(c, hipMalloc(&c->d_fnorm, sizeof(double) * fnorm.size()));
+        HIPCHK(c, hipMemcpy(c->d_Ja, cv->contravariant_vectors_host, sizeof(double) * 4 * NN * E,
+                            hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(c->d_ijn, cv->inverse_jacobian_host, sizeof(double) * NN * E, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(c->d_fnorm, fnorm.data(), sizeof(double) * fnorm.size(), hipMemcpyHostToDevice));
+    }
     return build_tables(c);
 }
 
@@ -299,6 +366,9 @@ KArgs base_args(tlw_ctx* c) {
     a.alpha_tmp = c->d_alpha_tmp;
     a.alpha_out = c->d_alpha;
     a.bc_scale = 1.0;
+    a.Ja = c->d_Ja;
+    a.ijn = c->d_ijn;
+    a.fnorm = c->d_fnorm;
     return a;
 }
 
@@ -320,8 +390,17 @@ template <int N, bool SC, bool PAR = false> void launch_volume_sc(tlw_ctx* c, co
     else if (c->ebv == ch[1]) launch_volume_eb<N, ch[1], SC, PAR>(c, a, kind);
     else launch_volume_eb<N, ch[2], SC, PAR>(c, a, kind);
 }
+template <int N, int EB> void launch_volume_curved_eb(tlw_ctx* c, const KArgs& a) {
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_LW, false, false, true>), dim3(c->vgrid),
+                       dim3(VGeo<N, EB>::NT), 0, c->stream, a);
+}
 template <int N> void launch_volume_t(tlw_ctx* c, const KArgs& a, int kind) {
-    if (c->parabolic) {
+    if (c->curved) {  // plain LW only (checked at creation)
+        constexpr const int* ch = N == 4 ? kEbChoices4 : kEbChoices3;
+        if (c->ebv == ch[0]) launch_volume_curved_eb<N, ch[0]>(c, a);
+        else if (c->ebv == ch[1]) launch_volume_curved_eb<N, ch[1]>(c, a);
+        else launch_volume_curved_eb<N, ch[2]>(c, a);
+    } else if (c->parabolic) {
         launch_volume_sc<N, false, true>(c, a, kind);
     } else if (c->sc) {
         // IndicatorHennemannGassner on uⁿ before the volume integral (dg_2d.jl:1665)
@@ -345,7 +424,9 @@ void launch_finalize(tlw_ctx* c, int reduce, int use_vol) {
 }
 template <int N> void launch_surface_t(tlw_ctx* c, const KArgs& a) {
     dim3 g(c->sgrid), b(kSurfBlock);
-    if (c->sc) {
+    if (c->curved) {
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_surface<N, false, false, true>), g, b, 0, c->stream, a);
+    } else if (c->sc) {
         if (c->has_boundary) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_surface<N, true, true>), g, b, 0, c->stream, a);
         else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_surface<N, false, true>), g, b, 0, c->stream, a);
     } else {
@@ -539,7 +620,7 @@ bool any_bad(const tlw_ctx* c) {
 }
 
 int create_common(int ninst, const tlw_mesh_desc* meshes, const tlw_basis_desc* basis, const tlw_solver_desc* solver,
-                  tlw_ctx** out, const tlw_parabolic_desc* par = nullptr);
+                  tlw_ctx** out, const tlw_parabolic_desc* par = nullptr, const tlw_curved_desc* cv = nullptr);
 
 }  // namespace
 
@@ -557,6 +638,15 @@ int tlw_create_parabolic(const tlw_mesh_desc* mesh, const tlw_basis_desc* basis,
         return TLW_EINVAL;
     }
     return create_common(1, mesh, basis, solver, out, parabolic);
+}
+
+int tlw_create_curved(const tlw_mesh_desc* mesh, const tlw_basis_desc* basis, const tlw_solver_desc* solver,
+                      const tlw_curved_desc* metrics, tlw_ctx** out) {
+    if (!metrics) {
+        if (out) *out = nullptr;
+        return TLW_EINVAL;
+    }
+    return create_common(1, mesh, basis, solver, out, nullptr, metrics);
 }
 
 int tlw_create_batch(int32_t n_instances, const tlw_mesh_desc* meshes, const tlw_basis_desc* basis,
@@ -577,6 +667,7 @@ int tlw_n_instances(const tlw_ctx* c, int32_t* n) {
 int tlw_rebuild_mesh(tlw_ctx* c, const tlw_mesh_desc* mesh) {
     if (!c) return TLW_EINVAL;
     if (c->ninst != 1) return fail(c, TLW_EINVAL, "tlw_rebuild_mesh needs a single-instance context");
+    if (c->curved) return fail(c, TLW_ENOTSUP, "a curvilinear context keeps its mesh (create a new one)");
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return upload_meshes(c, mesh, 1);
 }
@@ -703,7 +794,7 @@ int tlw_solve_init_adaptive(tlw_ctx* c, double t0, double t_final, double dt_ini
     if (!(abstol >= 0.0) || !(reltol >= 0.0) || !(abstol > 0.0 || reltol > 0.0) || !std::isfinite(abstol) ||
         !std::isfinite(reltol))
         return fail(c, TLW_EINVAL, "tolerances: abstol, reltol >= 0, not both zero");
-    if (c->N != 4 || c->time_disc != TLW_LW || c->parabolic)
+    if (c->N != 4 || c->time_disc != TLW_LW || c->parabolic || c->curved)
         return fail(c, TLW_ENOTSUP,
                     "adaptive steps need the temporal-error estimate of the Euler LW kernel for polydeg 4");
     if (c->positivity) return fail(c, TLW_ENOTSUP, "adaptive steps are not combined with the positivity limiter");
@@ -807,8 +898,8 @@ int tlw_adjoint_lw(tlw_ctx* c, const double* u, const double* lam_next, double* 
     if (!c) return TLW_EINVAL;
     if (!u || !lam_next || !lam) return fail(c, TLW_EINVAL, "null state pointer");
     if (!std::isfinite(dt) || !std::isfinite(t)) return fail(c, TLW_EINVAL, "t and dt must be finite");
-    if (c->time_disc != TLW_LW || c->sc || c->parabolic || c->has_boundary || c->hconst.flux_order != TLW_FLUX_TREE ||
-        c->ninst != 1)
+    if (c->time_disc != TLW_LW || c->sc || c->parabolic || c->curved || c->has_boundary ||
+        c->hconst.flux_order != TLW_FLUX_TREE || c->ninst != 1)
         return fail(c, TLW_ENOTSUP, "the adjoint covers plain LW on one periodic mesh with the TreeMesh flux order");
     if (lam == lam_next || lam == u) return fail(c, TLW_EINVAL, "lam must not alias lam_next or u");
     KArgs a = base_args(c);
@@ -837,6 +928,8 @@ int tlw_set_positivity_limiter(tlw_ctx* c, int32_t enable, double threshold_dens
         return fail(c, TLW_EINVAL, "positivity thresholds must not be NaN");
     if (enable && c->parabolic)
         return fail(c, TLW_ENOTSUP, "the positivity limiter is not combined with the hyperbolic-parabolic path");
+    if (enable && c->curved)
+        return fail(c, TLW_ENOTSUP, "the positivity limiter is not combined with the curvilinear path");
     if (enable && c->adaptive)
         return fail(c, TLW_ENOTSUP, "the positivity limiter is not combined with adaptive steps (tlw_solve_init first)");
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -965,7 +1058,7 @@ int tlw_get_aux(tlw_ctx* c, int32_t which, double* out, size_t len) {
 namespace {
 
 int create_common(int ninst, const tlw_mesh_desc* meshes, const tlw_basis_desc* basis, const tlw_solver_desc* solver,
-                  tlw_ctx** out, const tlw_parabolic_desc* par) {
+                  tlw_ctx** out, const tlw_parabolic_desc* par, const tlw_curved_desc* cv) {
     if (!out) return TLW_EINVAL;
     *out = nullptr;
     if (!meshes || !basis || !solver) return TLW_EINVAL;
@@ -1027,13 +1120,22 @@ int create_common(int ninst, const tlw_mesh_desc* meshes, const tlw_basis_desc* 
         h.visc_Pr = par->prandtl;
         h.visc_kappa = solver->gamma * (1.0 / (solver->gamma - 1.0)) / par->prandtl;
     }
+    if (cv) {
+        // StructuredMesh: plain FR, LW, periodic (boundaries are rejected at upload)
+        if (c->sc || c->time_disc != TLW_LW || c->parabolic || ninst != 1) {
+            delete c;
+            return TLW_ENOTSUP;
+        }
+        c->curved = true;
+        h.flux_order = TLW_FLUX_STRUCTURED;
+    }
     auto bail = [&](int rc) {
         tlw_destroy(c);
         return rc;
     };
     if (hipMalloc(&c->d_const, sizeof(DevConst)) != hipSuccess) return bail(TLW_EHIP);
     if (hipMemcpy(c->d_const, &h, sizeof(DevConst), hipMemcpyHostToDevice) != hipSuccess) return bail(TLW_EHIP);
-    int rc = upload_meshes(c, meshes, ninst);
+    int rc = upload_meshes(c, meshes, ninst, cv);
     if (rc) return bail(rc);
     *out = c;
     return TLW_OK;

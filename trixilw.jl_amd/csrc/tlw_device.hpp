@@ -134,6 +134,12 @@ struct KArgs {
     // viscous bound J⁻² max(4μ/(3ρ), μκ/(ρ Pr)) sit pv_off entries after the
     // advective ones (0: no parabolic terms)
     int pv_off;
+    // curvilinear StructuredMesh (null on a TreeMesh): per node the contravariant
+    // vectors (Ja11, Ja12, Ja21, Ja22) [E][n²][4] and inverse Jacobian [E][n²],
+    // per face node the shared face normal (n1, n2, |n|, 0) [E][4][n][4]
+    const double* Ja;
+    const double* ijn;
+    const double* fnorm;
     double t_arg, dt_arg;
     int use_state_dt;  // 1: t, dt from DevState (device loop); 0: from t_arg/dt_arg
     int step_mode;     // 1: out = u + dt * rhs; 0: out = rhs
@@ -1275,6 +1281,211 @@ __device__ __forceinline__ void mdrkp2_element(const double* uN, const double* g
 }
 
 // ---------------------------------------------------------------------------
+// Curvilinear StructuredMesh (lw_volume_kernel_3!/4! for StructuredMesh,
+// src/solvers/dgsem_structured/dg_2d.jl:1379-2105): the LW procedure with the
+// node's contravariant vectors J = (Ja11, Ja12, Ja21, Ja22) and inverse
+// Jacobian.  The flux time derivatives are formed for the Cartesian fluxes
+// (their time averages F̃, G̃ feed the face records, F̃·n) and turned into
+// contravariant ones, Ja¹·(f, g) and Ja²·(f, g), for the derivative
+// contractions; the volume term is D̂ of the contravariant time average.
+// f(uⁿ) is kept in registers (f0x, f0y) for the centred stencils.
+__device__ __forceinline__ void to_contravariant(const double* J, const double* f, const double* g, double* fc,
+                                                 double* gc) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        fc[v] = fma(J[0], f[v], J[1] * g[v]);
+        gc[v] = fma(J[2], f[v], J[3] * g[v]);
+    }
+}
+
+template <int N, int EB>
+__device__ __forceinline__ void lwc_element(const double* uN, const double* J, double invJ, double gm1, int i, int j,
+                                            bool act, int sidx, int sbase, double* s0, double* s1, double* s2,
+                                            const double* sDm, const double* sDh, double* Uc, double* F, double* G,
+                                            double* vol) {
+    using G_ = VGeo<N, EB>;
+    constexpr int n = G_::n, S = G_::S;
+    double f0x[4], f0y[4], cx[4], cy[4];
+    euler_fluxes(uN, gm1, f0x, f0y);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) { F[v] = f0x[v]; G[v] = f0y[v]; }
+    to_contravariant(J, f0x, f0y, cx, cy);
+    if (act) put8<S>(s0, sidx, cx, cy);
+    __syncthreads();
+    double w[4];
+    contract<n, S>(s0, sbase, i, j, sDm, w);
+    double up[4], um[4], upp[4], umm[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        w[v] *= invJ;  // ut
+        Uc[v] = fma(0.5, w[v], uN[v]);
+        up[v] = uN[v] + w[v];
+        um[v] = uN[v] - w[v];
+        upp[v] = fma(2.0, w[v], uN[v]);
+        umm[v] = fma(-2.0, w[v], uN[v]);
+    }
+    double fa[4], ga[4], dx[4], dy[4];
+    // stage 1: f_t = (-f(u++) + 8 f(u+) - 8 f(u-) + f(u--)) / 12
+    euler_fluxes(upp, gm1, fa, ga);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) { dx[v] = -fa[v]; dy[v] = -ga[v]; }
+    euler_fluxes(up, gm1, fa, ga);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) { dx[v] = fma(8.0, fa[v], dx[v]); dy[v] = fma(8.0, ga[v], dy[v]); }
+    euler_fluxes(um, gm1, fa, ga);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) { dx[v] = fma(-8.0, fa[v], dx[v]); dy[v] = fma(-8.0, ga[v], dy[v]); }
+    euler_fluxes(umm, gm1, fa, ga);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        dx[v] = (1.0 / 12.0) * (dx[v] + fa[v]);
+        dy[v] = (1.0 / 12.0) * (dy[v] + ga[v]);
+        F[v] = fma(0.5, dx[v], F[v]);
+        G[v] = fma(0.5, dy[v], G[v]);
+    }
+    to_contravariant(J, dx, dy, cx, cy);
+    if (act) put8<S>(s1, sidx, cx, cy);
+    __syncthreads();
+    contract<n, S>(s1, sbase, i, j, sDm, w);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        w[v] *= invJ;  // utt
+        Uc[v] = fma(1.0 / 6.0, w[v], Uc[v]);
+        up[v] = fma(0.5, w[v], up[v]);
+        um[v] = fma(0.5, w[v], um[v]);
+        upp[v] = fma(2.0, w[v], upp[v]);
+        umm[v] = fma(2.0, w[v], umm[v]);
+    }
+    if constexpr (N == 4) {
+        // stage 2: f_tt = (-f(u++) + 16 f(u+) - 30 f + 16 f(u-) - f(u--)) / 12
+        euler_fluxes(upp, gm1, fa, ga);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) { dx[v] = -fa[v]; dy[v] = -ga[v]; }
+        euler_fluxes(up, gm1, fa, ga);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            dx[v] = fma(16.0, fa[v], dx[v]);
+            dy[v] = fma(16.0, ga[v], dy[v]);
+            dx[v] = fma(-30.0, f0x[v], dx[v]);
+            dy[v] = fma(-30.0, f0y[v], dy[v]);
+        }
+        euler_fluxes(um, gm1, fa, ga);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) { dx[v] = fma(16.0, fa[v], dx[v]); dy[v] = fma(16.0, ga[v], dy[v]); }
+        euler_fluxes(umm, gm1, fa, ga);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            dx[v] = (1.0 / 12.0) * (dx[v] - fa[v]);
+            dy[v] = (1.0 / 12.0) * (dy[v] - ga[v]);
+        }
+    } else {
+        // stage 2: f_tt = f(u+) - 2 f + f(u-)
+        euler_fluxes(up, gm1, fa, ga);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            dx[v] = fma(-2.0, f0x[v], fa[v]);
+            dy[v] = fma(-2.0, f0y[v], ga[v]);
+        }
+        euler_fluxes(um, gm1, fa, ga);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) { dx[v] += fa[v]; dy[v] += ga[v]; }
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        F[v] = fma(1.0 / 6.0, dx[v], F[v]);
+        G[v] = fma(1.0 / 6.0, dy[v], G[v]);
+    }
+    to_contravariant(J, dx, dy, cx, cy);
+    if (act) put8<S>(s2, sidx, cx, cy);
+    __syncthreads();
+    contract<n, S>(s2, sbase, i, j, sDm, w);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        w[v] *= invJ;  // uttt
+        Uc[v] = fma(1.0 / 24.0, w[v], Uc[v]);
+        um[v] = fma(-1.0 / 6.0, w[v], um[v]);
+        up[v] = fma(1.0 / 6.0, w[v], up[v]);
+        umm[v] = fma(-4.0 / 3.0, w[v], umm[v]);
+        upp[v] = fma(4.0 / 3.0, w[v], upp[v]);
+    }
+    // stage 3: f_ttt = (f(u++) - 2 f(u+) + 2 f(u-) - f(u--)) / 2
+    euler_fluxes(upp, gm1, fa, ga);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) { dx[v] = fa[v]; dy[v] = ga[v]; }
+    euler_fluxes(up, gm1, fa, ga);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) { dx[v] = fma(-2.0, fa[v], dx[v]); dy[v] = fma(-2.0, ga[v], dy[v]); }
+    euler_fluxes(um, gm1, fa, ga);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) { dx[v] = fma(2.0, fa[v], dx[v]); dy[v] = fma(2.0, ga[v], dy[v]); }
+    euler_fluxes(umm, gm1, fa, ga);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        dx[v] = 0.5 * (dx[v] - fa[v]);
+        dy[v] = 0.5 * (dy[v] - ga[v]);
+        F[v] = fma(1.0 / 24.0, dx[v], F[v]);
+        G[v] = fma(1.0 / 24.0, dy[v], G[v]);
+    }
+    if constexpr (N == 4) {
+        to_contravariant(J, dx, dy, cx, cy);
+        if (act) put8<S>(s1, sidx, cx, cy);
+        __syncthreads();
+        contract<n, S>(s1, sbase, i, j, sDm, w);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            w[v] *= invJ;  // utttt
+            Uc[v] = fma(1.0 / 120.0, w[v], Uc[v]);
+            um[v] = fma(1.0 / 24.0, w[v], um[v]);
+            up[v] = fma(1.0 / 24.0, w[v], up[v]);
+            umm[v] = fma(2.0 / 3.0, w[v], umm[v]);
+            upp[v] = fma(2.0 / 3.0, w[v], upp[v]);
+        }
+        // stage 4: f_tttt = 0.5 (f(u++) - 4 f(u+) + 6 f - 4 f(u-) + f(u--)), with the
+        // reference's factor 0.5 (dg_2d.jl:2018-2021)
+        euler_fluxes(upp, gm1, fa, ga);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) { dx[v] = fa[v]; dy[v] = ga[v]; }
+        euler_fluxes(up, gm1, fa, ga);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            dx[v] = fma(-4.0, fa[v], dx[v]);
+            dy[v] = fma(-4.0, ga[v], dy[v]);
+            dx[v] = fma(6.0, f0x[v], dx[v]);
+            dy[v] = fma(6.0, f0y[v], dy[v]);
+        }
+        euler_fluxes(um, gm1, fa, ga);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) { dx[v] = fma(-4.0, fa[v], dx[v]); dy[v] = fma(-4.0, ga[v], dy[v]); }
+        euler_fluxes(umm, gm1, fa, ga);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            dx[v] = 0.5 * (dx[v] + fa[v]);
+            dy[v] = 0.5 * (dy[v] + ga[v]);
+            F[v] = fma(1.0 / 120.0, dx[v], F[v]);
+            G[v] = fma(1.0 / 120.0, dy[v], G[v]);
+        }
+    }
+    to_contravariant(J, F, G, cx, cy);
+    // polydeg 3 read s2 in its last contraction: every thread has finished
+    // before it is reused (polydeg 4 passed a barrier since)
+    if constexpr (N == 3) __syncthreads();
+    if (act) put8<S>(s2, sidx, cx, cy);
+    __syncthreads();
+    contract<n, S>(s2, sbase, i, j, sDh, vol);
+}
+
+// Curvilinear CFL speeds of a node (Trixi max_dt for curvilinear meshes): the
+// speeds transformed to the reference element, |Ja^k · (λ1, λ2)| |J⁻¹|.
+__device__ __forceinline__ void curved_speeds(const double* u, const double* J, double invJ, double gamma, double& m1,
+                                              double& m2, bool& bad) {
+    double l1, l2;
+    node_speeds(u, gamma, l1, l2, bad);
+    const double aj = fabs(invJ);
+    m1 = fabs(fma(J[0], l1, J[1] * l2)) * aj;
+    m2 = fabs(fma(J[2], l1, J[3] * l2)) * aj;
+}
+
+// ---------------------------------------------------------------------------
 // Volume kernel: element computation, then the interfaces inside the
 // workgroup, the lifted surface terms (weak form, ℓ̂ = 1/w at the end nodes),
 // the Jacobian and the update.  Faces leaving the workgroup write their face
@@ -1291,9 +1502,10 @@ template <bool TWO> constexpr int kVolWaves = TWO ? 2 : TLW_VOL_WAVES;
 
 // PAR: hyperbolic-parabolic (compressible Navier-Stokes) element procedures,
 // same register budget as shock capturing.
-template <int N, int EB, int KIND, bool SC, bool PAR = false>
+template <int N, int EB, int KIND, bool SC, bool PAR = false, bool CURV = false>
 __global__ __launch_bounds__((VGeo<N, EB>::NT), kVolWaves<SC || PAR>) void k_volume(KArgs a) {
     static_assert(!(SC && PAR), "shock capturing has no hyperbolic-parabolic form");
+    static_assert(!CURV || (KIND == VOL_LW && !SC && !PAR), "the curvilinear path is plain LW");
     using G_ = VGeo<N, EB>;
     constexpr int n = G_::n, NN = G_::NN, S = G_::S, NT = G_::NT;
     __shared__ double lds[G_::LDS];
@@ -1319,7 +1531,8 @@ __global__ __launch_bounds__((VGeo<N, EB>::NT), kVolWaves<SC || PAR>) void k_vol
     const double dt = a.use_state_dt ? st->dt : a.dt_arg;
     const DevConst* c = a.c;
     const double gamma = c->gamma, gm1 = gamma - 1.0;
-    const double invJ = a.inv_jac[e];
+    // inverse Jacobian: per element on a TreeMesh, per node on a curvilinear mesh
+    const double invJ = CURV ? a.ijn[(size_t)e * NN + node] : a.inv_jac[e];
     const size_t off = ((size_t)e * NN + node) * 4;
 
     if (tid < NN) {  // published by the first barrier of the element computation
@@ -1330,7 +1543,11 @@ __global__ __launch_bounds__((VGeo<N, EB>::NT), kVolWaves<SC || PAR>) void k_vol
     double uN[4], Uc[4], F[4], G[4], vol[4];
     double terr = 0.0;  // temporal error of the node (adaptive steps, LW N=4)
     load4(uin + off, uN);
-    if constexpr (PAR) {
+    if constexpr (CURV) {
+        double Jn[4];  // Ja11, Ja12, Ja21, Ja22 of the node: same layout as the state
+        load4(a.Ja + off, Jn);
+        lwc_element<N, EB>(uN, Jn, invJ, gm1, i, j, act, sidx, sbase, s0, s1, s2, sDm, sDh, Uc, F, G, vol);
+    } else if constexpr (PAR) {
         // BR1 gradients of uⁿ (and of us in MDRK stage 2), exchanged through a
         // buffer the element procedure writes only after its first barrier
         double* sg = (KIND == VOL_MDRK2) ? s2 : s1;
@@ -1434,7 +1651,24 @@ __global__ __launch_bounds__((VGeo<N, EB>::NT), kVolWaves<SC || PAR>) void k_vol
     const double* lamst = tree ? Uc : uN;
     const int nbx = fx >= 0 ? a.nbr[4 * e + fx] : -1;
     const int nby = fy >= 0 ? a.nbr[4 * e + fy] : -1;
-    {
+    if constexpr (CURV) {
+        // structured order with the interface's normal n (shared by both
+        // elements): F̃·n, |v(uⁿ)·n| and c(uⁿ)|n| (dg_2d.jl:2229-2264)
+        const double ir = rcp_nr(uN[0]);
+        const double v1 = uN[1] * ir, v2 = uN[2] * ir;
+        const double p = gm1 * (uN[3] - 0.5 * (uN[1] * v1 + uN[2] * v2));
+        const double cs = sqrt_gs(gamma * p * ir);
+        auto face = [&](int f, int l) {
+            double nf[4], Fn[4], r[RQ];
+            load4(a.fnorm + (((size_t)e * 4 + f) * n + l) * 4, nf);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) Fn[v] = fma(F[v], nf[0], G[v] * nf[1]);
+            make_record(r, Uc, Fn, fabs(fma(v1, nf[0], v2 * nf[1])), cs * nf[2]);
+            store_record<RQ>(rec + ((lec * 4 + f) * n + l) * RQ, r);
+        };
+        if (act && fx >= 0) face(fx, j);
+        if (act && fy >= 0) face(fy, i);
+    } else {
         double av1, av2, cs;
         speed_parts2(lamst, gamma, av1, av2, cs);
         auto fill = [&](double* r, const double* fnl) {
@@ -1594,8 +1828,9 @@ __device__ inline void boundary_record(const KArgs& a, int e, int face, int l, d
 // both see the same bits.  In step mode every lane then takes its node's
 // final value (corrected here or finalised by the volume kernel) into the
 // CFL maxima and the admissibility test.
-template <int N, bool BC, bool SC>
+template <int N, bool BC, bool SC, bool CURV = false>
 __global__ __launch_bounds__(kSurfBlock) void k_surface(KArgs a) {
+    static_assert(!(CURV && (BC || SC)), "the curvilinear path is periodic, without shock capturing");
     using G_ = SGeo<N>;
     constexpr int n = G_::n, NN = G_::NN, EB = G_::EB, S = G_::S;
     __shared__ double s_l1[S], s_l2[S], s_m[EB], s_lv[S], s_mv[EB];
@@ -1615,7 +1850,7 @@ __global__ __launch_bounds__(kSurfBlock) void k_surface(KArgs a) {
     const double dt = a.use_state_dt ? st->dt : a.dt_arg;
     const double t = a.use_state_dt ? st->t : a.t_arg;
     const DevConst* c = a.c;
-    const double invJ = a.inv_jac[e];
+    const double invJ = CURV ? a.ijn[(size_t)e * NN + node] : a.inv_jac[e];
     const int eblk = a.etile[e];
     const int fx = (i == 0) ? 0 : ((i == n - 1) ? 1 : -1);
     const int fy = (j == 0) ? 2 : ((j == n - 1) ? 3 : -1);
@@ -1674,7 +1909,13 @@ __global__ __launch_bounds__(kSurfBlock) void k_surface(KArgs a) {
     if (act) {
         double l1, l2;
         bool bad;
-        node_speeds(val, c->gamma, l1, l2, bad);
+        if constexpr (CURV) {
+            double Jn[4];
+            load4(a.Ja + ((size_t)e * NN + node) * 4, Jn);
+            curved_speeds(val, Jn, invJ, c->gamma, l1, l2, bad);
+        } else {
+            node_speeds(val, c->gamma, l1, l2, bad);
+        }
         if (bad) st->bad_state = 1;
         s_l1[lec * NN + node] = l1;
         s_l2[lec * NN + node] = l2;
@@ -1688,7 +1929,8 @@ __global__ __launch_bounds__(kSurfBlock) void k_surface(KArgs a) {
             m2 = dmax(m2, s_l2[lec * NN + k]);
             mv = dmax(mv, s_lv[lec * NN + k]);
         }
-        s_m[lec] = invJ * (m1 + m2);
+        // curvilinear speeds carry the node's |J⁻¹| already
+        s_m[lec] = CURV ? m1 + m2 : invJ * (m1 + m2);
         s_mv[lec] = mv;
     }
     __syncthreads();
@@ -1724,7 +1966,13 @@ __global__ __launch_bounds__(kSurfBlock) void k_max_speed(KArgs a) {
     load4(uin + ((size_t)e * NN + node) * 4, val);
     double l1, l2;
     bool bad;
-    node_speeds(val, a.c->gamma, l1, l2, bad);
+    if (a.Ja) {  // curvilinear mesh: speeds in reference space, |J⁻¹| included
+        double Jn[4];
+        load4(a.Ja + ((size_t)e * NN + node) * 4, Jn);
+        curved_speeds(val, Jn, a.ijn[(size_t)e * NN + node], a.c->gamma, l1, l2, bad);
+    } else {
+        node_speeds(val, a.c->gamma, l1, l2, bad);
+    }
     if (act && bad) st->bad_state = 1;
     const bool par = a.pv_off > 0;
     if (act) {
@@ -1740,7 +1988,7 @@ __global__ __launch_bounds__(kSurfBlock) void k_max_speed(KArgs a) {
             m2 = dmax(m2, s_l2[lec * NN + k]);
             mv = dmax(mv, s_lv[lec * NN + k]);
         }
-        s_m[lec] = a.inv_jac[e] * (m1 + m2);
+        s_m[lec] = a.Ja ? m1 + m2 : a.inv_jac[e] * (m1 + m2);
         s_mv[lec] = mv;
     }
     __syncthreads();

@@ -14,7 +14,7 @@ from .equations import (CompressibleEulerEquations2D, CompressibleNavierStokesDi
 from .integrator import (LWIntegrator, LWSolution, compute_dt, isapprox, isfinished, perform_step, set_dt,
                          set_t_and_iter, solve_lwfr)
 from .limiters import PositivityPreservingLimiterZhangShu, density, pressure
-from .mesh import TreeMesh
+from .mesh import StructuredMesh, TreeMesh, coordinates2mapping
 from .semidiscretization import (DGSEM, LW, MDRK, Adaptive, CFLBased, FirstOrderReconstruction, IndicatorHennemannGassner,
                                  LWUpdate, MUSCLHancockReconstruction, SemidiscretizationHyperbolic,
                                  SemidiscretizationHyperbolicParabolic, VolumeIntegralFR,
@@ -25,7 +25,8 @@ from .semidiscretization import (DGSEM, LW, MDRK, Adaptive, CFLBased, FirstOrder
 __all__ = [
     "LobattoLegendreBasis", "CompressibleEulerEquations2D", "ConstantState", "FourQuadrantRiemann",
     "IsentropicVortex", "LinearScalarAdvectionEquation2D", "LWIntegrator", "LWSolution", "compute_dt", "isapprox",
-    "isfinished", "perform_step", "set_dt", "set_t_and_iter", "solve_lwfr", "TreeMesh", "DGSEM", "LW", "MDRK",
+    "isfinished", "perform_step", "set_dt", "set_t_and_iter", "solve_lwfr", "TreeMesh",
+    "StructuredMesh", "coordinates2mapping", "DGSEM", "LW", "MDRK",
     "CFLBased", "Adaptive", "LWUpdate", "SemidiscretizationHyperbolic", "VolumeIntegralFR", "flux_lax_friedrichs",
     "get_time_discretization", "lw_fourier_cfl", "rhs", "rhs_mdrk1", "rhs_mdrk2", "semidiscretize", "trixi2lw",
     "FirstOrderReconstruction", "IndicatorHennemannGassner", "MUSCLHancockReconstruction",

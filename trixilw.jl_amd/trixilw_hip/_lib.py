@@ -28,7 +28,7 @@ EXPORTED = (
     "tlw_enable_kernel_timing", "tlw_kernel_times", "tlw_set_elements_per_workgroup",
     "tlw_get_elements_per_workgroup", "tlw_autotune", "tlw_set_positivity_limiter", "tlw_apply_positivity_limiter",
     "tlw_adjoint_lw", "tlw_create_batch", "tlw_n_instances", "tlw_batch_status", "tlw_enable_graphs",
-    "tlw_create_parabolic", "tlw_solve_init_adaptive", "tlw_rejected_steps",
+    "tlw_create_parabolic", "tlw_solve_init_adaptive", "tlw_rejected_steps", "tlw_create_curved",
 )
 
 
@@ -65,6 +65,13 @@ class SolverDesc(C.Structure):
         ("alpha_min", C.c_double),
         ("alpha_smooth", C.c_int32),
         ("inverse_vandermonde_host", C.POINTER(C.c_double)),
+    ]
+
+
+class CurvedDesc(C.Structure):
+    _fields_ = [
+        ("contravariant_vectors_host", C.POINTER(C.c_double)),
+        ("inverse_jacobian_host", C.POINTER(C.c_double)),
     ]
 
 
@@ -108,6 +115,9 @@ def lib():
     L.tlw_create_parabolic.restype = C.c_int
     L.tlw_create_parabolic.argtypes = [C.POINTER(MeshDesc), C.POINTER(BasisDesc), C.POINTER(SolverDesc),
                                        C.POINTER(ParabolicDesc), C.POINTER(vp)]
+    L.tlw_create_curved.restype = C.c_int
+    L.tlw_create_curved.argtypes = [C.POINTER(MeshDesc), C.POINTER(BasisDesc), C.POINTER(SolverDesc),
+                                    C.POINTER(CurvedDesc), C.POINTER(vp)]
     L.tlw_rebuild_mesh.restype = C.c_int
     L.tlw_rebuild_mesh.argtypes = [vp, C.POINTER(MeshDesc)]
     L.tlw_destroy.restype = None

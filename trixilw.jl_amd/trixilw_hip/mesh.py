@@ -1,4 +1,4 @@
-"""Uniformly refined 2-D TreeMesh on the host.
+"""Uniformly refined 2-D TreeMesh and curvilinear StructuredMesh on the host.
 
 Mirrors what TrixiLW reads from Trixi's TreeMesh and its containers
 (`cache.elements.inverse_jacobian`, `node_coordinates`, `cache.interfaces`,
@@ -89,3 +89,110 @@ class TreeMesh:
     def lexicographic_permutation(self):
         """perm such that u_lex = u[perm] maps this element order to e = ix + nx*iy."""
         return self.element_of_cell.copy()
+
+
+def coordinates2mapping(coordinates_min, coordinates_max):
+    """Trixi coordinates2mapping: the affine map of the reference square
+    [-1, 1]² onto the box [min, max]."""
+    x0, y0 = map(float, coordinates_min)
+    x1, y1 = map(float, coordinates_max)
+
+    def mapping(xi, eta):
+        return x0 + 0.5 * (xi + 1.0) * (x1 - x0), y0 + 0.5 * (eta + 1.0) * (y1 - y0)
+
+    return mapping
+
+
+class StructuredMesh:
+    """StructuredMesh(cells_per_dimension, mapping; periodicity=True), or
+    StructuredMesh(cells_per_dimension, coordinates_min, coordinates_max; periodicity=True).
+
+    ``mapping(xi, eta) -> (x, y)`` maps the global reference square [-1, 1]²
+    onto the domain and must accept numpy arrays.  Elements are numbered
+    e = cx + nx*cy like Trixi's linear indices.  The node coordinates and
+    metric terms follow Trixi's StructuredMesh cache (restated: the element's
+    GLL nodes placed in its cell of the reference square and mapped;
+    jacobian_matrix from the derivative matrix applied to the node
+    coordinates; contravariant vectors Ja¹ = (y_η, -x_η), Ja² = (-y_ξ, x_ξ);
+    inverse_jacobian = 1/(x_ξ y_η - x_η y_ξ)), which TrixiLW's StructuredMesh
+    kernels read (src/solvers/dgsem_structured/dg_2d.jl:1379-2290).  The
+    device path covers periodic meshes.
+    """
+
+    curved = True
+
+    def __init__(self, cells_per_dimension, mapping, coordinates_max=None, periodicity=True):
+        nx, ny = (int(c) for c in cells_per_dimension)
+        if nx < 1 or ny < 1:
+            raise ValueError("cells_per_dimension must be positive")
+        if coordinates_max is not None:
+            mapping = coordinates2mapping(mapping, coordinates_max)
+        if not callable(mapping):
+            raise TypeError("mapping must be callable (or give coordinates_min, coordinates_max)")
+        if isinstance(periodicity, bool):
+            periodicity = (periodicity, periodicity)
+        self.periodicity = tuple(bool(p) for p in periodicity)
+        self.mapping = mapping
+        self.nx, self.ny = nx, ny
+        self.n_elements = nx * ny
+        E = self.n_elements
+        e = np.arange(E)
+        self.ix, self.iy = e % nx, e // nx
+        nbr = np.full((E, 4), -1, dtype=np.int32)
+        px, py = self.periodicity
+        for face, (dx, dy) in enumerate(((-1, 0), (1, 0), (0, -1), (0, 1))):
+            jx, jy = self.ix + dx, self.iy + dy
+            if px:
+                jx = jx % nx
+            if py:
+                jy = jy % ny
+            ok = (jx >= 0) & (jx < nx) & (jy >= 0) & (jy < ny)
+            nbr[:, face] = np.where(ok, jx + nx * jy, -1)
+        self.neighbors = nbr
+        self._metrics = {}
+
+    @property
+    def ncells_per_dimension(self):
+        return (self.nx, self.ny)
+
+    def node_coordinates(self, nodes):
+        """x, y of every node, arrays [e, j, i] (Trixi's node_coordinates[:, i, j, e])."""
+        nodes = np.asarray(nodes, dtype=np.float64)
+        dx, dy = 2.0 / self.nx, 2.0 / self.ny
+        ox = -1.0 + self.ix * dx + dx / 2
+        oy = -1.0 + self.iy * dy + dy / 2
+        xi = ox[:, None, None] + dx / 2 * nodes[None, None, :]
+        eta = oy[:, None, None] + dy / 2 * nodes[None, :, None]
+        xi, eta = np.broadcast_arrays(xi, eta)
+        x, y = self.mapping(xi, eta)
+        shape = (self.n_elements, len(nodes), len(nodes))
+        return (np.ascontiguousarray(np.broadcast_to(x, shape), dtype=np.float64),
+                np.ascontiguousarray(np.broadcast_to(y, shape), dtype=np.float64))
+
+    def metrics(self, basis):
+        """(contravariant_vectors [E, j, i, 4] with Ja11, Ja12, Ja21, Ja22 per node,
+        inverse_jacobian [E, j, i]) for the basis' nodes and derivative matrix."""
+        key = id(basis)
+        if key not in self._metrics:
+            D = np.asarray(basis.derivative_matrix, dtype=np.float64)
+            x, y = self.node_coordinates(basis.nodes)
+            x_xi = np.einsum("ik,ejk->eji", D, x)
+            y_xi = np.einsum("ik,ejk->eji", D, y)
+            x_eta = np.einsum("jk,eki->eji", D, x)
+            y_eta = np.einsum("jk,eki->eji", D, y)
+            Ja = np.stack([y_eta, -x_eta, -y_xi, x_xi], axis=-1)
+            ijn = 1.0 / (x_xi * y_eta - x_eta * y_xi)
+            self._metrics[key] = (np.ascontiguousarray(Ja), np.ascontiguousarray(ijn))
+        return self._metrics[key]
+
+    def set_metrics(self, basis, contravariant_vectors, inverse_jacobian):
+        """Metric terms computed elsewhere (e.g. Trixi's cache.elements of the
+        same mesh) for the basis, in the layouts metrics() returns; used in
+        place of the host computation."""
+        n = len(basis.nodes)
+        Ja = np.ascontiguousarray(contravariant_vectors, dtype=np.float64).reshape(self.n_elements, n, n, 4)
+        ijn = np.ascontiguousarray(inverse_jacobian, dtype=np.float64).reshape(self.n_elements, n, n)
+        self._metrics[id(basis)] = (Ja, ijn)
+
+    def lexicographic_permutation(self):
+        return np.arange(self.n_elements)

@@ -144,6 +144,10 @@ class SemidiscretizationHyperbolic:
     condition on every non-periodic face, or None for a fully periodic mesh.
     ``surface_flux_order`` selects the TreeMesh ("tree") or Structured/P4est
     ("structured") argument order of the LW Rusanov flux.
+
+    On a curvilinear StructuredMesh (periodic, plain LW) the kernels of
+    src/solvers/dgsem_structured/dg_2d.jl apply: contravariant fluxes, per-node
+    inverse Jacobians, interface normals and the structured flux order.
     """
 
     def __init__(self, mesh, time_discretization, equations, initial_condition, solver, boundary_conditions=None,
@@ -198,11 +202,22 @@ class SemidiscretizationHyperbolic:
         self._h = h
 
     def _create(self, md, basis, sd, h):
+        if getattr(self.mesh, "curved", False):
+            k = self._keep
+            cd = _lib.CurvedDesc(_dptr(k["Ja"]), _dptr(k["ijn"]))
+            rc = _lib.lib().tlw_create_curved(C.byref(md), C.byref(basis), C.byref(sd), C.byref(cd), C.byref(h))
+            if rc != _lib.TLW_OK:
+                raise _lib.TLWError(rc, "tlw_create_curved failed (periodic mesh, plain LW and positive Jacobians "
+                                        "required)")
+            return
         rc = _lib.lib().tlw_create(C.byref(md), C.byref(basis), C.byref(sd), C.byref(h))
         if rc != _lib.TLW_OK:
             raise _lib.TLWError(rc, "tlw_create failed (invalid description or HIP error)")
 
     def _mesh_arrays(self, mesh):
+        if getattr(mesh, "curved", False):
+            Ja, ijn = mesh.metrics(self.solver.basis)
+            return {"nbr": np.ascontiguousarray(mesh.neighbors, dtype=np.int32), "Ja": Ja, "ijn": ijn}
         return {
             "nbr": np.ascontiguousarray(mesh.neighbors, dtype=np.int32),
             "invj": np.ascontiguousarray(mesh.inverse_jacobian, dtype=np.float64),
@@ -212,6 +227,10 @@ class SemidiscretizationHyperbolic:
 
     def _mesh_desc(self):
         k = self._keep
+        if "Ja" in k:  # curvilinear: the TreeMesh element arrays are not read
+            null = C.POINTER(C.c_double)()
+            return _lib.MeshDesc(int(self.mesh.n_elements), k["nbr"].ctypes.data_as(C.POINTER(C.c_int32)), null,
+                                 null, null)
         return _lib.MeshDesc(int(self.mesh.n_elements), k["nbr"].ctypes.data_as(C.POINTER(C.c_int32)),
                              _dptr(k["invj"]), _dptr(k["center"]), _dptr(k["length"]))
 
