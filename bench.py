@@ -44,7 +44,12 @@ WORKLOADS = {
     "lw_navierstokes_shear_1024x1024_N4": dict(level=10, N=4, td="lw", problem="ns_shear", cfl=0.5),
     "mdrk_navierstokes_shear_1024x1024_N3": dict(level=10, N=3, td="mdrk", problem="ns_shear", cfl=0.5),
     "lw_navierstokes_shear_256x256_N4": dict(level=8, N=4, td="lw", problem="ns_shear", cfl=0.5),
+    # isentropic vortex on a periodic curvilinear StructuredMesh: [0,20]^2 warped by
+    # x, y += 20·0.05 sin(2πs) sin(2πt) (s, t the unit-square coordinates), CFL 0.98·LW limit
+    "lw_euler_vortex_warped_1024x1024_N4": dict(nx=1024, N=4, td="lw", problem="vortex_warped"),
+    "lw_euler_vortex_warped_256x256_N4": dict(nx=256, N=4, td="lw", problem="vortex_warped"),
 }
+WARP_L, WARP_A = 20.0, 0.05
 NS_MU, NS_PR = 0.01, 0.72
 DEFAULT_WORKLOAD = "lw_euler_vortex_1024x1024_N4"
 
@@ -53,7 +58,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def alg_bytes_per_dof(N: int, td: str, parabolic: bool = False):
+def alg_bytes_per_dof(N: int, td: str, parabolic: bool = False, curved: bool = False):
     """Algorithmic HBM bytes per DOF-update: (whole step, volume kernel).
 
     Whole step (LW): read uⁿ + write uⁿ⁺¹ + one write and one read of the face
@@ -63,9 +68,17 @@ def alg_bytes_per_dof(N: int, td: str, parabolic: bool = False):
     MDRK: 8 (5·4 n² + 4·48 n) / n² per step (two stages).
     Navier-Stokes adds, per stage, the read of the neighbours' face states for
     the BR1 gradient (4 variables × n nodes × 4 faces): + 8·4·4 n / n².
+    A curvilinear mesh adds the node metrics (Ja: 4, J⁻¹: 1 double per node)
+    and the face normals (4 doubles per face node) to the volume kernel, and
+    the node metrics again to the surface kernel (Jacobian and CFL speeds):
+    volume + 8 (5 n² + 4·4 n) / n², step + 8 (2·5 n² + 4·4 n) / n².
     """
     n = N + 1
     br1 = 8.0 * 4 * 4 * n / (n * n) if parabolic else 0.0
+    if curved:
+        step = 8.0 * (2 * 4 * n * n + 2 * 3 * 4 * 4 * n + 2 * 5 * n * n + 4 * 4 * n) / (n * n)
+        vol = 8.0 * (2 * 4 * n * n + 3 * 4 * 4 * n + 5 * n * n + 4 * 4 * n) / (n * n)
+        return step, vol
     if td == "lw":
         step = 8.0 * (2 * 4 * n * n + 2 * 3 * 4 * 4 * n) / (n * n) + br1
         vol = 8.0 * (2 * 4 * n * n + 3 * 4 * 4 * n) / (n * n) + br1
@@ -177,6 +190,17 @@ def ns_shear(x, y, t=0.0, gamma=1.4):
     return np.stack([rho, rho * v1, rho * v2, p / (gamma - 1) + 0.5 * rho * (v1**2 + v2**2)], axis=-1)
 
 
+def warp_mapping(L, A):
+    """[0, L]² warped by A sin 2πs sin 2πt in both coordinates (periodic)."""
+
+    def mapping(xi, eta):
+        s, t = 0.5 * (xi + 1.0), 0.5 * (eta + 1.0)
+        w = A * np.sin(2.0 * np.pi * s) * np.sin(2.0 * np.pi * t)
+        return L * (s + w), L * (t + w)
+
+    return mapping
+
+
 def make_problem(wl):
     """Semidiscretization, initial condition, CFL number and the oracle's
     constructor arguments of a workload."""
@@ -196,10 +220,19 @@ def make_problem(wl):
         ic = ns_shear
         lo, hi, bc, periodic = -1.0, 1.0, None, True
         okw = dict(parabolic="navier_stokes", mu=NS_MU, prandtl=NS_PR)
+    elif problem == "vortex_warped":
+        import oracle_ctypes
+
+        ic = tlw.IsentropicVortex(center=(WARP_L / 2, WARP_L / 2), period=(WARP_L, WARP_L))
+        lo, hi, bc, periodic = 0.0, WARP_L, None, True
+        okw = dict(mapping=(oracle_ctypes.MAP_WARP, (WARP_L, WARP_A)))
     else:
         ic = tlw.IsentropicVortex()
         lo, hi, bc, periodic = -10.0, 10.0, None, True
-    mesh = tlw.TreeMesh((lo, lo), (hi, hi), wl["level"], periodicity=periodic, ordering="morton")
+    if problem == "vortex_warped":
+        mesh = tlw.StructuredMesh((wl["nx"], wl["nx"]), warp_mapping(WARP_L, WARP_A))
+    else:
+        mesh = tlw.TreeMesh((lo, lo), (hi, hi), wl["level"], periodicity=periodic, ordering="morton")
     solver = tlw.DGSEM(N, volume_integral=vi)
     if problem == "ns_shear":
         eqp = tlw.CompressibleNavierStokesDiffusion2D(eq, mu=NS_MU, Prandtl=NS_PR,
@@ -291,6 +324,42 @@ def measure_navier_stokes(steps, cpu_steps, threads, with_cpu=True):
     return out
 
 
+def measure_curved(steps, cpu_steps, threads, with_cpu=True):
+    """Sub-record: LW N=4 isentropic vortex on the warped 1024² StructuredMesh,
+    device loop throughput, volume kernel bytes per launch and the CPU oracle."""
+    name = "lw_euler_vortex_warped_1024x1024_N4"
+    wl = WORKLOADS[name]
+    semi, mesh, ic, cfl, (lo, hi), okw = make_problem(wl)
+    x, y = semi.node_coordinates()
+    u0 = np.ascontiguousarray(ic(x, y, 0.0))
+    r = run_steps(semi, u0, cfl, steps, 2)
+    dofs = semi.ndofs()
+    _, vol_b = alg_bytes_per_dof(4, "lw", curved=True)
+    out = {"workload": name, "value": r["value"], "unit": "DOF-updates/s",
+           "ms_per_step": round(r["ms_per_step"], 4), "steps": steps, "dofs": dofs,
+           "volume_avg_launch_ms": round(r["vol_ms"], 4), "surface_avg_launch_ms": round(r["surf_ms"], 4),
+           "volume_achieved_GBs": round(vol_b * dofs / (r["vol_ms"] * 1e-3) / 1e9, 2),
+           "volume_alg_bytes_per_dof": vol_b, "elements_per_workgroup": r["tile"], "tile_autotune_ms": r["tune"],
+           "config": f"isentropic vortex, periodic StructuredMesh [0,{WARP_L:g}]^2 warped by amplitude {WARP_A}, "
+                     f"{mesh.nx}x{mesh.ny}, polydeg 4, LW, CFL {cfl:.4f}"}
+    log(f"bench: curvilinear {r['value'] / 1e9:.3f} G DOF-updates/s, {r['ms_per_step']:.3f} ms/step, "
+        f"volume {r['vol_ms']:.3f} ms, surface {r['surf_ms']:.3f} ms")
+    semi.close()
+    if with_cpu:
+        import oracle_ctypes
+
+        nx = mesh.nx
+        o = oracle_ctypes.Oracle(4, nx, nx, lo, lo, (hi - lo) / nx, **okw)
+        xo, yo = o.node_coords()
+        v, el = cpu_baseline(o, np.ascontiguousarray(ic(xo, yo, 0.0)), cfl, False, dofs, cpu_steps, threads)
+        log(f"bench: curvilinear CPU {v / 1e6:.2f} M DOF-updates/s ({el:.1f} s)")
+        out["cpu_baseline"] = {"value": v, "unit": "DOF-updates/s", "cores": threads, "kind": "port",
+                               "sample": f"{cpu_steps} step(s) of the same mesh, C++/OpenMP oracle"}
+        out["gpu_over_cpu"] = round(r["value"] / v, 2)
+        del o
+    return out
+
+
 def measure_adaptive(semi, u0, attempts, tol=(1e-10, 1e-10), dt0=1e-5):
     """Sub-record: the adaptive (error-controlled) LW loop on the same mesh and
     state: `attempts` device-loop attempts after 30 untimed ones (the controller
@@ -340,6 +409,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-instance batch measurement")
     ap.add_argument("--no-ns", action="store_true", help="skip the Navier-Stokes sub-record")
     ap.add_argument("--no-adaptive", action="store_true", help="skip the adaptive-stepping sub-record")
+    ap.add_argument("--no-curved", action="store_true", help="skip the curvilinear-mesh sub-record")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device copy (the reference kernel for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -371,6 +441,7 @@ def main():
     N, td = wl["N"], wl["td"]
     sc = wl.get("problem") == "riemann_sc"
     par = wl.get("problem") == "ns_shear"
+    curved = wl.get("problem") == "vortex_warped"
     semi, mesh, ic, cfl, (lo, hi), okw = make_problem(wl)
     dofs = semi.ndofs()
     log(f"bench: {args.workload}: {mesh.n_elements} elements, {dofs} DOFs")
@@ -386,7 +457,7 @@ def main():
     log(f"bench: {value / 1e9:.3f} G DOF-updates/s, {ms_per_step:.3f} ms/step, "
         f"volume {vms_avg:.3f} ms x{vn}, surface {sms_avg:.3f} ms")
 
-    step_b, vol_b = alg_bytes_per_dof(N, td, parabolic=par)
+    step_b, vol_b = alg_bytes_per_dof(N, td, parabolic=par, curved=curved)
     vol_avg_s = vms_avg * 1e-3
     launches_per_step = 1 if td == "lw" else 2
     if vol_b is not None:
@@ -399,7 +470,7 @@ def main():
     roofline = {
         "bound": "hbm",
         "kernel": (("k_volume_lw4" if (td == "lw" and N == 4) else ("k_volume_lw3" if td == "lw" else "k_volume_mdrk*"))
-                   + ("_navier_stokes" if par else "")),
+                   + ("_navier_stokes" if par else "") + ("_curved" if curved else "")),
         "achieved": round(achieved, 2),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
@@ -414,7 +485,7 @@ def main():
     }
 
     adjoint = None
-    if td == "lw" and not sc and not par and not args.no_adjoint:
+    if td == "lw" and not sc and not par and not curved and not args.no_adjoint:
         # adjoint/forward step ratio: one adjoint step (tlw_adjoint_lw: face records + reverse
         # sweep) against one forward step of the loop above, same state and dt
         _lib.check(L.tlw_get_state(h, u0.ctypes.data_as(dp), u0.size), h)  # the state after the timed steps
@@ -460,6 +531,10 @@ def main():
         navier_stokes = measure_navier_stokes(max(3, min(args.steps, 10)), 1, threads,
                                               with_cpu=not args.no_cpu_baseline)
 
+    curved_rec = None
+    if args.workload == DEFAULT_WORKLOAD and not args.no_curved:
+        curved_rec = measure_curved(max(3, min(args.steps, 10)), 1, threads, with_cpu=not args.no_cpu_baseline)
+
     cpu = None
     if not args.no_cpu_baseline:
         import oracle_ctypes
@@ -492,7 +567,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": args.workload,
                    "mesh": (f"{mesh.nx}x{mesh.nx} TreeMesh [0,1]^2, time-averaged Dirichlet boundary" if sc else
-                            f"{mesh.nx}x{mesh.nx} periodic TreeMesh [{lo:g},{hi:g}]^2"),
+                            f"{mesh.nx}x{mesh.nx} periodic StructuredMesh [0,{WARP_L:g}]^2 warped by {WARP_A}"
+                            if curved else f"{mesh.nx}x{mesh.nx} periodic TreeMesh [{lo:g},{hi:g}]^2"),
                    "polydeg": N, "time_discretization": td.upper(), "dofs": dofs,
                    "initial_condition": ("four-quadrant Riemann problem (Lax-Liu 3)" if sc else
                                          "shear flow with density/pressure waves" if par else
@@ -508,6 +584,7 @@ def main():
         "batch": batch,
         "navier_stokes": navier_stokes,
         "adaptive": adaptive,
+        "curved": curved_rec,
     }
     print(json.dumps(out), flush=True)
     semi.close()
