@@ -156,9 +156,12 @@ int tlw_create_parabolic(const tlw_mesh_desc* mesh, const tlw_basis_desc* basis,
  * mesh->neighbors_host gives the face neighbours (a periodic mesh: no -1 entries);
  * mesh->inverse_jacobian_host, element_center_host and element_length_host are not read and
  * may be NULL.  The structured flux order applies whatever solver->surface_flux_order says.
- * Plain VolumeIntegralFR with TLW_LW; shock capturing, boundaries, MDRK return TLW_ENOTSUP, and
- * so do the positivity limiter, adaptive steps, tlw_adjoint_lw and tlw_rebuild_mesh on such a
- * context.  A non-positive inverse Jacobian returns TLW_EINVAL. */
+ * With TLW_MDRK, tlw_rhs_mdrk1/2 are rhs_mdrk1!/2! of src/solvers/dgsem_structured/
+ * multi_derivative_2d.jl:1-80 (mdrk_kernel_1!/2!, 82-445: F̃ = f/2 + f_t/8, then F2 + f_t(us)/3,
+ * in contravariant form for the volume term).  Plain VolumeIntegralFR; shock capturing and
+ * boundaries return TLW_ENOTSUP, and so do the positivity limiter, adaptive steps,
+ * tlw_adjoint_lw and tlw_rebuild_mesh on such a context.  A non-positive inverse Jacobian
+ * returns TLW_EINVAL. */
 int tlw_create_curved(const tlw_mesh_desc* mesh, const tlw_basis_desc* basis, const tlw_solver_desc* solver,
                       const tlw_curved_desc* metrics, tlw_ctx** out);
 int tlw_n_instances(const tlw_ctx* ctx, int32_t* n_instances);

@@ -10,6 +10,8 @@
 //   lw_volume_kernel_3! / 4!      dg_2d.jl:1379-1685, 1687-2105
 //   calc_interface_flux!          dg_2d.jl:2107-2290 (λ from uⁿ, jump on Ũ, F̃ in the
 //                                 right element's contravariant direction)
+//   rhs_mdrk1! / rhs_mdrk2!       structured/multi_derivative_2d.jl:1-80
+//   mdrk_kernel_1! / 2!           structured/multi_derivative_2d.jl:82-243, 245-445
 //   normal_product                src/solvers/fr.jl:146-150
 // and, from the un-vendored Trixi.jl (restated from their published
 // definitions): StructuredMesh node coordinates (the element's GLL nodes in the
@@ -325,6 +327,156 @@ template <class T, class Eq> struct CurvedSolver : Solver<T, Eq> {
         this->reset_du(du);
 #pragma omp parallel for schedule(static)
         for (int e = 0; e < mesh.E; ++e) lw_volume_curved(du, u, dt, e);
+        calc_interface_flux_curved(u);
+        surface_integral_and_jacobian_curved(du);
+    }
+
+    // mdrk_kernel_1! for StructuredMesh (structured/multi_derivative_2d.jl:82-243):
+    // Cartesian F = f/2 + f_t/8 (element cache) and F2 = f + f_t/6 (stage cache),
+    // contravariant F̃ = cv_f/2 + cv_f_t/8 accumulated separately from the
+    // contravariant fluxes of each Taylor state; U = u/2 + ut/8, U2 = u + ut/6.
+    // The low-order update u_np1_low (an adaptive-mode output) is left out.
+    void mdrk_curved_1(T* du, const T* u, double dt, int e) {
+        const auto& D = basis.D;
+        const auto& Dh = basis.Dhat;
+        T ut[MAXN][MAXN][NV], Ft[MAXN][MAXN][NV], Gt[MAXN][MAXN][NV];
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i)
+                for (int v = 0; v < NV; ++v) ut[j][i][v] = T(0);
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) {
+                const T* un = &u[idx(0, i, j, e)];
+                T f[NV], g[NV], fc[NV], gc[NV];
+                cv_fluxes(un, &Ja[4 * nidx(i, j, e)], f, g, fc, gc);
+                for (int v = 0; v < NV; ++v) {
+                    this->F[fidx(v, 0, i, j, e)] = 0.5 * f[v];
+                    this->F[fidx(v, 1, i, j, e)] = 0.5 * g[v];
+                    this->F2[fidx(v, 0, i, j, e)] = f[v];
+                    this->F2[fidx(v, 1, i, j, e)] = g[v];
+                    Ft[j][i][v] = 0.5 * fc[v];
+                    Gt[j][i][v] = 0.5 * gc[v];
+                    this->U[idx(v, i, j, e)] = 0.5 * un[v];
+                    this->U2[idx(v, i, j, e)] = un[v];
+                }
+                for (int ii = 0; ii < n; ++ii)
+                    for (int v = 0; v < NV; ++v) ut[j][ii][v] = ut[j][ii][v] + (-dt * D[ii][i]) * fc[v];
+                for (int jj = 0; jj < n; ++jj)
+                    for (int v = 0; v < NV; ++v) ut[jj][i][v] = ut[jj][i][v] + (-dt * D[jj][j]) * gc[v];
+            }
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i)
+                for (int v = 0; v < NV; ++v) ut[j][i][v] = ut[j][i][v] * ijn[nidx(i, j, e)];
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) {
+                T ft[NV], gt[NV], fct[NV], gct[NV];
+                mdrk_curved_ft(&u[idx(0, i, j, e)], ut[j][i], &Ja[4 * nidx(i, j, e)], ft, gt, fct, gct);
+                for (int v = 0; v < NV; ++v) {
+                    this->F[fidx(v, 0, i, j, e)] = this->F[fidx(v, 0, i, j, e)] + 0.125 * ft[v];
+                    this->F[fidx(v, 1, i, j, e)] = this->F[fidx(v, 1, i, j, e)] + 0.125 * gt[v];
+                    Ft[j][i][v] = Ft[j][i][v] + 0.125 * fct[v];
+                    Gt[j][i][v] = Gt[j][i][v] + 0.125 * gct[v];
+                    this->U[idx(v, i, j, e)] = this->U[idx(v, i, j, e)] + 0.125 * ut[j][i][v];
+                    this->F2[fidx(v, 0, i, j, e)] = this->F2[fidx(v, 0, i, j, e)] + (1.0 / 6.0) * ft[v];
+                    this->F2[fidx(v, 1, i, j, e)] = this->F2[fidx(v, 1, i, j, e)] + (1.0 / 6.0) * gt[v];
+                    this->U2[idx(v, i, j, e)] = this->U2[idx(v, i, j, e)] + (1.0 / 6.0) * ut[j][i][v];
+                }
+                for (int ii = 0; ii < n; ++ii)
+                    for (int v = 0; v < NV; ++v) du[idx(v, ii, j, e)] = du[idx(v, ii, j, e)] + Dh[ii][i] * Ft[j][i][v];
+                for (int jj = 0; jj < n; ++jj)
+                    for (int v = 0; v < NV; ++v) du[idx(v, i, jj, e)] = du[idx(v, i, jj, e)] + Dh[jj][j] * Gt[j][i][v];
+            }
+    }
+
+    // f_t of the MDRK stages, Cartesian and contravariant, from the contravariant
+    // fluxes of the four Taylor states u ± ut, u ± 2ut (multi_derivative_2d.jl:161-176)
+    void mdrk_curved_ft(const T* un, const T* ut, const double* J, T* ft, T* gt, T* fct, T* gct) const {
+        T um[NV], up[NV], umm[NV], upp[NV];
+        for (int v = 0; v < NV; ++v) {
+            um[v] = un[v] - ut[v];
+            up[v] = un[v] + ut[v];
+            umm[v] = un[v] - 2.0 * ut[v];
+            upp[v] = un[v] + 2.0 * ut[v];
+        }
+        T fm[NV], gm[NV], cfm[NV], cgm[NV], fp[NV], gp[NV], cfp[NV], cgp[NV];
+        T fmm[NV], gmm[NV], cfmm[NV], cgmm[NV], fpp[NV], gpp[NV], cfpp[NV], cgpp[NV];
+        cv_fluxes(um, J, fm, gm, cfm, cgm);
+        cv_fluxes(up, J, fp, gp, cfp, cgp);
+        cv_fluxes(umm, J, fmm, gmm, cfmm, cgmm);
+        cv_fluxes(upp, J, fpp, gpp, cfpp, cgpp);
+        for (int v = 0; v < NV; ++v) {
+            ft[v] = 1.0 / 12.0 * (-fpp[v] + 8.0 * fp[v] - 8.0 * fm[v] + fmm[v]);
+            gt[v] = 1.0 / 12.0 * (-gpp[v] + 8.0 * gp[v] - 8.0 * gm[v] + gmm[v]);
+            fct[v] = 1.0 / 12.0 * (-cfpp[v] + 8.0 * cfp[v] - 8.0 * cfm[v] + cfmm[v]);
+            gct[v] = 1.0 / 12.0 * (-cgpp[v] + 8.0 * cgp[v] - 8.0 * cgm[v] + cgmm[v]);
+        }
+    }
+
+    // mdrk_kernel_2! for StructuredMesh (structured/multi_derivative_2d.jl:245-445):
+    // F = F2 + f_t(us)/3 (Cartesian), F̃ = Ja·F2 + cv_f_t(us)/3, U = U2 + ust/3.
+    void mdrk_curved_2(T* du, double dt, int e) {
+        const auto& D = basis.D;
+        const auto& Dh = basis.Dhat;
+        T ust[MAXN][MAXN][NV], Ft[MAXN][MAXN][NV], Gt[MAXN][MAXN][NV];
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) {
+                const double* J = &Ja[4 * nidx(i, j, e)];
+                for (int v = 0; v < NV; ++v) {
+                    const T f2 = this->F2[fidx(v, 0, i, j, e)], g2 = this->F2[fidx(v, 1, i, j, e)];
+                    this->F[fidx(v, 0, i, j, e)] = f2;
+                    this->F[fidx(v, 1, i, j, e)] = g2;
+                    Ft[j][i][v] = J[0] * f2 + J[1] * g2;
+                    Gt[j][i][v] = J[2] * f2 + J[3] * g2;
+                    this->U[idx(v, i, j, e)] = this->U2[idx(v, i, j, e)];
+                    ust[j][i][v] = T(0);
+                }
+            }
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) {
+                T f[NV], g[NV], fc[NV], gc[NV];
+                cv_fluxes(&this->us[idx(0, i, j, e)], &Ja[4 * nidx(i, j, e)], f, g, fc, gc);
+                for (int ii = 0; ii < n; ++ii)
+                    for (int v = 0; v < NV; ++v) ust[j][ii][v] = ust[j][ii][v] + (-dt * D[ii][i]) * fc[v];
+                for (int jj = 0; jj < n; ++jj)
+                    for (int v = 0; v < NV; ++v) ust[jj][i][v] = ust[jj][i][v] + (-dt * D[jj][j]) * gc[v];
+            }
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i)
+                for (int v = 0; v < NV; ++v) ust[j][i][v] = ust[j][i][v] * ijn[nidx(i, j, e)];
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) {
+                T ft[NV], gt[NV], fct[NV], gct[NV];
+                mdrk_curved_ft(&this->us[idx(0, i, j, e)], ust[j][i], &Ja[4 * nidx(i, j, e)], ft, gt, fct, gct);
+                for (int v = 0; v < NV; ++v) {
+                    this->F[fidx(v, 0, i, j, e)] = this->F[fidx(v, 0, i, j, e)] + (1.0 / 3.0) * ft[v];
+                    this->F[fidx(v, 1, i, j, e)] = this->F[fidx(v, 1, i, j, e)] + (1.0 / 3.0) * gt[v];
+                    Ft[j][i][v] = Ft[j][i][v] + (1.0 / 3.0) * fct[v];
+                    Gt[j][i][v] = Gt[j][i][v] + (1.0 / 3.0) * gct[v];
+                    this->U[idx(v, i, j, e)] = this->U[idx(v, i, j, e)] + (1.0 / 3.0) * ust[j][i][v];
+                }
+                for (int ii = 0; ii < n; ++ii)
+                    for (int v = 0; v < NV; ++v) du[idx(v, ii, j, e)] = du[idx(v, ii, j, e)] + Dh[ii][i] * Ft[j][i][v];
+                for (int jj = 0; jj < n; ++jj)
+                    for (int v = 0; v < NV; ++v) du[idx(v, i, jj, e)] = du[idx(v, i, jj, e)] + Dh[jj][j] * Gt[j][i][v];
+            }
+    }
+
+    // rhs_mdrk1! / rhs_mdrk2! for StructuredMesh (multi_derivative_2d.jl:1-80):
+    // both stages evaluate the interface flux with λ of uⁿ
+    void rhs_mdrk1(T* du, const T* u, double t, double dt) override {
+        (void)t;
+        this->ensure_mdrk();
+        this->reset_du(du);
+#pragma omp parallel for schedule(static)
+        for (int e = 0; e < mesh.E; ++e) mdrk_curved_1(du, u, dt, e);
+        calc_interface_flux_curved(u);
+        surface_integral_and_jacobian_curved(du);
+    }
+    void rhs_mdrk2(T* du, const T* u, double t, double dt) override {
+        (void)t;
+        this->ensure_mdrk();
+        this->reset_du(du);
+#pragma omp parallel for schedule(static)
+        for (int e = 0; e < mesh.E; ++e) mdrk_curved_2(du, dt, e);
         calc_interface_flux_curved(u);
         surface_integral_and_jacobian_curved(du);
     }

@@ -1,6 +1,7 @@
-"""GPU parity of the curvilinear StructuredMesh LW path (periodic, compressible
-Euler) against the CPU oracle: rhs on warped and affine meshes, the curvilinear
-max_dt and K-step device-loop states within 1e-12 relative (L2 and L∞), every
+"""GPU parity of the curvilinear StructuredMesh LW and MDRK paths (periodic,
+compressible Euler) against the CPU oracle: rhs and both MDRK stages on warped
+and affine meshes, the curvilinear max_dt and K-step device-loop states (LW and
+MDRK) within 1e-12 relative (L2 and L∞), every
 volume tile, the affine mesh against the TreeMesh kernels with the structured
 flux order, free-stream preservation and conservation at 256×256, and the
 combinations the path rejects.  Inputs are seeded.
@@ -78,6 +79,58 @@ def test_curved_rhs_parity(oracle_mod, N, kind, A):
     F = semi.get_aux("F")[mesh.lexicographic_permutation()]
     assert_close(U, o.aux("U").reshape(U.shape), what="Ũ")
     assert_close(F, o.aux("F").reshape(F.shape), what="F̃")
+    semi.close()
+
+
+@pytest.mark.parametrize("N", [3, 4])
+def test_curved_rhs_mdrk_parity(oracle_mod, N):
+    """Both MDRK stages (mdrk_kernel_1!/2! for StructuredMesh) with Ũ and F̃."""
+    import trixilw_hip as tlw
+
+    torch = torch_mod()
+    print(f"curved MDRK rhs N={N}", flush=True)
+    semi, o, u_lex, mesh = setup_curved(oracle_mod, N, 8, 6, A=0.1, td="mdrk", keep_aux=True, seed=4)
+    dt = 0.4 * o.max_dt(u_lex)
+    d1_ref = o.rhs(u_lex, 0.2, dt, kind="mdrk1")
+    U1, F1 = o.aux("U"), o.aux("F")
+    us = u_lex + dt * d1_ref
+    o.set_us(us)
+    d2_ref = o.rhs(u_lex, 0.2, dt, kind="mdrk2")
+    u = to_dev(mesh, u_lex)
+    du = torch.empty_like(u)
+    semi.cache["dt"][0] = dt
+    tlw.rhs_mdrk1(du, u, semi, 0.2)
+    torch.cuda.synchronize()
+    assert_close(to_lex(mesh, du), d1_ref, what=f"curved mdrk1 N={N}")
+    U = semi.get_aux("U")[mesh.lexicographic_permutation()]
+    F = semi.get_aux("F")[mesh.lexicographic_permutation()]
+    assert_close(U, U1.reshape(U.shape), what="stage-1 Ũ")
+    assert_close(F, F1.reshape(F.shape), what="stage-1 F̃")
+    semi.set_us(to_dev(mesh, us))
+    tlw.rhs_mdrk2(du, u, semi, 0.2)
+    torch.cuda.synchronize()
+    assert_close(to_lex(mesh, du), d2_ref, what=f"curved mdrk2 N={N}")
+    U = semi.get_aux("U")[mesh.lexicographic_permutation()]
+    F = semi.get_aux("F")[mesh.lexicographic_permutation()]
+    assert_close(U, o.aux("U").reshape(U.shape), what="stage-2 Ũ")
+    assert_close(F, o.aux("F").reshape(F.shape), what="stage-2 F̃")
+    semi.close()
+
+
+@pytest.mark.parametrize("N,nx,K", [(3, 16, 20), (4, 8, 15)])
+def test_curved_mdrk_device_loop_k_steps(oracle_mod, N, nx, K):
+    import trixilw_hip as tlw
+
+    print(f"curved MDRK device loop N={N} {nx}x{nx} K={K}", flush=True)
+    semi, o, u_lex, mesh = setup_curved(oracle_mod, N, nx, nx, A=0.1, td="mdrk")
+    cfl = 0.5 * tlw.trixi2lw(0.9, semi.solver)
+    u_ref, t_ref, _ = o.solve(u_lex, 0.0, 1e9, cfl, max_steps=K, mdrk=True)
+    lw_update = tlw.semidiscretize(semi, semi.time_discretization, (0.0, float("inf")))
+    lw_update.u0.copy_(to_dev(mesh, u_lex))
+    sol = tlw.solve_lwfr(lw_update, (), 1e-6, None, time_step_computation=tlw.CFLBased(cfl), max_steps=K)
+    assert sol.stats.naccept == K
+    assert abs(sol.t - t_ref) <= 1e-13 * t_ref
+    assert_close(to_lex(mesh, sol.u), u_ref, what=f"curved MDRK N={N} {nx}x{nx} after {K} steps")
     semi.close()
 
 
@@ -236,9 +289,8 @@ def test_curved_rejects_unsupported(oracle_mod):
     semi.close()
     eq = tlw.CompressibleEulerEquations2D(GAMMA)
     ic = tlw.IsentropicVortex()
-    # MDRK, a non-periodic mesh, and a mirrored mapping (negative Jacobian)
+    # a non-periodic mesh, and a mirrored mapping (negative Jacobian)
     cases = [
-        (tlw.StructuredMesh((4, 4), warp_mapping(20.0, 0.05)), tlw.MDRK(), _lib.TLW_ENOTSUP),
         (tlw.StructuredMesh((4, 4), warp_mapping(20.0, 0.05), periodicity=False), tlw.LW(), _lib.TLW_ENOTSUP),
         (tlw.StructuredMesh((4, 4), lambda xi, eta: (-xi, eta)), tlw.LW(), _lib.TLW_EINVAL),
     ]

@@ -99,3 +99,67 @@ def test_warped_mesh_vortex_convergence(oracle_mod, N, eoc_min):
         errs.append(np.sqrt(np.mean((u - ic(x, y, t))[..., 0] ** 2)))
     eoc = np.log(errs[1] / errs[2]) / np.log(2)
     assert eoc > eoc_min, (errs, eoc)
+
+
+@pytest.mark.parametrize("N", [3, 4])
+def test_affine_mapping_mdrk_equals_treemesh_structured_order(oracle_mod, N):
+    """Both MDRK stages of the curvilinear kernels (mdrk_kernel_1!/2! for
+    StructuredMesh) on an affine box equal the TreeMesh MDRK stages with the
+    Structured flux order; the stage cache (F2, U2) carries over."""
+    nx, ny, L = 6, 4, 20.0
+    h = L / nx
+    ic = vortex(L)
+    tree = oracle_mod.Oracle(N, nx, ny, 0.0, 0.0, h, flux_order=1)
+    curv = oracle_mod.Oracle(N, nx, ny, 0.0, 0.0, h, mapping=(oracle_mod.MAP_AFFINE, (0.0, 0.0, L, h * ny)))
+    x, y = tree.node_coords()
+    rng = np.random.default_rng(31 + N)
+    u = ic(x, y, 0.0) * (1 + 1e-2 * rng.standard_normal(x.shape + (4,)))
+    dt = 0.4 * tree.max_dt(u)
+    d1t, d1c = tree.rhs(u, 0.3, dt, kind="mdrk1"), curv.rhs(u, 0.3, dt, kind="mdrk1")
+    assert rel_linf(d1c, d1t) < 1e-12
+    assert rel_linf(curv.aux("U"), tree.aux("U")) < 1e-13 and rel_linf(curv.aux("F"), tree.aux("F")) < 1e-12
+    us = u + dt * d1t
+    tree.set_us(us)
+    curv.set_us(us)
+    d2t, d2c = tree.rhs(u, 0.3, dt, kind="mdrk2"), curv.rhs(u, 0.3, dt, kind="mdrk2")
+    assert rel_linf(d2c, d2t) < 1e-12
+    # the stages differ from LW and from each other
+    assert rel_linf(d1c, curv.rhs(u, 0.3, dt)) > 1e-3 and rel_linf(d2c, d1c) > 1e-4
+
+
+@pytest.mark.parametrize("N", [3, 4])
+def test_warped_mesh_mdrk_free_stream_and_conservation(oracle_mod, N):
+    o = warped(oracle_mod, N, 6)
+    x, y = o.node_coords()
+    c = np.zeros(x.shape + (4,))
+    c[...] = [1.1, 0.4, -0.3, 2.7]
+    dt = 0.3 * o.max_dt(c)
+    assert np.max(np.abs(o.rhs(c, 0.0, dt, kind="mdrk1"))) < 1e-12
+    o.set_us(c)
+    assert np.max(np.abs(o.rhs(c, 0.0, dt, kind="mdrk2"))) < 1e-12
+    rng = np.random.default_rng(9)
+    u = vortex()(x, y, 0.0) * (1 + 1e-3 * rng.standard_normal(x.shape + (4,)))
+    _, w, _, _ = o.basis()
+    W = np.outer(w, w)[None] / o.aux("inverse_jacobian").reshape(x.shape)
+    d1 = o.rhs(u, 0.0, dt, kind="mdrk1")
+    o.set_us(u + dt * d1)
+    d2 = o.rhs(u, 0.0, dt, kind="mdrk2")
+    for du in (d1, d2):
+        total = np.einsum("eji,ejiv->v", W, du)
+        scale = np.einsum("eji,ejiv->v", W, np.abs(du))
+        assert np.all(np.abs(total) <= 1e-13 * scale)
+
+
+def test_warped_mesh_mdrk_vortex_convergence(oracle_mod):
+    """MDRK (two-stage fourth order) at polydeg 3 on the warped mesh."""
+    ic = vortex()
+    errs = []
+    for nx in (16, 32, 64):
+        o = warped(oracle_mod, 3, nx)
+        x, y = o.node_coords()
+        cfl = 0.9 * 4 * 0.170 * 0.5
+        u, t, _ = o.solve(ic(x, y, 0.0), 0.0, 1.0, cfl, mdrk=True)
+        assert abs(t - 1.0) < 1e-12
+        errs.append(np.sqrt(np.mean((u - ic(x, y, t))[..., 0] ** 2)))
+    eoc = np.log(errs[1] / errs[2]) / np.log(2)
+    assert eoc > 3.4, (errs, eoc)

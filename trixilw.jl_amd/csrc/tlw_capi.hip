@@ -390,16 +390,21 @@ template <int N, bool SC, bool PAR = false> void launch_volume_sc(tlw_ctx* c, co
     else if (c->ebv == ch[1]) launch_volume_eb<N, ch[1], SC, PAR>(c, a, kind);
     else launch_volume_eb<N, ch[2], SC, PAR>(c, a, kind);
 }
-template <int N, int EB> void launch_volume_curved_eb(tlw_ctx* c, const KArgs& a) {
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_LW, false, false, true>), dim3(c->vgrid),
-                       dim3(VGeo<N, EB>::NT), 0, c->stream, a);
+template <int N, int EB> void launch_volume_curved_eb(tlw_ctx* c, const KArgs& a, int kind) {
+    dim3 g(c->vgrid), b(VGeo<N, EB>::NT);
+    if (kind == VOL_LW)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_LW, false, false, true>), g, b, 0, c->stream, a);
+    else if (kind == VOL_MDRK1)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_MDRK1, false, false, true>), g, b, 0, c->stream, a);
+    else
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_MDRK2, false, false, true>), g, b, 0, c->stream, a);
 }
 template <int N> void launch_volume_t(tlw_ctx* c, const KArgs& a, int kind) {
-    if (c->curved) {  // plain LW only (checked at creation)
+    if (c->curved) {  // plain LW or MDRK (adaptive steps are rejected on such a context)
         constexpr const int* ch = N == 4 ? kEbChoices4 : kEbChoices3;
-        if (c->ebv == ch[0]) launch_volume_curved_eb<N, ch[0]>(c, a);
-        else if (c->ebv == ch[1]) launch_volume_curved_eb<N, ch[1]>(c, a);
-        else launch_volume_curved_eb<N, ch[2]>(c, a);
+        if (c->ebv == ch[0]) launch_volume_curved_eb<N, ch[0]>(c, a, kind);
+        else if (c->ebv == ch[1]) launch_volume_curved_eb<N, ch[1]>(c, a, kind);
+        else launch_volume_curved_eb<N, ch[2]>(c, a, kind);
     } else if (c->parabolic) {
         launch_volume_sc<N, false, true>(c, a, kind);
     } else if (c->sc) {
@@ -1121,8 +1126,8 @@ int create_common(int ninst, const tlw_mesh_desc* meshes, const tlw_basis_desc* 
         h.visc_kappa = solver->gamma * (1.0 / (solver->gamma - 1.0)) / par->prandtl;
     }
     if (cv) {
-        // StructuredMesh: plain FR, LW, periodic (boundaries are rejected at upload)
-        if (c->sc || c->time_disc != TLW_LW || c->parabolic || ninst != 1) {
+        // StructuredMesh: plain FR, LW or MDRK, periodic (boundaries are rejected at upload)
+        if (c->sc || c->parabolic || ninst != 1) {
             delete c;
             return TLW_ENOTSUP;
         }

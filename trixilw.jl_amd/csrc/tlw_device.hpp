@@ -1474,6 +1474,83 @@ __device__ __forceinline__ void lwc_element(const double* uN, const double* J, d
     contract<n, S>(s2, sbase, i, j, sDh, vol);
 }
 
+// mdrk_kernel_1! for StructuredMesh (structured/multi_derivative_2d.jl:82-243):
+// F̃ = f/2 + f_t/8 (Cartesian, for the records), its contravariant image for
+// the D̂ contraction, Ũ = u/2 + ut/8, ut from the contravariant fluxes.
+template <int N, int EB>
+__device__ __forceinline__ void mdrkc1_element(const double* uN, const double* J, double invJ, double gm1, int i,
+                                               int j, bool act, int sidx, int sbase, double* s0, double* s2,
+                                               const double* sDm, const double* sDh, double* Uc, double* F,
+                                               double* G, double* vol) {
+    using G_ = VGeo<N, EB>;
+    constexpr int n = G_::n, S = G_::S;
+    double f[4], g[4], cx[4], cy[4];
+    euler_fluxes(uN, gm1, f, g);
+    to_contravariant(J, f, g, cx, cy);
+    if (act) put8<S>(s0, sidx, cx, cy);
+    __syncthreads();
+    double ut[4];
+    contract<n, S>(s0, sbase, i, j, sDm, ut);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) ut[v] *= invJ;
+    double dx[4], dy[4];
+    mdrk_ft(uN, ut, gm1, dx, dy);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        F[v] = fma(0.125, dx[v], 0.5 * f[v]);
+        G[v] = fma(0.125, dy[v], 0.5 * g[v]);
+        Uc[v] = fma(0.125, ut[v], 0.5 * uN[v]);
+    }
+    to_contravariant(J, F, G, cx, cy);
+    if (act) put8<S>(s2, sidx, cx, cy);
+    __syncthreads();
+    contract<n, S>(s2, sbase, i, j, sDh, vol);
+}
+
+// mdrk_kernel_2! for StructuredMesh (structured/multi_derivative_2d.jl:245-445):
+// F̃ = F2 + f_t(us)/3, Ũ = U2 + ust/3, with F2 = f + f_t/6 and U2 = u + ut/6
+// recomputed from uⁿ instead of being stored by stage 1.
+template <int N, int EB>
+__device__ __forceinline__ void mdrkc2_element(const double* uN, const double* usn, const double* J, double invJ,
+                                               double gm1, int i, int j, bool act, int sidx, int sbase, double* s0,
+                                               double* s1, double* s2, const double* sDm, const double* sDh,
+                                               double* Uc, double* F, double* G, double* vol) {
+    using G_ = VGeo<N, EB>;
+    constexpr int n = G_::n, S = G_::S;
+    double f[4], g[4], cx[4], cy[4];
+    euler_fluxes(uN, gm1, f, g);
+    to_contravariant(J, f, g, cx, cy);
+    if (act) put8<S>(s0, sidx, cx, cy);
+    euler_fluxes(usn, gm1, F, G);
+    to_contravariant(J, F, G, cx, cy);
+    if (act) put8<S>(s1, sidx, cx, cy);
+    __syncthreads();
+    double ut[4], ust[4];
+    contract<n, S>(s0, sbase, i, j, sDm, ut);
+    contract<n, S>(s1, sbase, i, j, sDm, ust);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) { ut[v] *= invJ; ust[v] *= invJ; }
+    double dx[4], dy[4];
+    mdrk_ft(uN, ut, gm1, dx, dy);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        F[v] = fma(1.0 / 6.0, dx[v], f[v]);    // F2
+        G[v] = fma(1.0 / 6.0, dy[v], g[v]);
+        Uc[v] = fma(1.0 / 6.0, ut[v], uN[v]);  // U2
+    }
+    mdrk_ft(usn, ust, gm1, dx, dy);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        F[v] = fma(1.0 / 3.0, dx[v], F[v]);
+        G[v] = fma(1.0 / 3.0, dy[v], G[v]);
+        Uc[v] = fma(1.0 / 3.0, ust[v], Uc[v]);
+    }
+    to_contravariant(J, F, G, cx, cy);
+    if (act) put8<S>(s2, sidx, cx, cy);
+    __syncthreads();
+    contract<n, S>(s2, sbase, i, j, sDh, vol);
+}
+
 // Curvilinear CFL speeds of a node (Trixi max_dt for curvilinear meshes): the
 // speeds transformed to the reference element, |Ja^k · (λ1, λ2)| |J⁻¹|.
 __device__ __forceinline__ void curved_speeds(const double* u, const double* J, double invJ, double gamma, double& m1,
@@ -1505,7 +1582,7 @@ template <bool TWO> constexpr int kVolWaves = TWO ? 2 : TLW_VOL_WAVES;
 template <int N, int EB, int KIND, bool SC, bool PAR = false, bool CURV = false>
 __global__ __launch_bounds__((VGeo<N, EB>::NT), kVolWaves<SC || PAR>) void k_volume(KArgs a) {
     static_assert(!(SC && PAR), "shock capturing has no hyperbolic-parabolic form");
-    static_assert(!CURV || (KIND == VOL_LW && !SC && !PAR), "the curvilinear path is plain LW");
+    static_assert(!CURV || (KIND != VOL_LW_ADAPT && !SC && !PAR), "the curvilinear path is plain LW or MDRK");
     using G_ = VGeo<N, EB>;
     constexpr int n = G_::n, NN = G_::NN, S = G_::S, NT = G_::NT;
     __shared__ double lds[G_::LDS];
@@ -1546,7 +1623,16 @@ __global__ __launch_bounds__((VGeo<N, EB>::NT), kVolWaves<SC || PAR>) void k_vol
     if constexpr (CURV) {
         double Jn[4];  // Ja11, Ja12, Ja21, Ja22 of the node: same layout as the state
         load4(a.Ja + off, Jn);
-        lwc_element<N, EB>(uN, Jn, invJ, gm1, i, j, act, sidx, sbase, s0, s1, s2, sDm, sDh, Uc, F, G, vol);
+        if constexpr (KIND == VOL_LW) {
+            lwc_element<N, EB>(uN, Jn, invJ, gm1, i, j, act, sidx, sbase, s0, s1, s2, sDm, sDh, Uc, F, G, vol);
+        } else if constexpr (KIND == VOL_MDRK1) {
+            mdrkc1_element<N, EB>(uN, Jn, invJ, gm1, i, j, act, sidx, sbase, s0, s2, sDm, sDh, Uc, F, G, vol);
+        } else {
+            double usn[4];
+            load4(a.us + off, usn);
+            mdrkc2_element<N, EB>(uN, usn, Jn, invJ, gm1, i, j, act, sidx, sbase, s0, s1, s2, sDm, sDh, Uc, F, G,
+                                  vol);
+        }
     } else if constexpr (PAR) {
         // BR1 gradients of uⁿ (and of us in MDRK stage 2), exchanged through a
         // buffer the element procedure writes only after its first barrier
