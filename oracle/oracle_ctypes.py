@@ -51,6 +51,7 @@ class _Desc(C.Structure):
         ("curved", C.c_int),
         ("map_kind", C.c_int),
         ("map_par", C.c_double * 4),
+        ("reconstruction", C.c_int),
     ]
 
 
@@ -126,7 +127,7 @@ class Oracle:
     def __init__(self, polydeg, nx, ny, xmin, ymin, h, periodic=(True, True), equation="euler", gamma=1.4,
                  adv=(1.0, 1.0), flux_order=0, bc_kind=BV_NONE, bc_par=(), tolerances=(0.0, 0.0), nthreads=0,
                  shock_capturing=False, alpha_max=0.5, alpha_min=0.001, alpha_smooth=True, positivity=None,
-                 parabolic=None, mu=0.0, prandtl=0.72, mapping=None):
+                 parabolic=None, mu=0.0, prandtl=0.72, mapping=None, reconstruction="first_order"):
         """``positivity``: (threshold_density, threshold_pressure) of a Zhang-Shu
         limiter that ``solve`` applies as a stage limiter, or None.
         ``parabolic``: None, "laplace" (LaplaceDiffusion2D with diffusivity ``mu``,
@@ -135,7 +136,9 @@ class Oracle:
         gradient variables); periodic meshes only.
         ``mapping``: None (TreeMesh) or (MAP_AFFINE, (x0, y0, Lx, Ly)) /
         (MAP_WARP, (L, A)) for a periodic curvilinear StructuredMesh of nx × ny
-        cells (xmin, ymin, h are then unused)."""
+        cells (xmin, ymin, h are then unused).
+        ``reconstruction``: "first_order" or "muscl_hancock" subcell FV of the
+        shock-capturing blend (Euler)."""
         d = _Desc()
         d.polydeg, d.nx, d.ny = polydeg, nx, ny
         d.xmin, d.ymin, d.h = xmin, ymin, h
@@ -152,6 +155,7 @@ class Oracle:
         d.shock_capturing = int(shock_capturing)
         d.alpha_max, d.alpha_min = alpha_max, alpha_min
         d.alpha_smooth = int(alpha_smooth)
+        d.reconstruction = {"first_order": 0, "muscl_hancock": 1}[reconstruction]
         if positivity is not None:
             d.positivity = 1
             d.pos_threshold[0], d.pos_threshold[1] = positivity
@@ -165,7 +169,7 @@ class Oracle:
         self._desc = d
         self._h = lib().tlwo_create(C.byref(d))
         if not self._h:
-            raise ValueError("oracle supports polydeg 3 and 4")
+            raise ValueError("unsupported oracle configuration (polydeg 3 and 4; see tlwo_create)")
         self.n = polydeg + 1
         self.nv = 4 if equation == "euler" else 1
         self.E = nx * ny

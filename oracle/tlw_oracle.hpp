@@ -416,6 +416,7 @@ template <class T, class Eq> struct Solver {
     // shock capturing (VolumeIntegralFRShockCapturing with IndicatorHennemannGassner
     // on density_pressure and first-order subcell FV, src/solvers/fr.jl:63-86)
     bool sc = false;
+    bool muscl_hancock = false;  // MUSCLHancockReconstruction instead of FirstOrderReconstruction
     double alpha_max = 0.5, alpha_min = 0.001;
     bool alpha_smooth = true;
     std::vector<double> alpha, alpha_tmp, alpha_override;
@@ -1012,6 +1013,215 @@ template <class T, class Eq> struct Solver {
             }
     }
 
+    // ---------------- MUSCL-Hancock subcell reconstruction ----------------
+    // fv_kernel! with MUSCLHancockReconstruction (dg_2d.jl:284-324) calls
+    // calcflux_mh!, which the reference defines for curvilinear meshes only
+    // (structured/dg_2d.jl:313-538).  Restated here for the TreeMesh with its
+    // Cartesian metric: contravariant vectors (1, 0) and (0, 1) (every subcell
+    // normal Ja + Σ w D Ja is then the same unit vector) and the element's
+    // inverse Jacobian 2/h at every node.  Kept as the reference computes it:
+    //   - subcell geometry of compute_subcells (dg_2d.jl:1806-1824): faces
+    //     xs[0] = -1, xs[i] = -1 + w_1 + ... + w_i, extended nodes ξe[0] = -1,
+    //     ξe[1..n] = GLL nodes, ξe[n+1] = 1;
+    //   - uext copies the element's edge values outward, so at the end subcells
+    //     (ξe[0] = ξe[1] = -1) the one-sided differences are 0/0 = NaN and minmod
+    //     returns 0 (Julia: sign(NaN) is NaN and NaN != NaN);
+    //   - minmod(back, centred, forward, β = 2 - α, 0) (structured/dg_2d.jl:75-95);
+    //   - limit_slope on density then pressure (compressible_euler_2d.jl:9-21,
+    //     equations.jl:262-287), which scales u* by θ a second time;
+    //   - the half-step (structured/dg_2d.jl:396-475): subcell k (1-based) gets
+    //     -Δt/2 J⁻¹ f(its right face value) / w_{k+1} for k < n and
+    //     +Δt/2 J⁻¹ f(its left face value) / w_{k+1} for 1 < k < n, in both
+    //     directions and added to all four face values; the end subcells get no
+    //     other term;
+    //   - Rusanov fluxes between the evolved face values at the interior subcell
+    //     faces, zero at the element faces;
+    //   - elements with a face on the domain boundary use first-order FV
+    //     (structured/dg_2d.jl:319-327).
+    // Euler only (limit_slope has no other method in the reference).
+    static T sign_julia(const T& x) {
+        const double v = value(x);
+        if (v != v) return x;  // NaN
+        return v > 0 ? T(1.0) : (v < 0 ? T(-1.0) : T(0.0));
+    }
+    static bool isnan_(const T& x) { const double v = value(x); return v != v; }
+    static T min3_julia(const T& a, const T& b, const T& c) {
+        if (isnan_(a)) return a;
+        if (isnan_(b)) return b;
+        if (isnan_(c)) return c;
+        T m = value(b) < value(a) ? b : a;
+        return value(c) < value(m) ? c : m;
+    }
+    static T minmod_mh(const T& a, const T& b, const T& c, double beta) {
+        const T slope = min3_julia(beta * abs_(a), abs_(b), beta * abs_(c));
+        const T s1 = sign_julia(a), s2 = sign_julia(b), s3 = sign_julia(c);
+        const bool ne12 = isnan_(s1) || isnan_(s2) || value(s1) != value(s2);
+        const bool ne23 = isnan_(s2) || isnan_(s3) || value(s2) != value(s3);
+        if (ne12 || ne23) return T(0.0);
+        return s1 * slope;
+    }
+    // limit_variable_slope (equations.jl:262-287) for var = density (0) or pressure (1)
+    void limit_variable_slope(int var, T* slope, T* us_ll, T* us_rr, const T* ue, double xl, double xr) const {
+        auto vf = [&](const T* q) { return var == 0 ? eq.density(q) : eq.pressure(q); };
+        const T v_ll = vf(us_ll), v_rr = vf(us_rr), v_low = vf(ue);
+        const double eps = 1e-10;
+        const T threshold = 0.1 * v_low;
+        if (value(v_ll) < eps || value(v_rr) < eps) {
+            const T r_ll = abs_(threshold - v_low) / (abs_(v_ll - v_low) + 1e-13);
+            const T r_rr = abs_(threshold - v_low) / (abs_(v_rr - v_low) + 1e-13);
+            T theta = value(r_rr) < value(r_ll) ? r_rr : r_ll;
+            if (!(value(theta) < 1.0)) theta = T(1.0);
+            for (int v = 0; v < NV; ++v) {
+                slope[v] = slope[v] * theta;
+                us_ll[v] = ue[v] + (2.0 * theta) * (xl * slope[v]);
+                us_rr[v] = ue[v] + (2.0 * theta) * (xr * slope[v]);
+            }
+        }
+    }
+    // limit_slope (compressible_euler_2d.jl:9-21): face values ue + slope·x of the limited slope
+    void limit_slope(T* slope, T* ufl, T* us_ll, T* ufr, T* us_rr, const T* ue, double xl, double xr) const {
+        limit_variable_slope(0, slope, us_ll, us_rr, ue, xl, xr);
+        limit_variable_slope(1, slope, us_ll, us_rr, ue, xl, xr);
+        for (int v = 0; v < NV; ++v) {
+            ufl[v] = ue[v] + slope[v] * xl;
+            ufr[v] = ue[v] + slope[v] * xr;
+        }
+    }
+    bool boundary_adjacent(int e) const {
+        for (int f = 0; f < 4; ++f)
+            if (mesh.nbr[4 * e + f] < 0) return true;
+        return false;
+    }
+    // fv_kernel! with MUSCL-Hancock: the subcell fluxes, α-scaled FV term and fn_low
+    void fv_kernel_mh(T* du, const T* u, int e, double fv_scale, double dtmh) {
+        if (boundary_adjacent(e)) {
+            fv_kernel_fo(du, u, e, fv_scale, true);
+            return;
+        }
+        const auto& w = basis.weights;
+        double xs[MAXN + 1], xe[MAXN + 2];
+        xs[0] = -1.0;
+        for (int i = 1; i <= n; ++i) {
+            double acc = w[0];
+            for (int k = 1; k < i; ++k) acc += w[k];  // sum(weights[1:i]), left to right
+            xs[i] = -1.0 + acc;
+        }
+        xe[0] = -1.0;
+        for (int i = 1; i <= n; ++i) xe[i] = basis.nodes[i - 1];
+        xe[n + 1] = 1.0;
+        const double ij = mesh.inv_jac[e];
+        // uext[j][i] with 1-based subcells and the copies at 0 and n+1
+        T uext[MAXN + 2][MAXN + 2][NV];
+        for (int j = 1; j <= n; ++j)
+            for (int i = 1; i <= n; ++i)
+                for (int v = 0; v < NV; ++v) uext[j][i][v] = u[idx(v, i - 1, j - 1, e)];
+        for (int k = 1; k <= n; ++k)
+            for (int v = 0; v < NV; ++v) {
+                uext[k][0][v] = uext[k][1][v];
+                uext[k][n + 1][v] = uext[k][n][v];
+                uext[0][k][v] = uext[1][k][v];
+                uext[n + 1][k][v] = uext[n][k][v];
+            }
+        T uf[MAXN + 1][MAXN + 1][4][NV], unph[MAXN + 1][MAXN + 1][4][NV];
+        const double beta = 2.0 - fv_scale;
+        for (int j = 1; j <= n; ++j)
+            for (int i = 1; i <= n; ++i) {
+                const T* u_ = uext[j][i];
+                const T *ul = uext[j][i - 1], *ur = uext[j][i + 1], *ud = uext[j - 1][i], *uu = uext[j + 1][i];
+                const double dx1 = xe[i] - xe[i - 1], dx2 = xe[i + 1] - xe[i];
+                const double dy1 = xe[j] - xe[j - 1], dy2 = xe[j + 1] - xe[j];
+                T sx[NV], sy[NV];
+                for (int v = 0; v < NV; ++v) {
+                    // finite_differences (structured/dg_2d.jl:75-81)
+                    {
+                        const double a = -(dx2 / (dx1 * (dx1 + dx2))), b = (dx2 - dx1) / (dx1 * dx2),
+                                     c = (dx1 / (dx2 * (dx1 + dx2)));
+                        const T back = (u_[v] - ul[v]) / dx1, fwd = (ur[v] - u_[v]) / dx2;
+                        const T cent = a * ul[v] + b * u_[v] + c * ur[v];
+                        sx[v] = minmod_mh(back, cent, fwd, beta);
+                    }
+                    {
+                        const double a = -(dy2 / (dy1 * (dy1 + dy2))), b = (dy2 - dy1) / (dy1 * dy2),
+                                     c = (dy1 / (dy2 * (dy1 + dy2)));
+                        const T back = (u_[v] - ud[v]) / dy1, fwd = (uu[v] - u_[v]) / dy2;
+                        const T cent = a * ud[v] + b * u_[v] + c * uu[v];
+                        sy[v] = minmod_mh(back, cent, fwd, beta);
+                    }
+                }
+                const double xl = xs[i - 1] - xe[i], xr = xs[i] - xe[i];
+                const double yd = xs[j - 1] - xe[j], yu = xs[j] - xe[j];
+                T us_l[NV], us_r[NV], us_d[NV], us_u[NV];
+                for (int v = 0; v < NV; ++v) {
+                    uf[j][i][0][v] = u_[v] + sx[v] * xl;
+                    uf[j][i][1][v] = u_[v] + sx[v] * xr;
+                    uf[j][i][2][v] = u_[v] + sy[v] * yd;
+                    uf[j][i][3][v] = u_[v] + sy[v] * yu;
+                    us_l[v] = u_[v] + 2.0 * sx[v] * xl;
+                    us_r[v] = u_[v] + 2.0 * sx[v] * xr;
+                    us_d[v] = u_[v] + 2.0 * sy[v] * yd;
+                    us_u[v] = u_[v] + 2.0 * sy[v] * yu;
+                }
+                limit_slope(sx, uf[j][i][0], us_l, uf[j][i][1], us_r, u_, xl, xr);
+                limit_slope(sy, uf[j][i][2], us_d, uf[j][i][3], us_u, u_, yd, yu);
+                for (int q = 0; q < 4; ++q)
+                    for (int v = 0; v < NV; ++v) unph[j][i][q][v] = uf[j][i][q][v];
+            }
+        auto add_all = [&](int i, int j, const T* r) {
+            for (int q = 0; q < 4; ++q)
+                for (int v = 0; v < NV; ++v) unph[j][i][q][v] = unph[j][i][q][v] + r[v];
+        };
+        T f[NV], r[NV];
+        for (int j = 1; j <= n; ++j)
+            for (int i = 2; i <= n; ++i) {
+                const double dxi = xs[i] - xs[i - 1];
+                eq.flux(uf[j][i - 1][1], 1, f);
+                for (int v = 0; v < NV; ++v) r[v] = -0.5 * dtmh * ij / dxi * f[v];
+                add_all(i - 1, j, r);
+                if (i < n) {
+                    eq.flux(uf[j][i][0], 1, f);
+                    const double dxip1 = xs[i + 1] - xs[i];
+                    for (int v = 0; v < NV; ++v) r[v] = 0.5 * dtmh * ij / dxip1 * f[v];
+                    add_all(i, j, r);
+                }
+            }
+        for (int i = 1; i <= n; ++i)
+            for (int j = 2; j <= n; ++j) {
+                const double dyi = xs[j] - xs[j - 1];
+                eq.flux(uf[j - 1][i][3], 2, f);
+                for (int v = 0; v < NV; ++v) r[v] = -0.5 * dtmh * ij / dyi * f[v];
+                add_all(i, j - 1, r);
+                if (j < n) {
+                    eq.flux(uf[j][i][2], 2, f);
+                    const double dyip1 = xs[j + 1] - xs[j];
+                    for (int v = 0; v < NV; ++v) r[v] = 0.5 * dtmh * ij / dyip1 * f[v];
+                    add_all(i, j, r);
+                }
+            }
+        // subcell fluxes between evolved face values (0-based node indices as in fv_kernel_fo)
+        T fx[MAXN][MAXN + 1][NV], gy[MAXN + 1][MAXN][NV];
+        for (int j = 0; j < n; ++j)
+            for (int v = 0; v < NV; ++v) { fx[j][0][v] = T(0); fx[j][n][v] = T(0); }
+        for (int i = 0; i < n; ++i)
+            for (int v = 0; v < NV; ++v) { gy[0][i][v] = T(0); gy[n][i][v] = T(0); }
+        for (int j = 1; j <= n; ++j)
+            for (int i = 2; i <= n; ++i) rusanov(unph[j][i - 1][1], unph[j][i][0], 1, fx[j - 1][i - 1]);
+        for (int i = 1; i <= n; ++i)
+            for (int j = 2; j <= n; ++j) rusanov(unph[j - 1][i][3], unph[j][i][2], 2, gy[j - 1][i - 1]);
+        const auto& iw = basis.inv_w;
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i)
+                for (int v = 0; v < NV; ++v)
+                    du[idx(v, i, j, e)] = du[idx(v, i, j, e)] + fv_scale * (iw[i] * (fx[j][i + 1][v] - fx[j][i][v]) +
+                                                                             iw[j] * (gy[j + 1][i][v] - gy[j][i][v]));
+        for (int k = 0; k < n; ++k)
+            for (int v = 0; v < NV; ++v) {
+                fn_low[sidx(v, k, 0, e)] = fx[k][1][v];
+                fn_low[sidx(v, k, 1, e)] = fx[k][n - 1][v];
+                fn_low[sidx(v, k, 2, e)] = gy[1][k][v];
+                fn_low[sidx(v, k, 3, e)] = gy[n - 1][k][v];
+            }
+    }
+
     // compute_alp for VolumeIntegralFRShockCapturing on TreeMesh (dg_2d.jl:1879-1901):
     // average of the two blending factors, reset to 1 when the low-order update
     // of either adjacent node with the blended flux is not admissible.  J is the
@@ -1168,11 +1378,7 @@ template <class T, class Eq> struct Solver {
             const double a_e = sc ? alpha[e] : 0.0;
             if (basis.N == 4) lw_volume_kernel_4(du, u, dt, e, 1.0 - a_e);
             else lw_volume_kernel_3(du, u, dt, e, 1.0 - a_e);
-            if (sc) {
-                // pure_and_blended_element_ids!: alpha ≈ 0 (atol 1e-12) is pure DG
-                const bool dg_only = std::fabs(a_e) <= 1e-12;
-                fv_kernel_fo(du, u, e, a_e, !dg_only);
-            }
+            if (sc) sc_element(du, u, e, dt);
         }
         calc_interface_flux(u);
         if (!mesh.bd_elem.empty()) calc_boundary_flux(u, t, dt, 1.0);
@@ -1187,9 +1393,15 @@ template <class T, class Eq> struct Solver {
         else indicator_hg(u);
         sc_dt = dt;
     }
-    void sc_element(T* du, const T* u, int e) {
+    // pure_and_blended_element_ids!: alpha ≈ 0 (atol 1e-12) is pure DG (calc_fn_low!
+    // only); a blended element adds fv_kernel! of the reconstruction, whose
+    // MUSCL-Hancock half-step uses dtmh (0.5 dt in MDRK stage 1,
+    // tree/multi_derivative_2d.jl:199)
+    void sc_element(T* du, const T* u, int e, double dtmh) {
         const double a_e = alpha[e];
-        fv_kernel_fo(du, u, e, a_e, !(std::fabs(a_e) <= 1e-12));
+        const bool dg_only = std::fabs(a_e) <= 1e-12;
+        if (!dg_only && muscl_hancock) fv_kernel_mh(du, u, e, a_e, dtmh);
+        else fv_kernel_fo(du, u, e, a_e, !dg_only);
     }
 
     // calc_volume_integral_mdrk1!/2! with shock capturing (tree/multi_derivative_2d.jl:151-261):
@@ -1201,7 +1413,7 @@ template <class T, class Eq> struct Solver {
 #pragma omp parallel for schedule(static)
         for (int e = 0; e < mesh.E; ++e) {
             mdrk_kernel_1(du, u, dt, e, sc ? 1.0 - alpha[e] : 1.0);
-            if (sc) sc_element(du, u, e);
+            if (sc) sc_element(du, u, e, 0.5 * dt);
         }
         calc_interface_flux(u);
         if (!mesh.bd_elem.empty()) calc_boundary_flux(u, t, dt, 0.5);
@@ -1214,7 +1426,7 @@ template <class T, class Eq> struct Solver {
 #pragma omp parallel for schedule(static)
         for (int e = 0; e < mesh.E; ++e) {
             mdrk_kernel_2(du, dt, e, sc ? 1.0 - alpha[e] : 1.0);
-            if (sc) sc_element(du, u, e);
+            if (sc) sc_element(du, u, e, dt);
         }
         calc_interface_flux(u);
         if (!mesh.bd_elem.empty()) calc_boundary_flux(u, t, dt, 1.0);

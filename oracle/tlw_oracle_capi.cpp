@@ -41,6 +41,9 @@ struct tlwo_desc {
     int curved;
     int map_kind;
     double map_par[4];
+    // shock-capturing reconstruction: 0 FirstOrderReconstruction,
+    // 1 MUSCLHancockReconstruction (Euler only)
+    int reconstruction;
 };
 
 struct tlwo_ctx {
@@ -61,6 +64,7 @@ template <class S> static void configure(S& s, const tlwo_desc* d) {
     s.alpha_max = d->alpha_max;
     s.alpha_min = d->alpha_min;
     s.alpha_smooth = d->alpha_smooth != 0;
+    s.muscl_hancock = d->reconstruction == 1;
     s.positivity = d->positivity != 0;
     s.pos_threshold[0] = d->pos_threshold[0];
     s.pos_threshold[1] = d->pos_threshold[1];
@@ -77,6 +81,7 @@ void* tlwo_create(const tlwo_desc* d) {
         if ((d->parabolic == 1) != (d->equation == 1)) return nullptr;
     }
     if (d->curved && (d->parabolic || d->shock_capturing || !d->periodic_x || !d->periodic_y)) return nullptr;
+    if (d->reconstruction == 1 && (d->equation != 0 || !d->shock_capturing)) return nullptr;
     auto* c = new tlwo_ctx;
     c->equation = d->equation;
     if (d->curved) {
