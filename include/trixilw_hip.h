@@ -58,6 +58,9 @@ enum { TLW_BV_NONE = 0, TLW_BV_VORTEX = 1, TLW_BV_CONSTANT = 2, TLW_BV_RIEMANN4 
  * and the shock-capturing blending factors, indicator.cache.alpha) */
 enum { TLW_AUX_U = 0, TLW_AUX_F = 1, TLW_AUX_ALPHA = 2 };
 
+/* shock-capturing subcell reconstructions */
+enum { TLW_RECON_FIRST_ORDER = 0, TLW_RECON_MUSCL_HANCOCK = 1 };
+
 /* solver flags */
 enum { TLW_FLAG_KEEP_AUX = 1 /* store full Ũ [v,i,j,e] and F̃ [v,d,i,j,e] on every rhs */ };
 
@@ -94,6 +97,11 @@ typedef struct {
     double alpha_max, alpha_min;
     int32_t alpha_smooth;
     const double* inverse_vandermonde_host; /* [n*n] Trixi basis.inverse_vandermonde_legendre, column-major */
+    /* subcell FV reconstruction of the blend: TLW_RECON_FIRST_ORDER (FirstOrderReconstruction,
+     * dg_2d.jl:251-282) or TLW_RECON_MUSCL_HANCOCK (MUSCLHancockReconstruction, dg_2d.jl:284-324 with
+     * calcflux_mh! of src/solvers/dgsem_structured/dg_2d.jl:313-538 on the TreeMesh's Cartesian
+     * metric; elements with a face on the domain boundary use first order, as there) */
+    int32_t reconstruction;
 } tlw_solver_desc;
 
 /* Parabolic terms of SemidiscretizationHyperbolicParabolic

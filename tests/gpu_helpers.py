@@ -16,9 +16,10 @@ def torch_mod():
 
 def setup(oracle_mod, N, level, td="lw", periodic=True, domain=(-10.0, 10.0), bc=None, flux_order="tree",
           ordering="lexicographic", keep_aux=True, seed=0, amp=1e-3, ic=None, sc=False, alpha_max=0.5,
-          alpha_smooth=True):
+          alpha_smooth=True, recon="first_order"):
     """``sc`` selects VolumeIntegralFRShockCapturing (Hennemann-Gassner on
-    density_pressure, first-order subcell FV) on both sides."""
+    density_pressure) on both sides, with first-order subcell FV or, with
+    ``recon="muscl_hancock"``, the MUSCL-Hancock reconstruction."""
     import trixilw_hip as tlw
 
     eq = tlw.CompressibleEulerEquations2D(1.4)
@@ -27,7 +28,8 @@ def setup(oracle_mod, N, level, td="lw", periodic=True, domain=(-10.0, 10.0), bc
     vi = tlw.VolumeIntegralFR(tlw.MDRK() if td == "mdrk" else tlw.LW())
     if sc:
         ind = tlw.IndicatorHennemannGassner(eq, None, alpha_max=alpha_max, alpha_min=0.001, alpha_smooth=alpha_smooth)
-        vi = tlw.VolumeIntegralFRShockCapturing(ind, volume_integralFR=vi)
+        rec = tlw.MUSCLHancockReconstruction() if recon == "muscl_hancock" else tlw.FirstOrderReconstruction()
+        vi = tlw.VolumeIntegralFRShockCapturing(ind, volume_integralFR=vi, reconstruction=rec)
     solver = tlw.DGSEM(N, volume_integral=vi)
     semi = tlw.SemidiscretizationHyperbolic(mesh, tlw.get_time_discretization(solver), eq, ic, solver,
                                             boundary_conditions=bc, surface_flux_order=flux_order,
@@ -36,7 +38,8 @@ def setup(oracle_mod, N, level, td="lw", periodic=True, domain=(-10.0, 10.0), bc
     o = oracle_mod.Oracle(N, nx, nx, domain[0], domain[0], (domain[1] - domain[0]) / nx, periodic=(periodic,) * 2,
                           flux_order=0 if flux_order == "tree" else 1,
                           bc_kind=bc.kind if bc is not None else 0, bc_par=bc.params if bc is not None else (),
-                          shock_capturing=sc, alpha_max=alpha_max, alpha_min=0.001, alpha_smooth=alpha_smooth)
+                          shock_capturing=sc, alpha_max=alpha_max, alpha_min=0.001, alpha_smooth=alpha_smooth,
+                          reconstruction=recon if sc else "first_order")
     x, y = o.node_coords()
     rng = np.random.default_rng(seed)
     u_lex = ic(x, y, 0.0) * (1.0 + amp * rng.standard_normal(x.shape + (4,)))

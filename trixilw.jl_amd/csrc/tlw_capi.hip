@@ -1109,6 +1109,40 @@ int create_common(int ninst, const tlw_mesh_desc* meshes, const tlw_basis_desc* 
         h.alpha_max = solver->alpha_max;
         h.alpha_min = solver->alpha_min;
         h.alpha_smooth = solver->alpha_smooth;
+        if (solver->reconstruction != TLW_RECON_FIRST_ORDER && solver->reconstruction != TLW_RECON_MUSCL_HANCOCK) {
+            delete c;
+            return TLW_EINVAL;
+        }
+        h.recon = solver->reconstruction;
+        if (h.recon == TLW_RECON_MUSCL_HANCOCK) {
+            // compute_subcells (dg_2d.jl:1806-1824): faces xs[0] = -1, xs[i] = -1 + w_1 + ... + w_i
+            // (summed left to right), extended nodes xe[0] = -1, xe[1..n] = nodes, xe[n+1] = 1
+            double xs[9], xe[10];
+            xs[0] = -1.0;
+            for (int i = 1; i <= n; ++i) {
+                double acc = h.weights[0];
+                for (int k = 1; k < i; ++k) acc += h.weights[k];
+                xs[i] = -1.0 + acc;
+            }
+            xe[0] = -1.0;
+            for (int i = 1; i <= n; ++i) xe[i] = h.nodes[i - 1];
+            xe[n + 1] = 1.0;
+            for (int i = 0; i < n; ++i) {
+                const int I = i + 1;  // 1-based subcell of node i
+                const double d1 = xe[I] - xe[I - 1], d2 = xe[I + 1] - xe[I];
+                h.mh_dx1[i] = d1;
+                h.mh_dx2[i] = d2;
+                // finite_differences (structured/dg_2d.jl:75-81); the end nodes (d1 or d2 = 0)
+                // take zero slopes and never read these
+                const bool inner = d1 > 0.0 && d2 > 0.0;
+                h.mh_fa[i] = inner ? -(d2 / (d1 * (d1 + d2))) : 0.0;
+                h.mh_fb[i] = inner ? (d2 - d1) / (d1 * d2) : 0.0;
+                h.mh_fc[i] = inner ? (d1 / (d2 * (d1 + d2))) : 0.0;
+                h.mh_xl[i] = xs[I - 1] - xe[I];
+                h.mh_xr[i] = xs[I] - xe[I];
+                h.mh_hw[i] = (i + 2 <= n) ? xs[i + 2] - xs[i + 1] : 1.0;
+            }
+        }
     }
     if (par) {
         // CompressibleNavierStokesDiffusion2D(equations; mu, Prandtl,

@@ -72,6 +72,12 @@ struct DevConst {
     // CompressibleNavierStokesDiffusion2D (conservative gradient variables):
     // dynamic viscosity, Prandtl number, κ = γ/((γ-1) Pr)
     double visc_mu, visc_Pr, visc_kappa;
+    // MUSCL-Hancock subcell geometry per node i (compute_subcells): spacings
+    // to the neighbouring nodes, the centred-difference weights, the distances
+    // from the node to its subcell faces, and hw[k] = xs[k+2] - xs[k+1], the
+    // width the half-step divides by for subcell k
+    int recon;  // TLW_RECON_*
+    double mh_dx1[8], mh_dx2[8], mh_fa[8], mh_fb[8], mh_fc[8], mh_xl[8], mh_xr[8], mh_hw[8];
 };
 
 // Device-resident integrator state (LWIntegrator fields t, dt, iter and the
@@ -379,6 +385,117 @@ __device__ __forceinline__ bool admissible(const double* u, double gamma) {
     const double v1 = u[1] / rho, v2 = u[2] / rho;
     const double p = (gamma - 1.0) * (u[3] - 0.5 * (u[1] * v1 + u[2] * v2));
     return rho > 0.0 && p > 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// MUSCL-Hancock subcell reconstruction of a blended element (calcflux_mh!,
+// src/solvers/dgsem_structured/dg_2d.jl:313-538, on the TreeMesh's Cartesian
+// metric: unit normals, the element's J⁻¹ at every node).
+
+// minmod(a, b, c, β, 0) (structured/dg_2d.jl:83-95) of finite arguments:
+// s·min(β|a|, |b|, β|c|) if a, b, c have one sign, else 0
+__device__ __forceinline__ double minmod3(double a, double b, double c, double beta) {
+    const double m = fmin(fmin(beta * fabs(a), fabs(b)), beta * fabs(c));
+    const double sa = (a > 0.0) ? 1.0 : ((a < 0.0) ? -1.0 : 0.0);
+    const double sb = (b > 0.0) ? 1.0 : ((b < 0.0) ? -1.0 : 0.0);
+    const double sc = (c > 0.0) ? 1.0 : ((c < 0.0) ? -1.0 : 0.0);
+    return (sa != sb || sb != sc) ? 0.0 : sa * m;
+}
+
+// limit_variable_slope (equations.jl:262-287) for density (var 0) or pressure
+// (var 1): when a doubled extrapolation u* drops below 1e-10, the slope is
+// scaled by θ = min(|0.1 v - v| / (|v* - v| + 1e-13), 1) and u* recomputed
+// with 2θ and the scaled slope.
+__device__ __forceinline__ void mh_limit_var(int var, double* s, double* usl, double* usr, const double* u, double xl,
+                                             double xr, double gm1) {
+    auto val = [&](const double* q) {
+        return var == 0 ? q[0] : gm1 * (q[3] - 0.5 * (q[1] * q[1] + q[2] * q[2]) / q[0]);
+    };
+    const double vll = val(usl), vrr = val(usr), vlow = val(u);
+    if (vll < 1e-10 || vrr < 1e-10) {
+        const double thr = 0.1 * vlow;
+        const double rll = fabs(thr - vlow) / (fabs(vll - vlow) + 1e-13);
+        const double rrr = fabs(thr - vlow) / (fabs(vrr - vlow) + 1e-13);
+        const double th = fmin(fmin(rll, rrr), 1.0);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            s[v] *= th;
+            usl[v] = u[v] + (2.0 * th) * (xl * s[v]);
+            usr[v] = u[v] + (2.0 * th) * (xr * s[v]);
+        }
+    }
+}
+
+// limit_slope (compressible_euler_2d.jl:9-21): density then pressure, face values u + s x
+__device__ __forceinline__ void mh_face_pair(double* s, const double* u, double xl, double xr, double gm1, double* qlo,
+                                             double* qhi) {
+    double usl[4], usr[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        usl[v] = u[v] + 2.0 * s[v] * xl;
+        usr[v] = u[v] + 2.0 * s[v] * xr;
+    }
+    mh_limit_var(0, s, usl, usr, u, xl, xr, gm1);
+    mh_limit_var(1, s, usl, usr, u, xl, xr, gm1);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        qlo[v] = u[v] + s[v] * xl;
+        qhi[v] = u[v] + s[v] * xr;
+    }
+}
+
+// Face values of node (i, j) after the Hancock half-step.  s1 holds the
+// element's uⁿ (variable-major, stride S).  End nodes (ξ = ±1) keep zero
+// slope: the reference's one-sided differences there are 0/0.  Half-step
+// terms (all added to the four face values, in the reference's order):
+// +Δt/2 J⁻¹ f(left) / hw[i] for 0 < i < n-1, -Δt/2 J⁻¹ f(right) / hw[i] for
+// i < n-1, then the same in y.
+template <int N, int S>
+__device__ __forceinline__ void mh_faces(const double* s1, int sidx, int i, int j, const double* u, const DevConst* c,
+                                         double beta, double dtmh, double invJ, double gm1, double* ql, double* qr,
+                                         double* qd, double* qu) {
+    constexpr int n = N + 1;
+    double sx[4] = {0.0, 0.0, 0.0, 0.0}, sy[4] = {0.0, 0.0, 0.0, 0.0};
+    if (i > 0 && i < n - 1) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const double ul = s1[v * S + sidx - 1], ur = s1[v * S + sidx + 1];
+            const double back = (u[v] - ul) / c->mh_dx1[i], fwd = (ur - u[v]) / c->mh_dx2[i];
+            const double cent = fma(c->mh_fc[i], ur, fma(c->mh_fa[i], ul, c->mh_fb[i] * u[v]));
+            sx[v] = minmod3(back, cent, fwd, beta);
+        }
+    }
+    if (j > 0 && j < n - 1) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const double ud = s1[v * S + sidx - n], uu = s1[v * S + sidx + n];
+            const double back = (u[v] - ud) / c->mh_dx1[j], fwd = (uu - u[v]) / c->mh_dx2[j];
+            const double cent = fma(c->mh_fc[j], uu, fma(c->mh_fa[j], ud, c->mh_fb[j] * u[v]));
+            sy[v] = minmod3(back, cent, fwd, beta);
+        }
+    }
+    mh_face_pair(sx, u, c->mh_xl[i], c->mh_xr[i], gm1, ql, qr);
+    mh_face_pair(sy, u, c->mh_xl[j], c->mh_xr[j], gm1, qd, qu);
+    double fl[4], fr[4], fd[4], fu[4];
+    euler_flux_dir(ql, gm1, 1, fl);
+    euler_flux_dir(qr, gm1, 1, fr);
+    euler_flux_dir(qd, gm1, 2, fd);
+    euler_flux_dir(qu, gm1, 2, fu);
+    const double hm = (-0.5 * dtmh) * invJ, hp = (0.5 * dtmh) * invJ;
+    auto add = [&](double coef, const double* f) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const double r = coef * f[v];
+            ql[v] += r;
+            qr[v] += r;
+            qd[v] += r;
+            qu[v] += r;
+        }
+    };
+    if (i > 0 && i < n - 1) add(hp / c->mh_hw[i], fl);
+    if (i < n - 1) add(hm / c->mh_hw[i], fr);
+    if (j > 0 && j < n - 1) add(hp / c->mh_hw[j], fd);
+    if (j < n - 1) add(hm / c->mh_hw[j], fu);
 }
 
 // Shock-capturing interface flux (dg_2d.jl:1879-1946): the LW flux blended
@@ -1696,38 +1813,52 @@ __global__ __launch_bounds__((VGeo<N, EB>::NT), kVolWaves<SC || PAR>) void k_vol
             }
         }
         if (act && node == 0 && a.alpha_out) a.alpha_out[e] = alpha_e;
-        // first-order subcell FV (fv_kernel!, dg_2d.jl:251-282; Trixi calcflux_fv!):
-        // Rusanov fluxes between neighbouring nodes, zero at the element faces
+        // subcell FV (fv_kernel!, dg_2d.jl:251-324): Rusanov fluxes between the
+        // face values of neighbouring subcells, zero at the element faces.  First
+        // order (Trixi calcflux_fv!): the face values are the node values.  A
+        // blended element away from the domain boundary with MUSCL-Hancock takes
+        // the reconstructed, half-step-evolved face values (mh_faces).
 #pragma unroll
         for (int v = 0; v < 4; ++v)
             if (act) s1[v * S + sidx] = uN[v];
         __syncthreads();
+        const bool blended = !(fabs(alpha_e) <= 1e-12);  // pure_and_blended_element_ids!
+        double ql[4], qr[4], qd[4], qu[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) ql[v] = qr[v] = qd[v] = qu[v] = uN[v];
+        if (c->recon == 1 && blended && a.nbr[4 * e] >= 0 && a.nbr[4 * e + 1] >= 0 && a.nbr[4 * e + 2] >= 0 &&
+            a.nbr[4 * e + 3] >= 0) {
+            const double dtmh = (KIND == VOL_MDRK1) ? 0.5 * dt : dt;  // tree/multi_derivative_2d.jl:199
+            mh_faces<N, S>(s1, sidx, i, j, uN, c, 2.0 - alpha_e, dtmh, invJ, gm1, ql, qr, qd, qu);
+        }
+        // the right and upper face values go to the neighbouring subcells
+        if (act) put8<S>(s0, sidx, qr, qu);
+        __syncthreads();
         double fxl[4] = {0, 0, 0, 0}, gyl[4] = {0, 0, 0, 0}, w[4];
         if (i > 0) {
 #pragma unroll
-            for (int v = 0; v < 4; ++v) w[v] = s1[v * S + sidx - 1];
-            rusanov(w, uN, gamma, 1, fxl);
+            for (int v = 0; v < 4; ++v) w[v] = s0[v * S + sidx - 1];
+            rusanov(w, ql, gamma, 1, fxl);
         }
         if (j > 0) {
 #pragma unroll
-            for (int v = 0; v < 4; ++v) w[v] = s1[v * S + sidx - n];
-            rusanov(w, uN, gamma, 2, gyl);
+            for (int v = 0; v < 4; ++v) w[v] = s0[(4 + v) * S + sidx - n];
+            rusanov(w, qd, gamma, 2, gyl);
         }
-        if (act) put8<S>(s0, sidx, fxl, gyl);
+        if (act) put8<S>(s2, sidx, fxl, gyl);
         __syncthreads();
         const double iwi = 1.0 / c->weights[i], iwj = 1.0 / c->weights[j];
-        const bool blended = !(fabs(alpha_e) <= 1e-12);  // pure_and_blended_element_ids!
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            const double fxr = (i < n - 1) ? s0[v * S + sidx + 1] : 0.0;
-            const double gyr = (j < n - 1) ? s0[(4 + v) * S + sidx + n] : 0.0;
+            const double fxr = (i < n - 1) ? s2[v * S + sidx + 1] : 0.0;
+            const double gyr = (j < n - 1) ? s2[(4 + v) * S + sidx + n] : 0.0;
             const double fv = iwi * (fxr - fxl[v]) + iwj * (gyr - gyl[v]);
             vol[v] = (1.0 - alpha_e) * vol[v] + (blended ? alpha_e * fv : 0.0);
             // fn_low: the subcell fluxes next to the element faces (load_fn_low!)
-            fnlx[v] = (fx == 0) ? s0[v * S + sidx + 1] : fxl[v];
-            fnly[v] = (fy == 2) ? s0[(4 + v) * S + sidx + n] : gyl[v];
+            fnlx[v] = (fx == 0) ? s2[v * S + sidx + 1] : fxl[v];
+            fnly[v] = (fy == 2) ? s2[(4 + v) * S + sidx + n] : gyl[v];
         }
-        __syncthreads();  // s0 and s1 are reused for the face records
+        __syncthreads();  // s0, s1 and s2 are reused for the face records and interface sums
     }
     // --- face records into LDS (s0, s1 are free), one per face the node lies on.
     // λ ingredients: |v1|, |v2| and c of the λ state, once per node.

@@ -32,6 +32,9 @@ EXPORTED = (
 )
 
 
+TLW_RECON_FIRST_ORDER, TLW_RECON_MUSCL_HANCOCK = 0, 1
+
+
 class MeshDesc(C.Structure):
     _fields_ = [
         ("n_elements", C.c_int32),
@@ -65,6 +68,7 @@ class SolverDesc(C.Structure):
         ("alpha_min", C.c_double),
         ("alpha_smooth", C.c_int32),
         ("inverse_vandermonde_host", C.POINTER(C.c_double)),
+        ("reconstruction", C.c_int32),
     ]
 
 

@@ -60,7 +60,12 @@ class FirstOrderReconstruction:
 
 
 class MUSCLHancockReconstruction:
-    """Not implemented on the device: the reference has no TreeMesh method of calcflux_mh!."""
+    """MUSCL-Hancock subcell reconstruction tag (src/solvers/fr.jl:58): minmod
+    slopes with β = 2 - α, positivity-scaled face values, a half-step with the
+    face fluxes and Rusanov fluxes between the evolved face values
+    (calcflux_mh!, src/solvers/dgsem_structured/dg_2d.jl:313-538, on the
+    TreeMesh's Cartesian metric).  Elements with a face on the domain
+    boundary use first-order subcell FV, as in the reference."""
 
 
 @dataclass
@@ -85,8 +90,8 @@ class VolumeIntegralFRShockCapturing:
         self.volume_integralFR = volume_integralFR if volume_integralFR is not None else VolumeIntegralFR(LW())
         self.volume_flux_fv = volume_flux_fv
         self.reconstruction = reconstruction if reconstruction is not None else FirstOrderReconstruction()
-        if not isinstance(self.reconstruction, FirstOrderReconstruction):
-            raise NotImplementedError("only FirstOrderReconstruction is implemented on the device")
+        if not isinstance(self.reconstruction, (FirstOrderReconstruction, MUSCLHancockReconstruction)):
+            raise NotImplementedError("FirstOrderReconstruction or MUSCLHancockReconstruction")
         if volume_flux_fv != flux_lax_friedrichs or indicator.variable != density_pressure:
             raise NotImplementedError("flux_lax_friedrichs subcell fluxes and density_pressure only")
 
@@ -193,6 +198,8 @@ class SemidiscretizationHyperbolic:
             sd.alpha_max, sd.alpha_min = ind.alpha_max, ind.alpha_min
             sd.alpha_smooth = int(bool(ind.alpha_smooth))
             sd.inverse_vandermonde_host = _dptr(self._vinv)
+            sd.reconstruction = (_lib.TLW_RECON_MUSCL_HANCOCK if isinstance(vi.reconstruction, MUSCLHancockReconstruction)
+                                 else _lib.TLW_RECON_FIRST_ORDER)
         self._solver_desc = sd
         self._keep = self._mesh_arrays(mesh)
         md = self._mesh_desc()
