@@ -38,6 +38,14 @@ WORKLOADS = {
     # without the reference's positivity limiter, on the CPU oracle as on the device)
     "lw_euler_riemann_sc_256x256_N4": dict(level=8, N=4, td="lw", problem="riemann_sc", cfl=0.3),
     "lw_euler_riemann_sc_1024x1024_N4": dict(level=10, N=4, td="lw", problem="riemann_sc", cfl=0.3),
+    # the same with MUSCLHancockReconstruction (the reconstruction config 1 names; elements on the
+    # domain boundary use first order, as in the reference), CFL 0.1: without a limiter the
+    # reference's MUSCL-Hancock blend loses admissibility by step 200 at CFL 0.3 and by step 300 at
+    # 0.2 (128² and 256², on the oracle as on the device); first order survives 500 steps at 0.3
+    "lw_euler_riemann_mh_256x256_N4": dict(level=8, N=4, td="lw", problem="riemann_sc", cfl=0.1,
+                                           recon="muscl_hancock"),
+    "lw_euler_riemann_mh_1024x1024_N4": dict(level=10, N=4, td="lw", problem="riemann_sc", cfl=0.1,
+                                             recon="muscl_hancock"),
     # compressible Navier-Stokes (conservative gradient variables, μ = 0.01, Pr = 0.72, the
     # parameters of the reference's elixir_navierstokes_convergence.jl) on a periodic [-1,1]^2
     # TreeMesh, smooth shear/density/pressure waves, CFL 0.5
@@ -214,8 +222,11 @@ def make_problem(wl):
     if problem == "riemann_sc":
         ic = tlw.FourQuadrantRiemann()
         lo, hi, bc, periodic = 0.0, 1.0, ic, False
-        vi = tlw.VolumeIntegralFRShockCapturing(tlw.IndicatorHennemannGassner(eq, None, alpha_max=0.5), vi)
-        okw = dict(bc_kind=bc.kind, bc_par=bc.params, shock_capturing=True, alpha_max=0.5)
+        recon = wl.get("recon", "first_order")
+        rec = tlw.MUSCLHancockReconstruction() if recon == "muscl_hancock" else tlw.FirstOrderReconstruction()
+        vi = tlw.VolumeIntegralFRShockCapturing(tlw.IndicatorHennemannGassner(eq, None, alpha_max=0.5), vi,
+                                                reconstruction=rec)
+        okw = dict(bc_kind=bc.kind, bc_par=bc.params, shock_capturing=True, alpha_max=0.5, reconstruction=recon)
     elif problem == "ns_shear":
         ic = ns_shear
         lo, hi, bc, periodic = -1.0, 1.0, None, True
@@ -324,6 +335,42 @@ def measure_navier_stokes(steps, cpu_steps, threads, with_cpu=True):
     return out
 
 
+def measure_shock_capturing(steps, cpu_steps, threads, with_cpu=True):
+    """Sub-record: config 1 as the reference names it, the four-quadrant Riemann
+    problem at 256² with HG blending and MUSCL-Hancock subcell FV: device loop
+    throughput, kernel times and the CPU oracle."""
+    name = "lw_euler_riemann_mh_256x256_N4"
+    wl = WORKLOADS[name]
+    semi, mesh, ic, cfl, (lo, hi), okw = make_problem(wl)
+    x, y = semi.node_coordinates()
+    u0 = np.ascontiguousarray(ic(x, y, 0.0))
+    r = run_steps(semi, u0, cfl, steps, 2)
+    dofs = semi.ndofs()
+    out = {"workload": name, "value": r["value"], "unit": "DOF-updates/s", "ms_per_step": round(r["ms_per_step"], 4),
+           "steps": steps, "dofs": dofs, "volume_avg_launch_ms": round(r["vol_ms"], 4),
+           "surface_avg_launch_ms": round(r["surf_ms"], 4), "elements_per_workgroup": r["tile"],
+           "tile_autotune_ms": r["tune"],
+           "config": f"four-quadrant Riemann problem (Lax-Liu 3) on [0,1]^2, {mesh.nx}x{mesh.nx} TreeMesh, "
+                     f"time-averaged Dirichlet boundary, polydeg 4, LW, HG indicator on density_pressure "
+                     f"(alpha_max 0.5), MUSCL-Hancock subcell FV, CFL {cfl}"}
+    log(f"bench: shock capturing (MUSCL-Hancock) {r['value'] / 1e9:.3f} G DOF-updates/s, {r['ms_per_step']:.3f} "
+        f"ms/step, volume {r['vol_ms']:.3f} ms, surface {r['surf_ms']:.3f} ms")
+    semi.close()
+    if with_cpu:
+        import oracle_ctypes
+
+        nx = mesh.nx
+        o = oracle_ctypes.Oracle(4, nx, nx, lo, lo, (hi - lo) / nx, **okw)
+        xo, yo = o.node_coords()
+        v, el = cpu_baseline(o, np.ascontiguousarray(ic(xo, yo, 0.0)), cfl, False, dofs, cpu_steps, threads)
+        log(f"bench: shock capturing CPU {v / 1e6:.2f} M DOF-updates/s ({el:.1f} s)")
+        out["cpu_baseline"] = {"value": v, "unit": "DOF-updates/s", "cores": threads, "kind": "port",
+                               "sample": f"{cpu_steps} steps of the same mesh, C++/OpenMP oracle"}
+        out["gpu_over_cpu"] = round(r["value"] / v, 2)
+        del o
+    return out
+
+
 def measure_curved(steps, cpu_steps, threads, with_cpu=True):
     """Sub-record: LW N=4 isentropic vortex on the warped 1024² StructuredMesh,
     device loop throughput, volume kernel bytes per launch and the CPU oracle."""
@@ -410,6 +457,7 @@ def main():
     ap.add_argument("--no-ns", action="store_true", help="skip the Navier-Stokes sub-record")
     ap.add_argument("--no-adaptive", action="store_true", help="skip the adaptive-stepping sub-record")
     ap.add_argument("--no-curved", action="store_true", help="skip the curvilinear-mesh sub-record")
+    ap.add_argument("--no-sc", action="store_true", help="skip the shock-capturing (MUSCL-Hancock) sub-record")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device copy (the reference kernel for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -531,6 +579,11 @@ def main():
         navier_stokes = measure_navier_stokes(max(3, min(args.steps, 10)), 1, threads,
                                               with_cpu=not args.no_cpu_baseline)
 
+    sc_rec = None
+    if args.workload == DEFAULT_WORKLOAD and not args.no_sc:
+        sc_rec = measure_shock_capturing(max(10, min(5 * args.steps, 100)), 5, threads,
+                                         with_cpu=not args.no_cpu_baseline)
+
     curved_rec = None
     if args.workload == DEFAULT_WORKLOAD and not args.no_curved:
         curved_rec = measure_curved(max(3, min(args.steps, 10)), 1, threads, with_cpu=not args.no_cpu_baseline)
@@ -575,8 +628,9 @@ def main():
                                          "isentropic vortex strength 5, Mach 0.5"),
                    "parabolic": (f"CompressibleNavierStokesDiffusion2D mu {NS_MU}, Pr {NS_PR}, conservative "
                                  "gradients" if par else None),
-                   "shock_capturing": ("Hennemann-Gassner on density_pressure, alpha_max 0.5, first-order subcell FV"
-                                       if sc else None),
+                   "shock_capturing": (("Hennemann-Gassner on density_pressure, alpha_max 0.5, "
+                                        + ("MUSCL-Hancock" if wl.get("recon") == "muscl_hancock" else "first-order")
+                                        + " subcell FV") if sc else None),
                    "cfl": cfl, "elements_per_workgroup": tile, "tile_autotune_ms": tune},
         "roofline": roofline,
         "cpu_baseline": cpu,
@@ -585,6 +639,7 @@ def main():
         "navier_stokes": navier_stokes,
         "adaptive": adaptive,
         "curved": curved_rec,
+        "shock_capturing": sc_rec,
     }
     print(json.dumps(out), flush=True)
     semi.close()
