@@ -222,6 +222,18 @@ template <class T> struct Euler2D {
         fluxes(u, ff, gg);
         for (int v = 0; v < 4; ++v) f[v] = orientation == 1 ? ff[v] : gg[v];
     }
+    // Trixi flux(u, normal_direction, ::CompressibleEulerEquations2D): with
+    // v_n = v·n, (ρ v_n, ρ v_n v1 + p n1, ρ v_n v2 + p n2, (ρe + p) v_n)
+    void flux_normal(const T* u, const double* nrm, T* f) const {
+        T rho, v1, v2, p;
+        cons2prim(u, rho, v1, v2, p);
+        const T vn = v1 * nrm[0] + v2 * nrm[1];
+        const T rvn = rho * vn;
+        f[0] = rvn;
+        f[1] = rvn * v1 + p * nrm[0];
+        f[2] = rvn * v2 + p * nrm[1];
+        f[3] = (u[3] + p) * vn;
+    }
     void cons2prim(const T* u, T& rho, T& v1, T& v2, T& p) const {
         rho = u[0];
         v1 = u[1] / rho;
@@ -411,6 +423,10 @@ template <class T, class Eq> struct Solver {
     int n = 0;
     // 0: TreeMesh order (λ from Ũ, jump on uⁿ; dg_2d.jl:1928)
     // 1: Structured/P4est order (λ from uⁿ, jump on Ũ; structured/dg_2d.jl:2271)
+    // 2: the StructuredMesh conventions on this Cartesian mesh: Structured order
+    //    at interfaces, TreeMesh order at Dirichlet boundaries
+    //    (structured/dg_2d.jl:2374) and a full-step boundary time average in MDRK
+    //    stage 1 (structured/multi_derivative_2d.jl:18-20 passes no scaling)
     int flux_order = 0;
     double abstol = 0.0, reltol = 0.0;
     // shock capturing (VolumeIntegralFRShockCapturing with IndicatorHennemannGassner
@@ -1251,10 +1267,14 @@ template <class T, class Eq> struct Solver {
     // (F̃n, F̃n, uⁿ, uⁿ, Ũ, Ũ) (structured/dg_2d.jl:2271-2274).
     void lw_numflux(const T* fl, const T* fr, const T* Ul, const T* Ur, const T* ul, const T* ur, int orientation,
                     T* out) const {
-        const T* sl = flux_order == 0 ? Ul : ul;  // state for λ
-        const T* sr = flux_order == 0 ? Ur : ur;
-        const T* jl = flux_order == 0 ? ul : Ul;  // state for the jump
-        const T* jr = flux_order == 0 ? ur : Ur;
+        lw_numflux_ord(fl, fr, Ul, Ur, ul, ur, orientation, flux_order == 0 ? 0 : 1, out);
+    }
+    void lw_numflux_ord(const T* fl, const T* fr, const T* Ul, const T* Ur, const T* ul, const T* ur, int orientation,
+                        int order, T* out) const {
+        const T* sl = order == 0 ? Ul : ul;  // state for λ
+        const T* sr = order == 0 ? Ur : ur;
+        const T* jl = order == 0 ? ul : Ul;  // state for the jump
+        const T* jr = order == 0 ? ur : Ur;
         const T lam = eq.max_abs_speed_naive(sl, sr, orientation);
         for (int v = 0; v < NV; ++v) out[v] = 0.5 * (fl[v] + fr[v]) + (-0.5 * lam) * (jr[v] - jl[v]);
     }
@@ -1335,8 +1355,12 @@ template <class T, class Eq> struct Solver {
                 bval.eval(x, y, t, uod);
                 for (int v = 0; v < NV; ++v) uo[v] = T(uod[v]);
                 T Fn[NV];
-                if (dir % 2 == 0) lw_numflux(F_in, Fo, U_in, Uo, u_in, uo, o, Fn);
-                else lw_numflux(Fo, F_in, Uo, U_in, uo, u_in, o, Fn);
+                // flux_order 2: Structured order at interfaces, TreeMesh order at the
+                // boundary (a StructuredMesh's Dirichlet call, structured/dg_2d.jl:2374-2378
+                // into equations.jl:6-37)
+                const int bo = flux_order == 2 ? 0 : flux_order;
+                if (dir % 2 == 0) lw_numflux_ord(F_in, Fo, U_in, Uo, u_in, uo, o, bo, Fn);
+                else lw_numflux_ord(Fo, F_in, Uo, U_in, uo, u_in, o, bo, Fn);
                 for (int v = 0; v < NV; ++v) sfv[sidx(v, l, dir - 1, e)] = Fn[v];
             }
         }
@@ -1416,7 +1440,7 @@ template <class T, class Eq> struct Solver {
             if (sc) sc_element(du, u, e, 0.5 * dt);
         }
         calc_interface_flux(u);
-        if (!mesh.bd_elem.empty()) calc_boundary_flux(u, t, dt, 0.5);
+        if (!mesh.bd_elem.empty()) calc_boundary_flux(u, t, dt, flux_order == 2 ? 1.0 : 0.5);
         surface_integral_and_jacobian(du);
     }
     virtual void rhs_mdrk2(T* du, const T* u, double t, double dt) {

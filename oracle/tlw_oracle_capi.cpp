@@ -37,7 +37,8 @@ struct tlwo_desc {
     double visc_mu;  // diffusivity (Laplace) or dynamic viscosity μ (Navier-Stokes)
     double prandtl;
     // StructuredMesh(cells, mapping): 0 TreeMesh, 1 curvilinear with map_kind
-    // (MappingKind) and map_par; periodic, plain FR, LW
+    // (MappingKind) and map_par; plain FR, LW or MDRK, periodic or with the
+    // time-averaged Dirichlet boundary
     int curved;
     int map_kind;
     double map_par[4];
@@ -80,7 +81,7 @@ void* tlwo_create(const tlwo_desc* d) {
         if (!d->periodic_x || !d->periodic_y || d->shock_capturing) return nullptr;
         if ((d->parabolic == 1) != (d->equation == 1)) return nullptr;
     }
-    if (d->curved && (d->parabolic || d->shock_capturing || !d->periodic_x || !d->periodic_y)) return nullptr;
+    if (d->curved && (d->parabolic || d->shock_capturing)) return nullptr;
     if (d->reconstruction == 1 && (d->equation != 0 || !d->shock_capturing)) return nullptr;
     auto* c = new tlwo_ctx;
     c->equation = d->equation;

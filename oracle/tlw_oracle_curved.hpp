@@ -301,6 +301,68 @@ template <class T, class Eq> struct CurvedSolver : Solver<T, Eq> {
             }
     }
 
+    // calc_boundary_flux! for StructuredMesh (structured/dg_2d.jl:2297-2391) with a
+    // time-averaged Dirichlet value (equations.jl:6-37): the element's own
+    // contravariant vector at the boundary node as normal, F̃·n inside, the GLL
+    // time average of the boundary value and of its normal flux outside, and the
+    // LW Rusanov flux in the TreeMesh argument order (λ from Ũ, jump on uⁿ).
+    // scaling_factor is 1 in both MDRK stages on this mesh (the call passes none).
+    void calc_boundary_flux_curved(const T* u, double t, double dt) {
+        if constexpr (std::is_same<Eq, Euler2D<T>>::value) {
+            const auto& nodes = basis.nodes;
+            const auto& w = basis.weights;
+            const int nB = (int)mesh.bd_elem.size();
+#pragma omp parallel for schedule(static)
+            for (int B = 0; B < nB; ++B) {
+                const int e = mesh.bd_elem[B], o = mesh.bd_orient[B], side = mesh.bd_side[B], dir = mesh.bd_dir[B];
+                for (int l = 0; l < n; ++l) {
+                    int ii, jj;
+                    if (o == 1) { ii = (side == 1) ? n - 1 : 0; jj = l; }
+                    else { ii = l; jj = (side == 1) ? n - 1 : 0; }
+                    const size_t k = nidx(ii, jj, e);
+                    const double* J = &Ja[4 * k];
+                    const double sgn = ijn[k] > 0 ? 1.0 : (ijn[k] < 0 ? -1.0 : 0.0);
+                    const double nrm[2] = {sgn * (o == 1 ? J[0] : J[2]), sgn * (o == 1 ? J[1] : J[3])};
+                    const T* u_in = &u[idx(0, ii, jj, e)];
+                    const T* U_in = &this->U[idx(0, ii, jj, e)];
+                    T f_in[NV], Uo[NV], Fo[NV], uo[NV];
+                    for (int v = 0; v < NV; ++v) {
+                        f_in[v] = this->F[fidx(v, 0, ii, jj, e)] * nrm[0] + this->F[fidx(v, 1, ii, jj, e)] * nrm[1];
+                        Uo[v] = T(0);
+                        Fo[v] = T(0);
+                    }
+                    const double x = xn[k], y = yn[k];
+                    for (int q = 0; q < n; ++q) {
+                        const double ts = t + 0.5 * dt * (nodes[q] + 1.0);
+                        double ubd[4];
+                        this->bval.eval(x, y, ts, ubd);
+                        T ub[NV], fb[NV];
+                        for (int v = 0; v < NV; ++v) ub[v] = T(ubd[v]);
+                        eq.flux_normal(ub, nrm, fb);
+                        for (int v = 0; v < NV; ++v) {
+                            Uo[v] = Uo[v] + 0.5 * ub[v] * w[q];
+                            Fo[v] = Fo[v] + 0.5 * fb[v] * w[q];
+                        }
+                    }
+                    double uod[4];
+                    this->bval.eval(x, y, t, uod);
+                    for (int v = 0; v < NV; ++v) uo[v] = T(uod[v]);
+                    // surface_flux(F_l, F_r, Ũ_l, Ũ_r, uⁿ_l, uⁿ_r, n): λ from Ũ, jump on uⁿ
+                    const T* Ul = (dir % 2 == 0) ? U_in : Uo;
+                    const T* Ur = (dir % 2 == 0) ? Uo : U_in;
+                    const T* fl = (dir % 2 == 0) ? f_in : Fo;
+                    const T* fr = (dir % 2 == 0) ? Fo : f_in;
+                    const T* jl = (dir % 2 == 0) ? u_in : uo;
+                    const T* jr = (dir % 2 == 0) ? uo : u_in;
+                    const T lam = eq.max_abs_speed_naive_normal(Ul, Ur, nrm);
+                    for (int v = 0; v < NV; ++v)
+                        this->sfv[sidx(v, l, dir - 1, e)] =
+                            sgn * (0.5 * (fl[v] + fr[v]) + (-0.5 * lam) * (jr[v] - jl[v]));
+                }
+            }
+        }
+    }
+
     // Trixi calc_surface_integral! (StructuredMesh uses the TreeMesh form) and
     // apply_jacobian! with the node's J⁻¹
     void surface_integral_and_jacobian_curved(T* du) {
@@ -328,6 +390,7 @@ template <class T, class Eq> struct CurvedSolver : Solver<T, Eq> {
 #pragma omp parallel for schedule(static)
         for (int e = 0; e < mesh.E; ++e) lw_volume_curved(du, u, dt, e);
         calc_interface_flux_curved(u);
+        if (!mesh.bd_elem.empty()) calc_boundary_flux_curved(u, t, dt);
         surface_integral_and_jacobian_curved(du);
     }
 
@@ -463,21 +526,21 @@ template <class T, class Eq> struct CurvedSolver : Solver<T, Eq> {
     // rhs_mdrk1! / rhs_mdrk2! for StructuredMesh (multi_derivative_2d.jl:1-80):
     // both stages evaluate the interface flux with λ of uⁿ
     void rhs_mdrk1(T* du, const T* u, double t, double dt) override {
-        (void)t;
         this->ensure_mdrk();
         this->reset_du(du);
 #pragma omp parallel for schedule(static)
         for (int e = 0; e < mesh.E; ++e) mdrk_curved_1(du, u, dt, e);
         calc_interface_flux_curved(u);
+        if (!mesh.bd_elem.empty()) calc_boundary_flux_curved(u, t, dt);
         surface_integral_and_jacobian_curved(du);
     }
     void rhs_mdrk2(T* du, const T* u, double t, double dt) override {
-        (void)t;
         this->ensure_mdrk();
         this->reset_du(du);
 #pragma omp parallel for schedule(static)
         for (int e = 0; e < mesh.E; ++e) mdrk_curved_2(du, dt, e);
         calc_interface_flux_curved(u);
+        if (!mesh.bd_elem.empty()) calc_boundary_flux_curved(u, t, dt);
         surface_integral_and_jacobian_curved(du);
     }
 

@@ -163,3 +163,79 @@ def test_warped_mesh_mdrk_vortex_convergence(oracle_mod):
         errs.append(np.sqrt(np.mean((u - ic(x, y, t))[..., 0] ** 2)))
     eoc = np.log(errs[1] / errs[2]) / np.log(2)
     assert eoc > 3.4, (errs, eoc)
+
+
+@pytest.mark.parametrize("N", [3, 4])
+def test_affine_mapping_with_dirichlet_boundary_equals_treemesh(oracle_mod, N):
+    """A non-periodic affine StructuredMesh with the time-averaged Dirichlet
+    boundary (calc_boundary_flux! of structured/dg_2d.jl:2297-2391: the
+    element's own contravariant vector as normal, TreeMesh argument order,
+    full-step average in both MDRK stages) equals the TreeMesh solver in the
+    StructuredMesh conventions (flux_order 2), whose TreeMesh pieces the linear
+    closed form pins.  LW and both MDRK stages."""
+    from trixilw_hip.equations import IsentropicVortex
+
+    nx, ny, L = 6, 4, 20.0
+    h = L / nx
+    ic = IsentropicVortex(center=(L / 2, 0.4 * h * ny), period=(1e9, 1e9))
+    kw = dict(periodic=(False, False), bc_kind=ic.kind, bc_par=ic.params)
+    tree = oracle_mod.Oracle(N, nx, ny, 0.0, 0.0, h, flux_order=2, **kw)
+    curv = oracle_mod.Oracle(N, nx, ny, 0.0, 0.0, h, mapping=(oracle_mod.MAP_AFFINE, (0.0, 0.0, L, h * ny)), **kw)
+    x, y = tree.node_coords()
+    rng = np.random.default_rng(40 + N)
+    u = ic(x, y, 0.0) * (1 + 1e-2 * rng.standard_normal(x.shape + (4,)))
+    dt = 0.8 * tree.max_dt(u)
+    assert rel_linf(curv.rhs(u, 0.3, dt), tree.rhs(u, 0.3, dt)) < 1e-12
+    d1t, d1c = tree.rhs(u, 0.3, dt, kind="mdrk1"), curv.rhs(u, 0.3, dt, kind="mdrk1")
+    assert rel_linf(d1c, d1t) < 1e-12
+    us = u + 0.5 * dt * d1t
+    tree.set_us(us)
+    curv.set_us(us)
+    assert rel_linf(curv.rhs(u, 0.3, dt, kind="mdrk2"), tree.rhs(u, 0.3, dt, kind="mdrk2")) < 1e-12
+    # the boundary matters: a periodic mesh gives another residual
+    per = oracle_mod.Oracle(N, nx, ny, 0.0, 0.0, h, mapping=(oracle_mod.MAP_AFFINE, (0.0, 0.0, L, h * ny)))
+    assert rel_linf(per.rhs(u, 0.3, dt), curv.rhs(u, 0.3, dt)) > 1e-3
+
+
+@pytest.mark.parametrize("N", [3, 4])
+def test_warped_mesh_dirichlet_free_stream(oracle_mod, N):
+    """A constant boundary value equal to the constant state keeps it (LW and
+    MDRK stage 2): the boundary normal and its sign agree with the interior
+    metric terms.  MDRK stage 1 does not: the reference averages the boundary
+    data over the full step (structured/multi_derivative_2d.jl:18-20 passes no
+    scaling) while the stage-1 element flux is f/2 + f_t/8, so on a boundary
+    node F* = ½(f/2 + f)·n = ¾ f·n against f/2 inside; interior elements stay
+    at rest."""
+    from trixilw_hip.equations import ConstantState
+
+    bc = ConstantState(rho=1.1, v1=0.3, v2=-0.25, p=0.9)
+    o = warped(oracle_mod, N, 6, periodic=(False, False), bc_kind=bc.kind, bc_par=bc.params)
+    x, y = o.node_coords()
+    u = np.ascontiguousarray(bc(x, y, 0.0))
+    assert np.ptp(u[..., 1]) == 0.0 and abs(u[0, 0, 0, 1] - 0.33) < 1e-15
+    dt = 0.5 * o.max_dt(u)
+    assert np.max(np.abs(o.rhs(u, 0.0, dt))) < 1e-12
+    d1 = o.rhs(u, 0.0, dt, kind="mdrk1")
+    o.set_us(u)
+    assert np.max(np.abs(o.rhs(u, 0.0, dt, kind="mdrk2"))) < 1e-12
+    edge = np.any(o.neighbors() < 0, axis=1)
+    assert np.max(np.abs(d1[~edge])) < 1e-12 and np.max(np.abs(d1[edge])) > 1e-3
+
+
+@pytest.mark.parametrize("N,eoc_min", [(3, 3.4), (4, 3.8)])
+def test_warped_mesh_dirichlet_vortex_convergence(oracle_mod, N, eoc_min):
+    """The isentropic vortex with its exact solution as time-averaged Dirichlet
+    data on the non-periodic warped mesh converges at high order."""
+    from trixilw_hip.equations import IsentropicVortex
+
+    ic = IsentropicVortex(center=(10.0, 10.0), period=(1e9, 1e9))
+    errs = []
+    for nx in (16, 32, 64):
+        o = warped(oracle_mod, N, nx, periodic=(False, False), bc_kind=ic.kind, bc_par=ic.params)
+        x, y = o.node_coords()
+        cfl = 0.9 * (N + 1) * {3: 0.170, 4: 0.103}[N]
+        u, t, _ = o.solve(ic(x, y, 0.0), 0.0, 1.0, cfl)
+        assert abs(t - 1.0) < 1e-12
+        errs.append(np.sqrt(np.mean((u - ic(x, y, t))[..., 0] ** 2)))
+    eoc = np.log(errs[1] / errs[2]) / np.log(2)
+    assert eoc > eoc_min, (errs, eoc)
