@@ -126,6 +126,8 @@ typedef struct {
 typedef struct {
     const double* contravariant_vectors_host; /* [2*2*n*n*E] */
     const double* inverse_jacobian_host;      /* [n*n*E], every entry > 0 */
+    const double* node_coordinates_host;      /* [2*n*n*E] node_coordinates[d, i, j, element]; needed (for the
+                                                 boundary value function) when the mesh has boundary faces */
 } tlw_curved_desc;
 typedef struct tlw_ctx tlw_ctx;
 
@@ -161,13 +163,16 @@ int tlw_create_parabolic(const tlw_mesh_desc* mesh, const tlw_basis_desc* basis,
  * and calc_interface_flux! (dg_2d.jl:2194-2290: the normal of the right element's face node,
  * F* = ½(F̃_L·n + F̃_R·n) - ½ λ(uⁿ, n)(Ũ_R - Ũ_L) with λ = max|v·n| + max c |n|); tlw_max_dt and
  * the device loop use Trixi's curvilinear max_dt (per node |Ja^l · (λ1, λ2)| |J⁻¹|).
- * mesh->neighbors_host gives the face neighbours (a periodic mesh: no -1 entries);
- * mesh->inverse_jacobian_host, element_center_host and element_length_host are not read and
- * may be NULL.  The structured flux order applies whatever solver->surface_flux_order says.
+ * mesh->neighbors_host gives the face neighbours, -1 on a domain boundary, where the time-averaged
+ * Dirichlet condition of solver->boundary_kind applies as in calc_boundary_flux! for StructuredMesh
+ * (dg_2d.jl:2297-2391: the element's own face normal, TreeMesh argument order, the average over the
+ * full step in both MDRK stages); mesh->inverse_jacobian_host, element_center_host and
+ * element_length_host are not read and may be NULL.  The structured flux order applies whatever
+ * solver->surface_flux_order says.
  * With TLW_MDRK, tlw_rhs_mdrk1/2 are rhs_mdrk1!/2! of src/solvers/dgsem_structured/
  * multi_derivative_2d.jl:1-80 (mdrk_kernel_1!/2!, 82-445: F̃ = f/2 + f_t/8, then F2 + f_t(us)/3,
- * in contravariant form for the volume term).  Plain VolumeIntegralFR; shock capturing and
- * boundaries return TLW_ENOTSUP, and so do the positivity limiter, adaptive steps,
+ * in contravariant form for the volume term).  Plain VolumeIntegralFR; shock capturing returns
+ * TLW_ENOTSUP, and so do the positivity limiter, adaptive steps,
  * tlw_adjoint_lw and tlw_rebuild_mesh on such a context.  A non-positive inverse Jacobian
  * returns TLW_EINVAL. */
 int tlw_create_curved(const tlw_mesh_desc* mesh, const tlw_basis_desc* basis, const tlw_solver_desc* solver,

@@ -35,23 +35,28 @@ def warp_mapping(L, A):
     return mapping
 
 
-def setup_curved(oracle_mod, N, nx, ny, kind="warp", L=20.0, A=0.05, td="lw", seed=0, amp=1e-3, keep_aux=False):
+def setup_curved(oracle_mod, N, nx, ny, kind="warp", L=20.0, A=0.05, td="lw", seed=0, amp=1e-3, keep_aux=False,
+                 periodic=True):
+    """A non-periodic mesh takes the vortex itself (no periodic images) as
+    time-averaged Dirichlet data."""
     import trixilw_hip as tlw
 
     eq = tlw.CompressibleEulerEquations2D(GAMMA)
-    ic = tlw.IsentropicVortex(center=(L / 2, L / 2), period=(L, L * ny / nx if kind == "affine" else L))
+    period = (L, L * ny / nx if kind == "affine" else L) if periodic else (1e9, 1e9)
+    ic = tlw.IsentropicVortex(center=(L / 2, L / 2), period=period)
     if kind == "warp":
-        mesh = tlw.StructuredMesh((nx, ny), warp_mapping(L, A))
+        mesh = tlw.StructuredMesh((nx, ny), warp_mapping(L, A), periodicity=periodic)
         mp = (oracle_mod.MAP_WARP, (L, A))
     else:
         Ly = L * ny / nx
-        mesh = tlw.StructuredMesh((nx, ny), (0.0, 0.0), (L, Ly))
+        mesh = tlw.StructuredMesh((nx, ny), (0.0, 0.0), (L, Ly), periodicity=periodic)
         mp = (oracle_mod.MAP_AFFINE, (0.0, 0.0, L, Ly))
     solver = tlw.DGSEM(N, volume_integral=tlw.VolumeIntegralFR(tlw.MDRK() if td == "mdrk" else tlw.LW()))
-    o = oracle_mod.Oracle(N, nx, ny, 0.0, 0.0, L / nx, gamma=GAMMA, mapping=mp)
+    okw = {} if periodic else dict(periodic=(False, False), bc_kind=ic.kind, bc_par=ic.params)
+    o = oracle_mod.Oracle(N, nx, ny, 0.0, 0.0, L / nx, gamma=GAMMA, mapping=mp, **okw)
     mesh.set_metrics(solver.basis, o.aux("contravariant_vectors"), o.aux("inverse_jacobian"))
     semi = tlw.SemidiscretizationHyperbolic(mesh, tlw.get_time_discretization(solver), eq, ic, solver,
-                                            keep_aux=keep_aux)
+                                            boundary_conditions=None if periodic else ic, keep_aux=keep_aux)
     x, y = o.node_coords()
     rng = np.random.default_rng(seed)
     u_lex = ic(x, y, 0.0) * (1.0 + amp * rng.standard_normal(x.shape + (4,)))
@@ -131,6 +136,56 @@ def test_curved_mdrk_device_loop_k_steps(oracle_mod, N, nx, K):
     assert sol.stats.naccept == K
     assert abs(sol.t - t_ref) <= 1e-13 * t_ref
     assert_close(to_lex(mesh, sol.u), u_ref, what=f"curved MDRK N={N} {nx}x{nx} after {K} steps")
+    semi.close()
+
+
+@pytest.mark.parametrize("N", [3, 4])
+@pytest.mark.parametrize("td", ["lw", "mdrk"])
+def test_curved_dirichlet_rhs_parity(oracle_mod, N, td):
+    """Non-periodic warped mesh with the vortex as time-averaged Dirichlet data
+    (calc_boundary_flux! for StructuredMesh): rhs, or both MDRK stages."""
+    import trixilw_hip as tlw
+
+    torch = torch_mod()
+    print(f"curved Dirichlet N={N} {td}", flush=True)
+    semi, o, u_lex, mesh = setup_curved(oracle_mod, N, 8, 6, A=0.1, td=td, seed=7, periodic=False)
+    u = to_dev(mesh, u_lex)
+    du = torch.empty_like(u)
+    dt = 0.6 * o.max_dt(u_lex)
+    semi.cache["dt"][0] = dt
+    if td == "lw":
+        du_ref = o.rhs(u_lex, 0.25, dt)
+        tlw.rhs(du, u, semi, 0.25)
+        torch.cuda.synchronize()
+        assert_close(to_lex(mesh, du), du_ref, what=f"curved Dirichlet LW N={N}")
+    else:
+        d1 = o.rhs(u_lex, 0.25, dt, kind="mdrk1")
+        us = u_lex + 0.5 * dt * d1
+        o.set_us(us)
+        d2 = o.rhs(u_lex, 0.25, dt, kind="mdrk2")
+        tlw.rhs_mdrk1(du, u, semi, 0.25)
+        assert_close(to_lex(mesh, du), d1, what=f"curved Dirichlet mdrk1 N={N}")
+        semi.set_us(to_dev(mesh, us))
+        tlw.rhs_mdrk2(du, u, semi, 0.25)
+        torch.cuda.synchronize()
+        assert_close(to_lex(mesh, du), d2, what=f"curved Dirichlet mdrk2 N={N}")
+    semi.close()
+
+
+@pytest.mark.parametrize("N,nx,td,K", [(4, 8, "lw", 25), (3, 16, "lw", 25), (3, 16, "mdrk", 20)])
+def test_curved_dirichlet_device_loop_k_steps(oracle_mod, N, nx, td, K):
+    import trixilw_hip as tlw
+
+    print(f"curved Dirichlet device loop N={N} {td} K={K}", flush=True)
+    semi, o, u_lex, mesh = setup_curved(oracle_mod, N, nx, nx, A=0.1, td=td, periodic=False)
+    cfl = tlw.trixi2lw(0.9, semi.solver) * (0.5 if td == "mdrk" else 1.0)
+    u_ref, t_ref, _ = o.solve(u_lex, 0.0, 1e9, cfl, max_steps=K, mdrk=(td == "mdrk"))
+    lw_update = tlw.semidiscretize(semi, semi.time_discretization, (0.0, float("inf")))
+    lw_update.u0.copy_(to_dev(mesh, u_lex))
+    sol = tlw.solve_lwfr(lw_update, (), 1e-6, None, time_step_computation=tlw.CFLBased(cfl), max_steps=K)
+    assert sol.stats.naccept == K
+    assert abs(sol.t - t_ref) <= 1e-13 * t_ref
+    assert_close(to_lex(mesh, sol.u), u_ref, what=f"curved Dirichlet {td} N={N} after {K} steps")
     semi.close()
 
 
@@ -289,9 +344,9 @@ def test_curved_rejects_unsupported(oracle_mod):
     semi.close()
     eq = tlw.CompressibleEulerEquations2D(GAMMA)
     ic = tlw.IsentropicVortex()
-    # a non-periodic mesh, and a mirrored mapping (negative Jacobian)
+    # a non-periodic mesh without a boundary value function, and a mirrored mapping (negative Jacobian)
     cases = [
-        (tlw.StructuredMesh((4, 4), warp_mapping(20.0, 0.05), periodicity=False), tlw.LW(), _lib.TLW_ENOTSUP),
+        (tlw.StructuredMesh((4, 4), warp_mapping(20.0, 0.05), periodicity=False), tlw.LW(), _lib.TLW_EINVAL),
         (tlw.StructuredMesh((4, 4), lambda xi, eta: (-xi, eta)), tlw.LW(), _lib.TLW_EINVAL),
     ]
     for mesh_k, td, code in cases:

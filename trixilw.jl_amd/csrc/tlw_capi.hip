@@ -61,6 +61,8 @@ struct tlw_ctx {
     double* d_Ja = nullptr;
     double* d_ijn = nullptr;
     double* d_fnorm = nullptr;
+    double* d_fxy = nullptr;  // boundary face-node coordinates (curvilinear mesh with boundaries)
+    bool bad_launch = false;  // a launch was refused for a missing input (reported by check_launch)
     int pv_off = 0;           // offset of the viscous CFL partials (0: none)
     int ebv = 0;              // elements per volume workgroup
     bool positivity = false;  // Zhang-Shu limiter in the device loop
@@ -128,7 +130,7 @@ void free_mesh(tlw_ctx* c) {
     void* ptrs[] = {c->d_nbr,       c->d_invj,  c->d_center, c->d_length, c->d_trace,   c->d_buf[0],    c->d_buf[1],
                     c->d_us,        c->d_auxU,  c->d_auxF,   c->d_partial, c->d_alpha_tmp, c->d_alpha,
                     c->d_vblk,      c->d_sblk,  c->d_etile,  c->d_rng,    c->d_state,   c->d_cur,
-                    c->d_Ja,        c->d_ijn,   c->d_fnorm};
+                    c->d_Ja,        c->d_ijn,   c->d_fnorm,  c->d_fxy};
     for (void* p : ptrs) (void)hipFree(p);
     if (c->h_state) (void)hipHostFree(c->h_state);
     c->d_nbr = nullptr;
@@ -140,7 +142,7 @@ void free_mesh(tlw_ctx* c) {
     c->d_state = nullptr;
     c->h_state = nullptr;
     c->d_cur = nullptr;
-    c->d_Ja = c->d_ijn = c->d_fnorm = nullptr;
+    c->d_Ja = c->d_ijn = c->d_fnorm = c->d_fxy = nullptr;
 }
 
 // Workgroup tables for the current tile: volume workgroups of ebv elements and
@@ -209,7 +211,7 @@ int validate_mesh(tlw_ctx* c, const tlw_mesh_desc* m) {
 // interface normals of every face node, taken from the right element's face
 // node so that both elements of an interface use the same bits.
 int prepare_curved(tlw_ctx* c, const tlw_mesh_desc* m, const tlw_curved_desc* cv, std::vector<double>& fnorm,
-                   std::vector<double>& invj) {
+                   std::vector<double>& invj, std::vector<double>& fxy) {
     if (!cv || !cv->contravariant_vectors_host || !cv->inverse_jacobian_host)
         return fail(c, TLW_EINVAL, "curved description: missing arrays");
     const int E = m->n_elements, n = c->n, NN = n * n;
@@ -220,15 +222,29 @@ int prepare_curved(tlw_ctx* c, const tlw_mesh_desc* m, const tlw_curved_desc* cv
     for (size_t k = 0; k < (size_t)NN * E; ++k)
         if (!(ijn[k] > 0.0) || !std::isfinite(ijn[k]))
             return fail(c, TLW_EINVAL, "curved description: inverse Jacobian must be finite and > 0");
+    bool boundary = false;
+    for (size_t k = 0; k < (size_t)4 * E; ++k) boundary = boundary || m->neighbors_host[k] < 0;
+    const double* xy = cv->node_coordinates_host;
+    if (boundary && !xy) return fail(c, TLW_EINVAL, "curved description: node coordinates needed on a mesh with boundaries");
     fnorm.assign((size_t)E * 4 * n * 4, 0.0);
+    fxy.assign(boundary ? (size_t)E * 4 * n * 2 : 0, 0.0);
     invj.resize(E);
     for (int e = 0; e < E; ++e) {
         invj[e] = ijn[(size_t)e * NN];
         for (int f = 0; f < 4; ++f) {
-            const int right = (f & 1) ? m->neighbors_host[4 * e + f] : e;
+            const int nb = m->neighbors_host[4 * e + f];
+            // an interface takes the right element's face node; a domain boundary the element's own
+            const int right = (f & 1) ? (nb >= 0 ? nb : e) : e;
+            const bool own_plus = (f & 1) && nb < 0;
             for (int l = 0; l < n; ++l) {
-                // x faces: Ja¹ at the right element's node (1, l); y faces: Ja² at (l, 1)
-                const size_t node = (size_t)right * NN + (f < 2 ? (size_t)n * l : (size_t)l);
+                // x faces: Ja¹ at node (1, l) of the right element (own: (n, l)); y faces: Ja² at (l, 1) ((l, n))
+                const size_t loc = f < 2 ? (size_t)n * l + (own_plus ? n - 1 : 0)
+                                         : (size_t)l + (own_plus ? (size_t)n * (n - 1) : 0);
+                const size_t node = (size_t)right * NN + loc;
+                if (nb < 0) {
+                    fxy[((((size_t)e * 4 + f) * n + l) * 2)] = xy[2 * node];
+                    fxy[((((size_t)e * 4 + f) * n + l) * 2) + 1] = xy[2 * node + 1];
+                }
                 const double n1 = Ja[4 * node + (f < 2 ? 0 : 2)], n2 = Ja[4 * node + (f < 2 ? 1 : 3)];
                 double* o = &fnorm[(((size_t)e * 4 + f) * n + l) * 4];
                 o[0] = n1;
@@ -256,14 +272,13 @@ int upload_meshes(tlw_ctx* c, const tlw_mesh_desc* ms, int ninst, const tlw_curv
     for (int k = 0; k < ninst; ++k)
         for (size_t q = 0; q < (size_t)4 * ms[k].n_elements; ++q)
             if (ms[k].neighbors_host[q] < 0) boundary = true;
-    if (boundary && c->curved) return fail(c, TLW_ENOTSUP, "the curvilinear path needs a periodic mesh");
     if (boundary && c->hconst.bc_kind == TLW_BV_NONE)
         return fail(c, TLW_EINVAL, "mesh has boundary faces but no boundary value function was given");
     if (boundary && c->parabolic)
         return fail(c, TLW_ENOTSUP, "the hyperbolic-parabolic path needs a periodic mesh");
-    std::vector<double> fnorm, cinvj;
+    std::vector<double> fnorm, cinvj, fxy;
     if (c->curved) {
-        int rc = prepare_curved(c, &ms[0], cv, fnorm, cinvj);
+        int rc = prepare_curved(c, &ms[0], cv, fnorm, cinvj, fxy);
         if (rc) return rc;
     }
     free_mesh(c);
@@ -340,6 +355,10 @@ int upload_meshes(tlw_ctx* c, const tlw_mesh_desc* ms, int ninst, const tlw_curv
                             hipMemcpyHostToDevice));
         HIPCHK(c, hipMemcpy(c->d_ijn, cv->inverse_jacobian_host, sizeof(double) * NN * E, hipMemcpyHostToDevice));
         HIPCHK(c, hipMemcpy(c->d_fnorm, fnorm.data(), sizeof(double) * fnorm.size(), hipMemcpyHostToDevice));
+        if (!fxy.empty()) {
+            HIPCHK(c, hipMalloc(&c->d_fxy, sizeof(double) * fxy.size()));
+            HIPCHK(c, hipMemcpy(c->d_fxy, fxy.data(), sizeof(double) * fxy.size(), hipMemcpyHostToDevice));
+        }
     }
     return build_tables(c);
 }
@@ -369,6 +388,7 @@ KArgs base_args(tlw_ctx* c) {
     a.Ja = c->d_Ja;
     a.ijn = c->d_ijn;
     a.fnorm = c->d_fnorm;
+    a.fxy = c->d_fxy;
     return a;
 }
 
@@ -429,8 +449,14 @@ void launch_finalize(tlw_ctx* c, int reduce, int use_vol) {
 }
 template <int N> void launch_surface_t(tlw_ctx* c, const KArgs& a) {
     dim3 g(c->sgrid), b(kSurfBlock);
+    if (c->curved && c->has_boundary && a.in_sel == SEL_EXPLICIT && !a.u) {
+        // the curvilinear boundary faces read uⁿ: never launch without it
+        c->bad_launch = true;
+        return;
+    }
     if (c->curved) {
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_surface<N, false, false, true>), g, b, 0, c->stream, a);
+        if (c->has_boundary) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_surface<N, true, false, true>), g, b, 0, c->stream, a);
+        else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_surface<N, false, false, true>), g, b, 0, c->stream, a);
     } else if (c->sc) {
         if (c->has_boundary) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_surface<N, true, true>), g, b, 0, c->stream, a);
         else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_surface<N, false, true>), g, b, 0, c->stream, a);
@@ -458,6 +484,10 @@ void launch_max_speed(tlw_ctx* c, const KArgs& a) {
 }
 
 int check_launch(tlw_ctx* c) {
+    if (c->bad_launch) {
+        c->bad_launch = false;
+        return fail(c, TLW_EINVAL, "internal error: surface kernel launched without its input state");
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(c, TLW_EHIP, std::string("kernel launch: ") + hipGetErrorString(e));
     return TLW_OK;
@@ -521,7 +551,8 @@ int rhs_common(tlw_ctx* c, int kind, const double* u, double* du, double t, doub
     a.use_state_dt = 0;
     a.step_mode = 0;
     a.reduce = RED_NONE;
-    a.bc_scale = (kind == VOL_MDRK1) ? 0.5 : 1.0;  // tree/multi_derivative_2d.jl:34
+    // tree/multi_derivative_2d.jl:34; a StructuredMesh passes no scaling (structured/multi_derivative_2d.jl:18-20)
+    a.bc_scale = (kind == VOL_MDRK1 && !c->curved) ? 0.5 : 1.0;
     if (c->flags & TLW_FLAG_KEEP_AUX) {
         a.auxU = c->d_auxU;
         a.auxF = c->d_auxF;
@@ -553,6 +584,7 @@ void enqueue_step(tlw_ctx* c) {
         v.reduce = red;
         timed(c, 0, [&] { launch_volume(c, v, c->adaptive ? VOL_LW_ADAPT : VOL_LW); });
         KArgs sf = a;
+        sf.in_sel = SEL_CUR;  // uⁿ: read at curvilinear domain-boundary faces
         sf.out_sel = SEL_OTHER;
         sf.reduce = red;
         timed(c, 1, [&] { launch_surface(c, sf); });
@@ -565,7 +597,7 @@ void enqueue_step(tlw_ctx* c) {
         v1.reduce = RED_NONE;
         timed(c, 0, [&] { launch_volume(c, v1, VOL_MDRK1); });
         KArgs s1 = v1;
-        s1.bc_scale = 0.5;
+        s1.bc_scale = c->curved ? 1.0 : 0.5;
         timed(c, 1, [&] { launch_surface(c, s1); });
         // stage 2: uⁿ⁺¹ = uⁿ + dt rhs_mdrk2(uⁿ, us)  (mdrk.jl:21-24)
         KArgs v2 = a;

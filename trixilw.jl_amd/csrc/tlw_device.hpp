@@ -113,7 +113,7 @@ struct KArgs {
     const double* us;     // MDRK stage state
     double* out;          // volume: result; surface: corrected in place
     double* bufs[2];      // ping-pong state buffers for the device-resident loop
-    int in_sel, out_sel;  // BufSel
+    int in_sel, out_sel;  // BufSel (the surface kernel reads the input at curvilinear boundary faces)
     double* trace;        // [E][4][n][kTraceQ], written for faces leaving a volume workgroup
     const int* nbr;       // [E][4]
     const double* inv_jac;
@@ -146,6 +146,7 @@ struct KArgs {
     const double* Ja;
     const double* ijn;
     const double* fnorm;
+    const double* fxy;  // boundary face-node coordinates [E][4][n][2] (curvilinear mesh with boundaries)
     double t_arg, dt_arg;
     int use_state_dt;  // 1: t, dt from DevState (device loop); 0: from t_arg/dt_arg
     int step_mode;     // 1: out = u + dt * rhs; 0: out = rhs
@@ -1875,6 +1876,8 @@ __global__ __launch_bounds__((VGeo<N, EB>::NT), kVolWaves<SC || PAR>) void k_vol
         const double v1 = uN[1] * ir, v2 = uN[2] * ir;
         const double p = gm1 * (uN[3] - 0.5 * (uN[1] * v1 + uN[2] * v2));
         const double cs = sqrt_gs(gamma * p * ir);
+        // (a domain-boundary face is rewritten to the TreeMesh order of the
+        // Dirichlet call by the surface kernel, which has uⁿ at hand)
         auto face = [&](int f, int l) {
             double nf[4], Fn[4], r[RQ];
             load4(a.fnorm + (((size_t)e * 4 + f) * n + l) * 4, nf);
@@ -2040,6 +2043,42 @@ __device__ inline void boundary_record(const KArgs& a, int e, int face, int l, d
     make_record(r, tree ? uo : Uo, Fo, orientation == 1 ? av1 : av2, cs);
 }
 
+// Outer record of a domain-boundary face node of a curvilinear mesh
+// (calc_boundary_flux_by_direction!, structured/dg_2d.jl:2352-2391, into the
+// Dirichlet call of equations.jl:6-37): the GLL time average of the boundary
+// value and of its flux in the normal direction (Trixi flux(u, n):
+// (ρv_n, ρv_n v1 + p n1, ρv_n v2 + p n2, (ρe + p) v_n)) over the full step,
+// TreeMesh order (jump on uⁿ, λ from Ũ), the element's own face normal.
+template <int N>
+__device__ inline void boundary_record_curved(const KArgs& a, int e, int face, int l, double t, double dt, double* r) {
+    constexpr int n = N + 1;
+    const DevConst* c = a.c;
+    const size_t fo = ((size_t)e * 4 + face) * n + l;
+    const double x = a.fxy[2 * fo], y = a.fxy[2 * fo + 1];
+    double nf[4];
+    load4(a.fnorm + 4 * fo, nf);
+    const double gm1 = c->gamma - 1.0;
+    double Uo[4] = {0, 0, 0, 0}, Fo[4] = {0, 0, 0, 0}, uo[4];
+    for (int q = 0; q < n; ++q) {
+        const double ts = t + 0.5 * dt * (c->nodes[q] + 1.0);
+        double ub[4];
+        boundary_value(c, x, y, ts, ub);
+        const double rho = ub[0], v1 = ub[1] / rho, v2 = ub[2] / rho;
+        const double p = gm1 * (ub[3] - 0.5 * (ub[1] * v1 + ub[2] * v2));
+        const double vn = v1 * nf[0] + v2 * nf[1], rvn = rho * vn;
+        const double fb[4] = {rvn, rvn * v1 + p * nf[0], rvn * v2 + p * nf[1], (ub[3] + p) * vn};
+        for (int v = 0; v < 4; ++v) {
+            Uo[v] += 0.5 * ub[v] * c->weights[q];
+            Fo[v] += 0.5 * fb[v] * c->weights[q];
+        }
+    }
+    boundary_value(c, x, y, t, uo);
+    const double ir = rcp_nr(Uo[0]);
+    const double b1 = Uo[1] * ir, b2 = Uo[2] * ir;
+    const double pb = gm1 * (Uo[3] - 0.5 * (Uo[1] * b1 + Uo[2] * b2));
+    make_record(r, uo, Fo, fabs(fma(b1, nf[0], b2 * nf[1])), sqrt_gs(c->gamma * pb * ir) * nf[2]);
+}
+
 // Surface kernel: the faces that leave a volume workgroup (a.ebv elements).
 // Each interface is evaluated by both neighbours from identical inputs, so
 // both see the same bits.  In step mode every lane then takes its node's
@@ -2047,7 +2086,7 @@ __device__ inline void boundary_record(const KArgs& a, int e, int face, int l, d
 // CFL maxima and the admissibility test.
 template <int N, bool BC, bool SC, bool CURV = false>
 __global__ __launch_bounds__(kSurfBlock) void k_surface(KArgs a) {
-    static_assert(!(CURV && (BC || SC)), "the curvilinear path is periodic, without shock capturing");
+    static_assert(!(CURV && SC), "the curvilinear path has no shock capturing");
     using G_ = SGeo<N>;
     constexpr int n = G_::n, NN = G_::NN, EB = G_::EB, S = G_::S;
     __shared__ double s_l1[S], s_l2[S], s_m[EB], s_lv[S], s_mv[EB];
@@ -2096,8 +2135,22 @@ __global__ __launch_bounds__(kSurfBlock) void k_surface(KArgs a) {
                     face_numflux(Lr, Rr, Fs);
             } else {
                 // domain boundary: time-averaged Dirichlet state, no blending
-                // (calc_boundary_flux!, dg_2d.jl:1379-1454)
-                boundary_record<N>(a, e, f, l, t, dt, oth);
+                // (calc_boundary_flux!, dg_2d.jl:1379-1454; structured/dg_2d.jl:2297-2391)
+                if constexpr (CURV) {
+                    // the volume kernel's record holds Ũ, F̃·n and the speeds of uⁿ; the
+                    // Dirichlet call takes the TreeMesh order: jump on uⁿ, λ from Ũ
+                    double nf[4];
+                    load4(a.fnorm + (((size_t)e * 4 + f) * n + l) * 4, nf);
+                    const double ir = rcp_nr(own[0]);
+                    const double b1 = own[1] * ir, b2 = own[2] * ir;
+                    const double pb = (c->gamma - 1.0) * (own[3] - 0.5 * (own[1] * b1 + own[2] * b2));
+                    own[8] = fabs(fma(b1, nf[0], b2 * nf[1]));
+                    own[9] = sqrt_gs(c->gamma * pb * ir) * nf[2];
+                    load4(sel_in(a, cur) + ((size_t)e * NN + node) * 4, own);
+                    boundary_record_curved<N>(a, e, f, l, t, dt, oth);
+                } else {
+                    boundary_record<N>(a, e, f, l, t, dt, oth);
+                }
                 face_numflux(Lr, Rr, Fs);
             }
             const double wf = (f & 1) ? c->bnd_hi : -c->bnd_lo;

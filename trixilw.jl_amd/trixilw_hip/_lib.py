@@ -76,6 +76,7 @@ class CurvedDesc(C.Structure):
     _fields_ = [
         ("contravariant_vectors_host", C.POINTER(C.c_double)),
         ("inverse_jacobian_host", C.POINTER(C.c_double)),
+        ("node_coordinates_host", C.POINTER(C.c_double)),
     ]
 
 

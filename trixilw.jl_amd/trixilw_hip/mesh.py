@@ -115,8 +115,9 @@ class StructuredMesh:
     jacobian_matrix from the derivative matrix applied to the node
     coordinates; contravariant vectors Ja¹ = (y_η, -x_η), Ja² = (-y_ξ, x_ξ);
     inverse_jacobian = 1/(x_ξ y_η - x_η y_ξ)), which TrixiLW's StructuredMesh
-    kernels read (src/solvers/dgsem_structured/dg_2d.jl:1379-2290).  The
-    device path covers periodic meshes.
+    kernels read (src/solvers/dgsem_structured/dg_2d.jl:1379-2391).  A
+    non-periodic direction takes the time-averaged Dirichlet condition of the
+    semidiscretization's boundary value function.
     """
 
     curved = True

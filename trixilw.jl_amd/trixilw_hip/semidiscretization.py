@@ -150,7 +150,8 @@ class SemidiscretizationHyperbolic:
     ``surface_flux_order`` selects the TreeMesh ("tree") or Structured/P4est
     ("structured") argument order of the LW Rusanov flux.
 
-    On a curvilinear StructuredMesh (periodic, plain LW) the kernels of
+    On a curvilinear StructuredMesh (plain LW or MDRK, periodic or with the
+    time-averaged Dirichlet boundary) the kernels of
     src/solvers/dgsem_structured/dg_2d.jl apply: contravariant fluxes, per-node
     inverse Jacobians, interface normals and the structured flux order.
     """
@@ -211,11 +212,11 @@ class SemidiscretizationHyperbolic:
     def _create(self, md, basis, sd, h):
         if getattr(self.mesh, "curved", False):
             k = self._keep
-            cd = _lib.CurvedDesc(_dptr(k["Ja"]), _dptr(k["ijn"]))
+            cd = _lib.CurvedDesc(_dptr(k["Ja"]), _dptr(k["ijn"]), _dptr(k["xy"]))
             rc = _lib.lib().tlw_create_curved(C.byref(md), C.byref(basis), C.byref(sd), C.byref(cd), C.byref(h))
             if rc != _lib.TLW_OK:
-                raise _lib.TLWError(rc, "tlw_create_curved failed (periodic mesh, plain LW and positive Jacobians "
-                                        "required)")
+                raise _lib.TLWError(rc, "tlw_create_curved failed (plain FR, positive Jacobians and, with "
+                                        "boundary faces, a boundary value function required)")
             return
         rc = _lib.lib().tlw_create(C.byref(md), C.byref(basis), C.byref(sd), C.byref(h))
         if rc != _lib.TLW_OK:
@@ -224,7 +225,9 @@ class SemidiscretizationHyperbolic:
     def _mesh_arrays(self, mesh):
         if getattr(mesh, "curved", False):
             Ja, ijn = mesh.metrics(self.solver.basis)
-            return {"nbr": np.ascontiguousarray(mesh.neighbors, dtype=np.int32), "Ja": Ja, "ijn": ijn}
+            x, y = mesh.node_coordinates(self.solver.basis.nodes)
+            return {"nbr": np.ascontiguousarray(mesh.neighbors, dtype=np.int32), "Ja": Ja, "ijn": ijn,
+                    "xy": np.ascontiguousarray(np.stack([x, y], axis=-1))}
         return {
             "nbr": np.ascontiguousarray(mesh.neighbors, dtype=np.int32),
             "invj": np.ascontiguousarray(mesh.inverse_jacobian, dtype=np.float64),
