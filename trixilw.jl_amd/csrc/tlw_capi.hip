@@ -63,6 +63,7 @@ struct tlw_ctx {
     double* d_fnorm = nullptr;
     double* d_fxy = nullptr;  // boundary face-node coordinates (curvilinear mesh with boundaries)
     bool bad_launch = false;  // a launch was refused for a missing input (reported by check_launch)
+    size_t slot_off = 0;      // offset of the CFL maxima slots in d_partial
     int pv_off = 0;           // offset of the viscous CFL partials (0: none)
     int ebv = 0;              // elements per volume workgroup
     bool positivity = false;  // Zhang-Shu limiter in the device loop
@@ -320,7 +321,11 @@ int upload_meshes(tlw_ctx* c, const tlw_mesh_desc* ms, int ninst, const tlw_curv
     const int scap = (E + surf_eb(c->N) - 1) / surf_eb(c->N) + ninst;
     // hyperbolic-parabolic: the viscous partials follow in a second block of the same size
     c->pv_off = c->parabolic ? c->vcap + scap : 0;
-    HIPCHK(c, hipMalloc(&c->d_partial, sizeof(double) * (size_t)((c->parabolic ? 2 : 1) * (c->vcap + scap) + 1)));
+    // partials, then per instance 2·kSlots CFL/viscous maxima slots, zero between reductions
+    c->slot_off = (size_t)((c->parabolic ? 2 : 1) * (c->vcap + scap) + 1);
+    const size_t npart = c->slot_off + (size_t)ninst * 2 * kSlots;
+    HIPCHK(c, hipMalloc(&c->d_partial, sizeof(double) * npart));
+    HIPCHK(c, hipMemset(c->d_partial, 0, sizeof(double) * npart));
     HIPCHK(c, hipMalloc(&c->d_vblk, sizeof(int4) * (size_t)c->vcap));
     HIPCHK(c, hipMalloc(&c->d_sblk, sizeof(int4) * (size_t)scap));
     HIPCHK(c, hipMalloc(&c->d_etile, sizeof(int) * (size_t)E));
@@ -389,6 +394,7 @@ KArgs base_args(tlw_ctx* c) {
     a.ijn = c->d_ijn;
     a.fnorm = c->d_fnorm;
     a.fxy = c->d_fxy;
+    a.slots = c->d_partial + c->slot_off;
     return a;
 }
 
@@ -442,10 +448,10 @@ void launch_volume(tlw_ctx* c, const KArgs& a, int kind) {
 void launch_finalize(tlw_ctx* c, int reduce, int use_vol) {
     if (c->N == 4)
         hipLaunchKernelGGL(k_finalize<4>, dim3(c->ninst), dim3(kFinalizeBlock), 0, c->stream, c->d_state,
-                           c->d_partial, c->d_rng, use_vol, reduce, c->pv_off);
+                           c->d_partial, c->d_rng, use_vol, reduce, c->pv_off, c->d_partial + c->slot_off);
     else
         hipLaunchKernelGGL(k_finalize<3>, dim3(c->ninst), dim3(kFinalizeBlock), 0, c->stream, c->d_state,
-                           c->d_partial, c->d_rng, use_vol, reduce, c->pv_off);
+                           c->d_partial, c->d_rng, use_vol, reduce, c->pv_off, c->d_partial + c->slot_off);
 }
 template <int N> void launch_surface_t(tlw_ctx* c, const KArgs& a) {
     dim3 g(c->sgrid), b(kSurfBlock);

@@ -147,6 +147,7 @@ struct KArgs {
     const double* ijn;
     const double* fnorm;
     const double* fxy;  // boundary face-node coordinates [E][4][n][2] (curvilinear mesh with boundaries)
+    double* slots;      // per instance: kSlots CFL maxima, then kSlots viscous maxima (k_finalize resets them)
     double t_arg, dt_arg;
     int use_state_dt;  // 1: t, dt from DevState (device loop); 0: from t_arg/dt_arg
     int step_mode;     // 1: out = u + dt * rhs; 0: out = rhs
@@ -190,6 +191,18 @@ __device__ __forceinline__ void euler_flux_dir(const double* u, double gm1, int 
 }
 
 __device__ __forceinline__ double dmax(double x, double y) { return (y > x) ? y : x; }
+
+// The CFL maxima of the node-per-lane workgroups meet in kSlots slots per
+// instance by an integer atomic maximum: non-negative doubles order like their
+// bit patterns, and a maximum does not depend on the order, so the reduction is
+// deterministic.  NaN and 0 leave a slot as dmax would.
+constexpr int kSlots = 256;
+__device__ __forceinline__ void slot_max(double* slots, int inst, int which, double v) {
+    if (v > 0.0)
+        atomicMax(reinterpret_cast<unsigned long long*>(slots + ((size_t)inst * 2 + which) * kSlots +
+                                                        (blockIdx.x & (kSlots - 1))),
+                  (unsigned long long)__double_as_longlong(v));
+}
 
 // Square root of a positive normal argument: the hardware reciprocal
 // square-root estimate refined by two steps of Goldschmidt's iteration and a
@@ -2210,8 +2223,8 @@ __global__ __launch_bounds__(kSurfBlock) void k_surface(KArgs a) {
             bm = dmax(bm, s_m[k]);
             bv = dmax(bv, s_mv[k]);
         }
-        a.partial[a.vol_grid + blockIdx.x] = bm;
-        if (par) a.partial[a.pv_off + a.vol_grid + blockIdx.x] = bv;
+        slot_max(a.slots, blk.z, 0, bm);
+        if (par) slot_max(a.slots, blk.z, 1, bv);
     }
 }
 
@@ -2268,8 +2281,8 @@ __global__ __launch_bounds__(kSurfBlock) void k_max_speed(KArgs a) {
             bm = dmax(bm, s_m[k]);
             bv = dmax(bv, s_mv[k]);
         }
-        a.partial[a.vol_grid + blockIdx.x] = bm;
-        if (par) a.partial[a.pv_off + a.vol_grid + blockIdx.x] = bv;
+        slot_max(a.slots, blk.z, 0, bm);
+        if (par) slot_max(a.slots, blk.z, 1, bv);
     }
 }
 
@@ -2433,7 +2446,7 @@ __global__ __launch_bounds__(kSurfBlock) void k_limiter(KArgs a) {
     if (tid == 0) {
         double bm = 0.0;
         for (int k = 0; k < nel; ++k) bm = dmax(bm, s_theta[k]);
-        a.partial[a.vol_grid + blockIdx.x] = bm;
+        slot_max(a.slots, blk.z, 0, bm);
     }
 }
 
@@ -2489,7 +2502,7 @@ constexpr int kFinalizeBlock = 1024;
 // SemidiscretizationHyperbolicParabolic, lwfr.jl:192-210).
 template <int N>
 __global__ __launch_bounds__(kFinalizeBlock) void k_finalize(DevState* st_all, const double* partial, const int4* rng,
-                                                           int use_vol, int reduce, int pv_off) {
+                                                           int use_vol, int reduce, int pv_off, double* slots) {
     constexpr int n = N + 1;
     __shared__ double s[kFinalizeBlock], sv[kFinalizeBlock];
     DevState* st = st_all + blockIdx.x;
@@ -2513,27 +2526,23 @@ __global__ __launch_bounds__(kFinalizeBlock) void k_finalize(DevState* st_all, c
         if (threadIdx.x == 0) adaptive_control(st, s[0], (double)n);
         return;
     }
-    // eight independent accumulators keep several loads in flight per lane
-    auto scan = [&](const double* p, int b, int e, double* m) {
-        int k = b + (int)threadIdx.x;
-        for (; k + 7 * kFinalizeBlock < e; k += 8 * kFinalizeBlock) {
-#pragma unroll
-            for (int q = 0; q < 8; ++q) m[q] = dmax(m[q], p[k + q * kFinalizeBlock]);
+    // the instance's CFL (and viscous) slots, reset for the next reduction
+    {
+        const int t = (int)threadIdx.x;
+        double* sl = slots + (size_t)blockIdx.x * 2 * kSlots;
+        s[t] = 0.0;
+        sv[t] = 0.0;
+        if (t < kSlots) {
+            s[t] = sl[t];
+            sl[t] = 0.0;
+            if (pv_off > 0) {
+                sv[t] = sl[kSlots + t];
+                sl[kSlots + t] = 0.0;
+            }
         }
-        for (; k < e; k += kFinalizeBlock) m[0] = dmax(m[0], p[k]);
-    };
-    auto reduce_all = [&](const double* p, double* sh) {
-        double m[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        if (use_vol) scan(p, r.x, r.y, m);
-        scan(p, r.z, r.w, m);
-#pragma unroll
-        for (int q = 1; q < 8; ++q) m[0] = dmax(m[0], m[q]);
-        sh[threadIdx.x] = m[0];
-    };
-    reduce_all(partial, s);
-    if (pv_off > 0) reduce_all(partial + pv_off, sv);
+    }
     __syncthreads();
-    for (int w = kFinalizeBlock / 2; w > 0; w >>= 1) {
+    for (int w = kSlots / 2; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) {
             s[threadIdx.x] = dmax(s[threadIdx.x], s[threadIdx.x + w]);
             if (pv_off > 0) sv[threadIdx.x] = dmax(sv[threadIdx.x], sv[threadIdx.x + w]);
