@@ -295,6 +295,7 @@ template <class T> struct LinAdv2D {
     double a1 = 1.0, a2 = 1.0;
     void fluxes(const T* u, T* f, T* g) const { f[0] = a1 * u[0]; g[0] = a2 * u[0]; }
     void flux(const T* u, int orientation, T* f) const { f[0] = (orientation == 1 ? a1 : a2) * u[0]; }
+    void flux_normal(const T* u, const double* nrm, T* f) const { f[0] = (a1 * nrm[0] + a2 * nrm[1]) * u[0]; }
     void speed_parts(const T*, int orientation, T& vn_abs, T& c) const {
         vn_abs = T(std::fabs(orientation == 1 ? a1 : a2));
         c = T(0.0);

@@ -81,7 +81,11 @@ void* tlwo_create(const tlwo_desc* d) {
         if (!d->periodic_x || !d->periodic_y || d->shock_capturing) return nullptr;
         if ((d->parabolic == 1) != (d->equation == 1)) return nullptr;
     }
-    if (d->curved && (d->parabolic || d->shock_capturing)) return nullptr;
+    // curvilinear shock capturing: first order; Trixi's alpha smoothing on a
+    // StructuredMesh asserts a periodic mesh
+    const bool periodic = d->periodic_x && d->periodic_y;
+    if (d->curved && d->parabolic) return nullptr;
+    if (d->curved && d->shock_capturing && (d->reconstruction != 0 || (d->alpha_smooth && !periodic))) return nullptr;
     if (d->reconstruction == 1 && (d->equation != 0 || !d->shock_capturing)) return nullptr;
     auto* c = new tlwo_ctx;
     c->equation = d->equation;

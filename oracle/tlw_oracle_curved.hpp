@@ -130,7 +130,7 @@ template <class T, class Eq> struct CurvedSolver : Solver<T, Eq> {
     // contractions and the node's J⁻¹; the Cartesian time average F̃ is kept
     // for the interfaces.  Source terms are `nothing`; the temporal-error
     // estimate (adaptive mode) is not computed.
-    void lw_volume_curved(T* du, const T* u, double dt, int e) {
+    void lw_volume_curved(T* du, const T* u, double dt, int e, double alpha_dg = 1.0) {
         const auto& D = basis.D;
         const auto& Dh = basis.Dhat;
         const int N = basis.N;
@@ -248,15 +248,107 @@ template <class T, class Eq> struct CurvedSolver : Solver<T, Eq> {
         for (int j = 0; j < n; ++j)
             for (int i = 0; i < n; ++i) {
                 for (int ii = 0; ii < n; ++ii)
-                    for (int v = 0; v < NV; ++v) du[idx(v, ii, j, e)] = du[idx(v, ii, j, e)] + Dh[ii][i] * Fc[j][i][v];
+                    for (int v = 0; v < NV; ++v)
+                        du[idx(v, ii, j, e)] = du[idx(v, ii, j, e)] + (alpha_dg * Dh[ii][i]) * Fc[j][i][v];
                 for (int jj = 0; jj < n; ++jj)
-                    for (int v = 0; v < NV; ++v) du[idx(v, i, jj, e)] = du[idx(v, i, jj, e)] + Dh[jj][j] * Gc[j][i][v];
+                    for (int v = 0; v < NV; ++v)
+                        du[idx(v, i, jj, e)] = du[idx(v, i, jj, e)] + (alpha_dg * Dh[jj][j]) * Gc[j][i][v];
                 for (int v = 0; v < NV; ++v) {
                     this->U[idx(v, i, j, e)] = Uc[j][i][v];
                     this->F[fidx(v, 0, i, j, e)] = F[j][i][v];
                     this->F[fidx(v, 1, i, j, e)] = G[j][i][v];
                 }
             }
+    }
+
+    // ---------------- shock capturing on curvilinear meshes ----------------
+    // Subcell normal of direction d (1: ξ, 2: η) at subcell face k (0-based,
+    // 1..n-1) of row r (Trixi calcflux_fv! for curvilinear meshes): the
+    // contravariant vector Ja^d of the row's first node plus, for every subcell
+    // before the face, w_q Σ_m D[q][m] Ja^d(m), summed in that order.  With
+    // `quirk`, the face n-1 normal of the reference's calc_fn_low!
+    // (structured/dg_2d.jl:111-173), whose `for i in (2, nnodes(dg))` adds the
+    // q = 0 and q = n-2 terms only.
+    void subcell_normal(int d, int e, int r, int k, double* nrm, bool quirk = false) const {
+        const auto& D = basis.D;
+        const auto& w = basis.weights;
+        auto ja = [&](int m) {
+            const size_t node = d == 1 ? nidx(m, r, e) : nidx(r, m, e);
+            return &Ja[4 * node + (d == 1 ? 0 : 2)];
+        };
+        nrm[0] = ja(0)[0];
+        nrm[1] = ja(0)[1];
+        for (int q = 0; q < k; ++q) {
+            if (quirk && q != 0 && q != n - 2) continue;
+            for (int m = 0; m < n; ++m) {
+                const double c = w[q] * D[q][m];
+                nrm[0] = nrm[0] + c * ja(m)[0];
+                nrm[1] = nrm[1] + c * ja(m)[1];
+            }
+        }
+    }
+    // flux_lax_friedrichs with a normal direction (Trixi FluxPlusDissipation of
+    // flux_central and DissipationLocalLaxFriedrichs)
+    void rusanov_normal(const T* ul, const T* ur, const double* nrm, T* out) const {
+        T fl[NV], fr[NV];
+        eq.flux_normal(ul, nrm, fl);
+        eq.flux_normal(ur, nrm, fr);
+        const T lam = eq.max_abs_speed_naive_normal(ul, ur, nrm);
+        for (int v = 0; v < NV; ++v) out[v] = 0.5 * (fl[v] + fr[v]) + (-0.5 * lam) * (ur[v] - ul[v]);
+    }
+    // fv_kernel! with FirstOrderReconstruction on a curvilinear mesh (dg_2d.jl:251-282
+    // with Trixi calcflux_fv!): Rusanov fluxes with the subcell normals between
+    // neighbouring nodes, zero at the element faces, the α-scaled FV term and
+    // fn_low (load_fn_low!).  With add = false only fn_low, from the quirky
+    // normals of calc_fn_low! (the reference's pure-DG elements).
+    void fv_kernel_fo_curved(T* du, const T* u, int e, double fv_scale, bool add) {
+        T fx[MAXN][MAXN + 1][NV], gy[MAXN + 1][MAXN][NV];
+        for (int a = 0; a < n; ++a)
+            for (int v = 0; v < NV; ++v) fx[a][0][v] = fx[a][n][v] = gy[0][a][v] = gy[n][a][v] = T(0);
+        for (int r = 0; r < n; ++r)
+            for (int k = 1; k < n; ++k) {
+                if (!add && k != 1 && k != n - 1) continue;
+                double nrm[2];
+                subcell_normal(1, e, r, k, nrm, !add);
+                rusanov_normal(&u[idx(0, k - 1, r, e)], &u[idx(0, k, r, e)], nrm, fx[r][k]);
+                subcell_normal(2, e, r, k, nrm, !add);
+                rusanov_normal(&u[idx(0, r, k - 1, e)], &u[idx(0, r, k, e)], nrm, gy[k][r]);
+            }
+        if (add) {
+            const auto& iw = basis.inv_w;
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < n; ++i)
+                    for (int v = 0; v < NV; ++v)
+                        du[idx(v, i, j, e)] = du[idx(v, i, j, e)] + fv_scale * (iw[i] * (fx[j][i + 1][v] - fx[j][i][v]) +
+                                                                                 iw[j] * (gy[j + 1][i][v] - gy[j][i][v]));
+        }
+        for (int k = 0; k < n; ++k)
+            for (int v = 0; v < NV; ++v) {
+                this->fn_low[this->sidx(v, k, 0, e)] = fx[k][1][v];
+                this->fn_low[this->sidx(v, k, 1, e)] = fx[k][n - 1][v];
+                this->fn_low[this->sidx(v, k, 2, e)] = gy[1][k][v];
+                this->fn_low[this->sidx(v, k, 3, e)] = gy[n - 1][k][v];
+            }
+    }
+    void sc_element_curved(T* du, const T* u, int e) {
+        const double a_e = this->alpha[e];
+        fv_kernel_fo_curved(du, u, e, a_e, !(std::fabs(a_e) <= 1e-12));
+    }
+    // compute_alp for StructuredMesh (structured/dg_2d.jl:2169-2192): the nodes'
+    // own inverse Jacobians
+    double compute_alp_lr(const T* ul, const T* ur, int L, int R, double Jl, double Jr, double dt, const T* fn,
+                          const T* Fn_, const T* fin_ll, const T* fin_rr) const {
+        const double alp = 0.5 * (this->alpha[L] + this->alpha[R]);
+        T Fn[NV], low[NV];
+        for (int v = 0; v < NV; ++v) Fn[v] = (1.0 - alp) * Fn_[v] + alp * fn[v];
+        const double lx = 0.5;
+        const double cl = dt * Jl / (basis.weights[n - 1] * lx);
+        for (int v = 0; v < NV; ++v) low[v] = ul[v] - cl * (Fn[v] - fin_ll[v]);
+        if (!eq.is_admissible(low)) return 1.0;
+        const double cr = dt * Jr / (basis.weights[0] * lx);
+        for (int v = 0; v < NV; ++v) low[v] = ur[v] - cr * (fin_rr[v] - Fn[v]);
+        if (!eq.is_admissible(low)) return 1.0;
+        return alp;
     }
 
     // Normal of the face node shared by left and right elements: the right
@@ -291,11 +383,24 @@ template <class T, class Eq> struct CurvedSolver : Solver<T, Eq> {
                     const T* ull = &u[idx(0, li, lj, left)];
                     const T* urr = &u[idx(0, ri, rj, e)];
                     const T lam = eq.max_abs_speed_naive_normal(ull, urr, nrm);
+                    T Fn[NV];
+                    for (int v = 0; v < NV; ++v)
+                        Fn[v] = sgn * (0.5 * (fll[v] + frr[v]) -
+                                       0.5 * lam * (this->U[idx(v, ri, rj, e)] - this->U[idx(v, li, lj, left)]));
+                    if (this->sc) {
+                        // compute_low_order_flux and compute_alp (structured/dg_2d.jl:2276-2285)
+                        T fn[NV];
+                        rusanov_normal(ull, urr, nrm, fn);
+                        for (int v = 0; v < NV; ++v) fn[v] = sgn * fn[v];
+                        const T* fin_ll = &this->fn_low[sidx(0, l, 2 * o - 1, left)];
+                        const T* fin_rr = &this->fn_low[sidx(0, l, 2 * o - 2, e)];
+                        const double alp = compute_alp_lr(ull, urr, left, e, ijn[nidx(li, lj, left)],
+                                                          ijn[nidx(ri, rj, e)], this->sc_dt, fn, Fn, fin_ll, fin_rr);
+                        for (int v = 0; v < NV; ++v) Fn[v] = alp * fn[v] + (1.0 - alp) * Fn[v];
+                    }
                     for (int v = 0; v < NV; ++v) {
-                        const T Fn = sgn * (0.5 * (fll[v] + frr[v]) -
-                                            0.5 * lam * (this->U[idx(v, ri, rj, e)] - this->U[idx(v, li, lj, left)]));
-                        this->sfv[sidx(v, l, 2 * o - 1, left)] = Fn;
-                        this->sfv[sidx(v, l, 2 * o - 2, e)] = Fn;
+                        this->sfv[sidx(v, l, 2 * o - 1, left)] = Fn[v];
+                        this->sfv[sidx(v, l, 2 * o - 2, e)] = Fn[v];
                     }
                 }
             }
@@ -385,10 +490,13 @@ template <class T, class Eq> struct CurvedSolver : Solver<T, Eq> {
     }
 
     void rhs_lw(T* du, const T* u, double t, double dt) override {
-        (void)t;
         this->reset_du(du);
+        if (this->sc) this->sc_prepare(u, dt);
 #pragma omp parallel for schedule(static)
-        for (int e = 0; e < mesh.E; ++e) lw_volume_curved(du, u, dt, e);
+        for (int e = 0; e < mesh.E; ++e) {
+            lw_volume_curved(du, u, dt, e, this->sc ? 1.0 - this->alpha[e] : 1.0);
+            if (this->sc) sc_element_curved(du, u, e);
+        }
         calc_interface_flux_curved(u);
         if (!mesh.bd_elem.empty()) calc_boundary_flux_curved(u, t, dt);
         surface_integral_and_jacobian_curved(du);
@@ -399,7 +507,7 @@ template <class T, class Eq> struct CurvedSolver : Solver<T, Eq> {
     // contravariant F̃ = cv_f/2 + cv_f_t/8 accumulated separately from the
     // contravariant fluxes of each Taylor state; U = u/2 + ut/8, U2 = u + ut/6.
     // The low-order update u_np1_low (an adaptive-mode output) is left out.
-    void mdrk_curved_1(T* du, const T* u, double dt, int e) {
+    void mdrk_curved_1(T* du, const T* u, double dt, int e, double alpha_dg = 1.0) {
         const auto& D = basis.D;
         const auto& Dh = basis.Dhat;
         T ut[MAXN][MAXN][NV], Ft[MAXN][MAXN][NV], Gt[MAXN][MAXN][NV];
@@ -444,9 +552,11 @@ template <class T, class Eq> struct CurvedSolver : Solver<T, Eq> {
                     this->U2[idx(v, i, j, e)] = this->U2[idx(v, i, j, e)] + (1.0 / 6.0) * ut[j][i][v];
                 }
                 for (int ii = 0; ii < n; ++ii)
-                    for (int v = 0; v < NV; ++v) du[idx(v, ii, j, e)] = du[idx(v, ii, j, e)] + Dh[ii][i] * Ft[j][i][v];
+                    for (int v = 0; v < NV; ++v)
+                        du[idx(v, ii, j, e)] = du[idx(v, ii, j, e)] + (alpha_dg * Dh[ii][i]) * Ft[j][i][v];
                 for (int jj = 0; jj < n; ++jj)
-                    for (int v = 0; v < NV; ++v) du[idx(v, i, jj, e)] = du[idx(v, i, jj, e)] + Dh[jj][j] * Gt[j][i][v];
+                    for (int v = 0; v < NV; ++v)
+                        du[idx(v, i, jj, e)] = du[idx(v, i, jj, e)] + (alpha_dg * Dh[jj][j]) * Gt[j][i][v];
             }
     }
 
@@ -476,7 +586,7 @@ template <class T, class Eq> struct CurvedSolver : Solver<T, Eq> {
 
     // mdrk_kernel_2! for StructuredMesh (structured/multi_derivative_2d.jl:245-445):
     // F = F2 + f_t(us)/3 (Cartesian), F̃ = Ja·F2 + cv_f_t(us)/3, U = U2 + ust/3.
-    void mdrk_curved_2(T* du, double dt, int e) {
+    void mdrk_curved_2(T* du, double dt, int e, double alpha_dg = 1.0) {
         const auto& D = basis.D;
         const auto& Dh = basis.Dhat;
         T ust[MAXN][MAXN][NV], Ft[MAXN][MAXN][NV], Gt[MAXN][MAXN][NV];
@@ -517,9 +627,11 @@ template <class T, class Eq> struct CurvedSolver : Solver<T, Eq> {
                     this->U[idx(v, i, j, e)] = this->U[idx(v, i, j, e)] + (1.0 / 3.0) * ust[j][i][v];
                 }
                 for (int ii = 0; ii < n; ++ii)
-                    for (int v = 0; v < NV; ++v) du[idx(v, ii, j, e)] = du[idx(v, ii, j, e)] + Dh[ii][i] * Ft[j][i][v];
+                    for (int v = 0; v < NV; ++v)
+                        du[idx(v, ii, j, e)] = du[idx(v, ii, j, e)] + (alpha_dg * Dh[ii][i]) * Ft[j][i][v];
                 for (int jj = 0; jj < n; ++jj)
-                    for (int v = 0; v < NV; ++v) du[idx(v, i, jj, e)] = du[idx(v, i, jj, e)] + Dh[jj][j] * Gt[j][i][v];
+                    for (int v = 0; v < NV; ++v)
+                        du[idx(v, i, jj, e)] = du[idx(v, i, jj, e)] + (alpha_dg * Dh[jj][j]) * Gt[j][i][v];
             }
     }
 
@@ -528,8 +640,12 @@ template <class T, class Eq> struct CurvedSolver : Solver<T, Eq> {
     void rhs_mdrk1(T* du, const T* u, double t, double dt) override {
         this->ensure_mdrk();
         this->reset_du(du);
+        if (this->sc) this->sc_prepare(u, dt);
 #pragma omp parallel for schedule(static)
-        for (int e = 0; e < mesh.E; ++e) mdrk_curved_1(du, u, dt, e);
+        for (int e = 0; e < mesh.E; ++e) {
+            mdrk_curved_1(du, u, dt, e, this->sc ? 1.0 - this->alpha[e] : 1.0);
+            if (this->sc) sc_element_curved(du, u, e);
+        }
         calc_interface_flux_curved(u);
         if (!mesh.bd_elem.empty()) calc_boundary_flux_curved(u, t, dt);
         surface_integral_and_jacobian_curved(du);
@@ -537,8 +653,12 @@ template <class T, class Eq> struct CurvedSolver : Solver<T, Eq> {
     void rhs_mdrk2(T* du, const T* u, double t, double dt) override {
         this->ensure_mdrk();
         this->reset_du(du);
+        if (this->sc) this->sc_prepare(u, dt);
 #pragma omp parallel for schedule(static)
-        for (int e = 0; e < mesh.E; ++e) mdrk_curved_2(du, dt, e);
+        for (int e = 0; e < mesh.E; ++e) {
+            mdrk_curved_2(du, dt, e, this->sc ? 1.0 - this->alpha[e] : 1.0);
+            if (this->sc) sc_element_curved(du, u, e);
+        }
         calc_interface_flux_curved(u);
         if (!mesh.bd_elem.empty()) calc_boundary_flux_curved(u, t, dt);
         surface_integral_and_jacobian_curved(du);

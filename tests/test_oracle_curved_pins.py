@@ -239,3 +239,114 @@ def test_warped_mesh_dirichlet_vortex_convergence(oracle_mod, N, eoc_min):
         errs.append(np.sqrt(np.mean((u - ic(x, y, t))[..., 0] ** 2)))
     eoc = np.log(errs[1] / errs[2]) / np.log(2)
     assert eoc > eoc_min, (errs, eoc)
+
+
+# ---------------------------------------------------------------- shock capturing
+def _euler_flux_normal(u, nrm, g=1.4):
+    rho = u[..., 0]
+    v1, v2 = u[..., 1] / rho, u[..., 2] / rho
+    p = (g - 1) * (u[..., 3] - 0.5 * (u[..., 1] * v1 + u[..., 2] * v2))
+    vn = v1 * nrm[0] + v2 * nrm[1]
+    return np.stack([rho * vn, rho * vn * v1 + p * nrm[0], rho * vn * v2 + p * nrm[1], (u[..., 3] + p) * vn], -1)
+
+
+def _rusanov_normal(ul, ur, nrm, g=1.4):
+    def parts(u):
+        rho = u[..., 0]
+        v1, v2 = u[..., 1] / rho, u[..., 2] / rho
+        p = (g - 1) * (u[..., 3] - 0.5 * (u[..., 1] * v1 + u[..., 2] * v2))
+        return np.abs(v1 * nrm[0] + v2 * nrm[1]), np.sqrt(g * p / rho) * np.hypot(*nrm)
+
+    al, cl = parts(ul)
+    ar, cr = parts(ur)
+    lam = np.maximum(al, ar) + np.maximum(cl, cr)
+    return 0.5 * (_euler_flux_normal(ul, nrm, g) + _euler_flux_normal(ur, nrm, g)) - 0.5 * lam * (ur - ul)
+
+
+@pytest.mark.parametrize("N", [3, 4])
+@pytest.mark.parametrize("dtf", [0.4, 1.5])
+def test_affine_mapping_shock_capturing_equals_treemesh(oracle_mod, N, dtf):
+    """First-order shock capturing on an affine StructuredMesh (subcell normals
+    of Trixi's calcflux_fv!, per-node J⁻¹ in compute_alp, the calc_fn_low!
+    normals) equals the TreeMesh shock capturing with the Structured order:
+    LW and both MDRK stages, blending factors with exact zeros (pure DG), and at
+    1.5× the stable dt, where the admissibility fallback switches interfaces."""
+    from trixilw_hip.equations import IsentropicVortex
+
+    nx, ny, L = 6, 4, 20.0
+    h = L / nx
+    ic = IsentropicVortex(center=(L / 2, h * ny / 2), period=(L, h * ny))
+    tree = oracle_mod.Oracle(N, nx, ny, 0.0, 0.0, h, flux_order=1, shock_capturing=True)
+    curv = oracle_mod.Oracle(N, nx, ny, 0.0, 0.0, h, mapping=(oracle_mod.MAP_AFFINE, (0.0, 0.0, L, h * ny)),
+                             shock_capturing=True)
+    x, y = tree.node_coords()
+    rng = np.random.default_rng(50 + N)
+    u = ic(x, y, 0.0) * (1 + 5e-2 * rng.standard_normal(x.shape + (4,)))
+    alpha = np.where(rng.random(nx * ny) < 0.3, 0.0, rng.random(nx * ny))
+    for o in (tree, curv):
+        o.set_alpha(alpha)
+    dt = dtf * tree.max_dt(u)
+    # the affine metric terms carry the round-off of D applied to the coordinates (~1e-14
+    # relative), which the flux cancellation of the larger step amplifies to ~1e-12
+    tol = 1e-12 if dtf < 1 else 5e-12
+    assert rel_linf(curv.rhs(u, 0.2, dt), tree.rhs(u, 0.2, dt)) < tol
+    assert rel_linf(curv.aux("fn_low"), tree.aux("fn_low") * (h / 2)) < tol  # contravariant: |Ja| = h/2
+    d1 = tree.rhs(u, 0.2, dt, kind="mdrk1")
+    assert rel_linf(curv.rhs(u, 0.2, dt, kind="mdrk1"), d1) < tol
+    us = u + 0.5 * dt * d1
+    tree.set_us(us)
+    curv.set_us(us)
+    assert rel_linf(curv.rhs(u, 0.2, dt, kind="mdrk2"), tree.rhs(u, 0.2, dt, kind="mdrk2")) < tol
+
+
+@pytest.mark.parametrize("N", [3, 4])
+def test_warped_mesh_shock_capturing_free_stream_and_conservation(oracle_mod, N):
+    """The subcell normals telescope to the element face normals (SBP), so a
+    constant state stays at rest at any blending; blended residuals conserve."""
+    o = warped(oracle_mod, N, 6, A=0.1, shock_capturing=True)
+    x, y = o.node_coords()
+    c = np.zeros(x.shape + (4,))
+    c[...] = [1.1, 0.4, -0.3, 2.7]
+    rng = np.random.default_rng(60 + N)
+    for alpha in (np.ones(o.E), rng.random(o.E)):
+        o.set_alpha(alpha)
+        assert np.max(np.abs(o.rhs(c, 0.0, 0.4 * o.max_dt(c)))) < 1e-12
+    u = vortex()(x, y, 0.0) * (1 + 1e-2 * rng.standard_normal(x.shape + (4,)))
+    du = o.rhs(u, 0.0, 0.4 * o.max_dt(u))
+    _, w, _, _ = o.basis()
+    W = np.outer(w, w)[None] / o.aux("inverse_jacobian").reshape(x.shape)
+    total = np.einsum("eji,ejiv->v", W, du)
+    scale = np.einsum("eji,ejiv->v", W, np.abs(du))
+    assert np.all(np.abs(total) <= 1e-13 * scale)
+
+
+@pytest.mark.parametrize("N", [3, 4])
+def test_warped_mesh_fn_low_normals(oracle_mod, N):
+    """fn_low next to the + ξ face: blended elements use Trixi's accumulated
+    subcell normal Ja¹(1) + Σ_{q<n-1} w_q (D Ja¹)(q); pure-DG elements the
+    reference's calc_fn_low! normal, which adds the q = 1 and q = n-1 terms only
+    (its loop runs over the tuple (2, nnodes))."""
+    n = N + 1
+    o = warped(oracle_mod, N, 4, A=0.12, shock_capturing=True)
+    x, y = o.node_coords()
+    rng = np.random.default_rng(70 + N)
+    u = vortex()(x, y, 0.0) * (1 + 1e-2 * rng.standard_normal(x.shape + (4,)))
+    alpha = np.where(np.arange(o.E) % 2 == 0, 0.0, 0.6)
+    o.set_alpha(alpha)
+    o.rhs(u, 0.0, 0.3 * o.max_dt(u))
+    fl = o.aux("fn_low").reshape(o.E, 4, n, 4)
+    Ja = o.aux("contravariant_vectors").reshape(o.E, n, n, 4)  # [e, j, i, (Ja11, Ja12, Ja21, Ja22)]
+    _, w, D, _ = o.basis()
+    D = np.asarray(D).reshape(n, n)
+    for e in (0, 1):
+        for j in range(n):
+            ja = Ja[e, j, :, 0:2]  # Ja¹ along the row
+            dja = D @ ja  # (D Ja¹)(q)
+            qs = (0, n - 2) if alpha[e] == 0.0 else range(n - 1)
+            nrm = ja[0] + sum(w[q] * dja[q] for q in qs)
+            expect = _rusanov_normal(u[e, j, n - 2], u[e, j, n - 1], nrm)
+            np.testing.assert_allclose(fl[e, 1, j], expect, rtol=1e-12, atol=1e-13)
+    # the two normals differ on this curved mesh, so the check tells them apart
+    ja = Ja[0, 1, :, 0:2]
+    dja = D @ ja
+    assert np.max(np.abs(sum(w[q] * dja[q] for q in range(1, n - 2)))) > 1e-4
