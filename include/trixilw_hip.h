@@ -171,10 +171,16 @@ int tlw_create_parabolic(const tlw_mesh_desc* mesh, const tlw_basis_desc* basis,
  * solver->surface_flux_order says.
  * With TLW_MDRK, tlw_rhs_mdrk1/2 are rhs_mdrk1!/2! of src/solvers/dgsem_structured/
  * multi_derivative_2d.jl:1-80 (mdrk_kernel_1!/2!, 82-445: F̃ = f/2 + f_t/8, then F2 + f_t(us)/3,
- * in contravariant form for the volume term).  Plain VolumeIntegralFR; shock capturing returns
- * TLW_ENOTSUP, and so do the positivity limiter, adaptive steps,
- * tlw_adjoint_lw and tlw_rebuild_mesh on such a context.  A non-positive inverse Jacobian
- * returns TLW_EINVAL. */
+ * in contravariant form for the volume term).  VolumeIntegralFRShockCapturing with
+ * TLW_RECON_FIRST_ORDER blends as on the TreeMesh (IndicatorHennemannGassner on uⁿ) with Trixi's
+ * calcflux_fv! for curvilinear meshes (structured/dg_2d.jl:111-173: Rusanov fluxes with the
+ * subcell normals Ja^l(node 0) + Σ_{q<k} w_q (D Ja^l)(q); a pure-DG element's fn_low at subcell
+ * face n-1 takes the reference's calc_fn_low! normal, the q = 0 and q = n-2 terms only) and
+ * compute_alp with each face node's own inverse Jacobian (dg_2d.jl:2169-2192); boundary faces are
+ * not blended.  TLW_RECON_MUSCL_HANCOCK, and alpha smoothing on a mesh with boundaries (Trixi
+ * asserts a periodic StructuredMesh there), return TLW_ENOTSUP, and so do parabolic terms,
+ * several instances, the positivity limiter, adaptive steps, tlw_adjoint_lw and
+ * tlw_rebuild_mesh on such a context.  A non-positive inverse Jacobian returns TLW_EINVAL. */
 int tlw_create_curved(const tlw_mesh_desc* mesh, const tlw_basis_desc* basis, const tlw_solver_desc* solver,
                       const tlw_curved_desc* metrics, tlw_ctx** out);
 int tlw_n_instances(const tlw_ctx* ctx, int32_t* n_instances);

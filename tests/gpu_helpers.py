@@ -62,3 +62,56 @@ def to_lex(mesh, u_dev):
 def assert_close(a, b, tol=TOL, what=""):
     l2, li = rel_l2(a, b), rel_linf(a, b)
     assert l2 <= tol and li <= tol, f"{what}: rel L2 {l2:.3e}, rel Linf {li:.3e}"
+
+
+GAMMA = 1.4
+
+
+def warp_mapping(L, A):
+    """[0, L]² warped by A sin 2πs sin 2πt in both coordinates (s, t the unit
+    square coordinates); periodic, the warp vanishes on the domain boundary."""
+
+    def mapping(xi, eta):
+        s, t = 0.5 * (xi + 1.0), 0.5 * (eta + 1.0)
+        w = A * np.sin(2.0 * np.pi * s) * np.sin(2.0 * np.pi * t)
+        return L * (s + w), L * (t + w)
+
+    return mapping
+
+
+def setup_curved(oracle_mod, N, nx, ny, kind="warp", L=20.0, A=0.05, td="lw", seed=0, amp=1e-3, keep_aux=False,
+                 periodic=True, sc=False, alpha_max=0.5, alpha_smooth=True):
+    """Isentropic vortex on a warped or affine StructuredMesh of [0, L]², the
+    device semidiscretization and the oracle on the same mesh with the oracle's
+    metric terms on both sides.  A non-periodic mesh takes the vortex itself (no
+    periodic images) as time-averaged Dirichlet data.  ``sc`` selects
+    VolumeIntegralFRShockCapturing (Hennemann-Gassner on density_pressure,
+    first-order subcell FV)."""
+    import trixilw_hip as tlw
+
+    eq = tlw.CompressibleEulerEquations2D(GAMMA)
+    period = (L, L * ny / nx if kind == "affine" else L) if periodic else (1e9, 1e9)
+    ic = tlw.IsentropicVortex(center=(L / 2, L / 2), period=period)
+    if kind == "warp":
+        mesh = tlw.StructuredMesh((nx, ny), warp_mapping(L, A), periodicity=periodic)
+        mp = (oracle_mod.MAP_WARP, (L, A))
+    else:
+        Ly = L * ny / nx
+        mesh = tlw.StructuredMesh((nx, ny), (0.0, 0.0), (L, Ly), periodicity=periodic)
+        mp = (oracle_mod.MAP_AFFINE, (0.0, 0.0, L, Ly))
+    vi = tlw.VolumeIntegralFR(tlw.MDRK() if td == "mdrk" else tlw.LW())
+    if sc:
+        ind = tlw.IndicatorHennemannGassner(eq, None, alpha_max=alpha_max, alpha_min=0.001, alpha_smooth=alpha_smooth)
+        vi = tlw.VolumeIntegralFRShockCapturing(ind, volume_integralFR=vi)
+    solver = tlw.DGSEM(N, volume_integral=vi)
+    okw = {} if periodic else dict(periodic=(False, False), bc_kind=ic.kind, bc_par=ic.params)
+    if sc:
+        okw.update(shock_capturing=True, alpha_max=alpha_max, alpha_min=0.001, alpha_smooth=alpha_smooth)
+    o = oracle_mod.Oracle(N, nx, ny, 0.0, 0.0, L / nx, gamma=GAMMA, mapping=mp, **okw)
+    mesh.set_metrics(solver.basis, o.aux("contravariant_vectors"), o.aux("inverse_jacobian"))
+    semi = tlw.SemidiscretizationHyperbolic(mesh, tlw.get_time_discretization(solver), eq, ic, solver,
+                                            boundary_conditions=None if periodic else ic, keep_aux=keep_aux)
+    x, y = o.node_coords()
+    rng = np.random.default_rng(seed)
+    u_lex = ic(x, y, 0.0) * (1.0 + amp * rng.standard_normal(x.shape + (4,)))
+    return semi, o, u_lex, mesh

@@ -62,6 +62,8 @@ struct tlw_ctx {
     double* d_ijn = nullptr;
     double* d_fnorm = nullptr;
     double* d_fxy = nullptr;  // boundary face-node coordinates (curvilinear mesh with boundaries)
+    double* d_snrm = nullptr;  // subcell normals (curvilinear shock capturing) [E][n²][4]
+    double* d_sqrk = nullptr;  // calc_fn_low! normals of subcell face n-1 [E][n][4]
     bool bad_launch = false;  // a launch was refused for a missing input (reported by check_launch)
     size_t slot_off = 0;      // offset of the CFL maxima slots in d_partial
     int pv_off = 0;           // offset of the viscous CFL partials (0: none)
@@ -131,7 +133,7 @@ void free_mesh(tlw_ctx* c) {
     void* ptrs[] = {c->d_nbr,       c->d_invj,  c->d_center, c->d_length, c->d_trace,   c->d_buf[0],    c->d_buf[1],
                     c->d_us,        c->d_auxU,  c->d_auxF,   c->d_partial, c->d_alpha_tmp, c->d_alpha,
                     c->d_vblk,      c->d_sblk,  c->d_etile,  c->d_rng,    c->d_state,   c->d_cur,
-                    c->d_Ja,        c->d_ijn,   c->d_fnorm,  c->d_fxy};
+                    c->d_Ja,        c->d_ijn,   c->d_fnorm,  c->d_fxy,     c->d_snrm,      c->d_sqrk};
     for (void* p : ptrs) (void)hipFree(p);
     if (c->h_state) (void)hipHostFree(c->h_state);
     c->d_nbr = nullptr;
@@ -143,7 +145,7 @@ void free_mesh(tlw_ctx* c) {
     c->d_state = nullptr;
     c->h_state = nullptr;
     c->d_cur = nullptr;
-    c->d_Ja = c->d_ijn = c->d_fnorm = c->d_fxy = nullptr;
+    c->d_Ja = c->d_ijn = c->d_fnorm = c->d_fxy = c->d_snrm = c->d_sqrk = nullptr;
 }
 
 // Workgroup tables for the current tile: volume workgroups of ebv elements and
@@ -211,6 +213,30 @@ int validate_mesh(tlw_ctx* c, const tlw_mesh_desc* m) {
 // (the normal's sign factor of calc_interface_flux! is then 1), and the
 // interface normals of every face node, taken from the right element's face
 // node so that both elements of an interface use the same bits.
+// Subcell normals of Trixi's calcflux_fv! for curvilinear meshes: direction d
+// (1: ξ, 2: η), row r of element e, subcell face k: Ja^d of the row's first node
+// plus, for each subcell q < k, Σ_m w_q D[q][m] Ja^d(m), summed in that order;
+// with quirk, the q = 0 and q = n-2 terms only (the reference's calc_fn_low!,
+// structured/dg_2d.jl:111-173, loops over the tuple (2, nnodes)).
+void subcell_normal(const DevConst& h, int n, const double* Ja, int d, size_t e, int r, int k, bool quirk,
+                    double* nrm) {
+    const size_t NN = (size_t)n * n;
+    auto ja = [&](int m) {
+        const size_t node = e * NN + (d == 1 ? (size_t)n * r + m : (size_t)n * m + r);
+        return Ja + 4 * node + (d == 1 ? 0 : 2);
+    };
+    nrm[0] = ja(0)[0];
+    nrm[1] = ja(0)[1];
+    for (int q = 0; q < k; ++q) {
+        if (quirk && q != 0 && q != n - 2) continue;
+        for (int m = 0; m < n; ++m) {
+            const double cq = h.weights[q] * h.D[q][m];
+            nrm[0] = nrm[0] + cq * ja(m)[0];
+            nrm[1] = nrm[1] + cq * ja(m)[1];
+        }
+    }
+}
+
 int prepare_curved(tlw_ctx* c, const tlw_mesh_desc* m, const tlw_curved_desc* cv, std::vector<double>& fnorm,
                    std::vector<double>& invj, std::vector<double>& fxy) {
     if (!cv || !cv->contravariant_vectors_host || !cv->inverse_jacobian_host)
@@ -278,9 +304,28 @@ int upload_meshes(tlw_ctx* c, const tlw_mesh_desc* ms, int ninst, const tlw_curv
     if (boundary && c->parabolic)
         return fail(c, TLW_ENOTSUP, "the hyperbolic-parabolic path needs a periodic mesh");
     std::vector<double> fnorm, cinvj, fxy;
+    std::vector<double> snrm, sqrk;
     if (c->curved) {
         int rc = prepare_curved(c, &ms[0], cv, fnorm, cinvj, fxy);
         if (rc) return rc;
+        if (c->sc) {
+            const int n = c->n, NN = n * n, Ec = ms[0].n_elements;
+            snrm.assign((size_t)Ec * NN * 4, 0.0);
+            sqrk.assign((size_t)Ec * n * 4, 0.0);
+            const double* Ja = cv->contravariant_vectors_host;
+            for (size_t e = 0; e < (size_t)Ec; ++e) {
+                for (int jj = 0; jj < n; ++jj)
+                    for (int ii = 0; ii < n; ++ii) {
+                        double* o = &snrm[(e * NN + (size_t)ii + (size_t)n * jj) * 4];
+                        if (ii > 0) subcell_normal(c->hconst, n, Ja, 1, e, jj, ii, false, o);
+                        if (jj > 0) subcell_normal(c->hconst, n, Ja, 2, e, ii, jj, false, o + 2);
+                    }
+                for (int l = 0; l < n; ++l) {
+                    subcell_normal(c->hconst, n, Ja, 1, e, l, n - 1, true, &sqrk[(e * n + l) * 4]);
+                    subcell_normal(c->hconst, n, Ja, 2, e, l, n - 1, true, &sqrk[(e * n + l) * 4 + 2]);
+                }
+            }
+        }
     }
     free_mesh(c);
     c->aux_valid = c->alpha_valid = false;
@@ -364,6 +409,12 @@ int upload_meshes(tlw_ctx* c, const tlw_mesh_desc* ms, int ninst, const tlw_curv
             HIPCHK(c, hipMalloc(&c->d_fxy, sizeof(double) * fxy.size()));
             HIPCHK(c, hipMemcpy(c->d_fxy, fxy.data(), sizeof(double) * fxy.size(), hipMemcpyHostToDevice));
         }
+        if (!snrm.empty()) {
+            HIPCHK(c, hipMalloc(&c->d_snrm, sizeof(double) * snrm.size()));
+            HIPCHK(c, hipMalloc(&c->d_sqrk, sizeof(double) * sqrk.size()));
+            HIPCHK(c, hipMemcpy(c->d_snrm, snrm.data(), sizeof(double) * snrm.size(), hipMemcpyHostToDevice));
+            HIPCHK(c, hipMemcpy(c->d_sqrk, sqrk.data(), sizeof(double) * sqrk.size(), hipMemcpyHostToDevice));
+        }
     }
     return build_tables(c);
 }
@@ -395,6 +446,8 @@ KArgs base_args(tlw_ctx* c) {
     a.fnorm = c->d_fnorm;
     a.fxy = c->d_fxy;
     a.slots = c->d_partial + c->slot_off;
+    a.snrm = c->d_snrm;
+    a.sqrk = c->d_sqrk;
     return a;
 }
 
@@ -416,21 +469,29 @@ template <int N, bool SC, bool PAR = false> void launch_volume_sc(tlw_ctx* c, co
     else if (c->ebv == ch[1]) launch_volume_eb<N, ch[1], SC, PAR>(c, a, kind);
     else launch_volume_eb<N, ch[2], SC, PAR>(c, a, kind);
 }
-template <int N, int EB> void launch_volume_curved_eb(tlw_ctx* c, const KArgs& a, int kind) {
+template <int N, int EB, bool SC> void launch_volume_curved_eb(tlw_ctx* c, const KArgs& a, int kind) {
     dim3 g(c->vgrid), b(VGeo<N, EB>::NT);
     if (kind == VOL_LW)
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_LW, false, false, true>), g, b, 0, c->stream, a);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_LW, SC, false, true>), g, b, 0, c->stream, a);
     else if (kind == VOL_MDRK1)
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_MDRK1, false, false, true>), g, b, 0, c->stream, a);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_MDRK1, SC, false, true>), g, b, 0, c->stream, a);
     else
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_MDRK2, false, false, true>), g, b, 0, c->stream, a);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_MDRK2, SC, false, true>), g, b, 0, c->stream, a);
+}
+template <int N, bool SC> void launch_volume_curved(tlw_ctx* c, const KArgs& a, int kind) {
+    constexpr const int* ch = N == 4 ? kEbChoices4 : kEbChoices3;
+    if (c->ebv == ch[0]) launch_volume_curved_eb<N, ch[0], SC>(c, a, kind);
+    else if (c->ebv == ch[1]) launch_volume_curved_eb<N, ch[1], SC>(c, a, kind);
+    else launch_volume_curved_eb<N, ch[2], SC>(c, a, kind);
 }
 template <int N> void launch_volume_t(tlw_ctx* c, const KArgs& a, int kind) {
-    if (c->curved) {  // plain LW or MDRK (adaptive steps are rejected on such a context)
-        constexpr const int* ch = N == 4 ? kEbChoices4 : kEbChoices3;
-        if (c->ebv == ch[0]) launch_volume_curved_eb<N, ch[0]>(c, a, kind);
-        else if (c->ebv == ch[1]) launch_volume_curved_eb<N, ch[1]>(c, a, kind);
-        else launch_volume_curved_eb<N, ch[2]>(c, a, kind);
+    if (c->curved) {  // LW or MDRK (adaptive steps are rejected on such a context)
+        if (c->sc) {
+            hipLaunchKernelGGL(k_indicator<N>, dim3(c->sgrid), dim3(kSurfBlock), 0, c->stream, a, c->d_alpha_tmp);
+            launch_volume_curved<N, true>(c, a, kind);
+        } else {
+            launch_volume_curved<N, false>(c, a, kind);
+        }
     } else if (c->parabolic) {
         launch_volume_sc<N, false, true>(c, a, kind);
     } else if (c->sc) {
@@ -461,8 +522,14 @@ template <int N> void launch_surface_t(tlw_ctx* c, const KArgs& a) {
         return;
     }
     if (c->curved) {
-        if (c->has_boundary) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_surface<N, true, false, true>), g, b, 0, c->stream, a);
-        else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_surface<N, false, false, true>), g, b, 0, c->stream, a);
+        if (c->sc) {
+            if (c->has_boundary) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_surface<N, true, true, true>), g, b, 0, c->stream, a);
+            else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_surface<N, false, true, true>), g, b, 0, c->stream, a);
+        } else if (c->has_boundary) {
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_surface<N, true, false, true>), g, b, 0, c->stream, a);
+        } else {
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_surface<N, false, false, true>), g, b, 0, c->stream, a);
+        }
     } else if (c->sc) {
         if (c->has_boundary) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_surface<N, true, true>), g, b, 0, c->stream, a);
         else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_surface<N, false, true>), g, b, 0, c->stream, a);
@@ -1137,7 +1204,7 @@ int create_common(int ninst, const tlw_mesh_desc* meshes, const tlw_basis_desc* 
     h.flux_order = solver->surface_flux_order;
     c->sc = solver->shock_capturing != 0;
     if (c->sc) {
-        if (!solver->inverse_vandermonde_host || solver->surface_flux_order != TLW_FLUX_TREE ||
+        if (!solver->inverse_vandermonde_host || (solver->surface_flux_order != TLW_FLUX_TREE && !cv) ||
             !(solver->alpha_max >= 0.0) || !(solver->alpha_min >= 0.0)) {
             delete c;
             return TLW_EINVAL;
@@ -1198,8 +1265,9 @@ int create_common(int ninst, const tlw_mesh_desc* meshes, const tlw_basis_desc* 
         h.visc_kappa = solver->gamma * (1.0 / (solver->gamma - 1.0)) / par->prandtl;
     }
     if (cv) {
-        // StructuredMesh: plain FR, LW or MDRK, periodic (boundaries are rejected at upload)
-        if (c->sc || c->parabolic || ninst != 1) {
+        // StructuredMesh: LW or MDRK; shock capturing with the first-order subcell
+        // scheme (MUSCL-Hancock is rejected by the reference on such meshes)
+        if (c->parabolic || ninst != 1 || (c->sc && h.recon != TLW_RECON_FIRST_ORDER)) {
             delete c;
             return TLW_ENOTSUP;
         }
@@ -1214,6 +1282,8 @@ int create_common(int ninst, const tlw_mesh_desc* meshes, const tlw_basis_desc* 
     if (hipMemcpy(c->d_const, &h, sizeof(DevConst), hipMemcpyHostToDevice) != hipSuccess) return bail(TLW_EHIP);
     int rc = upload_meshes(c, meshes, ninst, cv);
     if (rc) return bail(rc);
+    // Trixi's alpha smoothing on a StructuredMesh assumes a periodic mesh
+    if (c->curved && c->sc && h.alpha_smooth && c->has_boundary) return bail(TLW_ENOTSUP);
     *out = c;
     return TLW_OK;
 }

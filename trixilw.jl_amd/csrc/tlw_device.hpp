@@ -148,6 +148,11 @@ struct KArgs {
     const double* fnorm;
     const double* fxy;  // boundary face-node coordinates [E][4][n][2] (curvilinear mesh with boundaries)
     double* slots;      // per instance: kSlots CFL maxima, then kSlots viscous maxima (k_finalize resets them)
+    // curvilinear shock capturing: per node the subcell normals of its lower ξ
+    // and η subcell faces [E][n²][4], and the calc_fn_low! normals of subcell
+    // face n-1 per row (ξ) and column (η) [E][n][4]
+    const double* snrm;
+    const double* sqrk;
     double t_arg, dt_arg;
     int use_state_dt;  // 1: t, dt from DevState (device loop); 0: from t_arg/dt_arg
     int step_mode;     // 1: out = u + dt * rhs; 0: out = rhs
@@ -531,6 +536,76 @@ __device__ __forceinline__ void face_numflux_sc(const double* L, const double* R
     bool ok = admissible(low, gamma);
 #pragma unroll
     for (int v = 0; v < 4; ++v) low[v] = R[v] - cr * (R[10 + v] - Fb[v]);
+    ok = ok && admissible(low, gamma);
+    if (!ok) alp = 1.0;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) Fs[v] = alp * fn[v] + (1.0 - alp) * Fh[v];
+}
+
+// Rusanov flux with a normal direction n (Trixi flux_lax_friedrichs(u_l, u_r,
+// n)): Trixi's normal flux (ρv_n, ρv_n v1 + p n1, ρv_n v2 + p n2, (ρe + p) v_n)
+// and λ = max|v·n| + max(c)|n|.
+__device__ __forceinline__ void rusanov_normal(const double* ul, const double* ur, double gamma, double n1, double n2,
+                                               double nn, double* out) {
+    const double gm1 = gamma - 1.0;
+    double fl[4], fr[4], vl, cl, vr, cr;
+    auto side = [&](const double* q, double* f, double& vabs, double& c) {
+        const double ir = rcp_nr(q[0]);
+        const double v1 = q[1] * ir, v2 = q[2] * ir;
+        const double p = gm1 * (q[3] - 0.5 * (q[1] * v1 + q[2] * v2));
+        const double vn = fma(v1, n1, v2 * n2), rvn = q[0] * vn;
+        f[0] = rvn;
+        f[1] = fma(rvn, v1, p * n1);
+        f[2] = fma(rvn, v2, p * n2);
+        f[3] = (q[3] + p) * vn;
+        vabs = fabs(vn);
+        c = sqrt_gs(gamma * p * ir) * nn;
+    };
+    side(ul, fl, vl, cl);
+    side(ur, fr, vr, cr);
+    const double lam = dmax(vl, vr) + dmax(cl, cr);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) out[v] = 0.5 * (fl[v] + fr[v]) + (-0.5 * lam) * (ur[v] - ul[v]);
+}
+
+// Shock-capturing interface flux on a curvilinear mesh (calc_interface_flux!
+// and compute_alp for StructuredMesh, structured/dg_2d.jl:2169-2288).  Records
+// (16 doubles): [0..3] Ũ, [4..7] F̃·n, [8..11] uⁿ, [12..15] fn_low.  The LW flux
+// takes λ from uⁿ and the jump on Ũ; the low-order flux is the Rusanov flux of
+// uⁿ with the same normal; the admissibility test of compute_alp uses each
+// side's own node J⁻¹.
+__device__ __forceinline__ void face_numflux_sc_curved(const double* L, const double* R, const double* nf, double JL,
+                                                       double JR, double aL, double aR, double gamma, double dt,
+                                                       double w_n, double w_1, double* Fs) {
+    double fn[4], Fh[4], Fb[4], low[4];
+    rusanov_normal(L + 8, R + 8, gamma, nf[0], nf[1], nf[2], fn);
+    // λ of uⁿ with the normal, as in the Rusanov flux above
+    double lam;
+    {
+        const double gm1 = gamma - 1.0;
+        double va[2], ca[2];
+        const double* q2[2] = {L + 8, R + 8};
+        for (int k = 0; k < 2; ++k) {
+            const double* q = q2[k];
+            const double ir = rcp_nr(q[0]);
+            const double v1 = q[1] * ir, v2 = q[2] * ir;
+            const double p = gm1 * (q[3] - 0.5 * (q[1] * v1 + q[2] * v2));
+            va[k] = fabs(fma(v1, nf[0], v2 * nf[1]));
+            ca[k] = sqrt_gs(gamma * p * ir) * nf[2];
+        }
+        lam = dmax(va[0], va[1]) + dmax(ca[0], ca[1]);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) Fh[v] = 0.5 * (L[4 + v] + R[4 + v]) + (-0.5 * lam) * (R[v] - L[v]);
+    double alp = 0.5 * (aL + aR);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) Fb[v] = (1.0 - alp) * Fh[v] + alp * fn[v];
+    const double cl = dt * JL / (w_n * 0.5), cr = dt * JR / (w_1 * 0.5);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) low[v] = L[8 + v] - cl * (Fb[v] - L[12 + v]);
+    bool ok = admissible(low, gamma);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) low[v] = R[8 + v] - cr * (R[12 + v] - Fb[v]);
     ok = ok && admissible(low, gamma);
     if (!ok) alp = 1.0;
 #pragma unroll
@@ -1713,11 +1788,12 @@ template <bool TWO> constexpr int kVolWaves = TWO ? 2 : TLW_VOL_WAVES;
 template <int N, int EB, int KIND, bool SC, bool PAR = false, bool CURV = false>
 __global__ __launch_bounds__((VGeo<N, EB>::NT), kVolWaves<SC || PAR>) void k_volume(KArgs a) {
     static_assert(!(SC && PAR), "shock capturing has no hyperbolic-parabolic form");
-    static_assert(!CURV || (KIND != VOL_LW_ADAPT && !SC && !PAR), "the curvilinear path is plain LW or MDRK");
+    static_assert(!CURV || (KIND != VOL_LW_ADAPT && !PAR), "the curvilinear path is LW or MDRK, Euler");
     using G_ = VGeo<N, EB>;
     constexpr int n = G_::n, NN = G_::NN, S = G_::S, NT = G_::NT;
     __shared__ double lds[G_::LDS];
     __shared__ double sDm[NN], sDh[NN], sD[NN];  // -dt D, D̂ and D, row-major
+    __shared__ double s_alpha[(SC && CURV) ? EB : 1];  // smoothed blending factors (curvilinear interfaces)
     double* s0 = lds;
     double* s1 = lds + 8 * S;
     double* s2 = lds + 16 * S;
@@ -1827,6 +1903,9 @@ __global__ __launch_bounds__((VGeo<N, EB>::NT), kVolWaves<SC || PAR>) void k_vol
             }
         }
         if (act && node == 0 && a.alpha_out) a.alpha_out[e] = alpha_e;
+        if constexpr (CURV) {
+            if (act && node == 0) s_alpha[lec] = alpha_e;
+        }
         // subcell FV (fv_kernel!, dg_2d.jl:251-324): Rusanov fluxes between the
         // face values of neighbouring subcells, zero at the element faces.  First
         // order (Trixi calcflux_fv!): the face values are the node values.  A
@@ -1840,7 +1919,7 @@ __global__ __launch_bounds__((VGeo<N, EB>::NT), kVolWaves<SC || PAR>) void k_vol
         double ql[4], qr[4], qd[4], qu[4];
 #pragma unroll
         for (int v = 0; v < 4; ++v) ql[v] = qr[v] = qd[v] = qu[v] = uN[v];
-        if (c->recon == 1 && blended && a.nbr[4 * e] >= 0 && a.nbr[4 * e + 1] >= 0 && a.nbr[4 * e + 2] >= 0 &&
+        if (!CURV && c->recon == 1 && blended && a.nbr[4 * e] >= 0 && a.nbr[4 * e + 1] >= 0 && a.nbr[4 * e + 2] >= 0 &&
             a.nbr[4 * e + 3] >= 0) {
             const double dtmh = (KIND == VOL_MDRK1) ? 0.5 * dt : dt;  // tree/multi_derivative_2d.jl:199
             mh_faces<N, S>(s1, sidx, i, j, uN, c, 2.0 - alpha_e, dtmh, invJ, gm1, ql, qr, qd, qu);
@@ -1852,12 +1931,26 @@ __global__ __launch_bounds__((VGeo<N, EB>::NT), kVolWaves<SC || PAR>) void k_vol
         if (i > 0) {
 #pragma unroll
             for (int v = 0; v < 4; ++v) w[v] = s0[v * S + sidx - 1];
-            rusanov(w, ql, gamma, 1, fxl);
+            if constexpr (CURV) {
+                // Trixi's subcell normal, or for a pure-DG element at subcell face n-1 the
+                // normal of the reference's calc_fn_low! (structured/dg_2d.jl:111-173)
+                const double* nq = (!blended && i == n - 1) ? a.sqrk + ((size_t)e * n + j) * 4
+                                                             : a.snrm + ((size_t)e * NN + node) * 4;
+                rusanov_normal(w, ql, gamma, nq[0], nq[1], sqrt(fma(nq[0], nq[0], nq[1] * nq[1])), fxl);
+            } else {
+                rusanov(w, ql, gamma, 1, fxl);
+            }
         }
         if (j > 0) {
 #pragma unroll
             for (int v = 0; v < 4; ++v) w[v] = s0[(4 + v) * S + sidx - n];
-            rusanov(w, qd, gamma, 2, gyl);
+            if constexpr (CURV) {
+                const double* nq = (!blended && j == n - 1) ? a.sqrk + ((size_t)e * n + i) * 4 + 2
+                                                             : a.snrm + ((size_t)e * NN + node) * 4 + 2;
+                rusanov_normal(w, qd, gamma, nq[0], nq[1], sqrt(fma(nq[0], nq[0], nq[1] * nq[1])), gyl);
+            } else {
+                rusanov(w, qd, gamma, 2, gyl);
+            }
         }
         if (act) put8<S>(s2, sidx, fxl, gyl);
         __syncthreads();
@@ -1891,16 +1984,27 @@ __global__ __launch_bounds__((VGeo<N, EB>::NT), kVolWaves<SC || PAR>) void k_vol
         const double cs = sqrt_gs(gamma * p * ir);
         // (a domain-boundary face is rewritten to the TreeMesh order of the
         // Dirichlet call by the surface kernel, which has uⁿ at hand)
-        auto face = [&](int f, int l) {
+        // with shock capturing: Ũ, F̃·n, uⁿ and fn_low (face_numflux_sc_curved)
+        auto face = [&](int f, int l, const double* fnl) {
             double nf[4], Fn[4], r[RQ];
             load4(a.fnorm + (((size_t)e * 4 + f) * n + l) * 4, nf);
 #pragma unroll
             for (int v = 0; v < 4; ++v) Fn[v] = fma(F[v], nf[0], G[v] * nf[1]);
-            make_record(r, Uc, Fn, fabs(fma(v1, nf[0], v2 * nf[1])), cs * nf[2]);
+            if constexpr (SC) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    r[v] = Uc[v];
+                    r[4 + v] = Fn[v];
+                    r[8 + v] = uN[v];
+                    r[12 + v] = fnl[v];
+                }
+            } else {
+                make_record(r, Uc, Fn, fabs(fma(v1, nf[0], v2 * nf[1])), cs * nf[2]);
+            }
             store_record<RQ>(rec + ((lec * 4 + f) * n + l) * RQ, r);
         };
-        if (act && fx >= 0) face(fx, j);
-        if (act && fy >= 0) face(fy, i);
+        if (act && fx >= 0) face(fx, j, fnlx);
+        if (act && fy >= 0) face(fy, i, fnly);
     } else {
         double av1, av2, cs;
         speed_parts2(lamst, gamma, av1, av2, cs);
@@ -1949,13 +2053,20 @@ __global__ __launch_bounds__((VGeo<N, EB>::NT), kVolWaves<SC || PAR>) void k_vol
                 double Lr[RQ], Rr[RQ], Fs[4];
                 load_record<RQ>(own, Lr);
                 load_record<RQ>(rec + (((ne - e0) * 4 + (f ^ 1)) * n + l) * RQ, Rr);
-                if constexpr (SC)
-                    face_numflux_sc(Lr, Rr, f < 2 ? 1 : 2, gamma, st->invJ0, dt, c->weights[n - 1], c->weights[0], Fs);
-                else
-                    face_numflux(Lr, Rr, Fs);
                 // + face of ls: node (n-1, l) or (l, n-1); - face of ne: node (0, l) or (l, 0)
                 const int nodeL = (f == 1) ? l * n + (n - 1) : (n - 1) * n + l;
                 const int nodeR = (f == 1) ? l * n : l;
+                if constexpr (SC && CURV) {
+                    double nf[4];
+                    load4(a.fnorm + (((size_t)es * 4 + f) * n + l) * 4, nf);
+                    face_numflux_sc_curved(Lr, Rr, nf, a.ijn[(size_t)es * NN + nodeL], a.ijn[(size_t)ne * NN + nodeR],
+                                           s_alpha[ls], s_alpha[ne - e0], gamma, dt, c->weights[n - 1], c->weights[0],
+                                           Fs);
+                } else if constexpr (SC) {
+                    face_numflux_sc(Lr, Rr, f < 2 ? 1 : 2, gamma, st->invJ0, dt, c->weights[n - 1], c->weights[0], Fs);
+                } else {
+                    face_numflux(Lr, Rr, Fs);
+                }
                 double* acc = (f == 1) ? accx : accy;
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
@@ -2099,7 +2210,6 @@ __device__ inline void boundary_record_curved(const KArgs& a, int e, int face, i
 // CFL maxima and the admissibility test.
 template <int N, bool BC, bool SC, bool CURV = false>
 __global__ __launch_bounds__(kSurfBlock) void k_surface(KArgs a) {
-    static_assert(!(CURV && SC), "the curvilinear path has no shock capturing");
     using G_ = SGeo<N>;
     constexpr int n = G_::n, NN = G_::NN, EB = G_::EB, S = G_::S;
     __shared__ double s_l1[S], s_l2[S], s_m[EB], s_lv[S], s_mv[EB];
@@ -2141,30 +2251,54 @@ __global__ __launch_bounds__(kSurfBlock) void k_surface(KArgs a) {
             const double* Rr = (f & 1) ? oth : own;
             if (!BC || nb >= 0) {
                 load_record<RQ>(a.trace + (((size_t)nb * 4 + (f ^ 1)) * n + l) * RQ, oth);
-                if constexpr (SC)
+                if constexpr (SC && CURV) {
+                    // the neighbour's mirrored face node and the element-wise blending factors
+                    const int nnode = (f == 0) ? l * n + (n - 1) : (f == 1) ? l * n : (f == 2) ? (n - 1) * n + l : l;
+                    const double Jo = a.ijn[(size_t)e * NN + node], Jn = a.ijn[(size_t)nb * NN + nnode];
+                    const double ao = a.alpha_out[e], an = a.alpha_out[nb];
+                    double nf[4];
+                    load4(a.fnorm + (((size_t)e * 4 + f) * n + l) * 4, nf);
+                    if (f & 1)
+                        face_numflux_sc_curved(Lr, Rr, nf, Jo, Jn, ao, an, c->gamma, dt, c->weights[n - 1],
+                                               c->weights[0], Fs);
+                    else
+                        face_numflux_sc_curved(Lr, Rr, nf, Jn, Jo, an, ao, c->gamma, dt, c->weights[n - 1],
+                                               c->weights[0], Fs);
+                } else if constexpr (SC) {
                     face_numflux_sc(Lr, Rr, f < 2 ? 1 : 2, c->gamma, st->invJ0, dt, c->weights[n - 1], c->weights[0],
                                     Fs);
-                else
+                } else {
                     face_numflux(Lr, Rr, Fs);
+                }
             } else {
                 // domain boundary: time-averaged Dirichlet state, no blending
                 // (calc_boundary_flux!, dg_2d.jl:1379-1454; structured/dg_2d.jl:2297-2391)
                 if constexpr (CURV) {
-                    // the volume kernel's record holds Ũ, F̃·n and the speeds of uⁿ; the
-                    // Dirichlet call takes the TreeMesh order: jump on uⁿ, λ from Ũ
-                    double nf[4];
+                    // the volume kernel's record holds Ũ and F̃·n (and uⁿ with shock
+                    // capturing); the Dirichlet call takes the TreeMesh order: jump on
+                    // uⁿ, λ from Ũ; no blending (structured/dg_2d.jl:2352-2391)
+                    double nf[4], in10[kTraceQ], out10[kTraceQ];
                     load4(a.fnorm + (((size_t)e * 4 + f) * n + l) * 4, nf);
                     const double ir = rcp_nr(own[0]);
                     const double b1 = own[1] * ir, b2 = own[2] * ir;
                     const double pb = (c->gamma - 1.0) * (own[3] - 0.5 * (own[1] * b1 + own[2] * b2));
-                    own[8] = fabs(fma(b1, nf[0], b2 * nf[1]));
-                    own[9] = sqrt_gs(c->gamma * pb * ir) * nf[2];
-                    load4(sel_in(a, cur) + ((size_t)e * NN + node) * 4, own);
-                    boundary_record_curved<N>(a, e, f, l, t, dt, oth);
+                    if constexpr (SC) {
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) in10[v] = own[8 + v];
+                    } else {
+                        load4(sel_in(a, cur) + ((size_t)e * NN + node) * 4, in10);
+                    }
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) in10[4 + v] = own[4 + v];
+                    in10[8] = fabs(fma(b1, nf[0], b2 * nf[1]));
+                    in10[9] = sqrt_gs(c->gamma * pb * ir) * nf[2];
+                    boundary_record_curved<N>(a, e, f, l, t, dt, out10);
+                    if (f & 1) face_numflux(in10, out10, Fs);
+                    else face_numflux(out10, in10, Fs);
                 } else {
                     boundary_record<N>(a, e, f, l, t, dt, oth);
+                    face_numflux(Lr, Rr, Fs);
                 }
-                face_numflux(Lr, Rr, Fs);
             }
             const double wf = (f & 1) ? c->bnd_hi : -c->bnd_lo;
 #pragma unroll

@@ -150,10 +150,13 @@ class SemidiscretizationHyperbolic:
     ``surface_flux_order`` selects the TreeMesh ("tree") or Structured/P4est
     ("structured") argument order of the LW Rusanov flux.
 
-    On a curvilinear StructuredMesh (plain LW or MDRK, periodic or with the
+    On a curvilinear StructuredMesh (LW or MDRK, periodic or with the
     time-averaged Dirichlet boundary) the kernels of
     src/solvers/dgsem_structured/dg_2d.jl apply: contravariant fluxes, per-node
     inverse Jacobians, interface normals and the structured flux order.
+    Shock capturing there uses the first-order subcell scheme with Trixi's
+    subcell normals (MUSCL-Hancock, and alpha smoothing with boundaries, raise
+    TLW_ENOTSUP).
     """
 
     def __init__(self, mesh, time_discretization, equations, initial_condition, solver, boundary_conditions=None,
@@ -215,8 +218,9 @@ class SemidiscretizationHyperbolic:
             cd = _lib.CurvedDesc(_dptr(k["Ja"]), _dptr(k["ijn"]), _dptr(k["xy"]))
             rc = _lib.lib().tlw_create_curved(C.byref(md), C.byref(basis), C.byref(sd), C.byref(cd), C.byref(h))
             if rc != _lib.TLW_OK:
-                raise _lib.TLWError(rc, "tlw_create_curved failed (plain FR, positive Jacobians and, with "
-                                        "boundary faces, a boundary value function required)")
+                raise _lib.TLWError(rc, "tlw_create_curved failed (positive Jacobians and, with boundary "
+                                        "faces, a boundary value function required; first-order shock "
+                                        "capturing only)")
             return
         rc = _lib.lib().tlw_create(C.byref(md), C.byref(basis), C.byref(sd), C.byref(h))
         if rc != _lib.TLW_OK:
