@@ -258,9 +258,11 @@ def make_problem(wl):
     return semi, mesh, ic, cfl, (lo, hi), okw
 
 
-def run_steps(semi, u0, cfl, steps, warmup, tile=0):
+def run_steps(semi, u0, cfl, steps, warmup, tile=0, repeats=0):
     """Device loop: optional tile autotune, `warmup` untimed steps, then `steps`
-    timed steps with HIP-event kernel timing.  Returns a dict of the results."""
+    timed steps with HIP-event kernel timing.  With `repeats`, the same `steps`
+    from the same initial state are timed that many more times (wall clock
+    only), for the median.  Returns a dict of the results."""
     import torch
     from trixilw_hip import _lib
 
@@ -294,10 +296,23 @@ def run_steps(semi, u0, cfl, steps, warmup, tile=0):
     _lib.check(L.tlw_enable_kernel_timing(h, 0), h)
     steps_done = int(it.value) - warmup
     assert steps_done == steps, (steps_done, steps)
+    rep_ms = []
+    for _ in range(repeats):
+        _lib.check(L.tlw_set_state(h, u0.ctypes.data_as(dp), u0.size), h)
+        _lib.check(L.tlw_solve_init(h, 0.0, float("inf"), cfl), h)
+        if warmup > 0:
+            _lib.check(L.tlw_advance(h, warmup, C.byref(t), C.byref(dt), C.byref(it), C.byref(fin)), h)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        _lib.check(L.tlw_advance(h, steps, C.byref(t), C.byref(dt), C.byref(it), C.byref(fin)), h)
+        torch.cuda.synchronize()
+        rep_ms.append(1e3 * (time.perf_counter() - t1) / steps)
+        assert int(it.value) == warmup + steps, (int(it.value), warmup, steps)
     dofs = semi.ndofs()
     return {"value": dofs * steps / el, "ms_per_step": 1e3 * el / steps, "elapsed": el,
             "vol_ms": vms.value / max(vn.value, 1), "vol_n": int(vn.value),
-            "surf_ms": sms.value / max(sn.value, 1), "surf_n": int(sn.value), "tile": tile, "tune": tune}
+            "surf_ms": sms.value / max(sn.value, 1), "surf_n": int(sn.value), "tile": tile, "tune": tune,
+            "repeat_ms": rep_ms}
 
 
 def measure_navier_stokes(steps, cpu_steps, threads, with_cpu=True):
@@ -451,6 +466,7 @@ def main():
     ap.add_argument("--workload", default=DEFAULT_WORKLOAD, choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--repeats", type=int, default=5, help="extra timed repeats of the main run (median)")
     ap.add_argument("--tile", type=int, default=0, help="elements per volume workgroup (0: autotune)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward step ratio")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-instance batch measurement")
@@ -498,8 +514,17 @@ def main():
     L = _lib.lib()
     h = semi.handle
     dp = C.POINTER(C.c_double)
-    r = run_steps(semi, u0, cfl, args.steps, args.warmup, args.tile)
+    r = run_steps(semi, u0, cfl, args.steps, args.warmup, args.tile, repeats=args.repeats)
     tile, tune = r["tile"], r["tune"]
+    repeats = None
+    if r["repeat_ms"]:
+        ms = sorted(r["repeat_ms"])
+        med = float(np.median(ms))
+        repeats = {"n": len(ms), "steps_each": args.steps, "median_ms_per_step": round(med, 4),
+                   "min_ms_per_step": round(ms[0], 4), "max_ms_per_step": round(ms[-1], 4),
+                   "median_value": dofs / (med * 1e-3),
+                   "note": "the same warm-up and timed steps from the same initial state, re-timed after the main run"}
+        log(f"bench: {len(ms)} repeats, median {med:.3f} ms/step ({dofs / med / 1e6:.3f} G DOF-updates/s)")
     value, ms_per_step = r["value"], r["ms_per_step"]
     vms_avg, sms_avg, vn = r["vol_ms"], r["surf_ms"], r["vol_n"]
     log(f"bench: {value / 1e9:.3f} G DOF-updates/s, {ms_per_step:.3f} ms/step, "
@@ -599,10 +624,14 @@ def main():
         log(f"bench: CPU oracle baseline, {args.cpu_steps} steps on {threads} threads")
         v_all, el_all = cpu_baseline(o, uo, cfl, td == "mdrk", dofs, args.cpu_steps, threads)
         log(f"bench: CPU {v_all / 1e6:.2f} M DOF-updates/s ({el_all:.1f} s)")
+        v_one, el_one = cpu_baseline(o, uo, cfl, td == "mdrk", dofs, 1, 1)
+        log(f"bench: CPU single thread {v_one / 1e6:.2f} M DOF-updates/s ({el_one:.1f} s)")
         cpu = {"value": v_all, "unit": "DOF-updates/s", "cores": threads, "kind": "port",
                "sample": f"{args.cpu_steps} steps of the same {nx}x{nx} N={N} mesh ({dofs} DOFs), "
                          f"C++/OpenMP oracle, g++ -O3 -march=x86-64-v3",
-               "gpu_over_cpu": round(value / v_all, 2)}
+               "gpu_over_cpu": round(value / v_all, 2),
+               "single_thread": {"value": v_one, "cores": 1, "sample": "1 step of the same mesh",
+                                 "gpu_over_cpu": round(value / v_one, 2)}}
         del o
 
     out = {
@@ -633,6 +662,7 @@ def main():
                                         + " subcell FV") if sc else None),
                    "cfl": cfl, "elements_per_workgroup": tile, "tile_autotune_ms": tune},
         "roofline": roofline,
+        "repeats": repeats,
         "cpu_baseline": cpu,
         "adjoint": adjoint,
         "batch": batch,
