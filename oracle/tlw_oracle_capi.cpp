@@ -8,6 +8,7 @@
 #include <memory>
 
 #include "tlw_oracle_curved.hpp"
+#include "tlw_oracle_curved_parabolic.hpp"
 #include "tlw_oracle_parabolic.hpp"
 
 using namespace tlwo;
@@ -84,12 +85,19 @@ void* tlwo_create(const tlwo_desc* d) {
     // curvilinear shock capturing: first order; Trixi's alpha smoothing on a
     // StructuredMesh asserts a periodic mesh
     const bool periodic = d->periodic_x && d->periodic_y;
-    if (d->curved && d->parabolic) return nullptr;
+    if (d->curved && d->parabolic == 1) return nullptr;
     if (d->curved && d->shock_capturing && (d->reconstruction != 0 || (d->alpha_smooth && !periodic))) return nullptr;
     if (d->reconstruction == 1 && (d->equation != 0 || !d->shock_capturing)) return nullptr;
     auto* c = new tlwo_ctx;
     c->equation = d->equation;
-    if (d->curved) {
+    if (d->curved && d->parabolic == 2) {
+        auto* p = new CurvedParabolicSolver<double, Euler2D<double>, NavierStokesDiffusion2D<double>>;
+        p->eq.gamma = d->gamma;
+        p->visc.set(d->gamma, d->visc_mu, d->prandtl);
+        configure(*p, d);
+        p->setup_curved_parabolic(d->map_kind, d->map_par);
+        c->euler.reset(p);
+    } else if (d->curved) {
         if (d->equation == 0) {
             auto* p = new CurvedSolver<double, Euler2D<double>>;
             p->eq.gamma = d->gamma;

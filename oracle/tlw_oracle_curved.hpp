@@ -53,8 +53,10 @@ inline void apply_mapping(int kind, const double* p, double xi, double eta, doub
     }
 }
 
-template <class T, class Eq> struct CurvedSolver : Solver<T, Eq> {
-    using B = Solver<T, Eq>;
+// Base: Solver<T, Eq>, or ParabolicSolver for the hyperbolic-parabolic
+// system on the curvilinear mesh (tlw_oracle_curved_parabolic.hpp).
+template <class T, class Eq, class Base = Solver<T, Eq>> struct CurvedSolver : Base {
+    using B = Base;
     static constexpr int NV = Eq::NV;
     using B::basis;
     using B::eq;
@@ -77,7 +79,7 @@ template <class T, class Eq> struct CurvedSolver : Solver<T, Eq> {
             if (which == 13) return &Ja;
             if (which == 14) return &ijn;
         }
-        return nullptr;
+        return B::extra_aux(which);
     }
 
     // Trixi StructuredMesh cache: node coordinates, Jacobian matrix from the

@@ -97,9 +97,11 @@ template <class T> struct NavierStokesDiffusion2D {
         fy[3] = (v1 * tau12 + v2 * tau22 + q2) * mu;
     }
     // compressible_navier_stokes_2d.jl:230-237: J⁻² max(4μ/(3ρ), μκ/(ρ Pr))
-    double lam_v(const T* u, double inv_jac) const {
+    double lam_v(const T* u, double inv_jac) const { return lam_v_scaled(u, inv_jac * inv_jac); }
+    // s max(4μ/(3ρ), μκ/(ρ Pr)); s = J⁻¹ of the node on a curvilinear mesh
+    double lam_v_scaled(const T* u, double s) const {
         const double rho = value(u[0]);
-        return inv_jac * inv_jac * std::max(4.0 * mu / (3.0 * rho), mu * kappa / (rho * Pr));
+        return s * std::max(4.0 * mu / (3.0 * rho), mu * kappa / (rho * Pr));
     }
 };
 
@@ -118,6 +120,13 @@ template <class T, class Eq, class Visc> struct ParabolicSolver : Solver<T, Eq> 
     std::vector<T> gsfv;     // surface values of the gradient computation [v, l, face, e]
     std::vector<T> Fv;       // cache_parabolic.Fv [v, d, i, j, e]
     std::vector<T> Fv2;      // cache_parabolic.mdrk_cache.Fv2
+
+    // Curvilinear mesh (set by CurvedParabolicSolver): per node the contravariant
+    // vectors [e][j][i][4] (Ja11, Ja12, Ja21, Ja22) and J⁻¹ [e][j][i]; null on a TreeMesh.
+    const double* cJa = nullptr;
+    const double* cijn = nullptr;
+    size_t cnode(int i, int j, int e) const { return ((size_t)e * n + j) * n + i; }
+    double node_ij(int i, int j, int e) const { return cijn ? cijn[cnode(i, j, e)] : mesh.inv_jac[e]; }
 
     void setup_parabolic() {
         const size_t nd = this->ndofs_total();
@@ -191,8 +200,11 @@ template <class T, class Eq, class Visc> struct ParabolicSolver : Solver<T, Eq> 
     }
 
     // Element-local gradient of a time derivative: wx = J⁻¹ D w (x), wy = J⁻¹ D w (y)
-    // ("Compute ∇u_t", dg_2d_parabolic.jl:832-859).
-    void local_grad(const T (*w)[MAXN][NV], T (*wx)[MAXN][NV], T (*wy)[MAXN][NV], double ij) const {
+    // ("Compute ∇u_t", dg_2d_parabolic.jl:832-859); on a curvilinear mesh
+    // compute_local_grad_u! (p4est/dg_2d_parabolic.jl:51-104): the reference-space
+    // derivatives D w, turned with the transposed contravariant vectors,
+    // wx = Ja11 w_ξ + Ja21 w_η, wy = Ja12 w_ξ + Ja22 w_η, then ×J⁻¹ of the node.
+    void local_grad(const T (*w)[MAXN][NV], T (*wx)[MAXN][NV], T (*wy)[MAXN][NV], int e) const {
         const auto& D = basis.D;
         for (int j = 0; j < n; ++j)
             for (int i = 0; i < n; ++i)
@@ -204,22 +216,50 @@ template <class T, class Eq, class Visc> struct ParabolicSolver : Solver<T, Eq> 
                 for (int jj = 0; jj < n; ++jj)
                     for (int v = 0; v < NV; ++v) wy[jj][i][v] = wy[jj][i][v] + D[jj][j] * w[j][i][v];
             }
+        if (cJa) {
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < n; ++i) {
+                    const double* J = &cJa[4 * cnode(i, j, e)];
+                    for (int v = 0; v < NV; ++v) {
+                        const T g1 = wx[j][i][v], g2 = wy[j][i][v];
+                        wx[j][i][v] = J[0] * g1 + J[2] * g2;
+                        wy[j][i][v] = J[1] * g1 + J[3] * g2;
+                    }
+                }
+        }
         for (int j = 0; j < n; ++j)
-            for (int i = 0; i < n; ++i)
+            for (int i = 0; i < n; ++i) {
+                const double ij = node_ij(i, j, e);
                 for (int v = 0; v < NV; ++v) {
                     wx[j][i][v] = wx[j][i][v] * ij;
                     wy[j][i][v] = wy[j][i][v] * ij;
                 }
+            }
     }
-    // acc[ii][j] += c D[ii][i] f[j][i] (x) and c D[jj][j] g[j][i] (y)
-    void add_divergence(const T (*f)[MAXN][NV], const T (*g)[MAXN][NV], double c, T (*acc)[MAXN][NV]) const {
+    // acc[ii][j] += c D[ii][i] f[j][i] (x) and c D[jj][j] g[j][i] (y); on a
+    // curvilinear mesh of the contravariant pair (Ja¹·(f, g), Ja²·(f, g)) of the
+    // node (contravariant_fluxes, p4est/dg_2d_parabolic.jl:4-38)
+    void add_divergence(const T (*f)[MAXN][NV], const T (*g)[MAXN][NV], double c, T (*acc)[MAXN][NV], int e) const {
         const auto& D = basis.D;
         for (int j = 0; j < n; ++j)
             for (int i = 0; i < n; ++i) {
+                T fc[NV], gc[NV];
+                if (cJa) {
+                    const double* J = &cJa[4 * cnode(i, j, e)];
+                    for (int v = 0; v < NV; ++v) {
+                        fc[v] = J[0] * f[j][i][v] + J[1] * g[j][i][v];
+                        gc[v] = J[2] * f[j][i][v] + J[3] * g[j][i][v];
+                    }
+                } else {
+                    for (int v = 0; v < NV; ++v) {
+                        fc[v] = f[j][i][v];
+                        gc[v] = g[j][i][v];
+                    }
+                }
                 for (int ii = 0; ii < n; ++ii)
-                    for (int v = 0; v < NV; ++v) acc[j][ii][v] = acc[j][ii][v] + (c * D[ii][i]) * f[j][i][v];
+                    for (int v = 0; v < NV; ++v) acc[j][ii][v] = acc[j][ii][v] + (c * D[ii][i]) * fc[v];
                 for (int jj = 0; jj < n; ++jj)
-                    for (int v = 0; v < NV; ++v) acc[jj][i][v] = acc[jj][i][v] + (c * D[jj][j]) * g[j][i][v];
+                    for (int v = 0; v < NV; ++v) acc[jj][i][v] = acc[jj][i][v] + (c * D[jj][j]) * gc[v];
             }
     }
 
@@ -250,7 +290,6 @@ template <class T, class Eq, class Visc> struct ParabolicSolver : Solver<T, Eq> 
     // terms are `nothing` and the u_np1 / u_np1_low temporal-error updates of
     // kernel 4 feed only the adaptive time stepping; both are omitted.
     void lw_volume_parabolic(T* du, const T* u, double dt, int e) {
-        const double ij = mesh.inv_jac[e];
         const int N = basis.N;
         static thread_local States S;
         T fa[MAXN][MAXN][NV], ga[MAXN][MAXN][NV], fv[MAXN][MAXN][NV], gv[MAXN][MAXN][NV];
@@ -283,11 +322,11 @@ template <class T, class Eq, class Visc> struct ParabolicSolver : Solver<T, Eq> 
                     }
                 }
             }
-        add_divergence(df, dg, -dt, w);
+        add_divergence(df, dg, -dt, w, e);
         for (int j = 0; j < n; ++j)
             for (int i = 0; i < n; ++i)
-                for (int v = 0; v < NV; ++v) w[j][i][v] = w[j][i][v] * ij;  // ut
-        local_grad(w, wx, wy, ij);
+                for (int v = 0; v < NV; ++v) w[j][i][v] = w[j][i][v] * node_ij(i, j, e);  // ut
+        local_grad(w, wx, wy, e);
         {
             const double c[4] = {1.0, -1.0, 2.0, -2.0};
             add_states(S, w, wx, wy, c);
@@ -320,11 +359,11 @@ template <class T, class Eq, class Visc> struct ParabolicSolver : Solver<T, Eq> 
         for (int j = 0; j < n; ++j)
             for (int i = 0; i < n; ++i)
                 for (int v = 0; v < NV; ++v) w[j][i][v] = T(0);
-        add_divergence(df, dg, 1.0, w);
+        add_divergence(df, dg, 1.0, w, e);
         for (int j = 0; j < n; ++j)
             for (int i = 0; i < n; ++i)
-                for (int v = 0; v < NV; ++v) w[j][i][v] = w[j][i][v] * (-dt * ij);  // utt
-        local_grad(w, wx, wy, ij);
+                for (int v = 0; v < NV; ++v) w[j][i][v] = w[j][i][v] * (-dt * node_ij(i, j, e));  // utt
+        local_grad(w, wx, wy, e);
         {
             const double c[4] = {0.5, 0.5, 2.0, 2.0};
             add_states(S, w, wx, wy, c);
@@ -370,11 +409,11 @@ template <class T, class Eq, class Visc> struct ParabolicSolver : Solver<T, Eq> 
         for (int j = 0; j < n; ++j)
             for (int i = 0; i < n; ++i)
                 for (int v = 0; v < NV; ++v) w[j][i][v] = T(0);
-        add_divergence(df, dg, -dt, w);
+        add_divergence(df, dg, -dt, w, e);
         for (int j = 0; j < n; ++j)
             for (int i = 0; i < n; ++i)
-                for (int v = 0; v < NV; ++v) w[j][i][v] = w[j][i][v] * ij;  // uttt
-        local_grad(w, wx, wy, ij);
+                for (int v = 0; v < NV; ++v) w[j][i][v] = w[j][i][v] * node_ij(i, j, e);  // uttt
+        local_grad(w, wx, wy, e);
         {
             const double c[4] = {1.0 / 6.0, -1.0 / 6.0, 4.0 / 3.0, -4.0 / 3.0};
             add_states(S, w, wx, wy, c);
@@ -408,11 +447,11 @@ template <class T, class Eq, class Visc> struct ParabolicSolver : Solver<T, Eq> 
             for (int j = 0; j < n; ++j)
                 for (int i = 0; i < n; ++i)
                     for (int v = 0; v < NV; ++v) w[j][i][v] = T(0);
-            add_divergence(df, dg, -dt, w);
+            add_divergence(df, dg, -dt, w, e);
             for (int j = 0; j < n; ++j)
                 for (int i = 0; i < n; ++i)
-                    for (int v = 0; v < NV; ++v) w[j][i][v] = w[j][i][v] * ij;  // utttt
-            local_grad(w, wx, wy, ij);
+                    for (int v = 0; v < NV; ++v) w[j][i][v] = w[j][i][v] * node_ij(i, j, e);  // utttt
+            local_grad(w, wx, wy, e);
             const double c[4] = {1.0 / 24.0, 1.0 / 24.0, 2.0 / 3.0, 2.0 / 3.0};
             add_states(S, w, wx, wy, c);
             for (int j = 0; j < n; ++j)
@@ -454,6 +493,14 @@ template <class T, class Eq, class Visc> struct ParabolicSolver : Solver<T, Eq> 
                 for (int v = 0; v < NV; ++v) {
                     Fn[v] = Fa[j][i][v] - Fva[j][i][v];
                     Gn[v] = Ga[j][i][v] - Gva[j][i][v];
+                }
+                if (cJa) {  // D̂ on the contravariant F̃ (cv_F, cv_G of the reference)
+                    const double* J = &cJa[4 * cnode(i, j, e)];
+                    for (int v = 0; v < NV; ++v) {
+                        const T a = Fn[v], b = Gn[v];
+                        Fn[v] = J[0] * a + J[1] * b;
+                        Gn[v] = J[2] * a + J[3] * b;
+                    }
                 }
                 for (int ii = 0; ii < n; ++ii)
                     for (int v = 0; v < NV; ++v) du[idx(v, ii, j, e)] = du[idx(v, ii, j, e)] + Dh[ii][i] * Fn[v];
@@ -497,7 +544,6 @@ template <class T, class Eq, class Visc> struct ParabolicSolver : Solver<T, Eq> 
     // U2 = u + ut/6.  The u_low update feeds only the admissibility machinery
     // (and names a field MDRKElementCache lacks) and is omitted.
     void mdrk_parabolic_1(T* du, const T* u, double dt, int e) {
-        const double ij = mesh.inv_jac[e];
         T Fa[MAXN][MAXN][NV], Ga[MAXN][MAXN][NV], Fva[MAXN][MAXN][NV], Gva[MAXN][MAXN][NV];
         T Fa2[MAXN][MAXN][NV], Ga2[MAXN][MAXN][NV], Fva2[MAXN][MAXN][NV], Gva2[MAXN][MAXN][NV];
         T Uc[MAXN][MAXN][NV], U2c[MAXN][MAXN][NV], ut[MAXN][MAXN][NV], utx[MAXN][MAXN][NV], uty[MAXN][MAXN][NV];
@@ -526,11 +572,11 @@ template <class T, class Eq, class Visc> struct ParabolicSolver : Solver<T, Eq> 
                     U2c[j][i][v] = un[v];
                 }
             }
-        add_divergence(df, dg, -dt, ut);
+        add_divergence(df, dg, -dt, ut, e);
         for (int j = 0; j < n; ++j)
             for (int i = 0; i < n; ++i)
-                for (int v = 0; v < NV; ++v) ut[j][i][v] = ut[j][i][v] * ij;
-        local_grad(ut, utx, uty, ij);
+                for (int v = 0; v < NV; ++v) ut[j][i][v] = ut[j][i][v] * node_ij(i, j, e);
+        local_grad(ut, utx, uty, e);
         for (int j = 0; j < n; ++j)
             for (int i = 0; i < n; ++i) {
                 T fta[NV], gta[NV], ftv[NV], gtv[NV];
@@ -564,7 +610,6 @@ template <class T, class Eq, class Visc> struct ParabolicSolver : Solver<T, Eq> 
     // mdrk_kernel_2! (multi_derivative_2d_parabolic.jl:506-711): with the
     // gradient of us, F̃ = F2 + f_t(us)/3 and Ũ = U2 + ust/3 (both parts).
     void mdrk_parabolic_2(T* du, double dt, int e) {
-        const double ij = mesh.inv_jac[e];
         const T* us = this->us.data();
         T Fa[MAXN][MAXN][NV], Ga[MAXN][MAXN][NV], Fva[MAXN][MAXN][NV], Gva[MAXN][MAXN][NV];
         T Uc[MAXN][MAXN][NV], ust[MAXN][MAXN][NV], ustx[MAXN][MAXN][NV], usty[MAXN][MAXN][NV];
@@ -590,11 +635,11 @@ template <class T, class Eq, class Visc> struct ParabolicSolver : Solver<T, Eq> 
                     dg[j][i][v] = ga[v] - gv[v];
                 }
             }
-        add_divergence(df, dg, -dt, ust);
+        add_divergence(df, dg, -dt, ust, e);
         for (int j = 0; j < n; ++j)
             for (int i = 0; i < n; ++i)
-                for (int v = 0; v < NV; ++v) ust[j][i][v] = ust[j][i][v] * ij;
-        local_grad(ust, ustx, usty, ij);
+                for (int v = 0; v < NV; ++v) ust[j][i][v] = ust[j][i][v] * node_ij(i, j, e);
+        local_grad(ust, ustx, usty, e);
         for (int j = 0; j < n; ++j)
             for (int i = 0; i < n; ++i) {
                 T fta[NV], gta[NV], ftv[NV], gtv[NV];
