@@ -178,11 +178,27 @@ int tlw_create_parabolic(const tlw_mesh_desc* mesh, const tlw_basis_desc* basis,
  * face n-1 takes the reference's calc_fn_low! normal, the q = 0 and q = n-2 terms only) and
  * compute_alp with each face node's own inverse Jacobian (dg_2d.jl:2169-2192); boundary faces are
  * not blended.  TLW_RECON_MUSCL_HANCOCK, and alpha smoothing on a mesh with boundaries (Trixi
- * asserts a periodic StructuredMesh there), return TLW_ENOTSUP, and so do parabolic terms,
- * several instances, the positivity limiter, adaptive steps, tlw_adjoint_lw and
- * tlw_rebuild_mesh on such a context.  A non-positive inverse Jacobian returns TLW_EINVAL. */
+ * asserts a periodic StructuredMesh there), return TLW_ENOTSUP, and so do several instances,
+ * the positivity limiter, adaptive steps, tlw_adjoint_lw and tlw_rebuild_mesh on such a
+ * context.  A non-positive inverse Jacobian returns TLW_EINVAL. */
 int tlw_create_curved(const tlw_mesh_desc* mesh, const tlw_basis_desc* basis, const tlw_solver_desc* solver,
                       const tlw_curved_desc* metrics, tlw_ctx** out);
+/* SemidiscretizationHyperbolicParabolic (compressible Navier-Stokes, as tlw_create_parabolic) on a
+ * periodic curvilinear mesh: the LW kernels of src/solvers/dgsem_p4est/dg_2d_parabolic.jl
+ * (641-1787: contravariant fluxes Ja·(fa - fv), u_t scaled by the node's J⁻¹, element-local
+ * gradients of the time derivatives by compute_local_grad_u!, 51-104, i.e. D in ξ and η turned
+ * with the transposed Ja and scaled by J⁻¹) with Trixi's curvilinear BR1 gradient (reference-space
+ * D̂ and lifted ½(u_L + u_R), the element's own outward normal, transposed Ja, J⁻¹), and the
+ * interfaces of tlw_create_curved (one normal per face node; F* = LW-Rusanov(F̃a) -
+ * ½(F̃v_L + F̃v_R)·n, p4est/dg_2d_parabolic.jl:1878-1960).  MDRK stages are the TreeMesh parabolic
+ * ones with these operators (the reference has no curvilinear MDRK parabolic kernel).  dt =
+ * cfl · min(curvilinear max_dt, 1/((λa + λv)·2(2N+1))), λv = 2(2N+1) max J⁻¹ max(4μ/(3ρ),
+ * μκ/(ρ Pr)) with the node's J⁻¹ (the reference's P4est max_dt reads inverse_jacobian[element] of
+ * a per-node array; this is the formula that reduces to its TreeMesh one on an affine mesh).
+ * A mesh with boundary faces, shock capturing and several instances return TLW_ENOTSUP. */
+int tlw_create_curved_parabolic(const tlw_mesh_desc* mesh, const tlw_basis_desc* basis,
+                                const tlw_solver_desc* solver, const tlw_curved_desc* metrics,
+                                const tlw_parabolic_desc* parabolic, tlw_ctx** out);
 int tlw_n_instances(const tlw_ctx* ctx, int32_t* n_instances);
 int tlw_batch_status(tlw_ctx* ctx, int32_t n_instances, double* t_host, double* dt_host, int64_t* iter_host,
                      int32_t* finished_host);

@@ -484,9 +484,27 @@ template <int N, bool SC> void launch_volume_curved(tlw_ctx* c, const KArgs& a, 
     else if (c->ebv == ch[1]) launch_volume_curved_eb<N, ch[1], SC>(c, a, kind);
     else launch_volume_curved_eb<N, ch[2], SC>(c, a, kind);
 }
+// compressible Navier-Stokes on the curvilinear mesh
+template <int N, int EB> void launch_volume_curved_par_eb(tlw_ctx* c, const KArgs& a, int kind) {
+    dim3 g(c->vgrid), b(VGeo<N, EB>::NT);
+    if (kind == VOL_LW)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_LW, false, true, true>), g, b, 0, c->stream, a);
+    else if (kind == VOL_MDRK1)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_MDRK1, false, true, true>), g, b, 0, c->stream, a);
+    else
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_MDRK2, false, true, true>), g, b, 0, c->stream, a);
+}
+template <int N> void launch_volume_curved_par(tlw_ctx* c, const KArgs& a, int kind) {
+    constexpr const int* ch = N == 4 ? kEbChoices4 : kEbChoices3;
+    if (c->ebv == ch[0]) launch_volume_curved_par_eb<N, ch[0]>(c, a, kind);
+    else if (c->ebv == ch[1]) launch_volume_curved_par_eb<N, ch[1]>(c, a, kind);
+    else launch_volume_curved_par_eb<N, ch[2]>(c, a, kind);
+}
 template <int N> void launch_volume_t(tlw_ctx* c, const KArgs& a, int kind) {
     if (c->curved) {  // LW or MDRK (adaptive steps are rejected on such a context)
-        if (c->sc) {
+        if (c->parabolic) {
+            launch_volume_curved_par<N>(c, a, kind);
+        } else if (c->sc) {
             hipLaunchKernelGGL(k_indicator<N>, dim3(c->sgrid), dim3(kSurfBlock), 0, c->stream, a, c->d_alpha_tmp);
             launch_volume_curved<N, true>(c, a, kind);
         } else {
@@ -757,6 +775,16 @@ int tlw_create_curved(const tlw_mesh_desc* mesh, const tlw_basis_desc* basis, co
         return TLW_EINVAL;
     }
     return create_common(1, mesh, basis, solver, out, nullptr, metrics);
+}
+
+int tlw_create_curved_parabolic(const tlw_mesh_desc* mesh, const tlw_basis_desc* basis,
+                                const tlw_solver_desc* solver, const tlw_curved_desc* metrics,
+                                const tlw_parabolic_desc* parabolic, tlw_ctx** out) {
+    if (!metrics || !parabolic) {
+        if (out) *out = nullptr;
+        return TLW_EINVAL;
+    }
+    return create_common(1, mesh, basis, solver, out, parabolic, metrics);
 }
 
 int tlw_create_batch(int32_t n_instances, const tlw_mesh_desc* meshes, const tlw_basis_desc* basis,
@@ -1266,8 +1294,9 @@ int create_common(int ninst, const tlw_mesh_desc* meshes, const tlw_basis_desc* 
     }
     if (cv) {
         // StructuredMesh: LW or MDRK; shock capturing with the first-order subcell
-        // scheme (MUSCL-Hancock is rejected by the reference on such meshes)
-        if (c->parabolic || ninst != 1 || (c->sc && h.recon != TLW_RECON_FIRST_ORDER)) {
+        // scheme (MUSCL-Hancock is rejected by the reference on such meshes), or
+        // compressible Navier-Stokes (periodic, checked at upload)
+        if (ninst != 1 || (c->sc && h.recon != TLW_RECON_FIRST_ORDER)) {
             delete c;
             return TLW_ENOTSUP;
         }

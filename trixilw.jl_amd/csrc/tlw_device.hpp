@@ -1129,13 +1129,42 @@ __device__ __forceinline__ double lam_visc(const double* u, double invJ, const D
     const double mu = c->visc_mu, rho = u[0];
     return invJ * invJ * dmax(4.0 * mu / (3.0 * rho), mu * c->visc_kappa / (rho * c->visc_Pr));
 }
+// The same with the node's J⁻¹ of a curvilinear mesh in place of J⁻² (the 2-D
+// Jacobian already scales as h²)
+__device__ __forceinline__ double lam_visc_curved(const double* u, double invJ, const DevConst* c) {
+    const double mu = c->visc_mu, rho = u[0];
+    return invJ * dmax(4.0 * mu / (3.0 * rho), mu * c->visc_kappa / (rho * c->visc_Pr));
+}
+
+// Contravariant fluxes of a Cartesian pair: Ja¹·(f, g), Ja²·(f, g)
+// (contravariant_flux, structured/dg_2d.jl:550-556).
+__device__ __forceinline__ void to_contravariant(const double* J, const double* f, const double* g, double* fc,
+                                                 double* gc) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        fc[v] = fma(J[0], f[v], J[1] * g[v]);
+        gc[v] = fma(J[2], f[v], J[3] * g[v]);
+    }
+}
+
+// Reference-space gradient (w_ξ, w_η) → Cartesian with the transposed
+// contravariant vectors J = (Ja11, Ja12, Ja21, Ja22) and the node's J⁻¹
+// (compute_local_grad_u!, p4est/dg_2d_parabolic.jl:51-104).
+__device__ __forceinline__ void grad_to_cartesian(const double* J, double invJ, double* gx, double* gy) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const double g1 = gx[v], g2 = gy[v];
+        gx[v] = fma(J[0], g1, J[2] * g2) * invJ;
+        gy[v] = fma(J[1], g1, J[3] * g2) * invJ;
+    }
+}
 
 // BR1 gradient of the node's state: s holds the element's state (put4 before
 // the barrier), src the state array the face neighbours are read from.
-template <int N, int EB>
+template <int N, int EB, bool CURV = false>
 __device__ __forceinline__ void br1_gradient(const double* s, const double* src, const int* nbr, const DevConst* c,
                                              int e, int i, int j, int sbase, double invJ, const double* sDh,
-                                             const double* uown, double* gx, double* gy) {
+                                             const double* uown, double* gx, double* gy, const double* Jn = nullptr) {
     using G_ = VGeo<N, EB>;
     constexpr int n = G_::n, NN = G_::NN, S = G_::S;
     contract_xy<n, S>(s, sbase, i, j, sDh, gx, gy);
@@ -1160,6 +1189,14 @@ __device__ __forceinline__ void br1_gradient(const double* s, const double* src,
 #pragma unroll
             for (int v = 0; v < 4; ++v) gy[v] = fma(0.5 * (uown[v] + un[v]), wf, gy[v]);
         }
+    }
+    if constexpr (CURV) {
+        // curvilinear BR1 (Trixi calc_gradient! for P4estMesh): the reference-space
+        // volume and lifted interface terms above, turned with the transposed
+        // metric; the face terms of the element's own outward normal ±Ja are the
+        // same linear map
+        grad_to_cartesian(Jn, invJ, gx, gy);
+        return;
     }
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
@@ -1201,13 +1238,21 @@ struct ParStates {
 
 // Time derivative of the next level and its element-local gradient:
 // w = J⁻¹ (-dt D) · (dx, dy) through sa, then (wx, wy) = J⁻¹ D w through sg.
-template <int N, int EB>
+template <int N, int EB, bool CURV = false>
 __device__ __forceinline__ void par_level(bool act, int sidx, int sbase, int i, int j, double invJ, double* sa,
                                           double* sg, const double* sDm, const double* sD, const double* dx,
-                                          const double* dy, double* w, double* wx, double* wy) {
+                                          const double* dy, double* w, double* wx, double* wy,
+                                          const double* Jn = nullptr) {
     using G_ = VGeo<N, EB>;
     constexpr int n = G_::n, S = G_::S;
-    if (act) put8<S>(sa, sidx, dx, dy);
+    if constexpr (CURV) {
+        // contravariant pair for the derivative contraction
+        double cx[4], cy[4];
+        to_contravariant(Jn, dx, dy, cx, cy);
+        if (act) put8<S>(sa, sidx, cx, cy);
+    } else {
+        if (act) put8<S>(sa, sidx, dx, dy);
+    }
     __syncthreads();
     contract<n, S>(sa, sbase, i, j, sDm, w);
 #pragma unroll
@@ -1215,10 +1260,14 @@ __device__ __forceinline__ void par_level(bool act, int sidx, int sbase, int i, 
     if (act) put4<S>(sg, sidx, w);
     __syncthreads();
     contract_xy<n, S>(sg, sbase, i, j, sD, wx, wy);
+    if constexpr (CURV) {
+        grad_to_cartesian(Jn, invJ, wx, wy);
+    } else {
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-        wx[v] *= invJ;
-        wy[v] *= invJ;
+        for (int v = 0; v < 4; ++v) {
+            wx[v] *= invJ;
+            wy[v] *= invJ;
+        }
     }
 }
 
@@ -1227,17 +1276,23 @@ __device__ __forceinline__ void par_level(bool act, int sidx, int sbase, int i, 
 // flux, with the states of ∇u advanced by the same Taylor coefficients.
 // Buffer use: s0 keeps f(uⁿ) for the 5- and 3-point stencils, s1 takes each
 // flux derivative, s2 each time derivative; the final F̃ exchange is in s2.
-template <int N, int EB>
+template <int N, int EB, bool CURV = false>
 __device__ __forceinline__ void lwp_element(const double* uN, const double* gxN, const double* gyN, double invJ,
                                             double gm1, const DevConst* c, int i, int j, bool act, int sidx,
                                             int sbase, double* s0, double* s1, double* s2, const double* sDm,
                                             const double* sD, const double* sDh, double* Uc, double* F, double* G,
-                                            double* vol) {
+                                            double* vol, const double* Jn = nullptr) {
     using G_ = VGeo<N, EB>;
     constexpr int n = G_::n, S = G_::S;
     ns_total_fluxes(uN, gxN, gyN, gm1, c, F, G);
     double w[4], wx[4], wy[4];
-    par_level<N, EB>(act, sidx, sbase, i, j, invJ, s0, s2, sDm, sD, F, G, w, wx, wy);  // ut
+    par_level<N, EB, CURV>(act, sidx, sbase, i, j, invJ, s0, s2, sDm, sD, F, G, w, wx, wy, Jn);  // ut
+    // on a curvilinear mesh s0 took the contravariant pair; the centred stencils
+    // read the node's own Cartesian f(uⁿ), written once every lane has passed
+    // the contraction of s0
+    if constexpr (CURV) {
+        if (act) put8<S>(s0, sidx, F, G);
+    }
     ParStates P;
     P.init(uN, gxN, gyN);
     P.add(1.0, -1.0, 2.0, -2.0, w, wx, wy);
@@ -1262,7 +1317,7 @@ __device__ __forceinline__ void lwp_element(const double* uN, const double* gxN,
         F[v] = fma(0.5, dx[v], F[v]);
         G[v] = fma(0.5, dy[v], G[v]);
     }
-    par_level<N, EB>(act, sidx, sbase, i, j, invJ, s1, s2, sDm, sD, dx, dy, w, wx, wy);  // utt
+    par_level<N, EB, CURV>(act, sidx, sbase, i, j, invJ, s1, s2, sDm, sD, dx, dy, w, wx, wy, Jn);  // utt
     P.add(0.5, 0.5, 2.0, 2.0, w, wx, wy);
 #pragma unroll
     for (int v = 0; v < 4; ++v) Uc[v] = fma(1.0 / 6.0, w[v], Uc[v]);
@@ -1305,7 +1360,7 @@ __device__ __forceinline__ void lwp_element(const double* uN, const double* gxN,
         F[v] = fma(1.0 / 6.0, dx[v], F[v]);
         G[v] = fma(1.0 / 6.0, dy[v], G[v]);
     }
-    par_level<N, EB>(act, sidx, sbase, i, j, invJ, s1, s2, sDm, sD, dx, dy, w, wx, wy);  // uttt
+    par_level<N, EB, CURV>(act, sidx, sbase, i, j, invJ, s1, s2, sDm, sD, dx, dy, w, wx, wy, Jn);  // uttt
     P.add(1.0 / 6.0, -1.0 / 6.0, 4.0 / 3.0, -4.0 / 3.0, w, wx, wy);
 #pragma unroll
     for (int v = 0; v < 4; ++v) Uc[v] = fma(1.0 / 24.0, w[v], Uc[v]);
@@ -1328,7 +1383,7 @@ __device__ __forceinline__ void lwp_element(const double* uN, const double* gxN,
         G[v] = fma(1.0 / 24.0, dy[v], G[v]);
     }
     if constexpr (N == 4) {
-        par_level<N, EB>(act, sidx, sbase, i, j, invJ, s1, s2, sDm, sD, dx, dy, w, wx, wy);  // utttt
+        par_level<N, EB, CURV>(act, sidx, sbase, i, j, invJ, s1, s2, sDm, sD, dx, dy, w, wx, wy, Jn);  // utttt
         P.add(1.0 / 24.0, 1.0 / 24.0, 2.0 / 3.0, 2.0 / 3.0, w, wx, wy);
 #pragma unroll
         for (int v = 0; v < 4; ++v) Uc[v] = fma(1.0 / 120.0, w[v], Uc[v]);
@@ -1358,7 +1413,13 @@ __device__ __forceinline__ void lwp_element(const double* uN, const double* gxN,
         }
     }
     __syncthreads();  // every thread has finished reading s2 before it is reused
-    if (act) put8<S>(s2, sidx, F, G);
+    if constexpr (CURV) {
+        double Fc[4], Gc[4];
+        to_contravariant(Jn, F, G, Fc, Gc);
+        if (act) put8<S>(s2, sidx, Fc, Gc);
+    } else {
+        if (act) put8<S>(s2, sidx, F, G);
+    }
     __syncthreads();
     contract<n, S>(s2, sbase, i, j, sDh, vol);
 }
@@ -1396,18 +1457,18 @@ __device__ __forceinline__ void mdrkp_ft(const double* u, const double* ut, cons
 
 // mdrk_kernel_1! with viscous terms (multi_derivative_2d_parabolic.jl:221-464):
 // F̃ = f/2 + f_t/8, Ũ = u/2 + ut/8 on the total flux.
-template <int N, int EB>
+template <int N, int EB, bool CURV = false>
 __device__ __forceinline__ void mdrkp1_element(const double* uN, const double* gxN, const double* gyN, double invJ,
                                                double gm1, const DevConst* c, int i, int j, bool act, int sidx,
                                                int sbase, double* s0, double* s1, double* s2, const double* sDm,
                                                const double* sD, const double* sDh, double* Uc, double* F,
-                                               double* G, double* vol) {
+                                               double* G, double* vol, const double* Jn = nullptr) {
     using G_ = VGeo<N, EB>;
     constexpr int n = G_::n, S = G_::S;
     double f[4], g[4];
     ns_total_fluxes(uN, gxN, gyN, gm1, c, f, g);
     double ut[4], utx[4], uty[4];
-    par_level<N, EB>(act, sidx, sbase, i, j, invJ, s0, s2, sDm, sD, f, g, ut, utx, uty);
+    par_level<N, EB, CURV>(act, sidx, sbase, i, j, invJ, s0, s2, sDm, sD, f, g, ut, utx, uty, Jn);
     double dx[4], dy[4];
     mdrkp_ft(uN, ut, gxN, utx, gyN, uty, gm1, c, dx, dy);
 #pragma unroll
@@ -1417,7 +1478,13 @@ __device__ __forceinline__ void mdrkp1_element(const double* uN, const double* g
         Uc[v] = fma(0.125, ut[v], 0.5 * uN[v]);
     }
     __syncthreads();  // every thread has finished reading s2 before it is reused
-    if (act) put8<S>(s2, sidx, F, G);
+    if constexpr (CURV) {
+        double Fc[4], Gc[4];
+        to_contravariant(Jn, F, G, Fc, Gc);
+        if (act) put8<S>(s2, sidx, Fc, Gc);
+    } else {
+        if (act) put8<S>(s2, sidx, F, G);
+    }
     __syncthreads();
     contract<n, S>(s2, sbase, i, j, sDh, vol);
 }
@@ -1425,19 +1492,25 @@ __device__ __forceinline__ void mdrkp1_element(const double* uN, const double* g
 // mdrk_kernel_2! with viscous terms (multi_derivative_2d_parabolic.jl:506-711):
 // F̃ = F2 + f_t(us)/3, Ũ = U2 + ust/3, with F2 = f + f_t/6 and U2 = u + ut/6
 // recomputed from uⁿ and its BR1 gradient.
-template <int N, int EB>
+template <int N, int EB, bool CURV = false>
 __device__ __forceinline__ void mdrkp2_element(const double* uN, const double* gxN, const double* gyN,
                                                const double* usn, const double* gxS, const double* gyS,
                                                double invJ, double gm1, const DevConst* c, int i, int j, bool act,
                                                int sidx, int sbase, double* s0, double* s1, double* s2,
                                                const double* sDm, const double* sD, const double* sDh, double* Uc,
-                                               double* F, double* G, double* vol) {
+                                               double* F, double* G, double* vol, const double* Jn = nullptr) {
     using G_ = VGeo<N, EB>;
     constexpr int n = G_::n, S = G_::S;
     double f[4], g[4], fs[4], gs[4];
     ns_total_fluxes(uN, gxN, gyN, gm1, c, f, g);
     ns_total_fluxes(usn, gxS, gyS, gm1, c, fs, gs);
-    if (act) {
+    if constexpr (CURV) {
+        double cx[4], cy[4];
+        to_contravariant(Jn, f, g, cx, cy);
+        if (act) put8<S>(s0, sidx, cx, cy);
+        to_contravariant(Jn, fs, gs, cx, cy);
+        if (act) put8<S>(s1, sidx, cx, cy);
+    } else if (act) {
         put8<S>(s0, sidx, f, g);
         put8<S>(s1, sidx, fs, gs);
     }
@@ -1458,12 +1531,17 @@ __device__ __forceinline__ void mdrkp2_element(const double* uN, const double* g
     double utx[4], uty[4], stx[4], sty[4];
     contract_xy<n, S>(s2, sbase, i, j, sD, utx, uty);
     contract_xy<n, S>(s2 + 4 * S, sbase, i, j, sD, stx, sty);
+    if constexpr (CURV) {
+        grad_to_cartesian(Jn, invJ, utx, uty);
+        grad_to_cartesian(Jn, invJ, stx, sty);
+    } else {
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-        utx[v] *= invJ;
-        uty[v] *= invJ;
-        stx[v] *= invJ;
-        sty[v] *= invJ;
+        for (int v = 0; v < 4; ++v) {
+            utx[v] *= invJ;
+            uty[v] *= invJ;
+            stx[v] *= invJ;
+            sty[v] *= invJ;
+        }
     }
     double dx[4], dy[4];
     mdrkp_ft(uN, ut, gxN, utx, gyN, uty, gm1, c, dx, dy);
@@ -1481,7 +1559,13 @@ __device__ __forceinline__ void mdrkp2_element(const double* uN, const double* g
         Uc[v] = fma(1.0 / 3.0, ust[v], Uc[v]);
     }
     __syncthreads();  // every thread has finished reading s2 before it is reused
-    if (act) put8<S>(s2, sidx, F, G);
+    if constexpr (CURV) {
+        double Fc[4], Gc[4];
+        to_contravariant(Jn, F, G, Fc, Gc);
+        if (act) put8<S>(s2, sidx, Fc, Gc);
+    } else {
+        if (act) put8<S>(s2, sidx, F, G);
+    }
     __syncthreads();
     contract<n, S>(s2, sbase, i, j, sDh, vol);
 }
@@ -1495,14 +1579,6 @@ __device__ __forceinline__ void mdrkp2_element(const double* uN, const double* g
 // contravariant ones, Ja¹·(f, g) and Ja²·(f, g), for the derivative
 // contractions; the volume term is D̂ of the contravariant time average.
 // f(uⁿ) is kept in registers (f0x, f0y) for the centred stencils.
-__device__ __forceinline__ void to_contravariant(const double* J, const double* f, const double* g, double* fc,
-                                                 double* gc) {
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-        fc[v] = fma(J[0], f[v], J[1] * g[v]);
-        gc[v] = fma(J[2], f[v], J[3] * g[v]);
-    }
-}
 
 template <int N, int EB>
 __device__ __forceinline__ void lwc_element(const double* uN, const double* J, double invJ, double gm1, int i, int j,
@@ -1788,7 +1864,8 @@ template <bool TWO> constexpr int kVolWaves = TWO ? 2 : TLW_VOL_WAVES;
 template <int N, int EB, int KIND, bool SC, bool PAR = false, bool CURV = false>
 __global__ __launch_bounds__((VGeo<N, EB>::NT), kVolWaves<SC || PAR>) void k_volume(KArgs a) {
     static_assert(!(SC && PAR), "shock capturing has no hyperbolic-parabolic form");
-    static_assert(!CURV || (KIND != VOL_LW_ADAPT && !PAR), "the curvilinear path is LW or MDRK, Euler");
+    static_assert(!CURV || KIND != VOL_LW_ADAPT, "the curvilinear path is LW or MDRK");
+    static_assert(!(CURV && PAR && SC), "no shock capturing with parabolic terms");
     using G_ = VGeo<N, EB>;
     constexpr int n = G_::n, NN = G_::NN, S = G_::S, NT = G_::NT;
     __shared__ double lds[G_::LDS];
@@ -1827,9 +1904,9 @@ __global__ __launch_bounds__((VGeo<N, EB>::NT), kVolWaves<SC || PAR>) void k_vol
     double uN[4], Uc[4], F[4], G[4], vol[4];
     double terr = 0.0;  // temporal error of the node (adaptive steps, LW N=4)
     load4(uin + off, uN);
-    if constexpr (CURV) {
-        double Jn[4];  // Ja11, Ja12, Ja21, Ja22 of the node: same layout as the state
-        load4(a.Ja + off, Jn);
+    double Jn[4] = {0.0, 0.0, 0.0, 0.0};  // Ja11, Ja12, Ja21, Ja22 of the node: same layout as the state
+    if constexpr (CURV) load4(a.Ja + off, Jn);
+    if constexpr (CURV && !PAR) {
         if constexpr (KIND == VOL_LW) {
             lwc_element<N, EB>(uN, Jn, invJ, gm1, i, j, act, sidx, sbase, s0, s1, s2, sDm, sDh, Uc, F, G, vol);
         } else if constexpr (KIND == VOL_MDRK1) {
@@ -1851,19 +1928,19 @@ __global__ __launch_bounds__((VGeo<N, EB>::NT), kVolWaves<SC || PAR>) void k_vol
             load4(a.us + off, usn);
             if (act) put4<S>(sg + 4 * S, sidx, usn);
             __syncthreads();
-            br1_gradient<N, EB>(sg, uin, a.nbr, c, e, i, j, sbase, invJ, sDh, uN, gxN, gyN);
-            br1_gradient<N, EB>(sg + 4 * S, a.us, a.nbr, c, e, i, j, sbase, invJ, sDh, usn, gxS, gyS);
-            mdrkp2_element<N, EB>(uN, gxN, gyN, usn, gxS, gyS, invJ, gm1, c, i, j, act, sidx, sbase, s0, s1, s2, sDm,
-                                  sD, sDh, Uc, F, G, vol);
+            br1_gradient<N, EB, CURV>(sg, uin, a.nbr, c, e, i, j, sbase, invJ, sDh, uN, gxN, gyN, Jn);
+            br1_gradient<N, EB, CURV>(sg + 4 * S, a.us, a.nbr, c, e, i, j, sbase, invJ, sDh, usn, gxS, gyS, Jn);
+            mdrkp2_element<N, EB, CURV>(uN, gxN, gyN, usn, gxS, gyS, invJ, gm1, c, i, j, act, sidx, sbase, s0, s1, s2,
+                                        sDm, sD, sDh, Uc, F, G, vol, Jn);
         } else {
             __syncthreads();
-            br1_gradient<N, EB>(sg, uin, a.nbr, c, e, i, j, sbase, invJ, sDh, uN, gxN, gyN);
+            br1_gradient<N, EB, CURV>(sg, uin, a.nbr, c, e, i, j, sbase, invJ, sDh, uN, gxN, gyN, Jn);
             if constexpr (KIND == VOL_LW)
-                lwp_element<N, EB>(uN, gxN, gyN, invJ, gm1, c, i, j, act, sidx, sbase, s0, s1, s2, sDm, sD, sDh, Uc, F,
-                                   G, vol);
+                lwp_element<N, EB, CURV>(uN, gxN, gyN, invJ, gm1, c, i, j, act, sidx, sbase, s0, s1, s2, sDm, sD, sDh,
+                                         Uc, F, G, vol, Jn);
             else
-                mdrkp1_element<N, EB>(uN, gxN, gyN, invJ, gm1, c, i, j, act, sidx, sbase, s0, s1, s2, sDm, sD, sDh,
-                                      Uc, F, G, vol);
+                mdrkp1_element<N, EB, CURV>(uN, gxN, gyN, invJ, gm1, c, i, j, act, sidx, sbase, s0, s1, s2, sDm, sD,
+                                            sDh, Uc, F, G, vol, Jn);
         }
     } else if (KIND == VOL_LW || KIND == VOL_LW_ADAPT) {
         if constexpr (N == 4 && KIND == VOL_LW_ADAPT)
@@ -2336,7 +2413,7 @@ __global__ __launch_bounds__(kSurfBlock) void k_surface(KArgs a) {
         if (bad) st->bad_state = 1;
         s_l1[lec * NN + node] = l1;
         s_l2[lec * NN + node] = l2;
-        s_lv[lec * NN + node] = par ? lam_visc(val, invJ, c) : 0.0;
+        s_lv[lec * NN + node] = par ? (CURV ? lam_visc_curved(val, invJ, c) : lam_visc(val, invJ, c)) : 0.0;
     }
     __syncthreads();
     if (act && node == 0) {
@@ -2395,7 +2472,9 @@ __global__ __launch_bounds__(kSurfBlock) void k_max_speed(KArgs a) {
     if (act) {
         s_l1[lec * NN + node] = l1;
         s_l2[lec * NN + node] = l2;
-        s_lv[lec * NN + node] = par ? lam_visc(val, a.inv_jac[e], a.c) : 0.0;
+        s_lv[lec * NN + node] = par ? (a.Ja ? lam_visc_curved(val, a.ijn[(size_t)e * NN + node], a.c)
+                                            : lam_visc(val, a.inv_jac[e], a.c))
+                                    : 0.0;
     }
     __syncthreads();
     if (act && node == 0) {

@@ -29,6 +29,7 @@ EXPORTED = (
     "tlw_get_elements_per_workgroup", "tlw_autotune", "tlw_set_positivity_limiter", "tlw_apply_positivity_limiter",
     "tlw_adjoint_lw", "tlw_create_batch", "tlw_n_instances", "tlw_batch_status", "tlw_enable_graphs",
     "tlw_create_parabolic", "tlw_solve_init_adaptive", "tlw_rejected_steps", "tlw_create_curved",
+    "tlw_create_curved_parabolic",
 )
 
 
@@ -123,6 +124,9 @@ def lib():
     L.tlw_create_curved.restype = C.c_int
     L.tlw_create_curved.argtypes = [C.POINTER(MeshDesc), C.POINTER(BasisDesc), C.POINTER(SolverDesc),
                                     C.POINTER(CurvedDesc), C.POINTER(vp)]
+    L.tlw_create_curved_parabolic.restype = C.c_int
+    L.tlw_create_curved_parabolic.argtypes = [C.POINTER(MeshDesc), C.POINTER(BasisDesc), C.POINTER(SolverDesc),
+                                              C.POINTER(CurvedDesc), C.POINTER(ParabolicDesc), C.POINTER(vp)]
     L.tlw_rebuild_mesh.restype = C.c_int
     L.tlw_rebuild_mesh.argtypes = [vp, C.POINTER(MeshDesc)]
     L.tlw_destroy.restype = None

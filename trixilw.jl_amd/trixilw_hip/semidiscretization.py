@@ -388,6 +388,8 @@ class SemidiscretizationHyperbolicParabolic(SemidiscretizationHyperbolic):
     conditions None).  rhs / rhs_mdrk1 / rhs_mdrk2, max_dt and the device loop
     then include the viscous terms (BR1 gradient, Taylor states of ∇u, central
     viscous interface flux, compute_dt's min of advective and viscous bounds).
+    On a periodic curvilinear StructuredMesh the P4est parabolic kernels'
+    curvilinear form applies (src/solvers/dgsem_p4est/dg_2d_parabolic.jl).
     """
 
     def __init__(self, mesh, time_discretization, equations, initial_condition, solver, boundary_conditions=None,
@@ -410,6 +412,15 @@ class SemidiscretizationHyperbolicParabolic(SemidiscretizationHyperbolic):
                          source_terms=source_terms, keep_aux=keep_aux, device=device)
 
     def _create(self, md, basis, sd, h):
+        if getattr(self.mesh, "curved", False):
+            k = self._keep
+            cd = _lib.CurvedDesc(_dptr(k["Ja"]), _dptr(k["ijn"]), _dptr(k["xy"]))
+            rc = _lib.lib().tlw_create_curved_parabolic(C.byref(md), C.byref(basis), C.byref(sd), C.byref(cd),
+                                                        C.byref(self._par), C.byref(h))
+            if rc != _lib.TLW_OK:
+                raise _lib.TLWError(rc, "tlw_create_curved_parabolic failed (periodic mesh and positive Jacobians "
+                                        "required)")
+            return
         rc = _lib.lib().tlw_create_parabolic(C.byref(md), C.byref(basis), C.byref(sd), C.byref(self._par),
                                              C.byref(h))
         if rc != _lib.TLW_OK:
