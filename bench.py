@@ -56,9 +56,16 @@ WORKLOADS = {
     # x, y += 20·0.05 sin(2πs) sin(2πt) (s, t the unit-square coordinates), CFL 0.98·LW limit
     "lw_euler_vortex_warped_1024x1024_N4": dict(nx=1024, N=4, td="lw", problem="vortex_warped"),
     "lw_euler_vortex_warped_256x256_N4": dict(nx=256, N=4, td="lw", problem="vortex_warped"),
+    # compressible Navier-Stokes on a periodic curvilinear mesh: the shear/density/pressure waves
+    # moved to [0,2]^2, warped by x, y += 2·0.1 sin(2πs) sin(2πt), mu 0.001 (Reynolds number 1000
+    # for unit velocity and length scales), Pr 0.72, CFL 0.5; the 128² MDRK N=3 case for 300 steps
+    # is the curved Navier-Stokes configuration of the north-star list
+    "mdrk_navierstokes_warped_128x128_N3": dict(nx=128, N=3, td="mdrk", problem="ns_warped", cfl=0.5),
+    "lw_navierstokes_warped_1024x1024_N4": dict(nx=1024, N=4, td="lw", problem="ns_warped", cfl=0.5),
 }
 WARP_L, WARP_A = 20.0, 0.05
 NS_MU, NS_PR = 0.01, 0.72
+CNS_L, CNS_A, CNS_MU = 2.0, 0.1, 0.001
 DEFAULT_WORKLOAD = "lw_euler_vortex_1024x1024_N4"
 
 
@@ -83,15 +90,13 @@ def alg_bytes_per_dof(N: int, td: str, parabolic: bool = False, curved: bool = F
     """
     n = N + 1
     br1 = 8.0 * 4 * 4 * n / (n * n) if parabolic else 0.0
-    if curved:
-        step = 8.0 * (2 * 4 * n * n + 2 * 3 * 4 * 4 * n + 2 * 5 * n * n + 4 * 4 * n) / (n * n)
-        vol = 8.0 * (2 * 4 * n * n + 3 * 4 * 4 * n + 5 * n * n + 4 * 4 * n) / (n * n)
-        return step, vol
+    cstep = 8.0 * (2 * 5 * n * n + 4 * 4 * n) / (n * n) if curved else 0.0
+    cvol = 8.0 * (5 * n * n + 4 * 4 * n) / (n * n) if curved else 0.0
     if td == "lw":
-        step = 8.0 * (2 * 4 * n * n + 2 * 3 * 4 * 4 * n) / (n * n) + br1
-        vol = 8.0 * (2 * 4 * n * n + 3 * 4 * 4 * n) / (n * n) + br1
+        step = 8.0 * (2 * 4 * n * n + 2 * 3 * 4 * 4 * n) / (n * n) + br1 + cstep
+        vol = 8.0 * (2 * 4 * n * n + 3 * 4 * 4 * n) / (n * n) + br1 + cvol
     else:
-        step = 8.0 * (5 * 4 * n * n + 4 * 48 * n) / (n * n) + (3 if parabolic else 0) * br1
+        step = 8.0 * (5 * 4 * n * n + 4 * 48 * n) / (n * n) + (3 if parabolic else 0) * br1 + 2 * cstep
         vol = None
     return step, vol
 
@@ -231,6 +236,15 @@ def make_problem(wl):
         ic = ns_shear
         lo, hi, bc, periodic = -1.0, 1.0, None, True
         okw = dict(parabolic="navier_stokes", mu=NS_MU, prandtl=NS_PR)
+    elif problem == "ns_warped":
+        import oracle_ctypes
+
+        def ic(x, y, t=0.0):
+            return ns_shear(x - 1.0, y - 1.0)
+
+        lo, hi, bc, periodic = 0.0, CNS_L, None, True
+        okw = dict(parabolic="navier_stokes", mu=CNS_MU, prandtl=NS_PR,
+                   mapping=(oracle_ctypes.MAP_WARP, (CNS_L, CNS_A)))
     elif problem == "vortex_warped":
         import oracle_ctypes
 
@@ -242,11 +256,13 @@ def make_problem(wl):
         lo, hi, bc, periodic = -10.0, 10.0, None, True
     if problem == "vortex_warped":
         mesh = tlw.StructuredMesh((wl["nx"], wl["nx"]), warp_mapping(WARP_L, WARP_A))
+    elif problem == "ns_warped":
+        mesh = tlw.StructuredMesh((wl["nx"], wl["nx"]), warp_mapping(CNS_L, CNS_A))
     else:
         mesh = tlw.TreeMesh((lo, lo), (hi, hi), wl["level"], periodicity=periodic, ordering="morton")
     solver = tlw.DGSEM(N, volume_integral=vi)
-    if problem == "ns_shear":
-        eqp = tlw.CompressibleNavierStokesDiffusion2D(eq, mu=NS_MU, Prandtl=NS_PR,
+    if problem in ("ns_shear", "ns_warped"):
+        eqp = tlw.CompressibleNavierStokesDiffusion2D(eq, mu=NS_MU if problem == "ns_shear" else CNS_MU, Prandtl=NS_PR,
                                                      gradient_variables=tlw.GradientVariablesConservative())
         semi = tlw.SemidiscretizationHyperbolicParabolic(mesh, tlw.get_time_discretization(solver), (eq, eqp), ic,
                                                          solver)
@@ -386,6 +402,43 @@ def measure_shock_capturing(steps, cpu_steps, threads, with_cpu=True):
     return out
 
 
+def measure_curved_navier_stokes(steps, cpu_steps, threads, with_cpu=True):
+    """Sub-record: MDRK N=3 compressible Navier-Stokes on the warped 128²
+    StructuredMesh (warp amplitude 0.1, Reynolds number 1000, Pr 0.72) for
+    `steps` steps of the device loop, and the CPU oracle."""
+    name = "mdrk_navierstokes_warped_128x128_N3"
+    wl = WORKLOADS[name]
+    semi, mesh, ic, cfl, (lo, hi), okw = make_problem(wl)
+    x, y = semi.node_coordinates()
+    u0 = np.ascontiguousarray(ic(x, y, 0.0))
+    r = run_steps(semi, u0, cfl, steps, 2)
+    dofs = semi.ndofs()
+    step_b, _ = alg_bytes_per_dof(3, "mdrk", parabolic=True, curved=True)
+    out = {"workload": name, "value": r["value"], "unit": "DOF-updates/s",
+           "ms_per_step": round(r["ms_per_step"], 4), "steps": steps, "dofs": dofs,
+           "volume_avg_launch_ms": round(r["vol_ms"], 4), "surface_avg_launch_ms": round(r["surf_ms"], 4),
+           "step_alg_bytes_per_dof": step_b, "elements_per_workgroup": r["tile"], "tile_autotune_ms": r["tune"],
+           "config": f"compressible Navier-Stokes, conservative gradients, mu {CNS_MU} (Re 1000), Pr {NS_PR}, "
+                     f"periodic StructuredMesh [0,{CNS_L:g}]^2 warped by amplitude {CNS_A}, {mesh.nx}x{mesh.ny}, "
+                     f"polydeg 3, MDRK, CFL {cfl}"}
+    log(f"bench: curvilinear Navier-Stokes MDRK {r['value'] / 1e9:.3f} G DOF-updates/s, {r['ms_per_step']:.3f} "
+        f"ms/step, volume {r['vol_ms']:.3f} ms x{r['vol_n']}, surface {r['surf_ms']:.3f} ms")
+    semi.close()
+    if with_cpu:
+        import oracle_ctypes
+
+        nx = mesh.nx
+        o = oracle_ctypes.Oracle(3, nx, nx, lo, lo, (hi - lo) / nx, **okw)
+        xo, yo = o.node_coords()
+        v, el = cpu_baseline(o, np.ascontiguousarray(ic(xo, yo, 0.0)), cfl, True, dofs, cpu_steps, threads)
+        log(f"bench: curvilinear Navier-Stokes CPU {v / 1e6:.2f} M DOF-updates/s ({el:.1f} s)")
+        out["cpu_baseline"] = {"value": v, "unit": "DOF-updates/s", "cores": threads, "kind": "port",
+                               "sample": f"{cpu_steps} steps of the same mesh, C++/OpenMP oracle"}
+        out["gpu_over_cpu"] = round(r["value"] / v, 2)
+        del o
+    return out
+
+
 def measure_curved(steps, cpu_steps, threads, with_cpu=True):
     """Sub-record: LW N=4 isentropic vortex on the warped 1024² StructuredMesh,
     device loop throughput, volume kernel bytes per launch and the CPU oracle."""
@@ -472,7 +525,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-instance batch measurement")
     ap.add_argument("--no-ns", action="store_true", help="skip the Navier-Stokes sub-record")
     ap.add_argument("--no-adaptive", action="store_true", help="skip the adaptive-stepping sub-record")
-    ap.add_argument("--no-curved", action="store_true", help="skip the curvilinear-mesh sub-record")
+    ap.add_argument("--no-curved", action="store_true", help="skip the curvilinear-mesh sub-records")
     ap.add_argument("--no-sc", action="store_true", help="skip the shock-capturing (MUSCL-Hancock) sub-record")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device copy (the reference kernel for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -504,8 +557,8 @@ def main():
     wl = WORKLOADS[args.workload]
     N, td = wl["N"], wl["td"]
     sc = wl.get("problem") == "riemann_sc"
-    par = wl.get("problem") == "ns_shear"
-    curved = wl.get("problem") == "vortex_warped"
+    par = wl.get("problem") in ("ns_shear", "ns_warped")
+    curved = wl.get("problem") in ("vortex_warped", "ns_warped")
     semi, mesh, ic, cfl, (lo, hi), okw = make_problem(wl)
     dofs = semi.ndofs()
     log(f"bench: {args.workload}: {mesh.n_elements} elements, {dofs} DOFs")
@@ -610,8 +663,10 @@ def main():
                                          with_cpu=not args.no_cpu_baseline)
 
     curved_rec = None
+    curved_ns = None
     if args.workload == DEFAULT_WORKLOAD and not args.no_curved:
         curved_rec = measure_curved(max(3, min(args.steps, 10)), 1, threads, with_cpu=not args.no_cpu_baseline)
+        curved_ns = measure_curved_navier_stokes(300, 20, threads, with_cpu=not args.no_cpu_baseline)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -649,14 +704,15 @@ def main():
         "data": "synthetic",
         "config": {"workload": args.workload,
                    "mesh": (f"{mesh.nx}x{mesh.nx} TreeMesh [0,1]^2, time-averaged Dirichlet boundary" if sc else
-                            f"{mesh.nx}x{mesh.nx} periodic StructuredMesh [0,{WARP_L:g}]^2 warped by {WARP_A}"
-                            if curved else f"{mesh.nx}x{mesh.nx} periodic TreeMesh [{lo:g},{hi:g}]^2"),
+                            f"{mesh.nx}x{mesh.nx} periodic StructuredMesh [0,{hi:g}]^2 warped by "
+                            f"{CNS_A if par else WARP_A}" if curved
+                            else f"{mesh.nx}x{mesh.nx} periodic TreeMesh [{lo:g},{hi:g}]^2"),
                    "polydeg": N, "time_discretization": td.upper(), "dofs": dofs,
                    "initial_condition": ("four-quadrant Riemann problem (Lax-Liu 3)" if sc else
                                          "shear flow with density/pressure waves" if par else
                                          "isentropic vortex strength 5, Mach 0.5"),
-                   "parabolic": (f"CompressibleNavierStokesDiffusion2D mu {NS_MU}, Pr {NS_PR}, conservative "
-                                 "gradients" if par else None),
+                   "parabolic": (f"CompressibleNavierStokesDiffusion2D mu {CNS_MU if curved else NS_MU}, "
+                                 f"Pr {NS_PR}, conservative gradients" if par else None),
                    "shock_capturing": (("Hennemann-Gassner on density_pressure, alpha_max 0.5, "
                                         + ("MUSCL-Hancock" if wl.get("recon") == "muscl_hancock" else "first-order")
                                         + " subcell FV") if sc else None),
@@ -669,6 +725,7 @@ def main():
         "navier_stokes": navier_stokes,
         "adaptive": adaptive,
         "curved": curved_rec,
+        "curved_navier_stokes": curved_ns,
         "shock_capturing": sc_rec,
     }
     print(json.dumps(out), flush=True)
