@@ -51,8 +51,11 @@ enum { TLW_LW = 0, TLW_MDRK = 1 };
 enum { TLW_FLUX_TREE = 0, TLW_FLUX_STRUCTURED = 1 };
 
 /* boundary value functions of the time-averaged Dirichlet condition
- * (src/equations/equations.jl:6-37) */
-enum { TLW_BV_NONE = 0, TLW_BV_VORTEX = 1, TLW_BV_CONSTANT = 2, TLW_BV_RIEMANN4 = 3 };
+ * (src/equations/equations.jl:6-37), or TLW_BV_SLIP_WALL: the reflecting wall of
+ * boundary_condition_slip_wall_vertical on x faces and boundary_condition_slip_wall_horizontal on
+ * y faces (equations.jl:95-139: mirrored uⁿ and Ũ, F_outer = (-F1, F2, -F3, -F4) resp.
+ * (-F1, -F2, F3, -F4), the LW flux in the boundary order of the Dirichlet call); TreeMesh only */
+enum { TLW_BV_NONE = 0, TLW_BV_VORTEX = 1, TLW_BV_CONSTANT = 2, TLW_BV_RIEMANN4 = 3, TLW_BV_SLIP_WALL = 4 };
 
 /* tlw_get_aux selectors (element_cache fields, src/solvers/dgsem_tree/containers.jl:73-98,
  * and the shock-capturing blending factors, indicator.cache.alpha) */

@@ -16,7 +16,7 @@ import numpy as np
 HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "liboracle_tlw.so"
 
-BV_NONE, BV_VORTEX, BV_CONSTANT, BV_RIEMANN4 = 0, 1, 2, 3
+BV_NONE, BV_VORTEX, BV_CONSTANT, BV_RIEMANN4, BV_SLIP_WALL = 0, 1, 2, 3, 4
 
 
 class _Desc(C.Structure):

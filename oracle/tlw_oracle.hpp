@@ -316,7 +316,10 @@ template <class T> struct LinAdv2D {
 // ---------------------------------------------------------------------------
 // Boundary value functions for the time-averaged Dirichlet condition
 // (src/equations/equations.jl:6-37).  Same set as the HIP library.
-enum BoundaryValueKind { BV_NONE = 0, BV_VORTEX = 1, BV_CONSTANT = 2, BV_RIEMANN4 = 3 };
+// BV_SLIP_WALL is no boundary value but the reference's reflecting wall
+// (boundary_condition_slip_wall_vertical on x faces, _horizontal on y faces,
+// src/equations/equations.jl:95-139).
+enum BoundaryValueKind { BV_NONE = 0, BV_VORTEX = 1, BV_CONSTANT = 2, BV_RIEMANN4 = 3, BV_SLIP_WALL = 4 };
 
 struct BoundaryValue {
     int kind = BV_NONE;
@@ -1340,6 +1343,23 @@ template <class T, class Eq> struct Solver {
                 T Uo[NV], Fo[NV], uo[NV];
                 for (int v = 0; v < NV; ++v) { Uo[v] = T(0); Fo[v] = T(0); }
                 const double dts = scaling_factor * dt;
+                if (bval.kind == BV_SLIP_WALL) {
+                    // mirror states: the normal momentum changes sign; the normal flux
+                    // keeps only its normal-momentum component (F_outer = (-F1, F2, -F3,
+                    // -F4) on x faces, (-F1, -F2, F3, -F4) on y faces)
+                    for (int v = 0; v < NV; ++v) {
+                        const bool m = (v == o);
+                        Uo[v] = m ? -U_in[v] : U_in[v];
+                        uo[v] = m ? -u_in[v] : u_in[v];
+                        Fo[v] = m ? F_in[v] : -F_in[v];
+                    }
+                    T Fn[NV];
+                    const int bo = flux_order == 2 ? 0 : flux_order;
+                    if (dir % 2 == 0) lw_numflux_ord(F_in, Fo, U_in, Uo, u_in, uo, o, bo, Fn);
+                    else lw_numflux_ord(Fo, F_in, Uo, U_in, uo, u_in, o, bo, Fn);
+                    for (int v = 0; v < NV; ++v) sfv[sidx(v, l, dir - 1, e)] = Fn[v];
+                    continue;
+                }
                 for (int q = 0; q < n; ++q) {
                     const double ts = t + 0.5 * dts * (nodes[q] + 1.0);
                     double ubd[4];

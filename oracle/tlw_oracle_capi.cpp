@@ -88,6 +88,8 @@ void* tlwo_create(const tlwo_desc* d) {
     if (d->curved && d->parabolic == 1) return nullptr;
     if (d->curved && d->shock_capturing && (d->reconstruction != 0 || (d->alpha_smooth && !periodic))) return nullptr;
     if (d->reconstruction == 1 && (d->equation != 0 || !d->shock_capturing)) return nullptr;
+    // the reflecting wall is the Euler TreeMesh one (x faces vertical, y faces horizontal)
+    if (d->bc_kind == BV_SLIP_WALL && (d->equation != 0 || d->curved)) return nullptr;
     auto* c = new tlwo_ctx;
     c->equation = d->equation;
     if (d->curved && d->parabolic == 2) {

@@ -1206,7 +1206,7 @@ int create_common(int ninst, const tlw_mesh_desc* meshes, const tlw_basis_desc* 
     if (solver->time_discretization != TLW_LW && solver->time_discretization != TLW_MDRK) return TLW_EINVAL;
     if (solver->surface_flux_order != TLW_FLUX_TREE && solver->surface_flux_order != TLW_FLUX_STRUCTURED)
         return TLW_EINVAL;
-    if (solver->boundary_kind < TLW_BV_NONE || solver->boundary_kind > TLW_BV_RIEMANN4) return TLW_EINVAL;
+    if (solver->boundary_kind < TLW_BV_NONE || solver->boundary_kind > TLW_BV_SLIP_WALL) return TLW_EINVAL;
     if (!(solver->gamma > 1.0)) return TLW_EINVAL;
     tlw_ctx* c = new tlw_ctx;
     c->N = basis->polydeg;
@@ -1297,6 +1297,10 @@ int create_common(int ninst, const tlw_mesh_desc* meshes, const tlw_basis_desc* 
         // scheme (MUSCL-Hancock is rejected by the reference on such meshes), or
         // compressible Navier-Stokes (periodic, checked at upload)
         if (ninst != 1 || (c->sc && h.recon != TLW_RECON_FIRST_ORDER)) {
+            delete c;
+            return TLW_ENOTSUP;
+        }
+        if (h.bc_kind == TLW_BV_SLIP_WALL) {  // the reference's curvilinear wall is another function
             delete c;
             return TLW_ENOTSUP;
         }

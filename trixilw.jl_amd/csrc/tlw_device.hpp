@@ -31,6 +31,7 @@
 
 namespace tlw {
 
+constexpr int kBcSlipWall = 4;  // boundary kind of the reflecting wall (TLW_BV_SLIP_WALL)
 constexpr int kTraceQ = 10;  // face record: jump state (4), normal flux (4), |v_n|, c
 // shock-capturing face record: + fn_low (4), blending factor, padding
 constexpr int kTraceQSC = 16;
@@ -2372,6 +2373,18 @@ __global__ __launch_bounds__(kSurfBlock) void k_surface(KArgs a) {
                     boundary_record_curved<N>(a, e, f, l, t, dt, out10);
                     if (f & 1) face_numflux(in10, out10, Fs);
                     else face_numflux(out10, in10, Fs);
+                } else if (c->bc_kind == kBcSlipWall) {
+                    // reflecting wall (equations.jl:95-139): the mirrored record, whatever
+                    // the flux order (both states are mirrored, the speeds kept)
+                    const int m = (f < 2) ? 1 : 2;
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        oth[v] = (v == m) ? -own[v] : own[v];
+                        oth[4 + v] = (v == m) ? own[4 + v] : -own[4 + v];
+                    }
+                    oth[8] = own[8];
+                    oth[9] = own[9];
+                    face_numflux(Lr, Rr, Fs);
                 } else {
                     boundary_record<N>(a, e, f, l, t, dt, oth);
                     face_numflux(Lr, Rr, Fs);

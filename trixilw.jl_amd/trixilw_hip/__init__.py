@@ -10,7 +10,8 @@ from .basis import LobattoLegendreBasis
 from .batch import SemidiscretizationBatch
 from .equations import (CompressibleEulerEquations2D, CompressibleNavierStokesDiffusion2D, ConstantState,
                         FourQuadrantRiemann, GradientVariablesConservative, GradientVariablesPrimitive,
-                        IsentropicVortex, LinearScalarAdvectionEquation2D)
+                        IsentropicVortex, LinearScalarAdvectionEquation2D, boundary_condition_slip_wall_horizontal,
+                        boundary_condition_slip_wall_vertical)
 from .integrator import (LWIntegrator, LWSolution, compute_dt, isapprox, isfinished, perform_step, set_dt,
                          set_t_and_iter, solve_lwfr)
 from .limiters import PositivityPreservingLimiterZhangShu, density, pressure
@@ -24,6 +25,7 @@ from .semidiscretization import (DGSEM, LW, MDRK, Adaptive, CFLBased, FirstOrder
 
 __all__ = [
     "LobattoLegendreBasis", "CompressibleEulerEquations2D", "ConstantState", "FourQuadrantRiemann",
+    "boundary_condition_slip_wall_vertical", "boundary_condition_slip_wall_horizontal",
     "IsentropicVortex", "LinearScalarAdvectionEquation2D", "LWIntegrator", "LWSolution", "compute_dt", "isapprox",
     "isfinished", "perform_step", "set_dt", "set_t_and_iter", "solve_lwfr", "TreeMesh",
     "StructuredMesh", "coordinates2mapping", "DGSEM", "LW", "MDRK",

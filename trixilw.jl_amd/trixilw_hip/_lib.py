@@ -15,7 +15,7 @@ LIB_PATH = Path(os.environ.get("TRIXILW_HIP_LIBRARY") or Path(__file__).resolve(
 TLW_OK, TLW_EINVAL, TLW_EHIP, TLW_EADMISSIBLE, TLW_ENOTSUP = 0, 1, 2, 3, 4
 TLW_LW, TLW_MDRK = 0, 1
 TLW_FLUX_TREE, TLW_FLUX_STRUCTURED = 0, 1
-TLW_BV_NONE, TLW_BV_VORTEX, TLW_BV_CONSTANT, TLW_BV_RIEMANN4 = 0, 1, 2, 3
+TLW_BV_NONE, TLW_BV_VORTEX, TLW_BV_CONSTANT, TLW_BV_RIEMANN4, TLW_BV_SLIP_WALL = 0, 1, 2, 3, 4
 TLW_AUX_U, TLW_AUX_F, TLW_AUX_ALPHA = 0, 1, 2
 TLW_FLAG_KEEP_AUX = 1
 TLW_PAR_NAVIER_STOKES = 1

@@ -15,7 +15,48 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-BV_NONE, BV_VORTEX, BV_CONSTANT, BV_RIEMANN4 = 0, 1, 2, 3
+BV_NONE, BV_VORTEX, BV_CONSTANT, BV_RIEMANN4, BV_SLIP_WALL = 0, 1, 2, 3, 4
+
+
+class _SlipWall:
+    """The reference's reflecting wall of the LW TreeMesh path
+    (src/equations/equations.jl:95-139): boundary_condition_slip_wall_vertical
+    mirrors uⁿ and Ũ in v1 and keeps only the x-momentum component of the
+    normal flux, _horizontal does the same in y.  Pass the vertical one for the
+    x faces and the horizontal one for the y faces (one kind per mesh)."""
+
+    kind = BV_SLIP_WALL
+    params = ()
+
+    def __init__(self, orientation):
+        self.orientation = orientation
+
+    def __repr__(self):
+        return "boundary_condition_slip_wall_" + ("vertical" if self.orientation == 1 else "horizontal")
+
+
+boundary_condition_slip_wall_vertical = _SlipWall(1)
+boundary_condition_slip_wall_horizontal = _SlipWall(2)
+
+
+def boundary_conditions_kind(bcs):
+    """One boundary-value description for every boundary face from the
+    reference's forms: a single function object, or the NamedTuple (x_neg,
+    x_pos, y_neg, y_pos) as a dict or 4-tuple.  Per-direction conditions
+    must agree (the same function, or the vertical slip wall on the x faces
+    and the horizontal one on the y faces)."""
+    if bcs is None or not isinstance(bcs, (dict, tuple, list)):
+        return bcs
+    seq = [bcs[k] for k in ("x_neg", "x_pos", "y_neg", "y_pos")] if isinstance(bcs, dict) else list(bcs)
+    if len(seq) != 4:
+        raise ValueError("per-direction boundary conditions: (x_neg, x_pos, y_neg, y_pos)")
+    if all(isinstance(b, _SlipWall) for b in seq):
+        if [b.orientation for b in seq] != [1, 1, 2, 2]:
+            raise NotImplementedError("the slip wall of x faces is the vertical one, of y faces the horizontal one")
+        return seq[0]
+    if all(b is seq[0] for b in seq):
+        return seq[0]
+    raise NotImplementedError("the device path takes one boundary condition for every boundary face")
 
 
 @dataclass

@@ -187,11 +187,14 @@ class SemidiscretizationHyperbolic:
         sd.gamma = equations.gamma
         sd.surface_flux_order = {"tree": _lib.TLW_FLUX_TREE, "structured": _lib.TLW_FLUX_STRUCTURED}[
             surface_flux_order]
-        if boundary_conditions is None:
+        from .equations import boundary_conditions_kind
+
+        bcd = boundary_conditions_kind(boundary_conditions)
+        if bcd is None:
             sd.boundary_kind = _lib.TLW_BV_NONE
         else:
-            sd.boundary_kind = int(boundary_conditions.kind)
-            for k, v in enumerate(boundary_conditions.params):
+            sd.boundary_kind = int(bcd.kind)
+            for k, v in enumerate(bcd.params):
                 sd.boundary_params[k] = float(v)
         sd.flags = _lib.TLW_FLAG_KEEP_AUX if keep_aux else 0
         vi = solver.volume_integral
