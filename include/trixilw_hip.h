@@ -120,6 +120,15 @@ typedef struct {
     double mu;                   /* dynamic viscosity */
     double prandtl;              /* Prandtl number; κ = γ / ((γ-1) Pr) */
     int32_t gradient_variables;  /* TLW_GRAD_CONSERVATIVE */
+    /* 0: periodic mesh.  1: BoundaryConditionNavierStokesWall(NoSlip(v_wall), Adiabatic(0)) on every
+     * boundary face (compressible_navier_stokes_2d.jl:81-124: the gradient's boundary value
+     * prim2cons(ρ, v_wall, p) of the node, the viscous wall flux (F̃v1, F̃v2, F̃v3, v_wall·(F̃v2, F̃v3))
+     * subtracted from the hyperbolic wall flux, dg_2d_parabolic.jl:1951-2085), with the wall velocity
+     * (v1, v2) of x_neg, x_pos, y_neg, y_pos in wall_velocity; LW on a TreeMesh with
+     * solver->boundary_kind = TLW_BV_SLIP_WALL (the reference's lid-driven cavity), TLW_ENOTSUP
+     * otherwise */
+    int32_t walls;
+    double wall_velocity[8];
 } tlw_parabolic_desc;
 /* Curvilinear StructuredMesh metrics (Trixi cache.elements of a StructuredMesh,
  * src/solvers/dgsem_structured/containers_2d.jl; TrixiLW's kernels for it in

@@ -52,6 +52,8 @@ class _Desc(C.Structure):
         ("map_kind", C.c_int),
         ("map_par", C.c_double * 4),
         ("reconstruction", C.c_int),
+        ("ns_walls", C.c_int),
+        ("wall_v", C.c_double * 8),
     ]
 
 
@@ -127,13 +129,17 @@ class Oracle:
     def __init__(self, polydeg, nx, ny, xmin, ymin, h, periodic=(True, True), equation="euler", gamma=1.4,
                  adv=(1.0, 1.0), flux_order=0, bc_kind=BV_NONE, bc_par=(), tolerances=(0.0, 0.0), nthreads=0,
                  shock_capturing=False, alpha_max=0.5, alpha_min=0.001, alpha_smooth=True, positivity=None,
-                 parabolic=None, mu=0.0, prandtl=0.72, mapping=None, reconstruction="first_order"):
+                 parabolic=None, mu=0.0, prandtl=0.72, mapping=None, reconstruction="first_order",
+                 ns_walls=None):
         """``positivity``: (threshold_density, threshold_pressure) of a Zhang-Shu
         limiter that ``solve`` applies as a stage limiter, or None.
         ``parabolic``: None, "laplace" (LaplaceDiffusion2D with diffusivity ``mu``,
         linear advection) or "navier_stokes" (CompressibleNavierStokesDiffusion2D,
         dynamic viscosity ``mu``, Prandtl number ``prandtl``, conservative
         gradient variables); periodic meshes only.
+        ``ns_walls``: with parabolic="navier_stokes" and bc_kind=BV_SLIP_WALL, no-slip
+        adiabatic walls on every boundary face with the wall velocities (v1, v2) of
+        x_neg, x_pos, y_neg, y_pos (LW).
         ``mapping``: None (TreeMesh) or (MAP_AFFINE, (x0, y0, Lx, Ly)) /
         (MAP_WARP, (L, A)) for a periodic curvilinear StructuredMesh of nx × ny
         cells (xmin, ymin, h are then unused).
@@ -160,6 +166,10 @@ class Oracle:
             d.positivity = 1
             d.pos_threshold[0], d.pos_threshold[1] = positivity
         d.parabolic = PARABOLIC[parabolic]
+        if ns_walls is not None:  # (v1, v2) of the walls x_neg, x_pos, y_neg, y_pos
+            d.ns_walls = 1
+            for k, (a, b) in enumerate(ns_walls):
+                d.wall_v[2 * k], d.wall_v[2 * k + 1] = float(a), float(b)
         d.visc_mu, d.prandtl = mu, prandtl
         if mapping is not None:
             d.curved = 1

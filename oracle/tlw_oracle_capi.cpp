@@ -46,6 +46,11 @@ struct tlwo_desc {
     // shock-capturing reconstruction: 0 FirstOrderReconstruction,
     // 1 MUSCLHancockReconstruction (Euler only)
     int reconstruction;
+    // Navier-Stokes with no-slip adiabatic walls on every boundary face (LW; the
+    // hyperbolic part takes bc_kind = BV_SLIP_WALL): wall velocity (v1, v2) per
+    // direction x_neg, x_pos, y_neg, y_pos
+    int ns_walls;
+    double wall_v[8];
 };
 
 struct tlwo_ctx {
@@ -79,7 +84,8 @@ void* tlwo_create(const tlwo_desc* d) {
     if (d->polydeg != 3 && d->polydeg != 4) return nullptr;
     if (d->nthreads > 0) omp_set_num_threads(d->nthreads);
     if (d->parabolic) {
-        if (!d->periodic_x || !d->periodic_y || d->shock_capturing) return nullptr;
+        const bool walls = d->ns_walls && d->parabolic == 2 && d->bc_kind == BV_SLIP_WALL && !d->curved;
+        if ((!walls && (!d->periodic_x || !d->periodic_y)) || d->shock_capturing) return nullptr;
         if ((d->parabolic == 1) != (d->equation == 1)) return nullptr;
     }
     // curvilinear shock capturing: first order; Trixi's alpha smoothing on a
@@ -120,6 +126,11 @@ void* tlwo_create(const tlwo_desc* d) {
         p->visc.set(d->gamma, d->visc_mu, d->prandtl);
         configure(*p, d);
         p->setup_parabolic();
+        p->ns_walls = d->ns_walls;
+        for (int k = 0; k < 4; ++k) {
+            p->wall_v[k][0] = d->wall_v[2 * k];
+            p->wall_v[k][1] = d->wall_v[2 * k + 1];
+        }
         c->euler.reset(p);
     } else if (d->parabolic == 1) {
         auto* p = new ParabolicSolver<double, LinAdv2D<double>, LaplaceDiffusion2D<double>>;

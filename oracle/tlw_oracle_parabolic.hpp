@@ -128,6 +128,11 @@ template <class T, class Eq, class Visc> struct ParabolicSolver : Solver<T, Eq> 
     size_t cnode(int i, int j, int e) const { return ((size_t)e * n + j) * n + i; }
     double node_ij(int i, int j, int e) const { return cijn ? cijn[cnode(i, j, e)] : mesh.inv_jac[e]; }
 
+    // No-slip adiabatic walls on every boundary face (Euler with the reflecting wall
+    // for the hyperbolic part): wall velocity per direction x_neg, x_pos, y_neg, y_pos
+    int ns_walls = 0;
+    double wall_v[4][2] = {};
+
     void setup_parabolic() {
         const size_t nd = this->ndofs_total();
         gx.assign(nd, T(0));
@@ -176,6 +181,26 @@ template <class T, class Eq, class Visc> struct ParabolicSolver : Solver<T, Eq> 
                     const T fl = 0.5 * (u[idx(v, li, lj, L)] + u[idx(v, ri, rj, R)]);
                     gsfv[sidx(v, l, ldir, L)] = fl;
                     gsfv[sidx(v, l, rdir, R)] = fl;
+                }
+            }
+        }
+        if (ns_walls) {
+            // Trixi calc_boundary_flux_by_direction_gradient!: the boundary value of the
+            // gradient variables is the wall condition's u_outer; for
+            // BoundaryConditionNavierStokesWall{NoSlip, Adiabatic} with conservative
+            // gradient variables prim2cons(ρ, v_wall, p) of the inner trace
+            // (compressible_navier_stokes_2d.jl:113-124)
+            const int nB = (int)mesh.bd_elem.size();
+#pragma omp parallel for schedule(static)
+            for (int B = 0; B < nB; ++B) {
+                const int e = mesh.bd_elem[B], o = mesh.bd_orient[B], side = mesh.bd_side[B], dir = mesh.bd_dir[B];
+                for (int l = 0; l < n; ++l) {
+                    int ii, jj;
+                    if (o == 1) { ii = (side == 1) ? n - 1 : 0; jj = l; }
+                    else { ii = l; jj = (side == 1) ? n - 1 : 0; }
+                    T uo[NV];
+                    wall_state(&u[idx(0, ii, jj, e)], wall_v[dir - 1], uo);
+                    for (int v = 0; v < NV; ++v) gsfv[sidx(v, l, dir - 1, e)] = uo[v];
                 }
             }
         }
@@ -681,13 +706,55 @@ template <class T, class Eq, class Visc> struct ParabolicSolver : Solver<T, Eq> 
         }
     }
 
+    // prim2cons(ρ, v_wall, p) of an inner state (Euler; the wall condition's u_outer)
+    void wall_state(const T* ui, const double* vw, T* uo) const {
+        if constexpr (NV == 4) {
+            const double g = this->eq.gamma;
+            const T rho = ui[0];
+            const T p = (g - 1.0) * (ui[3] - 0.5 * (ui[1] * ui[1] + ui[2] * ui[2]) / rho);
+            uo[0] = rho;
+            uo[1] = rho * vw[0];
+            uo[2] = rho * vw[1];
+            uo[3] = p / (g - 1.0) + 0.5 * rho * (vw[0] * vw[0] + vw[1] * vw[1]);
+        } else {
+            for (int v = 0; v < NV; ++v) uo[v] = ui[v];
+        }
+    }
+
+    // prolong2boundaries_visc_lw! + calc_boundary_flux_divergence_lw!
+    // (dg_2d_parabolic.jl:1951-2085) with the NoSlip/Adiabatic wall of
+    // compressible_navier_stokes_2d.jl:81-91: the boundary flux (the hyperbolic wall
+    // flux already in place) minus (F̃v1, F̃v2, F̃v3, v_wall · (F̃v2, F̃v3) + 0) of the
+    // inner element's time-averaged viscous flux in the face's orientation
+    void wall_divergence_flux() {
+        const int nB = (int)mesh.bd_elem.size();
+#pragma omp parallel for schedule(static)
+        for (int B = 0; B < nB; ++B) {
+            const int e = mesh.bd_elem[B], o = mesh.bd_orient[B], side = mesh.bd_side[B], dir = mesh.bd_dir[B];
+            const double* vw = wall_v[dir - 1];
+            for (int l = 0; l < n; ++l) {
+                int ii, jj;
+                if (o == 1) { ii = (side == 1) ? n - 1 : 0; jj = l; }
+                else { ii = l; jj = (side == 1) ? n - 1 : 0; }
+                T fv[NV];
+                for (int v = 0; v < NV; ++v) fv[v] = Fv[fidx(v, o - 1, ii, jj, e)];
+                if constexpr (NV == 4) fv[3] = vw[0] * fv[1] + vw[1] * fv[2] + 0.0;
+                for (int v = 0; v < NV; ++v) this->sfv[sidx(v, l, dir - 1, e)] = this->sfv[sidx(v, l, dir - 1, e)] - fv[v];
+            }
+        }
+    }
+
     void rhs_lw(T* du, const T* u, double t, double dt) override {
-        (void)t;
         this->reset_du(du);
         calc_gradient(u);
 #pragma omp parallel for schedule(static)
         for (int e = 0; e < mesh.E; ++e) lw_volume_parabolic(du, u, dt, e);
         calc_interface_flux_hp(u);
+        if (ns_walls && !mesh.bd_elem.empty()) {
+            // the hyperbolic wall flux from F̃a (element cache F), then the viscous part
+            this->calc_boundary_flux(u, t, dt, 1.0);
+            wall_divergence_flux();
+        }
         this->surface_integral_and_jacobian(du);
     }
     void rhs_mdrk1(T* du, const T* u, double t, double dt) override {

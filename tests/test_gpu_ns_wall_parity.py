@@ -1,0 +1,150 @@
+"""GPU parity of the Navier-Stokes LW path with no-slip adiabatic walls
+(BoundaryConditionNavierStokesWall{NoSlip, Adiabatic}, conservative gradient
+variables, with the reflecting wall for the hyperbolic part: the reference's
+lid-driven cavity, examples/tree_2d_dgsem/elixir_navierstokes_lid_driven_cavity.jl)
+against the CPU oracle: residuals of a perturbed cavity state and of plane
+Couette flow, K-step device-loop states within 1e-12 relative (L2 and L∞),
+and the combinations the path rejects.  Inputs are seeded."""
+import numpy as np
+import pytest
+
+from gpu_helpers import assert_close, to_dev, to_lex, torch_mod
+
+pytestmark = pytest.mark.gpu
+
+GAMMA, PR = 1.4, 0.72
+LID = (1.0, 0.0)
+
+
+def cavity_state(x, y, t=0.0):
+    """initial_condition_cavity of the reference example: ρ = 1, v = 0,
+    p = 1 / (Ma² γ) with Ma = 0.1."""
+    rho = np.ones_like(x)
+    p = np.full_like(x, 1.0 / (0.1 ** 2 * GAMMA))
+    z = np.zeros_like(x)
+    return np.stack([rho, z, z, p / (GAMMA - 1)], axis=-1)
+
+
+def setup_walls(oracle_mod, N, level, mu=0.01, periodic_x=False, ic=cavity_state, lid=LID, td="lw"):
+    import trixilw_hip as tlw
+
+    eq = tlw.CompressibleEulerEquations2D(GAMMA)
+    eqp = tlw.CompressibleNavierStokesDiffusion2D(eq, mu=mu, Prandtl=PR,
+                                                 gradient_variables=tlw.GradientVariablesConservative())
+    mesh = tlw.TreeMesh((-1.0, -1.0), (1.0, 1.0), level, periodicity=(periodic_x, False))
+    solver = tlw.DGSEM(N, volume_integral=tlw.VolumeIntegralFR(tlw.MDRK() if td == "mdrk" else tlw.LW()))
+    rest = tlw.BoundaryConditionNavierStokesWall(tlw.NoSlip(lambda x, t, e: (0.0, 0.0)), tlw.Adiabatic(0.0))
+    moving = tlw.BoundaryConditionNavierStokesWall(tlw.NoSlip(lambda x, t, e: lid), tlw.Adiabatic(0.0))
+    bc_hyp = {"x_neg": tlw.boundary_condition_slip_wall_vertical, "x_pos": tlw.boundary_condition_slip_wall_vertical,
+              "y_neg": tlw.boundary_condition_slip_wall_horizontal,
+              "y_pos": tlw.boundary_condition_slip_wall_horizontal}
+    bc_par = {"x_neg": rest, "x_pos": rest, "y_neg": rest, "y_pos": moving}
+    semi = tlw.SemidiscretizationHyperbolicParabolic(mesh, tlw.get_time_discretization(solver), (eq, eqp), ic,
+                                                     solver, boundary_conditions=(bc_hyp, bc_par))
+    nx = mesh.nx
+    o = oracle_mod.Oracle(N, nx, nx, -1.0, -1.0, 2.0 / nx, gamma=GAMMA, periodic=(periodic_x, False),
+                          bc_kind=oracle_mod.BV_SLIP_WALL, parabolic="navier_stokes", mu=mu, prandtl=PR,
+                          ns_walls=[(0.0, 0.0), (0.0, 0.0), (0.0, 0.0), lid])
+    return semi, o, mesh
+
+
+def perturbed(o, seed, amp=0.05):
+    """The cavity state with seeded velocity and density perturbations."""
+    x, y = o.node_coords()
+    u = cavity_state(x, y)
+    rng = np.random.default_rng(seed)
+    rho = u[..., 0] * (1 + amp * rng.standard_normal(x.shape))
+    v1 = 0.3 * np.sin(np.pi * (x + 1) / 2) * np.sin(np.pi * (y + 1)) + amp * rng.standard_normal(x.shape)
+    v2 = 0.2 * np.cos(np.pi * x / 2) * np.sin(np.pi * y) + amp * rng.standard_normal(x.shape)
+    p = (GAMMA - 1) * u[..., 3]
+    return np.stack([rho, rho * v1, rho * v2, p / (GAMMA - 1) + 0.5 * rho * (v1 ** 2 + v2 ** 2)], axis=-1)
+
+
+@pytest.mark.parametrize("N", [3, 4])
+def test_cavity_rhs_parity(oracle_mod, N):
+    import trixilw_hip as tlw
+
+    torch = torch_mod()
+    print(f"NS walls cavity rhs N={N}", flush=True)
+    semi, o, mesh = setup_walls(oracle_mod, N, 3)
+    u_lex = perturbed(o, 30 + N)
+    dt = 0.8 * o.max_dt(u_lex)
+    du_ref = o.rhs(u_lex, 0.0, dt)
+    u = to_dev(mesh, u_lex)
+    du = torch.empty_like(u)
+    semi.cache["dt"][0] = dt
+    tlw.rhs(du, u, semi, 0.0)
+    torch.cuda.synchronize()
+    assert_close(to_lex(mesh, du), du_ref, what=f"NS walls cavity N={N}")
+    semi.close()
+
+
+@pytest.mark.parametrize("N", [3, 4])
+def test_couette_rhs_parity(oracle_mod, N):
+    """Plane Couette flow between the resting wall y = -1 and the lid y = 1
+    (x periodic): energy grows by μ (U/H)², mass and momentum stay."""
+    import trixilw_hip as tlw
+
+    torch = torch_mod()
+    print(f"NS walls Couette N={N}", flush=True)
+    U, mu = 0.5, 0.02
+
+    def couette(x, y, t=0.0):
+        v1 = U * (y + 1.0) / 2.0
+        z = np.zeros_like(x)
+        return np.stack([z + 1.0, v1, z, z + 1.0 / (GAMMA - 1) + 0.5 * v1 ** 2], axis=-1)
+
+    semi, o, mesh = setup_walls(oracle_mod, N, 2, mu=mu, periodic_x=True, ic=couette, lid=(U, 0.0))
+    x, y = o.node_coords()
+    u_lex = couette(x, y)
+    dt = 0.5 * o.max_dt(u_lex)
+    du_ref = o.rhs(u_lex, 0.0, dt)
+    u = to_dev(mesh, u_lex)
+    du = torch.empty_like(u)
+    semi.cache["dt"][0] = dt
+    tlw.rhs(du, u, semi, 0.0)
+    torch.cuda.synchronize()
+    got = to_lex(mesh, du)
+    # the residual is the small remainder of O(J⁻¹ |F|) ≈ 10 terms that cancel, so the
+    # two computations are compared in absolute terms at that scale
+    assert np.max(np.abs(got - du_ref)) < 1e-11
+    assert np.max(np.abs(got[..., 3] - mu * (U / 2.0) ** 2)) < 1e-10
+    assert np.max(np.abs(got[..., :3])) < 1e-12
+    semi.close()
+
+
+@pytest.mark.parametrize("N,K", [(4, 20), (3, 25)])
+def test_cavity_device_loop_k_steps(oracle_mod, N, K):
+    import trixilw_hip as tlw
+
+    print(f"NS walls cavity device loop N={N} K={K}", flush=True)
+    semi, o, mesh = setup_walls(oracle_mod, N, 3)
+    u_lex = perturbed(o, 40 + N, amp=0.02)
+    cfl = 0.5
+    u_ref, t_ref, dts = o.solve(u_lex, 0.0, 1e9, cfl, max_steps=K)
+    lw_update = tlw.semidiscretize(semi, semi.time_discretization, (0.0, float("inf")))
+    lw_update.u0.copy_(to_dev(mesh, u_lex))
+    sol = tlw.solve_lwfr(lw_update, (), 1e-6, None, time_step_computation=tlw.CFLBased(cfl), max_steps=K)
+    assert sol.stats.naccept == K
+    assert abs(sol.t - t_ref) <= 1e-13 * t_ref
+    assert_close(to_lex(mesh, sol.u), u_ref, what=f"NS walls cavity N={N} after {K} steps")
+    semi.close()
+
+
+def test_ns_walls_reject_unsupported(oracle_mod):
+    import trixilw_hip as tlw
+    from trixilw_hip import _lib
+
+    torch_mod()
+    with pytest.raises(_lib.TLWError) as ei:  # MDRK with walls
+        setup_walls(oracle_mod, 3, 2, td="mdrk")
+    assert ei.value.code == _lib.TLW_ENOTSUP
+    eq = tlw.CompressibleEulerEquations2D(GAMMA)
+    eqp = tlw.CompressibleNavierStokesDiffusion2D(eq, mu=0.01, Prandtl=PR,
+                                                 gradient_variables=tlw.GradientVariablesConservative())
+    solver = tlw.DGSEM(3, volume_integral=tlw.VolumeIntegralFR(tlw.LW()))
+    mesh = tlw.TreeMesh((-1.0, -1.0), (1.0, 1.0), 2, periodicity=False)
+    with pytest.raises(NotImplementedError):  # a non-periodic mesh without parabolic walls
+        tlw.SemidiscretizationHyperbolicParabolic(mesh, tlw.LW(), (eq, eqp), cavity_state, solver,
+                                                  boundary_conditions=(tlw.boundary_condition_slip_wall_vertical,
+                                                                       None))

@@ -1,0 +1,85 @@
+"""Pins of the oracle's no-slip adiabatic walls of the Navier-Stokes LW path
+(BoundaryConditionNavierStokesWall{NoSlip, Adiabatic} with conservative
+gradient variables, src/equations/compressible_navier_stokes_2d.jl:81-124,
+with the reflecting wall for the hyperbolic part, as in the reference's
+lid-driven cavity; no GPU).
+
+- Plane Couette flow (ρ = 1, p = 1, v1 = U y / H, x periodic, wall y = 0 at
+  rest, wall y = H moving with U): the gradient, the viscous stress and the
+  viscous work are polynomials the scheme represents exactly, so mass and
+  momentum do not change and the energy grows by the viscous heating
+  μ (U/H)² at every node.
+- The BR1 gradient at a wall takes the wall state u_outer = prim2cons(ρ, v_wall,
+  p) as its interface value: for a linear state that slips along the wall the
+  gradient there is the exact one plus (u_outer - u)(2/h)/w_n (the lifting of
+  the full difference; an average would lift half of it).
+- Mass is conserved with walls; a fluid at rest stays at rest.
+"""
+import numpy as np
+import pytest
+
+MU, PR, H = 0.01, 0.72, 1.0
+
+
+def walled(oracle_mod, N, nx, lid=0.3, periodic_x=True):
+    return oracle_mod.Oracle(N, nx, nx, 0.0, 0.0, H / nx, periodic=(periodic_x, False),
+                             bc_kind=oracle_mod.BV_SLIP_WALL, parabolic="navier_stokes", mu=MU, prandtl=PR,
+                             ns_walls=[(0.0, 0.0), (0.0, 0.0), (0.0, 0.0), (lid, 0.0)])
+
+
+def shear(x, y, a, b):
+    rho, p = np.ones_like(x), np.ones_like(x)
+    v1 = a + b * y
+    return np.stack([rho, rho * v1, 0 * x, p / 0.4 + 0.5 * rho * v1 ** 2], axis=-1)
+
+
+@pytest.mark.parametrize("N", [3, 4])
+def test_couette_flow_heats_at_the_viscous_rate(oracle_mod, N):
+    U = 0.3
+    o = walled(oracle_mod, N, 4, lid=U)
+    x, y = o.node_coords()
+    u = shear(x, y, 0.0, U / H)
+    du = o.rhs(u, 0.0, 0.5 * o.max_dt(u))
+    assert np.max(np.abs(du[..., 0])) < 1e-13
+    assert np.max(np.abs(du[..., 1:3])) < 1e-13
+    # absolute round-off of the O(J⁻¹ |F|) terms that cancel (J⁻¹ = 8 here)
+    assert np.max(np.abs(du[..., 3] - MU * U ** 2 / H ** 2)) < 1e-10
+
+
+@pytest.mark.parametrize("N", [3, 4])
+def test_wall_gradient_lifts_the_full_wall_state(oracle_mod, N):
+    nx, a, b, U = 4, 0.05, 0.2, 0.3
+    o = walled(oracle_mod, N, nx, lid=U)
+    x, y = o.node_coords()
+    u = shear(x, y, a, b)
+    o.rhs(u, 0.0, 1e-4)
+    gy = o.aux("gradients_y").reshape(u.shape)
+    _, w, _, _ = o.basis()
+    n, h = N + 1, H / nx
+    top = [e for e in range(nx * nx) if e // nx == nx - 1]
+    bottom = [e for e in range(nx * nx) if e // nx == 0]
+    lift = (2.0 / h) / w[-1]
+    # momentum component: d(ρ v1)/dy = b, plus the lifted jump to the wall value
+    np.testing.assert_allclose(gy[top, n - 1, :, 1], b + (U - (a + b * H)) * lift, rtol=0, atol=1e-11)
+    np.testing.assert_allclose(gy[bottom, 0, :, 1], b - (0.0 - a) * lift, rtol=0, atol=1e-11)
+    # away from the walls the gradient is the exact one
+    np.testing.assert_allclose(gy[top, 1:n - 1, :, 1], b, rtol=0, atol=1e-11)
+
+
+@pytest.mark.parametrize("N", [3, 4])
+def test_walls_conserve_mass_and_keep_rest(oracle_mod, N):
+    from trixilw_hip.equations import IsentropicVortex
+
+    o = walled(oracle_mod, N, 4, periodic_x=False)
+    x, y = o.node_coords()
+    rest = shear(x, y, 0.0, 0.0)
+    o_rest = walled(oracle_mod, N, 4, lid=0.0, periodic_x=False)
+    assert np.max(np.abs(o_rest.rhs(rest, 0.0, 1e-3))) < 1e-12
+    ic = IsentropicVortex(center=(0.5, 0.5), period=(1e9, 1e9), strength=1.0)
+    rng = np.random.default_rng(5 + N)
+    u = ic(x, y, 0.0) * (1 + 1e-2 * rng.standard_normal(x.shape + (4,)))
+    du = o.rhs(u, 0.0, 0.5 * o.max_dt(u))
+    _, w, _, _ = o.basis()
+    W = np.outer(w, w)[None]
+    total = np.einsum("eji,eji->", W, du[..., 0])
+    assert abs(total) <= 1e-13 * np.einsum("eji,eji->", W, np.abs(du[..., 0]))

@@ -62,6 +62,7 @@ struct tlw_ctx {
     double* d_ijn = nullptr;
     double* d_fnorm = nullptr;
     double* d_fxy = nullptr;  // boundary face-node coordinates (curvilinear mesh with boundaries)
+    double* d_fvb = nullptr;   // Navier-Stokes walls: viscous time average of boundary face nodes
     double* d_snrm = nullptr;  // subcell normals (curvilinear shock capturing) [E][n²][4]
     double* d_sqrk = nullptr;  // calc_fn_low! normals of subcell face n-1 [E][n][4]
     bool bad_launch = false;  // a launch was refused for a missing input (reported by check_launch)
@@ -133,7 +134,8 @@ void free_mesh(tlw_ctx* c) {
     void* ptrs[] = {c->d_nbr,       c->d_invj,  c->d_center, c->d_length, c->d_trace,   c->d_buf[0],    c->d_buf[1],
                     c->d_us,        c->d_auxU,  c->d_auxF,   c->d_partial, c->d_alpha_tmp, c->d_alpha,
                     c->d_vblk,      c->d_sblk,  c->d_etile,  c->d_rng,    c->d_state,   c->d_cur,
-                    c->d_Ja,        c->d_ijn,   c->d_fnorm,  c->d_fxy,     c->d_snrm,      c->d_sqrk};
+                    c->d_Ja,        c->d_ijn,   c->d_fnorm,  c->d_fxy,     c->d_snrm,      c->d_sqrk,
+                    c->d_fvb};
     for (void* p : ptrs) (void)hipFree(p);
     if (c->h_state) (void)hipHostFree(c->h_state);
     c->d_nbr = nullptr;
@@ -145,7 +147,7 @@ void free_mesh(tlw_ctx* c) {
     c->d_state = nullptr;
     c->h_state = nullptr;
     c->d_cur = nullptr;
-    c->d_Ja = c->d_ijn = c->d_fnorm = c->d_fxy = c->d_snrm = c->d_sqrk = nullptr;
+    c->d_Ja = c->d_ijn = c->d_fnorm = c->d_fxy = c->d_snrm = c->d_sqrk = c->d_fvb = nullptr;
 }
 
 // Workgroup tables for the current tile: volume workgroups of ebv elements and
@@ -301,8 +303,8 @@ int upload_meshes(tlw_ctx* c, const tlw_mesh_desc* ms, int ninst, const tlw_curv
             if (ms[k].neighbors_host[q] < 0) boundary = true;
     if (boundary && c->hconst.bc_kind == TLW_BV_NONE)
         return fail(c, TLW_EINVAL, "mesh has boundary faces but no boundary value function was given");
-    if (boundary && c->parabolic)
-        return fail(c, TLW_ENOTSUP, "the hyperbolic-parabolic path needs a periodic mesh");
+    if (boundary && c->parabolic && !c->hconst.ns_walls)
+        return fail(c, TLW_ENOTSUP, "the hyperbolic-parabolic path needs a periodic mesh or walls");
     std::vector<double> fnorm, cinvj, fxy;
     std::vector<double> snrm, sqrk;
     if (c->curved) {
@@ -375,6 +377,7 @@ int upload_meshes(tlw_ctx* c, const tlw_mesh_desc* ms, int ninst, const tlw_curv
     HIPCHK(c, hipMalloc(&c->d_sblk, sizeof(int4) * (size_t)scap));
     HIPCHK(c, hipMalloc(&c->d_etile, sizeof(int) * (size_t)E));
     HIPCHK(c, hipMalloc(&c->d_rng, sizeof(int4) * (size_t)ninst));
+    if (c->hconst.ns_walls) HIPCHK(c, hipMalloc(&c->d_fvb, sizeof(double) * (size_t)E * 4 * c->n * 2));
     if (c->sc) {
         HIPCHK(c, hipMalloc(&c->d_alpha_tmp, sizeof(double) * (size_t)E));
         HIPCHK(c, hipMalloc(&c->d_alpha, sizeof(double) * (size_t)E));
@@ -447,6 +450,7 @@ KArgs base_args(tlw_ctx* c) {
     a.fxy = c->d_fxy;
     a.slots = c->d_partial + c->slot_off;
     a.snrm = c->d_snrm;
+    a.fvb = c->d_fvb;
     a.sqrk = c->d_sqrk;
     return a;
 }
@@ -460,7 +464,10 @@ template <int N, int EB, bool SC, bool PAR = false> void launch_volume_eb(tlw_ct
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_MDRK1, SC, PAR>), g, b, 0, c->stream, a);
     else if (kind == VOL_MDRK2)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_MDRK2, SC, PAR>), g, b, 0, c->stream, a);
-    else if constexpr (N == 4 && !PAR)
+    else if (kind == VOL_LW_WALL) {
+        if constexpr (PAR && !SC)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_LW_WALL, SC, PAR>), g, b, 0, c->stream, a);
+    } else if constexpr (N == 4 && !PAR)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_LW_ADAPT, SC, PAR>), g, b, 0, c->stream, a);
 }
 template <int N, bool SC, bool PAR = false> void launch_volume_sc(tlw_ctx* c, const KArgs& a, int kind) {
@@ -511,7 +518,7 @@ template <int N> void launch_volume_t(tlw_ctx* c, const KArgs& a, int kind) {
             launch_volume_curved<N, false>(c, a, kind);
         }
     } else if (c->parabolic) {
-        launch_volume_sc<N, false, true>(c, a, kind);
+        launch_volume_sc<N, false, true>(c, a, (kind == VOL_LW && c->hconst.ns_walls) ? (int)VOL_LW_WALL : kind);
     } else if (c->sc) {
         // IndicatorHennemannGassner on uⁿ before the volume integral (dg_2d.jl:1665)
         hipLaunchKernelGGL(k_indicator<N>, dim3(c->sgrid), dim3(kSurfBlock), 0, c->stream, a, c->d_alpha_tmp);
@@ -1282,7 +1289,10 @@ int create_common(int ninst, const tlw_mesh_desc* meshes, const tlw_basis_desc* 
         // gradient_variables = GradientVariablesConservative()), constant μ
         const bool ok = par->equations == TLW_PAR_NAVIER_STOKES && par->gradient_variables == TLW_GRAD_CONSERVATIVE &&
                         std::isfinite(par->mu) && par->mu >= 0.0 && std::isfinite(par->prandtl) && par->prandtl > 0.0;
-        const bool supported = !c->sc && ninst == 1;
+        // no-slip adiabatic walls (LW, TreeMesh, with the reflecting wall of the hyperbolic part)
+        const bool walls_ok = !par->walls || (solver->time_discretization == TLW_LW && !cv &&
+                                              solver->boundary_kind == TLW_BV_SLIP_WALL);
+        const bool supported = !c->sc && ninst == 1 && walls_ok;
         if (!ok || !supported) {
             delete c;
             return ok ? TLW_ENOTSUP : TLW_EINVAL;
@@ -1291,6 +1301,8 @@ int create_common(int ninst, const tlw_mesh_desc* meshes, const tlw_basis_desc* 
         h.visc_mu = par->mu;
         h.visc_Pr = par->prandtl;
         h.visc_kappa = solver->gamma * (1.0 / (solver->gamma - 1.0)) / par->prandtl;
+        h.ns_walls = par->walls ? 1 : 0;
+        for (int k = 0; k < 8; ++k) h.wall_v[k] = par->walls ? par->wall_velocity[k] : 0.0;
     }
     if (cv) {
         // StructuredMesh: LW or MDRK; shock capturing with the first-order subcell

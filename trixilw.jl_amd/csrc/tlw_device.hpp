@@ -73,6 +73,10 @@ struct DevConst {
     // CompressibleNavierStokesDiffusion2D (conservative gradient variables):
     // dynamic viscosity, Prandtl number, κ = γ/((γ-1) Pr)
     double visc_mu, visc_Pr, visc_kappa;
+    // no-slip adiabatic walls of the Navier-Stokes path on every boundary face:
+    // wall velocity (v1, v2) per face 0..3 (x_neg, x_pos, y_neg, y_pos)
+    int ns_walls;
+    double wall_v[8];
     // MUSCL-Hancock subcell geometry per node i (compute_subcells): spacings
     // to the neighbouring nodes, the centred-difference weights, the distances
     // from the node to its subcell faces, and hw[k] = xs[k+2] - xs[k+1], the
@@ -106,7 +110,9 @@ enum BufSel { SEL_EXPLICIT = 0, SEL_CUR = 1, SEL_OTHER = 2 };
 enum Reduce { RED_NONE = 0, RED_STEP = 1, RED_INIT = 2, RED_QUERY = 3, RED_ADAPT = 4 };
 // VOL_LW_ADAPT: the LW kernel with the temporal-error estimate of adaptive steps
 // (a separate instantiation, so the plain LW kernel carries none of its registers)
-enum VolKind { VOL_LW = 0, VOL_MDRK1 = 1, VOL_MDRK2 = 2, VOL_LW_ADAPT = 3 };
+// VOL_LW_WALL: the Navier-Stokes LW kernel that also keeps the viscous time
+// average of boundary face nodes (no-slip walls)
+enum VolKind { VOL_LW = 0, VOL_MDRK1 = 1, VOL_MDRK2 = 2, VOL_LW_ADAPT = 3, VOL_LW_WALL = 4 };
 
 struct KArgs {
     int E;
@@ -149,6 +155,9 @@ struct KArgs {
     const double* fnorm;
     const double* fxy;  // boundary face-node coordinates [E][4][n][2] (curvilinear mesh with boundaries)
     double* slots;      // per instance: kSlots CFL maxima, then kSlots viscous maxima (k_finalize resets them)
+    // Navier-Stokes walls: per boundary face node the momentum components of the
+    // time-averaged viscous flux in the face's orientation [E][4][n][2]
+    double* fvb;
     // curvilinear shock capturing: per node the subcell normals of its lower ξ
     // and η subcell faces [E][n²][4], and the calc_fn_low! normals of subcell
     // face n-1 per row (ξ) and column (η) [E][n][4]
@@ -1169,26 +1178,48 @@ __device__ __forceinline__ void br1_gradient(const double* s, const double* src,
     using G_ = VGeo<N, EB>;
     constexpr int n = G_::n, NN = G_::NN, S = G_::S;
     contract_xy<n, S>(s, sbase, i, j, sDh, gx, gy);
+    // at a no-slip wall the interface value is the wall state prim2cons(ρ, v_wall, p)
+    // of the node (Trixi calc_boundary_flux_by_direction_gradient! with
+    // compressible_navier_stokes_2d.jl:113-124)
+    auto wall_state = [&](int f, double* uo) {
+        const double vw1 = c->wall_v[2 * f], vw2 = c->wall_v[2 * f + 1];
+        const double rho = uown[0];
+        const double p = (c->gamma - 1.0) * (uown[3] - 0.5 * (uown[1] * uown[1] + uown[2] * uown[2]) / rho);
+        uo[0] = rho;
+        uo[1] = rho * vw1;
+        uo[2] = rho * vw2;
+        uo[3] = p / (c->gamma - 1.0) + 0.5 * rho * (vw1 * vw1 + vw2 * vw2);
+    };
     if (i == 0 || i == n - 1) {
         const int f = (i == 0) ? 0 : 1;
         const int nb = nbr[4 * e + f];
+        const double wf = f ? c->bnd_hi : -c->bnd_lo;
         if (nb >= 0) {
             double un[4];
             load4(src + ((size_t)nb * NN + j * n + (n - 1 - i)) * 4, un);
-            const double wf = f ? c->bnd_hi : -c->bnd_lo;
 #pragma unroll
             for (int v = 0; v < 4; ++v) gx[v] = fma(0.5 * (uown[v] + un[v]), wf, gx[v]);
+        } else if (c->ns_walls) {
+            double uo[4];
+            wall_state(f, uo);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) gx[v] = fma(uo[v], wf, gx[v]);
         }
     }
     if (j == 0 || j == n - 1) {
         const int f = (j == 0) ? 2 : 3;
         const int nb = nbr[4 * e + f];
+        const double wf = (f & 1) ? c->bnd_hi : -c->bnd_lo;
         if (nb >= 0) {
             double un[4];
             load4(src + ((size_t)nb * NN + (n - 1 - j) * n + i) * 4, un);
-            const double wf = (f & 1) ? c->bnd_hi : -c->bnd_lo;
 #pragma unroll
             for (int v = 0; v < 4; ++v) gy[v] = fma(0.5 * (uown[v] + un[v]), wf, gy[v]);
+        } else if (c->ns_walls) {
+            double uo[4];
+            wall_state(f, uo);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) gy[v] = fma(uo[v], wf, gy[v]);
         }
     }
     if constexpr (CURV) {
@@ -1235,6 +1266,15 @@ struct ParStates {
     __device__ __forceinline__ void flux(int q, double gm1, const DevConst* c, double* f, double* g) const {
         ns_total_fluxes(u[q], x[q], y[q], gm1, c, f, g);
     }
+    // momentum components of the viscous fluxes of state q: (fv1, fv2, gv1, gv2)
+    __device__ __forceinline__ void visc12(int q, double gm1, const DevConst* c, double* o) const {
+        double fx[4], fy[4];
+        ns_visc_fluxes(u[q], x[q], y[q], gm1, c, fx, fy);
+        o[0] = fx[1];
+        o[1] = fx[2];
+        o[2] = fy[1];
+        o[3] = fy[2];
+    }
 };
 
 // Time derivative of the next level and its element-local gradient:
@@ -1277,15 +1317,27 @@ __device__ __forceinline__ void par_level(bool act, int sidx, int sbase, int i, 
 // flux, with the states of ∇u advanced by the same Taylor coefficients.
 // Buffer use: s0 keeps f(uⁿ) for the 5- and 3-point stencils, s1 takes each
 // flux derivative, s2 each time derivative; the final F̃ exchange is in s2.
-template <int N, int EB, bool CURV = false>
+template <int N, int EB, bool CURV = false, bool WALL = false>
 __device__ __forceinline__ void lwp_element(const double* uN, const double* gxN, const double* gyN, double invJ,
                                             double gm1, const DevConst* c, int i, int j, bool act, int sidx,
                                             int sbase, double* s0, double* s1, double* s2, const double* sDm,
                                             const double* sD, const double* sDh, double* Uc, double* F, double* G,
-                                            double* vol, const double* Jn = nullptr) {
+                                            double* vol, const double* Jn = nullptr, double* vw = nullptr) {
     using G_ = VGeo<N, EB>;
     constexpr int n = G_::n, S = G_::S;
     ns_total_fluxes(uN, gxN, gyN, gm1, c, F, G);
+    // with walls: the time average of (fv1, fv2, gv1, gv2) alongside F̃ (v0: at uⁿ)
+    double v0[4];
+    if constexpr (WALL) {
+        double fx[4], fy[4];
+        ns_visc_fluxes(uN, gxN, gyN, gm1, c, fx, fy);
+        v0[0] = fx[1];
+        v0[1] = fx[2];
+        v0[2] = fy[1];
+        v0[3] = fy[2];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) vw[k] = v0[k];
+    }
     double w[4], wx[4], wy[4];
     par_level<N, EB, CURV>(act, sidx, sbase, i, j, invJ, s0, s2, sDm, sD, F, G, w, wx, wy, Jn);  // ut
     // on a curvilinear mesh s0 took the contravariant pair; the centred stencils
@@ -1317,6 +1369,15 @@ __device__ __forceinline__ void lwp_element(const double* uN, const double* gxN,
         dy[v] = (1.0 / 12.0) * (dy[v] + ga[v]);
         F[v] = fma(0.5, dx[v], F[v]);
         G[v] = fma(0.5, dy[v], G[v]);
+    }
+    if constexpr (WALL) {  // f_t of the viscous part, weight 1/2
+        double a0[4], a1[4], a2[4], a3[4];
+        P.visc12(0, gm1, c, a0);
+        P.visc12(1, gm1, c, a1);
+        P.visc12(2, gm1, c, a2);
+        P.visc12(3, gm1, c, a3);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) vw[k] = fma(0.5, (1.0 / 12.0) * (-a2[k] + 8.0 * a0[k] - 8.0 * a1[k] + a3[k]), vw[k]);
     }
     par_level<N, EB, CURV>(act, sidx, sbase, i, j, invJ, s1, s2, sDm, sD, dx, dy, w, wx, wy, Jn);  // utt
     P.add(0.5, 0.5, 2.0, 2.0, w, wx, wy);
@@ -1361,6 +1422,23 @@ __device__ __forceinline__ void lwp_element(const double* uN, const double* gxN,
         F[v] = fma(1.0 / 6.0, dx[v], F[v]);
         G[v] = fma(1.0 / 6.0, dy[v], G[v]);
     }
+    if constexpr (WALL) {  // f_tt of the viscous part, weight 1/6
+        double a0[4], a1[4];
+        P.visc12(0, gm1, c, a0);
+        P.visc12(1, gm1, c, a1);
+        if constexpr (N == 4) {
+            double a2[4], a3[4];
+            P.visc12(2, gm1, c, a2);
+            P.visc12(3, gm1, c, a3);
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                vw[k] = fma(1.0 / 6.0,
+                            (1.0 / 12.0) * (-a2[k] + 16.0 * a0[k] - 30.0 * v0[k] + 16.0 * a1[k] - a3[k]), vw[k]);
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) vw[k] = fma(1.0 / 6.0, a0[k] - 2.0 * v0[k] + a1[k], vw[k]);
+        }
+    }
     par_level<N, EB, CURV>(act, sidx, sbase, i, j, invJ, s1, s2, sDm, sD, dx, dy, w, wx, wy, Jn);  // uttt
     P.add(1.0 / 6.0, -1.0 / 6.0, 4.0 / 3.0, -4.0 / 3.0, w, wx, wy);
 #pragma unroll
@@ -1382,6 +1460,15 @@ __device__ __forceinline__ void lwp_element(const double* uN, const double* gxN,
         dy[v] = 0.5 * (dy[v] - ga[v]);
         F[v] = fma(1.0 / 24.0, dx[v], F[v]);
         G[v] = fma(1.0 / 24.0, dy[v], G[v]);
+    }
+    if constexpr (WALL) {  // f_ttt of the viscous part, weight 1/24
+        double a0[4], a1[4], a2[4], a3[4];
+        P.visc12(0, gm1, c, a0);
+        P.visc12(1, gm1, c, a1);
+        P.visc12(2, gm1, c, a2);
+        P.visc12(3, gm1, c, a3);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) vw[k] = fma(1.0 / 24.0, 0.5 * (a2[k] - 2.0 * a0[k] + 2.0 * a1[k] - a3[k]), vw[k]);
     }
     if constexpr (N == 4) {
         par_level<N, EB, CURV>(act, sidx, sbase, i, j, invJ, s1, s2, sDm, sD, dx, dy, w, wx, wy, Jn);  // utttt
@@ -1412,6 +1499,16 @@ __device__ __forceinline__ void lwp_element(const double* uN, const double* gxN,
             F[v] = fma(1.0 / 120.0, dx[v], F[v]);
             G[v] = fma(1.0 / 120.0, dy[v], G[v]);
         }
+    }
+    if constexpr (WALL && N == 4) {  // f_tttt of the viscous part (with the 0.5), weight 1/120
+        double a0[4], a1[4], a2[4], a3[4];
+        P.visc12(0, gm1, c, a0);
+        P.visc12(1, gm1, c, a1);
+        P.visc12(2, gm1, c, a2);
+        P.visc12(3, gm1, c, a3);
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            vw[k] = fma(1.0 / 120.0, 0.5 * (a2[k] - 4.0 * a0[k] + 6.0 * v0[k] - 4.0 * a1[k] + a3[k]), vw[k]);
     }
     __syncthreads();  // every thread has finished reading s2 before it is reused
     if constexpr (CURV) {
@@ -1866,6 +1963,7 @@ template <int N, int EB, int KIND, bool SC, bool PAR = false, bool CURV = false>
 __global__ __launch_bounds__((VGeo<N, EB>::NT), kVolWaves<SC || PAR>) void k_volume(KArgs a) {
     static_assert(!(SC && PAR), "shock capturing has no hyperbolic-parabolic form");
     static_assert(!CURV || KIND != VOL_LW_ADAPT, "the curvilinear path is LW or MDRK");
+    static_assert(KIND != VOL_LW_WALL || (PAR && !CURV && !SC), "walls: the TreeMesh Navier-Stokes LW kernel");
     static_assert(!(CURV && PAR && SC), "no shock capturing with parabolic terms");
     using G_ = VGeo<N, EB>;
     constexpr int n = G_::n, NN = G_::NN, S = G_::S, NT = G_::NT;
@@ -1906,6 +2004,7 @@ __global__ __launch_bounds__((VGeo<N, EB>::NT), kVolWaves<SC || PAR>) void k_vol
     double terr = 0.0;  // temporal error of the node (adaptive steps, LW N=4)
     load4(uin + off, uN);
     double Jn[4] = {0.0, 0.0, 0.0, 0.0};  // Ja11, Ja12, Ja21, Ja22 of the node: same layout as the state
+    double vwall[4] = {0.0, 0.0, 0.0, 0.0};  // walls: time-averaged (fv1, fv2, gv1, gv2)
     if constexpr (CURV) load4(a.Ja + off, Jn);
     if constexpr (CURV && !PAR) {
         if constexpr (KIND == VOL_LW) {
@@ -1936,9 +2035,9 @@ __global__ __launch_bounds__((VGeo<N, EB>::NT), kVolWaves<SC || PAR>) void k_vol
         } else {
             __syncthreads();
             br1_gradient<N, EB, CURV>(sg, uin, a.nbr, c, e, i, j, sbase, invJ, sDh, uN, gxN, gyN, Jn);
-            if constexpr (KIND == VOL_LW)
-                lwp_element<N, EB, CURV>(uN, gxN, gyN, invJ, gm1, c, i, j, act, sidx, sbase, s0, s1, s2, sDm, sD, sDh,
-                                         Uc, F, G, vol, Jn);
+            if constexpr (KIND == VOL_LW || KIND == VOL_LW_WALL)
+                lwp_element<N, EB, CURV, KIND == VOL_LW_WALL>(uN, gxN, gyN, invJ, gm1, c, i, j, act, sidx, sbase, s0,
+                                                             s1, s2, sDm, sD, sDh, Uc, F, G, vol, Jn, vwall);
             else
                 mdrkp1_element<N, EB, CURV>(uN, gxN, gyN, invJ, gm1, c, i, j, act, sidx, sbase, s0, s1, s2, sDm, sD,
                                             sDh, Uc, F, G, vol, Jn);
@@ -2053,6 +2152,19 @@ __global__ __launch_bounds__((VGeo<N, EB>::NT), kVolWaves<SC || PAR>) void k_vol
     const double* lamst = tree ? Uc : uN;
     const int nbx = fx >= 0 ? a.nbr[4 * e + fx] : -1;
     const int nby = fy >= 0 ? a.nbr[4 * e + fy] : -1;
+    if constexpr (KIND == VOL_LW_WALL) {
+        // the viscous time average of wall face nodes, for the surface kernel's wall flux
+        if (act && fx >= 0 && nbx < 0) {
+            double* q = a.fvb + (((size_t)e * 4 + fx) * n + j) * 2;
+            q[0] = vwall[0];
+            q[1] = vwall[1];
+        }
+        if (act && fy >= 0 && nby < 0) {
+            double* q = a.fvb + (((size_t)e * 4 + fy) * n + i) * 2;
+            q[0] = vwall[2];
+            q[1] = vwall[3];
+        }
+    }
     if constexpr (CURV) {
         // structured order with the interface's normal n (shared by both
         // elements): F̃·n, |v(uⁿ)·n| and c(uⁿ)|n| (dg_2d.jl:2229-2264)
@@ -2377,6 +2489,16 @@ __global__ __launch_bounds__(kSurfBlock) void k_surface(KArgs a) {
                     // reflecting wall (equations.jl:95-139): the mirrored record, whatever
                     // the flux order (both states are mirrored, the speeds kept)
                     const int m = (f < 2) ? 1 : 2;
+                    double fvw[2] = {0.0, 0.0};
+                    if (c->ns_walls) {
+                        // Navier-Stokes: the record holds F̃a - F̃v; the wall flux is built
+                        // from F̃a (its normal-momentum component: the others cancel against
+                        // their mirror images), then the wall's viscous flux is subtracted
+                        const double* q = a.fvb + (((size_t)e * 4 + f) * n + l) * 2;
+                        fvw[0] = q[0];
+                        fvw[1] = q[1];
+                        own[4 + m] += fvw[m - 1];
+                    }
 #pragma unroll
                     for (int v = 0; v < 4; ++v) {
                         oth[v] = (v == m) ? -own[v] : own[v];
@@ -2385,6 +2507,13 @@ __global__ __launch_bounds__(kSurfBlock) void k_surface(KArgs a) {
                     oth[8] = own[8];
                     oth[9] = own[9];
                     face_numflux(Lr, Rr, Fs);
+                    if (c->ns_walls) {
+                        // NoSlip/Adiabatic (compressible_navier_stokes_2d.jl:81-91):
+                        // (F̃v1, F̃v2, F̃v3, v_wall · (F̃v2, F̃v3)), F̃v1 = 0
+                        Fs[1] -= fvw[0];
+                        Fs[2] -= fvw[1];
+                        Fs[3] -= fma(c->wall_v[2 * f], fvw[0], c->wall_v[2 * f + 1] * fvw[1]);
+                    }
                 } else {
                     boundary_record<N>(a, e, f, l, t, dt, oth);
                     face_numflux(Lr, Rr, Fs);

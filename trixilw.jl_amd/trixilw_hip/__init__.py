@@ -8,7 +8,8 @@ library; the first call that needs the GPU does, and fails if it is missing.
 from .adjoint import adjoint_step, gradient_lw, kinetic_energy
 from .basis import LobattoLegendreBasis
 from .batch import SemidiscretizationBatch
-from .equations import (CompressibleEulerEquations2D, CompressibleNavierStokesDiffusion2D, ConstantState,
+from .equations import (Adiabatic, BoundaryConditionNavierStokesWall, CompressibleEulerEquations2D,
+                        CompressibleNavierStokesDiffusion2D, ConstantState, NoSlip,
                         FourQuadrantRiemann, GradientVariablesConservative, GradientVariablesPrimitive,
                         IsentropicVortex, LinearScalarAdvectionEquation2D, boundary_condition_slip_wall_horizontal,
                         boundary_condition_slip_wall_vertical)
@@ -25,7 +26,8 @@ from .semidiscretization import (DGSEM, LW, MDRK, Adaptive, CFLBased, FirstOrder
 
 __all__ = [
     "LobattoLegendreBasis", "CompressibleEulerEquations2D", "ConstantState", "FourQuadrantRiemann",
-    "boundary_condition_slip_wall_vertical", "boundary_condition_slip_wall_horizontal",
+    "boundary_condition_slip_wall_vertical", "boundary_condition_slip_wall_horizontal", "NoSlip", "Adiabatic",
+    "BoundaryConditionNavierStokesWall",
     "IsentropicVortex", "LinearScalarAdvectionEquation2D", "LWIntegrator", "LWSolution", "compute_dt", "isapprox",
     "isfinished", "perform_step", "set_dt", "set_t_and_iter", "solve_lwfr", "TreeMesh",
     "StructuredMesh", "coordinates2mapping", "DGSEM", "LW", "MDRK",

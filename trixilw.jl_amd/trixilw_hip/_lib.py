@@ -87,6 +87,8 @@ class ParabolicDesc(C.Structure):
         ("mu", C.c_double),
         ("prandtl", C.c_double),
         ("gradient_variables", C.c_int32),
+        ("walls", C.c_int32),
+        ("wall_velocity", C.c_double * 8),
     ]
 
 

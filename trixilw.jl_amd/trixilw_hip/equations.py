@@ -39,6 +39,45 @@ boundary_condition_slip_wall_vertical = _SlipWall(1)
 boundary_condition_slip_wall_horizontal = _SlipWall(2)
 
 
+class NoSlip:
+    """Trixi NoSlip(boundary_value_function): the wall velocity (x, t, equations)
+    -> (v1, v2).  The device takes a constant wall velocity: a callable is
+    evaluated once and must not depend on x or t; a pair is taken as is."""
+
+    def __init__(self, velocity):
+        if callable(velocity):
+            a = velocity((0.0, 0.0), 0.0, None)
+            b = velocity((0.37, -0.61), 1.3, None)
+            if tuple(map(float, a)) != tuple(map(float, b)):
+                raise NotImplementedError("the device path takes a constant wall velocity")
+            velocity = a
+        self.velocity = (float(velocity[0]), float(velocity[1]))
+
+
+class Adiabatic:
+    """Trixi Adiabatic(boundary_value_normal_flux_function): the normal heat flux;
+    the reference's wall condition for conservative gradient variables takes it as
+    given, and the device path supports the insulated wall (zero flux)."""
+
+    def __init__(self, normal_flux=0.0):
+        if callable(normal_flux):
+            normal_flux = normal_flux((0.0, 0.0), 0.0, None)
+        if float(normal_flux) != 0.0:
+            raise NotImplementedError("the device path implements the insulated wall (zero normal heat flux)")
+
+
+class BoundaryConditionNavierStokesWall:
+    """Trixi BoundaryConditionNavierStokesWall(velocity, heat) with NoSlip and
+    Adiabatic (the reference's lid-driven cavity, conservative gradient
+    variables: src/equations/compressible_navier_stokes_2d.jl:81-124)."""
+
+    def __init__(self, boundary_condition_velocity, boundary_condition_heat_flux):
+        if not isinstance(boundary_condition_velocity, NoSlip) or not isinstance(boundary_condition_heat_flux,
+                                                                                 Adiabatic):
+            raise NotImplementedError("the device path implements NoSlip velocity with Adiabatic heat walls")
+        self.velocity = boundary_condition_velocity.velocity
+
+
 def boundary_conditions_kind(bcs):
     """One boundary-value description for every boundary face from the
     reference's forms: a single function object, or the NamedTuple (x_neg,

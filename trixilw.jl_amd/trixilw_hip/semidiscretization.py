@@ -404,14 +404,25 @@ class SemidiscretizationHyperbolicParabolic(SemidiscretizationHyperbolic):
         eq, eqp = equations
         if not isinstance(eqp, CompressibleNavierStokesDiffusion2D):
             raise NotImplementedError("the device path implements CompressibleNavierStokesDiffusion2D")
+        from .equations import BoundaryConditionNavierStokesWall
+
         bcs = boundary_conditions if boundary_conditions is not None else (None, None)
-        if any(b is not None for b in bcs):
-            raise NotImplementedError("the hyperbolic-parabolic device path needs a periodic mesh")
+        bc_hyp, bc_par = bcs
         if isinstance(solver.volume_integral, VolumeIntegralFRShockCapturing):
             raise NotImplementedError("the reference's parabolic kernels take VolumeIntegralFR only")
         self.equations_parabolic = eqp
         self._par = _lib.ParabolicDesc(_lib.TLW_PAR_NAVIER_STOKES, eqp.mu, eqp.Pr, _lib.TLW_GRAD_CONSERVATIVE)
-        super().__init__(mesh, time_discretization, eq, initial_condition, solver, boundary_conditions=None,
+        if bc_par is not None:
+            # boundary_conditions_parabolic: the NamedTuple (x_neg, x_pos, y_neg, y_pos) of walls
+            seq = [bc_par[k] for k in ("x_neg", "x_pos", "y_neg", "y_pos")] if isinstance(bc_par, dict) else bc_par
+            if len(seq) != 4 or not all(isinstance(b, BoundaryConditionNavierStokesWall) for b in seq):
+                raise NotImplementedError("parabolic boundary conditions: four BoundaryConditionNavierStokesWall")
+            self._par.walls = 1
+            for k, b in enumerate(seq):
+                self._par.wall_velocity[2 * k], self._par.wall_velocity[2 * k + 1] = b.velocity
+        elif bc_hyp is not None:
+            raise NotImplementedError("the hyperbolic-parabolic device path needs a periodic mesh or walls")
+        super().__init__(mesh, time_discretization, eq, initial_condition, solver, boundary_conditions=bc_hyp,
                          source_terms=source_terms, keep_aux=keep_aux, device=device)
 
     def _create(self, md, basis, sd, h):
