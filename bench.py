@@ -423,6 +423,30 @@ def measure_curved_navier_stokes(steps, cpu_steps, threads, with_cpu=True):
                      f"polydeg 3, MDRK, CFL {cfl}"}
     log(f"bench: curvilinear Navier-Stokes MDRK {r['value'] / 1e9:.3f} G DOF-updates/s, {r['ms_per_step']:.3f} "
         f"ms/step, volume {r['vol_ms']:.3f} ms x{r['vol_n']}, surface {r['surf_ms']:.3f} ms")
+    # the same steps replayed from a captured HIP graph (tlw_enable_graphs): at this size the
+    # launch gaps are a visible share of the step
+    import torch
+    from trixilw_hip import _lib
+
+    L, h, dp = _lib.lib(), semi.handle, C.POINTER(C.c_double)
+    t, dt, it, fin = C.c_double(), C.c_double(), C.c_int64(), C.c_int32()
+    with torch.cuda.stream(torch.cuda.Stream()):
+        semi._bind_stream()
+        _lib.check(L.tlw_enable_graphs(h, 1), h)
+        for timed in (False, True):  # the first pass captures the graph
+            _lib.check(L.tlw_set_state(h, u0.ctypes.data_as(dp), u0.size), h)
+            _lib.check(L.tlw_solve_init(h, 0.0, float("inf"), cfl), h)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            _lib.check(L.tlw_advance(h, steps, C.byref(t), C.byref(dt), C.byref(it), C.byref(fin)), h)
+            torch.cuda.synchronize()
+            el = time.perf_counter() - t0
+        assert int(it.value) == steps, (int(it.value), steps)
+        _lib.check(L.tlw_enable_graphs(h, 0), h)
+    out["graph_replay"] = {"value": dofs * steps / el, "ms_per_step": round(1e3 * el / steps, 4),
+                           "note": "the same steps from the same initial state, replayed from one captured graph"}
+    log(f"bench: curvilinear Navier-Stokes MDRK, graph replay {dofs * steps / el / 1e9:.3f} G DOF-updates/s, "
+        f"{1e3 * el / steps:.3f} ms/step")
     semi.close()
     if with_cpu:
         import oracle_ctypes
