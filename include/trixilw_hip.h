@@ -129,6 +129,10 @@ typedef struct {
      * otherwise */
     int32_t walls;
     double wall_velocity[8];
+    /* per direction: 0 Adiabatic(0), 1 Isothermal(wall_temperature): the reference's isothermal wall
+     * (compressible_navier_stokes_2d.jl:93-111) returns TLW_ENOTSUP on the device (set 0) */
+    int32_t wall_isothermal[4];
+    double wall_temperature[4];
 } tlw_parabolic_desc;
 /* Curvilinear StructuredMesh metrics (Trixi cache.elements of a StructuredMesh,
  * src/solvers/dgsem_structured/containers_2d.jl; TrixiLW's kernels for it in

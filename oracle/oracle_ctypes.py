@@ -54,6 +54,8 @@ class _Desc(C.Structure):
         ("reconstruction", C.c_int),
         ("ns_walls", C.c_int),
         ("wall_v", C.c_double * 8),
+        ("wall_iso", C.c_int * 4),
+        ("wall_T", C.c_double * 4),
     ]
 
 
@@ -130,7 +132,7 @@ class Oracle:
                  adv=(1.0, 1.0), flux_order=0, bc_kind=BV_NONE, bc_par=(), tolerances=(0.0, 0.0), nthreads=0,
                  shock_capturing=False, alpha_max=0.5, alpha_min=0.001, alpha_smooth=True, positivity=None,
                  parabolic=None, mu=0.0, prandtl=0.72, mapping=None, reconstruction="first_order",
-                 ns_walls=None):
+                 ns_walls=None, wall_temperature=None):
         """``positivity``: (threshold_density, threshold_pressure) of a Zhang-Shu
         limiter that ``solve`` applies as a stage limiter, or None.
         ``parabolic``: None, "laplace" (LaplaceDiffusion2D with diffusivity ``mu``,
@@ -170,6 +172,9 @@ class Oracle:
             d.ns_walls = 1
             for k, (a, b) in enumerate(ns_walls):
                 d.wall_v[2 * k], d.wall_v[2 * k + 1] = float(a), float(b)
+            for k, T in enumerate(wall_temperature or (None,) * 4):  # None: adiabatic
+                if T is not None:
+                    d.wall_iso[k], d.wall_T[k] = 1, float(T)
         d.visc_mu, d.prandtl = mu, prandtl
         if mapping is not None:
             d.curved = 1

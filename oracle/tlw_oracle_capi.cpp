@@ -51,6 +51,8 @@ struct tlwo_desc {
     // direction x_neg, x_pos, y_neg, y_pos
     int ns_walls;
     double wall_v[8];
+    int wall_iso[4];     // 1: Isothermal with wall_T, 0: Adiabatic
+    double wall_T[4];
 };
 
 struct tlwo_ctx {
@@ -130,6 +132,8 @@ void* tlwo_create(const tlwo_desc* d) {
         for (int k = 0; k < 4; ++k) {
             p->wall_v[k][0] = d->wall_v[2 * k];
             p->wall_v[k][1] = d->wall_v[2 * k + 1];
+            p->wall_iso[k] = d->wall_iso[k];
+            p->wall_T[k] = d->wall_T[k];
         }
         c->euler.reset(p);
     } else if (d->parabolic == 1) {

@@ -132,6 +132,10 @@ template <class T, class Eq, class Visc> struct ParabolicSolver : Solver<T, Eq> 
     // for the hyperbolic part): wall velocity per direction x_neg, x_pos, y_neg, y_pos
     int ns_walls = 0;
     double wall_v[4][2] = {};
+    // heat condition per direction: 0 Adiabatic (zero normal heat flux), 1 Isothermal
+    // with the wall temperature wall_T (T = p/ρ, the reference's R = 1)
+    int wall_iso[4] = {0, 0, 0, 0};
+    double wall_T[4] = {0, 0, 0, 0};
 
     void setup_parabolic() {
         const size_t nd = this->ndofs_total();
@@ -199,7 +203,7 @@ template <class T, class Eq, class Visc> struct ParabolicSolver : Solver<T, Eq> 
                     if (o == 1) { ii = (side == 1) ? n - 1 : 0; jj = l; }
                     else { ii = l; jj = (side == 1) ? n - 1 : 0; }
                     T uo[NV];
-                    wall_state(&u[idx(0, ii, jj, e)], wall_v[dir - 1], uo);
+                    wall_state(&u[idx(0, ii, jj, e)], wall_v[dir - 1], uo, wall_iso[dir - 1], wall_T[dir - 1]);
                     for (int v = 0; v < NV; ++v) gsfv[sidx(v, l, dir - 1, e)] = uo[v];
                 }
             }
@@ -706,12 +710,14 @@ template <class T, class Eq, class Visc> struct ParabolicSolver : Solver<T, Eq> 
         }
     }
 
-    // prim2cons(ρ, v_wall, p) of an inner state (Euler; the wall condition's u_outer)
-    void wall_state(const T* ui, const double* vw, T* uo) const {
+    // the wall condition's u_outer of an inner state (Euler): prim2cons(ρ, v_wall, p) with
+    // the inner pressure (Adiabatic, compressible_navier_stokes_2d.jl:113-124) or
+    // p = ρ T_wall (Isothermal, 93-101)
+    void wall_state(const T* ui, const double* vw, T* uo, int iso = 0, double Tw = 0.0) const {
         if constexpr (NV == 4) {
             const double g = this->eq.gamma;
             const T rho = ui[0];
-            const T p = (g - 1.0) * (ui[3] - 0.5 * (ui[1] * ui[1] + ui[2] * ui[2]) / rho);
+            const T p = iso ? rho * Tw : (g - 1.0) * (ui[3] - 0.5 * (ui[1] * ui[1] + ui[2] * ui[2]) / rho);
             uo[0] = rho;
             uo[1] = rho * vw[0];
             uo[2] = rho * vw[1];
@@ -738,7 +744,9 @@ template <class T, class Eq, class Visc> struct ParabolicSolver : Solver<T, Eq> 
                 else { ii = l; jj = (side == 1) ? n - 1 : 0; }
                 T fv[NV];
                 for (int v = 0; v < NV; ++v) fv[v] = Fv[fidx(v, o - 1, ii, jj, e)];
-                if constexpr (NV == 4) fv[3] = vw[0] * fv[1] + vw[1] * fv[2] + 0.0;
+                // Adiabatic: the energy flux v_wall · (τ1n, τ2n) + 0; Isothermal: flux_inner
+                if constexpr (NV == 4)
+                    if (!wall_iso[dir - 1]) fv[3] = vw[0] * fv[1] + vw[1] * fv[2] + 0.0;
                 for (int v = 0; v < NV; ++v) this->sfv[sidx(v, l, dir - 1, e)] = this->sfv[sidx(v, l, dir - 1, e)] - fv[v];
             }
         }

@@ -139,6 +139,19 @@ def test_ns_walls_reject_unsupported(oracle_mod):
     with pytest.raises(_lib.TLWError) as ei:  # MDRK with walls
         setup_walls(oracle_mod, 3, 2, td="mdrk")
     assert ei.value.code == _lib.TLW_ENOTSUP
+    # isothermal walls are restated in the oracle only
+    eq0 = tlw.CompressibleEulerEquations2D(GAMMA)
+    eqp0 = tlw.CompressibleNavierStokesDiffusion2D(eq0, mu=0.01, Prandtl=PR,
+                                                  gradient_variables=tlw.GradientVariablesConservative())
+    hot = tlw.BoundaryConditionNavierStokesWall(tlw.NoSlip((0.0, 0.0)), tlw.Isothermal(1.2))
+    bc_hyp = (tlw.boundary_condition_slip_wall_vertical, tlw.boundary_condition_slip_wall_vertical,
+              tlw.boundary_condition_slip_wall_horizontal, tlw.boundary_condition_slip_wall_horizontal)
+    with pytest.raises(_lib.TLWError) as ei:
+        tlw.SemidiscretizationHyperbolicParabolic(tlw.TreeMesh((-1.0, -1.0), (1.0, 1.0), 2, periodicity=False),
+                                                  tlw.LW(), (eq0, eqp0), cavity_state,
+                                                  tlw.DGSEM(3, volume_integral=tlw.VolumeIntegralFR(tlw.LW())),
+                                                  boundary_conditions=(bc_hyp, (hot,) * 4))
+    assert ei.value.code == _lib.TLW_ENOTSUP
     eq = tlw.CompressibleEulerEquations2D(GAMMA)
     eqp = tlw.CompressibleNavierStokesDiffusion2D(eq, mu=0.01, Prandtl=PR,
                                                  gradient_variables=tlw.GradientVariablesConservative())

@@ -83,3 +83,53 @@ def test_walls_conserve_mass_and_keep_rest(oracle_mod, N):
     W = np.outer(w, w)[None]
     total = np.einsum("eji,eji->", W, du[..., 0])
     assert abs(total) <= 1e-13 * np.einsum("eji,eji->", W, np.abs(du[..., 0]))
+
+
+@pytest.mark.parametrize("N", [3, 4])
+def test_isothermal_wall_gradient_lifts_the_wall_temperature_state(oracle_mod, N):
+    """An isothermal wall's state is prim2cons(ρ, v_wall, ρ T_wall): for a
+    linear shear at uniform T = 1 the energy gradient at the lid is the exact
+    one plus the lifted jump to the wall state's energy."""
+    nx, a, b, U, Tw = 4, 0.05, 0.2, 0.3, 1.5
+    o = oracle_mod.Oracle(N, nx, nx, 0.0, 0.0, H / nx, periodic=(True, False), bc_kind=oracle_mod.BV_SLIP_WALL,
+                          parabolic="navier_stokes", mu=MU, prandtl=PR,
+                          ns_walls=[(0.0, 0.0), (0.0, 0.0), (0.0, 0.0), (U, 0.0)],
+                          wall_temperature=[None, None, None, Tw])
+    x, y = o.node_coords()
+    u = shear(x, y, a, b)
+    o.rhs(u, 0.0, 1e-4)
+    gy = o.aux("gradients_y").reshape(u.shape)
+    _, w, _, _ = o.basis()
+    n, h = N + 1, H / nx
+    top = [e for e in range(nx * nx) if e // nx == nx - 1]
+    v_top = a + b * H
+    E_in = 1.0 / 0.4 + 0.5 * v_top ** 2
+    E_out = Tw / 0.4 + 0.5 * U ** 2
+    dEdy = b * v_top  # d/dy of 0.5 (a + b y)^2 at y = H
+    np.testing.assert_allclose(gy[top, n - 1, :, 3], dEdy + (E_out - E_in) * (2.0 / h) / w[-1], rtol=0, atol=1e-10)
+
+
+@pytest.mark.parametrize("N", [3, 4])
+def test_isothermal_walls_pass_the_heat_flux(oracle_mod, N):
+    """Fluid at rest with a linear temperature profile between walls held at
+    its own end temperatures: the conduction flux is uniform, so with
+    isothermal walls (which pass the inner viscous flux) the energy does not
+    change, while adiabatic walls block the flux and the wall elements heat or
+    cool (a tiny dt keeps the time-derivative terms of the hydrostatic
+    imbalance out of the comparison)."""
+    nx = 4
+    x0 = 0.0
+    common = dict(periodic=(True, False), bc_kind=oracle_mod.BV_SLIP_WALL, parabolic="navier_stokes", mu=MU,
+                  prandtl=PR, ns_walls=[(0.0, 0.0)] * 4)
+    iso = oracle_mod.Oracle(N, nx, nx, x0, 0.0, H / nx, wall_temperature=[None, None, 1.0, 1.2], **common)
+    adi = oracle_mod.Oracle(N, nx, nx, x0, 0.0, H / nx, **common)
+    x, y = iso.node_coords()
+    T = 1.0 + 0.2 * y / H
+    rho = np.ones_like(x)
+    u = np.stack([rho, 0 * x, 0 * x, rho * T / 0.4], axis=-1)
+    dt = 1e-9
+    du_iso = iso.rhs(u, 0.0, dt)
+    du_adi = adi.rhs(u, 0.0, dt)
+    # (2e-8 seen at the wall elements: the dt-sized terms of the hydrostatic imbalance)
+    assert np.max(np.abs(du_iso[..., 3])) < 1e-6
+    assert np.max(np.abs(du_adi[..., 3])) > 1e-3

@@ -1290,8 +1290,11 @@ int create_common(int ninst, const tlw_mesh_desc* meshes, const tlw_basis_desc* 
         const bool ok = par->equations == TLW_PAR_NAVIER_STOKES && par->gradient_variables == TLW_GRAD_CONSERVATIVE &&
                         std::isfinite(par->mu) && par->mu >= 0.0 && std::isfinite(par->prandtl) && par->prandtl > 0.0;
         // no-slip adiabatic walls (LW, TreeMesh, with the reflecting wall of the hyperbolic part)
+        // (isothermal walls are not implemented here: TLW_ENOTSUP)
+        bool iso = false;
+        for (int k = 0; k < 4; ++k) iso = iso || (par->walls && par->wall_isothermal[k]);
         const bool walls_ok = !par->walls || (solver->time_discretization == TLW_LW && !cv &&
-                                              solver->boundary_kind == TLW_BV_SLIP_WALL);
+                                              solver->boundary_kind == TLW_BV_SLIP_WALL && !iso);
         const bool supported = !c->sc && ninst == 1 && walls_ok;
         if (!ok || !supported) {
             delete c;
@@ -1303,6 +1306,7 @@ int create_common(int ninst, const tlw_mesh_desc* meshes, const tlw_basis_desc* 
         h.visc_kappa = solver->gamma * (1.0 / (solver->gamma - 1.0)) / par->prandtl;
         h.ns_walls = par->walls ? 1 : 0;
         for (int k = 0; k < 8; ++k) h.wall_v[k] = par->walls ? par->wall_velocity[k] : 0.0;
+
     }
     if (cv) {
         // StructuredMesh: LW or MDRK; shock capturing with the first-order subcell

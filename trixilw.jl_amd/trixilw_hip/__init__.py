@@ -9,7 +9,7 @@ from .adjoint import adjoint_step, gradient_lw, kinetic_energy
 from .basis import LobattoLegendreBasis
 from .batch import SemidiscretizationBatch
 from .equations import (Adiabatic, BoundaryConditionNavierStokesWall, CompressibleEulerEquations2D,
-                        CompressibleNavierStokesDiffusion2D, ConstantState, NoSlip,
+                        CompressibleNavierStokesDiffusion2D, ConstantState, Isothermal, NoSlip,
                         FourQuadrantRiemann, GradientVariablesConservative, GradientVariablesPrimitive,
                         IsentropicVortex, LinearScalarAdvectionEquation2D, boundary_condition_slip_wall_horizontal,
                         boundary_condition_slip_wall_vertical)
@@ -26,7 +26,7 @@ from .semidiscretization import (DGSEM, LW, MDRK, Adaptive, CFLBased, FirstOrder
 
 __all__ = [
     "LobattoLegendreBasis", "CompressibleEulerEquations2D", "ConstantState", "FourQuadrantRiemann",
-    "boundary_condition_slip_wall_vertical", "boundary_condition_slip_wall_horizontal", "NoSlip", "Adiabatic",
+    "boundary_condition_slip_wall_vertical", "boundary_condition_slip_wall_horizontal", "NoSlip", "Adiabatic", "Isothermal",
     "BoundaryConditionNavierStokesWall",
     "IsentropicVortex", "LinearScalarAdvectionEquation2D", "LWIntegrator", "LWSolution", "compute_dt", "isapprox",
     "isfinished", "perform_step", "set_dt", "set_t_and_iter", "solve_lwfr", "TreeMesh",

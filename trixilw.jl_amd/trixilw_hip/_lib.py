@@ -89,6 +89,8 @@ class ParabolicDesc(C.Structure):
         ("gradient_variables", C.c_int32),
         ("walls", C.c_int32),
         ("wall_velocity", C.c_double * 8),
+        ("wall_isothermal", C.c_int32 * 4),
+        ("wall_temperature", C.c_double * 4),
     ]
 
 

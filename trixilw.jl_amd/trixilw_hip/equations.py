@@ -66,16 +66,32 @@ class Adiabatic:
             raise NotImplementedError("the device path implements the insulated wall (zero normal heat flux)")
 
 
+class Isothermal:
+    """Trixi Isothermal(boundary_value_function): the wall temperature T = p/ρ
+    (the reference's wall condition takes R = 1); the device takes a constant
+    temperature (a callable is evaluated once and must not depend on x or t)."""
+
+    def __init__(self, temperature):
+        if callable(temperature):
+            a = temperature((0.0, 0.0), 0.0, None)
+            if float(a) != float(temperature((0.37, -0.61), 1.3, None)):
+                raise NotImplementedError("the device path takes a constant wall temperature")
+            temperature = a
+        self.temperature = float(temperature)
+
+
 class BoundaryConditionNavierStokesWall:
     """Trixi BoundaryConditionNavierStokesWall(velocity, heat) with NoSlip and
-    Adiabatic (the reference's lid-driven cavity, conservative gradient
-    variables: src/equations/compressible_navier_stokes_2d.jl:81-124)."""
+    Adiabatic or Isothermal (conservative gradient variables:
+    src/equations/compressible_navier_stokes_2d.jl:81-124; the reference's
+    lid-driven cavity uses Adiabatic)."""
 
     def __init__(self, boundary_condition_velocity, boundary_condition_heat_flux):
         if not isinstance(boundary_condition_velocity, NoSlip) or not isinstance(boundary_condition_heat_flux,
-                                                                                 Adiabatic):
-            raise NotImplementedError("the device path implements NoSlip velocity with Adiabatic heat walls")
+                                                                                 (Adiabatic, Isothermal)):
+            raise NotImplementedError("the device path implements NoSlip velocity with Adiabatic or Isothermal walls")
         self.velocity = boundary_condition_velocity.velocity
+        self.temperature = getattr(boundary_condition_heat_flux, "temperature", None)
 
 
 def boundary_conditions_kind(bcs):

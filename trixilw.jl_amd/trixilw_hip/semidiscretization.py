@@ -420,6 +420,8 @@ class SemidiscretizationHyperbolicParabolic(SemidiscretizationHyperbolic):
             self._par.walls = 1
             for k, b in enumerate(seq):
                 self._par.wall_velocity[2 * k], self._par.wall_velocity[2 * k + 1] = b.velocity
+                if b.temperature is not None:
+                    self._par.wall_isothermal[k], self._par.wall_temperature[k] = 1, b.temperature
         elif bc_hyp is not None:
             raise NotImplementedError("the hyperbolic-parabolic device path needs a periodic mesh or walls")
         super().__init__(mesh, time_discretization, eq, initial_condition, solver, boundary_conditions=bc_hyp,
