@@ -162,7 +162,8 @@ int tlw_create(const tlw_mesh_desc* mesh, const tlw_basis_desc* basis, const tlw
 int tlw_create_batch(int32_t n_instances, const tlw_mesh_desc* meshes, const tlw_basis_desc* basis,
                      const tlw_solver_desc* solver, tlw_ctx** out);
 /* SemidiscretizationHyperbolicParabolic(mesh, time_disc, (equations, equations_parabolic), ic, solver)
- * (semidiscretization_hyperbolic_parabolic.jl:8-118) on a periodic mesh: every rhs and step of the
+ * (semidiscretization_hyperbolic_parabolic.jl:8-118) on a periodic mesh, or with the no-slip walls of
+ * tlw_parabolic_desc.walls on a non-periodic TreeMesh (LW): every rhs and step of the
  * context then follows rhs! / rhs_mdrk1! / rhs_mdrk2! of the parabolic path
  * (src/solvers/dgsem_tree/dg_2d_parabolic.jl:15-98, multi_derivative_2d_parabolic.jl:59-219):
  * BR1 gradient of the conservative variables, LW procedure on f_a - f_v with Taylor states of

@@ -387,8 +387,11 @@ class SemidiscretizationHyperbolicParabolic(SemidiscretizationHyperbolic):
     (src/semidiscretization/semidiscretization_hyperbolic_parabolic.jl:8-118).
 
     ``equations_parabolic`` is a CompressibleNavierStokesDiffusion2D with
-    GradientVariablesConservative; the mesh must be periodic (boundary
-    conditions None).  rhs / rhs_mdrk1 / rhs_mdrk2, max_dt and the device loop
+    GradientVariablesConservative.  The mesh is periodic (boundary conditions
+    None), or a TreeMesh with walls: ``boundary_conditions=(bc_hyperbolic,
+    bc_parabolic)`` with the reflecting walls for the hyperbolic part and four
+    BoundaryConditionNavierStokesWall(NoSlip(...), Adiabatic(0.0)) for x_neg,
+    x_pos, y_neg, y_pos (the reference's lid-driven cavity; LW).  rhs / rhs_mdrk1 / rhs_mdrk2, max_dt and the device loop
     then include the viscous terms (BR1 gradient, Taylor states of ∇u, central
     viscous interface flux, compute_dt's min of advective and viscous bounds).
     On a periodic curvilinear StructuredMesh the P4est parabolic kernels'
