@@ -114,17 +114,23 @@ def test_structured_flux_order_parity(oracle_mod):
     assert_close(to_lex(mesh, du), o.rhs(u_lex, 0.0, dt), what="structured order")
 
 
-def test_morton_ordering_parity(oracle_mod):
+@pytest.mark.parametrize("ordering", ["morton", "strips"])
+def test_element_ordering_parity(oracle_mod, ordering):
+    """Any element order gives the same residual (the bench uses two-row strips,
+    whose 10-element runs are 5×2 blocks)."""
     import trixilw_hip as tlw
 
     torch = torch_mod()
-    semi, o, u_lex, mesh = setup(oracle_mod, 4, 4, ordering="morton", seed=8)
+    semi, o, u_lex, mesh = setup(oracle_mod, 4, 4, ordering=ordering, seed=8)
     dt = 0.4 * o.max_dt(u_lex)
     u = to_dev(mesh, u_lex)
     du = torch.empty_like(u)
     semi.cache["dt"][0] = dt
-    tlw.rhs(du, u, semi, 0.0)
-    assert_close(to_lex(mesh, du), o.rhs(u_lex, 0.0, dt), what="morton")
+    for tile in (5, 10, 15):
+        semi.elements_per_workgroup(tile)
+        tlw.rhs(du, u, semi, 0.0)
+        torch.cuda.synchronize()
+        assert_close(to_lex(mesh, du), o.rhs(u_lex, 0.0, dt), what=f"{ordering} tile {tile}")
 
 
 @pytest.mark.parametrize("N,td", [(4, "lw"), (3, "lw"), (3, "mdrk"), (4, "mdrk")])
