@@ -80,7 +80,7 @@ def warp_mapping(L, A):
 
 
 def setup_curved(oracle_mod, N, nx, ny, kind="warp", L=20.0, A=0.05, td="lw", seed=0, amp=1e-3, keep_aux=False,
-                 periodic=True, sc=False, alpha_max=0.5, alpha_smooth=True):
+                 periodic=True, sc=False, alpha_max=0.5, alpha_smooth=True, ordering="lexicographic"):
     """Isentropic vortex on a warped or affine StructuredMesh of [0, L]², the
     device semidiscretization and the oracle on the same mesh with the oracle's
     metric terms on both sides.  A non-periodic mesh takes the vortex itself (no
@@ -93,11 +93,11 @@ def setup_curved(oracle_mod, N, nx, ny, kind="warp", L=20.0, A=0.05, td="lw", se
     period = (L, L * ny / nx if kind == "affine" else L) if periodic else (1e9, 1e9)
     ic = tlw.IsentropicVortex(center=(L / 2, L / 2), period=period)
     if kind == "warp":
-        mesh = tlw.StructuredMesh((nx, ny), warp_mapping(L, A), periodicity=periodic)
+        mesh = tlw.StructuredMesh((nx, ny), warp_mapping(L, A), periodicity=periodic, ordering=ordering)
         mp = (oracle_mod.MAP_WARP, (L, A))
     else:
         Ly = L * ny / nx
-        mesh = tlw.StructuredMesh((nx, ny), (0.0, 0.0), (L, Ly), periodicity=periodic)
+        mesh = tlw.StructuredMesh((nx, ny), (0.0, 0.0), (L, Ly), periodicity=periodic, ordering=ordering)
         mp = (oracle_mod.MAP_AFFINE, (0.0, 0.0, L, Ly))
     vi = tlw.VolumeIntegralFR(tlw.MDRK() if td == "mdrk" else tlw.LW())
     if sc:

@@ -22,13 +22,13 @@ def state(x, y, t=0.0):
     return shear_state(x - 1.0, y - 1.0)
 
 
-def setup_cns(oracle_mod, N, nx, td="lw", mu=0.01, A=0.08, keep_aux=False, seed=0, amp=1e-3):
+def setup_cns(oracle_mod, N, nx, td="lw", mu=0.01, A=0.08, keep_aux=False, seed=0, amp=1e-3, ordering="lexicographic"):
     import trixilw_hip as tlw
 
     eq = tlw.CompressibleEulerEquations2D(GAMMA)
     eqp = tlw.CompressibleNavierStokesDiffusion2D(eq, mu=mu, Prandtl=PR,
                                                  gradient_variables=tlw.GradientVariablesConservative())
-    mesh = tlw.StructuredMesh((nx, nx), warp_mapping(L, A))
+    mesh = tlw.StructuredMesh((nx, nx), warp_mapping(L, A), ordering=ordering)
     solver = tlw.DGSEM(N, volume_integral=tlw.VolumeIntegralFR(tlw.MDRK() if td == "mdrk" else tlw.LW()))
     o = oracle_mod.Oracle(N, nx, nx, 0.0, 0.0, L / nx, gamma=GAMMA, mapping=(oracle_mod.MAP_WARP, (L, A)),
                           parabolic="navier_stokes", mu=mu, prandtl=PR)
@@ -197,4 +197,27 @@ def test_curved_ns_rejects_unsupported(oracle_mod):
         with pytest.raises(_lib.TLWError) as ei:
             call()
         assert ei.value.code == _lib.TLW_ENOTSUP
+    semi.close()
+
+
+@pytest.mark.parametrize("td", ["lw", "mdrk"])
+def test_curved_ns_strips_ordering_parity(oracle_mod, td):
+    """The strips element order (two-row strips, the bench's order) gives the
+    oracle's residuals and MDRK stages at every tile size."""
+    import trixilw_hip as tlw
+
+    torch = torch_mod()
+    semi, o, u_lex, mesh = setup_cns(oracle_mod, 3, 8, td=td, ordering="strips")
+    dt = 0.5 * o.max_dt(u_lex)
+    kind = "mdrk1" if td == "mdrk" else "lw"
+    du_ref = o.rhs(u_lex, 0.1, dt, kind=kind)
+    u = to_dev(mesh, u_lex)
+    semi.cache["dt"][0] = dt
+    for eb in (4, 8, 16):
+        print(f"curved NS strips {td} tile {eb}", flush=True)
+        semi.elements_per_workgroup(eb)
+        du = torch.empty_like(u)
+        (tlw.rhs_mdrk1 if td == "mdrk" else tlw.rhs)(du, u, semi, 0.1)
+        torch.cuda.synchronize()
+        assert_close(to_lex(mesh, du), du_ref, what=f"curved NS strips {td} tile {eb}")
     semi.close()

@@ -312,3 +312,43 @@ def test_curved_rejects_unsupported(oracle_mod):
         with pytest.raises(_lib.TLWError) as ei:
             tlw.SemidiscretizationHyperbolic(mesh_k, td, eq, ic, solver)
         assert ei.value.code == code
+
+
+@pytest.mark.parametrize("periodic,sc", [(True, False), (False, False), (True, True)])
+def test_curved_strips_ordering_parity(oracle_mod, periodic, sc):
+    """The strips element order (two-row strips, the bench's order) on periodic,
+    Dirichlet and shock-capturing curvilinear meshes: the oracle's residuals at
+    every tile size, and 10 device-loop steps."""
+    import trixilw_hip as tlw
+
+    torch = torch_mod()
+    semi, o, u_lex, mesh = setup_curved(oracle_mod, 4, 8, 6, periodic=periodic, sc=sc, ordering="strips",
+                                        keep_aux=sc)
+    if sc:
+        from test_gpu_curved_sc_parity import with_jump
+
+        u_lex = with_jump(o, u_lex)
+    dt = 0.5 * o.max_dt(u_lex)
+    du_ref = o.rhs(u_lex, 0.0, dt)
+    u = to_dev(mesh, u_lex)
+    semi.cache["dt"][0] = dt
+    for eb in (5, 10, 15):
+        print(f"curved strips periodic={periodic} sc={sc} tile {eb}", flush=True)
+        semi.elements_per_workgroup(eb)
+        du = torch.empty_like(u)
+        tlw.rhs(du, u, semi, 0.0)
+        torch.cuda.synchronize()
+        assert_close(to_lex(mesh, du), du_ref, what=f"curved strips periodic={periodic} sc={sc} tile {eb}")
+        if sc:
+            alpha = semi.get_aux("alpha")[mesh.lexicographic_permutation()]
+            assert np.count_nonzero(o.aux("alpha")) > 2
+            assert np.max(np.abs(alpha - o.aux("alpha"))) < 1e-13
+    K = 10
+    u_ref, t_ref, _ = o.solve(u_lex, 0.0, 1e9, 0.5, max_steps=K)
+    lw_update = tlw.semidiscretize(semi, semi.time_discretization, (0.0, float("inf")))
+    lw_update.u0.copy_(to_dev(mesh, u_lex))
+    sol = tlw.solve_lwfr(lw_update, (), 1e-6, None, time_step_computation=tlw.CFLBased(0.5), max_steps=K)
+    assert sol.stats.naccept == K
+    assert abs(sol.t - t_ref) <= 1e-13 * t_ref
+    assert_close(to_lex(mesh, sol.u), u_ref, what=f"curved strips periodic={periodic} sc={sc} after {K} steps")
+    semi.close()

@@ -71,3 +71,36 @@ def test_affine_mapping_matches_treemesh_coordinates():
     Ja, ijn = curv.metrics(basis)
     assert np.allclose(ijn, 1.0 / (0.5 * 0.5), rtol=1e-13, atol=0)
     assert np.allclose(Ja[..., 0], 0.5, rtol=1e-13) and np.allclose(Ja[..., 3], 0.5, rtol=1e-13)
+
+
+@pytest.mark.parametrize("nx,ny,periodic", [(6, 4, True), (5, 7, False), (1, 3, True)])
+def test_strips_ordering_is_the_lexicographic_mesh_permuted(nx, ny, periodic):
+    """The strips-ordered mesh is the lexicographic one with its elements
+    renumbered: same cells, neighbours, node coordinates and metric terms
+    (host-computed and set from linear-index arrays) through the permutation,
+    and runs of 10 elements are 5×2 blocks away from the strip ends."""
+    from trixilw_hip.basis import LobattoLegendreBasis
+    from trixilw_hip.mesh import StructuredMesh
+
+    m = lambda o: StructuredMesh((nx, ny), warp_mapping(3.0, 0.1), periodicity=periodic, ordering=o)
+    lex, st = m("lexicographic"), m("strips")
+    p = st.lexicographic_permutation()  # linear index -> element
+    assert sorted(p.tolist()) == list(range(nx * ny))
+    assert np.array_equal(st.ix[p], lex.ix) and np.array_equal(st.iy[p], lex.iy)
+    # a neighbour relation is the same relation under the renumbering
+    back = np.where(lex.neighbors >= 0, p[np.maximum(lex.neighbors, 0)], -1)
+    assert np.array_equal(st.neighbors[p], back)
+    basis = LobattoLegendreBasis(3)
+    for a, b in zip(st.node_coordinates(basis.nodes), lex.node_coordinates(basis.nodes)):
+        assert np.array_equal(a[p], b)
+    Ja, ijn = lex.metrics(basis)
+    for a, b in zip(st.metrics(basis), (Ja, ijn)):
+        assert np.array_equal(a[p], b)
+    st.set_metrics(basis, Ja, ijn)
+    for a, b in zip(st.metrics(basis), (Ja, ijn)):
+        assert np.array_equal(a[p], b)
+    if ny >= 2 and nx >= 5:
+        run = np.arange(10)
+        assert set(zip(st.ix[run].tolist(), st.iy[run].tolist())) == {(i, j) for i in range(5) for j in range(2)}
+    with pytest.raises(ValueError):
+        m("hilbert")

@@ -115,8 +115,14 @@ class StructuredMesh:
     StructuredMesh(cells_per_dimension, coordinates_min, coordinates_max; periodicity=True).
 
     ``mapping(xi, eta) -> (x, y)`` maps the global reference square [-1, 1]²
-    onto the domain and must accept numpy arrays.  Elements are numbered
-    e = cx + nx*cy like Trixi's linear indices.  The node coordinates and
+    onto the domain and must accept numpy arrays.  With
+    ``ordering="lexicographic"`` (the default) elements are numbered
+    e = cx + nx*cy like Trixi's linear indices; ``ordering="strips"`` numbers
+    two-row strips column by column within a strip, so that a run of 10
+    elements is a 5×2 block (14 of its 40 face slots leave the run, against 22
+    for a 10×1 row).  ``lexicographic_permutation()`` maps the linear index
+    cx + nx*cy to the element, and ``set_metrics`` takes its arrays in linear
+    index order.  The node coordinates and
     metric terms follow Trixi's StructuredMesh cache (restated: the element's
     GLL nodes placed in its cell of the reference square and mapped;
     jacobian_matrix from the derivative matrix applied to the node
@@ -129,7 +135,8 @@ class StructuredMesh:
 
     curved = True
 
-    def __init__(self, cells_per_dimension, mapping, coordinates_max=None, periodicity=True):
+    def __init__(self, cells_per_dimension, mapping, coordinates_max=None, periodicity=True,
+                 ordering="lexicographic"):
         nx, ny = (int(c) for c in cells_per_dimension)
         if nx < 1 or ny < 1:
             raise ValueError("cells_per_dimension must be positive")
@@ -144,8 +151,18 @@ class StructuredMesh:
         self.nx, self.ny = nx, ny
         self.n_elements = nx * ny
         E = self.n_elements
-        e = np.arange(E)
-        self.ix, self.iy = e % nx, e // nx
+        lin = np.arange(E)
+        if ordering == "lexicographic":
+            order = lin
+        elif ordering == "strips":
+            order = np.lexsort((lin // nx % 2, lin % nx, lin // nx // 2))
+        else:
+            raise ValueError(f"unknown ordering {ordering!r}")
+        self.ordering = ordering
+        self._order = order  # element -> linear index
+        self._element_of = np.empty(E, dtype=np.int64)  # linear index -> element
+        self._element_of[order] = lin
+        self.ix, self.iy = order % nx, order // nx
         nbr = np.full((E, 4), -1, dtype=np.int32)
         px, py = self.periodicity
         for face, (dx, dy) in enumerate(((-1, 0), (1, 0), (0, -1), (0, 1))):
@@ -155,7 +172,7 @@ class StructuredMesh:
             if py:
                 jy = jy % ny
             ok = (jx >= 0) & (jx < nx) & (jy >= 0) & (jy < ny)
-            nbr[:, face] = np.where(ok, jx + nx * jy, -1)
+            nbr[:, face] = np.where(ok, self._element_of[np.where(ok, jx + nx * jy, 0)], -1)
         self.neighbors = nbr
         self._metrics = {}
 
@@ -195,12 +212,14 @@ class StructuredMesh:
 
     def set_metrics(self, basis, contravariant_vectors, inverse_jacobian):
         """Metric terms computed elsewhere (e.g. Trixi's cache.elements of the
-        same mesh) for the basis, in the layouts metrics() returns; used in
-        place of the host computation."""
+        same mesh) for the basis, in the layouts metrics() returns with the
+        elements in linear index order cx + nx*cy; used in place of the host
+        computation."""
         n = len(basis.nodes)
-        Ja = np.ascontiguousarray(contravariant_vectors, dtype=np.float64).reshape(self.n_elements, n, n, 4)
-        ijn = np.ascontiguousarray(inverse_jacobian, dtype=np.float64).reshape(self.n_elements, n, n)
+        Ja = np.asarray(contravariant_vectors, dtype=np.float64).reshape(self.n_elements, n, n, 4)
+        ijn = np.asarray(inverse_jacobian, dtype=np.float64).reshape(self.n_elements, n, n)
+        Ja, ijn = np.ascontiguousarray(Ja[self._order]), np.ascontiguousarray(ijn[self._order])
         self._metrics[id(basis)] = (Ja, ijn)
 
     def lexicographic_permutation(self):
-        return np.arange(self.n_elements)
+        return self._element_of.copy()
