@@ -251,6 +251,18 @@ int tlw_set_us(tlw_ctx* ctx, const double* us_dev);
 /* Trixi max_dt(u, t, mesh::TreeMesh{2}, ...) = 2 / (n * max_e J⁻¹ (max λ1 + max λ2));
  * compute_dt multiplies it by the CFL number (lwfr.jl:176-190). */
 int tlw_max_dt(tlw_ctx* ctx, const double* u_dev, double* max_dt_host);
+/* Allocation-free variant of tlw_max_dt, with no counterpart in the reference: for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Needs no context: u_dev is a TreeMesh Euler state [v,i,j,e] of n_elements
+ * elements at polydeg 3 or 4, inverse_jacobian_dev its [n_elements] inverse Jacobians; the result
+ * 2 / (n max_e J⁻¹ (max λ1 + max λ2)), the same bits tlw_max_dt gives, is written to max_dt_dev
+ * (device memory) in stream order on `stream` (hipStream_t, NULL = default), NaN if a node has
+ * ρ <= 0 or p <= 0.  No allocation, no host synchronisation.  Partial maxima use a fixed 2 MiB
+ * device array per GPU (held until the process ends): n_elements above 1 310 720 (polydeg 4) or
+ * 2 097 152 (polydeg 3) returns TLW_EINVAL.  Calls that overlap on one GPU share that array and
+ * race.  Errors: tlw_last_error(NULL) gives the message. */
+int tlw_max_dt_noalloc(const double* u_dev, const double* inverse_jacobian_dev, int32_t n_elements, int32_t polydeg,
+                       double gamma, double* max_dt_dev, void* stream);
 
 /* Device-resident time loop: the state lives in the context (tlw_set_state /
  * tlw_get_state / tlw_state_ptr).  tlw_solve_init sets t, the final time and the

@@ -20,6 +20,7 @@
 #include "../../include/trixilw_hip.h"
 #include "tlw_adjoint.hpp"
 #include "tlw_device.hpp"
+#include "tlw_noalloc.hpp"
 
 using namespace tlw;
 
@@ -754,6 +755,9 @@ bool any_bad(const tlw_ctx* c) {
     return false;
 }
 
+// message of the last failed context-free call on this thread (tlw_last_error(NULL))
+thread_local std::string g_noctx_err;
+
 int create_common(int ninst, const tlw_mesh_desc* meshes, const tlw_basis_desc* basis, const tlw_solver_desc* solver,
                   tlw_ctx** out, const tlw_parabolic_desc* par = nullptr, const tlw_curved_desc* cv = nullptr);
 
@@ -827,7 +831,10 @@ void tlw_destroy(tlw_ctx* c) {
     delete c;
 }
 
-const char* tlw_last_error(const tlw_ctx* c) { return c ? c->err.c_str() : "null context"; }
+const char* tlw_last_error(const tlw_ctx* c) {
+    if (c) return c->err.c_str();
+    return g_noctx_err.empty() ? "null context" : g_noctx_err.c_str();
+}
 
 int tlw_set_stream(tlw_ctx* c, void* stream) {
     if (!c) return TLW_EINVAL;
@@ -1196,6 +1203,24 @@ int tlw_get_aux(tlw_ctx* c, int32_t which, double* out, size_t len) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipMemcpy(out, src, sizeof(double) * len, hipMemcpyDeviceToHost));
     return TLW_OK;
+}
+
+int tlw_max_dt_noalloc(const double* u, const double* inv_jac, int32_t n_elements, int32_t polydeg, double gamma,
+                       double* max_dt, void* stream) {
+    if (!u || !inv_jac || !max_dt) {
+        g_noctx_err = "tlw_max_dt_noalloc: null pointer";
+        return TLW_EINVAL;
+    }
+    if (n_elements < 1 || (polydeg != 3 && polydeg != 4) || !(gamma > 1.0)) {
+        g_noctx_err = "tlw_max_dt_noalloc: n_elements >= 1, polydeg 3 or 4 and gamma > 1 required";
+        return TLW_EINVAL;
+    }
+    std::string err;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const int rc = polydeg == 4 ? tlw_noalloc::launch<4>(u, inv_jac, n_elements, gamma, max_dt, s, err)
+                                : tlw_noalloc::launch<3>(u, inv_jac, n_elements, gamma, max_dt, s, err);
+    if (rc) g_noctx_err = "tlw_max_dt_noalloc: " + err;
+    return rc;
 }
 
 }  // extern "C"

@@ -21,7 +21,7 @@ from .semidiscretization import (DGSEM, LW, MDRK, Adaptive, CFLBased, FirstOrder
                                  LWUpdate, MUSCLHancockReconstruction, SemidiscretizationHyperbolic,
                                  SemidiscretizationHyperbolicParabolic, VolumeIntegralFR,
                                  VolumeIntegralFRShockCapturing, density_pressure, flux_lax_friedrichs,
-                                 get_time_discretization, lw_fourier_cfl, rhs, rhs_mdrk1, rhs_mdrk2, semidiscretize,
+                                 get_time_discretization, lw_fourier_cfl, max_dt_noalloc, rhs, rhs_mdrk1, rhs_mdrk2, semidiscretize,
                                  trixi2lw)
 
 __all__ = [
@@ -32,7 +32,7 @@ __all__ = [
     "isfinished", "perform_step", "set_dt", "set_t_and_iter", "solve_lwfr", "TreeMesh",
     "StructuredMesh", "coordinates2mapping", "DGSEM", "LW", "MDRK",
     "CFLBased", "Adaptive", "LWUpdate", "SemidiscretizationHyperbolic", "VolumeIntegralFR", "flux_lax_friedrichs",
-    "get_time_discretization", "lw_fourier_cfl", "rhs", "rhs_mdrk1", "rhs_mdrk2", "semidiscretize", "trixi2lw",
+    "get_time_discretization", "lw_fourier_cfl", "max_dt_noalloc", "rhs", "rhs_mdrk1", "rhs_mdrk2", "semidiscretize", "trixi2lw",
     "FirstOrderReconstruction", "IndicatorHennemannGassner", "MUSCLHancockReconstruction",
     "VolumeIntegralFRShockCapturing", "density_pressure", "PositivityPreservingLimiterZhangShu", "density",
     "pressure", "adjoint_step", "gradient_lw", "kinetic_energy",

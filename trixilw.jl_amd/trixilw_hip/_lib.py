@@ -29,7 +29,7 @@ EXPORTED = (
     "tlw_get_elements_per_workgroup", "tlw_autotune", "tlw_set_positivity_limiter", "tlw_apply_positivity_limiter",
     "tlw_adjoint_lw", "tlw_create_batch", "tlw_n_instances", "tlw_batch_status", "tlw_enable_graphs",
     "tlw_create_parabolic", "tlw_solve_init_adaptive", "tlw_rejected_steps", "tlw_create_curved",
-    "tlw_create_curved_parabolic",
+    "tlw_create_curved_parabolic", "tlw_max_dt_noalloc",
 )
 
 
@@ -192,6 +192,8 @@ def lib():
     L.tlw_adjoint_lw.argtypes = [vp, vp, vp, vp, C.c_double, C.c_double]
     L.tlw_autotune.restype = C.c_int
     L.tlw_autotune.argtypes = [vp, i32, C.POINTER(i32), dp]
+    L.tlw_max_dt_noalloc.restype = C.c_int
+    L.tlw_max_dt_noalloc.argtypes = [vp, vp, i32, i32, C.c_double, vp, vp]
     _lib = L
     return L
 
@@ -199,7 +201,8 @@ def lib():
 def check(rc: int, handle=None) -> None:
     if rc == TLW_OK:
         return
-    msg = lib().tlw_last_error(handle).decode() if handle else "(no context)"
+    # a NULL handle gives the message of the last failed context-free call of this thread
+    msg = lib().tlw_last_error(handle).decode()
     if rc == TLW_EADMISSIBLE:
         raise NonAdmissibleState(rc, msg)
     raise TLWError(rc, msg)

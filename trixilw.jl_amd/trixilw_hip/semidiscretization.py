@@ -485,3 +485,24 @@ def semidiscretize(semi, time_discretization, tspan):
     u0_ode = semi.compute_coefficients(tspan[0])
     du_ode = _torch().empty_like(u0_ode)
     return LWUpdate(rhs, u0_ode, {"u0_ode": u0_ode, "du_ode": du_ode}, (float(tspan[0]), float(tspan[1])), semi)
+
+
+def max_dt_noalloc(u, inverse_jacobian, polydeg, gamma, out, stream=None):
+    """Trixi max_dt of a TreeMesh Euler state without a context or any device
+    allocation (tlw_max_dt_noalloc): u [E, n, n, 4] and inverse_jacobian [E]
+    are float64 device tensors, the result lands in out[0] (a float64 device
+    tensor) in stream order, NaN for a non-admissible state.  Calls that overlap
+    on one GPU share the library's scratch array and race."""
+    for name, x in (("u", u), ("inverse_jacobian", inverse_jacobian), ("out", out)):
+        if not x.is_cuda or str(x.dtype) != "torch.float64" or not x.is_contiguous():
+            raise ValueError(f"{name} must be a contiguous float64 device tensor")
+    n_el = int(inverse_jacobian.numel())
+    n = int(polydeg) + 1
+    if u.numel() != 4 * n * n * n_el:
+        raise ValueError("u must hold 4 (polydeg+1)^2 values per element")
+    if out.numel() < 1:
+        raise ValueError("out needs one element")
+    s = stream.cuda_stream if stream is not None else None
+    _lib.check(_lib.lib().tlw_max_dt_noalloc(C.c_void_p(u.data_ptr()), C.c_void_p(inverse_jacobian.data_ptr()), n_el,
+                                             int(polydeg), float(gamma), C.c_void_p(out.data_ptr()),
+                                             C.c_void_p(s) if s else None))
