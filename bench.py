@@ -502,6 +502,53 @@ def measure_curved(steps, cpu_steps, threads, with_cpu=True):
     return out
 
 
+def measure_gradient(nsteps=200, checkpoint_every=20, level=6, seed=2025, reps=3):
+    """The differentiation configuration: d KE(u_200) / d u0 on the 64² vortex,
+    N=4, 200 LW steps at 0.98 of the LW CFL limit, checkpoints every 20, the
+    initial density perturbed by a seeded Gaussian random field of amplitude
+    1e-2.  Times the forward device loop alone (200 steps, checkpoint copies
+    included) and the whole gradient (forward, segment re-forward and adjoint
+    steps); median of `reps` runs after one warm-up."""
+    import torch
+    import trixilw_hip as tlw
+
+    eq = tlw.CompressibleEulerEquations2D(1.4)
+    ic = tlw.IsentropicVortex()
+    mesh = tlw.TreeMesh((-10.0, -10.0), (10.0, 10.0), level, ordering="strips")
+    solver = tlw.DGSEM(4, volume_integral=tlw.VolumeIntegralFR(tlw.LW()))
+    semi = tlw.SemidiscretizationHyperbolic(mesh, tlw.LW(), eq, ic, solver)
+    x, y = semi.node_coordinates()
+    u_host = ic(x, y, 0.0)
+    u_host[..., 0] += tlw.gaussian_random_field(x, y, (20.0, 20.0), seed=seed, amplitude=1e-2)
+    u0 = torch.from_numpy(np.ascontiguousarray(u_host)).to("cuda")
+    cfl = tlw.trixi2lw(0.98, solver)
+    from trixilw_hip.adjoint import _forward_device
+
+    fwd, grad = [], []
+    for r in range(reps + 1):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _forward_device(semi, u0, nsteps, cfl, 0.0, None, checkpoint_every)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        tlw.gradient_lw(semi, u0, nsteps, cfl, checkpoint_every=checkpoint_every)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        if r > 0:
+            fwd.append(t1 - t0)
+            grad.append(t2 - t1)
+    f, g = float(np.median(fwd)), float(np.median(grad))
+    semi.close()
+    return {"config": f"{mesh.nx}x{mesh.nx} vortex N=4, {nsteps} LW steps, checkpoint every {checkpoint_every}, "
+                      f"GRF density perturbation 1e-2 (seed {seed})",
+            "forward_ms_per_step": round(1e3 * f / nsteps, 4), "gradient_ms_per_step": round(1e3 * g / nsteps, 4),
+            "gradient_over_forward": round(g / f, 3),
+            "reverse_over_forward": round((g - f) / f, 3),
+            "note": "wall clock of gradient_lw (device-loop forward with checkpoint copies, re-forward of each "
+                    "segment by rhs! + update, adjoint steps) over the forward device loop alone; "
+                    f"median of {reps}"}
+
+
 def measure_adaptive(semi, u0, attempts, tol=(1e-10, 1e-10), dt0=1e-5):
     """Sub-record: the adaptive (error-controlled) LW loop on the same mesh and
     state: `attempts` device-loop attempts after 30 untimed ones (the controller
@@ -662,9 +709,13 @@ def main():
         adj_ms = e0.elapsed_time(e1) / reps
         adjoint = {"step_ms": round(adj_ms, 4), "forward_step_ms": round(ms_per_step, 4),
                    "ratio": round(adj_ms / ms_per_step, 3), "reps": reps,
-                   "note": "one discrete-adjoint LW step (k_adj_records + k_adjoint) on the same mesh and state"}
+                   "note": "kernel-only: one discrete-adjoint LW step (k_adj_records + k_adjoint) on the same "
+                           "mesh and state as the forward step"}
         log(f"bench: adjoint step {adj_ms:.3f} ms = {adj_ms / ms_per_step:.2f} x forward step")
         del u, lam_a, lam_b
+        if args.workload == DEFAULT_WORKLOAD:
+            adjoint["gradient"] = measure_gradient()
+            log(f"bench: gradient {adjoint['gradient']}")
 
     adaptive = None
     if args.workload == DEFAULT_WORKLOAD and not args.no_adaptive:

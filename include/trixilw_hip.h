@@ -55,7 +55,8 @@ enum { TLW_FLUX_TREE = 0, TLW_FLUX_STRUCTURED = 1 };
  * boundary_condition_slip_wall_vertical on x faces and boundary_condition_slip_wall_horizontal on
  * y faces (equations.jl:95-139: mirrored uⁿ and Ũ, F_outer = (-F1, F2, -F3, -F4) resp.
  * (-F1, -F2, F3, -F4), the LW flux in the boundary order of the Dirichlet call); TreeMesh only */
-enum { TLW_BV_NONE = 0, TLW_BV_VORTEX = 1, TLW_BV_CONSTANT = 2, TLW_BV_RIEMANN4 = 3, TLW_BV_SLIP_WALL = 4 };
+enum { TLW_BV_NONE = 0, TLW_BV_VORTEX = 1, TLW_BV_CONSTANT = 2, TLW_BV_RIEMANN4 = 3, TLW_BV_SLIP_WALL = 4,
+       TLW_BV_TABLE = 5 /* caller-defined boundary_value_function: values uploaded by tlw_set_boundary_data */ };
 
 /* tlw_get_aux selectors (element_cache fields, src/solvers/dgsem_tree/containers.jl:73-98,
  * and the shock-capturing blending factors, indicator.cache.alpha) */
@@ -274,9 +275,35 @@ int tlw_max_dt_noalloc(const double* u_dev, const double* inverse_jacobian_dev, 
 int tlw_set_state(tlw_ctx* ctx, const double* u_host, size_t len);
 int tlw_get_state(tlw_ctx* ctx, double* u_host, size_t len);
 int tlw_state_ptr(tlw_ctx* ctx, double** u_dev);
+/* Stream-ordered device-to-device copies into / out of the loop's state (integrator.u held in
+ * device memory by the caller), the device counterparts of tlw_set_state / tlw_get_state. */
+int tlw_set_state_dev(tlw_ctx* ctx, const double* u_dev);
+int tlw_get_state_dev(tlw_ctx* ctx, double* u_dev);
 int tlw_solve_init(tlw_ctx* ctx, double t0, double t_final, double cfl);
 int tlw_advance(tlw_ctx* ctx, int32_t max_steps, double* t_host, double* dt_host, int64_t* iter_host,
                 int32_t* finished_host);
+/* BoundaryConditionDirichlet(boundary_value_function) for any function (src/equations/equations.jl:6-37;
+ * TreeMesh and curvilinear StructuredMesh): with solver->boundary_kind = TLW_BV_TABLE the caller
+ * evaluates its function on the host and uploads the states; the device forms the time averages of
+ * the state and its flux (the GLL rule over [t, t + s dt]) and the Rusanov flux as for the built-in
+ * functions.  tlw_boundary_nodes gives the number of boundary face nodes and (xy_host != NULL, len =
+ * 2 n_nodes) their coordinates [node][2] in table order: element, face (-x, +x, -y, +y), face node.
+ * tlw_set_boundary_data uploads the conservative states [node][level][4] (stream-ordered):
+ * time_levels = 1, a steady state used at every time (the device loop accepts it), or polydeg + 2:
+ * levels q = 0..polydeg at ts_q = t + ½ s dt (ξ_q + 1) with the GLL nodes ξ_q, then level polydeg+1 at
+ * t, for the next rhs call's t and dt (s = 0.5 for tlw_rhs_mdrk1 on a TreeMesh, the reference's
+ * scaling_factor, else 1); the device loop refuses such per-call data (TLW_ENOTSUP). */
+int tlw_boundary_nodes(tlw_ctx* ctx, int64_t* n_nodes, double* xy_host, size_t len);
+int tlw_set_boundary_data(tlw_ctx* ctx, const double* values_host, size_t len, int32_t time_levels);
+/* Per-step dt record and prescribed dts of the device loop on a single-instance context, for the
+ * checkpointed gradient driver (no counterpart in the reference, whose LWIntegrator keeps dt on the
+ * host).  tlw_record_dts(capacity): the dt of step k (k = iterations since the last tlw_solve_init,
+ * 0-based) is stored on the device while k < capacity (0 turns it off); tlw_get_dts copies the first
+ * n.  tlw_prescribe_dts(dts, n): step k takes dts[k] instead of cfl · max_dt while k < n (set_dt!'s
+ * truncation at the final time still applies; n = 0 turns it off); CFL steps only. */
+int tlw_record_dts(tlw_ctx* ctx, int64_t capacity);
+int tlw_get_dts(tlw_ctx* ctx, double* dts_host, int64_t n);
+int tlw_prescribe_dts(tlw_ctx* ctx, const double* dts_host, int64_t n);
 
 /* Adaptive (error-controlled) time stepping, solve_lwfr with time_step_computation = Adaptive()
  * (src/solvers/adaptive.jl:95-173, lwfr.jl:141-171): replaces tlw_solve_init.  dt starts at

@@ -16,7 +16,8 @@ import numpy as np
 HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "liboracle_tlw.so"
 
-BV_NONE, BV_VORTEX, BV_CONSTANT, BV_RIEMANN4, BV_SLIP_WALL = 0, 1, 2, 3, 4
+BV_NONE, BV_VORTEX, BV_CONSTANT, BV_RIEMANN4, BV_SLIP_WALL, BV_CALLBACK = 0, 1, 2, 3, 4, 5
+BC_FN = C.CFUNCTYPE(None, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double))
 
 
 class _Desc(C.Structure):
@@ -56,6 +57,7 @@ class _Desc(C.Structure):
         ("wall_v", C.c_double * 8),
         ("wall_iso", C.c_int * 4),
         ("wall_T", C.c_double * 4),
+        ("bc_fn", C.c_void_p),
     ]
 
 
@@ -132,7 +134,7 @@ class Oracle:
                  adv=(1.0, 1.0), flux_order=0, bc_kind=BV_NONE, bc_par=(), tolerances=(0.0, 0.0), nthreads=0,
                  shock_capturing=False, alpha_max=0.5, alpha_min=0.001, alpha_smooth=True, positivity=None,
                  parabolic=None, mu=0.0, prandtl=0.72, mapping=None, reconstruction="first_order",
-                 ns_walls=None, wall_temperature=None):
+                 ns_walls=None, wall_temperature=None, bc_fn=None):
         """``positivity``: (threshold_density, threshold_pressure) of a Zhang-Shu
         limiter that ``solve`` applies as a stage limiter, or None.
         ``parabolic``: None, "laplace" (LaplaceDiffusion2D with diffusivity ``mu``,
@@ -146,7 +148,9 @@ class Oracle:
         (MAP_WARP, (L, A)) for a periodic curvilinear StructuredMesh of nx × ny
         cells (xmin, ymin, h are then unused).
         ``reconstruction``: "first_order" or "muscl_hancock" subcell FV of the
-        shock-capturing blend (Euler)."""
+        shock-capturing blend (Euler).
+        ``bc_fn``: with bc_kind=BV_CALLBACK, the boundary value function
+        f(x, y, t) -> 4 conservative values, called pointwise (Python)."""
         d = _Desc()
         d.polydeg, d.nx, d.ny = polydeg, nx, ny
         d.xmin, d.ymin, d.h = xmin, ymin, h
@@ -181,6 +185,14 @@ class Oracle:
             d.map_kind = int(mapping[0])
             for k, v in enumerate(mapping[1]):
                 d.map_par[k] = v
+        if bc_fn is not None:
+            def _cb(x, y, t, out, f=bc_fn):
+                v = f(x, y, t)
+                for q in range(4):
+                    out[q] = float(v[q])
+
+            self._bc_cb = BC_FN(_cb)  # kept alive with the oracle
+            d.bc_fn = C.cast(self._bc_cb, C.c_void_p)
         self._desc = d
         self._h = lib().tlwo_create(C.byref(d))
         if not self._h:

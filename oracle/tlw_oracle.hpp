@@ -319,7 +319,9 @@ template <class T> struct LinAdv2D {
 // BV_SLIP_WALL is no boundary value but the reference's reflecting wall
 // (boundary_condition_slip_wall_vertical on x faces, _horizontal on y faces,
 // src/equations/equations.jl:95-139).
-enum BoundaryValueKind { BV_NONE = 0, BV_VORTEX = 1, BV_CONSTANT = 2, BV_RIEMANN4 = 3, BV_SLIP_WALL = 4 };
+// BV_CALLBACK: a caller-defined boundary_value_function (x, y, t) -> conservative
+// state, Trixi's BoundaryConditionDirichlet(f) for any f (equations.jl:6-37)
+enum BoundaryValueKind { BV_NONE = 0, BV_VORTEX = 1, BV_CONSTANT = 2, BV_RIEMANN4 = 3, BV_SLIP_WALL = 4, BV_CALLBACK = 5 };
 
 struct BoundaryValue {
     int kind = BV_NONE;
@@ -328,8 +330,13 @@ struct BoundaryValue {
     // constant: par = {rho, v1, v2, p, gamma}
     // four-quadrant Riemann: par = {x0, y0, gamma, then (rho, v1, v2, p) for
     //   quadrants NE, NW, SW, SE}
+    void (*fn)(double, double, double, double*) = nullptr;  // BV_CALLBACK
     void eval(double x, double y, double t, double* u) const {
         double rho, v1, v2, p, gamma;
+        if (kind == BV_CALLBACK) {
+            fn(x, y, t, u);
+            return;
+        }
         if (kind == BV_VORTEX) {
             const double eps = par[0], mach = par[1], Lx = par[4], Ly = par[5];
             gamma = par[6];

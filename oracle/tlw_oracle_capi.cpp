@@ -53,6 +53,8 @@ struct tlwo_desc {
     double wall_v[8];
     int wall_iso[4];     // 1: Isothermal with wall_T, 0: Adiabatic
     double wall_T[4];
+    // bc_kind = BV_CALLBACK: the boundary value function u(x, y, t)
+    void (*bc_fn)(double, double, double, double*);
 };
 
 struct tlwo_ctx {
@@ -67,6 +69,7 @@ template <class S> static void configure(S& s, const tlwo_desc* d) {
     s.flux_order = d->flux_order;
     s.bval.kind = d->bc_kind;
     for (int k = 0; k < 24; ++k) s.bval.par[k] = d->bc_par[k];
+    s.bval.fn = d->bc_fn;
     s.abstol = d->abstol;
     s.reltol = d->reltol;
     s.sc = d->shock_capturing != 0;
@@ -98,6 +101,7 @@ void* tlwo_create(const tlwo_desc* d) {
     if (d->reconstruction == 1 && (d->equation != 0 || !d->shock_capturing)) return nullptr;
     // the reflecting wall is the Euler TreeMesh one (x faces vertical, y faces horizontal)
     if (d->bc_kind == BV_SLIP_WALL && (d->equation != 0 || d->curved)) return nullptr;
+    if (d->bc_kind == BV_CALLBACK && !d->bc_fn) return nullptr;
     auto* c = new tlwo_ctx;
     c->equation = d->equation;
     if (d->curved && d->parabolic == 2) {

@@ -35,8 +35,12 @@ def setup(oracle_mod, N, level, td="lw", periodic=True, domain=(-10.0, 10.0), bc
                                             boundary_conditions=bc, surface_flux_order=flux_order,
                                             keep_aux=keep_aux)
     nx = mesh.nx
+    # a caller-defined BoundaryConditionDirichlet reaches the oracle as a pointwise callback
+    bc_fn = None
+    if isinstance(bc, tlw.BoundaryConditionDirichlet):
+        bc_fn = lambda x, y, t, f=bc.boundary_value_function: f((x, y), t, None)  # noqa: E731
     o = oracle_mod.Oracle(N, nx, nx, domain[0], domain[0], (domain[1] - domain[0]) / nx, periodic=(periodic,) * 2,
-                          flux_order=0 if flux_order == "tree" else 1,
+                          flux_order=0 if flux_order == "tree" else 1, bc_fn=bc_fn,
                           bc_kind=bc.kind if bc is not None else 0, bc_par=bc.params if bc is not None else (),
                           shock_capturing=sc, alpha_max=alpha_max, alpha_min=0.001, alpha_smooth=alpha_smooth,
                           reconstruction=recon if sc else "first_order")

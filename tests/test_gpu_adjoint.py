@@ -74,10 +74,57 @@ def test_checkpointed_gradient_matches_tangent_and_fd(oracle_mod):
     assert abs(dJ_adj - dJ_tan) <= 1e-10 * abs(dJ_tan), (dJ_adj, dJ_tan)
     # central differences on the device with the frozen dts
     h = 1e-6
-    Jp = tlw.gradient_lw(semi, to_dev(mesh, u_lex + h * v), 20, cfl, dts=dts, checkpoint_every=100)[0]
-    Jm = tlw.gradient_lw(semi, to_dev(mesh, u_lex - h * v), 20, cfl, dts=dts, checkpoint_every=100)[0]
+    Jp = tlw.objective_lw(semi, to_dev(mesh, u_lex + h * v), 20, dts)
+    Jm = tlw.objective_lw(semi, to_dev(mesh, u_lex - h * v), 20, dts)
     fd = (Jp - Jm) / (2 * h)
     assert abs(dJ_adj - fd) <= 1e-6 * abs(fd), (dJ_adj, fd)
+
+
+def test_c3_gradient_64x64_200_steps(oracle_mod):
+    """The differentiation configuration as stated: the 64² periodic vortex
+    (strength 5, Mach 0.5) on [-10,10]², N=4, 200 LW steps at 0.98 of the LW
+    CFL limit, checkpoints every 20 steps, an initial density perturbation by
+    a seeded Gaussian random field of amplitude 1e-2; d KE(u_200) along a
+    second such field: adjoint against the oracle's dual-number tangent
+    (≤ 1e-10) and device central differences with h = 1e-6 (≤ 1e-6).  The
+    forward run's dts (recorded on the device) match the oracle's CFL dts,
+    and J matches the oracle's, ≤ 1e-12."""
+    import trixilw_hip as tlw
+
+    torch = torch_mod()
+    ic = tlw.IsentropicVortex()
+    semi, o, u_lex, mesh = setup(oracle_mod, 4, 6, ic=ic, amp=0.0, keep_aux=False, ordering="strips")
+    x, y = o.node_coords()
+    u_lex = u_lex.copy()
+    u_lex[..., 0] += tlw.gaussian_random_field(x, y, (20.0, 20.0), seed=2025, amplitude=1e-2)
+    v = np.zeros_like(u_lex)
+    v[..., 0] = tlw.gaussian_random_field(x, y, (20.0, 20.0), seed=2026, amplitude=1.0)
+    cfl = tlw.trixi2lw(0.98, semi.solver)
+    nsteps = 200
+    J, grad, uK, dts = tlw.gradient_lw(semi, to_dev(mesh, u_lex), nsteps, cfl, checkpoint_every=20)
+    torch.cuda.synchronize()
+    # forward parity: the device loop's CFL dts and final state against the oracle's solve
+    u_ref, _, dts_ref = o.solve(u_lex, 0.0, 1e9, cfl, max_steps=nsteps)
+    assert len(dts_ref) == nsteps
+    assert np.max(np.abs(np.asarray(dts) - np.asarray(dts_ref)) / np.asarray(dts_ref)) <= 1e-12
+    uK_lex = to_lex(mesh, uK)
+    assert np.max(np.abs(uK_lex - u_ref)) <= 1e-12 * np.max(np.abs(u_ref))
+    _, w, _, _ = o.basis()
+    J_o, gJ = ke_and_grad_np(u_ref, w, 20.0 / mesh.nx)
+    assert abs(J - J_o) <= 1e-12 * abs(J_o)
+    # adjoint against the dual-number tangent through the oracle with the same dts
+    dJ_adj = float(np.sum(to_lex(mesh, grad) * v))
+    uK_t, duK = o.tangent_lw(u_lex, v, dts)
+    J_t, gJ_t = ke_and_grad_np(uK_t, w, 20.0 / mesh.nx)
+    dJ_tan = float(np.sum(gJ_t * duK))
+    assert abs(dJ_adj - dJ_tan) <= 1e-10 * abs(dJ_tan), (dJ_adj, dJ_tan)
+    # central differences of the device loop with the frozen dts
+    h = 1e-6
+    Jp = tlw.objective_lw(semi, to_dev(mesh, u_lex + h * v), nsteps, dts)
+    Jm = tlw.objective_lw(semi, to_dev(mesh, u_lex - h * v), nsteps, dts)
+    fd = (Jp - Jm) / (2 * h)
+    assert abs(dJ_adj - fd) <= 1e-6 * abs(fd), (dJ_adj, fd)
+    semi.close()
 
 
 def test_adjoint_rejects_unsupported(oracle_mod):

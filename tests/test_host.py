@@ -93,3 +93,22 @@ def test_set_dt_truncates():
     assert Semi.cache["dt"][0] == pytest.approx(0.1)
     set_t_and_iter(it, 0.1)
     assert it.t == pytest.approx(1.0) and it.iter == 1
+
+
+def test_gaussian_random_field_is_seeded_periodic_and_scaled():
+    """The differentiation configuration's perturbation: deterministic by seed,
+    periodic in both directions, standard deviation = amplitude over the
+    points, different for a different seed."""
+    import numpy as np
+    from trixilw_hip.adjoint import gaussian_random_field
+
+    rng = np.random.default_rng(0)
+    x, y = rng.uniform(-10, 10, 500), rng.uniform(-10, 10, 500)
+    f = gaussian_random_field(x, y, (20.0, 20.0), seed=2025, amplitude=1e-2)
+    assert np.array_equal(f, gaussian_random_field(x, y, (20.0, 20.0), seed=2025, amplitude=1e-2))
+    assert abs(np.std(f) - 1e-2) < 1e-15
+    raw = gaussian_random_field(np.concatenate([x, x + 20.0, x]), np.concatenate([y, y, y - 40.0]),
+                                (20.0, 20.0), seed=2025, amplitude=1.0)
+    assert np.allclose(raw[:500], raw[500:1000], atol=1e-12) and np.allclose(raw[:500], raw[1000:], atol=1e-12)
+    g = gaussian_random_field(x, y, (20.0, 20.0), seed=2026, amplitude=1e-2)
+    assert np.max(np.abs(f - g)) > 1e-3

@@ -84,6 +84,19 @@ struct tlw_ctx {
     double t_ms[3] = {0.0, 0.0, 0.0};
     int64_t t_n[3] = {0, 0, 0};
     // captured step sequences of tlw_advance, by step count (tlw_enable_graphs)
+    // caller-defined Dirichlet data (TLW_BV_TABLE): boundary face index per element face,
+    // face-node coordinates in table order, the uploaded table and its time levels (0: none yet)
+    int* d_bidx = nullptr;
+    long long n_bnodes = 0;
+    std::vector<double> bxy;
+    double* d_btab = nullptr;
+    size_t btab_cap = 0;
+    int btab_levels = 0;
+    // per-step dt record and prescribed dts of the device loop (tlw_record_dts, tlw_prescribe_dts)
+    double* d_dtlog = nullptr;
+    long long dtlog_cap = 0;
+    double* d_dtpresc = nullptr;
+    long long dtpresc_n = 0;
     bool graphs = false;
     std::map<int, hipGraphExec_t> graph_cache;
     hipStream_t graph_stream = nullptr;
@@ -136,7 +149,7 @@ void free_mesh(tlw_ctx* c) {
                     c->d_us,        c->d_auxU,  c->d_auxF,   c->d_partial, c->d_alpha_tmp, c->d_alpha,
                     c->d_vblk,      c->d_sblk,  c->d_etile,  c->d_rng,    c->d_state,   c->d_cur,
                     c->d_Ja,        c->d_ijn,   c->d_fnorm,  c->d_fxy,     c->d_snrm,      c->d_sqrk,
-                    c->d_fvb};
+                    c->d_fvb,       c->d_bidx,  c->d_btab};
     for (void* p : ptrs) (void)hipFree(p);
     if (c->h_state) (void)hipHostFree(c->h_state);
     c->d_nbr = nullptr;
@@ -149,6 +162,12 @@ void free_mesh(tlw_ctx* c) {
     c->h_state = nullptr;
     c->d_cur = nullptr;
     c->d_Ja = c->d_ijn = c->d_fnorm = c->d_fxy = c->d_snrm = c->d_sqrk = c->d_fvb = nullptr;
+    c->d_bidx = nullptr;
+    c->d_btab = nullptr;
+    c->btab_cap = 0;
+    c->btab_levels = 0;
+    c->n_bnodes = 0;
+    c->bxy.clear();
 }
 
 // Workgroup tables for the current tile: volume workgroups of ebv elements and
@@ -420,6 +439,41 @@ int upload_meshes(tlw_ctx* c, const tlw_mesh_desc* ms, int ninst, const tlw_curv
             HIPCHK(c, hipMemcpy(c->d_sqrk, sqrk.data(), sizeof(double) * sqrk.size(), hipMemcpyHostToDevice));
         }
     }
+    if (c->hconst.bc_kind == TLW_BV_TABLE) {
+        // boundary face nodes in table order: element, face (-x, +x, -y, +y), node
+        const int n = c->n;
+        std::vector<int> bidx(4 * (size_t)E, -1);
+        long long nb = 0;
+        for (size_t q = 0; q < 4 * (size_t)E; ++q)
+            if (nbr[q] < 0) bidx[q] = (int)nb++;
+        c->n_bnodes = nb * n;
+        c->bxy.assign(2 * (size_t)c->n_bnodes, 0.0);
+        for (int e = 0; e < E; ++e)
+            for (int f = 0; f < 4; ++f) {
+                const int b = bidx[4 * (size_t)e + f];
+                if (b < 0) continue;
+                for (int l = 0; l < n; ++l) {
+                    double x, y;
+                    if (c->curved) {
+                        x = fxy[((((size_t)e * 4 + f) * n + l) * 2)];
+                        y = fxy[((((size_t)e * 4 + f) * n + l) * 2) + 1];
+                    } else {  // the device's TreeMesh face-node coordinates
+                        const double half = 0.5 * length[e];
+                        if (f < 2) {
+                            x = center[2 * (size_t)e] + half * c->hconst.nodes[f == 0 ? 0 : n - 1];
+                            y = center[2 * (size_t)e + 1] + half * c->hconst.nodes[l];
+                        } else {
+                            x = center[2 * (size_t)e] + half * c->hconst.nodes[l];
+                            y = center[2 * (size_t)e + 1] + half * c->hconst.nodes[f == 2 ? 0 : n - 1];
+                        }
+                    }
+                    c->bxy[2 * ((size_t)b * n + l)] = x;
+                    c->bxy[2 * ((size_t)b * n + l) + 1] = y;
+                }
+            }
+        HIPCHK(c, hipMalloc(&c->d_bidx, sizeof(int) * bidx.size()));
+        HIPCHK(c, hipMemcpy(c->d_bidx, bidx.data(), sizeof(int) * bidx.size(), hipMemcpyHostToDevice));
+    }
     return build_tables(c);
 }
 
@@ -453,6 +507,9 @@ KArgs base_args(tlw_ctx* c) {
     a.snrm = c->d_snrm;
     a.fvb = c->d_fvb;
     a.sqrk = c->d_sqrk;
+    a.bidx = c->d_bidx;
+    a.btab = c->d_btab;
+    a.btab_levels = c->btab_levels;
     return a;
 }
 
@@ -533,12 +590,13 @@ void launch_volume(tlw_ctx* c, const KArgs& a, int kind) {
     else launch_volume_t<3>(c, a, kind);
 }
 void launch_finalize(tlw_ctx* c, int reduce, int use_vol) {
+    DtLog dl{c->d_dtlog, c->dtlog_cap, c->d_dtpresc, c->dtpresc_n};
     if (c->N == 4)
         hipLaunchKernelGGL(k_finalize<4>, dim3(c->ninst), dim3(kFinalizeBlock), 0, c->stream, c->d_state,
-                           c->d_partial, c->d_rng, use_vol, reduce, c->pv_off, c->d_partial + c->slot_off);
+                           c->d_partial, c->d_rng, use_vol, reduce, c->pv_off, c->d_partial + c->slot_off, dl);
     else
         hipLaunchKernelGGL(k_finalize<3>, dim3(c->ninst), dim3(kFinalizeBlock), 0, c->stream, c->d_state,
-                           c->d_partial, c->d_rng, use_vol, reduce, c->pv_off, c->d_partial + c->slot_off);
+                           c->d_partial, c->d_rng, use_vol, reduce, c->pv_off, c->d_partial + c->slot_off, dl);
 }
 template <int N> void launch_surface_t(tlw_ctx* c, const KArgs& a) {
     dim3 g(c->sgrid), b(kSurfBlock);
@@ -639,6 +697,8 @@ int rhs_common(tlw_ctx* c, int kind, const double* u, double* du, double t, doub
     if (!std::isfinite(dt) || !std::isfinite(t)) return fail(c, TLW_EINVAL, "t and dt must be finite");
     if (kind != VOL_LW && c->time_disc != TLW_MDRK)
         return fail(c, TLW_EINVAL, "rhs_mdrk needs a context created with TLW_MDRK");
+    if (c->hconst.bc_kind == TLW_BV_TABLE && c->has_boundary && c->btab_levels == 0)
+        return fail(c, TLW_EINVAL, "no boundary data uploaded (tlw_set_boundary_data)");
     KArgs a = base_args(c);
     a.u = u;
     a.us = c->d_us;
@@ -755,6 +815,16 @@ bool any_bad(const tlw_ctx* c) {
     return false;
 }
 
+// The device loop computes dt itself, so caller-defined boundary data must hold for
+// every time (one time level) there.
+int check_loop_boundary(tlw_ctx* c) {
+    if (c->hconst.bc_kind != TLW_BV_TABLE || !c->has_boundary) return TLW_OK;
+    if (c->btab_levels == 0) return fail(c, TLW_EINVAL, "no boundary data uploaded (tlw_set_boundary_data)");
+    if (c->btab_levels != 1)
+        return fail(c, TLW_ENOTSUP, "time-dependent boundary data needs the host loop (the device loop sets dt itself)");
+    return TLW_OK;
+}
+
 // message of the last failed context-free call on this thread (tlw_last_error(NULL))
 thread_local std::string g_noctx_err;
 
@@ -827,6 +897,8 @@ void tlw_destroy(tlw_ctx* c) {
     else (void)hipDeviceSynchronize();
     free_mesh(c);
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
+    (void)hipFree(c->d_dtlog);
+    (void)hipFree(c->d_dtpresc);
     (void)hipFree(c->d_const);
     delete c;
 }
@@ -907,6 +979,22 @@ int tlw_get_state(tlw_ctx* c, double* u, size_t len) {
     return TLW_OK;
 }
 
+int tlw_set_state_dev(tlw_ctx* c, const double* u) {
+    if (!c || !u) return fail(c, TLW_EINVAL, "null pointer");
+    for (int k = 0; k < c->ninst; ++k)
+        HIPCHK(c, hipMemcpyAsync(c->d_buf[c->host_cur[k]] + inst_offset(c, k), u + inst_offset(c, k),
+                                 sizeof(double) * inst_len(c, k), hipMemcpyDeviceToDevice, c->stream));
+    return TLW_OK;
+}
+
+int tlw_get_state_dev(tlw_ctx* c, double* u) {
+    if (!c || !u) return fail(c, TLW_EINVAL, "null pointer");
+    for (int k = 0; k < c->ninst; ++k)
+        HIPCHK(c, hipMemcpyAsync(u + inst_offset(c, k), c->d_buf[c->host_cur[k]] + inst_offset(c, k),
+                                 sizeof(double) * inst_len(c, k), hipMemcpyDeviceToDevice, c->stream));
+    return TLW_OK;
+}
+
 int tlw_state_ptr(tlw_ctx* c, double** u) {
     if (!c || !u) return TLW_EINVAL;
     for (int k = 1; k < c->ninst; ++k)
@@ -920,6 +1008,7 @@ int tlw_solve_init(tlw_ctx* c, double t0, double t_final, double cfl) {
     if (!c) return TLW_EINVAL;
     if (!std::isfinite(t0) || std::isnan(t_final) || !(cfl > 0.0))
         return fail(c, TLW_EINVAL, "t0 finite, t_final not NaN and cfl > 0 required");
+    if (int rcb = check_loop_boundary(c)) return rcb;
     HIPCHK(c, hipMemcpyAsync(c->d_cur, c->host_cur.data(), sizeof(int) * c->ninst, hipMemcpyHostToDevice,
                              c->stream));
     if (c->adaptive) drop_graphs(c);
@@ -950,6 +1039,7 @@ int tlw_solve_init_adaptive(tlw_ctx* c, double t0, double t_final, double dt_ini
         return fail(c, TLW_ENOTSUP,
                     "adaptive steps need the temporal-error estimate of the Euler LW kernel for polydeg 4");
     if (c->positivity) return fail(c, TLW_ENOTSUP, "adaptive steps are not combined with the positivity limiter");
+    if (int rcb = check_loop_boundary(c)) return rcb;
     HIPCHK(c, hipMemcpyAsync(c->d_cur, c->host_cur.data(), sizeof(int) * c->ninst, hipMemcpyHostToDevice,
                              c->stream));
     if (!c->adaptive) drop_graphs(c);
@@ -976,6 +1066,7 @@ int tlw_advance(tlw_ctx* c, int32_t max_steps, double* t_host, double* dt_host, 
     if (!c) return TLW_EINVAL;
     if (max_steps < 0) return fail(c, TLW_EINVAL, "max_steps < 0");
     if (max_steps > 0) {
+        if (int rcb = check_loop_boundary(c)) return rcb;
         if (c->graphs && !c->timing) {
             int rc = advance_graph(c, max_steps);
             if (rc) return rc;
@@ -1205,6 +1296,90 @@ int tlw_get_aux(tlw_ctx* c, int32_t which, double* out, size_t len) {
     return TLW_OK;
 }
 
+int tlw_boundary_nodes(tlw_ctx* c, int64_t* n_nodes, double* xy_host, size_t len) {
+    if (!c || !n_nodes) return fail(c, TLW_EINVAL, "null pointer");
+    if (c->hconst.bc_kind != TLW_BV_TABLE)
+        return fail(c, TLW_EINVAL, "context created without caller-defined boundary data (TLW_BV_TABLE)");
+    *n_nodes = c->n_bnodes;
+    if (!xy_host) return TLW_OK;
+    if (len != c->bxy.size()) return fail(c, TLW_EINVAL, "coordinate array length must be 2 * n_nodes");
+    std::memcpy(xy_host, c->bxy.data(), sizeof(double) * len);
+    return TLW_OK;
+}
+
+int tlw_set_boundary_data(tlw_ctx* c, const double* values, size_t len, int32_t time_levels) {
+    if (!c || !values) return fail(c, TLW_EINVAL, "null pointer");
+    if (c->hconst.bc_kind != TLW_BV_TABLE)
+        return fail(c, TLW_EINVAL, "context created without caller-defined boundary data (TLW_BV_TABLE)");
+    if (time_levels != 1 && time_levels != c->n + 1)
+        return fail(c, TLW_EINVAL, "time_levels must be 1 (steady) or polydeg + 2 (GLL time levels, then t)");
+    const size_t need = (size_t)c->n_bnodes * time_levels * 4;
+    if (len != need) return fail(c, TLW_EINVAL, "boundary data length must be 4 * time_levels * n_nodes");
+    for (size_t k = 0; k < len; k += 4)
+        if (!(values[k] > 0.0) || !std::isfinite(values[k]))
+            return fail(c, TLW_EINVAL, "boundary data: density must be finite and > 0");
+    if (need > c->btab_cap) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        drop_graphs(c);
+        (void)hipFree(c->d_btab);
+        c->d_btab = nullptr;
+        c->btab_cap = 0;
+        HIPCHK(c, hipMalloc(&c->d_btab, sizeof(double) * (need > 0 ? need : 4)));
+        c->btab_cap = need;
+    }
+    if (time_levels != c->btab_levels) drop_graphs(c);
+    if (need > 0)
+        HIPCHK(c, hipMemcpyAsync(c->d_btab, values, sizeof(double) * need, hipMemcpyHostToDevice, c->stream));
+    c->btab_levels = time_levels;
+    return TLW_OK;
+}
+
+int tlw_record_dts(tlw_ctx* c, int64_t capacity) {
+    if (!c) return TLW_EINVAL;
+    if (capacity < 0) return fail(c, TLW_EINVAL, "capacity < 0");
+    if (capacity > 0 && c->ninst != 1) return fail(c, TLW_ENOTSUP, "the dt record needs a single-instance context");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    drop_graphs(c);
+    (void)hipFree(c->d_dtlog);
+    c->d_dtlog = nullptr;
+    c->dtlog_cap = 0;
+    if (capacity > 0) {
+        HIPCHK(c, hipMalloc(&c->d_dtlog, sizeof(double) * (size_t)capacity));
+        HIPCHK(c, hipMemset(c->d_dtlog, 0, sizeof(double) * (size_t)capacity));
+        c->dtlog_cap = capacity;
+    }
+    return TLW_OK;
+}
+
+int tlw_get_dts(tlw_ctx* c, double* dts_host, int64_t n) {
+    if (!c || (!dts_host && n > 0)) return fail(c, TLW_EINVAL, "null pointer");
+    if (n < 0 || n > c->dtlog_cap) return fail(c, TLW_EINVAL, "n exceeds the dt record's capacity");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (n > 0) HIPCHK(c, hipMemcpy(dts_host, c->d_dtlog, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost));
+    return TLW_OK;
+}
+
+int tlw_prescribe_dts(tlw_ctx* c, const double* dts_host, int64_t n) {
+    if (!c || (!dts_host && n > 0)) return fail(c, TLW_EINVAL, "null pointer");
+    if (n < 0) return fail(c, TLW_EINVAL, "n < 0");
+    if (n > 0 && c->ninst != 1) return fail(c, TLW_ENOTSUP, "prescribed dts need a single-instance context");
+    if (n > 0 && c->adaptive) return fail(c, TLW_ENOTSUP, "prescribed dts replace CFL steps, not adaptive ones");
+    for (int64_t k = 0; k < n; ++k)
+        if (!(dts_host[k] > 0.0) || !std::isfinite(dts_host[k]))
+            return fail(c, TLW_EINVAL, "prescribed dts must be finite and > 0");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    drop_graphs(c);
+    (void)hipFree(c->d_dtpresc);
+    c->d_dtpresc = nullptr;
+    c->dtpresc_n = 0;
+    if (n > 0) {
+        HIPCHK(c, hipMalloc(&c->d_dtpresc, sizeof(double) * (size_t)n));
+        HIPCHK(c, hipMemcpy(c->d_dtpresc, dts_host, sizeof(double) * (size_t)n, hipMemcpyHostToDevice));
+        c->dtpresc_n = n;
+    }
+    return TLW_OK;
+}
+
 int tlw_max_dt_noalloc(const double* u, const double* inv_jac, int32_t n_elements, int32_t polydeg, double gamma,
                        double* max_dt, void* stream) {
     if (!u || !inv_jac || !max_dt) {
@@ -1238,7 +1413,7 @@ int create_common(int ninst, const tlw_mesh_desc* meshes, const tlw_basis_desc* 
     if (solver->time_discretization != TLW_LW && solver->time_discretization != TLW_MDRK) return TLW_EINVAL;
     if (solver->surface_flux_order != TLW_FLUX_TREE && solver->surface_flux_order != TLW_FLUX_STRUCTURED)
         return TLW_EINVAL;
-    if (solver->boundary_kind < TLW_BV_NONE || solver->boundary_kind > TLW_BV_SLIP_WALL) return TLW_EINVAL;
+    if (solver->boundary_kind < TLW_BV_NONE || solver->boundary_kind > TLW_BV_TABLE) return TLW_EINVAL;
     if (!(solver->gamma > 1.0)) return TLW_EINVAL;
     tlw_ctx* c = new tlw_ctx;
     c->N = basis->polydeg;

@@ -32,6 +32,7 @@
 namespace tlw {
 
 constexpr int kBcSlipWall = 4;  // boundary kind of the reflecting wall (TLW_BV_SLIP_WALL)
+constexpr int kBcTable = 5;     // caller-defined boundary data (TLW_BV_TABLE)
 constexpr int kTraceQ = 10;  // face record: jump state (4), normal flux (4), |v_n|, c
 // shock-capturing face record: + fn_low (4), blending factor, padding
 constexpr int kTraceQSC = 16;
@@ -163,6 +164,12 @@ struct KArgs {
     // face n-1 per row (ξ) and column (η) [E][n][4]
     const double* snrm;
     const double* sqrk;
+    // caller-defined Dirichlet data (bc_kind kBcTable): per element face the
+    // boundary face index or -1 [E][4], and the state at every boundary face node
+    // [face][n][levels][4]: levels 1 (steady) or n + 1 (GLL time levels, then t)
+    const int* bidx;
+    const double* btab;
+    int btab_levels;
     double t_arg, dt_arg;
     int use_state_dt;  // 1: t, dt from DevState (device loop); 0: from t_arg/dt_arg
     int step_mode;     // 1: out = u + dt * rhs; 0: out = rhs
@@ -338,6 +345,21 @@ __device__ __forceinline__ void load4(const double* p, double* u) {
     const double2 a = p2[0], b = p2[1];
     u[0] = a.x; u[1] = a.y; u[2] = b.x; u[3] = b.y;
 }
+
+// Boundary state of face node l of element e's face at GLL time level q (q = n:
+// the time t itself): the caller's table (kBcTable) or a built-in function.
+template <int n>
+__device__ __forceinline__ void boundary_state(const DevConst* c, const int* bidx, const double* btab, int levels,
+                                               int e, int face, int l, int q, double x, double y, double t,
+                                               double* u) {
+    if (c->bc_kind == kBcTable) {
+        const size_t row = ((size_t)bidx[4 * (size_t)e + face] * n + l) * levels + (levels == 1 ? 0 : q);
+        load4(btab + 4 * row, u);
+    } else {
+        boundary_value(c, x, y, t, u);
+    }
+}
+
 __device__ __forceinline__ void store4(double* p, const double* u) {
     double2* p2 = reinterpret_cast<double2*>(p);
     p2[0] = make_double2(u[0], u[1]);
@@ -2343,14 +2365,14 @@ __device__ inline void boundary_record(const KArgs& a, int e, int face, int l, d
     for (int q = 0; q < n; ++q) {
         const double ts = t + 0.5 * dts * (c->nodes[q] + 1.0);
         double ub[4], fb[4];
-        boundary_value(c, x, y, ts, ub);
+        boundary_state<n>(c, a.bidx, a.btab, a.btab_levels, e, face, l, q, x, y, ts, ub);
         euler_flux_dir(ub, gm1, orientation, fb);
         for (int v = 0; v < 4; ++v) {
             Uo[v] += 0.5 * sf * ub[v] * c->weights[q];
             Fo[v] += 0.5 * sf * fb[v] * c->weights[q];
         }
     }
-    boundary_value(c, x, y, t, uo);
+    boundary_state<n>(c, a.bidx, a.btab, a.btab_levels, e, face, l, n, x, y, t, uo);
     const bool tree = c->flux_order == 0;
     double av1, av2, cs;
     speed_parts2(tree ? Uo : uo, c->gamma, av1, av2, cs);
@@ -2376,7 +2398,7 @@ __device__ inline void boundary_record_curved(const KArgs& a, int e, int face, i
     for (int q = 0; q < n; ++q) {
         const double ts = t + 0.5 * dt * (c->nodes[q] + 1.0);
         double ub[4];
-        boundary_value(c, x, y, ts, ub);
+        boundary_state<n>(c, a.bidx, a.btab, a.btab_levels, e, face, l, q, x, y, ts, ub);
         const double rho = ub[0], v1 = ub[1] / rho, v2 = ub[2] / rho;
         const double p = gm1 * (ub[3] - 0.5 * (ub[1] * v1 + ub[2] * v2));
         const double vn = v1 * nf[0] + v2 * nf[1], rvn = rho * vn;
@@ -2386,7 +2408,7 @@ __device__ inline void boundary_record_curved(const KArgs& a, int e, int face, i
             Fo[v] += 0.5 * fb[v] * c->weights[q];
         }
     }
-    boundary_value(c, x, y, t, uo);
+    boundary_state<n>(c, a.bidx, a.btab, a.btab_levels, e, face, l, n, x, y, t, uo);
     const double ir = rcp_nr(Uo[0]);
     const double b1 = Uo[1] * ir, b2 = Uo[2] * ir;
     const double pb = gm1 * (Uo[3] - 0.5 * (Uo[1] * b1 + Uo[2] * b2));
@@ -2851,13 +2873,24 @@ __device__ inline void adaptive_control(DevState* st, double total, double k) {
 // then compute_dt / set_dt! / set_t_and_iter! / isfinished on the instance's
 // state (lwfr.jl:76-103, 146-148, 176-190; Trixi max_dt).
 constexpr int kFinalizeBlock = 1024;
+// Per-step dt record and prescription of a single-instance device loop: the dt
+// of step k (k = the iteration count since the last solve_init) is written to
+// log[k] while k < log_cap, and taken from presc[k] instead of cfl · max_dt
+// while k < presc_n (the truncation at the final time still applies).
+struct DtLog {
+    double* log;
+    long long log_cap;
+    const double* presc;
+    long long presc_n;
+};
 // With pv_off > 0 the viscous maxima (pv_off entries further on) give the
 // parabolic bound 1/((λa + λv)·2(2N+1)), λv = 2(2N+1)·max J⁻² max(4μ/(3ρ), μκ/(ρPr)),
 // and max_dt is the smaller of the two (compute_dt for
 // SemidiscretizationHyperbolicParabolic, lwfr.jl:192-210).
 template <int N>
 __global__ __launch_bounds__(kFinalizeBlock) void k_finalize(DevState* st_all, const double* partial, const int4* rng,
-                                                           int use_vol, int reduce, int pv_off, double* slots) {
+                                                           int use_vol, int reduce, int pv_off, double* slots,
+                                                           DtLog dl) {
     constexpr int n = N + 1;
     __shared__ double s[kFinalizeBlock], sv[kFinalizeBlock];
     DevState* st = st_all + blockIdx.x;
@@ -2926,8 +2959,11 @@ __global__ __launch_bounds__(kFinalizeBlock) void k_finalize(DevState* st_all, c
     }
     const double tf = st->t_final;
     double dt = st->cfl * max_dt;  // compute_dt
+    const long long it = st->iter;
+    if (dl.presc && it < dl.presc_n) dt = dl.presc[it];  // prescribed dt of step `it`
     if (t + dt > tf) dt = tf - t;  // set_dt!
     st->dt = dt;
+    if (dl.log && it < dl.log_cap) dl.log[it] = dt;
     st->finished = julia_isapprox(t, tf);
 }
 

@@ -5,10 +5,10 @@ SemidiscretizationHyperbolic, rhs!/rhs_mdrk1!/rhs_mdrk2!, solve_lwfr) over the
 C ABI of libtrixilw_hip.so.  Importing this package does not load the
 library; the first call that needs the GPU does, and fails if it is missing.
 """
-from .adjoint import adjoint_step, gradient_lw, kinetic_energy
+from .adjoint import adjoint_step, gaussian_random_field, gradient_lw, kinetic_energy, objective_lw
 from .basis import LobattoLegendreBasis
 from .batch import SemidiscretizationBatch
-from .equations import (Adiabatic, BoundaryConditionNavierStokesWall, CompressibleEulerEquations2D,
+from .equations import (Adiabatic, BoundaryConditionDirichlet, BoundaryConditionNavierStokesWall, CompressibleEulerEquations2D,
                         CompressibleNavierStokesDiffusion2D, ConstantState, Isothermal, NoSlip,
                         FourQuadrantRiemann, GradientVariablesConservative, GradientVariablesPrimitive,
                         IsentropicVortex, LinearScalarAdvectionEquation2D, boundary_condition_slip_wall_horizontal,
@@ -27,7 +27,7 @@ from .semidiscretization import (DGSEM, LW, MDRK, Adaptive, CFLBased, FirstOrder
 __all__ = [
     "LobattoLegendreBasis", "CompressibleEulerEquations2D", "ConstantState", "FourQuadrantRiemann",
     "boundary_condition_slip_wall_vertical", "boundary_condition_slip_wall_horizontal", "NoSlip", "Adiabatic", "Isothermal",
-    "BoundaryConditionNavierStokesWall",
+    "BoundaryConditionNavierStokesWall", "BoundaryConditionDirichlet",
     "IsentropicVortex", "LinearScalarAdvectionEquation2D", "LWIntegrator", "LWSolution", "compute_dt", "isapprox",
     "isfinished", "perform_step", "set_dt", "set_t_and_iter", "solve_lwfr", "TreeMesh",
     "StructuredMesh", "coordinates2mapping", "DGSEM", "LW", "MDRK",
@@ -35,7 +35,7 @@ __all__ = [
     "get_time_discretization", "lw_fourier_cfl", "max_dt_noalloc", "rhs", "rhs_mdrk1", "rhs_mdrk2", "semidiscretize", "trixi2lw",
     "FirstOrderReconstruction", "IndicatorHennemannGassner", "MUSCLHancockReconstruction",
     "VolumeIntegralFRShockCapturing", "density_pressure", "PositivityPreservingLimiterZhangShu", "density",
-    "pressure", "adjoint_step", "gradient_lw", "kinetic_energy",
+    "pressure", "adjoint_step", "gradient_lw", "kinetic_energy", "gaussian_random_field", "objective_lw",
     "SemidiscretizationBatch", "SemidiscretizationHyperbolicParabolic", "CompressibleNavierStokesDiffusion2D",
     "GradientVariablesConservative", "GradientVariablesPrimitive",
 ]

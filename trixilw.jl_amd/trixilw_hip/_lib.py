@@ -15,7 +15,7 @@ LIB_PATH = Path(os.environ.get("TRIXILW_HIP_LIBRARY") or Path(__file__).resolve(
 TLW_OK, TLW_EINVAL, TLW_EHIP, TLW_EADMISSIBLE, TLW_ENOTSUP = 0, 1, 2, 3, 4
 TLW_LW, TLW_MDRK = 0, 1
 TLW_FLUX_TREE, TLW_FLUX_STRUCTURED = 0, 1
-TLW_BV_NONE, TLW_BV_VORTEX, TLW_BV_CONSTANT, TLW_BV_RIEMANN4, TLW_BV_SLIP_WALL = 0, 1, 2, 3, 4
+TLW_BV_NONE, TLW_BV_VORTEX, TLW_BV_CONSTANT, TLW_BV_RIEMANN4, TLW_BV_SLIP_WALL, TLW_BV_TABLE = 0, 1, 2, 3, 4, 5
 TLW_AUX_U, TLW_AUX_F, TLW_AUX_ALPHA = 0, 1, 2
 TLW_FLAG_KEEP_AUX = 1
 TLW_PAR_NAVIER_STOKES = 1
@@ -29,7 +29,8 @@ EXPORTED = (
     "tlw_get_elements_per_workgroup", "tlw_autotune", "tlw_set_positivity_limiter", "tlw_apply_positivity_limiter",
     "tlw_adjoint_lw", "tlw_create_batch", "tlw_n_instances", "tlw_batch_status", "tlw_enable_graphs",
     "tlw_create_parabolic", "tlw_solve_init_adaptive", "tlw_rejected_steps", "tlw_create_curved",
-    "tlw_create_curved_parabolic", "tlw_max_dt_noalloc",
+    "tlw_create_curved_parabolic", "tlw_max_dt_noalloc", "tlw_record_dts", "tlw_get_dts", "tlw_prescribe_dts",
+    "tlw_set_state_dev", "tlw_get_state_dev", "tlw_boundary_nodes", "tlw_set_boundary_data",
 )
 
 
@@ -192,6 +193,20 @@ def lib():
     L.tlw_adjoint_lw.argtypes = [vp, vp, vp, vp, C.c_double, C.c_double]
     L.tlw_autotune.restype = C.c_int
     L.tlw_autotune.argtypes = [vp, i32, C.POINTER(i32), dp]
+    L.tlw_boundary_nodes.restype = C.c_int
+    L.tlw_boundary_nodes.argtypes = [vp, C.POINTER(C.c_int64), dp, sz]
+    L.tlw_set_boundary_data.restype = C.c_int
+    L.tlw_set_boundary_data.argtypes = [vp, dp, sz, i32]
+    L.tlw_record_dts.restype = C.c_int
+    L.tlw_record_dts.argtypes = [vp, C.c_int64]
+    L.tlw_get_dts.restype = C.c_int
+    L.tlw_get_dts.argtypes = [vp, dp, C.c_int64]
+    L.tlw_prescribe_dts.restype = C.c_int
+    L.tlw_prescribe_dts.argtypes = [vp, dp, C.c_int64]
+    L.tlw_set_state_dev.restype = C.c_int
+    L.tlw_set_state_dev.argtypes = [vp, vp]
+    L.tlw_get_state_dev.restype = C.c_int
+    L.tlw_get_state_dev.argtypes = [vp, vp]
     L.tlw_max_dt_noalloc.restype = C.c_int
     L.tlw_max_dt_noalloc.argtypes = [vp, vp, i32, i32, C.c_double, vp, vp]
     _lib = L

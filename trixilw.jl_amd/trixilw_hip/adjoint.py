@@ -2,9 +2,9 @@
 
 ``adjoint_step`` is one reverse step, λ = λ' + dt (∂rhs/∂u)ᵀ λ' (tlw_adjoint_lw).
 ``gradient_lw`` runs solve_lwfr's LW loop (compute_dt -> rhs! -> u += dt du,
-src/solvers/lwfr.jl:212-230) forward with checkpoints every ``checkpoint_every``
-steps, then sweeps back segment by segment: each segment is recomputed from its
-checkpoint and reversed with adjoint steps.  The CFL dts of the forward run are
+src/solvers/lwfr.jl:212-230) forward on the device with checkpoints every
+``checkpoint_every`` steps, then sweeps back segment by segment: each segment is
+recomputed from its checkpoint and reversed with adjoint steps.  The CFL dts of the forward run are
 frozen, so the gradient is that of the fixed-dt map u0 -> u_K (the max in
 compute_dt is not differentiable); the dts are returned for the same reason.
 """
@@ -56,29 +56,95 @@ def kinetic_energy(semi, u):
     return J, g
 
 
+def gaussian_random_field(x, y, period, seed, amplitude=1e-2, modes=8, correlation=0.15):
+    """Seeded periodic Gaussian random field at the points (x, y): a sum of
+    Fourier modes k = (k1, k2), |k1|, |k2| <= modes, with independent N(0, 1)
+    cosine and sine coefficients (numpy PCG64 via default_rng(seed)) weighted
+    by the Gaussian spectrum exp(-(2π|k| correlation)² / 2) of a field whose
+    correlation length is `correlation` times the period, scaled so that its
+    standard deviation over the points is `amplitude`."""
+    rng = np.random.default_rng(seed)
+    Lx, Ly = period
+    ks = np.arange(-modes, modes + 1)
+    f = np.zeros(np.shape(x))
+    for k1 in ks:
+        for k2 in ks:
+            a, b = rng.standard_normal(2)
+            if k1 == 0 and k2 == 0:
+                continue
+            w = np.exp(-0.5 * (2.0 * np.pi * correlation) ** 2 * (k1 * k1 + k2 * k2))
+            ph = 2.0 * np.pi * (k1 * np.asarray(x) / Lx + k2 * np.asarray(y) / Ly)
+            f = f + w * (a * np.cos(ph) + b * np.sin(ph))
+    return amplitude * f / np.std(f)
+
+
+def _forward_device(semi, u0, nsteps, cfl, t0, dts, checkpoint_every):
+    """solve_lwfr's LW loop on the device (tlw_solve_init / tlw_advance) from
+    u0, nsteps steps, the state copied out every checkpoint_every steps; dts
+    prescribed or recorded by the device.  Returns (u_K, dts, checkpoints)."""
+    torch = _torch()
+    L, h = _lib.lib(), semi.handle
+    semi._bind_stream()
+    dp = C.POINTER(C.c_double)
+    if dts is None:
+        _lib.check(L.tlw_prescribe_dts(h, None, 0), h)
+        _lib.check(L.tlw_record_dts(h, nsteps), h)
+    else:
+        d = np.ascontiguousarray(dts[:nsteps], dtype=np.float64)
+        _lib.check(L.tlw_record_dts(h, 0), h)
+        _lib.check(L.tlw_prescribe_dts(h, d.ctypes.data_as(dp), d.size), h)
+    try:
+        _lib.check(L.tlw_set_state_dev(h, C.c_void_p(u0.data_ptr())), h)
+        _lib.check(L.tlw_solve_init(h, float(t0), float("inf"), float(cfl)), h)
+        checkpoints = {}
+        it = 0
+        while True:
+            if checkpoint_every and it % checkpoint_every == 0 and it < nsteps:
+                c = torch.empty_like(u0)
+                _lib.check(L.tlw_get_state_dev(h, C.c_void_p(c.data_ptr())), h)
+                checkpoints[it] = c
+            if it >= nsteps:
+                break
+            k = min(checkpoint_every or nsteps, nsteps - it)
+            _lib.check(L.tlw_advance(h, k, None, None, None, None), h)
+            it += k
+        uK = torch.empty_like(u0)
+        _lib.check(L.tlw_get_state_dev(h, C.c_void_p(uK.data_ptr())), h)
+        if dts is None:
+            rec = np.zeros(nsteps)
+            _lib.check(L.tlw_get_dts(h, rec.ctypes.data_as(dp), nsteps), h)
+            dts = [float(v) for v in rec]
+        torch.cuda.current_stream(u0.device).synchronize()
+    finally:
+        L.tlw_record_dts(h, 0)
+        L.tlw_prescribe_dts(h, None, 0)
+    return uK, list(dts[:nsteps]), checkpoints
+
+
 def gradient_lw(semi, u0, nsteps, cfl, objective=kinetic_energy, checkpoint_every=20, t0=0.0, dts=None):
     """(J(u_K), ∇_{u0} J, u_K, dts) for K = nsteps LW steps from u0.
 
-    ``dts`` (optional) prescribes the steps; by default dt_k = cfl · max_dt(u_k)
-    as compute_dt does, and the dts used are returned."""
+    The forward sweep is solve_lwfr's loop on the device (no host
+    synchronisation inside a checkpoint segment), keeping the state every
+    ``checkpoint_every`` steps; ``dts`` (optional) prescribes the steps, by
+    default dt_k = cfl · max_dt(u_k) as compute_dt does on the device, and the
+    dts used are returned.  The reverse sweep recomputes each segment from its
+    checkpoint (rhs! + update, stream-ordered) and applies the adjoint steps."""
     torch = _torch()
     if checkpoint_every < 1:
         raise ValueError("checkpoint_every >= 1")
-    du = torch.empty_like(u0)
-    checkpoints = {}
-    dts = list(dts) if dts is not None else []
+    if dts is not None and len(dts) < nsteps:
+        raise ValueError("dts must hold nsteps values")
+    semi._check_state(u0, "u0")
+    u, dts, checkpoints = _forward_device(semi, u0, nsteps, cfl, t0, dts, checkpoint_every)
     times = []
-    u, t = u0.clone(), float(t0)
-    for k in range(nsteps):
-        if k % checkpoint_every == 0:
-            checkpoints[k] = u.clone()
-        if k >= len(dts):
-            dts.append(cfl * semi.max_dt(u))
+    t = float(t0)
+    for k in range(nsteps):  # set_t_and_iter!: t accumulates the dts in order
         times.append(t)
-        u = lw_step(u, semi, t, dts[k], du)
         t += dts[k]
     J, lam = objective(semi, u)
     lam_prev = torch.empty_like(lam)
+    du = torch.empty_like(u0)
     for seg in sorted(checkpoints, reverse=True):
         end = min(seg + checkpoint_every, nsteps)
         states = [checkpoints[seg]]
@@ -88,3 +154,11 @@ def gradient_lw(semi, u0, nsteps, cfl, objective=kinetic_energy, checkpoint_ever
             adjoint_step(lam_prev, states[k - seg], lam, semi, times[k], dts[k])
             lam, lam_prev = lam_prev, lam
     return J, lam, u, dts
+
+
+def objective_lw(semi, u0, nsteps, dts, objective=kinetic_energy, t0=0.0):
+    """J(u_K) of the device loop with prescribed dts (no gradient): the
+    forward map a finite-difference check differentiates."""
+    # the CFL number only enters the first (unused) CFL reduction: every dt is prescribed
+    u, _, _ = _forward_device(semi, u0, nsteps, 1.0, t0, dts, 0)
+    return objective(semi, u)[0]

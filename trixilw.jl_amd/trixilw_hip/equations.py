@@ -15,7 +15,32 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-BV_NONE, BV_VORTEX, BV_CONSTANT, BV_RIEMANN4, BV_SLIP_WALL = 0, 1, 2, 3, 4
+BV_NONE, BV_VORTEX, BV_CONSTANT, BV_RIEMANN4, BV_SLIP_WALL, BV_TABLE = 0, 1, 2, 3, 4, 5
+
+
+class BoundaryConditionDirichlet:
+    """Trixi BoundaryConditionDirichlet(boundary_value_function) with the LW
+    time average of src/equations/equations.jl:6-37, for any function:
+    ``boundary_value_function(x, t, equations)`` takes x = (x, y) as numpy
+    arrays and returns the conservative states [..., 4].  The host evaluates it
+    at the boundary face nodes and uploads the values before each rhs call
+    (TLW_BV_TABLE); ``steady=True`` declares a function that does not depend on
+    t: it is evaluated once and the device loop can then run with it."""
+
+    kind = BV_TABLE
+    params = ()
+
+    def __init__(self, boundary_value_function, steady=False):
+        if not callable(boundary_value_function):
+            raise TypeError("boundary_value_function must be callable")
+        self.boundary_value_function = boundary_value_function
+        self.steady = bool(steady)
+
+    def values(self, x, y, t, equations=None):
+        u = np.asarray(self.boundary_value_function((x, y), t, equations), dtype=np.float64)
+        if u.shape != np.shape(x) + (4,):
+            raise ValueError(f"boundary_value_function must return shape {np.shape(x) + (4,)}, got {u.shape}")
+        return u
 
 
 class _SlipWall:

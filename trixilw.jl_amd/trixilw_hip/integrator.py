@@ -164,7 +164,9 @@ def solve_lwfr(lw_update: LWUpdate, callbacks=(), dt_initial=1e-6, tolerances=No
             cb.affect(integrator)
     dev_limiters = all(isinstance(lim, PositivityPreservingLimiterZhangShu) for lim in limiters) and len(limiters) <= 1
     if device_loop is None:
-        device_loop = not callbacks and dev_limiters
+        # time-dependent caller-defined boundary data is evaluated by the host for each rhs
+        host_bc = getattr(semi, "_bc_table", None) is not None and not semi._bc_table.steady
+        device_loop = not callbacks and dev_limiters and not host_bc
     if device_loop:
         if callbacks or not dev_limiters:
             raise NotImplementedError("the device loop runs without callbacks and with at most one positivity limiter")

@@ -214,6 +214,49 @@ class SemidiscretizationHyperbolic:
         torch.cuda.init()
         self._create(md, basis, sd, h)
         self._h = h
+        self._setup_boundary_data()
+
+    def _setup_boundary_data(self):
+        """Caller-defined Dirichlet data: the boundary face-node coordinates in the
+        library's table order, and the steady values uploaded once."""
+        from .equations import BoundaryConditionDirichlet, boundary_conditions_kind
+
+        bcd = boundary_conditions_kind(self.boundary_conditions)
+        self._bc_table = bcd if isinstance(bcd, BoundaryConditionDirichlet) else None
+        if self._bc_table is None:
+            return
+        L = _lib.lib()
+        nb = C.c_int64()
+        _lib.check(L.tlw_boundary_nodes(self._h, C.byref(nb), None, 0), self._h)
+        xy = np.zeros((nb.value, 2))
+        _lib.check(L.tlw_boundary_nodes(self._h, C.byref(nb), _dptr(xy), xy.size), self._h)
+        self._bxy = xy
+        if self._bc_table.steady and nb.value > 0:
+            self._upload_boundary(np.ascontiguousarray(self._bc_table.values(xy[:, 0], xy[:, 1], 0.0,
+                                                                             self.equations)), 1)
+
+    def _upload_boundary(self, vals, levels):
+        vals = np.ascontiguousarray(vals, dtype=np.float64)
+        self._bind_stream()
+        _lib.check(_lib.lib().tlw_set_boundary_data(self._h, _dptr(vals), vals.size, int(levels)), self._h)
+
+    def _before_rhs(self, t, dt, stage):
+        """Per-call boundary data for the rhs about to run: the function at the
+        n GLL time levels of [t, t + s dt] (s = 0.5 for the first MDRK stage on a
+        TreeMesh, the reference's scaling_factor) and at t."""
+        bc = getattr(self, "_bc_table", None)
+        if bc is None or bc.steady or len(self._bxy) == 0:
+            return
+        nodes = self.solver.basis.nodes
+        n = len(nodes)
+        s = 0.5 if (stage == 1 and not getattr(self.mesh, "curved", False)) else 1.0
+        dts = s * dt
+        x, y = self._bxy[:, 0], self._bxy[:, 1]
+        vals = np.empty((len(x), n + 1, 4))
+        for q in range(n):
+            vals[:, q] = bc.values(x, y, t + 0.5 * dts * (nodes[q] + 1.0), self.equations)
+        vals[:, n] = bc.values(x, y, t, self.equations)
+        self._upload_boundary(vals, n + 1)
 
     def _create(self, md, basis, sd, h):
         if getattr(self.mesh, "curved", False):
@@ -447,11 +490,12 @@ class SemidiscretizationHyperbolicParabolic(SemidiscretizationHyperbolic):
                                     "error)")
 
 
-def _rhs_call(fn, du_ode, u_ode, semi, t):
+def _rhs_call(fn, du_ode, u_ode, semi, t, stage=0):
     semi._check_state(u_ode, "u")
     semi._check_state(du_ode, "du")
     semi._bind_stream()
     dt = semi.cache["dt"][0]
+    semi._before_rhs(float(t), float(dt), stage)
     _lib.check(fn(semi.handle, C.c_void_p(u_ode.data_ptr()), C.c_void_p(du_ode.data_ptr()), float(t), float(dt)),
                semi.handle)
 
@@ -463,12 +507,12 @@ def rhs(du_ode, u_ode, semi, t, tolerances=None):
 
 def rhs_mdrk1(du_ode, u_ode, semi, t, tolerances=None):
     """rhs_mdrk1!(du_ode, u_ode, semi, t): first MDRK stage."""
-    _rhs_call(_lib.lib().tlw_rhs_mdrk1, du_ode, u_ode, semi, t)
+    _rhs_call(_lib.lib().tlw_rhs_mdrk1, du_ode, u_ode, semi, t, stage=1)
 
 
 def rhs_mdrk2(du_ode, u_ode, semi, t, tolerances=None):
     """rhs_mdrk2!(du_ode, u_ode, semi, t): second MDRK stage; reads us from the cache."""
-    _rhs_call(_lib.lib().tlw_rhs_mdrk2, du_ode, u_ode, semi, t)
+    _rhs_call(_lib.lib().tlw_rhs_mdrk2, du_ode, u_ode, semi, t, stage=2)
 
 
 @dataclass
