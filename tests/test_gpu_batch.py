@@ -59,3 +59,40 @@ def test_graph_replay_matches_launches(oracle_mod):
     assert np.array_equal(u_a, u_b)
     assert np.array_equal(st_a["t"], st_b["t"])
     batch.close()
+
+
+def test_c4_draw_every_instance_matches_oracle(oracle_mod):
+    """The batching configuration's own 64-instance draw (seed 2024: meshes
+    16²..128², polydeg 3/4, Mach U[0.2,0.8], strength U[2,5]), each polydeg
+    group one batch context as the benchmark runs it (Morton order, graph
+    replay): every instance after 10 steps matches its own oracle solve,
+    state and time, ≤ 1e-12."""
+    import torch
+    import trixilw_hip as tlw
+
+    torch_mod()
+    K = 10
+    levels, degs, machs, strengths = tlw.draw_vortex_ensemble(64, 2024)
+    assert sorted(set(levels.tolist())) == [4, 5, 6, 7] and sorted(set(degs.tolist())) == [3, 4]
+    eq = tlw.CompressibleEulerEquations2D(1.4)
+    checked = 0
+    for N in (3, 4):
+        idx = np.nonzero(degs == N)[0]
+        meshes = [tlw.TreeMesh((-10.0, -10.0), (10.0, 10.0), int(levels[k]), ordering="morton") for k in idx]
+        ics = [tlw.IsentropicVortex(strength=float(strengths[k]), mach=float(machs[k])) for k in idx]
+        solver = tlw.DGSEM(N)
+        batch = tlw.SemidiscretizationBatch(meshes, tlw.LW(), eq, ics, solver)
+        cfl = tlw.trixi2lw(0.98, solver)
+        u, status = batch.solve(batch.compute_coefficients(), cfl, K, graphs=True, stream=torch.cuda.Stream())
+        assert np.all(status["iter"] == K)
+        for k, (mesh, ic) in enumerate(zip(meshes, ics)):
+            nx = mesh.nx
+            o = oracle_mod.Oracle(N, nx, nx, -10.0, -10.0, 20.0 / nx)
+            x, y = o.node_coords()
+            u_ref, t_ref, _ = o.solve(ic(x, y, 0.0), 0.0, 1e9, cfl, max_steps=K)
+            got = batch.split(u)[k][mesh.lexicographic_permutation()]
+            assert abs(status["t"][k] - t_ref) <= 1e-13 * t_ref, (N, k)
+            assert_close(got, u_ref, what=f"N={N} instance {k} ({nx}x{nx})")
+            checked += 1
+        batch.close()
+    assert checked == 64

@@ -179,6 +179,81 @@ def test_curved_ns_free_stream_and_conservation_at_scale():
     semi.close()
 
 
+def _tgv_semi(tlw, nx, metrics_from=None):
+    """The curved Navier-Stokes configuration's setup: MDRK N=3, Taylor-Green
+    vortex (U 1, ρ0 1, Mach 0.1) on [0,2]² warped by amplitude 0.1, μ = 0.001
+    (Re 1000), Pr 0.72."""
+    eq = tlw.CompressibleEulerEquations2D(GAMMA)
+    eqp = tlw.CompressibleNavierStokesDiffusion2D(eq, mu=1e-3, Prandtl=PR,
+                                                 gradient_variables=tlw.GradientVariablesConservative())
+    mesh = tlw.StructuredMesh((nx, nx), warp_mapping(L, 0.1), ordering="strips")
+    solver = tlw.DGSEM(3, volume_integral=tlw.VolumeIntegralFR(tlw.MDRK()))
+    if metrics_from is not None:
+        mesh.set_metrics(solver.basis, metrics_from.aux("contravariant_vectors"), metrics_from.aux("inverse_jacobian"))
+    ic = tlw.TaylorGreenVortex(U=1.0, rho0=1.0, mach=0.1, L=L)
+    return tlw.SemidiscretizationHyperbolicParabolic(mesh, tlw.MDRK(), (eq, eqp), ic, solver), mesh, ic, solver
+
+
+def test_c2_config_tgv_mdrk_parity(oracle_mod):
+    """The curved Navier-Stokes configuration's parameters (warp 0.1, Re 1000,
+    MDRK N=3, Taylor-Green vortex, CFL 0.5) on 32² for 50 device-loop steps,
+    strips element order, against the oracle's solve ≤ 1e-12."""
+    import trixilw_hip as tlw
+
+    torch_mod()
+    nx, K, cfl = 32, 50, 0.5
+    o = oracle_mod.Oracle(3, nx, nx, 0.0, 0.0, L / nx, gamma=GAMMA, mapping=(oracle_mod.MAP_WARP, (L, 0.1)),
+                          parabolic="navier_stokes", mu=1e-3, prandtl=PR)
+    semi, mesh, ic, solver = _tgv_semi(tlw, nx, metrics_from=o)
+    x, y = o.node_coords()
+    u_lex = ic(x, y)
+    u_ref, t_ref, _ = o.solve(u_lex, 0.0, 1e9, cfl, max_steps=K, mdrk=True)
+    upd = tlw.semidiscretize(semi, tlw.MDRK(), (0.0, 1e9))
+    upd.u0.copy_(to_dev(mesh, u_lex))
+    sol = tlw.solve_lwfr(upd, (), 1e-6, None, time_step_computation=tlw.CFLBased(cfl), max_steps=K)
+    assert sol.stats.naccept == K
+    assert abs(sol.t - t_ref) <= 1e-13 * t_ref
+    assert_close(to_lex(mesh, sol.u), u_ref, what="TGV MDRK 32² x 50 steps")
+    semi.close()
+
+
+def test_c2_config_128x128_300_steps_properties():
+    """The configuration at its full size (128², 300 MDRK steps, the host's
+    own metric terms, as the benchmark runs it): the state stays finite,
+    Σ w_i w_j J u of every variable is conserved to summation round-off, and
+    the kinetic energy decays monotonically (viscosity, no forcing)."""
+    import trixilw_hip as tlw
+
+    torch = torch_mod()
+    semi, mesh, ic, solver = _tgv_semi(tlw, 128)
+    _, ijn = mesh.metrics(solver.basis)
+    w = np.outer(solver.basis.weights, solver.basis.weights)[None] / ijn
+    wt = torch.from_numpy(np.ascontiguousarray(w)).cuda()
+    upd = tlw.semidiscretize(semi, tlw.MDRK(), (0.0, 1e9))
+    u = upd.u0.clone()
+    m0 = torch.einsum("eji,ejiv->v", wt, u)
+    scale = torch.einsum("eji,ejiv->v", wt, u.abs())
+
+    def kinetic(v):
+        return float(torch.einsum("eji,eji->", wt, 0.5 * (v[..., 1] ** 2 + v[..., 2] ** 2) / v[..., 0]))
+
+    ke = [kinetic(u)]
+    t = 0.0
+    for chunk in range(6):  # 6 x 50 steps, each chunk a fresh solve from the last state
+        upd = tlw.LWUpdate(upd.f, u, upd.soln_arrays, (t, 1e9), semi)
+        sol = tlw.solve_lwfr(upd, (), 1e-6, None, time_step_computation=tlw.CFLBased(0.5), max_steps=50)
+        assert sol.stats.naccept == 50
+        u, t = sol.u, sol.t
+        assert torch.isfinite(u).all()
+        ke.append(kinetic(u))
+    m1 = torch.einsum("eji,ejiv->v", wt, u)
+    assert torch.all((m1 - m0).abs() <= 1e-12 * scale), (m0, m1)
+    assert all(b < a for a, b in zip(ke, ke[1:])), ke
+    # Re 1000 over t ≈ 300 dt: a small but clearly resolved decay
+    assert 1e-6 < 1.0 - ke[-1] / ke[0] < 0.5, ke
+    semi.close()
+
+
 def test_curved_ns_rejects_unsupported(oracle_mod):
     import trixilw_hip as tlw
     from trixilw_hip import _lib

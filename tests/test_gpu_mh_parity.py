@@ -14,12 +14,12 @@ pytestmark = pytest.mark.gpu
 MH = "muscl_hancock"
 
 
-def riemann_setup(oracle_mod, N, level, td="lw", keep_aux=True):
+def riemann_setup(oracle_mod, N, level, td="lw", keep_aux=True, ordering="lexicographic"):
     import trixilw_hip as tlw
 
     ic = tlw.FourQuadrantRiemann()
     return setup(oracle_mod, N, level, td=td, periodic=False, domain=(0.0, 1.0), bc=ic, ic=ic, amp=0.0, sc=True,
-                 keep_aux=keep_aux, recon=MH)
+                 keep_aux=keep_aux, recon=MH, ordering=ordering)
 
 
 def blocky_state(nx, n, seed):
@@ -114,14 +114,15 @@ def test_mh_blocky_state_with_fallback_parity(oracle_mod, N):
     semi.close()
 
 
+@pytest.mark.parametrize("ordering", ["lexicographic", "strips"])
 @pytest.mark.parametrize("N,td,K", [(4, "lw", 30), (3, "lw", 30), (3, "mdrk", 20)])
-def test_mh_device_loop_k_steps(oracle_mod, N, td, K):
+def test_mh_device_loop_k_steps(oracle_mod, N, td, K, ordering):
     """Four-quadrant Riemann problem on a 32×32 mesh: K steps of the device
     loop against the oracle's solve loop (CFL 0.3·… as the bench runs it)."""
     import trixilw_hip as tlw
 
     print(f"MH device loop N={N} {td} K={K}", flush=True)
-    semi, o, u_lex, mesh = riemann_setup(oracle_mod, N, 5, td=td, keep_aux=False)
+    semi, o, u_lex, mesh = riemann_setup(oracle_mod, N, 5, td=td, keep_aux=False, ordering=ordering)
     cfl = 0.3 * (0.5 if td == "mdrk" else 1.0)
     u_ref, t_ref, dts = o.solve(u_lex, 0.0, 1e9, cfl, max_steps=K, mdrk=(td == "mdrk"))
     lw_update = tlw.semidiscretize(semi, semi.time_discretization, (0.0, float("inf")))

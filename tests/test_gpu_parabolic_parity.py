@@ -22,13 +22,13 @@ def shear_state(x, y, t=0.0, gamma=GAMMA):
     return np.stack([rho, rho * v1, rho * v2, p / (gamma - 1) + 0.5 * rho * (v1**2 + v2**2)], axis=-1)
 
 
-def setup_ns(oracle_mod, N, level, td="lw", mu=0.002, keep_aux=False, seed=0, amp=1e-3):
+def setup_ns(oracle_mod, N, level, td="lw", mu=0.002, keep_aux=False, seed=0, amp=1e-3, ordering="morton"):
     import trixilw_hip as tlw
 
     eq = tlw.CompressibleEulerEquations2D(GAMMA)
     eqp = tlw.CompressibleNavierStokesDiffusion2D(eq, mu=mu, Prandtl=PR,
                                                  gradient_variables=tlw.GradientVariablesConservative())
-    mesh = tlw.TreeMesh((-1.0, -1.0), (1.0, 1.0), level)
+    mesh = tlw.TreeMesh((-1.0, -1.0), (1.0, 1.0), level, ordering=ordering)
     solver = tlw.DGSEM(N, volume_integral=tlw.VolumeIntegralFR(tlw.MDRK() if td == "mdrk" else tlw.LW()))
     semi = tlw.SemidiscretizationHyperbolicParabolic(mesh, tlw.get_time_discretization(solver), (eq, eqp),
                                                      shear_state, solver, keep_aux=keep_aux)
@@ -117,13 +117,14 @@ def test_every_tile_matches_oracle(oracle_mod):
         semi.close()
 
 
+@pytest.mark.parametrize("ordering", ["morton", "strips"])
 @pytest.mark.parametrize("N,td,K", [(4, "lw", 25), (3, "lw", 25), (3, "mdrk", 20), (4, "mdrk", 15)])
-def test_parabolic_device_loop_k_steps(oracle_mod, N, td, K):
+def test_parabolic_device_loop_k_steps(oracle_mod, N, td, K, ordering):
     """K steps of the device loop (viscous CFL bound active) against the
     oracle's solve loop on a 16×16 mesh."""
     import trixilw_hip as tlw
 
-    semi, o, u_lex, mesh = setup_ns(oracle_mod, N, 4, td=td, mu=0.002)
+    semi, o, u_lex, mesh = setup_ns(oracle_mod, N, 4, td=td, mu=0.002, ordering=ordering)
     cfl = 0.9 if td == "lw" else 0.45
     u_ref, t_ref, dts = o.solve(u_lex, 0.0, 1e9, cfl, max_steps=K, mdrk=(td == "mdrk"))
     lw_update = tlw.semidiscretize(semi, semi.time_discretization, (0.0, float("inf")))

@@ -112,3 +112,37 @@ def test_gaussian_random_field_is_seeded_periodic_and_scaled():
     assert np.allclose(raw[:500], raw[500:1000], atol=1e-12) and np.allclose(raw[:500], raw[1000:], atol=1e-12)
     g = gaussian_random_field(x, y, (20.0, 20.0), seed=2026, amplitude=1e-2)
     assert np.max(np.abs(f - g)) > 1e-3
+
+
+def test_wall_functions_are_checked_on_the_wall_face_nodes():
+    """A NoSlip/Isothermal function reaches the device as one constant: it is
+    evaluated on the face nodes of the walls it is used on, and one that varies
+    there (even if it agrees at a few fixed points) is rejected."""
+    import pytest
+    import trixilw_hip as tlw
+    from trixilw_hip.semidiscretization import _wall_face_points
+
+    mesh = tlw.TreeMesh((0.0, 0.0), (1.0, 1.0), 3, periodicity=False)
+    nodes = tlw.LobattoLegendreBasis(4).nodes
+    top = _wall_face_points(mesh, 3, nodes)
+    assert len(top) == 8 * 5 and all(abs(y - 1.0) < 1e-15 for _, y in top)
+    assert tlw.NoSlip(lambda x, t, e: (1.0, 0.0)).resolve(top) == (1.0, 0.0)
+    assert tlw.NoSlip((0.5, 0.0)).resolve(top) == (0.5, 0.0)
+    # agrees at (0, 0) and (0.37, -0.61) but not along the lid
+    sneaky = tlw.NoSlip(lambda x, t, e: (1.0 if x[0] < 0.5 or x[0] == 0.37 else 2.0, 0.0))
+    with pytest.raises(NotImplementedError):
+        sneaky.resolve(top)
+    with pytest.raises(NotImplementedError):
+        tlw.Isothermal(lambda x, t, e: 1.0 + t).resolve(top)
+    assert tlw.Isothermal(lambda x, t, e: 1.2).resolve(top) == 1.2
+    eq = tlw.CompressibleEulerEquations2D(1.4)
+    eqp = tlw.CompressibleNavierStokesDiffusion2D(eq, mu=1e-3, Prandtl=0.72,
+                                                 gradient_variables=tlw.GradientVariablesConservative())
+    rest = tlw.BoundaryConditionNavierStokesWall(tlw.NoSlip((0.0, 0.0)), tlw.Adiabatic(0.0))
+    lid = tlw.BoundaryConditionNavierStokesWall(sneaky, tlw.Adiabatic(0.0))
+    solver = tlw.DGSEM(4)
+    with pytest.raises(NotImplementedError):  # raised on the host, before any device call
+        tlw.SemidiscretizationHyperbolicParabolic(
+            mesh, tlw.LW(), (eq, eqp), tlw.ConstantState(), solver,
+            boundary_conditions=((tlw.boundary_condition_slip_wall_vertical,) * 2
+                                 + (tlw.boundary_condition_slip_wall_horizontal,) * 2, (rest, rest, rest, lid)))

@@ -99,6 +99,13 @@ def test_strips_ordering_is_the_lexicographic_mesh_permuted(nx, ny, periodic):
     st.set_metrics(basis, Ja, ijn)
     for a, b in zip(st.metrics(basis), (Ja, ijn)):
         assert np.array_equal(a[p], b)
+    # a round trip in the mesh's own element order is the identity
+    own = [a.copy() for a in st.metrics(basis)]
+    st.set_metrics(basis, *own, order="element")
+    for a, b in zip(st.metrics(basis), own):
+        assert np.array_equal(a, b)
+    with pytest.raises(ValueError):
+        st.set_metrics(basis, *own, order="morton")
     if ny >= 2 and nx >= 5:
         run = np.arange(10)
         assert set(zip(st.ix[run].tolist(), st.iy[run].tolist())) == {(i, j) for i in range(5) for j in range(2)}

@@ -7,11 +7,11 @@ library; the first call that needs the GPU does, and fails if it is missing.
 """
 from .adjoint import adjoint_step, gaussian_random_field, gradient_lw, kinetic_energy, objective_lw
 from .basis import LobattoLegendreBasis
-from .batch import SemidiscretizationBatch
+from .batch import SemidiscretizationBatch, draw_vortex_ensemble
 from .equations import (Adiabatic, BoundaryConditionDirichlet, BoundaryConditionNavierStokesWall, CompressibleEulerEquations2D,
                         CompressibleNavierStokesDiffusion2D, ConstantState, Isothermal, NoSlip,
                         FourQuadrantRiemann, GradientVariablesConservative, GradientVariablesPrimitive,
-                        IsentropicVortex, LinearScalarAdvectionEquation2D, boundary_condition_slip_wall_horizontal,
+                        IsentropicVortex, LinearScalarAdvectionEquation2D, TaylorGreenVortex, boundary_condition_slip_wall_horizontal,
                         boundary_condition_slip_wall_vertical)
 from .integrator import (LWIntegrator, LWSolution, compute_dt, isapprox, isfinished, perform_step, set_dt,
                          set_t_and_iter, solve_lwfr)
@@ -28,7 +28,7 @@ __all__ = [
     "LobattoLegendreBasis", "CompressibleEulerEquations2D", "ConstantState", "FourQuadrantRiemann",
     "boundary_condition_slip_wall_vertical", "boundary_condition_slip_wall_horizontal", "NoSlip", "Adiabatic", "Isothermal",
     "BoundaryConditionNavierStokesWall", "BoundaryConditionDirichlet",
-    "IsentropicVortex", "LinearScalarAdvectionEquation2D", "LWIntegrator", "LWSolution", "compute_dt", "isapprox",
+    "IsentropicVortex", "TaylorGreenVortex", "LinearScalarAdvectionEquation2D", "LWIntegrator", "LWSolution", "compute_dt", "isapprox",
     "isfinished", "perform_step", "set_dt", "set_t_and_iter", "solve_lwfr", "TreeMesh",
     "StructuredMesh", "coordinates2mapping", "DGSEM", "LW", "MDRK",
     "CFLBased", "Adaptive", "LWUpdate", "SemidiscretizationHyperbolic", "VolumeIntegralFR", "flux_lax_friedrichs",
@@ -36,6 +36,6 @@ __all__ = [
     "FirstOrderReconstruction", "IndicatorHennemannGassner", "MUSCLHancockReconstruction",
     "VolumeIntegralFRShockCapturing", "density_pressure", "PositivityPreservingLimiterZhangShu", "density",
     "pressure", "adjoint_step", "gradient_lw", "kinetic_energy", "gaussian_random_field", "objective_lw",
-    "SemidiscretizationBatch", "SemidiscretizationHyperbolicParabolic", "CompressibleNavierStokesDiffusion2D",
+    "SemidiscretizationBatch", "draw_vortex_ensemble", "SemidiscretizationHyperbolicParabolic", "CompressibleNavierStokesDiffusion2D",
     "GradientVariablesConservative", "GradientVariablesPrimitive",
 ]

@@ -149,3 +149,16 @@ class SemidiscretizationBatch:
                                                it.ctypes.data_as(C.POINTER(C.c_int64)),
                                                fin.ctypes.data_as(C.POINTER(C.c_int32))), self._h)
         return {"t": t, "dt": dt, "iter": it, "finished": fin.astype(bool)}
+
+
+def draw_vortex_ensemble(n_instances=64, seed=2024):
+    """The batching configuration's draw with numpy default_rng(seed): per
+    instance a TreeMesh level in {4, 5, 6, 7} (16² .. 128² elements), polydeg
+    in {3, 4}, Mach number U[0.2, 0.8] and vortex strength U[2, 5], in this
+    order of draws.  Returns (levels, polydegs, machs, strengths) as arrays."""
+    rng = np.random.default_rng(seed)
+    levels = rng.integers(4, 8, n_instances)
+    degs = rng.choice([3, 4], n_instances)
+    machs = rng.uniform(0.2, 0.8, n_instances)
+    strengths = rng.uniform(2.0, 5.0, n_instances)
+    return levels, degs, machs, strengths

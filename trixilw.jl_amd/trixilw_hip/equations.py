@@ -66,17 +66,27 @@ boundary_condition_slip_wall_horizontal = _SlipWall(2)
 
 class NoSlip:
     """Trixi NoSlip(boundary_value_function): the wall velocity (x, t, equations)
-    -> (v1, v2).  The device takes a constant wall velocity: a callable is
-    evaluated once and must not depend on x or t; a pair is taken as is."""
+    -> (v1, v2).  The device takes a constant wall velocity: a pair is taken as
+    is; a callable is evaluated by the semidiscretization at every boundary face
+    node of its walls (at t = 0 and t = 1) and rejected unless it gives one
+    value there."""
 
     def __init__(self, velocity):
-        if callable(velocity):
-            a = velocity((0.0, 0.0), 0.0, None)
-            b = velocity((0.37, -0.61), 1.3, None)
-            if tuple(map(float, a)) != tuple(map(float, b)):
-                raise NotImplementedError("the device path takes a constant wall velocity")
-            velocity = a
-        self.velocity = (float(velocity[0]), float(velocity[1]))
+        self.function = velocity if callable(velocity) else None
+        self.velocity = None if callable(velocity) else (float(velocity[0]), float(velocity[1]))
+
+    def resolve(self, points):
+        """The constant wall velocity on the face-node coordinates `points`
+        [(x, y), ...] of the walls it is used on."""
+        if self.function is None:
+            return self.velocity
+        if not points:  # no boundary face in this direction (periodic): the value is unused
+            return tuple(float(v) for v in self.function((0.0, 0.0), 0.0, None))
+        vals = {tuple(float(v) for v in self.function((x, y), t, None)) for (x, y) in points for t in (0.0, 1.0)}
+        if len(vals) != 1:
+            raise NotImplementedError("the device path takes a constant wall velocity; this function varies "
+                                      "over the wall's face nodes or in time")
+        return vals.pop()
 
 
 class Adiabatic:
@@ -97,12 +107,21 @@ class Isothermal:
     temperature (a callable is evaluated once and must not depend on x or t)."""
 
     def __init__(self, temperature):
-        if callable(temperature):
-            a = temperature((0.0, 0.0), 0.0, None)
-            if float(a) != float(temperature((0.37, -0.61), 1.3, None)):
-                raise NotImplementedError("the device path takes a constant wall temperature")
-            temperature = a
-        self.temperature = float(temperature)
+        self.function = temperature if callable(temperature) else None
+        self.temperature = None if callable(temperature) else float(temperature)
+
+    def resolve(self, points):
+        """The constant wall temperature on the face-node coordinates `points`
+        (evaluated at t = 0 and t = 1; a varying function is rejected)."""
+        if self.function is None:
+            return self.temperature
+        if not points:  # no boundary face in this direction (periodic): the value is unused
+            return float(self.function((0.0, 0.0), 0.0, None))
+        vals = {float(self.function((x, y), t, None)) for (x, y) in points for t in (0.0, 1.0)}
+        if len(vals) != 1:
+            raise NotImplementedError("the device path takes a constant wall temperature; this function varies "
+                                      "over the wall's face nodes or in time")
+        return vals.pop()
 
 
 class BoundaryConditionNavierStokesWall:
@@ -115,8 +134,14 @@ class BoundaryConditionNavierStokesWall:
         if not isinstance(boundary_condition_velocity, NoSlip) or not isinstance(boundary_condition_heat_flux,
                                                                                  (Adiabatic, Isothermal)):
             raise NotImplementedError("the device path implements NoSlip velocity with Adiabatic or Isothermal walls")
-        self.velocity = boundary_condition_velocity.velocity
-        self.temperature = getattr(boundary_condition_heat_flux, "temperature", None)
+        self.boundary_condition_velocity = boundary_condition_velocity
+        self.boundary_condition_heat_flux = boundary_condition_heat_flux
+
+    def resolve(self, points):
+        """(wall velocity, wall temperature or None) on the wall's face nodes."""
+        heat = self.boundary_condition_heat_flux
+        return (self.boundary_condition_velocity.resolve(points),
+                heat.resolve(points) if isinstance(heat, Isothermal) else None)
 
 
 def boundary_conditions_kind(bcs):
@@ -229,6 +254,35 @@ class IsentropicVortex:
         T = 1 - (g - 1) * eps * eps / (8 * g * np.pi * np.pi) * np.exp(1 - r2)
         rho = T ** (1 / (g - 1))
         return CompressibleEulerEquations2D(g).prim2cons(rho, vinf - du * dy, vinf + du * dx, rho * T)
+
+
+@dataclass
+class TaylorGreenVortex:
+    """Two-dimensional Taylor-Green vortex on the periodic square [x0, x0 + L]²
+    (an initial condition, not a boundary value function): with k = 2π/L,
+    v1 = U sin(k(x - x0)) cos(k(y - y0)), v2 = -U cos(k(x - x0)) sin(k(y - y0)),
+    ρ = ρ0 and p = p0 + ρ0 U²/4 (cos 2k(x - x0) + cos 2k(y - y0)), the
+    incompressible vortex's pressure on a background p0 = ρ0 U² / (γ M²) of
+    Mach number M.  Viscosity makes it decay; its Reynolds number is
+    ρ0 U (L/2) / μ."""
+
+    U: float = 1.0
+    rho0: float = 1.0
+    mach: float = 0.1
+    L: float = 2.0
+    origin: tuple = (0.0, 0.0)
+    gamma: float = 1.4
+
+    def __call__(self, x, y, t=0.0, equations=None):
+        k = 2.0 * np.pi / self.L
+        X = k * (np.asarray(x, dtype=np.float64) - self.origin[0])
+        Y = k * (np.asarray(y, dtype=np.float64) - self.origin[1])
+        p0 = self.rho0 * self.U ** 2 / (self.gamma * self.mach ** 2)
+        v1 = self.U * np.sin(X) * np.cos(Y)
+        v2 = -self.U * np.cos(X) * np.sin(Y)
+        p = p0 + 0.25 * self.rho0 * self.U ** 2 * (np.cos(2 * X) + np.cos(2 * Y))
+        rho = self.rho0 * np.ones_like(X)
+        return CompressibleEulerEquations2D(self.gamma).prim2cons(rho, v1, v2, p)
 
 
 @dataclass

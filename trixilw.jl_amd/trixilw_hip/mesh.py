@@ -210,16 +210,20 @@ class StructuredMesh:
             self._metrics[key] = (np.ascontiguousarray(Ja), np.ascontiguousarray(ijn))
         return self._metrics[key]
 
-    def set_metrics(self, basis, contravariant_vectors, inverse_jacobian):
+    def set_metrics(self, basis, contravariant_vectors, inverse_jacobian, order="linear"):
         """Metric terms computed elsewhere (e.g. Trixi's cache.elements of the
-        same mesh) for the basis, in the layouts metrics() returns with the
-        elements in linear index order cx + nx*cy; used in place of the host
-        computation."""
+        same mesh) for the basis, in the layouts metrics() returns, used in
+        place of the host computation.  ``order`` names the element order of
+        the given arrays: "linear" (cx + nx*cy, Trixi's StructuredMesh order)
+        or "element" (this mesh's element order, as metrics() returns them)."""
+        if order not in ("linear", "element"):
+            raise ValueError("order must be 'linear' or 'element'")
         n = len(basis.nodes)
         Ja = np.asarray(contravariant_vectors, dtype=np.float64).reshape(self.n_elements, n, n, 4)
         ijn = np.asarray(inverse_jacobian, dtype=np.float64).reshape(self.n_elements, n, n)
-        Ja, ijn = np.ascontiguousarray(Ja[self._order]), np.ascontiguousarray(ijn[self._order])
-        self._metrics[id(basis)] = (Ja, ijn)
+        if order == "linear":
+            Ja, ijn = Ja[self._order], ijn[self._order]
+        self._metrics[id(basis)] = (np.ascontiguousarray(Ja), np.ascontiguousarray(ijn))
 
     def lexicographic_permutation(self):
         return self._element_of.copy()

@@ -465,9 +465,10 @@ class SemidiscretizationHyperbolicParabolic(SemidiscretizationHyperbolic):
                 raise NotImplementedError("parabolic boundary conditions: four BoundaryConditionNavierStokesWall")
             self._par.walls = 1
             for k, b in enumerate(seq):
-                self._par.wall_velocity[2 * k], self._par.wall_velocity[2 * k + 1] = b.velocity
-                if b.temperature is not None:
-                    self._par.wall_isothermal[k], self._par.wall_temperature[k] = 1, b.temperature
+                vel, temp = b.resolve(_wall_face_points(mesh, k, solver.basis.nodes))
+                self._par.wall_velocity[2 * k], self._par.wall_velocity[2 * k + 1] = vel
+                if temp is not None:
+                    self._par.wall_isothermal[k], self._par.wall_temperature[k] = 1, temp
         elif bc_hyp is not None:
             raise NotImplementedError("the hyperbolic-parabolic device path needs a periodic mesh or walls")
         super().__init__(mesh, time_discretization, eq, initial_condition, solver, boundary_conditions=bc_hyp,
@@ -488,6 +489,26 @@ class SemidiscretizationHyperbolicParabolic(SemidiscretizationHyperbolic):
         if rc != _lib.TLW_OK:
             raise _lib.TLWError(rc, "tlw_create_parabolic failed (invalid description, non-periodic mesh or HIP "
                                     "error)")
+
+
+def _wall_face_points(mesh, direction, nodes):
+    """Coordinates of the face nodes of a TreeMesh's boundary faces in one
+    direction (0 x_neg, 1 x_pos, 2 y_neg, 3 y_pos)."""
+    nbr = np.asarray(mesh.neighbors).reshape(-1, 4)
+    elems = np.nonzero(nbr[:, direction] < 0)[0]
+    if len(elems) == 0 or not hasattr(mesh, "element_center"):
+        return []
+    c = np.asarray(mesh.element_center).reshape(-1, 2)[elems]
+    half = 0.5 * np.asarray(mesh.element_length)[elems]
+    nodes = np.asarray(nodes)
+    side = -1.0 if direction % 2 == 0 else 1.0
+    if direction < 2:
+        x = np.repeat(c[:, 0] + side * half, len(nodes))
+        y = (c[:, 1][:, None] + half[:, None] * nodes[None, :]).ravel()
+    else:
+        x = (c[:, 0][:, None] + half[:, None] * nodes[None, :]).ravel()
+        y = np.repeat(c[:, 1] + side * half, len(nodes))
+    return list(zip(x.tolist(), y.tolist()))
 
 
 def _rhs_call(fn, du_ode, u_ode, semi, t, stage=0):
