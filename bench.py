@@ -168,67 +168,71 @@ def cpu_baseline(o, u0, cfl, mdrk, dofs, steps, threads):
 
 def measure_batch(steps=200, n_instances=64, seed=2024, single_value=None):
     """64 isentropic-vortex instances (mesh 16²..128², polydeg 3 or 4, Mach U[0.2, 0.8],
-    strength U[2, 5], drawn with numpy default_rng(seed)), grouped by polydeg into two batch
-    contexts; each group runs `steps` LW steps of its instances (own CFL dt per instance,
-    0.98·LW limit) replayed from a captured graph.  Returns the sub-record of the JSON line."""
+    strength U[2, 5], drawn with numpy default_rng(seed)): a batch context per polydeg joined
+    into one ensemble, so every LW step is one volume, one surface and one finalize launch for
+    all 64 instances (own CFL dt per instance, 0.98·LW limit), replayed from a captured graph;
+    the tile pair is chosen by timing both.  Returns the sub-record of the JSON line."""
     import torch
     import trixilw_hip as tlw
 
     levels, degs, machs, strengths = tlw.draw_vortex_ensemble(n_instances, seed)
     eq = tlw.CompressibleEulerEquations2D(1.4)
-    total_dofs, total_s, groups = 0, 0.0, {}
     stream = torch.cuda.Stream()
+    batches, cfls, u0s, counts = [], [], [], {}
     for N in (3, 4):
         idx = np.nonzero(degs == N)[0]
-        if len(idx) == 0:
-            continue
         meshes = [tlw.TreeMesh((-10.0, -10.0), (10.0, 10.0), int(levels[k]), ordering="morton") for k in idx]
         ics = [tlw.IsentropicVortex(strength=float(strengths[k]), mach=float(machs[k])) for k in idx]
         solver = tlw.DGSEM(int(N))
-        batch = tlw.SemidiscretizationBatch(meshes, tlw.LW(), eq, ics, solver)
-        cfl = tlw.trixi2lw(0.98, solver)
-        u0 = batch.compute_coefficients()
-        batch.set_stream(stream)
-        L = tlw._lib.lib()
-        h = batch.handle
-        dp = C.POINTER(C.c_double)
-        tlw._lib.check(L.tlw_set_state(h, u0.ctypes.data_as(dp), u0.size), h)
-        chosen, _ = batch_autotune(batch, u0)
-        tlw._lib.check(L.tlw_enable_graphs(h, 1), h)
-        tlw._lib.check(L.tlw_solve_init(h, 0.0, float("inf"), cfl), h)
-        batch.advance(5)  # capture + warm-up
-        tlw._lib.check(L.tlw_set_state(h, u0.ctypes.data_as(dp), u0.size), h)
-        tlw._lib.check(L.tlw_solve_init(h, 0.0, float("inf"), cfl), h)
+        b = tlw.SemidiscretizationBatch(meshes, tlw.LW(), eq, ics, solver)
+        batches.append(b)
+        cfls.append(tlw.trixi2lw(0.98, solver))
+        u0s.append(b.compute_coefficients())
+        counts[f"N{N}"] = {"instances": int(len(idx)), "dofs": int(b.ndofs())}
+    ens = tlw.SemidiscretizationEnsemble(batches)
+    total_dofs = ens.ndofs()
+    tune = {}
+    for tiles in ens.TILES:  # capture + warm-up, then a short timed run per tile pair
+        ens.set_tiles(*tiles)
+        ens.init(u0s, cfls, stream=stream, graphs=True)
+        ens.advance(20)
+        ens.init(u0s, cfls, stream=stream, graphs=True)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        batch.advance(steps)
+        ens.advance(20)
         torch.cuda.synchronize()
-        el = time.perf_counter() - t0
-        st = batch.status()
+        tune[str(tiles)] = round(1e3 * (time.perf_counter() - t0) / 20, 4)
+    best = min(ens.TILES, key=lambda tl: tune[str(tl)])
+    ens.set_tiles(*best)
+    ens.init(u0s, cfls, stream=stream, graphs=True)
+    ens.advance(steps)  # capture of the timed step count
+    ens.init(u0s, cfls, stream=stream, graphs=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ens.advance(steps)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    for b in batches:
+        st = b.status()
         assert np.all(st["iter"] == steps), st["iter"]
-        dofs = batch.ndofs()
-        groups[f"N{N}"] = {"instances": int(len(idx)), "dofs": int(dofs), "seconds": round(el, 5),
-                           "dof_updates_per_s": dofs * steps / el, "elements_per_workgroup": chosen}
-        total_dofs += dofs
-        total_s += el
-        batch.close()
-    value = total_dofs * steps / total_s
+    value = total_dofs * steps / el
     out = {"value": value, "unit": "DOF-updates/s", "instances": n_instances, "steps": steps,
-           "total_dofs": int(total_dofs), "groups": groups,
+           "total_dofs": int(total_dofs), "ms_per_step": round(1e3 * el / steps, 4), "groups": counts,
+           "tiles": list(best), "tile_ms_per_step": tune,
            "config": "64 isentropic-vortex instances, 16^2..128^2 elements, polydeg 3/4, Mach U[0.2,0.8], "
-                     "strength U[2,5], LW, graph-replayed steps, one batch context per polydeg"}
+                     "strength U[2,5], LW, one volume + one surface + one finalize launch per step for all "
+                     "instances (tlw_create_ensemble), graph-replayed steps"}
+    ens.close()
+    for b in batches:
+        b.close()
     if single_value:
-        # each polydeg group against one large instance of the same polydeg (1024² vortex, LW)
+        # against one large instance of each polydeg (1024² vortex, LW) on the same DOF mix
         singles = {4: single_value, 3: measure_single(3)}
-        for N in (3, 4):
-            g = groups.get(f"N{N}")
-            if g:
-                g["single_instance_value"] = singles[N]
-                g["vs_single_instance"] = round(g["dof_updates_per_s"] / singles[N], 3)
-        out["vs_single_instance"] = round(value / (total_dofs / sum(g["dofs"] / singles[int(k[1])]
-                                                                    for k, g in groups.items())), 3)
-        out["vs_single_note"] = ("the batch's DOF-updates/s over the single-instance rate of the same DOF mix "
-                                 "(each group's DOFs at its polydeg's 1024² rate)")
+        mix_time = sum(counts[f"N{N}"]["dofs"] / singles[N] for N in (3, 4))
+        out["single_instance_values"] = {f"N{N}": singles[N] for N in (3, 4)}
+        out["vs_single_instance"] = round(value / (total_dofs / mix_time), 3)
+        out["vs_single_note"] = ("the ensemble's DOF-updates/s over the single-instance rate of the same DOF mix "
+                                 "(each polydeg's DOFs at its 1024² single-instance rate)")
     return out
 
 
@@ -240,16 +244,6 @@ def measure_single(N, steps=20, warmup=3):
     r = run_steps(semi, np.ascontiguousarray(ic(x, y, 0.0)), cfl, steps, warmup)
     semi.close()
     return r["value"]
-
-
-def batch_autotune(batch, u0):
-    """Tile choice of a batch context on its initial state."""
-    import trixilw_hip as tlw
-
-    chosen = C.c_int32()
-    ms = (C.c_double * 3)()
-    tlw._lib.check(tlw._lib.lib().tlw_autotune(batch.handle, 2, C.byref(chosen), ms), batch.handle)
-    return chosen.value, list(ms)
 
 
 def ns_shear(x, y, t=0.0, gamma=1.4):

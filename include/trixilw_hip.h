@@ -220,6 +220,22 @@ int tlw_create_curved_parabolic(const tlw_mesh_desc* mesh, const tlw_basis_desc*
 int tlw_n_instances(const tlw_ctx* ctx, int32_t* n_instances);
 int tlw_batch_status(tlw_ctx* ctx, int32_t n_instances, double* t_host, double* dt_host, int64_t* iter_host,
                      int32_t* finished_host);
+/* An ensemble of uneven instances: one batch context of polydeg 3 and one of polydeg 4 (plain LW on
+ * periodic TreeMeshes, tiles (16, 10) or (8, 5) elements per volume workgroup) advanced by the same
+ * launches: each step is one volume, one surface and one finalize launch for every instance of both,
+ * optionally replayed from a captured HIP graph.  No counterpart in the reference (one
+ * semidiscretization per solve_lwfr).  The member contexts keep their states (tlw_set_state,
+ * tlw_get_state, tlw_batch_status work on them as before); tlw_ensemble_solve_init is tlw_solve_init
+ * on both (CFL number per context); tlw_ensemble_advance runs up to max_steps steps and reports whether
+ * every instance is finished.  The ensemble does not own the contexts: destroy it first. */
+typedef struct tlw_ensemble tlw_ensemble;
+int tlw_create_ensemble(tlw_ctx* ctx_a, tlw_ctx* ctx_b, tlw_ensemble** out);
+int tlw_ensemble_set_stream(tlw_ensemble* ens, void* stream);
+int tlw_ensemble_enable_graphs(tlw_ensemble* ens, int32_t enable);
+int tlw_ensemble_solve_init(tlw_ensemble* ens, double t0, double t_final, double cfl_polydeg3, double cfl_polydeg4);
+int tlw_ensemble_advance(tlw_ensemble* ens, int32_t max_steps, int32_t* finished_host);
+const char* tlw_ensemble_last_error(const tlw_ensemble* ens);
+void tlw_ensemble_destroy(tlw_ensemble* ens);
 /* Replay tlw_advance's steps from a captured HIP graph (one graph per step count, captured on
  * first use; needs an explicit stream from tlw_set_stream).  No counterpart in the reference. */
 int tlw_enable_graphs(tlw_ctx* ctx, int32_t enable);

@@ -96,3 +96,58 @@ def test_c4_draw_every_instance_matches_oracle(oracle_mod):
             checked += 1
         batch.close()
     assert checked == 64
+
+
+def _c4_groups(tlw, n_instances=64, seed=2024, ordering="morton"):
+    levels, degs, machs, strengths = tlw.draw_vortex_ensemble(n_instances, seed)
+    eq = tlw.CompressibleEulerEquations2D(1.4)
+    out = {}
+    for N in (3, 4):
+        idx = np.nonzero(degs == N)[0]
+        meshes = [tlw.TreeMesh((-10.0, -10.0), (10.0, 10.0), int(levels[k]), ordering=ordering) for k in idx]
+        ics = [tlw.IsentropicVortex(strength=float(strengths[k]), mach=float(machs[k])) for k in idx]
+        solver = tlw.DGSEM(N)
+        out[N] = (tlw.SemidiscretizationBatch(meshes, tlw.LW(), eq, ics, solver), tlw.trixi2lw(0.98, solver))
+    return out
+
+
+@pytest.mark.parametrize("tiles", [(16, 10), (8, 5)])
+@pytest.mark.parametrize("graphs", [False, True])
+def test_ensemble_one_launch_matches_separate_batches(oracle_mod, tiles, graphs):
+    """The 64-instance draw as one ensemble (one volume, surface and finalize
+    launch per step for both polynomial degrees) gives the same bits, states
+    and times, as each polydeg group advanced by its own batch context; and
+    one instance of each degree matches the oracle."""
+    import torch
+    import trixilw_hip as tlw
+
+    torch_mod()
+    K = 12
+    groups = _c4_groups(tlw)
+    stream = torch.cuda.Stream()
+    ref = {}
+    for N, (b, _) in groups.items():
+        b.set_stream(stream)
+        tlw._lib.check(tlw._lib.lib().tlw_set_elements_per_workgroup(b.handle, tiles[0] if N == 3 else tiles[1]),
+                       b.handle)
+        u_sep, st_sep = b.solve(b.compute_coefficients(), groups[N][1], K, graphs=False, stream=stream)
+        ref[N] = (u_sep, st_sep)
+    ens = tlw.SemidiscretizationEnsemble([groups[4][0], groups[3][0]])
+    ens.set_tiles(*tiles)
+    states, statuses = ens.solve([groups[3][0].compute_coefficients(), groups[4][0].compute_coefficients()],
+                                 [groups[3][1], groups[4][1]], K, stream=stream, graphs=graphs, chunk=5)
+    for N, u, st in zip((3, 4), states, statuses):
+        assert np.all(st["iter"] == K)
+        assert np.array_equal(u, ref[N][0]), N
+        assert np.array_equal(st["t"], ref[N][1]["t"]), N
+    for N, u in zip((3, 4), states):
+        b = groups[N][0]
+        mesh, ic = b.meshes[0], b.initial_conditions[0]
+        nx = mesh.nx
+        o = oracle_mod.Oracle(N, nx, nx, -10.0, -10.0, 20.0 / nx)
+        x, y = o.node_coords()
+        u_ref, _, _ = o.solve(ic(x, y, 0.0), 0.0, 1e9, groups[N][1], max_steps=K)
+        assert_close(b.split(u)[0][mesh.lexicographic_permutation()], u_ref, what=f"ensemble N={N} instance 0")
+    ens.close()
+    for b, _ in groups.values():
+        b.close()

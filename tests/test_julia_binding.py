@@ -106,6 +106,7 @@ def test_struct_layouts_agree():
 
 C_ARG = {
     "tlw_ctx*": {"Ptr{Cvoid}"}, "tlw_ctx**": {"Ref{Ptr{Cvoid}}"}, "double": {"Float64"},
+    "tlw_ensemble*": {"Ptr{Cvoid}"}, "tlw_ensemble**": {"Ref{Ptr{Cvoid}}"},
     "int32_t": {"Int32", "Cint"}, "int64_t": {"Int64"}, "size_t": {"Csize_t"}, "void*": {"Ptr{Cvoid}"},
     "double*": {"Ptr{Float64}", "Ref{Float64}"}, "double**": {"Ref{Ptr{Float64}}"},
     "int64_t*": {"Ref{Int64}", "Ptr{Int64}"}, "int32_t*": {"Ref{Int32}", "Ptr{Int32}"},

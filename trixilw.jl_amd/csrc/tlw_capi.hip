@@ -24,6 +24,19 @@
 
 using namespace tlw;
 
+struct tlw_ctx;
+// Two batch contexts, one of polydeg 3 and one of polydeg 4, advanced by the
+// same launches (tlw_create_ensemble).
+struct tlw_ensemble {
+    tlw_ctx* c3 = nullptr;
+    tlw_ctx* c4 = nullptr;
+    hipStream_t stream = nullptr;
+    bool graphs = false;
+    std::map<int, hipGraphExec_t> graph_cache;
+    hipStream_t graph_stream = nullptr;
+    std::string err;
+};
+
 struct tlw_ctx {
     int N = 0, n = 0, E = 0;
     int ninst = 1;
@@ -589,14 +602,16 @@ void launch_volume(tlw_ctx* c, const KArgs& a, int kind) {
     if (c->N == 4) launch_volume_t<4>(c, a, kind);
     else launch_volume_t<3>(c, a, kind);
 }
+FinArgs fin_args(tlw_ctx* c, int reduce, int use_vol) {
+    return FinArgs{c->d_state, c->d_partial, c->d_rng, use_vol, reduce, c->pv_off, c->d_partial + c->slot_off,
+                   DtLog{c->d_dtlog, c->dtlog_cap, c->d_dtpresc, c->dtpresc_n}};
+}
 void launch_finalize(tlw_ctx* c, int reduce, int use_vol) {
-    DtLog dl{c->d_dtlog, c->dtlog_cap, c->d_dtpresc, c->dtpresc_n};
+    const FinArgs f = fin_args(c, reduce, use_vol);
     if (c->N == 4)
-        hipLaunchKernelGGL(k_finalize<4>, dim3(c->ninst), dim3(kFinalizeBlock), 0, c->stream, c->d_state,
-                           c->d_partial, c->d_rng, use_vol, reduce, c->pv_off, c->d_partial + c->slot_off, dl);
+        hipLaunchKernelGGL(k_finalize<4>, dim3(c->ninst), dim3(kFinalizeBlock), 0, c->stream, f);
     else
-        hipLaunchKernelGGL(k_finalize<3>, dim3(c->ninst), dim3(kFinalizeBlock), 0, c->stream, c->d_state,
-                           c->d_partial, c->d_rng, use_vol, reduce, c->pv_off, c->d_partial + c->slot_off, dl);
+        hipLaunchKernelGGL(k_finalize<3>, dim3(c->ninst), dim3(kFinalizeBlock), 0, c->stream, f);
 }
 template <int N> void launch_surface_t(tlw_ctx* c, const KArgs& a) {
     dim3 g(c->sgrid), b(kSurfBlock);
@@ -823,6 +838,18 @@ int check_loop_boundary(tlw_ctx* c) {
     if (c->btab_levels != 1)
         return fail(c, TLW_ENOTSUP, "time-dependent boundary data needs the host loop (the device loop sets dt itself)");
     return TLW_OK;
+}
+
+// KArgs of one LW step of the device loop (the arguments enqueue_step gives
+// the volume and surface launches of a plain LW context).
+KArgs lw_step_args(tlw_ctx* c) {
+    KArgs a = base_args(c);
+    a.use_state_dt = 1;
+    a.step_mode = 1;
+    a.in_sel = SEL_CUR;
+    a.out_sel = SEL_OTHER;
+    a.reduce = RED_STEP;
+    return a;
 }
 
 // message of the last failed context-free call on this thread (tlw_last_error(NULL))
@@ -1396,6 +1423,157 @@ int tlw_max_dt_noalloc(const double* u, const double* inv_jac, int32_t n_element
                                 : tlw_noalloc::launch<3>(u, inv_jac, n_elements, gamma, max_dt, s, err);
     if (rc) g_noctx_err = "tlw_max_dt_noalloc: " + err;
     return rc;
+}
+
+int tlw_create_ensemble(tlw_ctx* a, tlw_ctx* b, tlw_ensemble** out) {
+    if (!out) return TLW_EINVAL;
+    *out = nullptr;
+    if (!a || !b || a == b) {
+        g_noctx_err = "tlw_create_ensemble: two distinct contexts required";
+        return TLW_EINVAL;
+    }
+    tlw_ctx* c3 = a->N == 3 ? a : b;
+    tlw_ctx* c4 = a->N == 3 ? b : a;
+    if (c3->N != 3 || c4->N != 4) {
+        g_noctx_err = "tlw_create_ensemble: one context of polydeg 3 and one of polydeg 4";
+        return TLW_EINVAL;
+    }
+    for (tlw_ctx* c : {c3, c4})
+        if (c->time_disc != TLW_LW || c->sc || c->parabolic || c->curved || c->has_boundary || c->positivity ||
+            c->adaptive || c->d_dtpresc || c->d_dtlog) {
+            g_noctx_err = "tlw_create_ensemble: plain LW contexts on periodic TreeMeshes (no shock capturing, "
+                          "parabolic terms, limiter, adaptive steps or dt record)";
+            return TLW_ENOTSUP;
+        }
+    auto* e = new tlw_ensemble;
+    e->c3 = c3;
+    e->c4 = c4;
+    e->stream = c3->stream;
+    *out = e;
+    return TLW_OK;
+}
+
+namespace {
+
+int ens_fail(tlw_ensemble* e, int code, const std::string& m) {
+    e->err = m;
+    return code;
+}
+
+void ens_drop_graphs(tlw_ensemble* e) {
+    for (auto& kv : e->graph_cache) (void)hipGraphExecDestroy(kv.second);
+    e->graph_cache.clear();
+}
+
+// One LW step of both contexts: the volume, surface and finalize launches of
+// each, merged into one launch apiece.  Tiles (16, 10) run 256-lane workgroups,
+// (8, 5) 128-lane ones.
+void enqueue_pair_step(tlw_ensemble* e) {
+    tlw_ctx* c3 = e->c3;
+    tlw_ctx* c4 = e->c4;
+    const KArgs a3 = lw_step_args(c3), a4 = lw_step_args(c4);
+    const dim3 gv(c3->vgrid + c4->vgrid);
+    if (c3->ebv == 16)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume_pair<16, 10>), gv, dim3(VGeo<4, 10>::NT), 0, e->stream, a3, a4,
+                           c3->vgrid);
+    else
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume_pair<8, 5>), gv, dim3(VGeo<4, 5>::NT), 0, e->stream, a3, a4,
+                           c3->vgrid);
+    hipLaunchKernelGGL(k_surface_pair, dim3(c3->sgrid + c4->sgrid), dim3(kSurfBlock), 0, e->stream, a3, a4,
+                       c3->sgrid);
+    hipLaunchKernelGGL(k_finalize_pair, dim3(c3->ninst + c4->ninst), dim3(kFinalizeBlock), 0, e->stream,
+                       fin_args(c3, RED_STEP, 0), fin_args(c4, RED_STEP, 0), c3->ninst);
+}
+
+}  // namespace
+
+int tlw_ensemble_set_stream(tlw_ensemble* e, void* stream) {
+    if (!e) return TLW_EINVAL;
+    e->stream = static_cast<hipStream_t>(stream);
+    e->c3->stream = e->stream;
+    e->c4->stream = e->stream;
+    return TLW_OK;
+}
+
+int tlw_ensemble_enable_graphs(tlw_ensemble* e, int32_t enable) {
+    if (!e) return TLW_EINVAL;
+    e->graphs = enable != 0;
+    if (!e->graphs) ens_drop_graphs(e);
+    return TLW_OK;
+}
+
+int tlw_ensemble_solve_init(tlw_ensemble* e, double t0, double t_final, double cfl3, double cfl4) {
+    if (!e) return TLW_EINVAL;
+    e->c3->stream = e->c4->stream = e->stream;
+    int rc = tlw_solve_init(e->c3, t0, t_final, cfl3);
+    if (rc) return ens_fail(e, rc, std::string("polydeg 3 context: ") + e->c3->err);
+    rc = tlw_solve_init(e->c4, t0, t_final, cfl4);
+    if (rc) return ens_fail(e, rc, std::string("polydeg 4 context: ") + e->c4->err);
+    return TLW_OK;
+}
+
+int tlw_ensemble_advance(tlw_ensemble* e, int32_t max_steps, int32_t* finished) {
+    if (!e) return TLW_EINVAL;
+    if (max_steps < 0) return ens_fail(e, TLW_EINVAL, "max_steps < 0");
+    tlw_ctx* c3 = e->c3;
+    tlw_ctx* c4 = e->c4;
+    const bool pair16 = c3->ebv == 16 && c4->ebv == 10, pair8 = c3->ebv == 8 && c4->ebv == 5;
+    if (!pair16 && !pair8)
+        return ens_fail(e, TLW_EINVAL, "tiles (elements per workgroup) must be (16, 10) or (8, 5)");
+    c3->stream = c4->stream = e->stream;
+    if (max_steps > 0) {
+        if (e->graphs) {
+            if (!e->stream) return ens_fail(e, TLW_EINVAL, "graph replay needs an explicit stream");
+            if (e->graph_stream != e->stream) {
+                ens_drop_graphs(e);
+                e->graph_stream = e->stream;
+            }
+            // a tile change re-launches different kernels: key the cache on the pair too
+            const int key = 2 * max_steps + (pair16 ? 1 : 0);
+            auto it = e->graph_cache.find(key);
+            if (it == e->graph_cache.end()) {
+                hipGraph_t g = nullptr;
+                hipError_t er = hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal);
+                if (er == hipSuccess) {
+                    for (int k = 0; k < max_steps; ++k) enqueue_pair_step(e);
+                    er = hipStreamEndCapture(e->stream, &g);
+                }
+                hipGraphExec_t ge = nullptr;
+                if (er == hipSuccess) er = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+                if (g) (void)hipGraphDestroy(g);
+                if (er != hipSuccess) return ens_fail(e, TLW_EHIP, std::string("graph capture: ") + hipGetErrorString(er));
+                it = e->graph_cache.emplace(key, ge).first;
+            }
+            hipError_t er = hipGraphLaunch(it->second, e->stream);
+            if (er != hipSuccess) return ens_fail(e, TLW_EHIP, std::string("graph launch: ") + hipGetErrorString(er));
+        } else {
+            for (int k = 0; k < max_steps; ++k) enqueue_pair_step(e);
+        }
+    }
+    hipError_t er = hipGetLastError();
+    if (er != hipSuccess) return ens_fail(e, TLW_EHIP, std::string("kernel launch: ") + hipGetErrorString(er));
+    bool fin = true, bad = false;
+    for (tlw_ctx* c : {c3, c4}) {
+        int rc = sync_state(c);
+        if (rc) return ens_fail(e, rc, c->err);
+        for (int k = 0; k < c->ninst; ++k) {
+            c->host_cur[k] = c->h_state[k].cur;
+            fin = fin && c->h_state[k].finished;
+        }
+        bad = bad || any_bad(c);
+    }
+    if (finished) *finished = fin ? 1 : 0;
+    if (bad) return ens_fail(e, TLW_EADMISSIBLE, "non-admissible state (rho <= 0 or p <= 0)");
+    return TLW_OK;
+}
+
+const char* tlw_ensemble_last_error(const tlw_ensemble* e) { return e ? e->err.c_str() : "null ensemble"; }
+
+void tlw_ensemble_destroy(tlw_ensemble* e) {
+    if (!e) return;
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    ens_drop_graphs(e);
+    delete e;
 }
 
 }  // extern "C"

@@ -402,11 +402,14 @@ __device__ __forceinline__ double* sel_out(const KArgs& a, int cur) {
 }
 
 // A workgroup-table entry, read once per workgroup into scalar registers.
-__device__ __forceinline__ int4 load_blk(const int4* t) {
-    const int4 b = t[blockIdx.x];
+// bid is the workgroup's index in its own table (blockIdx.x, or the index
+// within its half of a two-context launch).
+__device__ __forceinline__ int4 load_blk_at(const int4* t, int bid) {
+    const int4 b = t[bid];
     return make_int4(__builtin_amdgcn_readfirstlane(b.x), __builtin_amdgcn_readfirstlane(b.y),
                      __builtin_amdgcn_readfirstlane(b.z), 0);
 }
+__device__ __forceinline__ int4 load_blk(const int4* t) { return load_blk_at(t, blockIdx.x); }
 
 // Rusanov flux of one face node from the two face records (left = -side):
 // F* = ½(F̃n_L + F̃n_R) − ½ λ (q_R − q_L), λ = max(|v_n|) + max(c)
@@ -1981,21 +1984,22 @@ template <bool TWO> constexpr int kVolWaves = TWO ? 2 : TLW_VOL_WAVES;
 
 // PAR: hyperbolic-parabolic (compressible Navier-Stokes) element procedures,
 // same register budget as shock capturing.
-template <int N, int EB, int KIND, bool SC, bool PAR = false, bool CURV = false>
-__global__ __launch_bounds__((VGeo<N, EB>::NT), kVolWaves<SC || PAR>) void k_volume(KArgs a) {
+// The volume workgroup `bid` of a's table, with its LDS: lds [G_::LDS], sDm,
+// sDh, sD [NN] (-dt D, D̂ and D, row-major) and s_alpha [EB] (smoothed blending
+// factors of the curvilinear interfaces).
+template <int N, int EB, int KIND, bool SC, bool PAR, bool CURV>
+__device__ __forceinline__ void volume_body(const KArgs& a, int bid, double* lds, double* sDm, double* sDh, double* sD,
+                                            double* s_alpha) {
     static_assert(!(SC && PAR), "shock capturing has no hyperbolic-parabolic form");
     static_assert(!CURV || KIND != VOL_LW_ADAPT, "the curvilinear path is LW or MDRK");
     static_assert(KIND != VOL_LW_WALL || (PAR && !CURV && !SC), "walls: the TreeMesh Navier-Stokes LW kernel");
     static_assert(!(CURV && PAR && SC), "no shock capturing with parabolic terms");
     using G_ = VGeo<N, EB>;
     constexpr int n = G_::n, NN = G_::NN, S = G_::S, NT = G_::NT;
-    __shared__ double lds[G_::LDS];
-    __shared__ double sDm[NN], sDh[NN], sD[NN];  // -dt D, D̂ and D, row-major
-    __shared__ double s_alpha[(SC && CURV) ? EB : 1];  // smoothed blending factors (curvilinear interfaces)
     double* s0 = lds;
     double* s1 = lds + 8 * S;
     double* s2 = lds + 16 * S;
-    const int4 blk = load_blk(a.vblk);
+    const int4 blk = load_blk_at(a.vblk, bid);
     DevState* st = a.st + blk.z;
     if (a.use_state_dt && st->finished) return;
     const int tid = threadIdx.x;
@@ -2335,11 +2339,20 @@ __global__ __launch_bounds__((VGeo<N, EB>::NT), kVolWaves<SC || PAR>) void k_vol
         if (tid == 0) {
             double r = 0.0;
             for (int k = 0; k < nel; ++k) r += es[k];
-            a.partial[blockIdx.x] = r;
+            a.partial[bid] = r;
         }
     }
     // the CFL reduction of a step (and the admissibility test) is the surface
     // kernel's: it reads every node once after the last correction
+}
+
+template <int N, int EB, int KIND, bool SC, bool PAR = false, bool CURV = false>
+__global__ __launch_bounds__((VGeo<N, EB>::NT), kVolWaves<SC || PAR>) void k_volume(KArgs a) {
+    using G_ = VGeo<N, EB>;
+    __shared__ double lds[G_::LDS];
+    __shared__ double sDm[G_::NN], sDh[G_::NN], sD[G_::NN];
+    __shared__ double s_alpha[(SC && CURV) ? EB : 1];
+    volume_body<N, EB, KIND, SC, PAR, CURV>(a, blockIdx.x, lds, sDm, sDh, sD, s_alpha);
 }
 
 // Outer record of a domain-boundary face node: time average of the boundary
@@ -2421,11 +2434,11 @@ __device__ inline void boundary_record_curved(const KArgs& a, int e, int face, i
 // final value (corrected here or finalised by the volume kernel) into the
 // CFL maxima and the admissibility test.
 template <int N, bool BC, bool SC, bool CURV = false>
-__global__ __launch_bounds__(kSurfBlock) void k_surface(KArgs a) {
+__device__ __forceinline__ void surface_body(const KArgs& a, int bid) {
     using G_ = SGeo<N>;
     constexpr int n = G_::n, NN = G_::NN, EB = G_::EB, S = G_::S;
     __shared__ double s_l1[S], s_l2[S], s_m[EB], s_lv[S], s_mv[EB];
-    const int4 blk = load_blk(a.sblk);
+    const int4 blk = load_blk_at(a.sblk, bid);
     DevState* st = a.st + blk.z;
     if (a.use_state_dt && st->finished) return;
     const int tid = threadIdx.x;
@@ -2601,6 +2614,11 @@ __global__ __launch_bounds__(kSurfBlock) void k_surface(KArgs a) {
         slot_max(a.slots, blk.z, 0, bm);
         if (par) slot_max(a.slots, blk.z, 1, bv);
     }
+}
+
+template <int N, bool BC, bool SC, bool CURV = false>
+__global__ __launch_bounds__(kSurfBlock) void k_surface(KArgs a) {
+    surface_body<N, BC, SC, CURV>(a, blockIdx.x);
 }
 
 // CFL reduction over a whole state (Trixi max_dt) without an update: first dt
@@ -2887,14 +2905,26 @@ struct DtLog {
 // parabolic bound 1/((λa + λv)·2(2N+1)), λv = 2(2N+1)·max J⁻² max(4μ/(3ρ), μκ/(ρPr)),
 // and max_dt is the smaller of the two (compute_dt for
 // SemidiscretizationHyperbolicParabolic, lwfr.jl:192-210).
+// The arguments of one context's finalize launch.
+struct FinArgs {
+    DevState* st_all;
+    const double* partial;
+    const int4* rng;
+    int use_vol, reduce, pv_off;
+    double* slots;
+    DtLog dl;
+};
+
+// Instance `bid` of the context, with LDS s, sv [kFinalizeBlock].
 template <int N>
-__global__ __launch_bounds__(kFinalizeBlock) void k_finalize(DevState* st_all, const double* partial, const int4* rng,
-                                                           int use_vol, int reduce, int pv_off, double* slots,
-                                                           DtLog dl) {
+__device__ __forceinline__ void finalize_body(const FinArgs& f, int bid, double* s, double* sv) {
     constexpr int n = N + 1;
-    __shared__ double s[kFinalizeBlock], sv[kFinalizeBlock];
-    DevState* st = st_all + blockIdx.x;
-    const int4 r = rng[blockIdx.x];
+    const double* partial = f.partial;
+    const int reduce = f.reduce, pv_off = f.pv_off;
+    double* slots = f.slots;
+    const DtLog& dl = f.dl;
+    DevState* st = f.st_all + bid;
+    const int4 r = f.rng[bid];
     if (reduce == RED_STEP && st->finished) return;
     if (reduce == RED_STEP && st->adaptive) {
         // sum of the volume workgroups' temporal-error partials, then the controller
@@ -2917,7 +2947,7 @@ __global__ __launch_bounds__(kFinalizeBlock) void k_finalize(DevState* st_all, c
     // the instance's CFL (and viscous) slots, reset for the next reduction
     {
         const int t = (int)threadIdx.x;
-        double* sl = slots + (size_t)blockIdx.x * 2 * kSlots;
+        double* sl = slots + (size_t)bid * 2 * kSlots;
         s[t] = 0.0;
         sv[t] = 0.0;
         if (t < kSlots) {
@@ -2965,6 +2995,43 @@ __global__ __launch_bounds__(kFinalizeBlock) void k_finalize(DevState* st_all, c
     st->dt = dt;
     if (dl.log && it < dl.log_cap) dl.log[it] = dt;
     st->finished = julia_isapprox(t, tf);
+}
+
+template <int N>
+__global__ __launch_bounds__(kFinalizeBlock) void k_finalize(FinArgs f) {
+    __shared__ double s[kFinalizeBlock], sv[kFinalizeBlock];
+    finalize_body<N>(f, blockIdx.x, s, sv);
+}
+
+// Two contexts advanced by the same launches (tlw_create_ensemble): one of
+// polydeg 3 (workgroups [0, g3) of each launch) and one of polydeg 4, plain LW
+// on periodic meshes; each half runs its own context's kernel body on its own
+// tables, so every instance gets the bits of its own context's launches.
+template <int EB3, int EB4>
+__global__ __launch_bounds__((VGeo<4, EB4>::NT), kVolWaves<false>) void k_volume_pair(KArgs a3, KArgs a4, int g3) {
+    static_assert(VGeo<3, EB3>::NT == VGeo<4, EB4>::NT, "one block size for both halves");
+    constexpr int L = VGeo<3, EB3>::LDS > VGeo<4, EB4>::LDS ? VGeo<3, EB3>::LDS : VGeo<4, EB4>::LDS;
+    __shared__ double lds[L];
+    __shared__ double sDm[25], sDh[25], sD[25], s_alpha[1];
+    if ((int)blockIdx.x < g3)
+        volume_body<3, EB3, VOL_LW, false, false, false>(a3, blockIdx.x, lds, sDm, sDh, sD, s_alpha);
+    else
+        volume_body<4, EB4, VOL_LW, false, false, false>(a4, blockIdx.x - g3, lds, sDm, sDh, sD, s_alpha);
+}
+
+__global__ __launch_bounds__(kSurfBlock) void k_surface_pair(KArgs a3, KArgs a4, int g3) {
+    if ((int)blockIdx.x < g3)
+        surface_body<3, false, false, false>(a3, blockIdx.x);
+    else
+        surface_body<4, false, false, false>(a4, blockIdx.x - g3);
+}
+
+__global__ __launch_bounds__(kFinalizeBlock) void k_finalize_pair(FinArgs f3, FinArgs f4, int n3) {
+    __shared__ double s[kFinalizeBlock], sv[kFinalizeBlock];
+    if ((int)blockIdx.x < n3)
+        finalize_body<3>(f3, blockIdx.x, s, sv);
+    else
+        finalize_body<4>(f4, blockIdx.x - n3, s, sv);
 }
 
 // Initialisation of the device integrator state of every instance.

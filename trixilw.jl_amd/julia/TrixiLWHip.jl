@@ -196,6 +196,26 @@ function structured_neighbors(nx, ny)
     return nbr
 end
 
+# An ensemble of uneven instances: a polydeg-3 and a polydeg-4 batch context advanced by the same
+# launches (one volume, surface and finalize launch per step for all instances)
+function create_ensemble(ctx_a, ctx_b)
+    ens = Ref{Ptr{Cvoid}}(C_NULL)
+    check(C_NULL, ccall((:tlw_create_ensemble, lib), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ref{Ptr{Cvoid}}), ctx_a, ctx_b, ens))
+    return ens[]
+end
+ensemble_error(ens) = unsafe_string(ccall((:tlw_ensemble_last_error, lib), Cstring, (Ptr{Cvoid},), ens))
+ensemble_check(ens, rc) = rc == TLW_OK ? nothing : error("libtrixilw_hip error $rc: " * ensemble_error(ens))
+ensemble_set_stream!(ens, stream::Ptr{Cvoid}) =
+    ensemble_check(ens, ccall((:tlw_ensemble_set_stream, lib), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), ens, stream))
+ensemble_enable_graphs!(ens, on::Bool) =
+    ensemble_check(ens, ccall((:tlw_ensemble_enable_graphs, lib), Cint, (Ptr{Cvoid}, Int32), ens, on))
+ensemble_solve_init!(ens, t0, tf, cfl3, cfl4) = ensemble_check(ens, ccall((:tlw_ensemble_solve_init, lib), Cint,
+    (Ptr{Cvoid}, Float64, Float64, Float64, Float64), ens, t0, tf, cfl3, cfl4))
+ensemble_advance!(ens, max_steps) = (fin = Ref{Int32}(0);
+    ensemble_check(ens, ccall((:tlw_ensemble_advance, lib), Cint, (Ptr{Cvoid}, Int32, Ref{Int32}), ens, max_steps, fin));
+    fin[] != 0)
+ensemble_destroy!(ens) = ccall((:tlw_ensemble_destroy, lib), Cvoid, (Ptr{Cvoid},), ens)
+
 # ---------------------------------------------------------------------------------------------
 # rhs!, rhs_mdrk1!, rhs_mdrk2! (semidiscretization_hyperbolic.jl:4-56) on device arrays;
 # dt is semi.cache.dt[1] as in the reference

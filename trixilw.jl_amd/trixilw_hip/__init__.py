@@ -7,7 +7,7 @@ library; the first call that needs the GPU does, and fails if it is missing.
 """
 from .adjoint import adjoint_step, gaussian_random_field, gradient_lw, kinetic_energy, objective_lw
 from .basis import LobattoLegendreBasis
-from .batch import SemidiscretizationBatch, draw_vortex_ensemble
+from .batch import SemidiscretizationBatch, SemidiscretizationEnsemble, draw_vortex_ensemble
 from .equations import (Adiabatic, BoundaryConditionDirichlet, BoundaryConditionNavierStokesWall, CompressibleEulerEquations2D,
                         CompressibleNavierStokesDiffusion2D, ConstantState, Isothermal, NoSlip,
                         FourQuadrantRiemann, GradientVariablesConservative, GradientVariablesPrimitive,
@@ -36,6 +36,6 @@ __all__ = [
     "FirstOrderReconstruction", "IndicatorHennemannGassner", "MUSCLHancockReconstruction",
     "VolumeIntegralFRShockCapturing", "density_pressure", "PositivityPreservingLimiterZhangShu", "density",
     "pressure", "adjoint_step", "gradient_lw", "kinetic_energy", "gaussian_random_field", "objective_lw",
-    "SemidiscretizationBatch", "draw_vortex_ensemble", "SemidiscretizationHyperbolicParabolic", "CompressibleNavierStokesDiffusion2D",
+    "SemidiscretizationBatch", "SemidiscretizationEnsemble", "draw_vortex_ensemble", "SemidiscretizationHyperbolicParabolic", "CompressibleNavierStokesDiffusion2D",
     "GradientVariablesConservative", "GradientVariablesPrimitive",
 ]

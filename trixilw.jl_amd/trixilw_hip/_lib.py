@@ -31,6 +31,8 @@ EXPORTED = (
     "tlw_create_parabolic", "tlw_solve_init_adaptive", "tlw_rejected_steps", "tlw_create_curved",
     "tlw_create_curved_parabolic", "tlw_max_dt_noalloc", "tlw_record_dts", "tlw_get_dts", "tlw_prescribe_dts",
     "tlw_set_state_dev", "tlw_get_state_dev", "tlw_boundary_nodes", "tlw_set_boundary_data",
+    "tlw_create_ensemble", "tlw_ensemble_set_stream", "tlw_ensemble_enable_graphs", "tlw_ensemble_solve_init",
+    "tlw_ensemble_advance", "tlw_ensemble_last_error", "tlw_ensemble_destroy",
 )
 
 
@@ -193,6 +195,20 @@ def lib():
     L.tlw_adjoint_lw.argtypes = [vp, vp, vp, vp, C.c_double, C.c_double]
     L.tlw_autotune.restype = C.c_int
     L.tlw_autotune.argtypes = [vp, i32, C.POINTER(i32), dp]
+    L.tlw_create_ensemble.restype = C.c_int
+    L.tlw_create_ensemble.argtypes = [vp, vp, C.POINTER(vp)]
+    L.tlw_ensemble_set_stream.restype = C.c_int
+    L.tlw_ensemble_set_stream.argtypes = [vp, vp]
+    L.tlw_ensemble_enable_graphs.restype = C.c_int
+    L.tlw_ensemble_enable_graphs.argtypes = [vp, i32]
+    L.tlw_ensemble_solve_init.restype = C.c_int
+    L.tlw_ensemble_solve_init.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_double]
+    L.tlw_ensemble_advance.restype = C.c_int
+    L.tlw_ensemble_advance.argtypes = [vp, i32, C.POINTER(i32)]
+    L.tlw_ensemble_last_error.restype = C.c_char_p
+    L.tlw_ensemble_last_error.argtypes = [vp]
+    L.tlw_ensemble_destroy.restype = None
+    L.tlw_ensemble_destroy.argtypes = [vp]
     L.tlw_boundary_nodes.restype = C.c_int
     L.tlw_boundary_nodes.argtypes = [vp, C.POINTER(C.c_int64), dp, sz]
     L.tlw_set_boundary_data.restype = C.c_int

@@ -151,6 +151,98 @@ class SemidiscretizationBatch:
         return {"t": t, "dt": dt, "iter": it, "finished": fin.astype(bool)}
 
 
+class SemidiscretizationEnsemble:
+    """Uneven instances of two polynomial degrees in one set of launches
+    (tlw_create_ensemble): ``batches`` is a SemidiscretizationBatch of polydeg 3
+    and one of polydeg 4 (plain LW, periodic); every step of ``solve`` /
+    ``advance`` is one volume, one surface and one finalize launch for all
+    instances of both, optionally replayed from a captured HIP graph.  The
+    batches keep their states and statuses."""
+
+    TILES = ((16, 10), (8, 5))  # (polydeg 3, polydeg 4) elements per volume workgroup, equal block sizes
+
+    def __init__(self, batches):
+        if len(batches) != 2:
+            raise ValueError("an ensemble joins one polydeg-3 and one polydeg-4 batch")
+        self.batches = sorted(batches, key=lambda b: b.n)
+        h = C.c_void_p()
+        L = _lib.lib()
+        rc = L.tlw_create_ensemble(self.batches[0].handle, self.batches[1].handle, C.byref(h))
+        if rc != _lib.TLW_OK:
+            raise _lib.TLWError(rc, L.tlw_last_error(None).decode())
+        self._h = h
+        self.set_tiles(*self.TILES[0])
+
+    def _check(self, rc):
+        if rc != _lib.TLW_OK:
+            msg = _lib.lib().tlw_ensemble_last_error(self._h).decode()
+            raise (_lib.NonAdmissibleState if rc == _lib.TLW_EADMISSIBLE else _lib.TLWError)(rc, msg)
+
+    def set_tiles(self, eb3, eb4):
+        if (eb3, eb4) not in self.TILES:
+            raise ValueError(f"tiles must be one of {self.TILES}")
+        L = _lib.lib()
+        for b, eb in zip(self.batches, (eb3, eb4)):
+            _lib.check(L.tlw_set_elements_per_workgroup(b.handle, int(eb)), b.handle)
+        self.tiles = (eb3, eb4)
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _lib.lib().tlw_ensemble_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def ndofs(self):
+        return sum(b.ndofs() for b in self.batches)
+
+    def init(self, u0s, cfls, t0=0.0, t_final=float("inf"), stream=None, graphs=False):
+        """Loads the batches' initial states (host arrays, polydeg 3 first) and
+        runs tlw_ensemble_solve_init with a CFL number per batch."""
+        L = _lib.lib()
+        if stream is not None:
+            self._check(L.tlw_ensemble_set_stream(self._h, C.c_void_p(stream.cuda_stream)))
+        elif graphs:
+            raise ValueError("graph replay needs an explicit stream")
+        self._check(L.tlw_ensemble_enable_graphs(self._h, int(bool(graphs))))
+        dp = C.POINTER(C.c_double)
+        for b, u0 in zip(self.batches, u0s):
+            u = np.ascontiguousarray(u0, dtype=np.float64)
+            _lib.check(L.tlw_set_state(b.handle, u.ctypes.data_as(dp), u.size), b.handle)
+        self._check(L.tlw_ensemble_solve_init(self._h, float(t0), float(t_final), float(cfls[0]), float(cfls[1])))
+
+    def advance(self, nsteps):
+        """nsteps steps of every instance; returns True when all are finished."""
+        fin = C.c_int32()
+        self._check(_lib.lib().tlw_ensemble_advance(self._h, int(nsteps), C.byref(fin)))
+        return bool(fin.value)
+
+    def states(self):
+        """The batches' current states (host arrays, polydeg 3 first)."""
+        L = _lib.lib()
+        out = []
+        for b in self.batches:
+            u = np.empty((sum(b.sizes), b.n, b.n, 4))
+            _lib.check(L.tlw_get_state(b.handle, u.ctypes.data_as(C.POINTER(C.c_double)), u.size), b.handle)
+            out.append(u)
+        return out
+
+    def solve(self, u0s, cfls, nsteps, stream=None, graphs=False, chunk=None):
+        self.init(u0s, cfls, stream=stream, graphs=graphs)
+        chunk = chunk or nsteps
+        done = 0
+        while done < nsteps:
+            k = min(chunk, nsteps - done)
+            if self.advance(k):
+                break
+            done += k
+        return self.states(), [b.status() for b in self.batches]
+
+
 def draw_vortex_ensemble(n_instances=64, seed=2024):
     """The batching configuration's draw with numpy default_rng(seed): per
     instance a TreeMesh level in {4, 5, 6, 7} (16² .. 128² elements), polydeg
