@@ -354,6 +354,17 @@ int tlw_kernel_times(tlw_ctx* ctx, double* volume_ms, int64_t* volume_launches, 
 int tlw_adjoint_lw(tlw_ctx* ctx, const double* u_dev, const double* lam_next_dev, double* lam_dev, double t,
                    double dt);
 
+/* Discrete adjoint of one MDRK step (plain MDRK, periodic mesh, TreeMesh flux order): with
+ * us = u + dt rhs_mdrk1!(u) and u' = u + dt rhs_mdrk2!(u, us) (src/solvers/mdrk.jl:13-25),
+ * lam = lam_next + dt (∂R2/∂u)ᵀ lam_next + λ_us + dt (∂R1/∂u)ᵀ λ_us, λ_us = dt (∂R2/∂us)ᵀ lam_next.
+ * No counterpart in the reference; it differentiates the stage residuals of
+ * tree/multi_derivative_2d.jl:263-555 (F̃ = f/2 + f_t/8, then f + f_t(uⁿ)/6 + f_t(us)/3) and the
+ * TreeMesh Rusanov flux, with dt frozen.  us is the stage state of the step (device).  Uses the
+ * context's trace buffer and a state-sized buffer of its own as scratch; lam must not alias u, us or
+ * lam_next. */
+int tlw_adjoint_mdrk(tlw_ctx* ctx, const double* u_dev, const double* us_dev, const double* lam_next_dev,
+                     double* lam_dev, double t, double dt);
+
 /* PositivityPreservingLimiterZhangShu(thresholds = (threshold_density, threshold_pressure),
  * variables = (density, pressure)) as a stage limiter of solve_lwfr
  * (src/callbacks/callbacks_stage/positivity_zhang_shu.jl:10-41, positivity_zhang_shu_dg2d.jl:8-45;

@@ -100,6 +100,8 @@ def lib():
         L.tlwo_set_alpha.argtypes = [C.c_void_p, dp]
         L.tlwo_positivity_limiter.argtypes = [C.c_void_p, dp]
         L.tlwo_solve_fixed_dt.argtypes = [C.c_void_p, dp, C.c_double, dp, C.c_int]
+        L.tlwo_tangent.restype = C.c_int
+        L.tlwo_tangent.argtypes = [C.POINTER(_Desc), dp, dp, dp, C.c_int, C.c_int, dp, dp, dp]
         L.tlwo_tangent_lw.restype = C.c_int
         L.tlwo_tangent_lw.argtypes = [C.POINTER(_Desc), dp, dp, dp, C.c_int, dp, dp]
         L.tlwo_inverse_vandermonde.argtypes = [C.c_void_p, dp]
@@ -275,6 +277,20 @@ class Oracle:
         d = np.ascontiguousarray(dts, dtype=np.float64)
         u, du = np.zeros_like(u0), np.zeros_like(u0)
         rc = lib().tlwo_tangent_lw(C.byref(self._desc), _ptr(u0), _ptr(du0), _ptr(d), len(d), _ptr(u), _ptr(du))
+        if rc:
+            raise ValueError("tangent needs polydeg 3 or 4")
+        return u, du
+
+    def tangent(self, u0, du0, dts, mdrk=False, alpha=None):
+        """Forward-mode tangent of the fixed-dt LW or MDRK loop, optionally with
+        the blending factors frozen at ``alpha`` [E]: returns (u_K, du_K)."""
+        u0 = np.ascontiguousarray(u0, dtype=np.float64)
+        du0 = np.ascontiguousarray(du0, dtype=np.float64)
+        d = np.ascontiguousarray(dts, dtype=np.float64)
+        a = None if alpha is None else np.ascontiguousarray(alpha, dtype=np.float64)
+        u, du = np.zeros_like(u0), np.zeros_like(u0)
+        rc = lib().tlwo_tangent(C.byref(self._desc), _ptr(u0), _ptr(du0), _ptr(d), len(d), int(mdrk),
+                                None if a is None else _ptr(a), _ptr(u), _ptr(du))
         if rc:
             raise ValueError("tangent needs polydeg 3 or 4")
         return u, du

@@ -1589,10 +1589,12 @@ int solve(Solver<T, Eq>& s, T* u, double t0, double tf, double cfl, int max_step
     return it;
 }
 
-// The LW loop of solve_lwfr with a prescribed dt per step, as the gradient
-// checks use it: the CFL dts of a reference run are frozen, so the map
-// u0 -> u_K is differentiable (compute_dt's max is not).
-template <class T, class Eq> void solve_fixed_dt(Solver<T, Eq>& s, T* u, double t0, const double* dts, int nsteps) {
+// The LW (or MDRK) loop of solve_lwfr with a prescribed dt per step, as the
+// gradient checks use it: the CFL dts of a reference run are frozen, so the map
+// u0 -> u_K is differentiable (compute_dt's max is not).  MDRK: us = uprev +
+// dt rhs_mdrk1!(u), then u = uprev + dt rhs_mdrk2!(u) (mdrk.jl:13-25).
+template <class T, class Eq>
+void solve_fixed_dt(Solver<T, Eq>& s, T* u, double t0, const double* dts, int nsteps, bool mdrk = false) {
     const long long nd = (long long)s.ndofs_total();
     std::vector<T> du(nd), uprev(nd);
     double t = t0;
@@ -1601,7 +1603,14 @@ template <class T, class Eq> void solve_fixed_dt(Solver<T, Eq>& s, T* u, double 
         const double dt = dts[k];
 #pragma omp parallel for schedule(static)
         for (long long q = 0; q < nd; ++q) uprev[q] = u[q];
-        s.rhs_lw(du.data(), u, t, dt);
+        if (mdrk) {
+            s.rhs_mdrk1(du.data(), u, t, dt);
+#pragma omp parallel for schedule(static)
+            for (long long q = 0; q < nd; ++q) s.us[q] = uprev[q] + dt * du[q];
+            s.rhs_mdrk2(du.data(), u, t, dt);
+        } else {
+            s.rhs_lw(du.data(), u, t, dt);
+        }
 #pragma omp parallel for schedule(static)
         for (long long q = 0; q < nd; ++q) u[q] = uprev[q] + dt * du[q];
         if (s.positivity) s.positivity_limiter(u);

@@ -331,6 +331,27 @@ void tlwo_solve_fixed_dt(void* ctx, double* u, double t0, const double* dts, int
 
 // Tangent of the fixed-dt LW loop: u0 + ε du0 -> u_K + ε du_K with dual numbers
 // (the forward-mode gradient check; Euler only).
+// Forward-mode tangent of the fixed-dt loop (LW, or MDRK with mdrk != 0);
+// alpha (E values, or NULL) freezes the shock-capturing blending factors.
+int tlwo_tangent(const tlwo_desc* d, const double* u0, const double* du0, const double* dts, int nsteps, int mdrk,
+                 const double* alpha, double* u_out, double* du_out) {
+    if (d->polydeg != 3 && d->polydeg != 4) return 1;
+    if (d->nthreads > 0) omp_set_num_threads(d->nthreads);
+    Solver<Dual, Euler2D<Dual>> s;
+    s.eq.gamma = d->gamma;
+    configure(s, d);
+    if (alpha) s.alpha_override.assign(alpha, alpha + s.mesh.E);
+    const size_t nd = s.ndofs_total();
+    std::vector<Dual> u(nd);
+    for (size_t k = 0; k < nd; ++k) u[k] = Dual(u0[k], du0[k]);
+    solve_fixed_dt(s, u.data(), 0.0, dts, nsteps, mdrk != 0);
+    for (size_t k = 0; k < nd; ++k) {
+        u_out[k] = u[k].v;
+        du_out[k] = u[k].d;
+    }
+    return 0;
+}
+
 int tlwo_tangent_lw(const tlwo_desc* d, const double* u0, const double* du0, const double* dts, int nsteps,
                     double* u_out, double* du_out) {
     if (d->polydeg != 3 && d->polydeg != 4) return 1;

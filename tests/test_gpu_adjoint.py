@@ -127,6 +127,66 @@ def test_c3_gradient_64x64_200_steps(oracle_mod):
     semi.close()
 
 
+@pytest.mark.parametrize("N", [3, 4])
+def test_adjoint_mdrk_step_matches_tangent(oracle_mod, N):
+    """One MDRK step (both stages): <λ, v> = <λ', du'> with du' the oracle's
+    dual-number tangent of u' = u + dt R2(u, u + dt R1(u)) along v, ≤ 1e-12."""
+    import trixilw_hip as tlw
+
+    torch = torch_mod()
+    semi, o, u_lex, mesh = setup(oracle_mod, N, 3, td="mdrk", ordering="strips", seed=70 + N, amp=1e-2)
+    dt = 0.4 * o.max_dt(u_lex)
+    rng = np.random.default_rng(80 + N)
+    v = rng.standard_normal(u_lex.shape)
+    lam_next = rng.standard_normal(u_lex.shape)
+    _, du1 = o.tangent(u_lex, v, [dt], mdrk=True)
+    ref = float(np.sum(lam_next * du1))
+    u = to_dev(mesh, u_lex)
+    du = torch.empty_like(u)
+    us = tlw.adjoint.mdrk_stage_state(u, semi, 0.0, dt, du)
+    lam = torch.empty_like(u)
+    tlw.adjoint_step_mdrk(lam, u, us, to_dev(mesh, lam_next), semi, 0.0, dt)
+    torch.cuda.synchronize()
+    got = float(np.sum(to_lex(mesh, lam) * v))
+    assert abs(got - ref) <= 1e-12 * abs(ref), (got, ref)
+    assert abs(float(np.sum(lam_next * v)) - ref) > 1e-3 * abs(ref)
+    semi.close()
+
+
+def test_checkpointed_gradient_mdrk_matches_tangent_and_fd(oracle_mod):
+    """d KE(u_K) / d u0 through 20 MDRK steps (N=3, 16² vortex, checkpoints
+    every 5): the adjoint against the oracle's MDRK tangent (≤ 1e-10) and
+    device central differences with h = 1e-6 (≤ 1e-6), dts frozen."""
+    import trixilw_hip as tlw
+
+    torch = torch_mod()
+    ic = tlw.IsentropicVortex()
+    semi, o, u_lex, mesh = setup(oracle_mod, 3, 4, td="mdrk", ic=ic, amp=0.0, keep_aux=False)
+    x, y = o.node_coords()
+    u_lex = u_lex.copy()
+    u_lex[..., 0] += tlw.gaussian_random_field(x, y, (20.0, 20.0), seed=7, amplitude=1e-2)
+    v = np.zeros_like(u_lex)
+    v[..., 0] = tlw.gaussian_random_field(x, y, (20.0, 20.0), seed=8, amplitude=1.0)
+    cfl = 0.5 * tlw.trixi2lw(0.98, semi.solver)
+    J, grad, uK, dts = tlw.gradient_lw(semi, to_dev(mesh, u_lex), 20, cfl, checkpoint_every=5)
+    torch.cuda.synchronize()
+    u_ref, _, dts_ref = o.solve(u_lex, 0.0, 1e9, cfl, max_steps=20, mdrk=True)
+    assert np.max(np.abs(np.asarray(dts) - dts_ref) / dts_ref) <= 1e-12
+    dJ_adj = float(np.sum(to_lex(mesh, grad) * v))
+    _, w, _, _ = o.basis()
+    uK_t, duK = o.tangent(u_lex, v, dts, mdrk=True)
+    J_t, gJ = ke_and_grad_np(uK_t, w, 20.0 / mesh.nx)
+    dJ_tan = float(np.sum(gJ * duK))
+    assert abs(J - J_t) <= 1e-12 * abs(J_t)
+    assert abs(dJ_adj - dJ_tan) <= 1e-10 * abs(dJ_tan), (dJ_adj, dJ_tan)
+    h = 1e-6
+    Jp = tlw.objective_lw(semi, to_dev(mesh, u_lex + h * v), 20, dts)
+    Jm = tlw.objective_lw(semi, to_dev(mesh, u_lex - h * v), 20, dts)
+    fd = (Jp - Jm) / (2 * h)
+    assert abs(dJ_adj - fd) <= 1e-6 * abs(fd), (dJ_adj, fd)
+    semi.close()
+
+
 def test_adjoint_rejects_unsupported(oracle_mod):
     import trixilw_hip as tlw
 

@@ -229,3 +229,41 @@ def test_dual_tangent_matches_central_differences(oracle_mod):
     h = 1e-6
     fd = (o.solve_fixed_dt(u0 + h * v, dts) - o.solve_fixed_dt(u0 - h * v, dts)) / (2 * h)
     assert rel_linf(duK, fd) < 1e-7
+
+
+def test_dual_tangent_mdrk_and_frozen_blend_match_central_differences(oracle_mod):
+    """The dual-number tangent of the fixed-dt MDRK loop, and of the LW loop
+    with shock capturing at frozen blending factors, against central
+    differences of the same maps (the value part of the dual loop with the
+    same dts and factors); the MDRK value part equals the oracle's CFL solve
+    when fed that solve's dts."""
+    from trixilw_hip.equations import IsentropicVortex
+
+    o = make(oracle_mod, 3, 8)
+    x, y = o.node_coords()
+    u0 = IsentropicVortex()(x, y, 0.0)
+    rng = np.random.default_rng(13)
+    v = np.zeros_like(u0)
+    v[..., 0] = 1e-2 * rng.standard_normal(x.shape)
+    v[..., 2] = 1e-2 * rng.standard_normal(x.shape)
+    u_ref, _, dts = o.solve(u0, 0.0, 1e9, 0.3, max_steps=5, mdrk=True)
+    uK, duK = o.tangent(u0, v, dts, mdrk=True)
+    assert np.max(np.abs(uK - u_ref)) <= 1e-13 * np.max(np.abs(u_ref))
+    h = 1e-6
+    fd = (o.tangent(u0 + h * v, 0 * v, dts, mdrk=True)[0] - o.tangent(u0 - h * v, 0 * v, dts, mdrk=True)[0]) / (2 * h)
+    assert rel_linf(duK, fd) < 1e-7
+    # shock capturing with the blending factors frozen at a blocky pattern
+    osc = oracle_mod.Oracle(4, 8, 8, -10.0, -10.0, 2.5, shock_capturing=True, alpha_max=0.5)
+    x, y = osc.node_coords()
+    # node noise breaks the vortex's mirror symmetry, where the Rusanov speed's max has ties (kinks)
+    u0 = IsentropicVortex()(x, y, 0.0) * (1 + 1e-3 * rng.standard_normal(x.shape + (4,)))
+    v = np.zeros_like(u0)
+    v[..., 0] = 1e-2 * rng.standard_normal(x.shape)
+    v[..., 3] = 1e-2 * rng.standard_normal(x.shape)
+    alpha = rng.uniform(0.0, 0.5, 64) * (rng.uniform(size=64) < 0.5)
+    dts = [0.2 * osc.max_dt(u0)] * 4
+    uK, duK = osc.tangent(u0, v, dts, alpha=alpha)
+    fd = (osc.tangent(u0 + h * v, 0 * v, dts, alpha=alpha)[0] - osc.tangent(u0 - h * v, 0 * v, dts, alpha=alpha)[0]) / (2 * h)
+    assert rel_linf(duK, fd) < 1e-7
+    # the frozen factors matter
+    assert rel_linf(osc.tangent(u0, v, dts, alpha=0 * alpha)[1], duK) > 1e-4

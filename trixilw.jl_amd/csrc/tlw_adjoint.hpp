@@ -361,3 +361,299 @@ __global__ __launch_bounds__((VGeo<N, EB>::NT), 2) void k_adjoint(KArgs a) {
 }
 
 }  // namespace tlw
+
+namespace tlw {
+
+// ---------------------------------------------------------------------------
+// Discrete adjoint of one MDRK step on a periodic TreeMesh (plain FR, TreeMesh
+// flux order):  us = u + dt R1(u),  u' = u + dt R2(u, us)  (mdrk.jl:13-25), so
+//   λ_us = dt (∂R2/∂us)ᵀ λ',  λ = λ' + dt (∂R2/∂u)ᵀ λ' + λ_us + dt (∂R1/∂u)ᵀ λ_us.
+// R1 and R2 are the stage residuals of mdrk1_element / mdrk2_element
+// (tree/multi_derivative_2d.jl:263-555 via the structured kernels'
+// formulas, structured/multi_derivative_2d.jl:82-445):
+//   stage 1: ut = J⁻¹ C(f(u)),  F̃ = f(u)/2 + f_t(u, ut)/8,  Ũ = u/2 + ut/8;
+//   stage 2: ut, ust likewise from u and us,  F̃ = f(u) + f_t(u, ut)/6 +
+//            f_t(us, ust)/3,  Ũ = u + ut/6 + ust/3,
+// with f_t(u, ut) = (-f(u+2ut) + 8 f(u+ut) - 8 f(u-ut) + f(u-2ut))/12, C the
+// (-dt D) contraction, and the interface flux of the LW step (jump on uⁿ, λ
+// from Ũ).  Each stage is a record kernel (face node u, the λ ingredients of
+// Ũ, s̄ = -J⁻¹ dt λ_stage) and a reverse kernel, as for the LW step.
+
+struct AdjArgs {
+    int E;
+    const double* u;         // uⁿ
+    const double* us;        // stage 2: the stage state us
+    const double* lam;       // the adjoint of the stage's output (λ' or λ_us)
+    const double* lam_base;  // added to the result (stage 1: stage 2's λ), or null
+    double* lam_out;         // the adjoint of uⁿ
+    double* lam_us;          // stage 2: the adjoint of us
+    double* trace;           // [E][4][n][kAdjQ] face records
+    const int* nbr;
+    const double* inv_jac;
+    const DevConst* c;
+    double dt;
+};
+
+// FD coefficients and Taylor offsets of f_t(u, ut): kinds u+2ut, u+ut, u-ut, u-2ut
+struct MdrkFt {
+    static constexpr double a[4] = {2.0, 1.0, -1.0, -2.0};
+    static constexpr double c[4] = {-1.0 / 12.0, 8.0 / 12.0, -8.0 / 12.0, 1.0 / 12.0};
+};
+
+// ub += Σ_k c_k (∂f/∂w)ᵀ(dxb, dyb) at w_k = u + a_k ut, utb += a_k times the same
+__device__ __forceinline__ void mdrk_ft_vjp(const double* u, const double* ut, double gm1, const double* dxb,
+                                            const double* dyb, double* ub, double* utb) {
+#pragma unroll
+    for (int kind = 0; kind < 4; ++kind) {
+        double w[4], fb[4], gb[4], wb[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            w[v] = fma(MdrkFt::a[kind], ut[v], u[v]);
+            fb[v] = MdrkFt::c[kind] * dxb[v];
+            gb[v] = MdrkFt::c[kind] * dyb[v];
+        }
+        euler_fluxes_vjp(w, gm1, fb, gb, wb);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            ub[v] += wb[v];
+            utb[v] = fma(MdrkFt::a[kind], wb[v], utb[v]);
+        }
+    }
+}
+
+// Forward quantities of a stage the adjoint needs: ut (and ust) and Ũ.
+template <int N, int EB, int STAGE>
+__device__ __forceinline__ void mdrk_adj_forward(const double* uN, const double* usn, double invJ, double gm1, int i,
+                                                 int j, bool act, int sidx, int sbase, double* lds, const double* sDm,
+                                                 double* ut, double* ust, double* Uc) {
+    using G_ = VGeo<N, EB>;
+    constexpr int n = G_::n, S = G_::S;
+    double f[4], g[4];
+    euler_fluxes(uN, gm1, f, g);
+    if (act) put8<S>(lds, sidx, f, g);
+    if constexpr (STAGE == 2) {
+        euler_fluxes(usn, gm1, f, g);
+        if (act) put8<S>(lds + 8 * S, sidx, f, g);
+    }
+    __syncthreads();
+    contract<n, S>(lds, sbase, i, j, sDm, ut);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) ut[v] *= invJ;
+    if constexpr (STAGE == 2) {
+        contract<n, S>(lds + 8 * S, sbase, i, j, sDm, ust);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            ust[v] *= invJ;
+            Uc[v] = fma(1.0 / 3.0, ust[v], fma(1.0 / 6.0, ut[v], uN[v]));
+        }
+    } else {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Uc[v] = fma(0.125, ut[v], 0.5 * uN[v]);
+    }
+    __syncthreads();  // the exchange buffers are reused by the caller
+}
+
+template <int N, int EB, int STAGE>
+__global__ __launch_bounds__((VGeo<N, EB>::NT), 2) void k_adj_records_md(AdjArgs a) {
+    using G_ = VGeo<N, EB>;
+    constexpr int n = G_::n, NN = G_::NN;
+    __shared__ double lds[G_::LDS];
+    __shared__ double sDm[NN];
+    const int tid = threadIdx.x;
+    const int le = tid / NN, node = tid - (tid / NN) * NN;
+    const int i = node % n, j = node / n;
+    const int e0 = blockIdx.x * EB;
+    const int nel = min(EB, a.E - e0);
+    const bool act = le < nel;
+    const int lec = le < nel ? le : nel - 1;
+    const int e = e0 + lec;
+    const int sidx = lec * NN + node, sbase = lec * NN;
+    const DevConst* c = a.c;
+    const double dt = a.dt, invJ = a.inv_jac[e];
+    const size_t off = ((size_t)e * NN + node) * 4;
+    if (tid < NN) sDm[tid] = -dt * c->D[tid / n][tid % n];
+    double uN[4], usn[4] = {0, 0, 0, 0}, ut[4], ust[4], Uc[4];
+    load4(a.u + off, uN);
+    if constexpr (STAGE == 2) load4(a.us + off, usn);
+    mdrk_adj_forward<N, EB, STAGE>(uN, usn, invJ, c->gamma - 1.0, i, j, act, sidx, sbase, lds, sDm, ut, ust, Uc);
+    const int fx = (i == 0) ? 0 : ((i == n - 1) ? 1 : -1);
+    const int fy = (j == 0) ? 2 : ((j == n - 1) ? 3 : -1);
+    if (!act || (fx < 0 && fy < 0)) return;
+    double lam[4], r[kAdjQ], av1, av2, cs;
+    load4(a.lam + off, lam);
+    speed_parts2(Uc, c->gamma, av1, av2, cs);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        r[v] = uN[v];
+        r[6 + v] = (-invJ * dt) * lam[v];
+    }
+    r[5] = cs;
+    if (fx >= 0) {
+        r[4] = av1;
+        store_record<kAdjQ>(a.trace + (((size_t)e * 4 + fx) * n + j) * kAdjQ, r);
+    }
+    if (fy >= 0) {
+        r[4] = av2;
+        store_record<kAdjQ>(a.trace + (((size_t)e * 4 + fy) * n + i) * kAdjQ, r);
+    }
+}
+
+template <int N, int EB, int STAGE>
+__global__ __launch_bounds__((VGeo<N, EB>::NT), 2) void k_adjoint_md(AdjArgs a) {
+    using G_ = VGeo<N, EB>;
+    constexpr int n = G_::n, NN = G_::NN, S = G_::S;
+    __shared__ double lds[G_::LDS];
+    __shared__ double sDm[NN], sDh[NN];
+    const int tid = threadIdx.x;
+    const int le = tid / NN, node = tid - (tid / NN) * NN;
+    const int i = node % n, j = node / n;
+    const int e0 = blockIdx.x * EB;
+    const int nel = min(EB, a.E - e0);
+    const bool act = le < nel;
+    const int lec = le < nel ? le : nel - 1;
+    const int e = e0 + lec;
+    const int sidx = lec * NN + node, sbase = lec * NN;
+    const DevConst* c = a.c;
+    const double gamma = c->gamma, gm1 = gamma - 1.0;
+    const double dt = a.dt, invJ = a.inv_jac[e];
+    const size_t off = ((size_t)e * NN + node) * 4;
+    if (tid < NN) {
+        sDm[tid] = -dt * c->D[tid / n][tid % n];
+        sDh[tid] = c->Dhat[tid / n][tid % n];
+    }
+    double uN[4], usn[4] = {0, 0, 0, 0}, lam[4], ut[4], ust[4] = {0, 0, 0, 0}, Uc[4];
+    load4(a.u + off, uN);
+    if constexpr (STAGE == 2) load4(a.us + off, usn);
+    load4(a.lam + off, lam);
+    mdrk_adj_forward<N, EB, STAGE>(uN, usn, invJ, gm1, i, j, act, sidx, sbase, lds, sDm, ut, ust, Uc);
+
+    // --- interface fluxes of this node's faces (the LW step's: jump on uⁿ, λ from Ũ)
+    double sb[4], Fb[4] = {0, 0, 0, 0}, Gb[4] = {0, 0, 0, 0}, Ucb[4] = {0, 0, 0, 0}, ub[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int v = 0; v < 4; ++v) sb[v] = (-invJ * dt) * lam[v];
+    const int fx = (i == 0) ? 0 : ((i == n - 1) ? 1 : -1);
+    const int fy = (j == 0) ? 2 : ((j == n - 1) ? 3 : -1);
+    auto face = [&](int f, int l, double* Fnb) {
+        const int nb = a.nbr[4 * e + f];
+        double r[kAdjQ];
+        load_record<kAdjQ>(a.trace + (((size_t)nb * 4 + (f ^ 1)) * n + l) * kAdjQ, r);
+        const int orientation = f < 2 ? 1 : 2;
+        double av1, av2, cs;
+        speed_parts2(Uc, gamma, av1, av2, cs);
+        const double aown = orientation == 1 ? av1 : av2;
+        const bool own_left = (f & 1) != 0;
+        const double aL = own_left ? aown : r[4], aR = own_left ? r[4] : aown;
+        const double cL = own_left ? cs : r[5], cR = own_left ? r[5] : cs;
+        const double lamf = dmax(aL, aR) + dmax(cL, cR);
+        double lamb = 0.0;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const double sL = own_left ? sb[v] : r[6 + v];
+            const double sR = own_left ? r[6 + v] : sb[v];
+            const double Fsb = c->bnd_hi * sL - c->bnd_lo * sR;
+            const double jump = own_left ? r[v] - uN[v] : uN[v] - r[v];  // u_R - u_L
+            lamb = fma(-0.5 * Fsb, jump, lamb);
+            Fnb[v] = fma(0.5, Fsb, Fnb[v]);
+            ub[v] = fma(own_left ? 0.5 * lamf : -0.5 * lamf, Fsb, ub[v]);
+        }
+        const bool a_own = own_left ? !(aR > aL) : (aR > aL);
+        const bool c_own = own_left ? !(cR > cL) : (cR > cL);
+        if (a_own || c_own) {
+            double da[4], dc[4];
+            speed_grads(Uc, gamma, orientation, da, dc);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                if (a_own) Ucb[v] = fma(lamb, da[v], Ucb[v]);
+                if (c_own) Ucb[v] = fma(lamb, dc[v], Ucb[v]);
+            }
+        }
+    };
+    if (fx >= 0) face(fx, j, Fb);
+    if (fy >= 0) face(fy, i, Gb);
+
+    // --- volume term D̂·(F̃, G̃): (F̄, Ḡ) += D̂ᵀ s̄
+    double* ex0 = lds;
+    double* ex1 = lds + 4 * S;
+    if (act) put4<S>(ex0, sidx, sb);
+    __syncthreads();
+    {
+        double cf[4], cg[4];
+        contract_t<n, S>(ex0, sbase, i, j, sDh, cf, cg);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            Fb[v] += cf[v];
+            Gb[v] += cg[v];
+        }
+    }
+    // --- reverse of the stage's F̃ and Ũ
+    double f0b[4], g0b[4], utb[4], ustb[4] = {0, 0, 0, 0}, usb[4] = {0, 0, 0, 0};
+    {
+        constexpr double wf = STAGE == 2 ? 1.0 : 0.5;       // weight of f(u) in F̃
+        constexpr double wt = STAGE == 2 ? 1.0 / 6.0 : 0.125;  // weight of f_t(u, ut) and of ut in Ũ
+        constexpr double wu = STAGE == 2 ? 1.0 : 0.5;       // weight of u in Ũ
+        double dxb[4], dyb[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            ub[v] = fma(wu, Ucb[v], ub[v]);
+            utb[v] = wt * Ucb[v];
+            f0b[v] = wf * Fb[v];
+            g0b[v] = wf * Gb[v];
+            dxb[v] = wt * Fb[v];
+            dyb[v] = wt * Gb[v];
+        }
+        mdrk_ft_vjp(uN, ut, gm1, dxb, dyb, ub, utb);
+        if constexpr (STAGE == 2) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                ustb[v] = (1.0 / 3.0) * Ucb[v];
+                dxb[v] = (1.0 / 3.0) * Fb[v];
+                dyb[v] = (1.0 / 3.0) * Gb[v];
+            }
+            mdrk_ft_vjp(usn, ust, gm1, dxb, dyb, usb, ustb);
+        }
+    }
+    // ut = J⁻¹ C(f(u)) (and ust = J⁻¹ C(f(us))): (f̄, ḡ) += Cᵀ J⁻¹ ūt
+    __syncthreads();  // every lane has read ex0
+    {
+        double t4[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) t4[v] = invJ * utb[v];
+        if (act) put4<S>(ex1, sidx, t4);
+        if constexpr (STAGE == 2) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) t4[v] = invJ * ustb[v];
+            if (act) put4<S>(ex0, sidx, t4);
+        }
+    }
+    __syncthreads();
+    double cf[4], cg[4], wb[4];
+    contract_t<n, S>(ex1, sbase, i, j, sDm, cf, cg);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        f0b[v] += cf[v];
+        g0b[v] += cg[v];
+    }
+    euler_fluxes_vjp(uN, gm1, f0b, g0b, wb);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) ub[v] += wb[v];
+    if constexpr (STAGE == 2) {
+        contract_t<n, S>(ex0, sbase, i, j, sDm, cf, cg);
+        euler_fluxes_vjp(usn, gm1, cf, cg, wb);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) usb[v] += wb[v];
+        if (act) store4(a.lam_us + off, usb);
+    }
+    double out[4];
+    if (a.lam_base) {
+        double base[4];
+        load4(a.lam_base + off, base);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) out[v] = base[v] + (lam[v] + ub[v]);
+    } else {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) out[v] = lam[v] + ub[v];
+    }
+    if (act) store4(a.lam_out + off, out);
+}
+
+}  // namespace tlw

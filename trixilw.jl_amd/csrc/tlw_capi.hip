@@ -105,6 +105,7 @@ struct tlw_ctx {
     double* d_btab = nullptr;
     size_t btab_cap = 0;
     int btab_levels = 0;
+    double* d_adj_tmp = nullptr;  // the MDRK adjoint's λ_us (allocated on first use)
     // per-step dt record and prescribed dts of the device loop (tlw_record_dts, tlw_prescribe_dts)
     double* d_dtlog = nullptr;
     long long dtlog_cap = 0;
@@ -926,6 +927,7 @@ void tlw_destroy(tlw_ctx* c) {
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
     (void)hipFree(c->d_dtlog);
     (void)hipFree(c->d_dtpresc);
+    (void)hipFree(c->d_adj_tmp);
     (void)hipFree(c->d_const);
     delete c;
 }
@@ -1188,6 +1190,37 @@ int tlw_adjoint_lw(tlw_ctx* c, const double* u, const double* lam_next, double* 
         dim3 g((c->E + EB - 1) / EB), b(VGeo<3, EB>::NT);
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_adj_records<3, EB>), g, b, 0, c->stream, a);
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_adjoint<3, EB>), g, b, 0, c->stream, a);
+    }
+    return check_launch(c);
+}
+
+int tlw_adjoint_mdrk(tlw_ctx* c, const double* u, const double* us, const double* lam_next, double* lam, double t,
+                     double dt) {
+    if (!c) return TLW_EINVAL;
+    if (!u || !us || !lam_next || !lam) return fail(c, TLW_EINVAL, "null state pointer");
+    if (!std::isfinite(dt) || !std::isfinite(t)) return fail(c, TLW_EINVAL, "t and dt must be finite");
+    if (c->time_disc != TLW_MDRK || c->sc || c->parabolic || c->curved || c->has_boundary ||
+        c->hconst.flux_order != TLW_FLUX_TREE || c->ninst != 1)
+        return fail(c, TLW_ENOTSUP,
+                    "the MDRK adjoint covers plain MDRK on one periodic mesh with the TreeMesh flux order");
+    if (lam == lam_next || lam == u || lam == us) return fail(c, TLW_EINVAL, "lam must not alias u, us or lam_next");
+    if (!c->d_adj_tmp) HIPCHK(c, hipMalloc(&c->d_adj_tmp, sizeof(double) * state_len(c)));
+    AdjArgs a2{c->E, u, us, lam_next, nullptr, lam, c->d_adj_tmp, c->d_trace, c->d_nbr, c->d_invj, c->d_const, dt};
+    AdjArgs a1{c->E, u, nullptr, c->d_adj_tmp, lam, lam, nullptr, c->d_trace, c->d_nbr, c->d_invj, c->d_const, dt};
+    if (c->N == 4) {
+        constexpr int EB = kEBA<4>;
+        dim3 g((c->E + EB - 1) / EB), b(VGeo<4, EB>::NT);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_adj_records_md<4, EB, 2>), g, b, 0, c->stream, a2);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_adjoint_md<4, EB, 2>), g, b, 0, c->stream, a2);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_adj_records_md<4, EB, 1>), g, b, 0, c->stream, a1);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_adjoint_md<4, EB, 1>), g, b, 0, c->stream, a1);
+    } else {
+        constexpr int EB = kEBA<3>;
+        dim3 g((c->E + EB - 1) / EB), b(VGeo<3, EB>::NT);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_adj_records_md<3, EB, 2>), g, b, 0, c->stream, a2);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_adjoint_md<3, EB, 2>), g, b, 0, c->stream, a2);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_adj_records_md<3, EB, 1>), g, b, 0, c->stream, a1);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_adjoint_md<3, EB, 1>), g, b, 0, c->stream, a1);
     }
     return check_launch(c);
 }

@@ -339,6 +339,10 @@ prescribe_dts!(ctx, dts::Vector{Float64}) = check(ctx, ccall((:tlw_prescribe_dts
     (Ptr{Cvoid}, Ptr{Float64}, Int64), ctx, dts, length(dts)))
 adjoint_step!(lam, u, lam_next, ctx, t, dt) = check(ctx, ccall((:tlw_adjoint_lw, lib), Cint,
     (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Float64, Float64), ctx, u, lam_next, lam, t, dt))
+# MDRK: us = u + dt rhs_mdrk1!(u) is the step's stage state (recomputed by the caller)
+adjoint_step_mdrk!(lam, u, us, lam_next, ctx, t, dt) = check(ctx, ccall((:tlw_adjoint_mdrk, lib), Cint,
+    (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Float64, Float64),
+    ctx, u, us, lam_next, lam, t, dt))
 
 # Reverse-mode rule of a fixed-dt LW solve u0 -> u_K with checkpoints every k steps (loaded
 # into TrixiLW next to ChainRulesCore):

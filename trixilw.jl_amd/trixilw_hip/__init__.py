@@ -5,7 +5,7 @@ SemidiscretizationHyperbolic, rhs!/rhs_mdrk1!/rhs_mdrk2!, solve_lwfr) over the
 C ABI of libtrixilw_hip.so.  Importing this package does not load the
 library; the first call that needs the GPU does, and fails if it is missing.
 """
-from .adjoint import adjoint_step, gaussian_random_field, gradient_lw, kinetic_energy, objective_lw
+from .adjoint import adjoint_step, adjoint_step_mdrk, gaussian_random_field, gradient_lw, kinetic_energy, objective_lw
 from .basis import LobattoLegendreBasis
 from .batch import SemidiscretizationBatch, SemidiscretizationEnsemble, draw_vortex_ensemble
 from .equations import (Adiabatic, BoundaryConditionDirichlet, BoundaryConditionNavierStokesWall, CompressibleEulerEquations2D,
@@ -35,7 +35,7 @@ __all__ = [
     "get_time_discretization", "lw_fourier_cfl", "max_dt_noalloc", "rhs", "rhs_mdrk1", "rhs_mdrk2", "semidiscretize", "trixi2lw",
     "FirstOrderReconstruction", "IndicatorHennemannGassner", "MUSCLHancockReconstruction",
     "VolumeIntegralFRShockCapturing", "density_pressure", "PositivityPreservingLimiterZhangShu", "density",
-    "pressure", "adjoint_step", "gradient_lw", "kinetic_energy", "gaussian_random_field", "objective_lw",
+    "pressure", "adjoint_step", "adjoint_step_mdrk", "gradient_lw", "kinetic_energy", "gaussian_random_field", "objective_lw",
     "SemidiscretizationBatch", "SemidiscretizationEnsemble", "draw_vortex_ensemble", "SemidiscretizationHyperbolicParabolic", "CompressibleNavierStokesDiffusion2D",
     "GradientVariablesConservative", "GradientVariablesPrimitive",
 ]

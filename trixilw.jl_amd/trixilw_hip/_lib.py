@@ -32,7 +32,7 @@ EXPORTED = (
     "tlw_create_curved_parabolic", "tlw_max_dt_noalloc", "tlw_record_dts", "tlw_get_dts", "tlw_prescribe_dts",
     "tlw_set_state_dev", "tlw_get_state_dev", "tlw_boundary_nodes", "tlw_set_boundary_data",
     "tlw_create_ensemble", "tlw_ensemble_set_stream", "tlw_ensemble_enable_graphs", "tlw_ensemble_solve_init",
-    "tlw_ensemble_advance", "tlw_ensemble_last_error", "tlw_ensemble_destroy",
+    "tlw_ensemble_advance", "tlw_ensemble_last_error", "tlw_ensemble_destroy", "tlw_adjoint_mdrk",
 )
 
 
@@ -195,6 +195,8 @@ def lib():
     L.tlw_adjoint_lw.argtypes = [vp, vp, vp, vp, C.c_double, C.c_double]
     L.tlw_autotune.restype = C.c_int
     L.tlw_autotune.argtypes = [vp, i32, C.POINTER(i32), dp]
+    L.tlw_adjoint_mdrk.restype = C.c_int
+    L.tlw_adjoint_mdrk.argtypes = [vp, vp, vp, vp, vp, C.c_double, C.c_double]
     L.tlw_create_ensemble.restype = C.c_int
     L.tlw_create_ensemble.argtypes = [vp, vp, C.POINTER(vp)]
     L.tlw_ensemble_set_stream.restype = C.c_int

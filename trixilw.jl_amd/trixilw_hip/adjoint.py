@@ -32,6 +32,36 @@ def adjoint_step(lam, u, lam_next, semi, t, dt):
                                          C.c_void_p(lam.data_ptr()), float(t), float(dt)), semi.handle)
 
 
+def adjoint_step_mdrk(lam, u, us, lam_next, semi, t, dt):
+    """lam: the adjoint of uⁿ for one MDRK step (tlw_adjoint_mdrk), with us the
+    step's stage state uⁿ + dt rhs_mdrk1!(uⁿ) (device tensors)."""
+    for name, x in (("lam", lam), ("u", u), ("us", us), ("lam_next", lam_next)):
+        semi._check_state(x, name)
+    semi._bind_stream()
+    _lib.check(_lib.lib().tlw_adjoint_mdrk(semi.handle, C.c_void_p(u.data_ptr()), C.c_void_p(us.data_ptr()),
+                                           C.c_void_p(lam_next.data_ptr()), C.c_void_p(lam.data_ptr()), float(t),
+                                           float(dt)), semi.handle)
+
+
+def mdrk_stage_state(u, semi, t, dt, du):
+    """us = u + dt rhs_mdrk1!(u) (mdrk.jl:13-17), into a new tensor."""
+    from .semidiscretization import rhs_mdrk1
+
+    semi.cache["dt"][0] = dt
+    rhs_mdrk1(du, u, semi, t)
+    return _torch().add(u, du, alpha=dt)
+
+
+def mdrk_step(u, semi, t, dt, du):
+    """One MDRK step: us, then uⁿ⁺¹ = u + dt rhs_mdrk2!(u, us), into a new tensor."""
+    from .semidiscretization import rhs_mdrk2
+
+    us = mdrk_stage_state(u, semi, t, dt, du)
+    semi.set_us(us)
+    rhs_mdrk2(du, u, semi, t)
+    return _torch().add(u, du, alpha=dt)
+
+
 def lw_step(u, semi, t, dt, du):
     """u + dt rhs!(u) with cache.dt = dt, into a new tensor."""
     semi.cache["dt"][0] = dt
@@ -122,7 +152,8 @@ def _forward_device(semi, u0, nsteps, cfl, t0, dts, checkpoint_every):
 
 
 def gradient_lw(semi, u0, nsteps, cfl, objective=kinetic_energy, checkpoint_every=20, t0=0.0, dts=None):
-    """(J(u_K), ∇_{u0} J, u_K, dts) for K = nsteps LW steps from u0.
+    """(J(u_K), ∇_{u0} J, u_K, dts) for K = nsteps LW steps from u0 (MDRK steps
+    on an MDRK semidiscretization, with tlw_adjoint_mdrk).
 
     The forward sweep is solve_lwfr's loop on the device (no host
     synchronisation inside a checkpoint segment), keeping the state every
@@ -145,15 +176,27 @@ def gradient_lw(semi, u0, nsteps, cfl, objective=kinetic_energy, checkpoint_ever
     J, lam = objective(semi, u)
     lam_prev = torch.empty_like(lam)
     du = torch.empty_like(u0)
+    mdrk = _is_mdrk(semi)
+    step = mdrk_step if mdrk else lw_step
     for seg in sorted(checkpoints, reverse=True):
         end = min(seg + checkpoint_every, nsteps)
         states = [checkpoints[seg]]
         for k in range(seg, end - 1):
-            states.append(lw_step(states[-1], semi, times[k], dts[k], du))
+            states.append(step(states[-1], semi, times[k], dts[k], du))
         for k in range(end - 1, seg - 1, -1):
-            adjoint_step(lam_prev, states[k - seg], lam, semi, times[k], dts[k])
+            if mdrk:
+                us = mdrk_stage_state(states[k - seg], semi, times[k], dts[k], du)
+                adjoint_step_mdrk(lam_prev, states[k - seg], us, lam, semi, times[k], dts[k])
+            else:
+                adjoint_step(lam_prev, states[k - seg], lam, semi, times[k], dts[k])
             lam, lam_prev = lam_prev, lam
     return J, lam, u, dts
+
+
+def _is_mdrk(semi):
+    from .semidiscretization import MDRK
+
+    return isinstance(semi.time_discretization, MDRK)
 
 
 def objective_lw(semi, u0, nsteps, dts, objective=kinetic_energy, t0=0.0):
