@@ -345,12 +345,16 @@ int tlw_enable_kernel_timing(tlw_ctx* ctx, int32_t enable);
 int tlw_kernel_times(tlw_ctx* ctx, double* volume_ms, int64_t* volume_launches, double* surface_ms,
                      int64_t* surface_launches);
 
-/* Discrete adjoint of one LW step u' = u + dt rhs!(u) (plain LW, periodic mesh, TreeMesh
- * flux order): lam = lam_next + dt (∂rhs/∂u)ᵀ lam_next at the state u.  No counterpart in the
- * reference, whose gradients go through ForwardDiff; it differentiates the same rhs!
- * (dg_2d.jl:28-83 with lw_volume_kernel_4!/3!, 674-1267, and the TreeMesh Rusanov flux,
- * 1903-1946), with dt frozen.  The max in λ passes its derivative to the side the forward
- * max picked.  Uses the context's trace buffer as scratch; lam must not alias u or lam_next. */
+/* Discrete adjoint of one LW step u' = u + dt rhs!(u) (periodic mesh, TreeMesh flux order):
+ * lam = lam_next + dt (∂rhs/∂u)ᵀ lam_next at the state u.  No counterpart in the reference, whose
+ * gradients go through ForwardDiff; it differentiates the same rhs! (dg_2d.jl:28-83 with
+ * lw_volume_kernel_4!/3!, 674-1267, and the TreeMesh Rusanov flux, 1903-1946), with dt frozen.
+ * The max in λ passes its derivative to the side the forward max picked.  With
+ * VolumeIntegralFRShockCapturing and first-order subcell FV, the blend (1 - α) LW + α FV of the
+ * volume term (dg_2d.jl:1648-1804, fv_kernel! 251-282) and the blended interface flux
+ * alp fn + (1 - alp) F* (1873-1946) are differentiated with the blending factors α of the state u
+ * and the interface's admissibility decision frozen.  Uses the context's trace buffer (a buffer of
+ * its own with shock capturing) as scratch; lam must not alias u or lam_next. */
 int tlw_adjoint_lw(tlw_ctx* ctx, const double* u_dev, const double* lam_next_dev, double* lam_dev, double t,
                    double dt);
 

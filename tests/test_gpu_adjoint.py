@@ -187,6 +187,90 @@ def test_checkpointed_gradient_mdrk_matches_tangent_and_fd(oracle_mod):
     semi.close()
 
 
+def _blended_setup(oracle_mod, N, seed):
+    """A periodic [0,1]² 8x8 mesh with a square density/pressure jump (node noise
+    breaks mirror symmetry, where the Rusanov speed's max has ties), shock
+    capturing with first-order subcell FV on both sides."""
+    import trixilw_hip as tlw
+
+    eq = tlw.CompressibleEulerEquations2D(1.4)
+    mesh = tlw.TreeMesh((0.0, 0.0), (1.0, 1.0), 3, ordering="strips")
+    ind = tlw.IndicatorHennemannGassner(eq, None, alpha_max=0.5, alpha_min=0.001, alpha_smooth=True)
+    solver = tlw.DGSEM(N, volume_integral=tlw.VolumeIntegralFRShockCapturing(ind))
+
+    def square(x, y, t=0.0):
+        inside = (np.abs(x - 0.5) < 0.22) & (np.abs(y - 0.45) < 0.2)
+        rho = np.where(inside, 1.0, 0.4)
+        p = np.where(inside, 1.0, 0.3)
+        v1, v2 = 0.2 + 0 * x, -0.1 + 0 * x
+        return np.stack([rho, rho * v1, rho * v2, p / 0.4 + 0.5 * rho * (v1 ** 2 + v2 ** 2)], -1)
+
+    semi = tlw.SemidiscretizationHyperbolic(mesh, tlw.LW(), eq, square, solver, keep_aux=True)
+    o = oracle_mod.Oracle(N, 8, 8, 0.0, 0.0, 1.0 / 8, shock_capturing=True, alpha_max=0.5, alpha_min=0.001,
+                          alpha_smooth=True)
+    x, y = o.node_coords()
+    rng = np.random.default_rng(seed)
+    u_lex = square(x, y) * (1.0 + 1e-3 * rng.standard_normal(x.shape + (4,)))
+    return semi, o, u_lex, mesh
+
+
+@pytest.mark.parametrize("N", [3, 4])
+def test_adjoint_blended_step_matches_tangent(oracle_mod, N):
+    """One shock-capturing LW step (first-order subcell FV, blended interface
+    flux), α and the admissibility decision frozen at the state: <λ, v> =
+    <λ', du'> against the oracle's dual tangent, whose indicator works on the
+    value parts (so α is frozen the same way), ≤ 1e-12; the blend is active."""
+    import trixilw_hip as tlw
+
+    torch = torch_mod()
+    semi, o, u_lex, mesh = _blended_setup(oracle_mod, N, 90 + N)
+    dt = 0.3 * o.max_dt(u_lex)
+    u = to_dev(mesh, u_lex)
+    du = torch.empty_like(u)
+    semi.cache["dt"][0] = dt
+    tlw.rhs(du, u, semi, 0.0)
+    alpha = semi.get_aux("alpha")
+    assert (alpha > 0.05).sum() >= 4 and (alpha < 1e-12).sum() >= 4, alpha
+    rng = np.random.default_rng(100 + N)
+    v = rng.standard_normal(u_lex.shape)
+    lam_next = rng.standard_normal(u_lex.shape)
+    _, du1 = o.tangent(u_lex, v, [dt])
+    ref = float(np.sum(lam_next * du1))
+    lam = torch.empty_like(u)
+    tlw.adjoint_step(lam, u, to_dev(mesh, lam_next), semi, 0.0, dt)
+    torch.cuda.synchronize()
+    got = float(np.sum(to_lex(mesh, lam) * v))
+    assert abs(got - ref) <= 1e-12 * abs(ref), (got, ref)
+    # the blended terms matter: the frozen-α tangent differs from that of the unblended scheme
+    o_plain = oracle_mod.Oracle(N, 8, 8, 0.0, 0.0, 1.0 / 8)
+    _, du_plain = o_plain.tangent(u_lex, v, [dt])
+    assert abs(float(np.sum(lam_next * du_plain)) - ref) > 1e-6 * abs(ref)
+    semi.close()
+
+
+def test_checkpointed_gradient_blended_matches_tangent(oracle_mod):
+    """12 shock-capturing LW steps (N=4, checkpoints every 4) of the blocky
+    state: the adjoint gradient of the kinetic energy against the oracle's
+    dual tangent with α frozen per step, ≤ 1e-10."""
+    import trixilw_hip as tlw
+
+    torch = torch_mod()
+    semi, o, u_lex, mesh = _blended_setup(oracle_mod, 4, 5)
+    rng = np.random.default_rng(6)
+    v = rng.standard_normal(u_lex.shape) * 1e-2
+    cfl = 0.3
+    J, grad, uK, dts = tlw.gradient_lw(semi, to_dev(mesh, u_lex), 12, cfl, checkpoint_every=4)
+    torch.cuda.synchronize()
+    dJ_adj = float(np.sum(to_lex(mesh, grad) * v))
+    _, w, _, _ = o.basis()
+    uK_t, duK = o.tangent(u_lex, v, dts)
+    J_t, gJ = ke_and_grad_np(uK_t, w, 1.0 / 8)
+    dJ_tan = float(np.sum(gJ * duK))
+    assert abs(J - J_t) <= 1e-12 * abs(J_t)
+    assert abs(dJ_adj - dJ_tan) <= 1e-10 * abs(dJ_tan), (dJ_adj, dJ_tan)
+    semi.close()
+
+
 def test_adjoint_rejects_unsupported(oracle_mod):
     import trixilw_hip as tlw
 

@@ -657,3 +657,407 @@ __global__ __launch_bounds__((VGeo<N, EB>::NT), 2) void k_adjoint_md(AdjArgs a) 
 }
 
 }  // namespace tlw
+
+namespace tlw {
+
+// ---------------------------------------------------------------------------
+// Discrete adjoint of one LW step with VolumeIntegralFRShockCapturing and
+// first-order subcell FV (TreeMesh, periodic), the blending factors α and the
+// interface's admissibility decision frozen at the step's state:
+//   vol = (1 - α_e) D̂·(F̃, G̃) + α_e fv(u),  fv = w⁻¹ differences of the subcell
+//   Rusanov fluxes (fv_kernel!, dg_2d.jl:251-282);
+//   F* = alp fn(u_L, u_R) + (1 - alp) F*_LW,  alp = ½(α_L + α_R) or 1 where the
+//   low-order update is not admissible (compute_alp, dg_2d.jl:1873-1901).
+// α comes from k_indicator on the step's state, smoothed as the volume kernel
+// does; fn_low (the subcell fluxes next to the faces) enters only the frozen
+// decision.  Records: the forward shock-capturing face record (16 doubles)
+// and s̄ = -J⁻¹ dt λ'.
+
+constexpr int kAdjQSC = 20;
+
+// (ulb, urb) += the adjoint of rusanov(ul, ur, orientation) for the output adjoint ob
+__device__ __forceinline__ void rusanov_vjp(const double* ul, const double* ur, double gamma, int orientation,
+                                            const double* ob, double* ulb, double* urb) {
+    double al, cl, ar, cr;
+    speed_parts(ul, gamma, orientation, al, cl);
+    speed_parts(ur, gamma, orientation, ar, cr);
+    const double lam = dmax(al, ar) + dmax(cl, cr);
+    double hb[4], zero[4] = {0, 0, 0, 0}, wb[4], lamb = 0.0;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        hb[v] = 0.5 * ob[v];
+        lamb = fma(-0.5 * ob[v], ur[v] - ul[v], lamb);
+        ulb[v] = fma(0.5 * lam, ob[v], ulb[v]);
+        urb[v] = fma(-0.5 * lam, ob[v], urb[v]);
+    }
+    const double gm1 = gamma - 1.0;
+    euler_fluxes_vjp(ul, gm1, orientation == 1 ? hb : zero, orientation == 1 ? zero : hb, wb);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) ulb[v] += wb[v];
+    euler_fluxes_vjp(ur, gm1, orientation == 1 ? hb : zero, orientation == 1 ? zero : hb, wb);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) urb[v] += wb[v];
+    double da[4], dc[4];
+    const bool a_right = ar > al, c_right = cr > cl;  // dmax picks the right side only when larger
+    speed_grads(a_right ? ur : ul, gamma, orientation, da, dc);
+    double* pa = a_right ? urb : ulb;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) pa[v] = fma(lamb, da[v], pa[v]);
+    speed_grads(c_right ? ur : ul, gamma, orientation, da, dc);
+    double* pc = c_right ? urb : ulb;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) pc[v] = fma(lamb, dc[v], pc[v]);
+}
+
+// The interface blending factor of face_numflux_sc for the records L, R.
+__device__ __forceinline__ double sc_face_alp(const double* L, const double* R, int orientation, double gamma,
+                                              double J, double dt, double w_n, double w_1) {
+    double Fh[4], fn[4], Fb[4], low[4];
+    face_numflux(L, R, Fh);
+    rusanov(L, R, gamma, orientation, fn);
+    double alp = 0.5 * (L[14] + R[14]);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) Fb[v] = (1.0 - alp) * Fh[v] + alp * fn[v];
+    const double cl = dt * J / (w_n * 0.5), cr = dt * J / (w_1 * 0.5);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) low[v] = L[v] - cl * (Fb[v] - L[10 + v]);
+    bool ok = admissible(low, gamma);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) low[v] = R[v] - cr * (R[10 + v] - Fb[v]);
+    ok = ok && admissible(low, gamma);
+    return ok ? alp : 1.0;
+}
+
+// Forward quantities of the blended step at one node: Ũ, F̃, the time
+// derivatives (KEEP), α_e and the subcell fluxes next to the node's faces.
+template <int N, int EB, bool KEEP>
+__device__ __forceinline__ void adj_forward_sc(const AdjArgs& a, const double* alpha_tmp, const double* uN, int e,
+                                               int i, int j, bool act, int sidx, int sbase, double* lds, double* sDm,
+                                               double* sDh, double* Uc, double* F, double* G, double* tder,
+                                               double& alpha_e, double* fnlx, double* fnly) {
+    using G_ = VGeo<N, EB>;
+    constexpr int n = G_::n, NN = G_::NN, S = G_::S;
+    const DevConst* c = a.c;
+    const double dt = a.dt, invJ = a.inv_jac[e], gamma = c->gamma, gm1 = gamma - 1.0;
+    if (threadIdx.x < NN) {
+        sDm[threadIdx.x] = -dt * c->D[threadIdx.x / n][threadIdx.x % n];
+        sDh[threadIdx.x] = c->Dhat[threadIdx.x / n][threadIdx.x % n];
+    }
+    double vol[4];
+    if constexpr (N == 4)
+        lw4_element<N, EB, KEEP>(uN, dt, invJ, gm1, c, i, j, act, sidx, sbase, lds, lds + 8 * S, lds + 16 * S, sDm,
+                                 sDh, Uc, F, G, vol, tder);
+    else
+        lw3_element<N, EB, KEEP>(uN, dt, invJ, gm1, c, i, j, act, sidx, sbase, lds, lds + 8 * S, lds + 16 * S, sDm,
+                                 sDh, Uc, F, G, vol, tder);
+    alpha_e = alpha_tmp[e];
+    if (c->alpha_smooth) {
+#pragma unroll
+        for (int f = 0; f < 4; ++f) alpha_e = dmax(alpha_e, 0.5 * alpha_tmp[a.nbr[4 * e + f]]);
+    }
+    __syncthreads();  // the element procedure's last reads of s2 are done
+    double* su = lds;
+    if (act) put4<S>(su, sidx, uN);
+    __syncthreads();
+    double w[4];
+    // subcell flux between nodes (0, j) and (1, j), or (n-2, j) and (n-1, j); likewise in y
+    if (i == 0 || i == n - 1) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) w[v] = su[v * S + sidx + (i == 0 ? 1 : -1)];
+        if (i == 0) rusanov(uN, w, gamma, 1, fnlx);
+        else rusanov(w, uN, gamma, 1, fnlx);
+    }
+    if (j == 0 || j == n - 1) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) w[v] = su[v * S + sidx + (j == 0 ? n : -n)];
+        if (j == 0) rusanov(uN, w, gamma, 2, fnly);
+        else rusanov(w, uN, gamma, 2, fnly);
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ void sc_own_record(const double* uN, const double* Fn, const double* Uc, double gamma,
+                                              int orientation, const double* fnl, double alpha_e, double* r) {
+    double av1, av2, cs;
+    speed_parts2(Uc, gamma, av1, av2, cs);
+    make_record(r, uN, Fn, orientation == 1 ? av1 : av2, cs);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) r[10 + v] = fnl[v];
+    r[14] = alpha_e;
+    r[15] = 0.0;
+}
+
+template <int N, int EB>
+__global__ __launch_bounds__((VGeo<N, EB>::NT), 2) void k_adj_records_sc(AdjArgs a, const double* alpha_tmp) {
+    using G_ = VGeo<N, EB>;
+    constexpr int n = G_::n, NN = G_::NN;
+    __shared__ double lds[G_::LDS];
+    __shared__ double sDm[NN], sDh[NN];
+    const int tid = threadIdx.x;
+    const int le = tid / NN, node = tid - (tid / NN) * NN;
+    const int i = node % n, j = node / n;
+    const int e0 = blockIdx.x * EB;
+    const int nel = min(EB, a.E - e0);
+    const bool act = le < nel;
+    const int lec = le < nel ? le : nel - 1;
+    const int e = e0 + lec;
+    const int sidx = lec * NN + node, sbase = lec * NN;
+    const size_t off = ((size_t)e * NN + node) * 4;
+    double uN[4], Uc[4], F[4], G[4], fnlx[4] = {0, 0, 0, 0}, fnly[4] = {0, 0, 0, 0}, alpha_e;
+    load4(a.u + off, uN);
+    adj_forward_sc<N, EB, false>(a, alpha_tmp, uN, e, i, j, act, sidx, sbase, lds, sDm, sDh, Uc, F, G, nullptr,
+                                 alpha_e, fnlx, fnly);
+    const int fx = (i == 0) ? 0 : ((i == n - 1) ? 1 : -1);
+    const int fy = (j == 0) ? 2 : ((j == n - 1) ? 3 : -1);
+    if (!act || (fx < 0 && fy < 0)) return;
+    const double invJ = a.inv_jac[e], gamma = a.c->gamma;
+    double lam[4], r[kAdjQSC];
+    load4(a.lam + off, lam);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) r[16 + v] = (-invJ * a.dt) * lam[v];
+    if (fx >= 0) {
+        sc_own_record(uN, F, Uc, gamma, 1, fnlx, alpha_e, r);
+        store_record<kAdjQSC>(a.trace + (((size_t)e * 4 + fx) * n + j) * kAdjQSC, r);
+    }
+    if (fy >= 0) {
+        sc_own_record(uN, G, Uc, gamma, 2, fnly, alpha_e, r);
+        store_record<kAdjQSC>(a.trace + (((size_t)e * 4 + fy) * n + i) * kAdjQSC, r);
+    }
+}
+
+template <int N, int EB>
+__global__ __launch_bounds__((VGeo<N, EB>::NT), 1) void k_adjoint_sc(AdjArgs a, const double* alpha_tmp, double J0) {
+    using G_ = VGeo<N, EB>;
+    constexpr int n = G_::n, NN = G_::NN, S = G_::S;
+    __shared__ double lds[G_::LDS];
+    __shared__ double sDm[NN], sDh[NN];
+    const int tid = threadIdx.x;
+    const int le = tid / NN, node = tid - (tid / NN) * NN;
+    const int i = node % n, j = node / n;
+    const int e0 = blockIdx.x * EB;
+    const int nel = min(EB, a.E - e0);
+    const bool act = le < nel;
+    const int lec = le < nel ? le : nel - 1;
+    const int e = e0 + lec;
+    const int sidx = lec * NN + node, sbase = lec * NN;
+    const DevConst* c = a.c;
+    const double gamma = c->gamma, gm1 = gamma - 1.0;
+    const double dt = a.dt, invJ = a.inv_jac[e];
+    const size_t off = ((size_t)e * NN + node) * 4;
+    double uN[4], lam[4], Uc[4], F[4], G[4], ud[N][4], fnlx[4] = {0, 0, 0, 0}, fnly[4] = {0, 0, 0, 0}, alpha_e;
+    load4(a.u + off, uN);
+    load4(a.lam + off, lam);
+    adj_forward_sc<N, EB, true>(a, alpha_tmp, uN, e, i, j, act, sidx, sbase, lds, sDm, sDh, Uc, F, G, &ud[0][0],
+                                alpha_e, fnlx, fnly);
+    const bool blended = !(fabs(alpha_e) <= 1e-12);
+
+    // --- blended interface fluxes of this node's faces, alp frozen
+    double sb[4], Fb[4] = {0, 0, 0, 0}, Gb[4] = {0, 0, 0, 0}, Ucb[4] = {0, 0, 0, 0}, ub[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int v = 0; v < 4; ++v) sb[v] = (-invJ * dt) * lam[v];
+    const int fx = (i == 0) ? 0 : ((i == n - 1) ? 1 : -1);
+    const int fy = (j == 0) ? 2 : ((j == n - 1) ? 3 : -1);
+    auto face = [&](int f, int l, const double* Fn, const double* fnl, double* Fnb) {
+        const int nb = a.nbr[4 * e + f];
+        const int orientation = f < 2 ? 1 : 2;
+        double r[kAdjQSC], own[kAdjQSC];
+        load_record<kAdjQSC>(a.trace + (((size_t)nb * 4 + (f ^ 1)) * n + l) * kAdjQSC, r);
+        sc_own_record(uN, Fn, Uc, gamma, orientation, fnl, alpha_e, own);
+        const bool own_left = (f & 1) != 0;
+        const double* L = own_left ? own : r;
+        const double* R = own_left ? r : own;
+        const double alp = sc_face_alp(L, R, orientation, gamma, J0, dt, c->weights[n - 1], c->weights[0]);
+        const double lamf = dmax(L[8], R[8]) + dmax(L[9], R[9]);
+        double Fsb[4], lamb = 0.0;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const double sL = own_left ? sb[v] : r[16 + v];
+            const double sR = own_left ? r[16 + v] : sb[v];
+            Fsb[v] = c->bnd_hi * sL - c->bnd_lo * sR;
+            // the LW part, weighted 1 - alp
+            const double hb = (1.0 - alp) * Fsb[v];
+            lamb = fma(-0.5 * hb, R[v] - L[v], lamb);
+            Fnb[v] = fma(0.5, hb, Fnb[v]);
+            ub[v] = fma(own_left ? 0.5 * lamf : -0.5 * lamf, hb, ub[v]);
+            Fsb[v] *= alp;  // the low-order part's adjoint
+        }
+        const double aown = own[8], aoth = r[8], cown = own[9], coth = r[9];
+        const bool a_own = own_left ? !(aoth > aown) : (aown > aoth);
+        const bool c_own = own_left ? !(coth > cown) : (cown > coth);
+        if (a_own || c_own) {
+            double da[4], dc[4];
+            speed_grads(Uc, gamma, orientation, da, dc);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                if (a_own) Ucb[v] = fma(lamb, da[v], Ucb[v]);
+                if (c_own) Ucb[v] = fma(lamb, dc[v], Ucb[v]);
+            }
+        }
+        if (alp != 0.0) {
+            double ulb[4] = {0, 0, 0, 0}, urb[4] = {0, 0, 0, 0};
+            rusanov_vjp(L, R, gamma, orientation, Fsb, ulb, urb);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) ub[v] += own_left ? ulb[v] : urb[v];
+        }
+    };
+    if (fx >= 0) face(fx, j, F, fnlx, Fb);
+    if (fy >= 0) face(fy, i, G, fnly, Gb);
+
+    // --- subcell FV of a blended element: vol += α fv(u)
+    double* su = lds;               // u, then the adjoint sent to the left / lower neighbour
+    double* ss = lds + 4 * S;       // α s̄
+    double* sl = lds + 8 * S;       // [x: 4][y: 4] adjoint for the left / lower node
+    {
+        double sv[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) sv[v] = blended ? alpha_e * sb[v] : 0.0;
+        if (act) {
+            put4<S>(su, sidx, uN);
+            put4<S>(ss, sidx, sv);
+        }
+        __syncthreads();
+        const double iwi = 1.0 / c->weights[i], iwj = 1.0 / c->weights[j];
+        double bx[4] = {0, 0, 0, 0}, by[4] = {0, 0, 0, 0};
+        if (blended && i > 0) {  // subcell face between (i-1, j) and (i, j)
+            double w[4], ob[4];
+            const double iwl = 1.0 / c->weights[i - 1];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                w[v] = su[v * S + sidx - 1];
+                ob[v] = iwl * ss[v * S + sidx - 1] - iwi * sv[v];
+            }
+            rusanov_vjp(w, uN, gamma, 1, ob, bx, ub);
+        }
+        if (blended && j > 0) {
+            double w[4], ob[4];
+            const double iwl = 1.0 / c->weights[j - 1];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                w[v] = su[v * S + sidx - n];
+                ob[v] = iwl * ss[v * S + sidx - n] - iwj * sv[v];
+            }
+            rusanov_vjp(w, uN, gamma, 2, ob, by, ub);
+        }
+        if (act) {
+            put4<S>(sl, sidx, bx);
+            put4<S>(sl + 4 * S, sidx, by);
+        }
+        __syncthreads();
+        if (blended && i < n - 1) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) ub[v] += sl[v * S + sidx + 1];
+        }
+        if (blended && j < n - 1) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) ub[v] += sl[4 * S + v * S + sidx + n];
+        }
+        __syncthreads();
+    }
+
+    // --- volume term (1 - α) D̂·(F̃, G̃)
+    double* ex0 = lds;
+    double* ex1 = lds + 4 * S;
+    {
+        double t4[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) t4[v] = (1.0 - alpha_e) * sb[v];
+        if (act) put4<S>(ex0, sidx, t4);
+    }
+    __syncthreads();
+    {
+        double cf[4], cg[4];
+        contract_t<n, S>(ex0, sbase, i, j, sDh, cf, cg);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            Fb[v] += cf[v];
+            Gb[v] += cg[v];
+        }
+    }
+    // --- reverse of the LW procedure (as k_adjoint)
+    double F0b[4], G0b[4], udb[N][4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        F0b[v] = Fb[v];
+        G0b[v] = Gb[v];
+        ub[v] += Ucb[v];
+#pragma unroll
+        for (int d = 0; d < N; ++d) udb[d][v] = LWTaylor::w[d] * Ucb[v];
+    }
+    int buf = 1;
+#pragma unroll
+    for (int k = N; k >= 1; --k) {
+        double dxb[4], dyb[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            dxb[v] = LWTaylor::w[k - 1] * Fb[v];
+            dyb[v] = LWTaylor::w[k - 1] * Gb[v];
+        }
+        if (k < N) {
+            double* ex = buf ? ex1 : ex0;
+            buf ^= 1;
+            double t4[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) t4[v] = invJ * udb[k][v];
+            if (act) put4<S>(ex, sidx, t4);
+            __syncthreads();
+            double cf[4], cg[4];
+            contract_t<n, S>(ex, sbase, i, j, sDm, cf, cg);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                dxb[v] += cf[v];
+                dyb[v] += cg[v];
+            }
+        }
+        const double f0 = LWStencil<N>::c[k - 1][4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            F0b[v] = fma(f0, dxb[v], F0b[v]);
+            G0b[v] = fma(f0, dyb[v], G0b[v]);
+        }
+#pragma unroll
+        for (int kind = 0; kind < 4; ++kind) {
+            const double cc = LWStencil<N>::c[k - 1][kind];
+            if (cc == 0.0) continue;
+            double w[4], fb[4], gb[4], wb[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                double x = uN[v];
+#pragma unroll
+                for (int d = 0; d < k; ++d) x = fma(LWTaylor::a[kind][d], ud[d][v], x);
+                w[v] = x;
+                fb[v] = cc * dxb[v];
+                gb[v] = cc * dyb[v];
+            }
+            euler_fluxes_vjp(w, gm1, fb, gb, wb);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                ub[v] += wb[v];
+#pragma unroll
+                for (int d = 0; d < k; ++d) udb[d][v] = fma(LWTaylor::a[kind][d], wb[v], udb[d][v]);
+            }
+        }
+    }
+    {
+        double* ex = buf ? ex1 : ex0;
+        double t4[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) t4[v] = invJ * udb[0][v];
+        if (act) put4<S>(ex, sidx, t4);
+        __syncthreads();
+        double cf[4], cg[4];
+        contract_t<n, S>(ex, sbase, i, j, sDm, cf, cg);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            F0b[v] += cf[v];
+            G0b[v] += cg[v];
+        }
+    }
+    double wb[4];
+    euler_fluxes_vjp(uN, gm1, F0b, G0b, wb);
+    double out[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) out[v] = lam[v] + (ub[v] + wb[v]);
+    if (act) store4(a.lam_out + off, out);
+}
+
+}  // namespace tlw

@@ -105,7 +105,8 @@ struct tlw_ctx {
     double* d_btab = nullptr;
     size_t btab_cap = 0;
     int btab_levels = 0;
-    double* d_adj_tmp = nullptr;  // the MDRK adjoint's λ_us (allocated on first use)
+    double* d_adj_tmp = nullptr;    // the MDRK adjoint's λ_us (allocated on first use)
+    double* d_adj_trace = nullptr;  // face records of the blended adjoint (allocated on first use)
     // per-step dt record and prescribed dts of the device loop (tlw_record_dts, tlw_prescribe_dts)
     double* d_dtlog = nullptr;
     long long dtlog_cap = 0;
@@ -928,6 +929,7 @@ void tlw_destroy(tlw_ctx* c) {
     (void)hipFree(c->d_dtlog);
     (void)hipFree(c->d_dtpresc);
     (void)hipFree(c->d_adj_tmp);
+    (void)hipFree(c->d_adj_trace);
     (void)hipFree(c->d_const);
     delete c;
 }
@@ -1170,10 +1172,38 @@ int tlw_adjoint_lw(tlw_ctx* c, const double* u, const double* lam_next, double* 
     if (!c) return TLW_EINVAL;
     if (!u || !lam_next || !lam) return fail(c, TLW_EINVAL, "null state pointer");
     if (!std::isfinite(dt) || !std::isfinite(t)) return fail(c, TLW_EINVAL, "t and dt must be finite");
-    if (c->time_disc != TLW_LW || c->sc || c->parabolic || c->curved || c->has_boundary ||
-        c->hconst.flux_order != TLW_FLUX_TREE || c->ninst != 1)
-        return fail(c, TLW_ENOTSUP, "the adjoint covers plain LW on one periodic mesh with the TreeMesh flux order");
+    if (c->time_disc != TLW_LW || c->parabolic || c->curved || c->has_boundary ||
+        c->hconst.flux_order != TLW_FLUX_TREE || c->ninst != 1 || (c->sc && c->hconst.recon != TLW_RECON_FIRST_ORDER))
+        return fail(c, TLW_ENOTSUP,
+                    "the adjoint covers LW (plain or blended with first-order subcell FV) on one periodic mesh with "
+                    "the TreeMesh flux order");
     if (lam == lam_next || lam == u) return fail(c, TLW_EINVAL, "lam must not alias lam_next or u");
+    if (c->sc) {
+        // blending factors of the step's state (frozen), then the blended step's adjoint
+        if (!c->d_adj_trace)
+            HIPCHK(c, hipMalloc(&c->d_adj_trace, sizeof(double) * (size_t)c->E * 4 * c->n * kAdjQSC));
+        KArgs ia = base_args(c);
+        ia.u = u;
+        ia.in_sel = SEL_EXPLICIT;
+        ia.use_state_dt = 0;
+        AdjArgs aa{c->E, u, nullptr, lam_next, nullptr, lam, nullptr, c->d_adj_trace, c->d_nbr, c->d_invj, c->d_const,
+                   dt};
+        const double J0 = c->h_state[0].invJ0;
+        if (c->N == 4) {
+            constexpr int EB = kEBA<4>;
+            dim3 g((c->E + EB - 1) / EB), b(VGeo<4, EB>::NT);
+            hipLaunchKernelGGL(k_indicator<4>, dim3(c->sgrid), dim3(kSurfBlock), 0, c->stream, ia, c->d_alpha_tmp);
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_adj_records_sc<4, EB>), g, b, 0, c->stream, aa, c->d_alpha_tmp);
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_adjoint_sc<4, EB>), g, b, 0, c->stream, aa, c->d_alpha_tmp, J0);
+        } else {
+            constexpr int EB = kEBA<3>;
+            dim3 g((c->E + EB - 1) / EB), b(VGeo<3, EB>::NT);
+            hipLaunchKernelGGL(k_indicator<3>, dim3(c->sgrid), dim3(kSurfBlock), 0, c->stream, ia, c->d_alpha_tmp);
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_adj_records_sc<3, EB>), g, b, 0, c->stream, aa, c->d_alpha_tmp);
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_adjoint_sc<3, EB>), g, b, 0, c->stream, aa, c->d_alpha_tmp, J0);
+        }
+        return check_launch(c);
+    }
     KArgs a = base_args(c);
     a.u = u;
     a.us = lam_next;
