@@ -170,8 +170,9 @@ int tlw_create_batch(int32_t n_instances, const tlw_mesh_desc* meshes, const tlw
  * BR1 gradient of the conservative variables, LW procedure on f_a - f_v with Taylor states of
  * the gradient, interface flux LW-Rusanov - ½(F̃v_L + F̃v_R); tlw_max_dt and the device loop use
  * compute_dt's min of the advective and viscous bounds (src/solvers/lwfr.jl:192-210,
- * compressible_navier_stokes_2d.jl:204-261).  Plain VolumeIntegralFR only; the positivity
- * limiter and tlw_adjoint_lw return TLW_ENOTSUP on such a context.  With TLW_FLAG_KEEP_AUX,
+ * compressible_navier_stokes_2d.jl:204-261).  Plain VolumeIntegralFR only; tlw_adjoint_lw returns
+ * TLW_ENOTSUP on such a context; the positivity limiter applies as on any mesh (quadrature weights,
+ * positivity_zhang_shu_dg2d.jl:8-45), the next dt then from the limited state with the viscous bound.  With TLW_FLAG_KEEP_AUX,
  * TLW_AUX_F holds the total F̃a - F̃v. */
 int tlw_create_parabolic(const tlw_mesh_desc* mesh, const tlw_basis_desc* basis, const tlw_solver_desc* solver,
                          const tlw_parabolic_desc* parabolic, tlw_ctx** out);
@@ -197,8 +198,10 @@ int tlw_create_parabolic(const tlw_mesh_desc* mesh, const tlw_basis_desc* basis,
  * compute_alp with each face node's own inverse Jacobian (dg_2d.jl:2169-2192); boundary faces are
  * not blended.  TLW_RECON_MUSCL_HANCOCK, and alpha smoothing on a mesh with boundaries (Trixi
  * asserts a periodic StructuredMesh there), return TLW_ENOTSUP, and so do several instances,
- * the positivity limiter, adaptive steps, tlw_adjoint_lw and tlw_rebuild_mesh on such a
- * context.  A non-positive inverse Jacobian returns TLW_EINVAL. */
+ * adaptive steps, tlw_adjoint_lw and tlw_rebuild_mesh on such a context.  The positivity limiter
+ * (my_limiter_zhang_shu!, quadrature weights only, positivity_zhang_shu_dg2d.jl:8-45, for any
+ * AbstractMesh{2}) applies, the next dt then from the limited state with the curvilinear speeds.
+ * A non-positive inverse Jacobian returns TLW_EINVAL. */
 int tlw_create_curved(const tlw_mesh_desc* mesh, const tlw_basis_desc* basis, const tlw_solver_desc* solver,
                       const tlw_curved_desc* metrics, tlw_ctx** out);
 /* SemidiscretizationHyperbolicParabolic (compressible Navier-Stokes, as tlw_create_parabolic) on a

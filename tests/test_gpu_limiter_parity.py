@@ -79,3 +79,69 @@ def test_limited_riemann_loop_parity(oracle_mod, device_loop):
     assert sol.stats.naccept == K
     assert abs(sol.t - t_ref) <= 1e-13 * t_ref
     assert_close(to_lex(mesh, sol.u), u_ref, what=f"limited Riemann, device_loop={device_loop}")
+
+
+@pytest.mark.parametrize("td,K", [("lw", 12), ("mdrk", 8)])
+def test_limited_curved_loop_parity(oracle_mod, td, K):
+    """The limiter on the curvilinear StructuredMesh (weights only, any
+    AbstractMesh{2}): the warped vortex with a density threshold above the
+    vortex core's density, so elements are limited every step, device loop
+    against the oracle's limited solve (the next dt from the limited state with
+    the curvilinear speeds), ≤ 1e-12."""
+    import trixilw_hip as tlw
+    from gpu_helpers import setup_curved
+
+    semi, o0, u_lex, mesh = setup_curved(oracle_mod, 4, 8, 8, td=td)
+    thr = (0.75, 1e-3)
+    L = 20.0
+    o = oracle_mod.Oracle(4, 8, 8, 0.0, 0.0, L / 8, gamma=1.4, mapping=(oracle_mod.MAP_WARP, (L, 0.05)),
+                          positivity=thr)
+    cfl = 0.5 * tlw.trixi2lw(1.0, semi.solver) * (0.5 if td == "mdrk" else 1.0)
+    u_ref, t_ref, _ = o.solve(u_lex, 0.0, 1e9, cfl, max_steps=K, mdrk=(td == "mdrk"))
+    u_unl, _, _ = o0.solve(u_lex, 0.0, 1e9, cfl, max_steps=K, mdrk=(td == "mdrk"))
+    assert np.max(np.abs(u_unl - u_ref)) > 1e-4  # the limiter acts
+    upd = tlw.semidiscretize(semi, semi.time_discretization, (0.0, float("inf")))
+    upd.u0.copy_(to_dev(mesh, u_lex))
+    lim = tlw.PositivityPreservingLimiterZhangShu(thresholds=thr, variables=(tlw.density, tlw.pressure))
+    sol = tlw.solve_lwfr(upd, (), 1e-6, None, time_step_computation=tlw.CFLBased(cfl), limiters=(lim,),
+                         max_steps=K)
+    assert sol.stats.naccept == K
+    assert abs(sol.t - t_ref) <= 1e-13 * t_ref
+    assert_close(to_lex(mesh, sol.u), u_ref, what=f"limited curved {td}")
+    semi.close()
+
+
+@pytest.mark.parametrize("curved", [False, True])
+def test_limited_navier_stokes_loop_parity(oracle_mod, curved):
+    """The limiter with the hyperbolic-parabolic path (TreeMesh and warped
+    mesh): density threshold inside the shear state's density range, device
+    loop with the viscous dt bound against the oracle, ≤ 1e-12."""
+    import trixilw_hip as tlw
+
+    thr = (0.95, 1e-3)
+    K = 10
+    if curved:
+        from test_gpu_curved_parabolic_parity import setup_cns
+
+        semi, o0, u_lex, mesh = setup_cns(oracle_mod, 4, 8, A=0.08)
+        o = oracle_mod.Oracle(4, 8, 8, 0.0, 0.0, 2.0 / 8, gamma=1.4, mapping=(oracle_mod.MAP_WARP, (2.0, 0.08)),
+                              parabolic="navier_stokes", mu=0.01, prandtl=0.72, positivity=thr)
+    else:
+        from test_gpu_parabolic_parity import setup_ns
+
+        semi, o0, u_lex, mesh = setup_ns(oracle_mod, 4, 3, mu=0.002)
+        o = oracle_mod.Oracle(4, 8, 8, -1.0, -1.0, 2.0 / 8, gamma=1.4, parabolic="navier_stokes", mu=0.002,
+                              prandtl=0.72, positivity=thr)
+    cfl = 0.5
+    u_ref, t_ref, _ = o.solve(u_lex, 0.0, 1e9, cfl, max_steps=K)
+    u_unl, _, _ = o0.solve(u_lex, 0.0, 1e9, cfl, max_steps=K)
+    assert np.max(np.abs(u_unl - u_ref)) > 1e-4
+    upd = tlw.semidiscretize(semi, semi.time_discretization, (0.0, float("inf")))
+    upd.u0.copy_(to_dev(mesh, u_lex))
+    lim = tlw.PositivityPreservingLimiterZhangShu(thresholds=thr, variables=(tlw.density, tlw.pressure))
+    sol = tlw.solve_lwfr(upd, (), 1e-6, None, time_step_computation=tlw.CFLBased(cfl), limiters=(lim,),
+                         max_steps=K)
+    assert sol.stats.naccept == K
+    assert abs(sol.t - t_ref) <= 1e-13 * t_ref
+    assert_close(to_lex(mesh, sol.u), u_ref, what=f"limited Navier-Stokes curved={curved}")
+    semi.close()

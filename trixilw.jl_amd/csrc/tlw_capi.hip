@@ -1259,10 +1259,6 @@ int tlw_set_positivity_limiter(tlw_ctx* c, int32_t enable, double threshold_dens
     if (!c) return TLW_EINVAL;
     if (enable && (std::isnan(threshold_density) || std::isnan(threshold_pressure)))
         return fail(c, TLW_EINVAL, "positivity thresholds must not be NaN");
-    if (enable && c->parabolic)
-        return fail(c, TLW_ENOTSUP, "the positivity limiter is not combined with the hyperbolic-parabolic path");
-    if (enable && c->curved)
-        return fail(c, TLW_ENOTSUP, "the positivity limiter is not combined with the curvilinear path");
     if (enable && c->adaptive)
         return fail(c, TLW_ENOTSUP, "the positivity limiter is not combined with adaptive steps (tlw_solve_init first)");
     HIPCHK(c, hipStreamSynchronize(c->stream));

@@ -2817,31 +2817,49 @@ __global__ __launch_bounds__(kSurfBlock) void k_limiter(KArgs a) {
     }
     if (act && changed) store4(u + off, un);
     if (a.reduce == RED_NONE) return;
-    // per-element max of λ1, λ2 of the limited state, then one partial per workgroup
+    // per-element max of λ1, λ2 of the limited state (curvilinear speeds carry the
+    // node's |J⁻¹|; Navier-Stokes adds the viscous bound), one partial per workgroup
     double l1, l2;
     bool bad;
-    node_speeds(un, c->gamma, l1, l2, bad);
+    if (a.Ja) {
+        double Jn[4];
+        load4(a.Ja + off, Jn);
+        curved_speeds(un, Jn, a.ijn[(size_t)e * NN + node], c->gamma, l1, l2, bad);
+    } else {
+        node_speeds(un, c->gamma, l1, l2, bad);
+    }
     if (act && bad) st->bad_state = 1;
+    const bool par = a.pv_off > 0;
     double* s_l1 = s_var;
     double* s_l2 = s_q[0];
+    double* s_lv = s_q[1];
+    double* s_mv = s_q[2];
     if (act) {
         s_l1[sidx] = l1;
         s_l2[sidx] = l2;
+        s_lv[sidx] = par ? (a.Ja ? lam_visc_curved(un, a.ijn[(size_t)e * NN + node], c) : lam_visc(un, a.inv_jac[e], c))
+                         : 0.0;
     }
     __syncthreads();
     if (act && node == 0) {
-        double m1 = 0.0, m2 = 0.0;
+        double m1 = 0.0, m2 = 0.0, mv = 0.0;
         for (int k = 0; k < NN; ++k) {
             m1 = dmax(m1, s_l1[sbase + k]);
             m2 = dmax(m2, s_l2[sbase + k]);
+            mv = dmax(mv, s_lv[sbase + k]);
         }
-        s_theta[lec] = a.inv_jac[e] * (m1 + m2);
+        s_theta[lec] = a.Ja ? m1 + m2 : a.inv_jac[e] * (m1 + m2);
+        s_mv[lec] = mv;
     }
     __syncthreads();
     if (tid == 0) {
-        double bm = 0.0;
-        for (int k = 0; k < nel; ++k) bm = dmax(bm, s_theta[k]);
+        double bm = 0.0, bv = 0.0;
+        for (int k = 0; k < nel; ++k) {
+            bm = dmax(bm, s_theta[k]);
+            bv = dmax(bv, s_mv[k]);
+        }
         slot_max(a.slots, blk.z, 0, bm);
+        if (par) slot_max(a.slots, blk.z, 1, bv);
     }
 }
 
