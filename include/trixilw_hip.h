@@ -331,9 +331,11 @@ int tlw_prescribe_dts(tlw_ctx* ctx, const double* dts_host, int64_t n);
  * reference's `tolerances`), the admissibility test of the updated state, then the controller
  * (β = 0.6, -0.2, 0, k = nnodes): a rejected attempt (non-admissible state or factor <= 0.81)
  * restarts from uⁿ with min(factor, 0.95)·dt, an accepted one advances t and proposes
- * min(factor, 1.5)·dt.  tlw_advance's iter counts accepted steps.  Polydeg 4, LW, Euler only (the
- * reference computes the estimate only in that kernel on TreeMesh); not combined with the
- * positivity limiter (TLW_ENOTSUP).  A rejected step whose factor is not finite (NaN state)
+ * min(factor, 1.5)·dt.  tlw_advance's iter counts accepted steps.  Euler LW on a TreeMesh at
+ * polydeg 4 (the reference's TreeMesh estimate is in that kernel), or on a curvilinear StructuredMesh
+ * at polydeg 3 or 4 (lw_volume_kernel_3!/4! of src/solvers/dgsem_structured/dg_2d.jl: u_np1 with the
+ * contravariant F̃ and the node's J⁻¹, u_np1_low without the last stage's term, 1595-1681 and
+ * 1983-2077); not combined with the positivity limiter (TLW_ENOTSUP).  A rejected step whose factor is not finite (NaN state)
  * shrinks dt by 0.95. */
 int tlw_solve_init_adaptive(tlw_ctx* ctx, double t0, double t_final, double dt_initial, double abstol,
                             double reltol);

@@ -131,7 +131,7 @@ template <class T, class Eq, class Base = Solver<T, Eq>> struct CurvedSolver : B
     // TreeMesh procedure with contravariant fluxes in the derivative
     // contractions and the node's J⁻¹; the Cartesian time average F̃ is kept
     // for the interfaces.  Source terms are `nothing`; the temporal-error
-    // estimate (adaptive mode) is not computed.
+    // estimate of adaptive steps is computed as the reference does.
     void lw_volume_curved(T* du, const T* u, double dt, int e, double alpha_dg = 1.0) {
         const auto& D = basis.D;
         const auto& Dh = basis.Dhat;
@@ -246,6 +246,29 @@ template <class T, class Eq, class Base = Solver<T, Eq>> struct CurvedSolver : B
             add_states(1.0 / 120.0, 1.0 / 24.0, 1.0 / 24.0, 2.0 / 3.0, 2.0 / 3.0);
             const double wt[5] = {1.0, -4.0, 6.0, -4.0, 1.0};  // f_tttt, with the 0.5 of dg_2d.jl:2018-2021
             stage(wt, 0.5, 1.0 / 120.0);
+        }
+        // temporal error estimate of the adaptive controller (dg_2d.jl:1595-1619, 1664-1681 at N=3;
+        // 1983-2007, 2058-2077 at N=4): u_np1 = u + J⁻¹(node) (-dt D)·F̃ with the contravariant F̃,
+        // u_np1_low the same without the last stage's term (f_ttt/24 at N=3, f_tttt/120 at N=4);
+        // the numerator is that term's update itself, as in the TreeMesh kernel
+        {
+            const double wlast = (N == 4) ? 1.0 / 120.0 : 1.0 / 24.0;
+            advance(dfc, dgc);
+            T dlo[MAXN][MAXN][NV];
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < n; ++i)
+                    for (int v = 0; v < NV; ++v) dlo[j][i][v] = wlast * w[j][i][v];
+            advance(Fc, Gc);
+            T err = T(0);
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < n; ++i)
+                    for (int v = 0; v < NV; ++v) {
+                        const T hi = u[idx(v, i, j, e)] + w[j][i][v];
+                        const T lo = hi - dlo[j][i][v];
+                        const T r = dlo[j][i][v] / (this->abstol + this->reltol * max_(abs_(hi), abs_(lo)));
+                        err = err + r * r;
+                    }
+            this->temporal_errors[e] = err;
         }
         for (int j = 0; j < n; ++j)
             for (int i = 0; i < n; ++i) {

@@ -102,3 +102,29 @@ def test_adaptive_rejects_unsupported(oracle_mod):
         _lib.check(_lib.lib().tlw_solve_init_adaptive(h, 0.0, 1.0, 1e-3, 1e-6, 1e-6), h)
     assert ei.value.code == _lib.TLW_ENOTSUP
     semi.close()
+
+
+@pytest.mark.parametrize("N,dt_initial", [(4, 1e-3), (3, 1e-3), (4, 5.0)])
+def test_adaptive_curved_matches_oracle(oracle_mod, N, dt_initial):
+    """Adaptive steps on the warped StructuredMesh: the temporal-error estimate
+    of the curvilinear LW kernel (structured dg_2d.jl, N=3 and N=4) drives the
+    same controller; accepted and rejected counts, final time and state match
+    the oracle's solve_adaptive ≤ 1e-12."""
+    import trixilw_hip as tlw
+    from gpu_helpers import setup_curved
+
+    torch_mod()
+    semi, _, u_lex, mesh = setup_curved(oracle_mod, N, 6, 6)
+    L = 20.0
+    o = oracle_mod.Oracle(N, 6, 6, 0.0, 0.0, L / 6, gamma=1.4, mapping=(oracle_mod.MAP_WARP, (L, 0.05)),
+                          tolerances=(TOL["abstol"], TOL["reltol"]))
+    tf = 0.4
+    u_ref, t_ref, dts, nrej = o.solve_adaptive(u_lex, 0.0, tf, dt_initial, max_attempts=20000)
+    sol = run_device(semi, mesh, u_lex, tf, dt_initial)
+    assert sol.stats.naccept == len(dts) > 3
+    assert sol.stats.nreject == nrej
+    if dt_initial > 1.0:
+        assert nrej >= 1
+    assert abs(sol.t - t_ref) <= 1e-13 * t_ref
+    assert_close(to_lex(mesh, sol.u), u_ref, what=f"curved adaptive N={N} dt0={dt_initial}")
+    semi.close()

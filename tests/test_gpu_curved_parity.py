@@ -293,6 +293,9 @@ def test_curved_rejects_unsupported(oracle_mod):
     with pytest.raises(_lib.TLWError) as ei:
         tlw.adjoint_step(torch.empty_like(u), u, u.clone(), semi, 0.0, 1e-3)
     assert ei.value.code == _lib.TLW_ENOTSUP
+    semi.close()
+    # adaptive steps take the Euler LW estimate: an MDRK context has none
+    semi, o, u_lex, mesh = setup_curved(oracle_mod, 3, 4, 4, td="mdrk")
     lw_update = tlw.semidiscretize(semi, semi.time_discretization, (0.0, 1.0))
     with pytest.raises(_lib.TLWError) as ei:
         tlw.solve_lwfr(lw_update, (), 1e-6, {"abstol": 1e-8, "reltol": 1e-8}, time_step_computation=tlw.Adaptive(),

@@ -350,3 +350,37 @@ def test_warped_mesh_fn_low_normals(oracle_mod, N):
     ja = Ja[0, 1, :, 0:2]
     dja = D @ ja
     assert np.max(np.abs(sum(w[q] * dja[q] for q in range(1, n - 2)))) > 1e-4
+
+
+def test_affine_curved_temporal_errors_and_adaptive_solve_equal_treemesh(oracle_mod):
+    """The curvilinear kernel's temporal-error estimate (structured dg_2d.jl,
+    u_np1 vs u_np1_low without the last stage's term) on an affine mesh equals
+    the TreeMesh kernel's at N=4, element by element, and an adaptive solve then
+    takes the same steps, accepted and rejected; at N=3 the estimate is the f_ttt
+    term's update, nonzero and smaller on a finer mesh."""
+    nx, ny, L = 6, 4, 20.0
+    h = L / nx
+    ic = vortex(L)
+    tol = dict(tolerances=(1e-7, 1e-7))
+    tree = oracle_mod.Oracle(4, nx, ny, 0.0, 0.0, h, flux_order=1, **tol)
+    curv = oracle_mod.Oracle(4, nx, ny, 0.0, 0.0, h, mapping=(oracle_mod.MAP_AFFINE, (0.0, 0.0, L, h * ny)), **tol)
+    x, y = tree.node_coords()
+    u = ic(x, y, 0.0) * (1 + 1e-2 * np.random.default_rng(5).standard_normal(x.shape + (4,)))
+    dt = 0.8 * tree.max_dt(u)
+    tree.rhs(u, 0.0, dt)
+    curv.rhs(u, 0.0, dt)
+    te_t, te_c = tree.aux("temporal_errors"), curv.aux("temporal_errors")
+    assert np.max(te_t) > 0 and rel_linf(te_c, te_t) < 1e-9
+    ut, tt, dts_t, rt = tree.solve_adaptive(u, 0.0, 0.5, 1e-3)
+    uc, tc, dts_c, rc = curv.solve_adaptive(u, 0.0, 0.5, 1e-3)
+    assert len(dts_t) == len(dts_c) > 3 and rt == rc and abs(tt - tc) < 1e-13
+    assert np.allclose(dts_t, dts_c, rtol=1e-9, atol=0)
+    assert rel_linf(uc, ut) < 1e-10
+    errs = []
+    for m in (4, 8):
+        c3 = oracle_mod.Oracle(3, m, m, 0.0, 0.0, L / m, mapping=(oracle_mod.MAP_WARP, (L, 0.05)), **tol)
+        xc, yc = c3.node_coords()
+        u3 = ic(xc, yc, 0.0)
+        c3.rhs(u3, 0.0, 0.5 * c3.max_dt(u3))
+        errs.append(np.sum(c3.aux("temporal_errors")))
+    assert errs[0] > errs[1] > 0

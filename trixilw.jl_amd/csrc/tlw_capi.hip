@@ -553,6 +553,8 @@ template <int N, int EB, bool SC> void launch_volume_curved_eb(tlw_ctx* c, const
     dim3 g(c->vgrid), b(VGeo<N, EB>::NT);
     if (kind == VOL_LW)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_LW, SC, false, true>), g, b, 0, c->stream, a);
+    else if (kind == VOL_LW_ADAPT)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_LW_ADAPT, SC, false, true>), g, b, 0, c->stream, a);
     else if (kind == VOL_MDRK1)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_MDRK1, SC, false, true>), g, b, 0, c->stream, a);
     else
@@ -1066,9 +1068,10 @@ int tlw_solve_init_adaptive(tlw_ctx* c, double t0, double t_final, double dt_ini
     if (!(abstol >= 0.0) || !(reltol >= 0.0) || !(abstol > 0.0 || reltol > 0.0) || !std::isfinite(abstol) ||
         !std::isfinite(reltol))
         return fail(c, TLW_EINVAL, "tolerances: abstol, reltol >= 0, not both zero");
-    if (c->N != 4 || c->time_disc != TLW_LW || c->parabolic || c->curved)
+    if (c->time_disc != TLW_LW || c->parabolic || (c->N != 4 && !c->curved))
         return fail(c, TLW_ENOTSUP,
-                    "adaptive steps need the temporal-error estimate of the Euler LW kernel for polydeg 4");
+                    "adaptive steps need the temporal-error estimate of an Euler LW kernel (TreeMesh polydeg 4, "
+                    "curvilinear polydeg 3 or 4)");
     if (c->positivity) return fail(c, TLW_ENOTSUP, "adaptive steps are not combined with the positivity limiter");
     if (int rcb = check_loop_boundary(c)) return rcb;
     HIPCHK(c, hipMemcpyAsync(c->d_cur, c->host_cur.data(), sizeof(int) * c->ninst, hipMemcpyHostToDevice,

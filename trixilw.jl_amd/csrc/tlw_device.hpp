@@ -1707,7 +1707,8 @@ template <int N, int EB>
 __device__ __forceinline__ void lwc_element(const double* uN, const double* J, double invJ, double gm1, int i, int j,
                                             bool act, int sidx, int sbase, double* s0, double* s1, double* s2,
                                             const double* sDm, const double* sDh, double* Uc, double* F, double* G,
-                                            double* vol) {
+                                            double* vol, double* terr = nullptr, double abstol = 0.0,
+                                            double reltol = 0.0) {
     using G_ = VGeo<N, EB>;
     constexpr int n = G_::n, S = G_::S;
     double f0x[4], f0y[4], cx[4], cy[4];
@@ -1875,8 +1876,35 @@ __device__ __forceinline__ void lwc_element(const double* uN, const double* J, d
     // before it is reused (polydeg 4 passed a barrier since)
     if constexpr (N == 3) __syncthreads();
     if (act) put8<S>(s2, sidx, cx, cy);
+    // the last stage's contravariant term for the temporal error; s0 is read
+    // only by the first contraction
+    if (terr) {
+        double tx[4], ty[4];
+        to_contravariant(J, dx, dy, tx, ty);
+        if (act) put8<S>(s0, sidx, tx, ty);
+    }
     __syncthreads();
     contract<n, S>(s2, sbase, i, j, sDh, vol);
+    if (terr) {
+        // temporal error of the node (structured dg_2d.jl:1595-1681 at N=3, 1983-2077 at N=4):
+        // u_np1 = u + J⁻¹(-dt D)·F̃c, u_np1_low without the last stage's term, whose update
+        // itself is the numerator
+        constexpr double wlast = (N == 4) ? 1.0 / 120.0 : 1.0 / 24.0;
+        double a1[4], a2[4];
+        contract<n, S>(s2, sbase, i, j, sDm, a1);
+        contract<n, S>(s0, sbase, i, j, sDm, a2);
+        double e = 0.0;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const double d = invJ * (wlast * a2[v]);
+            const double hi = fma(invJ, a1[v], uN[v]);
+            const double lo = hi - d;
+            const double r = d / (abstol + reltol * dmax(fabs(hi), fabs(lo)));
+            e = fma(r, r, e);
+        }
+        *terr = e;
+        __syncthreads();  // s0 is reused for the face records
+    }
 }
 
 // mdrk_kernel_1! for StructuredMesh (structured/multi_derivative_2d.jl:82-243):
@@ -1991,7 +2019,7 @@ template <int N, int EB, int KIND, bool SC, bool PAR, bool CURV>
 __device__ __forceinline__ void volume_body(const KArgs& a, int bid, double* lds, double* sDm, double* sDh, double* sD,
                                             double* s_alpha) {
     static_assert(!(SC && PAR), "shock capturing has no hyperbolic-parabolic form");
-    static_assert(!CURV || KIND != VOL_LW_ADAPT, "the curvilinear path is LW or MDRK");
+    static_assert(!CURV || !PAR || KIND != VOL_LW_ADAPT, "adaptive steps: Euler LW");
     static_assert(KIND != VOL_LW_WALL || (PAR && !CURV && !SC), "walls: the TreeMesh Navier-Stokes LW kernel");
     static_assert(!(CURV && PAR && SC), "no shock capturing with parabolic terms");
     using G_ = VGeo<N, EB>;
@@ -2035,6 +2063,9 @@ __device__ __forceinline__ void volume_body(const KArgs& a, int bid, double* lds
     if constexpr (CURV && !PAR) {
         if constexpr (KIND == VOL_LW) {
             lwc_element<N, EB>(uN, Jn, invJ, gm1, i, j, act, sidx, sbase, s0, s1, s2, sDm, sDh, Uc, F, G, vol);
+        } else if constexpr (KIND == VOL_LW_ADAPT) {
+            lwc_element<N, EB>(uN, Jn, invJ, gm1, i, j, act, sidx, sbase, s0, s1, s2, sDm, sDh, Uc, F, G, vol,
+                               &terr, st->abstol, st->reltol);
         } else if constexpr (KIND == VOL_MDRK1) {
             mdrkc1_element<N, EB>(uN, Jn, invJ, gm1, i, j, act, sidx, sbase, s0, s2, sDm, sDh, Uc, F, G, vol);
         } else {
