@@ -107,10 +107,12 @@ def volume_kernel_bytes(mesh, N, tile, record_doubles=10, parabolic=False, curve
     each), the face record (`record_doubles` doubles per face node) of every
     face slot whose neighbour lies in another volume workgroup of `tile`
     consecutive elements (or on the domain boundary), and the element's
-    neighbour ids and inverse Jacobian; Navier-Stokes adds the neighbours' face
-    states its BR1 gradient reads, a curvilinear mesh the node metrics (Ja and
-    J⁻¹, 5 doubles per node) and face normals (4 per face node).  Faces whose
-    two elements share a workgroup are resolved in LDS and cost nothing."""
+    neighbour ids and inverse Jacobian; Navier-Stokes adds the neighbour's face
+    state (4 doubles per face node) its BR1 gradient reads across those same
+    leaving faces, a curvilinear mesh the node metrics (Ja and J⁻¹, 5 doubles
+    per node) and face normals (4 per face node).  Faces whose two elements
+    share a workgroup need no HBM traffic (the neighbour's state is in the
+    workgroup's own tile) and are not counted."""
     n = N + 1
     E = mesh.n_elements
     nbr = np.asarray(mesh.neighbors, dtype=np.int64).reshape(E, 4)
@@ -118,7 +120,7 @@ def volume_kernel_bytes(mesh, N, tile, record_doubles=10, parabolic=False, curve
     leaving = (nbr < 0) | (tiles[np.clip(nbr, 0, None)] != tiles[:, None])
     b = 8.0 * (2 * 4 * n * n * E + int(leaving.sum()) * n * record_doubles) + 24.0 * E
     if parabolic:
-        b += 8.0 * 4 * 4 * n * E
+        b += 8.0 * 4 * n * int(leaving.sum())
     if curved:
         b += 8.0 * (5 * n * n + 4 * 4 * n) * E
     return b, float(leaving.mean())
@@ -401,6 +403,7 @@ def measure_navier_stokes(steps, cpu_steps, threads, with_cpu=True):
            "volume_achieved_GBs": round(vol_bytes / (r["vol_ms"] * 1e-3) / 1e9, 2),
            "volume_alg_bytes_per_dof": round(vol_bytes / dofs, 2),
            "volume_valu": load_valu("lw_navierstokes_shear_1024x1024_N4", r["vol_ms"] * 1e-3),
+           "volume_traffic": load_traffic("lw_navierstokes_shear_1024x1024_N4")[0],
            "elements_per_workgroup": r["tile"], "tile_autotune_ms": r["tune"],
            "config": f"compressible Navier-Stokes, conservative gradients, mu {NS_MU}, Pr {NS_PR}, periodic "
                      f"[-1,1]^2 {mesh.nx}x{mesh.nx} TreeMesh, polydeg 4, LW, CFL {cfl}"}
