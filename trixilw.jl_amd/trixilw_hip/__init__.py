@@ -11,7 +11,7 @@ from .batch import SemidiscretizationBatch, SemidiscretizationEnsemble, draw_vor
 from .equations import (Adiabatic, BoundaryConditionDirichlet, BoundaryConditionNavierStokesWall, CompressibleEulerEquations2D,
                         CompressibleNavierStokesDiffusion2D, ConstantState, Isothermal, NoSlip,
                         FourQuadrantRiemann, GradientVariablesConservative, GradientVariablesPrimitive,
-                        IsentropicVortex, LinearScalarAdvectionEquation2D, TaylorGreenVortex, boundary_condition_slip_wall_horizontal,
+                        IsentropicVortex, TaylorGreenVortex, boundary_condition_slip_wall_horizontal,
                         boundary_condition_slip_wall_vertical)
 from .integrator import (LWIntegrator, LWSolution, compute_dt, isapprox, isfinished, perform_step, set_dt,
                          set_t_and_iter, solve_lwfr)
@@ -28,7 +28,7 @@ __all__ = [
     "LobattoLegendreBasis", "CompressibleEulerEquations2D", "ConstantState", "FourQuadrantRiemann",
     "boundary_condition_slip_wall_vertical", "boundary_condition_slip_wall_horizontal", "NoSlip", "Adiabatic", "Isothermal",
     "BoundaryConditionNavierStokesWall", "BoundaryConditionDirichlet",
-    "IsentropicVortex", "TaylorGreenVortex", "LinearScalarAdvectionEquation2D", "LWIntegrator", "LWSolution", "compute_dt", "isapprox",
+    "IsentropicVortex", "TaylorGreenVortex", "LWIntegrator", "LWSolution", "compute_dt", "isapprox",
     "isfinished", "perform_step", "set_dt", "set_t_and_iter", "solve_lwfr", "TreeMesh",
     "StructuredMesh", "coordinates2mapping", "DGSEM", "LW", "MDRK",
     "CFLBased", "Adaptive", "LWUpdate", "SemidiscretizationHyperbolic", "VolumeIntegralFR", "flux_lax_friedrichs",

@@ -181,11 +181,6 @@ class CompressibleEulerEquations2D:
         return rho, v1, v2, p
 
 
-@dataclass
-class LinearScalarAdvectionEquation2D:
-    advection_velocity: tuple = (1.0, 1.0)
-
-
 class GradientVariablesConservative:
     """Gradients of the conservative variables (TrixiLW's
     GradientVariablesConservative, src/equations/compressible_navier_stokes_2d.jl:14-57)."""
