@@ -16,8 +16,10 @@ import numpy as np
 HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "liboracle_tlw.so"
 
-BV_NONE, BV_VORTEX, BV_CONSTANT, BV_RIEMANN4, BV_SLIP_WALL, BV_CALLBACK = 0, 1, 2, 3, 4, 5
+BV_NONE, BV_VORTEX, BV_CONSTANT, BV_RIEMANN4, BV_SLIP_WALL, BV_CALLBACK, BV_CONVERGENCE_TEST = 0, 1, 2, 3, 4, 5, 6
+SRC_NONE, SRC_CONVERGENCE_TEST, SRC_CALLBACK = 0, 1, 2
 BC_FN = C.CFUNCTYPE(None, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double))
+SRC_FN = C.CFUNCTYPE(None, C.POINTER(C.c_double), C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double))
 
 
 class _Desc(C.Structure):
@@ -58,6 +60,8 @@ class _Desc(C.Structure):
         ("wall_iso", C.c_int * 4),
         ("wall_T", C.c_double * 4),
         ("bc_fn", C.c_void_p),
+        ("source_kind", C.c_int),
+        ("src_fn", C.c_void_p),
     ]
 
 
@@ -136,7 +140,7 @@ class Oracle:
                  adv=(1.0, 1.0), flux_order=0, bc_kind=BV_NONE, bc_par=(), tolerances=(0.0, 0.0), nthreads=0,
                  shock_capturing=False, alpha_max=0.5, alpha_min=0.001, alpha_smooth=True, positivity=None,
                  parabolic=None, mu=0.0, prandtl=0.72, mapping=None, reconstruction="first_order",
-                 ns_walls=None, wall_temperature=None, bc_fn=None):
+                 ns_walls=None, wall_temperature=None, bc_fn=None, source_terms=None):
         """``positivity``: (threshold_density, threshold_pressure) of a Zhang-Shu
         limiter that ``solve`` applies as a stage limiter, or None.
         ``parabolic``: None, "laplace" (LaplaceDiffusion2D with diffusivity ``mu``,
@@ -152,7 +156,10 @@ class Oracle:
         ``reconstruction``: "first_order" or "muscl_hancock" subcell FV of the
         shock-capturing blend (Euler).
         ``bc_fn``: with bc_kind=BV_CALLBACK, the boundary value function
-        f(x, y, t) -> 4 conservative values, called pointwise (Python)."""
+        f(x, y, t) -> 4 conservative values, called pointwise (Python).
+        ``source_terms``: None, "convergence_test" (Trixi's
+        source_terms_convergence_test) or a Python function s(u, x, y, t) -> 4
+        values, called pointwise (Euler on the TreeMesh)."""
         d = _Desc()
         d.polydeg, d.nx, d.ny = polydeg, nx, ny
         d.xmin, d.ymin, d.h = xmin, ymin, h
@@ -195,6 +202,17 @@ class Oracle:
 
             self._bc_cb = BC_FN(_cb)  # kept alive with the oracle
             d.bc_fn = C.cast(self._bc_cb, C.c_void_p)
+        if source_terms == "convergence_test":
+            d.source_kind = SRC_CONVERGENCE_TEST
+        elif source_terms is not None:
+            def _scb(u, x, y, t, out, f=source_terms):
+                v = f(np.array([u[0], u[1], u[2], u[3]]), x, y, t)
+                for q in range(4):
+                    out[q] = float(v[q])
+
+            self._src_cb = SRC_FN(_scb)
+            d.source_kind = SRC_CALLBACK
+            d.src_fn = C.cast(self._src_cb, C.c_void_p)
         self._desc = d
         self._h = lib().tlwo_create(C.byref(d))
         if not self._h:

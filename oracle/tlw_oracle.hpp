@@ -321,7 +321,61 @@ template <class T> struct LinAdv2D {
 // src/equations/equations.jl:95-139).
 // BV_CALLBACK: a caller-defined boundary_value_function (x, y, t) -> conservative
 // state, Trixi's BoundaryConditionDirichlet(f) for any f (equations.jl:6-37)
-enum BoundaryValueKind { BV_NONE = 0, BV_VORTEX = 1, BV_CONSTANT = 2, BV_RIEMANN4 = 3, BV_SLIP_WALL = 4, BV_CALLBACK = 5 };
+// BV_CONVERGENCE_TEST: Trixi's initial_condition_convergence_test for
+// CompressibleEulerEquations2D as a boundary value function (the reference's
+// elixir_euler_source_terms_nonperiodic.jl): ρ = ρv1 = ρv2 = φ, ρe = φ² with
+// φ = 2 + 0.1 sin(π(x + y - t)).
+enum BoundaryValueKind {
+    BV_NONE = 0, BV_VORTEX = 1, BV_CONSTANT = 2, BV_RIEMANN4 = 3, BV_SLIP_WALL = 4, BV_CALLBACK = 5,
+    BV_CONVERGENCE_TEST = 6
+};
+
+// The manufactured solution of Trixi's convergence test for the Euler
+// equations (initial_condition_convergence_test, c = 2, A = 0.1, L = 2, ω = π).
+inline void convergence_test_state(double x, double y, double t, double* u) {
+    const double phi = 2.0 + 0.1 * std::sin(3.141592653589793 * (x + y - t));
+    u[0] = u[1] = u[2] = phi;
+    u[3] = phi * phi;
+}
+
+// Source terms s(u, x, t) (the `source_terms` of SemidiscretizationHyperbolic),
+// which the LW kernels fold into the Taylor procedure (dg_2d.jl:89-176).
+// SRC_CONVERGENCE_TEST: Trixi's source_terms_convergence_test for
+// CompressibleEulerEquations2D, the residual ∂t u + ∇·f(u) of the state above
+// (with φ' = ∂x φ = π 0.1 cos(π(x + y - t)) and p = (γ-1)(φ² - φ)):
+// (φ', φ'(1 + τ), φ'(1 + τ), 2φ'(φ + τ)), τ = (2φ - 1)(γ - 1); it does not
+// depend on u.  SRC_CALLBACK: a caller's function of (u, x, y, t).
+enum SourceKind { SRC_NONE = 0, SRC_CONVERGENCE_TEST = 1, SRC_CALLBACK = 2 };
+
+struct SourceTerm {
+    int kind = SRC_NONE;
+    double gamma = 1.4;
+    void (*fn)(const double*, double, double, double, double*) = nullptr;
+    // NV = 4 (Euler); other equations take no source terms
+    template <int NV, class T> void eval(const T* u, double x, double y, double t, T* s) const {
+        if constexpr (NV != 4) {
+            for (int v = 0; v < NV; ++v) s[v] = T(0);
+            return;
+        } else if (kind == SRC_CONVERGENCE_TEST) {
+            const double w = 3.141592653589793;
+            const double arg = w * (x + y - t);
+            const double rho = 2.0 + 0.1 * std::sin(arg);
+            const double rho_x = w * 0.1 * std::cos(arg);
+            const double tmp = (2.0 * rho - 1.0) * (gamma - 1.0);
+            s[0] = T(rho_x);
+            s[1] = T(rho_x * (1.0 + tmp));
+            s[2] = s[1];
+            s[3] = T(2.0 * rho_x * (rho + tmp));
+        } else if (kind == SRC_CALLBACK) {
+            double uv[4], sv[4];
+            for (int v = 0; v < 4; ++v) uv[v] = value(u[v]);
+            fn(uv, x, y, t, sv);
+            for (int v = 0; v < 4; ++v) s[v] = T(sv[v]);
+        } else {
+            for (int v = 0; v < 4; ++v) s[v] = T(0);
+        }
+    }
+};
 
 struct BoundaryValue {
     int kind = BV_NONE;
@@ -335,6 +389,10 @@ struct BoundaryValue {
         double rho, v1, v2, p, gamma;
         if (kind == BV_CALLBACK) {
             fn(x, y, t, u);
+            return;
+        }
+        if (kind == BV_CONVERGENCE_TEST) {
+            convergence_test_state(x, y, t, u);
             return;
         }
         if (kind == BV_VORTEX) {
@@ -455,6 +513,11 @@ template <class T, class Eq> struct Solver {
     std::vector<T> fn_low;  // [v, l, face, e]
     std::vector<T> U, F, sfv, temporal_errors;
     std::vector<T> us, U2, F2;  // MDRK stage cache (containers.jl:83-98)
+    // source terms and the t of the current rhs call; S2 is the MDRK stage
+    // cache's source part (s + s_t/6 of stage 1, read by stage 2)
+    SourceTerm src;
+    double src_t = 0.0;
+    std::vector<T> S2;
 
     virtual ~Solver() = default;
     // node coordinates (TreeMesh: from the element's centre and edge length)
@@ -482,6 +545,109 @@ template <class T, class Eq> struct Solver {
         if (us.size() != nd) us.assign(nd, T(0));
         if (U2.size() != nd) U2.assign(nd, T(0));
         if (F2.size() != 2 * nd) F2.assign(2 * nd, T(0));
+        if (S2.size() != nd) S2.assign(nd, T(0));
+    }
+
+    // The source-term finite differences in time (dg_2d.jl:89-176) at the node
+    // (x, y): s(w, Δt) = source_terms(w, x, t + Δt), each stencil state at the
+    // time level it approximates.  The argument order of each helper is the
+    // reference's (u, up, upp, um, umm), so that its call sites below can pass
+    // the states exactly as the reference's kernels do.
+    void src_at(const T* w, double x, double y, double dtoff, T* s) const { src.template eval<NV>(w, x, y, src_t + dtoff, s); }
+    void src_t_N34(const T* up, const T* upp, const T* um, const T* umm, double x, double y, double dt, double t0,
+                   T* st) const {
+        T a[NV], b[NV], c[NV], d[NV];
+        src.template eval<NV>(upp, x, y, t0 + 2.0 * dt, a);
+        src.template eval<NV>(up, x, y, t0 + dt, b);
+        src.template eval<NV>(um, x, y, t0 - dt, c);
+        src.template eval<NV>(umm, x, y, t0 - 2.0 * dt, d);
+        for (int v = 0; v < NV; ++v) st[v] = (1.0 / 12.0) * (-a[v] + 8.0 * b[v] - 8.0 * c[v] + d[v]);
+    }
+    void src_tt_N23(const T* u, const T* up, const T* um, double x, double y, double dt, T* s) const {
+        T a[NV], b[NV], c[NV];
+        src.template eval<NV>(up, x, y, src_t + dt, a);
+        src.template eval<NV>(u, x, y, src_t, b);
+        src.template eval<NV>(um, x, y, src_t - dt, c);
+        for (int v = 0; v < NV; ++v) s[v] = a[v] - 2.0 * b[v] + c[v];
+    }
+    void src_ttt_N34(const T* up, const T* upp, const T* um, const T* umm, double x, double y, double dt,
+                     T* s) const {
+        T a[NV], b[NV], c[NV], d[NV];
+        src.template eval<NV>(upp, x, y, src_t + 2.0 * dt, a);
+        src.template eval<NV>(up, x, y, src_t + dt, b);
+        src.template eval<NV>(um, x, y, src_t - dt, c);
+        src.template eval<NV>(umm, x, y, src_t - 2.0 * dt, d);
+        for (int v = 0; v < NV; ++v) s[v] = 0.5 * (a[v] - 2.0 * b[v] + 2.0 * c[v] - d[v]);
+    }
+    void src_tttt_N4(const T* u, const T* up, const T* upp, const T* um, const T* umm, double x, double y, double dt,
+                     T* s) const {
+        T a[NV], b[NV], c[NV], d[NV], z[NV];
+        src.template eval<NV>(upp, x, y, src_t + 2.0 * dt, a);
+        src.template eval<NV>(up, x, y, src_t + dt, b);
+        src.template eval<NV>(u, x, y, src_t, z);
+        src.template eval<NV>(um, x, y, src_t - dt, c);
+        src.template eval<NV>(umm, x, y, src_t - 2.0 * dt, d);
+        for (int v = 0; v < NV; ++v) s[v] = a[v] - 4.0 * b[v] + 6.0 * z[v] - 4.0 * c[v] + d[v];
+    }
+    using NodeArr = T[MAXN][MAXN][NV];
+    // S = s(uⁿ, t), ut += dt s (dg_2d.jl:980-987: the source has no Jacobian factor)
+    void src_init(const T* u, int e, double dt, NodeArr& ut, NodeArr& Sc) const {
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) {
+                double x, y;
+                node_xy(i, j, e, x, y);
+                T s[NV];
+                src_at(&u[idx(0, i, j, e)], x, y, 0.0, s);
+                for (int v = 0; v < NV; ++v) {
+                    Sc[j][i][v] = s[v];
+                    ut[j][i][v] = ut[j][i][v] + dt * s[v];
+                }
+            }
+    }
+    // s_t of the stage-1 states: S += s_t/2, utt += dt s_t (1033-1045)
+    void src_stage_t(int e, double dt, const NodeArr& up, const NodeArr& upp, const NodeArr& um, const NodeArr& umm,
+                     NodeArr& utt, NodeArr& Sc) const {
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) {
+                double x, y;
+                node_xy(i, j, e, x, y);
+                T s[NV];
+                src_t_N34(up[j][i], upp[j][i], um[j][i], umm[j][i], x, y, dt, src_t, s);
+                for (int v = 0; v < NV; ++v) {
+                    Sc[j][i][v] = Sc[j][i][v] + 0.5 * s[v];
+                    utt[j][i][v] = utt[j][i][v] + dt * s[v];
+                }
+            }
+    }
+    // s_tt (three-point) of the stage-2 states: S += s_tt/6, uttt += dt s_tt (1093-1103)
+    void src_stage_tt(const T* u, int e, double dt, const NodeArr& up, const NodeArr& um, NodeArr& uttt,
+                      NodeArr& Sc) const {
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) {
+                double x, y;
+                node_xy(i, j, e, x, y);
+                T s[NV];
+                src_tt_N23(&u[idx(0, i, j, e)], up[j][i], um[j][i], x, y, dt, s);
+                for (int v = 0; v < NV; ++v) {
+                    Sc[j][i][v] = Sc[j][i][v] + (1.0 / 6.0) * s[v];
+                    uttt[j][i][v] = uttt[j][i][v] + dt * s[v];
+                }
+            }
+    }
+    // s_ttt of the stage-3 states: S += s_ttt/24, utttt += dt s_ttt (1148-1160)
+    void src_stage_ttt(int e, double dt, const NodeArr& up, const NodeArr& upp, const NodeArr& um,
+                       const NodeArr& umm, NodeArr& utttt, NodeArr& Sc) const {
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) {
+                double x, y;
+                node_xy(i, j, e, x, y);
+                T s[NV];
+                src_ttt_N34(up[j][i], upp[j][i], um[j][i], umm[j][i], x, y, dt, s);
+                for (int v = 0; v < NV; ++v) {
+                    Sc[j][i][v] = Sc[j][i][v] + (1.0 / 24.0) * s[v];
+                    utttt[j][i][v] = utttt[j][i][v] + dt * s[v];
+                }
+            }
     }
     size_t ndofs_total() const { return (size_t)mesh.E * n * n * NV; }
     size_t idx(int v, int i, int j, int e) const { return (((size_t)e * n + j) * n + i) * NV + v; }
@@ -490,16 +656,24 @@ template <class T, class Eq> struct Solver {
 
     // ---------------- LW volume kernel, N = 4 ----------------
     // lw_volume_kernel_4!, src/solvers/dgsem_tree/dg_2d.jl:918-1267.
-    // Source terms are `nothing` on this path (the calc_source* methods at
-    // dg_2d.jl:92-176 return zero), so their zero contributions are omitted.
+    // With source terms (src.kind != SRC_NONE) each time derivative gets dt times
+    // the source's own derivative at the stencil states of that point of the
+    // procedure (980-987, 1033-1045, 1093-1103, 1148-1160) and S = s + s_t/2 +
+    // s_tt/6 + s_ttt/24 + s_tttt/120 enters du as -S/J⁻¹ (1229-1247), so that the
+    // Jacobian turns it into +S.  s_tt takes the three-point difference
+    // (calc_source_tt_N23) and s_tttt the reference's argument order
+    // (u, up, um, upp, umm), which swaps the states of t + 2dt and t - dt; a
+    // source that does not depend on u, as the convergence test's, is not
+    // affected.  Without source terms these steps are skipped (they add zero).
     void lw_volume_kernel_4(T* du, const T* u, double dt, int e, double alpha) {
         const auto& D = basis.D;
         const auto& Dh = basis.Dhat;
         const double ij = mesh.inv_jac[e];
+        const bool has_src = src.kind != SRC_NONE;
         T f[MAXN][MAXN][NV], g[MAXN][MAXN][NV], Fc[MAXN][MAXN][NV], Gc[MAXN][MAXN][NV];
         T ut[MAXN][MAXN][NV], utt[MAXN][MAXN][NV], uttt[MAXN][MAXN][NV], utttt[MAXN][MAXN][NV];
         T Uc[MAXN][MAXN][NV], up[MAXN][MAXN][NV], um[MAXN][MAXN][NV], upp[MAXN][MAXN][NV], umm[MAXN][MAXN][NV];
-        T unp1[MAXN][MAXN][NV], unp1low[MAXN][MAXN][NV], dlow[MAXN][MAXN][NV];
+        T unp1[MAXN][MAXN][NV], unp1low[MAXN][MAXN][NV], dlow[MAXN][MAXN][NV], Sc[MAXN][MAXN][NV];
         for (int j = 0; j < n; ++j)
             for (int i = 0; i < n; ++i)
                 for (int v = 0; v < NV; ++v) ut[j][i][v] = utt[j][i][v] = uttt[j][i][v] = utttt[j][i][v] = dlow[j][i][v] = T(0);
@@ -522,6 +696,7 @@ template <class T, class Eq> struct Solver {
         for (int j = 0; j < n; ++j)
             for (int i = 0; i < n; ++i)
                 for (int v = 0; v < NV; ++v) ut[j][i][v] = ut[j][i][v] * ij;
+        if (has_src) src_init(u, e, dt, ut, Sc);
         // 995-1022: first Taylor states, f_t, utt
         for (int j = 0; j < n; ++j)
             for (int i = 0; i < n; ++i) {
@@ -553,6 +728,7 @@ template <class T, class Eq> struct Solver {
         for (int j = 0; j < n; ++j)
             for (int i = 0; i < n; ++i)
                 for (int v = 0; v < NV; ++v) utt[j][i][v] = utt[j][i][v] * ij;
+        if (has_src) src_stage_t(e, dt, up, upp, um, umm, utt, Sc);
         // 1047-1080: f_tt, uttt
         for (int j = 0; j < n; ++j)
             for (int i = 0; i < n; ++i) {
@@ -584,6 +760,7 @@ template <class T, class Eq> struct Solver {
         for (int j = 0; j < n; ++j)
             for (int i = 0; i < n; ++i)
                 for (int v = 0; v < NV; ++v) uttt[j][i][v] = uttt[j][i][v] * ij;
+        if (has_src) src_stage_tt(u, e, dt, up, um, uttt, Sc);
         // 1103-1136: f_ttt, utttt
         for (int j = 0; j < n; ++j)
             for (int i = 0; i < n; ++i) {
@@ -615,6 +792,7 @@ template <class T, class Eq> struct Solver {
         for (int j = 0; j < n; ++j)
             for (int i = 0; i < n; ++i)
                 for (int v = 0; v < NV; ++v) utttt[j][i][v] = utttt[j][i][v] * ij;
+        if (has_src) src_stage_ttt(e, dt, up, upp, um, umm, utttt, Sc);
         // 1160-1249: f_tttt (with the reference's 0.5 factor), F̃, volume term
         for (int j = 0; j < n; ++j)
             for (int i = 0; i < n; ++i) {
@@ -659,10 +837,20 @@ template <class T, class Eq> struct Solver {
                         du[idx(v, i, jj, e)] = du[idx(v, i, jj, e)] + (alpha * Dh[jj][j]) * Gc[j][i][v];
                         unp1[jj][i][v] = unp1[jj][i][v] + (-dt * ij * D[jj][j]) * Gc[j][i][v];
                     }
+                if (has_src) {
+                    // 1228-1233: the reference passes (u, up, um, upp, umm) to
+                    // calc_source_tttt_N4(u, up, upp, um, umm, ...)
+                    double x, y;
+                    node_xy(i, j, e, x, y);
+                    T s4[NV];
+                    src_tttt_N4(&u[idx(0, i, j, e)], up[j][i], um[j][i], upp[j][i], umm[j][i], x, y, dt, s4);
+                    for (int v = 0; v < NV; ++v) Sc[j][i][v] = Sc[j][i][v] + (1.0 / 120.0) * s4[v];
+                }
                 for (int v = 0; v < NV; ++v) {
                     F[fidx(v, 0, i, j, e)] = Fc[j][i][v];
                     F[fidx(v, 1, i, j, e)] = Gc[j][i][v];
                     U[idx(v, i, j, e)] = Uc[j][i][v];
+                    if (has_src) du[idx(v, i, j, e)] = du[idx(v, i, j, e)] + (-1.0 / ij) * Sc[j][i][v];
                 }
             }
         // 1251-1265: temporal error estimate for the adaptive controller.  The
@@ -681,13 +869,18 @@ template <class T, class Eq> struct Solver {
     }
 
     // ---------------- LW volume kernel, N = 3 ----------------
-    // lw_volume_kernel_3!, src/solvers/dgsem_tree/dg_2d.jl:674-916
+    // lw_volume_kernel_3!, src/solvers/dgsem_tree/dg_2d.jl:674-916.  Source
+    // terms as in the N = 4 kernel (732-740, 785-797, 841-851), with
+    // S = s + s_t/2 + s_tt/6 + s_ttt/24; s_ttt at the end takes the reference's
+    // arguments (u, up, upp, upp, umm) (894-896: the state of t + 2dt also at
+    // t - dt), which a source independent of u does not see.
     void lw_volume_kernel_3(T* du, const T* u, double dt, int e, double alpha) {
         const auto& D = basis.D;
         const auto& Dh = basis.Dhat;
         const double ij = mesh.inv_jac[e];
+        const bool has_src = src.kind != SRC_NONE;
         T f[MAXN][MAXN][NV], g[MAXN][MAXN][NV], Fc[MAXN][MAXN][NV], Gc[MAXN][MAXN][NV];
-        T ut[MAXN][MAXN][NV], utt[MAXN][MAXN][NV], uttt[MAXN][MAXN][NV];
+        T ut[MAXN][MAXN][NV], utt[MAXN][MAXN][NV], uttt[MAXN][MAXN][NV], Sc[MAXN][MAXN][NV];
         T Uc[MAXN][MAXN][NV], up[MAXN][MAXN][NV], um[MAXN][MAXN][NV], upp[MAXN][MAXN][NV], umm[MAXN][MAXN][NV];
         for (int j = 0; j < n; ++j)
             for (int i = 0; i < n; ++i)
@@ -709,6 +902,7 @@ template <class T, class Eq> struct Solver {
         for (int j = 0; j < n; ++j)
             for (int i = 0; i < n; ++i)
                 for (int v = 0; v < NV; ++v) ut[j][i][v] = ut[j][i][v] * ij;
+        if (has_src) src_init(u, e, dt, ut, Sc);
         // 740-769
         for (int j = 0; j < n; ++j)
             for (int i = 0; i < n; ++i) {
@@ -740,6 +934,7 @@ template <class T, class Eq> struct Solver {
         for (int j = 0; j < n; ++j)
             for (int i = 0; i < n; ++i)
                 for (int v = 0; v < NV; ++v) utt[j][i][v] = utt[j][i][v] * ij;
+        if (has_src) src_stage_t(e, dt, up, upp, um, umm, utt, Sc);
         // 799-827: f_tt = fp - 2f + fm (two-point second difference)
         for (int j = 0; j < n; ++j)
             for (int i = 0; i < n; ++i) {
@@ -769,6 +964,7 @@ template <class T, class Eq> struct Solver {
         for (int j = 0; j < n; ++j)
             for (int i = 0; i < n; ++i)
                 for (int v = 0; v < NV; ++v) uttt[j][i][v] = uttt[j][i][v] * ij;
+        if (has_src) src_stage_tt(u, e, dt, up, um, uttt, Sc);
         // 853-913: f_ttt, F̃, volume term
         for (int j = 0; j < n; ++j)
             for (int i = 0; i < n; ++i) {
@@ -797,10 +993,18 @@ template <class T, class Eq> struct Solver {
                 for (int jj = 0; jj < n; ++jj)
                     for (int v = 0; v < NV; ++v)
                         du[idx(v, i, jj, e)] = du[idx(v, i, jj, e)] + (alpha * Dh[jj][j]) * Gc[j][i][v];
+                if (has_src) {
+                    double x, y;
+                    node_xy(i, j, e, x, y);
+                    T s3[NV];
+                    src_ttt_N34(up[j][i], upp[j][i], upp[j][i], umm[j][i], x, y, dt, s3);
+                    for (int v = 0; v < NV; ++v) Sc[j][i][v] = Sc[j][i][v] + (1.0 / 24.0) * s3[v];
+                }
                 for (int v = 0; v < NV; ++v) {
                     U[idx(v, i, j, e)] = Uc[j][i][v];
                     F[fidx(v, 0, i, j, e)] = Fc[j][i][v];
                     F[fidx(v, 1, i, j, e)] = Gc[j][i][v];
+                    if (has_src) du[idx(v, i, j, e)] = du[idx(v, i, j, e)] + (-1.0 / ij) * Sc[j][i][v];
                 }
             }
     }
@@ -811,12 +1015,15 @@ template <class T, class Eq> struct Solver {
     // MDRKElementCache does not have (containers.jl:83-98), so the reference
     // raises there; that low-order update feeds only the admissibility
     // machinery and is left out.  du, Ũ, F̃, U2, F2 follow the kernel.
+    // Source terms (323-331, 351-379): ut += dt s(uⁿ, t); S = s/2 + s_t/8 enters
+    // du as -S/J⁻¹ and S2 = s + s_t/6 goes to the stage cache.
     void mdrk_kernel_1(T* du, const T* u, double dt, int e, double alpha) {
         const auto& D = basis.D;
         const auto& Dh = basis.Dhat;
         const double ij = mesh.inv_jac[e];
+        const bool has_src = src.kind != SRC_NONE;
         T Fc[MAXN][MAXN][NV], F2c[MAXN][MAXN][NV], Gc[MAXN][MAXN][NV], G2c[MAXN][MAXN][NV];
-        T ut[MAXN][MAXN][NV], Uc[MAXN][MAXN][NV], U2c[MAXN][MAXN][NV];
+        T ut[MAXN][MAXN][NV], Uc[MAXN][MAXN][NV], U2c[MAXN][MAXN][NV], Sc[MAXN][MAXN][NV];
         for (int j = 0; j < n; ++j)
             for (int i = 0; i < n; ++i)
                 for (int v = 0; v < NV; ++v) ut[j][i][v] = T(0);
@@ -836,6 +1043,7 @@ template <class T, class Eq> struct Solver {
         for (int j = 0; j < n; ++j)
             for (int i = 0; i < n; ++i)
                 for (int v = 0; v < NV; ++v) ut[j][i][v] = ut[j][i][v] * ij;
+        if (has_src) src_init(u, e, dt, ut, Sc);  // Sc = s; halved below (S = s/2, S2 = s)
         for (int j = 0; j < n; ++j)
             for (int i = 0; i < n; ++i) {
                 const T* un = &u[idx(0, i, j, e)];
@@ -862,6 +1070,17 @@ template <class T, class Eq> struct Solver {
                     G2c[j][i][v] = G2c[j][i][v] + (1.0 / 6.0) * gt;
                     U2c[j][i][v] = U2c[j][i][v] + (1.0 / 6.0) * ut[j][i][v];
                 }
+                T S1[NV], S2n[NV];
+                if (has_src) {
+                    double x, y;
+                    node_xy(i, j, e, x, y);
+                    T st[NV];
+                    src_t_N34(up, upp, um, umm, x, y, dt, src_t, st);
+                    for (int v = 0; v < NV; ++v) {
+                        S1[v] = 0.5 * Sc[j][i][v] + 0.125 * st[v];
+                        S2n[v] = Sc[j][i][v] + (1.0 / 6.0) * st[v];
+                    }
+                }
                 for (int ii = 0; ii < n; ++ii)
                     for (int v = 0; v < NV; ++v)
                         du[idx(v, ii, j, e)] = du[idx(v, ii, j, e)] + (alpha * Dh[ii][i]) * Fc[j][i][v];
@@ -875,15 +1094,22 @@ template <class T, class Eq> struct Solver {
                     U2[idx(v, i, j, e)] = U2c[j][i][v];
                     F2[fidx(v, 0, i, j, e)] = F2c[j][i][v];
                     F2[fidx(v, 1, i, j, e)] = G2c[j][i][v];
+                    if (has_src) {
+                        du[idx(v, i, j, e)] = du[idx(v, i, j, e)] + (-1.0 / ij) * S1[v];
+                        S2[idx(v, i, j, e)] = S2n[v];
+                    }
                 }
             }
     }
 
     // mdrk_kernel_2!, src/solvers/dgsem_tree/multi_derivative_2d.jl:423-555
+    // Source terms (487-522): S starts from the stage cache's S2, ust += dt
+    // s(us, t + dt/2), S += s_t/3 with s_t of the us stencil about t + dt/2.
     void mdrk_kernel_2(T* du, double dt, int e, double alpha) {
         const auto& D = basis.D;
         const auto& Dh = basis.Dhat;
         const double ij = mesh.inv_jac[e];
+        const bool has_src = src.kind != SRC_NONE;
         T Fc[MAXN][MAXN][NV], Gc[MAXN][MAXN][NV], ust[MAXN][MAXN][NV], Uc[MAXN][MAXN][NV];
         for (int j = 0; j < n; ++j)
             for (int i = 0; i < n; ++i)
@@ -906,6 +1132,15 @@ template <class T, class Eq> struct Solver {
         for (int j = 0; j < n; ++j)
             for (int i = 0; i < n; ++i)
                 for (int v = 0; v < NV; ++v) ust[j][i][v] = ust[j][i][v] * ij;
+        if (has_src)
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < n; ++i) {
+                    double x, y;
+                    node_xy(i, j, e, x, y);
+                    T s[NV];
+                    src_at(&us[idx(0, i, j, e)], x, y, 0.5 * dt, s);
+                    for (int v = 0; v < NV; ++v) ust[j][i][v] = ust[j][i][v] + dt * s[v];
+                }
         for (int j = 0; j < n; ++j)
             for (int i = 0; i < n; ++i) {
                 const T* usn = &us[idx(0, i, j, e)];
@@ -929,6 +1164,14 @@ template <class T, class Eq> struct Solver {
                     Gc[j][i][v] = Gc[j][i][v] + (1.0 / 3.0) * gt;
                     Uc[j][i][v] = Uc[j][i][v] + (1.0 / 3.0) * ust[j][i][v];
                 }
+                T Sn[NV];
+                if (has_src) {
+                    double x, y;
+                    node_xy(i, j, e, x, y);
+                    T st[NV];
+                    src_t_N34(up, upp, um, umm, x, y, dt, src_t + 0.5 * dt, st);
+                    for (int v = 0; v < NV; ++v) Sn[v] = S2[idx(v, i, j, e)] + (1.0 / 3.0) * st[v];
+                }
                 for (int ii = 0; ii < n; ++ii)
                     for (int v = 0; v < NV; ++v)
                         du[idx(v, ii, j, e)] = du[idx(v, ii, j, e)] + (alpha * Dh[ii][i]) * Fc[j][i][v];
@@ -939,6 +1182,7 @@ template <class T, class Eq> struct Solver {
                     U[idx(v, i, j, e)] = Uc[j][i][v];
                     F[fidx(v, 0, i, j, e)] = Fc[j][i][v];
                     F[fidx(v, 1, i, j, e)] = Gc[j][i][v];
+                    if (has_src) du[idx(v, i, j, e)] = du[idx(v, i, j, e)] + (-1.0 / ij) * Sn[v];
                 }
             }
     }
@@ -1422,6 +1666,7 @@ template <class T, class Eq> struct Solver {
     }
 
     virtual void rhs_lw(T* du, const T* u, double t, double dt) {
+        src_t = t;
         reset_du(du);  // reset_du!
         // calc_volume_integral! for VolumeIntegralFRShockCapturing (dg_2d.jl:1648-1804)
         if (sc) sc_prepare(u, dt);
@@ -1459,6 +1704,7 @@ template <class T, class Eq> struct Solver {
     // calc_volume_integral_mdrk1!/2! with shock capturing (tree/multi_derivative_2d.jl:151-261):
     // the MDRK kernel scaled by 1 - α plus first-order FV of uⁿ scaled by α.
     virtual void rhs_mdrk1(T* du, const T* u, double t, double dt) {
+        src_t = t;
         ensure_mdrk();
         reset_du(du);
         if (sc) sc_prepare(u, dt);
@@ -1472,6 +1718,7 @@ template <class T, class Eq> struct Solver {
         surface_integral_and_jacobian(du);
     }
     virtual void rhs_mdrk2(T* du, const T* u, double t, double dt) {
+        src_t = t;
         ensure_mdrk();
         reset_du(du);
         if (sc) sc_prepare(u, dt);

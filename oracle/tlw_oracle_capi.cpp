@@ -55,6 +55,10 @@ struct tlwo_desc {
     double wall_T[4];
     // bc_kind = BV_CALLBACK: the boundary value function u(x, y, t)
     void (*bc_fn)(double, double, double, double*);
+    // source terms (SourceKind; Euler on the TreeMesh): SRC_CALLBACK calls
+    // src_fn(u, x, y, t, s)
+    int source_kind;
+    void (*src_fn)(const double*, double, double, double, double*);
 };
 
 struct tlwo_ctx {
@@ -80,6 +84,9 @@ template <class S> static void configure(S& s, const tlwo_desc* d) {
     s.positivity = d->positivity != 0;
     s.pos_threshold[0] = d->pos_threshold[0];
     s.pos_threshold[1] = d->pos_threshold[1];
+    s.src.kind = d->source_kind;
+    s.src.gamma = d->gamma;
+    s.src.fn = d->src_fn;
     s.setup();
 }
 
@@ -102,6 +109,11 @@ void* tlwo_create(const tlwo_desc* d) {
     // the reflecting wall is the Euler TreeMesh one (x faces vertical, y faces horizontal)
     if (d->bc_kind == BV_SLIP_WALL && (d->equation != 0 || d->curved)) return nullptr;
     if (d->bc_kind == BV_CALLBACK && !d->bc_fn) return nullptr;
+    // source terms: the Euler kernels of the TreeMesh (LW N = 3, 4 and MDRK)
+    if (d->source_kind != SRC_NONE &&
+        (d->equation != 0 || d->parabolic || d->curved || (d->source_kind == SRC_CALLBACK && !d->src_fn) ||
+         d->source_kind < 0 || d->source_kind > SRC_CALLBACK))
+        return nullptr;
     auto* c = new tlwo_ctx;
     c->equation = d->equation;
     if (d->curved && d->parabolic == 2) {
@@ -336,6 +348,7 @@ void tlwo_solve_fixed_dt(void* ctx, double* u, double t0, const double* dts, int
 int tlwo_tangent(const tlwo_desc* d, const double* u0, const double* du0, const double* dts, int nsteps, int mdrk,
                  const double* alpha, double* u_out, double* du_out) {
     if (d->polydeg != 3 && d->polydeg != 4) return 1;
+    if (d->source_kind == SRC_CALLBACK) return 2;  // a callback sees values only: no tangent
     if (d->nthreads > 0) omp_set_num_threads(d->nthreads);
     Solver<Dual, Euler2D<Dual>> s;
     s.eq.gamma = d->gamma;
