@@ -56,7 +56,18 @@ enum { TLW_FLUX_TREE = 0, TLW_FLUX_STRUCTURED = 1 };
  * y faces (equations.jl:95-139: mirrored uⁿ and Ũ, F_outer = (-F1, F2, -F3, -F4) resp.
  * (-F1, -F2, F3, -F4), the LW flux in the boundary order of the Dirichlet call); TreeMesh only */
 enum { TLW_BV_NONE = 0, TLW_BV_VORTEX = 1, TLW_BV_CONSTANT = 2, TLW_BV_RIEMANN4 = 3, TLW_BV_SLIP_WALL = 4,
-       TLW_BV_TABLE = 5 /* caller-defined boundary_value_function: values uploaded by tlw_set_boundary_data */ };
+       TLW_BV_TABLE = 5 /* caller-defined boundary_value_function: values uploaded by tlw_set_boundary_data */,
+       TLW_BV_CONVERGENCE_TEST = 6 /* Trixi's initial_condition_convergence_test (Euler): ρ = ρv1 = ρv2 = φ,
+                                      ρe = φ², φ = 2 + 0.1 sin(π(x + y - t)) */ };
+
+/* source terms of SemidiscretizationHyperbolic (source_terms = ...), folded into the Taylor procedure
+ * as the reference does (src/solvers/dgsem_tree/dg_2d.jl:89-176 and the calls in
+ * lw_volume_kernel_3!/4!, 674-1267; tree/multi_derivative_2d.jl:323-379, 487-522):
+ * TLW_SRC_CONVERGENCE_TEST is Trixi's source_terms_convergence_test for CompressibleEulerEquations2D
+ * (the residual of the state of TLW_BV_CONVERGENCE_TEST).  Euler LW and MDRK on a TreeMesh, plain
+ * VolumeIntegralFR; shock capturing, parabolic terms, curvilinear meshes and adaptive steps with
+ * source terms return TLW_ENOTSUP. */
+enum { TLW_SRC_NONE = 0, TLW_SRC_CONVERGENCE_TEST = 1 };
 
 /* tlw_get_aux selectors (element_cache fields, src/solvers/dgsem_tree/containers.jl:73-98,
  * and the shock-capturing blending factors, indicator.cache.alpha) */
@@ -106,6 +117,7 @@ typedef struct {
      * calcflux_mh! of src/solvers/dgsem_structured/dg_2d.jl:313-538 on the TreeMesh's Cartesian
      * metric; elements with a face on the domain boundary use first order, as there) */
     int32_t reconstruction;
+    int32_t source_terms;           /* TLW_SRC_* */
 } tlw_solver_desc;
 
 /* Parabolic terms of SemidiscretizationHyperbolicParabolic

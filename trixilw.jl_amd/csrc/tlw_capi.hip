@@ -531,6 +531,18 @@ KArgs base_args(tlw_ctx* c) {
 // Kernel launches, dispatched on the polynomial degree and the tile size.
 template <int N, int EB, bool SC, bool PAR = false> void launch_volume_eb(tlw_ctx* c, const KArgs& a, int kind) {
     dim3 g(c->vgrid), b(VGeo<N, EB>::NT);
+    if constexpr (!SC && !PAR) {  // the Euler kernels with source terms
+        if (kind == VOL_LW_SRC) {
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_LW_SRC, false, false>), g, b, 0, c->stream, a);
+            return;
+        } else if (kind == VOL_MDRK1_SRC) {
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_MDRK1_SRC, false, false>), g, b, 0, c->stream, a);
+            return;
+        } else if (kind == VOL_MDRK2_SRC) {
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_MDRK2_SRC, false, false>), g, b, 0, c->stream, a);
+            return;
+        }
+    }
     if (kind == VOL_LW)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_LW, SC, PAR>), g, b, 0, c->stream, a);
     else if (kind == VOL_MDRK1)
@@ -598,6 +610,8 @@ template <int N> void launch_volume_t(tlw_ctx* c, const KArgs& a, int kind) {
         // IndicatorHennemannGassner on uⁿ before the volume integral (dg_2d.jl:1665)
         hipLaunchKernelGGL(k_indicator<N>, dim3(c->sgrid), dim3(kSurfBlock), 0, c->stream, a, c->d_alpha_tmp);
         launch_volume_sc<N, true>(c, a, kind);
+    } else if (c->hconst.src_kind != TLW_SRC_NONE && kind <= VOL_MDRK2) {
+        launch_volume_sc<N, false>(c, a, kind + VOL_LW_SRC);  // LW, MDRK1, MDRK2 with source terms
     } else {
         launch_volume_sc<N, false>(c, a, kind);
     }
@@ -1073,6 +1087,8 @@ int tlw_solve_init_adaptive(tlw_ctx* c, double t0, double t_final, double dt_ini
                     "adaptive steps need the temporal-error estimate of an Euler LW kernel (TreeMesh polydeg 4, "
                     "curvilinear polydeg 3 or 4)");
     if (c->positivity) return fail(c, TLW_ENOTSUP, "adaptive steps are not combined with the positivity limiter");
+    if (c->hconst.src_kind != TLW_SRC_NONE)
+        return fail(c, TLW_ENOTSUP, "adaptive steps are not combined with source terms");
     if (int rcb = check_loop_boundary(c)) return rcb;
     HIPCHK(c, hipMemcpyAsync(c->d_cur, c->host_cur.data(), sizeof(int) * c->ninst, hipMemcpyHostToDevice,
                              c->stream));
@@ -1176,10 +1192,11 @@ int tlw_adjoint_lw(tlw_ctx* c, const double* u, const double* lam_next, double* 
     if (!u || !lam_next || !lam) return fail(c, TLW_EINVAL, "null state pointer");
     if (!std::isfinite(dt) || !std::isfinite(t)) return fail(c, TLW_EINVAL, "t and dt must be finite");
     if (c->time_disc != TLW_LW || c->parabolic || c->curved || c->has_boundary ||
-        c->hconst.flux_order != TLW_FLUX_TREE || c->ninst != 1 || (c->sc && c->hconst.recon != TLW_RECON_FIRST_ORDER))
+        c->hconst.flux_order != TLW_FLUX_TREE || c->ninst != 1 || (c->sc && c->hconst.recon != TLW_RECON_FIRST_ORDER) ||
+        c->hconst.src_kind != TLW_SRC_NONE)
         return fail(c, TLW_ENOTSUP,
                     "the adjoint covers LW (plain or blended with first-order subcell FV) on one periodic mesh with "
-                    "the TreeMesh flux order");
+                    "the TreeMesh flux order and no source terms");
     if (lam == lam_next || lam == u) return fail(c, TLW_EINVAL, "lam must not alias lam_next or u");
     if (c->sc) {
         // blending factors of the step's state (frozen), then the blended step's adjoint
@@ -1233,9 +1250,10 @@ int tlw_adjoint_mdrk(tlw_ctx* c, const double* u, const double* us, const double
     if (!u || !us || !lam_next || !lam) return fail(c, TLW_EINVAL, "null state pointer");
     if (!std::isfinite(dt) || !std::isfinite(t)) return fail(c, TLW_EINVAL, "t and dt must be finite");
     if (c->time_disc != TLW_MDRK || c->sc || c->parabolic || c->curved || c->has_boundary ||
-        c->hconst.flux_order != TLW_FLUX_TREE || c->ninst != 1)
+        c->hconst.flux_order != TLW_FLUX_TREE || c->ninst != 1 || c->hconst.src_kind != TLW_SRC_NONE)
         return fail(c, TLW_ENOTSUP,
-                    "the MDRK adjoint covers plain MDRK on one periodic mesh with the TreeMesh flux order");
+                    "the MDRK adjoint covers plain MDRK on one periodic mesh with the TreeMesh flux order and no "
+                    "source terms");
     if (lam == lam_next || lam == u || lam == us) return fail(c, TLW_EINVAL, "lam must not alias u, us or lam_next");
     if (!c->d_adj_tmp) HIPCHK(c, hipMalloc(&c->d_adj_tmp, sizeof(double) * state_len(c)));
     AdjArgs a2{c->E, u, us, lam_next, nullptr, lam, c->d_adj_tmp, c->d_trace, c->d_nbr, c->d_invj, c->d_const, dt};
@@ -1502,9 +1520,9 @@ int tlw_create_ensemble(tlw_ctx* a, tlw_ctx* b, tlw_ensemble** out) {
     }
     for (tlw_ctx* c : {c3, c4})
         if (c->time_disc != TLW_LW || c->sc || c->parabolic || c->curved || c->has_boundary || c->positivity ||
-            c->adaptive || c->d_dtpresc || c->d_dtlog) {
+            c->adaptive || c->d_dtpresc || c->d_dtlog || c->hconst.src_kind != TLW_SRC_NONE) {
             g_noctx_err = "tlw_create_ensemble: plain LW contexts on periodic TreeMeshes (no shock capturing, "
-                          "parabolic terms, limiter, adaptive steps or dt record)";
+                          "parabolic terms, source terms, limiter, adaptive steps or dt record)";
             return TLW_ENOTSUP;
         }
     auto* e = new tlw_ensemble;
@@ -1653,8 +1671,11 @@ int create_common(int ninst, const tlw_mesh_desc* meshes, const tlw_basis_desc* 
     if (solver->time_discretization != TLW_LW && solver->time_discretization != TLW_MDRK) return TLW_EINVAL;
     if (solver->surface_flux_order != TLW_FLUX_TREE && solver->surface_flux_order != TLW_FLUX_STRUCTURED)
         return TLW_EINVAL;
-    if (solver->boundary_kind < TLW_BV_NONE || solver->boundary_kind > TLW_BV_TABLE) return TLW_EINVAL;
+    if (solver->boundary_kind < TLW_BV_NONE || solver->boundary_kind > TLW_BV_CONVERGENCE_TEST) return TLW_EINVAL;
     if (!(solver->gamma > 1.0)) return TLW_EINVAL;
+    if (solver->source_terms < TLW_SRC_NONE || solver->source_terms > TLW_SRC_CONVERGENCE_TEST) return TLW_EINVAL;
+    // source terms: the Euler LW and MDRK kernels of the TreeMesh
+    if (solver->source_terms != TLW_SRC_NONE && (solver->shock_capturing || par || cv)) return TLW_ENOTSUP;
     tlw_ctx* c = new tlw_ctx;
     c->N = basis->polydeg;
     c->n = c->N + 1;
@@ -1677,6 +1698,7 @@ int create_common(int ninst, const tlw_mesh_desc* meshes, const tlw_basis_desc* 
     for (int k = 0; k < 24; ++k) h.bc_par[k] = solver->boundary_params[k];
     h.bc_kind = solver->boundary_kind;
     h.flux_order = solver->surface_flux_order;
+    h.src_kind = solver->source_terms;
     c->sc = solver->shock_capturing != 0;
     if (c->sc) {
         if (!solver->inverse_vandermonde_host || (solver->surface_flux_order != TLW_FLUX_TREE && !cv) ||

@@ -29,10 +29,13 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "tlw_sincos.hpp"
+
 namespace tlw {
 
 constexpr int kBcSlipWall = 4;  // boundary kind of the reflecting wall (TLW_BV_SLIP_WALL)
 constexpr int kBcTable = 5;     // caller-defined boundary data (TLW_BV_TABLE)
+constexpr int kBcConvergenceTest = 6;  // the convergence test's state (TLW_BV_CONVERGENCE_TEST)
 constexpr int kTraceQ = 10;  // face record: jump state (4), normal flux (4), |v_n|, c
 // shock-capturing face record: + fn_low (4), blending factor, padding
 constexpr int kTraceQSC = 16;
@@ -84,6 +87,7 @@ struct DevConst {
     // width the half-step divides by for subcell k
     int recon;  // TLW_RECON_*
     double mh_dx1[8], mh_dx2[8], mh_fa[8], mh_fb[8], mh_fc[8], mh_xl[8], mh_xr[8], mh_hw[8];
+    int src_kind;  // SrcKind
 };
 
 // Device-resident integrator state (LWIntegrator fields t, dt, iter and the
@@ -113,7 +117,14 @@ enum Reduce { RED_NONE = 0, RED_STEP = 1, RED_INIT = 2, RED_QUERY = 3, RED_ADAPT
 // (a separate instantiation, so the plain LW kernel carries none of its registers)
 // VOL_LW_WALL: the Navier-Stokes LW kernel that also keeps the viscous time
 // average of boundary face nodes (no-slip walls)
-enum VolKind { VOL_LW = 0, VOL_MDRK1 = 1, VOL_MDRK2 = 2, VOL_LW_ADAPT = 3, VOL_LW_WALL = 4 };
+// VOL_*_SRC: the Euler LW and MDRK kernels with source terms (separate
+// instantiations, VOL_LW_SRC + the plain kind)
+enum VolKind {
+    VOL_LW = 0, VOL_MDRK1 = 1, VOL_MDRK2 = 2, VOL_LW_ADAPT = 3, VOL_LW_WALL = 4,
+    VOL_LW_SRC = 5, VOL_MDRK1_SRC = 6, VOL_MDRK2_SRC = 7
+};
+// source-term kinds of DevConst::src_kind
+enum SrcKind { SRC_NONE = 0, SRC_CONVERGENCE_TEST = 1 };
 
 struct KArgs {
     int E;
@@ -206,6 +217,79 @@ __device__ __forceinline__ void euler_fluxes(const double* u, double gm1, double
     g[3] = (E + p) * v2;
 }
 
+// ---------------------------------------------------------------------------
+// Source terms of the LW procedure (dg_2d.jl:89-176).  The device's source is
+// Trixi's source_terms_convergence_test for CompressibleEulerEquations2D, the
+// residual ∂t u + ∇·f(u) of the manufactured state ρ = ρv1 = ρv2 = φ, ρe = φ²,
+// φ = 2 + 0.1 sin(π(x + y - t)): with φ' = 0.1π cos(π(x + y - t)) and
+// τ = (2φ - 1)(γ - 1), s = (φ', φ'(1 + τ), φ'(1 + τ), 2φ'(φ + τ)).  It depends
+// on (x, t) only, so every time difference of s the reference forms at its
+// stencil states is a combination of s(x, t + k dt), k = -2..2, and the order
+// in which the reference passes those states does not change it.
+struct SrcPt {
+    double x, y, t, gamma;
+};
+__device__ __forceinline__ void src_eval(const SrcPt& p, double toff, double* s) {
+    const double w = 3.141592653589793;
+    double si, co;
+    sincos_cw(w * (p.x + p.y - (p.t + toff)), &si, &co);
+    const double rho = 2.0 + 0.1 * si;
+    const double rho_x = w * 0.1 * co;
+    const double tmp = (2.0 * rho - 1.0) * (p.gamma - 1.0);
+    s[0] = rho_x;
+    s[1] = rho_x * (1.0 + tmp);
+    s[2] = s[1];
+    s[3] = 2.0 * rho_x * (rho + tmp);
+}
+// dt-scaled time differences about p.t + t0 (calc_source_t_N34,
+// calc_source_tt_N23, calc_source_ttt_N34, calc_source_tttt_N4)
+__device__ __forceinline__ void src_t5(const SrcPt& p, double t0, double dt, double* o) {
+    double a[4], b[4];
+    src_eval(p, t0 + 2.0 * dt, a);
+    src_eval(p, t0 + dt, b);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) o[v] = 8.0 * b[v] - a[v];
+    src_eval(p, t0 - dt, a);
+    src_eval(p, t0 - 2.0 * dt, b);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) o[v] = (1.0 / 12.0) * (o[v] - 8.0 * a[v] + b[v]);
+}
+__device__ __forceinline__ void src_tt3(const SrcPt& p, double dt, double* o) {
+    double a[4], b[4];
+    src_eval(p, dt, a);
+    src_eval(p, 0.0, b);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) o[v] = a[v] - 2.0 * b[v];
+    src_eval(p, -dt, a);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) o[v] += a[v];
+}
+__device__ __forceinline__ void src_ttt5(const SrcPt& p, double dt, double* o) {
+    double a[4], b[4];
+    src_eval(p, 2.0 * dt, a);
+    src_eval(p, dt, b);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) o[v] = a[v] - 2.0 * b[v];
+    src_eval(p, -dt, a);
+    src_eval(p, -2.0 * dt, b);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) o[v] = 0.5 * (o[v] + 2.0 * a[v] - b[v]);
+}
+__device__ __forceinline__ void src_tttt5(const SrcPt& p, double dt, double* o) {
+    double a[4], b[4];
+    src_eval(p, 2.0 * dt, a);
+    src_eval(p, dt, b);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) o[v] = a[v] - 4.0 * b[v];
+    src_eval(p, 0.0, a);
+    src_eval(p, -dt, b);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) o[v] = o[v] + 6.0 * a[v] - 4.0 * b[v];
+    src_eval(p, -2.0 * dt, a);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) o[v] += a[v];
+}
+
 __device__ __forceinline__ void euler_flux_dir(const double* u, double gm1, int orientation, double* f) {
     double ff[4], gg[4];
     euler_fluxes(u, gm1, ff, gg);
@@ -280,6 +364,15 @@ __device__ __forceinline__ int julia_isapprox(double x, double y) {
 __device__ inline void boundary_value(const DevConst* c, double x, double y, double t, double* u) {
     const double* par = c->bc_par;
     double rho, v1, v2, p, gamma;
+    if (c->bc_kind == kBcConvergenceTest) {
+        // Trixi's initial_condition_convergence_test: ρ = ρv1 = ρv2 = φ, ρe = φ²
+        double si, co;
+        sincos_cw(3.141592653589793 * (x + y - t), &si, &co);
+        const double phi = 2.0 + 0.1 * si;
+        u[0] = u[1] = u[2] = phi;
+        u[3] = phi * phi;
+        return;
+    }
     if (c->bc_kind == 1) {  // isentropic vortex
         const double eps = par[0], mach = par[1], Lx = par[4], Ly = par[5];
         gamma = par[6];
@@ -684,23 +777,33 @@ __device__ __forceinline__ void node_speeds(const double* un, double gamma, doub
 // same with F̃ before the f_tttt term, compared variable by variable as
 // ((u_np1 - u_np1_low) / (abstol + reltol max(|u_np1|, |u_np1_low|)))², summed.
 // terr must be uniformly null or non-null over the workgroup (it adds barriers).
-template <int N, int EB, bool KEEP = false>
+// With SRC, the source terms of the node (*sp) enter every time derivative as
+// dt times the source's own difference (dg_2d.jl:980-987, 1033-1045, 1093-1103,
+// 1148-1160) and Ssrc receives S = s + s_t/2 + s_tt/6 + s_ttt/24 + s_tttt/120
+// (1228-1247), which the caller adds to the residual.
+template <int N, int EB, bool KEEP = false, bool SRC = false>
 __device__ __forceinline__ void lw4_element(const double* uN, double dt, double invJ, double gm1, const DevConst* c,
                                             int i, int j, bool act, int sidx, int sbase, double* s0, double* s1,
                                             double* s2, const double* sDm, const double* sDh, double* Uc, double* F,
                                             double* G, double* vol, double* tder = nullptr, double* terr = nullptr,
-                                            double abstol = 0.0, double reltol = 0.0) {
+                                            double abstol = 0.0, double reltol = 0.0, const SrcPt* sp = nullptr,
+                                            double* Ssrc = nullptr) {
     using G_ = VGeo<N, EB>;
     constexpr int n = G_::n, S = G_::S;
     euler_fluxes(uN, gm1, F, G);
     if (act) put8<S>(s0, sidx, F, G);
     __syncthreads();
-    double ut[4];
+    double ut[4], sk[4];
     contract<n, S>(s0, sbase, i, j, sDm, ut);
+    if constexpr (SRC) src_eval(*sp, 0.0, sk);
     double up[4], um[4], upp[4], umm[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         ut[v] *= invJ;
+        if constexpr (SRC) {
+            ut[v] = fma(dt, sk[v], ut[v]);
+            Ssrc[v] = sk[v];
+        }
         if constexpr (KEEP) tder[v] = ut[v];
         Uc[v] = fma(0.5, ut[v], uN[v]);
         up[v] = uN[v] + ut[v];
@@ -731,9 +834,14 @@ __device__ __forceinline__ void lw4_element(const double* uN, double dt, double 
     __syncthreads();
     double uk[4];
     contract<n, S>(s1, sbase, i, j, sDm, uk);
+    if constexpr (SRC) src_t5(*sp, 0.0, dt, sk);
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-        const double w = uk[v] * invJ;  // utt
+        double w = uk[v] * invJ;  // utt
+        if constexpr (SRC) {
+            w = fma(dt, sk[v], w);
+            Ssrc[v] = fma(0.5, sk[v], Ssrc[v]);
+        }
         if constexpr (KEEP) tder[4 + v] = w;
         Uc[v] = fma(1.0 / 6.0, w, Uc[v]);
         up[v] = fma(0.5, w, up[v]);
@@ -767,9 +875,14 @@ __device__ __forceinline__ void lw4_element(const double* uN, double dt, double 
     if (act) put8<S>(s2, sidx, dx, dy);
     __syncthreads();
     contract<n, S>(s2, sbase, i, j, sDm, uk);
+    if constexpr (SRC) src_tt3(*sp, dt, sk);
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-        const double w = uk[v] * invJ;  // uttt
+        double w = uk[v] * invJ;  // uttt
+        if constexpr (SRC) {
+            w = fma(dt, sk[v], w);
+            Ssrc[v] = fma(1.0 / 6.0, sk[v], Ssrc[v]);
+        }
         if constexpr (KEEP) tder[8 + v] = w;
         Uc[v] = fma(1.0 / 24.0, w, Uc[v]);
         um[v] = fma(-1.0 / 6.0, w, um[v]);
@@ -798,9 +911,14 @@ __device__ __forceinline__ void lw4_element(const double* uN, double dt, double 
     if (act) put8<S>(s1, sidx, dx, dy);
     __syncthreads();
     contract<n, S>(s1, sbase, i, j, sDm, uk);
+    if constexpr (SRC) src_ttt5(*sp, dt, sk);
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-        const double w = uk[v] * invJ;  // utttt
+        double w = uk[v] * invJ;  // utttt
+        if constexpr (SRC) {
+            w = fma(dt, sk[v], w);
+            Ssrc[v] = fma(1.0 / 24.0, sk[v], Ssrc[v]);
+        }
         if constexpr (KEEP) tder[12 + v] = w;
         Uc[v] = fma(1.0 / 120.0, w, Uc[v]);
         um[v] = fma(1.0 / 24.0, w, um[v]);
@@ -837,6 +955,11 @@ __device__ __forceinline__ void lw4_element(const double* uN, double dt, double 
     if (terr && act) put8<S>(s0, sidx, dx, dy);
     __syncthreads();
     contract<n, S>(s2, sbase, i, j, sDh, vol);
+    if constexpr (SRC) {
+        src_tttt5(*sp, dt, sk);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Ssrc[v] = fma(1.0 / 120.0, sk[v], Ssrc[v]);
+    }
     if (terr) {
         double a1[4], a2[4];
         contract<n, S>(s2, sbase, i, j, sDm, a1);  // -dt D F̃
@@ -859,23 +982,31 @@ __device__ __forceinline__ void lw4_element(const double* uN, double dt, double 
 
 // lw_volume_kernel_3!, dg_2d.jl:674-916
 // With KEEP, tder receives the time derivatives ut, utt, ... (4 per order),
-// the intermediates an adjoint sweep needs.
-template <int N, int EB, bool KEEP = false>
+// the intermediates an adjoint sweep needs.  With SRC, source terms as in
+// lw4_element (732-740, 785-797, 841-851, 891-911): S = s + s_t/2 + s_tt/6 +
+// s_ttt/24.
+template <int N, int EB, bool KEEP = false, bool SRC = false>
 __device__ __forceinline__ void lw3_element(const double* uN, double dt, double invJ, double gm1, const DevConst* c,
                                             int i, int j, bool act, int sidx, int sbase, double* s0, double* s1,
                                             double* s2, const double* sDm, const double* sDh, double* Uc, double* F,
-                                            double* G, double* vol, double* tder = nullptr) {
+                                            double* G, double* vol, double* tder = nullptr,
+                                            const SrcPt* sp = nullptr, double* Ssrc = nullptr) {
     using G_ = VGeo<N, EB>;
     constexpr int n = G_::n, S = G_::S;
     euler_fluxes(uN, gm1, F, G);
     if (act) put8<S>(s0, sidx, F, G);
     __syncthreads();
-    double ut[4];
+    double ut[4], sk[4];
     contract<n, S>(s0, sbase, i, j, sDm, ut);
+    if constexpr (SRC) src_eval(*sp, 0.0, sk);
     double up[4], um[4], upp[4], umm[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         ut[v] *= invJ;
+        if constexpr (SRC) {
+            ut[v] = fma(dt, sk[v], ut[v]);
+            Ssrc[v] = sk[v];
+        }
         if constexpr (KEEP) tder[v] = ut[v];
         Uc[v] = fma(0.5, ut[v], uN[v]);
         up[v] = uN[v] + ut[v];
@@ -905,9 +1036,14 @@ __device__ __forceinline__ void lw3_element(const double* uN, double dt, double 
     __syncthreads();
     double uk[4];
     contract<n, S>(s1, sbase, i, j, sDm, uk);
+    if constexpr (SRC) src_t5(*sp, 0.0, dt, sk);
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-        const double w = uk[v] * invJ;  // utt
+        double w = uk[v] * invJ;  // utt
+        if constexpr (SRC) {
+            w = fma(dt, sk[v], w);
+            Ssrc[v] = fma(0.5, sk[v], Ssrc[v]);
+        }
         if constexpr (KEEP) tder[4 + v] = w;
         Uc[v] = fma(1.0 / 6.0, w, Uc[v]);
         up[v] = fma(0.5, w, up[v]);
@@ -933,9 +1069,14 @@ __device__ __forceinline__ void lw3_element(const double* uN, double dt, double 
     if (act) put8<S>(s2, sidx, dx, dy);
     __syncthreads();
     contract<n, S>(s2, sbase, i, j, sDm, uk);
+    if constexpr (SRC) src_tt3(*sp, dt, sk);
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-        const double w = uk[v] * invJ;  // uttt
+        double w = uk[v] * invJ;  // uttt
+        if constexpr (SRC) {
+            w = fma(dt, sk[v], w);
+            Ssrc[v] = fma(1.0 / 6.0, sk[v], Ssrc[v]);
+        }
         if constexpr (KEEP) tder[8 + v] = w;
         Uc[v] = fma(1.0 / 24.0, w, Uc[v]);
         um[v] = fma(-1.0 / 6.0, w, um[v]);
@@ -965,6 +1106,11 @@ __device__ __forceinline__ void lw3_element(const double* uN, double dt, double 
     if (act) put8<S>(s2, sidx, F, G);
     __syncthreads();
     contract<n, S>(s2, sbase, i, j, sDh, vol);
+    if constexpr (SRC) {
+        src_ttt5(*sp, dt, sk);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Ssrc[v] = fma(1.0 / 24.0, sk[v], Ssrc[v]);
+    }
 }
 
 // f_t of the two-level Taylor states u ± ut, u ± 2ut:
@@ -996,12 +1142,14 @@ __device__ __forceinline__ void mdrk_ft(const double* u, const double* ut, doubl
 
 // mdrk_kernel_1!, tree/multi_derivative_2d.jl:263-420: F̃ = f/2 + f_t/8,
 // Ũ = u/2 + ut/8.  F2 = f + f_t/6 and U2 = u + ut/6 are recomputed by the
-// stage-2 kernel instead of being stored.
-template <int N, int EB>
+// stage-2 kernel instead of being stored.  With SRC (323-379): ut gets
+// dt s(t), and Ssrc = s/2 + s_t/8.
+template <int N, int EB, bool SRC = false>
 __device__ __forceinline__ void mdrk1_element(const double* uN, double dt, double invJ, double gm1, const DevConst* c,
                                               int i, int j, bool act, int sidx, int sbase, double* s0, double* s1,
                                               double* s2, const double* sDm, const double* sDh, double* Uc,
-                                              double* F, double* G, double* vol) {
+                                              double* F, double* G, double* vol, const SrcPt* sp = nullptr,
+                                              double* Ssrc = nullptr) {
     using G_ = VGeo<N, EB>;
     constexpr int n = G_::n, S = G_::S;
     double f[4], g[4];
@@ -1010,8 +1158,21 @@ __device__ __forceinline__ void mdrk1_element(const double* uN, double dt, doubl
     __syncthreads();
     double ut[4];
     contract<n, S>(s0, sbase, i, j, sDm, ut);
+    double sk[4];
+    if constexpr (SRC) src_eval(*sp, 0.0, sk);
 #pragma unroll
-    for (int v = 0; v < 4; ++v) ut[v] *= invJ;
+    for (int v = 0; v < 4; ++v) {
+        ut[v] *= invJ;
+        if constexpr (SRC) {
+            ut[v] = fma(dt, sk[v], ut[v]);
+            Ssrc[v] = 0.5 * sk[v];
+        }
+    }
+    if constexpr (SRC) {
+        src_t5(*sp, 0.0, dt, sk);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Ssrc[v] = fma(0.125, sk[v], Ssrc[v]);
+    }
     double dx[4], dy[4];
     mdrk_ft(uN, ut, gm1, dx, dy);
 #pragma unroll
@@ -1026,12 +1187,15 @@ __device__ __forceinline__ void mdrk1_element(const double* uN, double dt, doubl
 }
 
 // mdrk_kernel_2!, tree/multi_derivative_2d.jl:423-555: F̃ = F2 + f_t(us)/3,
-// Ũ = U2 + ust/3 with F2, U2 recomputed from uⁿ.
-template <int N, int EB>
+// Ũ = U2 + ust/3 with F2, U2 recomputed from uⁿ.  With SRC (487-522): the
+// recomputed stage-1 ut gets dt s(t) and S2 = s + s_t/6; ust gets
+// dt s(t + dt/2), and Ssrc = S2 + s_t(t + dt/2)/3.
+template <int N, int EB, bool SRC = false>
 __device__ __forceinline__ void mdrk2_element(const double* uN, const double* usn, double dt, double invJ, double gm1,
                                               const DevConst* c, int i, int j, bool act, int sidx, int sbase,
                                               double* s0, double* s1, double* s2, const double* sDm,
-                                              const double* sDh, double* Uc, double* F, double* G, double* vol) {
+                                              const double* sDh, double* Uc, double* F, double* G, double* vol,
+                                              const SrcPt* sp = nullptr, double* Ssrc = nullptr) {
     using G_ = VGeo<N, EB>;
     constexpr int n = G_::n, S = G_::S;
     double f[4], g[4], fs[4], gs[4];
@@ -1047,6 +1211,21 @@ __device__ __forceinline__ void mdrk2_element(const double* uN, const double* us
     contract<n, S>(s1, sbase, i, j, sDm, ust);
 #pragma unroll
     for (int v = 0; v < 4; ++v) { ut[v] *= invJ; ust[v] *= invJ; }
+    if constexpr (SRC) {
+        double sk[4], sh[4];
+        src_eval(*sp, 0.0, sk);
+        src_eval(*sp, 0.5 * dt, sh);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            ut[v] = fma(dt, sk[v], ut[v]);
+            ust[v] = fma(dt, sh[v], ust[v]);
+            Ssrc[v] = sk[v];
+        }
+        src_t5(*sp, 0.0, dt, sk);
+        src_t5(*sp, 0.5 * dt, dt, sh);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Ssrc[v] = fma(1.0 / 3.0, sh[v], fma(1.0 / 6.0, sk[v], Ssrc[v]));
+    }
     double dx[4], dy[4];
     mdrk_ft(uN, ut, gm1, dx, dy);
 #pragma unroll
@@ -2004,7 +2183,8 @@ __device__ __forceinline__ void curved_speeds(const double* u, const double* J, 
 // reduction here costs barriers and LDS passes worth 15% of this kernel).
 // Waves per SIMD the register budget is sized for: 3 (168 VGPRs) for the
 // plain scheme (TLW_VOL_WAVES at build time); 2 (256 VGPRs) with shock
-// capturing, whose subcell FV and blended face fluxes would otherwise spill.
+// capturing, whose subcell FV and blended face fluxes would otherwise spill,
+// and with source terms (the source's time differences at every stage).
 #ifndef TLW_VOL_WAVES
 #define TLW_VOL_WAVES 3
 #endif
@@ -2048,6 +2228,20 @@ __device__ __forceinline__ void volume_body(const KArgs& a, int bid, double* lds
     // inverse Jacobian: per element on a TreeMesh, per node on a curvilinear mesh
     const double invJ = CURV ? a.ijn[(size_t)e * NN + node] : a.inv_jac[e];
     const size_t off = ((size_t)e * NN + node) * 4;
+    // source terms: separate instantiations (KIND = VOL_LW_SRC + the plain kind);
+    // the node's coordinates from the element's centre and edge length, the
+    // time of the call; Ssrc = S, added to the residual at the end
+    constexpr bool SRC = KIND >= VOL_LW_SRC;
+    constexpr int KB = SRC ? KIND - VOL_LW_SRC : KIND;
+    static_assert(!SRC || (!SC && !PAR && !CURV), "source terms: the Euler TreeMesh kernels");
+    double Ssrc[4] = {0.0, 0.0, 0.0, 0.0};
+    SrcPt sp{0.0, 0.0, 0.0, gamma};
+    if constexpr (SRC) {
+        const double hh = 0.5 * a.length[e];
+        sp.x = a.center[2 * e] + hh * c->nodes[i];
+        sp.y = a.center[2 * e + 1] + hh * c->nodes[j];
+        sp.t = a.use_state_dt ? st->t : a.t_arg;
+    }
 
     if (tid < NN) {  // published by the first barrier of the element computation
         sDm[tid] = -dt * c->D[tid / n][tid % n];
@@ -2099,21 +2293,24 @@ __device__ __forceinline__ void volume_body(const KArgs& a, int bid, double* lds
                 mdrkp1_element<N, EB, CURV>(uN, gxN, gyN, invJ, gm1, c, i, j, act, sidx, sbase, s0, s1, s2, sDm, sD,
                                             sDh, Uc, F, G, vol, Jn);
         }
-    } else if (KIND == VOL_LW || KIND == VOL_LW_ADAPT) {
-        if constexpr (N == 4 && KIND == VOL_LW_ADAPT)
+    } else if (KB == VOL_LW || KB == VOL_LW_ADAPT) {
+        if constexpr (N == 4 && KB == VOL_LW_ADAPT)
             lw4_element<N, EB>(uN, dt, invJ, gm1, c, i, j, act, sidx, sbase, s0, s1, s2, sDm, sDh, Uc, F, G, vol,
                                nullptr, &terr, st->abstol, st->reltol);
         else if constexpr (N == 4)
-            lw4_element<N, EB>(uN, dt, invJ, gm1, c, i, j, act, sidx, sbase, s0, s1, s2, sDm, sDh, Uc, F, G, vol);
+            lw4_element<N, EB, false, SRC>(uN, dt, invJ, gm1, c, i, j, act, sidx, sbase, s0, s1, s2, sDm, sDh, Uc, F,
+                                           G, vol, nullptr, nullptr, 0.0, 0.0, &sp, Ssrc);
         else
-            lw3_element<N, EB>(uN, dt, invJ, gm1, c, i, j, act, sidx, sbase, s0, s1, s2, sDm, sDh, Uc, F, G, vol);
-    } else if (KIND == VOL_MDRK1) {
-        mdrk1_element<N, EB>(uN, dt, invJ, gm1, c, i, j, act, sidx, sbase, s0, s1, s2, sDm, sDh, Uc, F, G, vol);
+            lw3_element<N, EB, false, SRC>(uN, dt, invJ, gm1, c, i, j, act, sidx, sbase, s0, s1, s2, sDm, sDh, Uc, F,
+                                           G, vol, nullptr, &sp, Ssrc);
+    } else if (KB == VOL_MDRK1) {
+        mdrk1_element<N, EB, SRC>(uN, dt, invJ, gm1, c, i, j, act, sidx, sbase, s0, s1, s2, sDm, sDh, Uc, F, G, vol,
+                                  &sp, Ssrc);
     } else {
         double usn[4];
         load4(a.us + off, usn);
-        mdrk2_element<N, EB>(uN, usn, dt, invJ, gm1, c, i, j, act, sidx, sbase, s0, s1, s2, sDm, sDh, Uc, F, G,
-                             vol);
+        mdrk2_element<N, EB, SRC>(uN, usn, dt, invJ, gm1, c, i, j, act, sidx, sbase, s0, s1, s2, sDm, sDh, Uc, F, G,
+                                  vol, &sp, Ssrc);
     }
     if (act && a.auxU) {
         store4(a.auxU + off, Uc);
@@ -2341,7 +2538,8 @@ __device__ __forceinline__ void volume_body(const KArgs& a, int bid, double* lds
     double val[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-        const double d = (-invJ) * (vol[v] + surf[v]);
+        double d = (-invJ) * (vol[v] + surf[v]);
+        if constexpr (SRC) d += Ssrc[v];
         val[v] = a.step_mode ? fma(dt, d, uN[v]) : d;
     }
     if (act) store4(out + off, val);
@@ -2378,7 +2576,7 @@ __device__ __forceinline__ void volume_body(const KArgs& a, int bid, double* lds
 }
 
 template <int N, int EB, int KIND, bool SC, bool PAR = false, bool CURV = false>
-__global__ __launch_bounds__((VGeo<N, EB>::NT), kVolWaves<SC || PAR>) void k_volume(KArgs a) {
+__global__ __launch_bounds__((VGeo<N, EB>::NT), kVolWaves<SC || PAR || (KIND >= VOL_LW_SRC)>) void k_volume(KArgs a) {
     using G_ = VGeo<N, EB>;
     __shared__ double lds[G_::LDS];
     __shared__ double sDm[G_::NN], sDh[G_::NN], sD[G_::NN];

@@ -30,6 +30,9 @@ const TLW_BV_CONSTANT = Int32(2)
 const TLW_BV_RIEMANN4 = Int32(3)
 const TLW_BV_SLIP_WALL = Int32(4)
 const TLW_BV_TABLE = Int32(5)      # BoundaryConditionDirichlet(f) for any f: see set_boundary_data!
+const TLW_BV_CONVERGENCE_TEST = Int32(6)  # BoundaryConditionDirichlet(initial_condition_convergence_test)
+const TLW_SRC_NONE = Int32(0)
+const TLW_SRC_CONVERGENCE_TEST = Int32(1)  # source_terms = source_terms_convergence_test
 const TLW_AUX_U = Int32(0)
 const TLW_AUX_F = Int32(1)
 const TLW_AUX_ALPHA = Int32(2)
@@ -75,6 +78,7 @@ struct SolverDesc
     alpha_smooth::Int32
     inverse_vandermonde_host::Ptr{Float64}  # basis.inverse_vandermonde_legendre, column-major n×n
     reconstruction::Int32                 # TLW_RECON_FIRST_ORDER or TLW_RECON_MUSCL_HANCOCK
+    source_terms::Int32                   # TLW_SRC_*
 end
 
 # CompressibleNavierStokesDiffusion2D(equations; mu, Prandtl, gradient_variables) and its walls
@@ -99,10 +103,12 @@ end
 function SolverDesc(; time_discretization = TLW_LW, gamma = 1.4, surface_flux_order = TLW_FLUX_TREE,
                     boundary_kind = TLW_BV_NONE, boundary_params = ntuple(_ -> 0.0, 24), flags = Int32(0),
                     shock_capturing = Int32(0), alpha_max = 0.5, alpha_min = 0.001, alpha_smooth = Int32(1),
-                    inverse_vandermonde = Ptr{Float64}(C_NULL), reconstruction = TLW_RECON_FIRST_ORDER)
+                    inverse_vandermonde = Ptr{Float64}(C_NULL), reconstruction = TLW_RECON_FIRST_ORDER,
+                    source_terms = TLW_SRC_NONE)
     return SolverDesc(Int32(time_discretization), Float64(gamma), Int32(surface_flux_order), Int32(boundary_kind),
                       NTuple{24,Float64}(boundary_params), Int32(flags), Int32(shock_capturing), Float64(alpha_max),
-                      Float64(alpha_min), Int32(alpha_smooth), inverse_vandermonde, Int32(reconstruction))
+                      Float64(alpha_min), Int32(alpha_smooth), inverse_vandermonde, Int32(reconstruction),
+                      Int32(source_terms))
 end
 
 function ParabolicDesc(eqp; walls = nothing, wall_temperature = nothing)

@@ -11,7 +11,9 @@ from .batch import SemidiscretizationBatch, SemidiscretizationEnsemble, draw_vor
 from .equations import (Adiabatic, BoundaryConditionDirichlet, BoundaryConditionNavierStokesWall, CompressibleEulerEquations2D,
                         CompressibleNavierStokesDiffusion2D, ConstantState, Isothermal, NoSlip,
                         FourQuadrantRiemann, GradientVariablesConservative, GradientVariablesPrimitive,
-                        IsentropicVortex, TaylorGreenVortex, boundary_condition_slip_wall_horizontal,
+                        ConvergenceTest, IsentropicVortex, SourceTermsConvergenceTest, TaylorGreenVortex,
+                        boundary_condition_slip_wall_horizontal, initial_condition_convergence_test,
+                        source_terms_convergence_test,
                         boundary_condition_slip_wall_vertical)
 from .integrator import (LWIntegrator, LWSolution, compute_dt, isapprox, isfinished, perform_step, set_dt,
                          set_t_and_iter, solve_lwfr)
@@ -37,5 +39,6 @@ __all__ = [
     "VolumeIntegralFRShockCapturing", "density_pressure", "PositivityPreservingLimiterZhangShu", "density",
     "pressure", "adjoint_step", "adjoint_step_mdrk", "gradient_lw", "kinetic_energy", "gaussian_random_field", "objective_lw",
     "SemidiscretizationBatch", "SemidiscretizationEnsemble", "draw_vortex_ensemble", "SemidiscretizationHyperbolicParabolic", "CompressibleNavierStokesDiffusion2D",
-    "GradientVariablesConservative", "GradientVariablesPrimitive",
+    "GradientVariablesConservative", "GradientVariablesPrimitive", "ConvergenceTest", "SourceTermsConvergenceTest",
+    "initial_condition_convergence_test", "source_terms_convergence_test",
 ]

@@ -16,6 +16,8 @@ TLW_OK, TLW_EINVAL, TLW_EHIP, TLW_EADMISSIBLE, TLW_ENOTSUP = 0, 1, 2, 3, 4
 TLW_LW, TLW_MDRK = 0, 1
 TLW_FLUX_TREE, TLW_FLUX_STRUCTURED = 0, 1
 TLW_BV_NONE, TLW_BV_VORTEX, TLW_BV_CONSTANT, TLW_BV_RIEMANN4, TLW_BV_SLIP_WALL, TLW_BV_TABLE = 0, 1, 2, 3, 4, 5
+TLW_BV_CONVERGENCE_TEST = 6
+TLW_SRC_NONE, TLW_SRC_CONVERGENCE_TEST = 0, 1
 TLW_AUX_U, TLW_AUX_F, TLW_AUX_ALPHA = 0, 1, 2
 TLW_FLAG_KEEP_AUX = 1
 TLW_PAR_NAVIER_STOKES = 1
@@ -73,6 +75,7 @@ class SolverDesc(C.Structure):
         ("alpha_smooth", C.c_int32),
         ("inverse_vandermonde_host", C.POINTER(C.c_double)),
         ("reconstruction", C.c_int32),
+        ("source_terms", C.c_int32),
     ]
 
 

@@ -15,7 +15,8 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-BV_NONE, BV_VORTEX, BV_CONSTANT, BV_RIEMANN4, BV_SLIP_WALL, BV_TABLE = 0, 1, 2, 3, 4, 5
+BV_NONE, BV_VORTEX, BV_CONSTANT, BV_RIEMANN4, BV_SLIP_WALL, BV_TABLE, BV_CONVERGENCE_TEST = 0, 1, 2, 3, 4, 5, 6
+SRC_NONE, SRC_CONVERGENCE_TEST = 0, 1
 
 
 class BoundaryConditionDirichlet:
@@ -144,6 +145,18 @@ class BoundaryConditionNavierStokesWall:
                 heat.resolve(points) if isinstance(heat, Isothermal) else None)
 
 
+def _builtin_dirichlet(b):
+    """BoundaryConditionDirichlet of a boundary value function the device has
+    built in (the reference's elixirs wrap their initial condition, e.g.
+    BoundaryConditionDirichlet(initial_condition_convergence_test)) is that
+    function itself, which the device loop can evaluate; any other stays a
+    host-tabulated condition."""
+    if isinstance(b, BoundaryConditionDirichlet) and getattr(b.boundary_value_function, "kind", None) in (
+            BV_VORTEX, BV_CONSTANT, BV_RIEMANN4, BV_CONVERGENCE_TEST):
+        return b.boundary_value_function
+    return b
+
+
 def boundary_conditions_kind(bcs):
     """One boundary-value description for every boundary face from the
     reference's forms: a single function object, or the NamedTuple (x_neg,
@@ -151,8 +164,9 @@ def boundary_conditions_kind(bcs):
     must agree (the same function, or the vertical slip wall on the x faces
     and the horizontal one on the y faces)."""
     if bcs is None or not isinstance(bcs, (dict, tuple, list)):
-        return bcs
-    seq = [bcs[k] for k in ("x_neg", "x_pos", "y_neg", "y_pos")] if isinstance(bcs, dict) else list(bcs)
+        return _builtin_dirichlet(bcs)
+    seq = [_builtin_dirichlet(bcs[k]) for k in ("x_neg", "x_pos", "y_neg", "y_pos")] if isinstance(bcs, dict) \
+        else [_builtin_dirichlet(b) for b in bcs]
     if len(seq) != 4:
         raise ValueError("per-direction boundary conditions: (x_neg, x_pos, y_neg, y_pos)")
     if all(isinstance(b, _SlipWall) for b in seq):
@@ -298,6 +312,42 @@ class ConstantState:
         one = np.ones_like(np.asarray(x, dtype=np.float64))
         return CompressibleEulerEquations2D(self.gamma).prim2cons(self.rho * one, self.v1 * one, self.v2 * one,
                                                                   self.p * one)
+
+
+class ConvergenceTest:
+    """Trixi's initial_condition_convergence_test for CompressibleEulerEquations2D
+    (the manufactured solution of the reference's elixir_euler_source_terms*.jl):
+    ρ = ρv1 = ρv2 = φ and ρe = φ², φ = 2 + 0.1 sin(π(x + y - t)); the exact
+    solution with source_terms_convergence_test.  Also a boundary value
+    function (BoundaryConditionDirichlet of it)."""
+
+    kind = BV_CONVERGENCE_TEST
+    params = ()
+
+    def __call__(self, x, y, t, equations=None):
+        phi = 2.0 + 0.1 * np.sin(np.pi * (np.asarray(x, dtype=np.float64) + np.asarray(y, dtype=np.float64) - t))
+        return np.stack([phi, phi, phi, phi * phi], axis=-1)
+
+
+class SourceTermsConvergenceTest:
+    """Trixi's source_terms_convergence_test for CompressibleEulerEquations2D:
+    the residual ∂t u + ∇·f(u) of ConvergenceTest's state, which depends on
+    (x, t) only.  The device evaluates it inside the LW and MDRK procedures;
+    calling it here gives the same values on the host."""
+
+    kind = SRC_CONVERGENCE_TEST
+
+    def __call__(self, u, x, y, t, equations=None):
+        gamma = equations.gamma if equations is not None else 1.4
+        arg = np.pi * (np.asarray(x, dtype=np.float64) + np.asarray(y, dtype=np.float64) - t)
+        rho = 2.0 + 0.1 * np.sin(arg)
+        rho_x = np.pi * 0.1 * np.cos(arg)
+        tmp = (2.0 * rho - 1.0) * (gamma - 1.0)
+        return np.stack([rho_x, rho_x * (1.0 + tmp), rho_x * (1.0 + tmp), 2.0 * rho_x * (rho + tmp)], axis=-1)
+
+
+initial_condition_convergence_test = ConvergenceTest()
+source_terms_convergence_test = SourceTermsConvergenceTest()
 
 
 @dataclass

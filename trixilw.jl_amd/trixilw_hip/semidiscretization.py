@@ -161,8 +161,14 @@ class SemidiscretizationHyperbolic:
 
     def __init__(self, mesh, time_discretization, equations, initial_condition, solver, boundary_conditions=None,
                  source_terms=None, surface_flux_order="tree", keep_aux=False, device=None):
+        src_kind = _lib.TLW_SRC_NONE
         if source_terms is not None:
-            raise NotImplementedError("source terms are not on the device path")
+            from .equations import SRC_CONVERGENCE_TEST
+
+            if getattr(source_terms, "kind", None) != SRC_CONVERGENCE_TEST:
+                raise NotImplementedError("the device path implements source_terms_convergence_test")
+            src_kind = _lib.TLW_SRC_CONVERGENCE_TEST
+        self.source_terms = source_terms
         if not isinstance(equations, CompressibleEulerEquations2D):
             raise NotImplementedError("the device path implements CompressibleEulerEquations2D")
         torch = _torch()
@@ -197,6 +203,7 @@ class SemidiscretizationHyperbolic:
             for k, v in enumerate(bcd.params):
                 sd.boundary_params[k] = float(v)
         sd.flags = _lib.TLW_FLAG_KEEP_AUX if keep_aux else 0
+        sd.source_terms = src_kind
         vi = solver.volume_integral
         if isinstance(vi, VolumeIntegralFRShockCapturing):
             ind = vi.indicator
