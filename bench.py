@@ -461,6 +461,107 @@ def measure_shock_capturing(steps, cpu_steps, threads, with_cpu=True):
     return out
 
 
+def measure_config0(threads, with_cpu=True):
+    """Sub-record: the reference's CPU-runnable case itself, the isentropic vortex
+    on the periodic 32² TreeMesh at N=4, CFL 0.98·LW limit, solved to t = 1 by the
+    device loop (launched, then replayed from a captured HIP graph), and by the
+    CPU oracle on the host's cores."""
+    import torch
+    from trixilw_hip import _lib
+
+    name = "lw_euler_vortex_32x32_N4"
+    semi, mesh, ic, cfl, (lo, hi), okw = make_problem(WORKLOADS[name])
+    x, y = semi.node_coordinates()
+    u0 = np.ascontiguousarray(ic(x, y, 0.0))
+    dofs = semi.ndofs()
+    L, h, dp = _lib.lib(), semi.handle, C.POINTER(C.c_double)
+    t, dt, it, fin = C.c_double(), C.c_double(), C.c_int64(), C.c_int32()
+    runs = {}
+    with torch.cuda.stream(torch.cuda.Stream()):
+        semi._bind_stream()
+        for mode, passes in (("launched", (False, True)), ("graph_replay", (False, True))):
+            _lib.check(L.tlw_enable_graphs(h, int(mode == "graph_replay")), h)
+            for timed in passes:  # the untimed pass warms up (and captures the graph)
+                _lib.check(L.tlw_set_state(h, u0.ctypes.data_as(dp), u0.size), h)
+                _lib.check(L.tlw_solve_init(h, 0.0, 1.0, cfl), h)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                while True:
+                    _lib.check(L.tlw_advance(h, 200, C.byref(t), C.byref(dt), C.byref(it), C.byref(fin)), h)
+                    if fin.value:
+                        break
+                torch.cuda.synchronize()
+                el = time.perf_counter() - t0
+            runs[mode] = (el, int(it.value), t.value)
+        _lib.check(L.tlw_enable_graphs(h, 0), h)
+    el, nsteps, tend = runs["launched"]
+    u_dev = np.empty_like(u0)
+    _lib.check(L.tlw_get_state(h, u_dev.ctypes.data_as(dp), u_dev.size), h)
+    out = {"workload": name, "value": dofs * nsteps / el, "unit": "DOF-updates/s", "steps": nsteps, "t_final": tend,
+           "dofs": dofs, "ms_to_t1": round(1e3 * el, 3), "ms_per_step": round(1e3 * el / nsteps, 4),
+           "graph_replay": {"value": dofs * nsteps / runs["graph_replay"][0],
+                            "ms_to_t1": round(1e3 * runs["graph_replay"][0], 3),
+                            "ms_per_step": round(1e3 * runs["graph_replay"][0] / nsteps, 4)},
+           "config": f"isentropic vortex strength 5, Mach 0.5, periodic [-10,10]^2 TreeMesh 32x32, polydeg 4, LW, "
+                     f"CFL {cfl}, t = 0 .. 1"}
+    log(f"bench: config 0 (32^2 to t = 1): {nsteps} steps, {1e3 * el:.2f} ms launched, "
+        f"{1e3 * runs['graph_replay'][0]:.2f} ms graph-replayed")
+    semi.close()
+    if with_cpu:
+        import oracle_ctypes
+
+        oracle_ctypes.set_threads(threads)
+        o = oracle_ctypes.Oracle(4, mesh.nx, mesh.nx, lo, lo, (hi - lo) / mesh.nx, **okw)
+        xo, yo = o.node_coords()
+        t0 = time.perf_counter()
+        u_cpu, t_cpu, dts = o.solve(np.ascontiguousarray(ic(xo, yo, 0.0)), 0.0, 1.0, cfl)
+        el_cpu = time.perf_counter() - t0
+        u_lex = u_dev[mesh.lexicographic_permutation()]
+        out["cpu_baseline"] = {"value": dofs * len(dts) / el_cpu, "unit": "DOF-updates/s", "cores": threads,
+                               "kind": "port", "sample": f"the whole run to t = 1 ({len(dts)} steps), C++/OpenMP oracle",
+                               "seconds": round(el_cpu, 4), "steps": len(dts),
+                               "max_rel_diff_of_final_state": float(np.max(np.abs(u_lex - u_cpu)) /
+                                                                    np.max(np.abs(u_cpu)))}
+        out["gpu_over_cpu"] = round(out["value"] / out["cpu_baseline"]["value"], 2)
+        log(f"bench: config 0 CPU {el_cpu:.3f} s for {len(dts)} steps")
+        del o
+    return out
+
+
+def measure_mdrk_euler(steps, cpu_steps, threads, with_cpu=True):
+    """Sub-record: MDRK N=3 Euler on the periodic 1024² vortex mesh (two
+    volume/surface pairs per step): device-loop throughput, kernel times, the
+    step's bytes against HBM and the CPU oracle."""
+    name = "mdrk_euler_vortex_1024x1024_N3"
+    wl = WORKLOADS[name]
+    semi, mesh, ic, cfl, (lo, hi), okw = make_problem(wl)
+    x, y = semi.node_coordinates()
+    u0 = np.ascontiguousarray(ic(x, y, 0.0))
+    r = run_steps(semi, u0, cfl, steps, 2)
+    dofs = semi.ndofs()
+    step_b, _ = alg_bytes_per_dof(3, "mdrk")
+    out = {"workload": name, "value": r["value"], "unit": "DOF-updates/s", "ms_per_step": round(r["ms_per_step"], 4),
+           "steps": steps, "dofs": dofs, "volume_avg_launch_ms": round(r["vol_ms"], 4), "volume_launches": r["vol_n"],
+           "surface_avg_launch_ms": round(r["surf_ms"], 4), "step_alg_bytes_per_dof": step_b,
+           "step_frac": round(step_b * r["value"] / 1e9 / HBM_PEAK_GBS, 4),
+           "elements_per_workgroup": r["tile"], "tile_autotune_ms": r["tune"],
+           "config": f"isentropic vortex strength 5, Mach 0.5, periodic [-10,10]^2 TreeMesh {mesh.nx}x{mesh.nx}, "
+                     f"polydeg 3, MDRK, CFL {cfl}"}
+    log(f"bench: MDRK N=3 Euler {r['value'] / 1e9:.3f} G DOF-updates/s, {r['ms_per_step']:.3f} ms/step")
+    semi.close()
+    if with_cpu:
+        import oracle_ctypes
+
+        o = oracle_ctypes.Oracle(3, mesh.nx, mesh.nx, lo, lo, (hi - lo) / mesh.nx, **okw)
+        xo, yo = o.node_coords()
+        v, el = cpu_baseline(o, np.ascontiguousarray(ic(xo, yo, 0.0)), cfl, True, dofs, cpu_steps, threads)
+        out["cpu_baseline"] = {"value": v, "unit": "DOF-updates/s", "cores": threads, "kind": "port",
+                               "sample": f"{cpu_steps} step(s) of the same mesh, C++/OpenMP oracle"}
+        out["gpu_over_cpu"] = round(r["value"] / v, 2)
+        del o
+    return out
+
+
 def measure_curved_navier_stokes(steps, cpu_steps, threads, with_cpu=True):
     """Sub-record: MDRK N=3 compressible Navier-Stokes on the warped 128²
     StructuredMesh (warp amplitude 0.1, Reynolds number 1000, Pr 0.72) for
@@ -658,6 +759,8 @@ def main():
     ap.add_argument("--no-adaptive", action="store_true", help="skip the adaptive-stepping sub-record")
     ap.add_argument("--no-curved", action="store_true", help="skip the curvilinear-mesh sub-records")
     ap.add_argument("--no-sc", action="store_true", help="skip the shock-capturing (MUSCL-Hancock) sub-record")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the config-0 (32^2 to t = 1) and MDRK N=3 Euler sub-records")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device copy (the reference kernel for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -798,14 +901,19 @@ def main():
 
     sc_rec = None
     if args.workload == DEFAULT_WORKLOAD and not args.no_sc:
-        sc_rec = measure_shock_capturing(max(10, min(5 * args.steps, 100)), 5, threads,
-                                         with_cpu=not args.no_cpu_baseline)
+        # the configuration's own 500 steps (0.3 ms each at 256²)
+        sc_rec = measure_shock_capturing(500, 5, threads, with_cpu=not args.no_cpu_baseline)
 
     curved_rec = None
     curved_ns = None
     if args.workload == DEFAULT_WORKLOAD and not args.no_curved:
         curved_rec = measure_curved(max(3, min(args.steps, 10)), 1, threads, with_cpu=not args.no_cpu_baseline)
         curved_ns = measure_curved_navier_stokes(300, 20, threads, with_cpu=not args.no_cpu_baseline)
+
+    config0 = mdrk_euler = None
+    if args.workload == DEFAULT_WORKLOAD and not args.no_extra:
+        config0 = measure_config0(threads, with_cpu=not args.no_cpu_baseline)
+        mdrk_euler = measure_mdrk_euler(max(3, min(args.steps, 10)), 1, threads, with_cpu=not args.no_cpu_baseline)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -866,6 +974,8 @@ def main():
         "curved": curved_rec,
         "curved_navier_stokes": curved_ns,
         "shock_capturing": sc_rec,
+        "config0": config0,
+        "mdrk_euler": mdrk_euler,
     }
     print(json.dumps(out), flush=True)
     semi.close()
