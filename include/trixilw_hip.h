@@ -347,7 +347,9 @@ int tlw_prescribe_dts(tlw_ctx* ctx, const double* dts_host, int64_t n);
  * polydeg 4 (the reference's TreeMesh estimate is in that kernel), or on a curvilinear StructuredMesh
  * at polydeg 3 or 4 (lw_volume_kernel_3!/4! of src/solvers/dgsem_structured/dg_2d.jl: u_np1 with the
  * contravariant F̃ and the node's J⁻¹, u_np1_low without the last stage's term, 1595-1681 and
- * 1983-2077); not combined with the positivity limiter (TLW_ENOTSUP).  A rejected step whose factor is not finite (NaN state)
+ * 1983-2077), or compressible Navier-Stokes LW at polydeg 4 on a periodic TreeMesh (the total flux's
+ * estimate of lw_volume_kernel_4! in src/solvers/dgsem_tree/dg_2d_parabolic.jl:1749-1775, 1841-1855);
+ * not combined with the positivity limiter or source terms (TLW_ENOTSUP).  A rejected step whose factor is not finite (NaN state)
  * shrinks dt by 0.95. */
 int tlw_solve_init_adaptive(tlw_ctx* ctx, double t0, double t_final, double dt_initial, double abstol,
                             double reltol);

@@ -316,8 +316,11 @@ template <class T, class Eq, class Visc> struct ParabolicSolver : Solver<T, Eq> 
     }
 
     // lw_volume_kernel_3!/4! (dg_2d_parabolic.jl:725-1189, 1191-1860).  Source
-    // terms are `nothing` and the u_np1 / u_np1_low temporal-error updates of
-    // kernel 4 feed only the adaptive time stepping; both are omitted.
+    // terms are `nothing`.  Kernel 4 on the TreeMesh also gives the element's
+    // temporal error (1749-1775, 1841-1855): u_np1 = u - dt J⁻¹ D F̃ with the
+    // total F̃ = F̃a - F̃v, u_np1_low the same before the f_tttt terms, compared
+    // as in the Euler kernel; u_np1 - u_np1_low is formed as the update of the
+    // f_tttt term itself.
     void lw_volume_parabolic(T* du, const T* u, double dt, int e) {
         const int N = basis.N;
         static thread_local States S;
@@ -505,10 +508,41 @@ template <class T, class Eq, class Visc> struct ParabolicSolver : Solver<T, Eq> 
                         Fva[j][i][v] = Fva[j][i][v] + (1.0 / 120.0) * v4;
                         Ga[j][i][v] = Ga[j][i][v] + (1.0 / 120.0) * b4;
                         Gva[j][i][v] = Gva[j][i][v] + (1.0 / 120.0) * w4;
+                        df[j][i][v] = (1.0 / 120.0) * (a4 - v4);  // the f_tttt terms of F̃
+                        dg[j][i][v] = (1.0 / 120.0) * (b4 - w4);
                     }
                 }
+            if (!cJa) temporal_error(u, dt, e, Fa, Ga, Fva, Gva, df, dg);
         }
         finish_element(du, e, Fa, Ga, Fva, Gva, Uc, this->F, Fv);
+    }
+
+    // the element's temporal error of kernel 4 (dg_2d_parabolic.jl:1841-1855)
+    void temporal_error(const T* u, double dt, int e, const T (*Fa)[MAXN][NV], const T (*Ga)[MAXN][NV],
+                        const T (*Fva)[MAXN][NV], const T (*Gva)[MAXN][NV], const T (*d4x)[MAXN][NV],
+                        const T (*d4y)[MAXN][NV]) {
+        T Fx[MAXN][MAXN][NV], Fy[MAXN][MAXN][NV], hi[MAXN][MAXN][NV], d[MAXN][MAXN][NV];
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i)
+                for (int v = 0; v < NV; ++v) {
+                    Fx[j][i][v] = Fa[j][i][v] - Fva[j][i][v];
+                    Fy[j][i][v] = Ga[j][i][v] - Gva[j][i][v];
+                    hi[j][i][v] = d[j][i][v] = T(0);
+                }
+        add_divergence(Fx, Fy, -dt, hi, e);
+        add_divergence(d4x, d4y, -dt, d, e);
+        T err = T(0);
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i)
+                for (int v = 0; v < NV; ++v) {
+                    const double ij = node_ij(i, j, e);
+                    const T h = u[idx(v, i, j, e)] + ij * hi[j][i][v];
+                    const T dd = ij * d[j][i][v];
+                    const T lo = h - dd;
+                    const T r = dd / (this->abstol + this->reltol * max_(abs_(h), abs_(lo)));
+                    err = err + r * r;
+                }
+        this->temporal_errors[e] = err;
     }
 
     // du += D̂ (F̃a - F̃v), and the element caches U, F (advective) and Fv

@@ -128,3 +128,27 @@ def test_adaptive_curved_matches_oracle(oracle_mod, N, dt_initial):
     assert abs(sol.t - t_ref) <= 1e-13 * t_ref
     assert_close(to_lex(mesh, sol.u), u_ref, what=f"curved adaptive N={N} dt0={dt_initial}")
     semi.close()
+
+
+@pytest.mark.parametrize("dt_initial,tf", [(1e-5, 0.05), (0.5, 0.05)])
+def test_adaptive_navier_stokes_matches_oracle(oracle_mod, dt_initial, tf):
+    """Adaptive steps of the Navier-Stokes LW N=4 kernel on the periodic
+    TreeMesh (its temporal error, dg_2d_parabolic.jl:1841-1855): accepted and
+    rejected counts, final time and state equal the oracle's solve_adaptive
+    (≤1e-12), from a small dt_initial and from an unstable one."""
+    import trixilw_hip as tlw
+    from test_gpu_parabolic_parity import setup_ns
+
+    torch_mod()
+    semi, o0, u_lex, mesh = setup_ns(oracle_mod, 4, 3, mu=0.01)
+    o = oracle_mod.Oracle(4, mesh.nx, mesh.nx, -1.0, -1.0, 2.0 / mesh.nx, gamma=1.4, parabolic="navier_stokes",
+                          mu=0.01, prandtl=0.72, tolerances=(TOL["abstol"], TOL["reltol"]))
+    u_ref, t_ref, dts, nrej = o.solve_adaptive(u_lex, 0.0, tf, dt_initial, max_attempts=20000)
+    sol = run_device(semi, mesh, u_lex, tf, dt_initial)
+    assert sol.stats.naccept == len(dts) > 3
+    assert sol.stats.nreject == nrej
+    if dt_initial > 0.1:
+        assert nrej >= 1
+    assert abs(sol.t - t_ref) <= 1e-13 * t_ref
+    assert_close(to_lex(mesh, sol.u), u_ref, what=f"Navier-Stokes adaptive dt0={dt_initial}")
+    semi.close()

@@ -552,7 +552,7 @@ template <int N, int EB, bool SC, bool PAR = false> void launch_volume_eb(tlw_ct
     else if (kind == VOL_LW_WALL) {
         if constexpr (PAR && !SC)
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_LW_WALL, SC, PAR>), g, b, 0, c->stream, a);
-    } else if constexpr (N == 4 && !PAR)
+    } else if constexpr (N == 4)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_LW_ADAPT, SC, PAR>), g, b, 0, c->stream, a);
 }
 template <int N, bool SC, bool PAR = false> void launch_volume_sc(tlw_ctx* c, const KArgs& a, int kind) {
@@ -1082,10 +1082,10 @@ int tlw_solve_init_adaptive(tlw_ctx* c, double t0, double t_final, double dt_ini
     if (!(abstol >= 0.0) || !(reltol >= 0.0) || !(abstol > 0.0 || reltol > 0.0) || !std::isfinite(abstol) ||
         !std::isfinite(reltol))
         return fail(c, TLW_EINVAL, "tolerances: abstol, reltol >= 0, not both zero");
-    if (c->time_disc != TLW_LW || c->parabolic || (c->N != 4 && !c->curved))
+    if (c->time_disc != TLW_LW || (c->N != 4 && !c->curved) || (c->parabolic && (c->curved || c->hconst.ns_walls)))
         return fail(c, TLW_ENOTSUP,
-                    "adaptive steps need the temporal-error estimate of an Euler LW kernel (TreeMesh polydeg 4, "
-                    "curvilinear polydeg 3 or 4)");
+                    "adaptive steps need the temporal-error estimate of an LW kernel (Euler on a TreeMesh at polydeg "
+                    "4 or on a curvilinear mesh at polydeg 3 or 4; Navier-Stokes on a periodic TreeMesh at polydeg 4)");
     if (c->positivity) return fail(c, TLW_ENOTSUP, "adaptive steps are not combined with the positivity limiter");
     if (c->hconst.src_kind != TLW_SRC_NONE)
         return fail(c, TLW_ENOTSUP, "adaptive steps are not combined with source terms");

@@ -1526,7 +1526,8 @@ __device__ __forceinline__ void lwp_element(const double* uN, const double* gxN,
                                             double gm1, const DevConst* c, int i, int j, bool act, int sidx,
                                             int sbase, double* s0, double* s1, double* s2, const double* sDm,
                                             const double* sD, const double* sDh, double* Uc, double* F, double* G,
-                                            double* vol, const double* Jn = nullptr, double* vw = nullptr) {
+                                            double* vol, const double* Jn = nullptr, double* vw = nullptr,
+                                            double* terr = nullptr, double abstol = 0.0, double reltol = 0.0) {
     using G_ = VGeo<N, EB>;
     constexpr int n = G_::n, S = G_::S;
     ns_total_fluxes(uN, gxN, gyN, gm1, c, F, G);
@@ -1703,6 +1704,9 @@ __device__ __forceinline__ void lwp_element(const double* uN, const double* gxN,
             F[v] = fma(1.0 / 120.0, dx[v], F[v]);
             G[v] = fma(1.0 / 120.0, dy[v], G[v]);
         }
+        // s0 is read only at the own node since the first contraction: it takes
+        // the total f_tttt for the temporal error
+        if (terr && act) put8<S>(s0, sidx, dx, dy);
     }
     if constexpr (WALL && N == 4) {  // f_tttt of the viscous part (with the 0.5), weight 1/120
         double a0[4], a1[4], a2[4], a3[4];
@@ -1724,6 +1728,24 @@ __device__ __forceinline__ void lwp_element(const double* uN, const double* gxN,
     }
     __syncthreads();
     contract<n, S>(s2, sbase, i, j, sDh, vol);
+    if (terr) {
+        // temporal error of kernel 4 (dg_2d_parabolic.jl:1749-1775, 1841-1855): as in
+        // lw4_element, with the total F̃ and f_tttt
+        double a1[4], a2[4];
+        contract<n, S>(s2, sbase, i, j, sDm, a1);  // -dt D F̃
+        contract<n, S>(s0, sbase, i, j, sDm, a2);  // -dt D f_tttt
+        double e = 0.0;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const double d = invJ * ((1.0 / 120.0) * a2[v]);
+            const double hi = fma(invJ, a1[v], uN[v]);
+            const double lo = hi - d;
+            const double r = d / (abstol + reltol * dmax(fabs(hi), fabs(lo)));
+            e = fma(r, r, e);
+        }
+        *terr = e;
+        __syncthreads();  // s0 is reused for the face records
+    }
 }
 
 // f_t of the states w ± wt, w ± 2wt (and their gradients) for the MDRK stages
@@ -2199,7 +2221,8 @@ template <int N, int EB, int KIND, bool SC, bool PAR, bool CURV>
 __device__ __forceinline__ void volume_body(const KArgs& a, int bid, double* lds, double* sDm, double* sDh, double* sD,
                                             double* s_alpha) {
     static_assert(!(SC && PAR), "shock capturing has no hyperbolic-parabolic form");
-    static_assert(!CURV || !PAR || KIND != VOL_LW_ADAPT, "adaptive steps: Euler LW");
+    static_assert(!CURV || !PAR || KIND != VOL_LW_ADAPT, "adaptive steps: Euler LW, or Navier-Stokes LW on a TreeMesh");
+    static_assert(!PAR || KIND != VOL_LW_ADAPT || N == 4, "the parabolic temporal error is kernel 4's");
     static_assert(KIND != VOL_LW_WALL || (PAR && !CURV && !SC), "walls: the TreeMesh Navier-Stokes LW kernel");
     static_assert(!(CURV && PAR && SC), "no shock capturing with parabolic terms");
     using G_ = VGeo<N, EB>;
@@ -2289,6 +2312,9 @@ __device__ __forceinline__ void volume_body(const KArgs& a, int bid, double* lds
             if constexpr (KIND == VOL_LW || KIND == VOL_LW_WALL)
                 lwp_element<N, EB, CURV, KIND == VOL_LW_WALL>(uN, gxN, gyN, invJ, gm1, c, i, j, act, sidx, sbase, s0,
                                                              s1, s2, sDm, sD, sDh, Uc, F, G, vol, Jn, vwall);
+            else if constexpr (KIND == VOL_LW_ADAPT)
+                lwp_element<N, EB, CURV, false>(uN, gxN, gyN, invJ, gm1, c, i, j, act, sidx, sbase, s0, s1, s2, sDm,
+                                                sD, sDh, Uc, F, G, vol, Jn, vwall, &terr, st->abstol, st->reltol);
             else
                 mdrkp1_element<N, EB, CURV>(uN, gxN, gyN, invJ, gm1, c, i, j, act, sidx, sbase, s0, s1, s2, sDm, sD,
                                             sDh, Uc, F, G, vol, Jn);
