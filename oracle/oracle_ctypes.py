@@ -17,6 +17,7 @@ HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "liboracle_tlw.so"
 
 BV_NONE, BV_VORTEX, BV_CONSTANT, BV_RIEMANN4, BV_SLIP_WALL, BV_CALLBACK, BV_CONVERGENCE_TEST = 0, 1, 2, 3, 4, 5, 6
+BV_OUTFLOW, BV_SLIP_WALL_APPROX = 7, 8
 SRC_NONE, SRC_CONVERGENCE_TEST, SRC_CALLBACK = 0, 1, 2
 BC_FN = C.CFUNCTYPE(None, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double))
 SRC_FN = C.CFUNCTYPE(None, C.POINTER(C.c_double), C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double))

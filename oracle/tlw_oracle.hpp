@@ -325,9 +325,13 @@ template <class T> struct LinAdv2D {
 // CompressibleEulerEquations2D as a boundary value function (the reference's
 // elixir_euler_source_terms_nonperiodic.jl): ρ = ρv1 = ρv2 = φ, ρe = φ² with
 // φ = 2 + 0.1 sin(π(x + y - t)).
+// BV_OUTFLOW, BV_SLIP_WALL_APPROX: boundary_condition_outflow and
+// slip_wall_approximate (equations.jl:185-197, 251-260), conditions on uⁿ with
+// a normal direction (curvilinear mesh only; the solver applies them, eval
+// gives no value).
 enum BoundaryValueKind {
     BV_NONE = 0, BV_VORTEX = 1, BV_CONSTANT = 2, BV_RIEMANN4 = 3, BV_SLIP_WALL = 4, BV_CALLBACK = 5,
-    BV_CONVERGENCE_TEST = 6
+    BV_CONVERGENCE_TEST = 6, BV_OUTFLOW = 7, BV_SLIP_WALL_APPROX = 8
 };
 
 // The manufactured solution of Trixi's convergence test for the Euler

@@ -108,6 +108,10 @@ void* tlwo_create(const tlwo_desc* d) {
     if (d->reconstruction == 1 && (d->equation != 0 || !d->shock_capturing)) return nullptr;
     // the reflecting wall is the Euler TreeMesh one (x faces vertical, y faces horizontal)
     if (d->bc_kind == BV_SLIP_WALL && (d->equation != 0 || d->curved)) return nullptr;
+    // the outflow and approximate slip-wall conditions take a normal direction: curvilinear Euler
+    if ((d->bc_kind == BV_OUTFLOW || d->bc_kind == BV_SLIP_WALL_APPROX) &&
+        (d->equation != 0 || !d->curved || d->parabolic))
+        return nullptr;
     if (d->bc_kind == BV_CALLBACK && !d->bc_fn) return nullptr;
     // source terms: the Euler kernels of the TreeMesh (LW N = 3, 4 and MDRK)
     if (d->source_kind != SRC_NONE &&

@@ -455,6 +455,37 @@ template <class T, class Eq, class Base = Solver<T, Eq>> struct CurvedSolver : B
                     const double nrm[2] = {sgn * (o == 1 ? J[0] : J[2]), sgn * (o == 1 ? J[1] : J[3])};
                     const T* u_in = &u[idx(0, ii, jj, e)];
                     const T* U_in = &this->U[idx(0, ii, jj, e)];
+                    if (this->bval.kind == BV_OUTFLOW || this->bval.kind == BV_SLIP_WALL_APPROX) {
+                        // conditions on uⁿ with the face's normal (their P4est call,
+                        // p4est/dg_2d.jl:361-393, with the outward normal): outflow is the
+                        // normal flux of uⁿ; the approximate slip wall the Rusanov flux between
+                        // uⁿ and its reflection (get_reflection, equations.jl:175-183).  In the
+                        // +ξ/+η direction of the element's contravariant vector both give the
+                        // flux below: the outer state on the side of the boundary.
+                        T fb[NV];
+                        if (this->bval.kind == BV_OUTFLOW) {
+                            eq.flux_normal(u_in, nrm, fb);
+                        } else {
+                            T rho, v1, v2, p, ur[NV];
+                            eq.cons2prim(u_in, rho, v1, v2, p);
+                            const double ns = nrm[0] * nrm[0] + nrm[1] * nrm[1];
+                            const T vn = (v1 * nrm[0] + v2 * nrm[1]) / ns;
+                            const T w1 = v1 - 2.0 * vn * nrm[0], w2 = v2 - 2.0 * vn * nrm[1];
+                            ur[0] = rho;
+                            ur[1] = rho * w1;
+                            ur[2] = rho * w2;
+                            ur[3] = p / (eq.gamma - 1.0) + 0.5 * rho * (w1 * w1 + w2 * w2);
+                            const T* ul = (dir % 2 == 0) ? u_in : ur;
+                            const T* uq = (dir % 2 == 0) ? ur : u_in;
+                            T fl[NV], fr[NV];
+                            eq.flux_normal(ul, nrm, fl);
+                            eq.flux_normal(uq, nrm, fr);
+                            const T lam = eq.max_abs_speed_naive_normal(ul, uq, nrm);
+                            for (int v = 0; v < NV; ++v) fb[v] = 0.5 * (fl[v] + fr[v]) + (-0.5 * lam) * (uq[v] - ul[v]);
+                        }
+                        for (int v = 0; v < NV; ++v) this->sfv[sidx(v, l, dir - 1, e)] = sgn * fb[v];
+                        continue;
+                    }
                     T f_in[NV], Uo[NV], Fo[NV], uo[NV];
                     for (int v = 0; v < NV; ++v) {
                         f_in[v] = this->F[fidx(v, 0, ii, jj, e)] * nrm[0] + this->F[fidx(v, 1, ii, jj, e)] * nrm[1];
