@@ -1,7 +1,16 @@
+import os
 import sys
 from pathlib import Path
 
-import pytest
+# Under pytest-xdist each worker gets its share of the cores for the oracle's
+# OpenMP loops: 8 workers with 8 spinning threads each oversubscribe the host
+# many times over.  Set before any OpenMP runtime is loaded.
+_workers = int(os.environ.get("PYTEST_XDIST_WORKER_COUNT", "1") or 1)
+if _workers > 1:
+    os.environ.setdefault("OMP_NUM_THREADS", str(max(1, (os.cpu_count() or 1) // _workers)))
+    os.environ.setdefault("OMP_WAIT_POLICY", "passive")
+
+import pytest  # noqa: E402
 
 ROOT = Path(__file__).resolve().parents[1]
 for p in (ROOT / "trixilw.jl_amd", ROOT / "oracle", ROOT):
