@@ -58,7 +58,15 @@ enum { TLW_FLUX_TREE = 0, TLW_FLUX_STRUCTURED = 1 };
 enum { TLW_BV_NONE = 0, TLW_BV_VORTEX = 1, TLW_BV_CONSTANT = 2, TLW_BV_RIEMANN4 = 3, TLW_BV_SLIP_WALL = 4,
        TLW_BV_TABLE = 5 /* caller-defined boundary_value_function: values uploaded by tlw_set_boundary_data */,
        TLW_BV_CONVERGENCE_TEST = 6 /* Trixi's initial_condition_convergence_test (Euler): ρ = ρv1 = ρv2 = φ,
-                                      ρe = φ², φ = 2 + 0.1 sin(π(x + y - t)) */ };
+                                      ρe = φ², φ = 2 + 0.1 sin(π(x + y - t)) */,
+       /* conditions on uⁿ with the boundary node's contravariant vector as normal (curvilinear
+        * StructuredMesh, Euler only; TLW_ENOTSUP elsewhere), in the form of their P4est call
+        * (src/solvers/dgsem_p4est/dg_2d.jl:361-393).  The reference's StructuredMesh boundary call
+        * passes an extra direction argument that these two methods do not take. */
+       TLW_BV_OUTFLOW = 7 /* boundary_condition_outflow (equations.jl:185-197): the normal flux of uⁿ */,
+       TLW_BV_SLIP_WALL_APPROXIMATE = 8 /* slip_wall_approximate (equations.jl:251-260): the Rusanov flux
+                                           between uⁿ and its reflection about the normal, the reflected
+                                           state on the side of the boundary */ };
 
 /* source terms of SemidiscretizationHyperbolic (source_terms = ...), folded into the Taylor procedure
  * as the reference does (src/solvers/dgsem_tree/dg_2d.jl:89-176 and the calls in

@@ -84,13 +84,15 @@ def warp_mapping(L, A):
 
 
 def setup_curved(oracle_mod, N, nx, ny, kind="warp", L=20.0, A=0.05, td="lw", seed=0, amp=1e-3, keep_aux=False,
-                 periodic=True, sc=False, alpha_max=0.5, alpha_smooth=True, ordering="lexicographic"):
+                 periodic=True, sc=False, alpha_max=0.5, alpha_smooth=True, ordering="lexicographic", boundary=None):
     """Isentropic vortex on a warped or affine StructuredMesh of [0, L]², the
     device semidiscretization and the oracle on the same mesh with the oracle's
     metric terms on both sides.  A non-periodic mesh takes the vortex itself (no
     periodic images) as time-averaged Dirichlet data.  ``sc`` selects
     VolumeIntegralFRShockCapturing (Hennemann-Gassner on density_pressure,
-    first-order subcell FV)."""
+    first-order subcell FV).  ``boundary`` replaces the Dirichlet data of a
+    non-periodic mesh by a condition object with a kind code the oracle shares
+    (boundary_condition_outflow, slip_wall_approximate)."""
     import trixilw_hip as tlw
 
     eq = tlw.CompressibleEulerEquations2D(GAMMA)
@@ -108,13 +110,14 @@ def setup_curved(oracle_mod, N, nx, ny, kind="warp", L=20.0, A=0.05, td="lw", se
         ind = tlw.IndicatorHennemannGassner(eq, None, alpha_max=alpha_max, alpha_min=0.001, alpha_smooth=alpha_smooth)
         vi = tlw.VolumeIntegralFRShockCapturing(ind, volume_integralFR=vi)
     solver = tlw.DGSEM(N, volume_integral=vi)
-    okw = {} if periodic else dict(periodic=(False, False), bc_kind=ic.kind, bc_par=ic.params)
+    bc = ic if boundary is None else boundary
+    okw = {} if periodic else dict(periodic=(False, False), bc_kind=bc.kind, bc_par=bc.params)
     if sc:
         okw.update(shock_capturing=True, alpha_max=alpha_max, alpha_min=0.001, alpha_smooth=alpha_smooth)
     o = oracle_mod.Oracle(N, nx, ny, 0.0, 0.0, L / nx, gamma=GAMMA, mapping=mp, **okw)
     mesh.set_metrics(solver.basis, o.aux("contravariant_vectors"), o.aux("inverse_jacobian"))
     semi = tlw.SemidiscretizationHyperbolic(mesh, tlw.get_time_discretization(solver), eq, ic, solver,
-                                            boundary_conditions=None if periodic else ic, keep_aux=keep_aux)
+                                            boundary_conditions=None if periodic else bc, keep_aux=keep_aux)
     x, y = o.node_coords()
     rng = np.random.default_rng(seed)
     u_lex = ic(x, y, 0.0) * (1.0 + amp * rng.standard_normal(x.shape + (4,)))

@@ -1671,7 +1671,11 @@ int create_common(int ninst, const tlw_mesh_desc* meshes, const tlw_basis_desc* 
     if (solver->time_discretization != TLW_LW && solver->time_discretization != TLW_MDRK) return TLW_EINVAL;
     if (solver->surface_flux_order != TLW_FLUX_TREE && solver->surface_flux_order != TLW_FLUX_STRUCTURED)
         return TLW_EINVAL;
-    if (solver->boundary_kind < TLW_BV_NONE || solver->boundary_kind > TLW_BV_CONVERGENCE_TEST) return TLW_EINVAL;
+    if (solver->boundary_kind < TLW_BV_NONE || solver->boundary_kind > TLW_BV_SLIP_WALL_APPROXIMATE) return TLW_EINVAL;
+    // the conditions on uⁿ with a normal direction: the curvilinear Euler mesh
+    if ((solver->boundary_kind == TLW_BV_OUTFLOW || solver->boundary_kind == TLW_BV_SLIP_WALL_APPROXIMATE) &&
+        (!cv || par))
+        return TLW_ENOTSUP;
     if (!(solver->gamma > 1.0)) return TLW_EINVAL;
     if (solver->source_terms < TLW_SRC_NONE || solver->source_terms > TLW_SRC_CONVERGENCE_TEST) return TLW_EINVAL;
     // source terms: the Euler LW and MDRK kernels of the TreeMesh

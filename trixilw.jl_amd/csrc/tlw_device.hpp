@@ -36,6 +36,8 @@ namespace tlw {
 constexpr int kBcSlipWall = 4;  // boundary kind of the reflecting wall (TLW_BV_SLIP_WALL)
 constexpr int kBcTable = 5;     // caller-defined boundary data (TLW_BV_TABLE)
 constexpr int kBcConvergenceTest = 6;  // the convergence test's state (TLW_BV_CONVERGENCE_TEST)
+constexpr int kBcOutflow = 7;          // the normal flux of uⁿ (TLW_BV_OUTFLOW, curvilinear mesh)
+constexpr int kBcSlipWallApprox = 8;   // Rusanov with uⁿ's reflection (TLW_BV_SLIP_WALL_APPROXIMATE, curvilinear)
 constexpr int kTraceQ = 10;  // face record: jump state (4), normal flux (4), |v_n|, c
 // shock-capturing face record: + fn_low (4), blending factor, padding
 constexpr int kTraceQSC = 16;
@@ -2759,6 +2761,39 @@ __device__ __forceinline__ void surface_body(const KArgs& a, int bid) {
                     // uⁿ, λ from Ũ; no blending (structured/dg_2d.jl:2352-2391)
                     double nf[4], in10[kTraceQ], out10[kTraceQ];
                     load4(a.fnorm + (((size_t)e * 4 + f) * n + l) * 4, nf);
+                    if (c->bc_kind == kBcOutflow || c->bc_kind == kBcSlipWallApprox) {
+                        // conditions on uⁿ with the node's contravariant vector as normal
+                        // (boundary_condition_outflow, slip_wall_approximate,
+                        // equations.jl:185-197, 251-260): the normal flux of uⁿ, or the
+                        // Rusanov flux between uⁿ and its reflection with the reflected
+                        // state on the side of the boundary
+                        double un[4], ur[4];
+                        if constexpr (SC) {
+#pragma unroll
+                            for (int v = 0; v < 4; ++v) un[v] = own[8 + v];
+                        } else {
+                            load4(sel_in(a, cur) + ((size_t)e * NN + node) * 4, un);
+                        }
+                        if (c->bc_kind == kBcOutflow) {
+                            rusanov_normal(un, un, c->gamma, nf[0], nf[1], nf[2], Fs);
+                        } else {
+                            const double ir = rcp_nr(un[0]);
+                            const double v1 = un[1] * ir, v2 = un[2] * ir;
+                            const double p = (c->gamma - 1.0) * (un[3] - 0.5 * (un[1] * v1 + un[2] * v2));
+                            const double vn = fma(v1, nf[0], v2 * nf[1]) / (nf[2] * nf[2]);
+                            const double w1 = v1 - 2.0 * vn * nf[0], w2 = v2 - 2.0 * vn * nf[1];
+                            ur[0] = un[0];
+                            ur[1] = un[0] * w1;
+                            ur[2] = un[0] * w2;
+                            ur[3] = p / (c->gamma - 1.0) + 0.5 * un[0] * (w1 * w1 + w2 * w2);
+                            if (f & 1) rusanov_normal(un, ur, c->gamma, nf[0], nf[1], nf[2], Fs);
+                            else rusanov_normal(ur, un, c->gamma, nf[0], nf[1], nf[2], Fs);
+                        }
+                        const double wf = (f & 1) ? c->bnd_hi : -c->bnd_lo;
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) surf[v] = fma(Fs[v], wf, surf[v]);
+                        return;
+                    }
                     const double ir = rcp_nr(own[0]);
                     const double b1 = own[1] * ir, b2 = own[2] * ir;
                     const double pb = (c->gamma - 1.0) * (own[3] - 0.5 * (own[1] * b1 + own[2] * b2));

@@ -31,6 +31,8 @@ const TLW_BV_RIEMANN4 = Int32(3)
 const TLW_BV_SLIP_WALL = Int32(4)
 const TLW_BV_TABLE = Int32(5)      # BoundaryConditionDirichlet(f) for any f: see set_boundary_data!
 const TLW_BV_CONVERGENCE_TEST = Int32(6)  # BoundaryConditionDirichlet(initial_condition_convergence_test)
+const TLW_BV_OUTFLOW = Int32(7)               # boundary_condition_outflow (curvilinear StructuredMesh)
+const TLW_BV_SLIP_WALL_APPROXIMATE = Int32(8) # slip_wall_approximate (curvilinear StructuredMesh)
 const TLW_SRC_NONE = Int32(0)
 const TLW_SRC_CONVERGENCE_TEST = Int32(1)  # source_terms = source_terms_convergence_test
 const TLW_AUX_U = Int32(0)

@@ -12,7 +12,8 @@ from .equations import (Adiabatic, BoundaryConditionDirichlet, BoundaryCondition
                         CompressibleNavierStokesDiffusion2D, ConstantState, Isothermal, NoSlip,
                         FourQuadrantRiemann, GradientVariablesConservative, GradientVariablesPrimitive,
                         ConvergenceTest, IsentropicVortex, SourceTermsConvergenceTest, TaylorGreenVortex,
-                        boundary_condition_slip_wall_horizontal, initial_condition_convergence_test,
+                        boundary_condition_outflow, boundary_condition_slip_wall_horizontal,
+                        initial_condition_convergence_test, slip_wall_approximate,
                         source_terms_convergence_test,
                         boundary_condition_slip_wall_vertical)
 from .integrator import (LWIntegrator, LWSolution, compute_dt, isapprox, isfinished, perform_step, set_dt,
@@ -28,7 +29,8 @@ from .semidiscretization import (DGSEM, LW, MDRK, Adaptive, CFLBased, FirstOrder
 
 __all__ = [
     "LobattoLegendreBasis", "CompressibleEulerEquations2D", "ConstantState", "FourQuadrantRiemann",
-    "boundary_condition_slip_wall_vertical", "boundary_condition_slip_wall_horizontal", "NoSlip", "Adiabatic", "Isothermal",
+    "boundary_condition_slip_wall_vertical", "boundary_condition_slip_wall_horizontal",
+    "boundary_condition_outflow", "slip_wall_approximate", "NoSlip", "Adiabatic", "Isothermal",
     "BoundaryConditionNavierStokesWall", "BoundaryConditionDirichlet",
     "IsentropicVortex", "TaylorGreenVortex", "LWIntegrator", "LWSolution", "compute_dt", "isapprox",
     "isfinished", "perform_step", "set_dt", "set_t_and_iter", "solve_lwfr", "TreeMesh",

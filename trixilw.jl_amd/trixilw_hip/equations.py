@@ -16,6 +16,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 BV_NONE, BV_VORTEX, BV_CONSTANT, BV_RIEMANN4, BV_SLIP_WALL, BV_TABLE, BV_CONVERGENCE_TEST = 0, 1, 2, 3, 4, 5, 6
+BV_OUTFLOW, BV_SLIP_WALL_APPROXIMATE = 7, 8
 SRC_NONE, SRC_CONVERGENCE_TEST = 0, 1
 
 
@@ -63,6 +64,29 @@ class _SlipWall:
 
 boundary_condition_slip_wall_vertical = _SlipWall(1)
 boundary_condition_slip_wall_horizontal = _SlipWall(2)
+
+
+class _NormalCondition:
+    """A condition on uⁿ alone with the boundary node's normal direction, as
+    the reference's P4est boundary call uses it (src/solvers/dgsem_p4est/
+    dg_2d.jl:361-393): boundary_condition_outflow (equations.jl:185-197) is
+    the normal flux of uⁿ, slip_wall_approximate (equations.jl:251-260) the
+    Rusanov flux between uⁿ and its reflection about the normal.  The device
+    evaluates them on the curvilinear StructuredMesh (Euler, every boundary
+    face), with the element's contravariant vector at the node as normal."""
+
+    params = ()
+
+    def __init__(self, kind, name):
+        self.kind = kind
+        self._name = name
+
+    def __repr__(self):
+        return self._name
+
+
+boundary_condition_outflow = _NormalCondition(BV_OUTFLOW, "boundary_condition_outflow")
+slip_wall_approximate = _NormalCondition(BV_SLIP_WALL_APPROXIMATE, "slip_wall_approximate")
 
 
 class NoSlip:
