@@ -63,6 +63,13 @@ class _Desc(C.Structure):
         ("bc_fn", C.c_void_p),
         ("source_kind", C.c_int),
         ("src_fn", C.c_void_p),
+        ("n_tree", C.c_int),
+        ("tree_cx", C.c_void_p),
+        ("tree_cy", C.c_void_p),
+        ("tree_len", C.c_void_p),
+        ("tree_nbr", C.c_void_p),
+        ("n_mortars", C.c_int),
+        ("mortars", C.c_void_p),
     ]
 
 
@@ -92,6 +99,7 @@ def lib():
         L.tlwo_ndofs.restype = C.c_longlong
         L.tlwo_ndofs.argtypes = [C.c_void_p]
         L.tlwo_basis.argtypes = [C.c_void_p, dp, dp, dp, dp]
+        L.tlwo_mortar_matrices.argtypes = [C.c_void_p, dp, dp, dp, dp]
         L.tlwo_node_coords.argtypes = [C.c_void_p, dp, dp]
         L.tlwo_neighbors.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
         L.tlwo_max_dt.restype = C.c_double
@@ -141,7 +149,7 @@ class Oracle:
                  adv=(1.0, 1.0), flux_order=0, bc_kind=BV_NONE, bc_par=(), tolerances=(0.0, 0.0), nthreads=0,
                  shock_capturing=False, alpha_max=0.5, alpha_min=0.001, alpha_smooth=True, positivity=None,
                  parabolic=None, mu=0.0, prandtl=0.72, mapping=None, reconstruction="first_order",
-                 ns_walls=None, wall_temperature=None, bc_fn=None, source_terms=None):
+                 ns_walls=None, wall_temperature=None, bc_fn=None, source_terms=None, tree=None):
         """``positivity``: (threshold_density, threshold_pressure) of a Zhang-Shu
         limiter that ``solve`` applies as a stage limiter, or None.
         ``parabolic``: None, "laplace" (LaplaceDiffusion2D with diffusivity ``mu``,
@@ -160,7 +168,10 @@ class Oracle:
         f(x, y, t) -> 4 conservative values, called pointwise (Python).
         ``source_terms``: None, "convergence_test" (Trixi's
         source_terms_convergence_test) or a Python function s(u, x, y, t) -> 4
-        values, called pointwise (Euler on the TreeMesh)."""
+        values, called pointwise (Euler on the TreeMesh).
+        ``tree``: a TreeMesh with refined cells, (cx, cy, h, neighbors [E, 4],
+        mortars [M, 5]) as ``tree_mesh.connectivity`` gives them; nx, ny, xmin,
+        ymin, h and periodic are then unused and elements keep the given order."""
         d = _Desc()
         d.polydeg, d.nx, d.ny = polydeg, nx, ny
         d.xmin, d.ymin, d.h = xmin, ymin, h
@@ -214,6 +225,16 @@ class Oracle:
             self._src_cb = SRC_FN(_scb)
             d.source_kind = SRC_CALLBACK
             d.src_fn = C.cast(self._src_cb, C.c_void_p)
+        if tree is not None:
+            tcx, tcy, th, tnbr, tmort = tree
+            self._tree = (np.ascontiguousarray(tcx, dtype=np.float64), np.ascontiguousarray(tcy, dtype=np.float64),
+                          np.ascontiguousarray(th, dtype=np.float64), np.ascontiguousarray(tnbr, dtype=np.int32),
+                          np.ascontiguousarray(tmort, dtype=np.int32).reshape(-1, 5))
+            d.n_tree = len(self._tree[0])
+            d.tree_cx, d.tree_cy, d.tree_len, d.tree_nbr = (a.ctypes.data for a in self._tree[:4])
+            d.n_mortars = len(self._tree[4])
+            d.mortars = self._tree[4].ctypes.data if d.n_mortars else None
+            nx, ny = d.n_tree, 1
         self._desc = d
         self._h = lib().tlwo_create(C.byref(d))
         if not self._h:
@@ -231,6 +252,13 @@ class Oracle:
     @property
     def shape(self):
         return (self.E, self.n, self.n, self.nv)
+
+    def mortar_matrices(self):
+        """forward_lower, forward_upper, reverse_lower, reverse_upper ([n, n], row i = output node)."""
+        n = self.n
+        m = [np.zeros((n, n)) for _ in range(4)]
+        lib().tlwo_mortar_matrices(self._h, *[_ptr(a) for a in m])
+        return tuple(m)
 
     def basis(self):
         n = self.n

@@ -194,8 +194,71 @@ struct Basis {
         }
         for (int i = 0; i < n; ++i)
             for (int m = 0; m < n; ++m) Vinv[i][m] = (double)V[i][n + m];
+        init_mortar(x);
     }
     double Vinv[MAXN][MAXN] = {};
+
+    // Trixi's LobattoLegendreMortarL2 (Trixi.jl, src/solvers/dgsem/l2projection.jl;
+    // Trixi is an unpinned dependency of the reference's Project.toml), restated
+    // from its definition: forward_lower/upper interpolate the large face's
+    // polynomial to the GLL nodes of its lower half (ξ-1)/2 and upper half (ξ+1)/2;
+    // reverse_lower/upper are the L2 projection of the two halves back onto the
+    // large face, evaluated by (N+1)-point Gauss quadrature (exact for these
+    // degree-2N integrands) between interpolations GLL -> Gauss -> GLL.  Here the
+    // Lagrange polynomials are plain products in extended precision.
+    double fwd_lo[MAXN][MAXN] = {}, fwd_up[MAXN][MAXN] = {}, rev_lo[MAXN][MAXN] = {}, rev_up[MAXN][MAXN] = {};
+    static long double lagrange(const long double* xs, int m, int k, long double x) {
+        long double p = 1.0L;
+        for (int q = 0; q < m; ++q)
+            if (q != k) p *= (x - xs[q]) / (xs[k] - xs[q]);
+        return p;
+    }
+    void init_mortar(const long double* x) {
+        // Gauss-Legendre nodes and weights: Newton on L_n from Chebyshev-like guesses
+        long double g[MAXN], wg[MAXN];
+        const long double pi = 3.141592653589793238462643383279502884L;
+        for (int j = 0; j < n; ++j) {
+            long double xx = -cosl((j + 0.75L) * pi / (n + 0.5L)), dL = 1;
+            for (int it = 0; it < 100; ++it) {
+                long double L0 = 1, L1 = xx;
+                dL = 1;
+                for (int k = 2; k <= n; ++k) {
+                    const long double L2 = ((2.0L * k - 1) * xx * L1 - (k - 1.0L) * L0) / k;
+                    L0 = L1;
+                    L1 = L2;
+                }
+                if (n == 1) { L1 = xx; L0 = 1; }
+                dL = n * (xx * L1 - L0) / (xx * xx - 1.0L);
+                const long double d = -L1 / dL;
+                xx += d;
+                if (fabsl(d) <= 1e-19L) break;
+            }
+            g[j] = xx;
+            wg[j] = 2.0L / ((1.0L - xx * xx) * dL * dL);
+        }
+        long double l2g[MAXN][MAXN], g2l[MAXN][MAXN], op_lo[MAXN][MAXN], op_up[MAXN][MAXN];
+        for (int a = 0; a < n; ++a)
+            for (int k = 0; k < n; ++k) {
+                l2g[a][k] = lagrange(x, n, k, g[a]);  // GLL data -> Gauss node a
+                g2l[a][k] = lagrange(g, n, k, x[a]);  // Gauss data -> GLL node a
+                // projection on Gauss nodes: (Pf)(g_a) = Σ_k ½ ℓ_a(½(g_k ± 1)) w_k / w_a f(g_k)
+                op_lo[a][k] = 0.5L * lagrange(g, n, a, 0.5L * (g[k] - 1.0L)) * wg[k] / wg[a];
+                op_up[a][k] = 0.5L * lagrange(g, n, a, 0.5L * (g[k] + 1.0L)) * wg[k] / wg[a];
+            }
+        for (int i = 0; i < n; ++i)
+            for (int k = 0; k < n; ++k) {
+                fwd_lo[i][k] = (double)lagrange(x, n, k, 0.5L * (x[i] - 1.0L));
+                fwd_up[i][k] = (double)lagrange(x, n, k, 0.5L * (x[i] + 1.0L));
+                long double rl = 0, ru = 0;
+                for (int a = 0; a < n; ++a)
+                    for (int b = 0; b < n; ++b) {
+                        rl += g2l[i][a] * op_lo[a][b] * l2g[b][k];
+                        ru += g2l[i][a] * op_up[a][b] * l2g[b][k];
+                    }
+                rev_lo[i][k] = (double)rl;
+                rev_up[i][k] = (double)ru;
+            }
+    }
 };
 
 // ---------------------------------------------------------------------------
@@ -480,8 +543,49 @@ struct Mesh {
                     bd_dir.push_back(dir);
                 }
     }
-    double node_x(const Basis& b, int i, int e) const { return xmin + h * ((e % nx) + 0.5) + 0.5 * h * b.nodes[i]; }
-    double node_y(const Basis& b, int j, int e) const { return ymin + h * ((e / nx) + 0.5) + 0.5 * h * b.nodes[j]; }
+    // A TreeMesh with refined cells (Trixi TreeMesh leaves of different levels):
+    // element centres and edge lengths, the face neighbours (-1 domain boundary,
+    // -2 a face on a mortar) and Trixi's L2 mortar container (neighbor_ids
+    // lower, upper, large; orientations; large_sides), one mortar per large face
+    // with two small neighbours.
+    struct MortarRec { int lower, upper, large, orient, large_side; };
+    std::vector<double> cx, cy, len;
+    std::vector<MortarRec> mortars;
+    void init_general(int E_, const double* cx_, const double* cy_, const double* len_, const int* nbr_, int M,
+                      const int* mort) {
+        E = E_; nx = E_; ny = 1; h = 0.0; per_x = per_y = false;
+        cx.assign(cx_, cx_ + E);
+        cy.assign(cy_, cy_ + E);
+        len.assign(len_, len_ + E);
+        inv_jac.resize(E);
+        for (int e = 0; e < E; ++e) inv_jac[e] = 2.0 / len[e];
+        nbr.assign(nbr_, nbr_ + 4 * E);
+        mortars.resize(M);
+        for (int m = 0; m < M; ++m)
+            mortars[m] = {mort[5 * m], mort[5 * m + 1], mort[5 * m + 2], mort[5 * m + 3], mort[5 * m + 4]};
+        if_left.clear(); if_right.clear(); if_orient.clear();
+        for (int e = 0; e < E; ++e) {
+            if (nbr[4 * e + 1] >= 0) { if_left.push_back(e); if_right.push_back(nbr[4 * e + 1]); if_orient.push_back(1); }
+            if (nbr[4 * e + 3] >= 0) { if_left.push_back(e); if_right.push_back(nbr[4 * e + 3]); if_orient.push_back(2); }
+        }
+        bd_elem.clear(); bd_orient.clear(); bd_side.clear(); bd_dir.clear();
+        for (int dir = 1; dir <= 4; ++dir)
+            for (int e = 0; e < E; ++e)
+                if (nbr[4 * e + (dir - 1)] == -1) {
+                    bd_elem.push_back(e);
+                    bd_orient.push_back(dir <= 2 ? 1 : 2);
+                    bd_side.push_back((dir % 2 == 0) ? 1 : 2);
+                    bd_dir.push_back(dir);
+                }
+    }
+    double node_x(const Basis& b, int i, int e) const {
+        if (!cx.empty()) return cx[e] + 0.5 * len[e] * b.nodes[i];
+        return xmin + h * ((e % nx) + 0.5) + 0.5 * h * b.nodes[i];
+    }
+    double node_y(const Basis& b, int j, int e) const {
+        if (!cy.empty()) return cy[e] + 0.5 * len[e] * b.nodes[j];
+        return ymin + h * ((e / nx) + 0.5) + 0.5 * h * b.nodes[j];
+    }
 };
 
 // ---------------------------------------------------------------------------
@@ -1642,6 +1746,67 @@ template <class T, class Eq> struct Solver {
         }
     }
 
+    // prolong2mortars! + calc_mortar_flux! for the TreeMesh (dg_2d.jl:1456-1646):
+    // on each half of the mortar, uⁿ, Ũ and the normal component of F̃ of the
+    // small element's face node next to the large element's values interpolated
+    // by forward_lower/upper, the LW flux in the interface order (the large
+    // element on the left when large_side is 1); then Trixi's
+    // mortar_fluxes_to_elements!: each small element takes its half's fluxes,
+    // the large element reverse_upper·F*_upper + reverse_lower·F*_lower.
+    void calc_mortar_flux(const T* u) {
+        const int nM = (int)mesh.mortars.size();
+        auto face_node = [&](int dir, int l, int& i, int& j) {
+            if (dir < 2) { i = (dir == 0) ? 0 : n - 1; j = l; }
+            else { i = l; j = (dir == 2) ? 0 : n - 1; }
+        };
+#pragma omp parallel for schedule(static)
+        for (int m = 0; m < nM; ++m) {
+            const auto& M = mesh.mortars[m];
+            const int o = M.orient;
+            // 0-based faces: the large element's + face and the small elements' - faces
+            // when the large element is on the left (large_side 1), and the reverse
+            const int ldir = (M.large_side == 1) ? 2 * o - 1 : 2 * o - 2;
+            const int sdir = (M.large_side == 1) ? 2 * o - 2 : 2 * o - 1;
+            T fstar[2][MAXN][NV];
+            for (int half = 0; half < 2; ++half) {  // 0 lower, 1 upper
+                const int es = half ? M.upper : M.lower;
+                const double(*P)[MAXN] = half ? basis.fwd_up : basis.fwd_lo;
+                for (int l = 0; l < n; ++l) {
+                    int i, j;
+                    face_node(sdir, l, i, j);
+                    const T* us_ = &u[idx(0, i, j, es)];
+                    const T* Us_ = &U[idx(0, i, j, es)];
+                    const T* Fs_ = &F[fidx(0, o - 1, i, j, es)];
+                    T uL[NV], UL[NV], FL[NV];
+                    for (int v = 0; v < NV; ++v) { uL[v] = T(0); UL[v] = T(0); FL[v] = T(0); }
+                    for (int k = 0; k < n; ++k) {
+                        int ii, jj;
+                        face_node(ldir, k, ii, jj);
+                        for (int v = 0; v < NV; ++v) {
+                            uL[v] = uL[v] + P[l][k] * u[idx(v, ii, jj, M.large)];
+                            UL[v] = UL[v] + P[l][k] * U[idx(v, ii, jj, M.large)];
+                            FL[v] = FL[v] + P[l][k] * F[fidx(v, o - 1, ii, jj, M.large)];
+                        }
+                    }
+                    T Fn[NV];
+                    if (M.large_side == 1) lw_numflux(FL, Fs_, UL, Us_, uL, us_, o, Fn);
+                    else lw_numflux(Fs_, FL, Us_, UL, us_, uL, o, Fn);
+                    for (int v = 0; v < NV; ++v) {
+                        fstar[half][l][v] = Fn[v];
+                        sfv[sidx(v, l, sdir, es)] = Fn[v];
+                    }
+                }
+            }
+            for (int l = 0; l < n; ++l)
+                for (int v = 0; v < NV; ++v) {
+                    T acc = T(0);
+                    for (int k = 0; k < n; ++k) acc = acc + basis.rev_up[l][k] * fstar[1][k][v];
+                    for (int k = 0; k < n; ++k) acc = acc + basis.rev_lo[l][k] * fstar[0][k][v];
+                    sfv[sidx(v, l, ldir, M.large)] = acc;
+                }
+        }
+    }
+
     // Trixi calc_surface_integral! (weak form, TreeMesh 2-D) followed by
     // apply_jacobian! (du *= -J⁻¹), called at dg_2d.jl:75-80.
     void surface_integral_and_jacobian(T* du) {
@@ -1683,6 +1848,7 @@ template <class T, class Eq> struct Solver {
         }
         calc_interface_flux(u);
         if (!mesh.bd_elem.empty()) calc_boundary_flux(u, t, dt, 1.0);
+        if (!mesh.mortars.empty()) calc_mortar_flux(u);
         surface_integral_and_jacobian(du);
     }
 
@@ -1719,6 +1885,7 @@ template <class T, class Eq> struct Solver {
         }
         calc_interface_flux(u);
         if (!mesh.bd_elem.empty()) calc_boundary_flux(u, t, dt, flux_order == 2 ? 1.0 : 0.5);
+        if (!mesh.mortars.empty()) calc_mortar_flux(u);
         surface_integral_and_jacobian(du);
     }
     virtual void rhs_mdrk2(T* du, const T* u, double t, double dt) {
@@ -1733,6 +1900,7 @@ template <class T, class Eq> struct Solver {
         }
         calc_interface_flux(u);
         if (!mesh.bd_elem.empty()) calc_boundary_flux(u, t, dt, 1.0);
+        if (!mesh.mortars.empty()) calc_mortar_flux(u);
         surface_integral_and_jacobian(du);
     }
 

@@ -59,6 +59,17 @@ struct tlwo_desc {
     // src_fn(u, x, y, t, s)
     int source_kind;
     void (*src_fn)(const double*, double, double, double, double*);
+    // a TreeMesh with refined cells instead of the uniform nx x ny grid (n_tree > 0):
+    // element centres, edge lengths, face neighbours (-1 boundary, -2 on a mortar)
+    // and n_mortars records (lower, upper, large, orientation, large_side); plain FR,
+    // hyperbolic only
+    int n_tree;
+    const double* tree_cx;
+    const double* tree_cy;
+    const double* tree_len;
+    const int* tree_nbr;
+    int n_mortars;
+    const int* mortars;
 };
 
 struct tlwo_ctx {
@@ -69,7 +80,10 @@ struct tlwo_ctx {
 
 template <class S> static void configure(S& s, const tlwo_desc* d) {
     s.basis.init(d->polydeg);
-    s.mesh.init(d->nx, d->ny, d->xmin, d->ymin, d->h, d->periodic_x != 0, d->periodic_y != 0);
+    if (d->n_tree > 0)
+        s.mesh.init_general(d->n_tree, d->tree_cx, d->tree_cy, d->tree_len, d->tree_nbr, d->n_mortars, d->mortars);
+    else
+        s.mesh.init(d->nx, d->ny, d->xmin, d->ymin, d->h, d->periodic_x != 0, d->periodic_y != 0);
     s.flux_order = d->flux_order;
     s.bval.kind = d->bc_kind;
     for (int k = 0; k < 24; ++k) s.bval.par[k] = d->bc_par[k];
@@ -113,6 +127,11 @@ void* tlwo_create(const tlwo_desc* d) {
         (d->equation != 0 || !d->curved || d->parabolic))
         return nullptr;
     if (d->bc_kind == BV_CALLBACK && !d->bc_fn) return nullptr;
+    // refined TreeMesh: the hyperbolic plain-FR kernels (compute_alp and the indicator
+    // smoothing read the uniform mesh's Jacobian and neighbours)
+    if (d->n_tree > 0 && (d->curved || d->parabolic || d->shock_capturing || !d->tree_cx || !d->tree_cy ||
+                          !d->tree_len || !d->tree_nbr || d->n_mortars < 0 || (d->n_mortars > 0 && !d->mortars)))
+        return nullptr;
     // source terms: the Euler kernels of the TreeMesh (LW N = 3, 4 and MDRK)
     if (d->source_kind != SRC_NONE &&
         (d->equation != 0 || d->parabolic || d->curved || (d->source_kind == SRC_CALLBACK && !d->src_fn) ||
@@ -210,6 +229,20 @@ void tlwo_basis(void* ctx, double* nodes, double* weights, double* D, double* Dh
                 Dhat[i * n + k] = s.basis.Dhat[i][k];
             }
         }
+    });
+}
+
+// Trixi LobattoLegendreMortarL2 operators (n*n, row-major [output node][input node])
+void tlwo_mortar_matrices(void* ctx, double* fwd_lo, double* fwd_up, double* rev_lo, double* rev_up) {
+    DISPATCH(ctx, {
+        const int n = s.n;
+        for (int i = 0; i < n; ++i)
+            for (int k = 0; k < n; ++k) {
+                fwd_lo[i * n + k] = s.basis.fwd_lo[i][k];
+                fwd_up[i * n + k] = s.basis.fwd_up[i][k];
+                rev_lo[i * n + k] = s.basis.rev_lo[i][k];
+                rev_up[i * n + k] = s.basis.rev_up[i][k];
+            }
     });
 }
 
