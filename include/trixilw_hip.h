@@ -88,7 +88,8 @@ enum { TLW_RECON_FIRST_ORDER = 0, TLW_RECON_MUSCL_HANCOCK = 1 };
 enum { TLW_FLAG_KEEP_AUX = 1 /* store full Ũ [v,i,j,e] and F̃ [v,d,i,j,e] on every rhs */ };
 
 /* Mesh: Trixi TreeMesh element data (cache.elements) plus face neighbours
- * derived from cache.interfaces / cache.boundaries.  Conforming meshes only. */
+ * derived from cache.interfaces / cache.boundaries.  Faces between leaves of different levels are
+ * described by tlw_mortar_desc (tlw_create_nonconforming). */
 typedef struct {
     int32_t n_elements;
     const int32_t* neighbors_host;        /* [4*E]: neighbour of faces -x,+x,-y,+y; -1 = domain boundary */
@@ -96,6 +97,20 @@ typedef struct {
     const double* element_center_host;    /* [2*E] (x, y); used for boundary node coordinates */
     const double* element_length_host;    /* [E] */
 } tlw_mesh_desc;
+
+/* TreeMesh with refined cells (leaves of different levels, 2:1 across faces): a face whose
+ * neighbours across it are two leaves of the next level, or that is one of those two leaves' faces,
+ * carries neighbour TLW_NEIGHBOR_MORTAR and lies on one L2 mortar.  Each mortar record is Trixi's
+ * mortar container entry (src/solvers/dgsem_tree/containers_2d.jl in Trixi.jl, read by the
+ * reference's prolong2mortars!/calc_mortar_flux!, src/solvers/dgsem_tree/dg_2d.jl:1456-1646), 0-based:
+ * {lower element, upper element, large element, orientation (1: x face, 2: y face), large_side
+ * (1: the large element lies at -x/-y of the face, 2: at +x/+y)}; lower/upper is the small element at
+ * the lower/upper half of the face coordinate (y on x faces, x on y faces). */
+#define TLW_NEIGHBOR_MORTAR (-2)
+typedef struct {
+    int32_t n_mortars;
+    const int32_t* mortars_host;  /* [5*n_mortars] */
+} tlw_mortar_desc;
 
 /* Basis: Trixi LobattoLegendreBasis fields, arrays column-major as Julia stores them. */
 typedef struct {
@@ -265,6 +280,32 @@ int tlw_enable_graphs(tlw_ctx* ctx, int32_t enable);
 
 /* AMR or any host-side mesh change: rebuild the device mesh (src/callbacks/amr.jl:2-20 resize!) */
 int tlw_rebuild_mesh(tlw_ctx* ctx, const tlw_mesh_desc* mesh);
+
+/* A TreeMesh with refined cells (mortars, see tlw_mortar_desc): the rhs adds prolong2mortars! and
+ * calc_mortar_flux! of src/solvers/dgsem_tree/dg_2d.jl:1456-1646 (also in MDRK,
+ * tree/multi_derivative_2d.jl:37-46, 98-107) with Trixi's LobattoLegendreMortarL2 and
+ * mortar_fluxes_to_elements!: on each half of a mortar the large element's uⁿ, Ũ and F̃·n
+ * interpolated to the small face's nodes, the LW Rusanov flux in the interface order, the small
+ * element taking it and the large element its L2 projection.  Euler, LW or MDRK, VolumeIntegralFR,
+ * one instance; shock capturing, parabolic terms and curvilinear meshes return TLW_ENOTSUP, and so
+ * do the adjoints.  Source terms, the positivity limiter, adaptive steps (polydeg 4), the device
+ * loop and graphs work as on a uniform TreeMesh. */
+int tlw_create_nonconforming(const tlw_mesh_desc* mesh, const tlw_mortar_desc* mortars, const tlw_basis_desc* basis,
+                             const tlw_solver_desc* solver, tlw_ctx** out);
+/* AMR re-upload of a TreeMesh with refined cells (src/callbacks/amr.jl:2-208 resize! of the element,
+ * interface, boundary and mortar caches). */
+int tlw_rebuild_mesh_nonconforming(tlw_ctx* ctx, const tlw_mesh_desc* mesh, const tlw_mortar_desc* mortars);
+/* AMR solution transfer (Trixi's refine!/coarsen! of the DG solution, which the reference's AMR
+ * callback runs before src/callbacks/amr.jl:2-20 resizes the caches): u_new_dev [n_new elements] from
+ * u_old_dev [n_old elements], both in the context's state layout, with the mortar operators of the
+ * context's basis.  plan_host [5*n_new], per new element: {0, e} keeps old element e; {1, p, child}
+ * is a child of old element p (child 0 lower left, 1 lower right, 2 upper left, 3 upper right,
+ * forward_lower/upper in x then y); {2, c0, c1, c2, c3} is the parent of old elements c0..c3 (same
+ * child order, reverse_lower/upper, summed).  Independent of the context's mesh, so it runs between
+ * the host-side mesh change and tlw_rebuild_mesh_nonconforming.  Uploads the plan to a temporary
+ * device buffer and waits for the context's stream. */
+int tlw_transfer_solution(tlw_ctx* ctx, const double* u_old_dev, int32_t n_old, double* u_new_dev, int32_t n_new,
+                          const int32_t* plan_host);
 void tlw_destroy(tlw_ctx* ctx);
 const char* tlw_last_error(const tlw_ctx* ctx);
 

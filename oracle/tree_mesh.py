@@ -21,12 +21,13 @@ from __future__ import annotations
 import numpy as np
 
 
-def leaves_refined(coordinates_min, length, level, boxes=(), balance=True):
+def leaves_refined(coordinates_min, length, level, boxes=(), balance=True, periodic=(True, True)):
     """Leaves (cx, cy, h) of the square [x0, x0 + length]² at the initial
     level, refined once per box in order: every leaf whose centre lies strictly
     inside the box is split in four (Trixi refine_box!, src/meshes/
     abstract_tree.jl), then leaves are split until face neighbours differ by at
-    most one level (Trixi's 2:1 rebalance across faces)."""
+    most one level (Trixi's 2:1 rebalance across faces, periodic faces
+    included)."""
     x0, y0 = map(float, coordinates_min)
     h0 = float(length) / (1 << level)
     leaves = {(x0 + (i + 0.5) * h0, y0 + (j + 0.5) * h0, h0) for i in range(1 << level) for j in range(1 << level)}
@@ -42,18 +43,22 @@ def leaves_refined(coordinates_min, length, level, boxes=(), balance=True):
         for c in hit:
             leaves.remove(c)
             leaves |= split(c)
+        shifts = [(sx, sy) for sx in ((-length, 0.0, length) if periodic[0] else (0.0,))
+                  for sy in ((-length, 0.0, length) if periodic[1] else (0.0,))]
         while balance:
             L = np.array(sorted(leaves))
             bad = set()
             for a in range(len(L)):
-                cx, cy, h = L[a]
-                # leaves touching a's faces that are more than twice smaller
-                near = (np.abs(L[:, 0] - cx) <= 0.5 * (h + L[:, 2]) + 1e-12 * length) & \
-                       (np.abs(L[:, 1] - cy) <= 0.5 * (h + L[:, 2]) + 1e-12 * length)
-                face = near & ((np.abs(np.abs(L[:, 0] - cx) - 0.5 * (h + L[:, 2])) < 1e-12 * length) ^
-                               (np.abs(np.abs(L[:, 1] - cy) - 0.5 * (h + L[:, 2])) < 1e-12 * length))
-                if np.any(face & (L[:, 2] < 0.25 * h * (1 + 1e-9))):
-                    bad.add(tuple(L[a]))
+                cx0, cy0, h = L[a]
+                for sx, sy in shifts:  # the leaf and its periodic images
+                    cx, cy = cx0 + sx, cy0 + sy
+                    # leaves touching the faces that are more than twice smaller
+                    near = (np.abs(L[:, 0] - cx) <= 0.5 * (h + L[:, 2]) + 1e-12 * length) & \
+                           (np.abs(L[:, 1] - cy) <= 0.5 * (h + L[:, 2]) + 1e-12 * length)
+                    face = near & ((np.abs(np.abs(L[:, 0] - cx) - 0.5 * (h + L[:, 2])) < 1e-12 * length) ^
+                                   (np.abs(np.abs(L[:, 1] - cy) - 0.5 * (h + L[:, 2])) < 1e-12 * length))
+                    if np.any(face & (L[:, 2] < 0.25 * h * (1 + 1e-9))):
+                        bad.add(tuple(L[a]))
             if not bad:
                 break
             for c in bad:

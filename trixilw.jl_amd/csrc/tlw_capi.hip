@@ -85,7 +85,13 @@ struct tlw_ctx {
     int ebv = 0;              // elements per volume workgroup
     bool positivity = false;  // Zhang-Shu limiter in the device loop
     bool adaptive = false;    // error-controlled steps (tlw_solve_init_adaptive)
-    bool has_boundary = false;
+    bool has_boundary = false;  // a face with neighbour -1, or a mortar (the surface kernel's BC path)
+    // L2 mortars of a TreeMesh with refined cells: per mortar (orientation,
+    // large_side), the volume kernel's face-node values and the mortar fluxes
+    int n_mortars = 0;
+    int2* d_mort = nullptr;
+    double* d_mrec = nullptr;
+    double* d_mflux = nullptr;
     std::vector<int> host_cur;  // [ninst] state buffer holding each instance's uⁿ
     bool aux_valid = false;
     bool alpha_valid = false;  // blending factors of the last rhs or step (shock capturing)
@@ -164,7 +170,7 @@ void free_mesh(tlw_ctx* c) {
                     c->d_us,        c->d_auxU,  c->d_auxF,   c->d_partial, c->d_alpha_tmp, c->d_alpha,
                     c->d_vblk,      c->d_sblk,  c->d_etile,  c->d_rng,    c->d_state,   c->d_cur,
                     c->d_Ja,        c->d_ijn,   c->d_fnorm,  c->d_fxy,     c->d_snrm,      c->d_sqrk,
-                    c->d_fvb,       c->d_bidx,  c->d_btab};
+                    c->d_fvb,       c->d_bidx,  c->d_btab,   c->d_mort,   c->d_mrec,    c->d_mflux};
     for (void* p : ptrs) (void)hipFree(p);
     if (c->h_state) (void)hipHostFree(c->h_state);
     c->d_nbr = nullptr;
@@ -179,6 +185,9 @@ void free_mesh(tlw_ctx* c) {
     c->d_Ja = c->d_ijn = c->d_fnorm = c->d_fxy = c->d_snrm = c->d_sqrk = c->d_fvb = nullptr;
     c->d_bidx = nullptr;
     c->d_btab = nullptr;
+    c->d_mort = nullptr;
+    c->d_mrec = c->d_mflux = nullptr;
+    c->n_mortars = 0;
     c->btab_cap = 0;
     c->btab_levels = 0;
     c->n_bnodes = 0;
@@ -222,15 +231,16 @@ int build_tables(tlw_ctx* c) {
     return TLW_OK;
 }
 
-int validate_mesh(tlw_ctx* c, const tlw_mesh_desc* m) {
+int validate_mesh(tlw_ctx* c, const tlw_mesh_desc* m, bool mortars = false) {
     if (!m || m->n_elements <= 0 || !m->neighbors_host)
         return fail(c, TLW_EINVAL, "mesh description: missing neighbour table or n_elements <= 0");
     if (!c->curved && (!m->inverse_jacobian_host || !m->element_center_host || !m->element_length_host))
         return fail(c, TLW_EINVAL, "mesh description: missing arrays");
     const int E = m->n_elements;
+    const int lowest = mortars ? TLW_NEIGHBOR_MORTAR : -1;
     for (size_t k = 0; k < (size_t)4 * E; ++k) {
         const int nb = m->neighbors_host[k];
-        if (nb < -1 || nb >= E) return fail(c, TLW_EINVAL, "mesh description: neighbour id out of range");
+        if (nb < lowest || nb >= E) return fail(c, TLW_EINVAL, "mesh description: neighbour id out of range");
     }
     // conformity: the neighbour across a face must point back through the opposite face
     for (int e = 0; e < E; ++e)
@@ -322,20 +332,69 @@ int prepare_curved(tlw_ctx* c, const tlw_mesh_desc* m, const tlw_curved_desc* cv
 
 // Uploads ninst meshes as one element range each (element ids offset by the
 // instance's first element) and allocates everything that depends on E.
-int upload_meshes(tlw_ctx* c, const tlw_mesh_desc* ms, int ninst, const tlw_curved_desc* cv = nullptr) {
+// Mortar list of a TreeMesh with refined cells: every face marked
+// TLW_NEIGHBOR_MORTAR lies on exactly one mortar, in the role the record gives
+// it, and the geometry is a 2:1 face (small elements of half the large one's
+// length, side by side along the large face, lower before upper).  Returns the
+// device neighbour code of each mortar face in code[4*E] (-2 - (3 m + role)).
+int validate_mortars(tlw_ctx* c, const tlw_mesh_desc* m, const tlw_mortar_desc* md, std::vector<int>& code) {
+    const int E = m->n_elements, M = md->n_mortars;
+    if (M < 0 || (M > 0 && !md->mortars_host)) return fail(c, TLW_EINVAL, "mortar description: missing records");
+    code.assign(4 * (size_t)E, 0);
+    const double* xc = m->element_center_host;
+    const double* len = m->element_length_host;
+    for (int q = 0; q < M; ++q) {
+        const int* r = md->mortars_host + 5 * (size_t)q;
+        const int lower = r[0], upper = r[1], large = r[2], o = r[3], side = r[4];
+        if (lower < 0 || lower >= E || upper < 0 || upper >= E || large < 0 || large >= E || lower == upper ||
+            (o != 1 && o != 2) || (side != 1 && side != 2))
+            return fail(c, TLW_EINVAL, "mortar description: element id, orientation or large side out of range");
+        const int lface = (side == 1) ? 2 * o - 1 : 2 * o - 2, sface = lface ^ 1;
+        const int roles[3][2] = {{lower, sface}, {upper, sface}, {large, lface}};
+        for (int role = 0; role < 3; ++role) {
+            const size_t k = 4 * (size_t)roles[role][0] + roles[role][1];
+            if (m->neighbors_host[k] != TLW_NEIGHBOR_MORTAR || code[k] != 0)
+                return fail(c, TLW_EINVAL, "mortar description: a mortar face is not marked or lies on two mortars");
+            code[k] = -2 - (3 * q + role);
+        }
+        // geometry: half-size small elements next to each other, lower first along the face
+        const int ax = o - 1, tx = 1 - ax;
+        const double h = len[large], tol = 1e-9 * h;
+        if (std::fabs(len[lower] - 0.5 * h) > tol || std::fabs(len[upper] - 0.5 * h) > tol ||
+            std::fabs((xc[2 * upper + tx] - xc[2 * lower + tx]) - 0.5 * h) > tol ||
+            std::fabs(0.5 * (xc[2 * upper + tx] + xc[2 * lower + tx]) - xc[2 * large + tx]) > tol)
+            return fail(c, TLW_EINVAL, "mortar description: the small elements do not tile the large face");
+        (void)ax;
+    }
+    for (size_t k = 0; k < 4 * (size_t)E; ++k)
+        if (m->neighbors_host[k] == TLW_NEIGHBOR_MORTAR && code[k] == 0)
+            return fail(c, TLW_EINVAL, "mortar description: a face marked TLW_NEIGHBOR_MORTAR has no mortar");
+    return TLW_OK;
+}
+
+int upload_meshes(tlw_ctx* c, const tlw_mesh_desc* ms, int ninst, const tlw_curved_desc* cv = nullptr,
+                  const tlw_mortar_desc* md = nullptr) {
     if (ninst < 1) return fail(c, TLW_EINVAL, "at least one instance");
     if (c->curved && ninst != 1) return fail(c, TLW_ENOTSUP, "a curvilinear mesh is a single instance");
     long long total = 0;
     for (int k = 0; k < ninst; ++k) {
-        int rc = validate_mesh(c, &ms[k]);
+        int rc = validate_mesh(c, &ms[k], md != nullptr);
         if (rc) return rc;
         total += ms[k].n_elements;
     }
     if (total > (1LL << 30)) return fail(c, TLW_EINVAL, "too many elements");
+    std::vector<int> mcode;
+    const int M = md ? md->n_mortars : 0;
+    if (md) {
+        if (ninst != 1 || c->curved || c->sc || c->parabolic)
+            return fail(c, TLW_ENOTSUP, "mortars: a single-instance hyperbolic TreeMesh context with VolumeIntegralFR");
+        int rc = validate_mortars(c, &ms[0], md, mcode);
+        if (rc) return rc;
+    }
     bool boundary = false;
     for (int k = 0; k < ninst; ++k)
         for (size_t q = 0; q < (size_t)4 * ms[k].n_elements; ++q)
-            if (ms[k].neighbors_host[q] < 0) boundary = true;
+            if (ms[k].neighbors_host[q] == -1) boundary = true;
     if (boundary && c->hconst.bc_kind == TLW_BV_NONE)
         return fail(c, TLW_EINVAL, "mesh has boundary faces but no boundary value function was given");
     if (boundary && c->parabolic && !c->hconst.ns_walls)
@@ -368,7 +427,8 @@ int upload_meshes(tlw_ctx* c, const tlw_mesh_desc* ms, int ninst, const tlw_curv
     c->aux_valid = c->alpha_valid = false;
     c->ninst = ninst;
     c->E = (int)total;
-    c->has_boundary = boundary;
+    c->has_boundary = boundary || M > 0;
+    c->n_mortars = M;
     c->inst_first.assign(ninst + 1, 0);
     for (int k = 0; k < ninst; ++k) c->inst_first[k + 1] = c->inst_first[k] + ms[k].n_elements;
     c->host_cur.assign(ninst, 0);
@@ -379,7 +439,7 @@ int upload_meshes(tlw_ctx* c, const tlw_mesh_desc* ms, int ninst, const tlw_curv
         const int f0 = c->inst_first[k], Ek = ms[k].n_elements;
         for (size_t q = 0; q < (size_t)4 * Ek; ++q) {
             const int nb = ms[k].neighbors_host[q];
-            nbr[4 * (size_t)f0 + q] = nb < 0 ? -1 : nb + f0;
+            nbr[4 * (size_t)f0 + q] = nb == TLW_NEIGHBOR_MORTAR ? mcode[q] : (nb < 0 ? -1 : nb + f0);
         }
         if (c->curved) {  // element data of a TreeMesh: unused, J⁻¹ of each element's first node
             std::memcpy(&invj[f0], cinvj.data(), sizeof(double) * Ek);
@@ -413,6 +473,14 @@ int upload_meshes(tlw_ctx* c, const tlw_mesh_desc* ms, int ninst, const tlw_curv
     HIPCHK(c, hipMalloc(&c->d_etile, sizeof(int) * (size_t)E));
     HIPCHK(c, hipMalloc(&c->d_rng, sizeof(int4) * (size_t)ninst));
     if (c->hconst.ns_walls) HIPCHK(c, hipMalloc(&c->d_fvb, sizeof(double) * (size_t)E * 4 * c->n * 2));
+    if (M > 0) {
+        std::vector<int2> mo(M);
+        for (int q = 0; q < M; ++q) mo[q] = make_int2(md->mortars_host[5 * q + 3], md->mortars_host[5 * q + 4]);
+        HIPCHK(c, hipMalloc(&c->d_mort, sizeof(int2) * (size_t)M));
+        HIPCHK(c, hipMalloc(&c->d_mrec, sizeof(double) * (size_t)M * 3 * c->n * 12));
+        HIPCHK(c, hipMalloc(&c->d_mflux, sizeof(double) * (size_t)M * 3 * c->n * 4));
+        HIPCHK(c, hipMemcpy(c->d_mort, mo.data(), sizeof(int2) * (size_t)M, hipMemcpyHostToDevice));
+    }
     if (c->sc) {
         HIPCHK(c, hipMalloc(&c->d_alpha_tmp, sizeof(double) * (size_t)E));
         HIPCHK(c, hipMalloc(&c->d_alpha, sizeof(double) * (size_t)E));
@@ -460,7 +528,7 @@ int upload_meshes(tlw_ctx* c, const tlw_mesh_desc* ms, int ninst, const tlw_curv
         std::vector<int> bidx(4 * (size_t)E, -1);
         long long nb = 0;
         for (size_t q = 0; q < 4 * (size_t)E; ++q)
-            if (nbr[q] < 0) bidx[q] = (int)nb++;
+            if (nbr[q] == -1) bidx[q] = (int)nb++;
         c->n_bnodes = nb * n;
         c->bxy.assign(2 * (size_t)c->n_bnodes, 0.0);
         for (int e = 0; e < E; ++e)
@@ -525,6 +593,10 @@ KArgs base_args(tlw_ctx* c) {
     a.bidx = c->d_bidx;
     a.btab = c->d_btab;
     a.btab_levels = c->btab_levels;
+    a.n_mortars = c->n_mortars;
+    a.mort = c->d_mort;
+    a.mrec = c->d_mrec;
+    a.mflux = c->d_mflux;
     return a;
 }
 
@@ -637,6 +709,11 @@ template <int N> void launch_surface_t(tlw_ctx* c, const KArgs& a) {
         // the curvilinear boundary faces read uⁿ: never launch without it
         c->bad_launch = true;
         return;
+    }
+    if (c->n_mortars > 0) {  // mortar fluxes from the volume kernel's face-node values
+        constexpr int MPB = kMortarBlock / (N + 1);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mortar<N>), dim3((c->n_mortars + MPB - 1) / MPB), dim3(kMortarBlock), 0,
+                           c->stream, a);
     }
     if (c->curved) {
         if (c->sc) {
@@ -870,11 +947,92 @@ KArgs lw_step_args(tlw_ctx* c) {
     return a;
 }
 
+// Lagrange interpolation matrix from nodes xin to points xout (barycentric
+// form; a point on an input node takes that node's value).
+void interp_matrix(const double* xin, int n, const double* xout, int m, double* out /* [m][n] */) {
+    double wb[8];
+    for (int k = 0; k < n; ++k) {
+        wb[k] = 1.0;
+        for (int q = 0; q < n; ++q)
+            if (q != k) wb[k] *= xin[k] - xin[q];
+        wb[k] = 1.0 / wb[k];
+    }
+    for (int a = 0; a < m; ++a) {
+        int hit = -1;
+        for (int k = 0; k < n; ++k)
+            if (std::fabs(xout[a] - xin[k]) < 1e-15) hit = k;
+        double den = 0.0;
+        for (int k = 0; k < n; ++k) {
+            out[a * n + k] = (hit >= 0) ? (k == hit ? 1.0 : 0.0) : wb[k] / (xout[a] - xin[k]);
+            den += out[a * n + k];
+        }
+        if (hit < 0)
+            for (int k = 0; k < n; ++k) out[a * n + k] /= den;
+    }
+}
+
+// Trixi's LobattoLegendreMortarL2 for the basis nodes: forward_lower/upper
+// interpolate a face polynomial to the GLL nodes of its lower/upper half;
+// reverse_lower/upper project the halves' fluxes back onto the face in L2, by
+// Gauss quadrature between GLL -> Gauss and Gauss -> GLL interpolations
+// (Trixi.jl src/solvers/dgsem/l2projection.jl, calc_reverse_*(.., Val(:gauss))).
+void mortar_operators(DevConst& h, int n) {
+    double g[8], wg[8];
+    for (int j = 0; j < n; ++j) {  // Gauss-Legendre nodes: Newton on P_n
+        double x = -std::cos(M_PI * (j + 0.75) / (n + 0.5)), dP = 1.0;
+        for (int it = 0; it < 100; ++it) {
+            double p0 = 1.0, p1 = x;
+            for (int k = 2; k <= n; ++k) {
+                const double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
+                p0 = p1;
+                p1 = p2;
+            }
+            dP = n * (x * p1 - p0) / (x * x - 1.0);
+            const double d = p1 / dP;
+            x -= d;
+            if (std::fabs(d) < 1e-16) break;
+        }
+        g[j] = x;
+        wg[j] = 2.0 / ((1.0 - x * x) * dP * dP);
+    }
+    double half[2][8], ghalf[2][8];
+    for (int i = 0; i < n; ++i) {
+        half[0][i] = 0.5 * (h.nodes[i] - 1.0);
+        half[1][i] = 0.5 * (h.nodes[i] + 1.0);
+        ghalf[0][i] = 0.5 * (g[i] - 1.0);
+        ghalf[1][i] = 0.5 * (g[i] + 1.0);
+    }
+    double l2g[64], g2l[64], fw[64], lg[64];
+    interp_matrix(h.nodes, n, g, n, l2g);
+    interp_matrix(g, n, h.nodes, n, g2l);
+    for (int s = 0; s < 2; ++s) {
+        interp_matrix(h.nodes, n, half[s], n, fw);
+        interp_matrix(g, n, ghalf[s], n, lg);  // lg[j][i] = ℓ_i(½(g_j ± 1))
+        double op[64], t[64];
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) op[i * n + j] = 0.5 * lg[j * n + i] * wg[j] / wg[i];
+        for (int i = 0; i < n; ++i)
+            for (int k = 0; k < n; ++k) {
+                double acc = 0.0;
+                for (int j = 0; j < n; ++j) acc += op[i * n + j] * l2g[j * n + k];
+                t[i * n + k] = acc;
+            }
+        for (int i = 0; i < n; ++i)
+            for (int k = 0; k < n; ++k) {
+                double acc = 0.0;
+                for (int j = 0; j < n; ++j) acc += g2l[i * n + j] * t[j * n + k];
+                h.mrev[s][i][k] = acc;
+                h.mfwd[s][i][k] = fw[i * n + k];
+            }
+    }
+}
+
 // message of the last failed context-free call on this thread (tlw_last_error(NULL))
 thread_local std::string g_noctx_err;
 
 int create_common(int ninst, const tlw_mesh_desc* meshes, const tlw_basis_desc* basis, const tlw_solver_desc* solver,
-                  tlw_ctx** out, const tlw_parabolic_desc* par = nullptr, const tlw_curved_desc* cv = nullptr);
+                  tlw_ctx** out, const tlw_parabolic_desc* par = nullptr, const tlw_curved_desc* cv = nullptr,
+                  const tlw_mortar_desc* md = nullptr);
 
 }  // namespace
 
@@ -934,6 +1092,53 @@ int tlw_rebuild_mesh(tlw_ctx* c, const tlw_mesh_desc* mesh) {
     if (c->curved) return fail(c, TLW_ENOTSUP, "a curvilinear context keeps its mesh (create a new one)");
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return upload_meshes(c, mesh, 1);
+}
+
+int tlw_create_nonconforming(const tlw_mesh_desc* mesh, const tlw_mortar_desc* mortars, const tlw_basis_desc* basis,
+                             const tlw_solver_desc* solver, tlw_ctx** out) {
+    if (!mortars) {
+        if (out) *out = nullptr;
+        return TLW_EINVAL;
+    }
+    return create_common(1, mesh, basis, solver, out, nullptr, nullptr, mortars);
+}
+
+int tlw_transfer_solution(tlw_ctx* c, const double* u_old_dev, int32_t n_old, double* u_new_dev, int32_t n_new,
+                          const int32_t* plan_host) {
+    if (!c || !u_old_dev || !u_new_dev || !plan_host || n_old <= 0 || n_new <= 0) return TLW_EINVAL;
+    if (u_old_dev == u_new_dev) return fail(c, TLW_EINVAL, "tlw_transfer_solution: the states must not alias");
+    for (int e = 0; e < n_new; ++e) {
+        const int32_t* p = plan_host + 5 * (size_t)e;
+        const bool ok = (p[0] == 0 && p[1] >= 0 && p[1] < n_old) ||
+                        (p[0] == 1 && p[1] >= 0 && p[1] < n_old && p[2] >= 0 && p[2] < 4) ||
+                        (p[0] == 2 && p[1] >= 0 && p[1] < n_old && p[2] >= 0 && p[2] < n_old && p[3] >= 0 &&
+                         p[3] < n_old && p[4] >= 0 && p[4] < n_old);
+        if (!ok) return fail(c, TLW_EINVAL, "tlw_transfer_solution: plan entry out of range");
+    }
+    int* d_plan = nullptr;
+    HIPCHK(c, hipMalloc(&d_plan, sizeof(int32_t) * 5 * (size_t)n_new));
+    hipError_t e1 = hipMemcpyAsync(d_plan, plan_host, sizeof(int32_t) * 5 * (size_t)n_new, hipMemcpyHostToDevice,
+                                   c->stream);
+    if (e1 == hipSuccess) {
+        const long long total = (long long)n_new * c->n * c->n;
+        const dim3 g((unsigned)((total + kTransferBlock - 1) / kTransferBlock)), b(kTransferBlock);
+        if (c->N == 4) hipLaunchKernelGGL(k_transfer<4>, g, b, 0, c->stream, u_old_dev, u_new_dev, d_plan, n_new, c->d_const);
+        else hipLaunchKernelGGL(k_transfer<3>, g, b, 0, c->stream, u_old_dev, u_new_dev, d_plan, n_new, c->d_const);
+        e1 = hipGetLastError();
+    }
+    const hipError_t e2 = hipStreamSynchronize(c->stream);
+    (void)hipFree(d_plan);
+    if (e1 != hipSuccess) return fail(c, TLW_EHIP, std::string("tlw_transfer_solution: ") + hipGetErrorString(e1));
+    if (e2 != hipSuccess) return fail(c, TLW_EHIP, std::string("tlw_transfer_solution: ") + hipGetErrorString(e2));
+    return TLW_OK;
+}
+
+int tlw_rebuild_mesh_nonconforming(tlw_ctx* c, const tlw_mesh_desc* mesh, const tlw_mortar_desc* mortars) {
+    if (!c || !mortars) return TLW_EINVAL;
+    if (c->ninst != 1) return fail(c, TLW_EINVAL, "tlw_rebuild_mesh_nonconforming needs a single-instance context");
+    if (c->curved) return fail(c, TLW_ENOTSUP, "a curvilinear context keeps its mesh (create a new one)");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return upload_meshes(c, mesh, 1, nullptr, mortars);
 }
 
 void tlw_destroy(tlw_ctx* c) {
@@ -1661,7 +1866,7 @@ void tlw_ensemble_destroy(tlw_ensemble* e) {
 namespace {
 
 int create_common(int ninst, const tlw_mesh_desc* meshes, const tlw_basis_desc* basis, const tlw_solver_desc* solver,
-                  tlw_ctx** out, const tlw_parabolic_desc* par, const tlw_curved_desc* cv) {
+                  tlw_ctx** out, const tlw_parabolic_desc* par, const tlw_curved_desc* cv, const tlw_mortar_desc* md) {
     if (!out) return TLW_EINVAL;
     *out = nullptr;
     if (!meshes || !basis || !solver) return TLW_EINVAL;
@@ -1698,6 +1903,7 @@ int create_common(int ninst, const tlw_mesh_desc* meshes, const tlw_basis_desc* 
     }
     h.bnd_lo = 1.0 / h.weights[0];
     h.bnd_hi = 1.0 / h.weights[n - 1];
+    mortar_operators(h, n);
     h.gamma = solver->gamma;
     for (int k = 0; k < 24; ++k) h.bc_par[k] = solver->boundary_params[k];
     h.bc_kind = solver->boundary_kind;
@@ -1795,7 +2001,7 @@ int create_common(int ninst, const tlw_mesh_desc* meshes, const tlw_basis_desc* 
     };
     if (hipMalloc(&c->d_const, sizeof(DevConst)) != hipSuccess) return bail(TLW_EHIP);
     if (hipMemcpy(c->d_const, &h, sizeof(DevConst), hipMemcpyHostToDevice) != hipSuccess) return bail(TLW_EHIP);
-    int rc = upload_meshes(c, meshes, ninst, cv);
+    int rc = upload_meshes(c, meshes, ninst, cv, md);
     if (rc) return bail(rc);
     // Trixi's alpha smoothing on a StructuredMesh assumes a periodic mesh
     if (c->curved && c->sc && h.alpha_smooth && c->has_boundary) return bail(TLW_ENOTSUP);

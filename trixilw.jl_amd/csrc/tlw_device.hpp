@@ -90,6 +90,10 @@ struct DevConst {
     int recon;  // TLW_RECON_*
     double mh_dx1[8], mh_dx2[8], mh_fa[8], mh_fb[8], mh_fc[8], mh_xl[8], mh_xr[8], mh_hw[8];
     int src_kind;  // SrcKind
+    // L2 mortar operators of a TreeMesh with refined cells (LobattoLegendreMortarL2),
+    // row-major [output node][input node]: [0] lower half, [1] upper half
+    double mfwd[2][8][8];  // large face -> the half's GLL nodes
+    double mrev[2][8][8];  // the half's fluxes -> large face (L2 projection)
 };
 
 // Device-resident integrator state (LWIntegrator fields t, dt, iter and the
@@ -188,6 +192,15 @@ struct KArgs {
     int step_mode;     // 1: out = u + dt * rhs; 0: out = rhs
     int reduce;        // Reduce
     double bc_scale;   // scaling_factor of the Dirichlet time average
+    // L2 mortars of a TreeMesh with refined cells: a face on mortar m in role r
+    // (0 lower, 1 upper small element, 2 large element) has neighbour
+    // -2 - (3 m + r).  Per mortar (orientation 1|2, large_side 1|2); per role and
+    // face node the volume kernel's (jump state, normal flux, λ state) [M][3][n][12]
+    // and the mortar kernel's fluxes [M][3][n][4]
+    int n_mortars;
+    const int2* mort;
+    double* mrec;
+    double* mflux;
 };
 
 // ---------------------------------------------------------------------------
@@ -744,6 +757,19 @@ __device__ __forceinline__ void face_numflux_sc_curved(const double* L, const do
 
 // Face record of a face node: jump state, normal component of F̃ and the λ
 // ingredients |v_n| and c of the λ state (TreeMesh order: λ from Ũ, jump on uⁿ).
+// A face node on a mortar (neighbour code -2 - (3 m + role)): the jump state,
+// the normal flux and the λ state the mortar kernel interpolates or pairs.
+__device__ __forceinline__ void mortar_side(const KArgs& a, int code, int n, int l, const double* jump,
+                                            const double* Fn, const double* lam) {
+    double* q = a.mrec + ((size_t)(-2 - code) * n + l) * 12;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        q[v] = jump[v];
+        q[4 + v] = Fn[v];
+        q[8 + v] = lam[v];
+    }
+}
+
 __device__ __forceinline__ void make_record(double* r, const double* jump, const double* Fn, double vn_abs, double c) {
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
@@ -2493,12 +2519,14 @@ __device__ __forceinline__ void volume_body(const KArgs& a, int bid, double* lds
             make_record(r, jump, F, av1, cs);
             fill(r, fnlx);
             store_record<RQ>(rec + ((lec * 4 + fx) * n + j) * RQ, r);
+            if (nbx <= -2) mortar_side(a, nbx, n, j, jump, F, lamst);
         }
         if (act && fy >= 0) {
             double r[RQ];
             make_record(r, jump, G, av2, cs);
             fill(r, fnly);
             store_record<RQ>(rec + ((lec * 4 + fy) * n + i) * RQ, r);
+            if (nby <= -2) mortar_side(a, nby, n, i, jump, G, lamst);
         }
     }
     __syncthreads();
@@ -2752,6 +2780,9 @@ __device__ __forceinline__ void surface_body(const KArgs& a, int bid) {
                 } else {
                     face_numflux(Lr, Rr, Fs);
                 }
+            } else if (nb <= -2) {
+                // a face on an L2 mortar: the mortar kernel's flux of this element's role
+                load4(a.mflux + ((size_t)(-2 - nb) * n + l) * 4, Fs);
             } else {
                 // domain boundary: time-averaged Dirichlet state, no blending
                 // (calc_boundary_flux!, dg_2d.jl:1379-1454; structured/dg_2d.jl:2297-2391)
@@ -2909,6 +2940,114 @@ __device__ __forceinline__ void surface_body(const KArgs& a, int bid) {
 template <int N, bool BC, bool SC, bool CURV = false>
 __global__ __launch_bounds__(kSurfBlock) void k_surface(KArgs a) {
     surface_body<N, BC, SC, CURV>(a, blockIdx.x);
+}
+
+// AMR solution transfer between two TreeMesh leaf sets (Trixi's refine! and
+// coarsen! of the DG solution, src/callbacks_step/amr_dg2d.jl in Trixi.jl,
+// which the reference's AMR callback runs before resizing its caches,
+// src/callbacks/amr.jl:2-20): one lane per node of the new mesh.  Plan
+// [E_new][5]: (0, e) copies leaf e; (1, p, child) interpolates parent p to its
+// child (0 lower left, 1 lower right, 2 upper left, 3 upper right) with the
+// forward mortar operators in x, then y; (2, c0, c1, c2, c3) projects four
+// children onto their parent with the reverse operators, summed over children.
+constexpr int kTransferBlock = 256;
+template <int N>
+__global__ __launch_bounds__(kTransferBlock) void k_transfer(const double* __restrict__ uold, double* __restrict__ unew,
+                                                            const int* __restrict__ plan, int E_new, const DevConst* c) {
+    constexpr int n = N + 1, NN = n * n;
+    const long long gid = (long long)blockIdx.x * kTransferBlock + threadIdx.x;
+    if (gid >= (long long)E_new * NN) return;
+    const int e = (int)(gid / NN), node = (int)(gid - (long long)e * NN);
+    const int i = node % n, j = node / n;
+    const int* p = plan + 5 * (size_t)e;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    if (p[0] == 0) {
+        load4(uold + ((size_t)p[1] * NN + node) * 4, acc);
+    } else {
+        const int nsrc = (p[0] == 1) ? 1 : 4;
+        for (int q = 0; q < nsrc; ++q) {
+            const int src = (p[0] == 1) ? p[1] : p[1 + q];
+            const int child = (p[0] == 1) ? p[2] : q;
+            const double* Mx = (p[0] == 1) ? c->mfwd[child & 1][i] : c->mrev[child & 1][i];
+            const double* My = (p[0] == 1) ? c->mfwd[child >> 1][j] : c->mrev[child >> 1][j];
+            const double* us = uold + (size_t)src * NN * 4;
+            for (int l = 0; l < n; ++l) {
+                double t[4] = {0.0, 0.0, 0.0, 0.0};
+                for (int k = 0; k < n; ++k) {
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) t[v] = fma(Mx[k], us[((size_t)l * n + k) * 4 + v], t[v]);
+                }
+#pragma unroll
+                for (int v = 0; v < 4; ++v) acc[v] = fma(My[l], t[v], acc[v]);
+            }
+        }
+    }
+    store4(unew + ((size_t)e * NN + node) * 4, acc);
+}
+
+// L2 mortars of a TreeMesh with refined cells (prolong2mortars! and
+// calc_mortar_flux!, dg_2d.jl:1456-1646, with Trixi's
+// mortar_fluxes_to_elements!): one lane per mortar and face node.  On each
+// half the large element's jump state, normal flux and λ state are
+// interpolated to the node (forward operator of the half) and paired with
+// the small element's, the LW flux taken in the interface order (the large
+// element on the left when large_side is 1) and kept for the small element;
+// the large element's flux at the node is reverse_upper·F*_upper +
+// reverse_lower·F*_lower.  The surface kernel lifts both.
+constexpr int kMortarBlock = 64;
+template <int N>
+__global__ __launch_bounds__(kMortarBlock) void k_mortar(KArgs a) {
+    constexpr int n = N + 1, MPB = kMortarBlock / n;
+    __shared__ double fs[MPB][2][n][4];
+    if (a.use_state_dt && a.st[0].finished) return;
+    const DevConst* c = a.c;
+    const int tid = threadIdx.x, lm = tid / n, l = tid - lm * n;
+    const int m = blockIdx.x * MPB + lm;
+    const bool act = lm < MPB && m < a.n_mortars;
+    if (act) {
+        const int2 mo = a.mort[m];
+        const double* base = a.mrec + (size_t)m * 3 * n * 12;
+        const double* lg = base + 2 * n * 12;
+        for (int half = 0; half < 2; ++half) {
+            double jl[4] = {0, 0, 0, 0}, Fl[4] = {0, 0, 0, 0}, Ul[4] = {0, 0, 0, 0};
+#pragma unroll
+            for (int k = 0; k < n; ++k) {
+                const double p = c->mfwd[half][l][k];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    jl[v] = fma(p, lg[k * 12 + v], jl[v]);
+                    Fl[v] = fma(p, lg[k * 12 + 4 + v], Fl[v]);
+                    Ul[v] = fma(p, lg[k * 12 + 8 + v], Ul[v]);
+                }
+            }
+            const double* sm = base + ((size_t)half * n + l) * 12;
+            double rL[kTraceQ], rS[kTraceQ], Fs[4];
+            double aL, cL, aS, cS;
+            speed_parts(Ul, c->gamma, mo.x, aL, cL);
+            speed_parts(sm + 8, c->gamma, mo.x, aS, cS);
+            make_record(rL, jl, Fl, aL, cL);
+            make_record(rS, sm, sm + 4, aS, cS);
+            if (mo.y == 1) face_numflux(rL, rS, Fs);
+            else face_numflux(rS, rL, Fs);
+            store4(a.mflux + (((size_t)m * 3 + half) * n + l) * 4, Fs);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) fs[lm][half][l][v] = Fs[v];
+        }
+    }
+    __syncthreads();
+    if (act) {
+        double Fb[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            double acc = 0.0;
+#pragma unroll
+            for (int k = 0; k < n; ++k) acc = fma(c->mrev[1][l][k], fs[lm][1][k][v], acc);
+#pragma unroll
+            for (int k = 0; k < n; ++k) acc = fma(c->mrev[0][l][k], fs[lm][0][k][v], acc);
+            Fb[v] = acc;
+        }
+        store4(a.mflux + (((size_t)m * 3 + 2) * n + l) * 4, Fb);
+    }
 }
 
 // CFL reduction over a whole state (Trixi max_dt) without an update: first dt

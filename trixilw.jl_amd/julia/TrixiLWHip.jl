@@ -57,6 +57,14 @@ struct MeshDesc
     element_length_host::Ptr{Float64}     # [E]
 end
 
+# TreeMesh with refined cells: Trixi's mortar container (cache.mortars), 0-based
+# {lower, upper, large, orientation, large_side} per mortar; mortar faces carry neighbour -2
+const TLW_NEIGHBOR_MORTAR = Int32(-2)
+struct MortarDesc
+    n_mortars::Int32
+    mortars_host::Ptr{Int32}              # [5, M]
+end
+
 # LobattoLegendreBasis fields, column-major as Julia stores them
 struct BasisDesc
     polydeg::Int32
@@ -183,6 +191,23 @@ end
 # AMR: after DiffEqBase.resize! (src/callbacks/amr.jl:2-20) re-upload the host-built mesh
 rebuild_mesh!(ctx, mesh::MeshDesc) =
     check(ctx, ccall((:tlw_rebuild_mesh, lib), Cint, (Ptr{Cvoid}, Ref{MeshDesc}), ctx, mesh))
+
+# TreeMesh with refined cells (mortars), and its AMR re-upload after resize!
+function create_nonconforming(mesh::MeshDesc, mortars::MortarDesc, basis::BasisDesc, solver::SolverDesc)
+    ctx = Ref{Ptr{Cvoid}}(C_NULL)
+    check(C_NULL, ccall((:tlw_create_nonconforming, lib), Cint,
+                        (Ref{MeshDesc}, Ref{MortarDesc}, Ref{BasisDesc}, Ref{SolverDesc}, Ref{Ptr{Cvoid}}),
+                        mesh, mortars, basis, solver, ctx))
+    return ctx[]
+end
+rebuild_mesh_nonconforming!(ctx, mesh::MeshDesc, mortars::MortarDesc) =
+    check(ctx, ccall((:tlw_rebuild_mesh_nonconforming, lib), Cint, (Ptr{Cvoid}, Ref{MeshDesc}, Ref{MortarDesc}),
+                     ctx, mesh, mortars))
+# Trixi's refine!/coarsen! of the solution on the device: plan [5, E_new] (see the header)
+transfer_solution!(ctx, u_old_dev::Ptr{Float64}, n_old, u_new_dev::Ptr{Float64}, n_new, plan::Matrix{Int32}) =
+    check(ctx, ccall((:tlw_transfer_solution, lib), Cint,
+                     (Ptr{Cvoid}, Ptr{Float64}, Int32, Ptr{Float64}, Int32, Ptr{Int32}),
+                     ctx, u_old_dev, n_old, u_new_dev, n_new, plan))
 destroy!(ctx) = ccall((:tlw_destroy, lib), Cvoid, (Ptr{Cvoid},), ctx)
 set_stream!(ctx, stream::Ptr{Cvoid}) =
     check(ctx, ccall((:tlw_set_stream, lib), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), ctx, stream))

@@ -36,10 +36,21 @@ EXPORTED = (
     "tlw_set_state_dev", "tlw_get_state_dev", "tlw_boundary_nodes", "tlw_set_boundary_data",
     "tlw_create_ensemble", "tlw_ensemble_set_stream", "tlw_ensemble_enable_graphs", "tlw_ensemble_solve_init",
     "tlw_ensemble_advance", "tlw_ensemble_last_error", "tlw_ensemble_destroy", "tlw_adjoint_mdrk",
+    "tlw_create_nonconforming", "tlw_rebuild_mesh_nonconforming", "tlw_transfer_solution",
 )
 
 
 TLW_RECON_FIRST_ORDER, TLW_RECON_MUSCL_HANCOCK = 0, 1
+
+
+TLW_NEIGHBOR_MORTAR = -2
+
+
+class MortarDesc(C.Structure):
+    _fields_ = [
+        ("n_mortars", C.c_int32),
+        ("mortars_host", C.POINTER(C.c_int32)),
+    ]
 
 
 class MeshDesc(C.Structure):
@@ -140,6 +151,13 @@ def lib():
                                               C.POINTER(CurvedDesc), C.POINTER(ParabolicDesc), C.POINTER(vp)]
     L.tlw_rebuild_mesh.restype = C.c_int
     L.tlw_rebuild_mesh.argtypes = [vp, C.POINTER(MeshDesc)]
+    L.tlw_create_nonconforming.restype = C.c_int
+    L.tlw_create_nonconforming.argtypes = [C.POINTER(MeshDesc), C.POINTER(MortarDesc), C.POINTER(BasisDesc),
+                                           C.POINTER(SolverDesc), C.POINTER(vp)]
+    L.tlw_rebuild_mesh_nonconforming.restype = C.c_int
+    L.tlw_rebuild_mesh_nonconforming.argtypes = [vp, C.POINTER(MeshDesc), C.POINTER(MortarDesc)]
+    L.tlw_transfer_solution.restype = C.c_int
+    L.tlw_transfer_solution.argtypes = [vp, vp, C.c_int32, vp, C.c_int32, C.POINTER(C.c_int32)]
     L.tlw_destroy.restype = None
     L.tlw_destroy.argtypes = [vp]
     L.tlw_last_error.restype = C.c_char_p

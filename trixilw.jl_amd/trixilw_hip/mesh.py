@@ -1,4 +1,4 @@
-"""Uniformly refined 2-D TreeMesh and curvilinear StructuredMesh on the host.
+"""2-D TreeMesh (uniform or with refined cells) and curvilinear StructuredMesh on the host.
 
 Mirrors what TrixiLW reads from Trixi's TreeMesh and its containers
 (`cache.elements.inverse_jacobian`, `node_coordinates`, `cache.interfaces`,
@@ -9,6 +9,12 @@ leaves in Z-order like Trixi's uniformly refined tree, ``"lexicographic"``
 uses e = ix + nx*iy, and ``"strips"`` walks two-row strips column by column
 (runs of 10 elements are 5×2 blocks: the volume kernel's 10-element tiles then
 have fewer faces that leave the tile, so fewer face records cross HBM).
+
+A TreeMesh with ``refinement_patches`` (or one returned by ``refine`` /
+``coarsen``, the AMR operations) holds leaves of several levels, 2:1 across
+faces, numbered in Z-order; faces between levels are L2 mortars
+(``mortars``: lower, upper, large element, orientation, large side) and carry
+the neighbour code TLW_NEIGHBOR_MORTAR (-2).
 """
 from __future__ import annotations
 
@@ -24,10 +30,17 @@ def _morton_index(ix, iy, level):
 
 
 class TreeMesh:
-    """TreeMesh(coordinates_min, coordinates_max, initial_refinement_level, periodicity)."""
+    """TreeMesh(coordinates_min, coordinates_max, initial_refinement_level, periodicity;
+    refinement_patches): Trixi's TreeMesh constructor with its box patches
+    (``{"type": "box", "coordinates_min": ..., "coordinates_max": ...}``): each
+    patch refines once every leaf whose centre lies strictly inside the box
+    (Trixi refine_box!), and the tree is then refined further until leaves
+    sharing a face differ by at most one level (Trixi's rebalance)."""
+
+    mortars = np.zeros((0, 5), dtype=np.int32)
 
     def __init__(self, coordinates_min, coordinates_max, initial_refinement_level, periodicity=True,
-                 ordering="morton"):
+                 ordering="morton", refinement_patches=()):
         xmin, ymin = map(float, coordinates_min)
         xmax, ymax = map(float, coordinates_max)
         length = xmax - xmin
@@ -39,6 +52,17 @@ class TreeMesh:
         self.level = int(initial_refinement_level)
         self.coordinates_min = (xmin, ymin)
         self.coordinates_max = (xmax, ymax)
+        self.length = length
+        if refinement_patches:
+            leaves = {(self.level, i, j) for i in range(1 << self.level) for j in range(1 << self.level)}
+            for patch in refinement_patches:
+                if patch.get("type", "box") != "box":
+                    raise NotImplementedError("refinement patches of type 'box'")
+                (bx0, by0), (bx1, by1) = patch["coordinates_min"], patch["coordinates_max"]
+                hit = [c for c in leaves if bx0 < self._centre(c)[0] < bx1 and by0 < self._centre(c)[1] < by1]
+                leaves = self._balanced(self._split(leaves, hit))
+            self._set_leaves(leaves)
+            return
         nx = 1 << self.level
         self.nx = self.ny = nx
         self.h = length / nx
@@ -81,6 +105,159 @@ class TreeMesh:
             nbr[:, face] = np.where(ok, self.element_of_cell[cell], -1)
         self.neighbors = nbr
 
+    # ------------------------------------------------------------------ refined cells
+    def _centre(self, c):
+        lev, i, j = c
+        h = self.length / (1 << lev)
+        return self.coordinates_min[0] + (i + 0.5) * h, self.coordinates_min[1] + (j + 0.5) * h
+
+    @staticmethod
+    def _split(leaves, cells):
+        out = set(leaves)
+        for lev, i, j in cells:
+            out.remove((lev, i, j))
+            out |= {(lev + 1, 2 * i + a, 2 * j + b) for a in (0, 1) for b in (0, 1)}
+        return out
+
+    def _wrap(self, lev, i, j):
+        """The cell (lev, i, j) moved into the domain by periodicity, or None outside it."""
+        n = 1 << lev
+        if not 0 <= i < n:
+            if not self.periodicity[0]:
+                return None
+            i %= n
+        if not 0 <= j < n:
+            if not self.periodicity[1]:
+                return None
+            j %= n
+        return lev, i, j
+
+    @staticmethod
+    def _covering(leaves, c):
+        """The leaf containing cell c (c itself or an ancestor), or None if c is subdivided."""
+        lev, i, j = c
+        while lev >= 0:
+            if (lev, i, j) in leaves:
+                return lev, i, j
+            lev, i, j = lev - 1, i >> 1, j >> 1
+        return None
+
+    def _balanced(self, leaves):
+        """Split leaves until face neighbours differ by at most one level."""
+        leaves = set(leaves)
+        while True:
+            coarse = set()
+            for lev, i, j in leaves:
+                for di, dj in ((-1, 0), (1, 0), (0, -1), (0, 1)):
+                    c = self._wrap(lev, i + di, j + dj)
+                    if c is None:
+                        continue
+                    cov = self._covering(leaves, c)
+                    if cov is not None and cov[0] < lev - 1:
+                        coarse.add(cov)
+            if not coarse:
+                return leaves
+            leaves = self._split(leaves, coarse)
+
+    def _set_leaves(self, leaves):
+        """Element arrays, face neighbours and mortars of a set of leaves (lev, i, j)."""
+        leaves = set(leaves)
+        arr = np.array(sorted(leaves), dtype=np.int64)
+        top = int(arr[:, 0].max())
+        z = _morton_index(arr[:, 1] << (top - arr[:, 0]), arr[:, 2] << (top - arr[:, 0]), top)
+        arr = arr[np.lexsort((arr[:, 0], z))]  # Z-order of the leaves' lower-left corners
+        order = [tuple(int(q) for q in c) for c in arr]
+        self.leaves = arr
+        self.ordering = "morton"
+        self.max_level = top
+        E = len(order)
+        self.n_elements = E
+        index = {c: e for e, c in enumerate(order)}
+        h = self.length / (1 << self.leaves[:, 0])
+        self.element_length = h.astype(np.float64)
+        self.element_center = np.stack([self.coordinates_min[0] + (self.leaves[:, 1] + 0.5) * h,
+                                        self.coordinates_min[1] + (self.leaves[:, 2] + 0.5) * h], axis=1)
+        self.inverse_jacobian = 2.0 / self.element_length
+        nbr = np.full((E, 4), -1, dtype=np.int32)
+        mortars = []
+        for e, (lev, i, j) in enumerate(order):
+            for f, (di, dj) in enumerate(((-1, 0), (1, 0), (0, -1), (0, 1))):
+                c = self._wrap(lev, i + di, j + dj)
+                if c is None:
+                    continue
+                if c in index:
+                    nbr[e, f] = index[c]
+                    continue
+                cov = self._covering(leaves, c)
+                if cov is not None:  # a coarser neighbour: the small side of its mortar
+                    if cov[0] != lev - 1:
+                        raise ValueError("TreeMesh leaves are not 2:1 balanced")
+                    nbr[e, f] = -2
+                    continue
+                # two finer neighbours: this element is the large side of a mortar
+                _, ci, cj = c
+                a = 1 if di < 0 else 0           # column of the neighbour's children next to the face
+                b = 1 if dj < 0 else 0           # row of them
+                if di != 0:
+                    kids = [(lev + 1, 2 * ci + a, 2 * cj + q) for q in (0, 1)]
+                else:
+                    kids = [(lev + 1, 2 * ci + q, 2 * cj + b) for q in (0, 1)]
+                if not all(k in index for k in kids):
+                    raise ValueError("TreeMesh leaves are not 2:1 balanced")
+                nbr[e, f] = -2
+                mortars.append((index[kids[0]], index[kids[1]], e, 1 if di != 0 else 2, 1 if (di > 0 or dj > 0) else 2))
+        self.neighbors = nbr
+        self.mortars = np.array(mortars, dtype=np.int32).reshape(-1, 5)
+        self.nx = self.ny = None
+
+    @property
+    def refined(self):
+        return len(self.mortars) > 0 or hasattr(self, "leaves")
+
+    def _leaf_set(self):
+        if hasattr(self, "leaves"):
+            return {tuple(int(q) for q in c) for c in self.leaves}
+        return {(self.level, int(i), int(j)) for i, j in zip(self.ix, self.iy)}
+
+    def _from_leaves(self, leaves):
+        m = object.__new__(TreeMesh)
+        m.periodicity, m.level = self.periodicity, self.level
+        m.coordinates_min, m.coordinates_max, m.length = self.coordinates_min, self.coordinates_max, self.length
+        m._set_leaves(leaves)
+        return m
+
+    def refine(self, elements):
+        """AMR refinement (Trixi refine!): the TreeMesh with these elements split
+        in four, then rebalanced, and the plan that carries a solution over
+        (see ``transfer_plan``)."""
+        old = [tuple(int(q) for q in c) for c in self._leaf_list()]
+        new = self._from_leaves(self._balanced(self._split(set(old), {old[int(e)] for e in elements})))
+        return new, transfer_plan(old, new)
+
+    def coarsen(self, elements):
+        """AMR coarsening (Trixi coarsen!): every set of four sibling leaves that are
+        all listed becomes their parent, unless that breaks the 2:1 balance; the
+        TreeMesh and the plan that carries a solution over."""
+        old = [tuple(int(q) for q in c) for c in self._leaf_list()]
+        leaves = set(old)
+        marked = {old[int(e)] for e in elements}
+        parents = {(lev - 1, i >> 1, j >> 1) for lev, i, j in marked if lev > 0}
+        for p in sorted(parents):
+            lev, i, j = p
+            kids = {(lev + 1, 2 * i + a, 2 * j + b) for a in (0, 1) for b in (0, 1)}
+            if not kids <= marked or not kids <= leaves:
+                continue
+            trial = (leaves - kids) | {p}
+            if self._balanced(trial) == trial:
+                leaves = trial
+        new = self._from_leaves(leaves)
+        return new, transfer_plan(old, new)
+
+    def _leaf_list(self):
+        if hasattr(self, "leaves"):
+            return [tuple(int(q) for q in c) for c in self.leaves]
+        return [(self.level, int(i), int(j)) for i, j in zip(self.ix, self.iy)]
+
     @property
     def ncells_per_dimension(self):
         return self.nx
@@ -95,7 +272,30 @@ class TreeMesh:
 
     def lexicographic_permutation(self):
         """perm such that u_lex = u[perm] maps this element order to e = ix + nx*iy."""
+        if hasattr(self, "leaves"):
+            raise ValueError("a TreeMesh with refined cells has no lexicographic order")
         return self.element_of_cell.copy()
+
+
+def transfer_plan(old_leaves, new_mesh):
+    """Per element of new_mesh: (0, e_old, 0, 0, 0) for a kept leaf, (1, parent,
+    child, 0, 0) for a child of a refined leaf (child 0 lower left, 1 lower right,
+    2 upper left, 3 upper right) and (2, c0, c1, c2, c3) for the parent of four
+    coarsened leaves (same child order); the layout tlw_transfer_solution takes."""
+    index = {c: e for e, c in enumerate(old_leaves)}
+    plan = np.zeros((new_mesh.n_elements, 5), dtype=np.int32)
+    for e, c in enumerate(new_mesh.leaves):
+        lev, i, j = (int(q) for q in c)
+        if (lev, i, j) in index:
+            plan[e] = (0, index[(lev, i, j)], 0, 0, 0)
+        elif (lev - 1, i >> 1, j >> 1) in index:
+            plan[e] = (1, index[(lev - 1, i >> 1, j >> 1)], (i & 1) + 2 * (j & 1), 0, 0)
+        else:
+            kids = [(lev + 1, 2 * i + a, 2 * j + b) for b in (0, 1) for a in (0, 1)]
+            if not all(k in index for k in kids):
+                raise ValueError("a new leaf is neither kept, a child nor a parent of old leaves")
+            plan[e] = (2, *[index[k] for k in kids])
+    return plan
 
 
 def coordinates2mapping(coordinates_min, coordinates_max):

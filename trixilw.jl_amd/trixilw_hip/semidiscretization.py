@@ -275,9 +275,22 @@ class SemidiscretizationHyperbolic:
                                         "faces, a boundary value function required; first-order shock "
                                         "capturing only)")
             return
+        if getattr(self.mesh, "refined", False):
+            mo = self._mortar_desc()
+            rc = _lib.lib().tlw_create_nonconforming(C.byref(md), C.byref(mo), C.byref(basis), C.byref(sd),
+                                                     C.byref(h))
+            if rc != _lib.TLW_OK:
+                raise _lib.TLWError(rc, "tlw_create_nonconforming failed (mortars: Euler, VolumeIntegralFR, "
+                                        "a valid 2:1 mortar list)")
+            return
         rc = _lib.lib().tlw_create(C.byref(md), C.byref(basis), C.byref(sd), C.byref(h))
         if rc != _lib.TLW_OK:
             raise _lib.TLWError(rc, "tlw_create failed (invalid description or HIP error)")
+
+    def _mortar_desc(self):
+        self._keep["mortars"] = np.ascontiguousarray(self.mesh.mortars, dtype=np.int32).reshape(-1)
+        return _lib.MortarDesc(int(len(self.mesh.mortars)),
+                               self._keep["mortars"].ctypes.data_as(C.POINTER(C.c_int32)))
 
     def _mesh_arrays(self, mesh):
         if getattr(mesh, "curved", False):
@@ -306,7 +319,33 @@ class SemidiscretizationHyperbolic:
         self.mesh = mesh
         self._keep = self._mesh_arrays(mesh)
         md = self._mesh_desc()
-        _lib.check(_lib.lib().tlw_rebuild_mesh(self._h, C.byref(md)), self._h)
+        if getattr(mesh, "refined", False):
+            mo = self._mortar_desc()
+            _lib.check(_lib.lib().tlw_rebuild_mesh_nonconforming(self._h, C.byref(md), C.byref(mo)), self._h)
+        else:
+            _lib.check(_lib.lib().tlw_rebuild_mesh(self._h, C.byref(md)), self._h)
+
+    def adapt(self, u, new_mesh, plan):
+        """One AMR step of the reference's callback (Trixi's refine!/coarsen! of the
+        solution, then src/callbacks/amr.jl:2-208 resize!): the state u (device
+        tensor of the current mesh) carried to new_mesh by ``plan`` (as
+        TreeMesh.refine/coarsen return it) on the device, then the mesh
+        re-uploaded.  Returns the new state tensor."""
+        import torch
+
+        plan = np.ascontiguousarray(plan, dtype=np.int32)
+        n = self.solver.basis.nnodes()
+        old_E = self.mesh.n_elements
+        if tuple(u.shape) != (old_E, n, n, 4) or not u.is_cuda or u.dtype != torch.float64:
+            raise ValueError(f"u must be a float64 device tensor of shape {(old_E, n, n, 4)}")
+        u = u.contiguous()
+        unew = torch.empty((new_mesh.n_elements, n, n, 4), dtype=torch.float64, device=u.device)
+        torch.cuda.synchronize()
+        _lib.check(_lib.lib().tlw_transfer_solution(self._h, C.c_void_p(u.data_ptr()), int(old_E),
+                                                    C.c_void_p(unew.data_ptr()), int(new_mesh.n_elements),
+                                                    plan.ctypes.data_as(C.POINTER(C.c_int32))), self._h)
+        self.rebuild_mesh(new_mesh)
+        return unew
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
