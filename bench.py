@@ -32,6 +32,9 @@ WORKLOADS = {
     "lw_euler_vortex_256x256_N4": dict(level=8, N=4, td="lw"),
     "lw_euler_vortex_32x32_N4": dict(level=5, N=4, td="lw"),
     "mdrk_euler_vortex_1024x1024_N3": dict(level=10, N=3, td="mdrk"),
+    # 512² periodic vortex mesh whose central quarter [-5,5]^2 is refined once (458752 leaves,
+    # 1024 mortars): the mortar path at scale
+    "lw_euler_vortex_refined_512x512_box_N4": dict(level=9, N=4, td="lw", problem="vortex_refined"),
     # four-quadrant Riemann problem (Lax-Liu configuration 3) on [0,1]^2 with a time-averaged
     # Dirichlet boundary, Hennemann-Gassner indicator on density_pressure (alpha_max 0.5) and
     # first-order subcell FV blending; CFL 0.3 (the 0.98·LW limit loses positivity within 500 steps
@@ -314,6 +317,11 @@ def make_problem(wl):
         mesh = tlw.StructuredMesh((wl["nx"], wl["nx"]), warp_mapping(WARP_L, WARP_A), ordering="strips")
     elif problem == "ns_warped":
         mesh = tlw.StructuredMesh((wl["nx"], wl["nx"]), warp_mapping(CNS_L, CNS_A), ordering="strips")
+    elif problem == "vortex_refined":
+        box = {"type": "box", "coordinates_min": (-5.0, -5.0), "coordinates_max": (5.0, 5.0)}
+        mesh = tlw.TreeMesh((lo, lo), (hi, hi), wl["level"], periodicity=periodic, refinement_patches=(box,))
+        okw["tree"] = (mesh.element_center[:, 0], mesh.element_center[:, 1], mesh.element_length, mesh.neighbors,
+                       mesh.mortars)
     else:
         mesh = tlw.TreeMesh((lo, lo), (hi, hi), wl["level"], periodicity=periodic, ordering="strips")
     solver = tlw.DGSEM(N, volume_integral=vi)
@@ -562,6 +570,42 @@ def measure_mdrk_euler(steps, cpu_steps, threads, with_cpu=True):
     return out
 
 
+def measure_refined(steps, cpu_steps, threads, with_cpu=True, uniform_value=None):
+    """Sub-record: LW N=4 Euler on a TreeMesh with refined cells (the 512²
+    vortex mesh with its central quarter refined once; L2 mortars at the
+    refinement edge): device-loop throughput, kernel times (the surface time
+    includes the mortar kernel) and the CPU oracle on the same leaves."""
+    name = "lw_euler_vortex_refined_512x512_box_N4"
+    wl = WORKLOADS[name]
+    semi, mesh, ic, cfl, (lo, hi), okw = make_problem(wl)
+    x, y = semi.node_coordinates()
+    u0 = np.ascontiguousarray(ic(x, y, 0.0))
+    r = run_steps(semi, u0, cfl, steps, 2)
+    dofs = semi.ndofs()
+    out = {"workload": name, "value": r["value"], "unit": "DOF-updates/s", "ms_per_step": round(r["ms_per_step"], 4),
+           "steps": steps, "dofs": dofs, "elements": int(mesh.n_elements), "mortars": int(len(mesh.mortars)),
+           "volume_avg_launch_ms": round(r["vol_ms"], 4), "surface_and_mortar_avg_launch_ms": round(r["surf_ms"], 4),
+           "elements_per_workgroup": r["tile"], "tile_autotune_ms": r["tune"],
+           "config": "isentropic vortex strength 5, Mach 0.5, periodic [-10,10]^2 TreeMesh 512x512 with the "
+                     f"leaves of [-5,5]^2 refined once (Z-order), polydeg 4, LW, CFL {cfl}"}
+    if uniform_value:
+        out["vs_uniform_1024x1024"] = round(r["value"] / uniform_value, 4)
+    log(f"bench: refined TreeMesh {r['value'] / 1e9:.3f} G DOF-updates/s, {r['ms_per_step']:.3f} ms/step, "
+        f"{len(mesh.mortars)} mortars")
+    semi.close()
+    if with_cpu:
+        import oracle_ctypes
+
+        o = oracle_ctypes.Oracle(4, 1, 1, 0.0, 0.0, 1.0, **okw)
+        xo, yo = o.node_coords()
+        v, el = cpu_baseline(o, np.ascontiguousarray(ic(xo, yo, 0.0)), cfl, False, dofs, cpu_steps, threads)
+        out["cpu_baseline"] = {"value": v, "unit": "DOF-updates/s", "cores": threads, "kind": "port",
+                               "sample": f"{cpu_steps} step(s) of the same leaves, C++/OpenMP oracle"}
+        out["gpu_over_cpu"] = round(r["value"] / v, 2)
+        del o
+    return out
+
+
 def measure_curved_navier_stokes(steps, cpu_steps, threads, with_cpu=True):
     """Sub-record: MDRK N=3 compressible Navier-Stokes on the warped 128²
     StructuredMesh (warp amplitude 0.1, Reynolds number 1000, Pr 0.72) for
@@ -760,7 +804,7 @@ def main():
     ap.add_argument("--no-curved", action="store_true", help="skip the curvilinear-mesh sub-records")
     ap.add_argument("--no-sc", action="store_true", help="skip the shock-capturing (MUSCL-Hancock) sub-record")
     ap.add_argument("--no-extra", action="store_true",
-                    help="skip the config-0 (32^2 to t = 1) and MDRK N=3 Euler sub-records")
+                    help="skip the config-0 (32^2 to t = 1), MDRK N=3 Euler and refined-TreeMesh sub-records")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device copy (the reference kernel for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -910,17 +954,20 @@ def main():
         curved_rec = measure_curved(max(3, min(args.steps, 10)), 1, threads, with_cpu=not args.no_cpu_baseline)
         curved_ns = measure_curved_navier_stokes(300, 20, threads, with_cpu=not args.no_cpu_baseline)
 
-    config0 = mdrk_euler = None
+    config0 = mdrk_euler = refined = None
     if args.workload == DEFAULT_WORKLOAD and not args.no_extra:
         config0 = measure_config0(threads, with_cpu=not args.no_cpu_baseline)
         mdrk_euler = measure_mdrk_euler(max(3, min(args.steps, 10)), 1, threads, with_cpu=not args.no_cpu_baseline)
+        refined = measure_refined(max(3, min(args.steps, 10)), 1, threads, with_cpu=not args.no_cpu_baseline,
+                                  uniform_value=value)
 
     cpu = None
     if not args.no_cpu_baseline:
         import oracle_ctypes
 
-        nx = mesh.nx
-        o = oracle_ctypes.Oracle(N, nx, nx, lo, lo, (hi - lo) / nx, **okw)
+        nx = mesh.nx or 1
+        o = (oracle_ctypes.Oracle(N, 1, 1, 0.0, 0.0, 1.0, **okw) if "tree" in okw else
+             oracle_ctypes.Oracle(N, nx, nx, lo, lo, (hi - lo) / nx, **okw))
         xo, yo = o.node_coords()
         uo = np.ascontiguousarray(ic(xo, yo, 0.0))
         log(f"bench: CPU oracle baseline, {args.cpu_steps} steps on {threads} threads")
@@ -929,7 +976,7 @@ def main():
         v_one, el_one = cpu_baseline(o, uo, cfl, td == "mdrk", dofs, 1, 1)
         log(f"bench: CPU single thread {v_one / 1e6:.2f} M DOF-updates/s ({el_one:.1f} s)")
         cpu = {"value": v_all, "unit": "DOF-updates/s", "cores": threads, "kind": "port",
-               "sample": f"{args.cpu_steps} steps of the same {nx}x{nx} N={N} mesh ({dofs} DOFs), "
+               "sample": f"{args.cpu_steps} steps of the same N={N} mesh ({mesh.n_elements} elements, {dofs} DOFs), "
                          f"C++/OpenMP oracle, g++ -O3 -march=x86-64-v3",
                "gpu_over_cpu": round(value / v_all, 2),
                "single_thread": {"value": v_one, "cores": 1, "sample": "1 step of the same mesh",
@@ -953,6 +1000,8 @@ def main():
                    "mesh": (f"{mesh.nx}x{mesh.nx} TreeMesh [0,1]^2, time-averaged Dirichlet boundary" if sc else
                             f"{mesh.nx}x{mesh.nx} periodic StructuredMesh [0,{hi:g}]^2 warped by "
                             f"{CNS_A if par else WARP_A}" if curved
+                            else f"{mesh.n_elements}-leaf periodic TreeMesh [{lo:g},{hi:g}]^2 (512^2, [-5,5]^2 "
+                                 "refined once)" if "tree" in okw
                             else f"{mesh.nx}x{mesh.nx} periodic TreeMesh [{lo:g},{hi:g}]^2"),
                    "polydeg": N, "time_discretization": td.upper(), "dofs": dofs,
                    "initial_condition": ("four-quadrant Riemann problem (Lax-Liu 3)" if sc else
@@ -976,6 +1025,7 @@ def main():
         "shock_capturing": sc_rec,
         "config0": config0,
         "mdrk_euler": mdrk_euler,
+        "refined": refined,
     }
     print(json.dumps(out), flush=True)
     semi.close()

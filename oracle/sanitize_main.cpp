@@ -7,6 +7,7 @@
 // oracle; every scenario checks that its outputs are finite, and the
 // sanitizers report any out-of-bounds access, use after free, leak, signed
 // overflow or misaligned load on the way.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -143,6 +144,65 @@ void tangents(int N) {
     check_finite("tangent LW, frozen blend", duo);
 }
 
+// A periodic TreeMesh of [-10, 10]² with 4 x 4 cells whose central 2 x 2 are
+// refined once (28 leaves, 8 mortars), connected by brute-force geometry.
+struct Refined {
+    std::vector<double> cx, cy, len;
+    std::vector<int> nbr, mort;
+};
+Refined refined_mesh() {
+    Refined m;
+    for (int j = 0; j < 4; ++j)
+        for (int i = 0; i < 4; ++i) {
+            const double x = -10.0 + 5.0 * (i + 0.5), y = -10.0 + 5.0 * (j + 0.5);
+            if (i >= 1 && i <= 2 && j >= 1 && j <= 2) {
+                for (int b = 0; b < 2; ++b)
+                    for (int a = 0; a < 2; ++a) {
+                        m.cx.push_back(x + 2.5 * (a - 0.5));
+                        m.cy.push_back(y + 2.5 * (b - 0.5));
+                        m.len.push_back(2.5);
+                    }
+            } else {
+                m.cx.push_back(x);
+                m.cy.push_back(y);
+                m.len.push_back(5.0);
+            }
+        }
+    const int E = (int)m.cx.size();
+    m.nbr.assign(4 * (size_t)E, -3);
+    for (int e = 0; e < E; ++e)
+        for (int f = 0; f < 4; ++f) {
+            const int ax = f / 2;
+            const double sgn = (f % 2) ? 1.0 : -1.0;
+            const double* cn = ax ? m.cy.data() : m.cx.data();
+            const double* ct = ax ? m.cx.data() : m.cy.data();
+            double pos = cn[e] + sgn * 0.5 * m.len[e];
+            if (pos > 10.0 - 1e-9) pos -= 20.0;
+            if (pos < -10.0 + 1e-9) pos += 20.0;
+            std::vector<int> cand;
+            for (int k = 0; k < E; ++k) {
+                double opp = cn[k] - sgn * 0.5 * m.len[k];
+                if (opp > 10.0 - 1e-9) opp -= 20.0;
+                if (opp < -10.0 + 1e-9) opp += 20.0;
+                const double ov = std::min(ct[k] + 0.5 * m.len[k], ct[e] + 0.5 * m.len[e]) -
+                                  std::max(ct[k] - 0.5 * m.len[k], ct[e] - 0.5 * m.len[e]);
+                if (std::fabs(opp - pos) < 1e-9 && ov > 1e-9) cand.push_back(k);
+            }
+            if (cand.size() == 1 && m.len[cand[0]] == m.len[e]) {
+                m.nbr[4 * e + f] = cand[0];
+            } else {
+                m.nbr[4 * e + f] = -2;
+                if (cand.size() == 2) {
+                    const int lo = ct[cand[0]] < ct[cand[1]] ? cand[0] : cand[1];
+                    const int up = lo == cand[0] ? cand[1] : cand[0];
+                    const int rec[5] = {lo, up, e, ax + 1, sgn > 0 ? 1 : 2};
+                    m.mort.insert(m.mort.end(), rec, rec + 5);
+                }
+            }
+        }
+    return m;
+}
+
 }  // namespace
 
 int main() {
@@ -270,6 +330,26 @@ int main() {
             tlwo_destroy(c);
         }
         tangents(N);
+
+        // TreeMesh with refined cells: L2 mortars in LW and MDRK
+        {
+            static Refined m;
+            m = refined_mesh();
+            tlwo_desc t = base(N, 1, 0.0, 1.0);
+            t.n_tree = (int)m.cx.size();
+            t.tree_cx = m.cx.data();
+            t.tree_cy = m.cy.data();
+            t.tree_len = m.len.data();
+            t.tree_nbr = m.nbr.data();
+            t.n_mortars = (int)m.mort.size() / 5;
+            t.mortars = m.mort.data();
+            if (t.n_mortars != 8) {
+                std::printf("FAIL refined mesh: %d mortars\n", t.n_mortars);
+                ++g_failures;
+            }
+            exercise("refined LW", t, smooth, 0.3, false);
+            exercise("refined MDRK", t, smooth, 0.15, true);
+        }
     }
     std::printf("%s: %d failure(s)\n", g_failures ? "FAILED" : "PASSED", g_failures);
     return g_failures ? 1 : 0;
