@@ -85,7 +85,8 @@ struct tlw_ctx {
     int ebv = 0;              // elements per volume workgroup
     bool positivity = false;  // Zhang-Shu limiter in the device loop
     bool adaptive = false;    // error-controlled steps (tlw_solve_init_adaptive)
-    bool has_boundary = false;  // a face with neighbour -1, or a mortar (the surface kernel's BC path)
+    bool has_boundary = false;  // a face with neighbour -1, or a mortar (paths that refuse both test this)
+    bool bc_faces = false;      // a face with neighbour -1
     // L2 mortars of a TreeMesh with refined cells: per mortar (orientation,
     // large_side), the volume kernel's face-node values and the mortar fluxes
     int n_mortars = 0;
@@ -428,6 +429,7 @@ int upload_meshes(tlw_ctx* c, const tlw_mesh_desc* ms, int ninst, const tlw_curv
     c->ninst = ninst;
     c->E = (int)total;
     c->has_boundary = boundary || M > 0;
+    c->bc_faces = boundary;
     c->n_mortars = M;
     c->inst_first.assign(ninst + 1, 0);
     for (int k = 0; k < ninst; ++k) c->inst_first[k + 1] = c->inst_first[k] + ms[k].n_elements;
@@ -724,6 +726,11 @@ template <int N> void launch_surface_t(tlw_ctx* c, const KArgs& a) {
         } else {
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_surface<N, false, false, true>), g, b, 0, c->stream, a);
         }
+    } else if (c->n_mortars > 0) {  // plain FR on a TreeMesh with refined cells
+        if (c->bc_faces)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_surface<N, true, false, false, true>), g, b, 0, c->stream, a);
+        else
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_surface<N, false, false, false, true>), g, b, 0, c->stream, a);
     } else if (c->sc) {
         if (c->has_boundary) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_surface<N, true, true>), g, b, 0, c->stream, a);
         else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_surface<N, false, true>), g, b, 0, c->stream, a);

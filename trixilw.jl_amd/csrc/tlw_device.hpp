@@ -2718,7 +2718,9 @@ __device__ inline void boundary_record_curved(const KArgs& a, int e, int face, i
 // both see the same bits.  In step mode every lane then takes its node's
 // final value (corrected here or finalised by the volume kernel) into the
 // CFL maxima and the admissibility test.
-template <int N, bool BC, bool SC, bool CURV = false>
+// BC: the mesh has domain-boundary faces; MORT: it has faces on L2 mortars (a
+// mesh with mortars and no boundary face compiles no boundary code).
+template <int N, bool BC, bool SC, bool CURV = false, bool MORT = false>
 __device__ __forceinline__ void surface_body(const KArgs& a, int bid) {
     using G_ = SGeo<N>;
     constexpr int n = G_::n, NN = G_::NN, EB = G_::EB, S = G_::S;
@@ -2759,7 +2761,7 @@ __device__ __forceinline__ void surface_body(const KArgs& a, int bid) {
             load_record<RQ>(a.trace + (((size_t)e * 4 + f) * n + l) * RQ, own);
             const double* Lr = (f & 1) ? own : oth;
             const double* Rr = (f & 1) ? oth : own;
-            if (!BC || nb >= 0) {
+            if ((!BC && !MORT) || nb >= 0) {
                 load_record<RQ>(a.trace + (((size_t)nb * 4 + (f ^ 1)) * n + l) * RQ, oth);
                 if constexpr (SC && CURV) {
                     // the neighbour's mirrored face node and the element-wise blending factors
@@ -2780,10 +2782,10 @@ __device__ __forceinline__ void surface_body(const KArgs& a, int bid) {
                 } else {
                     face_numflux(Lr, Rr, Fs);
                 }
-            } else if (nb <= -2) {
+            } else if (MORT && nb <= -2) {
                 // a face on an L2 mortar: the mortar kernel's flux of this element's role
                 load4(a.mflux + ((size_t)(-2 - nb) * n + l) * 4, Fs);
-            } else {
+            } else if constexpr (BC) {
                 // domain boundary: time-averaged Dirichlet state, no blending
                 // (calc_boundary_flux!, dg_2d.jl:1379-1454; structured/dg_2d.jl:2297-2391)
                 if constexpr (CURV) {
@@ -2937,9 +2939,9 @@ __device__ __forceinline__ void surface_body(const KArgs& a, int bid) {
     }
 }
 
-template <int N, bool BC, bool SC, bool CURV = false>
+template <int N, bool BC, bool SC, bool CURV = false, bool MORT = false>
 __global__ __launch_bounds__(kSurfBlock) void k_surface(KArgs a) {
-    surface_body<N, BC, SC, CURV>(a, blockIdx.x);
+    surface_body<N, BC, SC, CURV, MORT>(a, blockIdx.x);
 }
 
 // AMR solution transfer between two TreeMesh leaf sets (Trixi's refine! and
