@@ -70,6 +70,8 @@ class _Desc(C.Structure):
         ("tree_nbr", C.c_void_p),
         ("n_mortars", C.c_int),
         ("mortars", C.c_void_p),
+        ("p4est", C.c_int),
+        ("rot", C.c_void_p),
     ]
 
 
@@ -100,6 +102,8 @@ def lib():
         L.tlwo_ndofs.argtypes = [C.c_void_p]
         L.tlwo_basis.argtypes = [C.c_void_p, dp, dp, dp, dp]
         L.tlwo_mortar_matrices.argtypes = [C.c_void_p, dp, dp, dp, dp]
+        L.tlwo_p4est_connectivity.restype = C.c_int
+        L.tlwo_p4est_connectivity.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_int)]
         L.tlwo_node_coords.argtypes = [C.c_void_p, dp, dp]
         L.tlwo_neighbors.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
         L.tlwo_max_dt.restype = C.c_double
@@ -149,7 +153,7 @@ class Oracle:
                  adv=(1.0, 1.0), flux_order=0, bc_kind=BV_NONE, bc_par=(), tolerances=(0.0, 0.0), nthreads=0,
                  shock_capturing=False, alpha_max=0.5, alpha_min=0.001, alpha_smooth=True, positivity=None,
                  parabolic=None, mu=0.0, prandtl=0.72, mapping=None, reconstruction="first_order",
-                 ns_walls=None, wall_temperature=None, bc_fn=None, source_terms=None, tree=None):
+                 ns_walls=None, wall_temperature=None, bc_fn=None, source_terms=None, tree=None, p4est=None):
         """``positivity``: (threshold_density, threshold_pressure) of a Zhang-Shu
         limiter that ``solve`` applies as a stage limiter, or None.
         ``parabolic``: None, "laplace" (LaplaceDiffusion2D with diffusivity ``mu``,
@@ -171,7 +175,10 @@ class Oracle:
         values, called pointwise (Euler on the TreeMesh).
         ``tree``: a TreeMesh with refined cells, (cx, cy, h, neighbors [E, 4],
         mortars [M, 5]) as ``tree_mesh.connectivity`` gives them; nx, ny, xmin,
-        ymin, h and periodic are then unused and elements keep the given order."""
+        ymin, h and periodic are then unused and elements keep the given order.
+        ``p4est``: with ``mapping``, a P4estMesh of the same nx × ny cells: None for a
+        StructuredMesh, True for aligned element frames, or an array of per-element
+        frame turns (0..3, ·90°) for the orientations of an unstructured mesh."""
         d = _Desc()
         d.polydeg, d.nx, d.ny = polydeg, nx, ny
         d.xmin, d.ymin, d.h = xmin, ymin, h
@@ -235,6 +242,11 @@ class Oracle:
             d.n_mortars = len(self._tree[4])
             d.mortars = self._tree[4].ctypes.data if d.n_mortars else None
             nx, ny = d.n_tree, 1
+        if p4est is not None:
+            d.p4est = 1
+            if p4est is not True:
+                self._rot = np.ascontiguousarray(p4est, dtype=np.int32).reshape(-1)
+                d.rot = self._rot.ctypes.data
         self._desc = d
         self._h = lib().tlwo_create(C.byref(d))
         if not self._h:
@@ -252,6 +264,19 @@ class Oracle:
     @property
     def shape(self):
         return (self.E, self.n, self.n, self.nv)
+
+    def p4est_connectivity(self):
+        """P4est interfaces [I, 5] (primary, primary face, secondary, secondary face,
+        flipped) and boundary faces [B, 2] (element, face), faces 0-based -ξ, +ξ, -η, +η."""
+        nb = C.c_int(0)
+        ni = lib().tlwo_p4est_connectivity(self._h, None, None, C.byref(nb))
+        if ni < 0:
+            raise ValueError("not a P4est oracle")
+        itf = np.zeros((ni, 5), dtype=np.int32)
+        bd = np.zeros((nb.value, 2), dtype=np.int32)
+        lib().tlwo_p4est_connectivity(self._h, itf.ctypes.data if ni else None, bd.ctypes.data if nb.value else None,
+                                      C.byref(nb))
+        return itf, bd
 
     def mortar_matrices(self):
         """forward_lower, forward_upper, reverse_lower, reverse_upper ([n, n], row i = output node)."""

@@ -70,6 +70,11 @@ struct tlwo_desc {
     const int* tree_nbr;
     int n_mortars;
     const int* mortars;
+    // P4estMesh{2} of the curvilinear mapping's nx x ny cells (curved = 1): each
+    // element's frame turned by rot[e]·90° (rot may be null), P4est interfaces
+    // and boundaries (Euler or linear advection, plain FR, hyperbolic only)
+    int p4est;
+    const int* rot;
 };
 
 struct tlwo_ctx {
@@ -118,6 +123,7 @@ void* tlwo_create(const tlwo_desc* d) {
     // StructuredMesh asserts a periodic mesh
     const bool periodic = d->periodic_x && d->periodic_y;
     if (d->curved && d->parabolic == 1) return nullptr;
+    if (d->p4est && (!d->curved || d->parabolic || d->shock_capturing || d->bc_kind == BV_SLIP_WALL)) return nullptr;
     if (d->curved && d->shock_capturing && (d->reconstruction != 0 || (d->alpha_smooth && !periodic))) return nullptr;
     if (d->reconstruction == 1 && (d->equation != 0 || !d->shock_capturing)) return nullptr;
     // the reflecting wall is the Euler TreeMesh one (x faces vertical, y faces horizontal)
@@ -151,14 +157,16 @@ void* tlwo_create(const tlwo_desc* d) {
             auto* p = new CurvedSolver<double, Euler2D<double>>;
             p->eq.gamma = d->gamma;
             configure(*p, d);
-            p->setup_curved(d->map_kind, d->map_par);
+            p->setup_curved(d->map_kind, d->map_par, d->p4est ? d->rot : nullptr);
+            if (d->p4est) p->setup_p4est();
             c->euler.reset(p);
         } else {
             auto* p = new CurvedSolver<double, LinAdv2D<double>>;
             p->eq.a1 = d->adv_a1;
             p->eq.a2 = d->adv_a2;
             configure(*p, d);
-            p->setup_curved(d->map_kind, d->map_par);
+            p->setup_curved(d->map_kind, d->map_par, d->p4est ? d->rot : nullptr);
+            if (d->p4est) p->setup_p4est();
             c->linadv.reset(p);
         }
     } else if (d->parabolic == 2) {
@@ -244,6 +252,32 @@ void tlwo_mortar_matrices(void* ctx, double* fwd_lo, double* fwd_up, double* rev
                 rev_up[i * n + k] = s.basis.rev_up[i][k];
             }
     });
+}
+
+// P4est interfaces {primary, primary face, secondary, secondary face, flipped} and
+// boundary faces {element, face} of a P4est context; returns the interface count
+// (boundary count in *nb); rows are written when the buffers are given
+int tlwo_p4est_connectivity(void* ctx, int* interfaces, int* boundaries, int* nb) {
+    int r = -1;
+    if (!ctx) return r;
+    auto* c = static_cast<tlwo_ctx*>(ctx);
+    auto fill = [&](auto* s) {
+        if (!s || !s->p4est) return;
+        r = (int)s->p4if.size();
+        if (nb) *nb = (int)s->p4bd_e.size();
+        for (size_t k = 0; interfaces && k < s->p4if.size(); ++k) {
+            const auto& q = s->p4if[k];
+            const int row[5] = {q.prim, q.pface, q.sec, q.sface, q.flip};
+            for (int m = 0; m < 5; ++m) interfaces[5 * k + m] = row[m];
+        }
+        for (size_t k = 0; boundaries && k < s->p4bd_e.size(); ++k) {
+            boundaries[2 * k] = s->p4bd_e[k];
+            boundaries[2 * k + 1] = s->p4bd_f[k];
+        }
+    };
+    if (c->euler) fill(dynamic_cast<CurvedSolver<double, Euler2D<double>>*>(c->euler.get()));
+    if (c->linadv) fill(dynamic_cast<CurvedSolver<double, LinAdv2D<double>>*>(c->linadv.get()));
+    return r;
 }
 
 // Node coordinates in [i, j, e] order

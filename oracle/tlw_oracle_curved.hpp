@@ -68,6 +68,17 @@ template <class T, class Eq, class Base = Solver<T, Eq>> struct CurvedSolver : B
     std::vector<double> xn, yn;  // node coordinates [e][j][i]
     std::vector<double> Ja;      // [e][j][i][4]: Ja11, Ja12, Ja21, Ja22
     std::vector<double> ijn;     // inverse Jacobian [e][j][i]
+    // P4estMesh{2} of trees_per_dimension cells and a mapping (p4est = true): the
+    // cells of the StructuredMesh, each element's local frame turned by rot[e]·90°
+    // against its cell (the orientations an unstructured quadrilateral mesh has);
+    // Trixi's P4est interface container (primary element and face, secondary
+    // element and face, flipped node order) and boundary faces, found from the
+    // cell adjacency.  Faces are 0-based -ξ, +ξ, -η, +η of the element's frame.
+    bool p4est = false;
+    std::vector<int> rot;
+    struct P4If { int prim, pface, sec, sface, flip; };
+    std::vector<P4If> p4if;
+    std::vector<int> p4bd_e, p4bd_f;
 
     size_t nidx(int i, int j, int e) const { return ((size_t)e * n + j) * n + i; }
     void node_xy(int i, int j, int e, double& x, double& y) const override {
@@ -84,20 +95,34 @@ template <class T, class Eq, class Base = Solver<T, Eq>> struct CurvedSolver : B
 
     // Trixi StructuredMesh cache: node coordinates, Jacobian matrix from the
     // derivative matrix, contravariant vectors, inverse Jacobian.
-    void setup_curved(int kind, const double* p) {
+    // local (ξ, η) of an element turned by r·90° -> its cell's (a, b)
+    static void turn(int r, double xi, double eta, double& a, double& b) {
+        switch (r & 3) {
+            case 0: a = xi; b = eta; break;
+            case 1: a = -eta; b = xi; break;
+            case 2: a = -xi; b = -eta; break;
+            default: a = eta; b = -xi; break;
+        }
+    }
+    void setup_curved(int kind, const double* p, const int* rotations = nullptr) {
         const int nx = mesh.nx, ny = mesh.ny, E = mesh.E;
         xn.assign((size_t)E * n * n, 0.0);
         yn.assign(xn.size(), 0.0);
         Ja.assign(4 * xn.size(), 0.0);
         ijn.assign(xn.size(), 0.0);
+        rot.assign(E, 0);
+        if (rotations)
+            for (int e = 0; e < E; ++e) rot[e] = rotations[e] & 3;
         const double dx = 2.0 / nx, dy = 2.0 / ny;
         for (int e = 0; e < E; ++e) {
             const int cx = e % nx, cy = e / nx;
             const double ox = -1.0 + cx * dx + dx / 2, oy = -1.0 + cy * dy + dy / 2;
             for (int j = 0; j < n; ++j)
-                for (int i = 0; i < n; ++i)
-                    apply_mapping(kind, p, ox + dx / 2 * basis.nodes[i], oy + dy / 2 * basis.nodes[j], xn[nidx(i, j, e)],
-                                  yn[nidx(i, j, e)]);
+                for (int i = 0; i < n; ++i) {
+                    double a, b;
+                    turn(rot[e], basis.nodes[i], basis.nodes[j], a, b);
+                    apply_mapping(kind, p, ox + dx / 2 * a, oy + dy / 2 * b, xn[nidx(i, j, e)], yn[nidx(i, j, e)]);
+                }
             const auto& D = basis.D;
             for (int j = 0; j < n; ++j)
                 for (int i = 0; i < n; ++i) {
@@ -524,6 +549,180 @@ template <class T, class Eq, class Base = Solver<T, Eq>> struct CurvedSolver : B
         }
     }
 
+    // ---- P4estMesh{2} -------------------------------------------------------
+    static void face_node(int n_, int f, int l, int& i, int& j) {
+        if (f < 2) { i = (f == 0) ? 0 : n_ - 1; j = l; }
+        else { i = l; j = (f == 2) ? 0 : n_ - 1; }
+    }
+    // Interfaces and boundary faces from the cell adjacency: local face f of
+    // element e points along its turned outward vector; the neighbour cell's
+    // element meets it with the face whose turned outward vector is opposite,
+    // and its node order is flipped when the turned tangents are opposite.
+    void setup_p4est() {
+        p4est = true;
+        const int nx = mesh.nx, ny = mesh.ny, E = mesh.E;
+        const int out[4][2] = {{-1, 0}, {1, 0}, {0, -1}, {0, 1}};
+        const int tan[4][2] = {{0, 1}, {0, 1}, {1, 0}, {1, 0}};
+        auto turned = [&](int r, const int* v, int* w) {
+            int a = v[0], b = v[1];
+            for (int k = 0; k < (r & 3); ++k) { const int t = a; a = -b; b = t; }
+            w[0] = a; w[1] = b;
+        };
+        p4if.clear(); p4bd_e.clear(); p4bd_f.clear();
+        for (int e = 0; e < E; ++e)
+            for (int f = 0; f < 4; ++f) {
+                int d[2], t[2];
+                turned(rot[e], out[f], d);
+                turned(rot[e], tan[f], t);
+                int cx = e % nx + d[0], cy = e / nx + d[1];
+                if (cx < 0 || cx >= nx) {
+                    if (!mesh.per_x) { p4bd_e.push_back(e); p4bd_f.push_back(f); continue; }
+                    cx = (cx + nx) % nx;
+                }
+                if (cy < 0 || cy >= ny) {
+                    if (!mesh.per_y) { p4bd_e.push_back(e); p4bd_f.push_back(f); continue; }
+                    cy = (cy + ny) % ny;
+                }
+                const int e2 = cx + nx * cy;
+                int f2 = -1, flip = 0;
+                for (int g = 0; g < 4; ++g) {
+                    int d2[2], t2[2];
+                    turned(rot[e2], out[g], d2);
+                    if (d2[0] == -d[0] && d2[1] == -d[1]) {
+                        f2 = g;
+                        turned(rot[e2], tan[g], t2);
+                        flip = (t2[0] != t[0] || t2[1] != t[1]) ? 1 : 0;
+                    }
+                }
+                // each interface once, primary = the lower (element, face)
+                if (e < e2 || (e == e2 && f < f2)) p4if.push_back({e, f, e2, f2, flip});
+            }
+    }
+    // outward normal of local face f at its node (i, j) (Trixi get_normal_direction)
+    void outward(int e, int f, int i, int j, double* nrm) const {
+        const double* J = &Ja[4 * nidx(i, j, e)];
+        const double s = (f & 1) ? 1.0 : -1.0;
+        nrm[0] = s * (f < 2 ? J[0] : J[2]);
+        nrm[1] = s * (f < 2 ? J[1] : J[3]);
+    }
+    // prolong2interfaces! + calc_interface_flux! for P4estMesh (p4est/dg_2d.jl:8-192),
+    // plain FR: each side's F̃ in the primary's outward direction from its own
+    // contravariant vector (the secondary's outward normal negated), λ from uⁿ with
+    // the primary's normal, jump on Ũ; the primary keeps F*, the secondary -F* at its
+    // own (possibly reversed) node.  Stored in the +ξ/+η convention of the surface
+    // integral: an outward flux of face f is (+1 | -1 for f odd | even) times it.
+    void calc_interface_flux_p4est(const T* u) {
+        const int nI = (int)p4if.size();
+#pragma omp parallel for schedule(static)
+        for (int I = 0; I < nI; ++I) {
+            const P4If& q = p4if[I];
+            for (int l = 0; l < n; ++l) {
+                const int ls = q.flip ? n - 1 - l : l;
+                int ip, jp, is, js;
+                face_node(n, q.pface, l, ip, jp);
+                face_node(n, q.sface, ls, is, js);
+                double np_[2], ns_[2];
+                outward(q.prim, q.pface, ip, jp, np_);
+                outward(q.sec, q.sface, is, js, ns_);
+                T fll[NV], frr[NV];
+                for (int v = 0; v < NV; ++v) {
+                    fll[v] = this->F[fidx(v, 0, ip, jp, q.prim)] * np_[0] + this->F[fidx(v, 1, ip, jp, q.prim)] * np_[1];
+                    frr[v] = this->F[fidx(v, 0, is, js, q.sec)] * (-ns_[0]) + this->F[fidx(v, 1, is, js, q.sec)] * (-ns_[1]);
+                }
+                const T* ull = &u[idx(0, ip, jp, q.prim)];
+                const T* urr = &u[idx(0, is, js, q.sec)];
+                const T lam = eq.max_abs_speed_naive_normal(ull, urr, np_);
+                const double sp = (q.pface & 1) ? 1.0 : -1.0, ss = (q.sface & 1) ? 1.0 : -1.0;
+                for (int v = 0; v < NV; ++v) {
+                    const T Fn = 0.5 * (fll[v] + frr[v]) -
+                                 0.5 * lam * (this->U[idx(v, is, js, q.sec)] - this->U[idx(v, ip, jp, q.prim)]);
+                    this->sfv[sidx(v, l, q.pface, q.prim)] = sp * Fn;
+                    this->sfv[sidx(v, ls, q.sface, q.sec)] = -ss * Fn;
+                }
+            }
+        }
+    }
+    // prolong2boundaries! + calc_boundary_flux! for P4estMesh (p4est/dg_2d.jl:283-393)
+    // with the conditions' P4est methods: the outward normal, uⁿ, Ũ and F̃·n inside;
+    // the time-averaged Dirichlet condition (equations.jl:66-93) takes
+    // surface_flux(F̃·n, F_outer, uⁿ, U_outer, Ũ, U_outer, n): λ from uⁿ and the time
+    // average U_outer, jump on Ũ; outflow and the approximate slip wall
+    // (equations.jl:185-197, 251-260) use uⁿ alone, the inner state first.
+    void calc_boundary_flux_p4est(const T* u, double t, double dt, double scaling_factor) {
+        if constexpr (std::is_same<Eq, Euler2D<T>>::value) {
+            const auto& nodes = basis.nodes;
+            const auto& w = basis.weights;
+            const int nB = (int)p4bd_e.size();
+#pragma omp parallel for schedule(static)
+            for (int B = 0; B < nB; ++B) {
+                const int e = p4bd_e[B], f = p4bd_f[B];
+                const double sf = (f & 1) ? 1.0 : -1.0;
+                for (int l = 0; l < n; ++l) {
+                    int i, j;
+                    face_node(n, f, l, i, j);
+                    double nrm[2];
+                    outward(e, f, i, j, nrm);
+                    const T* u_in = &u[idx(0, i, j, e)];
+                    const T* U_in = &this->U[idx(0, i, j, e)];
+                    T fb[NV];
+                    if (this->bval.kind == BV_OUTFLOW) {
+                        eq.flux_normal(u_in, nrm, fb);
+                    } else if (this->bval.kind == BV_SLIP_WALL_APPROX) {
+                        T rho, v1, v2, p, ur[NV];
+                        eq.cons2prim(u_in, rho, v1, v2, p);
+                        const double ns = nrm[0] * nrm[0] + nrm[1] * nrm[1];
+                        const T vn = (v1 * nrm[0] + v2 * nrm[1]) / ns;
+                        const T w1 = v1 - 2.0 * vn * nrm[0], w2 = v2 - 2.0 * vn * nrm[1];
+                        ur[0] = rho;
+                        ur[1] = rho * w1;
+                        ur[2] = rho * w2;
+                        ur[3] = p / (eq.gamma - 1.0) + 0.5 * rho * (w1 * w1 + w2 * w2);
+                        T fl[NV], fr[NV];
+                        eq.flux_normal(u_in, nrm, fl);
+                        eq.flux_normal(ur, nrm, fr);
+                        const T lam = eq.max_abs_speed_naive_normal(u_in, ur, nrm);
+                        for (int v = 0; v < NV; ++v) fb[v] = 0.5 * (fl[v] + fr[v]) + (-0.5 * lam) * (ur[v] - u_in[v]);
+                    } else {
+                        T f_in[NV], Uo[NV], Fo[NV];
+                        for (int v = 0; v < NV; ++v) {
+                            f_in[v] = this->F[fidx(v, 0, i, j, e)] * nrm[0] + this->F[fidx(v, 1, i, j, e)] * nrm[1];
+                            Uo[v] = T(0);
+                            Fo[v] = T(0);
+                        }
+                        const double x = xn[nidx(i, j, e)], y = yn[nidx(i, j, e)];
+                        const double dts = scaling_factor * dt;
+                        for (int q = 0; q < n; ++q) {
+                            const double ts = t + 0.5 * dts * (nodes[q] + 1.0);
+                            double ubd[4];
+                            this->bval.eval(x, y, ts, ubd);
+                            T ub[NV], fbq[NV];
+                            for (int v = 0; v < NV; ++v) ub[v] = T(ubd[v]);
+                            eq.flux_normal(ub, nrm, fbq);
+                            for (int v = 0; v < NV; ++v) {
+                                Uo[v] = Uo[v] + ub[v] * scaling_factor * w[q] / 2.0;
+                                Fo[v] = Fo[v] + fbq[v] * scaling_factor * w[q] / 2.0;
+                            }
+                        }
+                        const T lam = eq.max_abs_speed_naive_normal(u_in, Uo, nrm);
+                        for (int v = 0; v < NV; ++v) fb[v] = 0.5 * (f_in[v] + Fo[v]) - 0.5 * lam * (Uo[v] - U_in[v]);
+                    }
+                    for (int v = 0; v < NV; ++v) this->sfv[sidx(v, l, f, e)] = sf * fb[v];
+                }
+            }
+        }
+    }
+    // the interface and boundary phases of one rhs call (scaling_factor: MDRK stage 1 of
+    // the P4est mesh passes 0.5, tree/multi_derivative_2d.jl:34)
+    void surface_fluxes(const T* u, double t, double dt, double p4est_scaling) {
+        if (p4est) {
+            calc_interface_flux_p4est(u);
+            if (!p4bd_e.empty()) calc_boundary_flux_p4est(u, t, dt, p4est_scaling);
+        } else {
+            calc_interface_flux_curved(u);
+            if (!mesh.bd_elem.empty()) calc_boundary_flux_curved(u, t, dt);
+        }
+    }
+
     // Trixi calc_surface_integral! (StructuredMesh uses the TreeMesh form) and
     // apply_jacobian! with the node's J⁻¹
     void surface_integral_and_jacobian_curved(T* du) {
@@ -553,8 +752,7 @@ template <class T, class Eq, class Base = Solver<T, Eq>> struct CurvedSolver : B
             lw_volume_curved(du, u, dt, e, this->sc ? 1.0 - this->alpha[e] : 1.0);
             if (this->sc) sc_element_curved(du, u, e);
         }
-        calc_interface_flux_curved(u);
-        if (!mesh.bd_elem.empty()) calc_boundary_flux_curved(u, t, dt);
+        surface_fluxes(u, t, dt, 1.0);
         surface_integral_and_jacobian_curved(du);
     }
 
@@ -702,8 +900,7 @@ template <class T, class Eq, class Base = Solver<T, Eq>> struct CurvedSolver : B
             mdrk_curved_1(du, u, dt, e, this->sc ? 1.0 - this->alpha[e] : 1.0);
             if (this->sc) sc_element_curved(du, u, e);
         }
-        calc_interface_flux_curved(u);
-        if (!mesh.bd_elem.empty()) calc_boundary_flux_curved(u, t, dt);
+        surface_fluxes(u, t, dt, 0.5);
         surface_integral_and_jacobian_curved(du);
     }
     void rhs_mdrk2(T* du, const T* u, double t, double dt) override {
@@ -715,8 +912,7 @@ template <class T, class Eq, class Base = Solver<T, Eq>> struct CurvedSolver : B
             mdrk_curved_2(du, dt, e, this->sc ? 1.0 - this->alpha[e] : 1.0);
             if (this->sc) sc_element_curved(du, u, e);
         }
-        calc_interface_flux_curved(u);
-        if (!mesh.bd_elem.empty()) calc_boundary_flux_curved(u, t, dt);
+        surface_fluxes(u, t, dt, 1.0);
         surface_integral_and_jacobian_curved(du);
     }
 
