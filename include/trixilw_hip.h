@@ -181,6 +181,21 @@ typedef struct {
     const double* node_coordinates_host;      /* [2*n*n*E] node_coordinates[d, i, j, element]; needed (for the
                                                  boundary value function) when the mesh has boundary faces */
 } tlw_curved_desc;
+
+/* P4estMesh{2} (conforming: leaves of one level): the metric terms as in tlw_curved_desc plus Trixi's
+ * P4est interface container (src/solvers/dgsem_p4est/containers_2d.jl in Trixi.jl, read by the
+ * reference's prolong2interfaces!/calc_interface_flux!, src/solvers/dgsem_p4est/dg_2d.jl:8-192), its
+ * node_indices reduced to faces and a flip: per interface {primary element, primary face, secondary
+ * element, secondary face, flipped}, 0-based, faces 0..3 = -ξ, +ξ, -η, +η of the element's own frame
+ * (any relative orientation), flipped = 1 when the secondary's face nodes run against the primary's.
+ * mesh->neighbors_host holds each interface face's partner element and -1 on boundary faces. */
+typedef struct {
+    const double* contravariant_vectors_host; /* as in tlw_curved_desc */
+    const double* inverse_jacobian_host;
+    const double* node_coordinates_host;
+    int32_t n_interfaces;
+    const int32_t* interfaces_host; /* [5*n_interfaces] */
+} tlw_p4est_desc;
 typedef struct tlw_ctx tlw_ctx;
 
 /* SemidiscretizationHyperbolic(mesh, time_disc, equations, ic, solver) + cache
@@ -292,6 +307,18 @@ int tlw_rebuild_mesh(tlw_ctx* ctx, const tlw_mesh_desc* mesh);
  * loop and graphs work as on a uniform TreeMesh. */
 int tlw_create_nonconforming(const tlw_mesh_desc* mesh, const tlw_mortar_desc* mortars, const tlw_basis_desc* basis,
                              const tlw_solver_desc* solver, tlw_ctx** out);
+/* SemidiscretizationHyperbolic on a P4estMesh (tlw_p4est_desc): the curvilinear volume kernels of
+ * tlw_create_curved with the P4est surface terms of src/solvers/dgsem_p4est/dg_2d.jl:8-393: at an
+ * interface each side's F̃ in the primary's outward direction from its own contravariant vector, the
+ * LW Rusanov flux with λ from uⁿ and the jump on Ũ, the primary keeping F*, the secondary -F* at its own
+ * (possibly reversed) node; boundaries in the conditions' P4est methods with the outward normal:
+ * the time-averaged Dirichlet value (equations.jl:66-93: surface_flux(F̃·n, F_outer, uⁿ, U_outer, Ũ,
+ * U_outer, n)), TLW_BV_OUTFLOW and TLW_BV_SLIP_WALL_APPROXIMATE; MDRK stage 1 takes the boundary's half
+ * time average (scaling_factor 0.5, tree/multi_derivative_2d.jl:34).  LW or MDRK, VolumeIntegralFR,
+ * Euler; shock capturing, parabolic terms, source terms, TLW_BV_SLIP_WALL, mortars and the adjoints
+ * return TLW_ENOTSUP. */
+int tlw_create_p4est(const tlw_mesh_desc* mesh, const tlw_p4est_desc* p4est, const tlw_basis_desc* basis,
+                     const tlw_solver_desc* solver, tlw_ctx** out);
 /* AMR re-upload of a TreeMesh with refined cells (src/callbacks/amr.jl:2-208 resize! of the element,
  * interface, boundary and mortar caches). */
 int tlw_rebuild_mesh_nonconforming(tlw_ctx* ctx, const tlw_mesh_desc* mesh, const tlw_mortar_desc* mortars);

@@ -18,7 +18,8 @@ C_SCALAR = {"int32_t": (4, 4), "int": (4, 4), "double": (8, 8), "int64_t": (8, 8
 JL_SCALAR = {"Int32": (4, 4), "Cint": (4, 4), "Float64": (8, 8), "Int64": (8, 8), "Csize_t": (8, 8)}
 
 STRUCTS = {"tlw_mesh_desc": "MeshDesc", "tlw_basis_desc": "BasisDesc", "tlw_solver_desc": "SolverDesc",
-           "tlw_parabolic_desc": "ParabolicDesc", "tlw_curved_desc": "CurvedDesc", "tlw_mortar_desc": "MortarDesc"}
+           "tlw_parabolic_desc": "ParabolicDesc", "tlw_curved_desc": "CurvedDesc", "tlw_mortar_desc": "MortarDesc",
+           "tlw_p4est_desc": "P4estDesc"}
 
 
 def _strip_c_comments(s):
@@ -93,7 +94,7 @@ def test_struct_layouts_agree():
 
     hs, js = header_structs(), julia_structs()
     mirror = {"MeshDesc": L.MeshDesc, "BasisDesc": L.BasisDesc, "SolverDesc": L.SolverDesc,
-              "ParabolicDesc": L.ParabolicDesc, "CurvedDesc": L.CurvedDesc, "MortarDesc": L.MortarDesc}
+              "ParabolicDesc": L.ParabolicDesc, "CurvedDesc": L.CurvedDesc, "MortarDesc": L.MortarDesc, "P4estDesc": L.P4estDesc}
     assert set(hs) == set(STRUCTS)
     for cname, jname in STRUCTS.items():
         h_lay, h_size = _layout(hs[cname])

@@ -90,6 +90,9 @@ struct tlw_ctx {
     // L2 mortars of a TreeMesh with refined cells: per mortar (orientation,
     // large_side), the volume kernel's face-node values and the mortar fluxes
     int n_mortars = 0;
+    // P4estMesh: interface partner per element face (KArgs::plink)
+    bool p4est = false;
+    int4* d_plink = nullptr;
     int2* d_mort = nullptr;
     double* d_mrec = nullptr;
     double* d_mflux = nullptr;
@@ -171,7 +174,8 @@ void free_mesh(tlw_ctx* c) {
                     c->d_us,        c->d_auxU,  c->d_auxF,   c->d_partial, c->d_alpha_tmp, c->d_alpha,
                     c->d_vblk,      c->d_sblk,  c->d_etile,  c->d_rng,    c->d_state,   c->d_cur,
                     c->d_Ja,        c->d_ijn,   c->d_fnorm,  c->d_fxy,     c->d_snrm,      c->d_sqrk,
-                    c->d_fvb,       c->d_bidx,  c->d_btab,   c->d_mort,   c->d_mrec,    c->d_mflux};
+                    c->d_fvb,       c->d_bidx,  c->d_btab,   c->d_mort,   c->d_mrec,    c->d_mflux,
+                    c->d_plink};
     for (void* p : ptrs) (void)hipFree(p);
     if (c->h_state) (void)hipHostFree(c->h_state);
     c->d_nbr = nullptr;
@@ -188,6 +192,7 @@ void free_mesh(tlw_ctx* c) {
     c->d_btab = nullptr;
     c->d_mort = nullptr;
     c->d_mrec = c->d_mflux = nullptr;
+    c->d_plink = nullptr;
     c->n_mortars = 0;
     c->btab_cap = 0;
     c->btab_levels = 0;
@@ -244,7 +249,8 @@ int validate_mesh(tlw_ctx* c, const tlw_mesh_desc* m, bool mortars = false) {
         if (nb < lowest || nb >= E) return fail(c, TLW_EINVAL, "mesh description: neighbour id out of range");
     }
     // conformity: the neighbour across a face must point back through the opposite face
-    for (int e = 0; e < E; ++e)
+    // (a P4estMesh's faces meet in any orientation: its interface list is checked instead)
+    for (int e = 0; e < E && !c->p4est; ++e)
         for (int f = 0; f < 4; ++f) {
             const int nb = m->neighbors_host[4 * e + f];
             if (nb >= 0 && m->neighbors_host[4 * nb + (f ^ 1)] != e)
@@ -308,9 +314,10 @@ int prepare_curved(tlw_ctx* c, const tlw_mesh_desc* m, const tlw_curved_desc* cv
         invj[e] = ijn[(size_t)e * NN];
         for (int f = 0; f < 4; ++f) {
             const int nb = m->neighbors_host[4 * e + f];
-            // an interface takes the right element's face node; a domain boundary the element's own
-            const int right = (f & 1) ? (nb >= 0 ? nb : e) : e;
-            const bool own_plus = (f & 1) && nb < 0;
+            // an interface takes the right element's face node; a domain boundary the element's own;
+            // on a P4estMesh every face its own (each side of an interface uses its own normal)
+            const int right = c->p4est ? e : (f & 1) ? (nb >= 0 ? nb : e) : e;
+            const bool own_plus = (f & 1) && (nb < 0 || c->p4est);
             for (int l = 0; l < n; ++l) {
                 // x faces: Ja¹ at node (1, l) of the right element (own: (n, l)); y faces: Ja² at (l, 1) ((l, n))
                 const size_t loc = f < 2 ? (size_t)n * l + (own_plus ? n - 1 : 0)
@@ -374,7 +381,7 @@ int validate_mortars(tlw_ctx* c, const tlw_mesh_desc* m, const tlw_mortar_desc* 
 }
 
 int upload_meshes(tlw_ctx* c, const tlw_mesh_desc* ms, int ninst, const tlw_curved_desc* cv = nullptr,
-                  const tlw_mortar_desc* md = nullptr) {
+                  const tlw_mortar_desc* md = nullptr, const tlw_p4est_desc* pd = nullptr) {
     if (ninst < 1) return fail(c, TLW_EINVAL, "at least one instance");
     if (c->curved && ninst != 1) return fail(c, TLW_ENOTSUP, "a curvilinear mesh is a single instance");
     long long total = 0;
@@ -391,6 +398,30 @@ int upload_meshes(tlw_ctx* c, const tlw_mesh_desc* ms, int ninst, const tlw_curv
             return fail(c, TLW_ENOTSUP, "mortars: a single-instance hyperbolic TreeMesh context with VolumeIntegralFR");
         int rc = validate_mortars(c, &ms[0], md, mcode);
         if (rc) return rc;
+    }
+    std::vector<int4> plink;
+    if (pd) {
+        const int E0 = ms[0].n_elements, I = pd->n_interfaces;
+        if (ninst != 1 || I < 0 || (I > 0 && !pd->interfaces_host))
+            return fail(c, TLW_EINVAL, "P4est description: one instance and an interface list");
+        plink.assign(4 * (size_t)E0, make_int4(-1, -1, 0, 0));
+        for (int k = 0; k < I; ++k) {
+            const int* r = pd->interfaces_host + 5 * (size_t)k;
+            const int pe = r[0], pf = r[1], se = r[2], sf = r[3], fl = r[4];
+            if (pe < 0 || pe >= E0 || se < 0 || se >= E0 || pf < 0 || pf > 3 || sf < 0 || sf > 3 || (fl != 0 && fl != 1) ||
+                (pe == se && pf == sf))
+                return fail(c, TLW_EINVAL, "P4est description: interface entry out of range");
+            int4& a = plink[4 * (size_t)pe + pf];
+            int4& b = plink[4 * (size_t)se + sf];
+            if (a.x >= 0 || b.x >= 0) return fail(c, TLW_EINVAL, "P4est description: a face lies on two interfaces");
+            if (ms[0].neighbors_host[4 * (size_t)pe + pf] != se || ms[0].neighbors_host[4 * (size_t)se + sf] != pe)
+                return fail(c, TLW_EINVAL, "P4est description: neighbours_host disagrees with the interface list");
+            a = make_int4(se, sf, fl, 1);
+            b = make_int4(pe, pf, fl, 0);
+        }
+        for (size_t q = 0; q < 4 * (size_t)E0; ++q)
+            if (plink[q].x < 0 && ms[0].neighbors_host[q] != -1)
+                return fail(c, TLW_EINVAL, "P4est description: a face with a neighbour is on no interface");
     }
     bool boundary = false;
     for (int k = 0; k < ninst; ++k)
@@ -428,7 +459,8 @@ int upload_meshes(tlw_ctx* c, const tlw_mesh_desc* ms, int ninst, const tlw_curv
     c->aux_valid = c->alpha_valid = false;
     c->ninst = ninst;
     c->E = (int)total;
-    c->has_boundary = boundary || M > 0;
+    // the P4est interfaces are paired in the surface kernel's boundary-capable path
+    c->has_boundary = boundary || M > 0 || pd != nullptr;
     c->bc_faces = boundary;
     c->n_mortars = M;
     c->inst_first.assign(ninst + 1, 0);
@@ -441,7 +473,7 @@ int upload_meshes(tlw_ctx* c, const tlw_mesh_desc* ms, int ninst, const tlw_curv
         const int f0 = c->inst_first[k], Ek = ms[k].n_elements;
         for (size_t q = 0; q < (size_t)4 * Ek; ++q) {
             const int nb = ms[k].neighbors_host[q];
-            nbr[4 * (size_t)f0 + q] = nb == TLW_NEIGHBOR_MORTAR ? mcode[q] : (nb < 0 ? -1 : nb + f0);
+            nbr[4 * (size_t)f0 + q] = pd ? (nb < 0 ? -1 : -3) : nb == TLW_NEIGHBOR_MORTAR ? mcode[q] : (nb < 0 ? -1 : nb + f0);
         }
         if (c->curved) {  // element data of a TreeMesh: unused, J⁻¹ of each element's first node
             std::memcpy(&invj[f0], cinvj.data(), sizeof(double) * Ek);
@@ -475,6 +507,10 @@ int upload_meshes(tlw_ctx* c, const tlw_mesh_desc* ms, int ninst, const tlw_curv
     HIPCHK(c, hipMalloc(&c->d_etile, sizeof(int) * (size_t)E));
     HIPCHK(c, hipMalloc(&c->d_rng, sizeof(int4) * (size_t)ninst));
     if (c->hconst.ns_walls) HIPCHK(c, hipMalloc(&c->d_fvb, sizeof(double) * (size_t)E * 4 * c->n * 2));
+    if (pd) {
+        HIPCHK(c, hipMalloc(&c->d_plink, sizeof(int4) * plink.size()));
+        HIPCHK(c, hipMemcpy(c->d_plink, plink.data(), sizeof(int4) * plink.size(), hipMemcpyHostToDevice));
+    }
     if (M > 0) {
         std::vector<int2> mo(M);
         for (int q = 0; q < M; ++q) mo[q] = make_int2(md->mortars_host[5 * q + 3], md->mortars_host[5 * q + 4]);
@@ -595,6 +631,7 @@ KArgs base_args(tlw_ctx* c) {
     a.bidx = c->d_bidx;
     a.btab = c->d_btab;
     a.btab_levels = c->btab_levels;
+    a.plink = c->d_plink;
     a.n_mortars = c->n_mortars;
     a.mort = c->d_mort;
     a.mrec = c->d_mrec;
@@ -828,7 +865,7 @@ int rhs_common(tlw_ctx* c, int kind, const double* u, double* du, double t, doub
     a.step_mode = 0;
     a.reduce = RED_NONE;
     // tree/multi_derivative_2d.jl:34; a StructuredMesh passes no scaling (structured/multi_derivative_2d.jl:18-20)
-    a.bc_scale = (kind == VOL_MDRK1 && !c->curved) ? 0.5 : 1.0;
+    a.bc_scale = (kind == VOL_MDRK1 && (!c->curved || c->p4est)) ? 0.5 : 1.0;
     if (c->flags & TLW_FLAG_KEEP_AUX) {
         a.auxU = c->d_auxU;
         a.auxF = c->d_auxF;
@@ -873,7 +910,7 @@ void enqueue_step(tlw_ctx* c) {
         v1.reduce = RED_NONE;
         timed(c, 0, [&] { launch_volume(c, v1, VOL_MDRK1); });
         KArgs s1 = v1;
-        s1.bc_scale = c->curved ? 1.0 : 0.5;
+        s1.bc_scale = (c->curved && !c->p4est) ? 1.0 : 0.5;
         timed(c, 1, [&] { launch_surface(c, s1); });
         // stage 2: uⁿ⁺¹ = uⁿ + dt rhs_mdrk2(uⁿ, us)  (mdrk.jl:21-24)
         KArgs v2 = a;
@@ -1039,7 +1076,7 @@ thread_local std::string g_noctx_err;
 
 int create_common(int ninst, const tlw_mesh_desc* meshes, const tlw_basis_desc* basis, const tlw_solver_desc* solver,
                   tlw_ctx** out, const tlw_parabolic_desc* par = nullptr, const tlw_curved_desc* cv = nullptr,
-                  const tlw_mortar_desc* md = nullptr);
+                  const tlw_mortar_desc* md = nullptr, const tlw_p4est_desc* pd = nullptr);
 
 }  // namespace
 
@@ -1099,6 +1136,17 @@ int tlw_rebuild_mesh(tlw_ctx* c, const tlw_mesh_desc* mesh) {
     if (c->curved) return fail(c, TLW_ENOTSUP, "a curvilinear context keeps its mesh (create a new one)");
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return upload_meshes(c, mesh, 1);
+}
+
+int tlw_create_p4est(const tlw_mesh_desc* mesh, const tlw_p4est_desc* p4est, const tlw_basis_desc* basis,
+                     const tlw_solver_desc* solver, tlw_ctx** out) {
+    if (!p4est) {
+        if (out) *out = nullptr;
+        return TLW_EINVAL;
+    }
+    const tlw_curved_desc metrics{p4est->contravariant_vectors_host, p4est->inverse_jacobian_host,
+                                  p4est->node_coordinates_host};
+    return create_common(1, mesh, basis, solver, out, nullptr, &metrics, nullptr, p4est);
 }
 
 int tlw_create_nonconforming(const tlw_mesh_desc* mesh, const tlw_mortar_desc* mortars, const tlw_basis_desc* basis,
@@ -1873,7 +1921,8 @@ void tlw_ensemble_destroy(tlw_ensemble* e) {
 namespace {
 
 int create_common(int ninst, const tlw_mesh_desc* meshes, const tlw_basis_desc* basis, const tlw_solver_desc* solver,
-                  tlw_ctx** out, const tlw_parabolic_desc* par, const tlw_curved_desc* cv, const tlw_mortar_desc* md) {
+                  tlw_ctx** out, const tlw_parabolic_desc* par, const tlw_curved_desc* cv, const tlw_mortar_desc* md,
+                  const tlw_p4est_desc* pd) {
     if (!out) return TLW_EINVAL;
     *out = nullptr;
     if (!meshes || !basis || !solver) return TLW_EINVAL;
@@ -1888,6 +1937,10 @@ int create_common(int ninst, const tlw_mesh_desc* meshes, const tlw_basis_desc* 
     if ((solver->boundary_kind == TLW_BV_OUTFLOW || solver->boundary_kind == TLW_BV_SLIP_WALL_APPROXIMATE) &&
         (!cv || par))
         return TLW_ENOTSUP;
+    // P4estMesh: the hyperbolic plain-FR kernels of the curvilinear mesh
+    if (pd && (par || solver->shock_capturing || solver->boundary_kind == TLW_BV_SLIP_WALL ||
+               solver->source_terms != TLW_SRC_NONE || ninst != 1))
+        return TLW_ENOTSUP;
     if (!(solver->gamma > 1.0)) return TLW_EINVAL;
     if (solver->source_terms < TLW_SRC_NONE || solver->source_terms > TLW_SRC_CONVERGENCE_TEST) return TLW_EINVAL;
     // source terms: the Euler LW and MDRK kernels of the TreeMesh
@@ -1896,6 +1949,7 @@ int create_common(int ninst, const tlw_mesh_desc* meshes, const tlw_basis_desc* 
     c->N = basis->polydeg;
     c->n = c->N + 1;
     c->ebv = default_eb(c->N);
+    c->p4est = pd != nullptr;
     c->time_disc = solver->time_discretization;
     c->flags = solver->flags;
     DevConst& h = c->hconst;
@@ -2008,7 +2062,7 @@ int create_common(int ninst, const tlw_mesh_desc* meshes, const tlw_basis_desc* 
     };
     if (hipMalloc(&c->d_const, sizeof(DevConst)) != hipSuccess) return bail(TLW_EHIP);
     if (hipMemcpy(c->d_const, &h, sizeof(DevConst), hipMemcpyHostToDevice) != hipSuccess) return bail(TLW_EHIP);
-    int rc = upload_meshes(c, meshes, ninst, cv, md);
+    int rc = upload_meshes(c, meshes, ninst, cv, md, pd);
     if (rc) return bail(rc);
     // Trixi's alpha smoothing on a StructuredMesh assumes a periodic mesh
     if (c->curved && c->sc && h.alpha_smooth && c->has_boundary) return bail(TLW_ENOTSUP);

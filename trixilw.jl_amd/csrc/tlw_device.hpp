@@ -201,6 +201,10 @@ struct KArgs {
     const int2* mort;
     double* mrec;
     double* mflux;
+    // P4estMesh (null otherwise): per element face its interface partner (element,
+    // face, flipped node order, 1 if this element is the interface's primary);
+    // interface faces carry neighbour code -3, so every face record goes to HBM
+    const int4* plink;
 };
 
 // ---------------------------------------------------------------------------
@@ -2679,12 +2683,17 @@ __device__ inline void boundary_record(const KArgs& a, int e, int face, int l, d
 
 // Outer record of a domain-boundary face node of a curvilinear mesh
 // (calc_boundary_flux_by_direction!, structured/dg_2d.jl:2352-2391, into the
-// Dirichlet call of equations.jl:6-37): the GLL time average of the boundary
-// value and of its flux in the normal direction (Trixi flux(u, n):
-// (ρv_n, ρv_n v1 + p n1, ρv_n v2 + p n2, (ρe + p) v_n)) over the full step,
-// TreeMesh order (jump on uⁿ, λ from Ũ), the element's own face normal.
+// Dirichlet call of equations.jl:6-37): the GLL time averages (over
+// [t, t + s dt], s = a.bc_scale: 1 on a StructuredMesh, 0.5 in MDRK stage 1 of
+// a P4estMesh) of the boundary value and of its flux in the normal direction
+// (Trixi flux(u, n): (ρv_n, ρv_n v1 + p n1, ρv_n v2 + p n2, (ρe + p) v_n)) with
+// the element's own face normal, and λ's parts of the average.  The record's
+// jump state is u_outer(t) (the StructuredMesh call: TreeMesh order, jump on
+// uⁿ, λ from Ũ) or, with p4est_form, the time average itself (the P4est
+// method, equations.jl:66-93).
 template <int N>
-__device__ inline void boundary_record_curved(const KArgs& a, int e, int face, int l, double t, double dt, double* r) {
+__device__ inline void boundary_record_curved(const KArgs& a, int e, int face, int l, double t, double dt, double* r,
+                                              bool p4est_form = false) {
     constexpr int n = N + 1;
     const DevConst* c = a.c;
     const size_t fo = ((size_t)e * 4 + face) * n + l;
@@ -2692,9 +2701,10 @@ __device__ inline void boundary_record_curved(const KArgs& a, int e, int face, i
     double nf[4];
     load4(a.fnorm + 4 * fo, nf);
     const double gm1 = c->gamma - 1.0;
+    const double sf = a.bc_scale, dts = sf * dt;
     double Uo[4] = {0, 0, 0, 0}, Fo[4] = {0, 0, 0, 0}, uo[4];
     for (int q = 0; q < n; ++q) {
-        const double ts = t + 0.5 * dt * (c->nodes[q] + 1.0);
+        const double ts = t + 0.5 * dts * (c->nodes[q] + 1.0);
         double ub[4];
         boundary_state<n>(c, a.bidx, a.btab, a.btab_levels, e, face, l, q, x, y, ts, ub);
         const double rho = ub[0], v1 = ub[1] / rho, v2 = ub[2] / rho;
@@ -2702,15 +2712,15 @@ __device__ inline void boundary_record_curved(const KArgs& a, int e, int face, i
         const double vn = v1 * nf[0] + v2 * nf[1], rvn = rho * vn;
         const double fb[4] = {rvn, rvn * v1 + p * nf[0], rvn * v2 + p * nf[1], (ub[3] + p) * vn};
         for (int v = 0; v < 4; ++v) {
-            Uo[v] += 0.5 * ub[v] * c->weights[q];
-            Fo[v] += 0.5 * fb[v] * c->weights[q];
+            Uo[v] += 0.5 * sf * ub[v] * c->weights[q];
+            Fo[v] += 0.5 * sf * fb[v] * c->weights[q];
         }
     }
-    boundary_state<n>(c, a.bidx, a.btab, a.btab_levels, e, face, l, n, x, y, t, uo);
+    if (!p4est_form) boundary_state<n>(c, a.bidx, a.btab, a.btab_levels, e, face, l, n, x, y, t, uo);
     const double ir = rcp_nr(Uo[0]);
     const double b1 = Uo[1] * ir, b2 = Uo[2] * ir;
     const double pb = gm1 * (Uo[3] - 0.5 * (Uo[1] * b1 + Uo[2] * b2));
-    make_record(r, uo, Fo, fabs(fma(b1, nf[0], b2 * nf[1])), sqrt_gs(c->gamma * pb * ir) * nf[2]);
+    make_record(r, p4est_form ? Uo : uo, Fo, fabs(fma(b1, nf[0], b2 * nf[1])), sqrt_gs(c->gamma * pb * ir) * nf[2]);
 }
 
 // Surface kernel: the faces that leave a volume workgroup (a.ebv elements).
@@ -2794,6 +2804,56 @@ __device__ __forceinline__ void surface_body(const KArgs& a, int bid) {
                     // uⁿ, λ from Ũ; no blending (structured/dg_2d.jl:2352-2391)
                     double nf[4], in10[kTraceQ], out10[kTraceQ];
                     load4(a.fnorm + (((size_t)e * 4 + f) * n + l) * 4, nf);
+                    if (nb == -3) {
+                        // P4estMesh interface (prolong2interfaces!/calc_interface_flux!,
+                        // p4est/dg_2d.jl:8-192): each record holds F̃ in its element's +ξ/+η
+                        // direction (F̃·Ja of its own node); in the primary's outward
+                        // direction that is s_p·F̃_p and -s_s·F̃_s (s = +1 on + faces, -1 on
+                        // - faces); λ's parts from uⁿ, the jump on Ũ.  Both elements form
+                        // the same primary-outward flux; each keeps its +ξ/+η version.
+                        const int4 pl = a.plink[4 * e + f];
+                        const int lp = pl.z ? n - 1 - l : l;
+                        load_record<RQ>(a.trace + (((size_t)pl.x * 4 + pl.y) * n + lp) * RQ, oth);
+                        const double so = (f & 1) ? 1.0 : -1.0, sq = (pl.y & 1) ? 1.0 : -1.0;
+                        const double* P = pl.w ? own : oth;
+                        const double* Q = pl.w ? oth : own;
+                        const double sP = pl.w ? so : sq, sQ = pl.w ? sq : so;
+                        double Lq[kTraceQ], Rq[kTraceQ], Fo_[4];
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) {
+                            Lq[v] = P[v];
+                            Rq[v] = Q[v];
+                            Lq[4 + v] = sP * P[4 + v];
+                            Rq[4 + v] = -sQ * Q[4 + v];
+                        }
+                        Lq[8] = P[8];
+                        Lq[9] = P[9];
+                        Rq[8] = Q[8];
+                        Rq[9] = Q[9];
+                        face_numflux(Lq, Rq, Fo_);
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) Fs[v] = pl.w ? so * Fo_[v] : -so * Fo_[v];
+                        const double wf = (f & 1) ? c->bnd_hi : -c->bnd_lo;
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) surf[v] = fma(Fs[v], wf, surf[v]);
+                        return;
+                    }
+                    if (a.plink) {
+                        // P4estMesh boundary with the time-averaged Dirichlet value (the
+                        // P4est method, equations.jl:66-93): the volume kernel's record as
+                        // the inner state (jump on Ũ, λ from uⁿ), the outer record with the
+                        // time average as both jump state and λ state; inner first along
+                        // the outward normal, i.e. the outer record on the - side in +ξ/+η
+                        if (c->bc_kind != kBcOutflow && c->bc_kind != kBcSlipWallApprox) {
+                            boundary_record_curved<N>(a, e, f, l, t, dt, out10, true);
+                            if (f & 1) face_numflux(own, out10, Fs);
+                            else face_numflux(out10, own, Fs);
+                            const double wf = (f & 1) ? c->bnd_hi : -c->bnd_lo;
+#pragma unroll
+                            for (int v = 0; v < 4; ++v) surf[v] = fma(Fs[v], wf, surf[v]);
+                            return;
+                        }
+                    }
                     if (c->bc_kind == kBcOutflow || c->bc_kind == kBcSlipWallApprox) {
                         // conditions on uⁿ with the node's contravariant vector as normal
                         // (boundary_condition_outflow, slip_wall_approximate,

@@ -91,6 +91,16 @@ struct SolverDesc
     source_terms::Int32                   # TLW_SRC_*
 end
 
+# P4estMesh: the metric terms and Trixi's P4est interface container, node_indices reduced to
+# {primary element, primary face, secondary element, secondary face, flipped} (0-based)
+struct P4estDesc
+    contravariant_vectors_host::Ptr{Float64}
+    inverse_jacobian_host::Ptr{Float64}
+    node_coordinates_host::Ptr{Float64}
+    n_interfaces::Int32
+    interfaces_host::Ptr{Int32}               # [5, I]
+end
+
 # CompressibleNavierStokesDiffusion2D(equations; mu, Prandtl, gradient_variables) and its walls
 struct ParabolicDesc
     equations::Int32                      # TLW_PAR_NAVIER_STOKES
@@ -191,6 +201,15 @@ end
 # AMR: after DiffEqBase.resize! (src/callbacks/amr.jl:2-20) re-upload the host-built mesh
 rebuild_mesh!(ctx, mesh::MeshDesc) =
     check(ctx, ccall((:tlw_rebuild_mesh, lib), Cint, (Ptr{Cvoid}, Ref{MeshDesc}), ctx, mesh))
+
+# P4estMesh (conforming): the curvilinear kernels with the P4est surface terms
+function create_p4est(mesh::MeshDesc, p4est::P4estDesc, basis::BasisDesc, solver::SolverDesc)
+    ctx = Ref{Ptr{Cvoid}}(C_NULL)
+    check(C_NULL, ccall((:tlw_create_p4est, lib), Cint,
+                        (Ref{MeshDesc}, Ref{P4estDesc}, Ref{BasisDesc}, Ref{SolverDesc}, Ref{Ptr{Cvoid}}),
+                        mesh, p4est, basis, solver, ctx))
+    return ctx[]
+end
 
 # TreeMesh with refined cells (mortars), and its AMR re-upload after resize!
 function create_nonconforming(mesh::MeshDesc, mortars::MortarDesc, basis::BasisDesc, solver::SolverDesc)

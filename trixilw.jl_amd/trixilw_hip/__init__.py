@@ -19,7 +19,7 @@ from .equations import (Adiabatic, BoundaryConditionDirichlet, BoundaryCondition
 from .integrator import (LWIntegrator, LWSolution, compute_dt, isapprox, isfinished, perform_step, set_dt,
                          set_t_and_iter, solve_lwfr)
 from .limiters import PositivityPreservingLimiterZhangShu, density, pressure
-from .mesh import StructuredMesh, TreeMesh, coordinates2mapping
+from .mesh import P4estMesh, StructuredMesh, TreeMesh, coordinates2mapping
 from .semidiscretization import (DGSEM, LW, MDRK, Adaptive, CFLBased, FirstOrderReconstruction, IndicatorHennemannGassner,
                                  LWUpdate, MUSCLHancockReconstruction, SemidiscretizationHyperbolic,
                                  SemidiscretizationHyperbolicParabolic, VolumeIntegralFR,
@@ -34,7 +34,7 @@ __all__ = [
     "BoundaryConditionNavierStokesWall", "BoundaryConditionDirichlet",
     "IsentropicVortex", "TaylorGreenVortex", "LWIntegrator", "LWSolution", "compute_dt", "isapprox",
     "isfinished", "perform_step", "set_dt", "set_t_and_iter", "solve_lwfr", "TreeMesh",
-    "StructuredMesh", "coordinates2mapping", "DGSEM", "LW", "MDRK",
+    "StructuredMesh", "P4estMesh", "coordinates2mapping", "DGSEM", "LW", "MDRK",
     "CFLBased", "Adaptive", "LWUpdate", "SemidiscretizationHyperbolic", "VolumeIntegralFR", "flux_lax_friedrichs",
     "get_time_discretization", "lw_fourier_cfl", "max_dt_noalloc", "rhs", "rhs_mdrk1", "rhs_mdrk2", "semidiscretize", "trixi2lw",
     "FirstOrderReconstruction", "IndicatorHennemannGassner", "MUSCLHancockReconstruction",

@@ -36,7 +36,7 @@ EXPORTED = (
     "tlw_set_state_dev", "tlw_get_state_dev", "tlw_boundary_nodes", "tlw_set_boundary_data",
     "tlw_create_ensemble", "tlw_ensemble_set_stream", "tlw_ensemble_enable_graphs", "tlw_ensemble_solve_init",
     "tlw_ensemble_advance", "tlw_ensemble_last_error", "tlw_ensemble_destroy", "tlw_adjoint_mdrk",
-    "tlw_create_nonconforming", "tlw_rebuild_mesh_nonconforming", "tlw_transfer_solution",
+    "tlw_create_nonconforming", "tlw_rebuild_mesh_nonconforming", "tlw_transfer_solution", "tlw_create_p4est",
 )
 
 
@@ -99,6 +99,16 @@ class CurvedDesc(C.Structure):
     ]
 
 
+class P4estDesc(C.Structure):
+    _fields_ = [
+        ("contravariant_vectors_host", C.POINTER(C.c_double)),
+        ("inverse_jacobian_host", C.POINTER(C.c_double)),
+        ("node_coordinates_host", C.POINTER(C.c_double)),
+        ("n_interfaces", C.c_int32),
+        ("interfaces_host", C.POINTER(C.c_int32)),
+    ]
+
+
 class ParabolicDesc(C.Structure):
     _fields_ = [
         ("equations", C.c_int32),
@@ -156,6 +166,9 @@ def lib():
                                            C.POINTER(SolverDesc), C.POINTER(vp)]
     L.tlw_rebuild_mesh_nonconforming.restype = C.c_int
     L.tlw_rebuild_mesh_nonconforming.argtypes = [vp, C.POINTER(MeshDesc), C.POINTER(MortarDesc)]
+    L.tlw_create_p4est.restype = C.c_int
+    L.tlw_create_p4est.argtypes = [C.POINTER(MeshDesc), C.POINTER(P4estDesc), C.POINTER(BasisDesc),
+                                   C.POINTER(SolverDesc), C.POINTER(vp)]
     L.tlw_transfer_solution.restype = C.c_int
     L.tlw_transfer_solution.argtypes = [vp, vp, C.c_int32, vp, C.c_int32, C.POINTER(C.c_int32)]
     L.tlw_destroy.restype = None

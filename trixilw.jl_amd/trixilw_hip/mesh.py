@@ -427,3 +427,89 @@ class StructuredMesh:
 
     def lexicographic_permutation(self):
         return self._element_of.copy()
+
+
+class P4estMesh(StructuredMesh):
+    """P4estMesh(trees_per_dimension, mapping; periodicity, turns): Trixi's
+    P4estMesh of trees_per_dimension cells of the mapped reference square, one
+    element per tree (no refinement).  ``turns`` gives each element's local
+    frame as a multiple of 90° against its cell (all 0 for the aligned mesh Trixi
+    builds from a mapping; other values are the orientations of an unstructured
+    quadrilateral mesh, e.g. one read from an Abaqus file).  The node
+    coordinates and metric terms are those of the element's own frame; the
+    interfaces (primary element and face, secondary element and face, flipped
+    node order) are found by matching face end points in the reference square,
+    periodic directions wrapped.  Faces are numbered 0..3 = -ξ, +ξ, -η, +η of
+    the element's frame; ``neighbors`` holds each interface face's partner and
+    -1 on the domain boundary.  The device evaluates the reference's P4est
+    surface terms (src/solvers/dgsem_p4est/dg_2d.jl:8-393)."""
+
+    p4est = True
+
+    def __init__(self, trees_per_dimension, mapping, coordinates_max=None, periodicity=True, turns=None):
+        super().__init__(trees_per_dimension, mapping, coordinates_max, periodicity=periodicity)
+        E = self.n_elements
+        self.turns = np.zeros(E, dtype=np.int64) if turns is None else np.asarray(turns, dtype=np.int64) % 4
+        if self.turns.shape != (E,):
+            raise ValueError(f"turns must have one entry per element ({E})")
+        self._connect()
+
+    def _turned(self, a, b):
+        """Local (ξ, η) arrays [E, ...] -> the cells' frame."""
+        r = self.turns.reshape((-1,) + (1,) * (np.ndim(a) - 1))
+        ca = np.select([r == 0, r == 1, r == 2, r == 3], [a, -b, -a, b])
+        cb = np.select([r == 0, r == 1, r == 2, r == 3], [b, a, -b, -a])
+        return ca, cb
+
+    def _reference_xy(self, xi, eta):
+        dx, dy = 2.0 / self.nx, 2.0 / self.ny
+        ox = (-1.0 + self.ix * dx + dx / 2).reshape((-1,) + (1,) * (np.ndim(xi) - 1))
+        oy = (-1.0 + self.iy * dy + dy / 2).reshape((-1,) + (1,) * (np.ndim(xi) - 1))
+        a, b = self._turned(xi, eta)
+        return ox + dx / 2 * a, oy + dy / 2 * b
+
+    def node_coordinates(self, nodes):
+        """x, y of every node in the element's own frame, arrays [e, j, i]."""
+        nodes = np.asarray(nodes, dtype=np.float64)
+        n, E = len(nodes), self.n_elements
+        xi = np.broadcast_to(nodes[None, None, :], (E, n, n))
+        eta = np.broadcast_to(nodes[None, :, None], (E, n, n))
+        rx, ry = self._reference_xy(xi, eta)
+        x, y = self.mapping(rx, ry)
+        return np.ascontiguousarray(x, dtype=np.float64), np.ascontiguousarray(y, dtype=np.float64)
+
+    def _connect(self):
+        E = self.n_elements
+        # end points of each face (first and last face node, local coordinates)
+        ends = {0: ((-1, -1), (-1, 1)), 1: ((1, -1), (1, 1)), 2: ((-1, -1), (1, -1)), 3: ((-1, 1), (1, 1))}
+        tol = 1e-9
+
+        def canon(v, periodic):
+            if periodic and v > 1.0 - tol:
+                v -= 2.0
+            return round(v, 8) + 0.0
+
+        table = {}
+        nbr = np.full((E, 4), -1, dtype=np.int32)
+        for f in range(4):
+            (a0, b0), (a1, b1) = ends[f]
+            p0 = self._reference_xy(np.full(E, float(a0)), np.full(E, float(b0)))
+            p1 = self._reference_xy(np.full(E, float(a1)), np.full(E, float(b1)))
+            for e in range(E):
+                q0 = (canon(p0[0][e], self.periodicity[0]), canon(p0[1][e], self.periodicity[1]))
+                q1 = (canon(p1[0][e], self.periodicity[0]), canon(p1[1][e], self.periodicity[1]))
+                table.setdefault(frozenset((q0, q1)), []).append((e, f, q0))
+        interfaces = []
+        for sides in table.values():
+            if len(sides) == 1:
+                continue
+            if len(sides) != 2:
+                raise ValueError("a face is shared by more than two elements")
+            (e1, f1, s1), (e2, f2, s2) = sorted(sides)
+            nbr[e1, f1], nbr[e2, f2] = e2, e1
+            interfaces.append((e1, f1, e2, f2, 0 if s1 == s2 else 1))
+        self.neighbors = nbr
+        self.interfaces = np.array(sorted(interfaces), dtype=np.int32).reshape(-1, 5)
+
+    def lexicographic_permutation(self):
+        return np.arange(self.n_elements)

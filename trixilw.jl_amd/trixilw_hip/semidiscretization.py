@@ -256,7 +256,8 @@ class SemidiscretizationHyperbolic:
             return
         nodes = self.solver.basis.nodes
         n = len(nodes)
-        s = 0.5 if (stage == 1 and not getattr(self.mesh, "curved", False)) else 1.0
+        half = not getattr(self.mesh, "curved", False) or getattr(self.mesh, "p4est", False)
+        s = 0.5 if (stage == 1 and half) else 1.0
         dts = s * dt
         x, y = self._bxy[:, 0], self._bxy[:, 1]
         vals = np.empty((len(x), n + 1, 4))
@@ -266,6 +267,16 @@ class SemidiscretizationHyperbolic:
         self._upload_boundary(vals, n + 1)
 
     def _create(self, md, basis, sd, h):
+        if getattr(self.mesh, "p4est", False):
+            k = self._keep
+            k["interfaces"] = np.ascontiguousarray(self.mesh.interfaces, dtype=np.int32).reshape(-1)
+            pd = _lib.P4estDesc(_dptr(k["Ja"]), _dptr(k["ijn"]), _dptr(k["xy"]), int(len(self.mesh.interfaces)),
+                                k["interfaces"].ctypes.data_as(C.POINTER(C.c_int32)))
+            rc = _lib.lib().tlw_create_p4est(C.byref(md), C.byref(pd), C.byref(basis), C.byref(sd), C.byref(h))
+            if rc != _lib.TLW_OK:
+                raise _lib.TLWError(rc, "tlw_create_p4est failed (Euler, VolumeIntegralFR, a consistent interface "
+                                        "list; boundary faces need a boundary condition)")
+            return
         if getattr(self.mesh, "curved", False):
             k = self._keep
             cd = _lib.CurvedDesc(_dptr(k["Ja"]), _dptr(k["ijn"]), _dptr(k["xy"]))
