@@ -166,7 +166,9 @@ typedef struct {
     int32_t walls;
     double wall_velocity[8];
     /* per direction: 0 Adiabatic(0), 1 Isothermal(wall_temperature): the reference's isothermal wall
-     * (compressible_navier_stokes_2d.jl:93-111) returns TLW_ENOTSUP on the device (set 0) */
+     * (compressible_navier_stokes_2d.jl:93-111): the gradient's boundary value prim2cons(ρ, v_wall,
+     * ρ T_wall) (the reference's R = 1) and the inner viscous flux (F̃v1..F̃v4) as the wall's viscous flux;
+     * a temperature that is not finite and > 0 is TLW_EINVAL */
     int32_t wall_isothermal[4];
     double wall_temperature[4];
 } tlw_parabolic_desc;

@@ -1,5 +1,5 @@
-"""GPU parity of the Navier-Stokes LW path with no-slip adiabatic walls
-(BoundaryConditionNavierStokesWall{NoSlip, Adiabatic}, conservative gradient
+"""GPU parity of the Navier-Stokes LW path with no-slip adiabatic and isothermal walls
+(BoundaryConditionNavierStokesWall{NoSlip, Adiabatic | Isothermal}, conservative gradient
 variables, with the reflecting wall for the hyperbolic part: the reference's
 lid-driven cavity, examples/tree_2d_dgsem/elixir_navierstokes_lid_driven_cavity.jl)
 against the CPU oracle: residuals of a perturbed cavity state and of plane
@@ -25,7 +25,10 @@ def cavity_state(x, y, t=0.0):
     return np.stack([rho, z, z, p / (GAMMA - 1)], axis=-1)
 
 
-def setup_walls(oracle_mod, N, level, mu=0.01, periodic_x=False, ic=cavity_state, lid=LID, td="lw"):
+def setup_walls(oracle_mod, N, level, mu=0.01, periodic_x=False, ic=cavity_state, lid=LID, td="lw",
+                heat=(None, None, None, None)):
+    """``heat``: per direction (x_neg, x_pos, y_neg, y_pos) None for Adiabatic(0) or a
+    wall temperature for Isothermal(T)."""
     import trixilw_hip as tlw
 
     eq = tlw.CompressibleEulerEquations2D(GAMMA)
@@ -33,18 +36,20 @@ def setup_walls(oracle_mod, N, level, mu=0.01, periodic_x=False, ic=cavity_state
                                                  gradient_variables=tlw.GradientVariablesConservative())
     mesh = tlw.TreeMesh((-1.0, -1.0), (1.0, 1.0), level, periodicity=(periodic_x, False))
     solver = tlw.DGSEM(N, volume_integral=tlw.VolumeIntegralFR(tlw.MDRK() if td == "mdrk" else tlw.LW()))
-    rest = tlw.BoundaryConditionNavierStokesWall(tlw.NoSlip(lambda x, t, e: (0.0, 0.0)), tlw.Adiabatic(0.0))
-    moving = tlw.BoundaryConditionNavierStokesWall(tlw.NoSlip(lambda x, t, e: lid), tlw.Adiabatic(0.0))
+    def wall(v, T):
+        return tlw.BoundaryConditionNavierStokesWall(tlw.NoSlip(lambda x, t, e: v),
+                                                     tlw.Adiabatic(0.0) if T is None else tlw.Isothermal(T))
     bc_hyp = {"x_neg": tlw.boundary_condition_slip_wall_vertical, "x_pos": tlw.boundary_condition_slip_wall_vertical,
               "y_neg": tlw.boundary_condition_slip_wall_horizontal,
               "y_pos": tlw.boundary_condition_slip_wall_horizontal}
-    bc_par = {"x_neg": rest, "x_pos": rest, "y_neg": rest, "y_pos": moving}
+    bc_par = {"x_neg": wall((0.0, 0.0), heat[0]), "x_pos": wall((0.0, 0.0), heat[1]),
+              "y_neg": wall((0.0, 0.0), heat[2]), "y_pos": wall(lid, heat[3])}
     semi = tlw.SemidiscretizationHyperbolicParabolic(mesh, tlw.get_time_discretization(solver), (eq, eqp), ic,
                                                      solver, boundary_conditions=(bc_hyp, bc_par))
     nx = mesh.nx
     o = oracle_mod.Oracle(N, nx, nx, -1.0, -1.0, 2.0 / nx, gamma=GAMMA, periodic=(periodic_x, False),
                           bc_kind=oracle_mod.BV_SLIP_WALL, parabolic="navier_stokes", mu=mu, prandtl=PR,
-                          ns_walls=[(0.0, 0.0), (0.0, 0.0), (0.0, 0.0), lid])
+                          ns_walls=[(0.0, 0.0), (0.0, 0.0), (0.0, 0.0), lid], wall_temperature=list(heat))
     return semi, o, mesh
 
 
@@ -139,11 +144,11 @@ def test_ns_walls_reject_unsupported(oracle_mod):
     with pytest.raises(_lib.TLWError) as ei:  # MDRK with walls
         setup_walls(oracle_mod, 3, 2, td="mdrk")
     assert ei.value.code == _lib.TLW_ENOTSUP
-    # isothermal walls are restated in the oracle only
+    # an isothermal wall needs a finite positive temperature
     eq0 = tlw.CompressibleEulerEquations2D(GAMMA)
     eqp0 = tlw.CompressibleNavierStokesDiffusion2D(eq0, mu=0.01, Prandtl=PR,
                                                   gradient_variables=tlw.GradientVariablesConservative())
-    hot = tlw.BoundaryConditionNavierStokesWall(tlw.NoSlip((0.0, 0.0)), tlw.Isothermal(1.2))
+    hot = tlw.BoundaryConditionNavierStokesWall(tlw.NoSlip((0.0, 0.0)), tlw.Isothermal(-1.0))
     bc_hyp = (tlw.boundary_condition_slip_wall_vertical, tlw.boundary_condition_slip_wall_vertical,
               tlw.boundary_condition_slip_wall_horizontal, tlw.boundary_condition_slip_wall_horizontal)
     with pytest.raises(_lib.TLWError) as ei:
@@ -151,7 +156,7 @@ def test_ns_walls_reject_unsupported(oracle_mod):
                                                   tlw.LW(), (eq0, eqp0), cavity_state,
                                                   tlw.DGSEM(3, volume_integral=tlw.VolumeIntegralFR(tlw.LW())),
                                                   boundary_conditions=(bc_hyp, (hot,) * 4))
-    assert ei.value.code == _lib.TLW_ENOTSUP
+    assert ei.value.code == _lib.TLW_EINVAL
     eq = tlw.CompressibleEulerEquations2D(GAMMA)
     eqp = tlw.CompressibleNavierStokesDiffusion2D(eq, mu=0.01, Prandtl=PR,
                                                  gradient_variables=tlw.GradientVariablesConservative())
@@ -161,3 +166,48 @@ def test_ns_walls_reject_unsupported(oracle_mod):
         tlw.SemidiscretizationHyperbolicParabolic(mesh, tlw.LW(), (eq, eqp), cavity_state, solver,
                                                   boundary_conditions=(tlw.boundary_condition_slip_wall_vertical,
                                                                        None))
+
+
+# cavity temperature T = p/ρ (the reference's R = 1) is 1/(Ma² γ) ≈ 71.4: walls a little hotter or colder
+HOT = (70.0, 73.0, 71.0, 72.5)
+
+
+@pytest.mark.parametrize("N", [3, 4])
+@pytest.mark.parametrize("heat", [HOT, (None, 73.0, 70.5, None)])
+def test_isothermal_cavity_rhs_parity(oracle_mod, N, heat):
+    """Isothermal walls (compressible_navier_stokes_2d.jl:93-111): the gradient's
+    boundary value prim2cons(ρ, v_wall, ρ T_wall) and the inner viscous flux as the
+    wall's viscous flux, alone and mixed with adiabatic walls."""
+    import trixilw_hip as tlw
+
+    torch = torch_mod()
+    print(f"NS isothermal walls rhs N={N} heat={heat}", flush=True)
+    semi, o, mesh = setup_walls(oracle_mod, N, 3, heat=heat)
+    u_lex = perturbed(o, 60 + N)
+    u = to_dev(mesh, u_lex)
+    du = torch.empty_like(u)
+    dt = 0.4 * o.max_dt(u_lex)
+    semi.cache["dt"][0] = dt
+    du_ref = o.rhs(u_lex, 0.0, dt)
+    tlw.rhs(du, u, semi, 0.0)
+    torch.cuda.synchronize()
+    assert_close(to_lex(mesh, du), du_ref, what=f"NS isothermal walls N={N}")
+    semi.close()
+
+
+@pytest.mark.parametrize("N,K", [(4, 20), (3, 25)])
+def test_isothermal_cavity_device_loop_k_steps(oracle_mod, N, K):
+    import trixilw_hip as tlw
+
+    print(f"NS isothermal walls device loop N={N} K={K}", flush=True)
+    semi, o, mesh = setup_walls(oracle_mod, N, 3, heat=HOT)
+    u_lex = perturbed(o, 70 + N, amp=0.02)
+    cfl = 0.5
+    u_ref, t_ref, _ = o.solve(u_lex, 0.0, 1e9, cfl, max_steps=K)
+    lw_update = tlw.semidiscretize(semi, semi.time_discretization, (0.0, float("inf")))
+    lw_update.u0.copy_(to_dev(mesh, u_lex))
+    sol = tlw.solve_lwfr(lw_update, (), 1e-6, None, time_step_computation=tlw.CFLBased(cfl), max_steps=K)
+    assert sol.stats.naccept == K
+    assert abs(sol.t - t_ref) <= 1e-13 * t_ref
+    assert_close(to_lex(mesh, sol.u), u_ref, what=f"NS isothermal walls N={N} after {K} steps")
+    semi.close()

@@ -506,7 +506,7 @@ int upload_meshes(tlw_ctx* c, const tlw_mesh_desc* ms, int ninst, const tlw_curv
     HIPCHK(c, hipMalloc(&c->d_sblk, sizeof(int4) * (size_t)scap));
     HIPCHK(c, hipMalloc(&c->d_etile, sizeof(int) * (size_t)E));
     HIPCHK(c, hipMalloc(&c->d_rng, sizeof(int4) * (size_t)ninst));
-    if (c->hconst.ns_walls) HIPCHK(c, hipMalloc(&c->d_fvb, sizeof(double) * (size_t)E * 4 * c->n * 2));
+    if (c->hconst.ns_walls) HIPCHK(c, hipMalloc(&c->d_fvb, sizeof(double) * (size_t)E * 4 * c->n * 6));
     if (pd) {
         HIPCHK(c, hipMalloc(&c->d_plink, sizeof(int4) * plink.size()));
         HIPCHK(c, hipMemcpy(c->d_plink, plink.data(), sizeof(int4) * plink.size(), hipMemcpyHostToDevice));
@@ -661,8 +661,11 @@ template <int N, int EB, bool SC, bool PAR = false> void launch_volume_eb(tlw_ct
     else if (kind == VOL_MDRK2)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_MDRK2, SC, PAR>), g, b, 0, c->stream, a);
     else if (kind == VOL_LW_WALL) {
-        if constexpr (PAR && !SC)
+        // the wall kernel runs at one wave per SIMD: tiles of at most 4 waves
+        if constexpr (PAR && !SC && VGeo<N, EB>::NT <= 256)
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_LW_WALL, SC, PAR>), g, b, 0, c->stream, a);
+        else
+            c->bad_launch = true;
     } else if constexpr (N == 4)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_LW_ADAPT, SC, PAR>), g, b, 0, c->stream, a);
 }
@@ -1569,6 +1572,8 @@ int tlw_set_elements_per_workgroup(tlw_ctx* c, int32_t elements) {
     if (!eb_supported(c->N, elements))
         return fail(c, TLW_EINVAL, c->N == 4 ? "elements per workgroup must be 5, 10 or 15 for polydeg 4"
                                              : "elements per workgroup must be 4, 8 or 16 for polydeg 3");
+    if (c->hconst.ns_walls && elements * c->n * c->n > 256)
+        return fail(c, TLW_EINVAL, "Navier-Stokes walls: at most 256 nodes per workgroup (polydeg 4: 5 or 10)");
     if (elements == c->ebv) return TLW_OK;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->ebv = elements;
@@ -1606,6 +1611,7 @@ int tlw_autotune(tlw_ctx* c, int32_t reps, int32_t* chosen, double* ms_out) {
     double best_ms = 0.0;
     int rc = TLW_OK;
     for (int k = 0; k < 3 && rc == TLW_OK; ++k) {
+        if (c->hconst.ns_walls && eb_choices(c->N)[k] * c->n * c->n > 256) continue;  // not a wall tile
         rc = tlw_set_elements_per_workgroup(c, eb_choices(c->N)[k]);
         if (rc) break;
         rc = rhs_common(c, kind, u, scratch, 0.0, dt);  // warm-up
@@ -2022,12 +2028,18 @@ int create_common(int ninst, const tlw_mesh_desc* meshes, const tlw_basis_desc* 
         // gradient_variables = GradientVariablesConservative()), constant μ
         const bool ok = par->equations == TLW_PAR_NAVIER_STOKES && par->gradient_variables == TLW_GRAD_CONSERVATIVE &&
                         std::isfinite(par->mu) && par->mu >= 0.0 && std::isfinite(par->prandtl) && par->prandtl > 0.0;
-        // no-slip adiabatic walls (LW, TreeMesh, with the reflecting wall of the hyperbolic part)
-        // (isothermal walls are not implemented here: TLW_ENOTSUP)
-        bool iso = false;
-        for (int k = 0; k < 4; ++k) iso = iso || (par->walls && par->wall_isothermal[k]);
+        // no-slip adiabatic or isothermal walls (LW, TreeMesh, with the reflecting wall of the
+        // hyperbolic part); wall temperatures must be finite and positive
+        bool temps_ok = true;
+        for (int k = 0; k < 4; ++k)
+            if (par->walls && par->wall_isothermal[k])
+                temps_ok = temps_ok && std::isfinite(par->wall_temperature[k]) && par->wall_temperature[k] > 0.0;
         const bool walls_ok = !par->walls || (solver->time_discretization == TLW_LW && !cv &&
-                                              solver->boundary_kind == TLW_BV_SLIP_WALL && !iso);
+                                              solver->boundary_kind == TLW_BV_SLIP_WALL);
+        if (!temps_ok) {
+            delete c;
+            return TLW_EINVAL;
+        }
         const bool supported = !c->sc && ninst == 1 && walls_ok;
         if (!ok || !supported) {
             delete c;
@@ -2038,7 +2050,13 @@ int create_common(int ninst, const tlw_mesh_desc* meshes, const tlw_basis_desc* 
         h.visc_Pr = par->prandtl;
         h.visc_kappa = solver->gamma * (1.0 / (solver->gamma - 1.0)) / par->prandtl;
         h.ns_walls = par->walls ? 1 : 0;
+        // the wall kernel runs at one wave per SIMD: a tile of at most 4 waves
+        if (h.ns_walls && c->ebv * c->n * c->n > 256) c->ebv = 10;  // polydeg 4 (polydeg 3 tiles fit)
         for (int k = 0; k < 8; ++k) h.wall_v[k] = par->walls ? par->wall_velocity[k] : 0.0;
+        for (int k = 0; k < 4; ++k) {
+            h.wall_iso[k] = (par->walls && par->wall_isothermal[k]) ? 1 : 0;
+            h.wall_T[k] = h.wall_iso[k] ? par->wall_temperature[k] : 0.0;
+        }
 
     }
     if (cv) {

@@ -83,6 +83,9 @@ struct DevConst {
     // wall velocity (v1, v2) per face 0..3 (x_neg, x_pos, y_neg, y_pos)
     int ns_walls;
     double wall_v[8];
+    // per direction: 1 for Isothermal(wall_T) (NoSlip with a wall temperature), 0 Adiabatic(0)
+    int wall_iso[4];
+    double wall_T[4];
     // MUSCL-Hancock subcell geometry per node i (compute_subcells): spacings
     // to the neighbouring nodes, the centred-difference weights, the distances
     // from the node to its subcell faces, and hw[k] = xs[k+2] - xs[k+1], the
@@ -173,8 +176,9 @@ struct KArgs {
     const double* fnorm;
     const double* fxy;  // boundary face-node coordinates [E][4][n][2] (curvilinear mesh with boundaries)
     double* slots;      // per instance: kSlots CFL maxima, then kSlots viscous maxima (k_finalize resets them)
-    // Navier-Stokes walls: per boundary face node the momentum components of the
-    // time-averaged viscous flux in the face's orientation [E][4][n][2]
+    // Navier-Stokes walls: per boundary face node the time-averaged viscous flux
+    // of the face's orientation (momentum x, momentum y, energy), then the same
+    // components at uⁿ [E][4][n][6]
     double* fvb;
     // curvilinear shock capturing: per node the subcell normals of its lower ξ
     // and η subcell faces [E][n²][4], and the calc_fn_low! normals of subcell
@@ -1417,10 +1421,14 @@ __device__ __forceinline__ void br1_gradient(const double* s, const double* src,
     // at a no-slip wall the interface value is the wall state prim2cons(ρ, v_wall, p)
     // of the node (Trixi calc_boundary_flux_by_direction_gradient! with
     // compressible_navier_stokes_2d.jl:113-124)
+    // the gradient's boundary value at a no-slip wall: prim2cons(ρ, v_wall, p) with the
+    // node's pressure (Adiabatic) or p = ρ T_wall (Isothermal, the reference's R = 1;
+    // compressible_navier_stokes_2d.jl:93-101)
     auto wall_state = [&](int f, double* uo) {
         const double vw1 = c->wall_v[2 * f], vw2 = c->wall_v[2 * f + 1];
         const double rho = uown[0];
-        const double p = (c->gamma - 1.0) * (uown[3] - 0.5 * (uown[1] * uown[1] + uown[2] * uown[2]) / rho);
+        const double p = c->wall_iso[f] ? rho * c->wall_T[f]
+                                        : (c->gamma - 1.0) * (uown[3] - 0.5 * (uown[1] * uown[1] + uown[2] * uown[2]) / rho);
         uo[0] = rho;
         uo[1] = rho * vw1;
         uo[2] = rho * vw2;
@@ -1503,13 +1511,16 @@ struct ParStates {
         ns_total_fluxes(u[q], x[q], y[q], gm1, c, f, g);
     }
     // momentum components of the viscous fluxes of state q: (fv1, fv2, gv1, gv2)
-    __device__ __forceinline__ void visc12(int q, double gm1, const DevConst* c, double* o) const {
+    // components 1..3 of the viscous fluxes of state q: (fx1, fx2, fx3, fy1, fy2, fy3)
+    __device__ __forceinline__ void visc6(int q, double gm1, const DevConst* c, double* out) const {
         double fx[4], fy[4];
         ns_visc_fluxes(u[q], x[q], y[q], gm1, c, fx, fy);
-        o[0] = fx[1];
-        o[1] = fx[2];
-        o[2] = fy[1];
-        o[3] = fy[2];
+        out[0] = fx[1];
+        out[1] = fx[2];
+        out[2] = fx[3];
+        out[3] = fy[1];
+        out[4] = fy[2];
+        out[5] = fy[3];
     }
 };
 
@@ -1558,23 +1569,66 @@ __device__ __forceinline__ void lwp_element(const double* uN, const double* gxN,
                                             double gm1, const DevConst* c, int i, int j, bool act, int sidx,
                                             int sbase, double* s0, double* s1, double* s2, const double* sDm,
                                             const double* sD, const double* sDh, double* Uc, double* F, double* G,
-                                            double* vol, const double* Jn = nullptr, double* vw = nullptr,
+                                            double* vol, const double* Jn = nullptr, double* const* wq = nullptr,
                                             double* terr = nullptr, double abstol = 0.0, double reltol = 0.0) {
     using G_ = VGeo<N, EB>;
     constexpr int n = G_::n, S = G_::S;
     ns_total_fluxes(uN, gxN, gyN, gm1, c, F, G);
-    // with walls: the time average of (fv1, fv2, gv1, gv2) alongside F̃ (v0: at uⁿ)
-    double v0[4];
+    // with walls: the time average of the viscous flux of a wall face node in the
+    // face's orientation, alongside F̃, accumulated in the node's wall record
+    // wq[o] (o = 0: its x face, 1: its y face; null where that face is no wall):
+    // [time average (3), value at uⁿ (3)].  Only wall lanes touch memory here.
     if constexpr (WALL) {
         double fx[4], fy[4];
         ns_visc_fluxes(uN, gxN, gyN, gm1, c, fx, fy);
-        v0[0] = fx[1];
-        v0[1] = fx[2];
-        v0[2] = fy[1];
-        v0[3] = fy[2];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) vw[k] = v0[k];
+        for (int o = 0; o < 2; ++o) {
+            double* q = wq[o];
+            if (!q) continue;
+            const double* f = o ? fy : fx;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                q[k] = f[1 + k];
+                q[3 + k] = f[1 + k];
+            }
+        }
     }
+    // one stage of the wall average: weight · scale · (cv v(uⁿ) + Σ c_s v(state s)),
+    // the states u+, u-, u++, u-- one at a time, both orientations from one
+    // viscous-flux evaluation; only wall lanes enter
+    auto wall_stage = [&](const ParStates& Ps, double c0, double c1, double c2, double c3, double cv, double scale,
+                          double weight, bool outer) {
+        if constexpr (WALL) {
+            if (!wq[0] && !wq[1]) return;
+            double d[6], a6[6];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                d[k] = wq[0] ? cv * wq[0][3 + k] : 0.0;
+                d[3 + k] = wq[1] ? cv * wq[1][3 + k] : 0.0;
+            }
+            Ps.visc6(0, gm1, c, a6);
+#pragma unroll
+            for (int k = 0; k < 6; ++k) d[k] = fma(c0, a6[k], d[k]);
+            Ps.visc6(1, gm1, c, a6);
+#pragma unroll
+            for (int k = 0; k < 6; ++k) d[k] = fma(c1, a6[k], d[k]);
+            if (outer) {
+                Ps.visc6(2, gm1, c, a6);
+#pragma unroll
+                for (int k = 0; k < 6; ++k) d[k] = fma(c2, a6[k], d[k]);
+                Ps.visc6(3, gm1, c, a6);
+#pragma unroll
+                for (int k = 0; k < 6; ++k) d[k] = fma(c3, a6[k], d[k]);
+            }
+#pragma unroll
+            for (int o = 0; o < 2; ++o) {
+                double* q = wq[o];
+                if (!q) continue;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) q[k] = fma(weight, scale * d[3 * o + k], q[k]);
+            }
+        }
+    };
     double w[4], wx[4], wy[4];
     par_level<N, EB, CURV>(act, sidx, sbase, i, j, invJ, s0, s2, sDm, sD, F, G, w, wx, wy, Jn);  // ut
     // on a curvilinear mesh s0 took the contravariant pair; the centred stencils
@@ -1607,15 +1661,8 @@ __device__ __forceinline__ void lwp_element(const double* uN, const double* gxN,
         F[v] = fma(0.5, dx[v], F[v]);
         G[v] = fma(0.5, dy[v], G[v]);
     }
-    if constexpr (WALL) {  // f_t of the viscous part, weight 1/2
-        double a0[4], a1[4], a2[4], a3[4];
-        P.visc12(0, gm1, c, a0);
-        P.visc12(1, gm1, c, a1);
-        P.visc12(2, gm1, c, a2);
-        P.visc12(3, gm1, c, a3);
-#pragma unroll
-        for (int k = 0; k < 4; ++k) vw[k] = fma(0.5, (1.0 / 12.0) * (-a2[k] + 8.0 * a0[k] - 8.0 * a1[k] + a3[k]), vw[k]);
-    }
+    // f_t of the viscous part, weight 1/2
+    wall_stage(P, 8.0, -8.0, -1.0, 1.0, 0.0, 1.0 / 12.0, 0.5, true);
     par_level<N, EB, CURV>(act, sidx, sbase, i, j, invJ, s1, s2, sDm, sD, dx, dy, w, wx, wy, Jn);  // utt
     P.add(0.5, 0.5, 2.0, 2.0, w, wx, wy);
 #pragma unroll
@@ -1659,23 +1706,9 @@ __device__ __forceinline__ void lwp_element(const double* uN, const double* gxN,
         F[v] = fma(1.0 / 6.0, dx[v], F[v]);
         G[v] = fma(1.0 / 6.0, dy[v], G[v]);
     }
-    if constexpr (WALL) {  // f_tt of the viscous part, weight 1/6
-        double a0[4], a1[4];
-        P.visc12(0, gm1, c, a0);
-        P.visc12(1, gm1, c, a1);
-        if constexpr (N == 4) {
-            double a2[4], a3[4];
-            P.visc12(2, gm1, c, a2);
-            P.visc12(3, gm1, c, a3);
-#pragma unroll
-            for (int k = 0; k < 4; ++k)
-                vw[k] = fma(1.0 / 6.0,
-                            (1.0 / 12.0) * (-a2[k] + 16.0 * a0[k] - 30.0 * v0[k] + 16.0 * a1[k] - a3[k]), vw[k]);
-        } else {
-#pragma unroll
-            for (int k = 0; k < 4; ++k) vw[k] = fma(1.0 / 6.0, a0[k] - 2.0 * v0[k] + a1[k], vw[k]);
-        }
-    }
+    // f_tt of the viscous part, weight 1/6 (five points at N = 4, three at N = 3)
+    if constexpr (N == 4) wall_stage(P, 16.0, 16.0, -1.0, -1.0, -30.0, 1.0 / 12.0, 1.0 / 6.0, true);
+    else wall_stage(P, 1.0, 1.0, 0.0, 0.0, -2.0, 1.0, 1.0 / 6.0, false);
     par_level<N, EB, CURV>(act, sidx, sbase, i, j, invJ, s1, s2, sDm, sD, dx, dy, w, wx, wy, Jn);  // uttt
     P.add(1.0 / 6.0, -1.0 / 6.0, 4.0 / 3.0, -4.0 / 3.0, w, wx, wy);
 #pragma unroll
@@ -1698,15 +1731,8 @@ __device__ __forceinline__ void lwp_element(const double* uN, const double* gxN,
         F[v] = fma(1.0 / 24.0, dx[v], F[v]);
         G[v] = fma(1.0 / 24.0, dy[v], G[v]);
     }
-    if constexpr (WALL) {  // f_ttt of the viscous part, weight 1/24
-        double a0[4], a1[4], a2[4], a3[4];
-        P.visc12(0, gm1, c, a0);
-        P.visc12(1, gm1, c, a1);
-        P.visc12(2, gm1, c, a2);
-        P.visc12(3, gm1, c, a3);
-#pragma unroll
-        for (int k = 0; k < 4; ++k) vw[k] = fma(1.0 / 24.0, 0.5 * (a2[k] - 2.0 * a0[k] + 2.0 * a1[k] - a3[k]), vw[k]);
-    }
+    // f_ttt of the viscous part, weight 1/24
+    wall_stage(P, -2.0, 2.0, 1.0, -1.0, 0.0, 0.5, 1.0 / 24.0, true);
     if constexpr (N == 4) {
         par_level<N, EB, CURV>(act, sidx, sbase, i, j, invJ, s1, s2, sDm, sD, dx, dy, w, wx, wy, Jn);  // utttt
         P.add(1.0 / 24.0, 1.0 / 24.0, 2.0 / 3.0, 2.0 / 3.0, w, wx, wy);
@@ -1740,16 +1766,8 @@ __device__ __forceinline__ void lwp_element(const double* uN, const double* gxN,
         // the total f_tttt for the temporal error
         if (terr && act) put8<S>(s0, sidx, dx, dy);
     }
-    if constexpr (WALL && N == 4) {  // f_tttt of the viscous part (with the 0.5), weight 1/120
-        double a0[4], a1[4], a2[4], a3[4];
-        P.visc12(0, gm1, c, a0);
-        P.visc12(1, gm1, c, a1);
-        P.visc12(2, gm1, c, a2);
-        P.visc12(3, gm1, c, a3);
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-            vw[k] = fma(1.0 / 120.0, 0.5 * (a2[k] - 4.0 * a0[k] + 6.0 * v0[k] - 4.0 * a1[k] + a3[k]), vw[k]);
-    }
+    // f_tttt of the viscous part (with the 0.5), weight 1/120
+    if constexpr (N == 4) wall_stage(P, -4.0, -4.0, 1.0, 1.0, 6.0, 0.5, 1.0 / 120.0, true);
     __syncthreads();  // every thread has finished reading s2 before it is reused
     if constexpr (CURV) {
         double Fc[4], Gc[4];
@@ -2307,7 +2325,13 @@ __device__ __forceinline__ void volume_body(const KArgs& a, int bid, double* lds
     double terr = 0.0;  // temporal error of the node (adaptive steps, LW N=4)
     load4(uin + off, uN);
     double Jn[4] = {0.0, 0.0, 0.0, 0.0};  // Ja11, Ja12, Ja21, Ja22 of the node: same layout as the state
-    double vwall[4] = {0.0, 0.0, 0.0, 0.0};  // walls: time-averaged (fv1, fv2, gv1, gv2)
+    // walls: the node's wall records (its x face, its y face; null where that face is no wall)
+    double* wq[2] = {nullptr, nullptr};
+    if constexpr (KIND == VOL_LW_WALL) {
+        const int wfx = (i == 0) ? 0 : ((i == n - 1) ? 1 : -1), wfy = (j == 0) ? 2 : ((j == n - 1) ? 3 : -1);
+        if (act && wfx >= 0 && a.nbr[4 * e + wfx] == -1) wq[0] = a.fvb + (((size_t)e * 4 + wfx) * n + j) * 6;
+        if (act && wfy >= 0 && a.nbr[4 * e + wfy] == -1) wq[1] = a.fvb + (((size_t)e * 4 + wfy) * n + i) * 6;
+    }
     if constexpr (CURV) load4(a.Ja + off, Jn);
     if constexpr (CURV && !PAR) {
         if constexpr (KIND == VOL_LW) {
@@ -2343,10 +2367,10 @@ __device__ __forceinline__ void volume_body(const KArgs& a, int bid, double* lds
             br1_gradient<N, EB, CURV>(sg, uin, a.nbr, c, e, i, j, sbase, invJ, sDh, uN, gxN, gyN, Jn);
             if constexpr (KIND == VOL_LW || KIND == VOL_LW_WALL)
                 lwp_element<N, EB, CURV, KIND == VOL_LW_WALL>(uN, gxN, gyN, invJ, gm1, c, i, j, act, sidx, sbase, s0,
-                                                             s1, s2, sDm, sD, sDh, Uc, F, G, vol, Jn, vwall);
+                                                             s1, s2, sDm, sD, sDh, Uc, F, G, vol, Jn, wq);
             else if constexpr (KIND == VOL_LW_ADAPT)
                 lwp_element<N, EB, CURV, false>(uN, gxN, gyN, invJ, gm1, c, i, j, act, sidx, sbase, s0, s1, s2, sDm,
-                                                sD, sDh, Uc, F, G, vol, Jn, vwall, &terr, st->abstol, st->reltol);
+                                                sD, sDh, Uc, F, G, vol, Jn, wq, &terr, st->abstol, st->reltol);
             else
                 mdrkp1_element<N, EB, CURV>(uN, gxN, gyN, invJ, gm1, c, i, j, act, sidx, sbase, s0, s1, s2, sDm, sD,
                                             sDh, Uc, F, G, vol, Jn);
@@ -2464,19 +2488,6 @@ __device__ __forceinline__ void volume_body(const KArgs& a, int bid, double* lds
     const double* lamst = tree ? Uc : uN;
     const int nbx = fx >= 0 ? a.nbr[4 * e + fx] : -1;
     const int nby = fy >= 0 ? a.nbr[4 * e + fy] : -1;
-    if constexpr (KIND == VOL_LW_WALL) {
-        // the viscous time average of wall face nodes, for the surface kernel's wall flux
-        if (act && fx >= 0 && nbx < 0) {
-            double* q = a.fvb + (((size_t)e * 4 + fx) * n + j) * 2;
-            q[0] = vwall[0];
-            q[1] = vwall[1];
-        }
-        if (act && fy >= 0 && nby < 0) {
-            double* q = a.fvb + (((size_t)e * 4 + fy) * n + i) * 2;
-            q[0] = vwall[2];
-            q[1] = vwall[3];
-        }
-    }
     if constexpr (CURV) {
         // structured order with the interface's normal n (shared by both
         // elements): F̃·n, |v(uⁿ)·n| and c(uⁿ)|n| (dg_2d.jl:2229-2264)
@@ -2635,8 +2646,13 @@ __device__ __forceinline__ void volume_body(const KArgs& a, int bid, double* lds
     // kernel's: it reads every node once after the last correction
 }
 
+// the Navier-Stokes wall kernel (a rarely used configuration: the lid-driven
+// cavity's walls) runs at one wave per SIMD, so its extra wall averages live in
+// the accumulation registers instead of private memory
+template <int KIND, bool SC, bool PAR>
+constexpr int kVolWavesOf = KIND == VOL_LW_WALL ? 1 : kVolWaves<SC || PAR || (KIND >= VOL_LW_SRC)>;
 template <int N, int EB, int KIND, bool SC, bool PAR = false, bool CURV = false>
-__global__ __launch_bounds__((VGeo<N, EB>::NT), kVolWaves<SC || PAR || (KIND >= VOL_LW_SRC)>) void k_volume(KArgs a) {
+__global__ __launch_bounds__((VGeo<N, EB>::NT), (kVolWavesOf<KIND, SC, PAR>)) void k_volume(KArgs a) {
     using G_ = VGeo<N, EB>;
     __shared__ double lds[G_::LDS];
     __shared__ double sDm[G_::NN], sDh[G_::NN], sD[G_::NN];
@@ -2907,14 +2923,15 @@ __device__ __forceinline__ void surface_body(const KArgs& a, int bid) {
                     // reflecting wall (equations.jl:95-139): the mirrored record, whatever
                     // the flux order (both states are mirrored, the speeds kept)
                     const int m = (f < 2) ? 1 : 2;
-                    double fvw[2] = {0.0, 0.0};
+                    double fvw[3] = {0.0, 0.0, 0.0};
                     if (c->ns_walls) {
                         // Navier-Stokes: the record holds F̃a - F̃v; the wall flux is built
                         // from F̃a (its normal-momentum component: the others cancel against
                         // their mirror images), then the wall's viscous flux is subtracted
-                        const double* q = a.fvb + (((size_t)e * 4 + f) * n + l) * 2;
+                        const double* q = a.fvb + (((size_t)e * 4 + f) * n + l) * 6;
                         fvw[0] = q[0];
                         fvw[1] = q[1];
+                        fvw[2] = q[2];
                         own[4 + m] += fvw[m - 1];
                     }
 #pragma unroll
@@ -2927,10 +2944,12 @@ __device__ __forceinline__ void surface_body(const KArgs& a, int bid) {
                     face_numflux(Lr, Rr, Fs);
                     if (c->ns_walls) {
                         // NoSlip/Adiabatic (compressible_navier_stokes_2d.jl:81-91):
-                        // (F̃v1, F̃v2, F̃v3, v_wall · (F̃v2, F̃v3)), F̃v1 = 0
+                        // (F̃v1, F̃v2, F̃v3, v_wall · (F̃v2, F̃v3)), F̃v1 = 0; NoSlip/Isothermal
+                        // (103-109): the inner viscous flux itself
                         Fs[1] -= fvw[0];
                         Fs[2] -= fvw[1];
-                        Fs[3] -= fma(c->wall_v[2 * f], fvw[0], c->wall_v[2 * f + 1] * fvw[1]);
+                        Fs[3] -= c->wall_iso[f] ? fvw[2]
+                                                : fma(c->wall_v[2 * f], fvw[0], c->wall_v[2 * f + 1] * fvw[1]);
                     }
                 } else {
                     boundary_record<N>(a, e, f, l, t, dt, oth);
