@@ -160,9 +160,10 @@ typedef struct {
      * boundary face (compressible_navier_stokes_2d.jl:81-124: the gradient's boundary value
      * prim2cons(ρ, v_wall, p) of the node, the viscous wall flux (F̃v1, F̃v2, F̃v3, v_wall·(F̃v2, F̃v3))
      * subtracted from the hyperbolic wall flux, dg_2d_parabolic.jl:1951-2085), with the wall velocity
-     * (v1, v2) of x_neg, x_pos, y_neg, y_pos in wall_velocity; LW on a TreeMesh with
-     * solver->boundary_kind = TLW_BV_SLIP_WALL (the reference's lid-driven cavity), TLW_ENOTSUP
-     * otherwise */
+     * (v1, v2) of x_neg, x_pos, y_neg, y_pos in wall_velocity; LW or MDRK (each stage's own
+     * viscous time average at the wall, multi_derivative_2d_parabolic.jl:59-218) on a TreeMesh
+     * with solver->boundary_kind = TLW_BV_SLIP_WALL (the reference's lid-driven cavity),
+     * TLW_ENOTSUP otherwise */
     int32_t walls;
     double wall_velocity[8];
     /* per direction: 0 Adiabatic(0), 1 Isothermal(wall_temperature): the reference's isothermal wall

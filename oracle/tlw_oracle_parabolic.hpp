@@ -800,7 +800,6 @@ template <class T, class Eq, class Visc> struct ParabolicSolver : Solver<T, Eq> 
         this->surface_integral_and_jacobian(du);
     }
     void rhs_mdrk1(T* du, const T* u, double t, double dt) override {
-        (void)t;
         this->ensure_mdrk();
         if (Fv2.size() != Fv.size()) Fv2.assign(Fv.size(), T(0));
         this->reset_du(du);
@@ -808,10 +807,16 @@ template <class T, class Eq, class Visc> struct ParabolicSolver : Solver<T, Eq> 
 #pragma omp parallel for schedule(static)
         for (int e = 0; e < mesh.E; ++e) mdrk_parabolic_1(du, u, dt, e);
         calc_interface_flux_hp(u);
+        if (ns_walls && !mesh.bd_elem.empty()) {
+            // multi_derivative_2d_parabolic.jl:110-126: the hyperbolic wall flux with
+            // the stage's scaling 0.5 (which the reflecting wall does not read), then
+            // the wall's viscous flux from the stage's F̃v (f_v/2 + f_v,t/8)
+            this->calc_boundary_flux(u, t, dt, 0.5);
+            wall_divergence_flux();
+        }
         this->surface_integral_and_jacobian(du);
     }
     void rhs_mdrk2(T* du, const T* u, double t, double dt) override {
-        (void)t;
         this->ensure_mdrk();
         if (Fv2.size() != Fv.size()) Fv2.assign(Fv.size(), T(0));
         this->reset_du(du);
@@ -819,6 +824,12 @@ template <class T, class Eq, class Visc> struct ParabolicSolver : Solver<T, Eq> 
 #pragma omp parallel for schedule(static)
         for (int e = 0; e < mesh.E; ++e) mdrk_parabolic_2(du, dt, e);
         calc_interface_flux_hp(u);
+        if (ns_walls && !mesh.bd_elem.empty()) {
+            // multi_derivative_2d_parabolic.jl:192-207: scaling 1, the stage's F̃v
+            // (f_v + f_v,t/6 + f_v,t(us)/3) at the wall
+            this->calc_boundary_flux(u, t, dt, 1.0);
+            wall_divergence_flux();
+        }
         this->surface_integral_and_jacobian(du);
     }
 

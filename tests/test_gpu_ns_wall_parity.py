@@ -1,10 +1,11 @@
-"""GPU parity of the Navier-Stokes LW path with no-slip adiabatic and isothermal walls
+"""GPU parity of the Navier-Stokes LW and MDRK paths with no-slip adiabatic and isothermal walls
 (BoundaryConditionNavierStokesWall{NoSlip, Adiabatic | Isothermal}, conservative gradient
 variables, with the reflecting wall for the hyperbolic part: the reference's
 lid-driven cavity, examples/tree_2d_dgsem/elixir_navierstokes_lid_driven_cavity.jl)
 against the CPU oracle: residuals of a perturbed cavity state and of plane
-Couette flow, K-step device-loop states within 1e-12 relative (L2 and L∞),
-and the combinations the path rejects.  Inputs are seeded."""
+Couette flow, both MDRK stages (multi_derivative_2d_parabolic.jl:59-218),
+K-step device-loop states within 1e-12 relative (L2 and L∞), and the
+combinations the path rejects.  Inputs are seeded."""
 import numpy as np
 import pytest
 
@@ -141,9 +142,6 @@ def test_ns_walls_reject_unsupported(oracle_mod):
     from trixilw_hip import _lib
 
     torch_mod()
-    with pytest.raises(_lib.TLWError) as ei:  # MDRK with walls
-        setup_walls(oracle_mod, 3, 2, td="mdrk")
-    assert ei.value.code == _lib.TLW_ENOTSUP
     # an isothermal wall needs a finite positive temperature
     eq0 = tlw.CompressibleEulerEquations2D(GAMMA)
     eqp0 = tlw.CompressibleNavierStokesDiffusion2D(eq0, mu=0.01, Prandtl=PR,
@@ -210,4 +208,87 @@ def test_isothermal_cavity_device_loop_k_steps(oracle_mod, N, K):
     assert sol.stats.naccept == K
     assert abs(sol.t - t_ref) <= 1e-13 * t_ref
     assert_close(to_lex(mesh, sol.u), u_ref, what=f"NS isothermal walls N={N} after {K} steps")
+    semi.close()
+
+
+@pytest.mark.parametrize("N", [3, 4])
+@pytest.mark.parametrize("heat", [(None,) * 4, (None, 73.0, 70.5, None)])
+def test_cavity_mdrk_stage_parity(oracle_mod, N, heat):
+    """Both MDRK stages with walls: stage 1 takes the walls' viscous average
+    f_v/2 + f_v,t/8 (and the hyperbolic wall with its 0.5 scaling), stage 2
+    f_v + f_v,t/6 + f_v,t(us)/3 with the BR1 gradient of us at the walls."""
+    import trixilw_hip as tlw
+
+    torch = torch_mod()
+    print(f"NS walls MDRK stages N={N} heat={heat}", flush=True)
+    semi, o, mesh = setup_walls(oracle_mod, N, 3, td="mdrk", heat=heat)
+    u_lex = perturbed(o, 80 + N)
+    dt = 0.4 * o.max_dt(u_lex)
+    d1 = o.rhs(u_lex, 0.0, dt, kind="mdrk1")
+    us = u_lex + dt * d1
+    o.set_us(us)
+    d2 = o.rhs(u_lex, 0.0, dt, kind="mdrk2")
+    u = to_dev(mesh, u_lex)
+    du = torch.empty_like(u)
+    semi.cache["dt"][0] = dt
+    tlw.rhs_mdrk1(du, u, semi, 0.0)
+    torch.cuda.synchronize()
+    assert_close(to_lex(mesh, du), d1, what=f"NS walls mdrk1 N={N}")
+    semi.set_us(to_dev(mesh, us))
+    tlw.rhs_mdrk2(du, u, semi, 0.0)
+    torch.cuda.synchronize()
+    assert_close(to_lex(mesh, du), d2, what=f"NS walls mdrk2 N={N}")
+    semi.close()
+
+
+@pytest.mark.parametrize("N", [3, 4])
+def test_couette_mdrk_stage_parity(oracle_mod, N):
+    """Couette flow: stage 1 heats at half the viscous rate, stage 2 at the full
+    rate; mass and momentum stay."""
+    import trixilw_hip as tlw
+
+    torch = torch_mod()
+    print(f"NS walls Couette MDRK N={N}", flush=True)
+    U, mu = 0.5, 0.02
+
+    def couette(x, y, t=0.0):
+        v1 = U * (y + 1.0) / 2.0
+        z = np.zeros_like(x)
+        return np.stack([z + 1.0, v1, z, z + 1.0 / (GAMMA - 1) + 0.5 * v1 ** 2], axis=-1)
+
+    semi, o, mesh = setup_walls(oracle_mod, N, 2, mu=mu, periodic_x=True, ic=couette, lid=(U, 0.0), td="mdrk")
+    x, y = o.node_coords()
+    u_lex = couette(x, y)
+    dt = 0.5 * o.max_dt(u_lex)
+    u = to_dev(mesh, u_lex)
+    du = torch.empty_like(u)
+    semi.cache["dt"][0] = dt
+    tlw.rhs_mdrk1(du, u, semi, 0.0)
+    torch.cuda.synchronize()
+    g1 = to_lex(mesh, du)
+    semi.set_us(to_dev(mesh, u_lex + dt * g1))
+    tlw.rhs_mdrk2(du, u, semi, 0.0)
+    torch.cuda.synchronize()
+    g2 = to_lex(mesh, du)
+    for g, weight in ((g1, 0.5), (g2, 1.0)):
+        assert np.max(np.abs(g[..., :3])) < 1e-12
+        assert np.max(np.abs(g[..., 3] - weight * mu * (U / 2.0) ** 2)) < 1e-10
+    semi.close()
+
+
+@pytest.mark.parametrize("N,K,heat", [(4, 15, (None,) * 4), (3, 20, HOT)])
+def test_cavity_mdrk_device_loop_k_steps(oracle_mod, N, K, heat):
+    import trixilw_hip as tlw
+
+    print(f"NS walls MDRK device loop N={N} K={K} heat={heat}", flush=True)
+    semi, o, mesh = setup_walls(oracle_mod, N, 3, td="mdrk", heat=heat)
+    u_lex = perturbed(o, 90 + N, amp=0.02)
+    cfl = 0.25
+    u_ref, t_ref, _ = o.solve(u_lex, 0.0, 1e9, cfl, max_steps=K, mdrk=True)
+    lw_update = tlw.semidiscretize(semi, semi.time_discretization, (0.0, float("inf")))
+    lw_update.u0.copy_(to_dev(mesh, u_lex))
+    sol = tlw.solve_lwfr(lw_update, (), 1e-6, None, time_step_computation=tlw.CFLBased(cfl), max_steps=K)
+    assert sol.stats.naccept == K
+    assert abs(sol.t - t_ref) <= 1e-13 * t_ref
+    assert_close(to_lex(mesh, sol.u), u_ref, what=f"NS walls MDRK N={N} after {K} steps")
     semi.close()

@@ -14,6 +14,13 @@ lid-driven cavity; no GPU).
   gradient there is the exact one plus (u_outer - u)(2/h)/w_n (the lifting of
   the full difference; an average would lift half of it).
 - Mass is conserved with walls; a fluid at rest stays at rest.
+- MDRK with walls (multi_derivative_2d_parabolic.jl:59-218: both stages call
+  the hyperbolic wall flux and the wall's viscous flux from the stage's time
+  average F̃v): in Couette flow the energy residual of stage 1 is half the
+  viscous heating (F̃ = f/2 + f_t/8 with f_t = 0) and that of stage 2 the full
+  heating; mass is conserved by both stages; an MDRK solve with walls and an
+  LW solve at small steps reach the same state (the walls of the LW path are
+  pinned above, and the difference of the two time integrations is O(dt)).
 """
 import numpy as np
 import pytest
@@ -133,3 +140,58 @@ def test_isothermal_walls_pass_the_heat_flux(oracle_mod, N):
     # (2e-8 seen at the wall elements: the dt-sized terms of the hydrostatic imbalance)
     assert np.max(np.abs(du_iso[..., 3])) < 1e-6
     assert np.max(np.abs(du_adi[..., 3])) > 1e-3
+
+
+@pytest.mark.parametrize("N", [3, 4])
+def test_mdrk_couette_stages_heat_at_their_weights(oracle_mod, N):
+    U = 0.3
+    o = walled(oracle_mod, N, 4, lid=U)
+    x, y = o.node_coords()
+    u = shear(x, y, 0.0, U / H)
+    dt = 0.5 * o.max_dt(u)
+    heat = MU * U ** 2 / H ** 2
+    d1 = o.rhs(u, 0.0, dt, kind="mdrk1")
+    o.set_us(u + dt * d1)
+    d2 = o.rhs(u, 0.0, dt, kind="mdrk2")
+    for d, weight in ((d1, 0.5), (d2, 1.0)):
+        assert np.max(np.abs(d[..., :3])) < 1e-13
+        assert np.max(np.abs(d[..., 3] - weight * heat)) < 1e-10
+
+
+@pytest.mark.parametrize("N", [3, 4])
+def test_mdrk_walls_conserve_mass(oracle_mod, N):
+    from trixilw_hip.equations import IsentropicVortex
+
+    o = walled(oracle_mod, N, 4, periodic_x=False)
+    x, y = o.node_coords()
+    ic = IsentropicVortex(center=(0.5, 0.5), period=(1e9, 1e9), strength=1.0)
+    rng = np.random.default_rng(11 + N)
+    u = ic(x, y, 0.0) * (1 + 1e-2 * rng.standard_normal(x.shape + (4,)))
+    _, w, _, _ = o.basis()
+    W = np.outer(w, w)[None]
+    dt = 0.25 * o.max_dt(u)
+    d1 = o.rhs(u, 0.0, dt, kind="mdrk1")
+    o.set_us(u + dt * d1)
+    d2 = o.rhs(u, 0.0, dt, kind="mdrk2")
+    for d in (d1, d2):
+        assert abs(np.einsum("eji,eji->", W, d[..., 0])) <= 1e-13 * np.einsum("eji,eji->", W, np.abs(d[..., 0]))
+    u1, _, dts = o.solve(u, 0.0, 1e9, 0.2, max_steps=10, mdrk=True)
+    assert len(dts) == 10
+    m0, m1 = np.einsum("eji,eji->", W, u[..., 0]), np.einsum("eji,eji->", W, u1[..., 0])
+    assert abs(m1 - m0) <= 1e-14 * m0
+
+
+@pytest.mark.parametrize("N", [3, 4])
+def test_mdrk_with_walls_agrees_with_lw_at_small_steps(oracle_mod, N):
+    """A cavity with a moving lid from a density bump at rest: the MDRK and LW
+    solves at CFL 0.025 reach t = 0.05 within 1e-7 (each moves by 6e-5 from
+    CFL 0.025 to 0.00625); MDRK without stage 2's wall fluxes breaks this bound
+    (stage 1 enters the step at O(dt²): the Couette stages pin it)."""
+    o = walled(oracle_mod, N, 4, periodic_x=False)
+    x, y = o.node_coords()
+    rho = 1.0 + 0.1 * np.sin(np.pi * x) * np.sin(np.pi * y)
+    u = np.stack([rho, 0 * x, 0 * x, 1 / 0.4 + 0 * x], axis=-1)
+    u_md, t_md, _ = o.solve(u, 0.0, 0.05, 0.025, mdrk=True)
+    u_lw, t_lw, _ = o.solve(u, 0.0, 0.05, 0.025, mdrk=False)
+    assert t_md == t_lw == 0.05
+    assert np.max(np.abs(u_md - u_lw)) < 1e-7

@@ -660,12 +660,18 @@ template <int N, int EB, bool SC, bool PAR = false> void launch_volume_eb(tlw_ct
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_MDRK1, SC, PAR>), g, b, 0, c->stream, a);
     else if (kind == VOL_MDRK2)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_MDRK2, SC, PAR>), g, b, 0, c->stream, a);
-    else if (kind == VOL_LW_WALL) {
-        // the wall kernel runs at one wave per SIMD: tiles of at most 4 waves
-        if constexpr (PAR && !SC && VGeo<N, EB>::NT <= 256)
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_LW_WALL, SC, PAR>), g, b, 0, c->stream, a);
-        else
+    else if (vol_is_wall(kind)) {
+        // the LW wall kernel runs at one wave per SIMD: tiles of at most 4 waves
+        if constexpr (PAR && !SC && VGeo<N, EB>::NT <= 256) {
+            if (kind == VOL_LW_WALL)
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_LW_WALL, SC, PAR>), g, b, 0, c->stream, a);
+            else if (kind == VOL_MDRK1_WALL)
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_MDRK1_WALL, SC, PAR>), g, b, 0, c->stream, a);
+            else
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_MDRK2_WALL, SC, PAR>), g, b, 0, c->stream, a);
+        } else {
             c->bad_launch = true;
+        }
     } else if constexpr (N == 4)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_LW_ADAPT, SC, PAR>), g, b, 0, c->stream, a);
 }
@@ -719,7 +725,9 @@ template <int N> void launch_volume_t(tlw_ctx* c, const KArgs& a, int kind) {
             launch_volume_curved<N, false>(c, a, kind);
         }
     } else if (c->parabolic) {
-        launch_volume_sc<N, false, true>(c, a, (kind == VOL_LW && c->hconst.ns_walls) ? (int)VOL_LW_WALL : kind);
+        // walls: the instantiations that keep the wall face nodes' viscous averages
+        const int wk = kind == VOL_LW ? VOL_LW_WALL : kind == VOL_MDRK1 ? VOL_MDRK1_WALL : VOL_MDRK2_WALL;
+        launch_volume_sc<N, false, true>(c, a, (c->hconst.ns_walls && kind != VOL_LW_ADAPT) ? wk : kind);
     } else if (c->sc) {
         // IndicatorHennemannGassner on uⁿ before the volume integral (dg_2d.jl:1665)
         hipLaunchKernelGGL(k_indicator<N>, dim3(c->sgrid), dim3(kSurfBlock), 0, c->stream, a, c->d_alpha_tmp);
@@ -2028,14 +2036,13 @@ int create_common(int ninst, const tlw_mesh_desc* meshes, const tlw_basis_desc* 
         // gradient_variables = GradientVariablesConservative()), constant μ
         const bool ok = par->equations == TLW_PAR_NAVIER_STOKES && par->gradient_variables == TLW_GRAD_CONSERVATIVE &&
                         std::isfinite(par->mu) && par->mu >= 0.0 && std::isfinite(par->prandtl) && par->prandtl > 0.0;
-        // no-slip adiabatic or isothermal walls (LW, TreeMesh, with the reflecting wall of the
-        // hyperbolic part); wall temperatures must be finite and positive
+        // no-slip adiabatic or isothermal walls (LW or MDRK, TreeMesh, with the reflecting wall
+        // of the hyperbolic part); wall temperatures must be finite and positive
         bool temps_ok = true;
         for (int k = 0; k < 4; ++k)
             if (par->walls && par->wall_isothermal[k])
                 temps_ok = temps_ok && std::isfinite(par->wall_temperature[k]) && par->wall_temperature[k] > 0.0;
-        const bool walls_ok = !par->walls || (solver->time_discretization == TLW_LW && !cv &&
-                                              solver->boundary_kind == TLW_BV_SLIP_WALL);
+        const bool walls_ok = !par->walls || (!cv && solver->boundary_kind == TLW_BV_SLIP_WALL);
         if (!temps_ok) {
             delete c;
             return TLW_EINVAL;

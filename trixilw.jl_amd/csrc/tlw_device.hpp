@@ -124,14 +124,25 @@ enum BufSel { SEL_EXPLICIT = 0, SEL_CUR = 1, SEL_OTHER = 2 };
 enum Reduce { RED_NONE = 0, RED_STEP = 1, RED_INIT = 2, RED_QUERY = 3, RED_ADAPT = 4 };
 // VOL_LW_ADAPT: the LW kernel with the temporal-error estimate of adaptive steps
 // (a separate instantiation, so the plain LW kernel carries none of its registers)
-// VOL_LW_WALL: the Navier-Stokes LW kernel that also keeps the viscous time
-// average of boundary face nodes (no-slip walls)
+// VOL_LW_WALL, VOL_MDRK1_WALL, VOL_MDRK2_WALL: the Navier-Stokes LW and MDRK
+// kernels that also keep the viscous time average of boundary face nodes
+// (no-slip walls)
 // VOL_*_SRC: the Euler LW and MDRK kernels with source terms (separate
 // instantiations, VOL_LW_SRC + the plain kind)
 enum VolKind {
     VOL_LW = 0, VOL_MDRK1 = 1, VOL_MDRK2 = 2, VOL_LW_ADAPT = 3, VOL_LW_WALL = 4,
-    VOL_LW_SRC = 5, VOL_MDRK1_SRC = 6, VOL_MDRK2_SRC = 7
+    VOL_LW_SRC = 5, VOL_MDRK1_SRC = 6, VOL_MDRK2_SRC = 7, VOL_MDRK1_WALL = 8, VOL_MDRK2_WALL = 9
 };
+constexpr bool vol_is_src(int k) { return k >= VOL_LW_SRC && k <= VOL_MDRK2_SRC; }
+constexpr bool vol_is_wall(int k) { return k == VOL_LW_WALL || k == VOL_MDRK1_WALL || k == VOL_MDRK2_WALL; }
+// the plain kind of a source-term or wall instantiation
+constexpr int vol_base(int k) {
+    return vol_is_src(k)           ? k - VOL_LW_SRC
+           : k == VOL_LW_WALL      ? (int)VOL_LW
+           : k == VOL_MDRK1_WALL   ? (int)VOL_MDRK1
+           : k == VOL_MDRK2_WALL   ? (int)VOL_MDRK2
+                                   : k;
+}
 // source-term kinds of DevConst::src_kind
 enum SrcKind { SRC_NONE = 0, SRC_CONVERGENCE_TEST = 1 };
 
@@ -1798,12 +1809,16 @@ __device__ __forceinline__ void lwp_element(const double* uN, const double* gxN,
     }
 }
 
-// f_t of the states w ± wt, w ± 2wt (and their gradients) for the MDRK stages
+// f_t of the states w ± wt, w ± 2wt (and their gradients) for the MDRK stages;
+// WALL: also f_t of components 1..3 of the viscous fluxes in v6 (x, then y),
+// from the same evaluations
+template <bool WALL = false>
 __device__ __forceinline__ void mdrkp_ft(const double* u, const double* ut, const double* gx, const double* gtx,
                                          const double* gy, const double* gty, double gm1, const DevConst* c,
-                                         double* dx, double* dy) {
+                                         double* dx, double* dy, double* v6 = nullptr) {
     const double cq[4] = {2.0, 1.0, -1.0, -2.0};
     const double wq[4] = {-1.0, 8.0, -8.0, 1.0};
+    double a6[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int v = 0; v < 4; ++v) dx[v] = dy[v] = 0.0;
 #pragma unroll
@@ -1815,7 +1830,23 @@ __device__ __forceinline__ void mdrkp_ft(const double* u, const double* ut, cons
             sx[v] = fma(cq[q], gtx[v], gx[v]);
             sy[v] = fma(cq[q], gty[v], gy[v]);
         }
-        ns_total_fluxes(s, sx, sy, gm1, c, fa, ga);
+        if constexpr (WALL) {
+            double fx[4], fy[4];
+            euler_fluxes(s, gm1, fa, ga);
+            ns_visc_fluxes(s, sx, sy, gm1, c, fx, fy);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                fa[v] -= fx[v];
+                ga[v] -= fy[v];
+            }
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                a6[k] = fma(wq[q], fx[1 + k], a6[k]);
+                a6[3 + k] = fma(wq[q], fy[1 + k], a6[3 + k]);
+            }
+        } else {
+            ns_total_fluxes(s, sx, sy, gm1, c, fa, ga);
+        }
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             dx[v] = fma(wq[q], fa[v], dx[v]);
@@ -1827,24 +1858,66 @@ __device__ __forceinline__ void mdrkp_ft(const double* u, const double* ut, cons
         dx[v] *= 1.0 / 12.0;
         dy[v] *= 1.0 / 12.0;
     }
+    if constexpr (WALL) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) v6[k] = a6[k] * (1.0 / 12.0);
+    }
+}
+
+// No-slip walls: the wall records of a face node, wq[o] (o = 0: its x face,
+// 1: its y face; null where that face is no wall), take the time average of
+// components 1..3 of the viscous flux in the face's orientation: q = w·v6
+// (first term) or q += w·v6.  Only wall lanes touch memory.
+__device__ __forceinline__ void wall_acc(double* const* wq, double w, const double* v6, bool first) {
+#pragma unroll
+    for (int o = 0; o < 2; ++o) {
+        double* q = wq[o];
+        if (!q) continue;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) q[k] = first ? w * v6[3 * o + k] : fma(w, v6[3 * o + k], q[k]);
+    }
+}
+// components 1..3 of the viscous fluxes of a wall face node's state into its
+// wall records with weight w (the first term of the average)
+__device__ __forceinline__ void wall_first(double* const* wq, double w, const double* u, const double* gx,
+                                           const double* gy, double gm1, const DevConst* c) {
+    if (!wq[0] && !wq[1]) return;
+    double fx[4], fy[4], v6[6];
+    ns_visc_fluxes(u, gx, gy, gm1, c, fx, fy);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        v6[k] = fx[1 + k];
+        v6[3 + k] = fy[1 + k];
+    }
+    wall_acc(wq, w, v6, true);
 }
 
 // mdrk_kernel_1! with viscous terms (multi_derivative_2d_parabolic.jl:221-464):
-// F̃ = f/2 + f_t/8, Ũ = u/2 + ut/8 on the total flux.
-template <int N, int EB, bool CURV = false>
+// F̃ = f/2 + f_t/8, Ũ = u/2 + ut/8 on the total flux.  WALL: the wall records
+// of the node take the stage's viscous average f_v/2 + f_v,t/8 (the Fv the
+// reference's prolong2boundaries_visc_lw! reads).
+template <int N, int EB, bool CURV = false, bool WALL = false>
 __device__ __forceinline__ void mdrkp1_element(const double* uN, const double* gxN, const double* gyN, double invJ,
                                                double gm1, const DevConst* c, int i, int j, bool act, int sidx,
                                                int sbase, double* s0, double* s1, double* s2, const double* sDm,
                                                const double* sD, const double* sDh, double* Uc, double* F,
-                                               double* G, double* vol, const double* Jn = nullptr) {
+                                               double* G, double* vol, const double* Jn = nullptr,
+                                               double* const* wq = nullptr) {
     using G_ = VGeo<N, EB>;
     constexpr int n = G_::n, S = G_::S;
     double f[4], g[4];
     ns_total_fluxes(uN, gxN, gyN, gm1, c, f, g);
+    if constexpr (WALL) wall_first(wq, 0.5, uN, gxN, gyN, gm1, c);
     double ut[4], utx[4], uty[4];
     par_level<N, EB, CURV>(act, sidx, sbase, i, j, invJ, s0, s2, sDm, sD, f, g, ut, utx, uty, Jn);
     double dx[4], dy[4];
-    mdrkp_ft(uN, ut, gxN, utx, gyN, uty, gm1, c, dx, dy);
+    if constexpr (WALL) {
+        double v6[6];
+        mdrkp_ft<true>(uN, ut, gxN, utx, gyN, uty, gm1, c, dx, dy, v6);
+        wall_acc(wq, 0.125, v6, false);
+    } else {
+        mdrkp_ft(uN, ut, gxN, utx, gyN, uty, gm1, c, dx, dy);
+    }
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         F[v] = fma(0.125, dx[v], 0.5 * f[v]);
@@ -1865,18 +1938,21 @@ __device__ __forceinline__ void mdrkp1_element(const double* uN, const double* g
 
 // mdrk_kernel_2! with viscous terms (multi_derivative_2d_parabolic.jl:506-711):
 // F̃ = F2 + f_t(us)/3, Ũ = U2 + ust/3, with F2 = f + f_t/6 and U2 = u + ut/6
-// recomputed from uⁿ and its BR1 gradient.
-template <int N, int EB, bool CURV = false>
+// recomputed from uⁿ and its BR1 gradient.  WALL: the wall records take the
+// stage's viscous average f_v + f_v,t/6 + f_v,t(us)/3.
+template <int N, int EB, bool CURV = false, bool WALL = false>
 __device__ __forceinline__ void mdrkp2_element(const double* uN, const double* gxN, const double* gyN,
                                                const double* usn, const double* gxS, const double* gyS,
                                                double invJ, double gm1, const DevConst* c, int i, int j, bool act,
                                                int sidx, int sbase, double* s0, double* s1, double* s2,
                                                const double* sDm, const double* sD, const double* sDh, double* Uc,
-                                               double* F, double* G, double* vol, const double* Jn = nullptr) {
+                                               double* F, double* G, double* vol, const double* Jn = nullptr,
+                                               double* const* wq = nullptr) {
     using G_ = VGeo<N, EB>;
     constexpr int n = G_::n, S = G_::S;
     double f[4], g[4], fs[4], gs[4];
     ns_total_fluxes(uN, gxN, gyN, gm1, c, f, g);
+    if constexpr (WALL) wall_first(wq, 1.0, uN, gxN, gyN, gm1, c);
     ns_total_fluxes(usn, gxS, gyS, gm1, c, fs, gs);
     if constexpr (CURV) {
         double cx[4], cy[4];
@@ -1918,14 +1994,26 @@ __device__ __forceinline__ void mdrkp2_element(const double* uN, const double* g
         }
     }
     double dx[4], dy[4];
-    mdrkp_ft(uN, ut, gxN, utx, gyN, uty, gm1, c, dx, dy);
+    if constexpr (WALL) {
+        double v6[6];
+        mdrkp_ft<true>(uN, ut, gxN, utx, gyN, uty, gm1, c, dx, dy, v6);
+        wall_acc(wq, 1.0 / 6.0, v6, false);
+    } else {
+        mdrkp_ft(uN, ut, gxN, utx, gyN, uty, gm1, c, dx, dy);
+    }
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         F[v] = fma(1.0 / 6.0, dx[v], f[v]);
         G[v] = fma(1.0 / 6.0, dy[v], g[v]);
         Uc[v] = fma(1.0 / 6.0, ut[v], uN[v]);
     }
-    mdrkp_ft(usn, ust, gxS, stx, gyS, sty, gm1, c, dx, dy);
+    if constexpr (WALL) {
+        double v6[6];
+        mdrkp_ft<true>(usn, ust, gxS, stx, gyS, sty, gm1, c, dx, dy, v6);
+        wall_acc(wq, 1.0 / 3.0, v6, false);
+    } else {
+        mdrkp_ft(usn, ust, gxS, stx, gyS, sty, gm1, c, dx, dy);
+    }
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         F[v] = fma(1.0 / 3.0, dx[v], F[v]);
@@ -2273,7 +2361,7 @@ __device__ __forceinline__ void volume_body(const KArgs& a, int bid, double* lds
     static_assert(!(SC && PAR), "shock capturing has no hyperbolic-parabolic form");
     static_assert(!CURV || !PAR || KIND != VOL_LW_ADAPT, "adaptive steps: Euler LW, or Navier-Stokes LW on a TreeMesh");
     static_assert(!PAR || KIND != VOL_LW_ADAPT || N == 4, "the parabolic temporal error is kernel 4's");
-    static_assert(KIND != VOL_LW_WALL || (PAR && !CURV && !SC), "walls: the TreeMesh Navier-Stokes LW kernel");
+    static_assert(!vol_is_wall(KIND) || (PAR && !CURV && !SC), "walls: the TreeMesh Navier-Stokes kernels");
     static_assert(!(CURV && PAR && SC), "no shock capturing with parabolic terms");
     using G_ = VGeo<N, EB>;
     constexpr int n = G_::n, NN = G_::NN, S = G_::S, NT = G_::NT;
@@ -2304,8 +2392,9 @@ __device__ __forceinline__ void volume_body(const KArgs& a, int bid, double* lds
     // source terms: separate instantiations (KIND = VOL_LW_SRC + the plain kind);
     // the node's coordinates from the element's centre and edge length, the
     // time of the call; Ssrc = S, added to the residual at the end
-    constexpr bool SRC = KIND >= VOL_LW_SRC;
-    constexpr int KB = SRC ? KIND - VOL_LW_SRC : KIND;
+    constexpr bool SRC = vol_is_src(KIND);
+    constexpr bool WALL = vol_is_wall(KIND);
+    constexpr int KB = vol_base(KIND);
     static_assert(!SRC || (!SC && !PAR && !CURV), "source terms: the Euler TreeMesh kernels");
     double Ssrc[4] = {0.0, 0.0, 0.0, 0.0};
     SrcPt sp{0.0, 0.0, 0.0, gamma};
@@ -2327,7 +2416,7 @@ __device__ __forceinline__ void volume_body(const KArgs& a, int bid, double* lds
     double Jn[4] = {0.0, 0.0, 0.0, 0.0};  // Ja11, Ja12, Ja21, Ja22 of the node: same layout as the state
     // walls: the node's wall records (its x face, its y face; null where that face is no wall)
     double* wq[2] = {nullptr, nullptr};
-    if constexpr (KIND == VOL_LW_WALL) {
+    if constexpr (WALL) {
         const int wfx = (i == 0) ? 0 : ((i == n - 1) ? 1 : -1), wfy = (j == 0) ? 2 : ((j == n - 1) ? 3 : -1);
         if (act && wfx >= 0 && a.nbr[4 * e + wfx] == -1) wq[0] = a.fvb + (((size_t)e * 4 + wfx) * n + j) * 6;
         if (act && wfy >= 0 && a.nbr[4 * e + wfy] == -1) wq[1] = a.fvb + (((size_t)e * 4 + wfy) * n + i) * 6;
@@ -2350,30 +2439,30 @@ __device__ __forceinline__ void volume_body(const KArgs& a, int bid, double* lds
     } else if constexpr (PAR) {
         // BR1 gradients of uⁿ (and of us in MDRK stage 2), exchanged through a
         // buffer the element procedure writes only after its first barrier
-        double* sg = (KIND == VOL_MDRK2) ? s2 : s1;
+        double* sg = (KB == VOL_MDRK2) ? s2 : s1;
         double gxN[4], gyN[4];
         if (act) put4<S>(sg, sidx, uN);
-        if constexpr (KIND == VOL_MDRK2) {
+        if constexpr (KB == VOL_MDRK2) {
             double usn[4], gxS[4], gyS[4];
             load4(a.us + off, usn);
             if (act) put4<S>(sg + 4 * S, sidx, usn);
             __syncthreads();
             br1_gradient<N, EB, CURV>(sg, uin, a.nbr, c, e, i, j, sbase, invJ, sDh, uN, gxN, gyN, Jn);
             br1_gradient<N, EB, CURV>(sg + 4 * S, a.us, a.nbr, c, e, i, j, sbase, invJ, sDh, usn, gxS, gyS, Jn);
-            mdrkp2_element<N, EB, CURV>(uN, gxN, gyN, usn, gxS, gyS, invJ, gm1, c, i, j, act, sidx, sbase, s0, s1, s2,
-                                        sDm, sD, sDh, Uc, F, G, vol, Jn);
+            mdrkp2_element<N, EB, CURV, WALL>(uN, gxN, gyN, usn, gxS, gyS, invJ, gm1, c, i, j, act, sidx, sbase, s0,
+                                              s1, s2, sDm, sD, sDh, Uc, F, G, vol, Jn, wq);
         } else {
             __syncthreads();
             br1_gradient<N, EB, CURV>(sg, uin, a.nbr, c, e, i, j, sbase, invJ, sDh, uN, gxN, gyN, Jn);
-            if constexpr (KIND == VOL_LW || KIND == VOL_LW_WALL)
-                lwp_element<N, EB, CURV, KIND == VOL_LW_WALL>(uN, gxN, gyN, invJ, gm1, c, i, j, act, sidx, sbase, s0,
-                                                             s1, s2, sDm, sD, sDh, Uc, F, G, vol, Jn, wq);
+            if constexpr (KB == VOL_LW)
+                lwp_element<N, EB, CURV, WALL>(uN, gxN, gyN, invJ, gm1, c, i, j, act, sidx, sbase, s0, s1, s2, sDm,
+                                               sD, sDh, Uc, F, G, vol, Jn, wq);
             else if constexpr (KIND == VOL_LW_ADAPT)
                 lwp_element<N, EB, CURV, false>(uN, gxN, gyN, invJ, gm1, c, i, j, act, sidx, sbase, s0, s1, s2, sDm,
                                                 sD, sDh, Uc, F, G, vol, Jn, wq, &terr, st->abstol, st->reltol);
             else
-                mdrkp1_element<N, EB, CURV>(uN, gxN, gyN, invJ, gm1, c, i, j, act, sidx, sbase, s0, s1, s2, sDm, sD,
-                                            sDh, Uc, F, G, vol, Jn);
+                mdrkp1_element<N, EB, CURV, WALL>(uN, gxN, gyN, invJ, gm1, c, i, j, act, sidx, sbase, s0, s1, s2, sDm,
+                                                  sD, sDh, Uc, F, G, vol, Jn, wq);
         }
     } else if (KB == VOL_LW || KB == VOL_LW_ADAPT) {
         if constexpr (N == 4 && KB == VOL_LW_ADAPT)
@@ -2646,11 +2735,13 @@ __device__ __forceinline__ void volume_body(const KArgs& a, int bid, double* lds
     // kernel's: it reads every node once after the last correction
 }
 
-// the Navier-Stokes wall kernel (a rarely used configuration: the lid-driven
-// cavity's walls) runs at one wave per SIMD, so its extra wall averages live in
-// the accumulation registers instead of private memory
+// the Navier-Stokes LW and MDRK stage-2 wall kernels (a rarely used
+// configuration: the lid-driven cavity's walls) run at one wave per SIMD, so
+// their extra wall terms live in the accumulation registers instead of private
+// memory; the stage-1 wall kernel fits two waves
 template <int KIND, bool SC, bool PAR>
-constexpr int kVolWavesOf = KIND == VOL_LW_WALL ? 1 : kVolWaves<SC || PAR || (KIND >= VOL_LW_SRC)>;
+constexpr int kVolWavesOf =
+    (KIND == VOL_LW_WALL || KIND == VOL_MDRK2_WALL) ? 1 : kVolWaves<SC || PAR || vol_is_src(KIND)>;
 template <int N, int EB, int KIND, bool SC, bool PAR = false, bool CURV = false>
 __global__ __launch_bounds__((VGeo<N, EB>::NT), (kVolWavesOf<KIND, SC, PAR>)) void k_volume(KArgs a) {
     using G_ = VGeo<N, EB>;
