@@ -18,7 +18,7 @@ LIB_PATH = HERE / "liboracle_tlw.so"
 
 BV_NONE, BV_VORTEX, BV_CONSTANT, BV_RIEMANN4, BV_SLIP_WALL, BV_CALLBACK, BV_CONVERGENCE_TEST = 0, 1, 2, 3, 4, 5, 6
 BV_OUTFLOW, BV_SLIP_WALL_APPROX = 7, 8
-SRC_NONE, SRC_CONVERGENCE_TEST, SRC_CALLBACK = 0, 1, 2
+SRC_NONE, SRC_CONVERGENCE_TEST, SRC_CALLBACK, SRC_NS_CONVERGENCE = 0, 1, 2, 3
 BC_FN = C.CFUNCTYPE(None, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double))
 SRC_FN = C.CFUNCTYPE(None, C.POINTER(C.c_double), C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double))
 
@@ -125,6 +125,7 @@ def lib():
         L.tlwo_max_abs_speed_naive.restype = C.c_double
         L.tlwo_max_abs_speed_naive.argtypes = [dp, dp, C.c_int, C.c_double]
         L.tlwo_ns_viscous_fluxes.argtypes = [dp, dp, dp, C.c_double, C.c_double, C.c_double, dp, dp]
+        L.tlwo_ns_convergence.argtypes = [C.c_int, dp, dp, C.c_double, C.c_double, C.c_double, C.c_double, dp, dp]
         L.tlwo_solve_adaptive.restype = C.c_int
         L.tlwo_solve_adaptive.argtypes = [C.c_void_p, dp, C.c_double, C.c_double, C.c_double, C.c_int, dp, dp,
                                           C.POINTER(C.c_longlong)]
@@ -172,7 +173,10 @@ class Oracle:
         f(x, y, t) -> 4 conservative values, called pointwise (Python).
         ``source_terms``: None, "convergence_test" (Trixi's
         source_terms_convergence_test) or a Python function s(u, x, y, t) -> 4
-        values, called pointwise (Euler on the TreeMesh).
+        values, called pointwise (Euler on the TreeMesh); or
+        "navier_stokes_convergence_test" (the source of
+        elixir_navierstokes_convergence.jl, with parabolic="navier_stokes" on
+        the TreeMesh).
         ``tree``: a TreeMesh with refined cells, (cx, cy, h, neighbors [E, 4],
         mortars [M, 5]) as ``tree_mesh.connectivity`` gives them; nx, ny, xmin,
         ymin, h and periodic are then unused and elements keep the given order.
@@ -223,6 +227,8 @@ class Oracle:
             d.bc_fn = C.cast(self._bc_cb, C.c_void_p)
         if source_terms == "convergence_test":
             d.source_kind = SRC_CONVERGENCE_TEST
+        elif source_terms == "navier_stokes_convergence_test":
+            d.source_kind = SRC_NS_CONVERGENCE
         elif source_terms is not None:
             def _scb(u, x, y, t, out, f=source_terms):
                 v = f(np.array([u[0], u[1], u[2], u[3]]), x, y, t)
@@ -429,3 +435,15 @@ def set_threads(n: int) -> None:
 
 def max_threads() -> int:
     return int(lib().tlwo_max_threads())
+
+
+def ns_convergence(x, y, t, gamma=1.4, mu=0.01, prandtl=0.72):
+    """The oracle's Navier-Stokes manufactured solution and its source at the
+    points (x, y) and time t: (u [..., 4], s [..., 4])."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(np.broadcast_to(y, x.shape), dtype=np.float64)
+    u = np.zeros(x.shape + (4,))
+    s = np.zeros(x.shape + (4,))
+    lib().tlwo_ns_convergence(x.size, _ptr(x), _ptr(y), float(t), float(gamma), float(mu), float(prandtl), _ptr(u),
+                              _ptr(s))
+    return u, s

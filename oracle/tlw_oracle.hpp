@@ -405,6 +405,89 @@ inline void convergence_test_state(double x, double y, double t, double* u) {
     u[3] = phi * phi;
 }
 
+// The manufactured solution of the Navier-Stokes convergence test
+// (examples/tree_2d_dgsem/elixir_navierstokes_convergence.jl:31-45):
+// ρ = 2 + ½ sin πx cos πy cos πt, v1 = v2 = sin πx g(y) cos πt with
+// g(y) = log(y + 2)(1 - e^{-(y-1)/2}), p = ρ².
+inline void ns_convergence_state(double x, double y, double t, double gamma, double* u) {
+    const double pi = 3.141592653589793;
+    const double rho = 2.0 + 0.5 * std::sin(pi * x) * std::cos(pi * y) * std::cos(pi * t);
+    const double v = std::sin(pi * x) * std::log(y + 2.0) * (1.0 - std::exp(-0.5 * (y - 1.0))) * std::cos(pi * t);
+    const double p = rho * rho;
+    u[0] = rho;
+    u[1] = rho * v;
+    u[2] = rho * v;
+    u[3] = p / (gamma - 1.0) + 0.5 * rho * (v * v + v * v);
+}
+
+// Its source (the example's source_terms_navier_stokes_convergence_test,
+// elixir_navierstokes_convergence.jl:49-205): the residual of the manufactured
+// solution in the compressible Navier-Stokes equations with Trixi's viscous
+// fluxes (compressible_navier_stokes_2d.jl: τ with Stokes' hypothesis, heat
+// flux μ κ ∇T with κ = γ/((γ-1) Pr) and T = p/ρ),
+//   s = ∂t u + ∇·f_a(u) - ∇·f_v(u, ∇u),
+// assembled here from the closed-form first and second derivatives of ρ, v1,
+// v2 and p by the product rule, term by term of each conservation law.  The
+// source reads the example's μ and Pr, which are the equations' own.
+inline void ns_convergence_source(double x, double y, double t, double gamma, double mu, double prandtl,
+                                  double* s) {
+    const double pi = 3.141592653589793, A = 0.5;
+    const double sx = std::sin(pi * x), cx = std::cos(pi * x);
+    const double sy = std::sin(pi * y), cy = std::cos(pi * y);
+    const double st = std::sin(pi * t), ct = std::cos(pi * t);
+    // density and its derivatives
+    const double r = 2.0 + A * sx * cy * ct;
+    const double r_t = -pi * A * sx * cy * st;
+    const double r_x = pi * A * cx * cy * ct;
+    const double r_y = -pi * A * sx * sy * ct;
+    const double r_xx = -pi * pi * A * sx * cy * ct;
+    const double r_yy = r_xx;
+    // velocity φ = sin πx g(y) cos πt for both components
+    const double e = std::exp(-A * (y - 1.0)), L = std::log(y + 2.0), q = 1.0 / (y + 2.0);
+    const double g = L * (1.0 - e);
+    const double g1 = (1.0 - e) * q + A * L * e;
+    const double g2 = -(1.0 - e) * q * q + 2.0 * A * e * q - A * A * L * e;
+    const double v = sx * g * ct, v_t = -pi * sx * g * st;
+    const double v_x = pi * cx * g * ct, v_y = sx * g1 * ct;
+    const double v_xx = -pi * pi * sx * g * ct, v_xy = pi * cx * g1 * ct, v_yy = sx * g2 * ct;
+    const double v1 = v, v2 = v;
+    const double v1_t = v_t, v2_t = v_t, v1_x = v_x, v2_x = v_x, v1_y = v_y, v2_y = v_y;
+    const double v1_xx = v_xx, v2_xx = v_xx, v1_xy = v_xy, v2_xy = v_xy, v1_yy = v_yy, v2_yy = v_yy;
+    // pressure p = ρ²
+    const double p = r * r, p_x = 2.0 * r * r_x, p_y = 2.0 * r * r_y;
+    const double p_xx = 2.0 * (r_x * r_x + r * r_xx), p_yy = 2.0 * (r_y * r_y + r * r_yy);
+    // total energy E = p/(γ-1) + ρ|v|²/2
+    const double ig = 1.0 / (gamma - 1.0), k2 = v1 * v1 + v2 * v2;
+    const double E = p * ig + 0.5 * r * k2;
+    const double E_t = 2.0 * r * r_t * ig + 0.5 * r_t * k2 + r * (v1 * v1_t + v2 * v2_t);
+    const double E_x = p_x * ig + 0.5 * r_x * k2 + r * (v1 * v1_x + v2 * v2_x);
+    const double E_y = p_y * ig + 0.5 * r_y * k2 + r * (v1 * v1_y + v2 * v2_y);
+    // viscous stresses (without μ) and their divergence pieces
+    const double t11 = (4.0 / 3.0) * v1_x - (2.0 / 3.0) * v2_y, t12 = v1_y + v2_x;
+    const double t22 = (4.0 / 3.0) * v2_y - (2.0 / 3.0) * v1_x;
+    const double t11_x = (4.0 / 3.0) * v1_xx - (2.0 / 3.0) * v2_xy, t12_x = v1_xy + v2_xx;
+    const double t12_y = v1_yy + v2_xy, t22_y = (4.0 / 3.0) * v2_yy - (2.0 / 3.0) * v1_xy;
+    // ∇²T of T = p/ρ: T_xx = (p_xx ρ² - 2 p_x ρ ρ_x + 2 p ρ_x² - p ρ ρ_xx)/ρ³, same in y
+    const double ir3 = 1.0 / (r * r * r);
+    const double T_xx = (p_xx * r * r - 2.0 * p_x * r * r_x + 2.0 * p * r_x * r_x - p * r * r_xx) * ir3;
+    const double T_yy = (p_yy * r * r - 2.0 * p_y * r * r_y + 2.0 * p * r_y * r_y - p * r * r_yy) * ir3;
+    const double kappa = gamma * ig / prandtl;
+    // mass: ρ_t + (ρ v1)_x + (ρ v2)_y
+    s[0] = r_t + r_x * v1 + r * v1_x + r_y * v2 + r * v2_y;
+    // x momentum: (ρ v1)_t + (ρ v1² + p)_x + (ρ v1 v2)_y - μ (τ11_x + τ12_y)
+    s[1] = r_t * v1 + r * v1_t + r_x * v1 * v1 + 2.0 * r * v1 * v1_x + p_x + r_y * v1 * v2 + r * v1_y * v2 +
+           r * v1 * v2_y - mu * (t11_x + t12_y);
+    // y momentum: (ρ v2)_t + (ρ v1 v2)_x + (ρ v2² + p)_y - μ (τ12_x + τ22_y)
+    s[2] = r_t * v2 + r * v2_t + r_x * v1 * v2 + r * v1_x * v2 + r * v1 * v2_x + r_y * v2 * v2 +
+           2.0 * r * v2 * v2_y + p_y - mu * (t12_x + t22_y);
+    // energy: E_t + ((E + p) v1)_x + ((E + p) v2)_y
+    //         - μ ((v1 τ11 + v2 τ12)_x + (v1 τ12 + v2 τ22)_y + κ (T_xx + T_yy))
+    const double work_x = v1_x * t11 + v1 * t11_x + v2_x * t12 + v2 * t12_x;
+    const double work_y = v1_y * t12 + v1 * t12_y + v2_y * t22 + v2 * t22_y;
+    s[3] = E_t + (E_x + p_x) * v1 + (E + p) * v1_x + (E_y + p_y) * v2 + (E + p) * v2_y -
+           mu * (work_x + work_y + kappa * (T_xx + T_yy));
+}
+
 // Source terms s(u, x, t) (the `source_terms` of SemidiscretizationHyperbolic),
 // which the LW kernels fold into the Taylor procedure (dg_2d.jl:89-176).
 // SRC_CONVERGENCE_TEST: Trixi's source_terms_convergence_test for
@@ -412,11 +495,14 @@ inline void convergence_test_state(double x, double y, double t, double* u) {
 // (with φ' = ∂x φ = π 0.1 cos(π(x + y - t)) and p = (γ-1)(φ² - φ)):
 // (φ', φ'(1 + τ), φ'(1 + τ), 2φ'(φ + τ)), τ = (2φ - 1)(γ - 1); it does not
 // depend on u.  SRC_CALLBACK: a caller's function of (u, x, y, t).
-enum SourceKind { SRC_NONE = 0, SRC_CONVERGENCE_TEST = 1, SRC_CALLBACK = 2 };
+// SRC_NS_CONVERGENCE: the Navier-Stokes convergence source above (with the
+// hyperbolic-parabolic solver; it does not depend on u either).
+enum SourceKind { SRC_NONE = 0, SRC_CONVERGENCE_TEST = 1, SRC_CALLBACK = 2, SRC_NS_CONVERGENCE = 3 };
 
 struct SourceTerm {
     int kind = SRC_NONE;
     double gamma = 1.4;
+    double mu = 0.0, prandtl = 1.0;  // SRC_NS_CONVERGENCE
     void (*fn)(const double*, double, double, double, double*) = nullptr;
     // NV = 4 (Euler); other equations take no source terms
     template <int NV, class T> void eval(const T* u, double x, double y, double t, T* s) const {
@@ -433,6 +519,10 @@ struct SourceTerm {
             s[1] = T(rho_x * (1.0 + tmp));
             s[2] = s[1];
             s[3] = T(2.0 * rho_x * (rho + tmp));
+        } else if (kind == SRC_NS_CONVERGENCE) {
+            double sv[4];
+            ns_convergence_source(x, y, t, gamma, mu, prandtl, sv);
+            for (int v = 0; v < 4; ++v) s[v] = T(sv[v]);
         } else if (kind == SRC_CALLBACK) {
             double uv[4], sv[4];
             for (int v = 0; v < 4; ++v) uv[v] = value(u[v]);

@@ -55,8 +55,8 @@ struct tlwo_desc {
     double wall_T[4];
     // bc_kind = BV_CALLBACK: the boundary value function u(x, y, t)
     void (*bc_fn)(double, double, double, double*);
-    // source terms (SourceKind; Euler on the TreeMesh): SRC_CALLBACK calls
-    // src_fn(u, x, y, t, s)
+    // source terms (SourceKind; Euler on the TreeMesh, or SRC_NS_CONVERGENCE with
+    // the Navier-Stokes TreeMesh kernels): SRC_CALLBACK calls src_fn(u, x, y, t, s)
     int source_kind;
     void (*src_fn)(const double*, double, double, double, double*);
     // a TreeMesh with refined cells instead of the uniform nx x ny grid (n_tree > 0):
@@ -106,6 +106,8 @@ template <class S> static void configure(S& s, const tlwo_desc* d) {
     s.src.kind = d->source_kind;
     s.src.gamma = d->gamma;
     s.src.fn = d->src_fn;
+    s.src.mu = d->visc_mu;
+    s.src.prandtl = d->prandtl;
     s.setup();
 }
 
@@ -138,10 +140,15 @@ void* tlwo_create(const tlwo_desc* d) {
     if (d->n_tree > 0 && (d->curved || d->parabolic || d->shock_capturing || !d->tree_cx || !d->tree_cy ||
                           !d->tree_len || !d->tree_nbr || d->n_mortars < 0 || (d->n_mortars > 0 && !d->mortars)))
         return nullptr;
-    // source terms: the Euler kernels of the TreeMesh (LW N = 3, 4 and MDRK)
-    if (d->source_kind != SRC_NONE &&
-        (d->equation != 0 || d->parabolic || d->curved || (d->source_kind == SRC_CALLBACK && !d->src_fn) ||
-         d->source_kind < 0 || d->source_kind > SRC_CALLBACK))
+    // source terms: the Euler kernels of the TreeMesh (LW N = 3, 4 and MDRK); the
+    // Navier-Stokes convergence source with the TreeMesh Navier-Stokes kernels
+    if (d->source_kind < SRC_NONE || d->source_kind > SRC_NS_CONVERGENCE) return nullptr;
+    // (a callback source may also go with the TreeMesh Navier-Stokes kernels, where the
+    // oracle's pins give it sources of t alone)
+    if (d->source_kind == SRC_NS_CONVERGENCE && (d->equation != 0 || d->parabolic != 2 || d->curved)) return nullptr;
+    if (d->source_kind == SRC_CONVERGENCE_TEST && (d->equation != 0 || d->parabolic || d->curved)) return nullptr;
+    if (d->source_kind == SRC_CALLBACK &&
+        (d->equation != 0 || (d->parabolic && d->parabolic != 2) || d->curved || !d->src_fn))
         return nullptr;
     auto* c = new tlwo_ctx;
     c->equation = d->equation;
@@ -458,6 +465,15 @@ int tlwo_tangent_lw(const tlwo_desc* d, const double* u0, const double* du0, con
 extern "C" {
 // Pointwise viscous flux of CompressibleNavierStokesDiffusion2D with conservative
 // gradient variables, for the chain-rule check.
+// the Navier-Stokes manufactured solution and its source at n points
+void tlwo_ns_convergence(int n, const double* x, const double* y, double t, double gamma, double mu, double prandtl,
+                         double* u, double* s) {
+    for (int k = 0; k < n; ++k) {
+        ns_convergence_state(x[k], y[k], t, gamma, u + 4 * k);
+        ns_convergence_source(x[k], y[k], t, gamma, mu, prandtl, s + 4 * k);
+    }
+}
+
 void tlwo_ns_viscous_fluxes(const double* u, const double* gx, const double* gy, double gamma, double mu,
                             double prandtl, double* fx, double* fy) {
     NavierStokesDiffusion2D<double> v;

@@ -315,8 +315,61 @@ template <class T, class Eq, class Visc> struct ParabolicSolver : Solver<T, Eq> 
         visc.template fluxes<NV>(S.s[q][j][i], S.sx[q][j][i], S.sy[q][j][i], fv, gv);
     }
 
-    // lw_volume_kernel_3!/4! (dg_2d_parabolic.jl:725-1189, 1191-1860).  Source
-    // terms are `nothing`.  Kernel 4 on the TreeMesh also gives the element's
+    // Source terms of the hyperbolic-parabolic kernels (dg_2d_parabolic.jl:725-1860,
+    // multi_derivative_2d_parabolic.jl:221-711; only sources that do not depend on u
+    // are accepted with parabolic terms, but the stencil states are passed as the
+    // reference passes them).  Level `order` of the Taylor procedure: the source's
+    // time difference of that order at the states of the level before (s at uⁿ; s_t
+    // of calc_source_t_N34; s_tt five-point, calc_source_tt_N4, in kernel 4 and
+    // three-point, N23, in kernel 3; s_ttt of N34; s_tttt of N4), added as dt·s to the
+    // level's time derivative w before its gradient is formed (w may be null), and
+    // as wS·s to the element's S.
+    using PNodeArr = T[MAXN][MAXN][NV];
+    void src_level(int e, double dt, int order, const T* u, const States& St, T (*w)[MAXN][NV], PNodeArr& Sacc,
+                   double wS, bool first) const {
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) {
+                double x, y;
+                this->node_xy(i, j, e, x, y);
+                const T* un = &u[idx(0, i, j, e)];
+                const T *up = St.s[0][j][i], *um = St.s[1][j][i], *upp = St.s[2][j][i], *umm = St.s[3][j][i];
+                T sv[NV];
+                if (order == 0) {
+                    this->src_at(un, x, y, 0.0, sv);
+                } else if (order == 1) {
+                    this->src_t_N34(up, upp, um, umm, x, y, dt, this->src_t, sv);
+                } else if (order == 2 && basis.N == 4) {
+                    T a[NV], b[NV], z[NV], c[NV], d[NV];
+                    this->src_at(upp, x, y, 2.0 * dt, a);
+                    this->src_at(up, x, y, dt, b);
+                    this->src_at(un, x, y, 0.0, z);
+                    this->src_at(um, x, y, -dt, c);
+                    this->src_at(umm, x, y, -2.0 * dt, d);
+                    for (int v = 0; v < NV; ++v)
+                        sv[v] = (1.0 / 12.0) * (-a[v] + 16.0 * b[v] - 30.0 * z[v] + 16.0 * c[v] - d[v]);
+                } else if (order == 2) {
+                    this->src_tt_N23(un, up, um, x, y, dt, sv);
+                } else if (order == 3) {
+                    this->src_ttt_N34(up, upp, um, umm, x, y, dt, sv);
+                } else {
+                    this->src_tttt_N4(un, up, upp, um, umm, x, y, dt, sv);
+                }
+                for (int v = 0; v < NV; ++v) {
+                    Sacc[j][i][v] = first ? wS * sv[v] : Sacc[j][i][v] + wS * sv[v];
+                    if (w) w[j][i][v] = w[j][i][v] + dt * sv[v];
+                }
+            }
+    }
+    // du += -S/J⁻¹, which the Jacobian step turns into +S
+    void src_to_du(T* du, int e, const PNodeArr& Sacc) const {
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i)
+                for (int v = 0; v < NV; ++v)
+                    du[idx(v, i, j, e)] = du[idx(v, i, j, e)] + (-1.0 / node_ij(i, j, e)) * Sacc[j][i][v];
+    }
+
+    // lw_volume_kernel_3!/4! (dg_2d_parabolic.jl:725-1189, 1191-1860), with the
+    // source terms of src_level.  Kernel 4 on the TreeMesh also gives the element's
     // temporal error (1749-1775, 1841-1855): u_np1 = u - dt J⁻¹ D F̃ with the
     // total F̃ = F̃a - F̃v, u_np1_low the same before the f_tttt terms, compared
     // as in the Euler kernel; u_np1 - u_np1_low is formed as the update of the
@@ -328,6 +381,8 @@ template <class T, class Eq, class Visc> struct ParabolicSolver : Solver<T, Eq> 
         T Fa[MAXN][MAXN][NV], Ga[MAXN][MAXN][NV], Fva[MAXN][MAXN][NV], Gva[MAXN][MAXN][NV];
         T Uc[MAXN][MAXN][NV], w[MAXN][MAXN][NV], wx[MAXN][MAXN][NV], wy[MAXN][MAXN][NV];
         T df[MAXN][MAXN][NV], dg[MAXN][MAXN][NV];
+        const bool has_src = this->src.kind != SRC_NONE;
+        PNodeArr Sacc;
         for (int j = 0; j < n; ++j)
             for (int i = 0; i < n; ++i)
                 for (int v = 0; v < NV; ++v) w[j][i][v] = T(0);
@@ -358,6 +413,7 @@ template <class T, class Eq, class Visc> struct ParabolicSolver : Solver<T, Eq> 
         for (int j = 0; j < n; ++j)
             for (int i = 0; i < n; ++i)
                 for (int v = 0; v < NV; ++v) w[j][i][v] = w[j][i][v] * node_ij(i, j, e);  // ut
+        if (has_src) src_level(e, dt, 0, u, S, w, Sacc, 1.0, true);
         local_grad(w, wx, wy, e);
         {
             const double c[4] = {1.0, -1.0, 2.0, -2.0};
@@ -395,6 +451,7 @@ template <class T, class Eq, class Visc> struct ParabolicSolver : Solver<T, Eq> 
         for (int j = 0; j < n; ++j)
             for (int i = 0; i < n; ++i)
                 for (int v = 0; v < NV; ++v) w[j][i][v] = w[j][i][v] * (-dt * node_ij(i, j, e));  // utt
+        if (has_src) src_level(e, dt, 1, u, S, w, Sacc, 0.5, false);
         local_grad(w, wx, wy, e);
         {
             const double c[4] = {0.5, 0.5, 2.0, 2.0};
@@ -445,6 +502,7 @@ template <class T, class Eq, class Visc> struct ParabolicSolver : Solver<T, Eq> 
         for (int j = 0; j < n; ++j)
             for (int i = 0; i < n; ++i)
                 for (int v = 0; v < NV; ++v) w[j][i][v] = w[j][i][v] * node_ij(i, j, e);  // uttt
+        if (has_src) src_level(e, dt, 2, u, S, w, Sacc, 1.0 / 6.0, false);
         local_grad(w, wx, wy, e);
         {
             const double c[4] = {1.0 / 6.0, -1.0 / 6.0, 4.0 / 3.0, -4.0 / 3.0};
@@ -483,6 +541,7 @@ template <class T, class Eq, class Visc> struct ParabolicSolver : Solver<T, Eq> 
             for (int j = 0; j < n; ++j)
                 for (int i = 0; i < n; ++i)
                     for (int v = 0; v < NV; ++v) w[j][i][v] = w[j][i][v] * node_ij(i, j, e);  // utttt
+            if (has_src) src_level(e, dt, 3, u, S, w, Sacc, 1.0 / 24.0, false);
             local_grad(w, wx, wy, e);
             const double c[4] = {1.0 / 24.0, 1.0 / 24.0, 2.0 / 3.0, 2.0 / 3.0};
             add_states(S, w, wx, wy, c);
@@ -512,9 +571,13 @@ template <class T, class Eq, class Visc> struct ParabolicSolver : Solver<T, Eq> 
                         dg[j][i][v] = (1.0 / 120.0) * (b4 - w4);
                     }
                 }
+            if (has_src) src_level(e, dt, 4, u, S, nullptr, Sacc, 1.0 / 120.0, false);
             if (!cJa) temporal_error(u, dt, e, Fa, Ga, Fva, Gva, df, dg);
+        } else if (has_src) {
+            src_level(e, dt, 3, u, S, nullptr, Sacc, 1.0 / 24.0, false);  // kernel 3: S only
         }
         finish_element(du, e, Fa, Ga, Fva, Gva, Uc, this->F, Fv);
+        if (has_src) src_to_du(du, e, Sacc);
     }
 
     // the element's temporal error of kernel 4 (dg_2d_parabolic.jl:1841-1855)
@@ -602,10 +665,19 @@ template <class T, class Eq, class Visc> struct ParabolicSolver : Solver<T, Eq> 
         }
     }
 
+    // the four stencil states w ± wt, w ± 2wt of a node (up, um, upp, umm)
+    void stencil_states(const T* w, const T* wt, States& St, int i, int j) const {
+        const double c[4] = {1.0, -1.0, 2.0, -2.0};
+        for (int q = 0; q < 4; ++q)
+            for (int v = 0; v < NV; ++v) St.s[q][j][i][v] = w[v] + c[q] * wt[v];
+    }
+
     // mdrk_kernel_1! (multi_derivative_2d_parabolic.jl:221-464): F̃ = f/2 + f_t/8,
     // Ũ = u/2 + ut/8 for the advective and viscous parts; F2 = f + f_t/6,
     // U2 = u + ut/6.  The u_low update feeds only the admissibility machinery
-    // (and names a field MDRKElementCache lacks) and is omitted.
+    // (and names a field MDRKElementCache lacks) and is omitted.  Source terms
+    // (290-300, 380-392, 441-446): ut += dt s(uⁿ, t), S = s/2 + s_t/8 into du,
+    // S2 = s + s_t/6 to the stage cache.
     void mdrk_parabolic_1(T* du, const T* u, double dt, int e) {
         T Fa[MAXN][MAXN][NV], Ga[MAXN][MAXN][NV], Fva[MAXN][MAXN][NV], Gva[MAXN][MAXN][NV];
         T Fa2[MAXN][MAXN][NV], Ga2[MAXN][MAXN][NV], Fva2[MAXN][MAXN][NV], Gva2[MAXN][MAXN][NV];
@@ -639,7 +711,20 @@ template <class T, class Eq, class Visc> struct ParabolicSolver : Solver<T, Eq> 
         for (int j = 0; j < n; ++j)
             for (int i = 0; i < n; ++i)
                 for (int v = 0; v < NV; ++v) ut[j][i][v] = ut[j][i][v] * node_ij(i, j, e);
+        const bool has_src = this->src.kind != SRC_NONE;
+        static thread_local States St;
+        PNodeArr S1, S2c;
+        if (has_src) src_level(e, dt, 0, u, St, ut, S2c, 1.0, true);  // S2 = s, ut += dt s
         local_grad(ut, utx, uty, e);
+        if (has_src) {
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < n; ++i) {
+                    stencil_states(&u[idx(0, i, j, e)], ut[j][i], St, i, j);
+                    for (int v = 0; v < NV; ++v) S1[j][i][v] = 0.5 * S2c[j][i][v];
+                }
+            src_level(e, dt, 1, u, St, nullptr, S1, 0.125, false);
+            src_level(e, dt, 1, u, St, nullptr, S2c, 1.0 / 6.0, false);
+        }
         for (int j = 0; j < n; ++j)
             for (int i = 0; i < n; ++i) {
                 T fta[NV], gta[NV], ftv[NV], gtv[NV];
@@ -659,9 +744,11 @@ template <class T, class Eq, class Visc> struct ParabolicSolver : Solver<T, Eq> 
                 }
             }
         finish_element(du, e, Fa, Ga, Fva, Gva, Uc, this->F, Fv);
+        if (has_src) src_to_du(du, e, S1);
         for (int j = 0; j < n; ++j)
             for (int i = 0; i < n; ++i)
                 for (int v = 0; v < NV; ++v) {
+                    if (has_src) this->S2[idx(v, i, j, e)] = S2c[j][i][v];
                     this->U2[idx(v, i, j, e)] = U2c[j][i][v];
                     this->F2[fidx(v, 0, i, j, e)] = Fa2[j][i][v];
                     this->F2[fidx(v, 1, i, j, e)] = Ga2[j][i][v];
@@ -672,6 +759,9 @@ template <class T, class Eq, class Visc> struct ParabolicSolver : Solver<T, Eq> 
 
     // mdrk_kernel_2! (multi_derivative_2d_parabolic.jl:506-711): with the
     // gradient of us, F̃ = F2 + f_t(us)/3 and Ũ = U2 + ust/3 (both parts).
+    // Source terms (576-601, 668-674): ust += dt s(us, t) and S = S2 + s_t/3 with
+    // s_t of the us states, both at the call's t (where the Euler stage 2 takes
+    // t + dt/2).
     void mdrk_parabolic_2(T* du, double dt, int e) {
         const T* us = this->us.data();
         T Fa[MAXN][MAXN][NV], Ga[MAXN][MAXN][NV], Fva[MAXN][MAXN][NV], Gva[MAXN][MAXN][NV];
@@ -702,7 +792,19 @@ template <class T, class Eq, class Visc> struct ParabolicSolver : Solver<T, Eq> 
         for (int j = 0; j < n; ++j)
             for (int i = 0; i < n; ++i)
                 for (int v = 0; v < NV; ++v) ust[j][i][v] = ust[j][i][v] * node_ij(i, j, e);
+        const bool has_src = this->src.kind != SRC_NONE;
+        static thread_local States St;
+        PNodeArr Sn, Sdummy;
+        if (has_src) src_level(e, dt, 0, us, St, ust, Sdummy, 1.0, true);
         local_grad(ust, ustx, usty, e);
+        if (has_src) {
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < n; ++i) {
+                    stencil_states(&us[idx(0, i, j, e)], ust[j][i], St, i, j);
+                    for (int v = 0; v < NV; ++v) Sn[j][i][v] = this->S2[idx(v, i, j, e)];
+                }
+            src_level(e, dt, 1, us, St, nullptr, Sn, 1.0 / 3.0, false);
+        }
         for (int j = 0; j < n; ++j)
             for (int i = 0; i < n; ++i) {
                 T fta[NV], gta[NV], ftv[NV], gtv[NV];
@@ -717,6 +819,7 @@ template <class T, class Eq, class Visc> struct ParabolicSolver : Solver<T, Eq> 
                 }
             }
         finish_element(du, e, Fa, Ga, Fva, Gva, Uc, this->F, Fv);
+        if (has_src) src_to_du(du, e, Sn);
     }
 
     // prolong2interfaces_lw_parabolic! + calc_interface_flux_hyperbolic_parabolic!
@@ -787,6 +890,7 @@ template <class T, class Eq, class Visc> struct ParabolicSolver : Solver<T, Eq> 
     }
 
     void rhs_lw(T* du, const T* u, double t, double dt) override {
+        this->src_t = t;
         this->reset_du(du);
         calc_gradient(u);
 #pragma omp parallel for schedule(static)
@@ -800,6 +904,7 @@ template <class T, class Eq, class Visc> struct ParabolicSolver : Solver<T, Eq> 
         this->surface_integral_and_jacobian(du);
     }
     void rhs_mdrk1(T* du, const T* u, double t, double dt) override {
+        this->src_t = t;
         this->ensure_mdrk();
         if (Fv2.size() != Fv.size()) Fv2.assign(Fv.size(), T(0));
         this->reset_du(du);
@@ -817,6 +922,7 @@ template <class T, class Eq, class Visc> struct ParabolicSolver : Solver<T, Eq> 
         this->surface_integral_and_jacobian(du);
     }
     void rhs_mdrk2(T* du, const T* u, double t, double dt) override {
+        this->src_t = t;
         this->ensure_mdrk();
         if (Fv2.size() != Fv.size()) Fv2.assign(Fv.size(), T(0));
         this->reset_du(du);

@@ -144,7 +144,7 @@ constexpr int vol_base(int k) {
                                    : k;
 }
 // source-term kinds of DevConst::src_kind
-enum SrcKind { SRC_NONE = 0, SRC_CONVERGENCE_TEST = 1 };
+enum SrcKind { SRC_NONE = 0, SRC_CONVERGENCE_TEST = 1, SRC_NS_CONVERGENCE_TEST = 2 };
 
 struct KArgs {
     int E;
@@ -322,6 +322,87 @@ __device__ __forceinline__ void src_tttt5(const SrcPt& p, double dt, double* o) 
     src_eval(p, -2.0 * dt, a);
 #pragma unroll
     for (int v = 0; v < 4; ++v) o[v] += a[v];
+}
+
+// The Navier-Stokes convergence source (the source_terms of
+// examples/tree_2d_dgsem/elixir_navierstokes_convergence.jl:49-205): the
+// residual ∂t u + ∇·f_a - ∇·f_v of ρ = 2 + ½ sin πx cos πy cos πt,
+// v1 = v2 = sin πx g(y) cos πt with g = log(y + 2)(1 - e^{-(y-1)/2}), p = ρ²,
+// under Trixi's viscous fluxes (Stokes' hypothesis, heat flux μκ∇T with
+// κ = γ/((γ-1)Pr), T = p/ρ), by the product rule from the closed-form
+// derivatives.  The node's spatial factors are formed once (NsSrcPt), each
+// time level then needs sin and cos of πt only.  It does not depend on u.
+struct NsSrcPt {
+    double sx, cx, sy, cy, g, g1, g2, t, gamma, mu, prandtl;
+};
+__device__ __forceinline__ NsSrcPt ns_src_point(double x, double y, double t, const DevConst* c) {
+    constexpr double pi = 3.141592653589793, A = 0.5;
+    NsSrcPt p;
+    sincos_cw(pi * x, &p.sx, &p.cx);
+    sincos_cw(pi * y, &p.sy, &p.cy);
+    const double e = exp(-A * (y - 1.0)), L = log(y + 2.0), q = 1.0 / (y + 2.0);
+    p.g = L * (1.0 - e);
+    p.g1 = (1.0 - e) * q + A * L * e;
+    p.g2 = -(1.0 - e) * q * q + 2.0 * A * e * q - A * A * L * e;
+    p.t = t;
+    p.gamma = c->gamma;
+    p.mu = c->visc_mu;
+    p.prandtl = c->visc_Pr;
+    return p;
+}
+__device__ __forceinline__ void ns_src_eval(const NsSrcPt& p, double toff, double* s) {
+    constexpr double pi = 3.141592653589793, A = 0.5;
+    double st, ct;
+    sincos_cw(pi * (p.t + toff), &st, &ct);
+    const double sx = p.sx, cx = p.cx, sy = p.sy, cy = p.cy;
+    const double r = 2.0 + A * sx * cy * ct;
+    const double r_t = -pi * A * sx * cy * st;
+    const double r_x = pi * A * cx * cy * ct;
+    const double r_y = -pi * A * sx * sy * ct;
+    const double r_xx = -pi * pi * A * sx * cy * ct, r_yy = r_xx;
+    const double v = sx * p.g * ct, v_t = -pi * sx * p.g * st;
+    const double v_x = pi * cx * p.g * ct, v_y = sx * p.g1 * ct;
+    const double v_xx = -pi * pi * sx * p.g * ct, v_xy = pi * cx * p.g1 * ct, v_yy = sx * p.g2 * ct;
+    const double pr = r * r, p_x = 2.0 * r * r_x, p_y = 2.0 * r * r_y;
+    const double p_xx = 2.0 * (r_x * r_x + r * r_xx), p_yy = 2.0 * (r_y * r_y + r * r_yy);
+    const double ig = 1.0 / (p.gamma - 1.0), k2 = v * v + v * v;
+    const double E = pr * ig + 0.5 * r * k2;
+    const double E_t = 2.0 * r * r_t * ig + 0.5 * r_t * k2 + r * (v * v_t + v * v_t);
+    const double E_x = p_x * ig + 0.5 * r_x * k2 + r * (v * v_x + v * v_x);
+    const double E_y = p_y * ig + 0.5 * r_y * k2 + r * (v * v_y + v * v_y);
+    const double t11 = (4.0 / 3.0) * v_x - (2.0 / 3.0) * v_y, t12 = v_y + v_x;
+    const double t22 = (4.0 / 3.0) * v_y - (2.0 / 3.0) * v_x;
+    const double t11_x = (4.0 / 3.0) * v_xx - (2.0 / 3.0) * v_xy, t12_x = v_xy + v_xx;
+    const double t12_y = v_yy + v_xy, t22_y = (4.0 / 3.0) * v_yy - (2.0 / 3.0) * v_xy;
+    const double ir3 = 1.0 / (r * r * r);
+    const double T_xx = (p_xx * r * r - 2.0 * p_x * r * r_x + 2.0 * pr * r_x * r_x - pr * r * r_xx) * ir3;
+    const double T_yy = (p_yy * r * r - 2.0 * p_y * r * r_y + 2.0 * pr * r_y * r_y - pr * r * r_yy) * ir3;
+    const double kappa = p.gamma * ig / p.prandtl, mu = p.mu;
+    s[0] = r_t + r_x * v + r * v_x + r_y * v + r * v_y;
+    s[1] = r_t * v + r * v_t + r_x * v * v + 2.0 * r * v * v_x + p_x + r_y * v * v + r * v_y * v + r * v * v_y -
+           mu * (t11_x + t12_y);
+    s[2] = r_t * v + r * v_t + r_x * v * v + r * v_x * v + r * v * v_x + r_y * v * v + 2.0 * r * v * v_y + p_y -
+           mu * (t12_x + t22_y);
+    const double work_x = v_x * t11 + v * t11_x + v_x * t12 + v * t12_x;
+    const double work_y = v_y * t12 + v * t12_y + v_y * t22 + v * t22_y;
+    s[3] = E_t + (E_x + p_x) * v + (E + pr) * v_x + (E_y + p_y) * v + (E + pr) * v_y -
+           mu * (work_x + work_y + kappa * (T_xx + T_yy));
+}
+// s_t of calc_source_t_N34 about t: (-s(t+2dt) + 8 s(t+dt) - 8 s(t-dt) + s(t-2dt))/12
+__device__ __forceinline__ void ns_src_t5(const NsSrcPt& p, double dt, double* o) {
+    double a[4];
+    ns_src_eval(p, 2.0 * dt, a);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) o[v] = -a[v];
+    ns_src_eval(p, dt, a);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) o[v] = fma(8.0, a[v], o[v]);
+    ns_src_eval(p, -dt, a);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) o[v] = fma(-8.0, a[v], o[v]);
+    ns_src_eval(p, -2.0 * dt, a);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) o[v] = (1.0 / 12.0) * (o[v] + a[v]);
 }
 
 __device__ __forceinline__ void euler_flux_dir(const double* u, double gm1, int orientation, double* f) {
@@ -1536,12 +1617,13 @@ struct ParStates {
 };
 
 // Time derivative of the next level and its element-local gradient:
-// w = J⁻¹ (-dt D) · (dx, dy) through sa, then (wx, wy) = J⁻¹ D w through sg.
+// w = J⁻¹ (-dt D) · (dx, dy) (+ add, the level's source term dt·s) through sa,
+// then (wx, wy) = J⁻¹ D w through sg.
 template <int N, int EB, bool CURV = false>
 __device__ __forceinline__ void par_level(bool act, int sidx, int sbase, int i, int j, double invJ, double* sa,
                                           double* sg, const double* sDm, const double* sD, const double* dx,
                                           const double* dy, double* w, double* wx, double* wy,
-                                          const double* Jn = nullptr) {
+                                          const double* Jn = nullptr, const double* add = nullptr) {
     using G_ = VGeo<N, EB>;
     constexpr int n = G_::n, S = G_::S;
     if constexpr (CURV) {
@@ -1556,6 +1638,10 @@ __device__ __forceinline__ void par_level(bool act, int sidx, int sbase, int i, 
     contract<n, S>(sa, sbase, i, j, sDm, w);
 #pragma unroll
     for (int v = 0; v < 4; ++v) w[v] *= invJ;
+    if (add) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) w[v] += add[v];
+    }
     if (act) put4<S>(sg, sidx, w);
     __syncthreads();
     contract_xy<n, S>(sg, sbase, i, j, sD, wx, wy);
@@ -1895,21 +1981,37 @@ __device__ __forceinline__ void wall_first(double* const* wq, double w, const do
 // mdrk_kernel_1! with viscous terms (multi_derivative_2d_parabolic.jl:221-464):
 // F̃ = f/2 + f_t/8, Ũ = u/2 + ut/8 on the total flux.  WALL: the wall records
 // of the node take the stage's viscous average f_v/2 + f_v,t/8 (the Fv the
-// reference's prolong2boundaries_visc_lw! reads).
+// reference's prolong2boundaries_visc_lw! reads).  With the Navier-Stokes
+// convergence source (has_src; 290-300, 380-392, 441-446): ut += dt s(t) before
+// its gradient, and Ssrc = s/2 + s_t/8 for the residual.
 template <int N, int EB, bool CURV = false, bool WALL = false>
 __device__ __forceinline__ void mdrkp1_element(const double* uN, const double* gxN, const double* gyN, double invJ,
                                                double gm1, const DevConst* c, int i, int j, bool act, int sidx,
                                                int sbase, double* s0, double* s1, double* s2, const double* sDm,
                                                const double* sD, const double* sDh, double* Uc, double* F,
                                                double* G, double* vol, const double* Jn = nullptr,
-                                               double* const* wq = nullptr) {
+                                               double* const* wq = nullptr, bool has_src = false,
+                                               const NsSrcPt& sp = NsSrcPt{}, double dt = 0.0,
+                                               double* Ssrc = nullptr) {
     using G_ = VGeo<N, EB>;
     constexpr int n = G_::n, S = G_::S;
     double f[4], g[4];
     ns_total_fluxes(uN, gxN, gyN, gm1, c, f, g);
     if constexpr (WALL) wall_first(wq, 0.5, uN, gxN, gyN, gm1, c);
     double ut[4], utx[4], uty[4];
-    par_level<N, EB, CURV>(act, sidx, sbase, i, j, invJ, s0, s2, sDm, sD, f, g, ut, utx, uty, Jn);
+    if (has_src) {
+        double s[4], st[4], ds[4];
+        ns_src_eval(sp, 0.0, s);
+        ns_src_t5(sp, dt, st);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            ds[v] = dt * s[v];
+            Ssrc[v] = fma(0.125, st[v], 0.5 * s[v]);
+        }
+        par_level<N, EB, CURV>(act, sidx, sbase, i, j, invJ, s0, s2, sDm, sD, f, g, ut, utx, uty, Jn, ds);
+    } else {
+        par_level<N, EB, CURV>(act, sidx, sbase, i, j, invJ, s0, s2, sDm, sD, f, g, ut, utx, uty, Jn);
+    }
     double dx[4], dy[4];
     if constexpr (WALL) {
         double v6[6];
@@ -1939,7 +2041,10 @@ __device__ __forceinline__ void mdrkp1_element(const double* uN, const double* g
 // mdrk_kernel_2! with viscous terms (multi_derivative_2d_parabolic.jl:506-711):
 // F̃ = F2 + f_t(us)/3, Ũ = U2 + ust/3, with F2 = f + f_t/6 and U2 = u + ut/6
 // recomputed from uⁿ and its BR1 gradient.  WALL: the wall records take the
-// stage's viscous average f_v + f_v,t/6 + f_v,t(us)/3.
+// stage's viscous average f_v + f_v,t/6 + f_v,t(us)/3.  With the Navier-Stokes
+// convergence source (has_src; 576-601, 668-674): ut and ust both += dt s(t)
+// before their gradients (the reference evaluates stage 2's source at the
+// call's t), Ssrc = S2 + s_t/3 with S2 = s + s_t/6.
 template <int N, int EB, bool CURV = false, bool WALL = false>
 __device__ __forceinline__ void mdrkp2_element(const double* uN, const double* gxN, const double* gyN,
                                                const double* usn, const double* gxS, const double* gyS,
@@ -1947,7 +2052,9 @@ __device__ __forceinline__ void mdrkp2_element(const double* uN, const double* g
                                                int sidx, int sbase, double* s0, double* s1, double* s2,
                                                const double* sDm, const double* sD, const double* sDh, double* Uc,
                                                double* F, double* G, double* vol, const double* Jn = nullptr,
-                                               double* const* wq = nullptr) {
+                                               double* const* wq = nullptr, bool has_src = false,
+                                               const NsSrcPt& sp = NsSrcPt{}, double dt = 0.0,
+                                               double* Ssrc = nullptr) {
     using G_ = VGeo<N, EB>;
     constexpr int n = G_::n, S = G_::S;
     double f[4], g[4], fs[4], gs[4];
@@ -1972,6 +2079,17 @@ __device__ __forceinline__ void mdrkp2_element(const double* uN, const double* g
     for (int v = 0; v < 4; ++v) {
         ut[v] *= invJ;
         ust[v] *= invJ;
+    }
+    if (has_src) {
+        double s[4], st[4];
+        ns_src_eval(sp, 0.0, s);
+        ns_src_t5(sp, dt, st);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            ut[v] = fma(dt, s[v], ut[v]);
+            ust[v] = fma(dt, s[v], ust[v]);
+            Ssrc[v] = fma(1.0 / 3.0, st[v], fma(1.0 / 6.0, st[v], s[v]));
+        }
     }
     if (act) {
         put4<S>(s2, sidx, ut);
@@ -2442,6 +2560,18 @@ __device__ __forceinline__ void volume_body(const KArgs& a, int bid, double* lds
         double* sg = (KB == VOL_MDRK2) ? s2 : s1;
         double gxN[4], gyN[4];
         if (act) put4<S>(sg, sidx, uN);
+        // the Navier-Stokes convergence source (MDRK on the TreeMesh): the node's
+        // spatial factors once, at the call's t
+        NsSrcPt nsp{};
+        bool nsrc = false;
+        if constexpr (!CURV && (KB == VOL_MDRK1 || KB == VOL_MDRK2)) {
+            nsrc = c->src_kind == SRC_NS_CONVERGENCE_TEST;
+            if (nsrc) {
+                const double hh = 0.5 * a.length[e];
+                nsp = ns_src_point(a.center[2 * e] + hh * c->nodes[i], a.center[2 * e + 1] + hh * c->nodes[j],
+                                   a.use_state_dt ? st->t : a.t_arg, c);
+            }
+        }
         if constexpr (KB == VOL_MDRK2) {
             double usn[4], gxS[4], gyS[4];
             load4(a.us + off, usn);
@@ -2450,7 +2580,7 @@ __device__ __forceinline__ void volume_body(const KArgs& a, int bid, double* lds
             br1_gradient<N, EB, CURV>(sg, uin, a.nbr, c, e, i, j, sbase, invJ, sDh, uN, gxN, gyN, Jn);
             br1_gradient<N, EB, CURV>(sg + 4 * S, a.us, a.nbr, c, e, i, j, sbase, invJ, sDh, usn, gxS, gyS, Jn);
             mdrkp2_element<N, EB, CURV, WALL>(uN, gxN, gyN, usn, gxS, gyS, invJ, gm1, c, i, j, act, sidx, sbase, s0,
-                                              s1, s2, sDm, sD, sDh, Uc, F, G, vol, Jn, wq);
+                                              s1, s2, sDm, sD, sDh, Uc, F, G, vol, Jn, wq, nsrc, nsp, dt, Ssrc);
         } else {
             __syncthreads();
             br1_gradient<N, EB, CURV>(sg, uin, a.nbr, c, e, i, j, sbase, invJ, sDh, uN, gxN, gyN, Jn);
@@ -2462,7 +2592,7 @@ __device__ __forceinline__ void volume_body(const KArgs& a, int bid, double* lds
                                                 sD, sDh, Uc, F, G, vol, Jn, wq, &terr, st->abstol, st->reltol);
             else
                 mdrkp1_element<N, EB, CURV, WALL>(uN, gxN, gyN, invJ, gm1, c, i, j, act, sidx, sbase, s0, s1, s2, sDm,
-                                                  sD, sDh, Uc, F, G, vol, Jn, wq);
+                                                  sD, sDh, Uc, F, G, vol, Jn, wq, nsrc, nsp, dt, Ssrc);
         }
     } else if (KB == VOL_LW || KB == VOL_LW_ADAPT) {
         if constexpr (N == 4 && KB == VOL_LW_ADAPT)
@@ -2699,7 +2829,7 @@ __device__ __forceinline__ void volume_body(const KArgs& a, int bid, double* lds
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         double d = (-invJ) * (vol[v] + surf[v]);
-        if constexpr (SRC) d += Ssrc[v];
+        if constexpr (SRC || (PAR && !CURV && (KB == VOL_MDRK1 || KB == VOL_MDRK2))) d += Ssrc[v];
         val[v] = a.step_mode ? fma(dt, d, uN[v]) : d;
     }
     if (act) store4(out + off, val);

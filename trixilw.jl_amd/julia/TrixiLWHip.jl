@@ -35,6 +35,7 @@ const TLW_BV_OUTFLOW = Int32(7)               # boundary_condition_outflow (curv
 const TLW_BV_SLIP_WALL_APPROXIMATE = Int32(8) # slip_wall_approximate (curvilinear StructuredMesh)
 const TLW_SRC_NONE = Int32(0)
 const TLW_SRC_CONVERGENCE_TEST = Int32(1)  # source_terms = source_terms_convergence_test
+const TLW_SRC_NAVIER_STOKES_CONVERGENCE_TEST = Int32(2)  # source_terms_navier_stokes_convergence_test (MDRK)
 const TLW_AUX_U = Int32(0)
 const TLW_AUX_F = Int32(1)
 const TLW_AUX_ALPHA = Int32(2)

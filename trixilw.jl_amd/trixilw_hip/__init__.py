@@ -14,7 +14,10 @@ from .equations import (Adiabatic, BoundaryConditionDirichlet, BoundaryCondition
                         ConvergenceTest, IsentropicVortex, SourceTermsConvergenceTest, TaylorGreenVortex,
                         boundary_condition_outflow, boundary_condition_slip_wall_horizontal,
                         initial_condition_convergence_test, slip_wall_approximate,
-                        source_terms_convergence_test,
+                        source_terms_convergence_test, boundary_condition_periodic,
+                        NavierStokesConvergenceTest, SourceTermsNavierStokesConvergenceTest,
+                        initial_condition_navier_stokes_convergence_test,
+                        source_terms_navier_stokes_convergence_test,
                         boundary_condition_slip_wall_vertical)
 from .integrator import (LWIntegrator, LWSolution, compute_dt, isapprox, isfinished, perform_step, set_dt,
                          set_t_and_iter, solve_lwfr)
@@ -42,5 +45,7 @@ __all__ = [
     "pressure", "adjoint_step", "adjoint_step_mdrk", "gradient_lw", "kinetic_energy", "gaussian_random_field", "objective_lw",
     "SemidiscretizationBatch", "SemidiscretizationEnsemble", "draw_vortex_ensemble", "SemidiscretizationHyperbolicParabolic", "CompressibleNavierStokesDiffusion2D",
     "GradientVariablesConservative", "GradientVariablesPrimitive", "ConvergenceTest", "SourceTermsConvergenceTest",
-    "initial_condition_convergence_test", "source_terms_convergence_test",
+    "initial_condition_convergence_test", "source_terms_convergence_test", "boundary_condition_periodic",
+    "NavierStokesConvergenceTest", "SourceTermsNavierStokesConvergenceTest",
+    "initial_condition_navier_stokes_convergence_test", "source_terms_navier_stokes_convergence_test",
 ]

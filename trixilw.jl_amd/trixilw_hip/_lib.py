@@ -18,7 +18,7 @@ TLW_FLUX_TREE, TLW_FLUX_STRUCTURED = 0, 1
 TLW_BV_NONE, TLW_BV_VORTEX, TLW_BV_CONSTANT, TLW_BV_RIEMANN4, TLW_BV_SLIP_WALL, TLW_BV_TABLE = 0, 1, 2, 3, 4, 5
 TLW_BV_CONVERGENCE_TEST = 6
 TLW_BV_OUTFLOW, TLW_BV_SLIP_WALL_APPROXIMATE = 7, 8
-TLW_SRC_NONE, TLW_SRC_CONVERGENCE_TEST = 0, 1
+TLW_SRC_NONE, TLW_SRC_CONVERGENCE_TEST, TLW_SRC_NAVIER_STOKES_CONVERGENCE_TEST = 0, 1, 2
 TLW_AUX_U, TLW_AUX_F, TLW_AUX_ALPHA = 0, 1, 2
 TLW_FLAG_KEEP_AUX = 1
 TLW_PAR_NAVIER_STOKES = 1

@@ -17,7 +17,7 @@ import numpy as np
 
 BV_NONE, BV_VORTEX, BV_CONSTANT, BV_RIEMANN4, BV_SLIP_WALL, BV_TABLE, BV_CONVERGENCE_TEST = 0, 1, 2, 3, 4, 5, 6
 BV_OUTFLOW, BV_SLIP_WALL_APPROXIMATE = 7, 8
-SRC_NONE, SRC_CONVERGENCE_TEST = 0, 1
+SRC_NONE, SRC_CONVERGENCE_TEST, SRC_NS_CONVERGENCE_TEST = 0, 1, 2
 
 
 class BoundaryConditionDirichlet:
@@ -64,6 +64,17 @@ class _SlipWall:
 
 boundary_condition_slip_wall_vertical = _SlipWall(1)
 boundary_condition_slip_wall_horizontal = _SlipWall(2)
+
+
+class _Periodic:
+    """Trixi's boundary_condition_periodic: allowed in the per-direction forms
+    for the directions in which the mesh is periodic (no boundary faces)."""
+
+    def __repr__(self):
+        return "boundary_condition_periodic"
+
+
+boundary_condition_periodic = _Periodic()
 
 
 class _NormalCondition:
@@ -162,6 +173,11 @@ class BoundaryConditionNavierStokesWall:
         self.boundary_condition_velocity = boundary_condition_velocity
         self.boundary_condition_heat_flux = boundary_condition_heat_flux
 
+    @classmethod
+    def periodic(cls):
+        """The stand-in of a periodic direction (no wall faces there): its values are unused."""
+        return cls(NoSlip((0.0, 0.0)), Adiabatic(0.0))
+
     def resolve(self, points):
         """(wall velocity, wall temperature or None) on the wall's face nodes."""
         heat = self.boundary_condition_heat_flux
@@ -181,24 +197,32 @@ def _builtin_dirichlet(b):
     return b
 
 
-def boundary_conditions_kind(bcs):
+def boundary_conditions_kind(bcs, periodicity=None):
     """One boundary-value description for every boundary face from the
     reference's forms: a single function object, or the NamedTuple (x_neg,
     x_pos, y_neg, y_pos) as a dict or 4-tuple.  Per-direction conditions
     must agree (the same function, or the vertical slip wall on the x faces
-    and the horizontal one on the y faces)."""
+    and the horizontal one on the y faces); boundary_condition_periodic may
+    stand for the directions in which the mesh is periodic (``periodicity``)."""
     if bcs is None or not isinstance(bcs, (dict, tuple, list)):
         return _builtin_dirichlet(bcs)
     seq = [_builtin_dirichlet(bcs[k]) for k in ("x_neg", "x_pos", "y_neg", "y_pos")] if isinstance(bcs, dict) \
         else [_builtin_dirichlet(b) for b in bcs]
     if len(seq) != 4:
         raise ValueError("per-direction boundary conditions: (x_neg, x_pos, y_neg, y_pos)")
-    if all(isinstance(b, _SlipWall) for b in seq):
-        if [b.orientation for b in seq] != [1, 1, 2, 2]:
+    per = tuple(periodicity) if periodicity is not None else (False, False)
+    for k, b in enumerate(seq):
+        if isinstance(b, _Periodic) and not per[k // 2]:
+            raise ValueError("boundary_condition_periodic in a direction in which the mesh is not periodic")
+    live = [(k, b) for k, b in enumerate(seq) if not isinstance(b, _Periodic)]
+    if not live:
+        return None
+    if all(isinstance(b, _SlipWall) for _, b in live):
+        if any(b.orientation != (1 if k < 2 else 2) for k, b in live):
             raise NotImplementedError("the slip wall of x faces is the vertical one, of y faces the horizontal one")
-        return seq[0]
-    if all(b is seq[0] for b in seq):
-        return seq[0]
+        return live[0][1]
+    if all(b is live[0][1] for _, b in live):
+        return live[0][1]
     raise NotImplementedError("the device path takes one boundary condition for every boundary face")
 
 
@@ -372,6 +396,41 @@ class SourceTermsConvergenceTest:
 
 initial_condition_convergence_test = ConvergenceTest()
 source_terms_convergence_test = SourceTermsConvergenceTest()
+
+
+class NavierStokesConvergenceTest:
+    """initial_condition_navier_stokes_convergence_test of the reference's
+    examples/tree_2d_dgsem/elixir_navierstokes_convergence.jl:31-45:
+    ρ = 2 + ½ sin πx cos πy cos πt, v1 = v2 = sin πx log(y + 2)(1 - e^{-(y-1)/2}) cos πt,
+    p = ρ² (zero velocity on the walls y = ±1)."""
+
+    def __call__(self, x, y, t, equations=None):
+        gamma = getattr(equations, "gamma", 1.4)
+        x = np.asarray(x, dtype=np.float64)
+        y = np.asarray(y, dtype=np.float64)
+        rho = 2.0 + 0.5 * np.sin(np.pi * x) * np.cos(np.pi * y) * np.cos(np.pi * t)
+        v = np.sin(np.pi * x) * np.log(y + 2.0) * (1.0 - np.exp(-0.5 * (y - 1.0))) * np.cos(np.pi * t)
+        p = rho * rho
+        return np.stack([rho, rho * v, rho * v, p / (gamma - 1.0) + 0.5 * rho * (v * v + v * v)], axis=-1)
+
+
+class SourceTermsNavierStokesConvergenceTest:
+    """source_terms_navier_stokes_convergence_test of the same example: the
+    residual of NavierStokesConvergenceTest's state in the compressible
+    Navier-Stokes equations.  The reference's function reads the example's
+    mu() = 0.01 and prandtl_number() = 0.72, the values its equations take; the
+    device evaluates it with the context's μ and Pr, so a semidiscretization
+    accepts it only when those match ``mu`` and ``prandtl`` here.  It runs in the
+    Navier-Stokes MDRK stages on a TreeMesh."""
+
+    kind = SRC_NS_CONVERGENCE_TEST
+
+    def __init__(self, mu=0.01, prandtl=0.72):
+        self.mu, self.prandtl = float(mu), float(prandtl)
+
+
+initial_condition_navier_stokes_convergence_test = NavierStokesConvergenceTest()
+source_terms_navier_stokes_convergence_test = SourceTermsNavierStokesConvergenceTest()
 
 
 @dataclass

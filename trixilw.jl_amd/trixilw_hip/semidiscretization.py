@@ -163,11 +163,16 @@ class SemidiscretizationHyperbolic:
                  source_terms=None, surface_flux_order="tree", keep_aux=False, device=None):
         src_kind = _lib.TLW_SRC_NONE
         if source_terms is not None:
-            from .equations import SRC_CONVERGENCE_TEST
+            from .equations import SRC_CONVERGENCE_TEST, SRC_NS_CONVERGENCE_TEST
 
-            if getattr(source_terms, "kind", None) != SRC_CONVERGENCE_TEST:
-                raise NotImplementedError("the device path implements source_terms_convergence_test")
-            src_kind = _lib.TLW_SRC_CONVERGENCE_TEST
+            kind = getattr(source_terms, "kind", None)
+            if kind == SRC_CONVERGENCE_TEST:
+                src_kind = _lib.TLW_SRC_CONVERGENCE_TEST
+            elif kind == SRC_NS_CONVERGENCE_TEST:
+                src_kind = _lib.TLW_SRC_NAVIER_STOKES_CONVERGENCE_TEST
+            else:
+                raise NotImplementedError("the device path implements source_terms_convergence_test and "
+                                          "source_terms_navier_stokes_convergence_test")
         self.source_terms = source_terms
         if not isinstance(equations, CompressibleEulerEquations2D):
             raise NotImplementedError("the device path implements CompressibleEulerEquations2D")
@@ -195,7 +200,7 @@ class SemidiscretizationHyperbolic:
             surface_flux_order]
         from .equations import boundary_conditions_kind
 
-        bcd = boundary_conditions_kind(boundary_conditions)
+        bcd = boundary_conditions_kind(boundary_conditions, getattr(mesh, "periodicity", None))
         if bcd is None:
             sd.boundary_kind = _lib.TLW_BV_NONE
         else:
@@ -228,7 +233,7 @@ class SemidiscretizationHyperbolic:
         library's table order, and the steady values uploaded once."""
         from .equations import BoundaryConditionDirichlet, boundary_conditions_kind
 
-        bcd = boundary_conditions_kind(self.boundary_conditions)
+        bcd = boundary_conditions_kind(self.boundary_conditions, getattr(self.mesh, "periodicity", None))
         self._bc_table = bcd if isinstance(bcd, BoundaryConditionDirichlet) else None
         if self._bc_table is None:
             return
@@ -507,7 +512,7 @@ class SemidiscretizationHyperbolicParabolic(SemidiscretizationHyperbolic):
         eq, eqp = equations
         if not isinstance(eqp, CompressibleNavierStokesDiffusion2D):
             raise NotImplementedError("the device path implements CompressibleNavierStokesDiffusion2D")
-        from .equations import BoundaryConditionNavierStokesWall
+        from .equations import BoundaryConditionNavierStokesWall, boundary_condition_periodic
 
         bcs = boundary_conditions if boundary_conditions is not None else (None, None)
         bc_hyp, bc_par = bcs
@@ -515,9 +520,15 @@ class SemidiscretizationHyperbolicParabolic(SemidiscretizationHyperbolic):
             raise NotImplementedError("the reference's parabolic kernels take VolumeIntegralFR only")
         self.equations_parabolic = eqp
         self._par = _lib.ParabolicDesc(_lib.TLW_PAR_NAVIER_STOKES, eqp.mu, eqp.Pr, _lib.TLW_GRAD_CONSERVATIVE)
+        if (getattr(source_terms, "mu", eqp.mu), getattr(source_terms, "prandtl", eqp.Pr)) != (eqp.mu, eqp.Pr):
+            raise NotImplementedError("the Navier-Stokes convergence source takes the equations' mu and Prandtl")
         if bc_par is not None:
-            # boundary_conditions_parabolic: the NamedTuple (x_neg, x_pos, y_neg, y_pos) of walls
+            # boundary_conditions_parabolic: the NamedTuple (x_neg, x_pos, y_neg, y_pos) of walls;
+            # directions in which the mesh is periodic may name boundary_condition_periodic
             seq = [bc_par[k] for k in ("x_neg", "x_pos", "y_neg", "y_pos")] if isinstance(bc_par, dict) else bc_par
+            per = tuple(getattr(mesh, "periodicity", (False, False)))
+            seq = [BoundaryConditionNavierStokesWall.periodic() if (b is boundary_condition_periodic and per[k // 2])
+                   else b for k, b in enumerate(seq)]
             if len(seq) != 4 or not all(isinstance(b, BoundaryConditionNavierStokesWall) for b in seq):
                 raise NotImplementedError("parabolic boundary conditions: four BoundaryConditionNavierStokesWall")
             self._par.walls = 1
