@@ -1,0 +1,111 @@
+"""The reference's lid-driven cavity at Re = 1000 against Ghia, Ghia & Shin
+(1982), the centreline velocities the reference keeps in
+data/lid_driven_cavity_{y_vs_u1,x_vs_u2}_r1000.txt (copied to tests/golden/).
+The setup is examples/tree_2d_dgsem/elixir_navierstokes_lid_driven_cavity_ghia.jl:
+[0, 1]², 16² cells, polydeg 4, LW, μ = 0.001, Pr = 0.7, Ma 0.1 at rest, lid
+velocity (1, 0), reflecting walls for the hyperbolic part with no-slip adiabatic
+walls, CFL 0.98 to t = 25 (the time the reference's utils/extract_lid_driven_cavity.jl
+compares at).  The device loop runs the whole solve; the velocity along the
+vertical and horizontal centrelines (element-boundary nodes averaged, GLL
+Lagrange interpolation along the line) is compared with the table's interior
+rows within 0.03 (the device reaches 0.016 and 0.015).  This checks the wall
+path against data held by the reference, not against the oracle."""
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from gpu_helpers import to_dev, to_lex, torch_mod
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+GAMMA = 1.4
+
+
+def lagrange_row(nodes, xi):
+    """Values at xi of the Lagrange polynomials on the GLL nodes."""
+    L = np.ones(len(nodes))
+    for k, xk in enumerate(nodes):
+        for m, xm in enumerate(nodes):
+            if m != k:
+                L[k] *= (xi - xm) / (xk - xm)
+    return L
+
+
+def centreline(u_lex, nodes, nx, along, comp, coords):
+    """Velocity component `comp` (1: v1, 2: v2) along the line x = 0.5 (along =
+    "y") or y = 0.5 (along = "x") at the given coordinates on [0, 1]."""
+    n = len(nodes)
+    out = []
+    for c in coords:
+        k = min(int(c * nx), nx - 1)
+        L = lagrange_row(nodes, 2.0 * (c * nx - k) - 1.0)
+        vals = []
+        for side in (nx // 2 - 1, nx // 2):  # the two elements that share the centreline
+            node = n - 1 if side == nx // 2 - 1 else 0
+            if along == "y":  # e = ey * nx + ex, nodes [j, i]
+                line = u_lex[k * nx + side, :, node, :]
+            else:
+                line = u_lex[side * nx + k, node, :, :]
+            vel = line[:, comp] / line[:, 0]
+            vals.append(L @ vel)
+        out.append(0.5 * (vals[0] + vals[1]))
+    return np.array(out)
+
+
+def test_cavity_re1000_matches_ghia():
+    import trixilw_hip as tlw
+
+    torch = torch_mod()
+    eq = tlw.CompressibleEulerEquations2D(GAMMA)
+    eqp = tlw.CompressibleNavierStokesDiffusion2D(eq, mu=0.001, Prandtl=0.7,
+                                                 gradient_variables=tlw.GradientVariablesConservative())
+    solver = tlw.DGSEM(4, volume_integral=tlw.VolumeIntegralFR(tlw.LW()))
+    mesh = tlw.TreeMesh((0.0, 0.0), (1.0, 1.0), 4, periodicity=False)
+
+    def cavity(x, y, t=0.0, equations=None):
+        z = np.zeros_like(np.asarray(x, dtype=np.float64))
+        return eq.prim2cons(z + 1.0, z, z, z + 1.0 / (0.1 ** 2 * GAMMA))
+
+    lid = tlw.BoundaryConditionNavierStokesWall(tlw.NoSlip(lambda x, t, e: (1.0, 0.0)),
+                                                tlw.Adiabatic(lambda x, t, e: 0.0))
+    rest = tlw.BoundaryConditionNavierStokesWall(tlw.NoSlip(lambda x, t, e: (0.0, 0.0)),
+                                                 tlw.Adiabatic(lambda x, t, e: 0.0))
+    bc_hyp = {"x_neg": tlw.boundary_condition_slip_wall_vertical, "x_pos": tlw.boundary_condition_slip_wall_vertical,
+              "y_neg": tlw.boundary_condition_slip_wall_horizontal,
+              "y_pos": tlw.boundary_condition_slip_wall_horizontal}
+    bc_par = {"x_neg": rest, "y_neg": rest, "y_pos": lid, "x_pos": rest}
+    semi = tlw.SemidiscretizationHyperbolicParabolic(mesh, tlw.get_time_discretization(solver), (eq, eqp), cavity,
+                                                     solver, boundary_conditions=(bc_hyp, bc_par))
+    nodes = np.asarray(solver.basis.nodes)
+    n, nx = len(nodes), mesh.nx
+    xi = np.asarray(nodes)
+    x = (np.arange(nx)[None, None, :, None] + 0.5 * (xi[None, None, None, :] + 1.0)) / nx
+    y = (np.arange(nx)[:, None, None, None] + 0.5 * (xi[None, :, None, None] + 1.0)) / nx
+    X = np.broadcast_to(x, (nx, n, nx, n)).transpose(0, 2, 1, 3).reshape(nx * nx, n, n)
+    Y = np.broadcast_to(y, (nx, n, nx, n)).transpose(0, 2, 1, 3).reshape(nx * nx, n, n)
+    u0 = cavity(X, Y)
+    lw_update = tlw.semidiscretize(semi, semi.time_discretization, (0.0, 25.0))
+    lw_update.u0.copy_(to_dev(mesh, u0))
+    print("cavity Re 1000: device loop to t = 25", flush=True)
+    sol = tlw.solve_lwfr(lw_update, (), 1e-6, None, time_step_computation=tlw.CFLBased(0.98))
+    torch.cuda.synchronize()
+    print(f"cavity Re 1000: t = {sol.t}, {sol.stats.naccept} steps", flush=True)
+    assert abs(sol.t - 25.0) < 1e-12
+    u = to_lex(mesh, sol.u)
+    assert np.all(np.isfinite(u))
+    # the table's wall rows (0 and 1) hold the wall velocity, which DG imposes weakly:
+    # the interior rows are compared
+    gy = np.loadtxt(GOLDEN / "ghia_re1000_y_vs_u1.txt")
+    gx = np.loadtxt(GOLDEN / "ghia_re1000_x_vs_u2.txt")
+    gy = gy[(gy[:, 0] > 0.0) & (gy[:, 0] < 1.0)]
+    gx = gx[(gx[:, 0] > 0.0) & (gx[:, 0] < 1.0)]
+    u1 = centreline(u, nodes, nx, "y", 1, gy[:, 0])
+    u2 = centreline(u, nodes, nx, "x", 2, gx[:, 0])
+    d1, d2 = np.abs(u1 - gy[:, 1]), np.abs(u2 - gx[:, 1])
+    print(f"cavity Re 1000: max |u1 - Ghia| {d1.max():.4f}, max |u2 - Ghia| {d2.max():.4f}", flush=True)
+    print("u1", np.round(u1, 4).tolist(), "u2", np.round(u2, 4).tolist(), flush=True)
+    # measured: 0.016 (u1) and 0.015 (u2) after 378318 steps
+    assert d1.max() < 0.03 and d2.max() < 0.03
+    semi.close()
