@@ -79,7 +79,8 @@ enum { TLW_BV_NONE = 0, TLW_BV_VORTEX = 1, TLW_BV_CONSTANT = 2, TLW_BV_RIEMANN4 
  * examples/tree_2d_dgsem/elixir_navierstokes_convergence.jl (the residual of its manufactured
  * solution under the context's μ and Pr), in the Navier-Stokes MDRK stages on a TreeMesh
  * (multi_derivative_2d_parabolic.jl:221-711; tlw_create_parabolic with TLW_MDRK, periodic or with
- * walls); tlw_rhs_lw on such a context returns TLW_ENOTSUP. */
+ * walls) and in the LW Navier-Stokes kernels with walls (dg_2d_parabolic.jl:725-1860, as the
+ * example's walls need); an LW context without walls returns TLW_ENOTSUP. */
 enum { TLW_SRC_NONE = 0, TLW_SRC_CONVERGENCE_TEST = 1, TLW_SRC_NAVIER_STOKES_CONVERGENCE_TEST = 2 };
 
 /* tlw_get_aux selectors (element_cache fields, src/solvers/dgsem_tree/containers.jl:73-98,

@@ -862,8 +862,8 @@ int rhs_common(tlw_ctx* c, int kind, const double* u, double* du, double t, doub
     if (!std::isfinite(dt) || !std::isfinite(t)) return fail(c, TLW_EINVAL, "t and dt must be finite");
     if (kind != VOL_LW && c->time_disc != TLW_MDRK)
         return fail(c, TLW_EINVAL, "rhs_mdrk needs a context created with TLW_MDRK");
-    if (kind == VOL_LW && c->hconst.src_kind == TLW_SRC_NAVIER_STOKES_CONVERGENCE_TEST)
-        return fail(c, TLW_ENOTSUP, "the Navier-Stokes convergence source runs in the MDRK stages");
+    if (kind == VOL_LW && c->hconst.src_kind == TLW_SRC_NAVIER_STOKES_CONVERGENCE_TEST && !c->hconst.ns_walls)
+        return fail(c, TLW_ENOTSUP, "the Navier-Stokes convergence source runs in the MDRK stages or with walls");
     if (c->hconst.bc_kind == TLW_BV_TABLE && c->has_boundary && c->btab_levels == 0)
         return fail(c, TLW_EINVAL, "no boundary data uploaded (tlw_set_boundary_data)");
     KArgs a = base_args(c);
@@ -1965,7 +1965,7 @@ int create_common(int ninst, const tlw_mesh_desc* meshes, const tlw_basis_desc* 
     if (solver->source_terms == TLW_SRC_CONVERGENCE_TEST && (solver->shock_capturing || par || cv)) return TLW_ENOTSUP;
     if (solver->source_terms == TLW_SRC_NAVIER_STOKES_CONVERGENCE_TEST &&
         (!par || par->equations != TLW_PAR_NAVIER_STOKES || cv || pd || ninst != 1 ||
-         solver->time_discretization != TLW_MDRK))
+         (solver->time_discretization != TLW_MDRK && !par->walls)))
         return TLW_ENOTSUP;
     tlw_ctx* c = new tlw_ctx;
     c->N = basis->polydeg;

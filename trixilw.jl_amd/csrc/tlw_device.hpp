@@ -405,6 +405,31 @@ __device__ __forceinline__ void ns_src_t5(const NsSrcPt& p, double dt, double* o
     for (int v = 0; v < 4; ++v) o[v] = (1.0 / 12.0) * (o[v] + a[v]);
 }
 
+// The time difference of order k of the source at the node about t, as the
+// reference's LW kernels form it (calc_source / _t_N34 / _tt_N4 (five_tt) or
+// _tt_N23 / _ttt_N34 / _tttt_N4, dg_2d.jl:89-176).
+__device__ __forceinline__ void ns_src_diff(const NsSrcPt& p, double dt, int order, bool five_tt, double* o) {
+    double a[4], b[4], z[4], c[4], d[4];
+    ns_src_eval(p, 0.0, z);
+    if (order == 0) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) o[v] = z[v];
+        return;
+    }
+    ns_src_eval(p, 2.0 * dt, a);
+    ns_src_eval(p, dt, b);
+    ns_src_eval(p, -dt, c);
+    ns_src_eval(p, -2.0 * dt, d);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        if (order == 1) o[v] = (1.0 / 12.0) * (-a[v] + 8.0 * b[v] - 8.0 * c[v] + d[v]);
+        else if (order == 2 && five_tt) o[v] = (1.0 / 12.0) * (-a[v] + 16.0 * b[v] - 30.0 * z[v] + 16.0 * c[v] - d[v]);
+        else if (order == 2) o[v] = b[v] - 2.0 * z[v] + c[v];
+        else if (order == 3) o[v] = 0.5 * (a[v] - 2.0 * b[v] + 2.0 * c[v] - d[v]);
+        else o[v] = a[v] - 4.0 * b[v] + 6.0 * z[v] - 4.0 * c[v] + d[v];
+    }
+}
+
 __device__ __forceinline__ void euler_flux_dir(const double* u, double gm1, int orientation, double* f) {
     double ff[4], gg[4];
     euler_fluxes(u, gm1, ff, gg);
@@ -1667,10 +1692,27 @@ __device__ __forceinline__ void lwp_element(const double* uN, const double* gxN,
                                             int sbase, double* s0, double* s1, double* s2, const double* sDm,
                                             const double* sD, const double* sDh, double* Uc, double* F, double* G,
                                             double* vol, const double* Jn = nullptr, double* const* wq = nullptr,
-                                            double* terr = nullptr, double abstol = 0.0, double reltol = 0.0) {
+                                            double* terr = nullptr, double abstol = 0.0, double reltol = 0.0,
+                                            bool has_src = false, const NsSrcPt& sp = NsSrcPt{},
+                                            double* Ssrc = nullptr, double dtv = 0.0) {
     using G_ = VGeo<N, EB>;
     constexpr int n = G_::n, S = G_::S;
     ns_total_fluxes(uN, gxN, gyN, gm1, c, F, G);
+    // the Navier-Stokes convergence source (the wall instantiation, has_src;
+    // dg_2d_parabolic.jl:725-1860): level k of the Taylor procedure gets dt·s^(k)
+    // before its gradient is formed, S the same weights as F̃ (s^(k) the time
+    // difference of ns_src_diff; kernel 4 takes the five-point s_tt)
+    auto src_lvl = [&](int order, double weight, double* ds) {
+        if constexpr (WALL) {
+            double v4[4] = {0.0, 0.0, 0.0, 0.0};
+            if (has_src) ns_src_diff(sp, dtv, order, N == 4, v4);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                if (ds) ds[v] = dtv * v4[v];
+                if (has_src) Ssrc[v] = order == 0 ? weight * v4[v] : fma(weight, v4[v], Ssrc[v]);
+            }
+        }
+    };
     // with walls: the time average of the viscous flux of a wall face node in the
     // face's orientation, alongside F̃, accumulated in the node's wall record
     // wq[o] (o = 0: its x face, 1: its y face; null where that face is no wall):
@@ -1727,7 +1769,13 @@ __device__ __forceinline__ void lwp_element(const double* uN, const double* gxN,
         }
     };
     double w[4], wx[4], wy[4];
-    par_level<N, EB, CURV>(act, sidx, sbase, i, j, invJ, s0, s2, sDm, sD, F, G, w, wx, wy, Jn);  // ut
+    if constexpr (WALL) {
+        double ds[4];
+        src_lvl(0, 1.0, ds);
+        par_level<N, EB, CURV>(act, sidx, sbase, i, j, invJ, s0, s2, sDm, sD, F, G, w, wx, wy, Jn, ds);  // ut
+    } else {
+        par_level<N, EB, CURV>(act, sidx, sbase, i, j, invJ, s0, s2, sDm, sD, F, G, w, wx, wy, Jn);  // ut
+    }
     // on a curvilinear mesh s0 took the contravariant pair; the centred stencils
     // read the node's own Cartesian f(uⁿ), written once every lane has passed
     // the contraction of s0
@@ -1760,7 +1808,13 @@ __device__ __forceinline__ void lwp_element(const double* uN, const double* gxN,
     }
     // f_t of the viscous part, weight 1/2
     wall_stage(P, 8.0, -8.0, -1.0, 1.0, 0.0, 1.0 / 12.0, 0.5, true);
-    par_level<N, EB, CURV>(act, sidx, sbase, i, j, invJ, s1, s2, sDm, sD, dx, dy, w, wx, wy, Jn);  // utt
+    if constexpr (WALL) {
+        double ds[4];
+        src_lvl(1, 0.5, ds);
+        par_level<N, EB, CURV>(act, sidx, sbase, i, j, invJ, s1, s2, sDm, sD, dx, dy, w, wx, wy, Jn, ds);  // utt
+    } else {
+        par_level<N, EB, CURV>(act, sidx, sbase, i, j, invJ, s1, s2, sDm, sD, dx, dy, w, wx, wy, Jn);  // utt
+    }
     P.add(0.5, 0.5, 2.0, 2.0, w, wx, wy);
 #pragma unroll
     for (int v = 0; v < 4; ++v) Uc[v] = fma(1.0 / 6.0, w[v], Uc[v]);
@@ -1806,7 +1860,13 @@ __device__ __forceinline__ void lwp_element(const double* uN, const double* gxN,
     // f_tt of the viscous part, weight 1/6 (five points at N = 4, three at N = 3)
     if constexpr (N == 4) wall_stage(P, 16.0, 16.0, -1.0, -1.0, -30.0, 1.0 / 12.0, 1.0 / 6.0, true);
     else wall_stage(P, 1.0, 1.0, 0.0, 0.0, -2.0, 1.0, 1.0 / 6.0, false);
-    par_level<N, EB, CURV>(act, sidx, sbase, i, j, invJ, s1, s2, sDm, sD, dx, dy, w, wx, wy, Jn);  // uttt
+    if constexpr (WALL) {
+        double ds[4];
+        src_lvl(2, 1.0 / 6.0, ds);
+        par_level<N, EB, CURV>(act, sidx, sbase, i, j, invJ, s1, s2, sDm, sD, dx, dy, w, wx, wy, Jn, ds);  // uttt
+    } else {
+        par_level<N, EB, CURV>(act, sidx, sbase, i, j, invJ, s1, s2, sDm, sD, dx, dy, w, wx, wy, Jn);  // uttt
+    }
     P.add(1.0 / 6.0, -1.0 / 6.0, 4.0 / 3.0, -4.0 / 3.0, w, wx, wy);
 #pragma unroll
     for (int v = 0; v < 4; ++v) Uc[v] = fma(1.0 / 24.0, w[v], Uc[v]);
@@ -1831,7 +1891,13 @@ __device__ __forceinline__ void lwp_element(const double* uN, const double* gxN,
     // f_ttt of the viscous part, weight 1/24
     wall_stage(P, -2.0, 2.0, 1.0, -1.0, 0.0, 0.5, 1.0 / 24.0, true);
     if constexpr (N == 4) {
-        par_level<N, EB, CURV>(act, sidx, sbase, i, j, invJ, s1, s2, sDm, sD, dx, dy, w, wx, wy, Jn);  // utttt
+        if constexpr (WALL) {
+            double ds[4];
+            src_lvl(3, 1.0 / 24.0, ds);
+            par_level<N, EB, CURV>(act, sidx, sbase, i, j, invJ, s1, s2, sDm, sD, dx, dy, w, wx, wy, Jn, ds);
+        } else {
+            par_level<N, EB, CURV>(act, sidx, sbase, i, j, invJ, s1, s2, sDm, sD, dx, dy, w, wx, wy, Jn);  // utttt
+        }
         P.add(1.0 / 24.0, 1.0 / 24.0, 2.0 / 3.0, 2.0 / 3.0, w, wx, wy);
 #pragma unroll
         for (int v = 0; v < 4; ++v) Uc[v] = fma(1.0 / 120.0, w[v], Uc[v]);
@@ -1865,6 +1931,9 @@ __device__ __forceinline__ void lwp_element(const double* uN, const double* gxN,
     }
     // f_tttt of the viscous part (with the 0.5), weight 1/120
     if constexpr (N == 4) wall_stage(P, -4.0, -4.0, 1.0, 1.0, 6.0, 0.5, 1.0 / 120.0, true);
+    // the source's last difference enters S only: s_tttt/120 (kernel 4), s_ttt/24 (kernel 3)
+    if constexpr (N == 4) src_lvl(4, 1.0 / 120.0, nullptr);
+    else src_lvl(3, 1.0 / 24.0, nullptr);
     __syncthreads();  // every thread has finished reading s2 before it is reused
     if constexpr (CURV) {
         double Fc[4], Gc[4];
@@ -2564,7 +2633,7 @@ __device__ __forceinline__ void volume_body(const KArgs& a, int bid, double* lds
         // spatial factors once, at the call's t
         NsSrcPt nsp{};
         bool nsrc = false;
-        if constexpr (!CURV && (KB == VOL_MDRK1 || KB == VOL_MDRK2)) {
+        if constexpr (!CURV && (KB == VOL_MDRK1 || KB == VOL_MDRK2 || (WALL && KB == VOL_LW))) {
             nsrc = c->src_kind == SRC_NS_CONVERGENCE_TEST;
             if (nsrc) {
                 const double hh = 0.5 * a.length[e];
@@ -2586,7 +2655,8 @@ __device__ __forceinline__ void volume_body(const KArgs& a, int bid, double* lds
             br1_gradient<N, EB, CURV>(sg, uin, a.nbr, c, e, i, j, sbase, invJ, sDh, uN, gxN, gyN, Jn);
             if constexpr (KB == VOL_LW)
                 lwp_element<N, EB, CURV, WALL>(uN, gxN, gyN, invJ, gm1, c, i, j, act, sidx, sbase, s0, s1, s2, sDm,
-                                               sD, sDh, Uc, F, G, vol, Jn, wq);
+                                               sD, sDh, Uc, F, G, vol, Jn, wq, nullptr, 0.0, 0.0, nsrc, nsp, Ssrc,
+                                               dt);
             else if constexpr (KIND == VOL_LW_ADAPT)
                 lwp_element<N, EB, CURV, false>(uN, gxN, gyN, invJ, gm1, c, i, j, act, sidx, sbase, s0, s1, s2, sDm,
                                                 sD, sDh, Uc, F, G, vol, Jn, wq, &terr, st->abstol, st->reltol);
@@ -2829,7 +2899,8 @@ __device__ __forceinline__ void volume_body(const KArgs& a, int bid, double* lds
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         double d = (-invJ) * (vol[v] + surf[v]);
-        if constexpr (SRC || (PAR && !CURV && (KB == VOL_MDRK1 || KB == VOL_MDRK2))) d += Ssrc[v];
+        if constexpr (SRC || (PAR && !CURV && (KB == VOL_MDRK1 || KB == VOL_MDRK2 || (WALL && KB == VOL_LW))))
+            d += Ssrc[v];
         val[v] = a.step_mode ? fma(dt, d, uN[v]) : d;
     }
     if (act) store4(out + off, val);
