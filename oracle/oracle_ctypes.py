@@ -143,7 +143,7 @@ def _ptr(a: np.ndarray):
 AUX = {"U": 0, "F": 1, "temporal_errors": 2, "surface_flux_values": 3, "us": 4, "U2": 5, "F2": 6, "alpha": 7,
        "fn_low": 8, "gradients_x": 9, "gradients_y": 10, "Fv": 11, "Fv2": 12, "contravariant_vectors": 13,
        "inverse_jacobian": 14}
-MAP_AFFINE, MAP_WARP = 0, 1
+MAP_AFFINE, MAP_WARP, MAP_PARALLELOGRAM = 0, 1, 2
 PARABOLIC = {None: 0, "laplace": 1, "navier_stokes": 2}
 
 

@@ -37,13 +37,18 @@ namespace tlwo {
 //   MAP_AFFINE: the box [p0, p0 + p2] × [p1, p1 + p3];
 //   MAP_WARP:   [0, L]² (p0 = L) with x = L (s + A sin 2πs sin 2πt),
 //               y = L (t + A sin 2πs sin 2πt), s, t ∈ [0, 1], A = p1; the warp
-//               vanishes on the domain boundary, so the mesh is periodic.
-enum MappingKind { MAP_AFFINE = 0, MAP_WARP = 1 };
+//               vanishes on the domain boundary, so the mesh is periodic;
+//   MAP_PARALLELOGRAM: x = ξ + η, y = η (the mapping of the reference's
+//               elixir_euler_source_terms_parallelogram.jl; p unused).
+enum MappingKind { MAP_AFFINE = 0, MAP_WARP = 1, MAP_PARALLELOGRAM = 2 };
 
 inline void apply_mapping(int kind, const double* p, double xi, double eta, double& x, double& y) {
     if (kind == MAP_AFFINE) {
         x = p[0] + 0.5 * (xi + 1.0) * p[2];
         y = p[1] + 0.5 * (eta + 1.0) * p[3];
+    } else if (kind == MAP_PARALLELOGRAM) {
+        x = xi + eta;
+        y = eta;
     } else {
         const double pi = 3.141592653589793;
         const double s = 0.5 * (xi + 1.0), t = 0.5 * (eta + 1.0);
@@ -155,8 +160,14 @@ template <class T, class Eq, class Base = Solver<T, Eq>> struct CurvedSolver : B
     // lw_volume_kernel_3!/4! for StructuredMesh (dg_2d.jl:1379-2105): the
     // TreeMesh procedure with contravariant fluxes in the derivative
     // contractions and the node's J⁻¹; the Cartesian time average F̃ is kept
-    // for the interfaces.  Source terms are `nothing`; the temporal-error
-    // estimate of adaptive steps is computed as the reference does.
+    // for the interfaces.  The temporal-error estimate of adaptive steps is
+    // computed as the reference does (without source terms).  Source terms
+    // (1440-1448, 1505-1513, 1564-1570, 1646-1666 in kernel 3; 1750-1756,
+    // 1812-1819, 1884-1889, 1946-1953, 2039-2058 in kernel 4): each level's time
+    // derivative gets dt times the source's time difference of its order at the
+    // states of the level before (s; s_t of N34; s_tt five-point, tt_N4, in
+    // kernel 4, three-point, N23, in kernel 3; s_ttt of N34 -- in kernel 3 for S
+    // only; s_tttt of N4 for S only), S the F̃ weights, du += -S/J⁻¹ per node.
     void lw_volume_curved(T* du, const T* u, double dt, int e, double alpha_dg = 1.0) {
         const auto& D = basis.D;
         const auto& Dh = basis.Dhat;
@@ -245,13 +256,52 @@ template <class T, class Eq, class Base = Solver<T, Eq>> struct CurvedSolver : B
                     }
                 }
         };
+        const bool has_src = this->src.kind != SRC_NONE;
+        T Sacc[MAXN][MAXN][NV];
+        // source level `order` at the current states: w += dt s^(k), S += wS s^(k)
+        auto src_level = [&](int order, double wS, bool into_w) {
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < n; ++i) {
+                    double x, y;
+                    node_xy(i, j, e, x, y);
+                    const T* un = &u[idx(0, i, j, e)];
+                    const T *up = st[0][j][i], *um = st[1][j][i], *upp = st[2][j][i], *umm = st[3][j][i];
+                    T sv[NV];
+                    if (order == 0) {
+                        this->src_at(un, x, y, 0.0, sv);
+                    } else if (order == 1) {
+                        this->src_t_N34(up, upp, um, umm, x, y, dt, this->src_t, sv);
+                    } else if (order == 2 && N == 4) {
+                        T a[NV], b[NV], z[NV], c[NV], d[NV];
+                        this->src_at(upp, x, y, 2.0 * dt, a);
+                        this->src_at(up, x, y, dt, b);
+                        this->src_at(un, x, y, 0.0, z);
+                        this->src_at(um, x, y, -dt, c);
+                        this->src_at(umm, x, y, -2.0 * dt, d);
+                        for (int v = 0; v < NV; ++v)
+                            sv[v] = (1.0 / 12.0) * (-a[v] + 16.0 * b[v] - 30.0 * z[v] + 16.0 * c[v] - d[v]);
+                    } else if (order == 2) {
+                        this->src_tt_N23(un, up, um, x, y, dt, sv);
+                    } else if (order == 3) {
+                        this->src_ttt_N34(up, upp, um, umm, x, y, dt, sv);
+                    } else {
+                        this->src_tttt_N4(un, up, upp, um, umm, x, y, dt, sv);
+                    }
+                    for (int v = 0; v < NV; ++v) {
+                        Sacc[j][i][v] = order == 0 ? wS * sv[v] : Sacc[j][i][v] + wS * sv[v];
+                        if (into_w) w[j][i][v] = w[j][i][v] + dt * sv[v];
+                    }
+                }
+        };
         advance(fc, gc);  // ut
+        if (has_src) src_level(0, 1.0, true);
         add_states(0.5, 1.0, -1.0, 2.0, -2.0);
         {
             const double wt[5] = {-1.0, 8.0, 0.0, -8.0, 1.0};  // f_t
             stage(wt, 1.0 / 12.0, 0.5);
         }
         advance(dfc, dgc);  // utt
+        if (has_src) src_level(1, 0.5, true);
         add_states(1.0 / 6.0, 0.5, 0.5, 2.0, 2.0);
         if (N == 4) {
             const double wt[5] = {-1.0, 16.0, -30.0, 16.0, -1.0};  // f_tt
@@ -261,6 +311,7 @@ template <class T, class Eq, class Base = Solver<T, Eq>> struct CurvedSolver : B
             stage(wt, 1.0, 1.0 / 6.0);
         }
         advance(dfc, dgc);  // uttt
+        if (has_src) src_level(2, 1.0 / 6.0, true);
         add_states(1.0 / 24.0, 1.0 / 6.0, -1.0 / 6.0, 4.0 / 3.0, -4.0 / 3.0);
         {
             const double wt[5] = {1.0, -2.0, 0.0, 2.0, -1.0};  // f_ttt
@@ -268,9 +319,13 @@ template <class T, class Eq, class Base = Solver<T, Eq>> struct CurvedSolver : B
         }
         if (N == 4) {
             advance(dfc, dgc);  // utttt
+            if (has_src) src_level(3, 1.0 / 24.0, true);
             add_states(1.0 / 120.0, 1.0 / 24.0, 1.0 / 24.0, 2.0 / 3.0, 2.0 / 3.0);
             const double wt[5] = {1.0, -4.0, 6.0, -4.0, 1.0};  // f_tttt, with the 0.5 of dg_2d.jl:2018-2021
             stage(wt, 0.5, 1.0 / 120.0);
+            if (has_src) src_level(4, 1.0 / 120.0, false);
+        } else if (has_src) {
+            src_level(3, 1.0 / 24.0, false);
         }
         // temporal error estimate of the adaptive controller (dg_2d.jl:1595-1619, 1664-1681 at N=3;
         // 1983-2007, 2058-2077 at N=4): u_np1 = u + J⁻¹(node) (-dt D)·F̃ with the contravariant F̃,
@@ -307,6 +362,8 @@ template <class T, class Eq, class Base = Solver<T, Eq>> struct CurvedSolver : B
                     this->U[idx(v, i, j, e)] = Uc[j][i][v];
                     this->F[fidx(v, 0, i, j, e)] = F[j][i][v];
                     this->F[fidx(v, 1, i, j, e)] = G[j][i][v];
+                    if (has_src)
+                        du[idx(v, i, j, e)] = du[idx(v, i, j, e)] + (-1.0 / ijn[nidx(i, j, e)]) * Sacc[j][i][v];
                 }
             }
     }
@@ -745,6 +802,7 @@ template <class T, class Eq, class Base = Solver<T, Eq>> struct CurvedSolver : B
     }
 
     void rhs_lw(T* du, const T* u, double t, double dt) override {
+        this->src_t = t;
         this->reset_du(du);
         if (this->sc) this->sc_prepare(u, dt);
 #pragma omp parallel for schedule(static)
@@ -889,9 +947,18 @@ template <class T, class Eq, class Base = Solver<T, Eq>> struct CurvedSolver : B
             }
     }
 
+    // source terms in the curvilinear MDRK stages are not restated: such a call
+    // fills du with NaN
+    bool mdrk_src_refused(T* du) const {
+        if (this->src.kind == SRC_NONE) return false;
+        std::fill(du, du + this->ndofs_total(), T(std::numeric_limits<double>::quiet_NaN()));
+        return true;
+    }
+
     // rhs_mdrk1! / rhs_mdrk2! for StructuredMesh (multi_derivative_2d.jl:1-80):
     // both stages evaluate the interface flux with λ of uⁿ
     void rhs_mdrk1(T* du, const T* u, double t, double dt) override {
+        if (mdrk_src_refused(du)) return;
         this->ensure_mdrk();
         this->reset_du(du);
         if (this->sc) this->sc_prepare(u, dt);
@@ -904,6 +971,7 @@ template <class T, class Eq, class Base = Solver<T, Eq>> struct CurvedSolver : B
         surface_integral_and_jacobian_curved(du);
     }
     void rhs_mdrk2(T* du, const T* u, double t, double dt) override {
+        if (mdrk_src_refused(du)) return;
         this->ensure_mdrk();
         this->reset_du(du);
         if (this->sc) this->sc_prepare(u, dt);

@@ -76,6 +76,7 @@ struct tlw_ctx {
     double* d_ijn = nullptr;
     double* d_fnorm = nullptr;
     double* d_fxy = nullptr;  // boundary face-node coordinates (curvilinear mesh with boundaries)
+    double* d_nxy = nullptr;  // node coordinates [E][n*n][2] (curvilinear mesh with source terms)
     double* d_fvb = nullptr;   // Navier-Stokes walls: viscous time average of boundary face nodes
     double* d_snrm = nullptr;  // subcell normals (curvilinear shock capturing) [E][n²][4]
     double* d_sqrk = nullptr;  // calc_fn_low! normals of subcell face n-1 [E][n][4]
@@ -175,7 +176,7 @@ void free_mesh(tlw_ctx* c) {
                     c->d_vblk,      c->d_sblk,  c->d_etile,  c->d_rng,    c->d_state,   c->d_cur,
                     c->d_Ja,        c->d_ijn,   c->d_fnorm,  c->d_fxy,     c->d_snrm,      c->d_sqrk,
                     c->d_fvb,       c->d_bidx,  c->d_btab,   c->d_mort,   c->d_mrec,    c->d_mflux,
-                    c->d_plink};
+                    c->d_plink,     c->d_nxy};
     for (void* p : ptrs) (void)hipFree(p);
     if (c->h_state) (void)hipHostFree(c->h_state);
     c->d_nbr = nullptr;
@@ -193,6 +194,7 @@ void free_mesh(tlw_ctx* c) {
     c->d_mort = nullptr;
     c->d_mrec = c->d_mflux = nullptr;
     c->d_plink = nullptr;
+    c->d_nxy = nullptr;
     c->n_mortars = 0;
     c->btab_cap = 0;
     c->btab_levels = 0;
@@ -307,6 +309,8 @@ int prepare_curved(tlw_ctx* c, const tlw_mesh_desc* m, const tlw_curved_desc* cv
     for (size_t k = 0; k < (size_t)4 * E; ++k) boundary = boundary || m->neighbors_host[k] < 0;
     const double* xy = cv->node_coordinates_host;
     if (boundary && !xy) return fail(c, TLW_EINVAL, "curved description: node coordinates needed on a mesh with boundaries");
+    if (c->hconst.src_kind != TLW_SRC_NONE && !xy)
+        return fail(c, TLW_EINVAL, "curved description: node coordinates needed with source terms");
     fnorm.assign((size_t)E * 4 * n * 4, 0.0);
     fxy.assign(boundary ? (size_t)E * 4 * n * 2 : 0, 0.0);
     invj.resize(E);
@@ -553,6 +557,11 @@ int upload_meshes(tlw_ctx* c, const tlw_mesh_desc* ms, int ninst, const tlw_curv
             HIPCHK(c, hipMalloc(&c->d_fxy, sizeof(double) * fxy.size()));
             HIPCHK(c, hipMemcpy(c->d_fxy, fxy.data(), sizeof(double) * fxy.size(), hipMemcpyHostToDevice));
         }
+        if (c->hconst.src_kind != TLW_SRC_NONE) {  // the cache's node_coordinates[m, i, j, e] as they are
+            HIPCHK(c, hipMalloc(&c->d_nxy, sizeof(double) * 2 * NN * E));
+            HIPCHK(c, hipMemcpy(c->d_nxy, cv->node_coordinates_host, sizeof(double) * 2 * NN * E,
+                                hipMemcpyHostToDevice));
+        }
         if (!snrm.empty()) {
             HIPCHK(c, hipMalloc(&c->d_snrm, sizeof(double) * snrm.size()));
             HIPCHK(c, hipMalloc(&c->d_sqrk, sizeof(double) * sqrk.size()));
@@ -624,6 +633,7 @@ KArgs base_args(tlw_ctx* c) {
     a.ijn = c->d_ijn;
     a.fnorm = c->d_fnorm;
     a.fxy = c->d_fxy;
+    a.nxy = c->d_nxy;
     a.slots = c->d_partial + c->slot_off;
     a.snrm = c->d_snrm;
     a.fvb = c->d_fvb;
@@ -683,6 +693,13 @@ template <int N, bool SC, bool PAR = false> void launch_volume_sc(tlw_ctx* c, co
 }
 template <int N, int EB, bool SC> void launch_volume_curved_eb(tlw_ctx* c, const KArgs& a, int kind) {
     dim3 g(c->vgrid), b(VGeo<N, EB>::NT);
+    if constexpr (!SC) {  // the LW kernel with source terms
+        if (kind == VOL_LW_SRC) {
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_LW_SRC, false, false, true>), g, b, 0, c->stream,
+                               a);
+            return;
+        }
+    }
     if (kind == VOL_LW)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_LW, SC, false, true>), g, b, 0, c->stream, a);
     else if (kind == VOL_LW_ADAPT)
@@ -722,7 +739,8 @@ template <int N> void launch_volume_t(tlw_ctx* c, const KArgs& a, int kind) {
             hipLaunchKernelGGL(k_indicator<N>, dim3(c->sgrid), dim3(kSurfBlock), 0, c->stream, a, c->d_alpha_tmp);
             launch_volume_curved<N, true>(c, a, kind);
         } else {
-            launch_volume_curved<N, false>(c, a, kind);
+            const bool src = c->hconst.src_kind != TLW_SRC_NONE && kind == VOL_LW;
+            launch_volume_curved<N, false>(c, a, src ? (int)VOL_LW_SRC : kind);
         }
     } else if (c->parabolic) {
         // walls: the instantiations that keep the wall face nodes' viscous averages
@@ -1962,7 +1980,10 @@ int create_common(int ninst, const tlw_mesh_desc* meshes, const tlw_basis_desc* 
         return TLW_EINVAL;
     // source terms: the Euler LW and MDRK kernels of the TreeMesh; the Navier-Stokes
     // convergence source: the Navier-Stokes MDRK kernels of the TreeMesh
-    if (solver->source_terms == TLW_SRC_CONVERGENCE_TEST && (solver->shock_capturing || par || cv)) return TLW_ENOTSUP;
+    // (on a curvilinear StructuredMesh: LW without shock capturing, structured/dg_2d.jl:1379-2105)
+    if (solver->source_terms == TLW_SRC_CONVERGENCE_TEST &&
+        (solver->shock_capturing || par || (cv && (pd || solver->time_discretization != TLW_LW))))
+        return TLW_ENOTSUP;
     if (solver->source_terms == TLW_SRC_NAVIER_STOKES_CONVERGENCE_TEST &&
         (!par || par->equations != TLW_PAR_NAVIER_STOKES || cv || pd || ninst != 1 ||
          (solver->time_discretization != TLW_MDRK && !par->walls)))

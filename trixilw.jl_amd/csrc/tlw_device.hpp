@@ -186,6 +186,7 @@ struct KArgs {
     const double* ijn;
     const double* fnorm;
     const double* fxy;  // boundary face-node coordinates [E][4][n][2] (curvilinear mesh with boundaries)
+    const double* nxy;  // node coordinates [E][n*n][2] (curvilinear mesh with source terms)
     double* slots;      // per instance: kSlots CFL maxima, then kSlots viscous maxima (k_finalize resets them)
     // Navier-Stokes walls: per boundary face node the time-averaged viscous flux
     // of the face's orientation (momentum x, momentum y, energy), then the same
@@ -322,6 +323,38 @@ __device__ __forceinline__ void src_tttt5(const SrcPt& p, double dt, double* o) 
     src_eval(p, -2.0 * dt, a);
 #pragma unroll
     for (int v = 0; v < 4; ++v) o[v] += a[v];
+}
+
+// s_tt of calc_source_tt_N4 (five points), which the StructuredMesh kernel 4
+// takes: (-s(t+2dt) + 16 s(t+dt) - 30 s(t) + 16 s(t-dt) - s(t-2dt))/12
+__device__ __forceinline__ void src_tt5(const SrcPt& p, double dt, double* o) {
+    double a[4], b[4];
+    src_eval(p, 2.0 * dt, a);
+    src_eval(p, dt, b);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) o[v] = 16.0 * b[v] - a[v];
+    src_eval(p, 0.0, a);
+    src_eval(p, -dt, b);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) o[v] = o[v] - 30.0 * a[v] + 16.0 * b[v];
+    src_eval(p, -2.0 * dt, a);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) o[v] = (1.0 / 12.0) * (o[v] - a[v]);
+}
+// Source level k of the StructuredMesh LW kernels (structured/dg_2d.jl:1440-1666,
+// 1750-2058): s^(k) about t into sk (s; s_t of N34; s_tt five-point at polydeg
+// 4, three-point at 3; s_ttt of N34; s_tttt of N4), and S += weight·s^(k)
+// (S = s at k = 0)
+__device__ __forceinline__ void src_level_c(const SrcPt& p, double dt, int k, bool five_tt, double* sk, double* S,
+                                            double weight) {
+    if (k == 0) src_eval(p, 0.0, sk);
+    else if (k == 1) src_t5(p, 0.0, dt, sk);
+    else if (k == 2 && five_tt) src_tt5(p, dt, sk);
+    else if (k == 2) src_tt3(p, dt, sk);
+    else if (k == 3) src_ttt5(p, dt, sk);
+    else src_tttt5(p, dt, sk);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) S[v] = k == 0 ? weight * sk[v] : fma(weight, sk[v], S[v]);
 }
 
 // The Navier-Stokes convergence source (the source_terms of
@@ -2229,12 +2262,15 @@ __device__ __forceinline__ void mdrkp2_element(const double* uN, const double* g
 // contractions; the volume term is D̂ of the contravariant time average.
 // f(uⁿ) is kept in registers (f0x, f0y) for the centred stencils.
 
-template <int N, int EB>
+// SRC: with the source terms of the StructuredMesh kernels (src_level_c; sp the
+// node's point, Ssrc the S of the residual, dtv the step).
+template <int N, int EB, bool SRC = false>
 __device__ __forceinline__ void lwc_element(const double* uN, const double* J, double invJ, double gm1, int i, int j,
                                             bool act, int sidx, int sbase, double* s0, double* s1, double* s2,
                                             const double* sDm, const double* sDh, double* Uc, double* F, double* G,
                                             double* vol, double* terr = nullptr, double abstol = 0.0,
-                                            double reltol = 0.0) {
+                                            double reltol = 0.0, const SrcPt* sp = nullptr, double* Ssrc = nullptr,
+                                            double dtv = 0.0) {
     using G_ = VGeo<N, EB>;
     constexpr int n = G_::n, S = G_::S;
     double f0x[4], f0y[4], cx[4], cy[4];
@@ -2247,9 +2283,12 @@ __device__ __forceinline__ void lwc_element(const double* uN, const double* J, d
     double w[4];
     contract<n, S>(s0, sbase, i, j, sDm, w);
     double up[4], um[4], upp[4], umm[4];
+    double sk[4] = {0.0, 0.0, 0.0, 0.0};
+    if constexpr (SRC) src_level_c(*sp, dtv, 0, N == 4, sk, Ssrc, 1.0);
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         w[v] *= invJ;  // ut
+        if constexpr (SRC) w[v] = fma(dtv, sk[v], w[v]);
         Uc[v] = fma(0.5, w[v], uN[v]);
         up[v] = uN[v] + w[v];
         um[v] = uN[v] - w[v];
@@ -2279,9 +2318,11 @@ __device__ __forceinline__ void lwc_element(const double* uN, const double* J, d
     if (act) put8<S>(s1, sidx, cx, cy);
     __syncthreads();
     contract<n, S>(s1, sbase, i, j, sDm, w);
+    if constexpr (SRC) src_level_c(*sp, dtv, 1, N == 4, sk, Ssrc, 0.5);
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         w[v] *= invJ;  // utt
+        if constexpr (SRC) w[v] = fma(dtv, sk[v], w[v]);
         Uc[v] = fma(1.0 / 6.0, w[v], Uc[v]);
         up[v] = fma(0.5, w[v], up[v]);
         um[v] = fma(0.5, w[v], um[v]);
@@ -2331,9 +2372,11 @@ __device__ __forceinline__ void lwc_element(const double* uN, const double* J, d
     if (act) put8<S>(s2, sidx, cx, cy);
     __syncthreads();
     contract<n, S>(s2, sbase, i, j, sDm, w);
+    if constexpr (SRC) src_level_c(*sp, dtv, 2, N == 4, sk, Ssrc, 1.0 / 6.0);
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         w[v] *= invJ;  // uttt
+        if constexpr (SRC) w[v] = fma(dtv, sk[v], w[v]);
         Uc[v] = fma(1.0 / 24.0, w[v], Uc[v]);
         um[v] = fma(-1.0 / 6.0, w[v], um[v]);
         up[v] = fma(1.0 / 6.0, w[v], up[v]);
@@ -2363,9 +2406,11 @@ __device__ __forceinline__ void lwc_element(const double* uN, const double* J, d
         if (act) put8<S>(s1, sidx, cx, cy);
         __syncthreads();
         contract<n, S>(s1, sbase, i, j, sDm, w);
+        if constexpr (SRC) src_level_c(*sp, dtv, 3, true, sk, Ssrc, 1.0 / 24.0);
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             w[v] *= invJ;  // utttt
+            if constexpr (SRC) w[v] = fma(dtv, sk[v], w[v]);
             Uc[v] = fma(1.0 / 120.0, w[v], Uc[v]);
             um[v] = fma(1.0 / 24.0, w[v], um[v]);
             up[v] = fma(1.0 / 24.0, w[v], up[v]);
@@ -2397,6 +2442,8 @@ __device__ __forceinline__ void lwc_element(const double* uN, const double* J, d
             G[v] = fma(1.0 / 120.0, dy[v], G[v]);
         }
     }
+    // the source's last difference enters S only: s_tttt/120 (kernel 4), s_ttt/24 (kernel 3)
+    if constexpr (SRC) src_level_c(*sp, dtv, N == 4 ? 4 : 3, N == 4, sk, Ssrc, N == 4 ? 1.0 / 120.0 : 1.0 / 24.0);
     to_contravariant(J, F, G, cx, cy);
     // polydeg 3 read s2 in its last contraction: every thread has finished
     // before it is reused (polydeg 4 passed a barrier since)
@@ -2582,10 +2629,15 @@ __device__ __forceinline__ void volume_body(const KArgs& a, int bid, double* lds
     constexpr bool SRC = vol_is_src(KIND);
     constexpr bool WALL = vol_is_wall(KIND);
     constexpr int KB = vol_base(KIND);
-    static_assert(!SRC || (!SC && !PAR && !CURV), "source terms: the Euler TreeMesh kernels");
+    static_assert(!SRC || (!SC && !PAR && (!CURV || KB == VOL_LW)),
+                  "source terms: the Euler TreeMesh kernels and the curvilinear LW kernel");
     double Ssrc[4] = {0.0, 0.0, 0.0, 0.0};
     SrcPt sp{0.0, 0.0, 0.0, gamma};
-    if constexpr (SRC) {
+    if constexpr (SRC && CURV) {  // the node's coordinates of the curvilinear mesh
+        sp.x = a.nxy[((size_t)e * NN + node) * 2];
+        sp.y = a.nxy[((size_t)e * NN + node) * 2 + 1];
+        sp.t = a.use_state_dt ? st->t : a.t_arg;
+    } else if constexpr (SRC) {
         const double hh = 0.5 * a.length[e];
         sp.x = a.center[2 * e] + hh * c->nodes[i];
         sp.y = a.center[2 * e + 1] + hh * c->nodes[j];
@@ -2612,6 +2664,9 @@ __device__ __forceinline__ void volume_body(const KArgs& a, int bid, double* lds
     if constexpr (CURV && !PAR) {
         if constexpr (KIND == VOL_LW) {
             lwc_element<N, EB>(uN, Jn, invJ, gm1, i, j, act, sidx, sbase, s0, s1, s2, sDm, sDh, Uc, F, G, vol);
+        } else if constexpr (KIND == VOL_LW_SRC) {
+            lwc_element<N, EB, true>(uN, Jn, invJ, gm1, i, j, act, sidx, sbase, s0, s1, s2, sDm, sDh, Uc, F, G, vol,
+                                     nullptr, 0.0, 0.0, &sp, Ssrc, dt);
         } else if constexpr (KIND == VOL_LW_ADAPT) {
             lwc_element<N, EB>(uN, Jn, invJ, gm1, i, j, act, sidx, sbase, s0, s1, s2, sDm, sDh, Uc, F, G, vol,
                                &terr, st->abstol, st->reltol);
