@@ -73,10 +73,10 @@ enum { TLW_BV_NONE = 0, TLW_BV_VORTEX = 1, TLW_BV_CONSTANT = 2, TLW_BV_RIEMANN4 
  * lw_volume_kernel_3!/4!, 674-1267; tree/multi_derivative_2d.jl:323-379, 487-522):
  * TLW_SRC_CONVERGENCE_TEST is Trixi's source_terms_convergence_test for CompressibleEulerEquations2D
  * (the residual of the state of TLW_BV_CONVERGENCE_TEST).  Euler LW and MDRK on a TreeMesh, plain
- * VolumeIntegralFR, and LW on a curvilinear StructuredMesh (structured/dg_2d.jl:1379-2105, the
- * reference's elixir_euler_source_terms_parallelogram.jl; tlw_create_curved then needs the cache's
- * node_coordinates); shock capturing, parabolic terms, MDRK on a curvilinear mesh, P4est frames and
- * adaptive steps with source terms return TLW_ENOTSUP.
+ * VolumeIntegralFR, and LW and MDRK on a curvilinear StructuredMesh (structured/dg_2d.jl:1379-2105,
+ * structured/multi_derivative_2d.jl:82-445; the reference's elixir_euler_source_terms_parallelogram.jl;
+ * tlw_create_curved then needs the cache's node_coordinates); shock capturing, parabolic terms,
+ * P4est frames and adaptive steps with source terms return TLW_ENOTSUP.
  * TLW_SRC_NAVIER_STOKES_CONVERGENCE_TEST is source_terms_navier_stokes_convergence_test of
  * examples/tree_2d_dgsem/elixir_navierstokes_convergence.jl (the residual of its manufactured
  * solution under the context's μ and Pr), in the Navier-Stokes MDRK stages on a TreeMesh

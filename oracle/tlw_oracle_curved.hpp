@@ -849,10 +849,45 @@ template <class T, class Eq, class Base = Solver<T, Eq>> struct CurvedSolver : B
         for (int j = 0; j < n; ++j)
             for (int i = 0; i < n; ++i)
                 for (int v = 0; v < NV; ++v) ut[j][i][v] = ut[j][i][v] * ijn[nidx(i, j, e)];
+        // source terms (multi_derivative_2d.jl:146-155, 179-200): ut += dt s, S = s/2 + s_t/8
+        // into du, S2 = s + s_t/6 to the stage cache (s_t at the stencil states about t)
+        const bool has_src = this->src.kind != SRC_NONE;
+        T S1[MAXN][MAXN][NV];
+        if (has_src)
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < n; ++i) {
+                    double x, y;
+                    node_xy(i, j, e, x, y);
+                    T sv[NV];
+                    this->src_at(&u[idx(0, i, j, e)], x, y, 0.0, sv);
+                    for (int v = 0; v < NV; ++v) {
+                        S1[j][i][v] = 0.5 * sv[v];
+                        this->S2[idx(v, i, j, e)] = sv[v];
+                        ut[j][i][v] = ut[j][i][v] + dt * sv[v];
+                    }
+                }
         for (int j = 0; j < n; ++j)
             for (int i = 0; i < n; ++i) {
                 T ft[NV], gt[NV], fct[NV], gct[NV];
                 mdrk_curved_ft(&u[idx(0, i, j, e)], ut[j][i], &Ja[4 * nidx(i, j, e)], ft, gt, fct, gct);
+                if (has_src) {
+                    double x, y;
+                    node_xy(i, j, e, x, y);
+                    T up[NV], um[NV], upp[NV], umm[NV], stv[NV];
+                    for (int v = 0; v < NV; ++v) {
+                        const T w = u[idx(v, i, j, e)], d = ut[j][i][v];
+                        up[v] = w + d;
+                        um[v] = w - d;
+                        upp[v] = w + 2.0 * d;
+                        umm[v] = w - 2.0 * d;
+                    }
+                    this->src_t_N34(up, upp, um, umm, x, y, dt, this->src_t, stv);
+                    for (int v = 0; v < NV; ++v) {
+                        S1[j][i][v] = S1[j][i][v] + 0.125 * stv[v];
+                        this->S2[idx(v, i, j, e)] = this->S2[idx(v, i, j, e)] + (1.0 / 6.0) * stv[v];
+                        du[idx(v, i, j, e)] = du[idx(v, i, j, e)] + (-1.0 / ijn[nidx(i, j, e)]) * S1[j][i][v];
+                    }
+                }
                 for (int v = 0; v < NV; ++v) {
                     this->F[fidx(v, 0, i, j, e)] = this->F[fidx(v, 0, i, j, e)] + 0.125 * ft[v];
                     this->F[fidx(v, 1, i, j, e)] = this->F[fidx(v, 1, i, j, e)] + 0.125 * gt[v];
@@ -927,10 +962,43 @@ template <class T, class Eq, class Base = Solver<T, Eq>> struct CurvedSolver : B
         for (int j = 0; j < n; ++j)
             for (int i = 0; i < n; ++i)
                 for (int v = 0; v < NV; ++v) ust[j][i][v] = ust[j][i][v] * ijn[nidx(i, j, e)];
+        // source terms (multi_derivative_2d.jl:326-339, 358-385): ust += dt s(us, t) and S =
+        // s(us, t) + s_t/3 about t -- the reference loads S2 into S and then overwrites S
+        // with s, so stage 1's s + s_t/6 does not reach the step (restated as is)
+        const bool has_src = this->src.kind != SRC_NONE;
+        T Sn[MAXN][MAXN][NV];
+        if (has_src)
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < n; ++i) {
+                    double x, y;
+                    node_xy(i, j, e, x, y);
+                    T sv[NV];
+                    this->src_at(&this->us[idx(0, i, j, e)], x, y, 0.0, sv);
+                    for (int v = 0; v < NV; ++v) {
+                        Sn[j][i][v] = sv[v];
+                        ust[j][i][v] = ust[j][i][v] + dt * sv[v];
+                    }
+                }
         for (int j = 0; j < n; ++j)
             for (int i = 0; i < n; ++i) {
                 T ft[NV], gt[NV], fct[NV], gct[NV];
                 mdrk_curved_ft(&this->us[idx(0, i, j, e)], ust[j][i], &Ja[4 * nidx(i, j, e)], ft, gt, fct, gct);
+                if (has_src) {
+                    double x, y;
+                    node_xy(i, j, e, x, y);
+                    T up[NV], um[NV], upp[NV], umm[NV], stv[NV];
+                    for (int v = 0; v < NV; ++v) {
+                        const T w = this->us[idx(v, i, j, e)], d = ust[j][i][v];
+                        up[v] = w + d;
+                        um[v] = w - d;
+                        upp[v] = w + 2.0 * d;
+                        umm[v] = w - 2.0 * d;
+                    }
+                    this->src_t_N34(up, upp, um, umm, x, y, dt, this->src_t, stv);
+                    for (int v = 0; v < NV; ++v)
+                        du[idx(v, i, j, e)] = du[idx(v, i, j, e)] +
+                                              (-1.0 / ijn[nidx(i, j, e)]) * (Sn[j][i][v] + (1.0 / 3.0) * stv[v]);
+                }
                 for (int v = 0; v < NV; ++v) {
                     this->F[fidx(v, 0, i, j, e)] = this->F[fidx(v, 0, i, j, e)] + (1.0 / 3.0) * ft[v];
                     this->F[fidx(v, 1, i, j, e)] = this->F[fidx(v, 1, i, j, e)] + (1.0 / 3.0) * gt[v];
@@ -947,18 +1015,10 @@ template <class T, class Eq, class Base = Solver<T, Eq>> struct CurvedSolver : B
             }
     }
 
-    // source terms in the curvilinear MDRK stages are not restated: such a call
-    // fills du with NaN
-    bool mdrk_src_refused(T* du) const {
-        if (this->src.kind == SRC_NONE) return false;
-        std::fill(du, du + this->ndofs_total(), T(std::numeric_limits<double>::quiet_NaN()));
-        return true;
-    }
-
     // rhs_mdrk1! / rhs_mdrk2! for StructuredMesh (multi_derivative_2d.jl:1-80):
     // both stages evaluate the interface flux with λ of uⁿ
     void rhs_mdrk1(T* du, const T* u, double t, double dt) override {
-        if (mdrk_src_refused(du)) return;
+        this->src_t = t;
         this->ensure_mdrk();
         this->reset_du(du);
         if (this->sc) this->sc_prepare(u, dt);
@@ -971,7 +1031,7 @@ template <class T, class Eq, class Base = Solver<T, Eq>> struct CurvedSolver : B
         surface_integral_and_jacobian_curved(du);
     }
     void rhs_mdrk2(T* du, const T* u, double t, double dt) override {
-        if (mdrk_src_refused(du)) return;
+        this->src_t = t;
         this->ensure_mdrk();
         this->reset_du(du);
         if (this->sc) this->sc_prepare(u, dt);

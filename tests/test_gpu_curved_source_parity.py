@@ -4,9 +4,11 @@ lines of lw_volume_kernel_3!/4! in src/solvers/dgsem_structured/dg_2d.jl:
 _nonperiodic.jl) against the CPU oracle: the LW residual with Trixi's
 source_terms_convergence_test on the parallelogram x = ξ + η, y = η, on a
 warped mesh and on a box with the convergence test's Dirichlet data; K-step
-device-loop states within 1e-12 relative; the device's convergence order on
-the parallelogram; the configurations refused (MDRK, P4est frames).  Metric
-terms are the oracle's on both sides."""
+device-loop states within 1e-12 relative; both MDRK stages
+(structured/multi_derivative_2d.jl:82-445 with their source lines, stage 2's
+overwrite of S restated); the device's convergence order on the
+parallelogram; P4est frames refused.  Metric terms are the oracle's on both
+sides."""
 import numpy as np
 import pytest
 
@@ -72,18 +74,48 @@ def test_curved_source_rhs_parity(oracle_mod, N, kind):
     semi.close()
 
 
-@pytest.mark.parametrize("N,kind,K", [(4, "parallelogram", 20), (3, "box", 20)])
-def test_curved_source_device_loop_k_steps(oracle_mod, N, kind, K):
+@pytest.mark.parametrize("N,kind", [(3, "parallelogram"), (4, "warp"), (4, "box")])
+def test_curved_source_mdrk_stage_parity(oracle_mod, N, kind):
     import trixilw_hip as tlw
 
-    print(f"curved source device loop N={N} {kind} K={K}", flush=True)
-    semi, o, mesh = setup(oracle_mod, N, 8, kind)
+    torch = torch_mod()
+    print(f"curved source MDRK stages N={N} {kind}", flush=True)
+    semi, o, mesh = setup(oracle_mod, N, 8, kind, td="mdrk")
+    x, y = o.node_coords()
+    t = 0.3
+    u_np = exact(x, y, t) * (1 + 1e-3 * np.random.default_rng(20 + N).standard_normal(x.shape + (4,)))
+    dt = 0.3 * o.max_dt(u_np)
+    d1 = o.rhs(u_np, t, dt, kind="mdrk1")
+    us = u_np + dt * d1
+    o.set_us(us)
+    d2 = o.rhs(u_np, t, dt, kind="mdrk2")
+    semi.cache["dt"][0] = dt
+    u = dev(u_np)
+    du = torch.empty_like(u)
+    tlw.rhs_mdrk1(du, u, semi, t)
+    torch.cuda.synchronize()
+    assert_close(du.cpu().numpy(), d1, what=f"curved source mdrk1 N={N} {kind}")
+    semi.set_us(dev(us))
+    tlw.rhs_mdrk2(du, u, semi, t)
+    torch.cuda.synchronize()
+    assert_close(du.cpu().numpy(), d2, what=f"curved source mdrk2 N={N} {kind}")
+    semi.close()
+
+
+@pytest.mark.parametrize("N,kind,K,td", [(4, "parallelogram", 20, "lw"), (3, "box", 20, "lw"),
+                                         (3, "parallelogram", 15, "mdrk"), (4, "box", 15, "mdrk")])
+def test_curved_source_device_loop_k_steps(oracle_mod, N, kind, K, td):
+    import trixilw_hip as tlw
+
+    print(f"curved source device loop N={N} {kind} K={K} {td}", flush=True)
+    semi, o, mesh = setup(oracle_mod, N, 8, kind, td=td)
     x, y = o.node_coords()
     u_np = exact(x, y, 0.0)
-    u_ref, t_ref, _ = o.solve(u_np, 0.0, 1e9, 0.2, max_steps=K)
+    u_ref, t_ref, _ = o.solve(u_np, 0.0, 1e9, 0.2 if td == "lw" else 0.1, max_steps=K, mdrk=(td == "mdrk"))
     lw_update = tlw.semidiscretize(semi, semi.time_discretization, (0.0, float("inf")))
     lw_update.u0.copy_(dev(u_np))
-    sol = tlw.solve_lwfr(lw_update, (), 1e-6, None, time_step_computation=tlw.CFLBased(0.2), max_steps=K)
+    sol = tlw.solve_lwfr(lw_update, (), 1e-6, None, time_step_computation=tlw.CFLBased(0.2 if td == "lw" else 0.1),
+                         max_steps=K)
     assert sol.stats.naccept == K
     assert abs(sol.t - t_ref) <= 1e-13 * t_ref
     assert_close(sol.u.cpu().numpy(), u_ref, what=f"curved source N={N} {kind} after {K} steps")
@@ -122,9 +154,6 @@ def test_curved_source_refusals(oracle_mod):
     from trixilw_hip import _lib
 
     torch_mod()
-    with pytest.raises(_lib.TLWError) as ei:  # MDRK on a curvilinear mesh with source terms
-        setup(oracle_mod, 3, 4, td="mdrk")
-    assert ei.value.code == _lib.TLW_ENOTSUP
     eq = tlw.CompressibleEulerEquations2D(GAMMA)
     mesh = tlw.P4estMesh((4, 4), warp_mapping(2.0, 0.05), turns=np.arange(16) % 4)
     with pytest.raises(_lib.TLWError) as ei:  # P4est frames with source terms

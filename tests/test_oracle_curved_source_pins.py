@@ -10,8 +10,11 @@ and _nonperiodic.jl; no GPU).
   then exceeding it by dt⁴ s⁽⁴⁾/180 (three-point s_tt, no s_tttt level).
 - Trixi's convergence test on the parallelogram x = ξ + η, y = η (periodic)
   converges at order ≈ N + 1 in the density's L2 error at t = 0.5.
-- Source terms in the curvilinear MDRK stages are not restated: the oracle
-  fills such a residual with NaN.
+- The curvilinear MDRK stages (structured/multi_derivative_2d.jl:146-200,
+  326-385): on the same uniform state stage 1 gives s/2 + dt s'/8 and the step
+  s + dt s'/3 for a linear source, because the reference's stage 2 loads stage
+  1's S2 = s + s_t/6 into S and then overwrites S with s (the exact average
+  would be s + dt s'/2; the TreeMesh stages give it).
 """
 import numpy as np
 import pytest
@@ -57,8 +60,21 @@ def test_parallelogram_converges(oracle_mod, N, orders):
     assert eoc[0] > orders[0] and eoc[1] > orders[1], (errs, eoc)
 
 
-def test_curved_mdrk_with_source_is_refused(oracle_mod):
-    o = oracle_mod.Oracle(3, 4, 4, 0.0, 0.0, 1.0, gamma=1.4, mapping=(oracle_mod.MAP_WARP, (2.0, 0.05)),
-                          periodic=(True, True), source_terms="convergence_test")
-    x, y = o.node_coords()
-    assert np.all(np.isnan(o.rhs(exact(x, y, 0.0), 0.0, 1e-3, kind="mdrk1")))
+@pytest.mark.parametrize("N", [3, 4])
+def test_curved_mdrk_source_weights_on_a_uniform_state(oracle_mod, N):
+    t, dt = 0.3, 0.01
+    for a in ([0.3, -0.7], [0.3, -0.7, 0.5]):
+        P = np.polynomial.Polynomial(a)
+        o = oracle_mod.Oracle(N, 6, 6, 0.0, 0.0, 1.0, gamma=1.4, mapping=(oracle_mod.MAP_WARP, (2.0, 0.08)),
+                              periodic=(True, True), source_terms=lambda u, x, y, tt, P=P: P(tt) * W4)
+        x, _ = o.node_coords()
+        u = np.empty(x.shape + (4,))
+        u[:] = [1.0, 0.1, 0.2, 2.6]
+        d1 = o.rhs(u, t, dt, kind="mdrk1")
+        # s_t of N34 is dt s' for a polynomial of degree <= 4
+        np.testing.assert_allclose(d1, np.broadcast_to((P(t) / 2 + dt * P.deriv()(t) / 8) * W4, u.shape), rtol=0,
+                                   atol=1e-11)
+        o.set_us(u + dt * d1)
+        d2 = o.rhs(u, t, dt, kind="mdrk2")
+        np.testing.assert_allclose(d2, np.broadcast_to((P(t) + dt * P.deriv()(t) / 3) * W4, u.shape), rtol=0,
+                                   atol=1e-11)

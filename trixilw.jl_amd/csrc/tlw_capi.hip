@@ -693,10 +693,18 @@ template <int N, bool SC, bool PAR = false> void launch_volume_sc(tlw_ctx* c, co
 }
 template <int N, int EB, bool SC> void launch_volume_curved_eb(tlw_ctx* c, const KArgs& a, int kind) {
     dim3 g(c->vgrid), b(VGeo<N, EB>::NT);
-    if constexpr (!SC) {  // the LW kernel with source terms
+    if constexpr (!SC) {  // the LW and MDRK kernels with source terms
         if (kind == VOL_LW_SRC) {
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_LW_SRC, false, false, true>), g, b, 0, c->stream,
                                a);
+            return;
+        } else if (kind == VOL_MDRK1_SRC) {
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_MDRK1_SRC, false, false, true>), g, b, 0,
+                               c->stream, a);
+            return;
+        } else if (kind == VOL_MDRK2_SRC) {
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_MDRK2_SRC, false, false, true>), g, b, 0,
+                               c->stream, a);
             return;
         }
     }
@@ -739,8 +747,8 @@ template <int N> void launch_volume_t(tlw_ctx* c, const KArgs& a, int kind) {
             hipLaunchKernelGGL(k_indicator<N>, dim3(c->sgrid), dim3(kSurfBlock), 0, c->stream, a, c->d_alpha_tmp);
             launch_volume_curved<N, true>(c, a, kind);
         } else {
-            const bool src = c->hconst.src_kind != TLW_SRC_NONE && kind == VOL_LW;
-            launch_volume_curved<N, false>(c, a, src ? (int)VOL_LW_SRC : kind);
+            const bool src = c->hconst.src_kind != TLW_SRC_NONE && kind <= VOL_MDRK2;
+            launch_volume_curved<N, false>(c, a, src ? kind + (int)VOL_LW_SRC : kind);
         }
     } else if (c->parabolic) {
         // walls: the instantiations that keep the wall face nodes' viscous averages
@@ -1980,9 +1988,9 @@ int create_common(int ninst, const tlw_mesh_desc* meshes, const tlw_basis_desc* 
         return TLW_EINVAL;
     // source terms: the Euler LW and MDRK kernels of the TreeMesh; the Navier-Stokes
     // convergence source: the Navier-Stokes MDRK kernels of the TreeMesh
-    // (on a curvilinear StructuredMesh: LW without shock capturing, structured/dg_2d.jl:1379-2105)
-    if (solver->source_terms == TLW_SRC_CONVERGENCE_TEST &&
-        (solver->shock_capturing || par || (cv && (pd || solver->time_discretization != TLW_LW))))
+    // (on a curvilinear StructuredMesh: LW and MDRK without shock capturing, structured/dg_2d.jl:1379-2105,
+    // structured/multi_derivative_2d.jl:82-445)
+    if (solver->source_terms == TLW_SRC_CONVERGENCE_TEST && (solver->shock_capturing || par || (cv && pd)))
         return TLW_ENOTSUP;
     if (solver->source_terms == TLW_SRC_NAVIER_STOKES_CONVERGENCE_TEST &&
         (!par || par->equations != TLW_PAR_NAVIER_STOKES || cv || pd || ninst != 1 ||

@@ -2482,12 +2482,14 @@ __device__ __forceinline__ void lwc_element(const double* uN, const double* J, d
 
 // mdrk_kernel_1! for StructuredMesh (structured/multi_derivative_2d.jl:82-243):
 // F̃ = f/2 + f_t/8 (Cartesian, for the records), its contravariant image for
-// the D̂ contraction, Ũ = u/2 + ut/8, ut from the contravariant fluxes.
-template <int N, int EB>
+// the D̂ contraction, Ũ = u/2 + ut/8, ut from the contravariant fluxes.  SRC
+// (146-155, 179-200): ut += dt s, Ssrc = s/2 + s_t/8.
+template <int N, int EB, bool SRC = false>
 __device__ __forceinline__ void mdrkc1_element(const double* uN, const double* J, double invJ, double gm1, int i,
                                                int j, bool act, int sidx, int sbase, double* s0, double* s2,
                                                const double* sDm, const double* sDh, double* Uc, double* F,
-                                               double* G, double* vol) {
+                                               double* G, double* vol, const SrcPt* sp = nullptr,
+                                               double* Ssrc = nullptr, double dtv = 0.0) {
     using G_ = VGeo<N, EB>;
     constexpr int n = G_::n, S = G_::S;
     double f[4], g[4], cx[4], cy[4];
@@ -2499,6 +2501,16 @@ __device__ __forceinline__ void mdrkc1_element(const double* uN, const double* J
     contract<n, S>(s0, sbase, i, j, sDm, ut);
 #pragma unroll
     for (int v = 0; v < 4; ++v) ut[v] *= invJ;
+    if constexpr (SRC) {
+        double s[4], st[4];
+        src_eval(*sp, 0.0, s);
+        src_t5(*sp, 0.0, dtv, st);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            ut[v] = fma(dtv, s[v], ut[v]);
+            Ssrc[v] = fma(0.125, st[v], 0.5 * s[v]);
+        }
+    }
     double dx[4], dy[4];
     mdrk_ft(uN, ut, gm1, dx, dy);
 #pragma unroll
@@ -2515,12 +2527,15 @@ __device__ __forceinline__ void mdrkc1_element(const double* uN, const double* J
 
 // mdrk_kernel_2! for StructuredMesh (structured/multi_derivative_2d.jl:245-445):
 // F̃ = F2 + f_t(us)/3, Ũ = U2 + ust/3, with F2 = f + f_t/6 and U2 = u + ut/6
-// recomputed from uⁿ instead of being stored by stage 1.
-template <int N, int EB>
+// recomputed from uⁿ instead of being stored by stage 1.  SRC (326-339,
+// 358-385): ut and ust += dt s(t); Ssrc = s + s_t/3 about t, the reference's
+// stage 2 overwriting the S2 it loads (stage 1's s + s_t/6) with s.
+template <int N, int EB, bool SRC = false>
 __device__ __forceinline__ void mdrkc2_element(const double* uN, const double* usn, const double* J, double invJ,
                                                double gm1, int i, int j, bool act, int sidx, int sbase, double* s0,
                                                double* s1, double* s2, const double* sDm, const double* sDh,
-                                               double* Uc, double* F, double* G, double* vol) {
+                                               double* Uc, double* F, double* G, double* vol,
+                                               const SrcPt* sp = nullptr, double* Ssrc = nullptr, double dtv = 0.0) {
     using G_ = VGeo<N, EB>;
     constexpr int n = G_::n, S = G_::S;
     double f[4], g[4], cx[4], cy[4];
@@ -2536,6 +2551,17 @@ __device__ __forceinline__ void mdrkc2_element(const double* uN, const double* u
     contract<n, S>(s1, sbase, i, j, sDm, ust);
 #pragma unroll
     for (int v = 0; v < 4; ++v) { ut[v] *= invJ; ust[v] *= invJ; }
+    if constexpr (SRC) {
+        double s[4], st[4];
+        src_eval(*sp, 0.0, s);
+        src_t5(*sp, 0.0, dtv, st);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            ut[v] = fma(dtv, s[v], ut[v]);
+            ust[v] = fma(dtv, s[v], ust[v]);
+            Ssrc[v] = fma(1.0 / 3.0, st[v], s[v]);
+        }
+    }
     double dx[4], dy[4];
     mdrk_ft(uN, ut, gm1, dx, dy);
 #pragma unroll
@@ -2629,8 +2655,7 @@ __device__ __forceinline__ void volume_body(const KArgs& a, int bid, double* lds
     constexpr bool SRC = vol_is_src(KIND);
     constexpr bool WALL = vol_is_wall(KIND);
     constexpr int KB = vol_base(KIND);
-    static_assert(!SRC || (!SC && !PAR && (!CURV || KB == VOL_LW)),
-                  "source terms: the Euler TreeMesh kernels and the curvilinear LW kernel");
+    static_assert(!SRC || (!SC && !PAR), "source terms: the Euler kernels");
     double Ssrc[4] = {0.0, 0.0, 0.0, 0.0};
     SrcPt sp{0.0, 0.0, 0.0, gamma};
     if constexpr (SRC && CURV) {  // the node's coordinates of the curvilinear mesh
@@ -2667,6 +2692,14 @@ __device__ __forceinline__ void volume_body(const KArgs& a, int bid, double* lds
         } else if constexpr (KIND == VOL_LW_SRC) {
             lwc_element<N, EB, true>(uN, Jn, invJ, gm1, i, j, act, sidx, sbase, s0, s1, s2, sDm, sDh, Uc, F, G, vol,
                                      nullptr, 0.0, 0.0, &sp, Ssrc, dt);
+        } else if constexpr (KIND == VOL_MDRK1_SRC) {
+            mdrkc1_element<N, EB, true>(uN, Jn, invJ, gm1, i, j, act, sidx, sbase, s0, s2, sDm, sDh, Uc, F, G, vol,
+                                        &sp, Ssrc, dt);
+        } else if constexpr (KIND == VOL_MDRK2_SRC) {
+            double usn[4];
+            load4(a.us + off, usn);
+            mdrkc2_element<N, EB, true>(uN, usn, Jn, invJ, gm1, i, j, act, sidx, sbase, s0, s1, s2, sDm, sDh, Uc, F,
+                                        G, vol, &sp, Ssrc, dt);
         } else if constexpr (KIND == VOL_LW_ADAPT) {
             lwc_element<N, EB>(uN, Jn, invJ, gm1, i, j, act, sidx, sbase, s0, s1, s2, sDm, sDh, Uc, F, G, vol,
                                &terr, st->abstol, st->reltol);
