@@ -75,8 +75,9 @@ enum { TLW_BV_NONE = 0, TLW_BV_VORTEX = 1, TLW_BV_CONSTANT = 2, TLW_BV_RIEMANN4 
  * (the residual of the state of TLW_BV_CONVERGENCE_TEST).  Euler LW and MDRK on a TreeMesh, plain
  * VolumeIntegralFR, and LW and MDRK on a curvilinear StructuredMesh (structured/dg_2d.jl:1379-2105,
  * structured/multi_derivative_2d.jl:82-445; the reference's elixir_euler_source_terms_parallelogram.jl;
- * tlw_create_curved then needs the cache's node_coordinates); shock capturing, parabolic terms,
- * P4est frames and adaptive steps with source terms return TLW_ENOTSUP.
+ * tlw_create_curved then needs the cache's node_coordinates), also on a P4estMesh (tlw_create_p4est,
+ * whose volume kernels are the same); shock capturing, parabolic terms and adaptive steps with
+ * source terms return TLW_ENOTSUP.
  * TLW_SRC_NAVIER_STOKES_CONVERGENCE_TEST is source_terms_navier_stokes_convergence_test of
  * examples/tree_2d_dgsem/elixir_navierstokes_convergence.jl (the residual of its manufactured
  * solution under the context's μ and Pr), in the Navier-Stokes MDRK stages on a TreeMesh

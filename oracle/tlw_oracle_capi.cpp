@@ -146,8 +146,8 @@ void* tlwo_create(const tlwo_desc* d) {
     // (a callback source may also go with the TreeMesh Navier-Stokes kernels, where the
     // oracle's pins give it sources of t alone)
     if (d->source_kind == SRC_NS_CONVERGENCE && (d->equation != 0 || d->parabolic != 2 || d->curved)) return nullptr;
-    // (the curvilinear LW kernels take them too, without shock capturing or P4est frames)
-    const bool curved_src_ok = !d->curved || (!d->parabolic && !d->shock_capturing && !d->p4est);
+    // (the curvilinear and P4est LW and MDRK kernels take them too, without shock capturing)
+    const bool curved_src_ok = !d->curved || (!d->parabolic && !d->shock_capturing);
     if (d->source_kind == SRC_CONVERGENCE_TEST && (d->equation != 0 || d->parabolic || !curved_src_ok))
         return nullptr;
     if (d->source_kind == SRC_CALLBACK &&

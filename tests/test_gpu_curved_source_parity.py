@@ -7,8 +7,9 @@ warped mesh and on a box with the convergence test's Dirichlet data; K-step
 device-loop states within 1e-12 relative; both MDRK stages
 (structured/multi_derivative_2d.jl:82-445 with their source lines, stage 2's
 overwrite of S restated); the device's convergence order on the
-parallelogram; P4est frames refused.  Metric terms are the oracle's on both
-sides."""
+parallelogram; a P4estMesh with turned element frames (same volume kernels);
+shock capturing with source terms refused.  Metric terms are the oracle's on
+both sides."""
 import numpy as np
 import pytest
 
@@ -149,14 +150,72 @@ def test_parallelogram_device_eoc(oracle_mod):
     assert eoc[0] > 4.4 and eoc[1] > 4.0, (errs, eoc)
 
 
+@pytest.mark.parametrize("N,td", [(3, "lw"), (4, "lw"), (3, "mdrk"), (4, "mdrk")])
+def test_p4est_source_parity(oracle_mod, N, td):
+    """The convergence test on the warped P4estMesh (frames turned by random
+    multiples of 90°): the residual (both stages for MDRK) and 15 device-loop
+    steps against the oracle."""
+    import trixilw_hip as tlw
+
+    torch = torch_mod()
+    print(f"P4est source N={N} {td}", flush=True)
+    nx = 8
+    turns = np.random.default_rng(30 + N).integers(0, 4, nx * nx)
+    eq = tlw.CompressibleEulerEquations2D(GAMMA)
+    mesh = tlw.P4estMesh((nx, nx), warp_mapping(2.0, 0.05), turns=turns)
+    solver = tlw.DGSEM(N, volume_integral=tlw.VolumeIntegralFR(tlw.MDRK() if td == "mdrk" else tlw.LW()))
+    o = oracle_mod.Oracle(N, nx, nx, 0.0, 0.0, 1.0, gamma=GAMMA, mapping=(oracle_mod.MAP_WARP, (2.0, 0.05)),
+                          periodic=(True, True), p4est=turns, source_terms="convergence_test")
+    mesh.set_metrics(solver.basis, o.aux("contravariant_vectors"), o.aux("inverse_jacobian"))
+    semi = tlw.SemidiscretizationHyperbolic(mesh, tlw.get_time_discretization(solver), eq,
+                                            tlw.initial_condition_convergence_test, solver,
+                                            source_terms=tlw.source_terms_convergence_test)
+    x, y = o.node_coords()
+    t = 0.3
+    u_np = exact(x, y, t) * (1 + 1e-3 * np.random.default_rng(N).standard_normal(x.shape + (4,)))
+    dt = 0.3 * o.max_dt(u_np)
+    semi.cache["dt"][0] = dt
+    u = dev(u_np)
+    du = torch.empty_like(u)
+    if td == "lw":
+        ref = o.rhs(u_np, t, dt)
+        tlw.rhs(du, u, semi, t)
+        torch.cuda.synchronize()
+        assert_close(du.cpu().numpy(), ref, what=f"P4est source LW N={N}")
+    else:
+        d1 = o.rhs(u_np, t, dt, kind="mdrk1")
+        us = u_np + dt * d1
+        o.set_us(us)
+        d2 = o.rhs(u_np, t, dt, kind="mdrk2")
+        tlw.rhs_mdrk1(du, u, semi, t)
+        torch.cuda.synchronize()
+        assert_close(du.cpu().numpy(), d1, what=f"P4est source mdrk1 N={N}")
+        semi.set_us(dev(us))
+        tlw.rhs_mdrk2(du, u, semi, t)
+        torch.cuda.synchronize()
+        assert_close(du.cpu().numpy(), d2, what=f"P4est source mdrk2 N={N}")
+    cfl = 0.2 if td == "lw" else 0.1
+    u0 = exact(x, y, 0.0)
+    u_ref, t_ref, _ = o.solve(u0, 0.0, 1e9, cfl, max_steps=15, mdrk=(td == "mdrk"))
+    lw_update = tlw.semidiscretize(semi, semi.time_discretization, (0.0, float("inf")))
+    lw_update.u0.copy_(dev(u0))
+    sol = tlw.solve_lwfr(lw_update, (), 1e-6, None, time_step_computation=tlw.CFLBased(cfl), max_steps=15)
+    assert sol.stats.naccept == 15 and abs(sol.t - t_ref) <= 1e-13 * t_ref
+    assert_close(sol.u.cpu().numpy(), u_ref, what=f"P4est source {td} N={N} after 15 steps")
+    semi.close()
+
+
 def test_curved_source_refusals(oracle_mod):
     import trixilw_hip as tlw
     from trixilw_hip import _lib
 
     torch_mod()
     eq = tlw.CompressibleEulerEquations2D(GAMMA)
-    mesh = tlw.P4estMesh((4, 4), warp_mapping(2.0, 0.05), turns=np.arange(16) % 4)
-    with pytest.raises(_lib.TLWError) as ei:  # P4est frames with source terms
-        tlw.SemidiscretizationHyperbolic(mesh, tlw.LW(), eq, tlw.initial_condition_convergence_test, tlw.DGSEM(3),
+    mesh = tlw.StructuredMesh((4, 4), warp_mapping(2.0, 0.05))
+    ind = tlw.IndicatorHennemannGassner(eq, None, alpha_max=0.5)
+    vi = tlw.VolumeIntegralFRShockCapturing(ind, volume_integralFR=tlw.VolumeIntegralFR(tlw.LW()))
+    with pytest.raises(_lib.TLWError) as ei:  # shock capturing with source terms
+        tlw.SemidiscretizationHyperbolic(mesh, tlw.LW(), eq, tlw.initial_condition_convergence_test,
+                                         tlw.DGSEM(3, volume_integral=vi),
                                          source_terms=tlw.source_terms_convergence_test)
     assert ei.value.code == _lib.TLW_ENOTSUP

@@ -28,12 +28,15 @@ def exact(x, y, t):
 
 
 @pytest.mark.parametrize("N", [3, 4])
-def test_source_weights_on_a_uniform_state_of_a_warped_mesh(oracle_mod, N):
+@pytest.mark.parametrize("p4est", [False, True])
+def test_source_weights_on_a_uniform_state_of_a_warped_mesh(oracle_mod, N, p4est):
+    """(also with P4est element frames turned by random multiples of 90°)"""
     t, dt = 0.3, 0.01
+    turns = np.random.default_rng(4).integers(0, 4, 36) if p4est else None
     for deg in (1, 2, 3, 4):
         P = np.polynomial.Polynomial([0.3, -0.7, 0.5, 0.9, -0.4][:deg + 1])
         o = oracle_mod.Oracle(N, 6, 6, 0.0, 0.0, 1.0, gamma=1.4, mapping=(oracle_mod.MAP_WARP, (2.0, 0.08)),
-                              periodic=(True, True), source_terms=lambda u, x, y, tt, P=P: P(tt) * W4)
+                              periodic=(True, True), source_terms=lambda u, x, y, tt, P=P: P(tt) * W4, p4est=turns)
         x, _ = o.node_coords()
         u = np.empty(x.shape + (4,))
         u[:] = [1.0, 0.1, 0.2, 2.6]

@@ -193,6 +193,6 @@ def test_sources_are_refused_where_not_restated(oracle_mod):
     # (the curvilinear LW kernels take sources: tests/test_oracle_curved_source_pins.py)
     warp = (oracle_mod.MAP_WARP, (2.0, 0.05))
     for kw in (dict(equation="linear_advection"), dict(parabolic="navier_stokes", mu=0.01),
-               dict(mapping=warp, shock_capturing=True), dict(mapping=warp, p4est=True)):
+               dict(mapping=warp, shock_capturing=True)):
         with pytest.raises(ValueError):
             oracle_mod.Oracle(3, 4, 4, 0.0, 0.0, 0.5, source_terms="convergence_test", **kw)

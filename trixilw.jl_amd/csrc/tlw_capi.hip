@@ -1980,18 +1980,16 @@ int create_common(int ninst, const tlw_mesh_desc* meshes, const tlw_basis_desc* 
         (!cv || par))
         return TLW_ENOTSUP;
     // P4estMesh: the hyperbolic plain-FR kernels of the curvilinear mesh
-    if (pd && (par || solver->shock_capturing || solver->boundary_kind == TLW_BV_SLIP_WALL ||
-               solver->source_terms != TLW_SRC_NONE || ninst != 1))
+    if (pd && (par || solver->shock_capturing || solver->boundary_kind == TLW_BV_SLIP_WALL || ninst != 1))
         return TLW_ENOTSUP;
     if (!(solver->gamma > 1.0)) return TLW_EINVAL;
     if (solver->source_terms < TLW_SRC_NONE || solver->source_terms > TLW_SRC_NAVIER_STOKES_CONVERGENCE_TEST)
         return TLW_EINVAL;
     // source terms: the Euler LW and MDRK kernels of the TreeMesh; the Navier-Stokes
     // convergence source: the Navier-Stokes MDRK kernels of the TreeMesh
-    // (on a curvilinear StructuredMesh: LW and MDRK without shock capturing, structured/dg_2d.jl:1379-2105,
-    // structured/multi_derivative_2d.jl:82-445)
-    if (solver->source_terms == TLW_SRC_CONVERGENCE_TEST && (solver->shock_capturing || par || (cv && pd)))
-        return TLW_ENOTSUP;
+    // (on a curvilinear StructuredMesh or a P4estMesh, whose volume kernels are the same: LW and MDRK
+    // without shock capturing, structured/dg_2d.jl:1379-2105, structured/multi_derivative_2d.jl:82-445)
+    if (solver->source_terms == TLW_SRC_CONVERGENCE_TEST && (solver->shock_capturing || par)) return TLW_ENOTSUP;
     if (solver->source_terms == TLW_SRC_NAVIER_STOKES_CONVERGENCE_TEST &&
         (!par || par->equations != TLW_PAR_NAVIER_STOKES || cv || pd || ninst != 1 ||
          (solver->time_discretization != TLW_MDRK && !par->walls)))
