@@ -50,10 +50,10 @@ def test_hot_kernels_stay_out_of_private_memory(usage):
         # metrics and spill 68 B
         curved_sc = name.startswith("_ZN3tlw8k_volumeILi4E") and name.endswith("Lb1ELb0ELb1EEEvNS_5KArgsE")
         assert u["ScratchSize"] <= (96 if curved_sc else 64), (name, u)
-        # the Navier-Stokes LW and MDRK stage-2 wall kernels (KIND 4 and 9) run at one
-        # wave per SIMD by design, their wall terms in accumulation registers: no
-        # private memory at all
-        wall = re.match(r"_ZN3tlw8k_volumeILi\dELi\d+ELi[49]E", name) is not None
+        # the Navier-Stokes LW, MDRK stage-2 and adaptive wall kernels (KIND 4, 9 and 10)
+        # run at one wave per SIMD by design, their wall terms in accumulation
+        # registers: no private memory at all
+        wall = re.match(r"_ZN3tlw8k_volumeILi\dELi\d+ELi(4|9|10)E", name) is not None
         if wall:
             assert u["ScratchSize"] == 0 and u["Occupancy"] >= 1, (name, u)
         else:

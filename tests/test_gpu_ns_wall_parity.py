@@ -292,3 +292,35 @@ def test_cavity_mdrk_device_loop_k_steps(oracle_mod, N, K, heat):
     assert abs(sol.t - t_ref) <= 1e-13 * t_ref
     assert_close(to_lex(mesh, sol.u), u_ref, what=f"NS walls MDRK N={N} after {K} steps")
     semi.close()
+
+
+@pytest.mark.parametrize("dt_initial", [1e-5, 0.01])
+def test_cavity_adaptive_matches_oracle(oracle_mod, dt_initial):
+    """Adaptive steps with walls (the reference's Ghia utility runs the cavity
+    with Adaptive()): the LW N=4 wall kernel with the temporal error of
+    dg_2d_parabolic.jl:1749-1775, 1841-1855; accepted and rejected counts,
+    final time and state equal the oracle's solve_adaptive (≤1e-12), from a
+    small dt_initial and from an unstable one."""
+    import trixilw_hip as tlw
+
+    torch_mod()
+    print(f"NS walls adaptive dt0={dt_initial}", flush=True)
+    tol = {"abstol": 1e-6, "reltol": 1e-6}
+    semi, _, mesh = setup_walls(oracle_mod, 4, 3, heat=(None, 73.0, None, None))
+    o = oracle_mod.Oracle(4, mesh.nx, mesh.nx, -1.0, -1.0, 2.0 / mesh.nx, gamma=GAMMA, periodic=(False, False),
+                          bc_kind=oracle_mod.BV_SLIP_WALL, parabolic="navier_stokes", mu=0.01, prandtl=PR,
+                          ns_walls=[(0.0, 0.0), (0.0, 0.0), (0.0, 0.0), LID],
+                          wall_temperature=[None, 73.0, None, None], tolerances=(tol["abstol"], tol["reltol"]))
+    u_lex = perturbed(o, 50, amp=0.02)
+    tf = 0.02
+    u_ref, t_ref, dts, nrej = o.solve_adaptive(u_lex, 0.0, tf, dt_initial, max_attempts=20000)
+    lw_update = tlw.semidiscretize(semi, semi.time_discretization, (0.0, tf))
+    lw_update.u0.copy_(to_dev(mesh, u_lex))
+    sol = tlw.solve_lwfr(lw_update, (), dt_initial, tol, time_step_computation=tlw.Adaptive())
+    assert sol.stats.naccept == len(dts) > 3
+    assert sol.stats.nreject == nrej
+    if dt_initial > 1e-3:
+        assert nrej >= 1
+    assert abs(sol.t - t_ref) <= 1e-13 * t_ref
+    assert_close(to_lex(mesh, sol.u), u_ref, what=f"NS walls adaptive dt0={dt_initial}")
+    semi.close()

@@ -677,8 +677,13 @@ template <int N, int EB, bool SC, bool PAR = false> void launch_volume_eb(tlw_ct
                 hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_LW_WALL, SC, PAR>), g, b, 0, c->stream, a);
             else if (kind == VOL_MDRK1_WALL)
                 hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_MDRK1_WALL, SC, PAR>), g, b, 0, c->stream, a);
-            else
+            else if (kind == VOL_MDRK2_WALL)
                 hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_MDRK2_WALL, SC, PAR>), g, b, 0, c->stream, a);
+            else if constexpr (N == 4)  // VOL_LW_ADAPT_WALL: the temporal error is kernel 4's
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_volume<N, EB, VOL_LW_ADAPT_WALL, SC, PAR>), g, b, 0, c->stream,
+                                   a);
+            else
+                c->bad_launch = true;
         } else {
             c->bad_launch = true;
         }
@@ -752,8 +757,11 @@ template <int N> void launch_volume_t(tlw_ctx* c, const KArgs& a, int kind) {
         }
     } else if (c->parabolic) {
         // walls: the instantiations that keep the wall face nodes' viscous averages
-        const int wk = kind == VOL_LW ? VOL_LW_WALL : kind == VOL_MDRK1 ? VOL_MDRK1_WALL : VOL_MDRK2_WALL;
-        launch_volume_sc<N, false, true>(c, a, (c->hconst.ns_walls && kind != VOL_LW_ADAPT) ? wk : kind);
+        const int wk = kind == VOL_LW         ? VOL_LW_WALL
+                       : kind == VOL_MDRK1    ? VOL_MDRK1_WALL
+                       : kind == VOL_MDRK2    ? VOL_MDRK2_WALL
+                                              : VOL_LW_ADAPT_WALL;
+        launch_volume_sc<N, false, true>(c, a, c->hconst.ns_walls ? wk : kind);
     } else if (c->sc) {
         // IndicatorHennemannGassner on uⁿ before the volume integral (dg_2d.jl:1665)
         hipLaunchKernelGGL(k_indicator<N>, dim3(c->sgrid), dim3(kSurfBlock), 0, c->stream, a, c->d_alpha_tmp);
@@ -1381,10 +1389,11 @@ int tlw_solve_init_adaptive(tlw_ctx* c, double t0, double t_final, double dt_ini
     if (!(abstol >= 0.0) || !(reltol >= 0.0) || !(abstol > 0.0 || reltol > 0.0) || !std::isfinite(abstol) ||
         !std::isfinite(reltol))
         return fail(c, TLW_EINVAL, "tolerances: abstol, reltol >= 0, not both zero");
-    if (c->time_disc != TLW_LW || (c->N != 4 && !c->curved) || (c->parabolic && (c->curved || c->hconst.ns_walls)))
+    if (c->time_disc != TLW_LW || (c->N != 4 && !c->curved) || (c->parabolic && c->curved))
         return fail(c, TLW_ENOTSUP,
                     "adaptive steps need the temporal-error estimate of an LW kernel (Euler on a TreeMesh at polydeg "
-                    "4 or on a curvilinear mesh at polydeg 3 or 4; Navier-Stokes on a periodic TreeMesh at polydeg 4)");
+                    "4 or on a curvilinear mesh at polydeg 3 or 4; Navier-Stokes on a TreeMesh at polydeg 4, periodic "
+                    "or with walls)");
     if (c->positivity) return fail(c, TLW_ENOTSUP, "adaptive steps are not combined with the positivity limiter");
     if (c->hconst.src_kind != TLW_SRC_NONE)
         return fail(c, TLW_ENOTSUP, "adaptive steps are not combined with source terms");

@@ -131,16 +131,20 @@ enum Reduce { RED_NONE = 0, RED_STEP = 1, RED_INIT = 2, RED_QUERY = 3, RED_ADAPT
 // instantiations, VOL_LW_SRC + the plain kind)
 enum VolKind {
     VOL_LW = 0, VOL_MDRK1 = 1, VOL_MDRK2 = 2, VOL_LW_ADAPT = 3, VOL_LW_WALL = 4,
-    VOL_LW_SRC = 5, VOL_MDRK1_SRC = 6, VOL_MDRK2_SRC = 7, VOL_MDRK1_WALL = 8, VOL_MDRK2_WALL = 9
+    VOL_LW_SRC = 5, VOL_MDRK1_SRC = 6, VOL_MDRK2_SRC = 7, VOL_MDRK1_WALL = 8, VOL_MDRK2_WALL = 9,
+    VOL_LW_ADAPT_WALL = 10
 };
 constexpr bool vol_is_src(int k) { return k >= VOL_LW_SRC && k <= VOL_MDRK2_SRC; }
-constexpr bool vol_is_wall(int k) { return k == VOL_LW_WALL || k == VOL_MDRK1_WALL || k == VOL_MDRK2_WALL; }
+constexpr bool vol_is_wall(int k) {
+    return k == VOL_LW_WALL || k == VOL_MDRK1_WALL || k == VOL_MDRK2_WALL || k == VOL_LW_ADAPT_WALL;
+}
 // the plain kind of a source-term or wall instantiation
 constexpr int vol_base(int k) {
     return vol_is_src(k)           ? k - VOL_LW_SRC
            : k == VOL_LW_WALL      ? (int)VOL_LW
            : k == VOL_MDRK1_WALL   ? (int)VOL_MDRK1
            : k == VOL_MDRK2_WALL   ? (int)VOL_MDRK2
+           : k == VOL_LW_ADAPT_WALL ? (int)VOL_LW_ADAPT
                                    : k;
 }
 // source-term kinds of DevConst::src_kind
@@ -2620,7 +2624,7 @@ __device__ __forceinline__ void volume_body(const KArgs& a, int bid, double* lds
                                             double* s_alpha) {
     static_assert(!(SC && PAR), "shock capturing has no hyperbolic-parabolic form");
     static_assert(!CURV || !PAR || KIND != VOL_LW_ADAPT, "adaptive steps: Euler LW, or Navier-Stokes LW on a TreeMesh");
-    static_assert(!PAR || KIND != VOL_LW_ADAPT || N == 4, "the parabolic temporal error is kernel 4's");
+    static_assert(!PAR || vol_base(KIND) != VOL_LW_ADAPT || N == 4, "the parabolic temporal error is kernel 4's");
     static_assert(!vol_is_wall(KIND) || (PAR && !CURV && !SC), "walls: the TreeMesh Navier-Stokes kernels");
     static_assert(!(CURV && PAR && SC), "no shock capturing with parabolic terms");
     using G_ = VGeo<N, EB>;
@@ -2745,9 +2749,9 @@ __device__ __forceinline__ void volume_body(const KArgs& a, int bid, double* lds
                 lwp_element<N, EB, CURV, WALL>(uN, gxN, gyN, invJ, gm1, c, i, j, act, sidx, sbase, s0, s1, s2, sDm,
                                                sD, sDh, Uc, F, G, vol, Jn, wq, nullptr, 0.0, 0.0, nsrc, nsp, Ssrc,
                                                dt);
-            else if constexpr (KIND == VOL_LW_ADAPT)
-                lwp_element<N, EB, CURV, false>(uN, gxN, gyN, invJ, gm1, c, i, j, act, sidx, sbase, s0, s1, s2, sDm,
-                                                sD, sDh, Uc, F, G, vol, Jn, wq, &terr, st->abstol, st->reltol);
+            else if constexpr (KB == VOL_LW_ADAPT)
+                lwp_element<N, EB, CURV, WALL>(uN, gxN, gyN, invJ, gm1, c, i, j, act, sidx, sbase, s0, s1, s2, sDm,
+                                               sD, sDh, Uc, F, G, vol, Jn, wq, &terr, st->abstol, st->reltol);
             else
                 mdrkp1_element<N, EB, CURV, WALL>(uN, gxN, gyN, invJ, gm1, c, i, j, act, sidx, sbase, s0, s1, s2, sDm,
                                                   sD, sDh, Uc, F, G, vol, Jn, wq, nsrc, nsp, dt, Ssrc);
@@ -2992,7 +2996,7 @@ __device__ __forceinline__ void volume_body(const KArgs& a, int bid, double* lds
         val[v] = a.step_mode ? fma(dt, d, uN[v]) : d;
     }
     if (act) store4(out + off, val);
-    if constexpr (KIND == VOL_LW_ADAPT) {
+    if constexpr (KB == VOL_LW_ADAPT) {
         if (a.reduce != RED_ADAPT) return;
         // the workgroup's sum of its elements' temporal errors (adaptive.jl:48-64)
         double* se = lds;
@@ -3030,7 +3034,9 @@ __device__ __forceinline__ void volume_body(const KArgs& a, int bid, double* lds
 // memory; the stage-1 wall kernel fits two waves
 template <int KIND, bool SC, bool PAR>
 constexpr int kVolWavesOf =
-    (KIND == VOL_LW_WALL || KIND == VOL_MDRK2_WALL) ? 1 : kVolWaves<SC || PAR || vol_is_src(KIND)>;
+    (KIND == VOL_LW_WALL || KIND == VOL_MDRK2_WALL || KIND == VOL_LW_ADAPT_WALL)
+        ? 1
+        : kVolWaves<SC || PAR || vol_is_src(KIND)>;
 template <int N, int EB, int KIND, bool SC, bool PAR = false, bool CURV = false>
 __global__ __launch_bounds__((VGeo<N, EB>::NT), (kVolWavesOf<KIND, SC, PAR>)) void k_volume(KArgs a) {
     using G_ = VGeo<N, EB>;
