@@ -4,8 +4,9 @@ data/lid_driven_cavity_{y_vs_u1,x_vs_u2}_r1000.txt (copied to tests/golden/).
 The setup is examples/tree_2d_dgsem/elixir_navierstokes_lid_driven_cavity_ghia.jl:
 [0, 1]², 16² cells, polydeg 4, LW, μ = 0.001, Pr = 0.7, Ma 0.1 at rest, lid
 velocity (1, 0), reflecting walls for the hyperbolic part with no-slip adiabatic
-walls, CFL 0.98 to t = 25 (the time the reference's utils/extract_lid_driven_cavity.jl
-compares at).  The device loop runs the whole solve; the velocity along the
+walls, to t = 25 (the time the reference's utils/extract_lid_driven_cavity.jl
+compares at) with CFL 0.98 and, as that utility's error-based run, with
+Adaptive() at tolerances 1e-8.  The device loop runs the whole solve; the velocity along the
 vertical and horizontal centrelines (element-boundary nodes averaged, GLL
 Lagrange interpolation along the line) is compared with the table's interior
 rows within 0.03 (the device reaches 0.016 and 0.015).  This checks the wall
@@ -54,7 +55,8 @@ def centreline(u_lex, nodes, nx, along, comp, coords):
     return np.array(out)
 
 
-def test_cavity_re1000_matches_ghia():
+@pytest.mark.parametrize("stepping", ["cfl", "adaptive"])
+def test_cavity_re1000_matches_ghia(stepping):
     import trixilw_hip as tlw
 
     torch = torch_mod()
@@ -88,8 +90,12 @@ def test_cavity_re1000_matches_ghia():
     u0 = cavity(X, Y)
     lw_update = tlw.semidiscretize(semi, semi.time_discretization, (0.0, 25.0))
     lw_update.u0.copy_(to_dev(mesh, u0))
-    print("cavity Re 1000: device loop to t = 25", flush=True)
-    sol = tlw.solve_lwfr(lw_update, (), 1e-6, None, time_step_computation=tlw.CFLBased(0.98))
+    print(f"cavity Re 1000: device loop to t = 25 ({stepping})", flush=True)
+    if stepping == "cfl":
+        sol = tlw.solve_lwfr(lw_update, (), 1e-6, None, time_step_computation=tlw.CFLBased(0.98))
+    else:
+        sol = tlw.solve_lwfr(lw_update, (), 1e-3, {"abstol": 1e-8, "reltol": 1e-8},
+                             time_step_computation=tlw.Adaptive())
     torch.cuda.synchronize()
     print(f"cavity Re 1000: t = {sol.t}, {sol.stats.naccept} steps", flush=True)
     assert abs(sol.t - 25.0) < 1e-12
@@ -106,6 +112,6 @@ def test_cavity_re1000_matches_ghia():
     d1, d2 = np.abs(u1 - gy[:, 1]), np.abs(u2 - gx[:, 1])
     print(f"cavity Re 1000: max |u1 - Ghia| {d1.max():.4f}, max |u2 - Ghia| {d2.max():.4f}", flush=True)
     print("u1", np.round(u1, 4).tolist(), "u2", np.round(u2, 4).tolist(), flush=True)
-    # measured: 0.016 (u1) and 0.015 (u2) after 378318 steps
+    # measured: 0.016 (u1) and 0.015 (u2) after 378318 CFL steps or 115057 adaptive ones
     assert d1.max() < 0.03 and d2.max() < 0.03
     semi.close()
